@@ -1,0 +1,99 @@
+"""ctypes binding of libhdfe_hip.so, the C ABI declared in include/hdfe_hip.h.
+
+The library is the only compute path of this package: there is no CPU
+fallback.  Loading fails loudly (ImportError) when the library has not been
+built, and `require_device()` raises RuntimeError without a HIP device.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libhdfe_hip.so"
+BUILD_HINT = ("device='rocm' needs the HIP library built for gfx950: run "
+              "`python -m hdfe_stream_amd._build` (hipcc from ROCm) and use an AMD Instinct "
+              "MI355X")
+
+STATUS = {0: "ok", 1: "invalid argument", 2: "HIP error", 3: "out of device memory",
+          4: "no device"}
+
+_lib = None
+
+# (name, restype, argtypes) of every exported function; the header declares
+# exactly these.
+P = C.c_void_p
+I64 = C.c_int64
+I32 = C.c_int
+D = C.c_double
+SIGNATURES = [
+    ("hdfe_last_error", C.c_char_p, []),
+    ("hdfe_version", I32, []),
+    ("hdfe_device_info", I32, [C.c_char_p, I32, P, P]),
+    ("hdfe_csr_create", I32, [I64, I64, P, P, P, P, I32, P]),
+    ("hdfe_csr_info", I32, [P, P, P, P]),
+    ("hdfe_pcg_solve", I32, [P, I64, P, P, P, D, I64, I32, D, P, P, P, P, P, I32, P]),
+    ("hdfe_csr_matmat", I32, [P, I64, P, P, I32]),
+    ("hdfe_destroy", None, [P]),
+]
+
+
+def load():
+    """The loaded library (ctypes.CDLL), loading it on first use."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise ImportError(f"{LIB_PATH.name} not found. {BUILD_HINT}")
+        lib = C.CDLL(str(LIB_PATH))
+        for name, res, args in SIGNATURES:
+            try:
+                fn = getattr(lib, name)
+            except AttributeError:
+                continue
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def exported(name: str) -> bool:
+    return hasattr(load(), name)
+
+
+class HipError(RuntimeError):
+    pass
+
+
+def check(status: int, what: str) -> None:
+    """Raise for a non-zero status: ValueError for bad arguments, MemoryError
+    for device OOM, RuntimeError otherwise."""
+    if status == 0:
+        return
+    msg = load().hdfe_last_error().decode(errors="replace")
+    text = f"{what}: {STATUS.get(status, status)}: {msg}"
+    if status == 1:
+        raise ValueError(text)
+    if status == 3:
+        raise MemoryError(text)
+    raise HipError(text)
+
+
+def require_device() -> str:
+    """The device name; RuntimeError when no HIP device is visible."""
+    lib = load()
+    name = C.create_string_buffer(256)
+    free = C.c_int64(0)
+    total = C.c_int64(0)
+    st = lib.hdfe_device_info(name, 256, C.byref(free), C.byref(total))
+    if st == 4:
+        raise RuntimeError("device='rocm': no HIP device visible")
+    check(st, "hdfe_device_info")
+    return name.value.decode()
+
+
+def device_memory() -> tuple[int, int]:
+    lib = load()
+    free = C.c_int64(0)
+    total = C.c_int64(0)
+    check(lib.hdfe_device_info(None, 0, C.byref(free), C.byref(total)), "hdfe_device_info")
+    return int(free.value), int(total.value)

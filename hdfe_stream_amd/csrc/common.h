@@ -1,0 +1,77 @@
+// Shared helpers for the hdfe-stream HIP library (gfx950, wave64).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <string>
+
+namespace hdfe {
+
+// Last error text, per thread, for hdfe_last_error().
+void set_error(const std::string &msg);
+const char *get_error();
+
+#define HDFE_STATUS_OK 0
+#define HDFE_STATUS_ARG 1
+#define HDFE_STATUS_HIP 2
+#define HDFE_STATUS_NOMEM 3
+#define HDFE_STATUS_NODEV 4
+
+// Evaluate a HIP call; on failure record the error and return HDFE_STATUS_HIP
+// (HDFE_STATUS_NOMEM for allocation failures) from the enclosing function.
+#define HIP_TRY(expr)                                                                   \
+    do {                                                                                \
+        hipError_t _e = (expr);                                                         \
+        if (_e != hipSuccess) {                                                         \
+            hdfe::set_error(std::string(#expr) + ": " + hipGetErrorString(_e));         \
+            return _e == hipErrorOutOfMemory ? HDFE_STATUS_NOMEM : HDFE_STATUS_HIP;     \
+        }                                                                               \
+    } while (0)
+
+#define ARG_CHECK(cond, msg)                                                            \
+    do {                                                                                \
+        if (!(cond)) {                                                                  \
+            hdfe::set_error(msg);                                                       \
+            return HDFE_STATUS_ARG;                                                     \
+        }                                                                               \
+    } while (0)
+
+constexpr int kWave = 64;
+
+// Sum over the 64 lanes of a wavefront (butterfly: every lane gets the sum).
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+    return v;
+}
+
+// Sum over aligned groups of G lanes (G a power of two <= 64).
+template <int G, typename T>
+__device__ __forceinline__ T group_sum(T v) {
+#pragma unroll
+    for (int o = G / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+    return v;
+}
+
+// Block-wide sum of one value per thread, fixed tree (deterministic).
+// `lds` must hold blockDim.x / 64 values.  Result valid in every thread.
+template <typename T>
+__device__ __forceinline__ T block_sum(T v, T *lds) {
+    v = wave_sum(v);
+    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+    const int nw = blockDim.x / kWave;
+    __syncthreads();
+    if (lane == 0) lds[w] = v;
+    __syncthreads();
+    T s = T(0);
+    for (int i = 0; i < nw; ++i) s += lds[i];
+    return s;
+}
+
+// Row range of part `w` of `nparts` equal contiguous parts of [0, n).
+__host__ __device__ __forceinline__ int64_t part_begin(int64_t n, int64_t w, int64_t nparts) {
+    return (n * w) / nparts;
+}
+
+}  // namespace hdfe

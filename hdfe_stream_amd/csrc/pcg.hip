@@ -1,0 +1,1021 @@
+// Jacobi-preconditioned block CG on an explicit CSR matrix, for gfx950.
+//
+// Replaces the reference's device solve (core/gpu.py:186-253, explicit_solver
+// and pcg) and its CPU twin (core/solve.py:66-140, _SolveMixin._pcg).  The
+// algorithm is the reference's, column by column in lockstep: zero columns are
+// solved by 0, a warm start x0 may already solve a column, each column stops
+// when ||r|| <= tol ||b||, alpha is guarded by pAp > 0 and beta by rz != 0.
+//
+// Layout: every (L, k) block is row major, the reference's C order, so the
+// gather of one neighbour row in the SpMM reads k contiguous values.
+//
+// One iteration is three or four launches:
+//   [long rows]  rows longer than kLongThresh (the levels of small dimensions,
+//                e.g. years, which touch every firm) are split in chunks of
+//                kLongChunk nonzeros over many workgroups; the last workgroup
+//                adds the chunks of each row in a fixed order.
+//   spmm_dot     AP = S P for the other rows, with the pAp column sums fused;
+//                the last workgroup to finish forms alpha.
+//   step         X += alpha P, R -= alpha AP, with the sums of r^2 and r^2 dinv;
+//                the last workgroup forms rel, the active mask, beta, and the
+//                stop flag.
+//   direction    P = dinv R + beta P.
+// Column sums are per-workgroup partials over fixed row ranges added in a
+// fixed order by the last workgroup, so results do not depend on scheduling.
+// The host runs iterations in batches and reads the stop flag once per batch;
+// kernels launched after the stop return at once.
+//
+// precision 1 ("mixed"): the CG above runs in FP32 (values, vectors, SpMM)
+// with FP64 column sums, as the inner solver of an iterative refinement whose
+// residual b - S x is recomputed in FP64 with the FP64 matrix; columns stop
+// when that true FP64 residual meets tol.
+#include "common.h"
+
+#include <algorithm>
+#include <mutex>
+#include <vector>
+
+namespace hdfe {
+
+constexpr int kThreads = 256;
+constexpr int kLongThresh = 4096;  // nonzeros: rows above this are split
+constexpr int kLongChunk = 2048;   // nonzeros per workgroup of a split row
+constexpr int kMaxK = 1024;        // widest block of right-hand sides per call
+constexpr int kSmallK = 8;         // k <= 8: lanes over nonzeros; wider: over columns
+
+// Device-side solver state (one allocation; see State layout in Work).
+struct CgState {
+    double *rz, *alpha, *beta, *bnorm, *rel;  // (k) each
+    int *active;                              // (k)
+    int *done, *it, *it_final;                // scalars
+    unsigned *counter;                        // 4 last-block tickets
+    int k, maxiter;
+    double tol;
+};
+
+// ----------------------------------------------------------------- kernels
+
+// Grid-wide "last workgroup" detection: every workgroup publishes its writes
+// and takes a ticket; the one holding the last ticket sees all partials.
+__device__ __forceinline__ bool last_block(unsigned *counter) {
+    __shared__ bool is_last;
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned t = atomicAdd(counter, 1u);
+        is_last = (t == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (is_last) __threadfence();
+    return is_last;
+}
+
+// Sum part[w * k + j] over w < nw, for every column j, in a fixed order;
+// result through `out` (k values).  Called by one whole workgroup.
+__device__ void reduce_parts(const double *part, int nw, int k, int stride, double *out) {
+    for (int j = threadIdx.x; j < k; j += blockDim.x) {
+        double s = 0.0;
+        for (int w = 0; w < nw; ++w) s += part[(int64_t)w * stride + j];
+        out[j] = s;
+    }
+}
+
+// Split rows: partial sums of chunk c of long row lrow[c], then the last
+// workgroup adds each row's chunks.  Small k: threads over nonzeros.
+template <typename T, int K>
+__global__ __launch_bounds__(kThreads) void k_long_small(
+    const int *__restrict__ indptr, const int *__restrict__ indices, const T *__restrict__ val,
+    const int *__restrict__ chunk_row, const int *__restrict__ chunk_beg, int nchunk,
+    const int *__restrict__ long_rows, const int *__restrict__ long_cptr, int nlong,
+    const T *__restrict__ P, T *__restrict__ AP, double *__restrict__ lpart, CgState st) {
+    if (st.done && *st.done) return;
+    __shared__ double lds[kThreads / kWave];
+    const int c = blockIdx.x;
+    const int row = chunk_row[c];
+    const int s = chunk_beg[c], e = min(s + kLongChunk, indptr[row + 1]);
+    T acc[K];
+#pragma unroll
+    for (int j = 0; j < K; ++j) acc[j] = T(0);
+    for (int p = s + threadIdx.x; p < e; p += blockDim.x) {
+        const int col = indices[p];
+        const T v = val[p];
+#pragma unroll
+        for (int j = 0; j < K; ++j) acc[j] += v * P[(int64_t)col * K + j];
+    }
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        double t = block_sum<double>((double)acc[j], lds);
+        if (threadIdx.x == 0) lpart[(int64_t)c * K + j] = t;
+    }
+    if (!last_block(st.counter + 3)) return;
+    for (int t = threadIdx.x; t < nlong * K; t += blockDim.x) {
+        const int l = t / K, j = t % K;
+        double s2 = 0.0;
+        for (int q = long_cptr[l]; q < long_cptr[l + 1]; ++q) s2 += lpart[(int64_t)q * K + j];
+        AP[(int64_t)long_rows[l] * K + j] = (T)s2;
+    }
+    if (threadIdx.x == 0) st.counter[3] = 0;
+}
+
+// Split rows, wide k: waves over nonzeros, lanes over columns.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_long_wide(
+    const int *__restrict__ indptr, const int *__restrict__ indices, const T *__restrict__ val,
+    const int *__restrict__ chunk_row, const int *__restrict__ chunk_beg, int nchunk,
+    const int *__restrict__ long_rows, const int *__restrict__ long_cptr, int nlong,
+    const T *__restrict__ P, T *__restrict__ AP, double *__restrict__ lpart, CgState st) {
+    if (st.done && *st.done) return;
+    extern __shared__ double wlds[];  // [4][k]
+    const int k = st.k;
+    const int c = blockIdx.x;
+    const int row = chunk_row[c];
+    const int s = chunk_beg[c], e = min(s + kLongChunk, indptr[row + 1]);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int j0 = 0; j0 < k; j0 += 64) {
+        const int j = j0 + lane;
+        T acc = T(0);
+        if (j < k)
+            for (int p = s + wv; p < e; p += 4) acc += val[p] * P[(int64_t)indices[p] * k + j];
+        if (j < k) wlds[wv * k + j] = (double)acc;
+    }
+    __syncthreads();
+    for (int j = threadIdx.x; j < k; j += blockDim.x)
+        lpart[(int64_t)c * k + j] = wlds[j] + wlds[k + j] + wlds[2 * k + j] + wlds[3 * k + j];
+    if (!last_block(st.counter + 3)) return;
+    for (int t = threadIdx.x; t < nlong * k; t += blockDim.x) {
+        const int l = t / k, j = t % k;
+        double s2 = 0.0;
+        for (int q = long_cptr[l]; q < long_cptr[l + 1]; ++q) s2 += lpart[(int64_t)q * k + j];
+        AP[(int64_t)long_rows[l] * k + j] = (T)s2;
+    }
+    if (threadIdx.x == 0) st.counter[3] = 0;
+}
+
+// After the pAp sums: alpha = rz / pAp where active and pAp > 0, else 0
+// (core/solve.py:120).  One workgroup.
+__device__ void finish_alpha(const double *part, int nw, CgState st) {
+    __shared__ double tmp[kMaxK];
+    reduce_parts(part, nw, st.k, st.k, tmp);
+    __syncthreads();
+    for (int j = threadIdx.x; j < st.k; j += blockDim.x) {
+        const double pAp = tmp[j];
+        st.alpha[j] = (st.active[j] && pAp > 0.0) ? st.rz[j] / pAp : 0.0;
+    }
+    if (threadIdx.x == 0) st.counter[0] = 0;
+}
+
+// AP = S P (rows up to kLongThresh nonzeros; split rows come from the long
+// kernel) with part[w, j] = sum over the workgroup's rows of P AP.
+// Small k: G lanes per row, each over a stride of the row's nonzeros.
+template <typename T, int K, int G>
+__global__ __launch_bounds__(kThreads) void k_spmm_small(
+    const int *__restrict__ indptr, const int *__restrict__ indices, const T *__restrict__ val,
+    int64_t L, const T *__restrict__ P, T *__restrict__ AP, double *__restrict__ part,
+    int with_dot, CgState st) {
+    if (st.done && *st.done) return;
+    __shared__ double lds[kThreads / kWave];
+    const int w = blockIdx.x, nw = gridDim.x;
+    const int64_t r0 = part_begin(L, w, nw), r1 = part_begin(L, w + 1, nw);
+    const int sub = threadIdx.x % G, slot = threadIdx.x / G, nslot = blockDim.x / G;
+    double dot[K];
+#pragma unroll
+    for (int j = 0; j < K; ++j) dot[j] = 0.0;
+    for (int64_t base = r0; base < r1; base += nslot) {
+        const int64_t row = base + slot;
+        const bool ok = row < r1;
+        int s = 0, e = 0;
+        if (ok) {
+            s = indptr[row];
+            e = indptr[row + 1];
+        }
+        const bool is_long = (e - s) > kLongThresh;
+        T acc[K];
+#pragma unroll
+        for (int j = 0; j < K; ++j) acc[j] = T(0);
+        if (ok && !is_long) {
+            for (int p = s + sub; p < e; p += G) {
+                const int col = indices[p];
+                const T v = val[p];
+#pragma unroll
+                for (int j = 0; j < K; ++j) acc[j] += v * P[(int64_t)col * K + j];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < K; ++j) acc[j] = group_sum<G>(acc[j]);
+        if (ok && sub == 0) {
+#pragma unroll
+            for (int j = 0; j < K; ++j) {
+                T ap = acc[j];
+                if (is_long) ap = AP[row * K + j];
+                else AP[row * K + j] = ap;
+                dot[j] += (double)P[row * K + j] * (double)ap;
+            }
+        }
+    }
+    if (!with_dot) return;
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        double t = block_sum<double>(dot[j], lds);
+        if (threadIdx.x == 0) part[(int64_t)w * K + j] = t;
+    }
+    if (!last_block(st.counter + 0)) return;
+    finish_alpha(part, nw, st);
+}
+
+// Wide k: one wave per row, lanes over columns (j = lane + 64 t, t < 16).
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_spmm_wide(
+    const int *__restrict__ indptr, const int *__restrict__ indices, const T *__restrict__ val,
+    int64_t L, const T *__restrict__ P, T *__restrict__ AP, double *__restrict__ part,
+    int with_dot, CgState st) {
+    if (st.done && *st.done) return;
+    extern __shared__ double wlds[];  // [4][k]
+    const int k = st.k;
+    const int w = blockIdx.x, nw = gridDim.x;
+    const int64_t r0 = part_begin(L, w, nw), r1 = part_begin(L, w + 1, nw);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    constexpr int CPL = kMaxK / 64;
+    double dot[CPL];
+#pragma unroll
+    for (int t = 0; t < CPL; ++t) dot[t] = 0.0;
+    for (int64_t row = r0 + wv; row < r1; row += 4) {
+        const int s = indptr[row], e = indptr[row + 1];
+        const bool is_long = (e - s) > kLongThresh;
+#pragma unroll
+        for (int t = 0; t < CPL; ++t) {
+            const int j = lane + 64 * t;
+            if (64 * t >= k) break;
+            T acc = T(0);
+            if (!is_long && j < k)
+                for (int p = s; p < e; ++p) acc += val[p] * P[(int64_t)indices[p] * k + j];
+            if (j < k) {
+                if (is_long) acc = AP[row * k + j];
+                else AP[row * k + j] = acc;
+                dot[t] += (double)P[row * k + j] * (double)acc;
+            }
+        }
+    }
+    if (!with_dot) return;
+#pragma unroll
+    for (int t = 0; t < CPL; ++t) {
+        const int j = lane + 64 * t;
+        if (64 * t >= k) break;
+        if (j < k) wlds[wv * k + j] = dot[t];
+    }
+    __syncthreads();
+    for (int j = threadIdx.x; j < k; j += blockDim.x)
+        part[(int64_t)w * k + j] = wlds[j] + wlds[k + j] + wlds[2 * k + j] + wlds[3 * k + j];
+    if (!last_block(st.counter + 0)) return;
+    finish_alpha(part, nw, st);
+}
+
+// Column-blocked mapping for the vector kernels: TJ threads across columns,
+// TR = blockDim / TJ across rows; a thread owns columns tj + TJ c, c < CJ.
+struct ColMap {
+    int TJ, TR, CJ;
+};
+__host__ __device__ inline ColMap col_map(int k) {
+    int tj = 1;
+    while (tj < k && tj < kThreads) tj <<= 1;
+    ColMap m;
+    m.TJ = tj;
+    m.TR = kThreads / tj;
+    m.CJ = (k + tj - 1) / tj;
+    return m;
+}
+constexpr int kMaxCJ = kMaxK / kThreads;
+
+// Per-column sums over the workgroup's rows, written as part[w * stride + j].
+__device__ void store_col_parts(double (&acc)[kMaxCJ], ColMap m, int k, double *part,
+                                int64_t stride) {
+    __shared__ double lds[kThreads];
+    const int tj = threadIdx.x % m.TJ, tr = threadIdx.x / m.TJ;
+    for (int c = 0; c < m.CJ; ++c) {
+        __syncthreads();
+        lds[tr * m.TJ + tj] = acc[c];
+        __syncthreads();
+        const int j = tj + m.TJ * c;
+        if (tr == 0 && j < k) {
+            double s = 0.0;
+            for (int r = 0; r < m.TR; ++r) s += lds[r * m.TJ + tj];
+            part[(int64_t)blockIdx.x * stride + j] = s;
+        }
+    }
+}
+
+// X += alpha P; R -= alpha AP; sums of r^2 and r^2 dinv (core/kernels_base.py:444-462).
+// The last workgroup: rel = sqrt(rr) / bnorm, active = rel > tol, stop when no
+// column is active or at maxiter; beta = rz_new / rz (core/solve.py:123-139).
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_step(
+    int64_t L, T *__restrict__ X, T *__restrict__ R, const T *__restrict__ P,
+    const T *__restrict__ AP, const T *__restrict__ dinv, double *__restrict__ part, CgState st) {
+    if (*st.done) return;
+    const int k = st.k;
+    const ColMap m = col_map(k);
+    const int tj = threadIdx.x % m.TJ, tr = threadIdx.x / m.TJ;
+    const int w = blockIdx.x, nw = gridDim.x;
+    const int64_t r0 = part_begin(L, w, nw), r1 = part_begin(L, w + 1, nw);
+    double rr[kMaxCJ], rz[kMaxCJ];
+    for (int c = 0; c < kMaxCJ; ++c) rr[c] = rz[c] = 0.0;
+    for (int c = 0; c < m.CJ; ++c) {
+        const int j = tj + m.TJ * c;
+        if (j >= k) break;
+        const T a = (T)st.alpha[j];
+        for (int64_t i = r0 + tr; i < r1; i += m.TR) {
+            const int64_t e = i * k + j;
+            X[e] += a * P[e];
+            const T r = R[e] - a * AP[e];
+            R[e] = r;
+            const double r2 = (double)r * (double)r;
+            rr[c] += r2;
+            rz[c] += r2 * (double)dinv[i];
+        }
+    }
+    store_col_parts(rr, m, k, part, 2 * (int64_t)k);
+    store_col_parts(rz, m, k, part + k, 2 * (int64_t)k);
+    if (!last_block(st.counter + 1)) return;
+    __shared__ double s_rr[kMaxK];
+    __shared__ double s_rz[kMaxK];
+    __shared__ int any;
+    if (threadIdx.x == 0) any = 0;
+    reduce_parts(part, nw, k, 2 * k, s_rr);
+    reduce_parts(part + k, nw, k, 2 * k, s_rz);
+    __syncthreads();
+    for (int j = threadIdx.x; j < k; j += blockDim.x) {
+        const double rel = sqrt(s_rr[j]) / st.bnorm[j];
+        st.rel[j] = rel;
+        const int act = rel > st.tol;
+        st.active[j] = act;
+        if (act) atomicOr(&any, 1);
+    }
+    __syncthreads();
+    const int it = *st.it + 1;
+    if (any && it < st.maxiter) {
+        for (int j = threadIdx.x; j < k; j += blockDim.x) {
+            const double rzn = s_rz[j], rzo = st.rz[j];
+            st.beta[j] = st.active[j] ? rzn / (rzo != 0.0 ? rzo : 1.0) : 0.0;
+            st.rz[j] = rzn;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        *st.it = it;
+        if (!any || it >= st.maxiter) {
+            *st.done = any ? 2 : 1;  // 1 converged, 2 hit maxiter
+            *st.it_final = it;
+        }
+        st.counter[1] = 0;
+    }
+}
+
+// P = dinv R + beta P (core/kernels_base.py:465-474).
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_direction(int64_t L, T *__restrict__ P,
+                                                        const T *__restrict__ R,
+                                                        const T *__restrict__ dinv, CgState st) {
+    if (*st.done) return;
+    const int k = st.k;
+    const int64_t n = L * k;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = e / k;
+        const int j = (int)(e - i * k);
+        P[e] = dinv[i] * R[e] + (T)st.beta[j] * P[e];
+    }
+}
+
+// out[j] = sum_i A_ij B_ij (d_i if d), fixed-order; mode 1 also stores
+// sqrt(out) with 0 -> 1 (the column norms of b, core/solve.py:81-82).
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_colsum(int64_t L, int k, const T *__restrict__ A,
+                                                     const T *__restrict__ B,
+                                                     const T *__restrict__ d,
+                                                     double *__restrict__ part,
+                                                     double *__restrict__ out, int mode,
+                                                     unsigned *counter) {
+    const ColMap m = col_map(k);
+    const int tj = threadIdx.x % m.TJ, tr = threadIdx.x / m.TJ;
+    const int w = blockIdx.x, nw = gridDim.x;
+    const int64_t r0 = part_begin(L, w, nw), r1 = part_begin(L, w + 1, nw);
+    double acc[kMaxCJ];
+    for (int c = 0; c < kMaxCJ; ++c) acc[c] = 0.0;
+    for (int c = 0; c < m.CJ; ++c) {
+        const int j = tj + m.TJ * c;
+        if (j >= k) break;
+        for (int64_t i = r0 + tr; i < r1; i += m.TR) {
+            const int64_t e = i * k + j;
+            double v = (double)A[e] * (double)B[e];
+            if (d) v *= (double)d[i];
+            acc[c] += v;
+        }
+    }
+    store_col_parts(acc, m, k, part, k);
+    if (!last_block(counter)) return;
+    __shared__ double tmp[kMaxK];
+    reduce_parts(part, nw, k, k, tmp);
+    __syncthreads();
+    for (int j = threadIdx.x; j < k; j += blockDim.x) {
+        double v = tmp[j];
+        if (mode == 1) {
+            v = sqrt(v);
+            if (v == 0.0) v = 1.0;
+        }
+        out[j] = v;
+    }
+    if (threadIdx.x == 0) *counter = 0;
+}
+
+// Elementwise helpers.
+template <typename T>
+__global__ void k_set_start(int64_t L, int k, const double *__restrict__ b,
+                            const double *__restrict__ Sx, const int *__restrict__ zero,
+                            T *__restrict__ X, T *__restrict__ R, T *__restrict__ P,
+                            const T *__restrict__ dinv) {
+    // R = b - Sx (Sx may be null: R = b); zero columns get X = R = 0; P = dinv R.
+    const int64_t n = L * k;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = e / k;
+        const int j = (int)(e - i * k);
+        double r = b[e] - (Sx ? Sx[e] : 0.0);
+        if (zero[j]) {
+            r = 0.0;
+            X[e] = T(0);
+        }
+        R[e] = (T)r;
+        P[e] = dinv[i] * (T)r;
+    }
+}
+
+__global__ void k_cast_f64_to_f32(int64_t n, const double *__restrict__ a, float *__restrict__ b) {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n;
+         e += (int64_t)gridDim.x * blockDim.x)
+        b[e] = (float)a[e];
+}
+
+// Mixed precision: R32 = r64 / s_j for active columns (0 otherwise), X32 = 0,
+// P32 = dinv R32.  s_j = ||r64_j|| keeps the FP32 inner system well scaled.
+__global__ void k_inner_start(int64_t L, int k, const double *__restrict__ r64,
+                              const double *__restrict__ scale, const int *__restrict__ act,
+                              float *__restrict__ X, float *__restrict__ R, float *__restrict__ P,
+                              const float *__restrict__ dinv) {
+    const int64_t n = L * k;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = e / k;
+        const int j = (int)(e - i * k);
+        const float r = act[j] ? (float)(r64[e] / scale[j]) : 0.0f;
+        X[e] = 0.0f;
+        R[e] = r;
+        P[e] = dinv[i] * r;
+    }
+}
+
+// x64 += s_j X32 (active columns); r64 = b - Sx64 after the caller's SpMM.
+__global__ void k_refine_update(int64_t L, int k, double *__restrict__ x64,
+                                const float *__restrict__ X, const double *__restrict__ scale,
+                                const int *__restrict__ act) {
+    const int64_t n = L * k;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int j = (int)(e % k);
+        if (act[j]) x64[e] += scale[j] * (double)X[e];
+    }
+}
+
+__global__ void k_sub(int64_t n, const double *__restrict__ b, const double *__restrict__ Sx,
+                      double *__restrict__ r) {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n;
+         e += (int64_t)gridDim.x * blockDim.x)
+        r[e] = b[e] - Sx[e];
+}
+
+__global__ void k_zero_cols(int64_t L, int k, const int *__restrict__ zero, double *__restrict__ X) {
+    const int64_t n = L * k;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n;
+         e += (int64_t)gridDim.x * blockDim.x)
+        if (zero[e % k]) X[e] = 0.0;
+}
+
+}  // namespace hdfe
+
+// ====================================================================== host
+
+namespace hdfe {
+
+static int grid_for(int64_t n, int64_t per, int lo, int hi) {
+    int64_t g = (n + per - 1) / per;
+    return (int)std::max<int64_t>(lo, std::min<int64_t>(hi, g));
+}
+
+// Work buffers, cached on the matrix for the last block width used.
+struct Work {
+    int k = 0;
+    int64_t L = 0;
+    double *X64 = nullptr, *R64 = nullptr, *P64 = nullptr, *AP64 = nullptr, *B64 = nullptr;
+    float *X32 = nullptr, *R32 = nullptr, *P32 = nullptr, *AP32 = nullptr;
+    double *part = nullptr, *lpart = nullptr, *colbuf = nullptr;
+    double *sd = nullptr;  // rz alpha beta bnorm rel scale : 6k
+    int *si = nullptr;     // active(k) zero(k) outer_active(k) done it it_final
+    unsigned *cnt = nullptr;
+    void release() {
+        void *ps[] = {X64, R64, P64, AP64, B64, X32, R32, P32, AP32, part, lpart, colbuf, sd, si, cnt};
+        for (void *p : ps)
+            if (p) (void)hipFree(p);
+        *this = Work();
+    }
+};
+
+struct Matrix {
+    int device = 0;
+    int64_t L = 0, nnz = 0;
+    int *indptr = nullptr, *indices = nullptr;
+    double *val64 = nullptr, *dinv64 = nullptr;
+    float *val32 = nullptr, *dinv32 = nullptr;
+    int nlong = 0, nchunk = 0;
+    int *long_rows = nullptr, *long_cptr = nullptr, *chunk_row = nullptr, *chunk_beg = nullptr;
+    hipStream_t stream = nullptr;  // the solver's own stream
+    hipEvent_t ev = nullptr;
+    Work w;
+    std::mutex mu;
+};
+
+static int alloc_work(Matrix &M, int k) {
+    if (M.w.k == k && M.w.L == M.L) return HDFE_STATUS_OK;
+    M.w.release();
+    const int64_t Lk = std::max<int64_t>(1, M.L * (int64_t)k);
+    Work &W = M.w;
+    HIP_TRY(hipMalloc(&W.X64, Lk * 8));
+    HIP_TRY(hipMalloc(&W.R64, Lk * 8));
+    HIP_TRY(hipMalloc(&W.P64, Lk * 8));
+    HIP_TRY(hipMalloc(&W.AP64, Lk * 8));
+    HIP_TRY(hipMalloc(&W.B64, Lk * 8));
+    HIP_TRY(hipMalloc(&W.X32, Lk * 4));
+    HIP_TRY(hipMalloc(&W.R32, Lk * 4));
+    HIP_TRY(hipMalloc(&W.P32, Lk * 4));
+    HIP_TRY(hipMalloc(&W.AP32, Lk * 4));
+    HIP_TRY(hipMalloc(&W.part, (int64_t)4096 * 2 * k * 8));
+    HIP_TRY(hipMalloc(&W.lpart, std::max<int64_t>(1, (int64_t)M.nchunk * k) * 8));
+    HIP_TRY(hipMalloc(&W.colbuf, (int64_t)k * 8));
+    HIP_TRY(hipMalloc(&W.sd, (int64_t)6 * k * 8));
+    HIP_TRY(hipMalloc(&W.si, ((int64_t)3 * k + 8) * 4));
+    HIP_TRY(hipMalloc(&W.cnt, 8 * sizeof(unsigned)));
+    HIP_TRY(hipMemset(W.cnt, 0, 8 * sizeof(unsigned)));
+    HIP_TRY(hipMemset(W.sd, 0, (int64_t)6 * k * 8));
+    W.k = k;
+    W.L = M.L;
+    return HDFE_STATUS_OK;
+}
+
+static CgState make_state(Work &W, int k, double tol, int maxiter) {
+    CgState st;
+    st.rz = W.sd;
+    st.alpha = W.sd + k;
+    st.beta = W.sd + 2 * k;
+    st.bnorm = W.sd + 3 * k;
+    st.rel = W.sd + 4 * k;
+    st.active = W.si;
+    st.done = W.si + 3 * k;
+    st.it = W.si + 3 * k + 1;
+    st.it_final = W.si + 3 * k + 2;
+    st.counter = W.cnt;
+    st.k = k;
+    st.maxiter = maxiter;
+    st.tol = tol;
+    return st;
+}
+
+static int spmm_grid(int64_t L, int k) {
+    return k <= kSmallK ? grid_for(L, 256, 1, 2048) : grid_for(L, 128, 1, 512);
+}
+static int vec_grid(int64_t L, int k) {
+    return grid_for(L * (int64_t)k, 2048, 1, k <= kSmallK ? 2048 : 512);
+}
+
+// AP = S P (and, with with_dot, pAp sums and alpha).  `st.done` may be null.
+template <typename T>
+static void launch_spmm(Matrix &M, const T *val, const T *P, T *AP, double *part, double *lpart,
+                        int with_dot, CgState st, hipStream_t s) {
+    const int k = st.k;
+    if (M.nlong) {
+        if (k <= kSmallK) {
+#define HDFE_LONG(KK)                                                                         \
+    case KK:                                                                                  \
+        hipLaunchKernelGGL((k_long_small<T, KK>), dim3(M.nchunk), dim3(kThreads), 0, s,      \
+                           M.indptr, M.indices, val, M.chunk_row, M.chunk_beg, M.nchunk,      \
+                           M.long_rows, M.long_cptr, M.nlong, P, AP, lpart, st);             \
+        break;
+            switch (k) {
+                HDFE_LONG(1) HDFE_LONG(2) HDFE_LONG(3) HDFE_LONG(4)
+                HDFE_LONG(5) HDFE_LONG(6) HDFE_LONG(7) HDFE_LONG(8)
+            }
+#undef HDFE_LONG
+        } else {
+            hipLaunchKernelGGL((k_long_wide<T>), dim3(M.nchunk), dim3(kThreads), 4 * k * 8, s,
+                               M.indptr, M.indices, val, M.chunk_row, M.chunk_beg, M.nchunk,
+                               M.long_rows, M.long_cptr, M.nlong, P, AP, lpart, st);
+        }
+    }
+    const int g = spmm_grid(M.L, k);
+    if (k <= kSmallK) {
+#define HDFE_SPMM(KK)                                                                        \
+    case KK:                                                                                 \
+        hipLaunchKernelGGL((k_spmm_small<T, KK, 8>), dim3(g), dim3(kThreads), 0, s,         \
+                           M.indptr, M.indices, val, M.L, P, AP, part, with_dot, st);       \
+        break;
+        switch (k) {
+            HDFE_SPMM(1) HDFE_SPMM(2) HDFE_SPMM(3) HDFE_SPMM(4)
+            HDFE_SPMM(5) HDFE_SPMM(6) HDFE_SPMM(7) HDFE_SPMM(8)
+        }
+#undef HDFE_SPMM
+    } else {
+        hipLaunchKernelGGL((k_spmm_wide<T>), dim3(g), dim3(kThreads), 4 * k * 8, s, M.indptr,
+                           M.indices, val, M.L, P, AP, part, with_dot, st);
+    }
+}
+
+template <typename T>
+static void launch_colsum(int64_t L, int k, const T *A, const T *B, const T *d, double *part,
+                          double *out, int mode, unsigned *counter, hipStream_t s) {
+    hipLaunchKernelGGL((k_colsum<T>), dim3(vec_grid(L, k)), dim3(kThreads), 0, s, L, k, A, B, d,
+                       part, out, mode, counter);
+}
+
+// Iterate the CG already set up in (X, R, P, st.rz, st.active, st.bnorm);
+// returns the iterations run and whether every column converged.
+template <typename T>
+static int run_cg(Matrix &M, int k, const T *val, const T *dinv, T *X, T *R, T *P, T *AP,
+                  CgState st, hipStream_t s, int64_t *it_out, int *conv_out) {
+    Work &W = M.w;
+    HIP_TRY(hipMemsetAsync(st.done, 0, 3 * sizeof(int), s));
+    const int gv = vec_grid(M.L, k);
+    const int gd = grid_for(M.L * (int64_t)k, 1024, 1, 4096);
+    int host[2] = {0, 0};
+    int batch = 8;
+    for (int64_t launched = 0; launched < st.maxiter;) {
+        const int nb = (int)std::min<int64_t>(batch, st.maxiter - launched);
+        for (int b = 0; b < nb; ++b) {
+            launch_spmm<T>(M, val, P, AP, W.part, W.lpart, 1, st, s);
+            hipLaunchKernelGGL((k_step<T>), dim3(gv), dim3(kThreads), 0, s, M.L, X, R, P, AP, dinv,
+                               W.part, st);
+            hipLaunchKernelGGL((k_direction<T>), dim3(gd), dim3(kThreads), 0, s, M.L, P, R, dinv,
+                               st);
+        }
+        launched += nb;
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(host, st.done, sizeof(int), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipMemcpyAsync(host + 1, st.it_final, sizeof(int), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        if (host[0]) break;
+        if (batch < 64) batch *= 2;
+    }
+    if (!host[0]) {  // maxiter reached without the flag (maxiter == 0)
+        *it_out = 0;
+        *conv_out = 0;
+        return HDFE_STATUS_OK;
+    }
+    *it_out = host[1];
+    *conv_out = host[0] == 1;
+    return HDFE_STATUS_OK;
+}
+
+static int read_max(const double *dev, int k, hipStream_t s, double *out,
+                    std::vector<double> *all = nullptr) {
+    std::vector<double> h(k);
+    HIP_TRY(hipMemcpyAsync(h.data(), dev, k * 8, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    double m = 0.0;
+    for (double v : h) m = std::max(m, v);
+    *out = m;
+    if (all) *all = h;
+    return HDFE_STATUS_OK;
+}
+
+#define TRY(expr)                                  \
+    do {                                           \
+        int _st = (expr);                          \
+        if (_st != HDFE_STATUS_OK) return _st;     \
+    } while (0)
+
+// FP64 block PCG (core/solve.py:66-140).
+static int solve_f64(Matrix &M, int k, const uint8_t *zero_h, bool has_x0, double tol,
+                     int64_t maxiter, int64_t *it_out, int *conv_out, double *rel_out,
+                     hipStream_t s) {
+    Work &W = M.w;
+    CgState st = make_state(W, k, tol, (int)std::min<int64_t>(maxiter, INT32_MAX));
+    const int64_t Lk = M.L * (int64_t)k;
+    std::vector<int> zero(k), act(k);
+    bool all_zero = true;
+    for (int j = 0; j < k; ++j) {
+        zero[j] = zero_h ? zero_h[j] != 0 : 0;
+        act[j] = !zero[j];
+        all_zero &= (bool)zero[j];
+    }
+    int *zero_d = W.si + k;  // the zero mask lives after `active`
+    HIP_TRY(hipMemcpyAsync(zero_d, zero.data(), k * 4, hipMemcpyHostToDevice, s));
+    launch_colsum<double>(M.L, k, W.B64, W.B64, nullptr, W.part, st.bnorm, 1, W.cnt + 4, s);
+    const int ge = grid_for(Lk, 1024, 1, 4096);
+    CgState plain = st;
+    plain.done = nullptr;
+    if (has_x0) {
+        launch_spmm<double>(M, M.val64, W.X64, W.AP64, W.part, W.lpart, 0, plain, s);
+        hipLaunchKernelGGL(k_set_start<double>, dim3(ge), dim3(kThreads), 0, s, M.L, k, W.B64,
+                           W.AP64, zero_d, W.X64, W.R64, W.P64, M.dinv64);
+    } else {
+        HIP_TRY(hipMemsetAsync(W.X64, 0, Lk * 8, s));
+        hipLaunchKernelGGL(k_set_start<double>, dim3(ge), dim3(kThreads), 0, s, M.L, k, W.B64,
+                           (const double *)nullptr, zero_d, W.X64, W.R64, W.P64, M.dinv64);
+    }
+    HIP_TRY(hipGetLastError());
+    *it_out = 0;
+    *conv_out = 1;
+    *rel_out = 0.0;
+    if (all_zero) return HDFE_STATUS_OK;
+    if (has_x0) {
+        // rel = ||R|| / ||b||; columns already solved take no iterations
+        launch_colsum<double>(M.L, k, W.R64, W.R64, nullptr, W.part, W.colbuf, 0, W.cnt + 4, s);
+        std::vector<double> rr;
+        std::vector<double> bn(k);
+        double dummy;
+        TRY(read_max(W.colbuf, k, s, &dummy, &rr));
+        HIP_TRY(hipMemcpyAsync(bn.data(), st.bnorm, k * 8, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        bool any = false;
+        double rmax = 0.0;
+        for (int j = 0; j < k; ++j) {
+            double rel = std::sqrt(rr[j]) / bn[j];
+            rmax = std::max(rmax, rel);
+            act[j] = act[j] && rel > tol;
+            any |= (bool)act[j];
+        }
+        if (!any) {
+            *rel_out = rmax;
+            return HDFE_STATUS_OK;
+        }
+    }
+    HIP_TRY(hipMemcpyAsync(st.active, act.data(), k * 4, hipMemcpyHostToDevice, s));
+    launch_colsum<double>(M.L, k, W.R64, W.P64, nullptr, W.part, st.rz, 0, W.cnt + 4, s);
+    if (maxiter <= 0) {
+        *conv_out = 0;
+        return HDFE_STATUS_OK;
+    }
+    TRY(run_cg<double>(M, k, M.val64, M.dinv64, W.X64, W.R64, W.P64, W.AP64, st, s, it_out,
+                       conv_out));
+    return read_max(st.rel, k, s, rel_out);
+}
+
+// Mixed precision: FP32 CG inside FP64 iterative refinement.
+static int solve_mixed(Matrix &M, int k, const uint8_t *zero_h, bool has_x0, double tol,
+                       int64_t maxiter, double inner_tol, int64_t *it_out, int *conv_out,
+                       double *rel_out, int64_t *outer_out, hipStream_t s) {
+    Work &W = M.w;
+    const int64_t Lk = M.L * (int64_t)k;
+    std::vector<int> zero(k), act(k);
+    bool all_zero = true;
+    for (int j = 0; j < k; ++j) {
+        zero[j] = zero_h ? zero_h[j] != 0 : 0;
+        all_zero &= (bool)zero[j];
+    }
+    int *zero_d = W.si + k;
+    HIP_TRY(hipMemcpyAsync(zero_d, zero.data(), k * 4, hipMemcpyHostToDevice, s));
+    double *bnorm = W.sd + 3 * k, *scale = W.sd + 5 * k;
+    launch_colsum<double>(M.L, k, W.B64, W.B64, nullptr, W.part, bnorm, 1, W.cnt + 4, s);
+    std::vector<double> bn(k), rr(k), sc(k);
+    HIP_TRY(hipMemcpyAsync(bn.data(), bnorm, k * 8, hipMemcpyDeviceToHost, s));
+    const int ge = grid_for(Lk, 1024, 1, 4096);
+    if (!has_x0) HIP_TRY(hipMemsetAsync(W.X64, 0, Lk * 8, s));
+    hipLaunchKernelGGL(k_zero_cols, dim3(ge), dim3(kThreads), 0, s, M.L, k, zero_d, W.X64);
+    *it_out = 0;
+    *conv_out = 1;
+    *rel_out = 0.0;
+    *outer_out = 0;
+    if (all_zero) return HDFE_STATUS_OK;
+    *conv_out = 0;
+    int *outer_act = W.si + 2 * k;  // the refinement's own mask of open columns
+    CgState plain = make_state(W, k, tol, 0);
+    plain.done = nullptr;
+    int64_t total = 0;
+    bool first = true;
+    for (int outer = 0; outer < 64; ++outer) {
+        // the true residual, in FP64
+        if (first && !has_x0) {
+            HIP_TRY(hipMemcpyAsync(W.R64, W.B64, Lk * 8, hipMemcpyDeviceToDevice, s));
+        } else {
+            launch_spmm<double>(M, M.val64, W.X64, W.AP64, W.part, W.lpart, 0, plain, s);
+            hipLaunchKernelGGL(k_sub, dim3(ge), dim3(kThreads), 0, s, Lk, W.B64, W.AP64, W.R64);
+        }
+        first = false;
+        hipLaunchKernelGGL(k_zero_cols, dim3(ge), dim3(kThreads), 0, s, M.L, k, zero_d, W.R64);
+        launch_colsum<double>(M.L, k, W.R64, W.R64, nullptr, W.part, W.colbuf, 0, W.cnt + 4, s);
+        HIP_TRY(hipMemcpyAsync(rr.data(), W.colbuf, k * 8, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        bool any = false;
+        double rmax = 0.0;
+        for (int j = 0; j < k; ++j) {
+            const double rn = std::sqrt(rr[j]);
+            const double rel = rn / bn[j];
+            rmax = std::max(rmax, rel);
+            act[j] = !zero[j] && rel > tol;
+            any |= (bool)act[j];
+            sc[j] = rn > 0.0 ? rn : 1.0;
+        }
+        *rel_out = rmax;
+        if (!any) {
+            *conv_out = 1;
+            break;
+        }
+        if (total >= maxiter) {
+            *conv_out = 0;
+            break;
+        }
+        *outer_out = outer + 1;
+        // the FP32 inner solve of S d = r / ||r||
+        CgState st = make_state(W, k, inner_tol, (int)std::min<int64_t>(maxiter - total, INT32_MAX));
+        HIP_TRY(hipMemcpyAsync(scale, sc.data(), k * 8, hipMemcpyHostToDevice, s));
+        HIP_TRY(hipMemcpyAsync(st.active, act.data(), k * 4, hipMemcpyHostToDevice, s));
+        HIP_TRY(hipMemcpyAsync(outer_act, act.data(), k * 4, hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(k_inner_start, dim3(ge), dim3(kThreads), 0, s, M.L, k, W.R64, scale,
+                           st.active, W.X32, W.R32, W.P32, M.dinv32);
+        launch_colsum<float>(M.L, k, W.R32, W.R32, nullptr, W.part, st.bnorm, 1, W.cnt + 4, s);
+        launch_colsum<float>(M.L, k, W.R32, W.P32, nullptr, W.part, st.rz, 0, W.cnt + 4, s);
+        int64_t it = 0;
+        int conv = 0;
+        TRY(run_cg<float>(M, k, M.val32, M.dinv32, W.X32, W.R32, W.P32, W.AP32, st, s, &it, &conv));
+        total += std::max<int64_t>(it, 1);
+        hipLaunchKernelGGL(k_refine_update, dim3(ge), dim3(kThreads), 0, s, M.L, k, W.X64, W.X32,
+                           scale, outer_act);
+        HIP_TRY(hipGetLastError());
+    }
+    *it_out = total;
+    return HDFE_STATUS_OK;
+}
+
+}  // namespace hdfe
+
+// ================================================================== C ABI
+
+using namespace hdfe;
+
+extern "C" {
+
+typedef struct hdfe_solver hdfe_solver;
+
+int hdfe_csr_create(int64_t L, int64_t nnz, const int64_t *indptr, const int32_t *indices,
+                    const double *data, const double *dinv, int on_device, void **out) {
+    ARG_CHECK(out != nullptr, "hdfe_csr_create: null output handle");
+    ARG_CHECK(L >= 0 && nnz >= 0, "hdfe_csr_create: negative size");
+    ARG_CHECK(nnz < INT32_MAX && L < INT32_MAX,
+              "hdfe_csr_create: nnz and L must be below 2^31 (int32 CSR on the device)");
+    Matrix *M = new Matrix();
+    HIP_TRY(hipGetDevice(&M->device));
+    M->L = L;
+    M->nnz = nnz;
+    const hipMemcpyKind kind = on_device ? hipMemcpyDeviceToHost : hipMemcpyHostToHost;
+    std::vector<int64_t> ip(L + 1);
+    HIP_TRY(hipMemcpy(ip.data(), indptr, (L + 1) * 8, kind));
+    ARG_CHECK(ip[0] == 0 && ip[L] == nnz, "hdfe_csr_create: indptr must run from 0 to nnz");
+    std::vector<int> ip32(L + 1);
+    std::vector<int> lrows, lcptr(1, 0), crow, cbeg;
+    for (int64_t i = 0; i <= L; ++i) ip32[i] = (int)ip[i];
+    for (int64_t i = 0; i < L; ++i) {
+        ARG_CHECK(ip[i + 1] >= ip[i], "hdfe_csr_create: indptr must be non-decreasing");
+        const int64_t len = ip[i + 1] - ip[i];
+        if (len > kLongThresh) {
+            lrows.push_back((int)i);
+            for (int64_t p = ip[i]; p < ip[i + 1]; p += kLongChunk) {
+                crow.push_back((int)i);
+                cbeg.push_back((int)p);
+            }
+            lcptr.push_back((int)crow.size());
+        }
+    }
+    M->nlong = (int)lrows.size();
+    M->nchunk = (int)crow.size();
+    HIP_TRY(hipStreamCreateWithFlags(&M->stream, hipStreamNonBlocking));
+    HIP_TRY(hipEventCreateWithFlags(&M->ev, hipEventDisableTiming));
+    HIP_TRY(hipMalloc(&M->indptr, (L + 1) * 4));
+    HIP_TRY(hipMalloc(&M->indices, std::max<int64_t>(nnz, 1) * 4));
+    HIP_TRY(hipMalloc(&M->val64, std::max<int64_t>(nnz, 1) * 8));
+    HIP_TRY(hipMalloc(&M->val32, std::max<int64_t>(nnz, 1) * 4));
+    HIP_TRY(hipMalloc(&M->dinv64, std::max<int64_t>(L, 1) * 8));
+    HIP_TRY(hipMalloc(&M->dinv32, std::max<int64_t>(L, 1) * 4));
+    HIP_TRY(hipMemcpy(M->indptr, ip32.data(), (L + 1) * 4, hipMemcpyHostToDevice));
+    const hipMemcpyKind up = on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    if (nnz) {
+        HIP_TRY(hipMemcpy(M->indices, indices, nnz * 4, up));
+        HIP_TRY(hipMemcpy(M->val64, data, nnz * 8, up));
+    }
+    if (L) HIP_TRY(hipMemcpy(M->dinv64, dinv, L * 8, up));
+    const int g1 = grid_for(std::max<int64_t>(nnz, 1), 1024, 1, 4096);
+    hipLaunchKernelGGL(k_cast_f64_to_f32, dim3(g1), dim3(kThreads), 0, M->stream, nnz, M->val64,
+                       M->val32);
+    hipLaunchKernelGGL(k_cast_f64_to_f32, dim3(grid_for(std::max<int64_t>(L, 1), 1024, 1, 4096)),
+                       dim3(kThreads), 0, M->stream, L, M->dinv64, M->dinv32);
+    if (M->nlong) {
+        HIP_TRY(hipMalloc(&M->long_rows, M->nlong * 4));
+        HIP_TRY(hipMalloc(&M->long_cptr, (M->nlong + 1) * 4));
+        HIP_TRY(hipMalloc(&M->chunk_row, M->nchunk * 4));
+        HIP_TRY(hipMalloc(&M->chunk_beg, M->nchunk * 4));
+        HIP_TRY(hipMemcpy(M->long_rows, lrows.data(), M->nlong * 4, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(M->long_cptr, lcptr.data(), (M->nlong + 1) * 4, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(M->chunk_row, crow.data(), M->nchunk * 4, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(M->chunk_beg, cbeg.data(), M->nchunk * 4, hipMemcpyHostToDevice));
+    }
+    HIP_TRY(hipStreamSynchronize(M->stream));
+    *out = M;
+    return HDFE_STATUS_OK;
+}
+
+int hdfe_csr_info(void *h, int64_t *L, int64_t *nnz, int *nlong) {
+    ARG_CHECK(h != nullptr, "hdfe_csr_info: null handle");
+    Matrix *M = (Matrix *)h;
+    if (L) *L = M->L;
+    if (nnz) *nnz = M->nnz;
+    if (nlong) *nlong = M->nlong;
+    return HDFE_STATUS_OK;
+}
+
+int hdfe_pcg_solve(void *h, int64_t k, const double *b, const uint8_t *zero, const double *x0,
+                   double tol, int64_t maxiter, int precision, double inner_tol, double *X,
+                   int64_t *iterations, int *converged, double *max_rel, int64_t *outer,
+                   int on_device, void *stream) {
+    ARG_CHECK(h != nullptr, "hdfe_pcg_solve: null handle");
+    ARG_CHECK(k >= 1 && k <= kMaxK, "hdfe_pcg_solve: k must be in [1, 1024]");
+    ARG_CHECK(precision == 0 || precision == 1, "hdfe_pcg_solve: precision must be 0 or 1");
+    ARG_CHECK(b != nullptr && X != nullptr && iterations && converged && max_rel,
+              "hdfe_pcg_solve: null argument");
+    Matrix &M = *(Matrix *)h;
+    std::lock_guard<std::mutex> lock(M.mu);
+    HIP_TRY(hipSetDevice(M.device));
+    int st = alloc_work(M, (int)k);
+    if (st) return st;
+    hipStream_t s = M.stream;
+    hipStream_t caller = (hipStream_t)stream;
+    const int64_t Lk = M.L * k;
+    if (on_device) {  // order after the caller's stream
+        HIP_TRY(hipEventRecord(M.ev, caller));
+        HIP_TRY(hipStreamWaitEvent(s, M.ev, 0));
+    }
+    const hipMemcpyKind up = on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    if (Lk) HIP_TRY(hipMemcpyAsync(M.w.B64, b, Lk * 8, up, s));
+    if (x0 && Lk) HIP_TRY(hipMemcpyAsync(M.w.X64, x0, Lk * 8, up, s));
+    int64_t outer_n = 0;
+    if (precision == 0)
+        st = solve_f64(M, (int)k, zero, x0 != nullptr, tol, maxiter, iterations, converged, max_rel, s);
+    else
+        st = solve_mixed(M, (int)k, zero, x0 != nullptr, tol, maxiter,
+                         inner_tol > 0 ? inner_tol : 1e-5, iterations, converged, max_rel,
+                         &outer_n, s);
+    if (st) return st;
+    if (outer) *outer = outer_n;
+    const hipMemcpyKind down = on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+    if (Lk) HIP_TRY(hipMemcpyAsync(X, M.w.X64, Lk * 8, down, s));
+    if (on_device) {
+        HIP_TRY(hipEventRecord(M.ev, s));
+        HIP_TRY(hipStreamWaitEvent(caller, M.ev, 0));
+    } else {
+        HIP_TRY(hipStreamSynchronize(s));
+    }
+    return HDFE_STATUS_OK;
+}
+
+// y = S x (k columns, FP64), on the device; for tests of the operator.
+int hdfe_csr_matmat(void *h, int64_t k, const double *x, double *y, int on_device) {
+    ARG_CHECK(h != nullptr && x != nullptr && y != nullptr, "hdfe_csr_matmat: null argument");
+    ARG_CHECK(k >= 1 && k <= kMaxK, "hdfe_csr_matmat: k must be in [1, 1024]");
+    Matrix &M = *(Matrix *)h;
+    std::lock_guard<std::mutex> lock(M.mu);
+    HIP_TRY(hipSetDevice(M.device));
+    int st = alloc_work(M, (int)k);
+    if (st) return st;
+    hipStream_t s = M.stream;
+    const int64_t Lk = M.L * k;
+    const hipMemcpyKind up = on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    HIP_TRY(hipMemcpyAsync(M.w.P64, x, Lk * 8, up, s));
+    CgState plain = make_state(M.w, (int)k, 0.0, 0);
+    plain.done = nullptr;
+    launch_spmm<double>(M, M.val64, M.w.P64, M.w.AP64, M.w.part, M.w.lpart, 0, plain, s);
+    HIP_TRY(hipGetLastError());
+    const hipMemcpyKind down = on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+    HIP_TRY(hipMemcpyAsync(y, M.w.AP64, Lk * 8, down, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return HDFE_STATUS_OK;
+}
+
+void hdfe_destroy(void *h) {
+    if (!h) return;
+    Matrix *M = (Matrix *)h;
+    (void)hipSetDevice(M->device);
+    if (M->stream) (void)hipStreamSynchronize(M->stream);
+    M->w.release();
+    void *ps[] = {M->indptr, M->indices, M->val64, M->val32, M->dinv64, M->dinv32,
+                  M->long_rows, M->long_cptr, M->chunk_row, M->chunk_beg};
+    for (void *p : ps)
+        if (p) (void)hipFree(p);
+    if (M->ev) (void)hipEventDestroy(M->ev);
+    if (M->stream) (void)hipStreamDestroy(M->stream);
+    delete M;
+}
+
+}  // extern "C"

@@ -1,0 +1,181 @@
+"""The `device="rocm"` solve backend: the MI355X counterpart of the reference's
+CuPy plugin (hdfe_stream/core/gpu.py:33-51 `cuda_backend`, :186-196
+`explicit_solver`, :199-253 `pcg`).
+
+Same contract as the reference:
+  explicit_solver(S, dinv, tol, maxiter, debug=None) -> (solve, device name)
+  solve(b, zero=None, x0=None) -> (X, iterations, converged, max_rel)
+S (a scipy CSR, or device arrays) and dinv go to the device once; every call
+of `solve` reuses them.  b, x0 and X are host float64 (L, k) arrays, or torch
+tensors already on the device (then X comes back on the device, ordered on
+torch's current stream).  Non-convergence is reported, not raised.
+
+Beyond the reference: `precision="mixed"` runs FP32 CG inside FP64 iterative
+refinement (the residual b - S x recomputed in FP64), stopping on the true
+FP64 residual, so the tolerance means the same thing in both modes.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+from types import SimpleNamespace
+
+import numpy as np
+
+from . import _lib
+
+PRECISIONS = {"fp64": 0, "mixed": 1}
+
+
+def rocm_backend():
+    """What the device solver runs on: the HIP library and the device.
+    ImportError without the built library, RuntimeError without a device
+    (the reference's cuda_backend contract, core/gpu.py:33-51)."""
+    lib = _lib.load()
+    name = _lib.require_device()
+    return SimpleNamespace(
+        lib=lib, name=name,
+        free_bytes=lambda: _lib.device_memory()[0],
+        release=lambda: None,
+        csr=lambda S: DeviceCSR.from_scipy(S, None),
+    )
+
+
+def _is_torch(a) -> bool:
+    return type(a).__module__.startswith("torch")
+
+
+class DeviceCSR:
+    """An explicit S (and its Jacobi inverse diagonal) resident on the device."""
+
+    def __init__(self, handle, L: int, nnz: int):
+        self._h = handle
+        self.L = L
+        self.nnz = nnz
+
+    @classmethod
+    def from_scipy(cls, S, dinv):
+        S = S.tocsr()
+        L = S.shape[0]
+        indptr = np.ascontiguousarray(S.indptr, np.int64)
+        indices = np.ascontiguousarray(S.indices, np.int32)
+        data = np.ascontiguousarray(S.data, np.float64)
+        d = np.ascontiguousarray(np.zeros(L) if dinv is None else dinv, np.float64)
+        return cls._create(L, indptr, indices, data, d, 0)
+
+    @classmethod
+    def from_device(cls, L, indptr, indices, data, dinv):
+        """From torch device tensors: indptr int64 (L+1), indices int32,
+        data float64, dinv float64 (L)."""
+        return cls._create(L, indptr, indices, data, dinv, 1)
+
+    @classmethod
+    def _create(cls, L, indptr, indices, data, dinv, on_device):
+        lib = _lib.load()
+        h = C.c_void_p()
+        ptr = (lambda a: a.data_ptr()) if on_device else (lambda a: a.ctypes.data)
+        nnz = int(indices.shape[0]) if on_device else int(indptr[-1])
+        _lib.check(lib.hdfe_csr_create(L, nnz, ptr(indptr), ptr(indices), ptr(data), ptr(dinv),
+                                       on_device, C.byref(h)), "hdfe_csr_create")
+        return cls(h, L, nnz)
+
+    def nlong(self) -> int:
+        n = C.c_int(0)
+        _lib.check(_lib.load().hdfe_csr_info(self._h, None, None, C.byref(n)), "hdfe_csr_info")
+        return int(n.value)
+
+    def matmat(self, X):
+        """S X on the device (FP64)."""
+        lib = _lib.load()
+        if _is_torch(X):
+            import torch
+            X = X.contiguous()
+            k = X.shape[1] if X.dim() == 2 else 1
+            Y = torch.empty_like(X)
+            torch.cuda.current_stream().synchronize()
+            _lib.check(lib.hdfe_csr_matmat(self._h, k, X.data_ptr(), Y.data_ptr(), 1),
+                       "hdfe_csr_matmat")
+            return Y
+        X = np.ascontiguousarray(X, np.float64)
+        k = X.shape[1] if X.ndim == 2 else 1
+        Y = np.empty_like(X)
+        _lib.check(lib.hdfe_csr_matmat(self._h, k, X.ctypes.data, Y.ctypes.data, 0),
+                   "hdfe_csr_matmat")
+        return Y
+
+    def solve(self, b, zero=None, x0=None, tol=1e-10, maxiter=5000, precision="fp64",
+              inner_tol=1e-5, info=None):
+        """(X, iterations, converged, max_rel); see the module docstring."""
+        if precision not in PRECISIONS:
+            raise ValueError(f"precision must be one of {sorted(PRECISIONS)}")
+        lib = _lib.load()
+        dev = _is_torch(b)
+        if dev:
+            import torch
+            B = b.contiguous().to(torch.float64)
+            L, k = B.shape[0], (B.shape[1] if B.dim() == 2 else 1)
+            X = torch.empty_like(B)
+            X0 = None if x0 is None else x0.contiguous().to(torch.float64)
+            bp, xp, x0p = B.data_ptr(), X.data_ptr(), (None if X0 is None else X0.data_ptr())
+            stream = torch.cuda.current_stream().cuda_stream
+        else:
+            B = np.ascontiguousarray(np.asarray(b, np.float64))
+            L, k = B.shape[0], (B.shape[1] if B.ndim == 2 else 1)
+            X = np.empty_like(B)
+            X0 = None if x0 is None else np.ascontiguousarray(np.asarray(x0, np.float64))
+            bp, xp, x0p = B.ctypes.data, X.ctypes.data, (None if X0 is None else X0.ctypes.data)
+            stream = None
+        if L != self.L:
+            raise ValueError(f"b has {L} rows; S is {self.L} x {self.L}")
+        z = None if zero is None else np.ascontiguousarray(np.asarray(zero, np.uint8).reshape(-1))
+        if z is not None and len(z) != k:
+            raise ValueError("zero must have one entry per column of b")
+        it = C.c_int64(0)
+        conv = C.c_int(0)
+        rel = C.c_double(0.0)
+        outer = C.c_int64(0)
+        _lib.check(lib.hdfe_pcg_solve(self._h, k, bp, None if z is None else z.ctypes.data,
+                                      x0p, float(tol), int(maxiter), PRECISIONS[precision],
+                                      float(inner_tol), xp, C.byref(it), C.byref(conv),
+                                      C.byref(rel), C.byref(outer), int(dev), stream),
+                   "hdfe_pcg_solve")
+        if info is not None:
+            info["refinement_steps"] = int(outer.value)
+        return X, int(it.value), bool(conv.value), float(rel.value)
+
+    def close(self):
+        if self._h:
+            _lib.load().hdfe_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def explicit_solver(S, dinv, tol, maxiter, debug=None, precision="fp64", inner_tol=1e-5):
+    """(solve(b, zero, x0) -> (X, iterations, converged, rel), device name):
+    S and the preconditioner are copied to the device here, once, and stay
+    there for every right-hand side (core/gpu.py:186-196)."""
+    be = rocm_backend()
+    A = DeviceCSR.from_scipy(S, dinv)
+
+    def solve(b, zero=None, x0=None):
+        out = A.solve(b, zero, x0, tol=tol, maxiter=maxiter, precision=precision,
+                      inner_tol=inner_tol)
+        if debug is not None:
+            debug(f"  CG: {out[1]} iterations, max rel. residual {out[3]:.2e}")
+        return out
+    solve.matrix = A
+    return solve, be.name
+
+
+def pcg(A: DeviceCSR, b, zero=None, x0=None, tol=1e-10, maxiter=5000, debug=None,
+        precision="fp64"):
+    """Jacobi block PCG on the device (core/gpu.py:199-253)."""
+    out = A.solve(b, zero, x0, tol=tol, maxiter=maxiter, precision=precision)
+    if debug is not None:
+        debug(f"  CG: {out[1]} iterations, max rel. residual {out[3]:.2e}")
+    return out

@@ -1,0 +1,99 @@
+"""Seeded synthetic AKM panels of the shapes the benchmarks and tests use.
+
+The reference ships its own simulator (hdfe_stream/simulate.py:42-143,
+`simulate_akm`, on Polars).  These generators are the numpy panels of the
+workloads measured here: workers observed for `n_years` consecutive years, a
+uniform first firm, and each year a move with probability `p_move` to a
+uniform random firm.  Effects: worker alpha ~ N(0, 0.5^2), firm psi ~ N(0,
+0.3^2), year gamma ~ N(0, 0.1^2); covariates x1, x2 ~ N(0, 1);
+y = alpha + psi + gamma + 0.3 x1 - 0.2 x2 + eps, eps ~ N(0, 0.4^2).
+Ids are int64.  With `corr` > 0 the first firm is drawn by a Gaussian copula
+between the worker's alpha and the firms' psi ranks (sorting), which gives a
+positive row-level corr(alpha, psi).
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+__all__ = ["akm_panel", "weighted_panel", "poisson_models"]
+
+
+def _first_firm(rng, alpha, psi, corr):
+    n_workers, n_firms = len(alpha), len(psi)
+    if corr <= 0:
+        return rng.integers(0, n_firms, n_workers)
+    # Gaussian copula: latent = rho * z(alpha) + sqrt(1 - rho^2) * noise,
+    # mapped to the firm whose psi has the same rank among firms.
+    za = (alpha - alpha.mean()) / alpha.std()
+    lat = corr * za + np.sqrt(1.0 - corr * corr) * rng.standard_normal(n_workers)
+    q = np.argsort(np.argsort(lat)) / max(n_workers - 1, 1)
+    order = np.argsort(psi)
+    return order[np.minimum((q * n_firms).astype(np.int64), n_firms - 1)]
+
+
+def akm_panel(n_workers: int, n_firms: int, n_years: int = 8, p_move: float = 0.10,
+              seed: int = 1, corr: float = 0.0, copula: float | None = None) -> dict:
+    """A balanced AKM panel as a dict of numpy columns, rows ordered by
+    (worker, year): worker_id, firm_id, year (int64), x1, x2, y (float64),
+    plus the true effects alpha (per row), psi (per row) and gamma (per row).
+
+    `corr` is the copula correlation used for the first firm (0: uniform);
+    `copula` overrides it directly when given."""
+    rng = np.random.default_rng(seed)
+    n = n_workers * n_years
+    alpha_w = rng.normal(0.0, 0.5, n_workers)
+    psi_f = rng.normal(0.0, 0.3, n_firms)
+    gamma_t = rng.normal(0.0, 0.1, n_years)
+    rho = corr if copula is None else copula
+    firm = np.empty((n_workers, n_years), np.int64)
+    firm[:, 0] = _first_firm(rng, alpha_w, psi_f, rho)
+    for t in range(1, n_years):
+        move = rng.random(n_workers) < p_move
+        dest = rng.integers(0, n_firms, n_workers)
+        firm[:, t] = np.where(move, dest, firm[:, t - 1])
+    worker = np.repeat(np.arange(n_workers, dtype=np.int64), n_years)
+    year = np.tile(np.arange(n_years, dtype=np.int64) + 2000, n_workers)
+    firm = firm.reshape(-1)
+    x1 = rng.standard_normal(n)
+    x2 = rng.standard_normal(n)
+    eps = rng.normal(0.0, 0.4, n)
+    a = alpha_w[worker]
+    p = psi_f[firm]
+    g = gamma_t[year - 2000]
+    y = a + p + g + 0.3 * x1 - 0.2 * x2 + eps
+    return {"worker_id": worker, "firm_id": firm, "year": year, "x1": x1, "x2": x2, "y": y,
+            "alpha": a, "psi": p, "gamma": g}
+
+
+def weighted_panel(n_workers: int, n_firms: int, seed: int = 4, n_years: int = 8,
+                   p_move: float = 0.10) -> dict:
+    """`akm_panel` with row weights w ~ Uniform(0.5, 2.0) as float32."""
+    d = akm_panel(n_workers, n_firms, n_years=n_years, p_move=p_move, seed=seed)
+    rng = np.random.default_rng(seed + 1000)
+    d["w"] = rng.uniform(0.5, 2.0, len(d["y"])).astype(np.float32)
+    return d
+
+
+def poisson_models(n_models: int = 32, lo: int = 50_000, hi: int = 2_000_000,
+                   n_cov: int = 300, seed: int = 5, scale: float = 1.0) -> list:
+    """Uneven Poisson models: rows log-uniform in [lo, hi] (times `scale`),
+    worker rows/8 levels, firm rows/80, year 10, `n_cov` float32 covariates;
+    y ~ Poisson(exp(0.5 + 0.1 x1 + alpha + psi)), alpha, psi ~ N(0, 0.2^2)."""
+    rng = np.random.default_rng(seed)
+    sizes = np.exp(rng.uniform(np.log(lo), np.log(hi), n_models)) * scale
+    out = []
+    for n in sizes.astype(np.int64):
+        n = max(int(n), 160)
+        nw, nf = max(n // 8, 2), max(n // 80, 2)
+        worker = np.sort(rng.integers(0, nw, n))
+        firm = rng.integers(0, nf, n)
+        year = rng.integers(0, 10, n)
+        X = rng.standard_normal((n, n_cov)).astype(np.float32)
+        a = rng.normal(0.0, 0.2, nw)[worker]
+        p = rng.normal(0.0, 0.2, nf)[firm]
+        mu = np.exp(0.5 + 0.1 * X[:, 0].astype(np.float64) + a + p)
+        y = rng.poisson(mu).astype(np.float64)
+        out.append({"worker_id": worker.astype(np.int64), "firm_id": firm.astype(np.int64),
+                    "year": year.astype(np.int64), "X": X, "y": y})
+    return out

@@ -1,0 +1,548 @@
+"""CPU oracle of the hdfe-stream solve path -- TEST INFRASTRUCTURE ONLY.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may use
+this module, and only as the checker (or the timed CPU baseline).  The
+product (the `hdfe_stream_amd` package) never imports it.
+
+It restates, in numpy and in the C of `hdfe_oracle.c` (loaded with ctypes),
+the reference's FP64 algorithm:
+
+  pass 0   singleton restriction      core/passes.py:859-904
+           streamed-dimension choice  core/estimator.py:371-385
+           level codes in sorted order, rows sorted by (group, codes)
+                                      core/passes.py:311-394
+  pass 1   cells, global-mean centring, identifying groups, obs/cnt, the
+           within-cell scatter        core/passes.py:397-589
+  step 2   diag, rhs, explicit S, block Jacobi-PCG, zero-column test,
+           normalisation              core/solve.py:44-49, 66-194, 294-399
+  step 3   V' M_D V from cells, beta  core/inference.py:109-137, 833-876
+  pass 2   worker effects, residuals  core/kernels_base.py:543-619
+  graph    components and ranking     core/kernels_base.py:155-174,
+                                      core/passes.py:933-979
+           leave-one-out pruning      core/connected.py:503-562
+  KSS      Rademacher draws, centre/project rows, JLA moments and their
+           combination               core/kernels_groups.py:52-77,
+                                      kernels_inverse.py:34-108,
+                                      inverse.py:162-309, leaveout.py:130-282
+
+Each piece is pinned in tests/test_oracle.py against what the mathematics
+fixes (dense closed forms, scipy routines, brute force) and the golden
+vectors under tests/golden/.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import scipy.sparse as sp
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / "liboracle.so"
+RHS_ZERO_TOL = 1e-11     # core/solve.py:44
+
+_lib = None
+
+
+def build(force: bool = False) -> Path:
+    """Compile hdfe_oracle.c into liboracle.so with gcc and OpenMP."""
+    src = HERE / "hdfe_oracle.c"
+    if force or not LIB.exists() or LIB.stat().st_mtime < src.stat().st_mtime:
+        subprocess.check_call(["gcc", "-O2", "-fopenmp", "-fPIC", "-shared", "-o", str(LIB),
+                               str(src), "-lm"])
+    return LIB
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB.exists():
+            build()
+        _lib = C.CDLL(str(LIB))
+        _lib.orc_s_count.restype = C.c_int64
+        _lib.orc_bfs_components.restype = C.c_int64
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def set_threads(n: int) -> None:
+    lib().orc_set_threads(C.c_int(int(n)))
+
+
+def num_threads() -> int:
+    return int(lib().orc_num_threads())
+
+
+# ------------------------------------------------------------------ pass 0
+
+def singleton_restriction(fe_cols: list[np.ndarray]) -> np.ndarray:
+    """Boolean mask of the rows kept after dropping singletons: dimensions
+    checked in turn, each one's singletons dropped before the next is
+    counted, until every dimension has been checked since the last drop
+    (core/passes.py:859-904)."""
+    n = len(fe_cols[0])
+    keep = np.ones(n, bool)
+    dims = len(fe_cols)
+    found, quiet, i = False, 0, 0
+    while quiet < dims - int(found):
+        col = fe_cols[i % dims]
+        vals = col[keep]
+        u, inv, cnt = np.unique(vals, return_inverse=True, return_counts=True)
+        bad = cnt[inv] == 1
+        if bad.any():
+            idx = np.flatnonzero(keep)
+            keep[idx[bad]] = False
+            found = True
+            quiet = 0
+        else:
+            quiet += 1
+        i += 1
+    return keep
+
+
+def pass0(data: dict, y: str, xs: list[str], fe: list[str], weights: str | None = None,
+          fixef_rm: str = "singleton", sort_vars: bool = True) -> dict:
+    """Rows after the restrictions, with level codes in sorted level order,
+    sorted by (streamed group, other codes[, variables])."""
+    cols = [np.asarray(data[f]) for f in fe]
+    keep = singleton_restriction(cols) if fixef_rm == "singleton" else np.ones(len(cols[0]), bool)
+    cols = [c[keep] for c in cols]
+    names = [y] + [x for x in xs if x != y]
+    V = np.column_stack([np.asarray(data[v], np.float64)[keep] for v in names])
+    w = np.asarray(data[weights], np.float64)[keep] if weights else None
+    card = [len(np.unique(c)) for c in cols]
+    gi = max(range(len(fe)), key=lambda d: (card[d], -d))           # estimator.py:383
+    others = [d for d in range(len(fe)) if d != gi]
+    levels, codes = [], []
+    for d in others:
+        u, inv = np.unique(cols[d], return_inverse=True)
+        levels.append(u)
+        codes.append(inv.astype(np.int64))
+    gvals, ginv = np.unique(cols[gi], return_inverse=True)
+    keys = [ginv] + codes
+    if sort_vars:
+        keys = keys + [V[:, j] for j in range(V.shape[1])]
+    order = np.lexsort(tuple(reversed(keys)))
+    return {
+        "names": names, "fe": [fe[gi]] + [fe[d] for d in others], "n_obs": int(keep.sum()),
+        "keep": keep, "order": order,
+        "g": ginv[order].astype(np.int64), "g_levels": gvals,
+        "codes": (np.column_stack([c[order] for c in codes]) if codes
+                  else np.zeros((len(order), 0), np.int64)),
+        "levels": levels, "n_levels": [len(u) for u in levels],
+        "V": V[order], "w": None if w is None else w[order],
+    }
+
+
+# ------------------------------------------------------------------ pass 1
+
+def pass1(p0: dict) -> dict:
+    """Cells, identifying cells and the pass-1 accumulators
+    (core/passes.py:397-589)."""
+    g, codes, V, w = p0["g"], p0["codes"], p0["V"], p0["w"]
+    n_rows, D = codes.shape
+    m = V.shape[1]
+    means = V.mean(axis=0)                                  # unweighted, as pass 0's scan
+    raw_ss = ((V - means) ** 2).sum(axis=0)
+    Z = V - means
+    same = g[1:] == g[:-1]
+    for d in range(D):
+        same &= codes[1:, d] == codes[:-1, d]
+    cs = np.flatnonzero(np.concatenate(([True], ~same)))
+    bounds = np.append(cs, n_rows)
+    run = np.diff(bounds).astype(np.float64)
+    wz = Z if w is None else Z * w[:, None]
+    sums = np.add.reduceat(wz, cs, axis=0) if n_rows else np.zeros((0, m))
+    n = np.add.reduceat(w, cs) if w is not None else run
+    cg = g[cs]
+    ccodes = codes[cs]
+    offs = np.concatenate(([0], np.cumsum(p0["n_levels"])[:-1])).astype(np.int64)
+    L = int(sum(p0["n_levels"]))
+    obs = np.zeros(L)
+    cnt = np.zeros(L)
+    for d in range(D):
+        obs += np.bincount(offs[d] + ccodes[:, d], weights=n, minlength=L)
+        cnt += np.bincount(offs[d] + ccodes[:, d], weights=run, minlength=L)
+    # within-cell scatter (assembly="cells", kernels_base.py:374-422)
+    mean_c = sums / n[:, None]
+    dev = Z - np.repeat(mean_c, run.astype(np.int64), axis=0)
+    W = (dev * (w[:, None] if w is not None else 1.0)).T @ dev
+    # identifying groups: more than one cell
+    gstart = np.flatnonzero(np.concatenate(([True], cg[1:] != cg[:-1])))
+    ncell = np.diff(np.append(gstart, len(cg)))
+    keep_g = ncell > 1
+    keep_c = np.repeat(keep_g, ncell)
+    kept = ncell[keep_g]
+    starts = np.concatenate(([0], np.cumsum(kept))).astype(np.int64)
+    return {
+        "means": means, "raw_ss": raw_ss, "obs": obs, "cnt": cnt, "W_within": W,
+        "offs": offs, "L": L, "D": D, "m": m, "n_cells": len(cs),
+        "starts": starts, "codes": np.ascontiguousarray(ccodes[keep_c], dtype=np.uint32),
+        "n": np.ascontiguousarray(n[keep_c]), "sums": np.ascontiguousarray(sums[keep_c]),
+        "ident_groups": np.flatnonzero(keep_g),
+    }
+
+
+# ------------------------------------------------------------------ step 2
+
+def diag(c: dict) -> np.ndarray:
+    out = np.zeros(c["L"])
+    lib().orc_diag(C.c_int64(len(c["starts"]) - 1), _p(c["starts"]), C.c_int32(c["D"]),
+                   _p(c["codes"]), _p(c["n"]), _p(c["offs"]), _p(out))
+    return out
+
+
+def rhs(c: dict, sums: np.ndarray | None = None) -> np.ndarray:
+    s = np.ascontiguousarray(c["sums"] if sums is None else sums, dtype=np.float64)
+    m = s.shape[1]
+    out = np.zeros((c["L"], m))
+    lib().orc_rhs(C.c_int64(len(c["starts"]) - 1), _p(c["starts"]), C.c_int32(c["D"]),
+                  _p(c["codes"]), _p(c["n"]), C.c_int32(m), _p(s), _p(c["offs"]), _p(out))
+    return out
+
+
+def build_S(c: dict) -> sp.csr_matrix:
+    """The explicit S = D_o' M_0 D_o (core/solve.py:148-194)."""
+    L = c["L"]
+    G = len(c["starts"]) - 1
+    rnnz = np.zeros(L, np.int64)
+    nnz = lib().orc_s_count(C.c_int64(G), _p(c["starts"]), C.c_int32(c["D"]), _p(c["codes"]),
+                            _p(c["offs"]), C.c_int64(L), _p(rnnz))
+    indptr = np.zeros(L + 1, np.int64)
+    np.cumsum(rnnz, out=indptr[1:])
+    indices = np.empty(max(nnz, 1), np.int32)
+    data = np.empty(max(nnz, 1))
+    lib().orc_s_fill(C.c_int64(G), _p(c["starts"]), C.c_int32(c["D"]), _p(c["codes"]),
+                     _p(c["n"]), _p(c["offs"]), C.c_int64(L), _p(indptr), _p(indices),
+                     _p(data))
+    S = sp.csr_matrix((data[:nnz], indices[:nnz], indptr), shape=(L, L))
+    S.has_sorted_indices = True
+    return S
+
+
+def jacobi(d: np.ndarray) -> np.ndarray:
+    """core/solve.py:142-146."""
+    pos = d > 1e-12
+    return np.where(pos, 1.0 / np.where(pos, d, 1.0), 0.0)
+
+
+def pcg(S: sp.csr_matrix, dinv: np.ndarray, b: np.ndarray, zero=None, x0=None,
+        tol: float = 1e-10, maxiter: int = 5000):
+    """(X, iterations, converged, max_rel) -- core/solve.py:66-140."""
+    b = np.ascontiguousarray(np.asarray(b, np.float64).reshape(S.shape[0], -1))
+    L, k = b.shape
+    X = np.empty_like(b)
+    indptr = np.ascontiguousarray(S.indptr, np.int64)
+    indices = np.ascontiguousarray(S.indices, np.int32)
+    data = np.ascontiguousarray(S.data, np.float64)
+    z = None if zero is None else np.ascontiguousarray(np.asarray(zero, np.uint8))
+    xs = None if x0 is None else np.ascontiguousarray(np.asarray(x0, np.float64))
+    it = C.c_int64(0)
+    conv = C.c_int(0)
+    rel = C.c_double(0.0)
+    lib().orc_pcg(C.c_int64(L), _p(indptr), _p(indices), _p(data),
+                  _p(np.ascontiguousarray(dinv, np.float64)), C.c_int64(k), _p(b),
+                  None if z is None else _p(z), None if xs is None else _p(xs),
+                  C.c_double(tol), C.c_int64(maxiter), _p(X), C.byref(it), C.byref(conv),
+                  C.byref(rel))
+    return X, int(it.value), bool(conv.value), float(rel.value)
+
+
+def csr_matmat(S: sp.csr_matrix, X: np.ndarray) -> np.ndarray:
+    X = np.ascontiguousarray(X, np.float64).reshape(S.shape[0], -1)
+    out = np.empty_like(X)
+    lib().orc_csr_matmat(C.c_int64(S.shape[0]), _p(np.ascontiguousarray(S.indptr, np.int64)),
+                         _p(np.ascontiguousarray(S.indices, np.int32)),
+                         _p(np.ascontiguousarray(S.data, np.float64)), C.c_int64(X.shape[1]),
+                         _p(X), _p(out))
+    return out
+
+
+def stream_matvec(c: dict, P: np.ndarray) -> np.ndarray:
+    P = np.ascontiguousarray(P, np.float64).reshape(c["L"], -1)
+    out = np.empty_like(P)
+    lib().orc_stream_matvec(C.c_int64(len(c["starts"]) - 1), _p(c["starts"]), C.c_int32(c["D"]),
+                            _p(c["codes"]), _p(c["n"]), _p(c["offs"]), C.c_int64(c["L"]),
+                            C.c_int64(P.shape[1]), _p(P), _p(out))
+    return out
+
+
+def components(c: dict, n_levels: int, col: int = 0) -> np.ndarray:
+    parent = np.empty(n_levels, np.int64)
+    lib().orc_components(C.c_int64(len(c["starts"]) - 1), _p(c["starts"]), C.c_int32(c["D"]),
+                         _p(c["codes"]), C.c_int32(col), C.c_int64(n_levels), _p(parent))
+    return parent
+
+
+def rank_components(root: np.ndarray, cnt_f1: np.ndarray):
+    """Labels ranked by rows, ties to the smallest fe[1] code
+    (core/passes.py:933-950)."""
+    nl = len(root)
+    is_root = root == np.arange(nl)
+    dense = (np.cumsum(is_root) - 1)[root]
+    rows = np.bincount(dense, weights=cnt_f1, minlength=int(is_root.sum()))
+    order = np.argsort(-rows, kind="stable")
+    rank = np.empty(len(order), np.int64)
+    rank[order] = np.arange(len(order))
+    return rank[dense], len(order)
+
+
+def normalize(G: np.ndarray, comp: np.ndarray, obs: np.ndarray, n_levels: list[int]) -> np.ndarray:
+    """fe[1] effects: obs-weighted mean zero per component; the others: a
+    global obs-weighted mean of zero (core/solve.py:383-399)."""
+    G = np.array(G, dtype=np.float64, copy=True)
+    o = 0
+    nl = n_levels[0]
+    w = obs[o:o + nl]
+    cw = np.bincount(comp, weights=w, minlength=nl)
+    cw[cw == 0] = 1.0
+    blk = G[o:o + nl]
+    mean = np.column_stack([np.bincount(comp, weights=w * blk[:, j], minlength=nl)
+                            for j in range(G.shape[1])]) / cw[:, None]
+    G[o:o + nl] -= mean[comp]
+    o += nl
+    for nl in n_levels[1:]:
+        w = obs[o:o + nl]
+        G[o:o + nl] -= (w @ G[o:o + nl]) / max(w.sum(), 1.0)
+        o += nl
+    return G
+
+
+def assemble(c: dict, Gamma: np.ndarray) -> np.ndarray:
+    m = c["m"]
+    acc = np.zeros((m, m))
+    lib().orc_assemble(C.c_int64(len(c["starts"]) - 1), _p(c["starts"]), C.c_int32(c["D"]),
+                       _p(c["codes"]), _p(c["n"]), C.c_int32(m), _p(c["sums"]), _p(c["offs"]),
+                       _p(np.ascontiguousarray(Gamma, np.float64)), _p(acc))
+    return c["W_within"] + acc
+
+
+def pass2(p0: dict, c: dict, gam_y: np.ndarray, beta: np.ndarray, xcols: list[int]):
+    """Worker effects (per group) and residuals (per sorted row)."""
+    g, codes = p0["g"], p0["codes"]
+    n = len(g)
+    starts = np.concatenate(([0], np.flatnonzero(g[1:] != g[:-1]) + 1, [n])).astype(np.int64)
+    w = p0["w"] if p0["w"] is not None else np.ones(n)
+    y = np.ascontiguousarray(p0["V"][:, 0])
+    X = np.ascontiguousarray(p0["V"][:, xcols]) if xcols else np.zeros((n, 0))
+    G = len(starts) - 1
+    geff = np.empty(G)
+    e = np.empty(n)
+    lib().orc_pass2(C.c_int64(G), _p(starts), C.c_int32(codes.shape[1]),
+                    _p(np.ascontiguousarray(codes, np.int64)), _p(c["offs"]),
+                    _p(np.ascontiguousarray(w, np.float64)), _p(y), C.c_int32(X.shape[1]),
+                    _p(np.ascontiguousarray(X, np.float64)),
+                    _p(np.ascontiguousarray(beta, np.float64)),
+                    _p(np.ascontiguousarray(gam_y, np.float64)), _p(geff), _p(e))
+    return geff, e
+
+
+def fit(data: dict, y: str = "y", xs=("x1", "x2"), fe=("worker_id", "firm_id", "year"),
+        weights: str | None = None, tol: float = 1e-10, maxiter: int = 5000,
+        rhs_block: int = 8) -> dict:
+    """The whole FP64 fit of `y ~ xs | fe`, as the reference runs it."""
+    p0 = pass0(data, y, list(xs), list(fe), weights)
+    c = pass1(p0)
+    L = c["L"]
+    m = c["m"]
+    d = diag(c)
+    b = rhs(c)
+    S = build_S(c)
+    dinv = jacobi(d)
+    nl = p0["n_levels"]
+    comp, n_comp = rank_components(components(c, nl[0]), c["cnt"][:nl[0]])
+    Gamma = np.zeros((L, m))
+    its = []
+    for j0 in range(0, m, rhs_block):
+        j1 = min(m, j0 + rhs_block)
+        bb = b[:, j0:j1]
+        scale = np.sqrt(np.maximum(c["raw_ss"][j0:j1], 0.0))
+        zero = np.linalg.norm(bb, axis=0) <= RHS_ZERO_TOL * scale
+        X, it, conv, rel = pcg(S, dinv, bb, zero, None, tol, maxiter)
+        Gamma[:, j0:j1] = normalize(X, comp, c["obs"], nl)
+        its.append(it)
+    A = assemble(c, Gamma)
+    xj = list(range(1, m))
+    beta = np.linalg.solve(A[np.ix_(xj, xj)], A[xj, 0]) if xj else np.zeros(0)
+    gam_y = Gamma[:, 0] - Gamma[:, xj] @ beta
+    geff, e = pass2(p0, c, gam_y, beta, xj)
+    return {"beta": beta, "Gamma": Gamma, "gam_y": gam_y, "A": A, "S": S, "diag": d, "rhs": b,
+            "comp": comp, "n_components": n_comp, "iterations": its, "p0": p0, "cells": c,
+            "worker_effects": geff, "resid": e}
+
+
+# ------------------------------------------------------------------ KSS
+
+def rademacher(ordinal: int, n_rows: int, draws: int, seed: int, first_draw: int) -> np.ndarray:
+    out = np.empty((n_rows, draws))
+    lib().orc_rademacher(C.c_int64(ordinal), C.c_int64(n_rows), C.c_int64(draws),
+                         C.c_uint64(seed), C.c_int64(first_draw), _p(out))
+    return out
+
+
+def center_rows(starts, codes, offs, w, X, R, L):
+    G = len(starts) - 1
+    R = np.ascontiguousarray(R, np.float64)
+    m = R.shape[1]
+    kc = X.shape[1]
+    acc = np.zeros((L, m))
+    acc_x = np.zeros((kc, m))
+    lib().orc_center_rows(C.c_int64(G), _p(np.ascontiguousarray(starts, np.int64)),
+                          C.c_int32(codes.shape[1]), _p(np.ascontiguousarray(codes, np.int64)),
+                          _p(np.ascontiguousarray(offs, np.int64)),
+                          _p(np.ascontiguousarray(w, np.float64)), C.c_int32(kc),
+                          _p(np.ascontiguousarray(X, np.float64)), C.c_int32(m), _p(R), _p(acc),
+                          _p(acc_x))
+    return acc, acc_x
+
+
+def project_rows(starts, codes, offs, w, X, Cm, R, U):
+    G = len(starts) - 1
+    R = np.ascontiguousarray(R, np.float64)
+    m = R.shape[1]
+    kc = X.shape[1]
+    out = np.empty_like(R)
+    lib().orc_project_rows(C.c_int64(G), _p(np.ascontiguousarray(starts, np.int64)),
+                           C.c_int32(codes.shape[1]), _p(np.ascontiguousarray(codes, np.int64)),
+                           _p(np.ascontiguousarray(offs, np.int64)),
+                           _p(np.ascontiguousarray(w, np.float64)), C.c_int32(kc),
+                           _p(np.ascontiguousarray(X, np.float64)),
+                           _p(np.ascontiguousarray(Cm, np.float64)), C.c_int32(m), _p(R),
+                           _p(np.ascontiguousarray(U, np.float64)), _p(out))
+    return out
+
+
+def jla_moments(r: np.ndarray, proj: np.ndarray, moments: np.ndarray) -> None:
+    n, k = r.shape
+    lib().orc_jla_moments(C.c_int64(n), C.c_int32(k), _p(np.ascontiguousarray(r)),
+                          _p(np.ascontiguousarray(proj)), _p(moments))
+
+
+def combine_moments(moments: np.ndarray, n_draws: int):
+    """(P_bar, M_bar, factor) -- leaveout.py:209-282 (the per-row algebra)."""
+    P, M, mPP, mMM, mPM = (np.asarray(moments[i], np.float64) / n_draws for i in range(5))
+    total = P + M
+    safe = total > 0
+    P_bar = np.divide(P, total, out=np.zeros_like(P), where=safe)
+    M_bar = 1.0 - P_bar
+    V = (M_bar ** 2 * mPP + P_bar ** 2 * mMM - 2 * P_bar * M_bar * mPM) / n_draws
+    B = (M_bar * mPP - P_bar * mMM + (M_bar - P_bar) * mPM) / n_draws
+    usable = M_bar > 0
+    factor = np.ones_like(M_bar)
+    np.divide(V, M_bar ** 2, out=factor, where=usable)
+    factor = np.where(usable, 1.0 - factor + np.divide(B, M_bar, out=np.zeros_like(B),
+                                                       where=usable), 1.0)
+    return P_bar, M_bar, factor
+
+
+def jla_leverages(p0: dict, c: dict, S, dinv, n_draws: int, block: int, seed: int = 0,
+                  tol: float = 1e-10, maxiter: int = 5000):
+    """Leverages of the fixed-effect design (no covariates, unweighted) by
+    the reference's JLA: per block, centre, solve, project, accumulate the
+    five moments (leaveout.py:130-198, inverse.py:240-309)."""
+    g, codes = p0["g"], p0["codes"]
+    n = len(g)
+    starts = np.concatenate(([0], np.flatnonzero(g[1:] != g[:-1]) + 1, [n])).astype(np.int64)
+    w = np.ones(n) if p0["w"] is None else p0["w"]
+    X0 = np.zeros((n, 0))
+    moments = np.zeros((5, n))
+    drawn = 0
+    while drawn < n_draws:
+        size = min(block, n_draws - drawn)
+        R = rademacher(0, n, size, seed, drawn)
+        acc, _ = center_rows(starts, codes, c["offs"], w, X0, R, c["L"])
+        scale = np.sqrt(w @ (R * R))
+        zero = np.linalg.norm(acc, axis=0) <= RHS_ZERO_TOL * scale
+        U, _, _, _ = pcg(S, dinv, acc, zero, None, tol, maxiter)
+        proj = project_rows(starts, codes, c["offs"], w, X0, np.zeros((0, size)), R, U)
+        jla_moments(R, proj, moments)
+        drawn += size
+    return moments
+
+
+# ------------------------------------------------------------------ graph
+
+def articulation_points(indptr, indices, alive):
+    nv = len(indptr) - 1
+    cut = np.zeros(nv, np.uint8)
+    lib().orc_articulation_points(C.c_int64(nv), _p(np.ascontiguousarray(indptr, np.int64)),
+                                  _p(np.ascontiguousarray(indices, np.int64)), _p(cut),
+                                  _p(np.ascontiguousarray(alive, np.uint8)))
+    return cut.astype(bool)
+
+
+def bfs_components(indptr, indices, alive):
+    nv = len(indptr) - 1
+    label = np.empty(nv, np.int64)
+    nc = lib().orc_bfs_components(C.c_int64(nv), _p(np.ascontiguousarray(indptr, np.int64)),
+                                  _p(np.ascontiguousarray(indices, np.int64)),
+                                  _p(np.ascontiguousarray(alive, np.uint8)), _p(label))
+    return label, int(nc)
+
+
+def pair_graph(a: np.ndarray, b: np.ndarray, n0: int, n1: int, alive: np.ndarray):
+    """CSR adjacency in both directions of the distinct pairs (a, b), the
+    second dimension's vertices from n0 (core/kernels_graph.py:313-341)."""
+    keys = np.unique(a.astype(np.int64) * n1 + b.astype(np.int64))
+    ka, kb = keys // n1, n0 + keys % n1
+    ok = alive[ka] & alive[kb]
+    ka, kb = ka[ok], kb[ok]
+    nv = n0 + n1
+    deg = np.bincount(ka, minlength=nv) + np.bincount(kb, minlength=nv)
+    indptr = np.concatenate(([0], np.cumsum(deg))).astype(np.int64)
+    indices = np.empty(indptr[-1], np.int64)
+    pos = indptr[:-1].copy()
+    for x, y in zip(ka, kb):        # in key order, as _nb_pairs_fill
+        indices[pos[x]] = y
+        pos[x] += 1
+        indices[pos[y]] = x
+        pos[y] += 1
+    return keys, indptr, indices
+
+
+def leave_one_out(a: np.ndarray, b: np.ndarray, n0: int, n1: int):
+    """Leave-one-out connected set of the (worker a, firm b) rows: the
+    alive mask over the n0 + n1 vertices and the counts
+    (core/connected.py:503-562, ranking by rows)."""
+    nv = n0 + n1
+    alive = np.ones(nv, bool)
+    keys, indptr, indices = pair_graph(a, b, n0, n1, alive)
+    kk = a.astype(np.int64) * n1 + b.astype(np.int64)
+    _, counts = np.unique(kk, return_counts=True)
+    ka, kb = keys // n1, n0 + keys % n1
+
+    def keep_largest():
+        label, nc = bfs_components(indptr, indices, alive)
+        if nc == 0:
+            return 0
+        ok = (label[ka] >= 0) & (label[kb] >= 0)
+        size = np.bincount(label[ka][ok], weights=counts[ok], minlength=nc)
+        alive[:] &= label == int(np.argmax(size))
+        return nc
+
+    out = {"components_before": keep_largest()}
+    ok = alive[ka] & alive[kb]
+    rows0 = np.bincount(ka[ok], weights=counts[ok], minlength=n0)
+    single = (rows0 == 1) & alive[:n0]
+    out["single_observation"] = int(single.sum())
+    if out["single_observation"]:
+        alive[:n0] &= ~single
+        keep_largest()
+    cut_total, rounds = 0, 0
+    while True:
+        cut = articulation_points(indptr, indices, alive.astype(np.uint8))[:n0] & alive[:n0]
+        n_cut = int(cut.sum())
+        if not n_cut:
+            break
+        cut_total += n_cut
+        rounds += 1
+        alive[:n0] &= ~cut
+        keep_largest()
+    out.update(cut=cut_total, rounds=rounds)
+    return alive, out

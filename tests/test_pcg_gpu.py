@@ -1,0 +1,129 @@
+"""The device block PCG against the oracle's restatement of the reference's
+_pcg (core/solve.py:66-140) on the same S, right-hand sides, zero columns and
+starting guesses -- the pattern of the reference's
+tests/test_gpu.py:56-81."""
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def system(oracle_mod, small_panel):
+    O = oracle_mod
+    p0 = O.pass0(small_panel, "y", ["x1", "x2"], ["worker_id", "firm_id", "year"])
+    c = O.pass1(p0)
+    S = O.build_S(c)
+    dinv = O.jacobi(O.diag(c))
+    return S, dinv
+
+
+@pytest.fixture(scope="module")
+def long_system(oracle_mod):
+    """Year rows with more nonzeros than one workgroup takes (split rows)."""
+    from hdfe_stream_amd.simulate import akm_panel
+    O = oracle_mod
+    d = akm_panel(60_000, 9_000, seed=3)
+    p0 = O.pass0(d, "y", ["x1", "x2"], ["worker_id", "firm_id", "year"])
+    c = O.pass1(p0)
+    return O.build_S(c), O.jacobi(O.diag(c))
+
+
+def _rhs(S, k, seed=3):
+    rng = np.random.default_rng(seed)
+    return S @ rng.standard_normal((S.shape[0], k))
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_device_pcg_matches_the_oracle_pcg(oracle_mod, system, precision):
+    from hdfe_stream_amd.rocm import explicit_solver
+    S, dinv = system
+    b = _rhs(S, 4)
+    zero = np.array([False, True, False, False])
+    b[:, 1] = 0.0
+    x0 = np.zeros_like(b)
+    x0[:, 2] = np.linalg.lstsq(S.toarray(), b[:, 2], rcond=None)[0]   # already solved
+    solve, name = explicit_solver(S, dinv, 1e-11, 5000, precision=precision)
+    assert name
+    for start in (None, x0):
+        X_cpu, it_cpu, ok_cpu, _ = oracle_mod.pcg(S, dinv, b, zero, start, 1e-11, 5000)
+        X_dev, it_dev, ok_dev, rel = solve(b, zero, start)
+        assert ok_cpu and ok_dev
+        assert rel <= 1e-11
+        if precision == "fp64":
+            assert abs(it_cpu - it_dev) <= 1
+        assert np.all(X_dev[:, 1] == 0.0)
+        assert np.max(np.abs(X_dev - X_cpu)) < 1e-8 * np.max(np.abs(X_cpu))
+
+
+@pytest.mark.parametrize("k", [1, 3, 8, 20, 256])
+def test_device_spmm_is_the_operator(system, k):
+    from hdfe_stream_amd.rocm import DeviceCSR
+    S, dinv = system
+    A = DeviceCSR.from_scipy(S, dinv)
+    X = np.random.default_rng(k).standard_normal((S.shape[0], k))
+    np.testing.assert_allclose(A.matmat(X), S @ X, rtol=1e-12, atol=1e-12 * np.abs(S.data).max())
+
+
+@pytest.mark.parametrize("k", [1, 3, 40])
+def test_split_rows(long_system, k):
+    from hdfe_stream_amd.rocm import DeviceCSR
+    S, dinv = long_system
+    A = DeviceCSR.from_scipy(S, dinv)
+    assert A.nlong() > 0
+    X = np.random.default_rng(k).standard_normal((S.shape[0], k))
+    ref = S @ X
+    np.testing.assert_allclose(A.matmat(X), ref, rtol=1e-11, atol=1e-11 * np.abs(ref).max())
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_split_rows_solve(oracle_mod, long_system, precision):
+    from hdfe_stream_amd.rocm import DeviceCSR
+    S, dinv = long_system
+    A = DeviceCSR.from_scipy(S, dinv)
+    b = _rhs(S, 3, seed=5)
+    Xc, itc, okc, _ = oracle_mod.pcg(S, dinv, b, None, None, 1e-10, 5000)
+    Xd, itd, okd, rel = A.solve(b, tol=1e-10, precision=precision)
+    assert okc and okd and rel <= 1e-10
+    if precision == "fp64":
+        assert abs(itc - itd) <= 1
+    assert np.max(np.abs(Xd - Xc)) < 1e-7 * np.max(np.abs(Xc))
+
+
+@pytest.mark.parametrize("k", [16, 256])
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_wide_blocks(oracle_mod, system, k, precision):
+    from hdfe_stream_amd.rocm import DeviceCSR
+    S, dinv = system
+    A = DeviceCSR.from_scipy(S, dinv)
+    b = _rhs(S, k, seed=k)
+    Xc, itc, okc, _ = oracle_mod.pcg(S, dinv, b, None, None, 1e-10, 5000)
+    Xd, itd, okd, rel = A.solve(b, tol=1e-10, precision=precision)
+    assert okc and okd and rel <= 1e-10
+    if precision == "fp64":
+        assert abs(itc - itd) <= 1
+    assert np.max(np.abs(Xd - Xc)) < 1e-7 * np.max(np.abs(Xc))
+
+
+def test_all_zero_and_maxiter(system):
+    from hdfe_stream_amd.rocm import DeviceCSR
+    S, dinv = system
+    A = DeviceCSR.from_scipy(S, dinv)
+    b = _rhs(S, 2)
+    X, it, ok, rel = A.solve(b, zero=[True, True])
+    assert it == 0 and ok and np.all(X == 0)
+    X, it, ok, rel = A.solve(b, tol=1e-14, maxiter=3)
+    assert it == 3 and not ok and rel > 1e-14
+
+
+def test_device_tensors_in_and_out(system):
+    import torch
+    from hdfe_stream_amd.rocm import DeviceCSR
+    S, dinv = system
+    A = DeviceCSR.from_scipy(S, dinv)
+    b = _rhs(S, 3)
+    Xh, ith, _, _ = A.solve(b, tol=1e-10)
+    Xd, itd, ok, _ = A.solve(torch.from_numpy(b).cuda(), tol=1e-10)
+    assert ok and itd == ith
+    np.testing.assert_array_equal(Xd.cpu().numpy(), Xh)
