@@ -20,6 +20,21 @@ FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
          "-Wno-unused-result"]
 
 
+def torch_lib_dir() -> str | None:
+    """torch's lib directory: the library resolves the HIP runtime there
+    first, so that a process holds one HIP runtime (torch's)."""
+    try:
+        import importlib.util
+        spec = importlib.util.find_spec("torch")
+        if spec and spec.origin:
+            d = Path(spec.origin).parent / "lib"
+            if (d / "libamdhip64.so").exists():
+                return str(d)
+    except Exception:
+        pass
+    return None
+
+
 def sources() -> list[Path]:
     return [CSRC / s for s in SOURCES if (CSRC / s).exists()]
 
@@ -27,7 +42,9 @@ def sources() -> list[Path]:
 def build(verbose: bool = False) -> Path:
     """Compile every HIP source into one shared library (always rebuilds)."""
     tmp = LIB.with_suffix(".so.tmp")
-    cmd = ["hipcc", *FLAGS, "-I", str(CSRC), *map(str, sources()), "-o", str(tmp)]
+    rpath = torch_lib_dir()
+    link = [f"-Wl,-rpath,{rpath}"] if rpath else []
+    cmd = ["hipcc", *FLAGS, *link, "-I", str(CSRC), *map(str, sources()), "-o", str(tmp)]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         sys.stderr.write(res.stdout + res.stderr)

@@ -34,7 +34,21 @@ SIGNATURES = [
     ("hdfe_csr_info", I32, [P, P, P, P]),
     ("hdfe_pcg_solve", I32, [P, I64, P, P, P, D, I64, I32, D, P, P, P, P, P, I32, P]),
     ("hdfe_csr_matmat", I32, [P, I64, P, P, I32]),
+    ("hdfe_csr_export", I32, [P, P, P, P, P]),
+    ("hdfe_col_norms", I32, [P, I64, I64, P]),
+    ("hdfe_csr_timing", I32, [P, I32]),
+    ("hdfe_csr_timing_read", I32, [P, P, P]),
+    ("hdfe_memcpy", I32, [P, P, I64, I32]),
     ("hdfe_destroy", None, [P]),
+    ("hdfe_panel_create", I32, [I64, I32, P, I32, P, P, I32, I32, I32, I32, P]),
+    ("hdfe_panel_info", I32, [P, P, P, P]),
+    ("hdfe_panel_array", I32, [P, C.c_char_p, P, P]),
+    ("hdfe_panel_system", I32, [P, I32, P, P]),
+    ("hdfe_panel_components", I32, [P, P]),
+    ("hdfe_panel_normalize", I32, [P, P, I32, P]),
+    ("hdfe_panel_assemble", I32, [P, P, P]),
+    ("hdfe_panel_pass2", I32, [P, P, P, P, I32, P, P, P]),
+    ("hdfe_panel_destroy", None, [P]),
 ]
 
 
@@ -42,6 +56,9 @@ def load():
     """The loaded library (ctypes.CDLL), loading it on first use."""
     global _lib
     if _lib is None:
+        # torch first: its HIP runtime (same SONAMEs) is then the one the
+        # library binds to, so the process holds a single HIP runtime
+        import torch  # noqa: F401
         if not LIB_PATH.exists():
             raise ImportError(f"{LIB_PATH.name} not found. {BUILD_HINT}")
         lib = C.CDLL(str(LIB_PATH))

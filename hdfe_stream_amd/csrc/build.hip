@@ -1,0 +1,1940 @@
+// Passes 0 and 1 and the reduced system on the device, for gfx950.
+//
+// From the rows of a panel resident in HBM (fixed-effect id columns, the
+// variables, optional weights) to the explicit S = D_o' M_0 D_o, its Jacobi
+// diagonal and the right-hand side D_o' M_0 V, with everything kept on the
+// device for the solve, the normalisation, the assembly of V' M_D V and the
+// row pass that recovers the streamed effects.
+//
+// Reference steps restated (hdfe_stream/):
+//   factorize: codes in sorted level order        core/passes.py:329-335
+//   singletons, dimensions in turn                core/passes.py:859-904
+//   streamed dimension: highest cardinality       core/estimator.py:371-385
+//   rows sorted by (group, codes[, variables])    core/passes.py:340-394
+//   cells, centring, identifying groups, obs/cnt  core/passes.py:397-589
+//   diag / rhs                                    core/kernels_base.py:64-110
+//   S                                             core/kernels_base.py:192-358
+//   components and their ranking                  core/kernels_base.py:155-174,
+//                                                 core/passes.py:933-979
+//   normalisation                                 core/solve.py:383-399
+//   V' M_D V from cells                           core/kernels_base.py:510-540
+//   streamed effects and residuals                core/kernels_base.py:543-619
+//
+// Design (not the reference's numba loops): every step is a sort, a scan, a
+// segmented reduction or a gather over struct-of-arrays columns, so the
+// sums are formed in a fixed order and results do not depend on scheduling.
+// The entries of S between two "sparse" dimensions (many levels) come from a
+// sort-and-reduce of per-group level pairs; the entries that involve a
+// "dense" dimension (few levels, e.g. years, whose rows touch every firm) are
+// accumulated row by row into a dense panel, so that no level's row becomes a
+// sort bucket of millions of entries.
+#include <algorithm>
+#include <numeric>
+
+#include "devbuf.h"
+#include "solver.h"
+
+namespace hdfe {
+
+constexpr int kDenseMax = 64;  // a dimension with at most this many levels is dense
+constexpr int T256 = 256;
+
+// ------------------------------------------------------------ kernels
+
+__global__ void k_key_from_col(int64_t n, const int64_t *__restrict__ col, int64_t lo,
+                               uint64_t *__restrict__ key, int *__restrict__ idx) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        key[i] = (uint64_t)(col[i] - lo);
+        idx[i] = (int)i;
+    }
+}
+
+__global__ void k_iota(int64_t n, int *__restrict__ idx) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        idx[i] = (int)i;
+}
+
+__global__ void k_heads_u64(int64_t n, const uint64_t *__restrict__ key, int *__restrict__ head) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        head[i] = (i == 0 || key[i] != key[i - 1]) ? 1 : 0;
+}
+
+// code[idx[i]] = rank[i] - 1; levels[rank - 1] = value at each head.
+__global__ void k_scatter_codes(int64_t n, const uint64_t *__restrict__ key, int64_t lo,
+                                const int *__restrict__ idx, const int *__restrict__ head,
+                                const int *__restrict__ rank, int *__restrict__ code,
+                                int64_t *__restrict__ levels) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int r = rank[i] - 1;
+        code[idx[i]] = r;
+        if (head[i] && levels) levels[r] = (int64_t)key[i] + lo;
+    }
+}
+
+__global__ void k_minmax_partial(int64_t n, const int64_t *__restrict__ col,
+                                 int64_t *__restrict__ part) {
+    __shared__ int64_t lo[T256], hi[T256];
+    int64_t a = INT64_MAX, b = INT64_MIN;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        a = min(a, col[i]);
+        b = max(b, col[i]);
+    }
+    lo[threadIdx.x] = a;
+    hi[threadIdx.x] = b;
+    __syncthreads();
+    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+        if (threadIdx.x < o) {
+            lo[threadIdx.x] = min(lo[threadIdx.x], lo[threadIdx.x + o]);
+            hi[threadIdx.x] = max(hi[threadIdx.x], hi[threadIdx.x + o]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = lo[0];
+        part[2 * blockIdx.x + 1] = hi[0];
+    }
+}
+
+// Singletons: rows per level among the rows still kept.
+__global__ void k_level_count(int64_t n, const int *__restrict__ code,
+                              const uint8_t *__restrict__ keep, int *__restrict__ cnt) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        if (keep[i]) atomicAdd(&cnt[code[i]], 1);
+}
+
+__global__ void k_drop_single(int64_t n, const int *__restrict__ code,
+                              const int *__restrict__ cnt, uint8_t *__restrict__ keep,
+                              unsigned long long *__restrict__ dropped) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        if (keep[i] && cnt[code[i]] == 1) {
+            keep[i] = 0;
+            atomicAdd(dropped, 1ull);
+        }
+}
+
+// Levels still present after the restriction, for re-coding.
+__global__ void k_level_present(int64_t n, const int *__restrict__ code,
+                                const uint8_t *__restrict__ keep, int *__restrict__ present) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        if (keep[i]) present[code[i]] = 1;
+}
+
+__global__ void k_recode(int64_t n, int *__restrict__ code, const int *__restrict__ newcode) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        code[i] = newcode[code[i]];
+}
+
+__global__ void k_compact_levels(int64_t nl, const int *__restrict__ present,
+                                 const int *__restrict__ newcode, const int64_t *__restrict__ lv,
+                                 int64_t *__restrict__ out) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nl;
+         i += (int64_t)gridDim.x * blockDim.x)
+        if (present[i]) out[newcode[i]] = lv[i];
+}
+
+// Row sort keys: (g, code_1, ..., code_D) packed high to low; dropped rows
+// sort last.
+__global__ void k_pack_key(int64_t n, int D, const int *__restrict__ gcode,
+                           const int *__restrict__ codes, int64_t ld, const int *__restrict__ shift,
+                           int gshift, const uint8_t *__restrict__ keep,
+                           uint64_t *__restrict__ key, int *__restrict__ idx) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        uint64_t k = (uint64_t)gcode[i] << gshift;
+        for (int d = 0; d < D; ++d) k |= (uint64_t)codes[d * ld + i] << shift[d];
+        key[i] = keep[i] ? k : ~0ull;
+        idx[i] = (int)i;
+    }
+}
+
+// Order-preserving unsigned image of a double (ascending, -0 before +0).
+__device__ __forceinline__ uint64_t f64_key(double v) {
+    uint64_t u = __double_as_longlong(v);
+    return (u >> 63) ? ~u : (u | 0x8000000000000000ull);
+}
+
+__global__ void k_var_key(int64_t n, const double *__restrict__ V, int m, int j,
+                          const int *__restrict__ perm, uint64_t *__restrict__ key,
+                          int *__restrict__ idx) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int r = perm ? perm[i] : (int)i;
+        key[i] = f64_key(V[(int64_t)r * m + j]);
+        idx[i] = r;
+    }
+}
+
+// Sorted key of the rows in the current order perm: key[i] = full[perm[i]].
+__global__ void k_gather_key(int64_t n, const uint64_t *__restrict__ full,
+                             const int *__restrict__ perm, uint64_t *__restrict__ key) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        key[i] = full[perm[i]];
+}
+
+template <typename T>
+__global__ void k_gather(int64_t n, const int *__restrict__ perm, const T *__restrict__ src,
+                         T *__restrict__ dst) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = src[perm[i]];
+}
+
+// V (n, m) row major gathered from m column pointers by the row order perm.
+__global__ void k_gather_vars(int64_t n, int m, const double *const *__restrict__ cols,
+                              const int *__restrict__ perm, double *__restrict__ V) {
+    const int64_t nm = n * m;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < nm;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = e / m;
+        const int j = (int)(e - i * m);
+        V[e] = cols[j][perm[i]];
+    }
+}
+
+__global__ void k_gather_weights(int64_t n, const void *__restrict__ w, int f32,
+                                 const int *__restrict__ perm, double *__restrict__ out) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int r = perm[i];
+        out[i] = f32 ? (double)((const float *)w)[r] : ((const double *)w)[r];
+    }
+}
+
+// Run heads over sorted rows: a new group (gh) or a new cell (ch).
+__global__ void k_row_heads(int64_t n, int D, const int *__restrict__ g,
+                            const int *__restrict__ codes, int64_t ld, int *__restrict__ gh,
+                            int *__restrict__ ch) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        int ng = (i == 0) || g[i] != g[i - 1];
+        int nc = ng;
+        for (int d = 0; d < D && !nc; ++d) nc = codes[d * ld + i] != codes[d * ld + i - 1];
+        gh[i] = ng;
+        ch[i] = nc;
+    }
+}
+
+// starts[rank[i] - 1] = i at heads; starts[count] = n.
+__global__ void k_starts(int64_t n, const int *__restrict__ head, const int *__restrict__ rank,
+                         int *__restrict__ starts) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        if (head[i]) starts[rank[i] - 1] = (int)i;
+    if (blockIdx.x == 0 && threadIdx.x == 0) starts[rank[n - 1]] = (int)n;
+}
+
+// Column sums of V (n, m) in fixed row blocks: part[b * m + j].
+__global__ void k_colsum_rows(int64_t n, int m, const double *__restrict__ V,
+                              double *__restrict__ part) {
+    __shared__ double lds[T256];
+    const int64_t r0 = part_begin(n, blockIdx.x, gridDim.x), r1 = part_begin(n, blockIdx.x + 1, gridDim.x);
+    for (int j = 0; j < m; ++j) {
+        double s = 0.0;
+        for (int64_t i = r0 + threadIdx.x; i < r1; i += blockDim.x) s += V[i * m + j];
+        s = block_sum<double>(s, lds);
+        if (threadIdx.x == 0) part[(int64_t)blockIdx.x * m + j] = s;
+    }
+}
+
+// Squared deviations from the mean, same blocks.
+__global__ void k_colss_rows(int64_t n, int m, const double *__restrict__ V,
+                             const double *__restrict__ mean, double *__restrict__ part) {
+    __shared__ double lds[T256];
+    const int64_t r0 = part_begin(n, blockIdx.x, gridDim.x), r1 = part_begin(n, blockIdx.x + 1, gridDim.x);
+    for (int j = 0; j < m; ++j) {
+        double s = 0.0;
+        for (int64_t i = r0 + threadIdx.x; i < r1; i += blockDim.x) {
+            const double d = V[i * m + j] - mean[j];
+            s += d * d;
+        }
+        s = block_sum<double>(s, lds);
+        if (threadIdx.x == 0) part[(int64_t)blockIdx.x * m + j] = s;
+    }
+}
+
+// Cells: weight n_c (or rows), the weighted sums of the centred variables,
+// and per-block sums of the within-cell scatter (m <= 8).
+__global__ void k_cells(int64_t C, int m, const int *__restrict__ cstart,
+                        const double *__restrict__ V, const double *__restrict__ w,
+                        const double *__restrict__ mean, double *__restrict__ cn,
+                        double *__restrict__ crun, double *__restrict__ csums,
+                        double *__restrict__ wpart) {
+    __shared__ double lds[T256];
+    double W[64];
+    const int mm = m <= 8 ? m * m : 0;
+    for (int t = 0; t < 64; ++t) W[t] = 0.0;
+    const int64_t c0 = part_begin(C, blockIdx.x, gridDim.x), c1 = part_begin(C, blockIdx.x + 1, gridDim.x);
+    for (int64_t c = c0 + threadIdx.x; c < c1; c += blockDim.x) {
+        const int s = cstart[c], e = cstart[c + 1];
+        double nn = 0.0;
+        double sm[8];
+        for (int j = 0; j < 8; ++j) sm[j] = 0.0;
+        for (int i = s; i < e; ++i) {
+            const double wi = w ? w[i] : 1.0;
+            nn += wi;
+            if (m <= 8)
+                for (int j = 0; j < m; ++j) sm[j] += wi * (V[(int64_t)i * m + j] - mean[j]);
+        }
+        cn[c] = nn;
+        crun[c] = (double)(e - s);
+        if (m <= 8) {
+            for (int j = 0; j < m; ++j) csums[c * m + j] = sm[j];
+            if (mm) {
+                for (int i = s; i < e; ++i) {
+                    const double wi = w ? w[i] : 1.0;
+                    double dv[8];
+                    for (int j = 0; j < m; ++j) dv[j] = V[(int64_t)i * m + j] - mean[j] - sm[j] / nn;
+                    for (int a = 0; a < m; ++a)
+                        for (int b = 0; b < m; ++b) W[a * m + b] += wi * dv[a] * dv[b];
+                }
+            }
+        } else {
+            for (int j = 0; j < m; ++j) {
+                double t = 0.0;
+                for (int i = s; i < e; ++i)
+                    t += (w ? w[i] : 1.0) * (V[(int64_t)i * m + j] - mean[j]);
+                csums[c * m + j] = t;
+            }
+        }
+    }
+    for (int t = 0; t < mm; ++t) {
+        const double v = block_sum<double>(W[t], lds);
+        if (threadIdx.x == 0) wpart[(int64_t)blockIdx.x * mm + t] = v;
+    }
+}
+
+// obs / cnt per level over all cells (cnt in integer atomics: exact).
+__global__ void k_obs_cnt(int64_t C, int D, const int *__restrict__ cstart,
+                          const int *__restrict__ codes, int64_t ld,
+                          const int64_t *__restrict__ offs, const double *__restrict__ cn,
+                          unsigned long long *__restrict__ cnt, double *__restrict__ obs,
+                          int weighted) {
+    for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < C;
+         c += (int64_t)gridDim.x * blockDim.x) {
+        const int s = cstart[c];
+        const unsigned long long r = (unsigned long long)(cstart[c + 1] - s);
+        for (int d = 0; d < D; ++d) {
+            const int64_t a = offs[d] + codes[d * ld + s];
+            atomicAdd(&cnt[a], r);
+            if (weighted) atomicAdd(&obs[a], cn[c]);
+        }
+    }
+}
+
+__global__ void k_u64_to_f64(int64_t n, const unsigned long long *__restrict__ a,
+                             double *__restrict__ b, double *__restrict__ b2) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        b[i] = (double)a[i];
+        if (b2) b2[i] = (double)a[i];
+    }
+}
+
+// Cells per group (the group starts over cells), and the identifying flag.
+__global__ void k_group_cells(int64_t G, const int *__restrict__ gcstart,
+                              int *__restrict__ ident) {
+    for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < G;
+         g += (int64_t)gridDim.x * blockDim.x)
+        ident[g] = (gcstart[g + 1] - gcstart[g]) > 1 ? 1 : 0;
+}
+
+// Cells per identifying group (0 for the others; entry G is 0).
+__global__ void k_ident_ncells(int64_t G, const int *__restrict__ gcstart,
+                               const int *__restrict__ ident, int *__restrict__ out) {
+    for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g <= G;
+         g += (int64_t)gridDim.x * blockDim.x)
+        out[g] = (g < G && ident[g]) ? gcstart[g + 1] - gcstart[g] : 0;
+}
+
+// Scatter the identifying cells: for group g with ident, its cells go to
+// [ipos[g], ...) (ipos = exclusive sum of cells over identifying groups).
+__global__ void k_ident_cells(int64_t G, int D, int m, const int *__restrict__ gcstart,
+                              const int *__restrict__ ident, const int *__restrict__ gpos,
+                              const int *__restrict__ cpos, const int *__restrict__ cstart,
+                              const int *__restrict__ codes, int64_t ld,
+                              const double *__restrict__ cn, const double *__restrict__ csums,
+                              int *__restrict__ icodes, int64_t ild, double *__restrict__ in,
+                              double *__restrict__ isums, int *__restrict__ igstart,
+                              int *__restrict__ igroup) {
+    for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < G;
+         g += (int64_t)gridDim.x * blockDim.x) {
+        if (!ident[g]) continue;
+        const int q = gpos[g];
+        int o = cpos[g];
+        igstart[q] = o;
+        igroup[q] = (int)g;
+        for (int c = gcstart[g]; c < gcstart[g + 1]; ++c, ++o) {
+            const int r = cstart[c];
+            for (int d = 0; d < D; ++d) icodes[d * ild + o] = codes[d * ld + r];
+            in[o] = cn[c];
+            for (int j = 0; j < m; ++j) isums[(int64_t)o * m + j] = csums[(int64_t)c * m + j];
+        }
+    }
+}
+
+// Incidence keys: (group, level) for every (identifying cell, dimension).
+__global__ void k_inc_keys(int64_t M, int D, const int *__restrict__ cgroup,
+                           const int *__restrict__ icodes, int64_t ild,
+                           const int64_t *__restrict__ offs, int lbits,
+                           uint64_t *__restrict__ key, int *__restrict__ val) {
+    const int64_t n = M * D;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t c = e / D;
+        const int d = (int)(e - c * D);
+        key[e] = ((uint64_t)cgroup[c] << lbits) | (uint64_t)(offs[d] + icodes[d * ild + c]);
+        val[e] = (int)c;
+    }
+}
+
+__global__ void k_cell_group(int64_t Gi, const int *__restrict__ igstart, int *__restrict__ cg) {
+    for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < Gi;
+         g += (int64_t)gridDim.x * blockDim.x)
+        for (int c = igstart[g]; c < igstart[g + 1]; ++c) cg[c] = (int)g;
+}
+
+// Per incidence (a run of equal keys): group, level, u = sum n, s = sums.
+__global__ void k_inc_reduce(int64_t E, int m, int lbits, const uint64_t *__restrict__ key,
+                             const int *__restrict__ cell, const int *__restrict__ istart,
+                             const double *__restrict__ in, const double *__restrict__ isums,
+                             int *__restrict__ ig, int *__restrict__ il, double *__restrict__ iu,
+                             double *__restrict__ is) {
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < E;
+         q += (int64_t)gridDim.x * blockDim.x) {
+        const int s = istart[q], e = istart[q + 1];
+        const uint64_t k = key[s];
+        ig[q] = (int)(k >> lbits);
+        il[q] = (int)(k & ((1ull << lbits) - 1));
+        double u = 0.0;
+        for (int t = s; t < e; ++t) u += in[cell[t]];
+        iu[q] = u;
+        for (int j = 0; j < m; ++j) {
+            double v = 0.0;
+            for (int t = s; t < e; ++t) v += isums[(int64_t)cell[t] * m + j];
+            is[q * m + j] = v;
+        }
+    }
+}
+
+// Group weight N_g and vbar_g = sum of cell sums / N_g.
+__global__ void k_group_totals(int64_t Gi, int m, const int *__restrict__ igstart,
+                               const double *__restrict__ in, const double *__restrict__ isums,
+                               double *__restrict__ Ng, double *__restrict__ vbar) {
+    for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < Gi;
+         g += (int64_t)gridDim.x * blockDim.x) {
+        double N = 0.0;
+        for (int c = igstart[g]; c < igstart[g + 1]; ++c) N += in[c];
+        Ng[g] = N;
+        for (int j = 0; j < m; ++j) {
+            double v = 0.0;
+            for (int c = igstart[g]; c < igstart[g + 1]; ++c) v += isums[(int64_t)c * m + j];
+            vbar[g * m + j] = v / N;
+        }
+    }
+}
+
+// First incidence of each group (incidences are sorted by group, level).
+__global__ void k_group_inc_start(int64_t E, const int *__restrict__ ig,
+                                  int *__restrict__ gis, int64_t Gi) {
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < E;
+         q += (int64_t)gridDim.x * blockDim.x)
+        if (q == 0 || ig[q] != ig[q - 1]) gis[ig[q]] = (int)q;
+    if (blockIdx.x == 0 && threadIdx.x == 0) gis[Gi] = (int)E;
+}
+
+__global__ void k_count_levels(int64_t E, const int *__restrict__ il, int *__restrict__ cnt) {
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < E;
+         q += (int64_t)gridDim.x * blockDim.x)
+        atomicAdd(&cnt[il[q]], 1);
+}
+
+__global__ void k_level_keys(int64_t E, const int *__restrict__ ig, const int *__restrict__ il,
+                             int gbits, uint64_t *__restrict__ key, int *__restrict__ val) {
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < E;
+         q += (int64_t)gridDim.x * blockDim.x) {
+        key[q] = ((uint64_t)il[q] << gbits) | (uint64_t)ig[q];
+        val[q] = (int)q;
+    }
+}
+
+// diag[a] = sum over groups containing a of (u - u^2 / N_g)  (kernels_base.py:86-110)
+// rhs[a, j] = sum of (s_j - u vbar_j)                          (kernels_base.py:64-83)
+__global__ void k_diag_rhs(int64_t L, int m, const int *__restrict__ lptr,
+                           const int *__restrict__ lq, const int *__restrict__ ig,
+                           const double *__restrict__ iu, const double *__restrict__ is,
+                           const double *__restrict__ Ng, const double *__restrict__ vbar,
+                           double *__restrict__ diag, double *__restrict__ rhs) {
+    for (int64_t a = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; a < L;
+         a += (int64_t)gridDim.x * blockDim.x) {
+        double d = 0.0;
+        for (int t = lptr[a]; t < lptr[a + 1]; ++t) {
+            const int q = lq[t];
+            const double u = iu[q];
+            d += u - u * u / Ng[ig[q]];
+        }
+        diag[a] = d;
+        if (rhs)
+            for (int j = 0; j < m; ++j) {
+                double r = 0.0;
+                for (int t = lptr[a]; t < lptr[a + 1]; ++t) {
+                    const int q = lq[t];
+                    r += is[(int64_t)q * m + j] - iu[q] * vbar[(int64_t)ig[q] * m + j];
+                }
+                rhs[a * m + j] = r;
+            }
+    }
+}
+
+// Entries of S between sparse levels: per group, every ordered pair (a, b)
+// of its sparse incidences gives [a == b] u_a - u_a u_b / N_g; for sparse
+// dimensions d1 != d2 each cell adds n_c at its (a, b).  Count, then emit.
+__global__ void k_ss_count(int64_t Gi, const int *__restrict__ gis,
+                           const int *__restrict__ il, const uint8_t *__restrict__ sparse_lv,
+                           const int *__restrict__ igstart, int ncross,
+                           int64_t *__restrict__ cnt) {
+    for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < Gi;
+         g += (int64_t)gridDim.x * blockDim.x) {
+        int64_t q = 0;
+        for (int t = gis[g]; t < gis[g + 1]; ++t) q += sparse_lv[il[t]];
+        cnt[g] = q * q + (int64_t)ncross * (igstart[g + 1] - igstart[g]);
+    }
+}
+
+__global__ void k_ss_emit(int64_t Gi, int D, const int *__restrict__ gis,
+                          const int *__restrict__ il, const double *__restrict__ iu,
+                          const uint8_t *__restrict__ sparse_lv, const double *__restrict__ Ng,
+                          const int *__restrict__ igstart, const int *__restrict__ icodes,
+                          int64_t ild, const int64_t *__restrict__ offs,
+                          const uint8_t *__restrict__ sparse_dim, int lbits,
+                          const int64_t *__restrict__ pos, const double *__restrict__ in,
+                          uint64_t *__restrict__ key, double *__restrict__ val) {
+    for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < Gi;
+         g += (int64_t)gridDim.x * blockDim.x) {
+        int64_t o = pos[g];
+        const double N = Ng[g];
+        for (int t1 = gis[g]; t1 < gis[g + 1]; ++t1) {
+            const int a = il[t1];
+            if (!sparse_lv[a]) continue;
+            const double ua = iu[t1];
+            for (int t2 = gis[g]; t2 < gis[g + 1]; ++t2) {
+                const int b = il[t2];
+                if (!sparse_lv[b]) continue;
+                key[o] = ((uint64_t)a << lbits) | (uint64_t)b;
+                val[o] = (a == b ? ua : 0.0) - ua * iu[t2] / N;
+                ++o;
+            }
+        }
+        for (int c = igstart[g]; c < igstart[g + 1]; ++c)
+            for (int d1 = 0; d1 < D; ++d1) {
+                if (!sparse_dim[d1]) continue;
+                for (int d2 = 0; d2 < D; ++d2) {
+                    if (d2 == d1 || !sparse_dim[d2]) continue;
+                    key[o] = ((uint64_t)(offs[d1] + icodes[d1 * ild + c]) << lbits) |
+                             (uint64_t)(offs[d2] + icodes[d2 * ild + c]);
+                    val[o] = in[c];
+                    ++o;
+                }
+            }
+    }
+}
+
+// Sum runs of equal keys (sorted); out row/col/val at the run's rank.
+__global__ void k_reduce_runs(int64_t n, const uint64_t *__restrict__ key,
+                              const double *__restrict__ val, const int *__restrict__ head,
+                              const int *__restrict__ rank, int lbits, int *__restrict__ row,
+                              int *__restrict__ col, double *__restrict__ out) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        if (!head[i]) continue;
+        const uint64_t k = key[i];
+        double s = val[i];
+        for (int64_t t = i + 1; t < n && key[t] == k; ++t) s += val[t];
+        const int r = rank[i] - 1;
+        row[r] = (int)(k >> lbits);
+        col[r] = (int)(k & ((1ull << lbits) - 1));
+        out[r] = s;
+    }
+}
+
+// U[g, t] = u of dense level t in group g (0 when absent).
+__global__ void k_dense_u(int64_t E, const int *__restrict__ ig, const int *__restrict__ il,
+                          const double *__restrict__ iu, const int *__restrict__ dense_idx,
+                          int Ld, double *__restrict__ U) {
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < E;
+         q += (int64_t)gridDim.x * blockDim.x) {
+        const int t = dense_idx[il[q]];
+        if (t >= 0) U[(int64_t)ig[q] * Ld + t] = iu[q];
+    }
+}
+
+// Sparse row a x dense level t: sum over groups g containing a of
+// n_g(a, t) - u_a U[g, t] / N_g, present when t occurs in such a group.
+__global__ void k_sd(int64_t Ls, int Ld, const int *__restrict__ sparse_levels,
+                     const int *__restrict__ dense_levels, const int *__restrict__ lvl_dim,
+                     const int64_t *__restrict__ offs, const int *__restrict__ lptr,
+                     const int *__restrict__ lq, const int *__restrict__ ig,
+                     const double *__restrict__ iu, const double *__restrict__ Ng,
+                     const double *__restrict__ U, const int *__restrict__ igstart,
+                     const int *__restrict__ icodes, int64_t ild, const double *__restrict__ in,
+                     double *__restrict__ PD, uint8_t *__restrict__ PP) {
+    const int64_t n = Ls * (int64_t)Ld;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t s = e / Ld;
+        const int t = (int)(e - s * Ld);
+        const int a = sparse_levels[s];
+        const int da = lvl_dim[a];
+        const int la = a - (int)offs[da];
+        const int b = dense_levels[t];
+        const int db = lvl_dim[b];
+        const int lb = b - (int)offs[db];
+        double v = 0.0;
+        int pres = 0;
+        for (int p = lptr[a]; p < lptr[a + 1]; ++p) {
+            const int q = lq[p];
+            const int g = ig[q];
+            const double ut = U[(int64_t)g * Ld + t];
+            if (ut == 0.0) continue;
+            pres = 1;
+            double nat = 0.0;
+            for (int c = igstart[g]; c < igstart[g + 1]; ++c)
+                if (icodes[da * ild + c] == la && icodes[db * ild + c] == lb) nat += in[c];
+            v += nat - iu[q] * ut / Ng[g];
+        }
+        PD[e] = v;
+        PP[e] = (uint8_t)pres;
+    }
+}
+
+// Dense x dense block: per workgroup of groups, thread per (t1, t2) pair:
+// sum of n_g(t1, t2) - U_t1 U_t2 / N_g and a presence count; partials
+// reduced in a fixed order by k_dd_finish.
+__global__ void k_dd(int64_t Gi, int Ld, const int *__restrict__ dense_levels,
+                     const int *__restrict__ lvl_dim, const int64_t *__restrict__ offs,
+                     const double *__restrict__ U, const double *__restrict__ Ng,
+                     const int *__restrict__ igstart, const int *__restrict__ icodes,
+                     int64_t ild, const double *__restrict__ in, double *__restrict__ part,
+                     int *__restrict__ ppart) {
+    const int npair = Ld * Ld;
+    const int64_t g0 = part_begin(Gi, blockIdx.x, gridDim.x), g1 = part_begin(Gi, blockIdx.x + 1, gridDim.x);
+    for (int pr = threadIdx.x; pr < npair; pr += blockDim.x) {
+        const int t1 = pr / Ld, t2 = pr % Ld;
+        const int b1 = dense_levels[t1], b2 = dense_levels[t2];
+        const int d1 = lvl_dim[b1], d2 = lvl_dim[b2];
+        const int l1 = b1 - (int)offs[d1], l2 = b2 - (int)offs[d2];
+        double v = 0.0;
+        int pres = 0;
+        for (int64_t g = g0; g < g1; ++g) {
+            const double u1 = U[g * Ld + t1], u2 = U[g * Ld + t2];
+            if (u1 == 0.0 || u2 == 0.0) continue;
+            pres = 1;
+            double nat;
+            if (d1 == d2) {
+                nat = (t1 == t2) ? u1 : 0.0;
+            } else {
+                nat = 0.0;
+                for (int c = igstart[g]; c < igstart[g + 1]; ++c)
+                    if (icodes[d1 * ild + c] == l1 && icodes[d2 * ild + c] == l2) nat += in[c];
+            }
+            v += nat - u1 * u2 / Ng[g];
+        }
+        part[(int64_t)blockIdx.x * npair + pr] = v;
+        ppart[(int64_t)blockIdx.x * npair + pr] = pres;
+    }
+}
+
+__global__ void k_dd_finish(int nb, int npair, const double *__restrict__ part,
+                            const int *__restrict__ ppart, double *__restrict__ DD,
+                            uint8_t *__restrict__ DP) {
+    for (int pr = blockIdx.x * blockDim.x + threadIdx.x; pr < npair; pr += gridDim.x * blockDim.x) {
+        double v = 0.0;
+        int p = 0;
+        for (int b = 0; b < nb; ++b) {
+            v += part[(int64_t)b * npair + pr];
+            p |= ppart[(int64_t)b * npair + pr];
+        }
+        DD[pr] = v;
+        DP[pr] = (uint8_t)p;
+    }
+}
+
+// Row lengths of S.
+__global__ void k_row_nnz(int64_t L, int Ld, const int *__restrict__ ss_rowcnt,
+                          const int *__restrict__ sparse_idx, const int *__restrict__ dense_idx,
+                          const uint8_t *__restrict__ PP, const int *__restrict__ colpres,
+                          const uint8_t *__restrict__ DP, int *__restrict__ nnz) {
+    for (int64_t a = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; a < L;
+         a += (int64_t)gridDim.x * blockDim.x) {
+        int c = 0;
+        const int s = sparse_idx[a];
+        if (s >= 0) {
+            c = ss_rowcnt[s];
+            for (int t = 0; t < Ld; ++t) c += PP[(int64_t)s * Ld + t];
+        } else {
+            const int t = dense_idx[a];
+            c = colpres[t];
+            for (int t2 = 0; t2 < Ld; ++t2) c += DP[t * Ld + t2];
+        }
+        nnz[a] = c;
+    }
+}
+
+// Sparse rows: merge the sorted sparse columns with the present dense ones.
+__global__ void k_fill_sparse_rows(int64_t Ls, int Ld, const int *__restrict__ sparse_levels,
+                                   const int *__restrict__ dense_levels,
+                                   const int *__restrict__ ss_rowptr, const int *__restrict__ ss_col,
+                                   const double *__restrict__ ss_val, const uint8_t *__restrict__ PP,
+                                   const double *__restrict__ PD, const int *__restrict__ indptr,
+                                   int *__restrict__ indices, double *__restrict__ data) {
+    for (int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; s < Ls;
+         s += (int64_t)gridDim.x * blockDim.x) {
+        const int a = sparse_levels[s];
+        int o = indptr[a];
+        int p = ss_rowptr[s];
+        const int pe = ss_rowptr[s + 1];
+        int t = 0;
+        while (t < Ld && !PP[s * Ld + t]) ++t;
+        while (p < pe || t < Ld) {
+            const bool take_dense = t < Ld && (p >= pe || dense_levels[t] < ss_col[p]);
+            if (take_dense) {
+                indices[o] = dense_levels[t];
+                data[o] = PD[s * Ld + t];
+                ++t;
+                while (t < Ld && !PP[s * Ld + t]) ++t;
+            } else {
+                indices[o] = ss_col[p];
+                data[o] = ss_val[p];
+                ++p;
+            }
+            ++o;
+        }
+    }
+}
+
+// Dense rows, sparse columns: entry (t, a) at the rank of a among the
+// present sparse columns plus the present dense columns below a.
+__global__ void k_fill_dense_sparse(int64_t Ls, int Ld, const int *__restrict__ sparse_levels,
+                                    const int *__restrict__ dense_levels,
+                                    const uint8_t *__restrict__ PP, const double *__restrict__ PD,
+                                    const int *__restrict__ cpos, const uint8_t *__restrict__ DP,
+                                    const int *__restrict__ indptr, int *__restrict__ indices,
+                                    double *__restrict__ data) {
+    const int64_t n = Ls * (int64_t)Ld;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t s = e / Ld;
+        const int t = (int)(e - s * Ld);
+        if (!PP[e]) continue;
+        const int a = sparse_levels[s];
+        int below = 0;
+        for (int t2 = 0; t2 < Ld; ++t2) below += (DP[t * Ld + t2] && dense_levels[t2] < a);
+        const int o = indptr[dense_levels[t]] + cpos[(int64_t)t * (Ls + 1) + s] + below;
+        indices[o] = a;
+        data[o] = PD[e];
+    }
+}
+
+// Dense rows, dense columns.
+__global__ void k_fill_dense_dense(int64_t Ls, int Ld, const int *__restrict__ sparse_levels,
+                                   const int *__restrict__ dense_levels,
+                                   const int *__restrict__ cpos, const uint8_t *__restrict__ DP,
+                                   const double *__restrict__ DD, const int *__restrict__ indptr,
+                                   int *__restrict__ indices, double *__restrict__ data) {
+    const int npair = Ld * Ld;
+    for (int pr = blockIdx.x * blockDim.x + threadIdx.x; pr < npair; pr += gridDim.x * blockDim.x) {
+        const int t = pr / Ld, t2 = pr % Ld;
+        if (!DP[pr]) continue;
+        const int b = dense_levels[t2];
+        // present sparse columns below b: sparse levels are ascending
+        int lo = 0, hi = (int)Ls;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (sparse_levels[mid] < b) lo = mid + 1;
+            else hi = mid;
+        }
+        int below = cpos[(int64_t)t * (Ls + 1) + lo];
+        for (int u = 0; u < t2; ++u) below += DP[t * Ld + u];
+        const int o = indptr[dense_levels[t]] + below;
+        indices[o] = b;
+        data[o] = DD[pr];
+    }
+}
+
+__global__ void k_jacobi(int64_t L, const double *__restrict__ d, double *__restrict__ dinv) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < L;
+         i += (int64_t)gridDim.x * blockDim.x)
+        dinv[i] = d[i] > 1e-12 ? 1.0 / d[i] : 0.0;  // core/solve.py:142-146
+}
+
+__global__ void k_pp_col(int64_t Ls, int Ld, int t, const uint8_t *__restrict__ PP,
+                         int *__restrict__ col) {
+    for (int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; s < Ls;
+         s += (int64_t)gridDim.x * blockDim.x)
+        col[s] = PP[s * Ld + t];
+}
+
+__global__ void k_ss_rowcnt(int64_t n, const int *__restrict__ row, const int *__restrict__ sparse_idx,
+                            int *__restrict__ cnt) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        atomicAdd(&cnt[sparse_idx[row[i]]], 1);
+}
+
+// ------------------------------------------------------- components
+
+// Union-find on the levels of dimension 0 linked through identifying groups
+// (core/kernels_base.py:155-174); roots end as each component's smallest
+// level, whatever the order of the unions.
+__device__ int uf_root(int *parent, int x) {
+    while (true) {
+        int p = parent[x];
+        if (p == x) return x;
+        int gp = parent[p];
+        if (gp != p) atomicCAS(&parent[x], p, gp);
+        x = p;
+    }
+}
+
+__global__ void k_uf_init(int64_t n, int *parent) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        parent[i] = (int)i;
+}
+
+__global__ void k_uf_union(int64_t Gi, const int *__restrict__ igstart,
+                           const int *__restrict__ icodes, int *parent, int *changed) {
+    for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < Gi;
+         g += (int64_t)gridDim.x * blockDim.x) {
+        const int s = igstart[g], e = igstart[g + 1];
+        const int a0 = icodes[s];
+        for (int c = s + 1; c < e; ++c) {
+            int a = uf_root(parent, a0), b = uf_root(parent, icodes[c]);
+            while (a != b) {
+                if (a > b) {
+                    int t = a;
+                    a = b;
+                    b = t;
+                }
+                // hook the larger root under the smaller one
+                int old = atomicCAS(&parent[b], b, a);
+                if (old == b) {
+                    *changed = 1;
+                    break;
+                }
+                b = uf_root(parent, old);
+                a = uf_root(parent, a);
+            }
+        }
+    }
+}
+
+__global__ void k_uf_flatten(int64_t n, int *parent) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        parent[i] = uf_root(parent, (int)i);
+}
+
+// rows per component root (exact: integer counts)
+__global__ void k_comp_rows(int64_t n, const int *__restrict__ root,
+                            const double *__restrict__ cnt, unsigned long long *__restrict__ rows,
+                            int *__restrict__ is_root) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        atomicAdd(&rows[root[i]], (unsigned long long)cnt[i]);
+        is_root[i] = root[i] == (int)i;
+    }
+}
+
+// sort key of a root: rows descending, then root ascending (stable
+// argsort of -rows over roots in level order, core/passes.py:946-948)
+__global__ void k_comp_keys(int64_t n, const int *__restrict__ is_root,
+                            const unsigned long long *__restrict__ rows,
+                            const int *__restrict__ rpos, uint64_t *__restrict__ key,
+                            int *__restrict__ val) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        if (!is_root[i]) continue;
+        const int r = rpos[i];
+        key[r] = ((uint64_t)(0xFFFFFFFFull - (rows[i] & 0xFFFFFFFFull)) << 32) | (uint64_t)i;
+        val[r] = (int)i;
+    }
+}
+
+__global__ void k_comp_rank(int64_t nc, const int *__restrict__ sorted_roots,
+                            int *__restrict__ rank_of_root) {
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < nc;
+         r += (int64_t)gridDim.x * blockDim.x)
+        rank_of_root[sorted_roots[r]] = (int)r;
+}
+
+__global__ void k_comp_label(int64_t n, const int *__restrict__ root,
+                             const int *__restrict__ rank_of_root, int *__restrict__ comp) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        comp[i] = rank_of_root[root[i]];
+}
+
+__global__ void k_comp_level_keys(int64_t nl, const int *__restrict__ comp,
+                                  uint64_t *__restrict__ key, int *__restrict__ val,
+                                  int *__restrict__ ccnt) {
+    for (int64_t a = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; a < nl;
+         a += (int64_t)gridDim.x * blockDim.x) {
+        key[a] = ((uint64_t)comp[a] << 32) | (uint64_t)a;
+        val[a] = (int)a;
+        atomicAdd(&ccnt[comp[a]], 1);
+    }
+}
+
+// --------------------------------------------------- normalisation
+
+// Component means of Gamma's dimension-0 block, weighted by obs: levels
+// sorted by component (cptr over ncomp); thread per (component, column).
+__global__ void k_comp_means(int64_t nc, int k, const int *__restrict__ cptr,
+                             const int *__restrict__ clev, const double *__restrict__ obs,
+                             const double *__restrict__ Gm, double *__restrict__ mean) {
+    const int64_t n = nc * k;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t c = e / k;
+        const int j = (int)(e - c * k);
+        double sw = 0.0, s = 0.0;
+        for (int p = cptr[c]; p < cptr[c + 1]; ++p) {
+            const int a = clev[p];
+            sw += obs[a];
+            s += obs[a] * Gm[(int64_t)a * k + j];
+        }
+        mean[e] = s / (sw == 0.0 ? 1.0 : sw);
+    }
+}
+
+__global__ void k_sub_comp_means(int64_t nl, int k, const int *__restrict__ comp,
+                                 const double *__restrict__ mean, double *__restrict__ Gm) {
+    const int64_t n = nl * k;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t a = e / k;
+        const int j = (int)(e - a * k);
+        Gm[e] -= mean[(int64_t)comp[a] * k + j];
+    }
+}
+
+// Weighted column sums of a block of levels [o, o + nl) in fixed blocks.
+__global__ void k_wsum_block(int64_t nl, int k, int64_t o, const double *__restrict__ obs,
+                             const double *__restrict__ Gm, double *__restrict__ part) {
+    __shared__ double lds[T256];
+    const int64_t r0 = part_begin(nl, blockIdx.x, gridDim.x), r1 = part_begin(nl, blockIdx.x + 1, gridDim.x);
+    for (int j = 0; j <= k; ++j) {
+        double s = 0.0;
+        for (int64_t i = r0 + threadIdx.x; i < r1; i += blockDim.x)
+            s += j < k ? obs[o + i] * Gm[(o + i) * k + j] : obs[o + i];
+        s = block_sum<double>(s, lds);
+        if (threadIdx.x == 0) part[(int64_t)blockIdx.x * (k + 1) + j] = s;
+    }
+}
+
+__global__ void k_sub_global_mean(int64_t nl, int k, int64_t o, int nb,
+                                  const double *__restrict__ part, double *__restrict__ Gm) {
+    // every thread forms the means from the partials in the same order
+    const int64_t n = nl * k;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int j = (int)(e % k);
+        double s = 0.0, sw = 0.0;
+        for (int b = 0; b < nb; ++b) {
+            s += part[(int64_t)b * (k + 1) + j];
+            sw += part[(int64_t)b * (k + 1) + k];
+        }
+        Gm[o * k + e] -= s / (sw > 1.0 ? sw : 1.0);
+    }
+}
+
+// --------------------------------------------------- assembly / pass 2
+
+// V' M_D V between cells (core/kernels_base.py:510-540), m <= 8: per group
+// q_c = sums_c / n_c - sum_d Gamma[level] less the group mean.
+__global__ void k_assemble(int64_t Gi, int D, int m, const int *__restrict__ igstart,
+                           const int *__restrict__ icodes, int64_t ild,
+                           const int64_t *__restrict__ offs, const double *__restrict__ in,
+                           const double *__restrict__ isums, const double *__restrict__ Gm,
+                           double *__restrict__ part) {
+    __shared__ double lds[T256];
+    double A[64];
+    for (int t = 0; t < 64; ++t) A[t] = 0.0;
+    const int64_t g0 = part_begin(Gi, blockIdx.x, gridDim.x), g1 = part_begin(Gi, blockIdx.x + 1, gridDim.x);
+    for (int64_t g = g0 + threadIdx.x; g < g1; g += blockDim.x) {
+        const int s = igstart[g], e = igstart[g + 1];
+        double qg[8], N = 0.0;
+        for (int j = 0; j < m; ++j) qg[j] = 0.0;
+        for (int c = s; c < e; ++c) {
+            N += in[c];
+            for (int j = 0; j < m; ++j) {
+                double v = isums[(int64_t)c * m + j] / in[c];
+                for (int d = 0; d < D; ++d) v -= Gm[(offs[d] + icodes[d * ild + c]) * m + j];
+                qg[j] += in[c] * v;
+            }
+        }
+        for (int j = 0; j < m; ++j) qg[j] /= N;
+        for (int c = s; c < e; ++c) {
+            double q[8];
+            for (int j = 0; j < m; ++j) {
+                double v = isums[(int64_t)c * m + j] / in[c];
+                for (int d = 0; d < D; ++d) v -= Gm[(offs[d] + icodes[d * ild + c]) * m + j];
+                q[j] = v - qg[j];
+            }
+            for (int a = 0; a < m; ++a)
+                for (int b = 0; b < m; ++b) A[a * m + b] += in[c] * q[a] * q[b];
+        }
+    }
+    for (int t = 0; t < m * m; ++t) {
+        const double v = block_sum<double>(A[t], lds);
+        if (threadIdx.x == 0) part[(int64_t)blockIdx.x * m * m + t] = v;
+    }
+}
+
+// Streamed effects and residuals (core/kernels_base.py:543-619): per group
+// the weighted mean of y - x'beta - sum_d gam_y[level], the residual the
+// rest.  V column 0 is y, columns xcols[...] the regressors.
+__global__ void k_pass2(int64_t G, int D, int m, const int *__restrict__ gstart,
+                        const int *__restrict__ codes, int64_t ld,
+                        const int64_t *__restrict__ offs, const double *__restrict__ V,
+                        const double *__restrict__ w, const double *__restrict__ gam_y,
+                        const double *__restrict__ beta, const int *__restrict__ xcols, int kx,
+                        double *__restrict__ geff, double *__restrict__ resid) {
+    for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < G;
+         g += (int64_t)gridDim.x * blockDim.x) {
+        const int s = gstart[g], e = gstart[g + 1];
+        double Wg = 0.0, mu = 0.0;
+        for (int i = s; i < e; ++i) {
+            double u = V[(int64_t)i * m];
+            for (int j = 0; j < kx; ++j) u -= V[(int64_t)i * m + xcols[j]] * beta[j];
+            for (int d = 0; d < D; ++d) u -= gam_y[offs[d] + codes[d * ld + i]];
+            const double wi = w ? w[i] : 1.0;
+            Wg += wi;
+            mu += wi * u;
+            resid[i] = u;
+        }
+        mu /= Wg;
+        geff[g] = mu;
+        for (int i = s; i < e; ++i) resid[i] -= mu;
+    }
+}
+
+// gam_y = Gamma[:, 0] - Gamma[:, xcols] beta
+__global__ void k_gam_y(int64_t L, int m, const double *__restrict__ Gm,
+                        const double *__restrict__ beta, const int *__restrict__ xcols, int kx,
+                        double *__restrict__ out) {
+    for (int64_t a = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; a < L;
+         a += (int64_t)gridDim.x * blockDim.x) {
+        double v = Gm[a * m];
+        for (int j = 0; j < kx; ++j) v -= Gm[a * m + xcols[j]] * beta[j];
+        out[a] = v;
+    }
+}
+
+// ------------------------------------------------------------ host
+
+struct Panel {
+    hipStream_t s = nullptr;
+    Temp tmp;
+    int64_t n_in = 0, n = 0;  // rows given / kept
+    int nfe = 0, D = 0, m = 0, gdim = 0;
+    std::vector<int> dims;         // user index of each non-streamed dimension
+    std::vector<int64_t> nlev;     // their levels
+    std::vector<int64_t> offs_h;
+    int64_t G = 0, L = 0, C = 0, M = 0, Gi = 0, E = 0;
+    int weighted = 0;
+    int64_t singletons = 0, single_rounds = 0;
+    DevBuf<int> g, codes, gstart, cstart, cgstart;  // sorted rows; codes (D, n)
+    DevBuf<double> V, w;
+    DevBuf<int> perm;                      // sorted row -> input row
+    DevBuf<int64_t> glevels;               // streamed level values
+    std::vector<DevBuf<int64_t>> levels;   // other dimensions' level values
+    DevBuf<int64_t> offs;
+    DevBuf<double> means, raw_ss, W;
+    DevBuf<double> obs, cnt;
+    // identifying cells
+    DevBuf<int> icodes, igstart, igroup, cgroup;
+    DevBuf<double> in, isums;
+    // incidences
+    DevBuf<int> ig, il, gis, lptr, lq;
+    DevBuf<double> iu, is, Ng, vbar;
+    DevBuf<double> diag, rhs;
+    // components
+    DevBuf<int> comp, cptr, clev;
+    int64_t ncomp = 0;
+    // S
+    void *matrix = nullptr;
+    int64_t nnz = 0;
+    int n_sparse_ss = 0;
+    double t_build = 0.0;
+};
+
+static int factorize(Panel &P, const int64_t *col, int64_t n, DevBuf<int> &code,
+                     DevBuf<int64_t> &levels, int64_t &nl) {
+    hipStream_t s = P.s;
+    // min / max for the key range
+    const int nb = 256;
+    DevBuf<int64_t> part;
+    DEV_ALLOC(part, 2 * nb);
+    hipLaunchKernelGGL(k_minmax_partial, dim3(nb), dim3(T256), 0, s, n, col, part.p);
+    std::vector<int64_t> h;
+    HIP_TRY(to_host(h, part.p, 2 * nb, s));
+    int64_t lo = INT64_MAX, hi = INT64_MIN;
+    for (int b = 0; b < nb; ++b) {
+        lo = std::min(lo, h[2 * b]);
+        hi = std::max(hi, h[2 * b + 1]);
+    }
+    const int bits = std::max(1, bits_for((uint64_t)(hi - lo)));
+    DevBuf<uint64_t> k1, k2;
+    DevBuf<int> i1, i2, head, rank;
+    DEV_ALLOC(k1, n);
+    DEV_ALLOC(k2, n);
+    DEV_ALLOC(i1, n);
+    DEV_ALLOC(i2, n);
+    DEV_ALLOC(head, n);
+    DEV_ALLOC(rank, n);
+    hipLaunchKernelGGL(k_key_from_col, dim3(grid1(n)), dim3(T256), 0, s, n, col, lo, k1.p, i1.p);
+    HIP_TRY(sort_pairs(P.tmp, k1.p, k2.p, i1.p, i2.p, n, 0, bits, s));
+    hipLaunchKernelGGL(k_heads_u64, dim3(grid1(n)), dim3(T256), 0, s, n, k2.p, head.p);
+    HIP_TRY(inclusive_sum(P.tmp, head.p, rank.p, n, s));
+    int last = 0;
+    HIP_TRY(scalar_to_host(&last, rank.p + n - 1, s));
+    nl = last;
+    DEV_ALLOC(code, n);
+    DEV_ALLOC(levels, nl);
+    hipLaunchKernelGGL(k_scatter_codes, dim3(grid1(n)), dim3(T256), 0, s, n, k2.p, lo, i2.p,
+                       head.p, rank.p, code.p, levels.p);
+    HIP_TRY(hipGetLastError());
+    return HDFE_STATUS_OK;
+}
+
+#define TRY(expr)                              \
+    do {                                       \
+        int _st = (expr);                      \
+        if (_st != HDFE_STATUS_OK) return _st; \
+    } while (0)
+
+// Re-code dimension values to the levels present among kept rows.
+static int recode(Panel &P, int64_t n, DevBuf<int> &code, DevBuf<int64_t> &levels, int64_t &nl,
+                  const uint8_t *keep) {
+    hipStream_t s = P.s;
+    DevBuf<int> present, newcode;
+    DEV_ALLOC(present, nl + 1);
+    DEV_ALLOC(newcode, nl + 1);
+    HIP_TRY(hipMemsetAsync(present.p, 0, (nl + 1) * 4, s));
+    hipLaunchKernelGGL(k_level_present, dim3(grid1(n)), dim3(T256), 0, s, n, code.p, keep, present.p);
+    HIP_TRY(exclusive_sum(P.tmp, present.p, newcode.p, nl + 1, s));
+    int cnt = 0;
+    HIP_TRY(scalar_to_host(&cnt, newcode.p + nl, s));
+    DevBuf<int64_t> lv;
+    DEV_ALLOC(lv, cnt);
+    hipLaunchKernelGGL(k_compact_levels, dim3(grid1(nl)), dim3(T256), 0, s, nl, present.p,
+                       newcode.p, levels.p, lv.p);
+    hipLaunchKernelGGL(k_recode, dim3(grid1(n)), dim3(T256), 0, s, n, code.p, newcode.p);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(s));
+    levels = std::move(lv);
+    nl = cnt;
+    return HDFE_STATUS_OK;
+}
+
+static double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+}  // namespace hdfe
+
+#include <chrono>
+
+using namespace hdfe;
+
+extern "C" {
+
+void hdfe_destroy(void *h);  // pcg.hip
+
+// Passes 0 and 1 on the device.  fe: nfe int64 id columns (device), vars:
+// m float64 columns (device, y first), weights: NULL or float32/float64
+// (device).  stream_dim: -1 to choose the dimension with the most levels.
+// sort_vars: also order rows by the variables (the canonical row order of a
+// fit whose intermediates are kept, core/passes.py:340-368).
+int hdfe_panel_create(int64_t n_rows, int nfe, const int64_t *const *fe, int m,
+                      const double *const *vars, const void *weights, int weights_f32,
+                      int fixef_rm_singleton, int stream_dim, int sort_vars, void **out) {
+    ARG_CHECK(out && fe && vars, "hdfe_panel_create: null argument");
+    ARG_CHECK(n_rows > 0 && n_rows < INT32_MAX, "hdfe_panel_create: rows must be in [1, 2^31)");
+    ARG_CHECK(nfe >= 1 && nfe <= 8, "hdfe_panel_create: 1 to 8 fixed effects");
+    ARG_CHECK(m >= 1 && m <= 64, "hdfe_panel_create: 1 to 64 variables");
+    std::unique_ptr<Panel> P(new Panel());
+    const auto t0 = std::chrono::steady_clock::now();
+    HIP_TRY(hipStreamCreateWithFlags(&P->s, hipStreamNonBlocking));
+    hipStream_t s = P->s;
+    const int64_t n = n_rows;
+    P->n_in = n;
+    P->nfe = nfe;
+    P->m = m;
+    P->weighted = weights != nullptr;
+    // pass 0: codes for every dimension
+    std::vector<DevBuf<int>> code(nfe);
+    std::vector<DevBuf<int64_t>> lv(nfe);
+    std::vector<int64_t> nl(nfe);
+    for (int d = 0; d < nfe; ++d) TRY(factorize(*P, fe[d], n, code[d], lv[d], nl[d]));
+    DevBuf<uint8_t> keep;
+    DEV_ALLOC(keep, n);
+    HIP_TRY(hipMemsetAsync(keep.p, 1, n, s));
+    if (fixef_rm_singleton) {
+        // dimensions in turn until each has been checked since the last drop
+        DevBuf<int> cnt;
+        int64_t maxl = *std::max_element(nl.begin(), nl.end());
+        DEV_ALLOC(cnt, maxl);
+        DevBuf<unsigned long long> dropped;
+        DEV_ALLOC(dropped, 1);
+        bool found = false;
+        int quiet = 0;
+        int64_t i = 0;
+        std::vector<int64_t> rounds;
+        while (quiet < nfe - (found ? 1 : 0)) {
+            const int d = (int)(i % nfe);
+            HIP_TRY(hipMemsetAsync(cnt.p, 0, nl[d] * 4, s));
+            HIP_TRY(hipMemsetAsync(dropped.p, 0, 8, s));
+            hipLaunchKernelGGL(k_level_count, dim3(grid1(n)), dim3(T256), 0, s, n, code[d].p,
+                               keep.p, cnt.p);
+            hipLaunchKernelGGL(k_drop_single, dim3(grid1(n)), dim3(T256), 0, s, n, code[d].p,
+                               cnt.p, keep.p, dropped.p);
+            unsigned long long dr = 0;
+            HIP_TRY(scalar_to_host(&dr, dropped.p, s));
+            if (dr) {
+                found = true;
+                quiet = 0;
+                P->singletons += (int64_t)dr;
+                const int64_t rd = i / nfe;
+                if (rounds.empty() || rounds.back() != rd) rounds.push_back(rd);
+            } else {
+                ++quiet;
+            }
+            ++i;
+        }
+        P->single_rounds = (int64_t)rounds.size();
+        if (P->singletons)
+            for (int d = 0; d < nfe; ++d) TRY(recode(*P, n, code[d], lv[d], nl[d], keep.p));
+    }
+    P->n = n - P->singletons;
+    ARG_CHECK(P->n > 0, "no complete observations left after dropping singletons");
+    // the streamed dimension: the most levels, ties to the first listed
+    int gd = stream_dim;
+    if (gd < 0) {
+        gd = 0;
+        for (int d = 1; d < nfe; ++d)
+            if (nl[d] > nl[gd]) gd = d;
+    }
+    ARG_CHECK(gd < nfe, "hdfe_panel_create: stream_dim out of range");
+    P->gdim = gd;
+    for (int d = 0; d < nfe; ++d)
+        if (d != gd) P->dims.push_back(d);
+    const int D = (int)P->dims.size();
+    P->D = D;
+    P->G = nl[gd];
+    int64_t o = 0;
+    for (int t = 0; t < D; ++t) {
+        P->nlev.push_back(nl[P->dims[t]]);
+        P->offs_h.push_back(o);
+        o += nl[P->dims[t]];
+    }
+    P->L = o;
+    DEV_ALLOC(P->offs, std::max(D, 1));
+    if (D) HIP_TRY(hipMemcpyAsync(P->offs.p, P->offs_h.data(), D * 8, hipMemcpyHostToDevice, s));
+    // row order: (group, codes) packed, optionally after stable sorts by the
+    // variables (least significant first)
+    const int gbits = std::max(1, bits_for((uint64_t)P->G));
+    std::vector<int> shift(D);
+    int tot = 0;
+    for (int t = D - 1; t >= 0; --t) {
+        shift[t] = tot;
+        tot += std::max(1, bits_for((uint64_t)P->nlev[t]));
+    }
+    ARG_CHECK(tot + gbits <= 63, "hdfe_panel_create: level codes need more than 63 key bits");
+    DevBuf<int> codes_in, shift_d;
+    DEV_ALLOC(codes_in, std::max<int64_t>(1, (int64_t)D * n));
+    DEV_ALLOC(shift_d, std::max(D, 1));
+    for (int t = 0; t < D; ++t)
+        HIP_TRY(hipMemcpyAsync(codes_in.p + t * n, code[P->dims[t]].p, n * 4,
+                               hipMemcpyDeviceToDevice, s));
+    if (D) HIP_TRY(hipMemcpyAsync(shift_d.p, shift.data(), D * 4, hipMemcpyHostToDevice, s));
+    DevBuf<uint64_t> full, k1, k2;
+    DevBuf<int> i1, i2;
+    DEV_ALLOC(full, n);
+    DEV_ALLOC(k1, n);
+    DEV_ALLOC(k2, n);
+    DEV_ALLOC(i1, n);
+    DEV_ALLOC(i2, n);
+    hipLaunchKernelGGL(k_pack_key, dim3(grid1(n)), dim3(T256), 0, s, n, D, code[gd].p, codes_in.p,
+                       n, shift_d.p, tot, keep.p, full.p, i1.p);
+    DevBuf<double> Vin;
+    DevBuf<const double *> colp;
+    DEV_ALLOC(colp, m);
+    HIP_TRY(hipMemcpyAsync(colp.p, vars, m * sizeof(double *), hipMemcpyHostToDevice, s));
+    if (sort_vars) {
+        DEV_ALLOC(Vin, n * m);
+        DevBuf<int> ident;
+        DEV_ALLOC(ident, n);
+        // V in input order, row major, for the variable keys
+        hipLaunchKernelGGL(k_iota, dim3(grid1(n)), dim3(T256), 0, s, n, ident.p);
+        hipLaunchKernelGGL(k_gather_vars, dim3(grid1(n * m)), dim3(T256), 0, s, n, m, colp.p,
+                           ident.p, Vin.p);
+        bool first = true;
+        for (int j = m - 1; j >= 0; --j) {
+            hipLaunchKernelGGL(k_var_key, dim3(grid1(n)), dim3(T256), 0, s, n, Vin.p, m, j,
+                               first ? (const int *)nullptr : (const int *)i1.p, k1.p, i2.p);
+            HIP_TRY(sort_pairs(P->tmp, k1.p, k2.p, i2.p, i1.p, n, 0, 64, s));
+            first = false;
+        }
+        hipLaunchKernelGGL(k_gather_key, dim3(grid1(n)), dim3(T256), 0, s, n, full.p, i1.p, k1.p);
+    } else {
+        HIP_TRY(hipMemcpyAsync(k1.p, full.p, n * 8, hipMemcpyDeviceToDevice, s));
+    }
+    HIP_TRY(sort_pairs(P->tmp, k1.p, k2.p, i1.p, i2.p, n, 0, 64, s));
+    const int64_t nk = P->n;
+    P->perm = std::move(i2);
+    // gather the sorted columns (kept rows come first)
+    DEV_ALLOC(P->g, nk);
+    hipLaunchKernelGGL(k_gather<int>, dim3(grid1(nk)), dim3(T256), 0, s, nk, P->perm.p,
+                       code[gd].p, P->g.p);
+    DEV_ALLOC(P->codes, std::max<int64_t>(1, (int64_t)D * nk));
+    for (int t = 0; t < D; ++t)
+        hipLaunchKernelGGL(k_gather<int>, dim3(grid1(nk)), dim3(T256), 0, s, nk, P->perm.p,
+                           code[P->dims[t]].p, P->codes.p + t * nk);
+    DEV_ALLOC(P->V, nk * m);
+    hipLaunchKernelGGL(k_gather_vars, dim3(grid1(nk * m)), dim3(T256), 0, s, nk, m, colp.p,
+                       P->perm.p, P->V.p);
+    if (weights) {
+        DEV_ALLOC(P->w, nk);
+        hipLaunchKernelGGL(k_gather_weights, dim3(grid1(nk)), dim3(T256), 0, s, nk, weights,
+                           weights_f32, P->perm.p, P->w.p);
+    }
+    P->glevels = std::move(lv[gd]);
+    for (int t = 0; t < D; ++t) P->levels.push_back(std::move(lv[P->dims[t]]));
+    HIP_TRY(hipGetLastError());
+    // free pass-0 scratch
+    code.clear();
+    full.reset();
+    k1.reset();
+    k2.reset();
+    i1.reset();
+    Vin.reset();
+    keep.reset();
+    codes_in.reset();
+
+    // pass 1: means, cells, groups
+    const int nb = 512;
+    DevBuf<double> part;
+    DEV_ALLOC(part, (int64_t)nb * std::max(m * m, m) + 64);
+    std::vector<double> hp;
+    DEV_ALLOC(P->means, m);
+    DEV_ALLOC(P->raw_ss, m);
+    hipLaunchKernelGGL(k_colsum_rows, dim3(nb), dim3(T256), 0, s, nk, m, P->V.p, part.p);
+    HIP_TRY(to_host(hp, part.p, (int64_t)nb * m, s));
+    std::vector<double> mean(m, 0.0), ss(m, 0.0);
+    for (int b = 0; b < nb; ++b)
+        for (int j = 0; j < m; ++j) mean[j] += hp[(size_t)b * m + j];
+    for (int j = 0; j < m; ++j) mean[j] /= (double)nk;
+    HIP_TRY(hipMemcpyAsync(P->means.p, mean.data(), m * 8, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_colss_rows, dim3(nb), dim3(T256), 0, s, nk, m, P->V.p, P->means.p, part.p);
+    HIP_TRY(to_host(hp, part.p, (int64_t)nb * m, s));
+    for (int b = 0; b < nb; ++b)
+        for (int j = 0; j < m; ++j) ss[j] += hp[(size_t)b * m + j];
+    HIP_TRY(hipMemcpyAsync(P->raw_ss.p, ss.data(), m * 8, hipMemcpyHostToDevice, s));
+    DevBuf<int> gh, ch, grank, crank;
+    DEV_ALLOC(gh, nk);
+    DEV_ALLOC(ch, nk);
+    DEV_ALLOC(grank, nk);
+    DEV_ALLOC(crank, nk);
+    hipLaunchKernelGGL(k_row_heads, dim3(grid1(nk)), dim3(T256), 0, s, nk, D, P->g.p, P->codes.p,
+                       nk, gh.p, ch.p);
+    HIP_TRY(inclusive_sum(P->tmp, gh.p, grank.p, nk, s));
+    HIP_TRY(inclusive_sum(P->tmp, ch.p, crank.p, nk, s));
+    int ng = 0, nc = 0;
+    HIP_TRY(scalar_to_host(&ng, grank.p + nk - 1, s));
+    HIP_TRY(scalar_to_host(&nc, crank.p + nk - 1, s));
+    ARG_CHECK(ng == P->G, "internal: group count mismatch after sorting");
+    P->C = nc;
+    DEV_ALLOC(P->gstart, ng + 1);
+    DEV_ALLOC(P->cstart, nc + 1);
+    hipLaunchKernelGGL(k_starts, dim3(grid1(nk)), dim3(T256), 0, s, nk, gh.p, grank.p, P->gstart.p);
+    hipLaunchKernelGGL(k_starts, dim3(grid1(nk)), dim3(T256), 0, s, nk, ch.p, crank.p, P->cstart.p);
+    // group starts over cells: the cell rank of each group's first row
+    DevBuf<int> cghead, cgrank;
+    DEV_ALLOC(cghead, nc);
+    DEV_ALLOC(cgrank, nc);
+    DEV_ALLOC(P->cgstart, ng + 1);
+    {
+        // a cell starts a group when its first row does
+        hipLaunchKernelGGL(k_gather<int>, dim3(grid1(nc)), dim3(T256), 0, s, (int64_t)nc,
+                           P->cstart.p, gh.p, cghead.p);
+        HIP_TRY(inclusive_sum(P->tmp, cghead.p, cgrank.p, nc, s));
+        hipLaunchKernelGGL(k_starts, dim3(grid1(nc)), dim3(T256), 0, s, (int64_t)nc, cghead.p,
+                           cgrank.p, P->cgstart.p);
+    }
+    gh.reset();
+    ch.reset();
+    grank.reset();
+    crank.reset();
+    DevBuf<double> cn, crun, csums, wpart;
+    DEV_ALLOC(cn, nc);
+    DEV_ALLOC(crun, nc);
+    DEV_ALLOC(csums, (int64_t)nc * m);
+    const int mm = m <= 8 ? m * m : 0;
+    DEV_ALLOC(wpart, (int64_t)nb * std::max(mm, 1));
+    hipLaunchKernelGGL(k_cells, dim3(nb), dim3(T256), 0, s, (int64_t)nc, m, P->cstart.p, P->V.p,
+                       P->weighted ? P->w.p : (const double *)nullptr, P->means.p, cn.p, crun.p,
+                       csums.p, wpart.p);
+    DEV_ALLOC(P->W, std::max(mm, 1));
+    if (mm) {
+        HIP_TRY(to_host(hp, wpart.p, (int64_t)nb * mm, s));
+        std::vector<double> Wh(mm, 0.0);
+        for (int b = 0; b < nb; ++b)
+            for (int t = 0; t < mm; ++t) Wh[t] += hp[(size_t)b * mm + t];
+        HIP_TRY(hipMemcpyAsync(P->W.p, Wh.data(), mm * 8, hipMemcpyHostToDevice, s));
+    }
+    const int64_t L = P->L;
+    DEV_ALLOC(P->obs, std::max<int64_t>(L, 1));
+    DEV_ALLOC(P->cnt, std::max<int64_t>(L, 1));
+    {
+        DevBuf<unsigned long long> ic;
+        DEV_ALLOC(ic, std::max<int64_t>(L, 1));
+        HIP_TRY(hipMemsetAsync(ic.p, 0, std::max<int64_t>(L, 1) * 8, s));
+        HIP_TRY(hipMemsetAsync(P->obs.p, 0, std::max<int64_t>(L, 1) * 8, s));
+        if (D)
+            hipLaunchKernelGGL(k_obs_cnt, dim3(grid1(nc)), dim3(T256), 0, s, (int64_t)nc, D,
+                               P->cstart.p, P->codes.p, nk, P->offs.p, cn.p, ic.p, P->obs.p,
+                               P->weighted);
+        hipLaunchKernelGGL(k_u64_to_f64, dim3(grid1(L)), dim3(T256), 0, s, L, ic.p, P->cnt.p,
+                           P->weighted ? (double *)nullptr : P->obs.p);
+        HIP_TRY(hipStreamSynchronize(s));
+    }
+    // identifying groups and their cells
+    DevBuf<int> ident, gpos, gcells, cpos;
+    DEV_ALLOC(ident, ng + 1);
+    DEV_ALLOC(gpos, ng + 1);
+    DEV_ALLOC(gcells, ng + 1);
+    DEV_ALLOC(cpos, ng + 1);
+    HIP_TRY(hipMemsetAsync(ident.p, 0, (ng + 1) * 4, s));
+    hipLaunchKernelGGL(k_group_cells, dim3(grid1(ng)), dim3(T256), 0, s, (int64_t)ng,
+                       P->cgstart.p, ident.p);
+    HIP_TRY(exclusive_sum(P->tmp, ident.p, gpos.p, ng + 1, s));
+    int Gi = 0;
+    HIP_TRY(scalar_to_host(&Gi, gpos.p + ng, s));
+    P->Gi = Gi;
+    {
+        // cells of identifying groups, as an exclusive sum over groups
+        hipLaunchKernelGGL(k_ident_ncells, dim3(grid1(ng + 1)), dim3(T256), 0, s, (int64_t)ng,
+                           P->cgstart.p, ident.p, gcells.p);
+        HIP_TRY(exclusive_sum(P->tmp, gcells.p, cpos.p, ng + 1, s));
+        int Mi = 0;
+        HIP_TRY(scalar_to_host(&Mi, cpos.p + ng, s));
+        P->M = Mi;
+    }
+    const int64_t M = P->M;
+    DEV_ALLOC(P->icodes, std::max<int64_t>(1, (int64_t)D * M));
+    DEV_ALLOC(P->in, std::max<int64_t>(M, 1));
+    DEV_ALLOC(P->isums, std::max<int64_t>(M * m, 1));
+    DEV_ALLOC(P->igstart, Gi + 1);
+    DEV_ALLOC(P->igroup, std::max(Gi, 1));
+    hipLaunchKernelGGL(k_ident_cells, dim3(grid1(ng)), dim3(T256), 0, s, (int64_t)ng, D, m,
+                       P->cgstart.p, ident.p, gpos.p, cpos.p, P->cstart.p, P->codes.p, nk, cn.p,
+                       csums.p, P->icodes.p, M, P->in.p, P->isums.p, P->igstart.p, P->igroup.p);
+    HIP_TRY(hipMemcpyAsync(P->igstart.p + Gi, cpos.p + ng, 4, hipMemcpyDeviceToDevice, s));
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(s));
+    P->t_build = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    *out = P.release();
+    return HDFE_STATUS_OK;
+}
+
+// Integer facts of the panel: [n_in, n, G, C, M, Gi, L, D, gdim, singletons,
+// single_rounds, E, nnz, ncomp, m, weighted].
+int hdfe_panel_info(void *h, int64_t *info, int64_t *nlev, int *dims) {
+    ARG_CHECK(h && info, "hdfe_panel_info: null argument");
+    Panel &P = *(Panel *)h;
+    int64_t v[16] = {P.n_in, P.n, P.G, P.C, P.M, P.Gi, P.L, P.D, P.gdim, P.singletons,
+                     P.single_rounds, P.E, P.nnz, P.ncomp, P.m, P.weighted};
+    for (int i = 0; i < 16; ++i) info[i] = v[i];
+    for (int t = 0; t < P.D; ++t) {
+        if (nlev) nlev[t] = P.nlev[t];
+        if (dims) dims[t] = P.dims[t];
+    }
+    return HDFE_STATUS_OK;
+}
+
+// Device pointers of the panel's arrays, by name.
+int hdfe_panel_array(void *h, const char *name, void **ptr, int64_t *count) {
+    ARG_CHECK(h && name && ptr, "hdfe_panel_array: null argument");
+    Panel &P = *(Panel *)h;
+    std::string nm(name);
+    void *p = nullptr;
+    int64_t c = 0;
+    if (nm == "g") { p = P.g.p; c = P.n; }
+    else if (nm == "codes") { p = P.codes.p; c = P.D * P.n; }
+    else if (nm == "V") { p = P.V.p; c = P.n * P.m; }
+    else if (nm == "w") { p = P.w.p; c = P.weighted ? P.n : 0; }
+    else if (nm == "perm") { p = P.perm.p; c = P.n; }
+    else if (nm == "gstart") { p = P.gstart.p; c = P.G + 1; }
+    else if (nm == "cstart") { p = P.cstart.p; c = P.C + 1; }
+    else if (nm == "glevels") { p = P.glevels.p; c = P.G; }
+    else if (nm == "means") { p = P.means.p; c = P.m; }
+    else if (nm == "raw_ss") { p = P.raw_ss.p; c = P.m; }
+    else if (nm == "W") { p = P.W.p; c = P.m <= 8 ? P.m * P.m : 0; }
+    else if (nm == "obs") { p = P.obs.p; c = P.L; }
+    else if (nm == "cnt") { p = P.cnt.p; c = P.L; }
+    else if (nm == "icodes") { p = P.icodes.p; c = P.D * P.M; }
+    else if (nm == "in") { p = P.in.p; c = P.M; }
+    else if (nm == "isums") { p = P.isums.p; c = P.M * P.m; }
+    else if (nm == "igstart") { p = P.igstart.p; c = P.Gi + 1; }
+    else if (nm == "igroup") { p = P.igroup.p; c = P.Gi; }
+    else if (nm == "diag") { p = P.diag.p; c = P.diag.p ? P.L : 0; }
+    else if (nm == "rhs") { p = P.rhs.p; c = P.rhs.p ? P.L * P.m : 0; }
+    else if (nm == "comp") { p = P.comp.p; c = P.comp.p ? (P.D ? P.nlev[0] : 0) : 0; }
+    else if (nm == "offs") { p = P.offs.p; c = P.D; }
+    else if (nm.rfind("levels", 0) == 0) {
+        const int t = std::atoi(nm.c_str() + 6);
+        ARG_CHECK(t >= 0 && t < P.D, "hdfe_panel_array: no such levels array");
+        p = P.levels[t].p;
+        c = P.nlev[t];
+    } else {
+        set_error("hdfe_panel_array: unknown array " + nm);
+        return HDFE_STATUS_ARG;
+    }
+    *ptr = p;
+    if (count) *count = c;
+    return HDFE_STATUS_OK;
+}
+
+// Step 2's system: incidences, diag, rhs, the explicit S and its solver
+// handle (*matrix: owned by the panel; do not destroy).  with_rhs: build the
+// right-hand side of the panel's own variables.
+int hdfe_panel_system(void *h, int with_rhs, void **matrix, double *seconds) {
+    ARG_CHECK(h, "hdfe_panel_system: null handle");
+    Panel &P = *(Panel *)h;
+    hipStream_t s = P.s;
+    const auto t0 = std::chrono::steady_clock::now();
+    const int D = P.D, m = P.m;
+    const int64_t L = P.L, M = P.M, Gi = P.Gi;
+    ARG_CHECK(D >= 1 && L >= 1, "hdfe_panel_system: no dimension besides the streamed one");
+    // incidences (group, level) of the identifying cells
+    const int lbits = std::max(1, bits_for((uint64_t)L));
+    const int gbits = std::max(1, bits_for((uint64_t)std::max<int64_t>(Gi, 1)));
+    ARG_CHECK(lbits + gbits <= 64, "hdfe_panel_system: incidence keys need more than 64 bits");
+    const int64_t nE = M * D;
+    DEV_ALLOC(P.cgroup, std::max<int64_t>(M, 1));
+    hipLaunchKernelGGL(k_cell_group, dim3(grid1(Gi)), dim3(T256), 0, s, Gi, P.igstart.p, P.cgroup.p);
+    DevBuf<uint64_t> k1, k2;
+    DevBuf<int> v1, v2, head, rank;
+    DEV_ALLOC(k1, std::max<int64_t>(nE, 1));
+    DEV_ALLOC(k2, std::max<int64_t>(nE, 1));
+    DEV_ALLOC(v1, std::max<int64_t>(nE, 1));
+    DEV_ALLOC(v2, std::max<int64_t>(nE, 1));
+    DEV_ALLOC(head, std::max<int64_t>(nE, 1));
+    DEV_ALLOC(rank, std::max<int64_t>(nE, 1));
+    int E = 0;
+    if (nE) {
+        hipLaunchKernelGGL(k_inc_keys, dim3(grid1(nE)), dim3(T256), 0, s, M, D, P.cgroup.p,
+                           P.icodes.p, M, P.offs.p, lbits, k1.p, v1.p);
+        HIP_TRY(sort_pairs(P.tmp, k1.p, k2.p, v1.p, v2.p, nE, 0, lbits + gbits, s));
+        hipLaunchKernelGGL(k_heads_u64, dim3(grid1(nE)), dim3(T256), 0, s, nE, k2.p, head.p);
+        HIP_TRY(inclusive_sum(P.tmp, head.p, rank.p, nE, s));
+        HIP_TRY(scalar_to_host(&E, rank.p + nE - 1, s));
+    }
+    P.E = E;
+    DevBuf<int> istart;
+    DEV_ALLOC(istart, E + 1);
+    if (nE)
+        hipLaunchKernelGGL(k_starts, dim3(grid1(nE)), dim3(T256), 0, s, nE, head.p, rank.p, istart.p);
+    DEV_ALLOC(P.ig, std::max(E, 1));
+    DEV_ALLOC(P.il, std::max(E, 1));
+    DEV_ALLOC(P.iu, std::max(E, 1));
+    DEV_ALLOC(P.is, std::max<int64_t>((int64_t)E * m, 1));
+    DEV_ALLOC(P.Ng, std::max<int64_t>(Gi, 1));
+    DEV_ALLOC(P.vbar, std::max<int64_t>(Gi * m, 1));
+    DEV_ALLOC(P.gis, Gi + 1);
+    hipLaunchKernelGGL(k_inc_reduce, dim3(grid1(E)), dim3(T256), 0, s, (int64_t)E, m, lbits, k2.p,
+                       v2.p, istart.p, P.in.p, P.isums.p, P.ig.p, P.il.p, P.iu.p, P.is.p);
+    hipLaunchKernelGGL(k_group_totals, dim3(grid1(Gi)), dim3(T256), 0, s, Gi, m, P.igstart.p,
+                       P.in.p, P.isums.p, P.Ng.p, P.vbar.p);
+    hipLaunchKernelGGL(k_group_inc_start, dim3(grid1(E)), dim3(T256), 0, s, (int64_t)E, P.ig.p,
+                       P.gis.p, Gi);
+    // level CSR of the incidences (sorted by level, then group)
+    DevBuf<int> lcnt;
+    DEV_ALLOC(lcnt, L + 1);
+    DEV_ALLOC(P.lptr, L + 1);
+    DEV_ALLOC(P.lq, std::max(E, 1));
+    HIP_TRY(hipMemsetAsync(lcnt.p, 0, (L + 1) * 4, s));
+    hipLaunchKernelGGL(k_count_levels, dim3(grid1(E)), dim3(T256), 0, s, (int64_t)E, P.il.p, lcnt.p);
+    HIP_TRY(exclusive_sum(P.tmp, lcnt.p, P.lptr.p, L + 1, s));
+    if (E) {
+        hipLaunchKernelGGL(k_level_keys, dim3(grid1(E)), dim3(T256), 0, s, (int64_t)E, P.ig.p,
+                           P.il.p, gbits, k1.p, v1.p);
+        HIP_TRY(sort_pairs(P.tmp, k1.p, k2.p, v1.p, P.lq.p, E, 0, lbits + gbits, s));
+    }
+    DEV_ALLOC(P.diag, L);
+    if (with_rhs) DEV_ALLOC(P.rhs, L * m);
+    hipLaunchKernelGGL(k_diag_rhs, dim3(grid1(L)), dim3(T256), 0, s, L, m, P.lptr.p, P.lq.p, P.ig.p,
+                       P.iu.p, P.is.p, P.Ng.p, P.vbar.p, P.diag.p,
+                       with_rhs ? P.rhs.p : (double *)nullptr);
+    HIP_TRY(hipGetLastError());
+
+    // dense / sparse dimensions and their levels (ascending global index)
+    std::vector<uint8_t> sparse_dim(D), sparse_lv(L);
+    std::vector<int> sparse_idx(L, -1), dense_idx(L, -1), lvl_dim(L), sparse_levels, dense_levels;
+    {
+        // smallest dimensions first while the dense levels fit kDenseMax
+        std::vector<int> order(D);
+        std::iota(order.begin(), order.end(), 0);
+        std::stable_sort(order.begin(), order.end(),
+                         [&](int a, int b) { return P.nlev[a] < P.nlev[b]; });
+        int64_t ld = 0;
+        for (int t = 0; t < D; ++t) sparse_dim[t] = 1;
+        for (int t : order) {
+            if (P.nlev[t] <= kDenseMax && ld + P.nlev[t] <= kDenseMax && D > 1) {
+                sparse_dim[t] = 0;
+                ld += P.nlev[t];
+            }
+        }
+        // keep at least one sparse dimension
+        bool any_sparse = false;
+        for (int t = 0; t < D; ++t) any_sparse |= (bool)sparse_dim[t];
+        if (!any_sparse) sparse_dim[order.back()] = 1;
+    }
+    for (int t = 0; t < D; ++t)
+        for (int64_t l = 0; l < P.nlev[t]; ++l) {
+            const int a = (int)(P.offs_h[t] + l);
+            lvl_dim[a] = t;
+            sparse_lv[a] = sparse_dim[t];
+            if (sparse_dim[t]) {
+                sparse_idx[a] = (int)sparse_levels.size();
+                sparse_levels.push_back(a);
+            } else {
+                dense_idx[a] = (int)dense_levels.size();
+                dense_levels.push_back(a);
+            }
+        }
+    const int64_t Ls = (int64_t)sparse_levels.size();
+    const int Ld = (int)dense_levels.size();
+    int ncross = 0;
+    for (int d1 = 0; d1 < D; ++d1)
+        for (int d2 = 0; d2 < D; ++d2) ncross += (d1 != d2 && sparse_dim[d1] && sparse_dim[d2]);
+    DevBuf<uint8_t> d_sparse_lv, d_sparse_dim;
+    DevBuf<int> d_sparse_idx, d_dense_idx, d_lvl_dim, d_sparse_levels, d_dense_levels;
+    DEV_ALLOC(d_sparse_lv, L);
+    DEV_ALLOC(d_sparse_dim, D);
+    DEV_ALLOC(d_sparse_idx, L);
+    DEV_ALLOC(d_dense_idx, L);
+    DEV_ALLOC(d_lvl_dim, L);
+    DEV_ALLOC(d_sparse_levels, std::max<int64_t>(Ls, 1));
+    DEV_ALLOC(d_dense_levels, std::max(Ld, 1));
+    HIP_TRY(hipMemcpyAsync(d_sparse_lv.p, sparse_lv.data(), L, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(d_sparse_dim.p, sparse_dim.data(), D, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(d_sparse_idx.p, sparse_idx.data(), L * 4, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(d_dense_idx.p, dense_idx.data(), L * 4, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(d_lvl_dim.p, lvl_dim.data(), L * 4, hipMemcpyHostToDevice, s));
+    if (Ls) HIP_TRY(hipMemcpyAsync(d_sparse_levels.p, sparse_levels.data(), Ls * 4, hipMemcpyHostToDevice, s));
+    if (Ld) HIP_TRY(hipMemcpyAsync(d_dense_levels.p, dense_levels.data(), Ld * 4, hipMemcpyHostToDevice, s));
+
+    // sparse x sparse: emit, sort, reduce
+    DevBuf<int64_t> cnt, pos;
+    DEV_ALLOC(cnt, Gi + 1);
+    DEV_ALLOC(pos, Gi + 1);
+    HIP_TRY(hipMemsetAsync(cnt.p, 0, (Gi + 1) * 8, s));
+    hipLaunchKernelGGL(k_ss_count, dim3(grid1(Gi)), dim3(T256), 0, s, Gi, P.gis.p, P.il.p,
+                       d_sparse_lv.p, P.igstart.p, ncross, cnt.p);
+    HIP_TRY(exclusive_sum(P.tmp, cnt.p, pos.p, Gi + 1, s));
+    int64_t nent = 0;
+    HIP_TRY(scalar_to_host(&nent, pos.p + Gi, s));
+    ARG_CHECK(nent < INT32_MAX, "hdfe_panel_system: too many sparse entries for one sort");
+    DevBuf<uint64_t> ek1, ek2;
+    DevBuf<double> ev1, ev2;
+    DEV_ALLOC(ek1, std::max<int64_t>(nent, 1));
+    DEV_ALLOC(ek2, std::max<int64_t>(nent, 1));
+    DEV_ALLOC(ev1, std::max<int64_t>(nent, 1));
+    DEV_ALLOC(ev2, std::max<int64_t>(nent, 1));
+    k1.reset();
+    k2.reset();
+    v1.reset();
+    v2.reset();
+    hipLaunchKernelGGL(k_ss_emit, dim3(grid1(Gi)), dim3(T256), 0, s, Gi, D, P.gis.p, P.il.p,
+                       P.iu.p, d_sparse_lv.p, P.Ng.p, P.igstart.p, P.icodes.p, M, P.offs.p,
+                       d_sparse_dim.p, lbits, pos.p, P.in.p, ek1.p, ev1.p);
+    int nss = 0;
+    DevBuf<int> ss_row, ss_col, ss_rowcnt, ss_rowptr;
+    DevBuf<double> ss_val;
+    if (nent) {
+        HIP_TRY(sort_pairs(P.tmp, ek1.p, ek2.p, ev1.p, ev2.p, nent, 0, 2 * lbits, s));
+        head.reset();
+        rank.reset();
+        DEV_ALLOC(head, nent);
+        DEV_ALLOC(rank, nent);
+        hipLaunchKernelGGL(k_heads_u64, dim3(grid1(nent)), dim3(T256), 0, s, nent, ek2.p, head.p);
+        HIP_TRY(inclusive_sum(P.tmp, head.p, rank.p, nent, s));
+        HIP_TRY(scalar_to_host(&nss, rank.p + nent - 1, s));
+    }
+    DEV_ALLOC(ss_row, std::max(nss, 1));
+    DEV_ALLOC(ss_col, std::max(nss, 1));
+    DEV_ALLOC(ss_val, std::max(nss, 1));
+    DEV_ALLOC(ss_rowcnt, Ls + 1);
+    DEV_ALLOC(ss_rowptr, Ls + 1);
+    HIP_TRY(hipMemsetAsync(ss_rowcnt.p, 0, (Ls + 1) * 4, s));
+    if (nss) {
+        hipLaunchKernelGGL(k_reduce_runs, dim3(grid1(nent)), dim3(T256), 0, s, nent, ek2.p, ev2.p,
+                           head.p, rank.p, lbits, ss_row.p, ss_col.p, ss_val.p);
+        hipLaunchKernelGGL(k_ss_rowcnt, dim3(grid1(nss)), dim3(T256), 0, s, (int64_t)nss, ss_row.p,
+                           d_sparse_idx.p, ss_rowcnt.p);
+    }
+    HIP_TRY(exclusive_sum(P.tmp, ss_rowcnt.p, ss_rowptr.p, Ls + 1, s));
+    ek1.reset();
+    ek2.reset();
+    ev1.reset();
+    ev2.reset();
+    P.n_sparse_ss = nss;
+
+    // blocks with a dense dimension
+    DevBuf<double> U, PD, DD;
+    DevBuf<uint8_t> PP, DP;
+    DevBuf<int> colpres, cpos, colbuf;
+    DEV_ALLOC(PD, std::max<int64_t>(Ls * Ld, 1));
+    DEV_ALLOC(PP, std::max<int64_t>(Ls * Ld, 1));
+    DEV_ALLOC(DD, std::max(Ld * Ld, 1));
+    DEV_ALLOC(DP, std::max(Ld * Ld, 1));
+    DEV_ALLOC(colpres, std::max(Ld, 1));
+    DEV_ALLOC(cpos, std::max<int64_t>((int64_t)Ld * (Ls + 1), 1));
+    if (Ld) {
+        DEV_ALLOC(U, std::max<int64_t>(Gi * Ld, 1));
+        HIP_TRY(hipMemsetAsync(U.p, 0, std::max<int64_t>(Gi * Ld, 1) * 8, s));
+        hipLaunchKernelGGL(k_dense_u, dim3(grid1(E)), dim3(T256), 0, s, (int64_t)E, P.ig.p, P.il.p,
+                           P.iu.p, d_dense_idx.p, Ld, U.p);
+        hipLaunchKernelGGL(k_sd, dim3(grid1(Ls * Ld)), dim3(T256), 0, s, Ls, Ld, d_sparse_levels.p,
+                           d_dense_levels.p, d_lvl_dim.p, P.offs.p, P.lptr.p, P.lq.p, P.ig.p,
+                           P.iu.p, P.Ng.p, U.p, P.igstart.p, P.icodes.p, M, P.in.p, PD.p, PP.p);
+        const int nbd = 1024;
+        DevBuf<double> ddp;
+        DevBuf<int> ddpp;
+        DEV_ALLOC(ddp, (int64_t)nbd * Ld * Ld);
+        DEV_ALLOC(ddpp, (int64_t)nbd * Ld * Ld);
+        hipLaunchKernelGGL(k_dd, dim3(nbd), dim3(T256), 0, s, Gi, Ld, d_dense_levels.p,
+                           d_lvl_dim.p, P.offs.p, U.p, P.Ng.p, P.igstart.p, P.icodes.p, M,
+                           P.in.p, ddp.p, ddpp.p);
+        hipLaunchKernelGGL(k_dd_finish, dim3(grid1(Ld * Ld)), dim3(T256), 0, s, nbd, Ld * Ld,
+                           ddp.p, ddpp.p, DD.p, DP.p);
+        // per dense level: positions of the present sparse levels
+        DEV_ALLOC(colbuf, Ls + 1);
+        std::vector<int> cp(Ld);
+        for (int t = 0; t < Ld; ++t) {
+            HIP_TRY(hipMemsetAsync(colbuf.p + Ls, 0, 4, s));
+            hipLaunchKernelGGL(k_pp_col, dim3(grid1(Ls)), dim3(T256), 0, s, Ls, Ld, t, PP.p, colbuf.p);
+            HIP_TRY(exclusive_sum(P.tmp, colbuf.p, cpos.p + (int64_t)t * (Ls + 1), Ls + 1, s));
+        }
+        for (int t = 0; t < Ld; ++t)
+            HIP_TRY(hipMemcpyAsync(colpres.p + t, cpos.p + (int64_t)t * (Ls + 1) + Ls, 4,
+                                   hipMemcpyDeviceToDevice, s));
+        U.reset();
+    }
+    // CSR
+    DevBuf<int> rnnz, indptr;
+    DEV_ALLOC(rnnz, L + 1);
+    DEV_ALLOC(indptr, L + 1);
+    HIP_TRY(hipMemsetAsync(rnnz.p, 0, (L + 1) * 4, s));
+    hipLaunchKernelGGL(k_row_nnz, dim3(grid1(L)), dim3(T256), 0, s, L, Ld, ss_rowcnt.p,
+                       d_sparse_idx.p, d_dense_idx.p, PP.p, colpres.p, DP.p, rnnz.p);
+    HIP_TRY(exclusive_sum(P.tmp, rnnz.p, indptr.p, L + 1, s));
+    int nnz = 0;
+    HIP_TRY(scalar_to_host(&nnz, indptr.p + L, s));
+    P.nnz = nnz;
+    DevBuf<int> indices;
+    DevBuf<double> data, dinv;
+    DEV_ALLOC(indices, std::max(nnz, 1));
+    DEV_ALLOC(data, std::max(nnz, 1));
+    DEV_ALLOC(dinv, L);
+    hipLaunchKernelGGL(k_fill_sparse_rows, dim3(grid1(Ls)), dim3(T256), 0, s, Ls, Ld,
+                       d_sparse_levels.p, d_dense_levels.p, ss_rowptr.p, ss_col.p, ss_val.p, PP.p,
+                       PD.p, indptr.p, indices.p, data.p);
+    if (Ld) {
+        hipLaunchKernelGGL(k_fill_dense_sparse, dim3(grid1(Ls * Ld)), dim3(T256), 0, s, Ls, Ld,
+                           d_sparse_levels.p, d_dense_levels.p, PP.p, PD.p, cpos.p, DP.p, indptr.p,
+                           indices.p, data.p);
+        hipLaunchKernelGGL(k_fill_dense_dense, dim3(grid1(Ld * Ld)), dim3(T256), 0, s, Ls, Ld,
+                           d_sparse_levels.p, d_dense_levels.p, cpos.p, DP.p, DD.p, indptr.p,
+                           indices.p, data.p);
+    }
+    hipLaunchKernelGGL(k_jacobi, dim3(grid1(L)), dim3(T256), 0, s, L, P.diag.p, dinv.p);
+    HIP_TRY(hipGetLastError());
+    std::vector<int> ip;
+    HIP_TRY(to_host(ip, indptr.p, L + 1, s));
+    if (P.matrix) {
+        hdfe_destroy(P.matrix);
+        P.matrix = nullptr;
+    }
+    void *mat = nullptr;
+    TRY(matrix_create(L, nnz, ip, indices.p, data.p, dinv.p, true, &mat));
+    P.matrix = mat;
+    if (matrix) *matrix = mat;
+    if (seconds) *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return HDFE_STATUS_OK;
+}
+
+// Components of dimension 0's levels through the identifying groups, ranked
+// by rows (comp: dimension-0 levels, owned by the panel).  Returns the count.
+int hdfe_panel_components(void *h, int64_t *ncomp) {
+    ARG_CHECK(h, "hdfe_panel_components: null handle");
+    Panel &P = *(Panel *)h;
+    hipStream_t s = P.s;
+    ARG_CHECK(P.D >= 1, "hdfe_panel_components: no non-streamed dimension");
+    const int64_t nl = P.nlev[0];
+    DevBuf<int> parent, changed, is_root, rpos;
+    DEV_ALLOC(parent, nl);
+    DEV_ALLOC(changed, 1);
+    DEV_ALLOC(is_root, nl + 1);
+    DEV_ALLOC(rpos, nl + 1);
+    hipLaunchKernelGGL(k_uf_init, dim3(grid1(nl)), dim3(T256), 0, s, nl, parent.p);
+    for (int it = 0; it < 64; ++it) {
+        int ch = 0;
+        HIP_TRY(hipMemsetAsync(changed.p, 0, 4, s));
+        hipLaunchKernelGGL(k_uf_union, dim3(grid1(P.Gi)), dim3(T256), 0, s, P.Gi, P.igstart.p,
+                           P.icodes.p, parent.p, changed.p);
+        hipLaunchKernelGGL(k_uf_flatten, dim3(grid1(nl)), dim3(T256), 0, s, nl, parent.p);
+        HIP_TRY(scalar_to_host(&ch, changed.p, s));
+        if (!ch) break;
+    }
+    DevBuf<unsigned long long> rows;
+    DEV_ALLOC(rows, nl);
+    HIP_TRY(hipMemsetAsync(rows.p, 0, nl * 8, s));
+    HIP_TRY(hipMemsetAsync(is_root.p + nl, 0, 4, s));
+    hipLaunchKernelGGL(k_comp_rows, dim3(grid1(nl)), dim3(T256), 0, s, nl, parent.p, P.cnt.p,
+                       rows.p, is_root.p);
+    HIP_TRY(exclusive_sum(P.tmp, is_root.p, rpos.p, nl + 1, s));
+    int nc = 0;
+    HIP_TRY(scalar_to_host(&nc, rpos.p + nl, s));
+    DevBuf<uint64_t> k1, k2;
+    DevBuf<int> v1, v2, rank_of_root;
+    DEV_ALLOC(k1, nc);
+    DEV_ALLOC(k2, nc);
+    DEV_ALLOC(v1, nc);
+    DEV_ALLOC(v2, nc);
+    DEV_ALLOC(rank_of_root, nl);
+    hipLaunchKernelGGL(k_comp_keys, dim3(grid1(nl)), dim3(T256), 0, s, nl, is_root.p, rows.p,
+                       rpos.p, k1.p, v1.p);
+    HIP_TRY(sort_pairs(P.tmp, k1.p, k2.p, v1.p, v2.p, nc, 0, 64, s));
+    hipLaunchKernelGGL(k_comp_rank, dim3(grid1(nc)), dim3(T256), 0, s, (int64_t)nc, v2.p,
+                       rank_of_root.p);
+    DEV_ALLOC(P.comp, nl);
+    hipLaunchKernelGGL(k_comp_label, dim3(grid1(nl)), dim3(T256), 0, s, nl, parent.p,
+                       rank_of_root.p, P.comp.p);
+    // levels grouped by component, for the normalisation
+    DevBuf<uint64_t> lk1, lk2;
+    DevBuf<int> lv1, ccnt;
+    DEV_ALLOC(lk1, nl);
+    DEV_ALLOC(lk2, nl);
+    DEV_ALLOC(lv1, nl);
+    DEV_ALLOC(P.clev, nl);
+    DEV_ALLOC(ccnt, nc + 1);
+    DEV_ALLOC(P.cptr, nc + 1);
+    HIP_TRY(hipMemsetAsync(ccnt.p, 0, (nc + 1) * 4, s));
+    hipLaunchKernelGGL(k_comp_level_keys, dim3(grid1(nl)), dim3(T256), 0, s, nl, P.comp.p, lk1.p,
+                       lv1.p, ccnt.p);
+    HIP_TRY(sort_pairs(P.tmp, lk1.p, lk2.p, lv1.p, P.clev.p, nl, 0, 64, s));
+    HIP_TRY(exclusive_sum(P.tmp, ccnt.p, P.cptr.p, nc + 1, s));
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(s));
+    P.ncomp = nc;
+    if (ncomp) *ncomp = nc;
+    return HDFE_STATUS_OK;
+}
+
+// Normalise Gamma (L, k) on the device in place (core/solve.py:383-399).
+int hdfe_panel_normalize(void *h, double *Gm, int k, void *stream) {
+    ARG_CHECK(h && Gm, "hdfe_panel_normalize: null argument");
+    Panel &P = *(Panel *)h;
+    ARG_CHECK(P.comp.p, "hdfe_panel_normalize: components not computed");
+    hipStream_t s = stream ? (hipStream_t)stream : P.s;
+    const int64_t nl0 = P.nlev[0];
+    DevBuf<double> mean;
+    DEV_ALLOC(mean, P.ncomp * k);
+    hipLaunchKernelGGL(k_comp_means, dim3(grid1(P.ncomp * k)), dim3(T256), 0, s, P.ncomp, k,
+                       P.cptr.p, P.clev.p, P.obs.p, Gm, mean.p);
+    hipLaunchKernelGGL(k_sub_comp_means, dim3(grid1(nl0 * k)), dim3(T256), 0, s, nl0, k, P.comp.p,
+                       mean.p, Gm);
+    const int nb = 64;
+    DevBuf<double> part;
+    DEV_ALLOC(part, (int64_t)nb * (k + 1));
+    for (int t = 1; t < P.D; ++t) {
+        hipLaunchKernelGGL(k_wsum_block, dim3(nb), dim3(T256), 0, s, P.nlev[t], k, P.offs_h[t],
+                           P.obs.p, Gm, part.p);
+        hipLaunchKernelGGL(k_sub_global_mean, dim3(grid1(P.nlev[t] * k)), dim3(T256), 0, s,
+                           P.nlev[t], k, P.offs_h[t], nb, part.p, Gm);
+    }
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(s));
+    return HDFE_STATUS_OK;
+}
+
+// A = W_within + V' M_D V between cells (m x m, host out; m <= 8).
+int hdfe_panel_assemble(void *h, const double *Gm, double *A) {
+    ARG_CHECK(h && Gm && A, "hdfe_panel_assemble: null argument");
+    Panel &P = *(Panel *)h;
+    ARG_CHECK(P.m <= 8, "hdfe_panel_assemble: cell assembly needs m <= 8");
+    hipStream_t s = P.s;
+    const int m = P.m, nb = 512;
+    DevBuf<double> part;
+    DEV_ALLOC(part, (int64_t)nb * m * m);
+    hipLaunchKernelGGL(k_assemble, dim3(nb), dim3(T256), 0, s, P.Gi, P.D, m, P.igstart.p,
+                       P.icodes.p, P.M, P.offs.p, P.in.p, P.isums.p, Gm, part.p);
+    std::vector<double> hp, Wh;
+    HIP_TRY(to_host(hp, part.p, (int64_t)nb * m * m, s));
+    HIP_TRY(to_host(Wh, P.W.p, m * m, s));
+    for (int t = 0; t < m * m; ++t) {
+        double v = Wh[t];
+        for (int b = 0; b < nb; ++b) v += hp[(size_t)b * m * m + t];
+        A[t] = v;
+    }
+    return HDFE_STATUS_OK;
+}
+
+// The y-equation's effects gam_y (L, device out), the streamed effects
+// (G, device out) and the residuals (n, device out, sorted row order).
+int hdfe_panel_pass2(void *h, const double *Gm, const double *beta, const int *xcols, int kx,
+                     double *gam_y, double *geff, double *resid) {
+    ARG_CHECK(h && Gm && gam_y && geff && resid, "hdfe_panel_pass2: null argument");
+    Panel &P = *(Panel *)h;
+    hipStream_t s = P.s;
+    DevBuf<double> b;
+    DevBuf<int> xc;
+    DEV_ALLOC(b, std::max(kx, 1));
+    DEV_ALLOC(xc, std::max(kx, 1));
+    if (kx) {
+        HIP_TRY(hipMemcpyAsync(b.p, beta, kx * 8, hipMemcpyHostToDevice, s));
+        HIP_TRY(hipMemcpyAsync(xc.p, xcols, kx * 4, hipMemcpyHostToDevice, s));
+    }
+    hipLaunchKernelGGL(k_gam_y, dim3(grid1(P.L)), dim3(T256), 0, s, P.L, P.m, Gm, b.p, xc.p, kx, gam_y);
+    hipLaunchKernelGGL(k_pass2, dim3(grid1(P.G)), dim3(T256), 0, s, P.G, P.D, P.m, P.gstart.p,
+                       P.codes.p, P.n, P.offs.p, P.V.p, P.weighted ? P.w.p : (const double *)nullptr,
+                       gam_y, b.p, xc.p, kx, geff, resid);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(s));
+    return HDFE_STATUS_OK;
+}
+
+void hdfe_panel_destroy(void *h) {
+    if (!h) return;
+    Panel *P = (Panel *)h;
+    if (P->s) (void)hipStreamSynchronize(P->s);
+    if (P->matrix) hdfe_destroy(P->matrix);
+    if (P->s) (void)hipStreamDestroy(P->s);
+    delete P;
+}
+
+}  // extern "C"

@@ -1,0 +1,121 @@
+// Device buffers and hipCUB wrappers shared by the pipeline sources.
+#pragma once
+#include <hipcub/hipcub.hpp>
+
+#include <memory>
+#include <vector>
+
+#include "common.h"
+
+namespace hdfe {
+
+// An owning device allocation (hipMalloc/hipFree).
+template <typename T>
+struct DevBuf {
+    T *p = nullptr;
+    int64_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    DevBuf(DevBuf &&o) noexcept : p(o.p), n(o.n) {
+        o.p = nullptr;
+        o.n = 0;
+    }
+    DevBuf &operator=(DevBuf &&o) noexcept {
+        if (this != &o) {
+            reset();
+            p = o.p;
+            n = o.n;
+            o.p = nullptr;
+            o.n = 0;
+        }
+        return *this;
+    }
+    ~DevBuf() { reset(); }
+    void reset() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    hipError_t alloc(int64_t count) {
+        reset();
+        n = count;
+        return hipMalloc(&p, sizeof(T) * (size_t)std::max<int64_t>(count, 1));
+    }
+    T *get() const { return p; }
+};
+
+#define DEV_ALLOC(buf, count) HIP_TRY((buf).alloc(count))
+
+inline int grid1(int64_t n, int threads = 256, int cap = 65536) {
+    int64_t g = (n + threads - 1) / threads;
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return (int)g;
+}
+
+// Reusable temporary storage for hipCUB calls.
+struct Temp {
+    DevBuf<char> buf;
+    hipError_t ensure(size_t bytes) {
+        if ((int64_t)bytes <= buf.n) return hipSuccess;
+        return buf.alloc((int64_t)bytes + (1 << 20));
+    }
+};
+
+template <typename K, typename V>
+inline hipError_t sort_pairs(Temp &t, const K *kin, K *kout, const V *vin, V *vout, int64_t n,
+                             int begin_bit, int end_bit, hipStream_t s) {
+    size_t bytes = 0;
+    hipError_t e = hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, kin, kout, vin, vout,
+                                                      (int)n, begin_bit, end_bit, s);
+    if (e != hipSuccess) return e;
+    if ((e = t.ensure(bytes)) != hipSuccess) return e;
+    bytes = (size_t)t.buf.n;
+    return hipcub::DeviceRadixSort::SortPairs(t.buf.p, bytes, kin, kout, vin, vout, (int)n,
+                                              begin_bit, end_bit, s);
+}
+
+template <typename T, typename U>
+inline hipError_t inclusive_sum(Temp &t, const T *in, U *out, int64_t n, hipStream_t s) {
+    size_t bytes = 0;
+    hipError_t e = hipcub::DeviceScan::InclusiveSum(nullptr, bytes, in, out, (int)n, s);
+    if (e != hipSuccess) return e;
+    if ((e = t.ensure(bytes)) != hipSuccess) return e;
+    bytes = (size_t)t.buf.n;
+    return hipcub::DeviceScan::InclusiveSum(t.buf.p, bytes, in, out, (int)n, s);
+}
+
+template <typename T, typename U>
+inline hipError_t exclusive_sum(Temp &t, const T *in, U *out, int64_t n, hipStream_t s) {
+    size_t bytes = 0;
+    hipError_t e = hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, in, out, (int)n, s);
+    if (e != hipSuccess) return e;
+    if ((e = t.ensure(bytes)) != hipSuccess) return e;
+    bytes = (size_t)t.buf.n;
+    return hipcub::DeviceScan::ExclusiveSum(t.buf.p, bytes, in, out, (int)n, s);
+}
+
+template <typename T>
+inline hipError_t to_host(std::vector<T> &h, const T *d, int64_t n, hipStream_t s) {
+    h.resize((size_t)n);
+    if (n == 0) return hipSuccess;
+    hipError_t e = hipMemcpyAsync(h.data(), d, sizeof(T) * n, hipMemcpyDeviceToHost, s);
+    if (e != hipSuccess) return e;
+    return hipStreamSynchronize(s);
+}
+
+template <typename T>
+inline hipError_t scalar_to_host(T *h, const T *d, hipStream_t s) {
+    hipError_t e = hipMemcpyAsync(h, d, sizeof(T), hipMemcpyDeviceToHost, s);
+    if (e != hipSuccess) return e;
+    return hipStreamSynchronize(s);
+}
+
+inline int bits_for(uint64_t maxval) {
+    int b = 0;
+    while (b < 64 && (maxval >> b) != 0) ++b;
+    return b;
+}
+
+}  // namespace hdfe

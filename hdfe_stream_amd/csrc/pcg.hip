@@ -31,9 +31,13 @@
 // when that true FP64 residual meets tol.
 #include "common.h"
 
+#include <cmath>
 #include <algorithm>
+#include <memory>
 #include <mutex>
 #include <vector>
+
+#include "solver.h"
 
 namespace hdfe {
 
@@ -509,6 +513,16 @@ static int grid_for(int64_t n, int64_t per, int lo, int hi) {
     return (int)std::max<int64_t>(lo, std::min<int64_t>(hi, g));
 }
 
+// A pair of scratch allocations freed on scope exit.
+struct DevBuf2 {
+    double *p = nullptr;
+    void *extra = nullptr;
+    ~DevBuf2() {
+        if (p) (void)hipFree(p);
+        if (extra) (void)hipFree(extra);
+    }
+};
+
 // Work buffers, cached on the matrix for the last block width used.
 struct Work {
     int k = 0;
@@ -539,6 +553,14 @@ struct Matrix {
     hipEvent_t ev = nullptr;
     Work w;
     std::mutex mu;
+    // optional HIP-event timing of the SpMM launches of the CG iterations
+    bool timing = false;
+    std::vector<hipEvent_t> evpool;
+    double spmm_ms = 0.0;
+    int64_t spmm_n = 0;
+    ~Matrix() {
+        for (hipEvent_t e : evpool) (void)hipEventDestroy(e);
+    }
 };
 
 static int alloc_work(Matrix &M, int k) {
@@ -653,20 +675,40 @@ static int run_cg(Matrix &M, int k, const T *val, const T *dinv, T *X, T *R, T *
     const int gd = grid_for(M.L * (int64_t)k, 1024, 1, 4096);
     int host[2] = {0, 0};
     int batch = 8;
+    int64_t done_before = 0;  // iterations launched before this batch
     for (int64_t launched = 0; launched < st.maxiter;) {
         const int nb = (int)std::min<int64_t>(batch, st.maxiter - launched);
+        if (M.timing)
+            while ((int)M.evpool.size() < 2 * nb) {
+                hipEvent_t e;
+                HIP_TRY(hipEventCreate(&e));
+                M.evpool.push_back(e);
+            }
         for (int b = 0; b < nb; ++b) {
+            if (M.timing) HIP_TRY(hipEventRecord(M.evpool[2 * b], s));
             launch_spmm<T>(M, val, P, AP, W.part, W.lpart, 1, st, s);
+            if (M.timing) HIP_TRY(hipEventRecord(M.evpool[2 * b + 1], s));
             hipLaunchKernelGGL((k_step<T>), dim3(gv), dim3(kThreads), 0, s, M.L, X, R, P, AP, dinv,
                                W.part, st);
             hipLaunchKernelGGL((k_direction<T>), dim3(gd), dim3(kThreads), 0, s, M.L, P, R, dinv,
                                st);
         }
+        done_before = launched;
         launched += nb;
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipMemcpyAsync(host, st.done, sizeof(int), hipMemcpyDeviceToHost, s));
         HIP_TRY(hipMemcpyAsync(host + 1, st.it_final, sizeof(int), hipMemcpyDeviceToHost, s));
         HIP_TRY(hipStreamSynchronize(s));
+        if (M.timing) {
+            // iterations that ran (the ones after the stop return at once)
+            const int64_t ran = host[0] ? std::min<int64_t>(nb, host[1] - done_before) : nb;
+            for (int64_t b = 0; b < ran; ++b) {
+                float ms = 0.f;
+                HIP_TRY(hipEventElapsedTime(&ms, M.evpool[2 * b], M.evpool[2 * b + 1]));
+                M.spmm_ms += ms;
+                M.spmm_n += 1;
+            }
+        }
         if (host[0]) break;
         if (batch < 64) batch *= 2;
     }
@@ -851,39 +893,20 @@ static int solve_mixed(Matrix &M, int k, const uint8_t *zero_h, bool has_x0, dou
     return HDFE_STATUS_OK;
 }
 
-}  // namespace hdfe
-
-// ================================================================== C ABI
-
-using namespace hdfe;
-
-extern "C" {
-
-typedef struct hdfe_solver hdfe_solver;
-
-int hdfe_csr_create(int64_t L, int64_t nnz, const int64_t *indptr, const int32_t *indices,
-                    const double *data, const double *dinv, int on_device, void **out) {
-    ARG_CHECK(out != nullptr, "hdfe_csr_create: null output handle");
-    ARG_CHECK(L >= 0 && nnz >= 0, "hdfe_csr_create: negative size");
-    ARG_CHECK(nnz < INT32_MAX && L < INT32_MAX,
-              "hdfe_csr_create: nnz and L must be below 2^31 (int32 CSR on the device)");
-    Matrix *M = new Matrix();
+// A device matrix from a host int32 indptr and host or device indices,
+// values and inverse diagonal.
+int matrix_create(int64_t L, int64_t nnz, const std::vector<int> &ip32, const int *indices,
+                  const double *data, const double *dinv, bool on_device, void **out) {
+    std::unique_ptr<Matrix> M(new Matrix());
     HIP_TRY(hipGetDevice(&M->device));
     M->L = L;
     M->nnz = nnz;
-    const hipMemcpyKind kind = on_device ? hipMemcpyDeviceToHost : hipMemcpyHostToHost;
-    std::vector<int64_t> ip(L + 1);
-    HIP_TRY(hipMemcpy(ip.data(), indptr, (L + 1) * 8, kind));
-    ARG_CHECK(ip[0] == 0 && ip[L] == nnz, "hdfe_csr_create: indptr must run from 0 to nnz");
-    std::vector<int> ip32(L + 1);
     std::vector<int> lrows, lcptr(1, 0), crow, cbeg;
-    for (int64_t i = 0; i <= L; ++i) ip32[i] = (int)ip[i];
     for (int64_t i = 0; i < L; ++i) {
-        ARG_CHECK(ip[i + 1] >= ip[i], "hdfe_csr_create: indptr must be non-decreasing");
-        const int64_t len = ip[i + 1] - ip[i];
+        const int64_t len = (int64_t)ip32[i + 1] - ip32[i];
         if (len > kLongThresh) {
             lrows.push_back((int)i);
-            for (int64_t p = ip[i]; p < ip[i + 1]; p += kLongChunk) {
+            for (int64_t p = ip32[i]; p < ip32[i + 1]; p += kLongChunk) {
                 crow.push_back((int)i);
                 cbeg.push_back((int)p);
             }
@@ -923,8 +946,36 @@ int hdfe_csr_create(int64_t L, int64_t nnz, const int64_t *indptr, const int32_t
         HIP_TRY(hipMemcpy(M->chunk_beg, cbeg.data(), M->nchunk * 4, hipMemcpyHostToDevice));
     }
     HIP_TRY(hipStreamSynchronize(M->stream));
-    *out = M;
+    *out = M.release();
     return HDFE_STATUS_OK;
+}
+
+}  // namespace hdfe
+
+// ================================================================== C ABI
+
+using namespace hdfe;
+
+extern "C" {
+
+typedef struct hdfe_solver hdfe_solver;
+
+int hdfe_csr_create(int64_t L, int64_t nnz, const int64_t *indptr, const int32_t *indices,
+                    const double *data, const double *dinv, int on_device, void **out) {
+    ARG_CHECK(out != nullptr, "hdfe_csr_create: null output handle");
+    ARG_CHECK(L >= 0 && nnz >= 0, "hdfe_csr_create: negative size");
+    ARG_CHECK(nnz < INT32_MAX && L < INT32_MAX,
+              "hdfe_csr_create: nnz and L must be below 2^31 (int32 CSR on the device)");
+    const hipMemcpyKind kind = on_device ? hipMemcpyDeviceToHost : hipMemcpyHostToHost;
+    std::vector<int64_t> ip(L + 1);
+    HIP_TRY(hipMemcpy(ip.data(), indptr, (L + 1) * 8, kind));
+    ARG_CHECK(ip[0] == 0 && ip[L] == nnz, "hdfe_csr_create: indptr must run from 0 to nnz");
+    std::vector<int> ip32(L + 1);
+    for (int64_t i = 0; i <= L; ++i) {
+        if (i < L) ARG_CHECK(ip[i + 1] >= ip[i], "hdfe_csr_create: indptr must be non-decreasing");
+        ip32[i] = (int)ip[i];
+    }
+    return hdfe::matrix_create(L, nnz, ip32, indices, data, dinv, on_device != 0, out);
 }
 
 int hdfe_csr_info(void *h, int64_t *L, int64_t *nnz, int *nlong) {
@@ -1000,6 +1051,73 @@ int hdfe_csr_matmat(void *h, int64_t k, const double *x, double *y, int on_devic
     const hipMemcpyKind down = on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
     HIP_TRY(hipMemcpyAsync(y, M.w.AP64, Lk * 8, down, s));
     HIP_TRY(hipStreamSynchronize(s));
+    return HDFE_STATUS_OK;
+}
+
+// Copy S, as the device holds it, to host arrays: indptr int64 (L+1),
+// indices int32 (nnz), data float64 (nnz), dinv float64 (L); any may be NULL.
+int hdfe_csr_export(void *h, int64_t *indptr, int32_t *indices, double *data, double *dinv) {
+    ARG_CHECK(h != nullptr, "hdfe_csr_export: null handle");
+    Matrix &M = *(Matrix *)h;
+    HIP_TRY(hipSetDevice(M.device));
+    if (indptr) {
+        std::vector<int> ip(M.L + 1);
+        HIP_TRY(hipMemcpy(ip.data(), M.indptr, (M.L + 1) * 4, hipMemcpyDeviceToHost));
+        for (int64_t i = 0; i <= M.L; ++i) indptr[i] = ip[i];
+    }
+    if (indices && M.nnz) HIP_TRY(hipMemcpy(indices, M.indices, M.nnz * 4, hipMemcpyDeviceToHost));
+    if (data && M.nnz) HIP_TRY(hipMemcpy(data, M.val64, M.nnz * 8, hipMemcpyDeviceToHost));
+    if (dinv && M.L) HIP_TRY(hipMemcpy(dinv, M.dinv64, M.L * 8, hipMemcpyDeviceToHost));
+    return HDFE_STATUS_OK;
+}
+
+// Turn HIP-event timing of the CG's SpMM launches on or off (resets the
+// totals); read the totals (milliseconds, launches).
+int hdfe_csr_timing(void *h, int enable) {
+    ARG_CHECK(h != nullptr, "hdfe_csr_timing: null handle");
+    Matrix &M = *(Matrix *)h;
+    std::lock_guard<std::mutex> lock(M.mu);
+    M.timing = enable != 0;
+    M.spmm_ms = 0.0;
+    M.spmm_n = 0;
+    return HDFE_STATUS_OK;
+}
+
+int hdfe_csr_timing_read(void *h, double *ms, int64_t *launches) {
+    ARG_CHECK(h != nullptr, "hdfe_csr_timing_read: null handle");
+    Matrix &M = *(Matrix *)h;
+    if (ms) *ms = M.spmm_ms;
+    if (launches) *launches = M.spmm_n;
+    return HDFE_STATUS_OK;
+}
+
+// Column norms of a device (L, k) block, to the host: sqrt(sum_i X_ij^2),
+// reduced in a fixed order.
+int hdfe_col_norms(const double *X, int64_t L, int64_t k, double *out) {
+    ARG_CHECK(X && out && k >= 1 && k <= kMaxK, "hdfe_col_norms: bad argument");
+    DevBuf2 part, res;
+    unsigned *cnt = nullptr;
+    HIP_TRY(hipMalloc(&part.p, (int64_t)4096 * k * 8));
+    HIP_TRY(hipMalloc(&res.p, k * 8));
+    HIP_TRY(hipMalloc(&cnt, 8));
+    part.extra = cnt;
+    HIP_TRY(hipMemset(cnt, 0, 8));
+    launch_colsum<double>(L, (int)k, X, X, nullptr, part.p, res.p, 0, cnt, nullptr);
+    HIP_TRY(hipGetLastError());
+    std::vector<double> h(k);
+    HIP_TRY(hipMemcpy(h.data(), res.p, k * 8, hipMemcpyDeviceToHost));
+    for (int64_t j = 0; j < k; ++j) out[j] = std::sqrt(h[j]);
+    return HDFE_STATUS_OK;
+}
+
+// dst <- src, `bytes` bytes; kind 0 device to host, 1 host to device,
+// 2 device to device.
+int hdfe_memcpy(void *dst, const void *src, int64_t bytes, int kind) {
+    ARG_CHECK(kind >= 0 && kind <= 2, "hdfe_memcpy: kind must be 0, 1 or 2");
+    if (bytes <= 0) return HDFE_STATUS_OK;
+    const hipMemcpyKind k = kind == 0 ? hipMemcpyDeviceToHost
+                            : kind == 1 ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice;
+    HIP_TRY(hipMemcpy(dst, src, (size_t)bytes, k));
     return HDFE_STATUS_OK;
 }
 

@@ -1,0 +1,360 @@
+"""`device="rocm"` fits: the reference's estimator interface over the HIP path.
+
+Mirrors the parts of hdfe_stream's public API that reach the solve path:
+`feols_stream` (api.py:22-107) and `StreamingHDFE(...).fit` (core/estimator.py:
+69-246, 467-546) with the same options, validation and solver-info keys, and
+the same FP64 algorithm -- pass 0 (singletons, codes, row sort), pass 1
+(cells), step 2 (explicit S, block Jacobi-PCG, normalisation), step 3 (V' M_D V
+and beta) and pass 2 (streamed effects, residuals) -- all of it on the device
+(libhdfe_hip.so), with the panel's rows resident in HBM.
+
+Not mirrored (outside the solve path): Parquet streaming through Polars, the
+pyfixest formula language beyond `y ~ x1 + x2 | fe1 + fe2 + ...`, standard
+errors, IV, collinearity checks.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import re
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+from .rocm import DeviceCSR, PRECISIONS, rocm_backend
+
+RHS_ZERO_TOL = 1e-11      # core/solve.py:44
+
+
+def parse_formula(fml: str):
+    """`y ~ x1 + x2 | fe1 + fe2` -> ("y", ["x1", "x2"], ["fe1", "fe2"])."""
+    m = re.fullmatch(r"\s*([^~|]+?)\s*~\s*([^|]*?)\s*(?:\|\s*(.*?)\s*)?", fml)
+    if not m:
+        raise ValueError(f"cannot parse formula {fml!r}")
+    y = m.group(1).strip()
+    rhs = [t.strip() for t in m.group(2).split("+") if t.strip()]
+    xs = [t for t in rhs if t not in ("1", "0")]
+    fe = [t.strip() for t in (m.group(3) or "").split("+") if t.strip()]
+    return y, xs, fe
+
+
+def _column(data, name):
+    """A column of `data` (dict, pandas DataFrame, or Parquet path) as a
+    numpy array or torch tensor, unchanged."""
+    if isinstance(data, (str, bytes)) or hasattr(data, "__fspath__"):
+        import pyarrow.parquet as pq
+        return pq.read_table(data, columns=[name]).column(name).to_numpy()
+    col = data[name]
+    if hasattr(col, "to_numpy") and not type(col).__module__.startswith("torch"):
+        return col.to_numpy()
+    return col
+
+
+def _to_device(a, dtype):
+    import torch
+    if isinstance(a, torch.Tensor):
+        return a.to(device="cuda", dtype=dtype).contiguous()
+    return torch.from_numpy(np.ascontiguousarray(a)).to(device="cuda", dtype=dtype)
+
+
+class Panel:
+    """A panel's rows on the device after passes 0 and 1 (hdfe_panel_*)."""
+
+    INFO = ["n_in", "n", "G", "C", "M", "Gi", "L", "D", "gdim", "singletons",
+            "single_rounds", "E", "nnz", "ncomp", "m", "weighted"]
+
+    def __init__(self, fe_cols, var_cols, weights=None, fixef_rm="singleton", stream_dim=-1,
+                 sort_vars=False):
+        import torch
+        lib = _lib.load()
+        self._keep = []
+        fe_t = [_to_device(c, torch.int64) for c in fe_cols]
+        var_t = [_to_device(c, torch.float64) for c in var_cols]
+        w_t = None
+        w32 = 0
+        if weights is not None:
+            if getattr(weights, "dtype", None) in (np.float32, torch.float32):
+                w_t, w32 = _to_device(weights, torch.float32), 1
+            else:
+                w_t = _to_device(weights, torch.float64)
+        self._keep += fe_t + var_t + ([w_t] if w_t is not None else [])
+        n = int(fe_t[0].shape[0])
+        fe_p = (C.c_void_p * len(fe_t))(*[t.data_ptr() for t in fe_t])
+        var_p = (C.c_void_p * len(var_t))(*[t.data_ptr() for t in var_t])
+        h = C.c_void_p()
+        torch.cuda.synchronize()
+        lib.hdfe_panel_create.restype = C.c_int
+        _lib.check(lib.hdfe_panel_create(
+            C.c_int64(n), C.c_int(len(fe_t)), fe_p, C.c_int(len(var_t)), var_p,
+            C.c_void_p(w_t.data_ptr() if w_t is not None else None), C.c_int(w32),
+            C.c_int(1 if fixef_rm == "singleton" else 0), C.c_int(stream_dim),
+            C.c_int(1 if sort_vars else 0), C.byref(h)), "hdfe_panel_create")
+        self._h = h
+        self._keep = []      # the panel holds its own sorted copies
+        self.matrix = None
+        self.refresh()
+
+    def refresh(self):
+        info = (C.c_int64 * 16)()
+        nlev = (C.c_int64 * 8)()
+        dims = (C.c_int * 8)()
+        _lib.check(_lib.load().hdfe_panel_info(self._h, info, nlev, dims), "hdfe_panel_info")
+        self.info = dict(zip(self.INFO, [int(v) for v in info]))
+        D = self.info["D"]
+        self.n_levels = [int(nlev[t]) for t in range(D)]
+        self.dims = [int(dims[t]) for t in range(D)]
+        self.offs = np.concatenate(([0], np.cumsum(self.n_levels)[:-1])).astype(np.int64)
+
+    def ptr(self, name):
+        p = C.c_void_p()
+        n = C.c_int64()
+        _lib.check(_lib.load().hdfe_panel_array(self._h, name.encode(), C.byref(p), C.byref(n)),
+                   "hdfe_panel_array")
+        return p.value, int(n.value)
+
+    def array(self, name, dtype):
+        """A copy of a panel array on the host."""
+        p, n = self.ptr(name)
+        out = np.empty(n, dtype)
+        if n:
+            _lib.check(_lib.load().hdfe_memcpy(out.ctypes.data, C.c_void_p(p), out.nbytes, 0),
+                       "hdfe_memcpy")
+        return out
+
+    def tensor(self, name, dtype):
+        """A copy of a panel array on the device (torch)."""
+        import torch
+        p, n = self.ptr(name)
+        out = torch.empty(n, dtype=dtype, device="cuda")
+        if n:
+            torch.cuda.synchronize()
+            _lib.check(_lib.load().hdfe_memcpy(C.c_void_p(out.data_ptr()), C.c_void_p(p),
+                                               out.numel() * out.element_size(), 2), "hdfe_memcpy")
+        return out
+
+    def system(self, with_rhs=True):
+        h = C.c_void_p()
+        secs = C.c_double()
+        _lib.check(_lib.load().hdfe_panel_system(self._h, int(with_rhs), C.byref(h),
+                                                 C.byref(secs)), "hdfe_panel_system")
+        self.matrix = DeviceCSR(h, self.info["L"], 0, owned=False)
+        self.refresh()
+        self.matrix.nnz = self.info["nnz"]
+        return float(secs.value)
+
+    def components(self):
+        n = C.c_int64()
+        _lib.check(_lib.load().hdfe_panel_components(self._h, C.byref(n)), "hdfe_panel_components")
+        self.refresh()
+        return int(n.value)
+
+    def normalize(self, G):
+        import torch
+        torch.cuda.synchronize()
+        _lib.check(_lib.load().hdfe_panel_normalize(self._h, C.c_void_p(G.data_ptr()),
+                                                    C.c_int(G.shape[1]), None),
+                   "hdfe_panel_normalize")
+
+    def assemble(self, G):
+        import torch
+        m = self.info["m"]
+        A = np.empty((m, m))
+        torch.cuda.synchronize()
+        _lib.check(_lib.load().hdfe_panel_assemble(self._h, C.c_void_p(G.data_ptr()),
+                                                   A.ctypes.data), "hdfe_panel_assemble")
+        return A
+
+    def pass2(self, G, beta, xcols):
+        import torch
+        L, Gn, n = self.info["L"], self.info["G"], self.info["n"]
+        gam_y = torch.empty(L, dtype=torch.float64, device="cuda")
+        geff = torch.empty(Gn, dtype=torch.float64, device="cuda")
+        resid = torch.empty(n, dtype=torch.float64, device="cuda")
+        b = np.ascontiguousarray(beta, np.float64)
+        xc = np.ascontiguousarray(xcols, np.int32)
+        torch.cuda.synchronize()
+        _lib.check(_lib.load().hdfe_panel_pass2(
+            self._h, C.c_void_p(G.data_ptr()), b.ctypes.data, xc.ctypes.data, C.c_int(len(xc)),
+            C.c_void_p(gam_y.data_ptr()), C.c_void_p(geff.data_ptr()),
+            C.c_void_p(resid.data_ptr())), "hdfe_panel_pass2")
+        return gam_y, geff, resid
+
+    def close(self):
+        if getattr(self, "_h", None):
+            if self.matrix is not None:
+                self.matrix.close()         # the handle itself is the panel's
+            _lib.load().hdfe_panel_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+@dataclass
+class HDFEResult:
+    """What a fit returns: coefficients, effects, solver facts."""
+    coefnames: list
+    beta: np.ndarray
+    solver_info: dict
+    fe_names: list
+    fe_levels: dict
+    fe_values: dict
+    n_obs: int
+    n_components: int
+    A: np.ndarray
+    diagnostics: dict = field(default_factory=dict)
+    panel: object = None
+    Gamma: object = None
+    resid_sorted: object = None
+
+    def coef(self):
+        return dict(zip(self.coefnames, self.beta.tolist()))
+
+    def fixef(self, dim):
+        """(level values, effects) of one fixed-effect dimension."""
+        return self.fe_levels[dim], self.fe_values[dim]
+
+    def resid(self):
+        """Residuals in the source's row order (NaN for dropped rows)."""
+        perm = self.panel.array("perm", np.int32)[: self.n_obs]
+        r = self.resid_sorted.cpu().numpy()
+        out = np.full(self.panel.info["n_in"], np.nan)
+        out[perm] = r
+        return out
+
+
+class StreamingHDFE:
+    """The estimator with `device="rocm"` (core/estimator.py:69-246)."""
+
+    def __init__(self, y, x, fe, *, tol=1e-10, maxiter=5000, rhs_block=8, solver="auto",
+                 precond="jacobi", fixef_rm="singleton", weights=None, stream=None,
+                 device="rocm", precision="fp64", inner_tol=1e-5, keep_intermediates=False,
+                 verbose=False, spmm_timing=False, **ignored):
+        if device != "rocm":
+            raise ValueError("device must be 'rocm' (this package is the MI355X backend)")
+        if solver not in ("auto", "explicit", "stream_cg", "within"):
+            raise ValueError("solver must be 'auto', 'explicit', 'stream_cg' or 'within'")
+        if precond not in ("jacobi", "amg"):
+            raise ValueError("precond must be 'jacobi' or 'amg'")
+        if solver not in ("auto", "explicit") or precond != "jacobi":
+            raise ValueError("device='rocm' runs the explicit solver with the Jacobi "
+                             "preconditioner: use solver='auto' or 'explicit' and "
+                             "precond='jacobi'")
+        if fixef_rm not in ("none", "singleton"):
+            raise ValueError("fixef_rm must be 'none' or 'singleton'")
+        if precision not in PRECISIONS:
+            raise ValueError(f"precision must be one of {sorted(PRECISIONS)}")
+        self.backend = rocm_backend()       # fail now, without the library or a GPU
+        self.y = y
+        self.x = list(x or [])
+        self.fe = list(fe)
+        if len(set(self.fe)) < len(self.fe):
+            raise ValueError(f"duplicate fixed-effect dimensions in {self.fe}")
+        if stream is not None and stream not in self.fe:
+            raise ValueError(f"stream={stream!r} is not one of the fixed effects {self.fe}")
+        self.tol, self.maxiter = tol, maxiter
+        self.rhs_block = max(1, int(rhs_block))
+        self.fixef_rm = fixef_rm
+        self.weights = weights
+        self.stream = stream
+        self.precision = precision
+        self.inner_tol = inner_tol
+        self.keep_intermediates = keep_intermediates
+        self.verbose = verbose
+        self.spmm_timing = spmm_timing
+
+    def fit(self, data) -> HDFEResult:
+        import torch
+        t0 = time.time()
+        names = [self.y] + [v for v in self.x if v != self.y]
+        if len(self.fe) < 2:
+            raise ValueError("device='rocm' fits need at least two fixed effects "
+                             "(one streamed, one solved for)")
+        panel = Panel([_column(data, f) for f in self.fe], [_column(data, v) for v in names],
+                      None if self.weights is None else _column(data, self.weights),
+                      self.fixef_rm, -1 if self.stream is None else self.fe.index(self.stream),
+                      sort_vars=self.keep_intermediates)
+        info = {"solver": "explicit", "device": self.backend.name}
+        t1 = time.time()
+        s_secs = panel.system(with_rhs=True)
+        info.update(S_nnz=panel.info["nnz"],
+                    S_mb=round(panel.info["nnz"] * 12 / 1e6 + (panel.info["L"] + 1) * 4 / 1e6, 1),
+                    S_build_seconds=round(s_secs, 4), device_setup_seconds=0.0)
+        ncomp = panel.components()
+        L, m = panel.info["L"], panel.info["m"]
+        rhs = panel.tensor("rhs", torch.float64).view(L, m)
+        raw_ss = panel.array("raw_ss", np.float64)
+        if self.spmm_timing:
+            panel.matrix.timing(True)
+        t2 = time.time()
+        Gamma = torch.empty((L, m), dtype=torch.float64, device="cuda")
+        its, conv, rel, fe_spanned, steps = [], True, 0.0, [], []
+        for j0 in range(0, m, self.rhs_block):
+            j1 = min(m, j0 + self.rhs_block)
+            b = rhs[:, j0:j1].contiguous()
+            bn = np.empty(j1 - j0)
+            torch.cuda.synchronize()
+            _lib.check(_lib.load().hdfe_col_norms(C.c_void_p(b.data_ptr()), L, j1 - j0,
+                                                  bn.ctypes.data), "hdfe_col_norms")
+            zero = bn <= RHS_ZERO_TOL * np.sqrt(np.maximum(raw_ss[j0:j1], 0.0))
+            fe_spanned += [names[j0 + k] for k in np.flatnonzero(zero)]
+            sinfo = {}
+            X, it, c, r = panel.matrix.solve(b, zero, None, tol=self.tol, maxiter=self.maxiter,
+                                             precision=self.precision,
+                                             inner_tol=self.inner_tol, info=sinfo)
+            Gamma[:, j0:j1] = X
+            its.append(it)
+            steps.append(sinfo.get("refinement_steps", 0))
+            conv &= c
+            rel = max(rel, r)
+        t3 = time.time()
+        panel.normalize(Gamma)
+        A = panel.assemble(Gamma)
+        xj = list(range(1, m))
+        beta = np.linalg.solve(A[np.ix_(xj, xj)], A[xj, 0]) if xj else np.zeros(0)
+        gam_y, geff, resid = panel.pass2(Gamma, beta, xj)
+        torch.cuda.synchronize()
+        t4 = time.time()
+        info.update({"iterations": its if len(its) > 1 else its[0], "converged": bool(conv),
+                     "blocks": len(its), "seconds": round(t3 - t2, 4),
+                     "max_rel_residual": rel, "precision": self.precision})
+        if self.precision == "mixed":
+            info["refinement_steps"] = steps if len(steps) > 1 else steps[0]
+        if fe_spanned:
+            info["fe_spanned"] = fe_spanned
+        if self.spmm_timing:
+            info["spmm"] = panel.matrix.timing_read()
+        # effects per dimension, with their level values
+        fe_order = [self.fe[panel.info["gdim"]]] + [self.fe[d] for d in panel.dims]
+        gy = gam_y.cpu().numpy()
+        levels = {fe_order[0]: panel.array("glevels", np.int64)}
+        values = {fe_order[0]: geff.cpu().numpy()}
+        for t, d in enumerate(fe_order[1:]):
+            o, nl = int(panel.offs[t]), panel.n_levels[t]
+            levels[d] = panel.array(f"levels{t}", np.int64)
+            values[d] = gy[o:o + nl]
+        diag = {"seconds_pass01": round(t1 - t0, 4), "seconds_system": round(t2 - t1, 4),
+                "seconds_solve": round(t3 - t2, 4), "seconds_pass2": round(t4 - t3, 4),
+                "seconds_total": round(t4 - t0, 4), "singletons": panel.info["singletons"],
+                "cells": panel.info["C"], "identifying_cells": panel.info["M"],
+                "identifying_groups": panel.info["Gi"]}
+        res = HDFEResult(coefnames=names[1:], beta=beta, solver_info=info, fe_names=fe_order,
+                         fe_levels=levels, fe_values=values, n_obs=panel.info["n"],
+                         n_components=ncomp, A=A, diagnostics=diag, panel=panel, Gamma=Gamma,
+                         resid_sorted=resid)
+        if self.verbose:
+            print(f"fit: {diag}")
+        return res
+
+
+def feols_stream(fml: str, data, **options) -> HDFEResult:
+    """OLS with high-dimensional fixed effects on the device
+    (hdfe_stream/api.py:22-107, with device='rocm')."""
+    y, xs, fe = parse_formula(fml)
+    options.setdefault("device", "rocm")
+    return StreamingHDFE(y, xs, fe, **options).fit(data)

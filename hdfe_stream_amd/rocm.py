@@ -48,10 +48,11 @@ def _is_torch(a) -> bool:
 class DeviceCSR:
     """An explicit S (and its Jacobi inverse diagonal) resident on the device."""
 
-    def __init__(self, handle, L: int, nnz: int):
+    def __init__(self, handle, L: int, nnz: int, owned: bool = True):
         self._h = handle
         self.L = L
         self.nnz = nnz
+        self._owned = owned      # False: the handle belongs to a Panel
 
     @classmethod
     def from_scipy(cls, S, dinv):
@@ -143,10 +144,37 @@ class DeviceCSR:
             info["refinement_steps"] = int(outer.value)
         return X, int(it.value), bool(conv.value), float(rel.value)
 
+    def timing(self, enable: bool = True) -> None:
+        """HIP-event timing of the SpMM launches of later solves (resets)."""
+        _lib.check(_lib.load().hdfe_csr_timing(self._h, int(enable)), "hdfe_csr_timing")
+
+    def timing_read(self) -> dict:
+        ms = C.c_double()
+        n = C.c_int64()
+        _lib.check(_lib.load().hdfe_csr_timing_read(self._h, C.byref(ms), C.byref(n)),
+                   "hdfe_csr_timing_read")
+        return {"ms": float(ms.value), "launches": int(n.value)}
+
+    def export(self):
+        """(scipy CSR S, dinv) copied back from the device."""
+        import scipy.sparse as sp
+        L = self.L
+        nnz = C.c_int64()
+        _lib.check(_lib.load().hdfe_csr_info(self._h, None, C.byref(nnz), None), "hdfe_csr_info")
+        nnz = int(nnz.value)
+        indptr = np.empty(L + 1, np.int64)
+        indices = np.empty(max(nnz, 1), np.int32)
+        data = np.empty(max(nnz, 1))
+        dinv = np.empty(max(L, 1))
+        _lib.check(_lib.load().hdfe_csr_export(self._h, indptr.ctypes.data, indices.ctypes.data,
+                                               data.ctypes.data, dinv.ctypes.data),
+                   "hdfe_csr_export")
+        return sp.csr_matrix((data[:nnz], indices[:nnz], indptr), shape=(L, L)), dinv[:L]
+
     def close(self):
-        if self._h:
+        if self._h and self._owned:
             _lib.load().hdfe_destroy(self._h)
-            self._h = None
+        self._h = None
 
     def __del__(self):
         try:

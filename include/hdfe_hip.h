@@ -66,9 +66,84 @@ int hdfe_pcg_solve(void *h, int64_t k, const double *b, const uint8_t *zero, con
  * _nb_csr_matmat, core/kernels_base.py:361-371. */
 int hdfe_csr_matmat(void *h, int64_t k, const double *x, double *y, int on_device);
 
+/* Copy S as the device holds it to host arrays (indptr int64 (L+1),
+ * indices int32, data float64, dinv float64 (L)); any may be NULL.  For
+ * checks of the device-built S against the reference's (_build_explicit,
+ * core/solve.py:148-194). */
+int hdfe_csr_export(void *h, int64_t *indptr, int32_t *indices, double *data, double *dinv);
+
+/* HIP-event timing of the CG's SpMM launches of later solves on this handle
+ * (enable resets the totals) and the totals (milliseconds, launches).  The
+ * reference has no counterpart; its timings are the solver-info seconds
+ * (core/solve.py:370-381). */
+int hdfe_csr_timing(void *h, int enable);
+int hdfe_csr_timing_read(void *h, double *ms, int64_t *launches);
+
+/* Column norms sqrt(sum_i X_ij^2) of a device (L, k) block, to the host,
+ * in a fixed order: the zero-column test of core/solve.py:354-355. */
+int hdfe_col_norms(const double *X, int64_t L, int64_t k, double *out);
+
+/* dst <- src; kind 0 device to host, 1 host to device, 2 device to device. */
+int hdfe_memcpy(void *dst, const void *src, int64_t bytes, int kind);
+
 /* Free the handle and its device memory (the host S release of
  * core/solve.py:222-223 has no counterpart: nothing is kept on the host). */
 void hdfe_destroy(void *h);
+
+/* --------------------------------------------------- panel (passes 0-2) */
+
+/* Passes 0 and 1 on the device for a panel resident in HBM: fe (nfe device
+ * pointers to int64 id columns), vars (m device pointers to float64 columns,
+ * the outcome first), weights (NULL, or a float32 (weights_f32 = 1) or
+ * float64 device column).  Drops singletons when fixef_rm_singleton (dimensions
+ * in turn, core/passes.py:859-904), streams the dimension with the most levels
+ * unless stream_dim >= 0 (core/estimator.py:371-385), codes every other
+ * dimension in sorted level order and sorts the rows by (group, codes[,
+ * variables if sort_vars]) (core/passes.py:311-394), then collapses cells and
+ * keeps the identifying ones (core/passes.py:397-589).  *out: the panel. */
+int hdfe_panel_create(int64_t n_rows, int nfe, const int64_t *const *fe, int m,
+                      const double *const *vars, const void *weights, int weights_f32,
+                      int fixef_rm_singleton, int stream_dim, int sort_vars, void **out);
+
+/* info[16] = n_in, n, G, C, M, Gi, L, D, gdim, singletons, singleton rounds,
+ * incidences, nnz(S), components, m, weighted; nlev[D], dims[D] (may be NULL). */
+int hdfe_panel_info(void *h, int64_t *info, int64_t *nlev, int *dims);
+
+/* Device pointer and length of a panel array by name ("g", "codes", "V", "w",
+ * "perm", "gstart", "cstart", "glevels", "levels<t>", "means", "raw_ss", "W",
+ * "obs", "cnt", "icodes", "in", "isums", "igstart", "igroup", "diag", "rhs",
+ * "comp", "offs"). */
+int hdfe_panel_array(void *h, const char *name, void **ptr, int64_t *count);
+
+/* Step 2's system on the device: diag (core/kernels_base.py:86-110), the
+ * right-hand side (:64-83) and the explicit S (core/solve.py:148-194,
+ * core/kernels_base.py:192-358) with its solver handle in *matrix (owned by the
+ * panel: use it with hdfe_pcg_solve, never hdfe_destroy).  *seconds: wall time. */
+int hdfe_panel_system(void *h, int with_rhs, void **matrix, double *seconds);
+
+/* Components of the first non-streamed dimension through the identifying
+ * groups, ranked by rows with ties to the smallest level
+ * (core/kernels_base.py:155-174, core/passes.py:933-979). */
+int hdfe_panel_components(void *h, int64_t *ncomp);
+
+/* Normalise Gamma (L, k), device, in place: obs-weighted mean zero per
+ * component for the first dimension, overall for the others
+ * (core/solve.py:383-399).  stream may be NULL. */
+int hdfe_panel_normalize(void *h, double *Gamma, int k, void *stream);
+
+/* A = V' M_D V (m x m, host) from the cells: the within-cell scatter of pass 1
+ * plus the between-cell part (core/inference.py:109-115,
+ * core/kernels_base.py:510-540).  m <= 8. */
+int hdfe_panel_assemble(void *h, const double *Gamma, double *A);
+
+/* Pass 2 (core/kernels_base.py:543-619): gam_y = Gamma[:, 0] - Gamma[:, xcols]
+ * beta (L), the streamed effects geff (G) and residuals (n, sorted row
+ * order), all device outputs; beta and xcols (kx) on the host. */
+int hdfe_panel_pass2(void *h, const double *Gamma, const double *beta, const int *xcols, int kx,
+                     double *gam_y, double *geff, double *resid);
+
+/* Free the panel, its device arrays and its matrix. */
+void hdfe_panel_destroy(void *h);
 
 #ifdef __cplusplus
 }

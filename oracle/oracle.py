@@ -372,7 +372,9 @@ def fit(data: dict, y: str = "y", xs=("x1", "x2"), fe=("worker_id", "firm_id", "
     beta = np.linalg.solve(A[np.ix_(xj, xj)], A[xj, 0]) if xj else np.zeros(0)
     gam_y = Gamma[:, 0] - Gamma[:, xj] @ beta
     geff, e = pass2(p0, c, gam_y, beta, xj)
-    return {"beta": beta, "Gamma": Gamma, "gam_y": gam_y, "A": A, "S": S, "diag": d, "rhs": b,
+    resid_src = np.full(len(p0["keep"]), np.nan)
+    resid_src[np.flatnonzero(p0["keep"])[p0["order"]]] = e
+    return {"resid_source_order": resid_src,"beta": beta, "Gamma": Gamma, "gam_y": gam_y, "A": A, "S": S, "diag": d, "rhs": b,
             "comp": comp, "n_components": n_comp, "iterations": its, "p0": p0, "cells": c,
             "worker_effects": geff, "resid": e}
 
