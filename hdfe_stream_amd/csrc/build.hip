@@ -469,28 +469,35 @@ __global__ void k_level_keys(int64_t E, const int *__restrict__ ig, const int *_
 
 // diag[a] = sum over groups containing a of (u - u^2 / N_g)  (kernels_base.py:86-110)
 // rhs[a, j] = sum of (s_j - u vbar_j)                          (kernels_base.py:64-83)
+// One wave per level: lanes stride over the level's incidences, then a
+// fixed butterfly, so a level in millions of groups (a year) is spread over
+// 64 lanes.
 __global__ void k_diag_rhs(int64_t L, int m, const int *__restrict__ lptr,
                            const int *__restrict__ lq, const int *__restrict__ ig,
                            const double *__restrict__ iu, const double *__restrict__ is,
                            const double *__restrict__ Ng, const double *__restrict__ vbar,
                            double *__restrict__ diag, double *__restrict__ rhs) {
-    for (int64_t a = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; a < L;
-         a += (int64_t)gridDim.x * blockDim.x) {
+    const int lane = threadIdx.x & 63;
+    const int64_t waves = (int64_t)gridDim.x * (blockDim.x / 64);
+    for (int64_t a = blockIdx.x * (int64_t)(blockDim.x / 64) + threadIdx.x / 64; a < L;
+         a += waves) {
         double d = 0.0;
-        for (int t = lptr[a]; t < lptr[a + 1]; ++t) {
+        for (int t = lptr[a] + lane; t < lptr[a + 1]; t += 64) {
             const int q = lq[t];
             const double u = iu[q];
             d += u - u * u / Ng[ig[q]];
         }
-        diag[a] = d;
+        d = wave_sum(d);
+        if (lane == 0) diag[a] = d;
         if (rhs)
             for (int j = 0; j < m; ++j) {
                 double r = 0.0;
-                for (int t = lptr[a]; t < lptr[a + 1]; ++t) {
+                for (int t = lptr[a] + lane; t < lptr[a + 1]; t += 64) {
                     const int q = lq[t];
                     r += is[(int64_t)q * m + j] - iu[q] * vbar[(int64_t)ig[q] * m + j];
                 }
-                rhs[a * m + j] = r;
+                r = wave_sum(r);
+                if (lane == 0) rhs[a * m + j] = r;
             }
     }
 }
@@ -577,43 +584,77 @@ __global__ void k_dense_u(int64_t E, const int *__restrict__ ig, const int *__re
     }
 }
 
-// Sparse row a x dense level t: sum over groups g containing a of
-// n_g(a, t) - u_a U[g, t] / N_g, present when t occurs in such a group.
-__global__ void k_sd(int64_t Ls, int Ld, const int *__restrict__ sparse_levels,
-                     const int *__restrict__ dense_levels, const int *__restrict__ lvl_dim,
-                     const int64_t *__restrict__ offs, const int *__restrict__ lptr,
-                     const int *__restrict__ lq, const int *__restrict__ ig,
-                     const double *__restrict__ iu, const double *__restrict__ Ng,
-                     const double *__restrict__ U, const int *__restrict__ igstart,
-                     const int *__restrict__ icodes, int64_t ild, const double *__restrict__ in,
-                     double *__restrict__ PD, uint8_t *__restrict__ PP) {
+// Sparse x dense block, first per incidence: for incidence q = (group g,
+// sparse level a) and dense level t, C[q, t] = n_g(a, t) - u_a U[g, t] / N_g
+// (n_g(a, t): the weight of g's cells holding both a and t), with bit t of
+// F[q] set when t occurs in g.  One scan of g's cells per incidence.
+__global__ void k_sd_inc(int64_t E, int D, int Ld, const int *__restrict__ ig,
+                         const int *__restrict__ il, const double *__restrict__ iu,
+                         const int *__restrict__ sidx, const uint8_t *__restrict__ sparse_lv,
+                         const int *__restrict__ lvl_dim, const int64_t *__restrict__ offs,
+                         const uint8_t *__restrict__ sparse_dim, const int *__restrict__ dense_idx,
+                         const double *__restrict__ U, const double *__restrict__ Ng,
+                         const int *__restrict__ igstart, const int *__restrict__ icodes,
+                         int64_t ild, const double *__restrict__ in, double *__restrict__ Cq,
+                         uint64_t *__restrict__ Fq) {
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < E;
+         q += (int64_t)gridDim.x * blockDim.x) {
+        const int a = il[q];
+        if (!sparse_lv[a]) continue;
+        const int64_t r = sidx[q];
+        const int g = ig[q];
+        const int da = lvl_dim[a];
+        const int la = a - (int)offs[da];
+        double *row = Cq + r * Ld;
+        const double f = iu[q] / Ng[g];
+        uint64_t bits = 0;
+        for (int t = 0; t < Ld; ++t) {
+            const double ut = U[(int64_t)g * Ld + t];
+            row[t] = -f * ut;
+            if (ut != 0.0) bits |= 1ull << t;
+        }
+        for (int c = igstart[g]; c < igstart[g + 1]; ++c) {
+            if (icodes[da * ild + c] != la) continue;
+            for (int d = 0; d < D; ++d) {
+                if (sparse_dim[d]) continue;
+                const int t = dense_idx[offs[d] + icodes[d * ild + c]];
+                row[t] += in[c];
+            }
+        }
+        Fq[r] = bits;
+    }
+}
+
+// Then per (sparse level a, dense level t): the sum over a's incidences, in
+// level-CSR order (fixed), and the presence.
+__global__ void k_sd_sum(int64_t Ls, int Ld, const int *__restrict__ sparse_levels,
+                         const int *__restrict__ lptr, const int *__restrict__ lq,
+                         const int *__restrict__ sidx, const double *__restrict__ Cq,
+                         const uint64_t *__restrict__ Fq, double *__restrict__ PD,
+                         uint8_t *__restrict__ PP) {
     const int64_t n = Ls * (int64_t)Ld;
     for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n;
          e += (int64_t)gridDim.x * blockDim.x) {
         const int64_t s = e / Ld;
         const int t = (int)(e - s * Ld);
         const int a = sparse_levels[s];
-        const int da = lvl_dim[a];
-        const int la = a - (int)offs[da];
-        const int b = dense_levels[t];
-        const int db = lvl_dim[b];
-        const int lb = b - (int)offs[db];
         double v = 0.0;
-        int pres = 0;
+        uint64_t pres = 0;
         for (int p = lptr[a]; p < lptr[a + 1]; ++p) {
-            const int q = lq[p];
-            const int g = ig[q];
-            const double ut = U[(int64_t)g * Ld + t];
-            if (ut == 0.0) continue;
-            pres = 1;
-            double nat = 0.0;
-            for (int c = igstart[g]; c < igstart[g + 1]; ++c)
-                if (icodes[da * ild + c] == la && icodes[db * ild + c] == lb) nat += in[c];
-            v += nat - iu[q] * ut / Ng[g];
+            const int64_t r = sidx[lq[p]];
+            v += Cq[r * Ld + t];
+            pres |= Fq[r];
         }
         PD[e] = v;
-        PP[e] = (uint8_t)pres;
+        PP[e] = (uint8_t)((pres >> t) & 1ull);
     }
+}
+
+__global__ void k_sparse_flag(int64_t E, const int *__restrict__ il,
+                              const uint8_t *__restrict__ sparse_lv, int *__restrict__ flag) {
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q <= E;
+         q += (int64_t)gridDim.x * blockDim.x)
+        flag[q] = q < E ? (int)sparse_lv[il[q]] : 0;
 }
 
 // Dense x dense block: per workgroup of groups, thread per (t1, t2) pair:
@@ -1585,7 +1626,7 @@ int hdfe_panel_system(void *h, int with_rhs, void **matrix, double *seconds) {
     }
     DEV_ALLOC(P.diag, L);
     if (with_rhs) DEV_ALLOC(P.rhs, L * m);
-    hipLaunchKernelGGL(k_diag_rhs, dim3(grid1(L)), dim3(T256), 0, s, L, m, P.lptr.p, P.lq.p, P.ig.p,
+    hipLaunchKernelGGL(k_diag_rhs, dim3(grid1(L * 64)), dim3(T256), 0, s, L, m, P.lptr.p, P.lq.p, P.ig.p,
                        P.iu.p, P.is.p, P.Ng.p, P.vbar.p, P.diag.p,
                        with_rhs ? P.rhs.p : (double *)nullptr);
     HIP_TRY(hipGetLastError());
@@ -1718,9 +1759,29 @@ int hdfe_panel_system(void *h, int with_rhs, void **matrix, double *seconds) {
         HIP_TRY(hipMemsetAsync(U.p, 0, std::max<int64_t>(Gi * Ld, 1) * 8, s));
         hipLaunchKernelGGL(k_dense_u, dim3(grid1(E)), dim3(T256), 0, s, (int64_t)E, P.ig.p, P.il.p,
                            P.iu.p, d_dense_idx.p, Ld, U.p);
-        hipLaunchKernelGGL(k_sd, dim3(grid1(Ls * Ld)), dim3(T256), 0, s, Ls, Ld, d_sparse_levels.p,
-                           d_dense_levels.p, d_lvl_dim.p, P.offs.p, P.lptr.p, P.lq.p, P.ig.p,
-                           P.iu.p, P.Ng.p, U.p, P.igstart.p, P.icodes.p, M, P.in.p, PD.p, PP.p);
+        {
+            DevBuf<int> sflag, sidx;
+            DEV_ALLOC(sflag, (int64_t)E + 1);
+            DEV_ALLOC(sidx, (int64_t)E + 1);
+            hipLaunchKernelGGL(k_sparse_flag, dim3(grid1((int64_t)E + 1)), dim3(T256), 0, s,
+                               (int64_t)E, P.il.p, d_sparse_lv.p, sflag.p);
+            HIP_TRY(exclusive_sum(P.tmp, sflag.p, sidx.p, (int64_t)E + 1, s));
+            int Es = 0;
+            HIP_TRY(scalar_to_host(&Es, sidx.p + E, s));
+            DevBuf<double> Cq;
+            DevBuf<uint64_t> Fq;
+            DEV_ALLOC(Cq, std::max<int64_t>((int64_t)Es * Ld, 1));
+            DEV_ALLOC(Fq, std::max(Es, 1));
+            hipLaunchKernelGGL(k_sd_inc, dim3(grid1(E)), dim3(T256), 0, s, (int64_t)E, D, Ld,
+                               P.ig.p, P.il.p, P.iu.p, sidx.p, d_sparse_lv.p, d_lvl_dim.p,
+                               P.offs.p, d_sparse_dim.p, d_dense_idx.p, U.p, P.Ng.p,
+                               P.igstart.p, P.icodes.p, M, P.in.p, Cq.p, Fq.p);
+            hipLaunchKernelGGL(k_sd_sum, dim3(grid1(Ls * Ld)), dim3(T256), 0, s, Ls, Ld,
+                               d_sparse_levels.p, P.lptr.p, P.lq.p, sidx.p, Cq.p, Fq.p, PD.p,
+                               PP.p);
+            HIP_TRY(hipGetLastError());
+            HIP_TRY(hipStreamSynchronize(s));
+        }
         const int nbd = 1024;
         DevBuf<double> ddp;
         DevBuf<int> ddpp;

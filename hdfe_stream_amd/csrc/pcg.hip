@@ -923,18 +923,23 @@ int matrix_create(int64_t L, int64_t nnz, const std::vector<int> &ip32, const in
     HIP_TRY(hipMalloc(&M->val32, std::max<int64_t>(nnz, 1) * 4));
     HIP_TRY(hipMalloc(&M->dinv64, std::max<int64_t>(L, 1) * 8));
     HIP_TRY(hipMalloc(&M->dinv32, std::max<int64_t>(L, 1) * 4));
-    HIP_TRY(hipMemcpy(M->indptr, ip32.data(), (L + 1) * 4, hipMemcpyHostToDevice));
+    // every upload and the FP32 casts in order on the matrix's own stream (a
+    // device-to-device hipMemcpy may return before the copy has finished)
+    hipStream_t ms = M->stream;
+    HIP_TRY(hipMemcpyAsync(M->indptr, ip32.data(), (L + 1) * 4, hipMemcpyHostToDevice, ms));
     const hipMemcpyKind up = on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
     if (nnz) {
-        HIP_TRY(hipMemcpy(M->indices, indices, nnz * 4, up));
-        HIP_TRY(hipMemcpy(M->val64, data, nnz * 8, up));
+        HIP_TRY(hipMemcpyAsync(M->indices, indices, nnz * 4, up, ms));
+        HIP_TRY(hipMemcpyAsync(M->val64, data, nnz * 8, up, ms));
     }
-    if (L) HIP_TRY(hipMemcpy(M->dinv64, dinv, L * 8, up));
+    if (L) HIP_TRY(hipMemcpyAsync(M->dinv64, dinv, L * 8, up, ms));
     const int g1 = grid_for(std::max<int64_t>(nnz, 1), 1024, 1, 4096);
-    hipLaunchKernelGGL(k_cast_f64_to_f32, dim3(g1), dim3(kThreads), 0, M->stream, nnz, M->val64,
+    hipLaunchKernelGGL(k_cast_f64_to_f32, dim3(g1), dim3(kThreads), 0, ms, nnz, M->val64,
                        M->val32);
     hipLaunchKernelGGL(k_cast_f64_to_f32, dim3(grid_for(std::max<int64_t>(L, 1), 1024, 1, 4096)),
-                       dim3(kThreads), 0, M->stream, L, M->dinv64, M->dinv32);
+                       dim3(kThreads), 0, ms, L, M->dinv64, M->dinv32);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(ms));
     if (M->nlong) {
         HIP_TRY(hipMalloc(&M->long_rows, M->nlong * 4));
         HIP_TRY(hipMalloc(&M->long_cptr, (M->nlong + 1) * 4));
@@ -1118,6 +1123,31 @@ int hdfe_memcpy(void *dst, const void *src, int64_t bytes, int kind) {
     const hipMemcpyKind k = kind == 0 ? hipMemcpyDeviceToHost
                             : kind == 1 ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice;
     HIP_TRY(hipMemcpy(dst, src, (size_t)bytes, k));
+    if (kind == 2) HIP_TRY(hipDeviceSynchronize());  // a D2D copy may return early
+    return HDFE_STATUS_OK;
+}
+
+// y = S x in FP32 (the inner solver's operator: FP32 values, FP32 vectors),
+// host or device float32 blocks.
+int hdfe_csr_matmat_f32(void *h, int64_t k, const float *x, float *y, int on_device) {
+    ARG_CHECK(h != nullptr && x != nullptr && y != nullptr, "hdfe_csr_matmat_f32: null argument");
+    ARG_CHECK(k >= 1 && k <= kMaxK, "hdfe_csr_matmat_f32: k must be in [1, 1024]");
+    Matrix &M = *(Matrix *)h;
+    std::lock_guard<std::mutex> lock(M.mu);
+    HIP_TRY(hipSetDevice(M.device));
+    int st = alloc_work(M, (int)k);
+    if (st) return st;
+    hipStream_t s = M.stream;
+    const int64_t Lk = M.L * k;
+    const hipMemcpyKind up = on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    HIP_TRY(hipMemcpyAsync(M.w.P32, x, Lk * 4, up, s));
+    CgState plain = make_state(M.w, (int)k, 0.0, 0);
+    plain.done = nullptr;
+    launch_spmm<float>(M, M.val32, M.w.P32, M.w.AP32, M.w.part, M.w.lpart, 0, plain, s);
+    HIP_TRY(hipGetLastError());
+    const hipMemcpyKind down = on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+    HIP_TRY(hipMemcpyAsync(y, M.w.AP32, Lk * 4, down, s));
+    HIP_TRY(hipStreamSynchronize(s));
     return HDFE_STATUS_OK;
 }
 

@@ -104,6 +104,15 @@ class DeviceCSR:
                    "hdfe_csr_matmat")
         return Y
 
+    def matmat_f32(self, X):
+        """S X in FP32 on the device (host float32 in and out)."""
+        X = np.ascontiguousarray(X, np.float32)
+        k = X.shape[1] if X.ndim == 2 else 1
+        Y = np.empty_like(X)
+        _lib.check(_lib.load().hdfe_csr_matmat_f32(self._h, k, X.ctypes.data, Y.ctypes.data, 0),
+                   "hdfe_csr_matmat_f32")
+        return Y
+
     def solve(self, b, zero=None, x0=None, tol=1e-10, maxiter=5000, precision="fp64",
               inner_tol=1e-5, info=None):
         """(X, iterations, converged, max_rel); see the module docstring."""

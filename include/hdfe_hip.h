@@ -66,6 +66,10 @@ int hdfe_pcg_solve(void *h, int64_t k, const double *b, const uint8_t *zero, con
  * _nb_csr_matmat, core/kernels_base.py:361-371. */
 int hdfe_csr_matmat(void *h, int64_t k, const double *x, double *y, int on_device);
 
+/* y = S x in FP32 (FP32 values and vectors: the operator of the mixed
+ * precision inner solve), host or device float32 (L, k) blocks. */
+int hdfe_csr_matmat_f32(void *h, int64_t k, const float *x, float *y, int on_device);
+
 /* Copy S as the device holds it to host arrays (indptr int64 (L+1),
  * indices int32, data float64, dinv float64 (L)); any may be NULL.  For
  * checks of the device-built S against the reference's (_build_explicit,

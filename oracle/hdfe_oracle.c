@@ -674,3 +674,11 @@ int64_t orc_bfs_components(int64_t nv, const int64_t *indptr, const int64_t *ind
     free(queue);
     return comp;
 }
+
+/* The splitmix64 finaliser alone (the mixing step of orc_rademacher), for
+ * checking its constants against the generator's published outputs. */
+uint64_t orc_mix64(uint64_t x) {
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBULL;
+    return x ^ (x >> 31);
+}

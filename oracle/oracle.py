@@ -64,6 +64,8 @@ def lib():
         _lib = C.CDLL(str(LIB))
         _lib.orc_s_count.restype = C.c_int64
         _lib.orc_bfs_components.restype = C.c_int64
+        _lib.orc_mix64.restype = C.c_uint64
+        _lib.orc_mix64.argtypes = [C.c_uint64]
     return _lib
 
 
@@ -380,6 +382,10 @@ def fit(data: dict, y: str = "y", xs=("x1", "x2"), fe=("worker_id", "firm_id", "
 
 
 # ------------------------------------------------------------------ KSS
+
+def mix64(x: int) -> int:
+    return int(lib().orc_mix64(C.c_uint64(x)))
+
 
 def rademacher(ordinal: int, n_rows: int, draws: int, seed: int, first_draw: int) -> np.ndarray:
     out = np.empty((n_rows, draws))

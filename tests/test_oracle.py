@@ -1,0 +1,378 @@
+"""Pinning the oracle: each restated piece of the reference's algorithm is
+checked against what the mathematics fixes -- dense closed forms, scipy and
+numpy routines, brute force on small inputs -- and against published
+known-answer vectors, never against a retyping of its own formula."""
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+from scipy.sparse.csgraph import connected_components
+
+from conftest import rel
+
+FE = ["worker_id", "firm_id", "year"]
+
+
+def dense_design(c):
+    """The identifying cells' level indicators A (M x L)."""
+    M, D = c["codes"].shape
+    A = np.zeros((M, c["L"]))
+    for d in range(D):
+        A[np.arange(M), c["offs"][d] + c["codes"][:, d].astype(int)] = 1
+    return A
+
+
+def dense_S(c):
+    """S = sum over groups of A_g' N_g A_g - u_g u_g' / N_g."""
+    A = dense_design(c)
+    S = np.zeros((c["L"], c["L"]))
+    for g in range(len(c["starts"]) - 1):
+        s, e = c["starts"][g], c["starts"][g + 1]
+        Ag, n = A[s:e], c["n"][s:e]
+        u = Ag.T @ n
+        S += Ag.T @ (n[:, None] * Ag) - np.outer(u, u) / n.sum()
+    return S
+
+
+def dummy_fit(d, y, xs, fe, keep, w=None):
+    """OLS on explicit dummies for every fixed effect: beta and residuals."""
+    cols = [np.column_stack([np.asarray(d[x], float)[keep] for x in xs])]
+    for f in fe:
+        v = np.asarray(d[f])[keep]
+        u, inv = np.unique(v, return_inverse=True)
+        D = np.zeros((len(v), len(u)))
+        D[np.arange(len(v)), inv] = 1
+        cols.append(D)
+    Z = np.hstack(cols)
+    yy = np.asarray(d[y], float)[keep]
+    sw = np.ones(len(yy)) if w is None else np.sqrt(np.asarray(d[w], float)[keep])
+    theta = np.linalg.lstsq(Z * sw[:, None], yy * sw, rcond=None)[0]
+    return theta[: len(xs)], yy - Z @ theta
+
+
+@pytest.fixture(scope="module")
+def tiny():
+    from hdfe_stream_amd.simulate import akm_panel
+    return akm_panel(300, 40, seed=3)
+
+
+@pytest.mark.parametrize("fe", [FE, ["worker_id", "firm_id"], ["year", "worker_id", "firm_id"]])
+def test_S_diag_rhs_match_the_dense_closed_form(oracle_mod, tiny, fe):
+    O = oracle_mod
+    c = O.pass1(O.pass0(tiny, "y", ["x1", "x2"], fe))
+    S = dense_S(c)
+    Sx = O.build_S(c)
+    assert np.abs(Sx.toarray() - S).max() <= 1e-12 * np.abs(S).max()
+    # the sparsity: every pair of levels sharing an identifying group
+    A = dense_design(c)
+    share = np.zeros_like(S, bool)
+    for g in range(len(c["starts"]) - 1):
+        lv = np.flatnonzero(A[c["starts"][g]:c["starts"][g + 1]].any(axis=0))
+        share[np.ix_(lv, lv)] = True
+    assert Sx.nnz == share.sum()
+    np.testing.assert_allclose(O.diag(c), np.diag(S), rtol=1e-12, atol=1e-12)
+    # rhs = D_o' M_0 V with M_0 the within-group centring of the cell means
+    b = np.zeros((c["L"], c["m"]))
+    for g in range(len(c["starts"]) - 1):
+        s, e = c["starts"][g], c["starts"][g + 1]
+        sm, n = c["sums"][s:e], c["n"][s:e]
+        b += A[s:e].T @ (sm - n[:, None] * sm.sum(0) / n.sum())
+    assert np.abs(O.rhs(c) - b).max() <= 1e-12 * np.abs(b).max()
+    # the matrix-free operator is the same operator
+    P = np.random.default_rng(0).standard_normal((c["L"], 3))
+    np.testing.assert_allclose(O.stream_matvec(c, P), S @ P, rtol=1e-10, atol=1e-10)
+
+
+def test_pcg_solves_the_singular_system(oracle_mod, tiny):
+    O = oracle_mod
+    c = O.pass1(O.pass0(tiny, "y", ["x1", "x2"], FE))
+    S = O.build_S(c)
+    dinv = O.jacobi(O.diag(c))
+    b = O.rhs(c)
+    X, it, ok, rel_ = O.pcg(S, dinv, b, None, None, 1e-12, 5000)
+    assert ok and rel_ <= 1e-12 and it > 0
+    # against the minimum-norm dense solution, up to S's null space
+    Xd = np.linalg.pinv(S.toarray()) @ b
+    assert np.abs(S @ (X - Xd)).max() <= 1e-9 * np.abs(b).max()
+    # zero columns stay zero; a solved start takes no iterations
+    X2, it2, ok2, _ = O.pcg(S, dinv, b, np.array([False, True, False]), X, 1e-10, 5000)
+    assert ok2 and it2 == 0 and np.all(X2[:, 1] == 0)
+    # maxiter is honoured and non-convergence reported
+    _, it3, ok3, r3 = O.pcg(S, dinv, b, None, None, 1e-14, 2)
+    assert it3 == 2 and not ok3 and r3 > 1e-14
+
+
+@pytest.mark.parametrize("weights", [None, "w"])
+def test_fit_matches_ols_on_explicit_dummies(oracle_mod, weights):
+    from hdfe_stream_amd.simulate import weighted_panel
+    O = oracle_mod
+    d = weighted_panel(400, 30, seed=5)
+    d["w"] = d["w"].astype(np.float64)
+    r = O.fit(d, weights=weights, tol=1e-12)
+    beta, e = dummy_fit(d, "y", ["x1", "x2"], FE, r["p0"]["keep"], weights)
+    assert rel(r["beta"], beta) < 1e-9
+    e_src = r["resid_source_order"][r["p0"]["keep"]]
+    assert np.abs(e_src - e).max() < 1e-8
+
+
+def test_normalisation_invariants(oracle_mod, small_panel):
+    O = oracle_mod
+    r = O.fit(small_panel, tol=1e-12)
+    c, G = r["cells"], r["Gamma"]
+    nl = r["p0"]["n_levels"]
+    w = c["obs"][:nl[0]]
+    for k in range(r["n_components"]):
+        sel = r["comp"] == k
+        assert abs(w[sel] @ G[:nl[0]][sel, 0]) <= 1e-9 * w[sel].sum()
+    wy = c["obs"][nl[0]:]
+    assert abs(wy @ G[nl[0]:, 0]) <= 1e-9 * wy.sum()
+    # fitted values do not depend on the normalisation: the residuals match
+    # the dummy-variable fit
+    beta, e = dummy_fit(small_panel, "y", ["x1", "x2"], FE, r["p0"]["keep"])
+    assert np.abs(r["resid_source_order"][r["p0"]["keep"]] - e).max() < 1e-8
+
+
+def test_components_match_scipy_and_rank_by_rows(oracle_mod):
+    from hdfe_stream_amd.simulate import akm_panel
+    O = oracle_mod
+    d = akm_panel(3000, 900, p_move=0.02, seed=8)     # many components
+    p0 = O.pass0(d, "y", ["x1"], FE)
+    c = O.pass1(p0)
+    nf = p0["n_levels"][0]
+    comp, nc = O.rank_components(O.components(c, nf), c["cnt"][:nf])
+    # scipy on the firm graph: firms joined by a shared identifying worker
+    rows, cols = [], []
+    for g in range(len(c["starts"]) - 1):
+        f = np.unique(c["codes"][c["starts"][g]:c["starts"][g + 1], 0])
+        rows += [f[0]] * len(f)
+        cols += list(f)
+    G = sp.csr_matrix((np.ones(len(rows)), (rows, cols)), shape=(nf, nf))
+    n_sc, lab = connected_components(G, directed=False)
+    assert nc == n_sc > 1
+    # the same partition
+    pairs = set(zip(comp.tolist(), lab.tolist()))
+    assert len(pairs) == nc
+    # ranked by rows, ties to the component with the smallest firm code
+    size = np.bincount(comp, weights=c["cnt"][:nf])
+    first = np.array([np.flatnonzero(comp == k)[0] for k in range(nc)])
+    order = sorted(range(nc), key=lambda k: (-size[k], first[k]))
+    assert order == list(range(nc))
+
+
+def test_singleton_restriction_is_the_largest_fixed_point(oracle_mod):
+    O = oracle_mod
+    rng = np.random.default_rng(2)
+    n = 400
+    cols = [rng.integers(0, 150, n), rng.integers(0, 120, n), rng.integers(0, 5, n)]
+    keep = O.singleton_restriction(cols)
+    # a fixed point: no kept level holds a single kept row
+    for c in cols:
+        _, cnt = np.unique(c[keep], return_counts=True)
+        assert cnt.min() > 1
+    # the largest one: any row dropped is forced out by repeated removal
+    alive = np.ones(n, bool)
+    changed = True
+    while changed:
+        changed = False
+        for c in cols:
+            u, inv, cnt = np.unique(c[alive], return_inverse=True, return_counts=True)
+            bad = np.flatnonzero(alive)[cnt[inv] == 1]
+            if len(bad):
+                alive[bad] = False
+                changed = True
+    np.testing.assert_array_equal(keep, alive)
+    # and the order of the dimensions does not matter
+    np.testing.assert_array_equal(O.singleton_restriction(cols[::-1]), keep)
+
+
+def test_rademacher_pins_to_the_published_splitmix64_outputs(oracle_mod):
+    """splitmix64 with seed 0 outputs mix(k * 0x9E3779B97F4A7C15), k = 1, 2, ...
+    (Vigna's reference generator); the draw for (seed 0, row k, draw 0) is
+    exactly that mix, and its sign is the top bit."""
+    O = oracle_mod
+    published = [0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4, 0x06C45D188009454F,
+                 0xF88BB8A8724C81EC, 0x1B39896A51A8749B]
+    G = 0x9E3779B97F4A7C15
+    for k, v in enumerate(published, 1):
+        assert O.mix64((k * G) & (2**64 - 1)) == v
+    draws = O.rademacher(1, 5, 1, 0, 0)[:, 0]
+    np.testing.assert_array_equal(draws, [-1.0 if v >> 63 else 1.0 for v in published])
+
+
+def test_rademacher_contract(oracle_mod):
+    """The reference's contract (tests/test_inverse.py:242-253)."""
+    O = oracle_mod
+    whole = O.rademacher(0, 100, 3, 77, 0)
+    assert np.array_equal(O.rademacher(0, 100, 3, 77, 0), whole)
+    assert np.array_equal(O.rademacher(40, 60, 3, 77, 0), whole[40:])
+    assert set(np.unique(whole)) <= {-1.0, 1.0}
+    assert abs(whole.mean()) < 0.2
+    assert not np.array_equal(O.rademacher(0, 100, 3, 78, 0), whole)
+    # blocks of draws are stretches of one sequence
+    assert np.array_equal(O.rademacher(0, 100, 2, 77, 1), whole[:, 1:3])
+
+
+def test_golden_rademacher_table(oracle_mod):
+    from pathlib import Path
+    g = np.load(Path(__file__).parent / "golden" / "rademacher.npz")
+    np.testing.assert_array_equal(
+        oracle_mod.rademacher(int(g["ordinal"]), g["table"].shape[0], g["table"].shape[1],
+                              int(g["seed"]), int(g["first_draw"])), g["table"])
+
+
+def _brute_cut_vertices(indptr, indices, alive):
+    nv = len(indptr) - 1
+    A = sp.csr_matrix((np.ones(len(indices)), indices, indptr), shape=(nv, nv))
+    live = np.flatnonzero(alive)
+    base, _ = connected_components(A[live][:, live], directed=False)
+    cut = np.zeros(nv, bool)
+    for v in live:
+        rest = live[live != v]
+        n, _ = connected_components(A[rest][:, rest], directed=False)
+        cut[v] = n > base
+    return cut
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_articulation_points_match_brute_force(oracle_mod, seed):
+    O = oracle_mod
+    rng = np.random.default_rng(seed)
+    n0, n1 = 25, 10
+    a = rng.integers(0, n0, 40)
+    b = rng.integers(0, n1, 40)
+    alive = np.ones(n0 + n1, bool)
+    if seed % 3 == 0:
+        alive[rng.integers(0, n0 + n1, 4)] = False
+    _, indptr, indices = O.pair_graph(a, b, n0, n1, alive)
+    got = O.articulation_points(indptr, indices, alive.astype(np.uint8))
+    np.testing.assert_array_equal(got & alive, _brute_cut_vertices(indptr, indices, alive))
+
+
+def test_articulation_points_known_graphs(oracle_mod):
+    O = oracle_mod
+
+    def csr(edges, nv):
+        adj = [[] for _ in range(nv)]
+        for x, y in edges:
+            adj[x].append(y)
+            adj[y].append(x)
+        indptr = np.concatenate(([0], np.cumsum([len(v) for v in adj])))
+        return indptr, np.array([y for v in adj for y in v], np.int64)
+    # a path 0-1-2-3: 1 and 2 are cut vertices
+    ip, ix = csr([(0, 1), (1, 2), (2, 3)], 4)
+    assert O.articulation_points(ip, ix, np.ones(4, np.uint8)).tolist() == [False, True, True, False]
+    # a cycle has none
+    ip, ix = csr([(0, 1), (1, 2), (2, 3), (3, 0)], 4)
+    assert not O.articulation_points(ip, ix, np.ones(4, np.uint8)).any()
+    # a star: the centre
+    ip, ix = csr([(0, 1), (0, 2), (0, 3)], 4)
+    assert O.articulation_points(ip, ix, np.ones(4, np.uint8)).tolist() == [True, False, False, False]
+
+
+def test_leave_one_out_set_has_no_cut_worker(oracle_mod):
+    """The pruned network is connected, has no worker with one row and no
+    worker whose removal disconnects it; a brute-force restatement on the
+    explicit network gives the same set."""
+    O = oracle_mod
+    from hdfe_stream_amd.simulate import akm_panel
+    d = akm_panel(400, 60, p_move=0.15, seed=4)
+    a = np.unique(d["worker_id"], return_inverse=True)[1]
+    b = np.unique(d["firm_id"], return_inverse=True)[1]
+    n0, n1 = a.max() + 1, b.max() + 1
+    alive, info = O.leave_one_out(a, b, n0, n1)
+    keys, indptr, indices = O.pair_graph(a, b, n0, n1, alive)
+    label, nc = O.bfs_components(indptr, indices, alive.astype(np.uint8))
+    assert nc == 1
+    cut = _brute_cut_vertices(indptr, indices, alive)
+    assert not (cut[:n0] & alive[:n0]).any()
+    rows = np.bincount(a[alive[a] & alive[n0 + b]], minlength=n0)
+    assert (rows[alive[:n0]] >= 2).all()
+    # brute force: the same loop on the explicit network
+    al = np.ones(n0 + n1, bool)
+    _, ip0, ix0 = O.pair_graph(a, b, n0, n1, al)
+
+    def largest(al):
+        _, ip, ix = O.pair_graph(a, b, n0, n1, al)
+        A = sp.csr_matrix((np.ones(len(ix)), ix, ip), shape=(n0 + n1, n0 + n1))
+        live = np.flatnonzero(al)
+        _, lab = connected_components(A[live][:, live], directed=False)
+        ok = al[a] & al[n0 + b]
+        lab_full = np.full(n0 + n1, -1)
+        lab_full[live] = lab
+        size = np.bincount(lab_full[a[ok]], minlength=lab.max() + 1)
+        # ties to the component holding the smallest vertex
+        best = max(range(len(size)), key=lambda k: (size[k], -live[lab == k].min()))
+        out = al.copy()
+        out[live[lab != best]] = False
+        return out
+    al = largest(al)
+    rows = np.bincount(a[al[a] & al[n0 + b]], minlength=n0)
+    al[:n0] &= ~((rows == 1) & al[:n0])
+    al = largest(al)
+    while True:
+        _, ip, ix = O.pair_graph(a, b, n0, n1, al)
+        cv = _brute_cut_vertices(ip, ix, al)[:n0] & al[:n0]
+        if not cv.any():
+            break
+        al[:n0] &= ~cv
+        al = largest(al)
+    np.testing.assert_array_equal(alive, al)
+
+
+def test_projection_matches_the_dense_hat_matrix(oracle_mod, tiny):
+    """project_rows(centre_rows(R)) = A (A'WA)^+ A'W R for the full design of
+    the fixed effects (tests/test_inverse.py:75-114 in the reference)."""
+    O = oracle_mod
+    p0 = O.pass0(tiny, "y", ["x1"], FE)
+    c = O.pass1(p0)
+    n = len(p0["g"])
+    starts = np.concatenate(([0], np.flatnonzero(p0["g"][1:] != p0["g"][:-1]) + 1, [n]))
+    w = np.random.default_rng(1).uniform(0.5, 2.0, n)
+    R = np.random.default_rng(2).standard_normal((n, 4))
+    # the projection needs the system of these weights: rebuild the cells with w
+    p0w = dict(p0, w=w)
+    cw = O.pass1(p0w)
+    S = O.build_S(cw)
+    dinv = O.jacobi(O.diag(cw))
+    acc, _ = O.center_rows(starts, p0["codes"], cw["offs"], w, np.zeros((n, 0)), R, cw["L"])
+    U, _, ok, _ = O.pcg(S, dinv, acc, None, None, 1e-13, 5000)
+    assert ok
+    proj = O.project_rows(starts, p0["codes"], cw["offs"], w, np.zeros((n, 0)),
+                          np.zeros((0, 4)), R, U)
+    # dense: A = [worker dummies, other dims' dummies]
+    Dw = np.zeros((n, len(starts) - 1))
+    Dw[np.arange(n), np.repeat(np.arange(len(starts) - 1), np.diff(starts))] = 1
+    Do = np.zeros((n, cw["L"]))
+    for d in range(p0["codes"].shape[1]):
+        Do[np.arange(n), cw["offs"][d] + p0["codes"][:, d]] = 1
+    A = np.hstack([Dw, Do])
+    H = A @ np.linalg.pinv(A.T @ (w[:, None] * A)) @ (A.T * w)
+    np.testing.assert_allclose(proj, H @ R, rtol=0, atol=1e-9)
+
+
+def test_jla_leverages_estimate_the_exact_ones(oracle_mod):
+    """Mean within 0.005 and correlation above 0.9 of the exact leverages,
+    and invariance to the block size (tests/test_leaveout.py:66-101)."""
+    O = oracle_mod
+    from hdfe_stream_amd.simulate import akm_panel
+    d = akm_panel(300, 30, seed=6)
+    p0 = O.pass0(d, "y", ["x1"], FE)
+    c = O.pass1(p0)
+    S = O.build_S(c)
+    dinv = O.jacobi(O.diag(c))
+    mom = O.jla_leverages(p0, c, S, dinv, n_draws=1000, block=250, seed=3)
+    P, M, F = O.combine_moments(mom, 1000)
+    np.testing.assert_allclose(P + M, 1.0, rtol=0, atol=1e-12)
+    n = len(p0["g"])
+    starts = np.concatenate(([0], np.flatnonzero(p0["g"][1:] != p0["g"][:-1]) + 1, [n]))
+    Dw = np.zeros((n, len(starts) - 1))
+    Dw[np.arange(n), np.repeat(np.arange(len(starts) - 1), np.diff(starts))] = 1
+    Do = np.zeros((n, c["L"]))
+    for k in range(p0["codes"].shape[1]):
+        Do[np.arange(n), c["offs"][k] + p0["codes"][:, k]] = 1
+    A = np.hstack([Dw, Do])
+    exact = np.einsum("ij,ji->i", A, np.linalg.pinv(A.T @ A) @ A.T)
+    assert abs(P.mean() - exact.mean()) < 0.005
+    assert np.corrcoef(P, exact)[0, 1] > 0.9
+    mom2 = O.jla_leverages(p0, c, S, dinv, n_draws=1000, block=333, seed=3)
+    np.testing.assert_allclose(mom2, mom, rtol=1e-10, atol=1e-12)
