@@ -69,6 +69,45 @@ def cpu_baseline(workers_sample, firms_sample, full_workers, seed):
                       f"{full_workers:,}-worker panel"}
 
 
+def jla_phase(n_workers, n_firms, draws):
+    """JLA leverages on the KSS panel (observation level: worker, firm and
+    year effects): `draws` Rademacher right-hand sides solved as one block in
+    mixed precision, beside FP64 single right-hand-side solves."""
+    import torch
+    from hdfe_stream_amd.estimator import Panel
+    from hdfe_stream_amd.leaveout import jla_leverages
+    from hdfe_stream_amd.simulate import kss_panel
+    d = kss_panel(n_workers, n_firms, seed=3)
+    corr = float(np.corrcoef(d["alpha"], d["psi"])[0, 1])
+    cols = {k: torch.from_numpy(d[k]).cuda() for k in ("worker_id", "firm_id", "year", "y")}
+    P = Panel([cols["worker_id"], cols["firm_id"], cols["year"]], [cols["y"]], sort_vars=True)
+    P.system(with_rhs=False)
+    out = {"workload": f"{n_workers:,} workers x 8 years, {n_firms:,} firms, p_move 0.15, "
+                       f"corr(alpha, psi) {corr:.3f}, observation level, {draws} draws",
+           "levels": P.info["L"], "rows": P.info["n"], "S_nnz": P.info["nnz"]}
+    jla_leverages(P, n_draws=draws, block=draws, seed=0, precision="mixed")     # warm
+    tm = {}
+    t0 = time.perf_counter()
+    lev = jla_leverages(P, n_draws=draws, block=draws, seed=0, precision="mixed", timings=tm)
+    torch.cuda.synchronize()
+    out.update(wall_s=round(time.perf_counter() - t0, 4),
+               solve_s=round(tm["solve_seconds"], 4), row_pass_s=round(tm["row_pass_seconds"], 4),
+               rhs_solves_per_s=round(tm["rhs_solves_per_second"], 1),
+               iterations=lev.diagnostics["iterations"], mean_leverage=lev.diagnostics["mean_leverage"])
+    tm64 = {}
+    lev64 = jla_leverages(P, n_draws=draws, block=draws, seed=0, precision="fp64", timings=tm64)
+    out["fp64_block_rhs_solves_per_s"] = round(tm64["rhs_solves_per_second"], 1)
+    out["leverage_max_abs_diff_mixed_vs_fp64"] = float(
+        (lev.leverage - lev64.leverage).abs().max().item())
+    ts = {}
+    jla_leverages(P, n_draws=8, block=1, seed=0, precision="fp64", timings=ts)
+    out["fp64_single_rhs_solves_per_s"] = round(ts["rhs_solves_per_second"], 1)
+    out["speedup_mixed_block_vs_fp64_single"] = round(
+        tm["rhs_solves_per_second"] / ts["rhs_solves_per_second"], 2)
+    P.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -77,6 +116,10 @@ def main():
     ap.add_argument("--workers", type=int, default=5_000_000)
     ap.add_argument("--firms", type=int, default=400_000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-jla", action="store_true")
+    ap.add_argument("--jla-workers", type=int, default=1_000_000)
+    ap.add_argument("--jla-firms", type=int, default=100_000)
+    ap.add_argument("--jla-draws", type=int, default=256)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported "
@@ -92,8 +135,8 @@ def main():
     data = make_inputs(args.workers, args.firms, seed=2)
     log(f"inputs in HBM ({time.time() - t:.1f} s)")
 
-    def fit(precision, timing=False):
-        return feols_stream(FML, data, precision=precision, tol=1e-10, spmm_timing=timing)
+    def fit(precision):
+        return feols_stream(FML, data, precision=precision, tol=1e-10)
 
     for w in range(args.warmup):
         fit("fp64")
@@ -105,7 +148,7 @@ def main():
         for prec in ("fp64", "mixed"):
             torch.cuda.synchronize()
             t0 = time.perf_counter()
-            r = fit(prec, timing=True)
+            r = fit(prec)
             torch.cuda.synchronize()
             times[prec].append(time.perf_counter() - t0)
             results[prec] = r
@@ -118,22 +161,22 @@ def main():
     fe_err = float(np.max(np.abs(gymx - gy64)) / np.max(np.abs(gy64)))
     t_mixed = float(np.mean(times["mixed"]))
     t_f64 = float(np.mean(times["fp64"]))
-    # roofline of the SpMM (FP64 fit's: the FP32 inner SpMM is reported too)
-    info = f64.solver_info
+    # roofline of the SpMM: the same kernel and launch as in the FP64 fit's
+    # CG, timed with HIP events around a burst of launches on the solver's
+    # stream, on the last fit's matrix (still resident)
     L = f64.panel.info["L"]
     nnz = f64.panel.info["nnz"]
     k = 3
-    st = info.get("spmm", {})
-    roof = None
-    if st.get("launches"):
-        avg = st["ms"] / st["launches"] / 1e3
-        byt = spmm_bytes(nnz, L, k, 8)
-        ach = byt / avg / 1e9
-        roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "k_spmm_small<double,3,8> (+ split-row chunks)",
-                "bytes_per_launch": int(byt), "avg_launch_us": round(avg * 1e6, 2)}
-    st32 = mix.solver_info.get("spmm", {})
+    info = f64.solver_info
+    avg64 = f64.panel.matrix.spmm_time(k, reps=100, precision="fp64") * 1e-6
+    avg32 = f64.panel.matrix.spmm_time(k, reps=100, precision="mixed") * 1e-6
+    byt = spmm_bytes(nnz, L, k, 8)
+    ach = byt / avg64 / 1e9
+    roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": "k_spmm_small<double,3,8>", "bytes_per_launch": int(byt),
+            "avg_launch_us": round(avg64 * 1e6, 2)}
+    b32 = spmm_bytes(nnz, L, k, 4)
     line = {
         "metric": "AKM fit wall-s @5M workers (mixed FP32/FP64-refined, 1e-10 rel. residual)",
         "value": round(t_mixed, 4), "unit": "s", "n_gpus": 1, "steps": args.steps,
@@ -153,9 +196,13 @@ def main():
         "beta_rel_err_mixed_vs_fp64": beta_err,
         "fe_rel_err_mixed_vs_fp64": fe_err,
         "roofline": roof,
-        "spmm_fp32_avg_us": (round(st32["ms"] / st32["launches"] * 1e3, 2)
-                             if st32.get("launches") else None),
+        "spmm_fp32": {"kernel": "k_spmm_small<float,3,8>", "avg_launch_us": round(avg32 * 1e6, 2),
+                      "bytes_per_launch": int(b32), "achieved_GBps": round(b32 / avg32 / 1e9, 1)},
     }
+    if not args.no_jla:
+        del results, f64, mix
+        log("JLA phase")
+        line["jla"] = jla_phase(args.jla_workers, args.jla_firms, args.jla_draws)
     if not args.no_cpu:
         log("CPU baseline (oracle, 1/5 sample)")
         line["cpu_baseline"] = cpu_baseline(args.workers // 5, args.firms // 5, args.workers, seed=2)

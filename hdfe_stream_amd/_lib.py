@@ -38,6 +38,7 @@ SIGNATURES = [
     ("hdfe_csr_export", I32, [P, P, P, P, P]),
     ("hdfe_col_norms", I32, [P, I64, I64, P]),
     ("hdfe_csr_timing", I32, [P, I32]),
+    ("hdfe_csr_spmm_time", I32, [P, I64, I32, I32, P]),
     ("hdfe_csr_timing_read", I32, [P, P, P]),
     ("hdfe_memcpy", I32, [P, P, I64, I32]),
     ("hdfe_destroy", None, [P]),
@@ -50,6 +51,12 @@ SIGNATURES = [
     ("hdfe_panel_assemble", I32, [P, P, P]),
     ("hdfe_panel_pass2", I32, [P, P, P, P, I32, P, P, P]),
     ("hdfe_panel_destroy", None, [P]),
+    ("hdfe_rademacher", I32, [I64, I64, I32, C.c_uint64, I64, P]),
+    ("hdfe_rows_create", I32, [I64, I64, I32, P, P, P, P]),
+    ("hdfe_rows_center", I32, [P, P, P, P, I32, C.c_uint64, I64, I32, P]),
+    ("hdfe_rows_project_moments", I32, [P, P, P, P, I32, C.c_uint64, I64, I32, P, P, P]),
+    ("hdfe_jla_combine", I32, [I64, I64, P, P]),
+    ("hdfe_rows_destroy", None, [P]),
 ]
 
 

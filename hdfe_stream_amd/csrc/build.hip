@@ -100,12 +100,30 @@ __global__ void k_minmax_partial(int64_t n, const int64_t *__restrict__ col,
     }
 }
 
-// Singletons: rows per level among the rows still kept.
+constexpr int kLdsBins = 8192;  // dimensions up to this many levels count in LDS
+
+// Singletons: rows per level among the rows still kept.  A dimension with
+// few levels (years: every row hits one of a handful of counters) counts in
+// per-workgroup LDS bins first, then adds them to the global counters.
 __global__ void k_level_count(int64_t n, const int *__restrict__ code,
-                              const uint8_t *__restrict__ keep, int *__restrict__ cnt) {
+                              const uint8_t *__restrict__ keep, int *__restrict__ cnt, int nl) {
+    __shared__ int bins[kLdsBins];
+    const bool lds = nl <= kLdsBins;
+    if (lds) {
+        for (int t = threadIdx.x; t < nl; t += blockDim.x) bins[t] = 0;
+        __syncthreads();
+    }
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x)
-        if (keep[i]) atomicAdd(&cnt[code[i]], 1);
+        if (keep[i]) {
+            if (lds) atomicAdd(&bins[code[i]], 1);
+            else atomicAdd(&cnt[code[i]], 1);
+        }
+    if (lds) {
+        __syncthreads();
+        for (int t = threadIdx.x; t < nl; t += blockDim.x)
+            if (bins[t]) atomicAdd(&cnt[t], bins[t]);
+    }
 }
 
 __global__ void k_drop_single(int64_t n, const int *__restrict__ code,
@@ -313,21 +331,43 @@ __global__ void k_cells(int64_t C, int m, const int *__restrict__ cstart,
     }
 }
 
-// obs / cnt per level over all cells (cnt in integer atomics: exact).
-__global__ void k_obs_cnt(int64_t C, int D, const int *__restrict__ cstart,
-                          const int *__restrict__ codes, int64_t ld,
-                          const int64_t *__restrict__ offs, const double *__restrict__ cn,
-                          unsigned long long *__restrict__ cnt, double *__restrict__ obs,
-                          int weighted) {
+// obs / cnt per level of one dimension over all cells (cnt in integer
+// atomics: exact; obs, with weights, in float atomics).  Small dimensions
+// count in LDS bins first.
+__global__ void k_obs_cnt(int64_t C, const int *__restrict__ cstart,
+                          const int *__restrict__ code, int64_t off, int nl,
+                          const double *__restrict__ cn, unsigned long long *__restrict__ cnt,
+                          double *__restrict__ obs, int weighted) {
+    __shared__ unsigned bins[kLdsBins / 2];
+    __shared__ double wbins[kLdsBins / 2];
+    const bool lds = nl <= kLdsBins / 2;
+    if (lds) {
+        for (int t = threadIdx.x; t < nl; t += blockDim.x) {
+            bins[t] = 0;
+            wbins[t] = 0.0;
+        }
+        __syncthreads();
+    }
     for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < C;
          c += (int64_t)gridDim.x * blockDim.x) {
-        const int s = cstart[c];
-        const unsigned long long r = (unsigned long long)(cstart[c + 1] - s);
-        for (int d = 0; d < D; ++d) {
-            const int64_t a = offs[d] + codes[d * ld + s];
-            atomicAdd(&cnt[a], r);
-            if (weighted) atomicAdd(&obs[a], cn[c]);
+        const int s0 = cstart[c];
+        const unsigned r = (unsigned)(cstart[c + 1] - s0);
+        const int a = code[s0];
+        if (lds) {
+            atomicAdd(&bins[a], r);
+            if (weighted) atomicAdd(&wbins[a], cn[c]);
+        } else {
+            atomicAdd(&cnt[off + a], (unsigned long long)r);
+            if (weighted) atomicAdd(&obs[off + a], cn[c]);
         }
+    }
+    if (lds) {
+        __syncthreads();
+        for (int t = threadIdx.x; t < nl; t += blockDim.x)
+            if (bins[t]) {
+                atomicAdd(&cnt[off + t], (unsigned long long)bins[t]);
+                if (weighted) atomicAdd(&obs[off + t], wbins[t]);
+            }
     }
 }
 
@@ -452,10 +492,21 @@ __global__ void k_group_inc_start(int64_t E, const int *__restrict__ ig,
     if (blockIdx.x == 0 && threadIdx.x == 0) gis[Gi] = (int)E;
 }
 
-__global__ void k_count_levels(int64_t E, const int *__restrict__ il, int *__restrict__ cnt) {
-    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < E;
-         q += (int64_t)gridDim.x * blockDim.x)
-        atomicAdd(&cnt[il[q]], 1);
+// lptr[a] = first position of level a in the level-sorted keys (level in
+// the bits above gbits), by binary search; lptr[L] = E.
+__global__ void k_level_ptr(int64_t L, int64_t E, const uint64_t *__restrict__ key, int gbits,
+                            int *__restrict__ lptr) {
+    for (int64_t a = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; a <= L;
+         a += (int64_t)gridDim.x * blockDim.x) {
+        int64_t lo = 0, hi = E;
+        const uint64_t target = (uint64_t)a << gbits;
+        while (lo < hi) {
+            const int64_t mid = (lo + hi) >> 1;
+            if (key[mid] < target) lo = mid + 1;
+            else hi = mid;
+        }
+        lptr[a] = (int)lo;
+    }
 }
 
 __global__ void k_level_keys(int64_t E, const int *__restrict__ ig, const int *__restrict__ il,
@@ -469,37 +520,76 @@ __global__ void k_level_keys(int64_t E, const int *__restrict__ ig, const int *_
 
 // diag[a] = sum over groups containing a of (u - u^2 / N_g)  (kernels_base.py:86-110)
 // rhs[a, j] = sum of (s_j - u vbar_j)                          (kernels_base.py:64-83)
-// One wave per level: lanes stride over the level's incidences, then a
-// fixed butterfly, so a level in millions of groups (a year) is spread over
-// 64 lanes.
-__global__ void k_diag_rhs(int64_t L, int m, const int *__restrict__ lptr,
-                           const int *__restrict__ lq, const int *__restrict__ ig,
-                           const double *__restrict__ iu, const double *__restrict__ is,
-                           const double *__restrict__ Ng, const double *__restrict__ vbar,
-                           double *__restrict__ diag, double *__restrict__ rhs) {
+// Sums over a level's incidences in fixed chunks of kChunk incidences: one
+// wave per chunk (lanes strided, then a fixed butterfly) writes a partial; a
+// level in millions of groups (a year) is spread over many waves.  The
+// partials of a level are then added in chunk order.
+constexpr int kChunk = 2048;
+
+__global__ void k_diag_rhs_part(int64_t nchunk, int m, const int *__restrict__ chunk_lvl,
+                                const int *__restrict__ chunk_beg, const int *__restrict__ lptr,
+                                const int *__restrict__ lq, const int *__restrict__ ig,
+                                const double *__restrict__ iu, const double *__restrict__ is,
+                                const double *__restrict__ Ng, const double *__restrict__ vbar,
+                                double *__restrict__ part) {
     const int lane = threadIdx.x & 63;
     const int64_t waves = (int64_t)gridDim.x * (blockDim.x / 64);
-    for (int64_t a = blockIdx.x * (int64_t)(blockDim.x / 64) + threadIdx.x / 64; a < L;
-         a += waves) {
+    for (int64_t c = blockIdx.x * (int64_t)(blockDim.x / 64) + threadIdx.x / 64; c < nchunk;
+         c += waves) {
+        const int a = chunk_lvl[c];
+        const int b = chunk_beg[c], e = min(b + kChunk, lptr[a + 1]);
         double d = 0.0;
-        for (int t = lptr[a] + lane; t < lptr[a + 1]; t += 64) {
+        for (int t = b + lane; t < e; t += 64) {
             const int q = lq[t];
             const double u = iu[q];
             d += u - u * u / Ng[ig[q]];
         }
         d = wave_sum(d);
-        if (lane == 0) diag[a] = d;
+        if (lane == 0) part[c * (m + 1)] = d;
+        for (int j = 0; j < m; ++j) {
+            double r = 0.0;
+            for (int t = b + lane; t < e; t += 64) {
+                const int q = lq[t];
+                r += is[(int64_t)q * m + j] - iu[q] * vbar[(int64_t)ig[q] * m + j];
+            }
+            r = wave_sum(r);
+            if (lane == 0) part[c * (m + 1) + 1 + j] = r;
+        }
+    }
+}
+
+__global__ void k_diag_rhs_sum(int64_t L, int m, const int *__restrict__ cptr,
+                               const double *__restrict__ part, double *__restrict__ diag,
+                               double *__restrict__ rhs) {
+    for (int64_t a = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; a < L;
+         a += (int64_t)gridDim.x * blockDim.x) {
+        double d = 0.0;
+        for (int c = cptr[a]; c < cptr[a + 1]; ++c) d += part[(int64_t)c * (m + 1)];
+        diag[a] = d;
         if (rhs)
             for (int j = 0; j < m; ++j) {
                 double r = 0.0;
-                for (int t = lptr[a] + lane; t < lptr[a + 1]; t += 64) {
-                    const int q = lq[t];
-                    r += is[(int64_t)q * m + j] - iu[q] * vbar[(int64_t)ig[q] * m + j];
-                }
-                r = wave_sum(r);
-                if (lane == 0) rhs[a * m + j] = r;
+                for (int c = cptr[a]; c < cptr[a + 1]; ++c) r += part[(int64_t)c * (m + 1) + 1 + j];
+                rhs[a * m + j] = r;
             }
     }
+}
+
+// Chunks per level (at least one, so every level gets a partial).
+__global__ void k_level_nchunk(int64_t L, const int *__restrict__ lptr, int *__restrict__ nch) {
+    for (int64_t a = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; a <= L;
+         a += (int64_t)gridDim.x * blockDim.x)
+        nch[a] = a < L ? max(1, (lptr[a + 1] - lptr[a] + kChunk - 1) / kChunk) : 0;
+}
+
+__global__ void k_level_chunks(int64_t L, const int *__restrict__ lptr, const int *__restrict__ cptr,
+                               int *__restrict__ chunk_lvl, int *__restrict__ chunk_beg) {
+    for (int64_t a = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; a < L;
+         a += (int64_t)gridDim.x * blockDim.x)
+        for (int c = cptr[a], b = lptr[a]; c < cptr[a + 1]; ++c, b += kChunk) {
+            chunk_lvl[c] = (int)a;
+            chunk_beg[c] = b;
+        }
 }
 
 // Entries of S between sparse levels: per group, every ordered pair (a, b)
@@ -939,22 +1029,25 @@ __global__ void k_comp_level_keys(int64_t nl, const int *__restrict__ comp,
 // --------------------------------------------------- normalisation
 
 // Component means of Gamma's dimension-0 block, weighted by obs: levels
-// sorted by component (cptr over ncomp); thread per (component, column).
+// sorted by component (cptr over ncomp); one workgroup per component, its
+// threads strided over the levels, then a fixed tree.
 __global__ void k_comp_means(int64_t nc, int k, const int *__restrict__ cptr,
                              const int *__restrict__ clev, const double *__restrict__ obs,
                              const double *__restrict__ Gm, double *__restrict__ mean) {
-    const int64_t n = nc * k;
-    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n;
-         e += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t c = e / k;
-        const int j = (int)(e - c * k);
-        double sw = 0.0, s = 0.0;
-        for (int p = cptr[c]; p < cptr[c + 1]; ++p) {
-            const int a = clev[p];
-            sw += obs[a];
-            s += obs[a] * Gm[(int64_t)a * k + j];
+    __shared__ double lds[T256 / 64];
+    for (int64_t c = blockIdx.x; c < nc; c += gridDim.x) {
+        double sw = 0.0;
+        for (int p = cptr[c] + threadIdx.x; p < cptr[c + 1]; p += blockDim.x) sw += obs[clev[p]];
+        sw = block_sum<double>(sw, lds);
+        for (int j = 0; j < k; ++j) {
+            double s = 0.0;
+            for (int p = cptr[c] + threadIdx.x; p < cptr[c + 1]; p += blockDim.x) {
+                const int a = clev[p];
+                s += obs[a] * Gm[(int64_t)a * k + j];
+            }
+            s = block_sum<double>(s, lds);
+            if (threadIdx.x == 0) mean[c * k + j] = s / (sw == 0.0 ? 1.0 : sw);
         }
-        mean[e] = s / (sw == 0.0 ? 1.0 : sw);
     }
 }
 
@@ -1002,43 +1095,52 @@ __global__ void k_sub_global_mean(int64_t nl, int k, int64_t o, int nb,
 // --------------------------------------------------- assembly / pass 2
 
 // V' M_D V between cells (core/kernels_base.py:510-540), m <= 8: per group
-// q_c = sums_c / n_c - sum_d Gamma[level] less the group mean.
-__global__ void k_assemble(int64_t Gi, int D, int m, const int *__restrict__ igstart,
-                           const int *__restrict__ icodes, int64_t ild,
-                           const int64_t *__restrict__ offs, const double *__restrict__ in,
-                           const double *__restrict__ isums, const double *__restrict__ Gm,
-                           double *__restrict__ part) {
-    __shared__ double lds[T256];
-    double A[64];
-    for (int t = 0; t < 64; ++t) A[t] = 0.0;
+// q_c = sums_c / n_c - sum_d Gamma[level] less the group mean.  MM is m,
+// fixed at compile time so the m x m sums stay in registers.
+template <int MM>
+__global__ __launch_bounds__(T256) void k_assemble(
+    int64_t Gi, int D, const int *__restrict__ igstart, const int *__restrict__ icodes,
+    int64_t ild, const int64_t *__restrict__ offs, const double *__restrict__ in,
+    const double *__restrict__ isums, const double *__restrict__ Gm, double *__restrict__ part) {
+    __shared__ double lds[T256 / 64];
+    double A[MM * MM];
+#pragma unroll
+    for (int t = 0; t < MM * MM; ++t) A[t] = 0.0;
     const int64_t g0 = part_begin(Gi, blockIdx.x, gridDim.x), g1 = part_begin(Gi, blockIdx.x + 1, gridDim.x);
     for (int64_t g = g0 + threadIdx.x; g < g1; g += blockDim.x) {
         const int s = igstart[g], e = igstart[g + 1];
-        double qg[8], N = 0.0;
-        for (int j = 0; j < m; ++j) qg[j] = 0.0;
+        double qg[MM], N = 0.0;
+#pragma unroll
+        for (int j = 0; j < MM; ++j) qg[j] = 0.0;
         for (int c = s; c < e; ++c) {
             N += in[c];
-            for (int j = 0; j < m; ++j) {
-                double v = isums[(int64_t)c * m + j] / in[c];
-                for (int d = 0; d < D; ++d) v -= Gm[(offs[d] + icodes[d * ild + c]) * m + j];
+#pragma unroll
+            for (int j = 0; j < MM; ++j) {
+                double v = isums[(int64_t)c * MM + j] / in[c];
+                for (int d = 0; d < D; ++d) v -= Gm[(offs[d] + icodes[d * ild + c]) * MM + j];
                 qg[j] += in[c] * v;
             }
         }
-        for (int j = 0; j < m; ++j) qg[j] /= N;
+#pragma unroll
+        for (int j = 0; j < MM; ++j) qg[j] /= N;
         for (int c = s; c < e; ++c) {
-            double q[8];
-            for (int j = 0; j < m; ++j) {
-                double v = isums[(int64_t)c * m + j] / in[c];
-                for (int d = 0; d < D; ++d) v -= Gm[(offs[d] + icodes[d * ild + c]) * m + j];
+            double q[MM];
+#pragma unroll
+            for (int j = 0; j < MM; ++j) {
+                double v = isums[(int64_t)c * MM + j] / in[c];
+                for (int d = 0; d < D; ++d) v -= Gm[(offs[d] + icodes[d * ild + c]) * MM + j];
                 q[j] = v - qg[j];
             }
-            for (int a = 0; a < m; ++a)
-                for (int b = 0; b < m; ++b) A[a * m + b] += in[c] * q[a] * q[b];
+#pragma unroll
+            for (int a = 0; a < MM; ++a)
+#pragma unroll
+                for (int b = 0; b < MM; ++b) A[a * MM + b] += in[c] * q[a] * q[b];
         }
     }
-    for (int t = 0; t < m * m; ++t) {
+#pragma unroll
+    for (int t = 0; t < MM * MM; ++t) {
         const double v = block_sum<double>(A[t], lds);
-        if (threadIdx.x == 0) part[(int64_t)blockIdx.x * m * m + t] = v;
+        if (threadIdx.x == 0) part[(int64_t)blockIdx.x * MM * MM + t] = v;
     }
 }
 
@@ -1247,8 +1349,8 @@ int hdfe_panel_create(int64_t n_rows, int nfe, const int64_t *const *fe, int m,
             const int d = (int)(i % nfe);
             HIP_TRY(hipMemsetAsync(cnt.p, 0, nl[d] * 4, s));
             HIP_TRY(hipMemsetAsync(dropped.p, 0, 8, s));
-            hipLaunchKernelGGL(k_level_count, dim3(grid1(n)), dim3(T256), 0, s, n, code[d].p,
-                               keep.p, cnt.p);
+            hipLaunchKernelGGL(k_level_count, dim3(grid1(n, T256, 2048)), dim3(T256), 0, s, n,
+                               code[d].p, keep.p, cnt.p, (int)nl[d]);
             hipLaunchKernelGGL(k_drop_single, dim3(grid1(n)), dim3(T256), 0, s, n, code[d].p,
                                cnt.p, keep.p, dropped.p);
             unsigned long long dr = 0;
@@ -1453,9 +1555,10 @@ int hdfe_panel_create(int64_t n_rows, int nfe, const int64_t *const *fe, int m,
         DEV_ALLOC(ic, std::max<int64_t>(L, 1));
         HIP_TRY(hipMemsetAsync(ic.p, 0, std::max<int64_t>(L, 1) * 8, s));
         HIP_TRY(hipMemsetAsync(P->obs.p, 0, std::max<int64_t>(L, 1) * 8, s));
-        if (D)
-            hipLaunchKernelGGL(k_obs_cnt, dim3(grid1(nc)), dim3(T256), 0, s, (int64_t)nc, D,
-                               P->cstart.p, P->codes.p, nk, P->offs.p, cn.p, ic.p, P->obs.p,
+        for (int t = 0; t < D; ++t)
+            hipLaunchKernelGGL(k_obs_cnt, dim3(grid1(nc, T256, 2048)), dim3(T256), 0, s,
+                               (int64_t)nc, P->cstart.p, P->codes.p + (int64_t)t * nk,
+                               P->offs_h[t], (int)P->nlev[t], cn.p, ic.p, P->obs.p,
                                P->weighted);
         hipLaunchKernelGGL(k_u64_to_f64, dim3(grid1(L)), dim3(T256), 0, s, L, ic.p, P->cnt.p,
                            P->weighted ? (double *)nullptr : P->obs.p);
@@ -1612,23 +1715,39 @@ int hdfe_panel_system(void *h, int with_rhs, void **matrix, double *seconds) {
     hipLaunchKernelGGL(k_group_inc_start, dim3(grid1(E)), dim3(T256), 0, s, (int64_t)E, P.ig.p,
                        P.gis.p, Gi);
     // level CSR of the incidences (sorted by level, then group)
-    DevBuf<int> lcnt;
-    DEV_ALLOC(lcnt, L + 1);
     DEV_ALLOC(P.lptr, L + 1);
     DEV_ALLOC(P.lq, std::max(E, 1));
-    HIP_TRY(hipMemsetAsync(lcnt.p, 0, (L + 1) * 4, s));
-    hipLaunchKernelGGL(k_count_levels, dim3(grid1(E)), dim3(T256), 0, s, (int64_t)E, P.il.p, lcnt.p);
-    HIP_TRY(exclusive_sum(P.tmp, lcnt.p, P.lptr.p, L + 1, s));
     if (E) {
         hipLaunchKernelGGL(k_level_keys, dim3(grid1(E)), dim3(T256), 0, s, (int64_t)E, P.ig.p,
                            P.il.p, gbits, k1.p, v1.p);
         HIP_TRY(sort_pairs(P.tmp, k1.p, k2.p, v1.p, P.lq.p, E, 0, lbits + gbits, s));
     }
+    hipLaunchKernelGGL(k_level_ptr, dim3(grid1(L + 1)), dim3(T256), 0, s, L, (int64_t)E, k2.p,
+                       gbits, P.lptr.p);
     DEV_ALLOC(P.diag, L);
     if (with_rhs) DEV_ALLOC(P.rhs, L * m);
-    hipLaunchKernelGGL(k_diag_rhs, dim3(grid1(L * 64)), dim3(T256), 0, s, L, m, P.lptr.p, P.lq.p, P.ig.p,
-                       P.iu.p, P.is.p, P.Ng.p, P.vbar.p, P.diag.p,
-                       with_rhs ? P.rhs.p : (double *)nullptr);
+    {
+        DevBuf<int> nch, cptr, clvl, cbeg;
+        DEV_ALLOC(nch, L + 1);
+        DEV_ALLOC(cptr, L + 1);
+        hipLaunchKernelGGL(k_level_nchunk, dim3(grid1(L + 1)), dim3(T256), 0, s, L, P.lptr.p, nch.p);
+        HIP_TRY(exclusive_sum(P.tmp, nch.p, cptr.p, L + 1, s));
+        int nchunk = 0;
+        HIP_TRY(scalar_to_host(&nchunk, cptr.p + L, s));
+        DEV_ALLOC(clvl, nchunk);
+        DEV_ALLOC(cbeg, nchunk);
+        DevBuf<double> part;
+        DEV_ALLOC(part, (int64_t)nchunk * (m + 1));
+        hipLaunchKernelGGL(k_level_chunks, dim3(grid1(L)), dim3(T256), 0, s, L, P.lptr.p, cptr.p,
+                           clvl.p, cbeg.p);
+        hipLaunchKernelGGL(k_diag_rhs_part, dim3(grid1((int64_t)nchunk * 64)), dim3(T256), 0, s,
+                           (int64_t)nchunk, m, clvl.p, cbeg.p, P.lptr.p, P.lq.p, P.ig.p, P.iu.p,
+                           P.is.p, P.Ng.p, P.vbar.p, part.p);
+        hipLaunchKernelGGL(k_diag_rhs_sum, dim3(grid1(L)), dim3(T256), 0, s, L, m, cptr.p, part.p,
+                           P.diag.p, with_rhs ? P.rhs.p : (double *)nullptr);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipStreamSynchronize(s));
+    }
     HIP_TRY(hipGetLastError());
 
     // dense / sparse dimensions and their levels (ascending global index)
@@ -1925,7 +2044,8 @@ int hdfe_panel_normalize(void *h, double *Gm, int k, void *stream) {
     const int64_t nl0 = P.nlev[0];
     DevBuf<double> mean;
     DEV_ALLOC(mean, P.ncomp * k);
-    hipLaunchKernelGGL(k_comp_means, dim3(grid1(P.ncomp * k)), dim3(T256), 0, s, P.ncomp, k,
+    hipLaunchKernelGGL(k_comp_means, dim3((int)std::min<int64_t>(P.ncomp, 65536)), dim3(T256), 0, s,
+                       P.ncomp, k,
                        P.cptr.p, P.clev.p, P.obs.p, Gm, mean.p);
     hipLaunchKernelGGL(k_sub_comp_means, dim3(grid1(nl0 * k)), dim3(T256), 0, s, nl0, k, P.comp.p,
                        mean.p, Gm);
@@ -1952,8 +2072,17 @@ int hdfe_panel_assemble(void *h, const double *Gm, double *A) {
     const int m = P.m, nb = 512;
     DevBuf<double> part;
     DEV_ALLOC(part, (int64_t)nb * m * m);
-    hipLaunchKernelGGL(k_assemble, dim3(nb), dim3(T256), 0, s, P.Gi, P.D, m, P.igstart.p,
-                       P.icodes.p, P.M, P.offs.p, P.in.p, P.isums.p, Gm, part.p);
+    switch (m) {
+#define HDFE_ASM(MM)                                                                            \
+    case MM:                                                                                    \
+        hipLaunchKernelGGL(k_assemble<MM>, dim3(nb), dim3(T256), 0, s, P.Gi, P.D, P.igstart.p, \
+                           P.icodes.p, P.M, P.offs.p, P.in.p, P.isums.p, Gm, part.p);          \
+        break;
+        HDFE_ASM(1) HDFE_ASM(2) HDFE_ASM(3) HDFE_ASM(4) HDFE_ASM(5) HDFE_ASM(6) HDFE_ASM(7)
+        HDFE_ASM(8)
+#undef HDFE_ASM
+    }
+    HIP_TRY(hipGetLastError());
     std::vector<double> hp, Wh;
     HIP_TRY(to_host(hp, part.p, (int64_t)nb * m * m, s));
     HIP_TRY(to_host(Wh, P.W.p, m * m, s));

@@ -84,43 +84,6 @@ __device__ void reduce_parts(const double *part, int nw, int k, int stride, doub
     }
 }
 
-// Split rows: partial sums of chunk c of long row lrow[c], then the last
-// workgroup adds each row's chunks.  Small k: threads over nonzeros.
-template <typename T, int K>
-__global__ __launch_bounds__(kThreads) void k_long_small(
-    const int *__restrict__ indptr, const int *__restrict__ indices, const T *__restrict__ val,
-    const int *__restrict__ chunk_row, const int *__restrict__ chunk_beg, int nchunk,
-    const int *__restrict__ long_rows, const int *__restrict__ long_cptr, int nlong,
-    const T *__restrict__ P, T *__restrict__ AP, double *__restrict__ lpart, CgState st) {
-    if (st.done && *st.done) return;
-    __shared__ double lds[kThreads / kWave];
-    const int c = blockIdx.x;
-    const int row = chunk_row[c];
-    const int s = chunk_beg[c], e = min(s + kLongChunk, indptr[row + 1]);
-    T acc[K];
-#pragma unroll
-    for (int j = 0; j < K; ++j) acc[j] = T(0);
-    for (int p = s + threadIdx.x; p < e; p += blockDim.x) {
-        const int col = indices[p];
-        const T v = val[p];
-#pragma unroll
-        for (int j = 0; j < K; ++j) acc[j] += v * P[(int64_t)col * K + j];
-    }
-#pragma unroll
-    for (int j = 0; j < K; ++j) {
-        double t = block_sum<double>((double)acc[j], lds);
-        if (threadIdx.x == 0) lpart[(int64_t)c * K + j] = t;
-    }
-    if (!last_block(st.counter + 3)) return;
-    for (int t = threadIdx.x; t < nlong * K; t += blockDim.x) {
-        const int l = t / K, j = t % K;
-        double s2 = 0.0;
-        for (int q = long_cptr[l]; q < long_cptr[l + 1]; ++q) s2 += lpart[(int64_t)q * K + j];
-        AP[(int64_t)long_rows[l] * K + j] = (T)s2;
-    }
-    if (threadIdx.x == 0) st.counter[3] = 0;
-}
-
 // Split rows, wide k: waves over nonzeros, lanes over columns.
 template <typename T>
 __global__ __launch_bounds__(kThreads) void k_long_wide(
@@ -168,66 +131,137 @@ __device__ void finish_alpha(const double *part, int nw, CgState st) {
     if (threadIdx.x == 0) st.counter[0] = 0;
 }
 
-// AP = S P (rows up to kLongThresh nonzeros; split rows come from the long
-// kernel) with part[w, j] = sum over the workgroup's rows of P AP.
-// Small k: G lanes per row, each over a stride of the row's nonzeros.
+// AP = S P with, when with_dot, part[w, j] = sum over the workgroup's rows of
+// P AP and alpha formed by the last workgroup.  One launch: workgroups
+// [0, nchunk) take the chunks of the split rows (rows over kLongThresh
+// nonzeros: the levels of small dimensions), the others ranges of ordinary
+// rows with G lanes per row.  The last workgroup to finish adds each split
+// row's chunk partials in a fixed order, adds their P AP, and forms alpha.
 template <typename T, int K, int G>
 __global__ __launch_bounds__(kThreads) void k_spmm_small(
     const int *__restrict__ indptr, const int *__restrict__ indices, const T *__restrict__ val,
     int64_t L, const T *__restrict__ P, T *__restrict__ AP, double *__restrict__ part,
-    int with_dot, CgState st) {
+    int with_dot, const int *__restrict__ chunk_row, const int *__restrict__ chunk_beg,
+    int nchunk, const int *__restrict__ long_rows, const int *__restrict__ long_cptr, int nlong,
+    double *__restrict__ lpart, CgState st) {
     if (st.done && *st.done) return;
     __shared__ double lds[kThreads / kWave];
-    const int w = blockIdx.x, nw = gridDim.x;
-    const int64_t r0 = part_begin(L, w, nw), r1 = part_begin(L, w + 1, nw);
-    const int sub = threadIdx.x % G, slot = threadIdx.x / G, nslot = blockDim.x / G;
+    const int nw = gridDim.x - nchunk;
     double dot[K];
 #pragma unroll
     for (int j = 0; j < K; ++j) dot[j] = 0.0;
-    for (int64_t base = r0; base < r1; base += nslot) {
-        const int64_t row = base + slot;
-        const bool ok = row < r1;
-        int s = 0, e = 0;
-        if (ok) {
-            s = indptr[row];
-            e = indptr[row + 1];
-        }
-        const bool is_long = (e - s) > kLongThresh;
+    if ((int)blockIdx.x < nchunk) {
+        const int c = blockIdx.x;
+        const int row = chunk_row[c];
+        const int s = chunk_beg[c], e = min(s + kLongChunk, indptr[row + 1]);
         T acc[K];
 #pragma unroll
         for (int j = 0; j < K; ++j) acc[j] = T(0);
-        if (ok && !is_long) {
-            for (int p = s + sub; p < e; p += G) {
-                const int col = indices[p];
-                const T v = val[p];
+        for (int p = s + threadIdx.x; p < e; p += blockDim.x) {
+            const int col = indices[p];
+            const T v = val[p];
 #pragma unroll
-                for (int j = 0; j < K; ++j) acc[j] += v * P[(int64_t)col * K + j];
-            }
+            for (int j = 0; j < K; ++j) acc[j] += v * P[(int64_t)col * K + j];
         }
 #pragma unroll
-        for (int j = 0; j < K; ++j) acc[j] = group_sum<G>(acc[j]);
-        if (ok && sub == 0) {
+        for (int j = 0; j < K; ++j) {
+            double t = block_sum<double>((double)acc[j], lds);
+            if (threadIdx.x == 0) lpart[(int64_t)c * K + j] = t;
+        }
+    } else {
+        const int w = blockIdx.x - nchunk;
+        const int64_t r0 = part_begin(L, w, nw), r1 = part_begin(L, w + 1, nw);
+        const int sub = threadIdx.x % G, slot = threadIdx.x / G, nslot = blockDim.x / G;
+        for (int64_t base = r0; base < r1; base += nslot) {
+            const int64_t row = base + slot;
+            const bool ok = row < r1;
+            int s = 0, e = 0;
+            if (ok) {
+                s = indptr[row];
+                e = indptr[row + 1];
+            }
+            const bool is_long = (e - s) > kLongThresh;
+            T acc[K];
+#pragma unroll
+            for (int j = 0; j < K; ++j) acc[j] = T(0);
+            if (ok && !is_long) {
+                // four nonzeros per lane in flight: their column indices
+                // first, then all their P gathers
+                int p = s + sub;
+                for (; p + 3 * G < e; p += 4 * G) {
+                    const int c0 = indices[p], c1 = indices[p + G];
+                    const int c2 = indices[p + 2 * G], c3 = indices[p + 3 * G];
+                    const T v0 = val[p], v1 = val[p + G], v2 = val[p + 2 * G], v3 = val[p + 3 * G];
+#pragma unroll
+                    for (int j = 0; j < K; ++j)
+                        acc[j] += v0 * P[(int64_t)c0 * K + j] + v1 * P[(int64_t)c1 * K + j] +
+                                  v2 * P[(int64_t)c2 * K + j] + v3 * P[(int64_t)c3 * K + j];
+                }
+                for (; p < e; p += G) {
+                    const int col = indices[p];
+                    const T v = val[p];
+#pragma unroll
+                    for (int j = 0; j < K; ++j) acc[j] += v * P[(int64_t)col * K + j];
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < K; ++j) acc[j] = group_sum<G>(acc[j]);
+            if (ok && sub == 0 && !is_long) {
+#pragma unroll
+                for (int j = 0; j < K; ++j) {
+                    AP[row * K + j] = acc[j];
+                    dot[j] += (double)P[row * K + j] * (double)acc[j];
+                }
+            }
+        }
+        if (with_dot) {
 #pragma unroll
             for (int j = 0; j < K; ++j) {
-                T ap = acc[j];
-                if (is_long) ap = AP[row * K + j];
-                else AP[row * K + j] = ap;
-                dot[j] += (double)P[row * K + j] * (double)ap;
+                double t = block_sum<double>(dot[j], lds);
+                if (threadIdx.x == 0) part[(int64_t)w * K + j] = t;
             }
         }
     }
-    if (!with_dot) return;
-#pragma unroll
-    for (int j = 0; j < K; ++j) {
-        double t = block_sum<double>(dot[j], lds);
-        if (threadIdx.x == 0) part[(int64_t)w * K + j] = t;
-    }
+    if (!with_dot && nchunk == 0) return;
     if (!last_block(st.counter + 0)) return;
-    finish_alpha(part, nw, st);
+    // split rows: every thread takes a stride of each row's chunk partials,
+    // then a fixed tree; then their P AP
+    __shared__ double ldot[K];
+    if (threadIdx.x < K) ldot[threadIdx.x] = 0.0;
+    for (int l = 0; l < nlong; ++l) {
+        const int64_t row = long_rows[l];
+#pragma unroll
+        for (int j = 0; j < K; ++j) {
+            double s2 = 0.0;
+            for (int q = long_cptr[l] + threadIdx.x; q < long_cptr[l + 1]; q += blockDim.x)
+                s2 += lpart[(int64_t)q * K + j];
+            s2 = block_sum<double>(s2, lds);
+            if (threadIdx.x == 0) {
+                const T ap = (T)s2;
+                AP[row * K + j] = ap;
+                ldot[j] += (double)P[row * K + j] * (double)ap;
+            }
+        }
+    }
+    __syncthreads();
+    if (with_dot) {
+#pragma unroll
+        for (int j = 0; j < K; ++j) {
+            double pAp = 0.0;
+            for (int w = threadIdx.x; w < nw; w += blockDim.x) pAp += part[(int64_t)w * K + j];
+            pAp = block_sum<double>(pAp, lds) + ldot[j];
+            if (threadIdx.x == 0)
+                st.alpha[j] = (st.active[j] && pAp > 0.0) ? st.rz[j] / pAp : 0.0;
+        }
+    }
+    if (threadIdx.x == 0) st.counter[0] = 0;
 }
 
-// Wide k: one wave per row, lanes over columns (j = lane + 64 t, t < 16).
-template <typename T>
+// Wide k: one wave per row, lanes over columns (j = lane + 64 t, t < CPL).
+// Each nonzero's column index and value are loaded once (wave-uniform) and
+// feed CPL coalesced gathers of the neighbour's row of P; two nonzeros are in
+// flight at a time.
+template <typename T, int CPL>
 __global__ __launch_bounds__(kThreads) void k_spmm_wide(
     const int *__restrict__ indptr, const int *__restrict__ indices, const T *__restrict__ val,
     int64_t L, const T *__restrict__ P, T *__restrict__ AP, double *__restrict__ part,
@@ -238,24 +272,44 @@ __global__ __launch_bounds__(kThreads) void k_spmm_wide(
     const int w = blockIdx.x, nw = gridDim.x;
     const int64_t r0 = part_begin(L, w, nw), r1 = part_begin(L, w + 1, nw);
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    constexpr int CPL = kMaxK / 64;
     double dot[CPL];
 #pragma unroll
     for (int t = 0; t < CPL; ++t) dot[t] = 0.0;
     for (int64_t row = r0 + wv; row < r1; row += 4) {
-        const int s = indptr[row], e = indptr[row + 1];
+        const int s = __builtin_amdgcn_readfirstlane(indptr[row]);
+        const int e = __builtin_amdgcn_readfirstlane(indptr[row + 1]);
         const bool is_long = (e - s) > kLongThresh;
+        T acc[CPL];
+#pragma unroll
+        for (int t = 0; t < CPL; ++t) acc[t] = T(0);
+        if (!is_long) {
+            int p = s;
+            for (; p + 1 < e; p += 2) {
+                const int c0 = indices[p], c1 = indices[p + 1];
+                const T v0 = val[p], v1 = val[p + 1];
+                const T *p0 = P + (int64_t)c0 * k + lane;
+                const T *p1 = P + (int64_t)c1 * k + lane;
+#pragma unroll
+                for (int t = 0; t < CPL; ++t)
+                    if (lane + 64 * t < k) acc[t] += v0 * p0[64 * t] + v1 * p1[64 * t];
+            }
+            if (p < e) {
+                const int c0 = indices[p];
+                const T v0 = val[p];
+                const T *p0 = P + (int64_t)c0 * k + lane;
+#pragma unroll
+                for (int t = 0; t < CPL; ++t)
+                    if (lane + 64 * t < k) acc[t] += v0 * p0[64 * t];
+            }
+        }
 #pragma unroll
         for (int t = 0; t < CPL; ++t) {
             const int j = lane + 64 * t;
-            if (64 * t >= k) break;
-            T acc = T(0);
-            if (!is_long && j < k)
-                for (int p = s; p < e; ++p) acc += val[p] * P[(int64_t)indices[p] * k + j];
             if (j < k) {
-                if (is_long) acc = AP[row * k + j];
-                else AP[row * k + j] = acc;
-                dot[t] += (double)P[row * k + j] * (double)acc;
+                T a = acc[t];
+                if (is_long) a = AP[row * k + j];
+                else AP[row * k + j] = a;
+                dot[t] += (double)P[row * k + j] * (double)a;
             }
         }
     }
@@ -263,7 +317,6 @@ __global__ __launch_bounds__(kThreads) void k_spmm_wide(
 #pragma unroll
     for (int t = 0; t < CPL; ++t) {
         const int j = lane + 64 * t;
-        if (64 * t >= k) break;
         if (j < k) wlds[wv * k + j] = dot[t];
     }
     __syncthreads();
@@ -620,41 +673,36 @@ template <typename T>
 static void launch_spmm(Matrix &M, const T *val, const T *P, T *AP, double *part, double *lpart,
                         int with_dot, CgState st, hipStream_t s) {
     const int k = st.k;
-    if (M.nlong) {
-        if (k <= kSmallK) {
-#define HDFE_LONG(KK)                                                                         \
-    case KK:                                                                                  \
-        hipLaunchKernelGGL((k_long_small<T, KK>), dim3(M.nchunk), dim3(kThreads), 0, s,      \
-                           M.indptr, M.indices, val, M.chunk_row, M.chunk_beg, M.nchunk,      \
-                           M.long_rows, M.long_cptr, M.nlong, P, AP, lpart, st);             \
-        break;
-            switch (k) {
-                HDFE_LONG(1) HDFE_LONG(2) HDFE_LONG(3) HDFE_LONG(4)
-                HDFE_LONG(5) HDFE_LONG(6) HDFE_LONG(7) HDFE_LONG(8)
-            }
-#undef HDFE_LONG
-        } else {
-            hipLaunchKernelGGL((k_long_wide<T>), dim3(M.nchunk), dim3(kThreads), 4 * k * 8, s,
-                               M.indptr, M.indices, val, M.chunk_row, M.chunk_beg, M.nchunk,
-                               M.long_rows, M.long_cptr, M.nlong, P, AP, lpart, st);
-        }
-    }
     const int g = spmm_grid(M.L, k);
     if (k <= kSmallK) {
-#define HDFE_SPMM(KK)                                                                        \
-    case KK:                                                                                 \
-        hipLaunchKernelGGL((k_spmm_small<T, KK, 8>), dim3(g), dim3(kThreads), 0, s,         \
-                           M.indptr, M.indices, val, M.L, P, AP, part, with_dot, st);       \
+#define HDFE_SPMM(KK)                                                                          \
+    case KK:                                                                                   \
+        hipLaunchKernelGGL((k_spmm_small<T, KK, 8>), dim3(g + M.nchunk), dim3(kThreads), 0, s, \
+                           M.indptr, M.indices, val, M.L, P, AP, part, with_dot, M.chunk_row,  \
+                           M.chunk_beg, M.nchunk, M.long_rows, M.long_cptr, M.nlong, lpart,    \
+                           st);                                                                \
         break;
         switch (k) {
             HDFE_SPMM(1) HDFE_SPMM(2) HDFE_SPMM(3) HDFE_SPMM(4)
             HDFE_SPMM(5) HDFE_SPMM(6) HDFE_SPMM(7) HDFE_SPMM(8)
         }
 #undef HDFE_SPMM
-    } else {
-        hipLaunchKernelGGL((k_spmm_wide<T>), dim3(g), dim3(kThreads), 4 * k * 8, s, M.indptr,
-                           M.indices, val, M.L, P, AP, part, with_dot, st);
+        return;
     }
+    if (M.nlong)
+        hipLaunchKernelGGL((k_long_wide<T>), dim3(M.nchunk), dim3(kThreads), 4 * k * 8, s,
+                           M.indptr, M.indices, val, M.chunk_row, M.chunk_beg, M.nchunk,
+                           M.long_rows, M.long_cptr, M.nlong, P, AP, lpart, st);
+    const int cpl = (k + 63) / 64;
+#define HDFE_WIDE(C)                                                                        \
+    hipLaunchKernelGGL((k_spmm_wide<T, C>), dim3(g), dim3(kThreads), 4 * k * 8, s, M.indptr, \
+                       M.indices, val, M.L, P, AP, part, with_dot, st)
+    if (cpl <= 1) HDFE_WIDE(1);
+    else if (cpl <= 2) HDFE_WIDE(2);
+    else if (cpl <= 4) HDFE_WIDE(4);
+    else if (cpl <= 8) HDFE_WIDE(8);
+    else HDFE_WIDE(16);
+#undef HDFE_WIDE
 }
 
 template <typename T>
@@ -839,6 +887,7 @@ static int solve_mixed(Matrix &M, int k, const uint8_t *zero_h, bool has_x0, dou
     plain.done = nullptr;
     int64_t total = 0;
     bool first = true;
+    double prev = 1e300;
     for (int outer = 0; outer < 64; ++outer) {
         // the true residual, in FP64
         if (first && !has_x0) {
@@ -871,6 +920,13 @@ static int solve_mixed(Matrix &M, int k, const uint8_t *zero_h, bool has_x0, dou
             *conv_out = 0;
             break;
         }
+        // the true residual has reached its rounding floor: another step
+        // would not lower it by half
+        if (outer >= 2 && rmax > 0.5 * prev) {
+            *conv_out = 0;
+            break;
+        }
+        prev = rmax;
         *outer_out = outer + 1;
         // the FP32 inner solve of S d = r / ||r||
         CgState st = make_state(W, k, inner_tol, (int)std::min<int64_t>(maxiter - total, INT32_MAX));
@@ -1093,6 +1149,48 @@ int hdfe_csr_timing_read(void *h, double *ms, int64_t *launches) {
     Matrix &M = *(Matrix *)h;
     if (ms) *ms = M.spmm_ms;
     if (launches) *launches = M.spmm_n;
+    return HDFE_STATUS_OK;
+}
+
+// Average duration (microseconds) of the CG's SpMM kernel on this matrix:
+// `reps` back-to-back launches of the same launch configuration as in the
+// solve (k right-hand sides, with the fused pAp sums), between two HIP events
+// on the solver's stream.  precision 0: FP64, 1: FP32.
+int hdfe_csr_spmm_time(void *h, int64_t k, int reps, int precision, double *avg_us) {
+    ARG_CHECK(h != nullptr && avg_us != nullptr, "hdfe_csr_spmm_time: null argument");
+    ARG_CHECK(k >= 1 && k <= kMaxK && reps >= 1, "hdfe_csr_spmm_time: bad k or reps");
+    Matrix &M = *(Matrix *)h;
+    std::lock_guard<std::mutex> lock(M.mu);
+    HIP_TRY(hipSetDevice(M.device));
+    int st = alloc_work(M, (int)k);
+    if (st) return st;
+    hipStream_t s = M.stream;
+    const int64_t Lk = M.L * k;
+    CgState cs = make_state(M.w, (int)k, 0.0, 1);
+    HIP_TRY(hipMemsetAsync(cs.done, 0, 3 * sizeof(int), s));
+    HIP_TRY(hipMemsetAsync(cs.active, 0, k * 4, s));
+    HIP_TRY(hipMemsetAsync(M.w.P64, 0, Lk * 8, s));
+    HIP_TRY(hipMemsetAsync(M.w.P32, 0, Lk * 4, s));
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    for (int r = 0; r < 2; ++r) {  // warm
+        if (precision) launch_spmm<float>(M, M.val32, M.w.P32, M.w.AP32, M.w.part, M.w.lpart, 1, cs, s);
+        else launch_spmm<double>(M, M.val64, M.w.P64, M.w.AP64, M.w.part, M.w.lpart, 1, cs, s);
+    }
+    HIP_TRY(hipEventRecord(e0, s));
+    for (int r = 0; r < reps; ++r) {
+        if (precision) launch_spmm<float>(M, M.val32, M.w.P32, M.w.AP32, M.w.part, M.w.lpart, 1, cs, s);
+        else launch_spmm<double>(M, M.val64, M.w.P64, M.w.AP64, M.w.part, M.w.lpart, 1, cs, s);
+    }
+    HIP_TRY(hipEventRecord(e1, s));
+    HIP_TRY(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    HIP_TRY(hipEventDestroy(e0));
+    HIP_TRY(hipEventDestroy(e1));
+    HIP_TRY(hipGetLastError());
+    *avg_us = 1e3 * ms / reps;
     return HDFE_STATUS_OK;
 }
 

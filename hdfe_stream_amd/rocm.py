@@ -157,6 +157,14 @@ class DeviceCSR:
         """HIP-event timing of the SpMM launches of later solves (resets)."""
         _lib.check(_lib.load().hdfe_csr_timing(self._h, int(enable)), "hdfe_csr_timing")
 
+    def spmm_time(self, k: int, reps: int = 50, precision: str = "fp64") -> float:
+        """Average microseconds of the solve's SpMM kernel (k columns), from
+        HIP events around `reps` back-to-back launches on the solver's stream."""
+        us = C.c_double()
+        _lib.check(_lib.load().hdfe_csr_spmm_time(self._h, k, reps, PRECISIONS[precision],
+                                                  C.byref(us)), "hdfe_csr_spmm_time")
+        return float(us.value)
+
     def timing_read(self) -> dict:
         ms = C.c_double()
         n = C.c_int64()

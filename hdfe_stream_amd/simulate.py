@@ -16,7 +16,12 @@ from __future__ import annotations
 
 import numpy as np
 
-__all__ = ["akm_panel", "weighted_panel", "poisson_models"]
+__all__ = ["akm_panel", "kss_panel", "weighted_panel", "poisson_models"]
+
+# Row-level corr(alpha, psi) is about 0.6 times the copula parameter with
+# p_move 0.15 and 8 years (movers' later firms are uniform), so a copula of
+# 1/3 gives a row-level correlation of about 0.20.
+KSS_COPULA = 1.0 / 3.0
 
 
 def _first_firm(rng, alpha, psi, corr):
@@ -64,6 +69,14 @@ def akm_panel(n_workers: int, n_firms: int, n_years: int = 8, p_move: float = 0.
     y = a + p + g + 0.3 * x1 - 0.2 * x2 + eps
     return {"worker_id": worker, "firm_id": firm, "year": year, "x1": x1, "x2": x2, "y": y,
             "alpha": a, "psi": p, "gamma": g}
+
+
+def kss_panel(n_workers: int = 1_000_000, n_firms: int = 100_000, seed: int = 3,
+              n_years: int = 8) -> dict:
+    """The leave-out panel: `akm_panel` with p_move 0.15 and sorting, a
+    row-level corr(alpha, psi) of about 0.2."""
+    return akm_panel(n_workers, n_firms, n_years=n_years, p_move=0.15, seed=seed,
+                     copula=KSS_COPULA)
 
 
 def weighted_panel(n_workers: int, n_firms: int, seed: int = 4, n_years: int = 8,

@@ -83,6 +83,12 @@ int hdfe_csr_export(void *h, int64_t *indptr, int32_t *indices, double *data, do
 int hdfe_csr_timing(void *h, int enable);
 int hdfe_csr_timing_read(void *h, double *ms, int64_t *launches);
 
+/* Average duration in microseconds of the solve's SpMM kernel on this
+ * matrix (k right-hand sides, fused pAp sums): reps back-to-back launches
+ * between two HIP events on the solver's stream; precision 0 FP64, 1 FP32.
+ * No counterpart in the reference (measurement only). */
+int hdfe_csr_spmm_time(void *h, int64_t k, int reps, int precision, double *avg_us);
+
 /* Column norms sqrt(sum_i X_ij^2) of a device (L, k) block, to the host,
  * in a fixed order: the zero-column test of core/solve.py:354-355. */
 int hdfe_col_norms(const double *X, int64_t L, int64_t k, double *out);
@@ -148,6 +154,41 @@ int hdfe_panel_pass2(void *h, const double *Gamma, const double *beta, const int
 
 /* Free the panel, its device arrays and its matrix. */
 void hdfe_panel_destroy(void *h);
+
+/* ------------------------------------------------ leave-out row passes */
+
+/* Rademacher +/-1 draws (n, k) on the device: entry (i, j) from the
+ * counter-based splitmix64 hash of (seed, ordinal + i, first_draw + j)
+ * (core/kernels_groups.py:52-77, _nb_rademacher). */
+int hdfe_rademacher(int64_t ordinal, int64_t n, int k, uint64_t seed, int64_t first_draw,
+                    double *out);
+
+/* Row-pass state for a panel's sorted rows: the level-to-rows CSR of each of
+ * the D dimensions (codes (D, n) device; offs, n_levels host).  The reference
+ * iterates row files in whole groups instead (inverse.py:141-160). */
+int hdfe_rows_create(int64_t n, int64_t G, int D, const int *codes, const int64_t *offs_h,
+                     const int64_t *nlev_h, void **out);
+
+/* The reduced right-hand side of a block of k Rademacher vectors over the
+ * rows: rhs (L, k) = D_o' W M_0 R, the draws keyed on the sorted row ordinal,
+ * in the square-root weight metric when sqrtw (inverse.py:162-180,
+ * kernels_inverse.py:34-71, leaveout.py:179). */
+int hdfe_rows_center(void *h, const int *g, const int *gstart, const double *w, int sqrtw,
+                     uint64_t seed, int64_t first_draw, int k, double *rhs);
+
+/* The projection back to rows (A S^- A' r)_i from the reduced solution U
+ * (L, k), fused with the JLA sink: moments (5, n) += the block's sums of P,
+ * M, P^2, M^2, P M per row (kernels_inverse.py:74-108, leaveout.py:181-190). */
+int hdfe_rows_project_moments(void *h, const int *gstart, const int *codes, const double *w,
+                              int sqrtw, uint64_t seed, int64_t first_draw, int k,
+                              const int64_t *offs_d, const double *U, double *moments);
+
+/* Leverages from the moments, in place: rows 0, 1, 2 become P_bar, M_bar and
+ * the non-linearity factor (leaveout.py:209-282); stats (host, 4) = min, max,
+ * mean of P_bar, largest |P + M - 1|. */
+int hdfe_jla_combine(int64_t n, int64_t n_draws, double *moments, double *stats);
+
+void hdfe_rows_destroy(void *h);
 
 #ifdef __cplusplus
 }
