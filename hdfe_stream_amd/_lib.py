@@ -57,6 +57,7 @@ SIGNATURES = [
     ("hdfe_rows_project_moments", I32, [P, P, P, P, I32, C.c_uint64, I64, I32, P, P, P]),
     ("hdfe_jla_combine", I32, [I64, I64, P, P]),
     ("hdfe_rows_destroy", None, [P]),
+    ("hdfe_loo_prune", I32, [I64, P, P, P, P]),
 ]
 
 

@@ -1222,9 +1222,10 @@ struct Panel {
     double t_build = 0.0;
 };
 
-static int factorize(Panel &P, const int64_t *col, int64_t n, DevBuf<int> &code,
-                     DevBuf<int64_t> &levels, int64_t &nl) {
-    hipStream_t s = P.s;
+// Codes of an int64 column in sorted level order (core/passes.py:329-335):
+// code (n) and the level values (nl), on stream s.
+int factorize_col(hipStream_t s, Temp &tmp, const int64_t *col, int64_t n, DevBuf<int> &code,
+                  DevBuf<int64_t> &levels, int64_t &nl) {
     // min / max for the key range
     const int nb = 256;
     DevBuf<int64_t> part;
@@ -1247,9 +1248,9 @@ static int factorize(Panel &P, const int64_t *col, int64_t n, DevBuf<int> &code,
     DEV_ALLOC(head, n);
     DEV_ALLOC(rank, n);
     hipLaunchKernelGGL(k_key_from_col, dim3(grid1(n)), dim3(T256), 0, s, n, col, lo, k1.p, i1.p);
-    HIP_TRY(sort_pairs(P.tmp, k1.p, k2.p, i1.p, i2.p, n, 0, bits, s));
+    HIP_TRY(sort_pairs(tmp, k1.p, k2.p, i1.p, i2.p, n, 0, bits, s));
     hipLaunchKernelGGL(k_heads_u64, dim3(grid1(n)), dim3(T256), 0, s, n, k2.p, head.p);
-    HIP_TRY(inclusive_sum(P.tmp, head.p, rank.p, n, s));
+    HIP_TRY(inclusive_sum(tmp, head.p, rank.p, n, s));
     int last = 0;
     HIP_TRY(scalar_to_host(&last, rank.p + n - 1, s));
     nl = last;
@@ -1259,6 +1260,11 @@ static int factorize(Panel &P, const int64_t *col, int64_t n, DevBuf<int> &code,
                        head.p, rank.p, code.p, levels.p);
     HIP_TRY(hipGetLastError());
     return HDFE_STATUS_OK;
+}
+
+static int factorize(Panel &P, const int64_t *col, int64_t n, DevBuf<int> &code,
+                     DevBuf<int64_t> &levels, int64_t &nl) {
+    return factorize_col(P.s, P.tmp, col, n, code, levels, nl);
 }
 
 #define TRY(expr)                              \

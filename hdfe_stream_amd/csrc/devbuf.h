@@ -119,3 +119,9 @@ inline int bits_for(uint64_t maxval) {
 }
 
 }  // namespace hdfe
+
+namespace hdfe {
+// build.hip: codes of an int64 column in sorted level order.
+int factorize_col(hipStream_t s, Temp &tmp, const int64_t *col, int64_t n, DevBuf<int> &code,
+                  DevBuf<int64_t> &levels, int64_t &nl);
+}  // namespace hdfe

@@ -190,6 +190,17 @@ int hdfe_jla_combine(int64_t n, int64_t n_draws, double *moments, double *stats)
 
 void hdfe_rows_destroy(void *h);
 
+/* ------------------------------------------------- connected sets */
+
+/* The leave-one-out connected set of the rows (a[i], b[i]), a the workers
+ * (device int64 id columns): keep (n, device uint8) marks the rows whose worker
+ * and firm both survive; info (host, 8) = components before, workers with one
+ * row, cut workers, rounds, workers and firms left, distinct pairs, vertices.
+ * Replaces _find_components(leave_one_out=True) / _leave_one_out
+ * (core/connected.py:378-562) and its graph kernels (core/kernels_graph.py). */
+int hdfe_loo_prune(int64_t n, const int64_t *a_ids, const int64_t *b_ids, uint8_t *keep,
+                   int64_t *info);
+
 #ifdef __cplusplus
 }
 #endif
