@@ -58,6 +58,16 @@ SIGNATURES = [
     ("hdfe_jla_combine", I32, [I64, I64, P, P]),
     ("hdfe_rows_destroy", None, [P]),
     ("hdfe_loo_prune", I32, [I64, P, P, P, P]),
+    ("hdfe_rows_csr", I32, [P, P, P, P, P, P, P]),
+    ("hdfe_rows_chunk_sum", I32, [P, I32, P, P]),
+    ("hdfe_kss_group_weight", I32, [I64, P, P, P]),
+    ("hdfe_kss_reduce", I32, [P, I32, P, P, P, P, P, P, P]),
+    ("hdfe_kss_project", I32, [I64, I32, I32, P, P, I64, P, P, P, P, P, P, P]),
+    ("hdfe_kss_group_sums", I32, [I64, I32, P, P, P, P, P]),
+    ("hdfe_kss_trace_moments", I32, [I64, I32, P, P, I64, P, P, P, P, P, P]),
+    ("hdfe_kss_drop_null", I32, [I64, I64, I32, I64, I32, P, P, P, P, P, P, P]),
+    ("hdfe_kss_sigma2", I32, [I64, I32, P, P, P, P, P, P, P]),
+    ("hdfe_kss_plugin", I32, [I64, P, P, I64, P, P, P, P]),
 ]
 
 

@@ -462,6 +462,31 @@ int hdfe_jla_combine(int64_t n, int64_t n_draws, double *moments, double *stats)
     return HDFE_STATUS_OK;
 }
 
+// The level-to-rows CSR and its chunks (device pointers), for kss.hip.
+int hdfe_rows_csr(void *h, const int **rptr, const int **rrow, const int **cptr, const int **clvl,
+                  const int **cbeg, int64_t *nchunk) {
+    ARG_CHECK(h, "hdfe_rows_csr: null handle");
+    RowPasses &R = *(RowPasses *)h;
+    *rptr = R.rptr.p;
+    *rrow = R.rrow.p;
+    *cptr = R.cptr.p;
+    *clvl = R.clvl.p;
+    *cbeg = R.cbeg.p;
+    *nchunk = R.nchunk;
+    return HDFE_STATUS_OK;
+}
+
+// out (L, k) = per level, the sum of its chunks' partials (nchunk, k) in
+// chunk order.
+int hdfe_rows_chunk_sum(void *h, int k, const double *part, double *out) {
+    ARG_CHECK(h && part && out, "hdfe_rows_chunk_sum: null argument");
+    RowPasses &R = *(RowPasses *)h;
+    hipLaunchKernelGGL(k_center_sum, dim3(grid1(R.L * k)), dim3(256), 0, 0, R.L, k, R.cptr.p,
+                       part, out);
+    HIP_TRY(hipGetLastError());
+    return HDFE_STATUS_OK;
+}
+
 void hdfe_rows_destroy(void *h) {
     if (!h) return;
     RowPasses *R = (RowPasses *)h;

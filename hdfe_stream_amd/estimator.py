@@ -211,6 +211,8 @@ class HDFEResult:
     panel: object = None
     Gamma: object = None
     resid_sorted: object = None
+    gam_y: object = None
+    geff: object = None
 
     def coef(self):
         return dict(zip(self.coefnames, self.beta.tolist()))
@@ -346,7 +348,7 @@ class StreamingHDFE:
         res = HDFEResult(coefnames=names[1:], beta=beta, solver_info=info, fe_names=fe_order,
                          fe_levels=levels, fe_values=values, n_obs=panel.info["n"],
                          n_components=ncomp, A=A, diagnostics=diag, panel=panel, Gamma=Gamma,
-                         resid_sorted=resid)
+                         resid_sorted=resid, gam_y=gam_y, geff=geff)
         if self.verbose:
             print(f"fit: {diag}")
         return res

@@ -1,5 +1,6 @@
 """Leave-out (KSS) estimation on the device: JLA leverages by random
-projection, solved as blocks of many right-hand sides.
+projection and the Hutchinson bias term, each solved as blocks of many
+right-hand sides, and `leave_out_kss` leaving out an observation.
 
 Mirrors hdfe_stream's `jla_leverages` (leaveout.py:130-198) and
 `_combine_moments` (:209-282) over the operator `project_rows`
@@ -10,8 +11,17 @@ the five running moments.  The draws are keyed on the sorted row ordinal, as
 in the reference, so a panel whose rows are in the reference's canonical
 order (sorted by group, codes and variables) gets the same draws.
 
+The bias term mirrors `hutchinson_trace` / `_trace_block` (leaveout.py:
+321-448) over `apply_inverse` (inverse.py:311-460): coefficient-space
+Rademacher draws, two applications of the inverse per block, eight
+cross-moments.  `leave_out_kss` mirrors leaveout.py:1069-1255 with
+leave_out="observation" and `leave_out_components` (:634-805) with the
+default stayer rule ("own") and weighted centring.
+
 Not mirrored yet: covariates in the design (the Schur border, inverse.py:
-182-231); the leverages here are those of the fixed-effect design.
+182-231) -- the leverages and the bias are those of the fixed-effect design,
+and a model with covariates is refused; leaving out a match
+(leaveout_match.py); standard errors and the weak-identification diagnostic.
 """
 
 from __future__ import annotations
@@ -25,6 +35,7 @@ import numpy as np
 from . import _lib
 
 RHS_ZERO_TOL = 1e-11      # core/solve.py:44
+TRACE_SEED_OFFSET = 1_000_003     # leaveout.py:730
 
 
 @dataclass
@@ -155,3 +166,385 @@ def jla_leverages(panel, n_draws: int = 250, block: int | None = None, seed: int
                      "raw_sum_deviation": float(stats[3]), "solver": "explicit",
                      "converged": bool(conv), "iterations": its, "max_rel_residual": rel,
                      "precision": precision, "covariates": []})
+
+
+# ------------------------------------------------------------ bias term
+
+def _col_norms(X, rows, k):
+    out = np.empty(k)
+    if rows:
+        _lib.check(_lib.load().hdfe_col_norms(X.data_ptr(), rows, k, out.ctypes.data),
+                   "hdfe_col_norms")
+    else:
+        out[:] = 0.0
+    return out
+
+
+class CoefficientInverse:
+    """u = S^- v for coefficient-space blocks of a fitted Panel
+    (`apply_inverse`, inverse.py:311-374): levels (L, k) and a streamed block
+    (G, k), device tensors."""
+
+    def __init__(self, panel, tol=1e-10, maxiter=5000, precision="mixed"):
+        import torch
+        if panel.matrix is None:
+            panel.system(with_rhs=False)
+        self.panel = panel
+        self.tol, self.maxiter, self.precision = tol, maxiter, precision
+        info = panel.info
+        self.n, self.G, self.L, self.D = info["n"], info["G"], info["L"], info["D"]
+        self.ncomp = max(int(info["ncomp"]), 1)
+        self.g, _ = panel.ptr("g")
+        self.gstart, _ = panel.ptr("gstart")
+        self.codes, _ = panel.ptr("codes")
+        self.offs_d, _ = panel.ptr("offs")
+        self.lcomp, nc = panel.ptr("comp")
+        if not nc:
+            raise RuntimeError("the panel's components are not computed")
+        self.w = panel.ptr("w")[0] if info["weighted"] else None
+        self.offs = np.ascontiguousarray(panel.offs, np.int64)
+        self.nlev = np.ascontiguousarray(panel.n_levels, np.int64)
+        self.Wg = torch.empty(self.G, dtype=torch.float64, device="cuda")
+        _lib.check(_lib.load().hdfe_kss_group_weight(self.G, self.gstart, self.w,
+                                                     self.Wg.data_ptr()),
+                   "hdfe_kss_group_weight")
+        self.rows_state = _rows(panel)
+        self.iterations, self.converged, self.max_rel = [], True, 0.0
+
+    def drop_null(self, levels, streamed):
+        """In place: the component along S's null space removed
+        (inverse.py:376-460)."""
+        k = levels.shape[1]
+        _lib.check(_lib.load().hdfe_kss_drop_null(
+            self.G, self.L, self.D, self.ncomp, k, self.gstart, self.codes, self.lcomp,
+            self.offs.ctypes.data, self.nlev.ctypes.data, streamed.data_ptr(),
+            levels.data_ptr()), "hdfe_kss_drop_null")
+
+    def reduce(self, k, R=None, V=None, x=None):
+        """D_o' W diag(x) R for rows R (n, k), or D_o' W rho for the spread
+        streamed block V (G, k)."""
+        import torch
+        out = torch.empty((self.L, k), dtype=torch.float64, device="cuda")
+        _lib.check(_lib.load().hdfe_kss_reduce(
+            self.rows_state._h, k, self.g, self.w, None if x is None else x.data_ptr(),
+            None if R is None else R.data_ptr(), None if V is None else V.data_ptr(),
+            self.Wg.data_ptr(), out.data_ptr()), "hdfe_kss_reduce")
+        return out
+
+    def group_sums(self, R, x=None):
+        import torch
+        k = R.shape[1]
+        out = torch.empty((self.G, k), dtype=torch.float64, device="cuda")
+        _lib.check(_lib.load().hdfe_kss_group_sums(
+            self.G, k, self.gstart, self.w, None if x is None else x.data_ptr(), R.data_ptr(),
+            out.data_ptr()), "hdfe_kss_group_sums")
+        return out
+
+    def __call__(self, levels, streamed, want_rows=True):
+        """(U_levels (L, k), U_streamed (G, k), rows (A u)_i (n, k) or None)."""
+        import torch
+        levels = levels.to(torch.float64).contiguous().clone()
+        streamed = streamed.to(torch.float64).contiguous().clone()
+        k = levels.shape[1]
+        torch.cuda.synchronize()
+        self.drop_null(levels, streamed)
+        scale = np.sqrt(_col_norms(levels, self.L, k) ** 2 + _col_norms(streamed, self.G, k) ** 2)
+        rhs = levels - self.reduce(k, V=streamed)
+        torch.cuda.synchronize()
+        zero = _col_norms(rhs, self.L, k) <= RHS_ZERO_TOL * scale
+        U, it, conv, rel = self.panel.matrix.solve(rhs, zero, None, tol=self.tol,
+                                                   maxiter=self.maxiter,
+                                                   precision=self.precision)
+        torch.cuda.synchronize()
+        self.iterations.append(it)
+        self.converged &= conv
+        self.max_rel = max(self.max_rel, rel)
+        Ug = torch.empty((self.G, k), dtype=torch.float64, device="cuda")
+        rows = (torch.empty((self.n, k), dtype=torch.float64, device="cuda") if want_rows
+                else None)
+        _lib.check(_lib.load().hdfe_kss_project(
+            self.G, self.D, k, self.gstart, self.codes, self.n, self.offs_d, self.w,
+            self.Wg.data_ptr(), U.data_ptr(), streamed.data_ptr(),
+            None if rows is None else rows.data_ptr(), Ug.data_ptr()), "hdfe_kss_project")
+        return U, Ug, rows
+
+
+def _moment_divisor(total, rows):
+    """leaveout_se.py:202-214: T (R - 1) / R for R person-year rows."""
+    return total * (rows - 1.0) / rows if rows > 1 else total
+
+
+@dataclass
+class TraceEstimate:
+    """The three bias terms (leaveout.py:284-310)."""
+    var_psi: float
+    var_alpha: float
+    cov: float
+    n_draws: int
+    diagnostics: dict = field(default_factory=dict)
+
+    def as_dict(self):
+        return {"var(psi)": self.var_psi, "var(alpha)": self.var_alpha,
+                "cov(psi, alpha)": self.cov}
+
+
+def hutchinson_trace(panel, sigma2, n_draws: int = 250, block: int | None = None,
+                     seed: int = 0, psi: int = 0, tol: float = 1e-10, maxiter: int = 5000,
+                     precision: str = "mixed", timings: dict | None = None) -> TraceEstimate:
+    """tr(Q S^- A'W Omega A S^-) for var(psi), var(alpha) and their
+    covariance by Hutchinson's estimator (leaveout.py:321-448).
+
+    `sigma2` is the per-row variance estimate (device, the panel's sorted row
+    order); `psi` the index of the non-streamed dimension whose variance is
+    wanted (0: the first).  Per block of draws: Z (levels, then groups) from
+    the counter-based generator, u = S^- Z with its rows, v = A'W Sigma A u,
+    u2 = S^- v, and the eight moments of (Z, u2)."""
+    import torch
+    block = min(int(n_draws), 256) if block is None else int(block)
+    if block < 1 or n_draws < 1:
+        raise ValueError("n_draws and block must be at least 1")
+    inv = CoefficientInverse(panel, tol, maxiter, precision)
+    if not 0 <= psi < inv.D:
+        raise ValueError(f"psi must index a non-streamed dimension (0..{inv.D - 1})")
+    lib = _lib.load()
+    L, G = inv.L, inv.G
+    sig = sigma2.to(torch.float64).contiguous()
+    psi_codes = inv.codes + 4 * psi * inv.n
+    totals = np.zeros(8)
+    drawn, blocks = 0, 0
+    t0 = time.perf_counter()
+    while drawn < n_draws:
+        size = min(block, n_draws - drawn)
+        Z = torch.empty((L + G, size), dtype=torch.float64, device="cuda")
+        torch.cuda.synchronize()
+        _lib.check(lib.hdfe_rademacher(0, L + G, size, seed, drawn, Z.data_ptr()),
+                   "hdfe_rademacher")
+        Zl, Zg = Z[:L], Z[L:]
+        _, _, rows = inv(Zl, Zg)
+        vl = inv.reduce(size, R=rows, x=sig)
+        vg = inv.group_sums(rows, x=sig)
+        del rows
+        Ul, Ug, _ = inv(vl, vg, want_rows=False)
+        m8 = np.empty(8)
+        torch.cuda.synchronize()
+        _lib.check(lib.hdfe_kss_trace_moments(G, size, inv.gstart, psi_codes,
+                                              int(inv.offs[psi]), inv.w, Zl.data_ptr(),
+                                              Zg.data_ptr(), Ul.data_ptr(), Ug.data_ptr(),
+                                              m8.ctypes.data), "hdfe_kss_trace_moments")
+        totals += m8
+        drawn += size
+        blocks += 1
+    seconds = time.perf_counter() - t0
+    weight, a1a2, b1b2, cross, a1, a2, b1, b2 = totals / n_draws
+    var_psi = a1a2 / weight - (a1 / weight) * (a2 / weight)
+    var_alpha = b1b2 / weight - (b1 / weight) * (b2 / weight)
+    cov = 0.5 * (cross / weight - (a1 / weight) * (b2 / weight) - (b1 / weight) * (a2 / weight))
+    scale = weight / _moment_divisor(weight, inv.n)
+    if timings is not None:
+        timings.update(trace_seconds=seconds, trace_solves=2 * n_draws,
+                       trace_iterations=list(inv.iterations))
+    return TraceEstimate(
+        var_psi=float(scale * var_psi), var_alpha=float(scale * var_alpha),
+        cov=float(scale * cov), n_draws=n_draws,
+        diagnostics={"blocks": blocks, "n_obs": float(weight), "totals": totals,
+                     "converged": bool(inv.converged), "max_rel_residual": inv.max_rel,
+                     "iterations": list(inv.iterations), "precision": precision})
+
+
+# ------------------------------------------------------- the components
+
+@dataclass
+class LeaveOutComponents:
+    """KSS leave-out variance components with the plug-in ones and the bias
+    (leaveout.py:452-520): dicts keyed "var(psi)", "var(alpha)",
+    "cov(psi, alpha)"."""
+    plug_in: dict
+    bias: dict
+    leave_out: dict
+    n_obs: int
+    n_movers: int
+    leverages: JLALeverages
+    trace: TraceEstimate
+    diagnostics: dict = field(default_factory=dict)
+    pruning: dict | None = None
+    fit: object = None
+
+    def as_dict(self):
+        return dict(self.leave_out)
+
+    def summary(self) -> str:
+        lines = [f"KSS leave-out components ({self.n_obs:,} observations, "
+                 f"{self.n_movers:,} movers)",
+                 f"  {'':18s}{'plug-in':>12s}{'bias':>12s}{'leave-out':>12s}"]
+        for k in self.leave_out:
+            lines.append(f"  {k:18s}{self.plug_in[k]:12.6f}{self.bias[k]:12.6f}"
+                         f"{self.leave_out[k]:12.6f}")
+        return "\n".join(lines)
+
+
+def _movers(panel, psi):
+    """Rows whose group is seen at more than one level of dimension `psi`
+    (leaveout.py:928-947), as a device bool tensor."""
+    import torch
+    info = panel.info
+    n, G = info["n"], info["G"]
+    g = panel.tensor("g", torch.int32).long()
+    codes = panel.tensor("codes", torch.int32).view(info["D"], n)[psi].long()
+    lo = torch.full((G,), np.iinfo(np.int64).max, dtype=torch.int64, device="cuda")
+    hi = torch.full((G,), -1, dtype=torch.int64, device="cuda")
+    lo.scatter_reduce_(0, g, codes, "amin")
+    hi.scatter_reduce_(0, g, codes, "amax")
+    return (lo != hi)[g]
+
+
+def leave_out_components(result, n_draws: int = 250, block: int | None = None, seed: int = 0,
+                         psi: str | None = None, tol: float = 1e-10, maxiter: int = 5000,
+                         precision: str = "mixed", timings: dict | None = None
+                         ) -> LeaveOutComponents:
+    """KSS components for a fit leaving out an observation
+    (leaveout.py:634-805): JLA leverages, the leverage check, the plug-in
+    moments, sigma2 with weighted centring and own-estimate stayers, the
+    Hutchinson bias with seed + 1_000_003, and plug-in minus bias."""
+    import torch
+    panel = result.panel
+    info = panel.info
+    if info["m"] != 1:
+        raise NotImplementedError(
+            "leave-out components with covariates need the covariates in the design "
+            "(the Schur border); fit the fixed effects alone, as 'y ~ 1 | worker + firm'")
+    alpha = result.fe_names[0]
+    others = result.fe_names[1:]
+    psi = psi or others[0]
+    if psi not in others:
+        raise ValueError(f"psi must be a non-streamed dimension, one of {others}; got {psi!r}")
+    pi = others.index(psi)
+    n, G = info["n"], info["G"]
+    t = {} if timings is None else timings
+    t0 = time.perf_counter()
+    lev = jla_leverages(panel, n_draws=n_draws, block=block, seed=seed, tol=tol,
+                        maxiter=maxiter, precision=precision, timings=t)
+    t1 = time.perf_counter()
+    worst = lev.diagnostics["max_leverage"]
+    if worst >= 1.0 - 1e-9:
+        raise ValueError(
+            f"maximum leverage is {worst:.6f}: the panel is not leave-one-out connected "
+            "(leaving out one observation at a time), so the leave-out residual is "
+            "undefined for at least one observation. Refit with "
+            "connected='leave_one_out', or use leave_out_kss, which does")
+    movers = _movers(panel, pi)
+    lib = _lib.load()
+    gstart, _ = panel.ptr("gstart")
+    codes, _ = panel.ptr("codes")
+    w = panel.ptr("w")[0] if info["weighted"] else None
+    gam = result.gam_y.contiguous()
+    geff = result.geff.contiguous()
+    s6 = np.empty(6)
+    torch.cuda.synchronize()
+    _lib.check(lib.hdfe_kss_plugin(G, gstart, codes + 4 * pi * n, int(panel.offs[pi]),
+                                   gam.data_ptr(), geff.data_ptr(), w, s6.ctypes.data),
+               "hdfe_kss_plugin")
+    total, sa, sb, saa, sbb, sab = s6
+    ma, mb = sa / total, sb / total
+    scale = total / _moment_divisor(total, n)
+    plug_in = {"var(psi)": scale * (saa / total - ma * ma),
+               "var(alpha)": scale * (sbb / total - mb * mb),
+               "cov(psi, alpha)": scale * (sab / total - ma * mb)}
+    V, _ = panel.ptr("V")
+    sigma2 = torch.empty(n, dtype=torch.float64, device="cuda")
+    resid = result.resid_sorted.contiguous()
+    ybar = C.c_double()
+    _lib.check(lib.hdfe_kss_sigma2(n, info["m"], V, w, resid.data_ptr(),
+                                   lev.complement.data_ptr(), lev.factor.data_ptr(),
+                                   sigma2.data_ptr(), C.byref(ybar)), "hdfe_kss_sigma2")
+    t2 = time.perf_counter()
+    trace = hutchinson_trace(panel, sigma2, n_draws=n_draws, block=block,
+                             seed=seed + TRACE_SEED_OFFSET, psi=pi, tol=tol, maxiter=maxiter,
+                             precision=precision, timings=t)
+    t3 = time.perf_counter()
+    bias = trace.as_dict()
+    leave_out = {k: plug_in[k] - bias[k] for k in plug_in}
+    if info["weighted"]:
+        wt = panel.tensor("w", torch.float64)
+        sigma2_mean = float((sigma2 / wt).mean())
+    else:
+        sigma2_mean = float(sigma2.mean())
+    t.update(leverage_seconds=t1 - t0, sigma2_seconds=t2 - t1, bias_seconds=t3 - t2)
+    return LeaveOutComponents(
+        plug_in=plug_in, bias=bias, leave_out=leave_out, n_obs=n,
+        n_movers=int(movers.sum()), leverages=lev, trace=trace,
+        diagnostics={"psi": psi, "alpha": alpha, "stayers": "own", "max_leverage": worst,
+                     "leave_out": "observation", "sigma2_mean": sigma2_mean,
+                     "ybar": float(ybar.value), "covariates": [],
+                     "n_levels": {psi: panel.n_levels[pi], alpha: G},
+                     "sigma2": sigma2})
+
+
+def leave_out_kss(fml: str, data, *, psi: str | None = None, n_draws: int = 250,
+                  seed: int = 0, block: int | None = None, prune: bool = True,
+                  leave_out: str = "observation", stayers: str | None = None,
+                  precision: str = "mixed", tol: float = 1e-10, maxiter: int = 5000,
+                  timings: dict | None = None, **options) -> LeaveOutComponents:
+    """Kline-Saggio-Solvsten leave-out variance components, end to end, on
+    the device (leaveout.py:1069-1255): prune to the largest leave-one-out
+    connected set of (alpha, psi), fit with alpha streamed and no singleton
+    drop, then `leave_out_components`.
+
+    `alpha` is the formula's first fixed effect and `psi` (default: the
+    second) the dimension whose variance is wanted.  Only
+    leave_out="observation" and stayers="own" are implemented here."""
+    import torch
+
+    from .connected import leave_one_out
+    from .estimator import StreamingHDFE, _column, parse_formula
+
+    if leave_out not in ("match", "observation"):
+        raise ValueError("leave_out must be 'match' or 'observation'")
+    if leave_out == "match":
+        raise NotImplementedError("leaving out a match is not implemented on the device; "
+                                  "use leave_out='observation'")
+    if stayers not in (None, "own"):
+        raise NotImplementedError("only stayers='own' is implemented on the device")
+    if "|" not in fml:
+        raise ValueError(
+            f"{fml!r} has no fixed effects; leave-out estimation needs at least two "
+            "dimensions, as in 'y ~ x | worker_id + firm_id'")
+    y, xs, fe = parse_formula(fml)
+    if len(fe) < 2:
+        raise ValueError(f"{fml!r} has one fixed effect; leave-out estimation needs at least "
+                         "two, one of which is streamed")
+    alpha = fe[0]
+    psi = psi or fe[1]
+    if psi == alpha or psi not in fe:
+        raise ValueError(f"psi must be one of the fixed effects other than {alpha!r} "
+                         f"(which plays the part of alpha); got {psi!r} from {fe}")
+    options.setdefault("stream", alpha)
+    options.setdefault("fixef_rm", "none")
+    options.setdefault("tol", tol)
+    options.setdefault("maxiter", maxiter)
+    pruning = None
+    t = {} if timings is None else timings
+    t0 = time.perf_counter()
+    if prune:
+        keep, pinfo = leave_one_out(_column(data, alpha), _column(data, psi))
+        idx = torch.nonzero(keep, as_tuple=True)[0]
+        cols = [y] + list(xs) + list(fe) + ([options["weights"]] if options.get("weights")
+                                            else [])
+        sub = {}
+        for c in dict.fromkeys(cols):
+            v = _column(data, c)
+            if not isinstance(v, torch.Tensor):
+                v = torch.from_numpy(np.ascontiguousarray(v)).cuda()
+            sub[c] = v.cuda().index_select(0, idx)
+        data = sub
+        pruning = {"fe": [alpha, psi], **pinfo}
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    fit = StreamingHDFE(y, xs, fe, keep_intermediates=True, **options).fit(data)
+    t2 = time.perf_counter()
+    comps = leave_out_components(fit, n_draws=n_draws, block=block, seed=seed, psi=psi,
+                                 tol=tol, maxiter=maxiter, precision=precision, timings=t)
+    t.update(prune_seconds=t1 - t0, fit_seconds=t2 - t1,
+             total_seconds=time.perf_counter() - t0)
+    comps.pruning = pruning
+    comps.fit = fit
+    return comps

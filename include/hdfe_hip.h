@@ -188,7 +188,70 @@ int hdfe_rows_project_moments(void *h, const int *gstart, const int *codes, cons
  * mean of P_bar, largest |P + M - 1|. */
 int hdfe_jla_combine(int64_t n, int64_t n_draws, double *moments, double *stats);
 
+/* The level-to-rows CSR of a row-pass state (device pointers): rptr (L+1),
+ * rrow (n D), the chunks of at most 1024 rows (cptr (L+1), clvl, cbeg) and
+ * their count. */
+int hdfe_rows_csr(void *h, const int **rptr, const int **rrow, const int **cptr, const int **clvl,
+                  const int **cbeg, int64_t *nchunk);
+
+/* out (L, k) = per level, the sum in chunk order of part (nchunk, k). */
+int hdfe_rows_chunk_sum(void *h, int k, const double *part, double *out);
+
 void hdfe_rows_destroy(void *h);
+
+/* ------------------------------------------------- leave-out (KSS) */
+
+/* Group weights W_g (G, device) = sum of w over the group's rows (w NULL:
+ * ones); the denominators of _nb_spread_groups (kernels_inverse.py:133-152). */
+int hdfe_kss_group_weight(int64_t G, const int *gstart, const double *w, double *Wg);
+
+/* out (L, k) = D_o' W diag(x) R for a row block R (n, k), or D_o' W rho with
+ * rho the streamed block V (G, k) spread over the rows (V / W_g) when R is
+ * NULL (_nb_reduce_rows + _nb_spread_groups, kernels_inverse.py:111-152).
+ * x (n) may be NULL (ones). */
+int hdfe_kss_reduce(void *rows_h, int k, const int *g, const double *w, const double *x,
+                    const double *R, const double *V, const double *Wg, double *out);
+
+/* The rows (A u)_i (n, k; skipped when rows is NULL) and streamed block Ug
+ * (G, k) of a coefficient-space solution U (L, k) with streamed right-hand
+ * side V (G, k) (_nb_project_coef, kernels_inverse.py:156-188). */
+int hdfe_kss_project(int64_t G, int D, int k, const int *gstart, const int *codes, int64_t n,
+                     const int64_t *offs_d, const double *w, const double *Wg, const double *U,
+                     const double *V, double *rows, double *Ug);
+
+/* out (G, k) = per group sum of w x R (_nb_group_sums, core/kernels_groups.py:
+ * 16-27, with the sigma2 factor x of _trace_block's sink, leaveout.py:421-428). */
+int hdfe_kss_group_sums(int64_t G, int k, const int *gstart, const double *w, const double *x,
+                        const double *R, double *out);
+
+/* Hutchinson's eight cross-moments summed over rows and the k draws (host
+ * out, 8): sum w, w a1 a2, w b1 b2, w (a1 b2 + b1 a2), w a1, w a2, w b1, w b2
+ * with a the psi dimension's level value and b the group value of Z (1) and
+ * U (2) (_nb_trace_moments, kernels_inverse.py:191-231). */
+int hdfe_kss_trace_moments(int64_t G, int k, const int *gstart, const int *psi_codes,
+                           int64_t psi_off, const double *w, const double *Zl, const double *Zg,
+                           const double *Ul, const double *Ug, double *out);
+
+/* Remove, in place, the null-space part of a coefficient-space block (Vg
+ * (G, k), Vl (L, k), device): one basis vector per connected component and
+ * one per further dimension (_null_space / _drop_null_component,
+ * inverse.py:376-460).  lcomp: component of each first-dimension level
+ * (device); codes0: the rows' first-dimension codes; offs / nlev host. */
+int hdfe_kss_drop_null(int64_t G, int64_t L, int D, int64_t ncomp, int k, const int *gstart,
+                       const int *codes0, const int *lcomp, const int64_t *offs_h,
+                       const int64_t *nlev_h, double *Vg, double *Vl);
+
+/* sigma2_i = w_i (y_i - ybar) e_i / M_ii * factor_i, y column 0 of V (n, m),
+ * ybar its weighted mean (*ybar_out); device in/out
+ * (_leave_one_out_sigma2, leaveout.py:949-1016, centering "weighted"). */
+int hdfe_kss_sigma2(int64_t n, int m, const double *V, const double *w, const double *e,
+                    const double *Mb, const double *F, double *out, double *ybar_out);
+
+/* The plug-in moments' sums (host out, 6): sum w, w a, w b, w a^2, w b^2,
+ * w a b with a the row's psi effect gam[psi_off + code] and b its group
+ * effect geff[g] (_plug_in_moments, leaveout.py:807-832). */
+int hdfe_kss_plugin(int64_t G, const int *gstart, const int *psi_codes, int64_t psi_off,
+                    const double *gam, const double *geff, const double *w, double *out);
 
 /* ------------------------------------------------- connected sets */
 

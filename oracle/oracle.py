@@ -24,6 +24,11 @@ the reference's FP64 algorithm:
            combination               core/kernels_groups.py:52-77,
                                       kernels_inverse.py:34-108,
                                       inverse.py:162-309, leaveout.py:130-282
+           apply_inverse, null space, Hutchinson bias, sigma2, plug-in,
+           leave_out_kss leaving out an observation
+                                      inverse.py:311-460, kernels_inverse.py:
+                                      111-231, leaveout.py:321-448, 634-1016,
+                                      1069-1255, leaveout_se.py:202-214
 
 Each piece is pinned in tests/test_oracle.py against what the mathematics
 fixes (dense closed forms, scipy routines, brute force) and the golden
@@ -109,7 +114,7 @@ def singleton_restriction(fe_cols: list[np.ndarray]) -> np.ndarray:
 
 
 def pass0(data: dict, y: str, xs: list[str], fe: list[str], weights: str | None = None,
-          fixef_rm: str = "singleton", sort_vars: bool = True) -> dict:
+          fixef_rm: str = "singleton", sort_vars: bool = True, stream: str | None = None) -> dict:
     """Rows after the restrictions, with level codes in sorted level order,
     sorted by (streamed group, other codes[, variables])."""
     cols = [np.asarray(data[f]) for f in fe]
@@ -119,7 +124,8 @@ def pass0(data: dict, y: str, xs: list[str], fe: list[str], weights: str | None 
     V = np.column_stack([np.asarray(data[v], np.float64)[keep] for v in names])
     w = np.asarray(data[weights], np.float64)[keep] if weights else None
     card = [len(np.unique(c)) for c in cols]
-    gi = max(range(len(fe)), key=lambda d: (card[d], -d))           # estimator.py:383
+    gi = (fe.index(stream) if stream is not None
+          else max(range(len(fe)), key=lambda d: (card[d], -d)))     # estimator.py:371-385
     others = [d for d in range(len(fe)) if d != gi]
     levels, codes = [], []
     for d in others:
@@ -347,9 +353,9 @@ def pass2(p0: dict, c: dict, gam_y: np.ndarray, beta: np.ndarray, xcols: list[in
 
 def fit(data: dict, y: str = "y", xs=("x1", "x2"), fe=("worker_id", "firm_id", "year"),
         weights: str | None = None, tol: float = 1e-10, maxiter: int = 5000,
-        rhs_block: int = 8) -> dict:
+        rhs_block: int = 8, fixef_rm: str = "singleton", stream: str | None = None) -> dict:
     """The whole FP64 fit of `y ~ xs | fe`, as the reference runs it."""
-    p0 = pass0(data, y, list(xs), list(fe), weights)
+    p0 = pass0(data, y, list(xs), list(fe), weights, fixef_rm=fixef_rm, stream=stream)
     c = pass1(p0)
     L = c["L"]
     m = c["m"]
@@ -376,8 +382,9 @@ def fit(data: dict, y: str = "y", xs=("x1", "x2"), fe=("worker_id", "firm_id", "
     geff, e = pass2(p0, c, gam_y, beta, xj)
     resid_src = np.full(len(p0["keep"]), np.nan)
     resid_src[np.flatnonzero(p0["keep"])[p0["order"]]] = e
-    return {"resid_source_order": resid_src,"beta": beta, "Gamma": Gamma, "gam_y": gam_y, "A": A, "S": S, "diag": d, "rhs": b,
-            "comp": comp, "n_components": n_comp, "iterations": its, "p0": p0, "cells": c,
+    return {"resid_source_order": resid_src, "beta": beta, "Gamma": Gamma, "gam_y": gam_y,
+            "A": A, "S": S, "diag": d, "dinv": dinv, "rhs": b, "comp": comp,
+            "n_components": n_comp, "iterations": its, "p0": p0, "cells": c,
             "worker_effects": geff, "resid": e}
 
 
@@ -472,6 +479,182 @@ def jla_leverages(p0: dict, c: dict, S, dinv, n_draws: int, block: int, seed: in
         jla_moments(R, proj, moments)
         drawn += size
     return moments
+
+
+# ------------------------------------------------------- KSS bias term
+
+def design(p0: dict, c: dict):
+    """The sparse incidence matrices of the sorted rows: D_g (n, G) of the
+    streamed groups and D_o (n, L) of the other dimensions' levels."""
+    g, codes = p0["g"], p0["codes"]
+    n, D = codes.shape
+    G = int(g.max()) + 1 if n else 0
+    rows = np.arange(n)
+    Dg = sp.csr_matrix((np.ones(n), (rows, g)), shape=(n, G))
+    Do = sp.csr_matrix((np.ones(n * D), (np.repeat(rows, D),
+                                        (codes + c["offs"][None, :]).reshape(-1))),
+                       shape=(n, c["L"]))
+    return Dg, Do
+
+
+class KSSContext:
+    """What the bias term reads from a fit: incidences, weights, group
+    weights, the reduced S with its Jacobi inverse, and the null-space basis
+    of the full coefficient space (inverse.py:376-433)."""
+
+    def __init__(self, fitres: dict, tol: float = 1e-10, maxiter: int = 5000):
+        p0, c = fitres["p0"], fitres["cells"]
+        self.p0, self.c = p0, c
+        self.Dg, self.Do = design(p0, c)
+        n = self.Dg.shape[0]
+        self.w = np.ones(n) if p0["w"] is None else np.asarray(p0["w"], np.float64)
+        self.Wg = self.Dg.T @ self.w
+        self.S, self.dinv = fitres["S"], fitres["dinv"]
+        self.tol, self.maxiter = tol, maxiter
+        self.G, self.L = self.Dg.shape[1], c["L"]
+        self.g = p0["g"]
+        # basis: per component (+1 on its groups, -1 on its first-dimension
+        # levels); per further dimension (+1 first dimension, -1 that one)
+        comp, ncomp = fitres["comp"], fitres["n_components"]
+        nl, offs = p0["n_levels"], c["offs"]
+        gcomp = comp[p0["codes"][np.r_[0, np.flatnonzero(np.diff(self.g)) + 1], 0]]
+        vecs = []
+        for k in range(ncomp):
+            lv = np.zeros(self.L)
+            lv[offs[0]:offs[0] + nl[0]] = -(comp == k).astype(float)
+            vecs.append(np.concatenate([(gcomp == k).astype(float), lv]))
+        for t in range(1, len(nl)):
+            lv = np.zeros(self.L)
+            lv[offs[0]:offs[0] + nl[0]] = 1.0
+            lv[offs[t]:offs[t] + nl[t]] = -1.0
+            vecs.append(np.concatenate([np.zeros(self.G), lv]))
+        q, _ = np.linalg.qr(np.column_stack(vecs))
+        self.q_groups, self.q_levels = q[:self.G], q[self.G:]
+
+
+def drop_null(ctx: KSSContext, levels: np.ndarray, streamed: np.ndarray):
+    """inverse.py:446-460: the component along the null space removed."""
+    overlap = ctx.q_groups.T @ streamed + ctx.q_levels.T @ levels
+    return levels - ctx.q_levels @ overlap, streamed - ctx.q_groups @ overlap
+
+
+def apply_inverse(ctx: KSSContext, levels: np.ndarray, streamed: np.ndarray):
+    """u = S^- v for a coefficient-space block (inverse.py:311-374): drop the
+    null component, reduce the spread streamed block onto the levels
+    (kernels_inverse.py:111-152), solve, then recover the streamed block and
+    the rows (A u)_i (kernels_inverse.py:156-188).  Returns (U_levels,
+    U_groups, rows)."""
+    levels, streamed = drop_null(ctx, np.asarray(levels, np.float64),
+                                 np.asarray(streamed, np.float64))
+    scale = np.sqrt((levels * levels).sum(axis=0) + (streamed * streamed).sum(axis=0))
+    Wg = np.where(ctx.Wg > 0, ctx.Wg, 1.0)
+    rho = (streamed / Wg[:, None])[ctx.g]
+    rhs = levels - ctx.Do.T @ (ctx.w[:, None] * rho)
+    zero = np.linalg.norm(rhs, axis=0) <= RHS_ZERO_TOL * scale
+    U, _, _, _ = pcg(ctx.S, ctx.dinv, rhs, zero, None, ctx.tol, ctx.maxiter)
+    h = ctx.Do @ U
+    u0 = (ctx.Dg.T @ (ctx.w[:, None] * (rho - h))) / Wg[:, None]
+    return U, u0, h + u0[ctx.g]
+
+
+def trace_moments(ctx: KSSContext, psi_col: int, Zl, Zg, Ul, Ug) -> np.ndarray:
+    """The eight cross-moments summed over rows and draws
+    (kernels_inverse.py:191-231)."""
+    lv = ctx.c["offs"][psi_col] + ctx.p0["codes"][:, psi_col]
+    a1, a2, b1, b2 = Zl[lv], Ul[lv], Zg[ctx.g], Ug[ctx.g]
+    w = ctx.w[:, None]
+    return np.array([w.sum() * Zl.shape[1], (w * a1 * a2).sum(), (w * b1 * b2).sum(),
+                     (w * (a1 * b2 + b1 * a2)).sum(), (w * a1).sum(), (w * a2).sum(),
+                     (w * b1).sum(), (w * b2).sum()])
+
+
+def moment_divisor(total: float, rows: int) -> float:
+    """leaveout_se.py:202-214: total (R - 1) / R for R rows."""
+    return total * (rows - 1.0) / rows if rows > 1 else total
+
+
+def trace_components(totals: np.ndarray, n_draws: int, rows: int):
+    """(var_psi, var_alpha, cov) from the summed moments (leaveout.py:378-390)."""
+    weight, a1a2, b1b2, cross, a1, a2, b1, b2 = totals / n_draws
+    var_psi = a1a2 / weight - (a1 / weight) * (a2 / weight)
+    var_alpha = b1b2 / weight - (b1 / weight) * (b2 / weight)
+    cov = 0.5 * (cross / weight - (a1 / weight) * (b2 / weight) - (b1 / weight) * (a2 / weight))
+    scale = weight / moment_divisor(weight, rows)
+    return scale * var_psi, scale * var_alpha, scale * cov
+
+
+def hutchinson_trace(ctx: KSSContext, sigma2: np.ndarray, n_draws: int, block: int,
+                     seed: int, psi_col: int = 0):
+    """tr(Q S^- A'W Omega A S^-) for the three components by Hutchinson's
+    estimator with coefficient-space Rademacher draws keyed on (position,
+    draw), levels first then groups (leaveout.py:321-448)."""
+    L, G = ctx.L, ctx.G
+    totals = np.zeros(8)
+    drawn = 0
+    while drawn < n_draws:
+        size = min(block, n_draws - drawn)
+        Z = rademacher(0, L + G, size, seed, drawn)
+        Zl, Zg = Z[:L], Z[L:]
+        _, _, rows = apply_inverse(ctx, Zl, Zg)
+        weighted = rows * sigma2[:, None]
+        vl = ctx.Do.T @ (ctx.w[:, None] * weighted)
+        vg = ctx.Dg.T @ (ctx.w[:, None] * weighted)
+        Ul, Ug, _ = apply_inverse(ctx, vl, vg)
+        totals += trace_moments(ctx, psi_col, Zl, Zg, Ul, Ug)
+        drawn += size
+    return trace_components(totals, n_draws, len(ctx.g)), totals
+
+
+def sigma2(y: np.ndarray, e: np.ndarray, w: np.ndarray, complement: np.ndarray,
+           factor: np.ndarray) -> np.ndarray:
+    """w_i (y_i - ybar_w) e_i / M_ii * factor_i (leaveout.py:1000-1003)."""
+    ybar = float(w @ y / w.sum())
+    return w * (y - ybar) * e / complement * factor
+
+
+def plug_in(a: np.ndarray, b: np.ndarray, w: np.ndarray, rows: int):
+    """Weighted var(a), var(b), cov(a, b) over rows, divided by n - 1
+    (leaveout.py:807-832)."""
+    t = w.sum()
+    ma, mb = (w @ a) / t, (w @ b) / t
+    scale = t / moment_divisor(t, rows)
+    return (scale * ((w @ (a * a)) / t - ma * ma), scale * ((w @ (b * b)) / t - mb * mb),
+            scale * ((w @ (a * b)) / t - ma * mb))
+
+
+def leave_out_kss(data: dict, y: str = "y", xs=(), alpha: str = "worker_id",
+                  psi: str = "firm_id", others=(), n_draws: int = 250, block: int = 256,
+                  seed: int = 0, tol: float = 1e-10) -> dict:
+    """KSS leaving out an observation, end to end (leaveout.py:1069-1255,
+    634-805): leave-one-out pruning of the (alpha, psi) graph, the fit with
+    alpha streamed and no singleton drop, JLA leverages, sigma2 ("own"
+    stayers, weighted centring), the Hutchinson bias with seed + 1_000_003,
+    and the components plug-in minus bias.  Unweighted."""
+    a_ids, b_ids = np.asarray(data[alpha]), np.asarray(data[psi])
+    ua, ia = np.unique(a_ids, return_inverse=True)
+    ub, ib = np.unique(b_ids, return_inverse=True)
+    alive, prune = leave_one_out(ia, ib, len(ua), len(ub))
+    keep = alive[ia] & alive[len(ua) + ib]
+    sub = {k: np.asarray(v)[keep] for k, v in data.items()
+           if hasattr(v, "__len__") and len(v) == len(keep)}
+    fe = [alpha, psi] + list(others)
+    f = fit(sub, y, xs, fe, tol=tol, fixef_rm="none", stream=alpha)
+    p0, c = f["p0"], f["cells"]
+    n = len(p0["g"])
+    moments = jla_leverages(p0, c, f["S"], f["dinv"], n_draws, block, seed, tol)
+    _, M_bar, factor = combine_moments(moments, n_draws)
+    ctx = KSSContext(f, tol)
+    w = ctx.w
+    s2 = sigma2(p0["V"][:, 0], f["resid"], w, M_bar, factor)
+    psi_col = 0
+    a = f["gam_y"][c["offs"][psi_col] + p0["codes"][:, psi_col]]
+    b = f["worker_effects"][p0["g"]]
+    plug = plug_in(a, b, w, n)
+    bias, totals = hutchinson_trace(ctx, s2, n_draws, block, seed + 1_000_003, psi_col)
+    return {"plug_in": plug, "bias": bias,
+            "leave_out": tuple(p - q for p, q in zip(plug, bias)), "keep": keep,
+            "prune": prune, "fit": f, "sigma2": s2, "M_bar": M_bar, "factor": factor,
+            "moments": moments, "totals": totals, "n": n}
 
 
 # ------------------------------------------------------------------ graph

@@ -1,0 +1,143 @@
+"""Device KSS (leaving out an observation) against the oracle's restatement:
+the coefficient-space inverse, the Hutchinson bias draw for draw (the same
+counter-based coefficient-space draws), and leave_out_kss end to end on a
+pruned panel in the reference's canonical row order."""
+
+import numpy as np
+import pytest
+
+from conftest import rel
+
+pytestmark = pytest.mark.gpu
+
+FE2 = ["worker_id", "firm_id"]
+FE3 = ["worker_id", "firm_id", "year"]
+
+
+def _loo(O, d):
+    a, b = np.asarray(d["worker_id"]), np.asarray(d["firm_id"])
+    ua, ia = np.unique(a, return_inverse=True)
+    ub, ib = np.unique(b, return_inverse=True)
+    alive, _ = O.leave_one_out(ia, ib, len(ua), len(ub))
+    keep = alive[ia] & alive[len(ua) + ib]
+    return {k: np.asarray(v)[keep] for k, v in d.items()}
+
+
+def _fits(O, d, fe, tol=1e-12):
+    from hdfe_stream_amd.estimator import feols_stream
+    res = feols_stream("y ~ 1 | " + " + ".join(fe), d, stream=fe[0], fixef_rm="none",
+                       keep_intermediates=True, tol=tol)
+    ref = O.fit(d, "y", (), fe, tol=tol, fixef_rm="none", stream=fe[0])
+    return res, ref
+
+
+@pytest.mark.parametrize("fe", [FE2, FE3])
+def test_apply_inverse_matches_the_oracle(oracle_mod, fe):
+    import torch
+    from hdfe_stream_amd.leaveout import CoefficientInverse
+    from hdfe_stream_amd.simulate import akm_panel
+    O = oracle_mod
+    d = _loo(O, akm_panel(1500, 120, seed=6, p_move=0.2))
+    res, ref = _fits(O, d, fe)
+    print(f"apply_inverse fe={fe} rows={res.n_obs}", flush=True)
+    inv = CoefficientInverse(res.panel, tol=1e-12, precision="fp64")
+    ctx = O.KSSContext(ref, tol=1e-12)
+    assert (inv.G, inv.L) == (ctx.G, ctx.L)
+    rng = np.random.default_rng(3)
+    vl = rng.standard_normal((ctx.L, 5))
+    vg = rng.standard_normal((ctx.G, 5))
+    Ul, Ug, rows = inv(torch.from_numpy(vl).cuda(), torch.from_numpy(vg).cuda())
+    rUl, rUg, rrows = O.apply_inverse(ctx, vl, vg)
+    assert rel(rows.cpu().numpy(), rrows) < 1e-9
+    # both are Jacobi PCG from zero: the same solution, null-space part included
+    assert rel(Ul.cpu().numpy(), rUl) < 1e-8
+    assert rel(Ug.cpu().numpy(), rUg) < 1e-8
+    # the input is not modified
+    assert np.array_equal(vl, vl)
+
+
+@pytest.mark.parametrize("fe", [FE2, FE3])
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_hutchinson_matches_the_oracle(oracle_mod, fe, precision):
+    import torch
+    from hdfe_stream_amd.leaveout import hutchinson_trace
+    from hdfe_stream_amd.simulate import akm_panel
+    O = oracle_mod
+    d = _loo(O, akm_panel(1500, 120, seed=8, p_move=0.2))
+    res, ref = _fits(O, d, fe)
+    n = res.n_obs
+    print(f"hutchinson fe={fe} precision={precision} rows={n}", flush=True)
+    sigma2 = np.random.default_rng(5).uniform(0.05, 0.4, n)
+    tol = 1e-12 if precision == "fp64" else 1e-10
+    tr = hutchinson_trace(res.panel, torch.from_numpy(sigma2).cuda(), n_draws=48, block=20,
+                          seed=17, tol=tol, precision=precision)
+    ctx = O.KSSContext(ref, tol=1e-12)
+    (vp, va, vc), totals = O.hutchinson_trace(ctx, sigma2, 48, 20, seed=17)
+    got = np.array([tr.var_psi, tr.var_alpha, tr.cov])
+    want = np.array([vp, va, vc])
+    bound = 1e-8 if precision == "fp64" else 1e-6
+    assert np.abs(got - want).max() <= bound * np.abs(want).max()
+    assert tr.diagnostics["converged"]
+    # blocking does not change the estimate
+    tr2 = hutchinson_trace(res.panel, torch.from_numpy(sigma2).cuda(), n_draws=48, block=48,
+                           seed=17, tol=tol, precision=precision)
+    got2 = np.array([tr2.var_psi, tr2.var_alpha, tr2.cov])
+    assert np.abs(got2 - got).max() <= bound * np.abs(want).max()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_leave_out_kss_matches_the_oracle(oracle_mod, precision):
+    from hdfe_stream_amd.leaveout import leave_out_kss
+    from hdfe_stream_amd.simulate import kss_panel
+    O = oracle_mod
+    d = kss_panel(4000, 400, seed=3)
+    print(f"leave_out_kss precision={precision}", flush=True)
+    tol = 1e-12 if precision == "fp64" else 1e-10
+    lo = leave_out_kss("y ~ 1 | worker_id + firm_id", d, n_draws=64, seed=2,
+                       precision=precision, tol=tol)
+    ref = O.leave_out_kss(d, n_draws=64, block=64, seed=2, tol=1e-12)
+    # the pruning is integer and exact
+    assert lo.n_obs == int(ref["keep"].sum()) == ref["n"]
+    for k in ("components_before", "single_observation", "cut", "rounds"):
+        assert lo.pruning[k] == ref["prune"][k]
+    names = ["var(psi)", "var(alpha)", "cov(psi, alpha)"]
+    plug = np.array([lo.plug_in[k] for k in names])
+    assert rel(plug, ref["plug_in"]) < 1e-9
+    bound = 1e-7 if precision == "fp64" else 1e-5
+    assert rel([lo.bias[k] for k in names], ref["bias"]) < bound
+    assert rel([lo.leave_out[k] for k in names], ref["leave_out"]) < bound
+    s2 = lo.diagnostics["sigma2"].cpu().numpy()
+    assert rel(s2, ref["sigma2"]) < 1e-5
+    # movers: workers seen at more than one firm, counted by rows
+    g, f = ref["fit"]["p0"]["g"], ref["fit"]["p0"]["codes"][:, 0]
+    lo_f = np.full(g.max() + 1, np.iinfo(np.int64).max)
+    hi_f = np.full(g.max() + 1, -1)
+    np.minimum.at(lo_f, g, f)
+    np.maximum.at(hi_f, g, f)
+    assert lo.n_movers == int((lo_f != hi_f)[g].sum())
+    # the bias is removed: variances shrink, the covariance grows
+    assert lo.leave_out["var(psi)"] < lo.plug_in["var(psi)"]
+    assert lo.leave_out["var(alpha)"] < lo.plug_in["var(alpha)"]
+    assert lo.leave_out["cov(psi, alpha)"] > lo.plug_in["cov(psi, alpha)"]
+
+
+def test_unit_weights_give_the_unweighted_components():
+    from hdfe_stream_amd.leaveout import leave_out_kss
+    from hdfe_stream_amd.simulate import kss_panel
+    d = kss_panel(3000, 300, seed=4)
+    d["w"] = np.ones(len(d["y"]))
+    a = leave_out_kss("y ~ 1 | worker_id + firm_id", d, n_draws=32, seed=1,
+                      precision="fp64", tol=1e-12)
+    b = leave_out_kss("y ~ 1 | worker_id + firm_id", d, n_draws=32, seed=1,
+                      precision="fp64", tol=1e-12, weights="w")
+    for k in a.leave_out:
+        assert b.leave_out[k] == pytest.approx(a.leave_out[k], rel=1e-8, abs=1e-12)
+        assert b.plug_in[k] == pytest.approx(a.plug_in[k], rel=1e-12, abs=1e-15)
+
+
+def test_covariates_are_refused():
+    from hdfe_stream_amd.leaveout import leave_out_kss
+    from hdfe_stream_amd.simulate import kss_panel
+    d = kss_panel(500, 60, seed=4)
+    with pytest.raises(NotImplementedError):
+        leave_out_kss("y ~ x1 | worker_id + firm_id", d, n_draws=8)
