@@ -53,8 +53,9 @@ def spmm_bytes(nnz, L, k, vbytes):
 
 
 def cpu_baseline(workers_sample, firms_sample, full_workers, seed):
-    """The oracle's FP64 fit on a 1/5 sample, scaled linearly to the full
-    panel.  Returns the JSON object."""
+    """The oracle's FP64 fit on a sample of the workers (and firms, in the
+    same proportion), scaled linearly to the full panel.  Returns the JSON
+    object."""
     from hdfe_stream_amd.simulate import akm_panel
     from oracle import oracle as O
     O.build()
@@ -108,6 +109,68 @@ def jla_phase(n_workers, n_firms, draws):
     return out
 
 
+def kss_phase(n_workers, n_firms, draws):
+    """KSS leaving out an observation, end to end on the KSS panel:
+    leave-one-out pruning of (worker, firm), the fit of y ~ 1 | worker_id +
+    firm_id + year with workers streamed, `draws` JLA draws and `draws`
+    Hutchinson draws (two solves each), all blocks of `draws` right-hand
+    sides in mixed precision."""
+    import torch
+    from hdfe_stream_amd.leaveout import leave_out_kss
+    from hdfe_stream_amd.simulate import kss_panel
+    d = kss_panel(n_workers, n_firms, seed=3)
+    cols = {k: torch.from_numpy(d[k]).cuda() for k in ("worker_id", "firm_id", "year", "y")}
+    torch.cuda.synchronize()
+    tm = {}
+    t0 = time.perf_counter()
+    lo = leave_out_kss("y ~ 1 | worker_id + firm_id + year", cols, n_draws=draws, seed=0,
+                       block=draws, precision="mixed", timings=tm)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    out = {"workload": f"KSS leave-out (observation) y ~ 1 | worker_id + firm_id + year, "
+                       f"{n_workers:,} workers x 8 years, {n_firms:,} firms, p_move 0.15, "
+                       f"{draws} JLA + {draws} Hutchinson draws",
+           "wall_s": round(wall, 4), "rows_after_pruning": lo.n_obs, "movers": lo.n_movers,
+           "pruning": lo.pruning,
+           "seconds": {k: round(v, 4) for k, v in tm.items() if k.endswith("seconds")},
+           "trace_rhs_solves_per_s": round(2 * draws / tm["trace_seconds"], 1),
+           "plug_in": lo.plug_in, "bias": lo.bias, "leave_out": lo.leave_out,
+           "max_leverage": lo.diagnostics["max_leverage"]}
+    return out
+
+
+def copy_calibration(nbytes=1 << 30, reps=20):
+    """A plain copy kernel of known traffic: `nbytes` read and written per
+    launch, timed with HIP events."""
+    import ctypes as C
+    from hdfe_stream_amd import _lib
+    us = C.c_double()
+    _lib.check(_lib.load().hdfe_copy_calibrate(nbytes, reps, C.byref(us)), "hdfe_copy_calibrate")
+    return {"kernel": "k_copy_calibrate", "bytes_read": nbytes, "bytes_written": nbytes,
+            "avg_launch_us": round(us.value, 2),
+            "GBps": round(2 * nbytes / (us.value * 1e-6) / 1e9, 1)}
+
+
+def measured_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summary (FETCH_SIZE + WRITE_SIZE scaled by the copy kernel's factors),
+    or None when the summary is absent or for another kernel or workload."""
+    p = ROOT / "profiles" / "spmm_c2_traffic.txt"
+    if not p.exists():
+        return None
+    vals = {}
+    for line in p.read_text().splitlines():
+        if "=" in line and not line.startswith("#"):
+            k, v = line.split("=", 1)
+            vals[k.strip()] = v.strip()
+    if vals.get("kernel") != kernel or vals.get("workload") != workload:
+        return None
+    try:
+        return int(float(vals["traffic_bytes_per_launch"]))
+    except (KeyError, ValueError):
+        return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -120,6 +183,7 @@ def main():
     ap.add_argument("--jla-workers", type=int, default=1_000_000)
     ap.add_argument("--jla-firms", type=int, default=100_000)
     ap.add_argument("--jla-draws", type=int, default=256)
+    ap.add_argument("--no-kss", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported "
@@ -173,7 +237,9 @@ def main():
     byt = spmm_bytes(nnz, L, k, 8)
     ach = byt / avg64 / 1e9
     roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+            "frac": round(ach / HBM_PEAK_GBS, 4),
+            "traffic": measured_traffic("k_spmm_small<double,3,8>",
+                                        f"c2:{args.workers}x{args.firms}"),
             "kernel": "k_spmm_small<double,3,8>", "bytes_per_launch": int(byt),
             "avg_launch_us": round(avg64 * 1e6, 2)}
     b32 = spmm_bytes(nnz, L, k, 4)
@@ -198,14 +264,19 @@ def main():
         "roofline": roof,
         "spmm_fp32": {"kernel": "k_spmm_small<float,3,8>", "avg_launch_us": round(avg32 * 1e6, 2),
                       "bytes_per_launch": int(b32), "achieved_GBps": round(b32 / avg32 / 1e9, 1)},
+        "copy_calibration": copy_calibration(),
     }
-    if not args.no_jla:
+    if not args.no_jla or not args.no_kss:
         del results, f64, mix
+    if not args.no_jla:
         log("JLA phase")
         line["jla"] = jla_phase(args.jla_workers, args.jla_firms, args.jla_draws)
+    if not args.no_kss:
+        log("KSS phase")
+        line["kss"] = kss_phase(args.jla_workers, args.jla_firms, args.jla_draws)
     if not args.no_cpu:
-        log("CPU baseline (oracle, 1/5 sample)")
-        line["cpu_baseline"] = cpu_baseline(args.workers // 5, args.firms // 5, args.workers, seed=2)
+        log("CPU baseline (oracle, 1/8 sample)")
+        line["cpu_baseline"] = cpu_baseline(args.workers // 8, args.firms // 8, args.workers, seed=2)
     print(json.dumps(line), flush=True)
 
 

@@ -5,6 +5,15 @@
 
 namespace hdfe {
 static thread_local std::string g_error;
+
+// A plain streaming copy, 16 bytes per lane per step: the calibration
+// kernel for the HBM byte counters (its traffic is known exactly).
+__global__ void k_copy_calibrate(int64_t n16, const uint4 *__restrict__ src,
+                                 uint4 *__restrict__ dst) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n16;
+         i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
 void set_error(const std::string &msg) { g_error = msg; }
 const char *get_error() { return g_error.c_str(); }
 }  // namespace hdfe
@@ -37,6 +46,44 @@ int hdfe_device_info(char *name, int len, int64_t *free_bytes, int64_t *total_by
     HIP_TRY(hipMemGetInfo(&f, &t));
     if (free_bytes) *free_bytes = (int64_t)f;
     if (total_bytes) *total_bytes = (int64_t)t;
+    return HDFE_STATUS_OK;
+}
+
+// Copies `bytes` (a multiple of 16) between two fresh device buffers `reps`
+// times on the null stream; *avg_us: the mean launch duration from HIP
+// events.  Reads `bytes` and writes `bytes` per launch.
+int hdfe_copy_calibrate(int64_t bytes, int reps, double *avg_us) {
+    ARG_CHECK(bytes > 0 && bytes % 16 == 0 && reps >= 1 && avg_us,
+              "hdfe_copy_calibrate: bytes must be a positive multiple of 16");
+    void *a = nullptr, *b = nullptr;
+    HIP_TRY(hipMalloc(&a, bytes));
+    if (hipMalloc(&b, bytes) != hipSuccess) {
+        (void)hipFree(a);
+        hdfe::set_error("hdfe_copy_calibrate: out of device memory");
+        return HDFE_STATUS_NOMEM;
+    }
+    const int64_t n16 = bytes / 16;
+    (void)hipMemset(a, 1, bytes);
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    const int grid = 256 * 8 * 4;  // 4 workgroups of 256 per CU on 256 CUs, grid-stride
+    hipLaunchKernelGGL(hdfe::k_copy_calibrate, dim3(grid), dim3(256), 0, 0, n16,
+                       (const uint4 *)a, (uint4 *)b);
+    (void)hipEventRecord(e0, 0);
+    for (int r = 0; r < reps; ++r)
+        hipLaunchKernelGGL(hdfe::k_copy_calibrate, dim3(grid), dim3(256), 0, 0, n16,
+                           (const uint4 *)a, (uint4 *)b);
+    (void)hipEventRecord(e1, 0);
+    hipError_t err = hipEventSynchronize(e1);
+    float ms = 0.f;
+    if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    HIP_TRY(err);
+    *avg_us = 1e3 * ms / reps;
     return HDFE_STATUS_OK;
 }
 

@@ -155,6 +155,12 @@ int hdfe_panel_pass2(void *h, const double *Gamma, const double *beta, const int
 /* Free the panel, its device arrays and its matrix. */
 void hdfe_panel_destroy(void *h);
 
+/* Measurement only (no counterpart in the reference): copies `bytes` (a
+ * multiple of 16) between two fresh device buffers `reps` times with a plain
+ * streaming kernel (k_copy_calibrate) and returns the mean launch time in
+ * microseconds.  Its known traffic calibrates the HBM byte counters. */
+int hdfe_copy_calibrate(int64_t bytes, int reps, double *avg_us);
+
 /* ------------------------------------------------ leave-out row passes */
 
 /* Rademacher +/-1 draws (n, k) on the device: entry (i, j) from the
