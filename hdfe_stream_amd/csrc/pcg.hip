@@ -37,6 +37,7 @@
 #include <mutex>
 #include <vector>
 
+#include "devbuf.h"
 #include "solver.h"
 
 namespace hdfe {
@@ -46,41 +47,53 @@ constexpr int kLongThresh = 4096;  // nonzeros: rows above this are split
 constexpr int kLongChunk = 2048;   // nonzeros per workgroup of a split row
 constexpr int kMaxK = 1024;        // widest block of right-hand sides per call
 constexpr int kSmallK = 8;         // k <= 8: lanes over nonzeros; wider: over columns
+constexpr int64_t kL2Budget = 5 << 19;  // bytes of P an XCD's 4 MB L2 should hold (2.5 MB)
+constexpr int kBlkParts = 128;     // row parts per XCD of the blocked SpMM
 
 // Device-side solver state (one allocation; see State layout in Work).
 struct CgState {
     double *rz, *alpha, *beta, *bnorm, *rel;  // (k) each
     int *active;                              // (k)
     int *done, *it, *it_final;                // scalars
-    unsigned *counter;                        // 4 last-block tickets
+    unsigned *counter;                        // unused (kept for the layout)
     int k, maxiter;
     double tol;
 };
 
 // ----------------------------------------------------------------- kernels
 
-// Grid-wide "last workgroup" detection: every workgroup publishes its writes
-// and takes a ticket; the one holding the last ticket sees all partials.
-__device__ __forceinline__ bool last_block(unsigned *counter) {
-    __shared__ bool is_last;
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned t = atomicAdd(counter, 1u);
-        is_last = (t == gridDim.x - 1);
-    }
-    __syncthreads();
-    if (is_last) __threadfence();
-    return is_last;
-}
+constexpr int kFinThreads = 1024;  // threads of the one-workgroup finishing kernels
 
-// Sum part[w * k + j] over w < nw, for every column j, in a fixed order;
-// result through `out` (k values).  Called by one whole workgroup.
+// out[j] = the sum over w < nw of part[w * stride + j], for every column
+// j < k, by the whole workgroup: TJ threads across columns and TR = blockDim
+// / TJ across the partials (two accumulators each), then a fixed tree in LDS.
+// out may be shared or global memory.
 __device__ void reduce_parts(const double *part, int nw, int k, int stride, double *out) {
-    for (int j = threadIdx.x; j < k; j += blockDim.x) {
-        double s = 0.0;
-        for (int w = 0; w < nw; ++w) s += part[(int64_t)w * stride + j];
-        out[j] = s;
+    __shared__ double red[kFinThreads];
+    int tj = 1;
+    while (tj < k && tj < (int)blockDim.x) tj <<= 1;
+    const int TR = blockDim.x / tj;
+    const int c = threadIdx.x % tj, r = threadIdx.x / tj;
+    for (int j0 = 0; j0 < k; j0 += tj) {
+        const int j = j0 + c;
+        double a0 = 0.0, a1 = 0.0;
+        if (j < k) {
+            int w = r;
+            for (; w + TR < nw; w += 2 * TR) {
+                a0 += part[(int64_t)w * stride + j];
+                a1 += part[(int64_t)(w + TR) * stride + j];
+            }
+            if (w < nw) a0 += part[(int64_t)w * stride + j];
+        }
+        __syncthreads();
+        red[r * tj + c] = a0 + a1;
+        __syncthreads();
+        for (int h = TR / 2; h > 0; h >>= 1) {
+            if (r < h) red[r * tj + c] += red[(r + h) * tj + c];
+            __syncthreads();
+        }
+        if (r == 0 && j < k) out[j] = red[c];
+        __syncthreads();
     }
 }
 
@@ -108,19 +121,29 @@ __global__ __launch_bounds__(kThreads) void k_long_wide(
     __syncthreads();
     for (int j = threadIdx.x; j < k; j += blockDim.x)
         lpart[(int64_t)c * k + j] = wlds[j] + wlds[k + j] + wlds[2 * k + j] + wlds[3 * k + j];
-    if (!last_block(st.counter + 3)) return;
+}
+
+// One workgroup, after k_long_wide: each split row's AP from its chunk
+// partials in chunk order.
+template <typename T>
+__global__ __launch_bounds__(kFinThreads) void k_fin_long_wide(
+    const int *__restrict__ long_rows, const int *__restrict__ long_cptr, int nlong,
+    const double *__restrict__ lpart, T *__restrict__ AP, CgState st) {
+    if (st.done && *st.done) return;
+    const int k = st.k;
     for (int t = threadIdx.x; t < nlong * k; t += blockDim.x) {
         const int l = t / k, j = t % k;
         double s2 = 0.0;
         for (int q = long_cptr[l]; q < long_cptr[l + 1]; ++q) s2 += lpart[(int64_t)q * k + j];
         AP[(int64_t)long_rows[l] * k + j] = (T)s2;
     }
-    if (threadIdx.x == 0) st.counter[3] = 0;
 }
 
-// After the pAp sums: alpha = rz / pAp where active and pAp > 0, else 0
-// (core/solve.py:120).  One workgroup.
-__device__ void finish_alpha(const double *part, int nw, CgState st) {
+// One workgroup, after a wide SpMM: alpha = rz / pAp where active and
+// pAp > 0, else 0 (core/solve.py:120), from the nw workgroups' pAp sums.
+__global__ __launch_bounds__(kFinThreads) void k_fin_alpha(const double *__restrict__ part, int nw,
+                                                        CgState st) {
+    if (st.done && *st.done) return;
     __shared__ double tmp[kMaxK];
     reduce_parts(part, nw, st.k, st.k, tmp);
     __syncthreads();
@@ -128,7 +151,50 @@ __device__ void finish_alpha(const double *part, int nw, CgState st) {
         const double pAp = tmp[j];
         st.alpha[j] = (st.active[j] && pAp > 0.0) ? st.rz[j] / pAp : 0.0;
     }
-    if (threadIdx.x == 0) st.counter[0] = 0;
+}
+
+// One workgroup after a small-k SpMM (k_fin_small): each split row's AP from
+// its chunk partials (a wave per row and column: lanes over the chunks, then
+// the wave's fixed tree) and its P AP; with with_dot, the pAp sums of the nw
+// workgroups and alpha = rz / pAp where active and pAp > 0, else 0
+// (core/solve.py:120).
+template <typename T, int K>
+__device__ void spmm_finish(const T *__restrict__ P, T *__restrict__ AP,
+                            const double *__restrict__ part, int nw, int with_dot,
+                            const int *__restrict__ long_rows, const int *__restrict__ long_cptr,
+                            int nlong, const double *__restrict__ lpart, CgState st) {
+    const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
+    const int nwv = blockDim.x / kWave;
+    for (int q = wv; q < nlong * K; q += nwv) {
+        const int l = q / K, j = q % K;
+        double s2 = 0.0;
+        for (int c = long_cptr[l] + lane; c < long_cptr[l + 1]; c += kWave)
+            s2 += lpart[(int64_t)c * K + j];
+        s2 = wave_sum(s2);
+        if (lane == 0) AP[(int64_t)long_rows[l] * K + j] = (T)s2;
+    }
+    if (!with_dot) return;
+    __shared__ double pap[K];
+    reduce_parts(part, nw, K, K, pap);
+    __syncthreads();  // the split rows' AP, written above, are visible here
+    if (threadIdx.x < K) {
+        const int j = threadIdx.x;
+        double d = pap[j];
+        for (int l = 0; l < nlong; ++l) {
+            const int64_t row = long_rows[l];
+            d += (double)P[row * K + j] * (double)AP[row * K + j];
+        }
+        st.alpha[j] = (st.active[j] && d > 0.0) ? st.rz[j] / d : 0.0;
+    }
+}
+
+template <typename T, int K>
+__global__ __launch_bounds__(kFinThreads) void k_fin_small(
+    const T *__restrict__ P, T *__restrict__ AP, const double *__restrict__ part, int nw,
+    int with_dot, const int *__restrict__ long_rows, const int *__restrict__ long_cptr, int nlong,
+    const double *__restrict__ lpart, CgState st) {
+    if (st.done && *st.done) return;
+    spmm_finish<T, K>(P, AP, part, nw, with_dot, long_rows, long_cptr, nlong, lpart, st);
 }
 
 // AP = S P with, when with_dot, part[w, j] = sum over the workgroup's rows of
@@ -222,39 +288,127 @@ __global__ __launch_bounds__(kThreads) void k_spmm_small(
             }
         }
     }
-    if (!with_dot && nchunk == 0) return;
-    if (!last_block(st.counter + 0)) return;
-    // split rows: every thread takes a stride of each row's chunk partials,
-    // then a fixed tree; then their P AP
-    __shared__ double ldot[K];
-    if (threadIdx.x < K) ldot[threadIdx.x] = 0.0;
-    for (int l = 0; l < nlong; ++l) {
-        const int64_t row = long_rows[l];
+}
+
+// Blocked small-k SpMM, launch 1 of 2: the non-split rows' nonzeros are
+// stored again by column block (eight contiguous column ranges of equal
+// nonzero counts, block-major, rows ascending within a block, the row
+// pointers of all blocks in one array bptr (8 L + 1)).  Workgroup w < nblk
+// runs on XCD w % 8 (the dispatcher deals workgroups to the eight XCDs in
+// turn) and takes one group of ngrp groups of consecutive blocks -- the
+// XCD's group is xcd % ngrp -- over one part of the rows, so the rows of P
+// that an XCD gathers are its group's slice, which stays in that XCD's L2.
+// It writes that group's partial AP (L, K) for its rows.  Workgroups nblk..
+// take the split rows' chunks as k_spmm_small does.
+template <typename T, int K, int G>
+__global__ __launch_bounds__(kThreads) void k_spmm_blk(
+    const int *__restrict__ bptr, const int *__restrict__ bidx, const T *__restrict__ bval,
+    int64_t L, int ngrp, int nblk, const T *__restrict__ P, T *__restrict__ gpart,
+    const int *__restrict__ indptr, const int *__restrict__ indices, const T *__restrict__ val,
+    const int *__restrict__ chunk_row, const int *__restrict__ chunk_beg,
+    double *__restrict__ lpart, CgState st) {
+    if (st.done && *st.done) return;
+    __shared__ double lds[kThreads / kWave];
+    const int w = blockIdx.x;
+    if (w >= nblk) {
+        const int c = w - nblk;
+        const int row = chunk_row[c];
+        const int s = chunk_beg[c], e = min(s + kLongChunk, indptr[row + 1]);
+        T acc[K];
+#pragma unroll
+        for (int j = 0; j < K; ++j) acc[j] = T(0);
+        for (int p = s + threadIdx.x; p < e; p += blockDim.x) {
+            const int col = indices[p];
+            const T v = val[p];
+#pragma unroll
+            for (int j = 0; j < K; ++j) acc[j] += v * P[(int64_t)col * K + j];
+        }
 #pragma unroll
         for (int j = 0; j < K; ++j) {
-            double s2 = 0.0;
-            for (int q = long_cptr[l] + threadIdx.x; q < long_cptr[l + 1]; q += blockDim.x)
-                s2 += lpart[(int64_t)q * K + j];
-            s2 = block_sum<double>(s2, lds);
-            if (threadIdx.x == 0) {
-                const T ap = (T)s2;
-                AP[row * K + j] = ap;
-                ldot[j] += (double)P[row * K + j] * (double)ap;
+            double t = block_sum<double>((double)acc[j], lds);
+            if (threadIdx.x == 0) lpart[(int64_t)c * K + j] = t;
+        }
+        return;
+    }
+    const int xcd = w & 7;
+    const int grp = xcd % ngrp;
+    const int share = 8 / ngrp;               // XCDs working on one group
+    const int nparts = (nblk / 8) * share;    // row parts of one group
+    const int rp = (w >> 3) * share + xcd / ngrp;
+    const int64_t r0 = part_begin(L, rp, nparts), r1 = part_begin(L, rp + 1, nparts);
+    const int b0 = grp * share, b1 = b0 + share;
+    T *out = gpart + (int64_t)grp * L * K;
+    const int sub = threadIdx.x % G, slot = threadIdx.x / G, nslot = kThreads / G;
+    for (int64_t base = r0; base < r1; base += nslot) {
+        const int64_t row = base + slot;
+        T acc[K];
+#pragma unroll
+        for (int j = 0; j < K; ++j) acc[j] = T(0);
+        if (row < r1) {
+            for (int b = b0; b < b1; ++b) {
+                const int s = bptr[(int64_t)b * L + row], e = bptr[(int64_t)b * L + row + 1];
+                int p = s + sub;
+                for (; p + G < e; p += 2 * G) {
+                    const int c0 = bidx[p], c1 = bidx[p + G];
+                    const T v0 = bval[p], v1 = bval[p + G];
+#pragma unroll
+                    for (int j = 0; j < K; ++j)
+                        acc[j] += v0 * P[(int64_t)c0 * K + j] + v1 * P[(int64_t)c1 * K + j];
+                }
+                if (p < e) {
+                    const int c0 = bidx[p];
+                    const T v0 = bval[p];
+#pragma unroll
+                    for (int j = 0; j < K; ++j) acc[j] += v0 * P[(int64_t)c0 * K + j];
+                }
             }
         }
+#pragma unroll
+        for (int j = 0; j < K; ++j) acc[j] = group_sum<G>(acc[j]);
+        if (row < r1 && sub == 0) {
+#pragma unroll
+            for (int j = 0; j < K; ++j) out[row * K + j] = acc[j];
+        }
     }
-    __syncthreads();
+}
+
+// Blocked small-k SpMM, launch 2 of 2: AP = the sum of the groups' partials
+// in group order; with with_dot the pAp sums, and the last workgroup adds
+// the split rows and forms alpha as k_spmm_small does.
+template <typename T, int K>
+__global__ __launch_bounds__(kThreads) void k_spmm_combine(
+    int64_t L, int ngrp, const T *__restrict__ gpart, const T *__restrict__ P, T *__restrict__ AP,
+    double *__restrict__ part, int with_dot, const int *__restrict__ long_rows,
+    const int *__restrict__ long_cptr, int nlong, const double *__restrict__ lpart, CgState st) {
+    if (st.done && *st.done) return;
+    __shared__ double lds[kThreads / kWave];
+    const int nw = gridDim.x, w = blockIdx.x;
+    const int64_t r0 = part_begin(L, w, nw), r1 = part_begin(L, w + 1, nw);
+    double dot[K];
+#pragma unroll
+    for (int j = 0; j < K; ++j) dot[j] = 0.0;
+    const int64_t LK = L * K;
+    for (int64_t row = r0 + threadIdx.x; row < r1; row += blockDim.x) {
+        T acc[K];
+#pragma unroll
+        for (int j = 0; j < K; ++j) acc[j] = gpart[row * K + j];
+        for (int g = 1; g < ngrp; ++g) {
+#pragma unroll
+            for (int j = 0; j < K; ++j) acc[j] += gpart[g * LK + row * K + j];
+        }
+#pragma unroll
+        for (int j = 0; j < K; ++j) {
+            AP[row * K + j] = acc[j];
+            dot[j] += (double)P[row * K + j] * (double)acc[j];
+        }
+    }
     if (with_dot) {
 #pragma unroll
         for (int j = 0; j < K; ++j) {
-            double pAp = 0.0;
-            for (int w = threadIdx.x; w < nw; w += blockDim.x) pAp += part[(int64_t)w * K + j];
-            pAp = block_sum<double>(pAp, lds) + ldot[j];
-            if (threadIdx.x == 0)
-                st.alpha[j] = (st.active[j] && pAp > 0.0) ? st.rz[j] / pAp : 0.0;
+            double t = block_sum<double>(dot[j], lds);
+            if (threadIdx.x == 0) part[(int64_t)w * K + j] = t;
         }
     }
-    if (threadIdx.x == 0) st.counter[0] = 0;
 }
 
 // Wide k: one wave per row, lanes over columns (j = lane + 64 t, t < CPL).
@@ -322,8 +476,6 @@ __global__ __launch_bounds__(kThreads) void k_spmm_wide(
     __syncthreads();
     for (int j = threadIdx.x; j < k; j += blockDim.x)
         part[(int64_t)w * k + j] = wlds[j] + wlds[k + j] + wlds[2 * k + j] + wlds[3 * k + j];
-    if (!last_block(st.counter + 0)) return;
-    finish_alpha(part, nw, st);
 }
 
 // Column-blocked mapping for the vector kernels: TJ threads across columns,
@@ -360,9 +512,8 @@ __device__ void store_col_parts(double (&acc)[kMaxCJ], ColMap m, int k, double *
     }
 }
 
-// X += alpha P; R -= alpha AP; sums of r^2 and r^2 dinv (core/kernels_base.py:444-462).
-// The last workgroup: rel = sqrt(rr) / bnorm, active = rel > tol, stop when no
-// column is active or at maxiter; beta = rz_new / rz (core/solve.py:123-139).
+// X += alpha P; R -= alpha AP; per-workgroup sums of r^2 and r^2 dinv
+// (core/kernels_base.py:444-462), finished by k_fin_step.
 template <typename T>
 __global__ __launch_bounds__(kThreads) void k_step(
     int64_t L, T *__restrict__ X, T *__restrict__ R, const T *__restrict__ P,
@@ -391,7 +542,15 @@ __global__ __launch_bounds__(kThreads) void k_step(
     }
     store_col_parts(rr, m, k, part, 2 * (int64_t)k);
     store_col_parts(rz, m, k, part + k, 2 * (int64_t)k);
-    if (!last_block(st.counter + 1)) return;
+}
+
+// One workgroup after k_step: rel = sqrt(rr) / bnorm, active = rel > tol, stop
+// when no column is active or at maxiter; beta = rz_new / rz
+// (core/solve.py:123-139), from the nw workgroups' sums.
+__global__ __launch_bounds__(kFinThreads) void k_fin_step(const double *__restrict__ part, int nw,
+                                                       CgState st) {
+    if (*st.done) return;
+    const int k = st.k;
     __shared__ double s_rr[kMaxK];
     __shared__ double s_rz[kMaxK];
     __shared__ int any;
@@ -422,7 +581,6 @@ __global__ __launch_bounds__(kThreads) void k_step(
             *st.done = any ? 2 : 1;  // 1 converged, 2 hit maxiter
             *st.it_final = it;
         }
-        st.counter[1] = 0;
     }
 }
 
@@ -448,9 +606,7 @@ template <typename T>
 __global__ __launch_bounds__(kThreads) void k_colsum(int64_t L, int k, const T *__restrict__ A,
                                                      const T *__restrict__ B,
                                                      const T *__restrict__ d,
-                                                     double *__restrict__ part,
-                                                     double *__restrict__ out, int mode,
-                                                     unsigned *counter) {
+                                                     double *__restrict__ part) {
     const ColMap m = col_map(k);
     const int tj = threadIdx.x % m.TJ, tr = threadIdx.x / m.TJ;
     const int w = blockIdx.x, nw = gridDim.x;
@@ -468,7 +624,12 @@ __global__ __launch_bounds__(kThreads) void k_colsum(int64_t L, int k, const T *
         }
     }
     store_col_parts(acc, m, k, part, k);
-    if (!last_block(counter)) return;
+}
+
+// One workgroup after k_colsum: out[j] from the nw workgroups' sums.
+__global__ __launch_bounds__(kFinThreads) void k_fin_colsum(int k, const double *__restrict__ part,
+                                                         int nw, double *__restrict__ out,
+                                                         int mode) {
     __shared__ double tmp[kMaxK];
     reduce_parts(part, nw, k, k, tmp);
     __syncthreads();
@@ -480,7 +641,6 @@ __global__ __launch_bounds__(kThreads) void k_colsum(int64_t L, int k, const T *
         }
         out[j] = v;
     }
-    if (threadIdx.x == 0) *counter = 0;
 }
 
 // Elementwise helpers.
@@ -557,6 +717,56 @@ __global__ void k_zero_cols(int64_t L, int k, const int *__restrict__ zero, doub
 
 }  // namespace hdfe
 
+namespace hdfe {
+
+// Blocked layout of the non-split rows (see k_spmm_blk).
+__global__ void k_blk_rowcount(int64_t L, const int *__restrict__ indptr,
+                               const int *__restrict__ indices, const int *__restrict__ col_lo,
+                               int *__restrict__ cnt) {
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < L;
+         r += (int64_t)gridDim.x * blockDim.x) {
+        int c[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        const int s = indptr[r], e = indptr[r + 1];
+        if (e - s <= kLongThresh) {
+            int b = 0;
+            for (int p = s; p < e; ++p) {
+                const int col = indices[p];
+                while (b < 7 && col >= col_lo[b + 1]) ++b;
+                ++c[b];
+            }
+        }
+#pragma unroll
+        for (int b = 0; b < 8; ++b) cnt[(int64_t)b * L + r] = c[b];
+    }
+}
+
+__global__ void k_blk_fill(int64_t L, const int *__restrict__ indptr,
+                           const int *__restrict__ indices, const double *__restrict__ v64,
+                           const float *__restrict__ v32, const int *__restrict__ col_lo,
+                           const int *__restrict__ bptr, int *__restrict__ bidx,
+                           double *__restrict__ b64, float *__restrict__ b32) {
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < L;
+         r += (int64_t)gridDim.x * blockDim.x) {
+        const int s = indptr[r], e = indptr[r + 1];
+        if (e - s > kLongThresh) continue;
+        int b = 0;
+        int q = bptr[r];
+        for (int p = s; p < e; ++p) {
+            const int col = indices[p];
+            while (b < 7 && col >= col_lo[b + 1]) {
+                ++b;
+                q = bptr[(int64_t)b * L + r];
+            }
+            bidx[q] = col;
+            b64[q] = v64[p];
+            b32[q] = v32[p];
+            ++q;
+        }
+    }
+}
+
+}  // namespace hdfe
+
 // ====================================================================== host
 
 namespace hdfe {
@@ -583,11 +793,13 @@ struct Work {
     double *X64 = nullptr, *R64 = nullptr, *P64 = nullptr, *AP64 = nullptr, *B64 = nullptr;
     float *X32 = nullptr, *R32 = nullptr, *P32 = nullptr, *AP32 = nullptr;
     double *part = nullptr, *lpart = nullptr, *colbuf = nullptr;
+    void *gpart = nullptr;  // blocked SpMM: the groups' partial AP (8, L, k)
     double *sd = nullptr;  // rz alpha beta bnorm rel scale : 6k
     int *si = nullptr;     // active(k) zero(k) outer_active(k) done it it_final
     unsigned *cnt = nullptr;
     void release() {
-        void *ps[] = {X64, R64, P64, AP64, B64, X32, R32, P32, AP32, part, lpart, colbuf, sd, si, cnt};
+        void *ps[] = {X64, R64, P64, AP64, B64, X32, R32, P32, AP32, part, lpart, colbuf, gpart,
+                      sd, si, cnt};
         for (void *p : ps)
             if (p) (void)hipFree(p);
         *this = Work();
@@ -602,6 +814,13 @@ struct Matrix {
     float *val32 = nullptr, *dinv32 = nullptr;
     int nlong = 0, nchunk = 0;
     int *long_rows = nullptr, *long_cptr = nullptr, *chunk_row = nullptr, *chunk_beg = nullptr;
+    // blocked copy of the non-split rows for the small-k SpMM (k_spmm_blk)
+    bool blk = false;
+    int col_lo_h[9] = {0};
+    int *col_lo = nullptr, *bptr = nullptr, *bidx = nullptr;
+    double *bval64 = nullptr;
+    float *bval32 = nullptr;
+    int64_t bnnz = 0;
     hipStream_t stream = nullptr;  // the solver's own stream
     hipEvent_t ev = nullptr;
     Work w;
@@ -633,6 +852,7 @@ static int alloc_work(Matrix &M, int k) {
     HIP_TRY(hipMalloc(&W.part, (int64_t)4096 * 2 * k * 8));
     HIP_TRY(hipMalloc(&W.lpart, std::max<int64_t>(1, (int64_t)M.nchunk * k) * 8));
     HIP_TRY(hipMalloc(&W.colbuf, (int64_t)k * 8));
+    if (M.blk && k <= kSmallK) HIP_TRY(hipMalloc(&W.gpart, (int64_t)8 * Lk * 8));
     HIP_TRY(hipMalloc(&W.sd, (int64_t)6 * k * 8));
     HIP_TRY(hipMalloc(&W.si, ((int64_t)3 * k + 8) * 4));
     HIP_TRY(hipMalloc(&W.cnt, 8 * sizeof(unsigned)));
@@ -668,12 +888,105 @@ static int vec_grid(int64_t L, int k) {
     return grid_for(L * (int64_t)k, 2048, 1, k <= kSmallK ? 2048 : 512);
 }
 
+template <typename T>
+static const T *blk_values(const Matrix &M);
+template <>
+const double *blk_values<double>(const Matrix &M) { return M.bval64; }
+template <>
+const float *blk_values<float>(const Matrix &M) { return M.bval32; }
+
+// The blocked copy of the non-split rows (k_spmm_blk): eight column blocks
+// of about equal nonzero counts, on the matrix's stream.  S is symmetric, so
+// a column's nonzeros are counted by its row's length (ip32, host).
+static int build_blocked(Matrix &M, const std::vector<int> &ip32) {
+    const int64_t L = M.L;
+    hipStream_t ms = M.stream;
+    Temp tmp;
+    const int64_t total = (int64_t)ip32[L] - ip32[0];
+    if (total == 0) return HDFE_STATUS_OK;
+    // col_lo[b] = the first column whose prefix count reaches b / 8 of the total
+    M.col_lo_h[0] = 0;
+    int64_t acc = 0, c = 0;
+    for (int b = 1; b < 8; ++b) {
+        const int64_t target = total * b / 8;
+        while (c < L && acc < target) {
+            acc += (int64_t)ip32[c + 1] - ip32[c];
+            ++c;
+        }
+        M.col_lo_h[b] = (int)c;
+    }
+    M.col_lo_h[8] = (int)L;
+    HIP_TRY(hipMalloc(&M.col_lo, 9 * 4));
+    HIP_TRY(hipMemcpyAsync(M.col_lo, M.col_lo_h, 9 * 4, hipMemcpyHostToDevice, ms));
+    DevBuf<int> cnt;
+    DEV_ALLOC(cnt, 8 * L + 1);
+    HIP_TRY(hipMemsetAsync(cnt.p + 8 * L, 0, 4, ms));
+    hipLaunchKernelGGL(k_blk_rowcount, dim3(grid1(L)), dim3(kThreads), 0, ms, L, M.indptr,
+                       M.indices, M.col_lo, cnt.p);
+    HIP_TRY(hipMalloc(&M.bptr, (8 * L + 1) * 4));
+    HIP_TRY(exclusive_sum(tmp, cnt.p, M.bptr, 8 * L + 1, ms));
+    int last = 0;
+    HIP_TRY(scalar_to_host(&last, M.bptr + 8 * L, ms));
+    M.bnnz = last;
+    HIP_TRY(hipMalloc(&M.bidx, std::max<int64_t>(M.bnnz, 1) * 4));
+    HIP_TRY(hipMalloc(&M.bval64, std::max<int64_t>(M.bnnz, 1) * 8));
+    HIP_TRY(hipMalloc(&M.bval32, std::max<int64_t>(M.bnnz, 1) * 4));
+    hipLaunchKernelGGL(k_blk_fill, dim3(grid1(L)), dim3(kThreads), 0, ms, L, M.indptr, M.indices,
+                       M.val64, M.val32, M.col_lo, M.bptr, M.bidx, M.bval64, M.bval32);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(ms));
+    M.blk = true;
+    return HDFE_STATUS_OK;
+}
+
 // AP = S P (and, with with_dot, pAp sums and alpha).  `st.done` may be null.
 template <typename T>
 static void launch_spmm(Matrix &M, const T *val, const T *P, T *AP, double *part, double *lpart,
                         int with_dot, CgState st, hipStream_t s) {
     const int k = st.k;
     const int g = spmm_grid(M.L, k);
+    if (k <= kSmallK && M.blk && M.L * (int64_t)k * (int64_t)sizeof(T) > kL2Budget &&
+        M.w.gpart) {
+        // the fewest groups whose widest P slice fits the L2 budget
+        int ngrp = 8;
+        for (int ng = 2; ng <= 8; ng *= 2) {
+            int64_t widest = 0;
+            for (int q = 0; q < ng; ++q)
+                widest = std::max<int64_t>(
+                    widest, (int64_t)M.col_lo_h[(q + 1) * 8 / ng] - M.col_lo_h[q * 8 / ng]);
+            if (widest * k * (int64_t)sizeof(T) <= kL2Budget) {
+                ngrp = ng;
+                break;
+            }
+        }
+        const double per_row = (double)M.bnnz / ((double)std::max<int64_t>(M.L, 1) * ngrp);
+        const int G = per_row < 6.0 ? 1 : (per_row < 12.0 ? 2 : 4);
+        const int nblk = 8 * kBlkParts;
+        const T *bval = blk_values<T>(M);
+        T *gp = (T *)M.w.gpart;
+#define HDFE_BLK(KK, GG)                                                                          \
+    hipLaunchKernelGGL((k_spmm_blk<T, KK, GG>), dim3(nblk + M.nchunk), dim3(kThreads), 0, s,      \
+                       M.bptr, M.bidx, bval, M.L, ngrp, nblk, P, gp, M.indptr, M.indices, val,   \
+                       M.chunk_row, M.chunk_beg, lpart, st)
+#define HDFE_BLKG(KK)                   \
+    case KK:                            \
+        if (G == 1) HDFE_BLK(KK, 1);    \
+        else if (G == 2) HDFE_BLK(KK, 2); \
+        else HDFE_BLK(KK, 4);           \
+        hipLaunchKernelGGL((k_spmm_combine<T, KK>), dim3(g), dim3(kThreads), 0, s, M.L, ngrp, gp, \
+                           P, AP, part, with_dot, M.long_rows, M.long_cptr, M.nlong, lpart, st); \
+        if (with_dot || M.nlong)                                                                  \
+            hipLaunchKernelGGL((k_fin_small<T, KK>), dim3(1), dim3(kFinThreads), 0, s, P, AP, part, \
+                               g, with_dot, M.long_rows, M.long_cptr, M.nlong, lpart, st);        \
+        break;
+        switch (k) {
+            HDFE_BLKG(1) HDFE_BLKG(2) HDFE_BLKG(3) HDFE_BLKG(4)
+            HDFE_BLKG(5) HDFE_BLKG(6) HDFE_BLKG(7) HDFE_BLKG(8)
+        }
+#undef HDFE_BLKG
+#undef HDFE_BLK
+        return;
+    }
     if (k <= kSmallK) {
 #define HDFE_SPMM(KK)                                                                          \
     case KK:                                                                                   \
@@ -681,6 +994,10 @@ static void launch_spmm(Matrix &M, const T *val, const T *P, T *AP, double *part
                            M.indptr, M.indices, val, M.L, P, AP, part, with_dot, M.chunk_row,  \
                            M.chunk_beg, M.nchunk, M.long_rows, M.long_cptr, M.nlong, lpart,    \
                            st);                                                                \
+        if (with_dot || M.nlong)                                                               \
+            hipLaunchKernelGGL((k_fin_small<T, KK>), dim3(1), dim3(kFinThreads), 0, s, P, AP,  \
+                               part, g, with_dot, M.long_rows, M.long_cptr, M.nlong, lpart,    \
+                               st);                                                            \
         break;
         switch (k) {
             HDFE_SPMM(1) HDFE_SPMM(2) HDFE_SPMM(3) HDFE_SPMM(4)
@@ -689,10 +1006,13 @@ static void launch_spmm(Matrix &M, const T *val, const T *P, T *AP, double *part
 #undef HDFE_SPMM
         return;
     }
-    if (M.nlong)
+    if (M.nlong) {
         hipLaunchKernelGGL((k_long_wide<T>), dim3(M.nchunk), dim3(kThreads), 4 * k * 8, s,
                            M.indptr, M.indices, val, M.chunk_row, M.chunk_beg, M.nchunk,
                            M.long_rows, M.long_cptr, M.nlong, P, AP, lpart, st);
+        hipLaunchKernelGGL((k_fin_long_wide<T>), dim3(1), dim3(kFinThreads), 0, s, M.long_rows,
+                           M.long_cptr, M.nlong, lpart, AP, st);
+    }
     const int cpl = (k + 63) / 64;
 #define HDFE_WIDE(C)                                                                        \
     hipLaunchKernelGGL((k_spmm_wide<T, C>), dim3(g), dim3(kThreads), 4 * k * 8, s, M.indptr, \
@@ -703,13 +1023,17 @@ static void launch_spmm(Matrix &M, const T *val, const T *P, T *AP, double *part
     else if (cpl <= 8) HDFE_WIDE(8);
     else HDFE_WIDE(16);
 #undef HDFE_WIDE
+    if (with_dot)
+        hipLaunchKernelGGL(k_fin_alpha, dim3(1), dim3(kFinThreads), 0, s, part, g, st);
 }
 
 template <typename T>
 static void launch_colsum(int64_t L, int k, const T *A, const T *B, const T *d, double *part,
                           double *out, int mode, unsigned *counter, hipStream_t s) {
-    hipLaunchKernelGGL((k_colsum<T>), dim3(vec_grid(L, k)), dim3(kThreads), 0, s, L, k, A, B, d,
-                       part, out, mode, counter);
+    (void)counter;
+    const int g = vec_grid(L, k);
+    hipLaunchKernelGGL((k_colsum<T>), dim3(g), dim3(kThreads), 0, s, L, k, A, B, d, part);
+    hipLaunchKernelGGL(k_fin_colsum, dim3(1), dim3(kFinThreads), 0, s, k, part, g, out, mode);
 }
 
 // Iterate the CG already set up in (X, R, P, st.rz, st.active, st.bnorm);
@@ -738,6 +1062,7 @@ static int run_cg(Matrix &M, int k, const T *val, const T *dinv, T *X, T *R, T *
             if (M.timing) HIP_TRY(hipEventRecord(M.evpool[2 * b + 1], s));
             hipLaunchKernelGGL((k_step<T>), dim3(gv), dim3(kThreads), 0, s, M.L, X, R, P, AP, dinv,
                                W.part, st);
+            hipLaunchKernelGGL(k_fin_step, dim3(1), dim3(kFinThreads), 0, s, W.part, gv, st);
             hipLaunchKernelGGL((k_direction<T>), dim3(gd), dim3(kThreads), 0, s, M.L, P, R, dinv,
                                st);
         }
@@ -1007,6 +1332,10 @@ int matrix_create(int64_t L, int64_t nnz, const std::vector<int> &ip32, const in
         HIP_TRY(hipMemcpy(M->chunk_beg, cbeg.data(), M->nchunk * 4, hipMemcpyHostToDevice));
     }
     HIP_TRY(hipStreamSynchronize(M->stream));
+    if (L * (int64_t)kSmallK * 8 > kL2Budget && L < (INT32_MAX - 1) / 8) {
+        const int st = build_blocked(*M, ip32);
+        if (st) return st;
+    }
     *out = M.release();
     return HDFE_STATUS_OK;
 }
@@ -1256,7 +1585,8 @@ void hdfe_destroy(void *h) {
     if (M->stream) (void)hipStreamSynchronize(M->stream);
     M->w.release();
     void *ps[] = {M->indptr, M->indices, M->val64, M->val32, M->dinv64, M->dinv32,
-                  M->long_rows, M->long_cptr, M->chunk_row, M->chunk_beg};
+                  M->long_rows, M->long_cptr, M->chunk_row, M->chunk_beg, M->col_lo, M->bptr,
+                  M->bidx, M->bval64, M->bval32};
     for (void *p : ps)
         if (p) (void)hipFree(p);
     if (M->ev) (void)hipEventDestroy(M->ev);

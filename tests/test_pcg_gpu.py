@@ -141,3 +141,57 @@ def test_device_tensors_in_and_out(system):
     Xd, itd, ok, _ = A.solve(torch.from_numpy(b).cuda(), tol=1e-10)
     assert ok and itd == ith
     np.testing.assert_array_equal(Xd.cpu().numpy(), Xh)
+
+
+@pytest.fixture(scope="module")
+def big_system():
+    """A symmetric, diagonally dominant S at the 5M-worker panel's size
+    (400,008 levels): ~20 random couplings per level and 8 levels coupled to
+    every other (rows of 400,000 nonzeros), so that P at k >= 1 outgrows an
+    XCD's L2 and the small-k SpMM takes its column-blocked form."""
+    import scipy.sparse as sp
+    rng = np.random.default_rng(7)
+    Ls, Ld = 400_000, 8
+    L = Ls + Ld
+    r = np.repeat(np.arange(Ls), 10)
+    c = rng.integers(0, Ls, len(r))
+    v = -rng.uniform(0.1, 1.0, len(r))
+    rd = np.repeat(np.arange(Ls), Ld)
+    cd = Ls + np.tile(np.arange(Ld), Ls)
+    vd = -rng.uniform(0.01, 0.1, len(rd))
+    B = sp.csr_matrix((np.concatenate([v, vd]), (np.concatenate([r, rd]), np.concatenate([c, cd]))),
+                      shape=(L, L))
+    S = (B + B.T).tocsr()
+    S.setdiag(0.0)
+    S.eliminate_zeros()
+    S = (S + sp.diags(np.asarray(abs(S).sum(axis=1)).ravel() + 1.0)).tocsr()
+    S.sort_indices()
+    dinv = 1.0 / S.diagonal()
+    return S, dinv
+
+
+@pytest.mark.parametrize("k", [1, 3, 8])
+def test_blocked_spmm_is_the_operator(big_system, k):
+    from hdfe_stream_amd.rocm import DeviceCSR
+    S, dinv = big_system
+    A = DeviceCSR.from_scipy(S, dinv)
+    assert A.nlong() == 8
+    X = np.random.default_rng(k).standard_normal((S.shape[0], k))
+    ref = S @ X
+    np.testing.assert_allclose(A.matmat(X), ref, rtol=1e-12, atol=1e-12 * np.abs(ref).max())
+    got32 = A.matmat_f32(X.astype(np.float32))
+    assert np.max(np.abs(got32 - ref)) <= 1e-5 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_blocked_spmm_solve(oracle_mod, big_system, precision):
+    from hdfe_stream_amd.rocm import DeviceCSR
+    S, dinv = big_system
+    A = DeviceCSR.from_scipy(S, dinv)
+    b = _rhs(S, 3, seed=9)
+    Xc, itc, okc, _ = oracle_mod.pcg(S, dinv, b, None, None, 1e-10, 5000)
+    Xd, itd, okd, rel = A.solve(b, tol=1e-10, precision=precision)
+    assert okc and okd and rel <= 1e-10
+    if precision == "fp64":
+        assert abs(itc - itd) <= 1
+    assert np.max(np.abs(Xd - Xc)) <= 1e-8 * np.max(np.abs(Xc))
