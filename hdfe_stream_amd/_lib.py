@@ -69,6 +69,10 @@ SIGNATURES = [
     ("hdfe_kss_sigma2", I32, [I64, I32, P, P, P, P, P, P, P]),
     ("hdfe_kss_plugin", I32, [I64, P, P, I64, P, P, P, P]),
     ("hdfe_copy_calibrate", I32, [I64, I32, P]),
+    ("hdfe_panel_partial", I32, [P, P, P, I32, P]),
+    ("hdfe_adj_component_rows", I32, [I64, P, P, I64, P, P, P, P, P, P]),
+    ("hdfe_adj_xwr", I32, [I64, I32, P, P, I32, P, P, P]),
+    ("hdfe_adj_gradients", I32, [I64, I32, P, P, P, P, P, P, P, P]),
 ]
 
 

@@ -1172,6 +1172,35 @@ __global__ void k_pass2(int64_t G, int D, int m, const int *__restrict__ gstart,
     }
 }
 
+// The rows' columns cols (kc) of V less their fixed-effect fit: for each
+// group, u_i = V[i, col] - sum_d Gm[level_d(i), col], and out[i, c] = u_i
+// less the group's weighted mean of u (the covariates partialled on the
+// fixed effects, M_F x, from their reduced solutions in Gm (L, m)).
+__global__ void k_partial(int64_t G, int D, int m, const int *__restrict__ gstart,
+                          const int *__restrict__ codes, int64_t ld,
+                          const int64_t *__restrict__ offs, const double *__restrict__ V,
+                          const double *__restrict__ w, const double *__restrict__ Gm,
+                          const int *__restrict__ cols, int kc, double *__restrict__ out) {
+    for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < G;
+         g += (int64_t)gridDim.x * blockDim.x) {
+        const int s = gstart[g], e = gstart[g + 1];
+        for (int c = 0; c < kc; ++c) {
+            const int col = cols[c];
+            double Wg = 0.0, mu = 0.0;
+            for (int i = s; i < e; ++i) {
+                double u = V[(int64_t)i * m + col];
+                for (int d = 0; d < D; ++d) u -= Gm[(offs[d] + codes[d * ld + i]) * m + col];
+                const double wi = w ? w[i] : 1.0;
+                Wg += wi;
+                mu += wi * u;
+                out[(int64_t)i * kc + c] = u;
+            }
+            mu /= Wg;
+            for (int i = s; i < e; ++i) out[(int64_t)i * kc + c] -= mu;
+        }
+    }
+}
+
 // gam_y = Gamma[:, 0] - Gamma[:, xcols] beta
 __global__ void k_gam_y(int64_t L, int m, const double *__restrict__ Gm,
                         const double *__restrict__ beta, const int *__restrict__ xcols, int kx,
@@ -2119,6 +2148,25 @@ int hdfe_panel_pass2(void *h, const double *Gm, const double *beta, const int *x
     hipLaunchKernelGGL(k_pass2, dim3(grid1(P.G)), dim3(T256), 0, s, P.G, P.D, P.m, P.gstart.p,
                        P.codes.p, P.n, P.offs.p, P.V.p, P.weighted ? P.w.p : (const double *)nullptr,
                        gam_y, b.p, xc.p, kx, geff, resid);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(s));
+    return HDFE_STATUS_OK;
+}
+
+// out (n, kc, device) = columns cols (host, kc) of the panel's variables
+// partialled on the fixed effects, from the reduced solutions Gm (L, m).
+int hdfe_panel_partial(void *h, const double *Gm, const int *cols, int kc, double *out) {
+    ARG_CHECK(h && Gm && cols && out && kc >= 1, "hdfe_panel_partial: bad argument");
+    Panel &P = *(Panel *)h;
+    for (int c = 0; c < kc; ++c)
+        ARG_CHECK(cols[c] >= 0 && cols[c] < P.m, "hdfe_panel_partial: column out of range");
+    hipStream_t s = P.s;
+    DevBuf<int> cc;
+    DEV_ALLOC(cc, kc);
+    HIP_TRY(hipMemcpyAsync(cc.p, cols, kc * 4, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_partial, dim3(grid1(P.G)), dim3(T256), 0, s, P.G, P.D, P.m, P.gstart.p,
+                       P.codes.p, P.n, P.offs.p, P.V.p, P.weighted ? P.w.p : (const double *)nullptr,
+                       Gm, cc.p, kc, out);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(s));
     return HDFE_STATUS_OK;

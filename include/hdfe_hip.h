@@ -152,6 +152,13 @@ int hdfe_panel_assemble(void *h, const double *Gamma, double *A);
 int hdfe_panel_pass2(void *h, const double *Gamma, const double *beta, const int *xcols, int kx,
                      double *gam_y, double *geff, double *resid);
 
+/* out (n, kc, device) = the columns cols (host, kc) of the panel's variables
+ * partialled on the fixed effects (M_F x), from their reduced solutions Gm
+ * (L, m): per group, x less the levels' effects, less its weighted mean.
+ * The covariate block of the adjoint; no counterpart in the reference, whose
+ * adjoint hook is solve_normal_equations (normal_equations.py:115-231). */
+int hdfe_panel_partial(void *h, const double *Gm, const int *cols, int kc, double *out);
+
 /* Free the panel, its device arrays and its matrix. */
 void hdfe_panel_destroy(void *h);
 
@@ -269,6 +276,33 @@ int hdfe_kss_plugin(int64_t G, const int *gstart, const int *psi_codes, int64_t 
  * (core/connected.py:378-562) and its graph kernels (core/kernels_graph.py). */
 int hdfe_loo_prune(int64_t n, const int64_t *a_ids, const int64_t *b_ids, uint8_t *keep,
                    int64_t *info);
+
+/* ------------------------------------------------- adjoint (autograd)
+ * Row passes of the backward of a fitted regression: z_i'u for the adjoint
+ * solution u = (Z'WZ)^- g of a loss gradient g, and from it dL/dy_i =
+ * w_i z_i'u and dL/dw_i = e_i z_i'u.  The reference has no autograd; its
+ * adjoint hook is solve_normal_equations / predict (normal_equations.py:
+ * 65-231), whose z'u and w_i z_i'u these are. */
+
+/* Per row (device, n each): r_psi, r_alpha = the plug-in components'
+ * derivative in the row's psi and alpha effects over w_i, and direct = their
+ * derivative in w_i at fixed effects (leaveout.py:807-832).  coef (host, 10)
+ * = weights of var(psi), var(alpha), cov in the loss; the weighted means of
+ * psi and alpha; the divisor W (R-1)/R; W; the three components. */
+int hdfe_adj_component_rows(int64_t G, const int *gstart, const int *psi_codes, int64_t psi_off,
+                            const double *gam, const double *geff, const double *coef,
+                            double *r_psi, double *r_alpha, double *direct);
+
+/* out (host, kx) = X'W R over the covariate columns xcols (host) of V (n, m)
+ * and a row vector R (device). */
+int hdfe_adj_xwr(int64_t n, int m, const double *V, const int *xcols, int kx, const double *w,
+                 const double *R, double *out);
+
+/* gy, gw (n, device): zu_i = rows_i + Xt_i'u_X (rows may be NULL),
+ * gy = w zu, gw = e zu + direct (direct may be NULL); u_X host, kx <= 16. */
+int hdfe_adj_gradients(int64_t n, int kx, const double *rows, const double *Xt, const double *uX,
+                       const double *w, const double *e, const double *direct, double *gy,
+                       double *gw);
 
 #ifdef __cplusplus
 }

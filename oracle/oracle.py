@@ -29,6 +29,9 @@ the reference's FP64 algorithm:
                                       inverse.py:311-460, kernels_inverse.py:
                                       111-231, leaveout.py:321-448, 634-1016,
                                       1069-1255, leaveout_se.py:202-214
+  autograd a dense FP64 torch (CPU) restatement of the weighted fit and its
+           plug-in components, differentiated by torch's autograd: the check
+           of the device adjoint (the reference has no autograd)
 
 Each piece is pinned in tests/test_oracle.py against what the mathematics
 fixes (dense closed forms, scipy routines, brute force) and the golden
@@ -655,6 +658,44 @@ def leave_out_kss(data: dict, y: str = "y", xs=(), alpha: str = "worker_id",
             "leave_out": tuple(p - q for p, q in zip(plug, bias)), "keep": keep,
             "prune": prune, "fit": f, "sigma2": s2, "M_bar": M_bar, "factor": factor,
             "moments": moments, "totals": totals, "n": n}
+
+
+# ------------------------------------------------- differentiable fits
+
+def dense_fe_components(y, w, X, codes, psi: int = 1):
+    """[beta..., var(psi), var(alpha), cov(psi, alpha)] of the weighted least
+    squares fit of y on [X, dummies of every fixed effect] as dense FP64
+    torch operations on the CPU, so that torch's autograd differentiates it
+    in y and w.  `codes` are 0-based level codes per dimension, alpha the
+    first; one dummy of every further dimension is dropped (identification),
+    which shifts the effects by constants and leaves the components
+    unchanged.  The components are the weighted plug-in moments over rows
+    with the divisor W (R - 1) / R (leaveout.py:807-832,
+    leaveout_se.py:202-214)."""
+    import torch
+    n = y.shape[0]
+    kx = X.shape[1] if X is not None else 0
+    cols = [X] if kx else []
+    for d, c in enumerate(codes):
+        c = torch.as_tensor(np.asarray(c), dtype=torch.long)
+        D = torch.zeros((n, int(c.max()) + 1), dtype=torch.float64)
+        D[torch.arange(n), c] = 1.0
+        cols.append(D[:, 1:] if d > 0 else D)
+    Z = torch.cat(cols, dim=1)
+    A = Z.T @ (w[:, None] * Z)
+    theta = torch.linalg.solve(A, Z.T @ (w * y))
+    bounds = np.cumsum([0] + [b.shape[1] for b in cols])
+    fitted = [cols[j] @ theta[bounds[j]:bounds[j + 1]] for j in range(len(cols))]
+    off = 1 if kx else 0
+    alpha = fitted[off]
+    ps = fitted[off + psi]
+    W = w.sum()
+    Dp = W * (n - 1.0) / n
+    ma, mp = (w @ alpha) / W, (w @ ps) / W
+    vpsi = (w @ ((ps - mp) ** 2)) / Dp
+    valpha = (w @ ((alpha - ma) ** 2)) / Dp
+    cov = (w @ ((ps - mp) * (alpha - ma))) / Dp
+    return torch.cat([theta[:kx], torch.stack([vpsi, valpha, cov])])
 
 
 # ------------------------------------------------------------------ graph
