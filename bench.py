@@ -142,6 +142,71 @@ def kss_phase(n_workers, n_firms, draws):
     return out
 
 
+def adjoint_phase(n_workers, n_firms, fd_rows):
+    """The differentiable weighted fit: y ~ x1 + x2 | worker_id + firm_id +
+    year on the weighted panel (w ~ U(0.5, 2) float32), forward (the fit and
+    the plug-in components) and backward (one adjoint solve) for the loss
+    beta_1 + var(psi), timed; then central FP64 finite differences of the
+    device fit itself on `fd_rows` random rows, in w and in y."""
+    import torch
+    from hdfe_stream_amd.autograd import fe_components
+    from hdfe_stream_amd.simulate import weighted_panel
+    d = weighted_panel(n_workers, n_firms, seed=4)
+    cols = {k: torch.from_numpy(d[k]).cuda() for k in ("worker_id", "firm_id", "year", "x1", "x2")}
+    y = torch.from_numpy(d["y"]).cuda().requires_grad_(True)
+    w = torch.from_numpy(d["w"]).cuda().requires_grad_(True)
+    fml = "y ~ x1 + x2 | worker_id + firm_id + year"
+    g = torch.tensor([1.0, 0.0, 1.0, 0.0, 0.0], dtype=torch.float64, device="cuda")
+    out = fe_components(fml, cols, y=y, weights=w, tol=1e-13)      # warm
+    (out * g).sum().backward()
+    y.grad = w.grad = None
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    out = fe_components(fml, cols, y=y, weights=w, tol=1e-13)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    (out * g).sum().backward()
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    rng = np.random.default_rng(11)
+    rows = rng.choice(len(d["y"]), fd_rows, replace=False)
+    h = 0.05
+    pairs = {"w": [], "y": []}
+    for i in rows:
+        for kind in ("w", "y"):
+            vals = []
+            for sgn in (1.0, -1.0):
+                yy, ww = y.detach(), w.detach().double()
+                if kind == "w":
+                    ww = ww.clone()
+                    ww[i] += sgn * h
+                else:
+                    yy = yy.clone()
+                    yy[i] += sgn * h
+                vals.append(float((fe_components(fml, cols, y=yy, weights=ww, tol=1e-13) * g).sum()))
+            num = (vals[0] - vals[1]) / (2 * h)
+            an = float((w.grad if kind == "w" else y.grad)[i])
+            pairs[kind].append((num, an))
+
+    def err(kind):
+        # the largest FD-vs-adjoint gap relative to the gradients' RMS over
+        # the checked rows (a relative error per row is meaningless where a
+        # row's gradient is near zero)
+        a = np.array(pairs[kind])
+        rms = float(np.sqrt(np.mean(a[:, 1] ** 2)))
+        return float(np.max(np.abs(a[:, 0] - a[:, 1])) / rms), rms
+    ew, rw = err("w")
+    ey, ry = err("y")
+    return {"workload": f"weighted AKM y ~ x1 + x2 | worker_id + firm_id + year, "
+                        f"{n_workers:,} workers x 8 years, {n_firms:,} firms, w ~ U(0.5, 2) f32; "
+                        "loss beta_x1 + var(psi)",
+            "forward_s": round(t1 - t0, 4), "backward_s": round(t2 - t1, 4),
+            "outputs": [float(v) for v in out.detach().cpu()],
+            "fd_rows": int(fd_rows), "fd_step": h,
+            "fd_max_err_over_rms_w": ew, "grad_rms_w": rw,
+            "fd_max_err_over_rms_y": ey, "grad_rms_y": ry}
+
+
 def copy_calibration(nbytes=1 << 30, reps=20):
     """A plain copy kernel of known traffic: `nbytes` read and written per
     launch, timed with HIP events."""
@@ -187,6 +252,10 @@ def main():
     ap.add_argument("--jla-firms", type=int, default=100_000)
     ap.add_argument("--jla-draws", type=int, default=256)
     ap.add_argument("--no-kss", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--adjoint-workers", type=int, default=2_000_000)
+    ap.add_argument("--adjoint-firms", type=int, default=200_000)
+    ap.add_argument("--adjoint-fd-rows", type=int, default=64)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported "
@@ -270,7 +339,7 @@ def main():
                       "bytes_per_launch": int(b32), "achieved_GBps": round(b32 / avg32 / 1e9, 1)},
         "copy_calibration": copy_calibration(),
     }
-    if not args.no_jla or not args.no_kss:
+    if not (args.no_jla and args.no_kss and args.no_adjoint):
         del results, f64, mix
     if not args.no_jla:
         log("JLA phase")
@@ -278,6 +347,10 @@ def main():
     if not args.no_kss:
         log("KSS phase")
         line["kss"] = kss_phase(args.jla_workers, args.jla_firms, args.jla_draws)
+    if not args.no_adjoint:
+        log("adjoint phase")
+        line["adjoint"] = adjoint_phase(args.adjoint_workers, args.adjoint_firms,
+                                        args.adjoint_fd_rows)
     if not args.no_cpu:
         log("CPU baseline (oracle, 1/8 sample)")
         line["cpu_baseline"] = cpu_baseline(args.workers // 8, args.firms // 8, args.workers, seed=2)
