@@ -73,6 +73,16 @@ SIGNATURES = [
     ("hdfe_adj_component_rows", I32, [I64, P, P, I64, P, P, P, P, P, P]),
     ("hdfe_adj_xwr", I32, [I64, I32, P, P, I32, P, P, P]),
     ("hdfe_adj_gradients", I32, [I64, I32, P, P, P, P, P, P, P, P]),
+    ("hdfe_panel_reweight", I32, [P, P, P]),
+    ("hdfe_glm_rows", I32, [I32, P, P, P, I32, P, P, P, I32, P, P, I64, I32, P, P, P, P, P, P,
+                            P, P]),
+    ("hdfe_glm_rhs", I32, [P, I64, P, P, P, P, I32, P, P, P]),
+    ("hdfe_glm_partial", I32, [I32, P, P, P, P, P, P, I32, I32, P, P, I32, P, P, I64, I32, P, P,
+                               P, P, P, P]),
+    ("hdfe_glm_syrk", I32, [I32, P, I32, P, P, I64, P]),
+    ("hdfe_glm_coefficients", I32, [I64, I64, I32, P, P, P, P, P, P, P, P, I64, I32, P, P, P,
+                                    P]),
+    ("hdfe_glm_toward", I32, [I64, P, I32, P, P, P, P]),
 ]
 
 

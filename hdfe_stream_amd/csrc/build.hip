@@ -1345,6 +1345,139 @@ void hdfe_destroy(void *h);  // pcg.hip
 // (device).  stream_dim: -1 to choose the dimension with the most levels.
 // sort_vars: also order rows by the variables (the canonical row order of a
 // fit whose intermediates are kept, core/passes.py:340-368).
+// Pass 1 over the panel's sorted rows (V, w): means, cells, the weighted
+// cell sums, obs/cnt, identifying groups and cells (core/passes.py:397-589).
+// Run at creation and again by hdfe_panel_reweight.
+static int run_pass1(Panel &P) {
+    hipStream_t s = P.s;
+    const int64_t nk = P.n;
+    const int m = P.m;
+    const int D = P.D;
+    // pass 1: means, cells, groups
+    const int nb = 512;
+    DevBuf<double> part;
+    DEV_ALLOC(part, (int64_t)nb * std::max(m * m, m) + 64);
+    std::vector<double> hp;
+    DEV_ALLOC(P.means, m);
+    DEV_ALLOC(P.raw_ss, m);
+    hipLaunchKernelGGL(k_colsum_rows, dim3(nb), dim3(T256), 0, s, nk, m, P.V.p, part.p);
+    HIP_TRY(to_host(hp, part.p, (int64_t)nb * m, s));
+    std::vector<double> mean(m, 0.0), ss(m, 0.0);
+    for (int b = 0; b < nb; ++b)
+        for (int j = 0; j < m; ++j) mean[j] += hp[(size_t)b * m + j];
+    for (int j = 0; j < m; ++j) mean[j] /= (double)nk;
+    HIP_TRY(hipMemcpyAsync(P.means.p, mean.data(), m * 8, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_colss_rows, dim3(nb), dim3(T256), 0, s, nk, m, P.V.p, P.means.p, part.p);
+    HIP_TRY(to_host(hp, part.p, (int64_t)nb * m, s));
+    for (int b = 0; b < nb; ++b)
+        for (int j = 0; j < m; ++j) ss[j] += hp[(size_t)b * m + j];
+    HIP_TRY(hipMemcpyAsync(P.raw_ss.p, ss.data(), m * 8, hipMemcpyHostToDevice, s));
+    DevBuf<int> gh, ch, grank, crank;
+    DEV_ALLOC(gh, nk);
+    DEV_ALLOC(ch, nk);
+    DEV_ALLOC(grank, nk);
+    DEV_ALLOC(crank, nk);
+    hipLaunchKernelGGL(k_row_heads, dim3(grid1(nk)), dim3(T256), 0, s, nk, D, P.g.p, P.codes.p,
+                       nk, gh.p, ch.p);
+    HIP_TRY(inclusive_sum(P.tmp, gh.p, grank.p, nk, s));
+    HIP_TRY(inclusive_sum(P.tmp, ch.p, crank.p, nk, s));
+    int ng = 0, nc = 0;
+    HIP_TRY(scalar_to_host(&ng, grank.p + nk - 1, s));
+    HIP_TRY(scalar_to_host(&nc, crank.p + nk - 1, s));
+    ARG_CHECK(ng == P.G, "internal: group count mismatch after sorting");
+    P.C = nc;
+    DEV_ALLOC(P.gstart, ng + 1);
+    DEV_ALLOC(P.cstart, nc + 1);
+    hipLaunchKernelGGL(k_starts, dim3(grid1(nk)), dim3(T256), 0, s, nk, gh.p, grank.p, P.gstart.p);
+    hipLaunchKernelGGL(k_starts, dim3(grid1(nk)), dim3(T256), 0, s, nk, ch.p, crank.p, P.cstart.p);
+    // group starts over cells: the cell rank of each group's first row
+    DevBuf<int> cghead, cgrank;
+    DEV_ALLOC(cghead, nc);
+    DEV_ALLOC(cgrank, nc);
+    DEV_ALLOC(P.cgstart, ng + 1);
+    {
+        // a cell starts a group when its first row does
+        hipLaunchKernelGGL(k_gather<int>, dim3(grid1(nc)), dim3(T256), 0, s, (int64_t)nc,
+                           P.cstart.p, gh.p, cghead.p);
+        HIP_TRY(inclusive_sum(P.tmp, cghead.p, cgrank.p, nc, s));
+        hipLaunchKernelGGL(k_starts, dim3(grid1(nc)), dim3(T256), 0, s, (int64_t)nc, cghead.p,
+                           cgrank.p, P.cgstart.p);
+    }
+    gh.reset();
+    ch.reset();
+    grank.reset();
+    crank.reset();
+    DevBuf<double> cn, crun, csums, wpart;
+    DEV_ALLOC(cn, nc);
+    DEV_ALLOC(crun, nc);
+    DEV_ALLOC(csums, (int64_t)nc * m);
+    const int mm = m <= 8 ? m * m : 0;
+    DEV_ALLOC(wpart, (int64_t)nb * std::max(mm, 1));
+    hipLaunchKernelGGL(k_cells, dim3(nb), dim3(T256), 0, s, (int64_t)nc, m, P.cstart.p, P.V.p,
+                       P.weighted ? P.w.p : (const double *)nullptr, P.means.p, cn.p, crun.p,
+                       csums.p, wpart.p);
+    DEV_ALLOC(P.W, std::max(mm, 1));
+    if (mm) {
+        HIP_TRY(to_host(hp, wpart.p, (int64_t)nb * mm, s));
+        std::vector<double> Wh(mm, 0.0);
+        for (int b = 0; b < nb; ++b)
+            for (int t = 0; t < mm; ++t) Wh[t] += hp[(size_t)b * mm + t];
+        HIP_TRY(hipMemcpyAsync(P.W.p, Wh.data(), mm * 8, hipMemcpyHostToDevice, s));
+    }
+    const int64_t L = P.L;
+    DEV_ALLOC(P.obs, std::max<int64_t>(L, 1));
+    DEV_ALLOC(P.cnt, std::max<int64_t>(L, 1));
+    {
+        DevBuf<unsigned long long> ic;
+        DEV_ALLOC(ic, std::max<int64_t>(L, 1));
+        HIP_TRY(hipMemsetAsync(ic.p, 0, std::max<int64_t>(L, 1) * 8, s));
+        HIP_TRY(hipMemsetAsync(P.obs.p, 0, std::max<int64_t>(L, 1) * 8, s));
+        for (int t = 0; t < D; ++t)
+            hipLaunchKernelGGL(k_obs_cnt, dim3(grid1(nc, T256, 2048)), dim3(T256), 0, s,
+                               (int64_t)nc, P.cstart.p, P.codes.p + (int64_t)t * nk,
+                               P.offs_h[t], (int)P.nlev[t], cn.p, ic.p, P.obs.p,
+                               P.weighted);
+        hipLaunchKernelGGL(k_u64_to_f64, dim3(grid1(L)), dim3(T256), 0, s, L, ic.p, P.cnt.p,
+                           P.weighted ? (double *)nullptr : P.obs.p);
+        HIP_TRY(hipStreamSynchronize(s));
+    }
+    // identifying groups and their cells
+    DevBuf<int> ident, gpos, gcells, cpos;
+    DEV_ALLOC(ident, ng + 1);
+    DEV_ALLOC(gpos, ng + 1);
+    DEV_ALLOC(gcells, ng + 1);
+    DEV_ALLOC(cpos, ng + 1);
+    HIP_TRY(hipMemsetAsync(ident.p, 0, (ng + 1) * 4, s));
+    hipLaunchKernelGGL(k_group_cells, dim3(grid1(ng)), dim3(T256), 0, s, (int64_t)ng,
+                       P.cgstart.p, ident.p);
+    HIP_TRY(exclusive_sum(P.tmp, ident.p, gpos.p, ng + 1, s));
+    int Gi = 0;
+    HIP_TRY(scalar_to_host(&Gi, gpos.p + ng, s));
+    P.Gi = Gi;
+    {
+        // cells of identifying groups, as an exclusive sum over groups
+        hipLaunchKernelGGL(k_ident_ncells, dim3(grid1(ng + 1)), dim3(T256), 0, s, (int64_t)ng,
+                           P.cgstart.p, ident.p, gcells.p);
+        HIP_TRY(exclusive_sum(P.tmp, gcells.p, cpos.p, ng + 1, s));
+        int Mi = 0;
+        HIP_TRY(scalar_to_host(&Mi, cpos.p + ng, s));
+        P.M = Mi;
+    }
+    const int64_t M = P.M;
+    DEV_ALLOC(P.icodes, std::max<int64_t>(1, (int64_t)D * M));
+    DEV_ALLOC(P.in, std::max<int64_t>(M, 1));
+    DEV_ALLOC(P.isums, std::max<int64_t>(M * m, 1));
+    DEV_ALLOC(P.igstart, Gi + 1);
+    DEV_ALLOC(P.igroup, std::max(Gi, 1));
+    hipLaunchKernelGGL(k_ident_cells, dim3(grid1(ng)), dim3(T256), 0, s, (int64_t)ng, D, m,
+                       P.cgstart.p, ident.p, gpos.p, cpos.p, P.cstart.p, P.codes.p, nk, cn.p,
+                       csums.p, P.icodes.p, M, P.in.p, P.isums.p, P.igstart.p, P.igroup.p);
+    HIP_TRY(hipMemcpyAsync(P.igstart.p + Gi, cpos.p + ng, 4, hipMemcpyDeviceToDevice, s));
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(s));
+    return HDFE_STATUS_OK;
+}
+
 int hdfe_panel_create(int64_t n_rows, int nfe, const int64_t *const *fe, int m,
                       const double *const *vars, const void *weights, int weights_f32,
                       int fixef_rm_singleton, int stream_dim, int sort_vars, void **out) {
@@ -1511,128 +1644,7 @@ int hdfe_panel_create(int64_t n_rows, int nfe, const int64_t *const *fe, int m,
     keep.reset();
     codes_in.reset();
 
-    // pass 1: means, cells, groups
-    const int nb = 512;
-    DevBuf<double> part;
-    DEV_ALLOC(part, (int64_t)nb * std::max(m * m, m) + 64);
-    std::vector<double> hp;
-    DEV_ALLOC(P->means, m);
-    DEV_ALLOC(P->raw_ss, m);
-    hipLaunchKernelGGL(k_colsum_rows, dim3(nb), dim3(T256), 0, s, nk, m, P->V.p, part.p);
-    HIP_TRY(to_host(hp, part.p, (int64_t)nb * m, s));
-    std::vector<double> mean(m, 0.0), ss(m, 0.0);
-    for (int b = 0; b < nb; ++b)
-        for (int j = 0; j < m; ++j) mean[j] += hp[(size_t)b * m + j];
-    for (int j = 0; j < m; ++j) mean[j] /= (double)nk;
-    HIP_TRY(hipMemcpyAsync(P->means.p, mean.data(), m * 8, hipMemcpyHostToDevice, s));
-    hipLaunchKernelGGL(k_colss_rows, dim3(nb), dim3(T256), 0, s, nk, m, P->V.p, P->means.p, part.p);
-    HIP_TRY(to_host(hp, part.p, (int64_t)nb * m, s));
-    for (int b = 0; b < nb; ++b)
-        for (int j = 0; j < m; ++j) ss[j] += hp[(size_t)b * m + j];
-    HIP_TRY(hipMemcpyAsync(P->raw_ss.p, ss.data(), m * 8, hipMemcpyHostToDevice, s));
-    DevBuf<int> gh, ch, grank, crank;
-    DEV_ALLOC(gh, nk);
-    DEV_ALLOC(ch, nk);
-    DEV_ALLOC(grank, nk);
-    DEV_ALLOC(crank, nk);
-    hipLaunchKernelGGL(k_row_heads, dim3(grid1(nk)), dim3(T256), 0, s, nk, D, P->g.p, P->codes.p,
-                       nk, gh.p, ch.p);
-    HIP_TRY(inclusive_sum(P->tmp, gh.p, grank.p, nk, s));
-    HIP_TRY(inclusive_sum(P->tmp, ch.p, crank.p, nk, s));
-    int ng = 0, nc = 0;
-    HIP_TRY(scalar_to_host(&ng, grank.p + nk - 1, s));
-    HIP_TRY(scalar_to_host(&nc, crank.p + nk - 1, s));
-    ARG_CHECK(ng == P->G, "internal: group count mismatch after sorting");
-    P->C = nc;
-    DEV_ALLOC(P->gstart, ng + 1);
-    DEV_ALLOC(P->cstart, nc + 1);
-    hipLaunchKernelGGL(k_starts, dim3(grid1(nk)), dim3(T256), 0, s, nk, gh.p, grank.p, P->gstart.p);
-    hipLaunchKernelGGL(k_starts, dim3(grid1(nk)), dim3(T256), 0, s, nk, ch.p, crank.p, P->cstart.p);
-    // group starts over cells: the cell rank of each group's first row
-    DevBuf<int> cghead, cgrank;
-    DEV_ALLOC(cghead, nc);
-    DEV_ALLOC(cgrank, nc);
-    DEV_ALLOC(P->cgstart, ng + 1);
-    {
-        // a cell starts a group when its first row does
-        hipLaunchKernelGGL(k_gather<int>, dim3(grid1(nc)), dim3(T256), 0, s, (int64_t)nc,
-                           P->cstart.p, gh.p, cghead.p);
-        HIP_TRY(inclusive_sum(P->tmp, cghead.p, cgrank.p, nc, s));
-        hipLaunchKernelGGL(k_starts, dim3(grid1(nc)), dim3(T256), 0, s, (int64_t)nc, cghead.p,
-                           cgrank.p, P->cgstart.p);
-    }
-    gh.reset();
-    ch.reset();
-    grank.reset();
-    crank.reset();
-    DevBuf<double> cn, crun, csums, wpart;
-    DEV_ALLOC(cn, nc);
-    DEV_ALLOC(crun, nc);
-    DEV_ALLOC(csums, (int64_t)nc * m);
-    const int mm = m <= 8 ? m * m : 0;
-    DEV_ALLOC(wpart, (int64_t)nb * std::max(mm, 1));
-    hipLaunchKernelGGL(k_cells, dim3(nb), dim3(T256), 0, s, (int64_t)nc, m, P->cstart.p, P->V.p,
-                       P->weighted ? P->w.p : (const double *)nullptr, P->means.p, cn.p, crun.p,
-                       csums.p, wpart.p);
-    DEV_ALLOC(P->W, std::max(mm, 1));
-    if (mm) {
-        HIP_TRY(to_host(hp, wpart.p, (int64_t)nb * mm, s));
-        std::vector<double> Wh(mm, 0.0);
-        for (int b = 0; b < nb; ++b)
-            for (int t = 0; t < mm; ++t) Wh[t] += hp[(size_t)b * mm + t];
-        HIP_TRY(hipMemcpyAsync(P->W.p, Wh.data(), mm * 8, hipMemcpyHostToDevice, s));
-    }
-    const int64_t L = P->L;
-    DEV_ALLOC(P->obs, std::max<int64_t>(L, 1));
-    DEV_ALLOC(P->cnt, std::max<int64_t>(L, 1));
-    {
-        DevBuf<unsigned long long> ic;
-        DEV_ALLOC(ic, std::max<int64_t>(L, 1));
-        HIP_TRY(hipMemsetAsync(ic.p, 0, std::max<int64_t>(L, 1) * 8, s));
-        HIP_TRY(hipMemsetAsync(P->obs.p, 0, std::max<int64_t>(L, 1) * 8, s));
-        for (int t = 0; t < D; ++t)
-            hipLaunchKernelGGL(k_obs_cnt, dim3(grid1(nc, T256, 2048)), dim3(T256), 0, s,
-                               (int64_t)nc, P->cstart.p, P->codes.p + (int64_t)t * nk,
-                               P->offs_h[t], (int)P->nlev[t], cn.p, ic.p, P->obs.p,
-                               P->weighted);
-        hipLaunchKernelGGL(k_u64_to_f64, dim3(grid1(L)), dim3(T256), 0, s, L, ic.p, P->cnt.p,
-                           P->weighted ? (double *)nullptr : P->obs.p);
-        HIP_TRY(hipStreamSynchronize(s));
-    }
-    // identifying groups and their cells
-    DevBuf<int> ident, gpos, gcells, cpos;
-    DEV_ALLOC(ident, ng + 1);
-    DEV_ALLOC(gpos, ng + 1);
-    DEV_ALLOC(gcells, ng + 1);
-    DEV_ALLOC(cpos, ng + 1);
-    HIP_TRY(hipMemsetAsync(ident.p, 0, (ng + 1) * 4, s));
-    hipLaunchKernelGGL(k_group_cells, dim3(grid1(ng)), dim3(T256), 0, s, (int64_t)ng,
-                       P->cgstart.p, ident.p);
-    HIP_TRY(exclusive_sum(P->tmp, ident.p, gpos.p, ng + 1, s));
-    int Gi = 0;
-    HIP_TRY(scalar_to_host(&Gi, gpos.p + ng, s));
-    P->Gi = Gi;
-    {
-        // cells of identifying groups, as an exclusive sum over groups
-        hipLaunchKernelGGL(k_ident_ncells, dim3(grid1(ng + 1)), dim3(T256), 0, s, (int64_t)ng,
-                           P->cgstart.p, ident.p, gcells.p);
-        HIP_TRY(exclusive_sum(P->tmp, gcells.p, cpos.p, ng + 1, s));
-        int Mi = 0;
-        HIP_TRY(scalar_to_host(&Mi, cpos.p + ng, s));
-        P->M = Mi;
-    }
-    const int64_t M = P->M;
-    DEV_ALLOC(P->icodes, std::max<int64_t>(1, (int64_t)D * M));
-    DEV_ALLOC(P->in, std::max<int64_t>(M, 1));
-    DEV_ALLOC(P->isums, std::max<int64_t>(M * m, 1));
-    DEV_ALLOC(P->igstart, Gi + 1);
-    DEV_ALLOC(P->igroup, std::max(Gi, 1));
-    hipLaunchKernelGGL(k_ident_cells, dim3(grid1(ng)), dim3(T256), 0, s, (int64_t)ng, D, m,
-                       P->cgstart.p, ident.p, gpos.p, cpos.p, P->cstart.p, P->codes.p, nk, cn.p,
-                       csums.p, P->icodes.p, M, P->in.p, P->isums.p, P->igstart.p, P->igroup.p);
-    HIP_TRY(hipMemcpyAsync(P->igstart.p + Gi, cpos.p + ng, 4, hipMemcpyDeviceToDevice, s));
-    HIP_TRY(hipGetLastError());
-    HIP_TRY(hipStreamSynchronize(s));
+    TRY(run_pass1(*P));
     P->t_build = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     *out = P.release();
     return HDFE_STATUS_OK;
@@ -2149,6 +2161,23 @@ int hdfe_panel_pass2(void *h, const double *Gm, const double *beta, const int *x
                        P.codes.p, P.n, P.offs.p, P.V.p, P.weighted ? P.w.p : (const double *)nullptr,
                        gam_y, b.p, xc.p, kx, geff, resid);
     HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(s));
+    return HDFE_STATUS_OK;
+}
+
+// New variables V (n, m) and weights w (n), device, in the panel's sorted row
+// order, then pass 1 again; the rows, their order and the levels stay.  The
+// caller rebuilds the system (hdfe_panel_system).  For IRLS, whose every step
+// is a weighted fit with new weights and working response (core/irls.py:
+// 92-123).
+int hdfe_panel_reweight(void *h, const double *V, const double *w) {
+    ARG_CHECK(h && V && w, "hdfe_panel_reweight: null argument");
+    Panel &P = *(Panel *)h;
+    ARG_CHECK(P.weighted, "hdfe_panel_reweight: the panel was made without weights");
+    hipStream_t s = P.s;
+    HIP_TRY(hipMemcpyAsync(P.V.p, V, P.n * P.m * 8, hipMemcpyDeviceToDevice, s));
+    HIP_TRY(hipMemcpyAsync(P.w.p, w, P.n * 8, hipMemcpyDeviceToDevice, s));
+    TRY(run_pass1(P));
     HIP_TRY(hipStreamSynchronize(s));
     return HDFE_STATUS_OK;
 }

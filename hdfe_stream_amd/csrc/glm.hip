@@ -1,0 +1,576 @@
+// Poisson regression with fixed effects by IRLS on the device, many models
+// at once, for gfx950.
+//
+// Reference (hdfe_stream/):
+//   _row_terms, _nb_glm_rows       core/kernels_glm.py:99-211 (one fused row pass)
+//   out.W += V.T @ V               glm.py:990 (the dense X'WX)
+//   _wls, _update                  core/irls.py:100-144
+//   _nb_group_effects              core/kernels_glm.py:11-33
+//   _irls                          glm.py:743-838 (the loop, step-halving)
+//
+// Several models are one stacked panel: their fixed-effect levels are kept
+// apart, so the fixed-effect Gram matrix is block diagonal and one block CG
+// solves every model's [z, X] columns at once.  Rows of a model are
+// contiguous in the panel's sorted order; `rs` (models + 1) marks them.
+//
+// Each step's covariate Hessian X~'W X~ (X~ = X partialled on the fixed
+// effects) is a split-BF16 product on the matrix cores: s = sqrt(w) x~ in
+// FP32 is split s = hi + lo (two bf16), and s's' ~ hi hi' + hi lo' + lo hi'
+// (three v_mfma_f32_32x32x16_bf16, FP32 accumulation).  The coefficient
+// update is the Newton step beta += H^-1 X~'W (z~ - X~ beta) with that
+// Hessian and the FP64 score, so the converged beta solves the FP64 normal
+// equations whatever the Hessian's rounding (only the path differs from the
+// reference's, which solves A_xx beta = A_xz exactly each step).
+#include <cmath>
+#include <vector>
+
+#include "devbuf.h"
+
+namespace hdfe {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+constexpr int kGlmRowChunk = 1024;   // level chunks of the row CSR (rows.hip)
+constexpr int kSyrkThreads = 512;    // 8 waves
+constexpr int kSyrkMaxBlk = 8;       // 32x32 output blocks per wave
+
+// ------------------------------------------------------------ row pass
+
+// One IRLS row pass (core/kernels_glm.py:99-211, Poisson): per row its
+// linear predictor at (beta_m, fe, a) -- or the start log((y + ybar_m) / 2)
+// -- then mu = exp(eta), W = mu f, u = y / mu - 1, z = eta + u, and the
+// deviance and log-likelihood less its constant.  A wave per row (lanes over
+// the covariates); a workgroup per chunk of one model's rows; part[c] =
+// (dev, sum f (y eta - mu), f y, f) of chunk c (the fifth slot is
+// k_glm_lgamma's).
+__global__ __launch_bounds__(256) void k_glm_rows(
+    int nchunk, const int *__restrict__ ch_model, const int64_t *__restrict__ ch_begin,
+    const int64_t *__restrict__ ch_end, int init, const double *__restrict__ ybar,
+    const double *__restrict__ y, const float *__restrict__ X, int kx,
+    const int *__restrict__ g, const int *__restrict__ codes, int64_t ld, int D,
+    const int64_t *__restrict__ offs, const double *__restrict__ fe,
+    const double *__restrict__ a, const double *__restrict__ beta,
+    const double *__restrict__ f, double *__restrict__ w, double *__restrict__ z,
+    double *__restrict__ part) {
+    __shared__ double lds[4][4];
+    const int c = blockIdx.x;
+    const int m = ch_model[c];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    double acc[5] = {0, 0, 0, 0, 0};
+    for (int64_t i = ch_begin[c] + wv; i < ch_end[c]; i += 4) {
+        const double yi = y[i];
+        double eta;
+        if (init) {
+            eta = log(0.5 * (yi + ybar[m]));
+        } else {
+            double s = 0.0;
+            for (int j = lane; j < kx; j += 64) s += (double)X[i * kx + j] * beta[(int64_t)m * kx + j];
+            s = wave_sum(s);
+            eta = s + a[g[i]];
+            for (int d = 0; d < D; ++d) eta += fe[offs[d] + codes[d * ld + i]];
+        }
+        if (lane == 0) {
+            const double mu = exp(eta);
+            const double fi = f ? f[i] : 1.0;
+            const double u = mu > 0.0 ? yi / mu - 1.0 : 0.0;
+            const double xlogy = yi != 0.0 ? yi * log(yi) : 0.0;
+            acc[0] += fi * 2.0 * (xlogy - yi * eta - yi + mu);
+            acc[1] += fi * (yi * eta - mu);
+            acc[2] += fi * yi;
+            acc[3] += fi;
+            w[i] = mu * fi;
+            z[i] = eta + u;
+        }
+    }
+    if (lane == 0)
+        for (int q = 0; q < 4; ++q) lds[q][wv] = acc[q];
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        const int q = threadIdx.x;
+        part[(int64_t)c * 5 + q] = lds[q][0] + lds[q][1] + lds[q][2] + lds[q][3];
+    }
+}
+
+// part[c * 5 + 4] = sum of f lgamma(y + 1) over chunk c's rows (the
+// log-likelihood's constant, computed once).
+__global__ __launch_bounds__(256) void k_glm_lgamma(const int64_t *__restrict__ ch_begin,
+                                                    const int64_t *__restrict__ ch_end,
+                                                    const double *__restrict__ y,
+                                                    const double *__restrict__ f,
+                                                    double *__restrict__ part) {
+    __shared__ double lds[4];
+    const int c = blockIdx.x;
+    double s = 0.0;
+    for (int64_t i = ch_begin[c] + threadIdx.x; i < ch_end[c]; i += blockDim.x)
+        s += (f ? f[i] : 1.0) * lgamma(y[i] + 1.0);
+    s = block_sum<double>(s, lds);
+    if (threadIdx.x == 0) part[(int64_t)c * 5 + 4] = s;
+}
+
+// ----------------------------------------------- right-hand sides [z, X]
+
+// Weighted group means of v = [z, x] (G, C): a wave per group, lanes over
+// the C = kx + 1 columns.
+__global__ __launch_bounds__(256) void k_glm_group_means(
+    int64_t G, const int *__restrict__ gstart, const double *__restrict__ z,
+    const float *__restrict__ X, int kx, const double *__restrict__ w,
+    double *__restrict__ vbar) {
+    const int C = kx + 1;
+    const int lane = threadIdx.x & 63;
+    const int64_t waves = (int64_t)gridDim.x * 4;
+    for (int64_t gg = blockIdx.x * 4 + (threadIdx.x >> 6); gg < G; gg += waves) {
+        const int s = gstart[gg], e = gstart[gg + 1];
+        double W = 0.0;
+        for (int i = s; i < e; ++i) W += w[i];
+        const double inv = W > 0.0 ? 1.0 / W : 0.0;
+        for (int j = lane; j < C; j += 64) {
+            double acc = 0.0;
+            for (int i = s; i < e; ++i)
+                acc += w[i] * (j == 0 ? z[i] : (double)X[(int64_t)i * kx + j - 1]);
+            vbar[gg * C + j] = acc * inv;
+        }
+    }
+}
+
+// part[chunk, j] = sum over the chunk's rows i (of one level, the level-to-
+// rows CSR of rows.hip) of w_i (v_ij - vbar[g_i, j]): D_o' W M_0 v
+// (kernels_inverse.py:34-71 applied to the data), a wave per chunk.
+__global__ __launch_bounds__(256) void k_glm_center(
+    int64_t nchunk, const int *__restrict__ chunk_lvl, const int *__restrict__ chunk_beg,
+    const int *__restrict__ rptr, const int *__restrict__ rrow, const int *__restrict__ g,
+    const double *__restrict__ z, const float *__restrict__ X, int kx,
+    const double *__restrict__ w, const double *__restrict__ vbar, double *__restrict__ part) {
+    const int C = kx + 1;
+    const int lane = threadIdx.x & 63;
+    const int64_t waves = (int64_t)gridDim.x * 4;
+    for (int64_t c = blockIdx.x * 4 + (threadIdx.x >> 6); c < nchunk; c += waves) {
+        const int lv = chunk_lvl[c];
+        const int b = chunk_beg[c], e = min(b + kGlmRowChunk, rptr[lv + 1]);
+        for (int j = lane; j < C; j += 64) {
+            double acc = 0.0;
+            for (int p = b; p < e; ++p) {
+                const int i = rrow[p];
+                const double v = j == 0 ? z[i] : (double)X[(int64_t)i * kx + j - 1];
+                acc += w[i] * (v - vbar[(int64_t)g[i] * C + j]);
+            }
+            part[c * C + j] = acc;
+        }
+    }
+}
+
+// ------------------------------------------------------ partialled rows
+
+// For the groups of one model per workgroup (gb..ge): x~ = x less the levels'
+// effects in Gm (L, C) less its weighted group mean, likewise z~; the score
+// partial gpart[wg, j] = sum w_i x~_ij (z~_i - x~_i'beta_m); and s = sqrt(w)
+// x~ split into bf16 hi + lo, stored in rows of eight: element (p, j) of the
+// model's padded rows at ((p / 8) * Cp + j) * 8 + p % 8 from pbase.
+// A wave per group, lanes over columns (two sweeps of the group's rows: the
+// means, then the values).
+__global__ __launch_bounds__(256) void k_glm_partial(
+    const int *__restrict__ wg_model, const int64_t *__restrict__ wg_gb,
+    const int64_t *__restrict__ wg_ge, const int *__restrict__ gstart,
+    const int64_t *__restrict__ rs, const int64_t *__restrict__ pstart, int Cp,
+    const double *__restrict__ z, const float *__restrict__ X, int kx,
+    const double *__restrict__ w, const int *__restrict__ codes, int64_t ld, int D,
+    const int64_t *__restrict__ offs, const double *__restrict__ Gm,
+    const double *__restrict__ beta, __bf16 *__restrict__ hi, __bf16 *__restrict__ lo,
+    double *__restrict__ gpart) {
+    extern __shared__ double glds[];  // [4][kx]
+    const int C = kx + 1;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int m = wg_model[blockIdx.x];
+    const double *bm = beta + (int64_t)m * kx;
+    constexpr int kMaxJ = 8;  // columns per lane: C <= 512
+    double gacc[kMaxJ];
+    for (int t = 0; t < kMaxJ; ++t) gacc[t] = 0.0;
+    for (int64_t gg = wg_gb[blockIdx.x] + wv; gg < wg_ge[blockIdx.x]; gg += 4) {
+        const int s = gstart[gg], e = gstart[gg + 1];
+        double mean[kMaxJ], W = 0.0;
+        for (int t = 0; t < kMaxJ; ++t) mean[t] = 0.0;
+        for (int i = s; i < e; ++i) {
+            const double wi = w[i];
+            W += wi;
+            for (int t = 0; t < kMaxJ; ++t) {
+                const int j = lane + 64 * t;
+                if (j >= C) break;
+                double u = j == 0 ? z[i] : (double)X[(int64_t)i * kx + j - 1];
+                for (int d = 0; d < D; ++d) u -= Gm[(offs[d] + codes[d * ld + i]) * C + j];
+                mean[t] += wi * u;
+            }
+        }
+        const double inv = W > 0.0 ? 1.0 / W : 0.0;
+        for (int i = s; i < e; ++i) {
+            const double wi = w[i];
+            const double sw = sqrt(wi);
+            double v[kMaxJ];
+            double dot = 0.0, zt = 0.0;
+            for (int t = 0; t < kMaxJ; ++t) {
+                const int j = lane + 64 * t;
+                v[t] = 0.0;
+                if (j >= C) continue;
+                double u = j == 0 ? z[i] : (double)X[(int64_t)i * kx + j - 1];
+                for (int d = 0; d < D; ++d) u -= Gm[(offs[d] + codes[d * ld + i]) * C + j];
+                u -= mean[t] * inv;
+                v[t] = u;
+                if (j == 0) zt = u;
+                else dot += u * bm[j - 1];
+            }
+            dot = wave_sum(dot);
+            zt = __shfl(zt, 0, 64);
+            const double r = zt - dot;
+            const int64_t p = pstart[m] + (i - rs[m]);
+            const int64_t base = (p >> 3) * (int64_t)Cp * 8 + (p & 7);
+            for (int t = 0; t < kMaxJ; ++t) {
+                const int j = lane + 64 * t;
+                if (j < 1 || j >= C) continue;
+                gacc[t] += wi * v[t] * r;
+                const float sv = (float)(sw * v[t]);
+                const __bf16 h = (__bf16)sv;
+                const __bf16 l = (__bf16)(sv - (float)h);
+                hi[base + (int64_t)(j - 1) * 8] = h;
+                lo[base + (int64_t)(j - 1) * 8] = l;
+            }
+        }
+    }
+    // the workgroup's score partial: waves in order
+    for (int t = 0; t < kMaxJ; ++t) {
+        const int j = lane + 64 * t;
+        if (j >= 1 && j < C) glds[wv * kx + j - 1] = gacc[t];
+    }
+    __syncthreads();
+    for (int j = threadIdx.x; j < kx; j += blockDim.x)
+        gpart[(int64_t)blockIdx.x * kx + j] =
+            glds[j] + glds[kx + j] + glds[2 * kx + j] + glds[3 * kx + j];
+}
+
+// ------------------------------------------------ split-BF16 MFMA SYRK
+
+// Upper 32x32 blocks (bi <= bj) of s's over one model's padded rows
+// [kb, ke) (multiples of 16) for the blocks q0 + wave + 8 t of this launch:
+// the workgroup stages 16 rows of every column (hi and lo) in LDS, each wave
+// runs three MFMAs per block and step.  out[wg, q, lane * 16 + r]: the MFMA
+// accumulator as it lies in the lanes.
+__global__ __launch_bounds__(kSyrkThreads) void k_glm_syrk(
+    const int64_t *__restrict__ wg_kb, const int64_t *__restrict__ wg_ke, int Cp, int nb,
+    int q0, int nq, const __bf16 *__restrict__ hi, const __bf16 *__restrict__ lo,
+    float *__restrict__ out) {
+    extern __shared__ __bf16 slds[];  // [2: hi, lo][2 row groups][Cp][8]
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int64_t slice = (int64_t)2 * Cp * 8;  // bf16 per 16 rows of one array
+    __bf16 *sh = slds, *sl = slds + slice;
+    int bi[kSyrkMaxBlk], bj[kSyrkMaxBlk];
+    int nmine = 0;
+#pragma unroll
+    for (int t = 0; t < kSyrkMaxBlk; ++t) {
+        const int q = q0 + wv + 8 * t;
+        bi[t] = bj[t] = 0;
+        if (wv + 8 * t < nq) {
+            // q -> (bi, bj), bi <= bj, row-major over the upper triangle
+            int r = 0, qq = q;
+            while (qq >= nb - r) {
+                qq -= nb - r;
+                ++r;
+            }
+            bi[t] = r;
+            bj[t] = r + qq;
+            nmine = t + 1;
+        }
+    }
+    f32x16 acc[kSyrkMaxBlk];
+#pragma unroll
+    for (int t = 0; t < kSyrkMaxBlk; ++t)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) acc[t][r] = 0.f;
+    const int64_t kb = wg_kb[blockIdx.x], ke = wg_ke[blockIdx.x];
+    const int nvec = (int)(slice / 8);  // 16-byte vectors per array slice
+    for (int64_t k0 = kb; k0 < ke; k0 += 16) {
+        const uint4 *gh = (const uint4 *)(hi + (k0 >> 3) * (int64_t)Cp * 8);
+        const uint4 *gl = (const uint4 *)(lo + (k0 >> 3) * (int64_t)Cp * 8);
+        __syncthreads();
+        for (int v = threadIdx.x; v < nvec; v += blockDim.x) {
+            ((uint4 *)sh)[v] = gh[v];
+            ((uint4 *)sl)[v] = gl[v];
+        }
+        __syncthreads();
+        const int rg = lane >> 5, col = lane & 31;
+#pragma unroll
+        for (int t = 0; t < kSyrkMaxBlk; ++t) {
+            if (t >= nmine) continue;
+            const int64_t oa = ((int64_t)rg * Cp + bi[t] * 32 + col) * 8;
+            const int64_t ob = ((int64_t)rg * Cp + bj[t] * 32 + col) * 8;
+            const bf16x8 ah = *(const bf16x8 *)(sh + oa);
+            const bf16x8 al = *(const bf16x8 *)(sl + oa);
+            const bf16x8 bh = *(const bf16x8 *)(sh + ob);
+            const bf16x8 bl = *(const bf16x8 *)(sl + ob);
+            acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh, acc[t], 0, 0, 0);
+            acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl, acc[t], 0, 0, 0);
+            acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh, acc[t], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < kSyrkMaxBlk; ++t) {
+        if (t >= nmine) continue;
+        float *o = out + ((int64_t)blockIdx.x * nq + wv + 8 * t) * 1024 + lane * 16;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) o[r] = acc[t][r];
+    }
+}
+
+// A (nb 32, nb 32) of model m: the sum over its workgroups (wb..we) of the
+// blocks' accumulators, in workgroup order, in FP64; element (row, col) of
+// block (bi, bj) from lane col + 32 (row / 4 % 2), register 4 (row / 8) +
+// row % 4, and the transposed block below the diagonal.
+__global__ void k_glm_syrk_reduce(int wb, int we, int nq_launch, int q0, int nq, int nb,
+                                  const float *__restrict__ part, double *__restrict__ A) {
+    const int Cp = nb * 32;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < (int64_t)nq * 1024;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int ql = (int)(e / 1024);
+        const int idx = (int)(e % 1024);
+        const int lane = idx / 16, r = idx % 16;
+        double s = 0.0;
+        for (int wg = wb; wg < we; ++wg) s += (double)part[((int64_t)wg * nq_launch + ql) * 1024 + idx];
+        int q = q0 + ql, bi = 0;
+        while (q >= nb - bi) {
+            q -= nb - bi;
+            ++bi;
+        }
+        const int bj = bi + q;
+        const int row = 8 * (r / 4) + 4 * (lane / 32) + (r % 4);
+        const int col = lane % 32;
+        // a diagonal block holds (row, col) and (col, row) from different
+        // accumulation orders: its upper half alone is written, both ways
+        if (bi == bj && row > col) continue;
+        const int R = bi * 32 + row, Cc = bj * 32 + col;
+        A[(int64_t)R * Cp + Cc] = s;
+        A[(int64_t)Cc * Cp + R] = s;
+    }
+}
+
+// ------------------------------------------------------- coefficients
+
+// fe[l] = Gm[l, 0] - Gm[l, 1:] beta_{model(l)} (core/irls.py:135).
+__global__ void k_glm_fe(int64_t L, int kx, const double *__restrict__ Gm,
+                         const int *__restrict__ lmodel, const double *__restrict__ beta,
+                         double *__restrict__ fe) {
+    const int C = kx + 1;
+    for (int64_t l = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; l < L;
+         l += (int64_t)gridDim.x * blockDim.x) {
+        const double *bm = beta + (int64_t)lmodel[l] * kx;
+        double v = Gm[l * C];
+        for (int j = 0; j < kx; ++j) v -= Gm[l * C + 1 + j] * bm[j];
+        fe[l] = v;
+    }
+}
+
+// a[g] = the weighted group mean of z - x'beta_m - sum_d fe (the group's own
+// effect, core/kernels_glm.py:11-33): a wave per group.
+__global__ __launch_bounds__(256) void k_glm_group_effects(
+    int64_t G, const int *__restrict__ gstart, const int *__restrict__ gmodel,
+    const double *__restrict__ z, const float *__restrict__ X, int kx,
+    const double *__restrict__ w, const int *__restrict__ codes, int64_t ld, int D,
+    const int64_t *__restrict__ offs, const double *__restrict__ fe,
+    const double *__restrict__ beta, double *__restrict__ a) {
+    const int lane = threadIdx.x & 63;
+    const int64_t waves = (int64_t)gridDim.x * 4;
+    for (int64_t gg = blockIdx.x * 4 + (threadIdx.x >> 6); gg < G; gg += waves) {
+        const double *bm = beta + (int64_t)gmodel[gg] * kx;
+        double W = 0.0, S = 0.0;
+        for (int i = gstart[gg]; i < gstart[gg + 1]; ++i) {
+            double s = 0.0;
+            for (int j = lane; j < kx; j += 64) s += (double)X[(int64_t)i * kx + j] * bm[j];
+            s = wave_sum(s);
+            double u = z[i] - s;
+            for (int d = 0; d < D; ++d) u -= fe[offs[d] + codes[d * ld + i]];
+            W += w[i];
+            S += w[i] * u;
+        }
+        if (lane == 0) a[gg] = W > 0.0 ? S / W : 0.0;
+    }
+}
+
+// x = base + t_model (target - base), per entry's model (step-halving,
+// glm.py:810-823): fe by level, a by group, beta by model (kx each).
+__global__ void k_glm_toward(int64_t n, const int *__restrict__ model, int per,
+                             const double *__restrict__ t, const double *__restrict__ base,
+                             const double *__restrict__ target, double *__restrict__ out) {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = e / per;
+        const double tm = t[model ? model[i] : i];
+        out[e] = base[e] + tm * (target[e] - base[e]);
+    }
+}
+
+}  // namespace hdfe
+
+using namespace hdfe;
+
+extern "C" {
+
+int hdfe_rows_csr(void *h, const int **rptr, const int **rrow, const int **cptr, const int **clvl,
+                  const int **cbeg, int64_t *nchunk);
+int hdfe_rows_chunk_sum(void *h, int k, const double *part, double *out);
+
+// One IRLS row pass over nchunk chunks (model, rows [begin, end)), host
+// chunk lists; device in/out.  part (host, nchunk x 5): deviance, sum f (y
+// eta - mu), f y, f, and with init the sum of f lgamma(y + 1).
+int hdfe_glm_rows(int nchunk, const int *ch_model, const int64_t *ch_begin,
+                  const int64_t *ch_end, int init, const double *ybar, const double *y,
+                  const float *X, int kx, const int *g, const int *codes, int64_t ld, int D,
+                  const int64_t *offs, const double *fe, const double *a, const double *beta,
+                  const double *f, double *w, double *z, double *part_h) {
+    ARG_CHECK(nchunk >= 1 && y && w && z && part_h && (init || (X || kx == 0)),
+              "hdfe_glm_rows: bad argument");
+    DevBuf<int> cm;
+    DevBuf<int64_t> cb, ce;
+    DevBuf<double> part;
+    DEV_ALLOC(cm, nchunk);
+    DEV_ALLOC(cb, nchunk);
+    DEV_ALLOC(ce, nchunk);
+    DEV_ALLOC(part, (int64_t)nchunk * 5);
+    HIP_TRY(hipMemcpy(cm.p, ch_model, nchunk * 4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(cb.p, ch_begin, nchunk * 8, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(ce.p, ch_end, nchunk * 8, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemset(part.p, 0, (int64_t)nchunk * 5 * 8));
+    hipLaunchKernelGGL(k_glm_rows, dim3(nchunk), dim3(256), 0, 0, nchunk, cm.p, cb.p, ce.p, init,
+                       ybar, y, X, kx, g, codes, ld, D, offs, fe, a, beta, f, w, z, part.p);
+    if (init)
+        hipLaunchKernelGGL(k_glm_lgamma, dim3(nchunk), dim3(256), 0, 0, cb.p, ce.p, y, f, part.p);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpy(part_h, part.p, (int64_t)nchunk * 5 * 8, hipMemcpyDeviceToHost));
+    return HDFE_STATUS_OK;
+}
+
+// rhs (L, kx + 1) = D_o' W M_0 [z, X] (device), with vbar (G, kx + 1) scratch.
+int hdfe_glm_rhs(void *rows_h, int64_t G, const int *gstart, const int *g, const double *z,
+                 const float *X, int kx, const double *w, double *vbar, double *rhs) {
+    ARG_CHECK(rows_h && gstart && g && z && w && vbar && rhs && kx >= 0 && kx + 1 <= 512,
+              "hdfe_glm_rhs: bad argument");
+    const int C = kx + 1;
+    const int *rptr, *rrow, *cptr, *clvl, *cbeg;
+    int64_t nchunk;
+    int st = hdfe_rows_csr(rows_h, &rptr, &rrow, &cptr, &clvl, &cbeg, &nchunk);
+    if (st) return st;
+    hipLaunchKernelGGL(k_glm_group_means, dim3(grid1(G * 64)), dim3(256), 0, 0, G, gstart, z, X,
+                       kx, w, vbar);
+    DevBuf<double> part;
+    DEV_ALLOC(part, nchunk * (int64_t)C);
+    hipLaunchKernelGGL(k_glm_center, dim3(grid1(nchunk * 64)), dim3(256), 0, 0, nchunk, clvl,
+                       cbeg, rptr, rrow, g, z, X, kx, w, vbar, part.p);
+    HIP_TRY(hipGetLastError());
+    st = hdfe_rows_chunk_sum(rows_h, C, part.p, rhs);
+    if (st) return st;
+    HIP_TRY(hipDeviceSynchronize());
+    return HDFE_STATUS_OK;
+}
+
+// The partialled rows (split bf16, hi and lo (npad / 8, Cp, 8), device) and
+// the score partials of nwg workgroups (host lists: model, group range);
+// gpart (host, nwg x kx).
+int hdfe_glm_partial(int nwg, const int *wg_model, const int64_t *wg_gb, const int64_t *wg_ge,
+                     const int *gstart, const int64_t *rs_h, const int64_t *pstart_h, int nmodels,
+                     int Cp, const double *z, const float *X, int kx, const double *w,
+                     const int *codes, int64_t ld, int D, const int64_t *offs, const double *Gm,
+                     const double *beta, void *hi, void *lo, double *gpart_h) {
+    ARG_CHECK(nwg >= 1 && kx >= 1 && kx + 1 <= 512 && Cp >= kx && Cp % 32 == 0,
+              "hdfe_glm_partial: bad argument");
+    DevBuf<int> wm;
+    DevBuf<int64_t> gb, ge, rs, ps;
+    DevBuf<double> gpart;
+    DEV_ALLOC(wm, nwg);
+    DEV_ALLOC(gb, nwg);
+    DEV_ALLOC(ge, nwg);
+    DEV_ALLOC(rs, nmodels + 1);
+    DEV_ALLOC(ps, nmodels + 1);
+    DEV_ALLOC(gpart, (int64_t)nwg * kx);
+    HIP_TRY(hipMemcpy(wm.p, wg_model, nwg * 4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(gb.p, wg_gb, nwg * 8, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(ge.p, wg_ge, nwg * 8, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(rs.p, rs_h, (nmodels + 1) * 8, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(ps.p, pstart_h, (nmodels + 1) * 8, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_glm_partial, dim3(nwg), dim3(256), 4 * kx * 8, 0, wm.p, gb.p, ge.p,
+                       gstart, rs.p, ps.p, Cp, z, X, kx, w, codes, ld, D, offs, Gm, beta,
+                       (__bf16 *)hi, (__bf16 *)lo, gpart.p);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpy(gpart_h, gpart.p, (int64_t)nwg * kx * 8, hipMemcpyDeviceToHost));
+    return HDFE_STATUS_OK;
+}
+
+// A (nmodels, Cp, Cp, device FP64) = s's per model from the split rows:
+// model m's padded rows [pstart[m], pstart[m + 1]) (multiples of 16) in
+// workgroups of at most `rows_per_wg` rows.
+int hdfe_glm_syrk(int nmodels, const int64_t *pstart_h, int Cp, const void *hi, const void *lo,
+                  int64_t rows_per_wg, double *A) {
+    ARG_CHECK(nmodels >= 1 && Cp % 32 == 0 && Cp >= 32 && rows_per_wg >= 16 &&
+                  rows_per_wg % 16 == 0 && A,
+              "hdfe_glm_syrk: bad argument");
+    const int nb = Cp / 32;
+    const int nblk = nb * (nb + 1) / 2;
+    std::vector<int64_t> kb, ke;
+    std::vector<int> wb(nmodels + 1, 0);
+    for (int m = 0; m < nmodels; ++m) {
+        ARG_CHECK(pstart_h[m] % 16 == 0 && pstart_h[m + 1] % 16 == 0,
+                  "hdfe_glm_syrk: model rows must start at multiples of 16");
+        for (int64_t k = pstart_h[m]; k < pstart_h[m + 1]; k += rows_per_wg) {
+            kb.push_back(k);
+            ke.push_back(std::min(pstart_h[m + 1], k + rows_per_wg));
+        }
+        wb[m + 1] = (int)kb.size();
+    }
+    const int nwg = (int)kb.size();
+    HIP_TRY(hipMemset(A, 0, (int64_t)nmodels * Cp * Cp * 8));
+    if (nwg == 0) return HDFE_STATUS_OK;
+    DevBuf<int64_t> dkb, dke;
+    DEV_ALLOC(dkb, nwg);
+    DEV_ALLOC(dke, nwg);
+    HIP_TRY(hipMemcpy(dkb.p, kb.data(), nwg * 8, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dke.p, ke.data(), nwg * 8, hipMemcpyHostToDevice));
+    const int per_launch = 8 * kSyrkMaxBlk;
+    DevBuf<float> part;
+    DEV_ALLOC(part, (int64_t)nwg * per_launch * 1024);
+    const size_t lds = (size_t)2 * 2 * Cp * 8 * sizeof(__bf16);
+    for (int q0 = 0; q0 < nblk; q0 += per_launch) {
+        const int nq = std::min(per_launch, nblk - q0);
+        hipLaunchKernelGGL(k_glm_syrk, dim3(nwg), dim3(kSyrkThreads), lds, 0, dkb.p, dke.p, Cp, nb,
+                           q0, nq, (const __bf16 *)hi, (const __bf16 *)lo, part.p);
+        HIP_TRY(hipGetLastError());
+        for (int m = 0; m < nmodels; ++m) {
+            if (wb[m + 1] == wb[m]) continue;
+            hipLaunchKernelGGL(k_glm_syrk_reduce, dim3(grid1((int64_t)nq * 1024)), dim3(256), 0, 0,
+                               wb[m], wb[m + 1], nq, q0, nq, nb, part.p,
+                               A + (int64_t)m * Cp * Cp);
+        }
+        HIP_TRY(hipGetLastError());
+    }
+    HIP_TRY(hipDeviceSynchronize());
+    return HDFE_STATUS_OK;
+}
+
+// fe (L) and the group effects a (G) of new slopes beta (nmodels, kx, device).
+int hdfe_glm_coefficients(int64_t L, int64_t G, int kx, const double *Gm, const int *lmodel,
+                          const int *gmodel, const int *gstart, const double *z, const float *X,
+                          const double *w, const int *codes, int64_t ld, int D,
+                          const int64_t *offs, const double *beta, double *fe, double *a) {
+    hipLaunchKernelGGL(k_glm_fe, dim3(grid1(L)), dim3(256), 0, 0, L, kx, Gm, lmodel, beta, fe);
+    hipLaunchKernelGGL(k_glm_group_effects, dim3(grid1(G * 64)), dim3(256), 0, 0, G, gstart,
+                       gmodel, z, X, kx, w, codes, ld, D, offs, fe, beta, a);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipDeviceSynchronize());
+    return HDFE_STATUS_OK;
+}
+
+// out = base + t[model] (target - base) over n entries of `per` values each
+// (model NULL: entry i's own t[i]).
+int hdfe_glm_toward(int64_t n, const int *model, int per, const double *t, const double *base,
+                    const double *target, double *out) {
+    hipLaunchKernelGGL(k_glm_toward, dim3(grid1(n * per)), dim3(256), 0, 0, n * per, model, per,
+                       t, base, target, out);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipDeviceSynchronize());
+    return HDFE_STATUS_OK;
+}
+
+}  // extern "C"

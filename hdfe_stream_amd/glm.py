@@ -1,0 +1,491 @@
+"""Poisson regression with high-dimensional fixed effects on the device:
+`fepois_stream` (api.py:144-240) and `fepois_batch`, many independent
+models in one go.
+
+The algorithm is the reference's IRLS (glm.py:743-838, core/irls.py):
+Poisson's start eta = log((y + ybar) / 2), then per step one fused row pass
+(mu, W = mu, working response z = eta + y / mu - 1, deviance), one weighted
+fit of [z, X] on the fixed effects (diag, S and the reduced right-hand sides
+from the new weights, a block CG with the previous solution as the start),
+the slopes, the fixed effects and the group effects; a step that raises the
+deviance is halved back toward the current coefficients (up to 30 times);
+convergence when |dev - dev_old| / (0.1 + |dev|) < iwls_tol, or when a later
+step no longer lowers the deviance.  The separation check drops the levels
+whose outcome is zero on every row, one pass over the dimensions
+(glm.py:319-370), before the singleton drop.
+
+Different from the reference, on purpose, for the device:
+- every model of a batch is one stacked panel whose levels are kept apart
+  per model: one S (block diagonal), one block CG for all models' [z, X]
+  columns, one launch per row pass;
+- the slopes' Hessian X~'W X~ is a split-BF16 product on the matrix cores
+  and the slope update the Newton step beta += H^-1 X~'W (z~ - X~ beta) with
+  the FP64 score: the converged slopes solve the FP64 normal equations (the
+  steps' path differs from the reference's exact A_xx beta = A_xz by the
+  Hessian's rounding);
+- each model stops at its own convergence (its coefficients are frozen while
+  the others go on).
+
+Not mirrored: logit/probit, offsets, prior weights, standard errors, the
+bias correction, Newton's method for models with one fixed effect.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+
+MAX_HALVINGS = 30          # glm.py:96
+DEV_ROUNDING = 1e-12       # glm.py:100
+
+
+@dataclass
+class GLMResult:
+    """One model's Poisson fit."""
+    coefnames: list
+    beta: np.ndarray
+    deviance: float
+    loglik: float
+    iterations: int
+    converged: bool
+    n_obs: int
+    fe_names: list
+    fe_levels: dict
+    fe_values: dict
+    deviance_path: list = field(default_factory=list)
+    diagnostics: dict = field(default_factory=dict)
+    eta_source: object = None       # linear predictor per input row (NaN if dropped)
+
+    def coef(self):
+        return dict(zip(self.coefnames, self.beta.tolist()))
+
+    def fixef(self, dim):
+        return self.fe_levels[dim], self.fe_values[dim]
+
+
+def _design(fml, data):
+    """(y name, covariate blocks [(names, 2-D array)], fe names) of one model:
+    a covariate term may be a 2-D column (n, k), named term[0..k-1]."""
+    from .estimator import _column, parse_formula
+    yname, xs, fe = parse_formula(fml)
+    blocks = []
+    for x in xs:
+        col = _column(data, x)
+        arr = col
+        if getattr(arr, "ndim", 1) == 2:
+            k = arr.shape[1]
+            blocks.append(([f"{x}[{j}]" for j in range(k)], arr))
+        else:
+            blocks.append(([x], arr))
+    return yname, blocks, fe
+
+
+def fepois_batch(fml: str, datasets: list, iwls_tol: float = 1e-8, iwls_maxiter: int = 25,
+                 separation_check: bool = True, tol: float = 1e-10, maxiter: int = 5000,
+                 precision: str = "mixed", rows_per_syrk: int = 32768,
+                 timings: dict | None = None) -> list:
+    """Fit `fml` by Poisson IRLS on each dataset, all on the device at once.
+    Returns one GLMResult per dataset."""
+    import torch
+
+    from .estimator import Panel, _column
+    from .leaveout import RowPasses
+    if not 0 < iwls_tol < 1:
+        raise ValueError("iwls_tol must be in (0, 1)")
+    if iwls_maxiter < 1:
+        raise ValueError("iwls_maxiter must be at least 1")
+    lib = _lib.load()
+    t_start = time.perf_counter()
+    dev = "cuda"
+    M = len(datasets)
+    if M < 1:
+        raise ValueError("no data")
+    per = []
+    for d in datasets:
+        yname, blocks, fe = _design(fml, d)
+        if len(fe) < 2:
+            raise ValueError("device Poisson fits need at least two fixed effects")
+        y = torch.as_tensor(np.asarray(_column(d, yname)) if not torch.is_tensor(
+            _column(d, yname)) else _column(d, yname)).to(dev, torch.float64)
+        if float(y.min()) < 0:
+            raise ValueError("Poisson regression needs a nonnegative dependent variable; "
+                             f"its minimum is {float(y.min())}")
+        if float(y.max()) <= 0:
+            raise ValueError("the dependent variable is zero on every row")
+        xs = [torch.as_tensor(b).to(dev, torch.float32).reshape(len(y), -1)
+              for _, b in blocks]
+        X = torch.cat(xs, dim=1).contiguous() if xs else torch.zeros((len(y), 0), device=dev)
+        ids = [torch.as_tensor(np.asarray(_column(d, f)) if not torch.is_tensor(_column(d, f))
+                               else _column(d, f)).to(dev, torch.int64) for f in fe]
+        names = [n for nm, _ in blocks for n in nm]
+        per.append({"y": y, "X": X, "ids": ids, "names": names, "fe": fe, "n_in": len(y)})
+    kx = per[0]["X"].shape[1]
+    if any(p["X"].shape[1] != kx for p in per):
+        raise ValueError("every model of a batch needs the same covariates")
+    if kx < 1 or kx > 511:
+        raise ValueError("1 to 511 covariates")
+    fe_names = per[0]["fe"]
+    nfe = len(fe_names)
+    # separation: levels whose outcome is zero on every row, one pass over
+    # the dimensions on the rows still kept (glm.py:346-367)
+    for p in per:
+        keep = torch.ones(len(p["y"]), dtype=torch.bool, device=dev)
+        dropped = {}
+        if separation_check and bool((p["y"] == 0).any()):
+            for d, f in enumerate(fe_names):
+                u, inv = torch.unique(p["ids"][d], return_inverse=True)
+                ymax = torch.zeros(len(u), dtype=torch.float64, device=dev)
+                ymax.scatter_reduce_(0, inv[keep], p["y"][keep], "amax", include_self=True)
+                seen = torch.zeros(len(u), dtype=torch.bool, device=dev)
+                seen[inv[keep]] = True
+                bad = seen & (ymax == 0)
+                if bool(bad.any()):
+                    dropped[f] = int(bad.sum())
+                    keep &= ~bad[inv]
+        p["keep"] = keep
+        p["separation"] = {"checked": bool(separation_check), "levels": dropped,
+                           "observations": int((~keep).sum())}
+    # the stacked panel: every dimension's codes offset per model
+    stacked, src_model, src_row = [[] for _ in range(nfe)], [], []
+    offs_model = np.zeros((M, nfe), np.int64)
+    level_ids = [[None] * nfe for _ in range(M)]
+    tot = np.zeros(nfe, np.int64)
+    for m, p in enumerate(per):
+        rows = torch.nonzero(p["keep"], as_tuple=True)[0]
+        for d in range(nfe):
+            u, inv = torch.unique(p["ids"][d][rows], return_inverse=True)
+            level_ids[m][d] = u
+            offs_model[m, d] = tot[d]
+            stacked[d].append(inv + int(tot[d]))
+            tot[d] += len(u)
+        src_model.append(torch.full((len(rows),), m, dtype=torch.int32, device=dev))
+        src_row.append(rows)
+    fe_cols = [torch.cat(s) for s in stacked]
+    smodel = torch.cat(src_model)
+    y_all = torch.cat([p["y"][r] for p, r in zip(per, src_row)])
+    X_all = torch.cat([p["X"][r] for p, r in zip(per, src_row)])
+    n_stack = len(y_all)
+    ones = torch.ones(n_stack, dtype=torch.float64, device=dev)
+    panel = Panel(fe_cols, [y_all], weights=ones, fixef_rm="singleton", stream_dim=0)
+    info = panel.info
+    n, G, L, D = info["n"], info["G"], info["L"], info["D"]
+    perm = panel.tensor("perm", torch.int32).long()[:n]
+    rmodel = smodel[perm].contiguous()
+    y_s = y_all[perm].contiguous()
+    X_s = X_all[perm].contiguous()
+    del X_all
+    rm_host = rmodel.cpu().numpy()
+    rs = np.searchsorted(rm_host, np.arange(M + 1), side="left").astype(np.int64)
+    if np.any(np.diff(rm_host) < 0):
+        raise RuntimeError("internal: the stacked panel's rows are not grouped by model")
+    gstart_t = panel.tensor("gstart", torch.int32)
+    gs_host = gstart_t.cpu().numpy().astype(np.int64)
+    gmodel = rmodel[gstart_t[:-1].long()].contiguous()
+    gb_model = np.searchsorted(gs_host[:-1], rs, side="left").astype(np.int64)  # first group per model
+    codes_t = panel.tensor("codes", torch.int32).view(D, n)
+    lmodel = torch.zeros(L, dtype=torch.int32, device=dev)
+    for t in range(D):
+        lmodel[int(panel.offs[t]) + codes_t[t].long()] = rmodel
+    # pass 1, run again at every reweighting, reallocates the group starts:
+    # the device pointers are read afresh after each (ptr())
+    ptr = {}
+
+    def refresh_ptrs():
+        for k in ("g", "gstart", "codes", "offs"):
+            ptr[k] = panel.ptr(k)[0]
+    refresh_ptrs()
+    n_m = np.diff(rs)
+    ybar = torch.tensor([float(y_s[rs[m]:rs[m + 1]].mean()) if n_m[m] else 1.0
+                         for m in range(M)], dtype=torch.float64, device=dev)
+    # launch lists
+    ch_model, ch_b, ch_e = [], [], []
+    for m in range(M):
+        for b in range(int(rs[m]), int(rs[m + 1]), 65536):
+            ch_model.append(m)
+            ch_b.append(b)
+            ch_e.append(min(int(rs[m + 1]), b + 65536))
+    ch_model = np.array(ch_model, np.int32)
+    ch_b = np.array(ch_b, np.int64)
+    ch_e = np.array(ch_e, np.int64)
+    wg_model, wg_gb, wg_ge = [], [], []
+    for m in range(M):
+        for gb in range(int(gb_model[m]), int(gb_model[m + 1]), 512):
+            wg_model.append(m)
+            wg_gb.append(gb)
+            wg_ge.append(min(int(gb_model[m + 1]), gb + 512))
+    wg_model = np.array(wg_model, np.int32)
+    wg_gb = np.array(wg_gb, np.int64)
+    wg_ge = np.array(wg_ge, np.int64)
+    Cp = (kx + 31) // 32 * 32
+    pstart = np.zeros(M + 1, np.int64)
+    pstart[1:] = np.cumsum((n_m + 15) // 16 * 16)
+    npad = int(pstart[-1])
+    hi = torch.zeros((npad // 8, Cp, 8), dtype=torch.bfloat16, device=dev)
+    lo = torch.zeros((npad // 8, Cp, 8), dtype=torch.bfloat16, device=dev)
+    Cc = kx + 1
+    vbar = torch.empty((G, Cc), dtype=torch.float64, device=dev)
+    Abuf = torch.empty((M, Cp, Cp), dtype=torch.float64, device=dev)
+    rp = RowPasses(panel)
+    Vz = torch.empty((n, 1), dtype=torch.float64, device=dev)
+
+    def row_pass(beta, fe, a, init):
+        w = torch.empty(n, dtype=torch.float64, device=dev)
+        z = torch.empty(n, dtype=torch.float64, device=dev)
+        part = np.zeros((len(ch_model), 5))
+        torch.cuda.synchronize()
+        _lib.check(lib.hdfe_glm_rows(
+            len(ch_model), ch_model.ctypes.data, ch_b.ctypes.data, ch_e.ctypes.data, int(init),
+            ybar.data_ptr(), y_s.data_ptr(), X_s.data_ptr(), kx, ptr["g"], ptr["codes"], n, D,
+            ptr["offs"], None if init else fe.data_ptr(), None if init else a.data_ptr(),
+            None if init else beta.data_ptr(), None, w.data_ptr(), z.data_ptr(),
+            part.ctypes.data), "hdfe_glm_rows")
+        sums = np.zeros((M, 5))
+        np.add.at(sums, ch_model, part)
+        return w, z, sums
+
+    tm = {"row_pass": 0.0, "system": 0.0, "rhs": 0.0, "solve": 0.0, "partial": 0.0,
+          "syrk": 0.0, "update": 0.0}
+
+    def clock(key, t0):
+        torch.cuda.synchronize()
+        tm[key] += time.perf_counter() - t0
+
+    t0 = time.perf_counter()
+    w, z, sums = row_pass(None, None, None, True)
+    clock("row_pass", t0)
+    slg = sums[:, 4].copy()
+    dev_cur = sums[:, 0].copy()
+    beta = torch.zeros((M, kx), dtype=torch.float64, device=dev)
+    fe = torch.zeros(L, dtype=torch.float64, device=dev)
+    a = torch.zeros(G, dtype=torch.float64, device=dev)
+    init = np.ones(M, bool)
+    active = np.ones(M, bool)
+    converged = np.zeros(M, bool)
+    iters = np.zeros(M, np.int64)
+    halvings = np.zeros(M, np.int64)
+    paths = [[float(dev_cur[m])] for m in range(M)]
+    Gamma = None
+    ncomp = None
+    solver_its = []
+    for it in range(1, iwls_maxiter + 1):
+        # the weighted fit of [z, X] on the fixed effects
+        t0 = time.perf_counter()
+        Vz[:, 0] = z
+        torch.cuda.synchronize()
+        _lib.check(lib.hdfe_panel_reweight(panel._h, Vz.data_ptr(), w.data_ptr()),
+                   "hdfe_panel_reweight")
+        refresh_ptrs()
+        panel.system(with_rhs=False)
+        if ncomp is None:
+            ncomp = panel.components()
+        clock("system", t0)
+        t0 = time.perf_counter()
+        rhs = torch.empty((L, Cc), dtype=torch.float64, device=dev)
+        torch.cuda.synchronize()
+        _lib.check(lib.hdfe_glm_rhs(rp._h, G, ptr["gstart"], ptr["g"], z.data_ptr(), X_s.data_ptr(),
+                                    kx, w.data_ptr(), vbar.data_ptr(), rhs.data_ptr()),
+                   "hdfe_glm_rhs")
+        clock("rhs", t0)
+        t0 = time.perf_counter()
+        Gamma, its, ok, rel = panel.matrix.solve(rhs, None, Gamma, tol=tol, maxiter=maxiter,
+                                                 precision=precision)
+        solver_its.append(its)
+        clock("solve", t0)
+        # the slopes: Newton step with the split-BF16 Hessian and FP64 score
+        t0 = time.perf_counter()
+        gpart = np.zeros((len(wg_model), kx))
+        torch.cuda.synchronize()
+        _lib.check(lib.hdfe_glm_partial(
+            len(wg_model), wg_model.ctypes.data, wg_gb.ctypes.data, wg_ge.ctypes.data,
+            ptr["gstart"], rs.ctypes.data, pstart.ctypes.data, M, Cp, z.data_ptr(),
+            X_s.data_ptr(), kx, w.data_ptr(), ptr["codes"], n, D, ptr["offs"], Gamma.data_ptr(),
+            beta.data_ptr(), hi.data_ptr(), lo.data_ptr(), gpart.ctypes.data),
+            "hdfe_glm_partial")
+        clock("partial", t0)
+        t0 = time.perf_counter()
+        _lib.check(lib.hdfe_glm_syrk(M, pstart.ctypes.data, Cp, hi.data_ptr(), lo.data_ptr(),
+                                     int(rows_per_syrk), Abuf.data_ptr()), "hdfe_glm_syrk")
+        clock("syrk", t0)
+        t0 = time.perf_counter()
+        score = np.zeros((M, kx))
+        np.add.at(score, wg_model, gpart)
+        H = Abuf[:, :kx, :kx].cpu().numpy()
+        bcur = beta.cpu().numpy()
+        bnew = bcur.copy()
+        for m in range(M):
+            if active[m]:
+                bnew[m] = bcur[m] + np.linalg.solve(H[m], score[m])
+        beta_new = torch.from_numpy(bnew).to(dev)
+        fe_new = torch.empty(L, dtype=torch.float64, device=dev)
+        a_new = torch.empty(G, dtype=torch.float64, device=dev)
+        torch.cuda.synchronize()
+        _lib.check(lib.hdfe_glm_coefficients(
+            L, G, kx, Gamma.data_ptr(), lmodel.data_ptr(), gmodel.data_ptr(), ptr["gstart"],
+            z.data_ptr(), X_s.data_ptr(), w.data_ptr(), ptr["codes"], n, D, ptr["offs"],
+            beta_new.data_ptr(), fe_new.data_ptr(), a_new.data_ptr()), "hdfe_glm_coefficients")
+        clock("update", t0)
+
+        def toward(tv):
+            """Per model: current + t (new - current); t = 0 freezes a model."""
+            tt = torch.from_numpy(np.ascontiguousarray(tv, np.float64)).to(dev)
+            out_b = torch.empty_like(beta)
+            out_f = torch.empty_like(fe)
+            out_a = torch.empty_like(a)
+            torch.cuda.synchronize()
+            _lib.check(lib.hdfe_glm_toward(M, None, kx, tt.data_ptr(), beta.data_ptr(),
+                                           beta_new.data_ptr(), out_b.data_ptr()),
+                       "hdfe_glm_toward")
+            _lib.check(lib.hdfe_glm_toward(L, lmodel.data_ptr(), 1, tt.data_ptr(),
+                                           fe.data_ptr(), fe_new.data_ptr(), out_f.data_ptr()),
+                       "hdfe_glm_toward")
+            _lib.check(lib.hdfe_glm_toward(G, gmodel.data_ptr(), 1, tt.data_ptr(),
+                                           a.data_ptr(), a_new.data_ptr(), out_a.data_ptr()),
+                       "hdfe_glm_toward")
+            return out_b, out_f, out_a
+
+        t = active.astype(np.float64)
+        t0 = time.perf_counter()
+        cb, cf, ca = toward(t)
+        wc, zc, sums = row_pass(cb, cf, ca, False)
+        clock("row_pass", t0)
+        dev_new = sums[:, 0].copy()
+        tol_acc = max(iwls_tol, DEV_ROUNDING)
+
+        def accept(m):
+            return np.isfinite(dev_new[m]) and (
+                dev_new[m] < dev_cur[m]
+                or abs(dev_new[m] - dev_cur[m]) / (0.1 + abs(dev_cur[m])) < tol_acc)
+
+        bad = [m for m in range(M) if active[m] and not init[m] and not accept(m)]
+        for m in range(M):
+            if active[m] and init[m] and not np.isfinite(dev_new[m]):
+                raise FloatingPointError(f"model {m}: the first IRLS step gives a "
+                                         "non-finite deviance")
+        nh = 0
+        while bad:
+            if nh >= MAX_HALVINGS:
+                raise RuntimeError(f"models {bad}: IRLS step-halving failed")
+            nh += 1
+            for m in bad:
+                t[m] /= 2
+                halvings[m] += 1
+            t0 = time.perf_counter()
+            cb, cf, ca = toward(t)
+            wc, zc, sums = row_pass(cb, cf, ca, False)
+            clock("row_pass", t0)
+            for m in bad:
+                dev_new[m] = sums[m, 0]
+            bad = [m for m in bad if not (np.isfinite(dev_new[m]) and dev_new[m] < dev_cur[m])]
+        for m in range(M):
+            if not active[m]:
+                continue
+            crit = abs(dev_new[m] - dev_cur[m]) / (0.1 + abs(dev_cur[m]))
+            flat = (not init[m]) and dev_new[m] >= dev_cur[m]
+            iters[m] = it
+            paths[m].append(float(dev_new[m]))
+            dev_cur[m] = dev_new[m]
+            init[m] = False
+            if crit < iwls_tol or flat:
+                converged[m] = True
+                active[m] = False
+        beta, fe, a, w, z = cb, cf, ca, wc, zc
+        if not active.any():
+            break
+    # the reported effects: per model, each non-streamed dimension after the
+    # first at obs-weighted mean zero, the first per component (the
+    # normalisation of core/solve.py:383-399), the group effects absorbing
+    # the shifts; the linear predictor is unchanged
+    eta_s = _eta(y_s, fe, a, beta, row_pass)
+    fe_h = fe.cpu().numpy()
+    a_h = a.cpu().numpy()
+    lmod_h = lmodel.cpu().numpy()
+    obs = panel.array("obs", np.float64)
+    gmod_h = gmodel.cpu().numpy()
+    for t_ in range(1, D):
+        o, nl = int(panel.offs[t_]), panel.n_levels[t_]
+        seg = slice(o, o + nl)
+        for m in range(M):
+            sel = np.flatnonzero(lmod_h[seg] == m)
+            if not len(sel):
+                continue
+            wts = obs[seg][sel]
+            c = float(wts @ fe_h[seg][sel] / max(wts.sum(), 1e-300))
+            fe_h[o + sel] -= c
+            a_h[gmod_h == m] += c
+    o0, nl0 = int(panel.offs[0]), panel.n_levels[0]
+    comp = panel.array("comp", np.int32)
+    wts = obs[o0:o0 + nl0]
+    ncmp = int(comp.max()) + 1 if len(comp) else 0
+    cw = np.bincount(comp, weights=wts, minlength=ncmp)
+    cm = np.bincount(comp, weights=wts * fe_h[o0:o0 + nl0], minlength=ncmp) / np.where(cw > 0, cw, 1)
+    fe_h[o0:o0 + nl0] -= cm[comp]
+    # each component's shift goes to its groups: a group's component is its
+    # first-dimension level's
+    g0codes = codes_t[0][gstart_t[:-1].long()].cpu().numpy()
+    a_h += cm[comp[g0codes]]
+    eta_src_all = torch.full((n_stack,), float("nan"), dtype=torch.float64, device=dev)
+    eta_src_all[perm] = eta_s
+    perm_h = perm.cpu().numpy()
+    out = []
+    stack_off = np.concatenate(([0], np.cumsum([len(r) for r in src_row])))
+    glevels = panel.array("glevels", np.int64)
+    lv_arrays = [panel.array(f"levels{t_}", np.int64) for t_ in range(D)]
+    seconds = time.perf_counter() - t_start
+    for m, p in enumerate(per):
+        e_src = torch.full((p["n_in"],), float("nan"), dtype=torch.float64, device=dev)
+        e_src[src_row[m]] = eta_src_all[int(stack_off[m]):int(stack_off[m + 1])]
+        levels, values = {}, {}
+        gsel = np.flatnonzero(gmod_h == m)
+        ids0 = level_ids[m][0].cpu().numpy()
+        levels[fe_names[0]] = ids0[glevels[gsel] - offs_model[m, 0]]
+        values[fe_names[0]] = a_h[gsel]
+        for t_ in range(D):
+            d = panel.dims[t_]
+            o, nl = int(panel.offs[t_]), panel.n_levels[t_]
+            sel = np.flatnonzero(lmod_h[o:o + nl] == m)
+            idsd = level_ids[m][d].cpu().numpy()
+            levels[fe_names[d]] = idsd[lv_arrays[t_][sel] - offs_model[m, d]]
+            values[fe_names[d]] = fe_h[o + sel]
+        nm = int(n_m[m])
+        out.append(GLMResult(
+            coefnames=p["names"], beta=beta[m].cpu().numpy(), deviance=float(dev_cur[m]),
+            loglik=float(sums[m, 1] - slg[m]), iterations=int(iters[m]),
+            converged=bool(converged[m]), n_obs=nm, fe_names=[fe_names[0]] +
+            [fe_names[d] for d in panel.dims], fe_levels=levels, fe_values=values,
+            deviance_path=paths[m], eta_source=e_src,
+            diagnostics={"separation": p["separation"], "halvings": int(halvings[m]),
+                         "solver": "explicit", "precision": precision,
+                         "solver_iterations": solver_its, "batch": M,
+                         "seconds_batch": round(seconds, 4)}))
+    if timings is not None:
+        timings.update({k: round(v, 4) for k, v in tm.items()})
+        timings["total"] = round(seconds, 4)
+        timings["steps"] = int(iters.max())
+        timings["rows"] = int(n)
+        timings["levels"] = int(L)
+        timings["groups"] = int(G)
+    rp.close()
+    panel.close()
+    return out
+
+
+def _eta(y_s, fe, a, beta, row_pass):
+    """The linear predictor per sorted row at (beta, fe, a): z - u with the
+    row pass's u = y / mu - 1 recovered from w = mu."""
+    w, z, _ = row_pass(beta, fe, a, False)
+    return z - (y_s / w - 1.0)
+
+
+def fepois_stream(fml: str, data, iwls_tol: float = 1e-8, iwls_maxiter: int = 25,
+                  separation_check: bool = True, device: str = "rocm", **options) -> GLMResult:
+    """Poisson regression with fixed effects on the device (api.py:144-240,
+    with device='rocm')."""
+    if device != "rocm":
+        raise ValueError("device must be 'rocm' (this package is the MI355X backend)")
+    return fepois_batch(fml, [data], iwls_tol=iwls_tol, iwls_maxiter=iwls_maxiter,
+                        separation_check=separation_check, **options)[0]

@@ -152,6 +152,12 @@ int hdfe_panel_assemble(void *h, const double *Gamma, double *A);
 int hdfe_panel_pass2(void *h, const double *Gamma, const double *beta, const int *xcols, int kx,
                      double *gam_y, double *geff, double *resid);
 
+/* New variables V (n, m) and weights w (n), device, in the panel's sorted row
+ * order, and pass 1 again over the same rows and levels; the caller then
+ * rebuilds the system.  One weighted least-squares step of IRLS
+ * (core/irls.py:92-123: _wls after each row pass). */
+int hdfe_panel_reweight(void *h, const double *V, const double *w);
+
 /* out (n, kc, device) = the columns cols (host, kc) of the panel's variables
  * partialled on the fixed effects (M_F x), from their reduced solutions Gm
  * (L, m): per group, x less the levels' effects, less its weighted mean.
@@ -303,6 +309,57 @@ int hdfe_adj_xwr(int64_t n, int m, const double *V, const int *xcols, int kx, co
 int hdfe_adj_gradients(int64_t n, int kx, const double *rows, const double *Xt, const double *uX,
                        const double *w, const double *e, const double *direct, double *gy,
                        double *gw);
+
+/* ------------------------------------------------- Poisson IRLS (GLM)
+ * Many models as one stacked panel (levels kept apart per model), rows of a
+ * model contiguous in sorted order.  X: covariates (n, kx) float32, row
+ * major, sorted order. */
+
+/* One IRLS row pass (_nb_glm_rows / _row_terms, core/kernels_glm.py:99-211,
+ * Poisson): per row the linear predictor at (beta_m, fe, a), or the start
+ * log((y + ybar_m) / 2) with init; w = mu f, z = eta + y / mu - 1.  Chunks
+ * (host lists: model, row range) of one model each; part (host, nchunk x 5)
+ * = deviance, sum f (y eta - mu), f y, f, and with init sum f lgamma(y + 1). */
+int hdfe_glm_rows(int nchunk, const int *ch_model, const int64_t *ch_begin,
+                  const int64_t *ch_end, int init, const double *ybar, const double *y,
+                  const float *X, int kx, const int *g, const int *codes, int64_t ld, int D,
+                  const int64_t *offs, const double *fe, const double *a, const double *beta,
+                  const double *f, double *w, double *z, double *part_h);
+
+/* rhs (L, kx + 1) = D_o' W M_0 [z, X] (the cell sums' reduced right-hand
+ * side of _wls, core/irls.py:100-123), vbar (G, kx + 1) scratch. */
+int hdfe_glm_rhs(void *rows_h, int64_t G, const int *gstart, const int *g, const double *z,
+                 const float *X, int kx, const double *w, double *vbar, double *rhs);
+
+/* The covariates and working response partialled on the fixed effects
+ * (from Gm (L, kx + 1)); sqrt(w) X~ split into bf16 hi + lo in rows of eight
+ * ((p / 8, Cp, 8), model m's padded rows from pstart[m]); and per workgroup
+ * (host lists: model, group range) the score X~'W (z~ - X~ beta_m), gpart
+ * (host, nwg x kx). */
+int hdfe_glm_partial(int nwg, const int *wg_model, const int64_t *wg_gb, const int64_t *wg_ge,
+                     const int *gstart, const int64_t *rs_h, const int64_t *pstart_h, int nmodels,
+                     int Cp, const double *z, const float *X, int kx, const double *w,
+                     const int *codes, int64_t ld, int D, const int64_t *offs, const double *Gm,
+                     const double *beta, void *hi, void *lo, double *gpart_h);
+
+/* A (nmodels, Cp, Cp, FP64, device) = (hi + lo)'(hi + lo) per model over its
+ * padded rows, as hi'hi + hi'lo + lo'hi on the matrix cores (split-BF16, FP32
+ * accumulation per workgroup of rows_per_wg rows, FP64 across them): the
+ * dense X'WX of glm.py:990. */
+int hdfe_glm_syrk(int nmodels, const int64_t *pstart_h, int Cp, const void *hi, const void *lo,
+                  int64_t rows_per_wg, double *A);
+
+/* fe (L) = Gm[:, 0] - Gm[:, 1:] beta_model and the group effects a (G)
+ * (_update, core/irls.py:127-144; _nb_group_effects, kernels_glm.py:11-33). */
+int hdfe_glm_coefficients(int64_t L, int64_t G, int kx, const double *Gm, const int *lmodel,
+                          const int *gmodel, const int *gstart, const double *z, const float *X,
+                          const double *w, const int *codes, int64_t ld, int D,
+                          const int64_t *offs, const double *beta, double *fe, double *a);
+
+/* out = base + t[model] (target - base) over n entries of `per` values
+ * (model NULL: t[i]): step-halving, _State.toward (core/irls.py:44-48). */
+int hdfe_glm_toward(int64_t n, const int *model, int per, const double *t, const double *base,
+                    const double *target, double *out);
 
 #ifdef __cplusplus
 }

@@ -29,6 +29,9 @@ the reference's FP64 algorithm:
                                       inverse.py:311-460, kernels_inverse.py:
                                       111-231, leaveout.py:321-448, 634-1016,
                                       1069-1255, leaveout_se.py:202-214
+  GLM      Poisson IRLS: separation, start, weighted fits, step-halving,
+           convergence                glm.py:319-370, 743-838, core/irls.py:
+                                      100-144, core/families.py:69-100
   autograd a dense FP64 torch (CPU) restatement of the weighted fit and its
            plug-in components, differentiated by torch's autograd: the check
            of the device adjoint (the reference has no autograd)
@@ -696,6 +699,88 @@ def dense_fe_components(y, w, X, codes, psi: int = 1):
     valpha = (w @ ((alpha - ma) ** 2)) / Dp
     cov = (w @ ((ps - mp) * (alpha - ma))) / Dp
     return torch.cat([theta[:kx], torch.stack([vpsi, valpha, cov])])
+
+
+# ---------------------------------------------------------- Poisson IRLS
+
+def poisson_separation(y: np.ndarray, fe_cols: list) -> np.ndarray:
+    """Rows kept by the separation check: one pass over the dimensions, each
+    dropping the levels whose outcome is zero on every kept row
+    (glm.py:346-367, Poisson)."""
+    keep = np.ones(len(y), bool)
+    if not np.any(y == 0):
+        return keep
+    for col in fe_cols:
+        u, inv = np.unique(col, return_inverse=True)
+        ymax = np.full(len(u), -np.inf)
+        np.maximum.at(ymax, inv[keep], y[keep])
+        bad = ymax == 0
+        keep &= ~bad[inv]
+    return keep
+
+
+def fepois(data: dict, y: str, xs, fe, iwls_tol: float = 1e-8, iwls_maxiter: int = 25,
+           tol: float = 1e-12) -> dict:
+    """Poisson IRLS with fixed effects as the reference runs it (glm.py:743-838,
+    core/irls.py:100-144, families.py:69-100): separation check, singleton
+    drop, the start eta = log((y + ybar) / 2), then per step the weighted fit
+    of z = eta + y / mu - 1 on [X, fixed effects] with W = mu (the sparse FP64
+    fit above, its fitted values z - resid the new linear predictor), halving
+    of steps that raise the deviance, and |dev - dev_old| / (0.1 + |dev|) <
+    iwls_tol.  Returns beta, the linear predictor and deviance per kept row
+    (source order), the kept-row mask and the step count."""
+    yv = np.asarray(data[y], np.float64)
+    cols = [np.asarray(data[f]) for f in fe]
+    keep = poisson_separation(yv, cols)
+    keep[keep] &= singleton_restriction([c[keep] for c in cols])
+    sub = {k: np.asarray(v)[keep] for k, v in data.items()
+           if hasattr(v, "__len__") and len(v) == len(keep)}
+    yk = sub[y].astype(np.float64)
+
+    def deviance(eta):
+        mu = np.exp(eta)
+        xlogy = np.where(yk != 0, yk * np.log(np.where(yk != 0, yk, 1.0)), 0.0)
+        return float(np.sum(2.0 * (xlogy - yk * eta - yk + mu)))
+
+    eta = np.log((yk + yk.mean()) / 2.0)
+    dev = deviance(eta)
+    beta = np.zeros(len(xs))
+    init = True
+    path = [dev]
+    it = 0
+    converged = False
+    for it in range(1, iwls_maxiter + 1):
+        mu = np.exp(eta)
+        w = mu
+        z = eta + yk / mu - 1.0
+        d = dict(sub, __z__=z, __w__=w)
+        f = fit(d, "__z__", list(xs), list(fe), weights="__w__", tol=tol, fixef_rm="none",
+                stream=fe[0])
+        eta_new = z - f["resid_source_order"]
+        beta_new = f["beta"]
+        dev_new = deviance(eta_new)
+        if not init and not (np.isfinite(dev_new) and (
+                dev_new < dev or abs(dev_new - dev) / (0.1 + abs(dev)) < max(iwls_tol, 1e-12))):
+            t = 1.0
+            target_eta, target_beta = eta_new, beta_new
+            for _ in range(30):
+                t /= 2
+                eta_new = eta + t * (target_eta - eta)
+                beta_new = beta + t * (target_beta - beta)
+                dev_new = deviance(eta_new)
+                if np.isfinite(dev_new) and dev_new < dev:
+                    break
+            else:
+                raise RuntimeError("IRLS step-halving failed")
+        crit = abs(dev_new - dev) / (0.1 + abs(dev))
+        flat = (not init) and dev_new >= dev
+        eta, beta, dev, init = eta_new, beta_new, dev_new, False
+        path.append(dev)
+        if crit < iwls_tol or flat:
+            converged = True
+            break
+    return {"beta": beta, "eta": eta, "deviance": dev, "keep": keep, "iterations": it,
+            "converged": converged, "path": path, "y": yk, "data": sub}
 
 
 # ------------------------------------------------------------------ graph
