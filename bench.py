@@ -33,6 +33,7 @@ FML = "y ~ x1 + x2 | worker_id + firm_id + year"
 # an XCD's L2, so it runs column-blocked as two launches, timed together.
 SPMM_KERNEL = "k_spmm_blk<double,3,G> + k_spmm_combine<double,3>"
 HBM_PEAK_GBS = 8000.0
+BF16_PEAK_TFLOPS = 2500.0     # MI355X dense bf16 matrix peak
 
 
 def log(msg):
@@ -207,6 +208,48 @@ def adjoint_phase(n_workers, n_firms, fd_rows):
             "fd_max_err_over_rms_y": ey, "grad_rms_y": ry}
 
 
+def glm_phase(n_models, lo, hi, n_cov):
+    """Uneven Poisson models with three fixed effects and `n_cov` covariates,
+    all fitted as one batch (fepois_batch): mixed-precision solves timed,
+    FP64 solves beside them for the slopes' agreement.  The split-BF16 SYRK
+    of the covariate Hessian is the MFMA-bound kernel: its algorithmic flops
+    per IRLS step are sum_i 2 n_i c^2 (c = n_cov + 1, as the reference's
+    V'V), against the BF16 dense peak over three (the split's three
+    products)."""
+    import torch
+    from hdfe_stream_amd.glm import fepois_batch
+    from hdfe_stream_amd.simulate import poisson_models_device
+    ds = poisson_models_device(n_models, lo, hi, n_cov, seed=5)
+    rows = int(sum(len(d["y"]) for d in ds))
+    fml = "y ~ X | worker_id + firm_id + year"
+    tm = {}
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res = fepois_batch(fml, ds, precision="mixed", timings=tm)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    tm64 = {}
+    res64 = fepois_batch(fml, ds, precision="fp64", timings=tm64)
+    err = max(float(np.max(np.abs(a.beta - b.beta)) / np.max(np.abs(b.beta)))
+              for a, b in zip(res, res64))
+    c = n_cov + 1
+    flops_step = 2.0 * rows * c * c
+    steps = tm["steps"]
+    syrk_tflops = flops_step * steps / tm["syrk"] / 1e12 if tm["syrk"] > 0 else float("nan")
+    peak3 = BF16_PEAK_TFLOPS / 3.0
+    return {"workload": f"{n_models} Poisson models, rows log-uniform in [{lo:,}, {hi:,}] "
+                        f"({rows:,} rows), worker/firm/year fixed effects, {n_cov} float32 "
+                        "covariates; fepois_batch, mixed-precision solves",
+            "wall_s": round(wall, 4), "timings_s": tm, "fp64_wall_s": tm64["total"],
+            "irls_steps": steps, "converged": all(r.converged for r in res),
+            "max_iterations": int(max(r.iterations for r in res)),
+            "beta_rel_err_mixed_vs_fp64": err,
+            "syrk": {"kernel": "k_glm_syrk (split-BF16, v_mfma_f32_32x32x16_bf16 x3)",
+                     "flops_per_step": flops_step, "achieved_TFLOPs": round(syrk_tflops, 1),
+                     "peak_TFLOPs_bf16_over_3": round(peak3, 1),
+                     "frac": round(syrk_tflops / peak3, 4)}}
+
+
 def copy_calibration(nbytes=1 << 30, reps=20):
     """A plain copy kernel of known traffic: `nbytes` read and written per
     launch, timed with HIP events."""
@@ -253,6 +296,11 @@ def main():
     ap.add_argument("--jla-draws", type=int, default=256)
     ap.add_argument("--no-kss", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-glm", action="store_true")
+    ap.add_argument("--glm-models", type=int, default=32)
+    ap.add_argument("--glm-lo", type=int, default=50_000)
+    ap.add_argument("--glm-hi", type=int, default=2_000_000)
+    ap.add_argument("--glm-cov", type=int, default=300)
     ap.add_argument("--adjoint-workers", type=int, default=2_000_000)
     ap.add_argument("--adjoint-firms", type=int, default=200_000)
     ap.add_argument("--adjoint-fd-rows", type=int, default=64)
@@ -339,7 +387,7 @@ def main():
                       "bytes_per_launch": int(b32), "achieved_GBps": round(b32 / avg32 / 1e9, 1)},
         "copy_calibration": copy_calibration(),
     }
-    if not (args.no_jla and args.no_kss and args.no_adjoint):
+    if not (args.no_jla and args.no_kss and args.no_adjoint and args.no_glm):
         del results, f64, mix
     if not args.no_jla:
         log("JLA phase")
@@ -351,6 +399,9 @@ def main():
         log("adjoint phase")
         line["adjoint"] = adjoint_phase(args.adjoint_workers, args.adjoint_firms,
                                         args.adjoint_fd_rows)
+    if not args.no_glm:
+        log("GLM phase")
+        line["glm"] = glm_phase(args.glm_models, args.glm_lo, args.glm_hi, args.glm_cov)
     if not args.no_cpu:
         log("CPU baseline (oracle, 1/8 sample)")
         line["cpu_baseline"] = cpu_baseline(args.workers // 8, args.firms // 8, args.workers, seed=2)

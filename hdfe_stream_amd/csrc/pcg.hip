@@ -815,7 +815,8 @@ struct Matrix {
     int nlong = 0, nchunk = 0;
     int *long_rows = nullptr, *long_cptr = nullptr, *chunk_row = nullptr, *chunk_beg = nullptr;
     // blocked copy of the non-split rows for the small-k SpMM (k_spmm_blk)
-    bool blk = false;
+    bool blk = false, blk_tried = false;
+    std::vector<int> ip_host;  // host row pointers (the blocked layout's balance)
     int col_lo_h[9] = {0};
     int *col_lo = nullptr, *bptr = nullptr, *bidx = nullptr;
     double *bval64 = nullptr;
@@ -835,7 +836,18 @@ struct Matrix {
     }
 };
 
+static int build_blocked(Matrix &M, const std::vector<int> &ip32);
+
 static int alloc_work(Matrix &M, int k) {
+    // the blocked layout for small k, once, when P outgrows an XCD's L2
+    if (k <= kSmallK && !M.blk_tried) {
+        M.blk_tried = true;
+        if (M.L * (int64_t)kSmallK * 8 > kL2Budget && M.L < (INT32_MAX - 1) / 8) {
+            M.w.release();
+            const int st = build_blocked(M, M.ip_host);
+            if (st) return st;
+        }
+    }
     if (M.w.k == k && M.w.L == M.L) return HDFE_STATUS_OK;
     M.w.release();
     const int64_t Lk = std::max<int64_t>(1, M.L * (int64_t)k);
@@ -1019,7 +1031,10 @@ static void launch_spmm(Matrix &M, const T *val, const T *P, T *AP, double *part
                        M.indices, val, M.L, P, AP, part, with_dot, st)
     if (cpl <= 1) HDFE_WIDE(1);
     else if (cpl <= 2) HDFE_WIDE(2);
+    else if (cpl <= 3) HDFE_WIDE(3);
     else if (cpl <= 4) HDFE_WIDE(4);
+    else if (cpl <= 5) HDFE_WIDE(5);
+    else if (cpl <= 6) HDFE_WIDE(6);
     else if (cpl <= 8) HDFE_WIDE(8);
     else HDFE_WIDE(16);
 #undef HDFE_WIDE
@@ -1332,10 +1347,7 @@ int matrix_create(int64_t L, int64_t nnz, const std::vector<int> &ip32, const in
         HIP_TRY(hipMemcpy(M->chunk_beg, cbeg.data(), M->nchunk * 4, hipMemcpyHostToDevice));
     }
     HIP_TRY(hipStreamSynchronize(M->stream));
-    if (L * (int64_t)kSmallK * 8 > kL2Budget && L < (INT32_MAX - 1) / 8) {
-        const int st = build_blocked(*M, ip32);
-        if (st) return st;
-    }
+    M->ip_host = ip32;  // kept for the blocked layout, built at the first small-k use
     *out = M.release();
     return HDFE_STATUS_OK;
 }

@@ -292,9 +292,10 @@ def fepois_batch(fml: str, datasets: list, iwls_tol: float = 1e-8, iwls_maxiter:
                    "hdfe_glm_rhs")
         clock("rhs", t0)
         t0 = time.perf_counter()
+        sinfo = {}
         Gamma, its, ok, rel = panel.matrix.solve(rhs, None, Gamma, tol=tol, maxiter=maxiter,
-                                                 precision=precision)
-        solver_its.append(its)
+                                                 precision=precision, info=sinfo)
+        solver_its.append((its, sinfo.get("refinement_steps", 0), bool(ok), float(rel)))
         clock("solve", t0)
         # the slopes: Newton step with the split-BF16 Hessian and FP64 score
         t0 = time.perf_counter()
@@ -466,6 +467,7 @@ def fepois_batch(fml: str, datasets: list, iwls_tol: float = 1e-8, iwls_maxiter:
         timings.update({k: round(v, 4) for k, v in tm.items()})
         timings["total"] = round(seconds, 4)
         timings["steps"] = int(iters.max())
+        timings["solver_iterations"] = solver_its
         timings["rows"] = int(n)
         timings["levels"] = int(L)
         timings["groups"] = int(G)

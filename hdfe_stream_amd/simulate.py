@@ -16,7 +16,8 @@ from __future__ import annotations
 
 import numpy as np
 
-__all__ = ["akm_panel", "kss_panel", "weighted_panel", "poisson_models"]
+__all__ = ["akm_panel", "kss_panel", "weighted_panel", "poisson_models",
+           "poisson_models_device"]
 
 # Row-level corr(alpha, psi) is about 0.6 times the copula parameter with
 # p_move 0.15 and 8 years (movers' later firms are uniform), so a copula of
@@ -109,4 +110,33 @@ def poisson_models(n_models: int = 32, lo: int = 50_000, hi: int = 2_000_000,
         y = rng.poisson(mu).astype(np.float64)
         out.append({"worker_id": worker.astype(np.int64), "firm_id": firm.astype(np.int64),
                     "year": year.astype(np.int64), "X": X, "y": y})
+    return out
+
+
+def poisson_models_device(n_models: int = 32, lo: int = 50_000, hi: int = 2_000_000,
+                          n_cov: int = 300, seed: int = 5) -> list:
+    """`poisson_models`' shapes generated on the GPU (torch's seeded CUDA
+    generator), for the sizes whose covariates would take minutes to draw on
+    the host: rows log-uniform in [lo, hi] (numpy, `seed`), worker rows/8
+    levels (rows sorted by worker), firm rows/80, year 10, `n_cov` float32
+    covariates N(0, 1), y ~ Poisson(exp(0.5 + 0.1 x1 + alpha + psi)) with
+    alpha, psi ~ N(0, 0.2^2).  Device tensors."""
+    import torch
+    rng = np.random.default_rng(seed)
+    sizes = np.exp(rng.uniform(np.log(lo), np.log(hi), n_models)).astype(np.int64)
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(seed)
+    out = []
+    for n in sizes:
+        n = max(int(n), 160)
+        nw, nf = max(n // 8, 2), max(n // 80, 2)
+        worker = torch.sort(torch.randint(0, nw, (n,), generator=gen, device="cuda")).values
+        firm = torch.randint(0, nf, (n,), generator=gen, device="cuda")
+        year = torch.randint(0, 10, (n,), generator=gen, device="cuda")
+        X = torch.randn((n, n_cov), generator=gen, device="cuda", dtype=torch.float32)
+        a = 0.2 * torch.randn(nw, generator=gen, device="cuda", dtype=torch.float64)
+        p = 0.2 * torch.randn(nf, generator=gen, device="cuda", dtype=torch.float64)
+        mu = torch.exp(0.5 + 0.1 * X[:, 0].double() + a[worker] + p[firm])
+        y = torch.poisson(mu, generator=gen)
+        out.append({"worker_id": worker, "firm_id": firm, "year": year, "X": X, "y": y})
     return out
