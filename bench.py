@@ -114,32 +114,36 @@ def jla_phase(n_workers, n_firms, draws):
 
 
 def kss_phase(n_workers, n_firms, draws):
-    """KSS leaving out an observation, end to end on the KSS panel:
-    leave-one-out pruning of (worker, firm), the fit of y ~ 1 | worker_id +
-    firm_id + year with workers streamed, `draws` JLA draws and `draws`
-    Hutchinson draws (two solves each), all blocks of `draws` right-hand
-    sides in mixed precision."""
+    """KSS end to end on the KSS panel, leaving out a match (the reference's
+    default) and an observation: leave-one-out pruning of (worker, firm), the
+    fit of y ~ 1 | worker_id + firm_id + year with workers streamed (at match
+    level then the collapse and the weighted match fit), `draws` JLA draws
+    and `draws` Hutchinson draws (two solves each), all blocks of `draws`
+    right-hand sides in mixed precision."""
     import torch
     from hdfe_stream_amd.leaveout import leave_out_kss
     from hdfe_stream_amd.simulate import kss_panel
     d = kss_panel(n_workers, n_firms, seed=3)
     cols = {k: torch.from_numpy(d[k]).cuda() for k in ("worker_id", "firm_id", "year", "y")}
     torch.cuda.synchronize()
-    tm = {}
-    t0 = time.perf_counter()
-    lo = leave_out_kss("y ~ 1 | worker_id + firm_id + year", cols, n_draws=draws, seed=0,
-                       block=draws, precision="mixed", timings=tm)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    out = {"workload": f"KSS leave-out (observation) y ~ 1 | worker_id + firm_id + year, "
+    out = {"workload": f"KSS leave-out y ~ 1 | worker_id + firm_id + year, "
                        f"{n_workers:,} workers x 8 years, {n_firms:,} firms, p_move 0.15, "
-                       f"{draws} JLA + {draws} Hutchinson draws",
-           "wall_s": round(wall, 4), "rows_after_pruning": lo.n_obs, "movers": lo.n_movers,
-           "pruning": lo.pruning,
-           "seconds": {k: round(v, 4) for k, v in tm.items() if k.endswith("seconds")},
-           "trace_rhs_solves_per_s": round(2 * draws / tm["trace_seconds"], 1),
-           "plug_in": lo.plug_in, "bias": lo.bias, "leave_out": lo.leave_out,
-           "max_leverage": lo.diagnostics["max_leverage"]}
+                       f"{draws} JLA + {draws} Hutchinson draws, mixed precision"}
+    for level in ("match", "observation"):
+        tm = {}
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        lo = leave_out_kss("y ~ 1 | worker_id + firm_id + year", cols, n_draws=draws, seed=0,
+                           block=draws, precision="mixed", timings=tm, leave_out=level)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        out[level] = {
+            "wall_s": round(wall, 4), "person_years": lo.n_obs, "movers": lo.n_movers,
+            "matches": lo.diagnostics.get("n_matches"), "pruning": lo.pruning,
+            "seconds": {k: round(v, 4) for k, v in tm.items() if k.endswith("seconds")},
+            "trace_rhs_solves_per_s": round(2 * draws / tm["trace_seconds"], 1),
+            "plug_in": lo.plug_in, "bias": lo.bias, "leave_out": lo.leave_out,
+            "max_leverage": lo.diagnostics["max_leverage"]}
     return out
 
 

@@ -66,7 +66,7 @@ SIGNATURES = [
     ("hdfe_kss_group_sums", I32, [I64, I32, P, P, P, P, P]),
     ("hdfe_kss_trace_moments", I32, [I64, I32, P, P, I64, P, P, P, P, P, P]),
     ("hdfe_kss_drop_null", I32, [I64, I64, I32, I64, I32, P, P, P, P, P, P, P]),
-    ("hdfe_kss_sigma2", I32, [I64, I32, P, P, P, P, P, P, P]),
+    ("hdfe_kss_sigma2", I32, [I64, I32, P, P, I32, P, P, P, P, P]),
     ("hdfe_kss_plugin", I32, [I64, P, P, I64, P, P, P, P]),
     ("hdfe_copy_calibrate", I32, [I64, I32, P]),
     ("hdfe_panel_partial", I32, [P, P, P, I32, P]),

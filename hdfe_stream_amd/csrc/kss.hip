@@ -258,15 +258,43 @@ __global__ void k_wmean_part(int64_t n, int m, const double *__restrict__ V,
     }
 }
 
-// sigma2_i = w_i (y_i - ybar) e_i / M_ii * factor_i (leaveout.py:949-1016,
-// centering "weighted", every row its own estimate).
+// sigma2_i (leaveout.py:949-1016), every row its own estimate: centering
+// "weighted" (mode 0) w_i (y_i - ybar_w) e_i / M_ii * factor_i; "reference"
+// (mode 1) (sqrt(w_i) y_i - the mean of sqrt(w) y) sqrt(w_i) e_i / M_ii *
+// factor_i, `ybar` the respective mean.
 __global__ void k_sigma2(int64_t n, int m, const double *__restrict__ V,
-                         const double *__restrict__ w, double ybar,
+                         const double *__restrict__ w, double ybar, int mode,
                          const double *__restrict__ e, const double *__restrict__ Mb,
                          const double *__restrict__ F, double *__restrict__ out) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x)
-        out[i] = (w ? w[i] : 1.0) * (V[i * m] - ybar) * e[i] / Mb[i] * F[i];
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const double wi = w ? w[i] : 1.0;
+        if (mode == 1) {
+            const double r = sqrt(wi);
+            out[i] = (r * V[i * m] - ybar) * r * e[i] / Mb[i] * F[i];
+        } else {
+            out[i] = wi * (V[i * m] - ybar) * e[i] / Mb[i] * F[i];
+        }
+    }
+}
+
+// Partial sums for the reference centring's mean: part[b] = (count, sum
+// sqrt(w) y).
+__global__ void k_rootmean_part(int64_t n, int m, const double *__restrict__ V,
+                                const double *__restrict__ w, double *__restrict__ part) {
+    __shared__ double lds[4];
+    const int64_t r0 = part_begin(n, blockIdx.x, gridDim.x), r1 = part_begin(n, blockIdx.x + 1, gridDim.x);
+    double c = 0.0, s = 0.0;
+    for (int64_t i = r0 + threadIdx.x; i < r1; i += blockDim.x) {
+        c += 1.0;
+        s += sqrt(w ? w[i] : 1.0) * V[i * m];
+    }
+    c = block_sum<double>(c, lds);
+    s = block_sum<double>(s, lds);
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = c;
+        part[2 * blockIdx.x + 1] = s;
+    }
 }
 
 // The plug-in moments' sums over rows (leaveout.py:807-832): per block,
@@ -515,15 +543,20 @@ int hdfe_kss_drop_null(int64_t G, int64_t L, int D, int64_t ncomp, int k, const 
 }
 
 // sigma2 (n, device out) from the outcome (column 0 of V (n, m)), the
-// residuals, the leverage complements and factors; *ybar_out: the outcome's
-// weighted mean.
-int hdfe_kss_sigma2(int64_t n, int m, const double *V, const double *w, const double *e,
-                    const double *Mb, const double *F, double *out, double *ybar_out) {
-    ARG_CHECK(V && e && Mb && F && out, "hdfe_kss_sigma2: null argument");
+// residuals, the leverage complements and factors; centering 0 "weighted",
+// 1 "reference"; *ybar_out: the mean the outcome was centred at.
+int hdfe_kss_sigma2(int64_t n, int m, const double *V, const double *w, int centering,
+                    const double *e, const double *Mb, const double *F, double *out,
+                    double *ybar_out) {
+    ARG_CHECK(V && e && Mb && F && out && (centering == 0 || centering == 1),
+              "hdfe_kss_sigma2: bad argument");
     const int nb = 256;
     DevBuf<double> part;
     DEV_ALLOC(part, 2 * nb);
-    hipLaunchKernelGGL(k_wmean_part, dim3(nb), dim3(256), 0, 0, n, m, V, w, part.p);
+    if (centering == 1)
+        hipLaunchKernelGGL(k_rootmean_part, dim3(nb), dim3(256), 0, 0, n, m, V, w, part.p);
+    else
+        hipLaunchKernelGGL(k_wmean_part, dim3(nb), dim3(256), 0, 0, n, m, V, w, part.p);
     std::vector<double> h;
     HIP_TRY(to_host(h, part.p, 2 * nb, 0));
     double sw = 0.0, swy = 0.0;
@@ -532,7 +565,8 @@ int hdfe_kss_sigma2(int64_t n, int m, const double *V, const double *w, const do
         swy += h[2 * b + 1];
     }
     const double ybar = swy / sw;
-    hipLaunchKernelGGL(k_sigma2, dim3(grid1(n)), dim3(256), 0, 0, n, m, V, w, ybar, e, Mb, F, out);
+    hipLaunchKernelGGL(k_sigma2, dim3(grid1(n)), dim3(256), 0, 0, n, m, V, w, ybar, centering, e,
+                       Mb, F, out);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipDeviceSynchronize());
     if (ybar_out) *ybar_out = ybar;

@@ -18,10 +18,15 @@ cross-moments.  `leave_out_kss` mirrors leaveout.py:1069-1255 with
 leave_out="observation" and `leave_out_components` (:634-805) with the
 default stayer rule ("own") and weighted centring.
 
-Not mirrored yet: covariates in the design (the Schur border, inverse.py:
-182-231) -- the leverages and the bias are those of the fixed-effect design,
-and a model with covariates is refused; leaving out a match
-(leaveout_match.py); standard errors and the weak-identification diagnostic.
+Leaving out a match (the reference's default) follows leaveout_match.py:
+the row model's worker and firm effects plus residual, collapsed to spells,
+refitted with the spells' lengths as weights, and KSS on that fit with the
+within-match stayer rule and the reference centring.
+
+Not mirrored yet: covariates in the leave-out design (the Schur border,
+inverse.py:182-231) -- at observation level a model with covariates is
+refused (at match level the row model may have them: they are partialled out
+before the collapse); standard errors and the weak-identification diagnostic.
 """
 
 from __future__ import annotations
@@ -290,7 +295,8 @@ class TraceEstimate:
 
 def hutchinson_trace(panel, sigma2, n_draws: int = 250, block: int | None = None,
                      seed: int = 0, psi: int = 0, tol: float = 1e-10, maxiter: int = 5000,
-                     precision: str = "mixed", timings: dict | None = None) -> TraceEstimate:
+                     precision: str = "mixed", timings: dict | None = None,
+                     person_years: int | None = None) -> TraceEstimate:
     """tr(Q S^- A'W Omega A S^-) for var(psi), var(alpha) and their
     covariance by Hutchinson's estimator (leaveout.py:321-448).
 
@@ -339,7 +345,7 @@ def hutchinson_trace(panel, sigma2, n_draws: int = 250, block: int | None = None
     var_psi = a1a2 / weight - (a1 / weight) * (a2 / weight)
     var_alpha = b1b2 / weight - (b1 / weight) * (b2 / weight)
     cov = 0.5 * (cross / weight - (a1 / weight) * (b2 / weight) - (b1 / weight) * (a2 / weight))
-    scale = weight / _moment_divisor(weight, inv.n)
+    scale = weight / _moment_divisor(weight, inv.n if person_years is None else person_years)
     if timings is not None:
         timings.update(trace_seconds=seconds, trace_solves=2 * n_draws,
                        trace_iterations=list(inv.iterations))
@@ -368,6 +374,7 @@ class LeaveOutComponents:
     diagnostics: dict = field(default_factory=dict)
     pruning: dict | None = None
     fit: object = None
+    match_fit: object = None
 
     def as_dict(self):
         return dict(self.leave_out)
@@ -399,12 +406,25 @@ def _movers(panel, psi):
 
 def leave_out_components(result, n_draws: int = 250, block: int | None = None, seed: int = 0,
                          psi: str | None = None, tol: float = 1e-10, maxiter: int = 5000,
-                         precision: str = "mixed", timings: dict | None = None
-                         ) -> LeaveOutComponents:
-    """KSS components for a fit leaving out an observation
-    (leaveout.py:634-805): JLA leverages, the leverage check, the plug-in
-    moments, sigma2 with weighted centring and own-estimate stayers, the
-    Hutchinson bias with seed + 1_000_003, and plug-in minus bias."""
+                         precision: str = "mixed", timings: dict | None = None,
+                         stayers: str = "own", stayer_sigma2=None, centering: str = "weighted",
+                         person_years: int | None = None) -> LeaveOutComponents:
+    """KSS components for a fit (leaveout.py:634-805): JLA leverages, the
+    leverage check, the plug-in moments, sigma2, the Hutchinson bias with
+    seed + 1_000_003, and plug-in minus bias.
+
+    stayers "own" (every row its own sigma2) or "within_match" with
+    `stayer_sigma2` (device, the fit's sorted rows; used on stayers' rows,
+    leaveout.py:1012-1016); centering "weighted" or "reference"
+    (leaveout.py:996-1003); `person_years` the R of the divisor T (R - 1) / R
+    when the rows are matches (leaveout_se.py:216-220).  With "within_match"
+    only movers' rows are held to the leverage check (leaveout.py:712-724)."""
+    if stayers not in ("own", "within_match"):
+        raise ValueError("stayers must be 'own' or 'within_match' on the device")
+    if (stayers == "within_match") != (stayer_sigma2 is not None):
+        raise ValueError("stayers='within_match' needs stayer_sigma2")
+    if centering not in ("weighted", "reference"):
+        raise ValueError("centering must be 'weighted' or 'reference'")
     import torch
     panel = result.panel
     info = panel.info
@@ -419,19 +439,25 @@ def leave_out_components(result, n_draws: int = 250, block: int | None = None, s
         raise ValueError(f"psi must be a non-streamed dimension, one of {others}; got {psi!r}")
     pi = others.index(psi)
     n, G = info["n"], info["G"]
+    R = n if person_years is None else int(person_years)
     t = {} if timings is None else timings
     t0 = time.perf_counter()
     lev = jla_leverages(panel, n_draws=n_draws, block=block, seed=seed, tol=tol,
                         maxiter=maxiter, precision=precision, timings=t)
     t1 = time.perf_counter()
-    worst = lev.diagnostics["max_leverage"]
+    movers = _movers(panel, pi)
+    match_level = stayers == "within_match"
+    if match_level:
+        worst = float(lev.leverage[movers].max()) if bool(movers.any()) else 0.0
+    else:
+        worst = lev.diagnostics["max_leverage"]
     if worst >= 1.0 - 1e-9:
+        unit = "match" if match_level else "observation"
         raise ValueError(
             f"maximum leverage is {worst:.6f}: the panel is not leave-one-out connected "
-            "(leaving out one observation at a time), so the leave-out residual is "
-            "undefined for at least one observation. Refit with "
+            f"(leaving out one {unit} at a time), so the leave-out residual is "
+            f"undefined for at least one {unit}. Refit with "
             "connected='leave_one_out', or use leave_out_kss, which does")
-    movers = _movers(panel, pi)
     lib = _lib.load()
     gstart, _ = panel.ptr("gstart")
     codes, _ = panel.ptr("codes")
@@ -445,7 +471,7 @@ def leave_out_components(result, n_draws: int = 250, block: int | None = None, s
                "hdfe_kss_plugin")
     total, sa, sb, saa, sbb, sab = s6
     ma, mb = sa / total, sb / total
-    scale = total / _moment_divisor(total, n)
+    scale = total / _moment_divisor(total, R)
     plug_in = {"var(psi)": scale * (saa / total - ma * ma),
                "var(alpha)": scale * (sbb / total - mb * mb),
                "cov(psi, alpha)": scale * (sab / total - ma * mb)}
@@ -453,13 +479,16 @@ def leave_out_components(result, n_draws: int = 250, block: int | None = None, s
     sigma2 = torch.empty(n, dtype=torch.float64, device="cuda")
     resid = result.resid_sorted.contiguous()
     ybar = C.c_double()
-    _lib.check(lib.hdfe_kss_sigma2(n, info["m"], V, w, resid.data_ptr(),
-                                   lev.complement.data_ptr(), lev.factor.data_ptr(),
-                                   sigma2.data_ptr(), C.byref(ybar)), "hdfe_kss_sigma2")
+    _lib.check(lib.hdfe_kss_sigma2(n, info["m"], V, w, 1 if centering == "reference" else 0,
+                                   resid.data_ptr(), lev.complement.data_ptr(),
+                                   lev.factor.data_ptr(), sigma2.data_ptr(), C.byref(ybar)),
+               "hdfe_kss_sigma2")
+    if match_level:
+        sigma2 = torch.where(movers, sigma2, stayer_sigma2.to(torch.float64))
     t2 = time.perf_counter()
     trace = hutchinson_trace(panel, sigma2, n_draws=n_draws, block=block,
                              seed=seed + TRACE_SEED_OFFSET, psi=pi, tol=tol, maxiter=maxiter,
-                             precision=precision, timings=t)
+                             precision=precision, timings=t, person_years=R)
     t3 = time.perf_counter()
     bias = trace.as_dict()
     leave_out = {k: plug_in[k] - bias[k] for k in plug_in}
@@ -472,26 +501,106 @@ def leave_out_components(result, n_draws: int = 250, block: int | None = None, s
     return LeaveOutComponents(
         plug_in=plug_in, bias=bias, leave_out=leave_out, n_obs=n,
         n_movers=int(movers.sum()), leverages=lev, trace=trace,
-        diagnostics={"psi": psi, "alpha": alpha, "stayers": "own", "max_leverage": worst,
-                     "leave_out": "observation", "sigma2_mean": sigma2_mean,
+        diagnostics={"psi": psi, "alpha": alpha, "stayers": stayers, "max_leverage": worst,
+                     "leave_out": "match" if match_level else "observation",
+                     "centering": centering, "sigma2_mean": sigma2_mean,
                      "ybar": float(ybar.value), "covariates": [],
                      "n_levels": {psi: panel.n_levels[pi], alpha: G},
                      "sigma2": sigma2})
 
 
+def _leave_out_match(fit, psi, n_draws, block, seed, tol, maxiter, precision, timings):
+    """KSS leaving out a whole (worker, firm) match (leaveout_match.py:
+    156-214): the row fit's y~ = the worker effect + the firm effect + the
+    residual collapsed to one row per match (its weighted mean, weight the
+    spell's total weight; leaveout_match.py:63-88), the weighted fit of
+    y ~ 1 | alpha + psi on the matches, sigma_for_stayers on the stayers'
+    person-years (leaveout_match.py:91-132), then `leave_out_components` with
+    the within-match stayer rule, the reference centring and the person-year
+    divisor."""
+    import torch
+
+    from .estimator import StreamingHDFE
+    panel = fit.panel
+    info = panel.info
+    n, D = info["n"], info["D"]
+    others = fit.fe_names[1:]
+    pi = others.index(psi)
+    if pi != 0:
+        raise NotImplementedError("leaving out a match needs psi to be the second fixed effect "
+                                  "of the formula")
+    alpha = fit.fe_names[0]
+    g = panel.tensor("g", torch.int32).long()
+    c0 = panel.tensor("codes", torch.int32).view(D, n)[0].long()
+    w = panel.tensor("w", torch.float64) if info["weighted"] else \
+        torch.ones(n, dtype=torch.float64, device="cuda")
+    off0 = int(panel.offs[0])
+    ytil = fit.geff[g] + fit.gam_y[off0 + c0] + fit.resid_sorted
+    head = torch.ones(n, dtype=torch.bool, device="cuda")
+    head[1:] = (g[1:] != g[:-1]) | (c0[1:] != c0[:-1])
+    mid = torch.cumsum(head.long(), 0) - 1
+    nm = int(mid[-1]) + 1
+    T = torch.zeros(nm, dtype=torch.float64, device="cuda").index_add_(0, mid, w)
+    ybar = torch.zeros(nm, dtype=torch.float64, device="cuda").index_add_(0, mid, w * ytil) / T
+    first = torch.nonzero(head, as_tuple=True)[0]
+    glev = panel.tensor("glevels", torch.int64)
+    lev0 = panel.tensor("levels0", torch.int64)
+    mdata = {alpha: glev[g[first]], psi: lev0[c0[first]], "__ybar__": ybar, "__T__": T}
+    mfit = StreamingHDFE("__ybar__", [], [alpha, psi], weights="__T__", stream=alpha,
+                         fixef_rm="none", keep_intermediates=True, tol=tol,
+                         maxiter=maxiter).fit(mdata)
+    mp = mfit.panel
+    # the match fit factorises the same workers and firms: the same codes
+    if not (torch.equal(mp.tensor("glevels", torch.int64), glev)
+            and torch.equal(mp.tensor("levels0", torch.int64), lev0)):
+        raise RuntimeError("internal: the match fit's levels differ from the row fit's")
+    # sigma_for_stayers on the person-year rows
+    ybar_all = float((w * ytil).sum() / w.sum())
+    mn = mp.info["n"]
+    mg = mp.tensor("g", torch.int32).long()
+    fitted = mfit.geff[g] + mfit.gam_y[int(mp.offs[0]) + c0]
+    W_worker = torch.zeros(info["G"], dtype=torch.float64, device="cuda").index_add_(0, g, w)[g]
+    r = (ytil - ybar_all) * (ytil - fitted) / (1.0 - w / W_worker)
+    firms = torch.zeros(info["G"], dtype=torch.long, device="cuda").index_add_(
+        0, g[first], torch.ones_like(first))
+    stay_row = firms[g] == 1
+    num = torch.zeros(nm, dtype=torch.float64, device="cuda").index_add_(
+        0, mid[stay_row], (w * w * r)[stay_row])
+    den = torch.zeros(nm, dtype=torch.float64, device="cuda").index_add_(
+        0, mid[stay_row], w[stay_row])
+    st_tab = torch.where(den > 0, num / torch.where(den > 0, den, torch.ones_like(den)),
+                         torch.full_like(num, float("nan")))
+    # the table's rows in the match fit's sorted order
+    perm = mp.tensor("perm", torch.int32).long()[:mn]
+    st_sorted = st_tab[perm].contiguous()
+    comps = leave_out_components(mfit, n_draws=n_draws, block=block, seed=seed, psi=psi,
+                                 tol=tol, maxiter=maxiter, precision=precision, timings=timings,
+                                 stayers="within_match", stayer_sigma2=st_sorted,
+                                 centering="reference", person_years=n)
+    comps.n_obs = n
+    comps.diagnostics["n_matches"] = mn
+    comps.diagnostics["person_years"] = n
+    comps.match_fit = mfit
+    return comps
+
+
 def leave_out_kss(fml: str, data, *, psi: str | None = None, n_draws: int = 250,
                   seed: int = 0, block: int | None = None, prune: bool = True,
-                  leave_out: str = "observation", stayers: str | None = None,
+                  leave_out: str = "match", stayers: str | None = None,
+                  centering: str = "reference",
                   precision: str = "mixed", tol: float = 1e-10, maxiter: int = 5000,
                   timings: dict | None = None, **options) -> LeaveOutComponents:
     """Kline-Saggio-Solvsten leave-out variance components, end to end, on
     the device (leaveout.py:1069-1255): prune to the largest leave-one-out
     connected set of (alpha, psi), fit with alpha streamed and no singleton
-    drop, then `leave_out_components`.
+    drop, then leave out a match (the default, `_leave_out_match`) or an
+    observation (`leave_out_components`).
 
     `alpha` is the formula's first fixed effect and `psi` (default: the
-    second) the dimension whose variance is wanted.  Only
-    leave_out="observation" and stayers="own" are implemented here."""
+    second) the dimension whose variance is wanted.  At observation level the
+    stayer rule is "own" and the centring weighted; at match level the
+    within-match rule and the reference centring (the reference's defaults
+    there)."""
     import torch
 
     from .connected import leave_one_out
@@ -499,9 +608,14 @@ def leave_out_kss(fml: str, data, *, psi: str | None = None, n_draws: int = 250,
 
     if leave_out not in ("match", "observation"):
         raise ValueError("leave_out must be 'match' or 'observation'")
-    if leave_out == "match":
-        raise NotImplementedError("leaving out a match is not implemented on the device; "
-                                  "use leave_out='observation'")
+    if centering not in ("reference", "weighted"):
+        raise ValueError("centering must be 'reference' or 'weighted'")
+    if leave_out == "match" and stayers is not None:
+        raise ValueError("stayers applies when leaving out an observation; leaving out a "
+                         "match uses LeaveOutTwoWay's within-match rule")
+    if leave_out == "match" and centering != "reference":
+        raise NotImplementedError("leaving out a match uses the reference centring on the "
+                                  "device")
     if stayers not in (None, "own"):
         raise NotImplementedError("only stayers='own' is implemented on the device")
     if "|" not in fml:
@@ -541,8 +655,11 @@ def leave_out_kss(fml: str, data, *, psi: str | None = None, n_draws: int = 250,
     t1 = time.perf_counter()
     fit = StreamingHDFE(y, xs, fe, keep_intermediates=True, **options).fit(data)
     t2 = time.perf_counter()
-    comps = leave_out_components(fit, n_draws=n_draws, block=block, seed=seed, psi=psi,
-                                 tol=tol, maxiter=maxiter, precision=precision, timings=t)
+    if leave_out == "match":
+        comps = _leave_out_match(fit, psi, n_draws, block, seed, tol, maxiter, precision, t)
+    else:
+        comps = leave_out_components(fit, n_draws=n_draws, block=block, seed=seed, psi=psi,
+                                     tol=tol, maxiter=maxiter, precision=precision, timings=t)
     t.update(prune_seconds=t1 - t0, fit_seconds=t2 - t1,
              total_seconds=time.perf_counter() - t0)
     comps.pruning = pruning

@@ -260,11 +260,14 @@ int hdfe_kss_drop_null(int64_t G, int64_t L, int D, int64_t ncomp, int k, const 
                        const int *codes0, const int *lcomp, const int64_t *offs_h,
                        const int64_t *nlev_h, double *Vg, double *Vl);
 
-/* sigma2_i = w_i (y_i - ybar) e_i / M_ii * factor_i, y column 0 of V (n, m),
- * ybar its weighted mean (*ybar_out); device in/out
- * (_leave_one_out_sigma2, leaveout.py:949-1016, centering "weighted"). */
-int hdfe_kss_sigma2(int64_t n, int m, const double *V, const double *w, const double *e,
-                    const double *Mb, const double *F, double *out, double *ybar_out);
+/* sigma2_i from y (column 0 of V (n, m)), the residuals e, the leverage
+ * complements Mb and factors F, device in/out (_leave_one_out_sigma2,
+ * leaveout.py:949-1016): centering 0 "weighted", w_i (y_i - ybar_w) e_i /
+ * Mb_i F_i; centering 1 "reference", (sqrt(w_i) y_i - mean sqrt(w) y)
+ * sqrt(w_i) e_i / Mb_i F_i; *ybar_out the mean used. */
+int hdfe_kss_sigma2(int64_t n, int m, const double *V, const double *w, int centering,
+                    const double *e, const double *Mb, const double *F, double *out,
+                    double *ybar_out);
 
 /* The plug-in moments' sums (host out, 6): sum w, w a, w b, w a^2, w b^2,
  * w a b with a the row's psi effect gam[psi_off + code] and b its group

@@ -464,24 +464,32 @@ def combine_moments(moments: np.ndarray, n_draws: int):
 
 def jla_leverages(p0: dict, c: dict, S, dinv, n_draws: int, block: int, seed: int = 0,
                   tol: float = 1e-10, maxiter: int = 5000):
-    """Leverages of the fixed-effect design (no covariates, unweighted) by
-    the reference's JLA: per block, centre, solve, project, accumulate the
-    five moments (leaveout.py:130-198, inverse.py:240-309)."""
+    """Leverages of the fixed-effect design (no covariates) by the
+    reference's JLA: per block, centre, solve, project, accumulate the five
+    moments (leaveout.py:130-198, inverse.py:240-309).  With weights the
+    operator is the square-root-weight one, W^1/2 A S^- A' W^1/2: the draws
+    enter divided by sqrt(w) and the projection leaves multiplied by it
+    (inverse.py:299-305, leaveout.py:179)."""
     g, codes = p0["g"], p0["codes"]
     n = len(g)
     starts = np.concatenate(([0], np.flatnonzero(g[1:] != g[:-1]) + 1, [n])).astype(np.int64)
-    w = np.ones(n) if p0["w"] is None else p0["w"]
+    weighted = p0["w"] is not None
+    w = np.ones(n) if not weighted else np.asarray(p0["w"], np.float64)
+    root = np.sqrt(w)
     X0 = np.zeros((n, 0))
     moments = np.zeros((5, n))
     drawn = 0
     while drawn < n_draws:
         size = min(block, n_draws - drawn)
         R = rademacher(0, n, size, seed, drawn)
-        acc, _ = center_rows(starts, codes, c["offs"], w, X0, R, c["L"])
-        scale = np.sqrt(w @ (R * R))
+        Rin = R / root[:, None] if weighted else R
+        acc, _ = center_rows(starts, codes, c["offs"], w, X0, Rin, c["L"])
+        scale = np.sqrt(w @ (Rin * Rin))
         zero = np.linalg.norm(acc, axis=0) <= RHS_ZERO_TOL * scale
         U, _, _, _ = pcg(S, dinv, acc, zero, None, tol, maxiter)
-        proj = project_rows(starts, codes, c["offs"], w, X0, np.zeros((0, size)), R, U)
+        proj = project_rows(starts, codes, c["offs"], w, X0, np.zeros((0, size)), Rin, U)
+        if weighted:
+            proj = proj * root[:, None]
         jla_moments(R, proj, moments)
         drawn += size
     return moments
@@ -590,7 +598,7 @@ def trace_components(totals: np.ndarray, n_draws: int, rows: int):
 
 
 def hutchinson_trace(ctx: KSSContext, sigma2: np.ndarray, n_draws: int, block: int,
-                     seed: int, psi_col: int = 0):
+                     seed: int, psi_col: int = 0, n_rows: int | None = None):
     """tr(Q S^- A'W Omega A S^-) for the three components by Hutchinson's
     estimator with coefficient-space Rademacher draws keyed on (position,
     draw), levels first then groups (leaveout.py:321-448)."""
@@ -608,7 +616,7 @@ def hutchinson_trace(ctx: KSSContext, sigma2: np.ndarray, n_draws: int, block: i
         Ul, Ug, _ = apply_inverse(ctx, vl, vg)
         totals += trace_moments(ctx, psi_col, Zl, Zg, Ul, Ug)
         drawn += size
-    return trace_components(totals, n_draws, len(ctx.g)), totals
+    return trace_components(totals, n_draws, len(ctx.g) if n_rows is None else n_rows), totals
 
 
 def sigma2(y: np.ndarray, e: np.ndarray, w: np.ndarray, complement: np.ndarray,
@@ -661,6 +669,81 @@ def leave_out_kss(data: dict, y: str = "y", xs=(), alpha: str = "worker_id",
             "leave_out": tuple(p - q for p, q in zip(plug, bias)), "keep": keep,
             "prune": prune, "fit": f, "sigma2": s2, "M_bar": M_bar, "factor": factor,
             "moments": moments, "totals": totals, "n": n}
+
+
+def leave_out_match(data: dict, y: str = "y", xs=(), alpha: str = "worker_id",
+                    psi: str = "firm_id", others=(), n_draws: int = 250, block: int = 256,
+                    seed: int = 0, tol: float = 1e-10) -> dict:
+    """KSS leaving out a whole match (leaveout_match.py:156-214, 63-132;
+    leaveout.py:634-805 with stayers "within_match" and the reference
+    centring): prune as leave_out_kss, fit the row model, collapse y~ = the
+    worker effect + the firm effect + the residual to one row per (worker,
+    firm) match (its weighted mean, weight = the spell's total weight), fit
+    y ~ 1 | alpha + psi on the matches with those weights, JLA leverages in
+    the square-root-weight metric, sigma2 = (sqrt(T) ybar - its mean over
+    matches) sqrt(T) e / M_ii * factor for movers' matches and
+    sigma_for_stayers for stayers', the Hutchinson bias, the plug-in moments
+    with the person-year divisor.  Unweighted rows."""
+    a_ids, b_ids = np.asarray(data[alpha]), np.asarray(data[psi])
+    ua, ia = np.unique(a_ids, return_inverse=True)
+    ub, ib = np.unique(b_ids, return_inverse=True)
+    alive, prune = leave_one_out(ia, ib, len(ua), len(ub))
+    keep = alive[ia] & alive[len(ua) + ib]
+    sub = {k: np.asarray(v)[keep] for k, v in data.items()
+           if hasattr(v, "__len__") and len(v) == len(keep)}
+    f = fit(sub, y, xs, [alpha, psi] + list(others), tol=tol, fixef_rm="none", stream=alpha)
+    p0, c = f["p0"], f["cells"]
+    g, pc = p0["g"], p0["codes"][:, 0]
+    n = len(g)
+    w_row = np.ones(n)
+    ytil = f["worker_effects"][g] + f["gam_y"][c["offs"][0] + pc] + f["resid"]
+    head = np.concatenate(([True], (g[1:] != g[:-1]) | (pc[1:] != pc[:-1])))
+    mid = np.cumsum(head) - 1
+    nm = int(mid[-1]) + 1
+    T = np.bincount(mid, weights=w_row, minlength=nm)
+    ybar = np.bincount(mid, weights=w_row * ytil, minlength=nm) / T
+    first = np.flatnonzero(head)
+    R = n
+    mdata = {alpha: p0["g_levels"][g[first]], psi: p0["levels"][0][pc[first]], "y": ybar,
+             "w": T}
+    mf = fit(mdata, "y", (), [alpha, psi], weights="w", tol=tol, fixef_rm="none", stream=alpha)
+    mp0, mc = mf["p0"], mf["cells"]
+    mg, mpc = mp0["g"], mp0["codes"][:, 0]
+    # the match fit keeps the table's (worker, firm) order
+    assert np.array_equal(mg, g[first]) and np.array_equal(mpc, pc[first])
+    moments = jla_leverages(mp0, mc, mf["S"], mf["dinv"], n_draws, block, seed, tol)
+    _, M_bar, factor = combine_moments(moments, n_draws)
+    P_bar = 1.0 - M_bar
+    nf = np.bincount(g, minlength=int(g.max()) + 1)          # rows per worker
+    firms = np.bincount(mg, minlength=int(g.max()) + 1)      # matches per worker
+    movers = firms[mg] > 1
+    worst = float(P_bar[movers].max()) if movers.any() else 0.0
+    a = mf["gam_y"][mc["offs"][0] + mpc]
+    b = mf["worker_effects"][mg]
+    Tm = mp0["w"]
+    plug = plug_in(a, b, Tm, R)
+    root = np.sqrt(Tm)
+    yt = root * mp0["V"][:, 0]
+    s2 = np.full(nm, np.nan)
+    s2[movers] = ((yt[movers] - yt.mean()) * root[movers] * mf["resid"][movers]
+                  / M_bar[movers] * factor[movers])
+    # sigma_for_stayers (leaveout_match.py:91-132) on the person-year rows
+    ybar_all = float(w_row @ ytil / w_row.sum())
+    fitted = mf["worker_effects"][g] + mf["gam_y"][mc["offs"][0] + pc]
+    Wk = np.bincount(g, weights=w_row)[g]
+    r = (ytil - ybar_all) * (ytil - fitted) / (1.0 - w_row / Wk)
+    st_rows = firms[g] == 1
+    num = np.bincount(mid[st_rows], weights=(w_row ** 2 * r)[st_rows], minlength=nm)
+    den = np.bincount(mid[st_rows], weights=w_row[st_rows], minlength=nm)
+    s2[~movers] = num[~movers] / den[~movers]
+    ctx = KSSContext(mf, tol)
+    bias, totals = hutchinson_trace(ctx, s2, n_draws, block, seed + 1_000_003, 0, n_rows=R)
+    del nf
+    return {"plug_in": plug, "bias": bias,
+            "leave_out": tuple(p - q for p, q in zip(plug, bias)), "keep": keep,
+            "prune": prune, "fit": f, "match_fit": mf, "sigma2": s2, "M_bar": M_bar,
+            "factor": factor, "moments": moments, "person_years": R, "n_matches": nm,
+            "movers": movers, "max_leverage": worst}
 
 
 # ------------------------------------------------- differentiable fits

@@ -10,8 +10,10 @@ import pytest
     ("y ~ 1 | worker_id + firm_id", {"psi": "worker_id"}, ValueError),
     ("y ~ 1 | worker_id + firm_id", {"psi": "year"}, ValueError),
     ("y ~ 1 | worker_id + firm_id", {"leave_out": "spell"}, ValueError),
-    ("y ~ 1 | worker_id + firm_id", {"leave_out": "match"}, NotImplementedError),
-    ("y ~ 1 | worker_id + firm_id", {"stayers": "drop"}, NotImplementedError),
+    ("y ~ 1 | worker_id + firm_id", {"stayers": "own"}, ValueError),     # match level
+    ("y ~ 1 | worker_id + firm_id", {"centering": "mean"}, ValueError),
+    ("y ~ 1 | worker_id + firm_id", {"leave_out": "observation", "stayers": "drop"},
+     NotImplementedError),
 ])
 def test_bad_arguments_are_refused_before_any_work(fml, kw, err):
     from hdfe_stream_amd.leaveout import leave_out_kss

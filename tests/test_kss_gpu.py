@@ -94,7 +94,7 @@ def test_leave_out_kss_matches_the_oracle(oracle_mod, precision):
     print(f"leave_out_kss precision={precision}", flush=True)
     tol = 1e-12 if precision == "fp64" else 1e-10
     lo = leave_out_kss("y ~ 1 | worker_id + firm_id", d, n_draws=64, seed=2,
-                       precision=precision, tol=tol)
+                       precision=precision, tol=tol, leave_out="observation")
     ref = O.leave_out_kss(d, n_draws=64, block=64, seed=2, tol=1e-12)
     # the pruning is integer and exact
     assert lo.n_obs == int(ref["keep"].sum()) == ref["n"]
@@ -127,9 +127,9 @@ def test_unit_weights_give_the_unweighted_components():
     d = kss_panel(3000, 300, seed=4)
     d["w"] = np.ones(len(d["y"]))
     a = leave_out_kss("y ~ 1 | worker_id + firm_id", d, n_draws=32, seed=1,
-                      precision="fp64", tol=1e-12)
+                      precision="fp64", tol=1e-12, leave_out="observation")
     b = leave_out_kss("y ~ 1 | worker_id + firm_id", d, n_draws=32, seed=1,
-                      precision="fp64", tol=1e-12, weights="w")
+                      precision="fp64", tol=1e-12, weights="w", leave_out="observation")
     for k in a.leave_out:
         assert b.leave_out[k] == pytest.approx(a.leave_out[k], rel=1e-8, abs=1e-12)
         assert b.plug_in[k] == pytest.approx(a.plug_in[k], rel=1e-12, abs=1e-15)
@@ -140,4 +140,40 @@ def test_covariates_are_refused():
     from hdfe_stream_amd.simulate import kss_panel
     d = kss_panel(500, 60, seed=4)
     with pytest.raises(NotImplementedError):
-        leave_out_kss("y ~ x1 | worker_id + firm_id", d, n_draws=8)
+        leave_out_kss("y ~ x1 | worker_id + firm_id", d, n_draws=8, leave_out="observation")
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_leave_out_match_matches_the_oracle(oracle_mod, precision):
+    from hdfe_stream_amd.leaveout import leave_out_kss
+    from hdfe_stream_amd.simulate import kss_panel
+    O = oracle_mod
+    d = kss_panel(4000, 400, seed=3)
+    print(f"leave_out_kss match, {precision}", flush=True)
+    tol = 1e-12 if precision == "fp64" else 1e-10
+    lo = leave_out_kss("y ~ 1 | worker_id + firm_id", d, n_draws=64, seed=2,
+                       precision=precision, tol=tol)
+    ref = O.leave_out_match(d, n_draws=64, block=64, seed=2, tol=1e-12)
+    assert lo.diagnostics["leave_out"] == "match"
+    assert lo.n_obs == ref["person_years"]
+    assert lo.diagnostics["n_matches"] == ref["n_matches"]
+    assert lo.n_movers == int(ref["movers"].sum())
+    names = ["var(psi)", "var(alpha)", "cov(psi, alpha)"]
+    assert rel([lo.plug_in[k] for k in names], ref["plug_in"]) < 1e-9
+    bound = 1e-7 if precision == "fp64" else 1e-5
+    assert rel([lo.bias[k] for k in names], ref["bias"]) < bound
+    assert rel([lo.leave_out[k] for k in names], ref["leave_out"]) < bound
+    s2 = lo.diagnostics["sigma2"].cpu().numpy()
+    assert rel(s2, ref["sigma2"]) < 1e-5
+
+
+def test_leave_out_match_with_covariates_and_years():
+    """The row model's covariates and further effects are partialled out
+    before the collapse, so they are allowed at match level."""
+    from hdfe_stream_amd.leaveout import leave_out_kss
+    from hdfe_stream_amd.simulate import kss_panel
+    d = kss_panel(3000, 300, seed=4)
+    d["x1"] = np.random.default_rng(0).standard_normal(len(d["y"]))
+    lo = leave_out_kss("y ~ x1 | worker_id + firm_id + year", d, n_draws=32, seed=1)
+    assert lo.diagnostics["leave_out"] == "match"
+    assert lo.leave_out["var(psi)"] < lo.plug_in["var(psi)"]

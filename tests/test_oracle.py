@@ -376,3 +376,31 @@ def test_jla_leverages_estimate_the_exact_ones(oracle_mod):
     assert np.corrcoef(P, exact)[0, 1] > 0.9
     mom2 = O.jla_leverages(p0, c, S, dinv, n_draws=1000, block=333, seed=3)
     np.testing.assert_allclose(mom2, mom, rtol=1e-10, atol=1e-12)
+
+
+def test_weighted_jla_estimates_the_square_root_weight_leverages(oracle_mod):
+    """With weights the leverages are those of W^1/2 A (A'WA)^+ A' W^1/2
+    (inverse.py:299-305): mean within 0.005, correlation above 0.9."""
+    O = oracle_mod
+    from hdfe_stream_amd.simulate import weighted_panel
+    d = weighted_panel(300, 30, seed=6)
+    p0 = O.pass0(d, "y", ["x1"], FE, weights="w")
+    c = O.pass1(p0)
+    S = O.build_S(c)
+    dinv = O.jacobi(O.diag(c))
+    mom = O.jla_leverages(p0, c, S, dinv, n_draws=1000, block=250, seed=3)
+    P, M, F = O.combine_moments(mom, 1000)
+    n = len(p0["g"])
+    starts = np.concatenate(([0], np.flatnonzero(p0["g"][1:] != p0["g"][:-1]) + 1, [n]))
+    Dw = np.zeros((n, len(starts) - 1))
+    Dw[np.arange(n), np.repeat(np.arange(len(starts) - 1), np.diff(starts))] = 1
+    Do = np.zeros((n, c["L"]))
+    for k in range(p0["codes"].shape[1]):
+        Do[np.arange(n), c["offs"][k] + p0["codes"][:, k]] = 1
+    A = np.hstack([Dw, Do])
+    w = p0["w"]
+    r = np.sqrt(w)
+    H = (r[:, None] * A) @ np.linalg.pinv(A.T @ (w[:, None] * A)) @ (A.T * r)
+    exact = np.diag(H)
+    assert abs(P.mean() - exact.mean()) < 0.005
+    assert np.corrcoef(P, exact)[0, 1] > 0.9

@@ -219,3 +219,80 @@ def test_leave_out_kss_approaches_the_exact_estimator(oracle_mod):
     np.testing.assert_allclose(r["leave_out"], np.subtract(r["plug_in"], r["bias"]), rtol=1e-15)
     a = f["gam_y"][ctx.c["offs"][0] + ctx.p0["codes"][:, 0]]
     np.testing.assert_allclose(r["plug_in"][0], np.var(a, ddof=1), rtol=1e-12)
+
+
+def test_match_level_plug_in_equals_the_row_level_one(oracle_mod):
+    """Collapsing y~ = alpha + psi + e to match means and refitting with the
+    spells' lengths as weights gives back the row fit's effects (the
+    within-match sum of squares does not involve them), so the plug-in
+    moments with the person-year divisor are the row level's
+    (leaveout_match.py:63-88)."""
+    O = oracle_mod
+    from hdfe_stream_amd.simulate import kss_panel
+    d = kss_panel(1500, 150, seed=5)
+    r = O.leave_out_match(d, n_draws=8, block=8, seed=0)
+    f = r["fit"]
+    p0, c = f["p0"], f["cells"]
+    a = f["gam_y"][c["offs"][0] + p0["codes"][:, 0]]
+    b = f["worker_effects"][p0["g"]]
+    row = O.plug_in(a, b, np.ones(len(a)), len(a))
+    np.testing.assert_allclose(r["plug_in"], row, rtol=1e-7, atol=1e-12)
+
+
+def test_stayer_sigma2_is_the_within_spell_leave_one_out_product(oracle_mod):
+    """A stayer's single match has leverage 1, and sigma_for_stayers
+    (leaveout_match.py:91-132) uses its person-years instead: the mean over
+    the spell of (y~_it - ybar) times y~_it less the mean of the spell's
+    other years."""
+    O = oracle_mod
+    from hdfe_stream_amd.simulate import kss_panel
+    d = kss_panel(1500, 150, seed=5)
+    r = O.leave_out_match(d, n_draws=8, block=8, seed=0)
+    f = r["fit"]
+    p0, c = f["p0"], f["cells"]
+    g, pc = p0["g"], p0["codes"][:, 0]
+    ytil = f["worker_effects"][g] + f["gam_y"][c["offs"][0] + pc] + f["resid"]
+    ybar_all = ytil.mean()
+    mg = r["match_fit"]["p0"]["g"]
+    stay = np.flatnonzero(~r["movers"])[:20]
+    for m in stay:
+        rows = np.flatnonzero(g == mg[m])
+        yy = ytil[rows]
+        T = len(yy)
+        loo = np.array([yy[i] - (yy.sum() - yy[i]) / (T - 1) for i in range(T)])
+        want = np.mean((yy - ybar_all) * loo)
+        assert r["sigma2"][m] == pytest.approx(want, rel=1e-9, abs=1e-14)
+
+
+def test_match_level_bias_approaches_the_exact_trace(oracle_mod):
+    """With many draws the match-level bias is close to the exact trace of
+    the weighted collapsed design with the same sigma2, and the leverages to
+    the exact square-root-weight ones."""
+    O = oracle_mod
+    from hdfe_stream_amd.simulate import kss_panel
+    d = kss_panel(250, 25, seed=6)
+    r = O.leave_out_match(d, n_draws=4000, block=500, seed=1)
+    mf = r["match_fit"]
+    ctx = O.KSSContext(mf)
+    Dg, Do = O.design(mf["p0"], mf["cells"])
+    A = np.hstack([Dg.toarray(), Do.toarray()])
+    w = ctx.w
+    nm = A.shape[0]
+    G = ctx.G
+    s2 = r["sigma2"]
+    Sp = np.linalg.pinv(A.T @ (w[:, None] * A))
+    lev = np.einsum("ij,ji->i", np.sqrt(w)[:, None] * A, Sp @ (A.T * np.sqrt(w)))
+    assert abs((1 - r["M_bar"]).mean() - lev.mean()) < 0.01
+    M = Sp @ A.T @ ((w * s2)[:, None] * A) @ Sp
+    Ep = np.zeros((nm, A.shape[1]))
+    Ep[np.arange(nm), G + ctx.c["offs"][0] + ctx.p0["codes"][:, 0]] = 1
+    Ea = np.zeros((nm, A.shape[1]))
+    Ea[np.arange(nm), ctx.g] = 1
+    W = w.sum()
+    Cm = np.diag(w) / W - np.outer(w, w) / W ** 2
+    R = r["person_years"]
+    scale = W / O.moment_divisor(W, R)
+    exact = [np.trace(Q @ M) * scale for Q in
+             (Ep.T @ Cm @ Ep, Ea.T @ Cm @ Ea, 0.5 * (Ep.T @ Cm @ Ea + Ea.T @ Cm @ Ep))]
+    for b, e in zip(r["bias"], exact):
+        assert abs(b - e) < 0.1 * abs(e)
