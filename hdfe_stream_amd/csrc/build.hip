@@ -532,28 +532,40 @@ __global__ void k_diag_rhs_part(int64_t nchunk, int m, const int *__restrict__ c
                                 const double *__restrict__ iu, const double *__restrict__ is,
                                 const double *__restrict__ Ng, const double *__restrict__ vbar,
                                 double *__restrict__ part) {
+    // one sweep of the chunk's incidences per block of eight columns (one
+    // sweep in all for m <= 8): diag and the rhs columns together
     const int lane = threadIdx.x & 63;
     const int64_t waves = (int64_t)gridDim.x * (blockDim.x / 64);
     for (int64_t c = blockIdx.x * (int64_t)(blockDim.x / 64) + threadIdx.x / 64; c < nchunk;
          c += waves) {
         const int a = chunk_lvl[c];
         const int b = chunk_beg[c], e = min(b + kChunk, lptr[a + 1]);
-        double d = 0.0;
-        for (int t = b + lane; t < e; t += 64) {
-            const int q = lq[t];
-            const double u = iu[q];
-            d += u - u * u / Ng[ig[q]];
-        }
-        d = wave_sum(d);
-        if (lane == 0) part[c * (m + 1)] = d;
-        for (int j = 0; j < m; ++j) {
-            double r = 0.0;
+        for (int j0 = 0; j0 < m || j0 == 0; j0 += 8) {
+            const int nj = min(8, m - j0);
+            double d = 0.0, r[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) r[j] = 0.0;
             for (int t = b + lane; t < e; t += 64) {
                 const int q = lq[t];
-                r += is[(int64_t)q * m + j] - iu[q] * vbar[(int64_t)ig[q] * m + j];
+                const double u = iu[q];
+                const int gq = ig[q];
+                if (j0 == 0) d += u - u * u / Ng[gq];
+#pragma unroll
+                for (int j = 0; j < 8; ++j)
+                    if (j < nj)
+                        r[j] += is[(int64_t)q * m + j0 + j] - u * vbar[(int64_t)gq * m + j0 + j];
             }
-            r = wave_sum(r);
-            if (lane == 0) part[c * (m + 1) + 1 + j] = r;
+            if (j0 == 0) {
+                d = wave_sum(d);
+                if (lane == 0) part[c * (m + 1)] = d;
+            }
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                if (j >= nj) break;
+                const double v = wave_sum(r[j]);
+                if (lane == 0) part[c * (m + 1) + 1 + j0 + j] = v;
+            }
+            if (m == 0) break;
         }
     }
 }
@@ -976,13 +988,27 @@ __global__ void k_uf_flatten(int64_t n, int *parent) {
 }
 
 // rows per component root (exact: integer counts)
+// A wave whose active lanes share one root adds their sum with one atomic
+// (a giant component would otherwise serialise every level on one address).
 __global__ void k_comp_rows(int64_t n, const int *__restrict__ root,
                             const double *__restrict__ cnt, unsigned long long *__restrict__ rows,
                             int *__restrict__ is_root) {
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        atomicAdd(&rows[root[i]], (unsigned long long)cnt[i]);
-        is_root[i] = root[i] == (int)i;
+    const int64_t span = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n; base += span) {
+        const int64_t i = base + threadIdx.x;
+        const bool ok = i < n;
+        const int r = ok ? root[i] : -1;
+        const unsigned long long c = ok ? (unsigned long long)cnt[i] : 0ull;
+        if (ok) is_root[i] = r == (int)i;
+        const int r0 = __shfl(r, 0, 64);
+        const bool uniform = __all(!ok || r == r0) && r0 >= 0;
+        if (uniform) {
+            unsigned long long s = c;
+            for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+            if ((threadIdx.x & 63) == 0) atomicAdd(&rows[r0], s);
+        } else if (ok) {
+            atomicAdd(&rows[r], c);
+        }
     }
 }
 
@@ -1015,39 +1041,88 @@ __global__ void k_comp_label(int64_t n, const int *__restrict__ root,
         comp[i] = rank_of_root[root[i]];
 }
 
-__global__ void k_comp_level_keys(int64_t nl, const int *__restrict__ comp,
-                                  uint64_t *__restrict__ key, int *__restrict__ val,
-                                  int *__restrict__ ccnt) {
+__global__ void k_comp_level_keys(int64_t nl, int shift, const int *__restrict__ comp,
+                                  uint64_t *__restrict__ key, int *__restrict__ val) {
     for (int64_t a = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; a < nl;
          a += (int64_t)gridDim.x * blockDim.x) {
-        key[a] = ((uint64_t)comp[a] << 32) | (uint64_t)a;
+        key[a] = ((uint64_t)comp[a] << shift) | (uint64_t)a;
         val[a] = (int)a;
-        atomicAdd(&ccnt[comp[a]], 1);
     }
+}
+
+// cptr (nc + 1) from the levels sorted by component: where each component's
+// run starts (every component has a level, its root).
+__global__ void k_comp_counts(int64_t nl, int shift, const uint64_t *__restrict__ skey,
+                              int *__restrict__ cptr) {
+    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < nl;
+         p += (int64_t)gridDim.x * blockDim.x) {
+        const int c = (int)(skey[p] >> shift);
+        const int prev = p > 0 ? (int)(skey[p - 1] >> shift) : -1;
+        for (int q = prev + 1; q <= c; ++q) cptr[q] = (int)p;
+        if (p == nl - 1) cptr[c + 1] = (int)nl;
+    }
+}
+
+// Chunks of at most kCompChunk sorted levels within one component.
+constexpr int kCompChunk = 4096;
+
+__global__ void k_comp_nchunk(int64_t nc, const int *__restrict__ cptr, int *__restrict__ nch) {
+    for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c <= nc;
+         c += (int64_t)gridDim.x * blockDim.x)
+        nch[c] = c < nc ? max(1, (cptr[c + 1] - cptr[c] + kCompChunk - 1) / kCompChunk) : 0;
+}
+
+__global__ void k_comp_chunk_ends(int64_t nc, const int *__restrict__ cptr,
+                                  const int *__restrict__ chp, int *__restrict__ ch_end) {
+    for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < nc;
+         c += (int64_t)gridDim.x * blockDim.x)
+        for (int q = chp[c]; q < chp[c + 1]; ++q) ch_end[q] = cptr[c + 1];
+}
+
+__global__ void k_comp_chunks(int64_t nc, const int *__restrict__ cptr,
+                              const int *__restrict__ chp, int *__restrict__ ch_beg) {
+    for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < nc;
+         c += (int64_t)gridDim.x * blockDim.x)
+        for (int q = chp[c], b = cptr[c]; q < chp[c + 1]; ++q, b += kCompChunk) ch_beg[q] = b;
 }
 
 // --------------------------------------------------- normalisation
 
-// Component means of Gamma's dimension-0 block, weighted by obs: levels
-// sorted by component (cptr over ncomp); one workgroup per component, its
-// threads strided over the levels, then a fixed tree.
-__global__ void k_comp_means(int64_t nc, int k, const int *__restrict__ cptr,
-                             const int *__restrict__ clev, const double *__restrict__ obs,
-                             const double *__restrict__ Gm, double *__restrict__ mean) {
+// Component means of Gamma's dimension-0 block, weighted by obs, from the
+// levels sorted by component: a workgroup per chunk (at most kCompChunk
+// levels of one component) writes (sum obs, sum obs G_j), then each
+// component adds its chunks in order.
+__global__ void k_comp_chunk_sums(int64_t nchunk, int k, const int *__restrict__ ch_beg,
+                                  const int *__restrict__ chp_end, const int *__restrict__ clev,
+                                  const double *__restrict__ obs, const double *__restrict__ Gm,
+                                  double *__restrict__ part) {
     __shared__ double lds[T256 / 64];
-    for (int64_t c = blockIdx.x; c < nc; c += gridDim.x) {
-        double sw = 0.0;
-        for (int p = cptr[c] + threadIdx.x; p < cptr[c + 1]; p += blockDim.x) sw += obs[clev[p]];
-        sw = block_sum<double>(sw, lds);
-        for (int j = 0; j < k; ++j) {
-            double s = 0.0;
-            for (int p = cptr[c] + threadIdx.x; p < cptr[c + 1]; p += blockDim.x) {
+    for (int64_t q = blockIdx.x; q < nchunk; q += gridDim.x) {
+        const int b = ch_beg[q], e = min(b + kCompChunk, chp_end[q]);
+        for (int j = -1; j < k; ++j) {
+            double v = 0.0;
+            for (int p = b + threadIdx.x; p < e; p += blockDim.x) {
                 const int a = clev[p];
-                s += obs[a] * Gm[(int64_t)a * k + j];
+                v += j < 0 ? obs[a] : obs[a] * Gm[(int64_t)a * k + j];
             }
-            s = block_sum<double>(s, lds);
-            if (threadIdx.x == 0) mean[c * k + j] = s / (sw == 0.0 ? 1.0 : sw);
+            v = block_sum<double>(v, lds);
+            if (threadIdx.x == 0) part[q * (k + 1) + j + 1] = v;
         }
+    }
+}
+
+__global__ void k_comp_means(int64_t nc, int k, const int *__restrict__ chp,
+                             const double *__restrict__ part, double *__restrict__ mean) {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < nc * k;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t c = e / k;
+        const int j = (int)(e - c * k);
+        double sw = 0.0, s = 0.0;
+        for (int q = chp[c]; q < chp[c + 1]; ++q) {
+            sw += part[(int64_t)q * (k + 1)];
+            s += part[(int64_t)q * (k + 1) + 1 + j];
+        }
+        mean[e] = s / (sw == 0.0 ? 1.0 : sw);
     }
 }
 
@@ -1243,7 +1318,8 @@ struct Panel {
     DevBuf<double> diag, rhs;
     // components
     DevBuf<int> comp, cptr, clev;
-    int64_t ncomp = 0;
+    DevBuf<int> cchp, cch_beg, cch_end;  // chunks of the component-sorted levels
+    int64_t ncomp = 0, ncchunk = 0;
     // S
     void *matrix = nullptr;
     int64_t nnz = 0;
@@ -1612,7 +1688,8 @@ int hdfe_panel_create(int64_t n_rows, int nfe, const int64_t *const *fe, int m,
     } else {
         HIP_TRY(hipMemcpyAsync(k1.p, full.p, n * 8, hipMemcpyDeviceToDevice, s));
     }
-    HIP_TRY(sort_pairs(P->tmp, k1.p, k2.p, i1.p, i2.p, n, 0, 64, s));
+    // kept keys are below 2^(tot + gbits); a dropped row's key has that bit set
+    HIP_TRY(sort_pairs(P->tmp, k1.p, k2.p, i1.p, i2.p, n, 0, tot + gbits + 1, s));
     const int64_t nk = P->n;
     P->perm = std::move(i2);
     // gather the sorted columns (kept rows come first)
@@ -2063,18 +2140,34 @@ int hdfe_panel_components(void *h, int64_t *ncomp) {
                        rank_of_root.p, P.comp.p);
     // levels grouped by component, for the normalisation
     DevBuf<uint64_t> lk1, lk2;
-    DevBuf<int> lv1, ccnt;
+    DevBuf<int> lv1, nch;
     DEV_ALLOC(lk1, nl);
     DEV_ALLOC(lk2, nl);
     DEV_ALLOC(lv1, nl);
     DEV_ALLOC(P.clev, nl);
-    DEV_ALLOC(ccnt, nc + 1);
     DEV_ALLOC(P.cptr, nc + 1);
-    HIP_TRY(hipMemsetAsync(ccnt.p, 0, (nc + 1) * 4, s));
-    hipLaunchKernelGGL(k_comp_level_keys, dim3(grid1(nl)), dim3(T256), 0, s, nl, P.comp.p, lk1.p,
-                       lv1.p, ccnt.p);
-    HIP_TRY(sort_pairs(P.tmp, lk1.p, lk2.p, lv1.p, P.clev.p, nl, 0, 64, s));
-    HIP_TRY(exclusive_sum(P.tmp, ccnt.p, P.cptr.p, nc + 1, s));
+    const int lshift = std::max(1, bits_for((uint64_t)nl));
+    const int kbits = lshift + std::max(1, bits_for((uint64_t)nc));
+    hipLaunchKernelGGL(k_comp_level_keys, dim3(grid1(nl)), dim3(T256), 0, s, nl, lshift, P.comp.p,
+                       lk1.p, lv1.p);
+    HIP_TRY(sort_pairs(P.tmp, lk1.p, lk2.p, lv1.p, P.clev.p, nl, 0, std::min(kbits, 64), s));
+    hipLaunchKernelGGL(k_comp_counts, dim3(grid1(nl)), dim3(T256), 0, s, nl, lshift, lk2.p,
+                       P.cptr.p);
+    // chunks of the sorted levels for the normalisation's sums
+    DEV_ALLOC(nch, nc + 1);
+    DEV_ALLOC(P.cchp, nc + 1);
+    hipLaunchKernelGGL(k_comp_nchunk, dim3(grid1(nc + 1)), dim3(T256), 0, s, (int64_t)nc,
+                       P.cptr.p, nch.p);
+    HIP_TRY(exclusive_sum(P.tmp, nch.p, P.cchp.p, nc + 1, s));
+    int nchunk = 0;
+    HIP_TRY(scalar_to_host(&nchunk, P.cchp.p + nc, s));
+    P.ncchunk = nchunk;
+    DEV_ALLOC(P.cch_beg, std::max(nchunk, 1));
+    DEV_ALLOC(P.cch_end, std::max(nchunk, 1));
+    hipLaunchKernelGGL(k_comp_chunks, dim3(grid1(nc)), dim3(T256), 0, s, (int64_t)nc, P.cptr.p,
+                       P.cchp.p, P.cch_beg.p);
+    hipLaunchKernelGGL(k_comp_chunk_ends, dim3(grid1(nc)), dim3(T256), 0, s, (int64_t)nc,
+                       P.cptr.p, P.cchp.p, P.cch_end.p);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(s));
     P.ncomp = nc;
@@ -2091,9 +2184,13 @@ int hdfe_panel_normalize(void *h, double *Gm, int k, void *stream) {
     const int64_t nl0 = P.nlev[0];
     DevBuf<double> mean;
     DEV_ALLOC(mean, P.ncomp * k);
-    hipLaunchKernelGGL(k_comp_means, dim3((int)std::min<int64_t>(P.ncomp, 65536)), dim3(T256), 0, s,
-                       P.ncomp, k,
-                       P.cptr.p, P.clev.p, P.obs.p, Gm, mean.p);
+    DevBuf<double> cpart;
+    DEV_ALLOC(cpart, std::max<int64_t>(P.ncchunk, 1) * (k + 1));
+    hipLaunchKernelGGL(k_comp_chunk_sums, dim3((int)std::min<int64_t>(P.ncchunk, 65536)),
+                       dim3(T256), 0, s, P.ncchunk, k, P.cch_beg.p, P.cch_end.p, P.clev.p,
+                       P.obs.p, Gm, cpart.p);
+    hipLaunchKernelGGL(k_comp_means, dim3(grid1(P.ncomp * k)), dim3(T256), 0, s, P.ncomp, k,
+                       P.cchp.p, cpart.p, mean.p);
     hipLaunchKernelGGL(k_sub_comp_means, dim3(grid1(nl0 * k)), dim3(T256), 0, s, nl0, k, P.comp.p,
                        mean.p, Gm);
     const int nb = 64;
@@ -2116,7 +2213,7 @@ int hdfe_panel_assemble(void *h, const double *Gm, double *A) {
     Panel &P = *(Panel *)h;
     ARG_CHECK(P.m <= 8, "hdfe_panel_assemble: cell assembly needs m <= 8");
     hipStream_t s = P.s;
-    const int m = P.m, nb = 512;
+    const int m = P.m, nb = 4096;
     DevBuf<double> part;
     DEV_ALLOC(part, (int64_t)nb * m * m);
     switch (m) {
