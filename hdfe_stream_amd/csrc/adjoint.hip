@@ -113,10 +113,10 @@ int hdfe_adj_xwr(int64_t n, int m, const double *V, const int *xcols, int kx, co
     ARG_CHECK(V && xcols && R && out && kx >= 1, "hdfe_adj_xwr: bad argument");
     const int nb = 256;
     DevBuf<int> xc;
-    DEV_ALLOC(xc, kx);
+    DEV_ALLOC(xc, kx, nullptr);
     HIP_TRY(hipMemcpy(xc.p, xcols, kx * 4, hipMemcpyHostToDevice));
     DevBuf<double> part;
-    DEV_ALLOC(part, (int64_t)nb * kx);
+    DEV_ALLOC(part, (int64_t)nb * kx, nullptr);
     hipLaunchKernelGGL(k_adj_xwr, dim3(nb), dim3(256), 0, 0, n, m, V, xc.p, kx, w, R, part.p);
     HIP_TRY(hipGetLastError());
     std::vector<double> h;

@@ -1334,7 +1334,7 @@ int factorize_col(hipStream_t s, Temp &tmp, const int64_t *col, int64_t n, DevBu
     // min / max for the key range
     const int nb = 256;
     DevBuf<int64_t> part;
-    DEV_ALLOC(part, 2 * nb);
+    DEV_ALLOC(part, 2 * nb, s);
     hipLaunchKernelGGL(k_minmax_partial, dim3(nb), dim3(T256), 0, s, n, col, part.p);
     std::vector<int64_t> h;
     HIP_TRY(to_host(h, part.p, 2 * nb, s));
@@ -1346,12 +1346,12 @@ int factorize_col(hipStream_t s, Temp &tmp, const int64_t *col, int64_t n, DevBu
     const int bits = std::max(1, bits_for((uint64_t)(hi - lo)));
     DevBuf<uint64_t> k1, k2;
     DevBuf<int> i1, i2, head, rank;
-    DEV_ALLOC(k1, n);
-    DEV_ALLOC(k2, n);
-    DEV_ALLOC(i1, n);
-    DEV_ALLOC(i2, n);
-    DEV_ALLOC(head, n);
-    DEV_ALLOC(rank, n);
+    DEV_ALLOC(k1, n, s);
+    DEV_ALLOC(k2, n, s);
+    DEV_ALLOC(i1, n, s);
+    DEV_ALLOC(i2, n, s);
+    DEV_ALLOC(head, n, s);
+    DEV_ALLOC(rank, n, s);
     hipLaunchKernelGGL(k_key_from_col, dim3(grid1(n)), dim3(T256), 0, s, n, col, lo, k1.p, i1.p);
     HIP_TRY(sort_pairs(tmp, k1.p, k2.p, i1.p, i2.p, n, 0, bits, s));
     hipLaunchKernelGGL(k_heads_u64, dim3(grid1(n)), dim3(T256), 0, s, n, k2.p, head.p);
@@ -1359,8 +1359,8 @@ int factorize_col(hipStream_t s, Temp &tmp, const int64_t *col, int64_t n, DevBu
     int last = 0;
     HIP_TRY(scalar_to_host(&last, rank.p + n - 1, s));
     nl = last;
-    DEV_ALLOC(code, n);
-    DEV_ALLOC(levels, nl);
+    DEV_ALLOC(code, n, s);
+    DEV_ALLOC(levels, nl, s);
     hipLaunchKernelGGL(k_scatter_codes, dim3(grid1(n)), dim3(T256), 0, s, n, k2.p, lo, i2.p,
                        head.p, rank.p, code.p, levels.p);
     HIP_TRY(hipGetLastError());
@@ -1383,15 +1383,15 @@ static int recode(Panel &P, int64_t n, DevBuf<int> &code, DevBuf<int64_t> &level
                   const uint8_t *keep) {
     hipStream_t s = P.s;
     DevBuf<int> present, newcode;
-    DEV_ALLOC(present, nl + 1);
-    DEV_ALLOC(newcode, nl + 1);
+    DEV_ALLOC(present, nl + 1, s);
+    DEV_ALLOC(newcode, nl + 1, s);
     HIP_TRY(hipMemsetAsync(present.p, 0, (nl + 1) * 4, s));
     hipLaunchKernelGGL(k_level_present, dim3(grid1(n)), dim3(T256), 0, s, n, code.p, keep, present.p);
     HIP_TRY(exclusive_sum(P.tmp, present.p, newcode.p, nl + 1, s));
     int cnt = 0;
     HIP_TRY(scalar_to_host(&cnt, newcode.p + nl, s));
     DevBuf<int64_t> lv;
-    DEV_ALLOC(lv, cnt);
+    DEV_ALLOC(lv, cnt, s);
     hipLaunchKernelGGL(k_compact_levels, dim3(grid1(nl)), dim3(T256), 0, s, nl, present.p,
                        newcode.p, levels.p, lv.p);
     hipLaunchKernelGGL(k_recode, dim3(grid1(n)), dim3(T256), 0, s, n, code.p, newcode.p);
@@ -1432,10 +1432,10 @@ static int run_pass1(Panel &P) {
     // pass 1: means, cells, groups
     const int nb = 512;
     DevBuf<double> part;
-    DEV_ALLOC(part, (int64_t)nb * std::max(m * m, m) + 64);
+    DEV_ALLOC(part, (int64_t)nb * std::max(m * m, m) + 64, s);
     std::vector<double> hp;
-    DEV_ALLOC(P.means, m);
-    DEV_ALLOC(P.raw_ss, m);
+    DEV_ALLOC(P.means, m, s);
+    DEV_ALLOC(P.raw_ss, m, s);
     hipLaunchKernelGGL(k_colsum_rows, dim3(nb), dim3(T256), 0, s, nk, m, P.V.p, part.p);
     HIP_TRY(to_host(hp, part.p, (int64_t)nb * m, s));
     std::vector<double> mean(m, 0.0), ss(m, 0.0);
@@ -1449,10 +1449,10 @@ static int run_pass1(Panel &P) {
         for (int j = 0; j < m; ++j) ss[j] += hp[(size_t)b * m + j];
     HIP_TRY(hipMemcpyAsync(P.raw_ss.p, ss.data(), m * 8, hipMemcpyHostToDevice, s));
     DevBuf<int> gh, ch, grank, crank;
-    DEV_ALLOC(gh, nk);
-    DEV_ALLOC(ch, nk);
-    DEV_ALLOC(grank, nk);
-    DEV_ALLOC(crank, nk);
+    DEV_ALLOC(gh, nk, s);
+    DEV_ALLOC(ch, nk, s);
+    DEV_ALLOC(grank, nk, s);
+    DEV_ALLOC(crank, nk, s);
     hipLaunchKernelGGL(k_row_heads, dim3(grid1(nk)), dim3(T256), 0, s, nk, D, P.g.p, P.codes.p,
                        nk, gh.p, ch.p);
     HIP_TRY(inclusive_sum(P.tmp, gh.p, grank.p, nk, s));
@@ -1462,15 +1462,15 @@ static int run_pass1(Panel &P) {
     HIP_TRY(scalar_to_host(&nc, crank.p + nk - 1, s));
     ARG_CHECK(ng == P.G, "internal: group count mismatch after sorting");
     P.C = nc;
-    DEV_ALLOC(P.gstart, ng + 1);
-    DEV_ALLOC(P.cstart, nc + 1);
+    DEV_ALLOC(P.gstart, ng + 1, s);
+    DEV_ALLOC(P.cstart, nc + 1, s);
     hipLaunchKernelGGL(k_starts, dim3(grid1(nk)), dim3(T256), 0, s, nk, gh.p, grank.p, P.gstart.p);
     hipLaunchKernelGGL(k_starts, dim3(grid1(nk)), dim3(T256), 0, s, nk, ch.p, crank.p, P.cstart.p);
     // group starts over cells: the cell rank of each group's first row
     DevBuf<int> cghead, cgrank;
-    DEV_ALLOC(cghead, nc);
-    DEV_ALLOC(cgrank, nc);
-    DEV_ALLOC(P.cgstart, ng + 1);
+    DEV_ALLOC(cghead, nc, s);
+    DEV_ALLOC(cgrank, nc, s);
+    DEV_ALLOC(P.cgstart, ng + 1, s);
     {
         // a cell starts a group when its first row does
         hipLaunchKernelGGL(k_gather<int>, dim3(grid1(nc)), dim3(T256), 0, s, (int64_t)nc,
@@ -1484,15 +1484,15 @@ static int run_pass1(Panel &P) {
     grank.reset();
     crank.reset();
     DevBuf<double> cn, crun, csums, wpart;
-    DEV_ALLOC(cn, nc);
-    DEV_ALLOC(crun, nc);
-    DEV_ALLOC(csums, (int64_t)nc * m);
+    DEV_ALLOC(cn, nc, s);
+    DEV_ALLOC(crun, nc, s);
+    DEV_ALLOC(csums, (int64_t)nc * m, s);
     const int mm = m <= 8 ? m * m : 0;
-    DEV_ALLOC(wpart, (int64_t)nb * std::max(mm, 1));
+    DEV_ALLOC(wpart, (int64_t)nb * std::max(mm, 1), s);
     hipLaunchKernelGGL(k_cells, dim3(nb), dim3(T256), 0, s, (int64_t)nc, m, P.cstart.p, P.V.p,
                        P.weighted ? P.w.p : (const double *)nullptr, P.means.p, cn.p, crun.p,
                        csums.p, wpart.p);
-    DEV_ALLOC(P.W, std::max(mm, 1));
+    DEV_ALLOC(P.W, std::max(mm, 1), s);
     if (mm) {
         HIP_TRY(to_host(hp, wpart.p, (int64_t)nb * mm, s));
         std::vector<double> Wh(mm, 0.0);
@@ -1501,11 +1501,11 @@ static int run_pass1(Panel &P) {
         HIP_TRY(hipMemcpyAsync(P.W.p, Wh.data(), mm * 8, hipMemcpyHostToDevice, s));
     }
     const int64_t L = P.L;
-    DEV_ALLOC(P.obs, std::max<int64_t>(L, 1));
-    DEV_ALLOC(P.cnt, std::max<int64_t>(L, 1));
+    DEV_ALLOC(P.obs, std::max<int64_t>(L, 1), s);
+    DEV_ALLOC(P.cnt, std::max<int64_t>(L, 1), s);
     {
         DevBuf<unsigned long long> ic;
-        DEV_ALLOC(ic, std::max<int64_t>(L, 1));
+        DEV_ALLOC(ic, std::max<int64_t>(L, 1), s);
         HIP_TRY(hipMemsetAsync(ic.p, 0, std::max<int64_t>(L, 1) * 8, s));
         HIP_TRY(hipMemsetAsync(P.obs.p, 0, std::max<int64_t>(L, 1) * 8, s));
         for (int t = 0; t < D; ++t)
@@ -1519,10 +1519,10 @@ static int run_pass1(Panel &P) {
     }
     // identifying groups and their cells
     DevBuf<int> ident, gpos, gcells, cpos;
-    DEV_ALLOC(ident, ng + 1);
-    DEV_ALLOC(gpos, ng + 1);
-    DEV_ALLOC(gcells, ng + 1);
-    DEV_ALLOC(cpos, ng + 1);
+    DEV_ALLOC(ident, ng + 1, s);
+    DEV_ALLOC(gpos, ng + 1, s);
+    DEV_ALLOC(gcells, ng + 1, s);
+    DEV_ALLOC(cpos, ng + 1, s);
     HIP_TRY(hipMemsetAsync(ident.p, 0, (ng + 1) * 4, s));
     hipLaunchKernelGGL(k_group_cells, dim3(grid1(ng)), dim3(T256), 0, s, (int64_t)ng,
                        P.cgstart.p, ident.p);
@@ -1540,11 +1540,11 @@ static int run_pass1(Panel &P) {
         P.M = Mi;
     }
     const int64_t M = P.M;
-    DEV_ALLOC(P.icodes, std::max<int64_t>(1, (int64_t)D * M));
-    DEV_ALLOC(P.in, std::max<int64_t>(M, 1));
-    DEV_ALLOC(P.isums, std::max<int64_t>(M * m, 1));
-    DEV_ALLOC(P.igstart, Gi + 1);
-    DEV_ALLOC(P.igroup, std::max(Gi, 1));
+    DEV_ALLOC(P.icodes, std::max<int64_t>(1, (int64_t)D * M), s);
+    DEV_ALLOC(P.in, std::max<int64_t>(M, 1), s);
+    DEV_ALLOC(P.isums, std::max<int64_t>(M * m, 1), s);
+    DEV_ALLOC(P.igstart, Gi + 1, s);
+    DEV_ALLOC(P.igroup, std::max(Gi, 1), s);
     hipLaunchKernelGGL(k_ident_cells, dim3(grid1(ng)), dim3(T256), 0, s, (int64_t)ng, D, m,
                        P.cgstart.p, ident.p, gpos.p, cpos.p, P.cstart.p, P.codes.p, nk, cn.p,
                        csums.p, P.icodes.p, M, P.in.p, P.isums.p, P.igstart.p, P.igroup.p);
@@ -1576,15 +1576,15 @@ int hdfe_panel_create(int64_t n_rows, int nfe, const int64_t *const *fe, int m,
     std::vector<int64_t> nl(nfe);
     for (int d = 0; d < nfe; ++d) TRY(factorize(*P, fe[d], n, code[d], lv[d], nl[d]));
     DevBuf<uint8_t> keep;
-    DEV_ALLOC(keep, n);
+    DEV_ALLOC(keep, n, s);
     HIP_TRY(hipMemsetAsync(keep.p, 1, n, s));
     if (fixef_rm_singleton) {
         // dimensions in turn until each has been checked since the last drop
         DevBuf<int> cnt;
         int64_t maxl = *std::max_element(nl.begin(), nl.end());
-        DEV_ALLOC(cnt, maxl);
+        DEV_ALLOC(cnt, maxl, s);
         DevBuf<unsigned long long> dropped;
-        DEV_ALLOC(dropped, 1);
+        DEV_ALLOC(dropped, 1, s);
         bool found = false;
         int quiet = 0;
         int64_t i = 0;
@@ -1637,7 +1637,7 @@ int hdfe_panel_create(int64_t n_rows, int nfe, const int64_t *const *fe, int m,
         o += nl[P->dims[t]];
     }
     P->L = o;
-    DEV_ALLOC(P->offs, std::max(D, 1));
+    DEV_ALLOC(P->offs, std::max(D, 1), s);
     if (D) HIP_TRY(hipMemcpyAsync(P->offs.p, P->offs_h.data(), D * 8, hipMemcpyHostToDevice, s));
     // row order: (group, codes) packed, optionally after stable sorts by the
     // variables (least significant first)
@@ -1650,29 +1650,29 @@ int hdfe_panel_create(int64_t n_rows, int nfe, const int64_t *const *fe, int m,
     }
     ARG_CHECK(tot + gbits <= 63, "hdfe_panel_create: level codes need more than 63 key bits");
     DevBuf<int> codes_in, shift_d;
-    DEV_ALLOC(codes_in, std::max<int64_t>(1, (int64_t)D * n));
-    DEV_ALLOC(shift_d, std::max(D, 1));
+    DEV_ALLOC(codes_in, std::max<int64_t>(1, (int64_t)D * n), s);
+    DEV_ALLOC(shift_d, std::max(D, 1), s);
     for (int t = 0; t < D; ++t)
         HIP_TRY(hipMemcpyAsync(codes_in.p + t * n, code[P->dims[t]].p, n * 4,
                                hipMemcpyDeviceToDevice, s));
     if (D) HIP_TRY(hipMemcpyAsync(shift_d.p, shift.data(), D * 4, hipMemcpyHostToDevice, s));
     DevBuf<uint64_t> full, k1, k2;
     DevBuf<int> i1, i2;
-    DEV_ALLOC(full, n);
-    DEV_ALLOC(k1, n);
-    DEV_ALLOC(k2, n);
-    DEV_ALLOC(i1, n);
-    DEV_ALLOC(i2, n);
+    DEV_ALLOC(full, n, s);
+    DEV_ALLOC(k1, n, s);
+    DEV_ALLOC(k2, n, s);
+    DEV_ALLOC(i1, n, s);
+    DEV_ALLOC(i2, n, s);
     hipLaunchKernelGGL(k_pack_key, dim3(grid1(n)), dim3(T256), 0, s, n, D, code[gd].p, codes_in.p,
                        n, shift_d.p, tot, keep.p, full.p, i1.p);
     DevBuf<double> Vin;
     DevBuf<const double *> colp;
-    DEV_ALLOC(colp, m);
+    DEV_ALLOC(colp, m, s);
     HIP_TRY(hipMemcpyAsync(colp.p, vars, m * sizeof(double *), hipMemcpyHostToDevice, s));
     if (sort_vars) {
-        DEV_ALLOC(Vin, n * m);
+        DEV_ALLOC(Vin, n * m, s);
         DevBuf<int> ident;
-        DEV_ALLOC(ident, n);
+        DEV_ALLOC(ident, n, s);
         // V in input order, row major, for the variable keys
         hipLaunchKernelGGL(k_iota, dim3(grid1(n)), dim3(T256), 0, s, n, ident.p);
         hipLaunchKernelGGL(k_gather_vars, dim3(grid1(n * m)), dim3(T256), 0, s, n, m, colp.p,
@@ -1693,18 +1693,18 @@ int hdfe_panel_create(int64_t n_rows, int nfe, const int64_t *const *fe, int m,
     const int64_t nk = P->n;
     P->perm = std::move(i2);
     // gather the sorted columns (kept rows come first)
-    DEV_ALLOC(P->g, nk);
+    DEV_ALLOC(P->g, nk, s);
     hipLaunchKernelGGL(k_gather<int>, dim3(grid1(nk)), dim3(T256), 0, s, nk, P->perm.p,
                        code[gd].p, P->g.p);
-    DEV_ALLOC(P->codes, std::max<int64_t>(1, (int64_t)D * nk));
+    DEV_ALLOC(P->codes, std::max<int64_t>(1, (int64_t)D * nk), s);
     for (int t = 0; t < D; ++t)
         hipLaunchKernelGGL(k_gather<int>, dim3(grid1(nk)), dim3(T256), 0, s, nk, P->perm.p,
                            code[P->dims[t]].p, P->codes.p + t * nk);
-    DEV_ALLOC(P->V, nk * m);
+    DEV_ALLOC(P->V, nk * m, s);
     hipLaunchKernelGGL(k_gather_vars, dim3(grid1(nk * m)), dim3(T256), 0, s, nk, m, colp.p,
                        P->perm.p, P->V.p);
     if (weights) {
-        DEV_ALLOC(P->w, nk);
+        DEV_ALLOC(P->w, nk, s);
         hipLaunchKernelGGL(k_gather_weights, dim3(grid1(nk)), dim3(T256), 0, s, nk, weights,
                            weights_f32, P->perm.p, P->w.p);
     }
@@ -1801,16 +1801,16 @@ int hdfe_panel_system(void *h, int with_rhs, void **matrix, double *seconds) {
     const int gbits = std::max(1, bits_for((uint64_t)std::max<int64_t>(Gi, 1)));
     ARG_CHECK(lbits + gbits <= 64, "hdfe_panel_system: incidence keys need more than 64 bits");
     const int64_t nE = M * D;
-    DEV_ALLOC(P.cgroup, std::max<int64_t>(M, 1));
+    DEV_ALLOC(P.cgroup, std::max<int64_t>(M, 1), s);
     hipLaunchKernelGGL(k_cell_group, dim3(grid1(Gi)), dim3(T256), 0, s, Gi, P.igstart.p, P.cgroup.p);
     DevBuf<uint64_t> k1, k2;
     DevBuf<int> v1, v2, head, rank;
-    DEV_ALLOC(k1, std::max<int64_t>(nE, 1));
-    DEV_ALLOC(k2, std::max<int64_t>(nE, 1));
-    DEV_ALLOC(v1, std::max<int64_t>(nE, 1));
-    DEV_ALLOC(v2, std::max<int64_t>(nE, 1));
-    DEV_ALLOC(head, std::max<int64_t>(nE, 1));
-    DEV_ALLOC(rank, std::max<int64_t>(nE, 1));
+    DEV_ALLOC(k1, std::max<int64_t>(nE, 1), s);
+    DEV_ALLOC(k2, std::max<int64_t>(nE, 1), s);
+    DEV_ALLOC(v1, std::max<int64_t>(nE, 1), s);
+    DEV_ALLOC(v2, std::max<int64_t>(nE, 1), s);
+    DEV_ALLOC(head, std::max<int64_t>(nE, 1), s);
+    DEV_ALLOC(rank, std::max<int64_t>(nE, 1), s);
     int E = 0;
     if (nE) {
         hipLaunchKernelGGL(k_inc_keys, dim3(grid1(nE)), dim3(T256), 0, s, M, D, P.cgroup.p,
@@ -1822,16 +1822,16 @@ int hdfe_panel_system(void *h, int with_rhs, void **matrix, double *seconds) {
     }
     P.E = E;
     DevBuf<int> istart;
-    DEV_ALLOC(istart, E + 1);
+    DEV_ALLOC(istart, E + 1, s);
     if (nE)
         hipLaunchKernelGGL(k_starts, dim3(grid1(nE)), dim3(T256), 0, s, nE, head.p, rank.p, istart.p);
-    DEV_ALLOC(P.ig, std::max(E, 1));
-    DEV_ALLOC(P.il, std::max(E, 1));
-    DEV_ALLOC(P.iu, std::max(E, 1));
-    DEV_ALLOC(P.is, std::max<int64_t>((int64_t)E * m, 1));
-    DEV_ALLOC(P.Ng, std::max<int64_t>(Gi, 1));
-    DEV_ALLOC(P.vbar, std::max<int64_t>(Gi * m, 1));
-    DEV_ALLOC(P.gis, Gi + 1);
+    DEV_ALLOC(P.ig, std::max(E, 1), s);
+    DEV_ALLOC(P.il, std::max(E, 1), s);
+    DEV_ALLOC(P.iu, std::max(E, 1), s);
+    DEV_ALLOC(P.is, std::max<int64_t>((int64_t)E * m, 1), s);
+    DEV_ALLOC(P.Ng, std::max<int64_t>(Gi, 1), s);
+    DEV_ALLOC(P.vbar, std::max<int64_t>(Gi * m, 1), s);
+    DEV_ALLOC(P.gis, Gi + 1, s);
     hipLaunchKernelGGL(k_inc_reduce, dim3(grid1(E)), dim3(T256), 0, s, (int64_t)E, m, lbits, k2.p,
                        v2.p, istart.p, P.in.p, P.isums.p, P.ig.p, P.il.p, P.iu.p, P.is.p);
     hipLaunchKernelGGL(k_group_totals, dim3(grid1(Gi)), dim3(T256), 0, s, Gi, m, P.igstart.p,
@@ -1839,8 +1839,8 @@ int hdfe_panel_system(void *h, int with_rhs, void **matrix, double *seconds) {
     hipLaunchKernelGGL(k_group_inc_start, dim3(grid1(E)), dim3(T256), 0, s, (int64_t)E, P.ig.p,
                        P.gis.p, Gi);
     // level CSR of the incidences (sorted by level, then group)
-    DEV_ALLOC(P.lptr, L + 1);
-    DEV_ALLOC(P.lq, std::max(E, 1));
+    DEV_ALLOC(P.lptr, L + 1, s);
+    DEV_ALLOC(P.lq, std::max(E, 1), s);
     if (E) {
         hipLaunchKernelGGL(k_level_keys, dim3(grid1(E)), dim3(T256), 0, s, (int64_t)E, P.ig.p,
                            P.il.p, gbits, k1.p, v1.p);
@@ -1848,20 +1848,20 @@ int hdfe_panel_system(void *h, int with_rhs, void **matrix, double *seconds) {
     }
     hipLaunchKernelGGL(k_level_ptr, dim3(grid1(L + 1)), dim3(T256), 0, s, L, (int64_t)E, k2.p,
                        gbits, P.lptr.p);
-    DEV_ALLOC(P.diag, L);
-    if (with_rhs) DEV_ALLOC(P.rhs, L * m);
+    DEV_ALLOC(P.diag, L, s);
+    if (with_rhs) DEV_ALLOC(P.rhs, L * m, s);
     {
         DevBuf<int> nch, cptr, clvl, cbeg;
-        DEV_ALLOC(nch, L + 1);
-        DEV_ALLOC(cptr, L + 1);
+        DEV_ALLOC(nch, L + 1, s);
+        DEV_ALLOC(cptr, L + 1, s);
         hipLaunchKernelGGL(k_level_nchunk, dim3(grid1(L + 1)), dim3(T256), 0, s, L, P.lptr.p, nch.p);
         HIP_TRY(exclusive_sum(P.tmp, nch.p, cptr.p, L + 1, s));
         int nchunk = 0;
         HIP_TRY(scalar_to_host(&nchunk, cptr.p + L, s));
-        DEV_ALLOC(clvl, nchunk);
-        DEV_ALLOC(cbeg, nchunk);
+        DEV_ALLOC(clvl, nchunk, s);
+        DEV_ALLOC(cbeg, nchunk, s);
         DevBuf<double> part;
-        DEV_ALLOC(part, (int64_t)nchunk * (m + 1));
+        DEV_ALLOC(part, (int64_t)nchunk * (m + 1), s);
         hipLaunchKernelGGL(k_level_chunks, dim3(grid1(L)), dim3(T256), 0, s, L, P.lptr.p, cptr.p,
                            clvl.p, cbeg.p);
         hipLaunchKernelGGL(k_diag_rhs_part, dim3(grid1((int64_t)nchunk * 64)), dim3(T256), 0, s,
@@ -1916,13 +1916,13 @@ int hdfe_panel_system(void *h, int with_rhs, void **matrix, double *seconds) {
         for (int d2 = 0; d2 < D; ++d2) ncross += (d1 != d2 && sparse_dim[d1] && sparse_dim[d2]);
     DevBuf<uint8_t> d_sparse_lv, d_sparse_dim;
     DevBuf<int> d_sparse_idx, d_dense_idx, d_lvl_dim, d_sparse_levels, d_dense_levels;
-    DEV_ALLOC(d_sparse_lv, L);
-    DEV_ALLOC(d_sparse_dim, D);
-    DEV_ALLOC(d_sparse_idx, L);
-    DEV_ALLOC(d_dense_idx, L);
-    DEV_ALLOC(d_lvl_dim, L);
-    DEV_ALLOC(d_sparse_levels, std::max<int64_t>(Ls, 1));
-    DEV_ALLOC(d_dense_levels, std::max(Ld, 1));
+    DEV_ALLOC(d_sparse_lv, L, s);
+    DEV_ALLOC(d_sparse_dim, D, s);
+    DEV_ALLOC(d_sparse_idx, L, s);
+    DEV_ALLOC(d_dense_idx, L, s);
+    DEV_ALLOC(d_lvl_dim, L, s);
+    DEV_ALLOC(d_sparse_levels, std::max<int64_t>(Ls, 1), s);
+    DEV_ALLOC(d_dense_levels, std::max(Ld, 1), s);
     HIP_TRY(hipMemcpyAsync(d_sparse_lv.p, sparse_lv.data(), L, hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemcpyAsync(d_sparse_dim.p, sparse_dim.data(), D, hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemcpyAsync(d_sparse_idx.p, sparse_idx.data(), L * 4, hipMemcpyHostToDevice, s));
@@ -1933,8 +1933,8 @@ int hdfe_panel_system(void *h, int with_rhs, void **matrix, double *seconds) {
 
     // sparse x sparse: emit, sort, reduce
     DevBuf<int64_t> cnt, pos;
-    DEV_ALLOC(cnt, Gi + 1);
-    DEV_ALLOC(pos, Gi + 1);
+    DEV_ALLOC(cnt, Gi + 1, s);
+    DEV_ALLOC(pos, Gi + 1, s);
     HIP_TRY(hipMemsetAsync(cnt.p, 0, (Gi + 1) * 8, s));
     hipLaunchKernelGGL(k_ss_count, dim3(grid1(Gi)), dim3(T256), 0, s, Gi, P.gis.p, P.il.p,
                        d_sparse_lv.p, P.igstart.p, ncross, cnt.p);
@@ -1944,10 +1944,10 @@ int hdfe_panel_system(void *h, int with_rhs, void **matrix, double *seconds) {
     ARG_CHECK(nent < INT32_MAX, "hdfe_panel_system: too many sparse entries for one sort");
     DevBuf<uint64_t> ek1, ek2;
     DevBuf<double> ev1, ev2;
-    DEV_ALLOC(ek1, std::max<int64_t>(nent, 1));
-    DEV_ALLOC(ek2, std::max<int64_t>(nent, 1));
-    DEV_ALLOC(ev1, std::max<int64_t>(nent, 1));
-    DEV_ALLOC(ev2, std::max<int64_t>(nent, 1));
+    DEV_ALLOC(ek1, std::max<int64_t>(nent, 1), s);
+    DEV_ALLOC(ek2, std::max<int64_t>(nent, 1), s);
+    DEV_ALLOC(ev1, std::max<int64_t>(nent, 1), s);
+    DEV_ALLOC(ev2, std::max<int64_t>(nent, 1), s);
     k1.reset();
     k2.reset();
     v1.reset();
@@ -1962,17 +1962,17 @@ int hdfe_panel_system(void *h, int with_rhs, void **matrix, double *seconds) {
         HIP_TRY(sort_pairs(P.tmp, ek1.p, ek2.p, ev1.p, ev2.p, nent, 0, 2 * lbits, s));
         head.reset();
         rank.reset();
-        DEV_ALLOC(head, nent);
-        DEV_ALLOC(rank, nent);
+        DEV_ALLOC(head, nent, s);
+        DEV_ALLOC(rank, nent, s);
         hipLaunchKernelGGL(k_heads_u64, dim3(grid1(nent)), dim3(T256), 0, s, nent, ek2.p, head.p);
         HIP_TRY(inclusive_sum(P.tmp, head.p, rank.p, nent, s));
         HIP_TRY(scalar_to_host(&nss, rank.p + nent - 1, s));
     }
-    DEV_ALLOC(ss_row, std::max(nss, 1));
-    DEV_ALLOC(ss_col, std::max(nss, 1));
-    DEV_ALLOC(ss_val, std::max(nss, 1));
-    DEV_ALLOC(ss_rowcnt, Ls + 1);
-    DEV_ALLOC(ss_rowptr, Ls + 1);
+    DEV_ALLOC(ss_row, std::max(nss, 1), s);
+    DEV_ALLOC(ss_col, std::max(nss, 1), s);
+    DEV_ALLOC(ss_val, std::max(nss, 1), s);
+    DEV_ALLOC(ss_rowcnt, Ls + 1, s);
+    DEV_ALLOC(ss_rowptr, Ls + 1, s);
     HIP_TRY(hipMemsetAsync(ss_rowcnt.p, 0, (Ls + 1) * 4, s));
     if (nss) {
         hipLaunchKernelGGL(k_reduce_runs, dim3(grid1(nent)), dim3(T256), 0, s, nent, ek2.p, ev2.p,
@@ -1991,21 +1991,21 @@ int hdfe_panel_system(void *h, int with_rhs, void **matrix, double *seconds) {
     DevBuf<double> U, PD, DD;
     DevBuf<uint8_t> PP, DP;
     DevBuf<int> colpres, cpos, colbuf;
-    DEV_ALLOC(PD, std::max<int64_t>(Ls * Ld, 1));
-    DEV_ALLOC(PP, std::max<int64_t>(Ls * Ld, 1));
-    DEV_ALLOC(DD, std::max(Ld * Ld, 1));
-    DEV_ALLOC(DP, std::max(Ld * Ld, 1));
-    DEV_ALLOC(colpres, std::max(Ld, 1));
-    DEV_ALLOC(cpos, std::max<int64_t>((int64_t)Ld * (Ls + 1), 1));
+    DEV_ALLOC(PD, std::max<int64_t>(Ls * Ld, 1), s);
+    DEV_ALLOC(PP, std::max<int64_t>(Ls * Ld, 1), s);
+    DEV_ALLOC(DD, std::max(Ld * Ld, 1), s);
+    DEV_ALLOC(DP, std::max(Ld * Ld, 1), s);
+    DEV_ALLOC(colpres, std::max(Ld, 1), s);
+    DEV_ALLOC(cpos, std::max<int64_t>((int64_t)Ld * (Ls + 1), 1), s);
     if (Ld) {
-        DEV_ALLOC(U, std::max<int64_t>(Gi * Ld, 1));
+        DEV_ALLOC(U, std::max<int64_t>(Gi * Ld, 1), s);
         HIP_TRY(hipMemsetAsync(U.p, 0, std::max<int64_t>(Gi * Ld, 1) * 8, s));
         hipLaunchKernelGGL(k_dense_u, dim3(grid1(E)), dim3(T256), 0, s, (int64_t)E, P.ig.p, P.il.p,
                            P.iu.p, d_dense_idx.p, Ld, U.p);
         {
             DevBuf<int> sflag, sidx;
-            DEV_ALLOC(sflag, (int64_t)E + 1);
-            DEV_ALLOC(sidx, (int64_t)E + 1);
+            DEV_ALLOC(sflag, (int64_t)E + 1, s);
+            DEV_ALLOC(sidx, (int64_t)E + 1, s);
             hipLaunchKernelGGL(k_sparse_flag, dim3(grid1((int64_t)E + 1)), dim3(T256), 0, s,
                                (int64_t)E, P.il.p, d_sparse_lv.p, sflag.p);
             HIP_TRY(exclusive_sum(P.tmp, sflag.p, sidx.p, (int64_t)E + 1, s));
@@ -2013,8 +2013,8 @@ int hdfe_panel_system(void *h, int with_rhs, void **matrix, double *seconds) {
             HIP_TRY(scalar_to_host(&Es, sidx.p + E, s));
             DevBuf<double> Cq;
             DevBuf<uint64_t> Fq;
-            DEV_ALLOC(Cq, std::max<int64_t>((int64_t)Es * Ld, 1));
-            DEV_ALLOC(Fq, std::max(Es, 1));
+            DEV_ALLOC(Cq, std::max<int64_t>((int64_t)Es * Ld, 1), s);
+            DEV_ALLOC(Fq, std::max(Es, 1), s);
             hipLaunchKernelGGL(k_sd_inc, dim3(grid1(E)), dim3(T256), 0, s, (int64_t)E, D, Ld,
                                P.ig.p, P.il.p, P.iu.p, sidx.p, d_sparse_lv.p, d_lvl_dim.p,
                                P.offs.p, d_sparse_dim.p, d_dense_idx.p, U.p, P.Ng.p,
@@ -2028,15 +2028,15 @@ int hdfe_panel_system(void *h, int with_rhs, void **matrix, double *seconds) {
         const int nbd = 1024;
         DevBuf<double> ddp;
         DevBuf<int> ddpp;
-        DEV_ALLOC(ddp, (int64_t)nbd * Ld * Ld);
-        DEV_ALLOC(ddpp, (int64_t)nbd * Ld * Ld);
+        DEV_ALLOC(ddp, (int64_t)nbd * Ld * Ld, s);
+        DEV_ALLOC(ddpp, (int64_t)nbd * Ld * Ld, s);
         hipLaunchKernelGGL(k_dd, dim3(nbd), dim3(T256), 0, s, Gi, Ld, d_dense_levels.p,
                            d_lvl_dim.p, P.offs.p, U.p, P.Ng.p, P.igstart.p, P.icodes.p, M,
                            P.in.p, ddp.p, ddpp.p);
         hipLaunchKernelGGL(k_dd_finish, dim3(grid1(Ld * Ld)), dim3(T256), 0, s, nbd, Ld * Ld,
                            ddp.p, ddpp.p, DD.p, DP.p);
         // per dense level: positions of the present sparse levels
-        DEV_ALLOC(colbuf, Ls + 1);
+        DEV_ALLOC(colbuf, Ls + 1, s);
         std::vector<int> cp(Ld);
         for (int t = 0; t < Ld; ++t) {
             HIP_TRY(hipMemsetAsync(colbuf.p + Ls, 0, 4, s));
@@ -2050,8 +2050,8 @@ int hdfe_panel_system(void *h, int with_rhs, void **matrix, double *seconds) {
     }
     // CSR
     DevBuf<int> rnnz, indptr;
-    DEV_ALLOC(rnnz, L + 1);
-    DEV_ALLOC(indptr, L + 1);
+    DEV_ALLOC(rnnz, L + 1, s);
+    DEV_ALLOC(indptr, L + 1, s);
     HIP_TRY(hipMemsetAsync(rnnz.p, 0, (L + 1) * 4, s));
     hipLaunchKernelGGL(k_row_nnz, dim3(grid1(L)), dim3(T256), 0, s, L, Ld, ss_rowcnt.p,
                        d_sparse_idx.p, d_dense_idx.p, PP.p, colpres.p, DP.p, rnnz.p);
@@ -2061,9 +2061,9 @@ int hdfe_panel_system(void *h, int with_rhs, void **matrix, double *seconds) {
     P.nnz = nnz;
     DevBuf<int> indices;
     DevBuf<double> data, dinv;
-    DEV_ALLOC(indices, std::max(nnz, 1));
-    DEV_ALLOC(data, std::max(nnz, 1));
-    DEV_ALLOC(dinv, L);
+    DEV_ALLOC(indices, std::max(nnz, 1), s);
+    DEV_ALLOC(data, std::max(nnz, 1), s);
+    DEV_ALLOC(dinv, L, s);
     hipLaunchKernelGGL(k_fill_sparse_rows, dim3(grid1(Ls)), dim3(T256), 0, s, Ls, Ld,
                        d_sparse_levels.p, d_dense_levels.p, ss_rowptr.p, ss_col.p, ss_val.p, PP.p,
                        PD.p, indptr.p, indices.p, data.p);
@@ -2100,10 +2100,10 @@ int hdfe_panel_components(void *h, int64_t *ncomp) {
     ARG_CHECK(P.D >= 1, "hdfe_panel_components: no non-streamed dimension");
     const int64_t nl = P.nlev[0];
     DevBuf<int> parent, changed, is_root, rpos;
-    DEV_ALLOC(parent, nl);
-    DEV_ALLOC(changed, 1);
-    DEV_ALLOC(is_root, nl + 1);
-    DEV_ALLOC(rpos, nl + 1);
+    DEV_ALLOC(parent, nl, s);
+    DEV_ALLOC(changed, 1, s);
+    DEV_ALLOC(is_root, nl + 1, s);
+    DEV_ALLOC(rpos, nl + 1, s);
     hipLaunchKernelGGL(k_uf_init, dim3(grid1(nl)), dim3(T256), 0, s, nl, parent.p);
     for (int it = 0; it < 64; ++it) {
         int ch = 0;
@@ -2115,7 +2115,7 @@ int hdfe_panel_components(void *h, int64_t *ncomp) {
         if (!ch) break;
     }
     DevBuf<unsigned long long> rows;
-    DEV_ALLOC(rows, nl);
+    DEV_ALLOC(rows, nl, s);
     HIP_TRY(hipMemsetAsync(rows.p, 0, nl * 8, s));
     HIP_TRY(hipMemsetAsync(is_root.p + nl, 0, 4, s));
     hipLaunchKernelGGL(k_comp_rows, dim3(grid1(nl)), dim3(T256), 0, s, nl, parent.p, P.cnt.p,
@@ -2125,27 +2125,27 @@ int hdfe_panel_components(void *h, int64_t *ncomp) {
     HIP_TRY(scalar_to_host(&nc, rpos.p + nl, s));
     DevBuf<uint64_t> k1, k2;
     DevBuf<int> v1, v2, rank_of_root;
-    DEV_ALLOC(k1, nc);
-    DEV_ALLOC(k2, nc);
-    DEV_ALLOC(v1, nc);
-    DEV_ALLOC(v2, nc);
-    DEV_ALLOC(rank_of_root, nl);
+    DEV_ALLOC(k1, nc, s);
+    DEV_ALLOC(k2, nc, s);
+    DEV_ALLOC(v1, nc, s);
+    DEV_ALLOC(v2, nc, s);
+    DEV_ALLOC(rank_of_root, nl, s);
     hipLaunchKernelGGL(k_comp_keys, dim3(grid1(nl)), dim3(T256), 0, s, nl, is_root.p, rows.p,
                        rpos.p, k1.p, v1.p);
     HIP_TRY(sort_pairs(P.tmp, k1.p, k2.p, v1.p, v2.p, nc, 0, 64, s));
     hipLaunchKernelGGL(k_comp_rank, dim3(grid1(nc)), dim3(T256), 0, s, (int64_t)nc, v2.p,
                        rank_of_root.p);
-    DEV_ALLOC(P.comp, nl);
+    DEV_ALLOC(P.comp, nl, s);
     hipLaunchKernelGGL(k_comp_label, dim3(grid1(nl)), dim3(T256), 0, s, nl, parent.p,
                        rank_of_root.p, P.comp.p);
     // levels grouped by component, for the normalisation
     DevBuf<uint64_t> lk1, lk2;
     DevBuf<int> lv1, nch;
-    DEV_ALLOC(lk1, nl);
-    DEV_ALLOC(lk2, nl);
-    DEV_ALLOC(lv1, nl);
-    DEV_ALLOC(P.clev, nl);
-    DEV_ALLOC(P.cptr, nc + 1);
+    DEV_ALLOC(lk1, nl, s);
+    DEV_ALLOC(lk2, nl, s);
+    DEV_ALLOC(lv1, nl, s);
+    DEV_ALLOC(P.clev, nl, s);
+    DEV_ALLOC(P.cptr, nc + 1, s);
     const int lshift = std::max(1, bits_for((uint64_t)nl));
     const int kbits = lshift + std::max(1, bits_for((uint64_t)nc));
     hipLaunchKernelGGL(k_comp_level_keys, dim3(grid1(nl)), dim3(T256), 0, s, nl, lshift, P.comp.p,
@@ -2154,16 +2154,16 @@ int hdfe_panel_components(void *h, int64_t *ncomp) {
     hipLaunchKernelGGL(k_comp_counts, dim3(grid1(nl)), dim3(T256), 0, s, nl, lshift, lk2.p,
                        P.cptr.p);
     // chunks of the sorted levels for the normalisation's sums
-    DEV_ALLOC(nch, nc + 1);
-    DEV_ALLOC(P.cchp, nc + 1);
+    DEV_ALLOC(nch, nc + 1, s);
+    DEV_ALLOC(P.cchp, nc + 1, s);
     hipLaunchKernelGGL(k_comp_nchunk, dim3(grid1(nc + 1)), dim3(T256), 0, s, (int64_t)nc,
                        P.cptr.p, nch.p);
     HIP_TRY(exclusive_sum(P.tmp, nch.p, P.cchp.p, nc + 1, s));
     int nchunk = 0;
     HIP_TRY(scalar_to_host(&nchunk, P.cchp.p + nc, s));
     P.ncchunk = nchunk;
-    DEV_ALLOC(P.cch_beg, std::max(nchunk, 1));
-    DEV_ALLOC(P.cch_end, std::max(nchunk, 1));
+    DEV_ALLOC(P.cch_beg, std::max(nchunk, 1), s);
+    DEV_ALLOC(P.cch_end, std::max(nchunk, 1), s);
     hipLaunchKernelGGL(k_comp_chunks, dim3(grid1(nc)), dim3(T256), 0, s, (int64_t)nc, P.cptr.p,
                        P.cchp.p, P.cch_beg.p);
     hipLaunchKernelGGL(k_comp_chunk_ends, dim3(grid1(nc)), dim3(T256), 0, s, (int64_t)nc,
@@ -2183,9 +2183,9 @@ int hdfe_panel_normalize(void *h, double *Gm, int k, void *stream) {
     hipStream_t s = stream ? (hipStream_t)stream : P.s;
     const int64_t nl0 = P.nlev[0];
     DevBuf<double> mean;
-    DEV_ALLOC(mean, P.ncomp * k);
+    DEV_ALLOC(mean, P.ncomp * k, s);
     DevBuf<double> cpart;
-    DEV_ALLOC(cpart, std::max<int64_t>(P.ncchunk, 1) * (k + 1));
+    DEV_ALLOC(cpart, std::max<int64_t>(P.ncchunk, 1) * (k + 1), s);
     hipLaunchKernelGGL(k_comp_chunk_sums, dim3((int)std::min<int64_t>(P.ncchunk, 65536)),
                        dim3(T256), 0, s, P.ncchunk, k, P.cch_beg.p, P.cch_end.p, P.clev.p,
                        P.obs.p, Gm, cpart.p);
@@ -2195,7 +2195,7 @@ int hdfe_panel_normalize(void *h, double *Gm, int k, void *stream) {
                        mean.p, Gm);
     const int nb = 64;
     DevBuf<double> part;
-    DEV_ALLOC(part, (int64_t)nb * (k + 1));
+    DEV_ALLOC(part, (int64_t)nb * (k + 1), s);
     for (int t = 1; t < P.D; ++t) {
         hipLaunchKernelGGL(k_wsum_block, dim3(nb), dim3(T256), 0, s, P.nlev[t], k, P.offs_h[t],
                            P.obs.p, Gm, part.p);
@@ -2215,7 +2215,7 @@ int hdfe_panel_assemble(void *h, const double *Gm, double *A) {
     hipStream_t s = P.s;
     const int m = P.m, nb = 4096;
     DevBuf<double> part;
-    DEV_ALLOC(part, (int64_t)nb * m * m);
+    DEV_ALLOC(part, (int64_t)nb * m * m, s);
     switch (m) {
 #define HDFE_ASM(MM)                                                                            \
     case MM:                                                                                    \
@@ -2247,8 +2247,8 @@ int hdfe_panel_pass2(void *h, const double *Gm, const double *beta, const int *x
     hipStream_t s = P.s;
     DevBuf<double> b;
     DevBuf<int> xc;
-    DEV_ALLOC(b, std::max(kx, 1));
-    DEV_ALLOC(xc, std::max(kx, 1));
+    DEV_ALLOC(b, std::max(kx, 1), s);
+    DEV_ALLOC(xc, std::max(kx, 1), s);
     if (kx) {
         HIP_TRY(hipMemcpyAsync(b.p, beta, kx * 8, hipMemcpyHostToDevice, s));
         HIP_TRY(hipMemcpyAsync(xc.p, xcols, kx * 4, hipMemcpyHostToDevice, s));
@@ -2288,7 +2288,7 @@ int hdfe_panel_partial(void *h, const double *Gm, const int *cols, int kc, doubl
         ARG_CHECK(cols[c] >= 0 && cols[c] < P.m, "hdfe_panel_partial: column out of range");
     hipStream_t s = P.s;
     DevBuf<int> cc;
-    DEV_ALLOC(cc, kc);
+    DEV_ALLOC(cc, kc, s);
     HIP_TRY(hipMemcpyAsync(cc.p, cols, kc * 4, hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(k_partial, dim3(grid1(P.G)), dim3(T256), 0, s, P.G, P.D, P.m, P.gstart.p,
                        P.codes.p, P.n, P.offs.p, P.V.p, P.weighted ? P.w.p : (const double *)nullptr,
@@ -2301,10 +2301,13 @@ int hdfe_panel_partial(void *h, const double *Gm, const int *cols, int kc, doubl
 void hdfe_panel_destroy(void *h) {
     if (!h) return;
     Panel *P = (Panel *)h;
-    if (P->s) (void)hipStreamSynchronize(P->s);
+    // the panel's arrays may have been read on other streams; its buffers
+    // are freed on its stream, which must outlive them
+    (void)hipDeviceSynchronize();
     if (P->matrix) hdfe_destroy(P->matrix);
-    if (P->s) (void)hipStreamDestroy(P->s);
+    hipStream_t s = P->s;
     delete P;
+    if (s) (void)hipStreamDestroy(s);
 }
 
 }  // extern "C"

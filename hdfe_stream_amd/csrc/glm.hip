@@ -427,10 +427,10 @@ int hdfe_glm_rows(int nchunk, const int *ch_model, const int64_t *ch_begin,
     DevBuf<int> cm;
     DevBuf<int64_t> cb, ce;
     DevBuf<double> part;
-    DEV_ALLOC(cm, nchunk);
-    DEV_ALLOC(cb, nchunk);
-    DEV_ALLOC(ce, nchunk);
-    DEV_ALLOC(part, (int64_t)nchunk * 5);
+    DEV_ALLOC(cm, nchunk, nullptr);
+    DEV_ALLOC(cb, nchunk, nullptr);
+    DEV_ALLOC(ce, nchunk, nullptr);
+    DEV_ALLOC(part, (int64_t)nchunk * 5, nullptr);
     HIP_TRY(hipMemcpy(cm.p, ch_model, nchunk * 4, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(cb.p, ch_begin, nchunk * 8, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(ce.p, ch_end, nchunk * 8, hipMemcpyHostToDevice));
@@ -457,7 +457,7 @@ int hdfe_glm_rhs(void *rows_h, int64_t G, const int *gstart, const int *g, const
     hipLaunchKernelGGL(k_glm_group_means, dim3(grid1(G * 64)), dim3(256), 0, 0, G, gstart, z, X,
                        kx, w, vbar);
     DevBuf<double> part;
-    DEV_ALLOC(part, nchunk * (int64_t)C);
+    DEV_ALLOC(part, nchunk * (int64_t)C, nullptr);
     hipLaunchKernelGGL(k_glm_center, dim3(grid1(nchunk * 64)), dim3(256), 0, 0, nchunk, clvl,
                        cbeg, rptr, rrow, g, z, X, kx, w, vbar, part.p);
     HIP_TRY(hipGetLastError());
@@ -480,12 +480,12 @@ int hdfe_glm_partial(int nwg, const int *wg_model, const int64_t *wg_gb, const i
     DevBuf<int> wm;
     DevBuf<int64_t> gb, ge, rs, ps;
     DevBuf<double> gpart;
-    DEV_ALLOC(wm, nwg);
-    DEV_ALLOC(gb, nwg);
-    DEV_ALLOC(ge, nwg);
-    DEV_ALLOC(rs, nmodels + 1);
-    DEV_ALLOC(ps, nmodels + 1);
-    DEV_ALLOC(gpart, (int64_t)nwg * kx);
+    DEV_ALLOC(wm, nwg, nullptr);
+    DEV_ALLOC(gb, nwg, nullptr);
+    DEV_ALLOC(ge, nwg, nullptr);
+    DEV_ALLOC(rs, nmodels + 1, nullptr);
+    DEV_ALLOC(ps, nmodels + 1, nullptr);
+    DEV_ALLOC(gpart, (int64_t)nwg * kx, nullptr);
     HIP_TRY(hipMemcpy(wm.p, wg_model, nwg * 4, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(gb.p, wg_gb, nwg * 8, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(ge.p, wg_ge, nwg * 8, hipMemcpyHostToDevice));
@@ -524,13 +524,13 @@ int hdfe_glm_syrk(int nmodels, const int64_t *pstart_h, int Cp, const void *hi, 
     HIP_TRY(hipMemset(A, 0, (int64_t)nmodels * Cp * Cp * 8));
     if (nwg == 0) return HDFE_STATUS_OK;
     DevBuf<int64_t> dkb, dke;
-    DEV_ALLOC(dkb, nwg);
-    DEV_ALLOC(dke, nwg);
+    DEV_ALLOC(dkb, nwg, nullptr);
+    DEV_ALLOC(dke, nwg, nullptr);
     HIP_TRY(hipMemcpy(dkb.p, kb.data(), nwg * 8, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(dke.p, ke.data(), nwg * 8, hipMemcpyHostToDevice));
     const int per_launch = 8 * kSyrkMaxBlk;
     DevBuf<float> part;
-    DEV_ALLOC(part, (int64_t)nwg * per_launch * 1024);
+    DEV_ALLOC(part, (int64_t)nwg * per_launch * 1024, nullptr);
     const size_t lds = (size_t)2 * 2 * Cp * 8 * sizeof(__bf16);
     for (int q0 = 0; q0 < nblk; q0 += per_launch) {
         const int nq = std::min(per_launch, nblk - q0);

@@ -223,15 +223,15 @@ int hdfe_loo_prune(int64_t n, const int64_t *a_ids, const int64_t *b_ids, uint8_
     const int kbits = std::max(1, bits_for((uint64_t)n0 * (uint64_t)n1));
     DevBuf<uint64_t> k1, k2;
     DevBuf<int> head, rank;
-    DEV_ALLOC(k1, n);
-    DEV_ALLOC(k2, n);
-    DEV_ALLOC(head, n);
-    DEV_ALLOC(rank, n);
+    DEV_ALLOC(k1, n, s);
+    DEV_ALLOC(k2, n, s);
+    DEV_ALLOC(head, n, s);
+    DEV_ALLOC(rank, n, s);
     hipLaunchKernelGGL(k_pair_keys, dim3(grid1(n)), dim3(256), 0, s, n, ca.p, cb.p, n1, k1.p);
     {
         size_t bytes = 0;
         HIP_TRY(hipcub::DeviceRadixSort::SortKeys(nullptr, bytes, k1.p, k2.p, (int)n, 0, kbits, s));
-        HIP_TRY(tmp.ensure(bytes));
+        HIP_TRY(tmp.ensure(bytes, s));
         bytes = (size_t)tmp.buf.n;
         HIP_TRY(hipcub::DeviceRadixSort::SortKeys(tmp.buf.p, bytes, k1.p, k2.p, (int)n, 0, kbits, s));
     }
@@ -241,21 +241,21 @@ int hdfe_loo_prune(int64_t n, const int64_t *a_ids, const int64_t *b_ids, uint8_
     HIP_TRY(scalar_to_host(&np, rank.p + n - 1, s));
     DevBuf<uint64_t> ukey;
     DevBuf<int> ustart;
-    DEV_ALLOC(ukey, np);
-    DEV_ALLOC(ustart, np + 1);
+    DEV_ALLOC(ukey, np, s);
+    DEV_ALLOC(ustart, np + 1, s);
     hipLaunchKernelGGL(k_run_unique, dim3(grid1(n)), dim3(256), 0, s, n, k2.p, head.p, rank.p,
                        ukey.p, ustart.p);
     // directed edges, sorted: the CSR
     const int64_t ne = 2 * (int64_t)np;
     DevBuf<uint64_t> e1, e2;
-    DEV_ALLOC(e1, ne);
-    DEV_ALLOC(e2, ne);
+    DEV_ALLOC(e1, ne, s);
+    DEV_ALLOC(e2, ne, s);
     hipLaunchKernelGGL(k_edges, dim3(grid1(np)), dim3(256), 0, s, (int64_t)np, ukey.p, n0, n1, e1.p);
     {
         const int vbits = std::max(1, bits_for((uint64_t)(n0 + n1)));
         size_t bytes = 0;
         HIP_TRY(hipcub::DeviceRadixSort::SortKeys(nullptr, bytes, e1.p, e2.p, (int)ne, 0, 32 + vbits, s));
-        HIP_TRY(tmp.ensure(bytes));
+        HIP_TRY(tmp.ensure(bytes, s));
         bytes = (size_t)tmp.buf.n;
         HIP_TRY(hipcub::DeviceRadixSort::SortKeys(tmp.buf.p, bytes, e1.p, e2.p, (int)ne, 0,
                                                   32 + vbits, s));
@@ -263,8 +263,8 @@ int hdfe_loo_prune(int64_t n, const int64_t *a_ids, const int64_t *b_ids, uint8_
     const int64_t nv = n0 + n1;
     DevBuf<int64_t> indptr;
     DevBuf<int> dst;
-    DEV_ALLOC(indptr, nv + 1);
-    DEV_ALLOC(dst, ne);
+    DEV_ALLOC(indptr, nv + 1, s);
+    DEV_ALLOC(dst, ne, s);
     hipLaunchKernelGGL(k_csr_ptr, dim3(grid1(nv + 1)), dim3(256), 0, s, nv, ne, e2.p, indptr.p);
     hipLaunchKernelGGL(k_csr_dst, dim3(grid1(ne)), dim3(256), 0, s, ne, e2.p, dst.p);
     HIP_TRY(hipGetLastError());
@@ -315,7 +315,7 @@ int hdfe_loo_prune(int64_t n, const int64_t *a_ids, const int64_t *b_ids, uint8_
     for (int64_t v = 0; v < n0; ++v) a0 += alive[v];
     for (int64_t v = n0; v < nv; ++v) a1 += alive[v];
     DevBuf<uint8_t> alive_d;
-    DEV_ALLOC(alive_d, nv);
+    DEV_ALLOC(alive_d, nv, s);
     HIP_TRY(hipMemcpyAsync(alive_d.p, alive.data(), nv, hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(k_keep_rows, dim3(grid1(n)), dim3(256), 0, s, n, ca.p, cb.p, n0, alive_d.p,
                        keep);

@@ -358,7 +358,7 @@ int hdfe_kss_reduce(void *rows_h, int k, const int *g, const double *w, const do
     if (st) return st;
     // the chunk sums, then per level in chunk order (rows.hip's k_center_sum)
     DevBuf<double> part;
-    DEV_ALLOC(part, nchunk * (int64_t)k);
+    DEV_ALLOC(part, nchunk * (int64_t)k, nullptr);
     hipLaunchKernelGGL(k_reduce_levels, dim3(grid1(nchunk * 64)), dim3(256), 0, 0, nchunk, k, clvl,
                        cbeg, rptr, rrow, g, w, x, R, V, Wg, part.p);
     HIP_TRY(hipGetLastError());
@@ -396,7 +396,7 @@ int hdfe_kss_trace_moments(int64_t G, int k, const int *gstart, const int *psi_c
                            const double *Ul, const double *Ug, double *out) {
     const int nb = 512;
     DevBuf<double> part;
-    DEV_ALLOC(part, (int64_t)nb * k * 8);
+    DEV_ALLOC(part, (int64_t)nb * k * 8, nullptr);
     hipLaunchKernelGGL(k_trace_moments, dim3(nb), dim3(256), 0, 0, G, k, gstart, psi_codes,
                        psi_off, w, Zl, Zg, Ul, Ug, part.p);
     HIP_TRY(hipGetLastError());
@@ -421,8 +421,8 @@ int hdfe_kss_drop_null(int64_t G, int64_t L, int D, int64_t ncomp, int k, const 
     ARG_CHECK(D >= 1 && ncomp >= 1 && k >= 1, "hdfe_kss_drop_null: bad argument");
     const int64_t n1 = nlev_h[0];
     DevBuf<int> gcomp, ldim;
-    DEV_ALLOC(gcomp, G);
-    DEV_ALLOC(ldim, L);
+    DEV_ALLOC(gcomp, G, nullptr);
+    DEV_ALLOC(ldim, L, nullptr);
     hipLaunchKernelGGL(k_group_comp, dim3(grid1(G)), dim3(256), 0, 0, G, gstart, codes0, lcomp,
                        gcomp.p);
     std::vector<int> hd(L);
@@ -430,15 +430,15 @@ int hdfe_kss_drop_null(int64_t G, int64_t L, int D, int64_t ncomp, int k, const 
         for (int64_t a = offs_h[d]; a < offs_h[d] + nlev_h[d]; ++a) hd[a] = d;
     HIP_TRY(hipMemcpy(ldim.p, hd.data(), L * 4, hipMemcpyHostToDevice));
     DevBuf<int64_t> offs_d, nlev_d;
-    DEV_ALLOC(offs_d, D);
-    DEV_ALLOC(nlev_d, D);
+    DEV_ALLOC(offs_d, D, nullptr);
+    DEV_ALLOC(nlev_d, D, nullptr);
     HIP_TRY(hipMemcpy(offs_d.p, offs_h, D * 8, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(nlev_d.p, nlev_h, D * 8, hipMemcpyHostToDevice));
     const int q = (int)ncomp + D - 1;
     DevBuf<double> sg, s1, so;
-    DEV_ALLOC(sg, ncomp * k);
-    DEV_ALLOC(s1, ncomp * k);
-    DEV_ALLOC(so, std::max(D - 1, 1) * (int64_t)k);
+    DEV_ALLOC(sg, ncomp * k, nullptr);
+    DEV_ALLOC(s1, ncomp * k, nullptr);
+    DEV_ALLOC(so, std::max(D - 1, 1) * (int64_t)k, nullptr);
     HIP_TRY(hipMemset(sg.p, 0, ncomp * k * 8));
     HIP_TRY(hipMemset(s1.p, 0, ncomp * k * 8));
     std::vector<double> hsg, hs1, hso;
@@ -446,7 +446,7 @@ int hdfe_kss_drop_null(int64_t G, int64_t L, int D, int64_t ncomp, int k, const 
         // one component (a leave-one-out connected panel): block sums
         const int nb = 256;
         DevBuf<double> part;
-        DEV_ALLOC(part, (int64_t)nb * k);
+        DEV_ALLOC(part, (int64_t)nb * k, nullptr);
         std::vector<double> hp;
         auto colsum = [&](const double *X, int64_t lo, int64_t hi, std::vector<double> &dst,
                           size_t at) -> int {
@@ -529,8 +529,8 @@ int hdfe_kss_drop_null(int64_t G, int64_t L, int D, int64_t ncomp, int k, const 
             B[(size_t)r * k + j] = s / A[(size_t)r * q + r];
         }
     DevBuf<double> cc, co;
-    DEV_ALLOC(cc, ncomp * k);
-    DEV_ALLOC(co, std::max(D - 1, 1) * (int64_t)k);
+    DEV_ALLOC(cc, ncomp * k, nullptr);
+    DEV_ALLOC(co, std::max(D - 1, 1) * (int64_t)k, nullptr);
     HIP_TRY(hipMemcpy(cc.p, B.data(), ncomp * k * 8, hipMemcpyHostToDevice));
     if (D > 1)
         HIP_TRY(hipMemcpy(co.p, B.data() + (size_t)ncomp * k, (D - 1) * (int64_t)k * 8,
@@ -552,7 +552,7 @@ int hdfe_kss_sigma2(int64_t n, int m, const double *V, const double *w, int cent
               "hdfe_kss_sigma2: bad argument");
     const int nb = 256;
     DevBuf<double> part;
-    DEV_ALLOC(part, 2 * nb);
+    DEV_ALLOC(part, 2 * nb, nullptr);
     if (centering == 1)
         hipLaunchKernelGGL(k_rootmean_part, dim3(nb), dim3(256), 0, 0, n, m, V, w, part.p);
     else
@@ -578,7 +578,7 @@ int hdfe_kss_plugin(int64_t G, const int *gstart, const int *psi_codes, int64_t 
                     const double *gam, const double *geff, const double *w, double *out) {
     const int nb = 256;
     DevBuf<double> part;
-    DEV_ALLOC(part, 6 * nb);
+    DEV_ALLOC(part, 6 * nb, nullptr);
     hipLaunchKernelGGL(k_plugin_part, dim3(nb), dim3(256), 0, 0, G, gstart, psi_codes, psi_off, gam,
                        geff, w, part.p);
     HIP_TRY(hipGetLastError());

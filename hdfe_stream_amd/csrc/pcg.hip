@@ -931,7 +931,7 @@ static int build_blocked(Matrix &M, const std::vector<int> &ip32) {
     HIP_TRY(hipMalloc(&M.col_lo, 9 * 4));
     HIP_TRY(hipMemcpyAsync(M.col_lo, M.col_lo_h, 9 * 4, hipMemcpyHostToDevice, ms));
     DevBuf<int> cnt;
-    DEV_ALLOC(cnt, 8 * L + 1);
+    DEV_ALLOC(cnt, 8 * L + 1, ms);
     HIP_TRY(hipMemsetAsync(cnt.p + 8 * L, 0, 4, ms));
     hipLaunchKernelGGL(k_blk_rowcount, dim3(grid1(L)), dim3(kThreads), 0, ms, L, M.indptr,
                        M.indices, M.col_lo, cnt.p);

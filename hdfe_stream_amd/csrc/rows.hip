@@ -341,19 +341,19 @@ int hdfe_rows_create(int64_t n, int64_t G, int D, const int *codes, const int64_
     const int rbits = std::max(1, bits_for((uint64_t)n));
     const int lbits = std::max(1, bits_for((uint64_t)L));
     ARG_CHECK(rbits + lbits <= 64, "hdfe_rows_create: keys need more than 64 bits");
-    DEV_ALLOC(R->rptr, L + 1);
-    DEV_ALLOC(R->rrow, n * D);
+    DEV_ALLOC(R->rptr, L + 1, s);
+    DEV_ALLOC(R->rrow, n * D, s);
     DevBuf<uint64_t> k1, k2;
     DevBuf<int> dummy, dummy2;
-    DEV_ALLOC(k1, n);
-    DEV_ALLOC(k2, n);
+    DEV_ALLOC(k1, n, s);
+    DEV_ALLOC(k2, n, s);
     for (int d = 0; d < D; ++d) {
         hipLaunchKernelGGL(k_level_row_keys, dim3(grid1(n)), dim3(256), 0, s, n, codes + d * n,
                            offs_h[d], rbits, k1.p);
         size_t bytes = 0;
         HIP_TRY(hipcub::DeviceRadixSort::SortKeys(nullptr, bytes, k1.p, k2.p, (int)n, 0,
                                                   rbits + lbits, s));
-        HIP_TRY(R->tmp.ensure(bytes));
+        HIP_TRY(R->tmp.ensure(bytes, s));
         bytes = (size_t)R->tmp.buf.n;
         HIP_TRY(hipcub::DeviceRadixSort::SortKeys(R->tmp.buf.p, bytes, k1.p, k2.p, (int)n, 0,
                                                   rbits + lbits, s));
@@ -372,15 +372,15 @@ int hdfe_rows_create(int64_t n, int64_t G, int D, const int *codes, const int64_
         HIP_TRY(hipStreamSynchronize(s));
     }
     DevBuf<int> nch;
-    DEV_ALLOC(nch, L + 1);
-    DEV_ALLOC(R->cptr, L + 1);
+    DEV_ALLOC(nch, L + 1, s);
+    DEV_ALLOC(R->cptr, L + 1, s);
     hipLaunchKernelGGL(k_chunk_count, dim3(grid1(L + 1)), dim3(256), 0, s, L, R->rptr.p, nch.p);
     HIP_TRY(exclusive_sum(R->tmp, nch.p, R->cptr.p, L + 1, s));
     int nchunk = 0;
     HIP_TRY(scalar_to_host(&nchunk, R->cptr.p + L, s));
     R->nchunk = nchunk;
-    DEV_ALLOC(R->clvl, nchunk);
-    DEV_ALLOC(R->cbeg, nchunk);
+    DEV_ALLOC(R->clvl, nchunk, s);
+    DEV_ALLOC(R->cbeg, nchunk, s);
     hipLaunchKernelGGL(k_chunk_fill, dim3(grid1(L)), dim3(256), 0, s, L, R->rptr.p, R->cptr.p,
                        R->clvl.p, R->cbeg.p);
     HIP_TRY(hipGetLastError());
@@ -391,8 +391,8 @@ int hdfe_rows_create(int64_t n, int64_t G, int D, const int *codes, const int64_
 
 static int ensure_k(RowPasses &R, int k) {
     if (R.kcap >= k) return HDFE_STATUS_OK;
-    DEV_ALLOC(R.Rbar, R.G * (int64_t)k);
-    DEV_ALLOC(R.part, R.nchunk * (int64_t)k);
+    DEV_ALLOC(R.Rbar, R.G * (int64_t)k, R.s);
+    DEV_ALLOC(R.part, R.nchunk * (int64_t)k, R.s);
     R.kcap = k;
     return HDFE_STATUS_OK;
 }
@@ -442,7 +442,7 @@ int hdfe_jla_combine(int64_t n, int64_t n_draws, double *moments, double *stats)
     ARG_CHECK(moments && stats && n >= 1 && n_draws >= 1, "hdfe_jla_combine: bad argument");
     const int nb = 256;
     DevBuf<double> part;
-    DEV_ALLOC(part, nb * 4);
+    DEV_ALLOC(part, nb * 4, nullptr);
     hipLaunchKernelGGL(k_combine_moments, dim3(nb), dim3(256), 0, 0, n, (double)n_draws, moments,
                        part.p);
     HIP_TRY(hipGetLastError());
@@ -490,11 +490,11 @@ int hdfe_rows_chunk_sum(void *h, int k, const double *part, double *out) {
 void hdfe_rows_destroy(void *h) {
     if (!h) return;
     RowPasses *R = (RowPasses *)h;
-    if (R->s) {
-        (void)hipStreamSynchronize(R->s);
-        (void)hipStreamDestroy(R->s);
-    }
+    // buffers are freed on the passes' stream, which must outlive them
+    (void)hipDeviceSynchronize();
+    hipStream_t s = R->s;
     delete R;
+    if (s) (void)hipStreamDestroy(s);
 }
 
 }  // extern "C"
