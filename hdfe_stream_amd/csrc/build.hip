@@ -102,6 +102,30 @@ __global__ void k_minmax_partial(int64_t n, const int64_t *__restrict__ col,
 
 constexpr int kLdsBins = 8192;  // dimensions up to this many levels count in LDS
 
+// Kernels over runs (a group's cells or rows) give each run W adjacent lanes,
+// W near the mean run length: a wave then reads the consecutive runs of 64/W
+// groups with coalesced loads, where a thread per run would stride.  Sums
+// over a run's lanes use a fixed butterfly (deterministic).
+template <int W>
+__device__ __forceinline__ double run_sum(double x) {
+#pragma unroll
+    for (int o = W / 2; o > 0; o >>= 1) x += __shfl_xor(x, o, W);
+    return x;
+}
+
+static int run_width(int64_t items, int64_t runs) {
+    const int64_t avg = runs > 0 ? (items + runs - 1) / runs : 1;
+    return avg <= 4 ? 4 : avg <= 8 ? 8 : avg <= 16 ? 16 : 32;
+}
+
+#define HDFE_BY_WIDTH(w, LAUNCH) \
+    switch (w) {                  \
+    case 4: LAUNCH(4); break;     \
+    case 8: LAUNCH(8); break;     \
+    case 16: LAUNCH(16); break;   \
+    default: LAUNCH(32); break;   \
+    }
+
 // Singletons: rows per level among the rows still kept.  A dimension with
 // few levels (years: every row hits one of a handful of counters) counts in
 // per-workgroup LDS bins first, then adds them to the global counters.
@@ -398,6 +422,7 @@ __global__ void k_ident_ncells(int64_t G, const int *__restrict__ gcstart,
 
 // Scatter the identifying cells: for group g with ident, its cells go to
 // [ipos[g], ...) (ipos = exclusive sum of cells over identifying groups).
+template <int W>
 __global__ void k_ident_cells(int64_t G, int D, int m, const int *__restrict__ gcstart,
                               const int *__restrict__ ident, const int *__restrict__ gpos,
                               const int *__restrict__ cpos, const int *__restrict__ cstart,
@@ -406,15 +431,17 @@ __global__ void k_ident_cells(int64_t G, int D, int m, const int *__restrict__ g
                               int *__restrict__ icodes, int64_t ild, double *__restrict__ in,
                               double *__restrict__ isums, int *__restrict__ igstart,
                               int *__restrict__ igroup) {
-    for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < G;
-         g += (int64_t)gridDim.x * blockDim.x) {
+    const int sub = threadIdx.x % W;
+    for (int64_t g = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / W; g < G;
+         g += (int64_t)gridDim.x * blockDim.x / W) {
         if (!ident[g]) continue;
-        const int q = gpos[g];
-        int o = cpos[g];
-        igstart[q] = o;
-        igroup[q] = (int)g;
-        for (int c = gcstart[g]; c < gcstart[g + 1]; ++c, ++o) {
-            const int r = cstart[c];
+        const int q = gpos[g], o0 = cpos[g], c0 = gcstart[g];
+        if (sub == 0) {
+            igstart[q] = o0;
+            igroup[q] = (int)g;
+        }
+        for (int c = c0 + sub; c < gcstart[g + 1]; c += W) {
+            const int r = cstart[c], o = o0 + (c - c0);
             for (int d = 0; d < D; ++d) icodes[d * ild + o] = codes[d * ld + r];
             in[o] = cn[c];
             for (int j = 0; j < m; ++j) isums[(int64_t)o * m + j] = csums[(int64_t)c * m + j];
@@ -437,10 +464,12 @@ __global__ void k_inc_keys(int64_t M, int D, const int *__restrict__ cgroup,
     }
 }
 
+template <int W>
 __global__ void k_cell_group(int64_t Gi, const int *__restrict__ igstart, int *__restrict__ cg) {
-    for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < Gi;
-         g += (int64_t)gridDim.x * blockDim.x)
-        for (int c = igstart[g]; c < igstart[g + 1]; ++c) cg[c] = (int)g;
+    const int sub = threadIdx.x % W;
+    for (int64_t g = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / W; g < Gi;
+         g += (int64_t)gridDim.x * blockDim.x / W)
+        for (int c = igstart[g] + sub; c < igstart[g + 1]; c += W) cg[c] = (int)g;
 }
 
 // Per incidence (a run of equal keys): group, level, u = sum n, s = sums.
@@ -467,18 +496,23 @@ __global__ void k_inc_reduce(int64_t E, int m, int lbits, const uint64_t *__rest
 }
 
 // Group weight N_g and vbar_g = sum of cell sums / N_g.
+template <int W>
 __global__ void k_group_totals(int64_t Gi, int m, const int *__restrict__ igstart,
                                const double *__restrict__ in, const double *__restrict__ isums,
                                double *__restrict__ Ng, double *__restrict__ vbar) {
-    for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < Gi;
-         g += (int64_t)gridDim.x * blockDim.x) {
+    const int sub = threadIdx.x % W;
+    for (int64_t g = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / W; g < Gi;
+         g += (int64_t)gridDim.x * blockDim.x / W) {
+        const int s = igstart[g], e = igstart[g + 1];
         double N = 0.0;
-        for (int c = igstart[g]; c < igstart[g + 1]; ++c) N += in[c];
-        Ng[g] = N;
+        for (int c = s + sub; c < e; c += W) N += in[c];
+        N = run_sum<W>(N);
+        if (sub == 0) Ng[g] = N;
         for (int j = 0; j < m; ++j) {
             double v = 0.0;
-            for (int c = igstart[g]; c < igstart[g + 1]; ++c) v += isums[(int64_t)c * m + j];
-            vbar[g * m + j] = v / N;
+            for (int c = s + sub; c < e; c += W) v += isums[(int64_t)c * m + j];
+            v = run_sum<W>(v);
+            if (sub == 0) vbar[g * m + j] = v / N;
         }
     }
 }
@@ -1172,7 +1206,8 @@ __global__ void k_sub_global_mean(int64_t nl, int k, int64_t o, int nb,
 // V' M_D V between cells (core/kernels_base.py:510-540), m <= 8: per group
 // q_c = sums_c / n_c - sum_d Gamma[level] less the group mean.  MM is m,
 // fixed at compile time so the m x m sums stay in registers.
-template <int MM>
+// A group's cells run over W adjacent lanes.
+template <int MM, int W>
 __global__ __launch_bounds__(T256) void k_assemble(
     int64_t Gi, int D, const int *__restrict__ igstart, const int *__restrict__ icodes,
     int64_t ild, const int64_t *__restrict__ offs, const double *__restrict__ in,
@@ -1181,35 +1216,39 @@ __global__ __launch_bounds__(T256) void k_assemble(
     double A[MM * MM];
 #pragma unroll
     for (int t = 0; t < MM * MM; ++t) A[t] = 0.0;
+    const int sub = threadIdx.x % W;
     const int64_t g0 = part_begin(Gi, blockIdx.x, gridDim.x), g1 = part_begin(Gi, blockIdx.x + 1, gridDim.x);
-    for (int64_t g = g0 + threadIdx.x; g < g1; g += blockDim.x) {
+    for (int64_t g = g0 + threadIdx.x / W; g < g1; g += blockDim.x / W) {
         const int s = igstart[g], e = igstart[g + 1];
         double qg[MM], N = 0.0;
 #pragma unroll
         for (int j = 0; j < MM; ++j) qg[j] = 0.0;
-        for (int c = s; c < e; ++c) {
-            N += in[c];
+        for (int c = s + sub; c < e; c += W) {
+            const double ic = in[c];
+            N += ic;
 #pragma unroll
             for (int j = 0; j < MM; ++j) {
-                double v = isums[(int64_t)c * MM + j] / in[c];
+                double v = isums[(int64_t)c * MM + j] / ic;
                 for (int d = 0; d < D; ++d) v -= Gm[(offs[d] + icodes[d * ild + c]) * MM + j];
-                qg[j] += in[c] * v;
+                qg[j] += ic * v;
             }
         }
+        N = run_sum<W>(N);
 #pragma unroll
-        for (int j = 0; j < MM; ++j) qg[j] /= N;
-        for (int c = s; c < e; ++c) {
+        for (int j = 0; j < MM; ++j) qg[j] = run_sum<W>(qg[j]) / N;
+        for (int c = s + sub; c < e; c += W) {
+            const double ic = in[c];
             double q[MM];
 #pragma unroll
             for (int j = 0; j < MM; ++j) {
-                double v = isums[(int64_t)c * MM + j] / in[c];
+                double v = isums[(int64_t)c * MM + j] / ic;
                 for (int d = 0; d < D; ++d) v -= Gm[(offs[d] + icodes[d * ild + c]) * MM + j];
                 q[j] = v - qg[j];
             }
 #pragma unroll
             for (int a = 0; a < MM; ++a)
 #pragma unroll
-                for (int b = 0; b < MM; ++b) A[a * MM + b] += in[c] * q[a] * q[b];
+                for (int b = 0; b < MM; ++b) A[a * MM + b] += ic * q[a] * q[b];
         }
     }
 #pragma unroll
@@ -1222,28 +1261,35 @@ __global__ __launch_bounds__(T256) void k_assemble(
 // Streamed effects and residuals (core/kernels_base.py:543-619): per group
 // the weighted mean of y - x'beta - sum_d gam_y[level], the residual the
 // rest.  V column 0 is y, columns xcols[...] the regressors.
+// A group's rows run over W adjacent lanes; a lane keeps its first row's
+// value in a register and corrects any further rows in place.
+template <int W>
 __global__ void k_pass2(int64_t G, int D, int m, const int *__restrict__ gstart,
                         const int *__restrict__ codes, int64_t ld,
                         const int64_t *__restrict__ offs, const double *__restrict__ V,
                         const double *__restrict__ w, const double *__restrict__ gam_y,
                         const double *__restrict__ beta, const int *__restrict__ xcols, int kx,
                         double *__restrict__ geff, double *__restrict__ resid) {
-    for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < G;
-         g += (int64_t)gridDim.x * blockDim.x) {
-        const int s = gstart[g], e = gstart[g + 1];
-        double Wg = 0.0, mu = 0.0;
-        for (int i = s; i < e; ++i) {
+    const int sub = threadIdx.x % W;
+    for (int64_t g = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / W; g < G;
+         g += (int64_t)gridDim.x * blockDim.x / W) {
+        const int s = gstart[g], e = gstart[g + 1], i0 = s + sub;
+        double Wg = 0.0, mu = 0.0, u0 = 0.0;
+        for (int i = i0; i < e; i += W) {
             double u = V[(int64_t)i * m];
             for (int j = 0; j < kx; ++j) u -= V[(int64_t)i * m + xcols[j]] * beta[j];
             for (int d = 0; d < D; ++d) u -= gam_y[offs[d] + codes[d * ld + i]];
             const double wi = w ? w[i] : 1.0;
             Wg += wi;
             mu += wi * u;
-            resid[i] = u;
+            if (i == i0) u0 = u;
+            else resid[i] = u;
         }
-        mu /= Wg;
-        geff[g] = mu;
-        for (int i = s; i < e; ++i) resid[i] -= mu;
+        Wg = run_sum<W>(Wg);
+        mu = run_sum<W>(mu) / Wg;
+        if (sub == 0) geff[g] = mu;
+        if (i0 < e) resid[i0] = u0 - mu;
+        for (int i = i0 + W; i < e; i += W) resid[i] -= mu;
     }
 }
 
@@ -1251,18 +1297,20 @@ __global__ void k_pass2(int64_t G, int D, int m, const int *__restrict__ gstart,
 // group, u_i = V[i, col] - sum_d Gm[level_d(i), col], and out[i, c] = u_i
 // less the group's weighted mean of u (the covariates partialled on the
 // fixed effects, M_F x, from their reduced solutions in Gm (L, m)).
+template <int W>
 __global__ void k_partial(int64_t G, int D, int m, const int *__restrict__ gstart,
                           const int *__restrict__ codes, int64_t ld,
                           const int64_t *__restrict__ offs, const double *__restrict__ V,
                           const double *__restrict__ w, const double *__restrict__ Gm,
                           const int *__restrict__ cols, int kc, double *__restrict__ out) {
-    for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < G;
-         g += (int64_t)gridDim.x * blockDim.x) {
+    const int sub = threadIdx.x % W;
+    for (int64_t g = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / W; g < G;
+         g += (int64_t)gridDim.x * blockDim.x / W) {
         const int s = gstart[g], e = gstart[g + 1];
         for (int c = 0; c < kc; ++c) {
             const int col = cols[c];
             double Wg = 0.0, mu = 0.0;
-            for (int i = s; i < e; ++i) {
+            for (int i = s + sub; i < e; i += W) {
                 double u = V[(int64_t)i * m + col];
                 for (int d = 0; d < D; ++d) u -= Gm[(offs[d] + codes[d * ld + i]) * m + col];
                 const double wi = w ? w[i] : 1.0;
@@ -1270,8 +1318,9 @@ __global__ void k_partial(int64_t G, int D, int m, const int *__restrict__ gstar
                 mu += wi * u;
                 out[(int64_t)i * kc + c] = u;
             }
-            mu /= Wg;
-            for (int i = s; i < e; ++i) out[(int64_t)i * kc + c] -= mu;
+            Wg = run_sum<W>(Wg);
+            mu = run_sum<W>(mu) / Wg;
+            for (int i = s + sub; i < e; i += W) out[(int64_t)i * kc + c] -= mu;
         }
     }
 }
@@ -1545,9 +1594,13 @@ static int run_pass1(Panel &P) {
     DEV_ALLOC(P.isums, std::max<int64_t>(M * m, 1), s);
     DEV_ALLOC(P.igstart, Gi + 1, s);
     DEV_ALLOC(P.igroup, std::max(Gi, 1), s);
-    hipLaunchKernelGGL(k_ident_cells, dim3(grid1(ng)), dim3(T256), 0, s, (int64_t)ng, D, m,
-                       P.cgstart.p, ident.p, gpos.p, cpos.p, P.cstart.p, P.codes.p, nk, cn.p,
-                       csums.p, P.icodes.p, M, P.in.p, P.isums.p, P.igstart.p, P.igroup.p);
+#define HDFE_L(W)                                                                                \
+    hipLaunchKernelGGL(k_ident_cells<W>, dim3(grid1((int64_t)ng * W)), dim3(T256), 0, s,        \
+                       (int64_t)ng, D, m, P.cgstart.p, ident.p, gpos.p, cpos.p, P.cstart.p,     \
+                       P.codes.p, nk, cn.p, csums.p, P.icodes.p, M, P.in.p, P.isums.p,         \
+                       P.igstart.p, P.igroup.p)
+    HDFE_BY_WIDTH(run_width(P.C, ng), HDFE_L)
+#undef HDFE_L
     HIP_TRY(hipMemcpyAsync(P.igstart.p + Gi, cpos.p + ng, 4, hipMemcpyDeviceToDevice, s));
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(s));
@@ -1802,7 +1855,12 @@ int hdfe_panel_system(void *h, int with_rhs, void **matrix, double *seconds) {
     ARG_CHECK(lbits + gbits <= 64, "hdfe_panel_system: incidence keys need more than 64 bits");
     const int64_t nE = M * D;
     DEV_ALLOC(P.cgroup, std::max<int64_t>(M, 1), s);
-    hipLaunchKernelGGL(k_cell_group, dim3(grid1(Gi)), dim3(T256), 0, s, Gi, P.igstart.p, P.cgroup.p);
+    const int cw = run_width(M, Gi);
+#define HDFE_L(W)                                                                       \
+    hipLaunchKernelGGL(k_cell_group<W>, dim3(grid1(Gi * W)), dim3(T256), 0, s, Gi,      \
+                       P.igstart.p, P.cgroup.p)
+    HDFE_BY_WIDTH(cw, HDFE_L)
+#undef HDFE_L
     DevBuf<uint64_t> k1, k2;
     DevBuf<int> v1, v2, head, rank;
     DEV_ALLOC(k1, std::max<int64_t>(nE, 1), s);
@@ -1834,8 +1892,11 @@ int hdfe_panel_system(void *h, int with_rhs, void **matrix, double *seconds) {
     DEV_ALLOC(P.gis, Gi + 1, s);
     hipLaunchKernelGGL(k_inc_reduce, dim3(grid1(E)), dim3(T256), 0, s, (int64_t)E, m, lbits, k2.p,
                        v2.p, istart.p, P.in.p, P.isums.p, P.ig.p, P.il.p, P.iu.p, P.is.p);
-    hipLaunchKernelGGL(k_group_totals, dim3(grid1(Gi)), dim3(T256), 0, s, Gi, m, P.igstart.p,
-                       P.in.p, P.isums.p, P.Ng.p, P.vbar.p);
+#define HDFE_L(W)                                                                        \
+    hipLaunchKernelGGL(k_group_totals<W>, dim3(grid1(Gi * W)), dim3(T256), 0, s, Gi, m,  \
+                       P.igstart.p, P.in.p, P.isums.p, P.Ng.p, P.vbar.p)
+    HDFE_BY_WIDTH(cw, HDFE_L)
+#undef HDFE_L
     hipLaunchKernelGGL(k_group_inc_start, dim3(grid1(E)), dim3(T256), 0, s, (int64_t)E, P.ig.p,
                        P.gis.p, Gi);
     // level CSR of the incidences (sorted by level, then group)
@@ -2213,18 +2274,24 @@ int hdfe_panel_assemble(void *h, const double *Gm, double *A) {
     Panel &P = *(Panel *)h;
     ARG_CHECK(P.m <= 8, "hdfe_panel_assemble: cell assembly needs m <= 8");
     hipStream_t s = P.s;
-    const int m = P.m, nb = 4096;
+    const int m = P.m, nb = 4096, cw = run_width(P.M, P.Gi);
     DevBuf<double> part;
     DEV_ALLOC(part, (int64_t)nb * m * m, s);
     switch (m) {
-#define HDFE_ASM(MM)                                                                            \
-    case MM:                                                                                    \
-        hipLaunchKernelGGL(k_assemble<MM>, dim3(nb), dim3(T256), 0, s, P.Gi, P.D, P.igstart.p, \
-                           P.icodes.p, P.M, P.offs.p, P.in.p, P.isums.p, Gm, part.p);          \
+#define HDFE_ASMW(MM, W)                                                                     \
+    hipLaunchKernelGGL((k_assemble<MM, W>), dim3(nb), dim3(T256), 0, s, P.Gi, P.D, P.igstart.p, \
+                       P.icodes.p, P.M, P.offs.p, P.in.p, P.isums.p, Gm, part.p)
+#define HDFE_ASM(MM)                                          \
+    case MM:                                                  \
+        if (cw == 4) HDFE_ASMW(MM, 4);                        \
+        else if (cw == 8) HDFE_ASMW(MM, 8);                   \
+        else if (cw == 16) HDFE_ASMW(MM, 16);                 \
+        else HDFE_ASMW(MM, 32);                               \
         break;
         HDFE_ASM(1) HDFE_ASM(2) HDFE_ASM(3) HDFE_ASM(4) HDFE_ASM(5) HDFE_ASM(6) HDFE_ASM(7)
         HDFE_ASM(8)
 #undef HDFE_ASM
+#undef HDFE_ASMW
     }
     HIP_TRY(hipGetLastError());
     std::vector<double> hp, Wh;
@@ -2254,9 +2321,13 @@ int hdfe_panel_pass2(void *h, const double *Gm, const double *beta, const int *x
         HIP_TRY(hipMemcpyAsync(xc.p, xcols, kx * 4, hipMemcpyHostToDevice, s));
     }
     hipLaunchKernelGGL(k_gam_y, dim3(grid1(P.L)), dim3(T256), 0, s, P.L, P.m, Gm, b.p, xc.p, kx, gam_y);
-    hipLaunchKernelGGL(k_pass2, dim3(grid1(P.G)), dim3(T256), 0, s, P.G, P.D, P.m, P.gstart.p,
-                       P.codes.p, P.n, P.offs.p, P.V.p, P.weighted ? P.w.p : (const double *)nullptr,
-                       gam_y, b.p, xc.p, kx, geff, resid);
+#define HDFE_L(W)                                                                            \
+    hipLaunchKernelGGL(k_pass2<W>, dim3(grid1(P.G * W)), dim3(T256), 0, s, P.G, P.D, P.m,    \
+                       P.gstart.p, P.codes.p, P.n, P.offs.p, P.V.p,                          \
+                       P.weighted ? P.w.p : (const double *)nullptr, gam_y, b.p, xc.p, kx,   \
+                       geff, resid)
+    HDFE_BY_WIDTH(run_width(P.n, P.G), HDFE_L)
+#undef HDFE_L
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(s));
     return HDFE_STATUS_OK;
@@ -2290,9 +2361,12 @@ int hdfe_panel_partial(void *h, const double *Gm, const int *cols, int kc, doubl
     DevBuf<int> cc;
     DEV_ALLOC(cc, kc, s);
     HIP_TRY(hipMemcpyAsync(cc.p, cols, kc * 4, hipMemcpyHostToDevice, s));
-    hipLaunchKernelGGL(k_partial, dim3(grid1(P.G)), dim3(T256), 0, s, P.G, P.D, P.m, P.gstart.p,
-                       P.codes.p, P.n, P.offs.p, P.V.p, P.weighted ? P.w.p : (const double *)nullptr,
-                       Gm, cc.p, kc, out);
+#define HDFE_L(W)                                                                            \
+    hipLaunchKernelGGL(k_partial<W>, dim3(grid1(P.G * W)), dim3(T256), 0, s, P.G, P.D, P.m,  \
+                       P.gstart.p, P.codes.p, P.n, P.offs.p, P.V.p,                          \
+                       P.weighted ? P.w.p : (const double *)nullptr, Gm, cc.p, kc, out)
+    HDFE_BY_WIDTH(run_width(P.n, P.G), HDFE_L)
+#undef HDFE_L
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(s));
     return HDFE_STATUS_OK;
