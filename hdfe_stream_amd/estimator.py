@@ -59,6 +59,23 @@ def _to_device(a, dtype):
     return torch.from_numpy(np.ascontiguousarray(a)).to(device="cuda", dtype=dtype)
 
 
+def _pinned(n, dtype):
+    """A numpy array of n elements in page-locked host memory (it keeps its
+    torch buffer alive)."""
+    import torch
+    t = torch.empty(n, dtype=torch.from_numpy(np.empty(0, dtype)).dtype, pin_memory=True)
+    return t.numpy()
+
+
+def _host(t):
+    """A device tensor's host copy through page-locked memory."""
+    out = _pinned(t.numel(), np.dtype(str(t.dtype).replace("torch.", "")))
+    if t.numel():
+        import torch
+        torch.from_numpy(out).copy_(t.reshape(-1))
+    return out.reshape(tuple(t.shape))
+
+
 class Panel:
     """A panel's rows on the device after passes 0 and 1 (hdfe_panel_*)."""
 
@@ -114,10 +131,15 @@ class Panel:
                    "hdfe_panel_array")
         return p.value, int(n.value)
 
-    def array(self, name, dtype):
-        """A copy of a panel array on the host."""
+    def array(self, name, dtype, pinned=False):
+        """A copy of a panel array on the host; with `pinned`, in page-locked
+        memory from torch's caching host allocator (a DMA copy at full link
+        speed, where pageable memory goes through staging buffers)."""
         p, n = self.ptr(name)
-        out = np.empty(n, dtype)
+        if pinned and n:
+            out = _pinned(n, dtype)
+        else:
+            out = np.empty(n, dtype)
         if n:
             _lib.check(_lib.load().hdfe_memcpy(out.ctypes.data, C.c_void_p(p), out.nbytes, 0),
                        "hdfe_memcpy")
@@ -333,12 +355,12 @@ class StreamingHDFE:
             info["spmm"] = panel.matrix.timing_read()
         # effects per dimension, with their level values
         fe_order = [self.fe[panel.info["gdim"]]] + [self.fe[d] for d in panel.dims]
-        gy = gam_y.cpu().numpy()
-        levels = {fe_order[0]: panel.array("glevels", np.int64)}
-        values = {fe_order[0]: geff.cpu().numpy()}
+        gy = _host(gam_y)
+        levels = {fe_order[0]: panel.array("glevels", np.int64, pinned=True)}
+        values = {fe_order[0]: _host(geff)}
         for t, d in enumerate(fe_order[1:]):
             o, nl = int(panel.offs[t]), panel.n_levels[t]
-            levels[d] = panel.array(f"levels{t}", np.int64)
+            levels[d] = panel.array(f"levels{t}", np.int64, pinned=True)
             values[d] = gy[o:o + nl]
         diag = {"seconds_pass01": round(t1 - t0, 4), "seconds_system": round(t2 - t1, 4),
                 "seconds_solve": round(t3 - t2, 4), "seconds_pass2": round(t4 - t3, 4),
