@@ -11,3 +11,7 @@ __version__ = "0.1.0"
 
 from . import rocm  # noqa: F401
 from .rocm import DeviceCSR, explicit_solver, rocm_backend  # noqa: F401
+from .connected import ConnectedComponents, connected_components  # noqa: F401,E402
+from .estimator import feols_stream  # noqa: F401,E402
+from .glm import fepois_stream  # noqa: F401,E402
+from .leaveout import leave_out_kss  # noqa: F401,E402

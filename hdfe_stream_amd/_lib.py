@@ -58,6 +58,11 @@ SIGNATURES = [
     ("hdfe_jla_combine", I32, [I64, I64, P, P]),
     ("hdfe_rows_destroy", None, [P]),
     ("hdfe_loo_prune", I32, [I64, P, P, P, P]),
+    ("hdfe_components", I32, [I64, P, P, I32, I32, I32, I32, P]),
+    ("hdfe_components_info", I32, [P, P]),
+    ("hdfe_components_levels", I32, [P, I32, P, P]),
+    ("hdfe_components_sizes", I32, [P, P]),
+    ("hdfe_components_destroy", None, [P]),
     ("hdfe_rows_csr", I32, [P, P, P, P, P, P, P]),
     ("hdfe_rows_chunk_sum", I32, [P, I32, P, P]),
     ("hdfe_kss_group_weight", I32, [I64, P, P, P]),

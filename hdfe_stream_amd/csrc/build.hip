@@ -1421,12 +1421,6 @@ static int factorize(Panel &P, const int64_t *col, int64_t n, DevBuf<int> &code,
     return factorize_col(P.s, P.tmp, col, n, code, levels, nl);
 }
 
-#define TRY(expr)                              \
-    do {                                       \
-        int _st = (expr);                      \
-        if (_st != HDFE_STATUS_OK) return _st; \
-    } while (0)
-
 // Re-code dimension values to the levels present among kept rows.
 static int recode(Panel &P, int64_t n, DevBuf<int> &code, DevBuf<int64_t> &levels, int64_t &nl,
                   const uint8_t *keep) {

@@ -36,6 +36,13 @@ const char *get_error();
         }                                                                               \
     } while (0)
 
+// Propagate a non-OK status from a call that returns one.
+#define TRY(expr)                              \
+    do {                                       \
+        int _st = (expr);                      \
+        if (_st != HDFE_STATUS_OK) return _st; \
+    } while (0)
+
 constexpr int kWave = 64;
 
 // Sum over the 64 lanes of a wavefront (butterfly: every lane gets the sum).

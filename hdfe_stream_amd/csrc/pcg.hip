@@ -1122,12 +1122,6 @@ static int read_max(const double *dev, int k, hipStream_t s, double *out,
     return HDFE_STATUS_OK;
 }
 
-#define TRY(expr)                                  \
-    do {                                           \
-        int _st = (expr);                          \
-        if (_st != HDFE_STATUS_OK) return _st;     \
-    } while (0)
-
 // FP64 block PCG (core/solve.py:66-140).
 static int solve_f64(Matrix &M, int k, const uint8_t *zero_h, bool has_x0, double tol,
                      int64_t maxiter, int64_t *it_out, int *conv_out, double *rel_out,

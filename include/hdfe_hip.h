@@ -286,6 +286,34 @@ int hdfe_kss_plugin(int64_t G, const int *gstart, const int *psi_codes, int64_t 
 int hdfe_loo_prune(int64_t n, const int64_t *a_ids, const int64_t *b_ids, uint8_t *keep,
                    int64_t *info);
 
+/* Standalone connected components of the graph whose vertices are the levels
+ * of two id columns (a_ids, b_ids: device int64, n rows) and whose edges are
+ * the rows.  min_deg0/min_deg1 >= 1 first prune to the (k0, k1)-core of the
+ * distinct pairs; leave_one_out then keeps the leave-one-out connected set;
+ * rank_by -1 ranks components by rows, 0 or 1 by that dimension's levels.
+ * Levels are coded in sorted order; components are numbered by their
+ * smallest first-dimension level and ranked by size, stably.  *out is a
+ * handle read by the three calls below and freed by hdfe_components_destroy.
+ * Replaces connected_components (core/connected.py:211-338),
+ * _find_components (:378-493) and _nb_peel / _nb_union_keys /
+ * _nb_dense_labels / _nb_pair_rows (core/kernels_graph.py:187-284). */
+int hdfe_components(int64_t n, const int64_t *a_ids, const int64_t *b_ids, int min_deg0,
+                    int min_deg1, int leave_one_out, int rank_by, void **out);
+/* info (host, 16): levels of each dimension, components, rows, distinct
+ * pairs; min_degree's rounds, removed levels of each dimension and removed
+ * rows; the leave-one-out step's components before, workers with one row,
+ * cut workers, rounds, removed levels of each dimension and removed rows
+ * (the reference's diagnostics, core/connected.py:418-456). */
+int hdfe_components_info(void *h, int64_t *info);
+/* Dimension dim's level ids, sorted, and their components (-1: removed):
+ * host arrays of info[dim] entries (ConnectedComponents.levels,
+ * core/connected.py:474-480). */
+int hdfe_components_levels(void *h, int dim, int64_t *levels, int64_t *comp);
+/* Sizes (host, info[2] x 3), largest first: rows and the levels of each
+ * dimension (ConnectedComponents.sizes, core/connected.py:481-483). */
+int hdfe_components_sizes(void *h, int64_t *sizes);
+void hdfe_components_destroy(void *h);
+
 /* ------------------------------------------------- adjoint (autograd)
  * Row passes of the backward of a fitted regression: z_i'u for the adjoint
  * solution u = (Z'WZ)^- g of a loss gradient g, and from it dL/dy_i =

@@ -20,6 +20,9 @@ the reference's FP64 algorithm:
   graph    components and ranking     core/kernels_base.py:155-174,
                                       core/passes.py:933-979
            leave-one-out pruning      core/connected.py:503-562
+           standalone components: (k0, k1)-core peel, union-find labels in
+           root order, ranking        core/connected.py:378-493,
+                                      core/kernels_graph.py:187-262
   KSS      Rademacher draws, centre/project rows, JLA moments and their
            combination               core/kernels_groups.py:52-77,
                                       kernels_inverse.py:34-108,
@@ -906,12 +909,14 @@ def pair_graph(a: np.ndarray, b: np.ndarray, n0: int, n1: int, alive: np.ndarray
     return keys, indptr, indices
 
 
-def leave_one_out(a: np.ndarray, b: np.ndarray, n0: int, n1: int):
+def leave_one_out(a: np.ndarray, b: np.ndarray, n0: int, n1: int, alive=None, rank_by=None):
     """Leave-one-out connected set of the (worker a, firm b) rows: the
     alive mask over the n0 + n1 vertices and the counts
-    (core/connected.py:503-562, ranking by rows)."""
+    (core/connected.py:503-562).  `alive` on entry (default all) is the set
+    left by min_degree; rank_by None ranks components by rows, 0 or 1 by the
+    levels of that dimension (:525-536)."""
     nv = n0 + n1
-    alive = np.ones(nv, bool)
+    alive = np.ones(nv, bool) if alive is None else np.asarray(alive, bool).copy()
     keys, indptr, indices = pair_graph(a, b, n0, n1, alive)
     kk = a.astype(np.int64) * n1 + b.astype(np.int64)
     _, counts = np.unique(kk, return_counts=True)
@@ -921,8 +926,12 @@ def leave_one_out(a: np.ndarray, b: np.ndarray, n0: int, n1: int):
         label, nc = bfs_components(indptr, indices, alive)
         if nc == 0:
             return 0
-        ok = (label[ka] >= 0) & (label[kb] >= 0)
-        size = np.bincount(label[ka][ok], weights=counts[ok], minlength=nc)
+        if rank_by is None:
+            ok = (label[ka] >= 0) & (label[kb] >= 0)
+            size = np.bincount(label[ka][ok], weights=counts[ok], minlength=nc)
+        else:
+            lab = label[:n0] if rank_by == 0 else label[n0:]
+            size = np.bincount(lab[lab >= 0], minlength=nc)
         alive[:] &= label == int(np.argmax(size))
         return nc
 
@@ -946,3 +955,104 @@ def leave_one_out(a: np.ndarray, b: np.ndarray, n0: int, n1: int):
         keep_largest()
     out.update(cut=cut_total, rounds=rounds)
     return alive, out
+
+
+def peel(keys: np.ndarray, n0: int, n1: int, k0: int, k1: int, alive: np.ndarray) -> int:
+    """The (k0, k1)-core of the distinct pairs keys = a * n1 + b, in place on
+    alive (core/kernels_graph.py:187-221): every level with fewer alive
+    partners than its dimension's k goes, all at once per round; returns the
+    rounds that removed something."""
+    ka, kb = keys // n1, n0 + keys % n1
+    thr = np.where(np.arange(n0 + n1) < n0, k0, k1)
+    rounds = 0
+    while True:
+        ok = alive[ka] & alive[kb]
+        deg = (np.bincount(ka[ok], minlength=n0 + n1) + np.bincount(kb[ok], minlength=n0 + n1))
+        drop = alive & (deg < thr)
+        if not drop.any():
+            return rounds
+        alive &= ~drop
+        rounds += 1
+
+
+def union_labels(keys: np.ndarray, n0: int, n1: int, alive: np.ndarray):
+    """Union-find over the alive pairs, each root the smaller vertex
+    (core/kernels_graph.py:224-240), then labels 0, 1, ... in the order of
+    the roots, -1 for dead vertices (:243-262).  Pure Python: small graphs."""
+    nv = n0 + n1
+    parent = np.arange(nv, dtype=np.int64)
+
+    def find(x):
+        while parent[x] != x:
+            parent[x] = parent[parent[x]]
+            x = parent[x]
+        return x
+
+    for key in keys.tolist():
+        a, b = key // n1, n0 + key % n1
+        if not (alive[a] and alive[b]):
+            continue
+        ra, rb = find(a), find(b)
+        if ra != rb:
+            if ra < rb:
+                parent[rb] = ra
+            else:
+                parent[ra] = rb
+    label = np.full(nv, -1, np.int64)
+    nc = 0
+    for x in range(nv):
+        r = find(x)
+        if not alive[x]:
+            continue
+        if r == x:
+            label[x] = nc
+            nc += 1
+        else:
+            label[x] = label[r]
+    return label, nc
+
+
+def connected_components(a: np.ndarray, b: np.ndarray, n0: int, n1: int, min_degree=(1, 1),
+                         leave_one_out_set: bool = False, rank_by=None):
+    """Components of the graph of the level codes (a, b) of the rows
+    (core/connected.py:378-493): the (k0, k1)-core first, then the
+    leave-one-out set or union-find, then the ranking by size (rows, or the
+    levels of dimension rank_by), stable so that ties go to the component
+    with the smallest first-dimension level.  Returns (comp per vertex,
+    sizes (nc, 3): rows and levels of each dimension, diagnostics)."""
+    kk = a.astype(np.int64) * n1 + b.astype(np.int64)
+    keys, counts = np.unique(kk, return_counts=True)
+    nv = n0 + n1
+    alive = np.ones(nv, bool)
+    diag = {"rows": int(len(a)), "distinct_pairs": int(len(keys))}
+    ka, kb = keys // n1, n0 + keys % n1
+
+    def rows_alive(al):
+        return int(counts[al[ka] & al[kb]].sum())
+
+    if tuple(min_degree) != (1, 1):
+        diag["pruning_rounds"] = peel(keys, n0, n1, min_degree[0], min_degree[1], alive)
+        diag["pruned_levels"] = (int(n0 - alive[:n0].sum()), int(n1 - alive[n0:].sum()))
+        diag["pruned_rows"] = len(a) - rows_alive(alive)
+    if leave_one_out_set:
+        before = alive.copy()
+        alive, lo = leave_one_out(a, b, n0, n1, alive=alive, rank_by=rank_by)
+        lo["levels"] = (int((before[:n0] & ~alive[:n0]).sum()),
+                        int((before[n0:] & ~alive[n0:]).sum()))
+        lo["rows"] = rows_alive(before) - rows_alive(alive)
+        diag["leave_one_out"] = lo
+        label = np.where(alive, 0, -1).astype(np.int64)
+        nc = int(alive.any())
+    else:
+        label, nc = union_labels(keys, n0, n1, alive)
+    ok = (label[ka] >= 0) & (label[kb] >= 0)
+    rows = np.bincount(label[ka][ok], weights=counts[ok], minlength=nc).astype(np.int64)
+    c0 = np.bincount(label[:n0][label[:n0] >= 0], minlength=nc)
+    c1 = np.bincount(label[n0:][label[n0:] >= 0], minlength=nc)
+    key = rows if rank_by is None else (c0 if rank_by == 0 else c1)
+    order = np.argsort(-key, kind="stable")
+    rank = np.empty(nc, np.int64)
+    rank[order] = np.arange(nc)
+    comp = np.where(label >= 0, rank[np.maximum(label, 0)], -1)
+    sizes = np.stack([rows[order], c0[order], c1[order]], axis=1) if nc else np.zeros((0, 3), np.int64)
+    return comp, sizes, diag
