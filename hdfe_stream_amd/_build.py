@@ -12,7 +12,7 @@ from pathlib import Path
 PKG = Path(__file__).resolve().parent
 CSRC = PKG / "csrc"
 LIB = PKG / "libhdfe_hip.so"
-SOURCES = ["util.hip", "pcg.hip", "build.hip", "rows.hip", "kss.hip", "graph.hip", "adjoint.hip",
+SOURCES = ["util.hip", "pcg.hip", "build.hip", "rows.hip", "kss.hip", "graph.hip", "adjoint.hip", "se.hip",
            "glm.hip"]
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
          # packed FP32 instructions stay off: hipcc would otherwise emit them

@@ -26,7 +26,9 @@ within-match stayer rule and the reference centring.
 Not mirrored yet: covariates in the leave-out design (the Schur border,
 inverse.py:182-231) -- at observation level a model with covariates is
 refused (at match level the row model may have them: they are partialled out
-before the collapse); standard errors and the weak-identification diagnostic.
+before the collapse); standard errors leaving out a match, and the
+weak-identification diagnostic.  Standard errors leaving out an observation
+are in `leaveout_se`.
 """
 
 from __future__ import annotations
@@ -371,6 +373,7 @@ class LeaveOutComponents:
     n_movers: int
     leverages: JLALeverages
     trace: TraceEstimate
+    se: object = None
     diagnostics: dict = field(default_factory=dict)
     pruning: dict | None = None
     fit: object = None
@@ -386,6 +389,8 @@ class LeaveOutComponents:
         for k in self.leave_out:
             lines.append(f"  {k:18s}{self.plug_in[k]:12.6f}{self.bias[k]:12.6f}"
                          f"{self.leave_out[k]:12.6f}")
+        if self.se is not None:
+            lines += ["", self.se.summary()]
         return "\n".join(lines)
 
 
@@ -408,7 +413,8 @@ def leave_out_components(result, n_draws: int = 250, block: int | None = None, s
                          psi: str | None = None, tol: float = 1e-10, maxiter: int = 5000,
                          precision: str = "mixed", timings: dict | None = None,
                          stayers: str = "own", stayer_sigma2=None, centering: str = "weighted",
-                         person_years: int | None = None) -> LeaveOutComponents:
+                         person_years: int | None = None, se: bool = False, se_draws=None,
+                         se_trace: bool = True, smooth_bins=None) -> LeaveOutComponents:
     """KSS components for a fit (leaveout.py:634-805): JLA leverages, the
     leverage check, the plug-in moments, sigma2, the Hutchinson bias with
     seed + 1_000_003, and plug-in minus bias.
@@ -418,7 +424,10 @@ def leave_out_components(result, n_draws: int = 250, block: int | None = None, s
     leaveout.py:1012-1016); centering "weighted" or "reference"
     (leaveout.py:996-1003); `person_years` the R of the divisor T (R - 1) / R
     when the rows are matches (leaveout_se.py:216-220).  With "within_match"
-    only movers' rows are held to the leverage check (leaveout.py:712-724)."""
+    only movers' rows are held to the leverage check (leaveout.py:712-724).
+    With `se`, the standard errors of `leaveout_se.component_variances`
+    (seed + 5_500_011, `se_draws` draws, default n_draws; leaveout.py:
+    759-775), leaving out an observation."""
     if stayers not in ("own", "within_match"):
         raise ValueError("stayers must be 'own' or 'within_match' on the device")
     if (stayers == "within_match") != (stayer_sigma2 is not None):
@@ -497,10 +506,25 @@ def leave_out_components(result, n_draws: int = 250, block: int | None = None, s
         sigma2_mean = float((sigma2 / wt).mean())
     else:
         sigma2_mean = float(sigma2.mean())
-    t.update(leverage_seconds=t1 - t0, sigma2_seconds=t2 - t1, bias_seconds=t3 - t2)
+    errors = None
+    if se:
+        if match_level:
+            raise NotImplementedError("standard errors leaving out a match are not built on the "
+                                      "device; use leave_out='observation'")
+        from .leaveout_se import BII_SEED_OFFSET, component_variances
+        inv = CoefficientInverse(panel, tol, maxiter, precision)
+        y = panel.tensor("V", torch.float64).view(n, info["m"])[:, 0].contiguous()
+        errors = component_variances(
+            inv, y, sigma2, lev, psi=pi, rows=R,
+            n_draws=n_draws if se_draws is None else int(se_draws), block=block,
+            seed=seed + BII_SEED_OFFSET, trace=se_trace, smooth_bins=smooth_bins)
+        errors.diagnostics.update(psi=psi, alpha=alpha, covariates=[])
+    t4 = time.perf_counter()
+    t.update(leverage_seconds=t1 - t0, sigma2_seconds=t2 - t1, bias_seconds=t3 - t2,
+             se_seconds=t4 - t3)
     return LeaveOutComponents(
         plug_in=plug_in, bias=bias, leave_out=leave_out, n_obs=n,
-        n_movers=int(movers.sum()), leverages=lev, trace=trace,
+        n_movers=int(movers.sum()), leverages=lev, trace=trace, se=errors,
         diagnostics={"psi": psi, "alpha": alpha, "stayers": stayers, "max_leverage": worst,
                      "leave_out": "match" if match_level else "observation",
                      "centering": centering, "sigma2_mean": sigma2_mean,
@@ -589,7 +613,8 @@ def leave_out_kss(fml: str, data, *, psi: str | None = None, n_draws: int = 250,
                   leave_out: str = "match", stayers: str | None = None,
                   centering: str = "reference",
                   precision: str = "mixed", tol: float = 1e-10, maxiter: int = 5000,
-                  timings: dict | None = None, **options) -> LeaveOutComponents:
+                  timings: dict | None = None, se: bool = False, se_draws=None,
+                  se_trace: bool = True, smooth_bins=None, **options) -> LeaveOutComponents:
     """Kline-Saggio-Solvsten leave-out variance components, end to end, on
     the device (leaveout.py:1069-1255): prune to the largest leave-one-out
     connected set of (alpha, psi), fit with alpha streamed and no singleton
@@ -600,7 +625,9 @@ def leave_out_kss(fml: str, data, *, psi: str | None = None, n_draws: int = 250,
     second) the dimension whose variance is wanted.  At observation level the
     stayer rule is "own" and the centring weighted; at match level the
     within-match rule and the reference centring (the reference's defaults
-    there)."""
+    there).  `se`, `se_draws`, `se_trace` and `smooth_bins` ask for the
+    components' standard errors (leaveout_se.py), leaving out an
+    observation."""
     import torch
 
     from .connected import leave_one_out
@@ -618,6 +645,9 @@ def leave_out_kss(fml: str, data, *, psi: str | None = None, n_draws: int = 250,
                                   "device")
     if stayers not in (None, "own"):
         raise NotImplementedError("only stayers='own' is implemented on the device")
+    if se and leave_out == "match":
+        raise NotImplementedError("standard errors leaving out a match are not built on the "
+                                  "device; use leave_out='observation'")
     if "|" not in fml:
         raise ValueError(
             f"{fml!r} has no fixed effects; leave-out estimation needs at least two "
@@ -659,7 +689,9 @@ def leave_out_kss(fml: str, data, *, psi: str | None = None, n_draws: int = 250,
         comps = _leave_out_match(fit, psi, n_draws, block, seed, tol, maxiter, precision, t)
     else:
         comps = leave_out_components(fit, n_draws=n_draws, block=block, seed=seed, psi=psi,
-                                     tol=tol, maxiter=maxiter, precision=precision, timings=t)
+                                     tol=tol, maxiter=maxiter, precision=precision, timings=t,
+                                     se=se, se_draws=se_draws, se_trace=se_trace,
+                                     smooth_bins=smooth_bins)
     t.update(prune_seconds=t1 - t0, fit_seconds=t2 - t1,
              total_seconds=time.perf_counter() - t0)
     comps.pruning = pruning

@@ -314,6 +314,44 @@ int hdfe_components_levels(void *h, int dim, int64_t *levels, int64_t *comp);
 int hdfe_components_sizes(void *h, int64_t *sizes);
 void hdfe_components_destroy(void *h);
 
+/* ------------------------------------------ KSS standard errors (se.hip)
+ * Row passes of the components' standard errors, in the square-root-weight
+ * metric; blocks of s vectors are row-major (rows, s); b and s3 are (3, n). */
+
+/* The first solve's stacked input of applying C: [v, b_k v] for a shared v
+ * (n, s), or v_k = z sqrt(max(s_k, 0)) and [v_k, b_k v_k] per component from
+ * draws z (n, s); out (n, (1 or 3) s + 3 s).  Replaces the stacking of
+ * _apply_c_passes (leaveout_se.py:515-530). */
+int hdfe_se_stack(int64_t n, int s, const double *v, const double *z, const double *s3,
+                  const double *b, double *out);
+/* C v at row level from the two solves' read-backs base and thq, reduced in
+ * a fixed order (host out): mode 0, v = yt (s = 1): 4 sum s_k (C y)_k^2 and
+ * yt'(C y)_k; mode 1, per-component draws: sum_i s_k,i sum_j (C v)_kij^2.
+ * u, c2, udot, odot: the rank-two term of the reference centring.
+ * Replaces _apply_c_passes' pass 3 with collect_y / _cc_trace's collect
+ * (leaveout_se.py:565-592, 722-731, 911-925). */
+int hdfe_se_assemble(int64_t n, int s, int mode, const double *v, const double *z,
+                     const double *s3, const double *b, const double *w, const double *base,
+                     const double *thq, const double *u, double c2, const double *udot,
+                     const double *odot, const double *yt, double *out);
+/* out (3, n) += w_i (sum r1^2, sum r2^2, sum r1 r2) of rows (n, 2 s).
+ * Replaces _nb_bii_moments (kernels_se.py:75-98). */
+int hdfe_se_bii(int64_t n, int s, const double *rows, const double *w, double *out);
+/* out (G, s) = per group, sum of w_i Ul[psi_off + psi_code_i]: the
+ * covariance's cross sum over the groups.  Replaces the out_groups half of
+ * _nb_cov_cross (kernels_se.py:44-72). */
+int hdfe_se_cross_groups(int64_t G, int s, const int *gstart, const int *psi_codes,
+                         int64_t psi_off, const double *w, const double *Ul, double *out);
+/* The smoother's cells: per cell of the nb x nb grid of (P_ii, B_ii)
+ * quantile ranks, the sum of sigma2 and the rows (host tot, cnt).  Replaces
+ * fit_smoother's binning (leaveout_se.py:1024-1033). */
+int hdfe_se_cell_sums(int64_t n, const double *lev, const double *bii, const double *sigma2,
+                      const double *kp, const double *kb, int nb, double *tot, double *cnt);
+/* The smoothed sigma2, bilinear between cell centres.  Replaces smooth_eval
+ * (leaveout_se.py:1038-1054). */
+int hdfe_se_smooth(int64_t n, const double *lev, const double *bii, const double *kp,
+                   const double *kb, const double *means, int nb, double *out);
+
 /* ------------------------------------------------- adjoint (autograd)
  * Row passes of the backward of a fitted regression: z_i'u for the adjoint
  * solution u = (Z'WZ)^- g of a loss gradient g, and from it dL/dy_i =

@@ -1056,3 +1056,244 @@ def connected_components(a: np.ndarray, b: np.ndarray, n0: int, n1: int, min_deg
     comp = np.where(label >= 0, rank[np.maximum(label, 0)], -1)
     sizes = np.stack([rows[order], c0[order], c1[order]], axis=1) if nc else np.zeros((0, 3), np.int64)
     return comp, sizes, diag
+
+
+# ------------------------------------------------------ KSS standard errors
+#
+# leaveout_se.py:1-49, 128-950: in the square-root-weight metric X~ = W^(1/2) A,
+# the kernel of component k is C_k = B_k - (diag(b_k) M + M diag(b_k)) / 2 with
+# B_k = X~ S^- Q_k S^- X~', M = I - X~ S^- X~', b_k = B_k,ii / M_ii (times the
+# leverage pass's factor), and V-hat = 4 sum_i s_i (C y~)_i^2 - 2 tr(C Om C Om).
+
+SE_COMPONENTS = ("var(psi)", "var(alpha)", "cov(psi, alpha)")
+SE_QUANTILE_SAMPLE = 2_000_000     # leaveout_se.py:62
+SE_BII_OFFSET = 5_500_011          # leaveout.py:772
+SE_TRACE_OFFSET = 7_700_017        # leaveout_se.py:752
+
+
+class SELayout:
+    """Weight totals per level and group, the grand total, the divisor and
+    the psi block (leaveout_se.py:135-180)."""
+
+    def __init__(self, ctx: KSSContext, psi_col: int, rows: int):
+        self.ctx = ctx
+        w = ctx.w
+        self.level_w = ctx.Do.T @ w
+        self.group_w = ctx.Dg.T @ w
+        self.total = float(self.group_w.sum())
+        self.norm = moment_divisor(self.total, rows)
+        o = int(ctx.c["offs"][psi_col])
+        self.psi = slice(o, o + int(ctx.p0["n_levels"][psi_col]))
+        self.psi_level = o + ctx.p0["codes"][:, psi_col]
+        self.root = np.sqrt(w)
+        self.inv_root = np.divide(1.0, self.root, out=np.zeros_like(self.root),
+                                  where=self.root > 0)
+
+
+def se_reduce(lay: SELayout, R: np.ndarray):
+    """X~' R in coefficient space: (levels, groups) of sum_i sqrt(w_i) R_i (the
+    reference feeds R / sqrt(w) to reductions that apply w)."""
+    ctx = lay.ctx
+    x = lay.root[:, None] * R
+    return ctx.Do.T @ x, ctx.Dg.T @ x
+
+
+def bii_rows(lay: SELayout, n_draws: int, block: int, seed: int) -> np.ndarray:
+    """B_ii of the three components by random projection, (3, n)
+    (leaveout_se.py:228-327): per draw q, the two solves of L_psi'q and
+    L_alpha'q (weight-centred selectors over sqrt(norm)), read back at the
+    rows in the metric, squared and crossed."""
+    ctx = lay.ctx
+    n, L, G = len(ctx.w), ctx.L, ctx.G
+    out = np.zeros((3, n))
+    drawn = 0
+    sq = np.sqrt(lay.norm)
+    while drawn < n_draws:
+        size = min(block, n_draws - drawn)
+        q = rademacher(0, n, size, seed, drawn)
+        lv, gs = se_reduce(lay, q)
+        grand = lv[lay.psi].sum(axis=0) / lay.total
+        rl = np.zeros((L, 2 * size))
+        rl[lay.psi, :size] = (lv[lay.psi] - np.outer(lay.level_w[lay.psi], grand)) / sq
+        rg = np.zeros((G, 2 * size))
+        rg[:, size:] = (gs - np.outer(lay.group_w, grand)) / sq
+        _, _, rows = apply_inverse(ctx, rl, rg)
+        r1 = lay.root[:, None] * rows[:, :size]
+        r2 = lay.root[:, None] * rows[:, size:]
+        out[0] += (r1 * r1).sum(axis=1)
+        out[1] += (r2 * r2).sum(axis=1)
+        out[2] += (r1 * r2).sum(axis=1)
+        drawn += size
+    return out / n_draws
+
+
+def apply_q(lay: SELayout, k: int, Ul: np.ndarray, Ug: np.ndarray):
+    """Q_k u in coefficient space (leaveout_se.py:339-385): the variances by
+    weight-centred scaling, the covariance by crossing the two dimensions
+    over the rows."""
+    ctx = lay.ctx
+    s = Ul.shape[1]
+    ol, og = np.zeros((ctx.L, s)), np.zeros((ctx.G, s))
+    lw = lay.level_w[lay.psi]
+    psi_mean = (lw @ Ul[lay.psi]) / lay.total
+    alpha_mean = (lay.group_w @ Ug) / lay.total
+    if k == 0:
+        ol[lay.psi] = lw[:, None] * (Ul[lay.psi] - psi_mean) / lay.norm
+    elif k == 1:
+        og[:] = lay.group_w[:, None] * (Ug - alpha_mean) / lay.norm
+    else:
+        w = ctx.w[:, None]
+        cross_l = ctx.Do.T @ (w * Ug[ctx.g])
+        cross_g = ctx.Dg.T @ (w * Ul[lay.psi_level])
+        ol[lay.psi] = (cross_l[lay.psi] - lw[:, None] * alpha_mean) / (2 * lay.norm)
+        og[:] = (cross_g - lay.group_w[:, None] * psi_mean) / (2 * lay.norm)
+    return ol, og
+
+
+def apply_c(lay: SELayout, v: np.ndarray, b: np.ndarray, per_component: bool = False,
+            rank_two=None) -> np.ndarray:
+    """C_k v for the three components, (n, 3, s) (leaveout_se.py:426-596):
+    v is (n, s) shared or (n, 3, s) one per component; b (3, n).  Solve 1:
+    S^- X~'[v, b_k v]; Q_k on the input block; solve 2; then
+    C v = B v - (b .* M v + M (b .* v)) / 2, plus rank_two = (u (3, n), c):
+    c (1 u_k' + u_k 1') v."""
+    ctx = lay.ctx
+    n_in = 3 if per_component else 1
+    s = v.shape[-1]
+    part = (lambda k: v[:, k, :]) if per_component else (lambda k: v)
+    stack = np.concatenate([part(k) for k in range(n_in)]
+                           + [part(k) * b[k][:, None] for k in range(3)], axis=1)
+    lv, gs = se_reduce(lay, stack)
+    U0l, U0g, base = apply_inverse(ctx, lv, gs)
+    ql, qg = np.zeros((ctx.L, 3 * s)), np.zeros((ctx.G, 3 * s))
+    inc = lambda k: slice((k if per_component else 0) * s, ((k if per_component else 0) + 1) * s)
+    bc = lambda k: slice((n_in + k) * s, (n_in + k + 1) * s)
+    for k in range(3):
+        ol, og = apply_q(lay, k, U0l[:, inc(k)], U0g[:, inc(k)])
+        ql[:, k * s:(k + 1) * s] = ol
+        qg[:, k * s:(k + 1) * s] = og
+    _, _, thq = apply_inverse(ctx, ql, qg)
+    root = lay.root[:, None]
+    cv = np.empty((len(ctx.w), 3, s))
+    for k in range(3):
+        vk = part(k)
+        Mv = vk - root * base[:, inc(k)]
+        Bv = root * thq[:, k * s:(k + 1) * s]
+        Mbv = vk * b[k][:, None] - root * base[:, bc(k)]
+        cv[:, k, :] = Bv - 0.5 * (b[k][:, None] * Mv + Mbv)
+        if rank_two is not None:
+            u, c = rank_two
+            cv[:, k, :] += c * ((u[k] @ vk)[None, :] + np.outer(u[k], vk.sum(axis=0)))
+    return cv
+
+
+def m_times(lay: SELayout, b: np.ndarray) -> np.ndarray:
+    """u_k = M b_k for the three row vectors (leaveout_se.py:387-424)."""
+    lv, gs = se_reduce(lay, b.T)
+    _, _, base = apply_inverse(lay.ctx, lv, gs)
+    return b - (lay.root[:, None] * base).T
+
+
+def auto_bins(n: int, dims: int = 2) -> int:
+    """Bins per axis, n^(1/6) in two dimensions (leaveout_se.py:955-964)."""
+    power = 1.0 / 6.0 if dims == 2 else 1.0 / 3.0
+    return int(np.clip(round(n ** power), 2, 500))
+
+
+def smoother_knots(values: np.ndarray, n_bins: int, take: np.ndarray) -> np.ndarray:
+    """Quantile knots made strictly increasing (leaveout_se.py:967-973)."""
+    q = np.quantile(np.asarray(values[take], np.float64), np.linspace(0.0, 1.0, n_bins + 1))
+    q = np.maximum.accumulate(q)
+    span = max(float(q[-1] - q[0]), 1e-300)
+    return q + np.arange(len(q)) * span * 1e-12
+
+
+def smoother_rank(knots: np.ndarray, values: np.ndarray) -> np.ndarray:
+    return np.interp(np.asarray(values, np.float64), knots, np.linspace(0.0, 1.0, len(knots)))
+
+
+def smoother_sample(n: int) -> np.ndarray:
+    """The rows the knots are taken from (leaveout_se.py:1021-1023)."""
+    if n <= SE_QUANTILE_SAMPLE:
+        return np.arange(n)
+    return np.sort(np.random.default_rng(0).choice(n, SE_QUANTILE_SAMPLE, replace=False))
+
+
+def fit_smoother(sigma2, leverage, b_ii, n_bins=None):
+    """Cell means of sigma2 on the quantile grid of (P_ii, B_ii)
+    (leaveout_se.py:982-1035): (knots_p, knots_b, means, n_bins)."""
+    n = len(sigma2)
+    nb = auto_bins(n, 2) if n_bins is None else int(n_bins)
+    if nb < 1:
+        return None
+    take = smoother_sample(n)
+    kp, kb = smoother_knots(leverage, nb, take), smoother_knots(b_ii, nb, take)
+    cp = np.minimum((smoother_rank(kp, leverage) * nb).astype(np.int64), nb - 1)
+    cb = np.minimum((smoother_rank(kb, b_ii) * nb).astype(np.int64), nb - 1)
+    flat = cp * nb + cb
+    tot = np.bincount(flat, weights=np.asarray(sigma2, np.float64), minlength=nb * nb)
+    cnt = np.bincount(flat, minlength=nb * nb).astype(np.float64)
+    overall = float(tot.sum() / max(cnt.sum(), 1.0))
+    means = np.where(cnt > 0, tot / np.maximum(cnt, 1.0), overall).reshape(nb, nb)
+    return kp, kb, means, nb
+
+
+def smooth_eval(table, leverage, b_ii) -> np.ndarray:
+    """Bilinear between cell centres (leaveout_se.py:1038-1054)."""
+    kp, kb, means, nb = table
+
+    def at(knots, values):
+        f = smoother_rank(knots, values) * nb - 0.5
+        j0 = np.clip(np.floor(f).astype(np.int64), 0, nb - 2)
+        return j0, np.clip(f - j0, 0.0, 1.0)
+
+    i0, a = at(kp, leverage)
+    j0, c = at(kb, b_ii)
+    return ((1.0 - a) * (1.0 - c) * means[i0, j0] + a * (1.0 - c) * means[i0 + 1, j0]
+            + (1.0 - a) * c * means[i0, j0 + 1] + a * c * means[i0 + 1, j0 + 1])
+
+
+def component_variances(lay: SELayout, y: np.ndarray, sigma2: np.ndarray, leverage,
+                        complement, factor, n_draws: int, block: int, seed: int,
+                        trace: bool = True, smooth_bins=None, b_ii=None, stayer_rows=None,
+                        centering: str = "weighted", reported=(0, 1, 2)) -> dict:
+    """V-hat and the standard errors of the three components
+    (leaveout_se.py:599-784); `seed` is the one component_variances receives
+    (the caller's seed + 5_500_011), the trace's draws use seed + 7_700_017."""
+    ctx = lay.ctx
+    n = len(ctx.w)
+    if b_ii is None:
+        b_ii = bii_rows(lay, n_draws, block, seed)
+    b = b_ii * factor[None, :] / complement[None, :]
+    if stayer_rows is not None:
+        b[:, stayer_rows] = 0.0
+    rank_two = (m_times(lay, b), 0.5 / n) if centering == "reference" else None
+    tables = [fit_smoother(sigma2, leverage, b_ii[k], smooth_bins) for k in range(3)]
+    s = np.stack([sigma2 if t is None else smooth_eval(t, leverage, b_ii[k])
+                  for k, t in enumerate(tables)])
+    w = ctx.w
+    ybar = float(w @ y / w.sum())
+    yt = lay.root * (y - (ybar if centering == "weighted" else 0.0))
+    cy = apply_c(lay, yt[:, None], b, rank_two=rank_two)[:, :, 0]
+    first = np.array([4.0 * s[k] @ cy[:, k] ** 2 for k in range(3)])
+    theta = np.array([yt @ cy[:, k] for k in range(3)])
+    total = np.zeros(3)
+    if trace:
+        drawn = 0
+        while drawn < n_draws:
+            size = min(block, n_draws - drawn)
+            z = rademacher(0, n, size, seed + SE_TRACE_OFFSET, drawn)
+            v = np.stack([z * np.sqrt(np.maximum(s[k], 0.0))[:, None] for k in range(3)], axis=1)
+            cv = apply_c(lay, v, b, per_component=True, rank_two=rank_two)
+            for k in range(3):
+                total[k] += s[k] @ (cv[:, k, :] ** 2).sum(axis=1)
+            drawn += size
+        total /= n_draws
+    variance = first - 2.0 * total
+    for k in range(3):
+        if k not in reported:
+            variance[k] = first[k] = total[k] = theta[k] = np.nan
+    return {"variance": variance, "first": first, "trace": 2.0 * total, "theta": theta,
+            "se": np.where(variance > 0, np.sqrt(np.abs(variance)), np.nan),
+            "se_conservative": np.where(first > 0, np.sqrt(np.abs(first)), np.nan),
+            "b_ii": b_ii, "b": b, "s": s}

@@ -14,6 +14,7 @@ import pytest
     ("y ~ 1 | worker_id + firm_id", {"centering": "mean"}, ValueError),
     ("y ~ 1 | worker_id + firm_id", {"leave_out": "observation", "stayers": "drop"},
      NotImplementedError),
+    ("y ~ 1 | worker_id + firm_id", {"se": True}, NotImplementedError),   # match level
 ])
 def test_bad_arguments_are_refused_before_any_work(fml, kw, err):
     from hdfe_stream_amd.leaveout import leave_out_kss

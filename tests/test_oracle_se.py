@@ -1,0 +1,184 @@
+"""The oracle's KSS standard errors (leaveout_se.py) pinned by dense algebra
+on a small weighted panel: the kernel C of each component built from
+explicit matrices -- X~ = W^(1/2) A, S^- = pinv(X~'X~), B = X~ S^- Q S^- X~',
+M = I - X~ S^- X~' -- and every streamed piece checked against it with the
+same draws, plus the estimator's identity y~'C y~ = plug-in - bias."""
+
+import numpy as np
+import pytest
+
+from conftest import rel
+
+FE = ["worker_id", "firm_id"]
+
+
+def _loo(O, d):
+    a, b = np.asarray(d["worker_id"]), np.asarray(d["firm_id"])
+    ua, ia = np.unique(a, return_inverse=True)
+    ub, ib = np.unique(b, return_inverse=True)
+    alive, _ = O.leave_one_out(ia, ib, len(ua), len(ub))
+    keep = alive[ia] & alive[len(ua) + ib]
+    return {k: np.asarray(v)[keep] for k, v in d.items()}
+
+
+@pytest.fixture(scope="module")
+def dense(oracle_mod):
+    from hdfe_stream_amd.simulate import kss_panel
+    O = oracle_mod
+    d = kss_panel(220, 30, seed=12)
+    d["w"] = np.random.default_rng(3).uniform(0.5, 2.0, len(d["y"]))
+    d = _loo(O, d)
+    f = O.fit(d, "y", (), FE, weights="w", tol=1e-13, fixef_rm="none", stream="worker_id")
+    ctx = O.KSSContext(f, tol=1e-13)
+    n = len(ctx.w)
+    lay = O.SELayout(ctx, 0, n)
+    A = np.hstack([ctx.Dg.toarray(), ctx.Do.toarray()])          # coefficients: groups, levels
+    Xt = lay.root[:, None] * A
+    Sm = np.linalg.pinv(Xt.T @ Xt)
+    P = Xt @ Sm @ Xt.T
+    M = np.eye(n) - P
+    G, L = ctx.G, ctx.L
+    psi = np.zeros((n, G + L))
+    psi[np.arange(n), G + lay.psi_level] = 1.0
+    alp = np.zeros((n, G + L))
+    alp[np.arange(n), ctx.g] = 1.0
+    T = lay.total
+
+    def centred(Z):
+        return Z - (ctx.w @ Z)[None, :] / T
+
+    Lp = lay.root[:, None] * centred(psi) / np.sqrt(lay.norm)
+    La = lay.root[:, None] * centred(alp) / np.sqrt(lay.norm)
+    Q = [Lp.T @ Lp, La.T @ La, 0.5 * (Lp.T @ La + La.T @ Lp)]
+    B = [Xt @ Sm @ q @ Sm @ Xt.T for q in Q]
+    bvec = np.stack([np.diag(Bk) / np.diag(M) for Bk in B])
+    Cm = [B[k] - 0.5 * (np.diag(bvec[k]) @ M + M @ np.diag(bvec[k])) for k in range(3)]
+    return dict(O=O, f=f, ctx=ctx, lay=lay, Xt=Xt, Sm=Sm, M=M, Lp=Lp, La=La, B=B, b=bvec,
+                C=Cm, d=d, n=n)
+
+
+def test_the_kernel_has_a_zero_diagonal(dense):
+    for Ck in dense["C"]:
+        assert np.abs(np.diag(Ck)).max() < 1e-10 * np.abs(Ck).max()
+
+
+@pytest.mark.parametrize("per_component", [False, True])
+def test_apply_c_is_the_dense_kernel(dense, per_component):
+    O, lay, n = dense["O"], dense["lay"], dense["n"]
+    rng = np.random.default_rng(5)
+    v = rng.standard_normal((n, 3, 4) if per_component else (n, 4))
+    cv = O.apply_c(lay, v, dense["b"], per_component=per_component)
+    for k in range(3):
+        want = dense["C"][k] @ (v[:, k, :] if per_component else v)
+        assert rel(cv[:, k, :], want) < 1e-8
+
+
+def test_rank_two_term(dense):
+    O, lay, n = dense["O"], dense["lay"], dense["n"]
+    u = np.random.default_rng(6).standard_normal((3, n))
+    v = np.random.default_rng(7).standard_normal((n, 2))
+    cv = O.apply_c(lay, v, dense["b"], rank_two=(u, 0.3))
+    one = np.ones(n)
+    for k in range(3):
+        K = dense["C"][k] + 0.3 * (np.outer(one, u[k]) + np.outer(u[k], one))
+        assert rel(cv[:, k, :], K @ v) < 1e-8
+
+
+def test_m_times(dense):
+    O, lay = dense["O"], dense["lay"]
+    b = np.random.default_rng(8).standard_normal((3, dense["n"]))
+    assert rel(O.m_times(lay, b), (dense["M"] @ b.T).T) < 1e-8
+
+
+def test_bii_rows_with_the_same_draws(dense):
+    O, lay, n = dense["O"], dense["lay"], dense["n"]
+    nd, seed = 24, 11
+    got = O.bii_rows(lay, nd, 10, seed)
+    q = O.rademacher(0, n, nd, seed, 0)
+    E = q @ q.T / nd
+    R1 = dense["Xt"] @ dense["Sm"] @ dense["Lp"].T
+    R2 = dense["Xt"] @ dense["Sm"] @ dense["La"].T
+    want = [((R1 @ E) * R1).sum(1), ((R2 @ E) * R2).sum(1), ((R1 @ E) * R2).sum(1)]
+    for k in range(3):
+        assert rel(got[k], want[k]) < 1e-8
+    # the expectation over draws is the diagonal of B (E[qq'] = I)
+    exact = [R1 @ R1.T, R2 @ R2.T, 0.5 * (R1 @ R2.T + R2 @ R1.T)]
+    for k in range(3):
+        np.testing.assert_allclose(np.diag(exact[k]), np.diag(dense["B"][k]), rtol=1e-8,
+                                   atol=1e-14)
+
+
+def test_y_c_y_is_plug_in_minus_bias(dense):
+    """With exact leverages and B_ii, theta = y~'C y~ equals the plug-in less
+    sum_i B_ii sigma2_i, sigma2_i the leave-out variance (leaveout_se.py:10-14)."""
+    O, lay, f, ctx, n = dense["O"], dense["lay"], dense["f"], dense["ctx"], dense["n"]
+    M = dense["M"]
+    y = f["p0"]["V"][:, 0]
+    comp = np.diag(M)
+    s2 = O.sigma2(y, f["resid"], ctx.w, comp, np.ones(n))
+    res = O.component_variances(lay, y, s2, np.diag(np.eye(n) - M), comp, np.ones(n), 8, 8, 0,
+                                trace=False, b_ii=np.stack([np.diag(B) for B in dense["B"]]))
+    c = f["cells"]
+    a = f["gam_y"][c["offs"][0] + f["p0"]["codes"][:, 0]]
+    bb = f["worker_effects"][f["p0"]["g"]]
+    plug = np.array(O.plug_in(a, bb, ctx.w, n))
+    bias = np.array([np.diag(B) @ s2 for B in dense["B"]])
+    assert rel(res["theta"], plug - bias) < 1e-8
+    # and the quadratic forms themselves: y~'B y~ is the plug-in
+    yt = lay.root * (y - ctx.w @ y / ctx.w.sum())
+    assert rel([yt @ B @ yt for B in dense["B"]], plug) < 1e-8
+
+
+def test_trace_term_with_the_same_draws(dense):
+    O, lay, n = dense["O"], dense["lay"], dense["n"]
+    rng = np.random.default_rng(9)
+    s2 = rng.uniform(0.1, 0.5, n)
+    f = dense["f"]
+    nd, seed = 6, 21
+    res = O.component_variances(lay, f["p0"]["V"][:, 0], s2, rng.uniform(0.1, 0.9, n),
+                                np.diag(dense["M"]), np.ones(n), nd, 4, seed,
+                                b_ii=np.stack([np.diag(B) for B in dense["B"]]), smooth_bins=0)
+    # smooth_bins=0: no smoother, s = sigma2 itself
+    z = O.rademacher(0, n, nd, seed + O.SE_TRACE_OFFSET, 0)
+    Om = np.sqrt(np.maximum(s2, 0.0))[:, None] * z
+    want = np.array([(s2[:, None] * (dense["C"][k] @ Om) ** 2).sum() / nd for k in range(3)])
+    assert rel(res["trace"], 2.0 * want) < 1e-8
+    # the first term
+    yt = lay.root * (f["p0"]["V"][:, 0] - dense["ctx"].w @ f["p0"]["V"][:, 0] / lay.total)
+    first = np.array([4.0 * s2 @ (dense["C"][k] @ yt) ** 2 for k in range(3)])
+    assert rel(res["first"], first) < 1e-8
+    np.testing.assert_allclose(res["variance"], res["first"] - res["trace"], rtol=1e-12)
+
+
+def test_smoother(oracle_mod):
+    O = oracle_mod
+    rng = np.random.default_rng(1)
+    n = 5000
+    lev, bii = rng.uniform(0, 1, n), rng.exponential(1.0, n)
+    assert O.auto_bins(n) == round(n ** (1 / 6))
+    # a constant is reproduced exactly, everywhere
+    t = O.fit_smoother(np.full(n, 0.7), lev, bii)
+    np.testing.assert_allclose(O.smooth_eval(t, lev, bii), 0.7, rtol=1e-14)
+    # cell means: each row's cell is found by its ranks, and the means are
+    # those of the rows in the cell
+    s2 = rng.standard_normal(n)
+    kp, kb, means, nb = O.fit_smoother(s2, lev, bii, n_bins=4)
+    take = np.arange(n)
+    np.testing.assert_array_equal(kp[:-1] <= kp[1:], True)
+    assert kp[0] == lev.min() and kb[0] == bii.min()
+    cp = np.minimum((O.smoother_rank(kp, lev) * 4).astype(int), 3)
+    cb = np.minimum((O.smoother_rank(kb, bii) * 4).astype(int), 3)
+    for i in range(4):
+        for j in range(4):
+            sel = (cp == i) & (cb == j)
+            assert means[i, j] == pytest.approx(s2[sel].mean(), rel=1e-12)
+    # quartile cells of independent uniform ranks hold about n / 16 rows each
+    counts = np.bincount(cp * 4 + cb, minlength=16)
+    assert counts.min() > n / 16 * 0.7
+    del take
+    # the evaluation interpolates between cell centres: at a centre (half way
+    # between two knots, as the rank is linear there) it is the cell's mean
+    mid_p = 0.5 * (kp[1] + kp[2])
+    mid_b = 0.5 * (kb[2] + kb[3])
+    assert O.smooth_eval((kp, kb, means, 4), np.array([mid_p]), np.array([mid_b]))[0] == \
+        pytest.approx(means[1, 2], rel=1e-9)

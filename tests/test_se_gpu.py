@@ -1,0 +1,114 @@
+"""Device standard errors of the KSS components (leaveout_se.py) against the
+oracle's restatement, draw for draw: with the oracle's own leverages and
+sigma2 (every piece to solver tolerance), and end to end through
+leave_out_kss(se=True)."""
+
+import numpy as np
+import pytest
+
+from conftest import rel
+
+pytestmark = pytest.mark.gpu
+
+FE = ["worker_id", "firm_id"]
+
+
+def _loo(O, d):
+    a, b = np.asarray(d["worker_id"]), np.asarray(d["firm_id"])
+    ua, ia = np.unique(a, return_inverse=True)
+    ub, ib = np.unique(b, return_inverse=True)
+    alive, _ = O.leave_one_out(ia, ib, len(ua), len(ub))
+    keep = alive[ia] & alive[len(ua) + ib]
+    return {k: np.asarray(v)[keep] for k, v in d.items()}
+
+
+@pytest.mark.parametrize("weighted", [False, True])
+def test_component_variances_match_the_oracle(oracle_mod, weighted):
+    import torch
+    from hdfe_stream_amd.estimator import feols_stream
+    from hdfe_stream_amd.leaveout import CoefficientInverse, JLALeverages
+    from hdfe_stream_amd.leaveout_se import component_variances
+    from hdfe_stream_amd.simulate import kss_panel
+    O = oracle_mod
+    d = kss_panel(1200, 150, seed=5)
+    if weighted:
+        d["w"] = np.random.default_rng(2).uniform(0.5, 2.0, len(d["y"]))
+    d = _loo(O, d)
+    wkw = {"weights": "w"} if weighted else {}
+    print(f"component_variances weighted={weighted}", flush=True)
+    res = feols_stream("y ~ 1 | worker_id + firm_id", d, stream="worker_id", fixef_rm="none",
+                       keep_intermediates=True, tol=1e-12, **wkw)
+    f = O.fit(d, "y", (), FE, weights="w" if weighted else None, tol=1e-12, fixef_rm="none",
+              stream="worker_id")
+    ctx = O.KSSContext(f, tol=1e-12)
+    n = len(ctx.w)
+    assert res.n_obs == n
+    rng = np.random.default_rng(4)
+    M_bar = rng.uniform(0.3, 0.95, n)
+    factor = rng.uniform(0.9, 1.1, n)
+    s2 = O.sigma2(f["p0"]["V"][:, 0], f["resid"], ctx.w, M_bar, factor)
+    seed, nd = 9, 20
+    lay = O.SELayout(ctx, 0, n)
+    ref = O.component_variances(lay, f["p0"]["V"][:, 0], s2, 1.0 - M_bar, M_bar, factor, nd, 7,
+                                seed)
+    inv = CoefficientInverse(res.panel, tol=1e-12, precision="fp64")
+    cu = lambda a: torch.from_numpy(np.ascontiguousarray(a, np.float64)).cuda()
+    lev = JLALeverages(leverage=cu(1.0 - M_bar), complement=cu(M_bar), factor=cu(factor),
+                       n_draws=nd)
+    y = res.panel.tensor("V", torch.float64)
+    out = component_variances(inv, y, cu(s2), lev, n_draws=nd, block=6, seed=seed)
+    bii = out.diagnostics["b_ii"].cpu().numpy()
+    for k in range(3):
+        assert rel(bii[k], ref["b_ii"][k]) < 1e-8
+    np.testing.assert_allclose(out.diagnostics["smoothed_sigma2"].cpu().numpy(), ref["s"],
+                               rtol=1e-7, atol=1e-12 * np.abs(ref["s"]).max())
+    names = ["var(psi)", "var(alpha)", "cov(psi, alpha)"]
+    for key, part in (("first", out.first_term), ("theta", out.theta),
+                      ("trace", out.trace_term), ("variance", out.variance)):
+        got = np.array([part[nm] for nm in names])
+        assert rel(got, ref[key]) < 1e-7, key
+    for k, nm in enumerate(names):
+        if ref["variance"][k] > 0:
+            assert out.se[nm] == pytest.approx(np.sqrt(ref["variance"][k]), rel=1e-7)
+
+
+def test_leave_out_kss_standard_errors(oracle_mod):
+    from hdfe_stream_amd.leaveout import leave_out_kss
+    from hdfe_stream_amd.simulate import kss_panel
+    O = oracle_mod
+    d = kss_panel(4000, 400, seed=3)
+    print("leave_out_kss se=True", flush=True)
+    lo = leave_out_kss("y ~ 1 | worker_id + firm_id", d, n_draws=64, seed=2, precision="fp64",
+                       tol=1e-12, leave_out="observation", se=True, se_draws=32, block=16)
+    ref = O.leave_out_kss(d, n_draws=64, block=64, seed=2, tol=1e-12)
+    f = ref["fit"]
+    ctx = O.KSSContext(f, tol=1e-12)
+    n = ref["n"]
+    lay = O.SELayout(ctx, 0, n)
+    want = O.component_variances(lay, f["p0"]["V"][:, 0], ref["sigma2"], 1.0 - ref["M_bar"],
+                                 ref["M_bar"], ref["factor"], 32, 16, 2 + O.SE_BII_OFFSET)
+    names = ["var(psi)", "var(alpha)", "cov(psi, alpha)"]
+    got = lo.se
+    assert got is not None and got.n_draws == 32
+    # the leverages and sigma2 agree to ~1e-9; the smoother's cells follow them
+    for key, part in (("first", got.first_term), ("theta", got.theta),
+                      ("variance", got.variance)):
+        assert rel([part[nm] for nm in names], want[key]) < 1e-5, key
+    # theta = y~'C y~ checks the point estimate: close to plug-in - bias
+    for nm in names:
+        assert got.theta[nm] == pytest.approx(lo.leave_out[nm], rel=0.1,
+                                              abs=0.1 * abs(lo.plug_in[nm]))
+        assert got.se[nm] > 0 and got.se[nm] <= got.se_conservative[nm]
+    assert "standard errors" in lo.summary()
+
+
+def test_without_the_trace_the_errors_are_conservative():
+    from hdfe_stream_amd.leaveout import leave_out_kss
+    from hdfe_stream_amd.simulate import kss_panel
+    d = kss_panel(2000, 200, seed=6)
+    lo = leave_out_kss("y ~ 1 | worker_id + firm_id", d, n_draws=16, seed=1,
+                       leave_out="observation", se=True, se_trace=False)
+    for nm, v in lo.se.se.items():
+        assert lo.se.trace_term[nm] == 0.0
+        assert v == pytest.approx(lo.se.se_conservative[nm], rel=1e-12)
+    assert lo.se.n_draws == 0
