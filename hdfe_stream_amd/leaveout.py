@@ -26,9 +26,8 @@ within-match stayer rule and the reference centring.
 Not mirrored yet: covariates in the leave-out design (the Schur border,
 inverse.py:182-231) -- at observation level a model with covariates is
 refused (at match level the row model may have them: they are partialled out
-before the collapse); standard errors leaving out a match, and the
-weak-identification diagnostic.  Standard errors leaving out an observation
-are in `leaveout_se`.
+before the collapse), and the weak-identification diagnostic.  The
+components' standard errors are in `leaveout_se`.
 """
 
 from __future__ import annotations
@@ -414,7 +413,9 @@ def leave_out_components(result, n_draws: int = 250, block: int | None = None, s
                          precision: str = "mixed", timings: dict | None = None,
                          stayers: str = "own", stayer_sigma2=None, centering: str = "weighted",
                          person_years: int | None = None, se: bool = False, se_draws=None,
-                         se_trace: bool = True, smooth_bins=None) -> LeaveOutComponents:
+                         se_trace: bool = True, smooth_bins=None,
+                         se_variance: str = "person_year",
+                         match_within=None) -> LeaveOutComponents:
     """KSS components for a fit (leaveout.py:634-805): JLA leverages, the
     leverage check, the plug-in moments, sigma2, the Hutchinson bias with
     seed + 1_000_003, and plug-in minus bias.
@@ -427,7 +428,12 @@ def leave_out_components(result, n_draws: int = 250, block: int | None = None, s
     only movers' rows are held to the leverage check (leaveout.py:712-724).
     With `se`, the standard errors of `leaveout_se.component_variances`
     (seed + 5_500_011, `se_draws` draws, default n_draws; leaveout.py:
-    759-775), leaving out an observation."""
+    759-775).  Leaving out a match (stayers "within_match") the variance of
+    each match row is `_match_se_sigma2`'s (leaveout.py:848-894): sigma2 with
+    weighted centring on movers' matches; "person_year" adds it over T to
+    `match_within` (device, each match's within-match variance, which a
+    stayer's match gets alone), "match" takes it with the stayers' values; b
+    is zero on stayers' matches and var(alpha) gets no standard error."""
     if stayers not in ("own", "within_match"):
         raise ValueError("stayers must be 'own' or 'within_match' on the device")
     if (stayers == "within_match") != (stayer_sigma2 is not None):
@@ -508,17 +514,36 @@ def leave_out_components(result, n_draws: int = 250, block: int | None = None, s
         sigma2_mean = float(sigma2.mean())
     errors = None
     if se:
-        if match_level:
-            raise NotImplementedError("standard errors leaving out a match are not built on the "
-                                      "device; use leave_out='observation'")
         from .leaveout_se import BII_SEED_OFFSET, component_variances
+        if se_variance not in ("person_year", "match"):
+            raise ValueError("se_variance must be 'person_year' or 'match'")
         inv = CoefficientInverse(panel, tol, maxiter, precision)
         y = panel.tensor("V", torch.float64).view(n, info["m"])[:, 0].contiguous()
+        sigma2_se, options = sigma2, {}
+        if match_level:
+            blk = torch.empty(n, dtype=torch.float64, device="cuda")
+            yb = C.c_double()
+            torch.cuda.synchronize()
+            _lib.check(lib.hdfe_kss_sigma2(n, info["m"], V, w, 0, resid.data_ptr(),
+                                           lev.complement.data_ptr(), lev.factor.data_ptr(),
+                                           blk.data_ptr(), C.byref(yb)), "hdfe_kss_sigma2")
+            if se_variance == "match":
+                sigma2_se = torch.where(movers, blk, stayer_sigma2.to(torch.float64))
+            else:
+                if match_within is None:
+                    raise ValueError("se_variance='person_year' needs the within-match "
+                                     "variances")
+                wt = panel.tensor("w", torch.float64)
+                mw = match_within.to(torch.float64)
+                sigma2_se = torch.where(movers, mw + blk / wt, mw)
+            options = {"stayer_rows": ~movers, "centering": centering, "reported": (0, 2)}
         errors = component_variances(
-            inv, y, sigma2, lev, psi=pi, rows=R,
+            inv, y, sigma2_se.contiguous(), lev, psi=pi, rows=R,
             n_draws=n_draws if se_draws is None else int(se_draws), block=block,
-            seed=seed + BII_SEED_OFFSET, trace=se_trace, smooth_bins=smooth_bins)
-        errors.diagnostics.update(psi=psi, alpha=alpha, covariates=[])
+            seed=seed + BII_SEED_OFFSET, trace=se_trace, smooth_bins=smooth_bins, **options)
+        errors.diagnostics.update(psi=psi, alpha=alpha, covariates=[],
+                                  se_variance=se_variance if match_level else None,
+                                  sigma2=sigma2_se)
     t4 = time.perf_counter()
     t.update(leverage_seconds=t1 - t0, sigma2_seconds=t2 - t1, bias_seconds=t3 - t2,
              se_seconds=t4 - t3)
@@ -533,7 +558,9 @@ def leave_out_components(result, n_draws: int = 250, block: int | None = None, s
                      "sigma2": sigma2})
 
 
-def _leave_out_match(fit, psi, n_draws, block, seed, tol, maxiter, precision, timings):
+def _leave_out_match(fit, psi, n_draws, block, seed, tol, maxiter, precision, timings,
+                     se=False, se_draws=None, se_trace=True, smooth_bins=None,
+                     se_variance="person_year"):
     """KSS leaving out a whole (worker, firm) match (leaveout_match.py:
     156-214): the row fit's y~ = the worker effect + the firm effect + the
     residual collapsed to one row per match (its weighted mean, weight the
@@ -597,10 +624,21 @@ def _leave_out_match(fit, psi, n_draws, block, seed, tol, maxiter, precision, ti
     # the table's rows in the match fit's sorted order
     perm = mp.tensor("perm", torch.int32).long()[:mn]
     st_sorted = st_tab[perm].contiguous()
+    within = None
+    if se and se_variance == "person_year":
+        # each match's within-match variance of y~, (1/T) sum (y~ - its mean)^2
+        # over the spell's rows, unweighted (leaveout_match.py:135-153)
+        cnt = torch.zeros(nm, dtype=torch.float64, device="cuda").index_add_(
+            0, mid, torch.ones_like(ytil))
+        mu = torch.zeros(nm, dtype=torch.float64, device="cuda").index_add_(0, mid, ytil) / cnt
+        within = (torch.zeros(nm, dtype=torch.float64, device="cuda").index_add_(
+            0, mid, (ytil - mu[mid]) ** 2) / cnt)[perm].contiguous()
     comps = leave_out_components(mfit, n_draws=n_draws, block=block, seed=seed, psi=psi,
                                  tol=tol, maxiter=maxiter, precision=precision, timings=timings,
                                  stayers="within_match", stayer_sigma2=st_sorted,
-                                 centering="reference", person_years=n)
+                                 centering="reference", person_years=n, se=se,
+                                 se_draws=se_draws, se_trace=se_trace, smooth_bins=smooth_bins,
+                                 se_variance=se_variance, match_within=within)
     comps.n_obs = n
     comps.diagnostics["n_matches"] = mn
     comps.diagnostics["person_years"] = n
@@ -614,7 +652,8 @@ def leave_out_kss(fml: str, data, *, psi: str | None = None, n_draws: int = 250,
                   centering: str = "reference",
                   precision: str = "mixed", tol: float = 1e-10, maxiter: int = 5000,
                   timings: dict | None = None, se: bool = False, se_draws=None,
-                  se_trace: bool = True, smooth_bins=None, **options) -> LeaveOutComponents:
+                  se_trace: bool = True, smooth_bins=None, se_variance: str = "person_year",
+                  **options) -> LeaveOutComponents:
     """Kline-Saggio-Solvsten leave-out variance components, end to end, on
     the device (leaveout.py:1069-1255): prune to the largest leave-one-out
     connected set of (alpha, psi), fit with alpha streamed and no singleton
@@ -626,8 +665,10 @@ def leave_out_kss(fml: str, data, *, psi: str | None = None, n_draws: int = 250,
     stayer rule is "own" and the centring weighted; at match level the
     within-match rule and the reference centring (the reference's defaults
     there).  `se`, `se_draws`, `se_trace` and `smooth_bins` ask for the
-    components' standard errors (leaveout_se.py), leaving out an
-    observation."""
+    components' standard errors (leaveout_se.py); leaving out a match,
+    `se_variance` is how each match's error variance is estimated
+    ("person_year", the reference implementation's, unweighted only, or
+    "match") and var(alpha) gets none (leaveout.py:1136-1170)."""
     import torch
 
     from .connected import leave_one_out
@@ -645,9 +686,12 @@ def leave_out_kss(fml: str, data, *, psi: str | None = None, n_draws: int = 250,
                                   "device")
     if stayers not in (None, "own"):
         raise NotImplementedError("only stayers='own' is implemented on the device")
-    if se and leave_out == "match":
-        raise NotImplementedError("standard errors leaving out a match are not built on the "
-                                  "device; use leave_out='observation'")
+    if se_variance not in ("match", "person_year"):
+        raise ValueError("se_variance must be 'match' or 'person_year'")
+    if (leave_out == "match" and se and se_variance == "person_year"
+            and options.get("weights") is not None):
+        raise ValueError("se_variance='person_year' is LeaveOutTwoWay's, which has no user "
+                         "weights; use se_variance='match' with a weighted fit")
     if "|" not in fml:
         raise ValueError(
             f"{fml!r} has no fixed effects; leave-out estimation needs at least two "
@@ -686,7 +730,9 @@ def leave_out_kss(fml: str, data, *, psi: str | None = None, n_draws: int = 250,
     fit = StreamingHDFE(y, xs, fe, keep_intermediates=True, **options).fit(data)
     t2 = time.perf_counter()
     if leave_out == "match":
-        comps = _leave_out_match(fit, psi, n_draws, block, seed, tol, maxiter, precision, t)
+        comps = _leave_out_match(fit, psi, n_draws, block, seed, tol, maxiter, precision, t,
+                                 se=se, se_draws=se_draws, se_trace=se_trace,
+                                 smooth_bins=smooth_bins, se_variance=se_variance)
     else:
         comps = leave_out_components(fit, n_draws=n_draws, block=block, seed=seed, psi=psi,
                                      tol=tol, maxiter=maxiter, precision=precision, timings=t,

@@ -676,7 +676,8 @@ def leave_out_kss(data: dict, y: str = "y", xs=(), alpha: str = "worker_id",
 
 def leave_out_match(data: dict, y: str = "y", xs=(), alpha: str = "worker_id",
                     psi: str = "firm_id", others=(), n_draws: int = 250, block: int = 256,
-                    seed: int = 0, tol: float = 1e-10) -> dict:
+                    seed: int = 0, tol: float = 1e-10, se: bool = False, se_draws=None,
+                    se_block: int = 16, se_variance: str = "person_year") -> dict:
     """KSS leaving out a whole match (leaveout_match.py:156-214, 63-132;
     leaveout.py:634-805 with stayers "within_match" and the reference
     centring): prune as leave_out_kss, fit the row model, collapse y~ = the
@@ -686,7 +687,12 @@ def leave_out_match(data: dict, y: str = "y", xs=(), alpha: str = "worker_id",
     the square-root-weight metric, sigma2 = (sqrt(T) ybar - its mean over
     matches) sqrt(T) e / M_ii * factor for movers' matches and
     sigma_for_stayers for stayers', the Hutchinson bias, the plug-in moments
-    with the person-year divisor.  Unweighted rows."""
+    with the person-year divisor.  Unweighted rows.  With `se`, the standard
+    errors of var(psi) and the covariance (leaveout.py:759-775, 848-894):
+    sigma2 recomputed with weighted centring on movers' matches, per match
+    the within-match variance plus that over T ("person_year") or that alone
+    with the stayers' values ("match"), b zero on stayers' matches, and the
+    reference-centred kernel's rank-two term."""
     a_ids, b_ids = np.asarray(data[alpha]), np.asarray(data[psi])
     ua, ia = np.unique(a_ids, return_inverse=True)
     ub, ib = np.unique(b_ids, return_inverse=True)
@@ -742,11 +748,28 @@ def leave_out_match(data: dict, y: str = "y", xs=(), alpha: str = "worker_id",
     ctx = KSSContext(mf, tol)
     bias, totals = hutchinson_trace(ctx, s2, n_draws, block, seed + 1_000_003, 0, n_rows=R)
     del nf
-    return {"plug_in": plug, "bias": bias,
-            "leave_out": tuple(p - q for p, q in zip(plug, bias)), "keep": keep,
-            "prune": prune, "fit": f, "match_fit": mf, "sigma2": s2, "M_bar": M_bar,
-            "factor": factor, "moments": moments, "person_years": R, "n_matches": nm,
-            "movers": movers, "max_leverage": worst}
+    out = {"plug_in": plug, "bias": bias,
+           "leave_out": tuple(p - q for p, q in zip(plug, bias)), "keep": keep,
+           "prune": prune, "fit": f, "match_fit": mf, "sigma2": s2, "M_bar": M_bar,
+           "factor": factor, "moments": moments, "person_years": R, "n_matches": nm,
+           "movers": movers, "max_leverage": worst}
+    if se:
+        yv = mp0["V"][:, 0]
+        ybw = float(Tm @ yv / Tm.sum())
+        blk = Tm * (yv - ybw) * mf["resid"] / M_bar * factor
+        if se_variance == "match":
+            sig = np.where(movers, blk, s2)
+        else:
+            within = (np.bincount(mid, weights=(ytil - ybar[mid]) ** 2, minlength=nm)
+                      / np.bincount(mid, minlength=nm))
+            sig = np.where(movers, within + blk / Tm, within)
+        lay = SELayout(ctx, 0, R)
+        out["se"] = component_variances(lay, yv, sig, P_bar, M_bar, factor,
+                                        n_draws if se_draws is None else se_draws, se_block,
+                                        seed + SE_BII_OFFSET, stayer_rows=~movers,
+                                        centering="reference", reported=(0, 2))
+        out["se_sigma2"] = sig
+    return out
 
 
 # ------------------------------------------------- differentiable fits

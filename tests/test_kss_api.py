@@ -14,7 +14,8 @@ import pytest
     ("y ~ 1 | worker_id + firm_id", {"centering": "mean"}, ValueError),
     ("y ~ 1 | worker_id + firm_id", {"leave_out": "observation", "stayers": "drop"},
      NotImplementedError),
-    ("y ~ 1 | worker_id + firm_id", {"se": True}, NotImplementedError),   # match level
+    ("y ~ 1 | worker_id + firm_id", {"se": True, "se_variance": "row"}, ValueError),
+    ("y ~ 1 | worker_id + firm_id", {"se": True, "weights": "w"}, ValueError),  # person_year
 ])
 def test_bad_arguments_are_refused_before_any_work(fml, kw, err):
     from hdfe_stream_amd.leaveout import leave_out_kss

@@ -112,3 +112,27 @@ def test_without_the_trace_the_errors_are_conservative():
         assert lo.se.trace_term[nm] == 0.0
         assert v == pytest.approx(lo.se.se_conservative[nm], rel=1e-12)
     assert lo.se.n_draws == 0
+
+
+@pytest.mark.parametrize("se_variance", ["person_year", "match"])
+def test_leave_out_match_standard_errors(oracle_mod, se_variance):
+    from hdfe_stream_amd.leaveout import leave_out_kss
+    from hdfe_stream_amd.simulate import kss_panel
+    O = oracle_mod
+    d = kss_panel(4000, 400, seed=3)
+    print(f"leave_out_kss match se_variance={se_variance}", flush=True)
+    lo = leave_out_kss("y ~ 1 | worker_id + firm_id", d, n_draws=64, seed=2, precision="fp64",
+                       tol=1e-12, se=True, se_draws=32, block=16, se_variance=se_variance)
+    ref = O.leave_out_match(d, n_draws=64, block=64, seed=2, tol=1e-12, se=True, se_draws=32,
+                            se_block=16, se_variance=se_variance)
+    assert lo.diagnostics["leave_out"] == "match"
+    s2 = lo.se.diagnostics["sigma2"].cpu().numpy()
+    assert rel(s2, ref["se_sigma2"]) < 1e-6
+    want = ref["se"]
+    for key, part in (("first", lo.se.first_term), ("theta", lo.se.theta),
+                      ("variance", lo.se.variance)):
+        got = np.array([part["var(psi)"], part["cov(psi, alpha)"]])
+        assert rel(got, want[key][[0, 2]]) < 1e-5, key
+        assert np.isnan(part["var(alpha)"]) and np.isnan(want[key][1])
+    assert np.isnan(lo.se.se["var(alpha)"])
+    assert lo.se.se["var(psi)"] > 0
