@@ -115,7 +115,8 @@ def jla_phase(n_workers, n_firms, draws):
 
 def kss_phase(n_workers, n_firms, draws):
     """KSS end to end on the KSS panel, leaving out a match (the reference's
-    default) and an observation: leave-one-out pruning of (worker, firm), the
+    default) and an observation, then at match level with standard errors:
+    leave-one-out pruning of (worker, firm), the
     fit of y ~ 1 | worker_id + firm_id + year with workers streamed (at match
     level then the collapse and the weighted match fit), `draws` JLA draws
     and `draws` Hutchinson draws (two solves each), all blocks of `draws`
@@ -144,6 +145,19 @@ def kss_phase(n_workers, n_firms, draws):
             "trace_rhs_solves_per_s": round(2 * draws / tm["trace_seconds"], 1),
             "plug_in": lo.plug_in, "bias": lo.bias, "leave_out": lo.leave_out,
             "max_leverage": lo.diagnostics["max_leverage"]}
+    # with standard errors (the reference's default match level): B_ii by
+    # `draws` projections and the trace term by `draws` draws, blocks of 64
+    tm = {}
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    lo = leave_out_kss("y ~ 1 | worker_id + firm_id + year", cols, n_draws=draws, seed=0,
+                       block=64, precision="mixed", timings=tm, se=True)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    out["match_se"] = {
+        "wall_s": round(wall, 4), "se_seconds": round(tm["se_seconds"], 4),
+        "se_draws": draws, "se": lo.se.se, "se_conservative": lo.se.se_conservative,
+        "trace_term": lo.se.trace_term, "theta": lo.se.theta, "leave_out": lo.leave_out}
     return out
 
 
