@@ -162,7 +162,8 @@ __device__ __forceinline__ double se_rank(const double *__restrict__ kn, int nk,
     return slope * (x - kn[lo]) + f0;
 }
 
-// The smoother's cell of each row (leaveout_se.py:1024-1029).
+// The smoother's cell of each row (leaveout_se.py:1024-1029); without bii,
+// the leverage's alone.
 __global__ void k_se_cells(int64_t n, const double *__restrict__ lev,
                            const double *__restrict__ bii, const double *__restrict__ kp,
                            const double *__restrict__ kb, int nb, uint32_t *__restrict__ cell,
@@ -170,8 +171,12 @@ __global__ void k_se_cells(int64_t n, const double *__restrict__ lev,
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x) {
         const int64_t cp = min((int64_t)(se_rank(kp, nb + 1, lev[i]) * nb), (int64_t)nb - 1);
-        const int64_t cb = min((int64_t)(se_rank(kb, nb + 1, bii[i]) * nb), (int64_t)nb - 1);
-        cell[i] = (uint32_t)(cp * nb + cb);
+        if (bii) {
+            const int64_t cb = min((int64_t)(se_rank(kb, nb + 1, bii[i]) * nb), (int64_t)nb - 1);
+            cell[i] = (uint32_t)(cp * nb + cb);
+        } else {
+            cell[i] = (uint32_t)cp;
+        }
         idx[i] = (int)i;
     }
 }
@@ -207,7 +212,8 @@ __global__ void k_se_cell_sums(int64_t n, int ncell, const uint32_t *__restrict_
     }
 }
 
-// The smoothed sigma2, bilinear between cell centres (leaveout_se.py:1038-1054).
+// The smoothed sigma2, bilinear between cell centres (leaveout_se.py:1038-1054);
+// without bii, linear along the leverage.
 __global__ void k_se_smooth(int64_t n, const double *__restrict__ lev,
                             const double *__restrict__ bii, const double *__restrict__ kp,
                             const double *__restrict__ kb, const double *__restrict__ means,
@@ -215,10 +221,15 @@ __global__ void k_se_smooth(int64_t n, const double *__restrict__ lev,
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x) {
         const double fp = se_rank(kp, nb + 1, lev[i]) * nb - 0.5;
-        const double fb = se_rank(kb, nb + 1, bii[i]) * nb - 0.5;
         const int64_t i0 = min(max((int64_t)floor(fp), (int64_t)0), (int64_t)nb - 2);
+        const double a = fmin(fmax(fp - i0, 0.0), 1.0);
+        if (!bii) {
+            out[i] = (1.0 - a) * means[i0] + a * means[i0 + 1];
+            continue;
+        }
+        const double fb = se_rank(kb, nb + 1, bii[i]) * nb - 0.5;
         const int64_t j0 = min(max((int64_t)floor(fb), (int64_t)0), (int64_t)nb - 2);
-        const double a = fmin(fmax(fp - i0, 0.0), 1.0), c = fmin(fmax(fb - j0, 0.0), 1.0);
+        const double c = fmin(fmax(fb - j0, 0.0), 1.0);
         out[i] = (1.0 - a) * (1.0 - c) * means[i0 * nb + j0] + a * (1.0 - c) * means[(i0 + 1) * nb + j0] +
                  (1.0 - a) * c * means[i0 * nb + j0 + 1] + a * c * means[(i0 + 1) * nb + j0 + 1];
     }
@@ -288,13 +299,14 @@ int hdfe_se_cross_groups(int64_t G, int s, const int *gstart, const int *psi_cod
 
 // The smoother's cell sums: per cell of the nb x nb grid of (P_ii, B_ii)
 // ranks through the knots kp, kb (device, nb + 1 each), the sum of sigma2
-// and the row count; tot, cnt host (nb * nb).
+// and the row count; tot, cnt host (nb * nb).  With bii and kb null, the nb
+// cells of the leverage's ranks alone.
 int hdfe_se_cell_sums(int64_t n, const double *lev, const double *bii, const double *sigma2,
                       const double *kp, const double *kb, int nb, double *tot, double *cnt) {
-    ARG_CHECK(n >= 1 && n < INT32_MAX && nb >= 2 && nb <= 500 && lev && bii && sigma2 && kp &&
-                  kb && tot && cnt, "hdfe_se_cell_sums: bad argument");
+    ARG_CHECK(n >= 1 && n < INT32_MAX && nb >= 2 && nb <= 500 && lev && sigma2 && kp &&
+                  (!bii == !kb) && tot && cnt, "hdfe_se_cell_sums: bad argument");
     hipStream_t s = nullptr;
-    const int ncell = nb * nb;
+    const int ncell = bii ? nb * nb : nb;
     Temp tmp;
     DevBuf<uint32_t> c1, c2;
     DevBuf<int> i1, i2;
@@ -319,10 +331,11 @@ int hdfe_se_cell_sums(int64_t n, const double *lev, const double *bii, const dou
     return HDFE_STATUS_OK;
 }
 
-// The smoothed sigma2 of every row (see k_se_smooth); means (device, nb x nb).
+// The smoothed sigma2 of every row (see k_se_smooth); means (device, nb x nb,
+// or nb with bii and kb null).
 int hdfe_se_smooth(int64_t n, const double *lev, const double *bii, const double *kp,
                    const double *kb, const double *means, int nb, double *out) {
-    ARG_CHECK(n >= 1 && nb >= 2 && lev && bii && kp && kb && means && out,
+    ARG_CHECK(n >= 1 && nb >= 2 && lev && kp && (!bii == !kb) && means && out,
               "hdfe_se_smooth: bad argument");
     hipLaunchKernelGGL(k_se_smooth, dim3(grid1(n)), dim3(256), 0, 0, n, lev, bii, kp, kb, means,
                        nb, out);

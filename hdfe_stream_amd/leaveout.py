@@ -26,8 +26,8 @@ within-match stayer rule and the reference centring.
 Not mirrored yet: covariates in the leave-out design (the Schur border,
 inverse.py:182-231) -- at observation level a model with covariates is
 refused (at match level the row model may have them: they are partialled out
-before the collapse), and the weak-identification diagnostic.  The
-components' standard errors are in `leaveout_se`.
+before the collapse).  The components' standard errors are in
+`leaveout_se`, the weak-identification diagnostic in `leaveout_weakid`.
 """
 
 from __future__ import annotations
@@ -373,6 +373,7 @@ class LeaveOutComponents:
     leverages: JLALeverages
     trace: TraceEstimate
     se: object = None
+    weak_id: object = None
     diagnostics: dict = field(default_factory=dict)
     pruning: dict | None = None
     fit: object = None
@@ -390,6 +391,8 @@ class LeaveOutComponents:
                          f"{self.leave_out[k]:12.6f}")
         if self.se is not None:
             lines += ["", self.se.summary()]
+        if self.weak_id is not None:
+            lines += ["", self.weak_id.summary()]
         return "\n".join(lines)
 
 
@@ -414,8 +417,8 @@ def leave_out_components(result, n_draws: int = 250, block: int | None = None, s
                          stayers: str = "own", stayer_sigma2=None, centering: str = "weighted",
                          person_years: int | None = None, se: bool = False, se_draws=None,
                          se_trace: bool = True, smooth_bins=None,
-                         se_variance: str = "person_year",
-                         match_within=None) -> LeaveOutComponents:
+                         se_variance: str = "person_year", match_within=None,
+                         diagnose=None, diagnose_draws: int = 64) -> LeaveOutComponents:
     """KSS components for a fit (leaveout.py:634-805): JLA leverages, the
     leverage check, the plug-in moments, sigma2, the Hutchinson bias with
     seed + 1_000_003, and plug-in minus bias.
@@ -433,7 +436,9 @@ def leave_out_components(result, n_draws: int = 250, block: int | None = None, s
     weighted centring on movers' matches; "person_year" adds it over T to
     `match_within` (device, each match's within-match variance, which a
     stayer's match gets alone), "match" takes it with the stayers' values; b
-    is zero on stayers' matches and var(alpha) gets no standard error."""
+    is zero on stayers' matches and var(alpha) gets no standard error.
+    `diagnose` (default: `se`) runs the weak-identification diagnostic with
+    `diagnose_draws` trace draws and seed + 9_900_019 (leaveout.py:749-758)."""
     if stayers not in ("own", "within_match"):
         raise ValueError("stayers must be 'own' or 'within_match' on the device")
     if (stayers == "within_match") != (stayer_sigma2 is not None):
@@ -512,6 +517,15 @@ def leave_out_components(result, n_draws: int = 250, block: int | None = None, s
         sigma2_mean = float((sigma2 / wt).mean())
     else:
         sigma2_mean = float(sigma2.mean())
+    weak_id = None
+    if bool(se) if diagnose is None else diagnose:
+        from .leaveout_weakid import SEED_OFFSET, weak_id_diagnostics
+        y_d = panel.tensor("V", torch.float64).view(n, info["m"])[:, 0].contiguous()
+        weak_id = weak_id_diagnostics(CoefficientInverse(panel, tol, maxiter, precision), y_d,
+                                      sigma2, lev, psi=pi, rows=R, n_draws=int(diagnose_draws),
+                                      block=8 if block is None else int(block),
+                                      seed=seed + SEED_OFFSET)
+        weak_id.diagnostics.update(psi=psi, alpha=alpha, covariates=[])
     errors = None
     if se:
         from .leaveout_se import BII_SEED_OFFSET, component_variances
@@ -549,7 +563,7 @@ def leave_out_components(result, n_draws: int = 250, block: int | None = None, s
              se_seconds=t4 - t3)
     return LeaveOutComponents(
         plug_in=plug_in, bias=bias, leave_out=leave_out, n_obs=n,
-        n_movers=int(movers.sum()), leverages=lev, trace=trace, se=errors,
+        n_movers=int(movers.sum()), leverages=lev, trace=trace, se=errors, weak_id=weak_id,
         diagnostics={"psi": psi, "alpha": alpha, "stayers": stayers, "max_leverage": worst,
                      "leave_out": "match" if match_level else "observation",
                      "centering": centering, "sigma2_mean": sigma2_mean,
@@ -560,7 +574,7 @@ def leave_out_components(result, n_draws: int = 250, block: int | None = None, s
 
 def _leave_out_match(fit, psi, n_draws, block, seed, tol, maxiter, precision, timings,
                      se=False, se_draws=None, se_trace=True, smooth_bins=None,
-                     se_variance="person_year"):
+                     se_variance="person_year", diagnose=None, diagnose_draws=64):
     """KSS leaving out a whole (worker, firm) match (leaveout_match.py:
     156-214): the row fit's y~ = the worker effect + the firm effect + the
     residual collapsed to one row per match (its weighted mean, weight the
@@ -638,7 +652,8 @@ def _leave_out_match(fit, psi, n_draws, block, seed, tol, maxiter, precision, ti
                                  stayers="within_match", stayer_sigma2=st_sorted,
                                  centering="reference", person_years=n, se=se,
                                  se_draws=se_draws, se_trace=se_trace, smooth_bins=smooth_bins,
-                                 se_variance=se_variance, match_within=within)
+                                 se_variance=se_variance, match_within=within,
+                                 diagnose=diagnose, diagnose_draws=diagnose_draws)
     comps.n_obs = n
     comps.diagnostics["n_matches"] = mn
     comps.diagnostics["person_years"] = n
@@ -653,7 +668,7 @@ def leave_out_kss(fml: str, data, *, psi: str | None = None, n_draws: int = 250,
                   precision: str = "mixed", tol: float = 1e-10, maxiter: int = 5000,
                   timings: dict | None = None, se: bool = False, se_draws=None,
                   se_trace: bool = True, smooth_bins=None, se_variance: str = "person_year",
-                  **options) -> LeaveOutComponents:
+                  diagnose=None, diagnose_draws: int = 64, **options) -> LeaveOutComponents:
     """Kline-Saggio-Solvsten leave-out variance components, end to end, on
     the device (leaveout.py:1069-1255): prune to the largest leave-one-out
     connected set of (alpha, psi), fit with alpha streamed and no singleton
@@ -668,7 +683,9 @@ def leave_out_kss(fml: str, data, *, psi: str | None = None, n_draws: int = 250,
     components' standard errors (leaveout_se.py); leaving out a match,
     `se_variance` is how each match's error variance is estimated
     ("person_year", the reference implementation's, unweighted only, or
-    "match") and var(alpha) gets none (leaveout.py:1136-1170)."""
+    "match") and var(alpha) gets none (leaveout.py:1136-1170).  `diagnose`
+    (default: `se`) and `diagnose_draws` run the weak-identification
+    diagnostic (leaveout_weakid.py)."""
     import torch
 
     from .connected import leave_one_out
@@ -732,12 +749,14 @@ def leave_out_kss(fml: str, data, *, psi: str | None = None, n_draws: int = 250,
     if leave_out == "match":
         comps = _leave_out_match(fit, psi, n_draws, block, seed, tol, maxiter, precision, t,
                                  se=se, se_draws=se_draws, se_trace=se_trace,
-                                 smooth_bins=smooth_bins, se_variance=se_variance)
+                                 smooth_bins=smooth_bins, se_variance=se_variance,
+                                 diagnose=diagnose, diagnose_draws=diagnose_draws)
     else:
         comps = leave_out_components(fit, n_draws=n_draws, block=block, seed=seed, psi=psi,
                                      tol=tol, maxiter=maxiter, precision=precision, timings=t,
                                      se=se, se_draws=se_draws, se_trace=se_trace,
-                                     smooth_bins=smooth_bins)
+                                     smooth_bins=smooth_bins, diagnose=diagnose,
+                                     diagnose_draws=diagnose_draws)
     t.update(prune_seconds=t1 - t0, fit_seconds=t2 - t1,
              total_seconds=time.perf_counter() - t0)
     comps.pruning = pruning

@@ -250,38 +250,42 @@ def _knots(values_host: np.ndarray, n_bins: int) -> np.ndarray:
     return q + np.arange(len(q)) * span * 1e-12
 
 
-def smoothed_sigma2(sigma2, leverage, b_ii, n_bins=None):
-    """sigma2 smoothed on the grid of (P_ii, B_ii) quantile ranks: cell means
-    interpolated bilinearly between cell centres (leaveout_se.py:982-1054);
-    device (n,).  n_bins < 1: sigma2 itself."""
+def smoothed_sigma2(sigma2, leverage, b_ii=None, n_bins=None):
+    """sigma2 smoothed on the grid of (P_ii, B_ii) quantile ranks -- or of the
+    leverage alone without b_ii -- cell means interpolated (bi)linearly
+    between cell centres (leaveout_se.py:982-1054); device (n,).
+    n_bins < 1: sigma2 itself."""
     import torch
     n = int(sigma2.shape[0])
-    nb = auto_bins(n, 2) if n_bins is None else int(n_bins)
+    dims = 1 if b_ii is None else 2
+    nb = auto_bins(n, dims) if n_bins is None else int(n_bins)
     if nb < 1:
         return sigma2.to(torch.float64).contiguous()
     if nb == 1:
         raise ValueError("smooth_bins must be 0 (no smoothing) or at least 2")
     take = torch.from_numpy(_sample(n)).cuda()
     lev = leverage.to(torch.float64).contiguous()
-    bii = b_ii.to(torch.float64).contiguous()
-    kp = _knots(lev[take].cpu().numpy(), nb)
-    kb = _knots(bii[take].cpu().numpy(), nb)
-    kp_d, kb_d = torch.from_numpy(kp).cuda(), torch.from_numpy(kb).cuda()
-    tot, cnt = np.empty(nb * nb), np.empty(nb * nb)
+    kp_d = torch.from_numpy(_knots(lev[take].cpu().numpy(), nb)).cuda()
+    bii = kb_d = None
+    if b_ii is not None:
+        bii = b_ii.to(torch.float64).contiguous()
+        kb_d = torch.from_numpy(_knots(bii[take].cpu().numpy(), nb)).cuda()
+    ncell = nb ** dims
+    tot, cnt = np.empty(ncell), np.empty(ncell)
     s2 = sigma2.to(torch.float64).contiguous()
     lib = _lib.load()
+    ptr = (lambda t: None if t is None else t.data_ptr())
     torch.cuda.synchronize()
-    _lib.check(lib.hdfe_se_cell_sums(n, lev.data_ptr(), bii.data_ptr(), s2.data_ptr(),
-                                     kp_d.data_ptr(), kb_d.data_ptr(), nb, tot.ctypes.data,
+    _lib.check(lib.hdfe_se_cell_sums(n, lev.data_ptr(), ptr(bii), s2.data_ptr(),
+                                     kp_d.data_ptr(), ptr(kb_d), nb, tot.ctypes.data,
                                      cnt.ctypes.data), "hdfe_se_cell_sums")
     overall = float(tot.sum() / max(cnt.sum(), 1.0))
     means = np.where(cnt > 0, tot / np.maximum(cnt, 1.0), overall)
     means_d = torch.from_numpy(means).cuda()
     out = torch.empty(n, dtype=torch.float64, device="cuda")
     torch.cuda.synchronize()
-    _lib.check(lib.hdfe_se_smooth(n, lev.data_ptr(), bii.data_ptr(), kp_d.data_ptr(),
-                                  kb_d.data_ptr(), means_d.data_ptr(), nb, out.data_ptr()),
-               "hdfe_se_smooth")
+    _lib.check(lib.hdfe_se_smooth(n, lev.data_ptr(), ptr(bii), kp_d.data_ptr(), ptr(kb_d),
+                                  means_d.data_ptr(), nb, out.data_ptr()), "hdfe_se_smooth")
     return out
 
 

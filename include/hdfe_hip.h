@@ -343,11 +343,13 @@ int hdfe_se_bii(int64_t n, int s, const double *rows, const double *w, double *o
 int hdfe_se_cross_groups(int64_t G, int s, const int *gstart, const int *psi_codes,
                          int64_t psi_off, const double *w, const double *Ul, double *out);
 /* The smoother's cells: per cell of the nb x nb grid of (P_ii, B_ii)
- * quantile ranks, the sum of sigma2 and the rows (host tot, cnt).  Replaces
- * fit_smoother's binning (leaveout_se.py:1024-1033). */
+ * quantile ranks, the sum of sigma2 and the rows (host tot, cnt); with bii
+ * and kb null, the nb cells of the leverage alone.  Replaces fit_smoother's
+ * binning (leaveout_se.py:1024-1033). */
 int hdfe_se_cell_sums(int64_t n, const double *lev, const double *bii, const double *sigma2,
                       const double *kp, const double *kb, int nb, double *tot, double *cnt);
-/* The smoothed sigma2, bilinear between cell centres.  Replaces smooth_eval
+/* The smoothed sigma2, bilinear between cell centres (linear along the
+ * leverage with bii and kb null).  Replaces smooth_eval
  * (leaveout_se.py:1038-1054). */
 int hdfe_se_smooth(int64_t n, const double *lev, const double *bii, const double *kp,
                    const double *kb, const double *means, int nb, double *out);

@@ -1320,3 +1320,369 @@ def component_variances(lay: SELayout, y: np.ndarray, sigma2: np.ndarray, levera
             "se": np.where(variance > 0, np.sqrt(np.abs(variance)), np.nan),
             "se_conservative": np.where(first > 0, np.sqrt(np.abs(first)), np.nan),
             "b_ii": b_ii, "b": b, "s": s}
+
+
+# -------------------------------------------- weak-identification diagnostic
+#
+# leaveout_weakid.py: Lanczos on Q S^-, self-adjoint in <x, y> = x' S^- y, the
+# three components in lockstep (one solve of three columns per step), from
+# x0 = Q r; the leading Ritz values with Cullum-Willoughby filtering; the Ritz
+# vectors by retracing; sum lambda^2 = tr((S^- Q)^2) by deflated Hutchinson;
+# KSS's q rule and the first-stage F of the leading direction.
+
+WEAK_ID_THRESHOLD = 0.1            # leaveout_weakid.py:67
+WEAKID_SEED_OFFSET = 9_900_019     # leaveout.py:758
+WEAKID_SUM_OFFSET = 3_300_007      # leaveout_weakid.py:651
+SCRATCH_BYTES = 32 << 20           # core/estimator.py:77 (scratch_mb)
+
+
+def q_block(lay: SELayout, comps, Ul: np.ndarray, Ug: np.ndarray):
+    """Q applied column by column, column j with component comps[j]
+    (leaveout_weakid.py:164-179)."""
+    comps = np.asarray(comps)
+    ol, og = np.zeros_like(Ul), np.zeros_like(Ug)
+    for k in range(3):
+        cols = np.flatnonzero(comps == k)
+        if len(cols):
+            a, b = apply_q(lay, k, Ul[:, cols], Ug[:, cols])
+            ol[:, cols], og[:, cols] = a, b
+    return ol, og
+
+
+def _cdot(a, b):
+    return sum((x * y).sum(axis=0) for x, y in zip(a, b))
+
+
+def krylov_plan(lay: SELayout, max_iter: int):
+    """Steps per component and whether its basis is kept for full
+    reorthogonalization (leaveout_weakid.py:186-215)."""
+    n_psi = lay.psi.stop - lay.psi.start
+    G, L = lay.ctx.G, lay.ctx.L
+    dims = [max(n_psi - 1, 0), max(L + 1, 1), max(2 * (n_psi - 1), 0)]
+    support = [n_psi, G, n_psi + G]
+    caps = [min(max_iter, d) for d in dims]
+    reorth = [cap * 2 * rows * 8 <= SCRATCH_BYTES for cap, rows in zip(caps, support)]
+    return caps, reorth
+
+
+def lanczos(lay: SELayout, n_iter: int, seed: int, ritz=None, check=None, every: int = 20,
+            caps=None, reorth=None):
+    """The three-component recurrence (leaveout_weakid.py:217-383); blocks are
+    (levels (L, 3), groups (G, 3)).  Returns (alphas, betas, lengths, steps),
+    or with `ritz` the Ritz combinations (X, Y) of 3k columns."""
+    ctx = lay.ctx
+    L, G = ctx.L, ctx.G
+    psi = lay.psi
+    caps = [n_iter] * 3 if caps is None else list(caps)
+    reorth = [False] * 3 if reorth is None else list(reorth)
+    comps = [0, 1, 2]
+
+    def support(c, blocks):
+        parts = []
+        if c in (0, 2):
+            parts.append(blocks[0][psi, c])
+        if c in (1, 2):
+            parts.append(blocks[1][:, c])
+        return np.concatenate(parts) if len(parts) > 1 else parts[0].copy()
+
+    def project(blocks):
+        for c in (0, 2):
+            blocks[0][psi, c] -= blocks[0][psi, c].mean()
+        for c in (1, 2):
+            blocks[1][:, c] -= blocks[1][:, c].mean()
+
+    def scatter(c, flat, blocks):
+        at = 0
+        if c in (0, 2):
+            n = psi.stop - psi.start
+            blocks[0][psi, c] -= flat[at:at + n]
+            at += n
+        if c in (1, 2):
+            blocks[1][:, c] -= flat[at:]
+
+    # x0 = Q r, r Rademacher keyed on (position, column): levels then groups
+    rl = rademacher(0, L, 3, seed, 0)
+    rg = rademacher(L, G, 3, seed, 0)
+    ql, qg = q_block(lay, comps, rl, rg)
+    x = [ql, qg]
+    project(x)
+    y = list(apply_inverse(ctx, x[0], x[1])[:2])
+    norm = np.sqrt(np.maximum(_cdot(x, y), 0.0))
+    live = (norm > 0) & (np.array(caps) > 0)
+    for blk in (*x, *y):
+        blk[:, live] /= norm[live]
+        blk[:, ~live] = 0.0
+    x_prev = [np.zeros((L, 3)), np.zeros((G, 3))]
+    beta_prev = np.zeros(3)
+    basis = [([], []) for _ in comps]
+    alphas, betas = [[] for _ in comps], [[] for _ in comps]
+    lengths = np.zeros(3, dtype=int)
+    acc = None
+    if ritz is not None:
+        k = ritz[0].shape[1]
+        acc = ([np.zeros((L, 3 * k)), np.zeros((G, 3 * k))],
+               [np.zeros((L, 3 * k)), np.zeros((G, 3 * k))])
+    beta_max = np.zeros(3)
+    steps = 0
+    for j in range(n_iter):
+        if not live.any():
+            break
+        if (check is not None and j > 0 and j % every == 0
+                and check([np.array(a) for a in alphas], [np.array(b) for b in betas],
+                          lengths, j)):
+            break
+        steps = j + 1
+        if acc is not None:
+            for c in np.flatnonzero(live):
+                if j < len(ritz[c]):
+                    cols = slice(c * k, (c + 1) * k)
+                    for ax, ay, xb, yb in zip(acc[0], acc[1], x, y):
+                        ax[:, cols] += np.outer(xb[:, c], ritz[c][j])
+                        ay[:, cols] += np.outer(yb[:, c], ritz[c][j])
+        for c in np.flatnonzero(live):
+            if reorth[c]:
+                basis[c][0].append(support(c, x))
+                basis[c][1].append(support(c, y))
+        ql, qg = q_block(lay, comps, y[0], y[1])
+        w = [ql, qg]
+        for wb, xp in zip(w, x_prev):
+            wb -= beta_prev * xp
+        alpha = _cdot(w, y)
+        for wb, xb in zip(w, x):
+            wb -= alpha * xb
+        fix = _cdot(w, y)
+        for wb, xb in zip(w, x):
+            wb -= fix * xb
+        alpha = alpha + fix
+        project(w)
+        for c in np.flatnonzero(live):
+            if not reorth[c] or not basis[c][0]:
+                continue
+            for _ in range(2):
+                wc = support(c, w)
+                Xs, Ys = np.column_stack(basis[c][0]), np.column_stack(basis[c][1])
+                scatter(c, Xs @ (Ys.T @ wc), w)
+        project(w)
+        z = list(apply_inverse(ctx, w[0], w[1])[:2])
+        beta = np.sqrt(np.maximum(_cdot(w, z), 0.0))
+        beta_max = np.maximum(beta_max, beta)
+        for c in np.flatnonzero(live):
+            alphas[c].append(float(alpha[c]))
+            betas[c].append(float(beta[c]))
+            lengths[c] = j + 1
+        live = live & (beta > 1e-8 * np.maximum(beta_max, 1e-300)) & (lengths < np.array(caps))
+        x_prev = x
+        x = [wb.copy() for wb in w]
+        y = z
+        for blk in (*x, *y):
+            blk[:, live] /= beta[live]
+            blk[:, ~live] = 0.0
+        beta_prev = np.where(live, beta, 0.0)
+    if acc is not None:
+        return acc
+    return [np.array(a) for a in alphas], [np.array(b) for b in betas], lengths, steps
+
+
+def ritz_top(alpha: np.ndarray, beta: np.ndarray, top: int):
+    """The leading genuine Ritz values, their error bounds and eigenvectors:
+    ghosts merged, lone spurious values dropped by Cullum-Willoughby
+    (leaveout_weakid.py:385-433)."""
+    from scipy.linalg import eigh_tridiagonal, eigvalsh_tridiagonal
+    m = len(alpha)
+    if m == 0:
+        return np.array([]), np.array([]), np.zeros((0, 0))
+    off = beta[:m - 1]
+    theta, S = eigh_tridiagonal(alpha, off)
+    bounds = np.abs(beta[m - 1] * S[-1, :])
+    norm = max(float(np.abs(theta).max()), 1e-300)
+    merge_tol = 1e-8 * norm
+    order = np.argsort(theta)
+    keep = []
+    for idx in order:
+        if keep and abs(theta[idx] - theta[keep[-1]]) <= merge_tol:
+            if bounds[idx] < bounds[keep[-1]]:
+                keep[-1] = idx
+            continue
+        keep.append(idx)
+    mult = {}
+    for idx in order:
+        head = min(keep, key=lambda q: abs(theta[q] - theta[idx]))
+        mult[head] = mult.get(head, 0) + 1
+    if m > 1:
+        hat = eigvalsh_tridiagonal(alpha[1:], off[1:]) if m > 2 else alpha[1:]
+        spur = 1e-11 * norm
+        keep = [q for q in keep if mult[q] > 1 or np.min(np.abs(hat - theta[q])) > spur]
+    keep = sorted(keep, key=lambda q: -abs(theta[q]))[:top]
+    return theta[keep], bounds[keep], S[:, keep]
+
+
+def s_orthonormalize(X, Y, k: int):
+    """Modified Gram-Schmidt of each component's k Ritz vectors in
+    <x, y> = x'S^- y (leaveout_weakid.py:436-465)."""
+    for c in range(3):
+        base = c * k
+        for a in range(k):
+            ca = base + a
+            for p in range(a):
+                cp = base + p
+                coef = sum(xb[:, ca] @ yb[:, cp] for xb, yb in zip(X, Y))
+                for xb, yb in zip(X, Y):
+                    xb[:, ca] -= coef * xb[:, cp]
+                    yb[:, ca] -= coef * yb[:, cp]
+            n2 = sum(xb[:, ca] @ yb[:, ca] for xb, yb in zip(X, Y))
+            lead = sum(xb[:, base] @ yb[:, base] for xb, yb in zip(X, Y))
+            if n2 <= 1e-20 * max(lead, 1e-300) or n2 <= 0:
+                for blk in (*X, *Y):
+                    blk[:, ca] = 0.0
+                continue
+            for blk in (*X, *Y):
+                blk[:, ca] *= 1.0 / np.sqrt(n2)
+    return X, Y
+
+
+def sum_sq_eigenvalues(lay: SELayout, n_draws: int, block: int, seed: int, X, Y, k: int,
+                       settled=None, min_draws: int = 16):
+    """tr((S^- Q)^2) per component: the deflated directions exactly, the
+    rest by Hutchinson over zeta = X~'xi (leaveout_weakid.py:467-551).
+    Returns (estimates, Monte Carlo standard errors, draws)."""
+    ctx = lay.ctx
+    n = len(ctx.w)
+    comps = [c for c in range(3) for _ in range(k)]
+    ql, qg = q_block(lay, comps, Y[0], Y[1])
+    zl, zg, _ = apply_inverse(ctx, ql, qg)
+    exact = ((ql * zl).sum(0) + (qg * zg).sum(0)).reshape(3, k).sum(axis=1)
+    draws, drawn = [], 0
+    size = max(1, block // 3)
+    while drawn < n_draws:
+        m = min(size, n_draws - drawn)
+        xi = rademacher(0, n, m, seed, drawn)
+        zeta = list(se_reduce(lay, xi))
+        yl, yg, _ = apply_inverse(ctx, zeta[0], zeta[1])
+        y = [yl, yg]
+        yd = [np.empty((b.shape[0], 3 * m)) for b in y]
+        for c in range(3):
+            cols = slice(c * k, (c + 1) * k)
+            proj = sum(zb.T @ Yb[:, cols] for zb, Yb in zip(zeta, Y))
+            out = slice(c * m, (c + 1) * m)
+            for d, yb, Yb in zip(yd, y, Y):
+                d[:, out] = yb - Yb[:, cols] @ proj.T
+        ql, qg = q_block(lay, [c for c in range(3) for _ in range(m)], yd[0], yd[1])
+        bl, bg, _ = apply_inverse(ctx, ql, qg)
+        draws.append(((ql * bl).sum(0) + (qg * bg).sum(0)).reshape(3, m))
+        drawn += m
+        vals = np.hstack(draws)
+        if settled is not None and min_draws <= drawn < n_draws and \
+                settled(*_summarize(exact, vals)):
+            break
+    est, se = _summarize(exact, np.hstack(draws))
+    return est, se, drawn
+
+
+def _summarize(exact, values):
+    n = values.shape[1]
+    se = values.std(axis=1, ddof=1) / np.sqrt(n) if n > 1 else np.full(values.shape[0], np.nan)
+    return exact + values.mean(axis=1), se
+
+
+def fit_smoother_1d(sigma2, leverage, n_bins=None):
+    """sigma2 smoothed on the leverage alone (leaveout_se.py:982-1035, one
+    axis): (knots, means, n_bins) or None."""
+    n = len(sigma2)
+    nb = auto_bins(n, 1) if n_bins is None else int(n_bins)
+    if nb < 1:
+        return None
+    kp = smoother_knots(leverage, nb, smoother_sample(n))
+    cp = np.minimum((smoother_rank(kp, leverage) * nb).astype(np.int64), nb - 1)
+    tot = np.bincount(cp, weights=np.asarray(sigma2, np.float64), minlength=nb)
+    cnt = np.bincount(cp, minlength=nb).astype(np.float64)
+    overall = float(tot.sum() / max(cnt.sum(), 1.0))
+    return kp, np.where(cnt > 0, tot / np.maximum(cnt, 1.0), overall), nb
+
+
+def smooth_eval_1d(table, leverage):
+    kp, means, nb = table
+    f = smoother_rank(kp, leverage) * nb - 0.5
+    i0 = np.clip(np.floor(f).astype(np.int64), 0, nb - 2)
+    a = np.clip(f - i0, 0.0, 1.0)
+    return (1.0 - a) * means[i0] + a * means[i0 + 1]
+
+
+def weak_id_diagnostics(lay: SELayout, y: np.ndarray, sigma2: np.ndarray, leverage,
+                        n_draws: int = 64, block: int = 8, seed: int = 0, max_iter: int = 150,
+                        tol: float = 1e-4, top: int = 3, smooth_bins=None) -> dict:
+    """KSS's eigenvalue diagnostic (leaveout_weakid.py:553-744): Lanczos to
+    the leading eigenvalues, deflated sum lambda^2, the q rule, and
+    F = b1^2 / V[b1] of the leading direction."""
+    ctx = lay.ctx
+    caps, reorth = krylov_plan(lay, max_iter)
+
+    def converged_all(alphas, betas, lengths, steps):
+        for c in range(3):
+            if lengths[c] < steps:
+                continue
+            theta, bounds, _ = ritz_top(alphas[c], betas[c], top)
+            lead = float(np.abs(theta[0])) if len(theta) else 0.0
+            if np.any(bounds[:2] > tol * max(lead, 1e-300)):
+                return False
+        return True
+
+    alphas, betas, lengths, n_iter = lanczos(lay, max_iter, seed, check=converged_all,
+                                             every=10, caps=caps, reorth=reorth)
+    checked = [ritz_top(alphas[c], betas[c], top) for c in range(3)]
+    n_coef = ctx.L + ctx.G
+    k = top if 2 * 3 * top * n_coef * 8 <= SCRATCH_BYTES else 1
+    ritz = []
+    for _t, _b, S in checked:
+        coef = np.zeros((S.shape[0], k))
+        coef[:, :min(k, S.shape[1])] = S[:, :k]
+        ritz.append(coef)
+    X, Y = lanczos(lay, n_iter, seed, ritz=ritz, caps=caps, reorth=reorth)
+    X, Y = s_orthonormalize(X, Y, k)
+    u = [blk[:, ::k].copy() for blk in Y]
+    leading = [t for t, _b, _S in checked]
+
+    def settled(est, se):
+        for c in range(3):
+            if not est[c] > 0 or not np.isfinite(se[c]):
+                return False
+            r = leading[c] ** 2 / est[c]
+            if np.any(np.abs(r - WEAK_ID_THRESHOLD) < 2.0 * (se[c] / est[c]) * r):
+                return False
+        return True
+
+    sum_sq, sum_sq_se, used = sum_sq_eigenvalues(lay, n_draws, block, seed + WEAKID_SUM_OFFSET,
+                                                 X, Y, k, settled=settled)
+    table = fit_smoother_1d(sigma2, leverage, smooth_bins)
+    sm = sigma2 if table is None else smooth_eval_1d(table, leverage)
+    w = ctx.w
+    ybar = float(w @ y / w.sum())
+    yt = lay.root * (y - ybar)
+    rows = ctx.Do @ u[0] + u[1][ctx.g]
+    x1 = rows * lay.root[:, None]
+    num, den = x1.T @ yt, (x1 ** 2).T @ sm
+    f_stat = np.where(den > 0, num ** 2 / np.where(den > 0, den, 1.0), np.nan)
+    out = {"eigenvalues": [], "bounds": [], "ratios": [], "q": [], "q_capped": [],
+           "q_borderline": [], "converged": [], "sum_sq": sum_sq, "sum_sq_se": sum_sq_se,
+           "f_stat": f_stat, "steps": n_iter, "lengths": lengths, "deflated": k,
+           "n_draws": used, "caps": caps, "reorth": reorth, "lead": u}
+    for c in range(3):
+        theta, b, _ = checked[c]
+        lead = float(np.abs(theta[0])) if len(theta) else 0.0
+        out["eigenvalues"].append(theta)
+        out["bounds"].append(b)
+        out["converged"].append(bool(lengths[c] < n_iter or lengths[c] >= caps[c]
+                                     or np.all(b[:2] <= tol * max(lead, 1e-300))))
+        r = theta ** 2 / sum_sq[c] if sum_sq[c] > 0 else np.zeros_like(theta)
+        out["ratios"].append(r)
+        cnt = 0
+        for v in r:
+            if v >= WEAK_ID_THRESHOLD:
+                cnt += 1
+            else:
+                break
+        out["q"].append(cnt)
+        out["q_capped"].append(bool(cnt == len(r) and cnt > 0))
+        rel_se = sum_sq_se[c] / sum_sq[c] if sum_sq[c] > 0 else np.inf
+        out["q_borderline"].append(bool(np.any(np.abs(r - WEAK_ID_THRESHOLD)
+                                               < 2.0 * rel_se * r)))
+    return out

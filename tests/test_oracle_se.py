@@ -182,3 +182,78 @@ def test_smoother(oracle_mod):
     mid_b = 0.5 * (kb[2] + kb[3])
     assert O.smooth_eval((kp, kb, means, 4), np.array([mid_p]), np.array([mid_b]))[0] == \
         pytest.approx(means[1, 2], rel=1e-9)
+
+
+# ------------------------------------------------ weak-identification diagnostic
+
+def _dense_A(dense, k):
+    """S^- Q_k in the coefficient order (groups, levels)."""
+    lay = dense["lay"]
+    Sfull = dense["Xt"].T @ dense["Xt"]
+    Lp, La = dense["Lp"], dense["La"]
+    Q = [Lp.T @ Lp, La.T @ La, 0.5 * (Lp.T @ La + La.T @ Lp)][k]
+    return np.linalg.pinv(Sfull) @ Q, Sfull, Q, lay
+
+
+def test_lanczos_finds_the_leading_eigenvalues(dense):
+    O = dense["O"]
+    out = O.weak_id_diagnostics(dense["lay"], dense["f"]["p0"]["V"][:, 0],
+                                np.random.default_rng(1).uniform(0.1, 0.4, dense["n"]),
+                                np.random.default_rng(2).uniform(0.2, 0.9, dense["n"]),
+                                n_draws=24, seed=5, tol=1e-10, max_iter=400)
+    G = dense["ctx"].G
+    for k in range(3):
+        A, Sfull, Q, lay = _dense_A(dense, k)
+        ev = np.linalg.eigvals(A).real
+        ev = ev[np.argsort(-np.abs(ev))][:3]
+        got = out["eigenvalues"][k]
+        np.testing.assert_allclose(got[:len(ev)], ev[:len(got)], rtol=1e-7,
+                                   atol=1e-9 * abs(ev[0]))
+        # the lead vector: an eigenvector of S^- Q, S-normalized, up to a part
+        # in S's null space (the streamed block's recovery leaves one; Q, X~u
+        # and u'Su do not see it)
+        u = np.concatenate([out["lead"][1][:, k], out["lead"][0][:, k]])
+        assert u @ Sfull @ u == pytest.approx(1.0, rel=1e-6)
+        wS, VS = np.linalg.eigh(Sfull)
+        null = VS[:, wS < 1e-9 * wS.max()]
+        u_range = u - null @ (null.T @ u)
+        assert rel(A @ u, got[0] * u_range) < 1e-6
+        # sum lambda^2: exact part plus a Monte Carlo remainder, within its error
+        exact = np.trace(A @ A)
+        assert abs(out["sum_sq"][k] - exact) <= 5 * out["sum_sq_se"][k] + 1e-9 * exact
+        r = np.array(got) ** 2 / out["sum_sq"][k]
+        np.testing.assert_allclose(out["ratios"][k], r, rtol=1e-12)
+    del G
+
+
+def test_first_stage_f(dense):
+    O, lay, ctx, n = dense["O"], dense["lay"], dense["ctx"], dense["n"]
+    rng = np.random.default_rng(3)
+    s2, lev = rng.uniform(0.1, 0.4, n), rng.uniform(0.2, 0.9, n)
+    y = dense["f"]["p0"]["V"][:, 0]
+    out = O.weak_id_diagnostics(lay, y, s2, lev, n_draws=8, seed=1)
+    tab = O.fit_smoother_1d(s2, lev)
+    sm = O.smooth_eval_1d(tab, lev)
+    yt = lay.root * (y - ctx.w @ y / ctx.w.sum())
+    A = np.hstack([ctx.Dg.toarray(), ctx.Do.toarray()])
+    for k in range(3):
+        u = np.concatenate([out["lead"][1][:, k], out["lead"][0][:, k]])
+        x1 = lay.root * (A @ u)
+        assert out["f_stat"][k] == pytest.approx((x1 @ yt) ** 2 / ((x1 ** 2) @ sm), rel=1e-10)
+
+
+def test_ritz_filter_drops_a_lone_spurious_value(oracle_mod):
+    O = oracle_mod
+    # a tridiagonal whose first row decouples: the trailing block's values
+    # are eigenvalues of T with its first row and column deleted, lone ones,
+    # unseen by the starting vector -- spurious, dropped; 2.0 is genuine
+    alpha = np.array([2.0, 5.0, 1.0, 3.0])
+    beta = np.array([1e-14, 0.5, 0.2, 1e-3])
+    theta, bounds, S = O.ritz_top(alpha, beta, 4)
+    np.testing.assert_allclose(theta, [2.0], rtol=1e-12)
+    # coupled, every value is genuine
+    theta, _b, _S = O.ritz_top(alpha, np.array([0.7, 0.5, 0.2, 1e-3]), 4)
+    assert len(theta) == 4
+    # merged copies: two equal eigenvalues count once
+    theta, _b, _S = O.ritz_top(np.array([1.0, 1.0]), np.array([0.0, 1e-3]), 4)
+    assert len(theta) == 1 and theta[0] == pytest.approx(1.0)

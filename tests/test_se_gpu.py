@@ -136,3 +136,39 @@ def test_leave_out_match_standard_errors(oracle_mod, se_variance):
         assert np.isnan(part["var(alpha)"]) and np.isnan(want[key][1])
     assert np.isnan(lo.se.se["var(alpha)"])
     assert lo.se.se["var(psi)"] > 0
+
+
+def test_weak_id_matches_the_oracle(oracle_mod):
+    import torch
+    from hdfe_stream_amd.estimator import feols_stream
+    from hdfe_stream_amd.leaveout import CoefficientInverse, JLALeverages
+    from hdfe_stream_amd.leaveout_weakid import weak_id_diagnostics
+    from hdfe_stream_amd.simulate import kss_panel
+    O = oracle_mod
+    d = _loo(O, kss_panel(1200, 150, seed=5))
+    print("weak_id_diagnostics", flush=True)
+    res = feols_stream("y ~ 1 | worker_id + firm_id", d, stream="worker_id", fixef_rm="none",
+                       keep_intermediates=True, tol=1e-12)
+    f = O.fit(d, "y", (), FE, tol=1e-12, fixef_rm="none", stream="worker_id")
+    ctx = O.KSSContext(f, tol=1e-12)
+    n = len(ctx.w)
+    rng = np.random.default_rng(6)
+    s2, lev = rng.uniform(0.1, 0.4, n), rng.uniform(0.2, 0.9, n)
+    y = f["p0"]["V"][:, 0]
+    ref = O.weak_id_diagnostics(O.SELayout(ctx, 0, n), y, s2, lev, n_draws=32, block=8, seed=7)
+    cu = lambda a: torch.from_numpy(np.ascontiguousarray(a, np.float64)).cuda()
+    levs = JLALeverages(leverage=cu(lev), complement=cu(1.0 - lev), factor=cu(np.ones(n)),
+                        n_draws=1)
+    inv = CoefficientInverse(res.panel, tol=1e-12, precision="fp64")
+    out = weak_id_diagnostics(inv, res.panel.tensor("V", torch.float64), cu(s2), levs,
+                              n_draws=32, block=8, seed=7)
+    names = ["var(psi)", "var(alpha)", "cov(psi, alpha)"]
+    assert out.diagnostics["lanczos_steps"] == ref["steps"]
+    for k, nm in enumerate(names):
+        np.testing.assert_allclose(out.eigenvalues[nm], ref["eigenvalues"][k], rtol=1e-6)
+        assert out.sum_sq[nm] == pytest.approx(ref["sum_sq"][k], rel=1e-6)
+        assert out.f_stat[nm] == pytest.approx(ref["f_stat"][k], rel=1e-5)
+        assert out.q[nm] == ref["q"][k]
+        np.testing.assert_allclose(out.ratios[nm], ref["ratios"][k], rtol=1e-5)
+    assert out.diagnostics["n_draws"] == ref["n_draws"]
+    assert "weak-identification" in out.summary()
