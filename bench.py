@@ -379,7 +379,8 @@ def main():
             "frac": round(ach / HBM_PEAK_GBS, 4),
             "traffic": measured_traffic(kname, f"c2:{args.workers}x{args.firms}"),
             "kernel": kname, "bytes_per_launch": int(byt),
-            "avg_launch_us": round(avg64 * 1e6, 2)}
+            "avg_launch_us": round(avg64 * 1e6, 2),
+            "tuned_row_parts_per_xcd": f64.panel.matrix.blk_parts(k, "fp64")}
     b32 = spmm_bytes(nnz, L, k, 4)
     line = {
         "metric": "AKM fit wall-s @5M workers (mixed FP32/FP64-refined, 1e-10 rel. residual)",
@@ -402,7 +403,8 @@ def main():
         "roofline": roof,
         "spmm_fp32": {"kernel": "k_spmm_blk<float,3,G> + k_spmm_combine<float,3>",
                       "avg_launch_us": round(avg32 * 1e6, 2),
-                      "bytes_per_launch": int(b32), "achieved_GBps": round(b32 / avg32 / 1e9, 1)},
+                      "bytes_per_launch": int(b32), "achieved_GBps": round(b32 / avg32 / 1e9, 1),
+                      "tuned_row_parts_per_xcd": f64.panel.matrix.blk_parts(k, "mixed")},
         "copy_calibration": copy_calibration(),
     }
     if not (args.no_jla and args.no_kss and args.no_adjoint and args.no_glm):

@@ -33,6 +33,8 @@
 
 #include <cmath>
 #include <algorithm>
+#include <array>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <vector>
@@ -47,8 +49,9 @@ constexpr int kLongThresh = 4096;  // nonzeros: rows above this are split
 constexpr int kLongChunk = 2048;   // nonzeros per workgroup of a split row
 constexpr int kMaxK = 1024;        // widest block of right-hand sides per call
 constexpr int kSmallK = 8;         // k <= 8: lanes over nonzeros; wider: over columns
-constexpr int64_t kL2Budget = 5 << 19;  // bytes of P an XCD's 4 MB L2 should hold (2.5 MB)
+constexpr int64_t kL2Budget = 3 << 20;  // bytes of P an XCD's 4 MB L2 should hold (3 MB)
 constexpr int kBlkParts = 128;     // row parts per XCD of the blocked SpMM
+constexpr int kDenseLongMax = 16;  // split rows the blocked SpMM takes as dense columns
 
 // Device-side solver state (one allocation; see State layout in Work).
 struct CgState {
@@ -158,20 +161,37 @@ __global__ __launch_bounds__(kFinThreads) void k_fin_alpha(const double *__restr
 // the wave's fixed tree) and its P AP; with with_dot, the pAp sums of the nw
 // workgroups and alpha = rz / pAp where active and pAp > 0, else 0
 // (core/solve.py:120).
+// With ll (the split rows' block among themselves, nlong x nlong) the
+// partials hold only the split rows' products with the other rows, and
+// sum_t' ll[l, t'] P[long t'] is added last.
 template <typename T, int K>
 __device__ void spmm_finish(const T *__restrict__ P, T *__restrict__ AP,
                             const double *__restrict__ part, int nw, int with_dot,
                             const int *__restrict__ long_rows, const int *__restrict__ long_cptr,
-                            int nlong, const double *__restrict__ lpart, CgState st) {
+                            int nlong, const double *__restrict__ lpart, CgState st,
+                            const T *__restrict__ ll = nullptr) {
     const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
     const int nwv = blockDim.x / kWave;
     for (int q = wv; q < nlong * K; q += nwv) {
         const int l = q / K, j = q % K;
-        double s2 = 0.0;
-        for (int c = long_cptr[l] + lane; c < long_cptr[l + 1]; c += kWave)
-            s2 += lpart[(int64_t)c * K + j];
-        s2 = wave_sum(s2);
-        if (lane == 0) AP[(int64_t)long_rows[l] * K + j] = (T)s2;
+        // four loads in flight per lane, their sums added in a fixed order
+        double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+        const int c1 = long_cptr[l + 1];
+        int c = long_cptr[l] + lane;
+        for (; c + 3 * kWave < c1; c += 4 * kWave) {
+            a0 += lpart[(int64_t)c * K + j];
+            a1 += lpart[(int64_t)(c + kWave) * K + j];
+            a2 += lpart[(int64_t)(c + 2 * kWave) * K + j];
+            a3 += lpart[(int64_t)(c + 3 * kWave) * K + j];
+        }
+        for (; c < c1; c += kWave) a0 += lpart[(int64_t)c * K + j];
+        double s2 = wave_sum((a0 + a1) + (a2 + a3));
+        if (lane == 0) {
+            if (ll)
+                for (int t = 0; t < nlong; ++t)
+                    s2 += (double)ll[l * nlong + t] * (double)P[(int64_t)long_rows[t] * K + j];
+            AP[(int64_t)long_rows[l] * K + j] = (T)s2;
+        }
     }
     if (!with_dot) return;
     __shared__ double pap[K];
@@ -192,9 +212,9 @@ template <typename T, int K>
 __global__ __launch_bounds__(kFinThreads) void k_fin_small(
     const T *__restrict__ P, T *__restrict__ AP, const double *__restrict__ part, int nw,
     int with_dot, const int *__restrict__ long_rows, const int *__restrict__ long_cptr, int nlong,
-    const double *__restrict__ lpart, CgState st) {
+    const double *__restrict__ lpart, CgState st, const T *__restrict__ ll = nullptr) {
     if (st.done && *st.done) return;
-    spmm_finish<T, K>(P, AP, part, nw, with_dot, long_rows, long_cptr, nlong, lpart, st);
+    spmm_finish<T, K>(P, AP, part, nw, with_dot, long_rows, long_cptr, nlong, lpart, st, ll);
 }
 
 // AP = S P with, when with_dot, part[w, j] = sum over the workgroup's rows of
@@ -373,17 +393,25 @@ __global__ __launch_bounds__(kThreads) void k_spmm_blk(
 }
 
 // Blocked small-k SpMM, launch 2 of 2: AP = the sum of the groups' partials
-// in group order; with with_dot the pAp sums, and the last workgroup adds
-// the split rows and forms alpha as k_spmm_small does.
+// in group order; with with_dot the pAp sums.  With nld dense long columns
+// (the split rows taken by symmetry: dt[t * L + row] = S[row, long t] for the
+// other rows, column-major), each row also adds sum_t dt[t, row] P[long t],
+// and each wave then forms, for its split rows t, the workgroup's partial of
+// AP[long t] = sum_row dt[t, row] P[row] into
+// dlpart[(t * nw + w) * K + j] (added in workgroup order by k_fin_small).
 template <typename T, int K>
 __global__ __launch_bounds__(kThreads) void k_spmm_combine(
     int64_t L, int ngrp, const T *__restrict__ gpart, const T *__restrict__ P, T *__restrict__ AP,
-    double *__restrict__ part, int with_dot, const int *__restrict__ long_rows,
-    const int *__restrict__ long_cptr, int nlong, const double *__restrict__ lpart, CgState st) {
+    double *__restrict__ part, int with_dot, int nld, const T *__restrict__ dt,
+    const int *__restrict__ long_rows, double *__restrict__ dlpart, CgState st) {
     if (st.done && *st.done) return;
     __shared__ double lds[kThreads / kWave];
+    __shared__ T plong[kDenseLongMax * K];
     const int nw = gridDim.x, w = blockIdx.x;
     const int64_t r0 = part_begin(L, w, nw), r1 = part_begin(L, w + 1, nw);
+    for (int q = threadIdx.x; q < nld * K; q += blockDim.x)
+        plong[q] = P[(int64_t)long_rows[q / K] * K + q % K];
+    if (nld) __syncthreads();
     double dot[K];
 #pragma unroll
     for (int j = 0; j < K; ++j) dot[j] = 0.0;
@@ -396,10 +424,33 @@ __global__ __launch_bounds__(kThreads) void k_spmm_combine(
 #pragma unroll
             for (int j = 0; j < K; ++j) acc[j] += gpart[g * LK + row * K + j];
         }
+        for (int t = 0; t < nld; ++t) {
+            const T d = dt[t * L + row];
+#pragma unroll
+            for (int j = 0; j < K; ++j) acc[j] += d * plong[t * K + j];
+        }
 #pragma unroll
         for (int j = 0; j < K; ++j) {
             AP[row * K + j] = acc[j];
             dot[j] += (double)P[row * K + j] * (double)acc[j];
+        }
+    }
+    if (nld) {
+        const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
+        for (int t = wv; t < nld; t += kThreads / kWave) {
+            double ta[K];
+#pragma unroll
+            for (int j = 0; j < K; ++j) ta[j] = 0.0;
+            for (int64_t row = r0 + lane; row < r1; row += kWave) {
+                const double d = (double)dt[t * L + row];
+#pragma unroll
+                for (int j = 0; j < K; ++j) ta[j] += d * (double)P[row * K + j];
+            }
+#pragma unroll
+            for (int j = 0; j < K; ++j) {
+                const double v = wave_sum(ta[j]);
+                if (lane == 0) dlpart[((int64_t)t * nw + w) * K + j] = v;
+            }
         }
     }
     if (with_dot) {
@@ -719,10 +770,12 @@ __global__ void k_zero_cols(int64_t L, int k, const int *__restrict__ zero, doub
 
 namespace hdfe {
 
-// Blocked layout of the non-split rows (see k_spmm_blk).
+// Blocked layout of the non-split rows (see k_spmm_blk); with slot (the
+// dense-long slot of each column, -1 for the others), a row's entries at the
+// split rows' columns are left out of the blocks.
 __global__ void k_blk_rowcount(int64_t L, const int *__restrict__ indptr,
                                const int *__restrict__ indices, const int *__restrict__ col_lo,
-                               int *__restrict__ cnt) {
+                               const int *__restrict__ slot, int *__restrict__ cnt) {
     for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < L;
          r += (int64_t)gridDim.x * blockDim.x) {
         int c[8] = {0, 0, 0, 0, 0, 0, 0, 0};
@@ -732,7 +785,7 @@ __global__ void k_blk_rowcount(int64_t L, const int *__restrict__ indptr,
             for (int p = s; p < e; ++p) {
                 const int col = indices[p];
                 while (b < 7 && col >= col_lo[b + 1]) ++b;
-                ++c[b];
+                if (!slot || slot[col] < 0) ++c[b];
             }
         }
 #pragma unroll
@@ -744,7 +797,9 @@ __global__ void k_blk_fill(int64_t L, const int *__restrict__ indptr,
                            const int *__restrict__ indices, const double *__restrict__ v64,
                            const float *__restrict__ v32, const int *__restrict__ col_lo,
                            const int *__restrict__ bptr, int *__restrict__ bidx,
-                           double *__restrict__ b64, float *__restrict__ b32) {
+                           double *__restrict__ b64, float *__restrict__ b32,
+                           const int *__restrict__ slot, double *__restrict__ dt64,
+                           float *__restrict__ dt32) {
     for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < L;
          r += (int64_t)gridDim.x * blockDim.x) {
         const int s = indptr[r], e = indptr[r + 1];
@@ -757,10 +812,34 @@ __global__ void k_blk_fill(int64_t L, const int *__restrict__ indptr,
                 ++b;
                 q = bptr[(int64_t)b * L + r];
             }
+            if (slot && slot[col] >= 0) {
+                dt64[(int64_t)slot[col] * L + r] = v64[p];
+                dt32[(int64_t)slot[col] * L + r] = v32[p];
+                continue;
+            }
             bidx[q] = col;
             b64[q] = v64[p];
             b32[q] = v32[p];
             ++q;
+        }
+    }
+}
+
+// The split rows' block among themselves: ll[t, slot(col)] = S[long t, col]
+// for the split rows' entries at split rows' columns (a workgroup per row).
+__global__ void k_ll_fill(int nlong, const int *__restrict__ long_rows,
+                          const int *__restrict__ indptr, const int *__restrict__ indices,
+                          const double *__restrict__ v64, const float *__restrict__ v32,
+                          const int *__restrict__ slot, double *__restrict__ ll64,
+                          float *__restrict__ ll32) {
+    const int t = blockIdx.x;
+    if (t >= nlong) return;
+    const int r = long_rows[t];
+    for (int p = indptr[r] + threadIdx.x; p < indptr[r + 1]; p += blockDim.x) {
+        const int sl = slot[indices[p]];
+        if (sl >= 0) {
+            ll64[t * nlong + sl] = v64[p];
+            ll32[t * nlong + sl] = v32[p];
         }
     }
 }
@@ -822,6 +901,14 @@ struct Matrix {
     double *bval64 = nullptr;
     float *bval32 = nullptr;
     int64_t bnnz = 0;
+    // dense long columns: the split rows taken by symmetry in the blocked SpMM
+    bool dl = false;
+    int *dl_slot = nullptr, *dl_cptr = nullptr;  // (L) slot or -1; (kSmallK, nlong + 1)
+    double *dt64 = nullptr, *ll64 = nullptr;     // (nlong, L) column-major; (nlong, nlong)
+    float *dt32 = nullptr, *ll32 = nullptr;
+    // the blocked SpMM's row parts per XCD, chosen by timing (0: the default),
+    // per precision (0: FP64, 1: FP32) and k
+    int blk_parts[2][kSmallK + 1] = {};
     hipStream_t stream = nullptr;  // the solver's own stream
     hipEvent_t ev = nullptr;
     Work w;
@@ -837,6 +924,8 @@ struct Matrix {
 };
 
 static int build_blocked(Matrix &M, const std::vector<int> &ip32);
+static int spmm_grid(int64_t L, int k);
+static int tune_blocked_both(Matrix &M, int k);
 
 static int alloc_work(Matrix &M, int k) {
     // the blocked layout for small k, once, when P outgrows an XCD's L2
@@ -862,7 +951,8 @@ static int alloc_work(Matrix &M, int k) {
     HIP_TRY(hipMalloc(&W.P32, Lk * 4));
     HIP_TRY(hipMalloc(&W.AP32, Lk * 4));
     HIP_TRY(hipMalloc(&W.part, (int64_t)4096 * 2 * k * 8));
-    HIP_TRY(hipMalloc(&W.lpart, std::max<int64_t>(1, (int64_t)M.nchunk * k) * 8));
+    const int64_t dlp = M.dl ? (int64_t)M.nlong * spmm_grid(M.L, k) : 0;
+    HIP_TRY(hipMalloc(&W.lpart, std::max<int64_t>(1, std::max<int64_t>(M.nchunk, dlp) * k) * 8));
     HIP_TRY(hipMalloc(&W.colbuf, (int64_t)k * 8));
     if (M.blk && k <= kSmallK) HIP_TRY(hipMalloc(&W.gpart, (int64_t)8 * Lk * 8));
     HIP_TRY(hipMalloc(&W.sd, (int64_t)6 * k * 8));
@@ -872,7 +962,7 @@ static int alloc_work(Matrix &M, int k) {
     HIP_TRY(hipMemset(W.sd, 0, (int64_t)6 * k * 8));
     W.k = k;
     W.L = M.L;
-    return HDFE_STATUS_OK;
+    return tune_blocked_both(M, k);
 }
 
 static CgState make_state(Work &W, int k, double tol, int maxiter) {
@@ -906,6 +996,18 @@ template <>
 const double *blk_values<double>(const Matrix &M) { return M.bval64; }
 template <>
 const float *blk_values<float>(const Matrix &M) { return M.bval32; }
+template <typename T>
+static const T *dl_values(const Matrix &M);
+template <>
+const double *dl_values<double>(const Matrix &M) { return M.dt64; }
+template <>
+const float *dl_values<float>(const Matrix &M) { return M.dt32; }
+template <typename T>
+static const T *ll_values(const Matrix &M);
+template <>
+const double *ll_values<double>(const Matrix &M) { return M.ll64; }
+template <>
+const float *ll_values<float>(const Matrix &M) { return M.ll32; }
 
 // The blocked copy of the non-split rows (k_spmm_blk): eight column blocks
 // of about equal nonzero counts, on the matrix's stream.  S is symmetric, so
@@ -916,16 +1018,52 @@ static int build_blocked(Matrix &M, const std::vector<int> &ip32) {
     Temp tmp;
     const int64_t total = (int64_t)ip32[L] - ip32[0];
     if (total == 0) return HDFE_STATUS_OK;
-    // col_lo[b] = the first column whose prefix count reaches b / 8 of the total
+    // the split rows as dense long columns: few of them, each coupled to most
+    // of the other rows (a small dimension's levels: years)
+    std::vector<int> lr(M.nlong), slot_h(L, -1);
+    int64_t long_nnz = 0;
+    if (M.nlong) {
+        HIP_TRY(hipMemcpy(lr.data(), M.long_rows, M.nlong * 4, hipMemcpyDeviceToHost));
+        for (int t = 0; t < M.nlong; ++t) {
+            slot_h[lr[t]] = t;
+            long_nnz += (int64_t)ip32[lr[t] + 1] - ip32[lr[t]];
+        }
+    }
+    M.dl = M.nlong >= 1 && M.nlong <= kDenseLongMax &&
+           4 * long_nnz >= (int64_t)M.nlong * (L - M.nlong) && (int64_t)M.nlong * L < INT32_MAX;
+    // col_lo[b] = the first column whose prefix count reaches b / 8 of the
+    // total (S is symmetric: a column's count is its row's length); dense
+    // long columns are not in the blocks
+    int64_t btotal = total;
+    if (M.dl) btotal -= long_nnz;
     M.col_lo_h[0] = 0;
     int64_t acc = 0, c = 0;
     for (int b = 1; b < 8; ++b) {
-        const int64_t target = total * b / 8;
+        const int64_t target = btotal * b / 8;
         while (c < L && acc < target) {
-            acc += (int64_t)ip32[c + 1] - ip32[c];
+            if (!(M.dl && slot_h[c] >= 0)) acc += (int64_t)ip32[c + 1] - ip32[c];
             ++c;
         }
         M.col_lo_h[b] = (int)c;
+    }
+    if (M.dl) {
+        // per k <= kSmallK, the combine's partials of split row t start at t * grid
+        std::vector<int> cptr((int64_t)kSmallK * (M.nlong + 1));
+        for (int k = 1; k <= kSmallK; ++k)
+            for (int t = 0; t <= M.nlong; ++t)
+                cptr[(k - 1) * (M.nlong + 1) + t] = t * spmm_grid(L, k);
+        HIP_TRY(hipMalloc(&M.dl_cptr, cptr.size() * 4));
+        HIP_TRY(hipMemcpyAsync(M.dl_cptr, cptr.data(), cptr.size() * 4, hipMemcpyHostToDevice, ms));
+        HIP_TRY(hipMalloc(&M.dl_slot, L * 4));
+        HIP_TRY(hipMemcpyAsync(M.dl_slot, slot_h.data(), L * 4, hipMemcpyHostToDevice, ms));
+        HIP_TRY(hipMalloc(&M.dt64, (int64_t)M.nlong * L * 8));
+        HIP_TRY(hipMalloc(&M.dt32, (int64_t)M.nlong * L * 4));
+        HIP_TRY(hipMalloc(&M.ll64, (int64_t)M.nlong * M.nlong * 8));
+        HIP_TRY(hipMalloc(&M.ll32, (int64_t)M.nlong * M.nlong * 4));
+        HIP_TRY(hipMemsetAsync(M.dt64, 0, (int64_t)M.nlong * L * 8, ms));
+        HIP_TRY(hipMemsetAsync(M.dt32, 0, (int64_t)M.nlong * L * 4, ms));
+        HIP_TRY(hipMemsetAsync(M.ll64, 0, (int64_t)M.nlong * M.nlong * 8, ms));
+        HIP_TRY(hipMemsetAsync(M.ll32, 0, (int64_t)M.nlong * M.nlong * 4, ms));
     }
     M.col_lo_h[8] = (int)L;
     HIP_TRY(hipMalloc(&M.col_lo, 9 * 4));
@@ -934,7 +1072,7 @@ static int build_blocked(Matrix &M, const std::vector<int> &ip32) {
     DEV_ALLOC(cnt, 8 * L + 1, ms);
     HIP_TRY(hipMemsetAsync(cnt.p + 8 * L, 0, 4, ms));
     hipLaunchKernelGGL(k_blk_rowcount, dim3(grid1(L)), dim3(kThreads), 0, ms, L, M.indptr,
-                       M.indices, M.col_lo, cnt.p);
+                       M.indices, M.col_lo, M.dl ? (const int *)M.dl_slot : nullptr, cnt.p);
     HIP_TRY(hipMalloc(&M.bptr, (8 * L + 1) * 4));
     HIP_TRY(exclusive_sum(tmp, cnt.p, M.bptr, 8 * L + 1, ms));
     int last = 0;
@@ -944,7 +1082,11 @@ static int build_blocked(Matrix &M, const std::vector<int> &ip32) {
     HIP_TRY(hipMalloc(&M.bval64, std::max<int64_t>(M.bnnz, 1) * 8));
     HIP_TRY(hipMalloc(&M.bval32, std::max<int64_t>(M.bnnz, 1) * 4));
     hipLaunchKernelGGL(k_blk_fill, dim3(grid1(L)), dim3(kThreads), 0, ms, L, M.indptr, M.indices,
-                       M.val64, M.val32, M.col_lo, M.bptr, M.bidx, M.bval64, M.bval32);
+                       M.val64, M.val32, M.col_lo, M.bptr, M.bidx, M.bval64, M.bval32,
+                       M.dl ? (const int *)M.dl_slot : nullptr, M.dt64, M.dt32);
+    if (M.dl)
+        hipLaunchKernelGGL(k_ll_fill, dim3(M.nlong), dim3(kThreads), 0, ms, M.nlong, M.long_rows,
+                           M.indptr, M.indices, M.val64, M.val32, M.dl_slot, M.ll64, M.ll32);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(ms));
     M.blk = true;
@@ -954,7 +1096,7 @@ static int build_blocked(Matrix &M, const std::vector<int> &ip32) {
 // AP = S P (and, with with_dot, pAp sums and alpha).  `st.done` may be null.
 template <typename T>
 static void launch_spmm(Matrix &M, const T *val, const T *P, T *AP, double *part, double *lpart,
-                        int with_dot, CgState st, hipStream_t s) {
+                        int with_dot, CgState st, hipStream_t s, int force_parts = 0) {
     const int k = st.k;
     const int g = spmm_grid(M.L, k);
     if (k <= kSmallK && M.blk && M.L * (int64_t)k * (int64_t)sizeof(T) > kL2Budget &&
@@ -973,11 +1115,17 @@ static void launch_spmm(Matrix &M, const T *val, const T *P, T *AP, double *part
         }
         const double per_row = (double)M.bnnz / ((double)std::max<int64_t>(M.L, 1) * ngrp);
         const int G = per_row < 6.0 ? 1 : (per_row < 12.0 ? 2 : 4);
-        const int nblk = 8 * kBlkParts;
+        const int tuned = M.blk_parts[sizeof(T) == 4 ? 1 : 0][k];
+        const int nblk = 8 * (force_parts ? force_parts : (tuned ? tuned : kBlkParts));
         const T *bval = blk_values<T>(M);
         T *gp = (T *)M.w.gpart;
+        // dense long columns: the combine's partials of the split rows, g per row
+        const int nld = M.dl ? M.nlong : 0;
+        const T *dt = dl_values<T>(M), *ll = M.dl ? ll_values<T>(M) : nullptr;
+        const int *lcp = M.dl ? M.dl_cptr + (k - 1) * (M.nlong + 1) : M.long_cptr;
+        const int nchunk = M.dl ? 0 : M.nchunk;
 #define HDFE_BLK(KK, GG)                                                                          \
-    hipLaunchKernelGGL((k_spmm_blk<T, KK, GG>), dim3(nblk + M.nchunk), dim3(kThreads), 0, s,      \
+    hipLaunchKernelGGL((k_spmm_blk<T, KK, GG>), dim3(nblk + nchunk), dim3(kThreads), 0, s,        \
                        M.bptr, M.bidx, bval, M.L, ngrp, nblk, P, gp, M.indptr, M.indices, val,   \
                        M.chunk_row, M.chunk_beg, lpart, st)
 #define HDFE_BLKG(KK)                   \
@@ -986,10 +1134,10 @@ static void launch_spmm(Matrix &M, const T *val, const T *P, T *AP, double *part
         else if (G == 2) HDFE_BLK(KK, 2); \
         else HDFE_BLK(KK, 4);           \
         hipLaunchKernelGGL((k_spmm_combine<T, KK>), dim3(g), dim3(kThreads), 0, s, M.L, ngrp, gp, \
-                           P, AP, part, with_dot, M.long_rows, M.long_cptr, M.nlong, lpart, st); \
+                           P, AP, part, with_dot, nld, dt, M.long_rows, lpart, st);             \
         if (with_dot || M.nlong)                                                                  \
             hipLaunchKernelGGL((k_fin_small<T, KK>), dim3(1), dim3(kFinThreads), 0, s, P, AP, part, \
-                               g, with_dot, M.long_rows, M.long_cptr, M.nlong, lpart, st);        \
+                               g, with_dot, M.long_rows, lcp, M.nlong, lpart, st, ll);            \
         break;
         switch (k) {
             HDFE_BLKG(1) HDFE_BLKG(2) HDFE_BLKG(3) HDFE_BLKG(4)
@@ -1120,6 +1268,70 @@ static int read_max(const double *dev, int k, hipStream_t s, double *out,
     *out = m;
     if (all) *all = h;
     return HDFE_STATUS_OK;
+}
+
+// The blocked SpMM's launch shape by timing: the row parts per XCD (how many
+// workgroups share an XCD's column group) that give the shortest product on
+// this matrix, once per precision and k.  Only the row partition changes --
+// every row's sum, and every partial's order, is the same whatever is chosen
+// -- so the results are bit-identical for any choice.  Kept process-wide per
+// matrix shape, so repeated fits of one panel time it once.
+static std::mutex g_tune_mu;
+static std::map<std::array<int64_t, 6>, int> g_tune;
+constexpr int kTuneParts[] = {64, 128, 256, 512, 1024};
+
+template <typename T>
+static int tune_blocked(Matrix &M, int k) {
+    if (!(k <= kSmallK && M.blk && M.L * (int64_t)k * (int64_t)sizeof(T) > kL2Budget &&
+          M.w.gpart))
+        return HDFE_STATUS_OK;
+    const int prec = sizeof(T) == 4 ? 1 : 0;
+    const std::array<int64_t, 6> key = {M.L, M.nnz, M.bnnz, M.nlong, k, prec};
+    {
+        std::lock_guard<std::mutex> lock(g_tune_mu);
+        auto it = g_tune.find(key);
+        if (it != g_tune.end()) {
+            M.blk_parts[prec][k] = it->second;
+            return HDFE_STATUS_OK;
+        }
+    }
+    hipStream_t s = M.stream;
+    T *P = sizeof(T) == 4 ? (T *)M.w.P32 : (T *)M.w.P64;
+    T *AP = sizeof(T) == 4 ? (T *)M.w.AP32 : (T *)M.w.AP64;
+    const T *val = sizeof(T) == 4 ? (const T *)M.val32 : (const T *)M.val64;
+    HIP_TRY(hipMemsetAsync(P, 0, M.L * (int64_t)k * sizeof(T), s));
+    CgState cs = make_state(M.w, k, 0.0, 1);
+    cs.done = nullptr;
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    float best = 1e30f;
+    int choice = kBlkParts;
+    for (int parts : kTuneParts) {
+        launch_spmm<T>(M, val, P, AP, M.w.part, M.w.lpart, 1, cs, s, parts);
+        HIP_TRY(hipEventRecord(e0, s));
+        for (int r = 0; r < 3; ++r) launch_spmm<T>(M, val, P, AP, M.w.part, M.w.lpart, 1, cs, s, parts);
+        HIP_TRY(hipEventRecord(e1, s));
+        HIP_TRY(hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best) {
+            best = ms;
+            choice = parts;
+        }
+    }
+    HIP_TRY(hipEventDestroy(e0));
+    HIP_TRY(hipEventDestroy(e1));
+    HIP_TRY(hipGetLastError());
+    M.blk_parts[prec][k] = choice;
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    g_tune[key] = choice;
+    return HDFE_STATUS_OK;
+}
+
+static int tune_blocked_both(Matrix &M, int k) {
+    TRY(tune_blocked<double>(M, k));
+    return tune_blocked<float>(M, k);
 }
 
 // FP64 block PCG (core/solve.py:66-140).
@@ -1491,6 +1703,22 @@ int hdfe_csr_timing_read(void *h, double *ms, int64_t *launches) {
 // `reps` back-to-back launches of the same launch configuration as in the
 // solve (k right-hand sides, with the fused pAp sums), between two HIP events
 // on the solver's stream.  precision 0: FP64, 1: FP32.
+// The blocked SpMM's row parts per XCD for k columns in one precision (0 FP64,
+// 1 FP32): *parts gets the tuned choice (0: not tuned, the default of 128),
+// after set >= 1 overrides it.  The choice never changes a result.
+int hdfe_csr_blk_parts(void *h, int k, int precision, int set, int *parts) {
+    ARG_CHECK(h && parts && k >= 1 && k <= kSmallK && (precision == 0 || precision == 1),
+              "hdfe_csr_blk_parts: bad argument");
+    Matrix &M = *(Matrix *)h;
+    std::lock_guard<std::mutex> lock(M.mu);
+    if (set >= 1) {
+        ARG_CHECK(set <= 4096, "hdfe_csr_blk_parts: at most 4096 parts per XCD");
+        M.blk_parts[precision][k] = set;
+    }
+    *parts = M.blk_parts[precision][k];
+    return HDFE_STATUS_OK;
+}
+
 int hdfe_csr_spmm_time(void *h, int64_t k, int reps, int precision, double *avg_us) {
     ARG_CHECK(h != nullptr && avg_us != nullptr, "hdfe_csr_spmm_time: null argument");
     ARG_CHECK(k >= 1 && k <= kMaxK && reps >= 1, "hdfe_csr_spmm_time: bad k or reps");
@@ -1592,7 +1820,8 @@ void hdfe_destroy(void *h) {
     M->w.release();
     void *ps[] = {M->indptr, M->indices, M->val64, M->val32, M->dinv64, M->dinv32,
                   M->long_rows, M->long_cptr, M->chunk_row, M->chunk_beg, M->col_lo, M->bptr,
-                  M->bidx, M->bval64, M->bval32};
+                  M->bidx, M->bval64, M->bval32, M->dl_slot, M->dl_cptr, M->dt64, M->dt32,
+                  M->ll64, M->ll32};
     for (void *p : ps)
         if (p) (void)hipFree(p);
     if (M->ev) (void)hipEventDestroy(M->ev);

@@ -165,6 +165,14 @@ class DeviceCSR:
                                                   C.byref(us)), "hdfe_csr_spmm_time")
         return float(us.value)
 
+    def blk_parts(self, k: int, precision: str = "fp64", set: int = 0) -> int:
+        """The blocked SpMM's tuned row parts per XCD for k columns (0: not
+        tuned); `set` overrides it.  Results do not depend on it."""
+        out = C.c_int()
+        _lib.check(_lib.load().hdfe_csr_blk_parts(self._h, int(k), PRECISIONS[precision],
+                                                  int(set), C.byref(out)), "hdfe_csr_blk_parts")
+        return int(out.value)
+
     def timing_read(self) -> dict:
         ms = C.c_double()
         n = C.c_int64()

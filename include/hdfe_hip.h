@@ -83,6 +83,14 @@ int hdfe_csr_export(void *h, int64_t *indptr, int32_t *indices, double *data, do
 int hdfe_csr_timing(void *h, int enable);
 int hdfe_csr_timing_read(void *h, double *ms, int64_t *launches);
 
+/* The blocked small-k SpMM's row parts per XCD for k (1..8) columns in one
+ * precision (0 FP64, 1 FP32), as chosen by timing at the matrix's first use of
+ * that k (0: untuned, 128); set >= 1 overrides it.  Only the row partition
+ * depends on it, so results are bit-identical for every value.  No
+ * counterpart in the reference (its SpMM is scipy's / cuSPARSE's); for the
+ * launch autotuner's report and tests. */
+int hdfe_csr_blk_parts(void *h, int k, int precision, int set, int *parts);
+
 /* Average duration in microseconds of the solve's SpMM kernel on this
  * matrix (k right-hand sides, fused pAp sums): reps back-to-back launches
  * between two HIP events on the solver's stream; precision 0 FP64, 1 FP32.

@@ -159,7 +159,11 @@ def big_system():
     rd = np.repeat(np.arange(Ls), Ld)
     cd = Ls + np.tile(np.arange(Ld), Ls)
     vd = -rng.uniform(0.01, 0.1, len(rd))
-    B = sp.csr_matrix((np.concatenate([v, vd]), (np.concatenate([r, rd]), np.concatenate([c, cd]))),
+    # the long rows coupled among themselves too (as years are)
+    ra, ca = np.triu_indices(Ld, 1)
+    va = -rng.uniform(1.0, 5.0, len(ra))
+    B = sp.csr_matrix((np.concatenate([v, vd, va]),
+                       (np.concatenate([r, rd, Ls + ra]), np.concatenate([c, cd, Ls + ca]))),
                       shape=(L, L))
     S = (B + B.T).tocsr()
     S.setdiag(0.0)
@@ -195,3 +199,23 @@ def test_blocked_spmm_solve(oracle_mod, big_system, precision):
     if precision == "fp64":
         assert abs(itc - itd) <= 1
     assert np.max(np.abs(Xd - Xc)) <= 1e-8 * np.max(np.abs(Xc))
+
+
+@pytest.mark.parametrize("k", [1, 3])
+def test_autotuned_launch_shape_leaves_results_bit_identical(big_system, k):
+    """The blocked SpMM's row parts per XCD are chosen by timing; every choice
+    partitions the same per-row sums, so the products are bit-identical."""
+    from hdfe_stream_amd.rocm import DeviceCSR
+    S, dinv = big_system
+    A = DeviceCSR.from_scipy(S, dinv)
+    X = np.random.default_rng(k).standard_normal((S.shape[0], k))
+    first = A.matmat(X)
+    tuned = A.blk_parts(k, "fp64")
+    assert tuned in (64, 128, 256, 512, 1024)
+    for parts in (64, 1024, 100):
+        A.blk_parts(k, "fp64", set=parts)
+        np.testing.assert_array_equal(A.matmat(X), first)
+        A.blk_parts(k, "mixed", set=parts)
+        a32 = A.matmat_f32(X.astype(np.float32))
+        A.blk_parts(k, "mixed", set=128)
+        np.testing.assert_array_equal(A.matmat_f32(X.astype(np.float32)), a32)
