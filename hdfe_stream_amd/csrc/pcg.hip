@@ -59,6 +59,8 @@ struct CgState {
     int *active;                              // (k)
     int *done, *it, *it_final;                // scalars
     unsigned *counter;                        // unused (kept for the layout)
+    double *dtol;                             // tol, on the device (k_set_params)
+    int *dmaxiter;                            // maxiter, on the device
     int k, maxiter;
     double tol;
 };
@@ -595,6 +597,16 @@ __global__ __launch_bounds__(kThreads) void k_step(
     store_col_parts(rz, m, k, part + k, 2 * (int64_t)k);
 }
 
+// The solve's tolerance and iteration cap into the state, so that the
+// iterations' kernels take the same arguments for every solve (a captured
+// graph of them is replayed with new ones).
+__global__ void k_set_params(CgState st, double tol, int maxiter) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        *st.dtol = tol;
+        *st.dmaxiter = maxiter;
+    }
+}
+
 // One workgroup after k_step: rel = sqrt(rr) / bnorm, active = rel > tol, stop
 // when no column is active or at maxiter; beta = rz_new / rz
 // (core/solve.py:123-139), from the nw workgroups' sums.
@@ -609,16 +621,18 @@ __global__ __launch_bounds__(kFinThreads) void k_fin_step(const double *__restri
     reduce_parts(part, nw, k, 2 * k, s_rr);
     reduce_parts(part + k, nw, k, 2 * k, s_rz);
     __syncthreads();
+    const double tol = *st.dtol;
+    const int maxiter = *st.dmaxiter;
     for (int j = threadIdx.x; j < k; j += blockDim.x) {
         const double rel = sqrt(s_rr[j]) / st.bnorm[j];
         st.rel[j] = rel;
-        const int act = rel > st.tol;
+        const int act = rel > tol;
         st.active[j] = act;
         if (act) atomicOr(&any, 1);
     }
     __syncthreads();
     const int it = *st.it + 1;
-    if (any && it < st.maxiter) {
+    if (any && it < maxiter) {
         for (int j = threadIdx.x; j < k; j += blockDim.x) {
             const double rzn = s_rz[j], rzo = st.rz[j];
             st.beta[j] = st.active[j] ? rzn / (rzo != 0.0 ? rzo : 1.0) : 0.0;
@@ -628,7 +642,7 @@ __global__ __launch_bounds__(kFinThreads) void k_fin_step(const double *__restri
     __syncthreads();
     if (threadIdx.x == 0) {
         *st.it = it;
-        if (!any || it >= st.maxiter) {
+        if (!any || it >= maxiter) {
             *st.done = any ? 2 : 1;  // 1 converged, 2 hit maxiter
             *st.it_final = it;
         }
@@ -909,6 +923,15 @@ struct Matrix {
     // the blocked SpMM's row parts per XCD, chosen by timing (0: the default),
     // per precision (0: FP64, 1: FP32) and k
     int blk_parts[2][kSmallK + 1] = {};
+    // captured graphs of batches of CG iterations, per (buffers, k, precision,
+    // batch); a key's first use runs eagerly, the second captures
+    std::map<std::array<uintptr_t, 9>, hipGraphExec_t> cg_graphs;
+    std::map<std::array<uintptr_t, 9>, int> cg_seen;
+    void drop_graphs() {
+        for (auto &kv : cg_graphs) (void)hipGraphExecDestroy(kv.second);
+        cg_graphs.clear();
+        cg_seen.clear();
+    }
     hipStream_t stream = nullptr;  // the solver's own stream
     hipEvent_t ev = nullptr;
     Work w;
@@ -932,12 +955,14 @@ static int alloc_work(Matrix &M, int k) {
     if (k <= kSmallK && !M.blk_tried) {
         M.blk_tried = true;
         if (M.L * (int64_t)kSmallK * 8 > kL2Budget && M.L < (INT32_MAX - 1) / 8) {
+            M.drop_graphs();
             M.w.release();
             const int st = build_blocked(M, M.ip_host);
             if (st) return st;
         }
     }
     if (M.w.k == k && M.w.L == M.L) return HDFE_STATUS_OK;
+    M.drop_graphs();  // their kernels point at the work buffers
     M.w.release();
     const int64_t Lk = std::max<int64_t>(1, M.L * (int64_t)k);
     Work &W = M.w;
@@ -955,11 +980,11 @@ static int alloc_work(Matrix &M, int k) {
     HIP_TRY(hipMalloc(&W.lpart, std::max<int64_t>(1, std::max<int64_t>(M.nchunk, dlp) * k) * 8));
     HIP_TRY(hipMalloc(&W.colbuf, (int64_t)k * 8));
     if (M.blk && k <= kSmallK) HIP_TRY(hipMalloc(&W.gpart, (int64_t)8 * Lk * 8));
-    HIP_TRY(hipMalloc(&W.sd, (int64_t)6 * k * 8));
+    HIP_TRY(hipMalloc(&W.sd, ((int64_t)6 * k + 1) * 8));
     HIP_TRY(hipMalloc(&W.si, ((int64_t)3 * k + 8) * 4));
     HIP_TRY(hipMalloc(&W.cnt, 8 * sizeof(unsigned)));
     HIP_TRY(hipMemset(W.cnt, 0, 8 * sizeof(unsigned)));
-    HIP_TRY(hipMemset(W.sd, 0, (int64_t)6 * k * 8));
+    HIP_TRY(hipMemset(W.sd, 0, ((int64_t)6 * k + 1) * 8));
     W.k = k;
     W.L = M.L;
     return tune_blocked_both(M, k);
@@ -977,6 +1002,8 @@ static CgState make_state(Work &W, int k, double tol, int maxiter) {
     st.it = W.si + 3 * k + 1;
     st.it_final = W.si + 3 * k + 2;
     st.counter = W.cnt;
+    st.dtol = W.sd + 6 * k;
+    st.dmaxiter = W.si + 3 * k + 3;
     st.k = k;
     st.maxiter = maxiter;
     st.tol = tol;
@@ -1206,8 +1233,21 @@ static int run_cg(Matrix &M, int k, const T *val, const T *dinv, T *X, T *R, T *
                   CgState st, hipStream_t s, int64_t *it_out, int *conv_out) {
     Work &W = M.w;
     HIP_TRY(hipMemsetAsync(st.done, 0, 3 * sizeof(int), s));
+    hipLaunchKernelGGL(k_set_params, dim3(1), dim3(64), 0, s, st, st.tol, st.maxiter);
     const int gv = vec_grid(M.L, k);
     const int gd = grid_for(M.L * (int64_t)k, 1024, 1, 4096);
+    // one iteration: SpMM (+ pAp, alpha), step, its finish (rel, stop, beta),
+    // direction -- every kernel returns at once after the stop
+    auto iteration = [&](int b) {
+        if (M.timing) (void)hipEventRecord(M.evpool[2 * b], s);
+        launch_spmm<T>(M, val, P, AP, W.part, W.lpart, 1, st, s);
+        if (M.timing) (void)hipEventRecord(M.evpool[2 * b + 1], s);
+        hipLaunchKernelGGL((k_step<T>), dim3(gv), dim3(kThreads), 0, s, M.L, X, R, P, AP, dinv,
+                           W.part, st);
+        hipLaunchKernelGGL(k_fin_step, dim3(1), dim3(kFinThreads), 0, s, W.part, gv, st);
+        hipLaunchKernelGGL((k_direction<T>), dim3(gd), dim3(kThreads), 0, s, M.L, P, R, dinv,
+                           st);
+    };
     int host[2] = {0, 0};
     int batch = 8;
     int64_t done_before = 0;  // iterations launched before this batch
@@ -1219,15 +1259,33 @@ static int run_cg(Matrix &M, int k, const T *val, const T *dinv, T *X, T *R, T *
                 HIP_TRY(hipEventCreate(&e));
                 M.evpool.push_back(e);
             }
-        for (int b = 0; b < nb; ++b) {
-            if (M.timing) HIP_TRY(hipEventRecord(M.evpool[2 * b], s));
-            launch_spmm<T>(M, val, P, AP, W.part, W.lpart, 1, st, s);
-            if (M.timing) HIP_TRY(hipEventRecord(M.evpool[2 * b + 1], s));
-            hipLaunchKernelGGL((k_step<T>), dim3(gv), dim3(kThreads), 0, s, M.L, X, R, P, AP, dinv,
-                               W.part, st);
-            hipLaunchKernelGGL(k_fin_step, dim3(1), dim3(kFinThreads), 0, s, W.part, gv, st);
-            hipLaunchKernelGGL((k_direction<T>), dim3(gd), dim3(kThreads), 0, s, M.L, P, R, dinv,
-                               st);
+        hipGraphExec_t ex = nullptr;
+        if (!M.timing) {
+            // a batch of nb iterations as a graph: its kernels' arguments are
+            // fixed for these buffers (tol and maxiter are read on the device)
+            const std::array<uintptr_t, 9> key = {
+                (uintptr_t)val, (uintptr_t)dinv, (uintptr_t)X, (uintptr_t)R, (uintptr_t)P,
+                (uintptr_t)AP, (uintptr_t)k, sizeof(T), (uintptr_t)nb};
+            auto it = M.cg_graphs.find(key);
+            if (it != M.cg_graphs.end()) {
+                ex = it->second;
+            } else if (M.cg_seen[key]++ >= 1) {
+                hipGraph_t g = nullptr;
+                HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+                for (int b = 0; b < nb; ++b) iteration(b);
+                const hipError_t ce = hipStreamEndCapture(s, &g);
+                HIP_TRY(ce);
+                HIP_TRY(hipGetLastError());
+                const hipError_t ie = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(g);
+                HIP_TRY(ie);
+                M.cg_graphs[key] = ex;
+            }
+        }
+        if (ex) {
+            HIP_TRY(hipGraphLaunch(ex, s));
+        } else {
+            for (int b = 0; b < nb; ++b) iteration(b);
         }
         done_before = launched;
         launched += nb;
@@ -1817,6 +1875,7 @@ void hdfe_destroy(void *h) {
     Matrix *M = (Matrix *)h;
     (void)hipSetDevice(M->device);
     if (M->stream) (void)hipStreamSynchronize(M->stream);
+    M->drop_graphs();
     M->w.release();
     void *ps[] = {M->indptr, M->indices, M->val64, M->val32, M->dinv64, M->dinv32,
                   M->long_rows, M->long_cptr, M->chunk_row, M->chunk_beg, M->col_lo, M->bptr,

@@ -219,3 +219,38 @@ def test_autotuned_launch_shape_leaves_results_bit_identical(big_system, k):
         a32 = A.matmat_f32(X.astype(np.float32))
         A.blk_parts(k, "mixed", set=128)
         np.testing.assert_array_equal(A.matmat_f32(X.astype(np.float32)), a32)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_graph_replays_match_the_eager_solve(big_system, precision):
+    """Batches of CG iterations run eagerly on a key's first use, are captured
+    as a graph on the second and replayed after: every solve, with eager work
+    and synchronisations between them, is bit-identical to the eager one."""
+    import torch
+    from hdfe_stream_amd.rocm import DeviceCSR
+    S, dinv = big_system
+    A = DeviceCSR.from_scipy(S, dinv)
+    b = _rhs(S, 3, seed=4)
+    X = np.random.default_rng(2).standard_normal((S.shape[0], 3))
+    outs, its = [], []
+    for r in range(4):   # eager, capture, replay, replay
+        Xs, it, ok, _ = A.solve(b, tol=1e-10, precision=precision)
+        assert ok
+        outs.append(Xs)
+        its.append(it)
+        A.matmat(X)                       # eager work between the replays
+        torch.cuda.synchronize()
+    for Xs, it in zip(outs[1:], its[1:]):
+        np.testing.assert_array_equal(Xs, outs[0])
+        assert it == its[0]
+    # the eager path with per-launch timing gives the same bits again
+    A.timing(True)
+    Xt, itt, _, _ = A.solve(b, tol=1e-10, precision=precision)
+    A.timing(False)
+    np.testing.assert_array_equal(Xt, outs[0])
+    # another tolerance on the same graphs: the device-side parameters (the
+    # mixed solve's FP32 inner tolerance may already reach it in one step)
+    X8, it8, ok8, rel8 = A.solve(b, tol=1e-6, precision=precision)
+    assert ok8 and rel8 <= 1e-6
+    if precision == "fp64":
+        assert it8 < its[0]
