@@ -840,16 +840,19 @@ __global__ void k_blk_fill(int64_t L, const int *__restrict__ indptr,
 }
 
 // The split rows' block among themselves: ll[t, slot(col)] = S[long t, col]
-// for the split rows' entries at split rows' columns (a workgroup per row).
+// for the split rows' entries at split rows' columns (kLlParts workgroups per
+// row, each a stride of its entries).
+constexpr int kLlParts = 64;
 __global__ void k_ll_fill(int nlong, const int *__restrict__ long_rows,
                           const int *__restrict__ indptr, const int *__restrict__ indices,
                           const double *__restrict__ v64, const float *__restrict__ v32,
                           const int *__restrict__ slot, double *__restrict__ ll64,
                           float *__restrict__ ll32) {
-    const int t = blockIdx.x;
+    const int t = blockIdx.x / kLlParts, part = blockIdx.x % kLlParts;
     if (t >= nlong) return;
     const int r = long_rows[t];
-    for (int p = indptr[r] + threadIdx.x; p < indptr[r + 1]; p += blockDim.x) {
+    for (int p = indptr[r] + part * blockDim.x + threadIdx.x; p < indptr[r + 1];
+         p += kLlParts * blockDim.x) {
         const int sl = slot[indices[p]];
         if (sl >= 0) {
             ll64[t * nlong + sl] = v64[p];
@@ -1112,8 +1115,8 @@ static int build_blocked(Matrix &M, const std::vector<int> &ip32) {
                        M.val64, M.val32, M.col_lo, M.bptr, M.bidx, M.bval64, M.bval32,
                        M.dl ? (const int *)M.dl_slot : nullptr, M.dt64, M.dt32);
     if (M.dl)
-        hipLaunchKernelGGL(k_ll_fill, dim3(M.nlong), dim3(kThreads), 0, ms, M.nlong, M.long_rows,
-                           M.indptr, M.indices, M.val64, M.val32, M.dl_slot, M.ll64, M.ll32);
+        hipLaunchKernelGGL(k_ll_fill, dim3(M.nlong * kLlParts), dim3(kThreads), 0, ms, M.nlong,
+                           M.long_rows, M.indptr, M.indices, M.val64, M.val32, M.dl_slot, M.ll64, M.ll32);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(ms));
     M.blk = true;
