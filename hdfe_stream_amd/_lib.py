@@ -60,8 +60,8 @@ SIGNATURES = [
     ("hdfe_loo_prune", I32, [I64, P, P, P, P]),
     ("hdfe_components", I32, [I64, P, P, I32, I32, I32, I32, P]),
     ("hdfe_csr_blk_parts", I32, [P, I32, I32, I32, P]),
-    ("hdfe_se_stack", I32, [I64, I32, P, P, P, P, P]),
-    ("hdfe_se_assemble", I32, [I64, I32, I32, P, P, P, P, P, P, P, P, D, P, P, P, P]),
+    ("hdfe_se_stack", I32, [I64, I32, P, P, P, P, I32, P]),
+    ("hdfe_se_assemble", I32, [I64, I32, I32, P, P, P, P, P, P, P, P, D, P, P, P, P, P, I32, P]),
     ("hdfe_se_bii", I32, [I64, I32, P, P, P]),
     ("hdfe_se_cross_groups", I32, [I64, I32, P, P, I64, P, P, P]),
     ("hdfe_se_cell_sums", I32, [I64, P, P, P, P, P, I32, P, P]),

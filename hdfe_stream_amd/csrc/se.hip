@@ -27,11 +27,11 @@ constexpr int kSeBlocks = 1024;  // workgroups of the reducing passes
 
 // The stacked inputs of the first solve (leaveout_se.py:522-530): with a
 // shared input v (n, s): [v, b_0 v, b_1 v, b_2 v]; per component (z != null):
-// v_k = z sqrt(max(s_k, 0)) and [v_0, v_1, v_2, b_0 v_0, b_1 v_1, b_2 v_2].
-// b and s3 are (3, n).
+// v_k = z sqrt(max(s_k, 0)) -- zero for a component not in the bit mask --
+// and [v_0, v_1, v_2, b_0 v_0, b_1 v_1, b_2 v_2].  b and s3 are (3, n).
 __global__ void k_se_stack(int64_t n, int s, const double *__restrict__ v,
                            const double *__restrict__ z, const double *__restrict__ s3,
-                           const double *__restrict__ b, double *__restrict__ out) {
+                           const double *__restrict__ b, int mask, double *__restrict__ out) {
     const int nin = z ? 3 : 1, wide = nin + 3;
     const int64_t total = n * s;
     for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
@@ -42,7 +42,8 @@ __global__ void k_se_stack(int64_t n, int s, const double *__restrict__ v,
         if (z) {
             const double zz = z[e];
 #pragma unroll
-            for (int k = 0; k < 3; ++k) vk[k] = zz * sqrt(fmax(s3[k * n + i], 0.0));
+            for (int k = 0; k < 3; ++k)
+                vk[k] = ((mask >> k) & 1) ? zz * sqrt(fmax(s3[k * n + i], 0.0)) : 0.0;
         } else {
             vk[0] = vk[1] = vk[2] = v[e];
         }
@@ -57,19 +58,25 @@ __global__ void k_se_stack(int64_t n, int s, const double *__restrict__ v,
 //   M v    = v_k - sqrt(w) base[in(k)],    B v = sqrt(w) thq[k]
 //   M(b v) = b_k v_k - sqrt(w) base[b(k)]
 //   C v    = B v - (b_k M v + M(b v)) / 2  [+ c (u_k'v 1 + (1'v) u_k)]
+// With x1 (3, n) and defl = (lambda (3), x1_k'v_k (3, s)), B v loses
+// lambda_k x1_k (x1_k'v_k): C2 of the deflated B2 (leaveout_se.py:579-580).
 // mode 0 (C y, s = 1): part[w] = (4 s_k (Cy)_k^2, yt (Cy)_k) for k < 3;
-// mode 1 (trace): part[w] = (s_k sum_j (C v)_kj^2) for k < 3.
+// mode 1 (trace): part[w] = (s_k sum_j (C v)_kj^2) for k < 3;
+// mode 2 (C2 y, s = 1, s floored at 0): (2 x1_k s_k (C2y)_k, 4 s_k (C2y)_k^2,
+//   yt (C2y)_k) for k < 3; mode 3 (trace, s floored at 0): as mode 1.
 // base (n, wide s), thq (n, 3 s); u (3, n), udot/odot (3, s) or null.
 __global__ __launch_bounds__(256) void k_se_assemble(
     int64_t n, int s, int mode, const double *__restrict__ v, const double *__restrict__ z,
     const double *__restrict__ s3, const double *__restrict__ b, const double *__restrict__ w,
     const double *__restrict__ base, const double *__restrict__ thq, const double *__restrict__ u,
     double c2, const double *__restrict__ udot, const double *__restrict__ odot,
-    const double *__restrict__ yt, double *__restrict__ part) {
+    const double *__restrict__ yt, const double *__restrict__ x1,
+    const double *__restrict__ defl, int mask, double *__restrict__ part) {
     __shared__ double lds[256 / 64];
     const int nin = z ? 3 : 1, wide = nin + 3;
     const int64_t r0 = part_begin(n, blockIdx.x, gridDim.x), r1 = part_begin(n, blockIdx.x + 1, gridDim.x);
-    double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    double acc[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    const bool floor0 = mode >= 2;
     const int64_t total = (r1 - r0) * s;
     for (int64_t e = threadIdx.x; e < total; e += blockDim.x) {
         const int64_t i = r0 + e / s;
@@ -80,23 +87,30 @@ __global__ __launch_bounds__(256) void k_se_assemble(
         const double zz = z ? z[i * s + j] : 0.0;
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-            const double sk = s3[k * n + i];
-            const double vk = z ? zz * sqrt(fmax(sk, 0.0)) : v[i * s + j];
+            const double sr = s3[k * n + i];
+            const double sk = floor0 ? fmax(sr, 0.0) : sr;
+            const bool on = (mask >> k) & 1;
+            const double vk = z ? (on ? zz * sqrt(fmax(sr, 0.0)) : 0.0) : v[i * s + j];
             const double bk = b[k * n + i];
             const double Mv = vk - root * bs[(z ? k : 0) * s + j];
-            const double Bv = root * tq[k * s + j];
+            double Bv = root * tq[k * s + j];
+            if (x1) Bv -= defl[k] * x1[k * n + i] * defl[3 + k * s + j];
             const double Mbv = vk * bk - root * bs[(nin + k) * s + j];
             double cv = Bv - 0.5 * (bk * Mv + Mbv);
             if (u) cv += c2 * (udot[k * s + j] + u[k * n + i] * odot[k * s + j]);
             if (mode == 0) {
                 acc[k] += 4.0 * sk * cv * cv;
                 acc[3 + k] += yt[i] * cv;
-            } else {
+            } else if (mode == 2) {
+                acc[k] += 2.0 * x1[k * n + i] * sk * cv;
+                acc[3 + k] += 4.0 * sk * cv * cv;
+                acc[6 + k] += yt[i] * cv;
+            } else if (on) {
                 acc[k] += sk * cv * cv;
             }
         }
     }
-    const int nout = mode == 0 ? 6 : 3;
+    const int nout = mode == 0 ? 6 : mode == 2 ? 9 : 3;
     for (int t = 0; t < nout; ++t) {
         const double r = block_sum<double>(acc[t], lds);
         if (threadIdx.x == 0) part[(int64_t)blockIdx.x * nout + t] = r;
@@ -241,30 +255,35 @@ using namespace hdfe;
 
 extern "C" {
 
-// The stacked first-solve input (see k_se_stack); out (n, (nin + 3) s).
+// The stacked first-solve input (see k_se_stack); out (n, (nin + 3) s); mask
+// the components with draws (bits 0-2).
 int hdfe_se_stack(int64_t n, int s, const double *v, const double *z, const double *s3,
-                  const double *b, double *out) {
+                  const double *b, int mask, double *out) {
     ARG_CHECK(n >= 1 && s >= 1 && b && out && (v || (z && s3)), "hdfe_se_stack: bad argument");
-    hipLaunchKernelGGL(k_se_stack, dim3(grid1(n * s)), dim3(256), 0, 0, n, s, v, z, s3, b, out);
+    hipLaunchKernelGGL(k_se_stack, dim3(grid1(n * s)), dim3(256), 0, 0, n, s, v, z, s3, b, mask,
+                       out);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipDeviceSynchronize());
     return HDFE_STATUS_OK;
 }
 
 // C v reduced (see k_se_assemble): out (host) = mode 0: first (3), theta (3);
-// mode 1: sum_i s_k,i sum_j (C v)_kij^2 (3).
+// mode 1: sum_i s_k,i sum_j (C v)_kij^2 (3); mode 2: s12 (3), first (3),
+// quad (3); mode 3: as mode 1 with s floored, components in mask.
 int hdfe_se_assemble(int64_t n, int s, int mode, const double *v, const double *z,
                      const double *s3, const double *b, const double *w, const double *base,
                      const double *thq, const double *u, double c2, const double *udot,
-                     const double *odot, const double *yt, double *out) {
-    ARG_CHECK(n >= 1 && s >= 1 && (mode == 0 || mode == 1) && s3 && b && base && thq && out &&
-                  (v || z) && (mode == 1 || yt) && (!u || (udot && odot)),
+                     const double *odot, const double *yt, const double *x1, const double *defl,
+                     int mask, double *out) {
+    ARG_CHECK(n >= 1 && s >= 1 && mode >= 0 && mode <= 3 && s3 && b && base && thq && out &&
+                  (v || z) && ((mode & 1) || yt) && (!u || (udot && odot)) &&
+                  (!x1 == !defl) && (mode != 2 || x1),
               "hdfe_se_assemble: bad argument");
-    const int nb = (int)std::min<int64_t>(kSeBlocks, n), nout = mode == 0 ? 6 : 3;
+    const int nb = (int)std::min<int64_t>(kSeBlocks, n), nout = mode == 0 ? 6 : mode == 2 ? 9 : 3;
     DevBuf<double> part;
     DEV_ALLOC(part, (int64_t)nb * nout, nullptr);
     hipLaunchKernelGGL(k_se_assemble, dim3(nb), dim3(256), 0, 0, n, s, mode, v, z, s3, b, w,
-                       base, thq, u, c2, udot, odot, yt, part.p);
+                       base, thq, u, c2, udot, odot, yt, x1, defl, mask, part.p);
     HIP_TRY(hipGetLastError());
     std::vector<double> h;
     HIP_TRY(to_host(h, part.p, (int64_t)nb * nout, nullptr));

@@ -383,12 +383,53 @@ class LeaveOutComponents:
         return dict(self.leave_out)
 
     def summary(self) -> str:
+        """The components; with standard errors, the normal intervals at the
+        run's confidence (flagged * where the diagnostic finds q >= 1) and
+        the q = 1 intervals (leaveout.py:519-627)."""
+        from scipy.special import ndtri
+        level = self.diagnostics.get("confidence", 0.95)
+        zn = float(ndtri(0.5 + level / 2.0))
         lines = [f"KSS leave-out components ({self.n_obs:,} observations, "
                  f"{self.n_movers:,} movers)",
-                 f"  {'':18s}{'plug-in':>12s}{'bias':>12s}{'leave-out':>12s}"]
+                 f"  {'':18s}{'plug-in':>12s}{'bias':>12s}{'leave-out':>12s}"
+                 + (f"{'se':>11s}{f'{level:.0%} interval':>28s}" if self.se is not None
+                    else "")]
         for k in self.leave_out:
-            lines.append(f"  {k:18s}{self.plug_in[k]:12.6f}{self.bias[k]:12.6f}"
-                         f"{self.leave_out[k]:12.6f}")
+            row = (f"  {k:18s}{self.plug_in[k]:12.6f}{self.bias[k]:12.6f}"
+                   f"{self.leave_out[k]:12.6f}")
+            if self.se is not None:
+                se = self.se.se[k]
+                row += f"{se:11.6f}"
+                if np.isfinite(se):
+                    weak = self.weak_id is not None and self.weak_id.q[k] >= 1
+                    lo, hi = self.leave_out[k] - zn * se, self.leave_out[k] + zn * se
+                    row += f"{f'[{lo:.6f}, {hi:.6f}]':>28s}" + ("*" if weak else "")
+            lines.append(row)
+        weak = (self.se.weak or {}) if self.se is not None else {}
+        shown = [k for k in self.leave_out if "interval" in weak.get(k, {})]
+        if self.se is not None and self.weak_id is not None and any(
+                self.weak_id.q[k] >= 1 for k in self.leave_out):
+            lines.append("  * q >= 1: weakly identified, the normal interval understates "
+                         "the uncertainty")
+        if shown:
+            lines += ["", f"q = 1 intervals, valid under weak identification "
+                          f"({weak[shown[0]]['confidence']:.0%})",
+                      f"  {'':18s}{'leave-out':>12s}{'interval':>28s}{'kappa':>9s}"
+                      f"{'z':>8s}{'vs normal':>11s}"]
+            for k in shown:
+                piece = weak[k]
+                lo, hi = piece["interval"]
+                normal = 2.0 * zn * self.se.se[k]
+                ratio = (hi - lo) / normal if np.isfinite(normal) and normal > 0 else np.nan
+                lines.append(f"  {k:18s}{self.leave_out[k]:12.6f}"
+                             f"{f'[{lo:.6f}, {hi:.6f}]':>28s}{piece['kappa']:9.3f}"
+                             f"{piece['z']:8.4f}{ratio:10.2f}x")
+            if any(piece.get("conservative") for piece in weak.values()):
+                lines.append("  V[theta1] fell back to its first term (conservative) for "
+                             "at least one component")
+            if any(self.weak_id.q[k] >= 2 for k in shown):
+                lines.append("  q >= 2 somewhere: the q = 1 interval ignores the second "
+                             "direction and may undercover")
         if self.se is not None:
             lines += ["", self.se.summary()]
         if self.weak_id is not None:
@@ -418,7 +459,8 @@ def leave_out_components(result, n_draws: int = 250, block: int | None = None, s
                          person_years: int | None = None, se: bool = False, se_draws=None,
                          se_trace: bool = True, smooth_bins=None,
                          se_variance: str = "person_year", match_within=None,
-                         diagnose=None, diagnose_draws: int = 64) -> LeaveOutComponents:
+                         diagnose=None, diagnose_draws: int = 64, weak_interval: bool = True,
+                         confidence: float = 0.95) -> LeaveOutComponents:
     """KSS components for a fit (leaveout.py:634-805): JLA leverages, the
     leverage check, the plug-in moments, sigma2, the Hutchinson bias with
     seed + 1_000_003, and plug-in minus bias.
@@ -438,7 +480,13 @@ def leave_out_components(result, n_draws: int = 250, block: int | None = None, s
     stayer's match gets alone), "match" takes it with the stayers' values; b
     is zero on stayers' matches and var(alpha) gets no standard error.
     `diagnose` (default: `se`) runs the weak-identification diagnostic with
-    `diagnose_draws` trace draws and seed + 9_900_019 (leaveout.py:749-758)."""
+    `diagnose_draws` trace draws and seed + 9_900_019 (leaveout.py:749-758).
+    With `se` and `weak_interval`, each component it flags (q >= 1) also
+    gets KSS's q = 1 interval at `confidence` in se.weak[name]["interval"]:
+    the Andrews-Mikusheva projection of the ellipse of (b1, theta1), theta1
+    = leave-out - lambda_1 (b1^2 - V[b1] raw) (leaveout.py:777-793)."""
+    if not 0.0 < confidence < 1.0:
+        raise ValueError("confidence must lie in (0, 1)")
     if stayers not in ("own", "within_match"):
         raise ValueError("stayers must be 'own' or 'within_match' on the device")
     if (stayers == "within_match") != (stayer_sigma2 is not None):
@@ -550,14 +598,25 @@ def leave_out_components(result, n_draws: int = 250, block: int | None = None, s
                 wt = panel.tensor("w", torch.float64)
                 mw = match_within.to(torch.float64)
                 sigma2_se = torch.where(movers, mw + blk / wt, mw)
-            options = {"stayer_rows": ~movers, "centering": centering, "reported": (0, 2)}
+            options = {"stayer_rows": ~movers, "centering": centering, "reported": (0, 2),
+                       "sigma2_point": sigma2}
         errors = component_variances(
             inv, y, sigma2_se.contiguous(), lev, psi=pi, rows=R,
             n_draws=n_draws if se_draws is None else int(se_draws), block=block,
-            seed=seed + BII_SEED_OFFSET, trace=se_trace, smooth_bins=smooth_bins, **options)
+            seed=seed + BII_SEED_OFFSET, trace=se_trace, smooth_bins=smooth_bins,
+            weak_id=weak_id if weak_interval else None, **options)
         errors.diagnostics.update(psi=psi, alpha=alpha, covariates=[],
                                   se_variance=se_variance if match_level else None,
                                   sigma2=sigma2_se)
+        if errors.weak:
+            from .am_interval import am_interval
+            for name, piece in errors.weak.items():
+                theta1 = leave_out[name] - piece["lambda_1"] * (piece["b1"] ** 2
+                                                                - piece["v_raw"])
+                lo, hi, extra = am_interval(piece["lambda_1"], piece["b1"], theta1,
+                                            piece["sigma"], 1.0 - confidence)
+                piece.update(theta1=float(theta1), interval=(lo, hi), confidence=confidence,
+                             **extra)
     t4 = time.perf_counter()
     t.update(leverage_seconds=t1 - t0, sigma2_seconds=t2 - t1, bias_seconds=t3 - t2,
              se_seconds=t4 - t3)
@@ -565,6 +624,7 @@ def leave_out_components(result, n_draws: int = 250, block: int | None = None, s
         plug_in=plug_in, bias=bias, leave_out=leave_out, n_obs=n,
         n_movers=int(movers.sum()), leverages=lev, trace=trace, se=errors, weak_id=weak_id,
         diagnostics={"psi": psi, "alpha": alpha, "stayers": stayers, "max_leverage": worst,
+                     "confidence": confidence,
                      "leave_out": "match" if match_level else "observation",
                      "centering": centering, "sigma2_mean": sigma2_mean,
                      "ybar": float(ybar.value), "covariates": [],
@@ -574,7 +634,8 @@ def leave_out_components(result, n_draws: int = 250, block: int | None = None, s
 
 def _leave_out_match(fit, psi, n_draws, block, seed, tol, maxiter, precision, timings,
                      se=False, se_draws=None, se_trace=True, smooth_bins=None,
-                     se_variance="person_year", diagnose=None, diagnose_draws=64):
+                     se_variance="person_year", diagnose=None, diagnose_draws=64,
+                     weak_interval=True, confidence=0.95):
     """KSS leaving out a whole (worker, firm) match (leaveout_match.py:
     156-214): the row fit's y~ = the worker effect + the firm effect + the
     residual collapsed to one row per match (its weighted mean, weight the
@@ -653,7 +714,8 @@ def _leave_out_match(fit, psi, n_draws, block, seed, tol, maxiter, precision, ti
                                  centering="reference", person_years=n, se=se,
                                  se_draws=se_draws, se_trace=se_trace, smooth_bins=smooth_bins,
                                  se_variance=se_variance, match_within=within,
-                                 diagnose=diagnose, diagnose_draws=diagnose_draws)
+                                 diagnose=diagnose, diagnose_draws=diagnose_draws,
+                                 weak_interval=weak_interval, confidence=confidence)
     comps.n_obs = n
     comps.diagnostics["n_matches"] = mn
     comps.diagnostics["person_years"] = n
@@ -668,7 +730,8 @@ def leave_out_kss(fml: str, data, *, psi: str | None = None, n_draws: int = 250,
                   precision: str = "mixed", tol: float = 1e-10, maxiter: int = 5000,
                   timings: dict | None = None, se: bool = False, se_draws=None,
                   se_trace: bool = True, smooth_bins=None, se_variance: str = "person_year",
-                  diagnose=None, diagnose_draws: int = 64, **options) -> LeaveOutComponents:
+                  diagnose=None, diagnose_draws: int = 64, weak_interval: bool = True,
+                  confidence: float = 0.95, **options) -> LeaveOutComponents:
     """Kline-Saggio-Solvsten leave-out variance components, end to end, on
     the device (leaveout.py:1069-1255): prune to the largest leave-one-out
     connected set of (alpha, psi), fit with alpha streamed and no singleton
@@ -685,7 +748,9 @@ def leave_out_kss(fml: str, data, *, psi: str | None = None, n_draws: int = 250,
     ("person_year", the reference implementation's, unweighted only, or
     "match") and var(alpha) gets none (leaveout.py:1136-1170).  `diagnose`
     (default: `se`) and `diagnose_draws` run the weak-identification
-    diagnostic (leaveout_weakid.py)."""
+    diagnostic (leaveout_weakid.py); with `se` and `weak_interval` every
+    component it flags gets the q = 1 interval at `confidence`, the level of
+    the normal intervals too (leaveout.py:1144-1158)."""
     import torch
 
     from .connected import leave_one_out
@@ -750,13 +815,15 @@ def leave_out_kss(fml: str, data, *, psi: str | None = None, n_draws: int = 250,
         comps = _leave_out_match(fit, psi, n_draws, block, seed, tol, maxiter, precision, t,
                                  se=se, se_draws=se_draws, se_trace=se_trace,
                                  smooth_bins=smooth_bins, se_variance=se_variance,
-                                 diagnose=diagnose, diagnose_draws=diagnose_draws)
+                                 diagnose=diagnose, diagnose_draws=diagnose_draws,
+                                 weak_interval=weak_interval, confidence=confidence)
     else:
         comps = leave_out_components(fit, n_draws=n_draws, block=block, seed=seed, psi=psi,
                                      tol=tol, maxiter=maxiter, precision=precision, timings=t,
                                      se=se, se_draws=se_draws, se_trace=se_trace,
                                      smooth_bins=smooth_bins, diagnose=diagnose,
-                                     diagnose_draws=diagnose_draws)
+                                     diagnose_draws=diagnose_draws,
+                                     weak_interval=weak_interval, confidence=confidence)
     t.update(prune_seconds=t1 - t0, fit_seconds=t2 - t1,
              total_seconds=time.perf_counter() - t0)
     comps.pruning = pruning

@@ -17,6 +17,8 @@ and its variance is estimated by V-hat = 4 sum_i s_i (C y~)_i^2 -
   m_times               M b (the reference centring)     :387-424
   fit / eval smoother   sigma2 on the (P_ii, B_ii) grid  :955-1054
   component_variances   first term, trace, V-hat, SEs    :599-784, 891-937
+  eigen_rows            x1 = X~ u of a lead vector       :486-495
+  weak_moments          (b1, theta1) moments for q = 1   :786-889
 
 The row passes are hand-written kernels (se.hip, kss.hip, rows.hip); the
 solves are the panel's block PCG through `CoefficientInverse`.  The draws
@@ -35,6 +37,7 @@ from . import _lib
 COMPONENTS = ("var(psi)", "var(alpha)", "cov(psi, alpha)")
 BII_SEED_OFFSET = 5_500_011        # leaveout.py:772
 TRACE_SEED_OFFSET = 7_700_017      # leaveout_se.py:752
+WEAK_SEED_OFFSET = 8_800_021       # leaveout_se.py:747
 QUANTILE_SAMPLE = 2_000_000        # leaveout_se.py:62
 
 
@@ -42,7 +45,8 @@ QUANTILE_SAMPLE = 2_000_000        # leaveout_se.py:62
 class ComponentSE:
     """Standard errors of the three components and the parts they came from
     (leaveout_se.py:74-126): dicts keyed by component; NaN where none is
-    reported."""
+    reported.  `weak` holds, per component the diagnostic flags (q >= 1),
+    the moments of the q = 1 interval (see `weak_moments`)."""
     se: dict
     variance: dict
     first_term: dict
@@ -172,14 +176,19 @@ def apply_q(lay: SELayout, k: int, Ul, Ug):
     return ol, og
 
 
-def apply_c(lay: SELayout, b, v=None, z=None, s3=None, rank_two=None, yt=None):
+def apply_c(lay: SELayout, b, v=None, z=None, s3=None, rank_two=None, yt=None, deflate=None,
+            weak: bool = False, mask: int = 7):
     """C_k v for the three components, reduced (leaveout_se.py:426-596).
 
     With a shared input v (n, s) and `yt`: (first (3), theta (3)) of
     4 sum s_k (C v)_k^2 and yt'(C v)_k (s = 1, v = yt).  Per component,
-    v_k = z sqrt(max(s_k, 0)) from draws z (n, s): sum_i s_k sum_j (C v)_k^2
-    (3).  b and s3 are (3, n); rank_two = (u (3, n), c) adds
-    c (1 u_k' + u_k 1') v."""
+    v_k = z sqrt(max(s_k, 0)) from draws z (n, s), zero for a component not
+    in the bit `mask`: sum_i s_k sum_j (C v)_k^2 (3).  b and s3 are (3, n);
+    rank_two = (u (3, n), c) adds c (1 u_k' + u_k 1') v; deflate =
+    (lambdas (3,), x1 (3, n)) makes it the C2 of B2 = B - lambda_k x1_k x1_k'.
+    `weak` floors s at zero and, for the shared input, reduces to (s12 (3),
+    first (3), quad (3)): 2 sum x1_k s_k (C2 v)_k, 4 sum s_k (C2 v)_k^2 and
+    yt'(C2 v)_k (leaveout_se.py:843-853)."""
     import torch
     lib = _lib.load()
     n = lay.n
@@ -187,17 +196,25 @@ def apply_c(lay: SELayout, b, v=None, z=None, s3=None, rank_two=None, yt=None):
     s = (z if per else v).shape[1]
     n_in = 3 if per else 1
     wide = n_in + 3
+    if weak and deflate is None:
+        raise ValueError("the weak moments need the deflation")
     stack = torch.empty((n, wide * s), dtype=torch.float64, device="cuda")
     torch.cuda.synchronize()
     _lib.check(lib.hdfe_se_stack(n, s, None if per else v.data_ptr(),
                                  z.data_ptr() if per else None, s3.data_ptr(), b.data_ptr(),
-                                 stack.data_ptr()), "hdfe_se_stack")
+                                 int(mask), stack.data_ptr()), "hdfe_se_stack")
+    vin = [stack[:, (k if per else 0) * s:((k if per else 0) + 1) * s] for k in range(3)]
     udot = odot = None
     if rank_two is not None:
         u = rank_two[0]
-        vin = [stack[:, (k if per else 0) * s:((k if per else 0) + 1) * s] for k in range(3)]
         udot = torch.stack([u[k][None, :] @ vin[k] for k in range(3)]).reshape(3, s).contiguous()
         odot = torch.stack([vin[k].sum(0) for k in range(3)]).contiguous()
+    x1 = defl = None
+    if deflate is not None:
+        x1 = deflate[1]
+        along = torch.stack([x1[k][None, :] @ vin[k] for k in range(3)]).reshape(3 * s)
+        lam = torch.as_tensor(np.asarray(deflate[0], np.float64)).cuda()
+        defl = torch.cat([lam, along]).contiguous()
     lv, gs = lay.reduce(stack)
     U0l, U0g, base = lay.inv(lv, gs)
     del lv, gs
@@ -210,15 +227,18 @@ def apply_c(lay: SELayout, b, v=None, z=None, s3=None, rank_two=None, yt=None):
         qg[:, k * s:(k + 1) * s] = og
     del U0l, U0g, stack
     _, _, thq = lay.inv(ql, qg)
-    out = np.empty(6 if not per else 3)
+    mode = (1 if per else 0) + (2 if weak else 0)
+    out = np.empty({0: 6, 2: 9}.get(mode, 3))
     torch.cuda.synchronize()
     _lib.check(lib.hdfe_se_assemble(
-        n, s, 1 if per else 0, None if per else v.data_ptr(), z.data_ptr() if per else None,
+        n, s, mode, None if per else v.data_ptr(), z.data_ptr() if per else None,
         s3.data_ptr(), b.data_ptr(), lay.w.data_ptr() if lay.weighted else None,
         base.data_ptr(), thq.data_ptr(), None if rank_two is None else rank_two[0].data_ptr(),
         0.0 if rank_two is None else float(rank_two[1]),
         None if udot is None else udot.data_ptr(), None if odot is None else odot.data_ptr(),
-        None if yt is None else yt.data_ptr(), out.ctypes.data), "hdfe_se_assemble")
+        None if yt is None else yt.data_ptr(), None if x1 is None else x1.data_ptr(),
+        None if defl is None else defl.data_ptr(), int(mask), out.ctypes.data),
+        "hdfe_se_assemble")
     return out
 
 
@@ -289,10 +309,90 @@ def smoothed_sigma2(sigma2, leverage, b_ii=None, n_bins=None):
     return out
 
 
+def eigen_rows(lay: SELayout, Ul, Ug):
+    """x1 = X~ u: sqrt(w_i) (A u)_i for coefficient blocks Ul (L, c) and
+    Ug (G, c) (leaveout_se.py:486-495); device (n, c)."""
+    import torch
+    inv = lay.inv
+    panel = inv.panel
+    g = panel.tensor("g", torch.int32).long()
+    codes = panel.tensor("codes", torch.int32).view(inv.D, inv.n).long()
+    au = Ug[g]
+    for d in range(inv.D):
+        au = au + Ul[int(inv.offs[d]) + codes[d]]
+    return au * lay.root[:, None]
+
+
+def weak_moments(lay: SELayout, y, sigma2, s3, leverages, b_ii, lams, Ul, Ug, components, *,
+                 n_draws: int, block: int, seed: int, stayer_rows=None,
+                 centering: str = "weighted") -> dict:
+    """The moments of the q = 1 interval per component in `components`
+    (leaveout_se.py:786-889): b1 = x1'y~ with x1 = X~ u1 (u1 the lead
+    vector, u1'S u1 = 1), V[b1] raw from `sigma2` (the point estimate's),
+    and the covariance of (b1, theta1), theta1 = y~'C2 y~ for the C2 of
+    B2 = B - lambda_1 x1 x1':
+
+        V[b1] = sum x1^2 s,   Cov = 2 sum x1 s (C2 y~),
+        V[theta1] = 4 sum s (C2 y~)^2 - 2 tr(C2 Om C2 Om),
+
+    s the smoothed sigma2 `s3` floored at zero.  Where subtracting the trace
+    leaves the matrix indefinite, V[theta1] falls back to its first term
+    ("conservative").  The trace's draws use `seed` directly."""
+    import torch
+    n = lay.n
+    x1 = eigen_rows(lay, Ul, Ug).T.contiguous()                 # (3, n)
+    w = lay.w
+    y = y.to(torch.float64).contiguous()
+    ybar = float((w * y).sum() / w.sum())
+    yt_c = lay.root * (y - ybar)
+    b1 = (x1 @ yt_c).cpu().numpy()
+    v_raw = ((x1 * x1) @ sigma2.to(torch.float64)).cpu().numpy()
+    sf = s3.clamp_min(0.0)
+    s11 = (x1 * x1 * sf).sum(1).cpu().numpy()
+    lam_d = torch.as_tensor(np.asarray(lams, np.float64)).cuda()
+    b2 = ((b_ii - lam_d[:, None] * x1 * x1)
+          * (leverages.factor / leverages.complement)[None, :]).contiguous()
+    if stayer_rows is not None:
+        b2[:, stayer_rows] = 0.0
+    rank_two = (m_times(lay, b2), 0.5 / n) if centering == "reference" else None
+    yt = (lay.root * (y - (ybar if centering == "weighted" else 0.0))).contiguous()
+    deflate = (lams, x1)
+    fy = apply_c(lay, b2, v=yt[:, None].contiguous(), s3=s3, rank_two=rank_two, yt=yt,
+                 deflate=deflate, weak=True)
+    s12, first, quad = fy[:3], fy[3:6], fy[6:]
+    mask = sum(1 << k for k in components)
+    total = np.zeros(3)
+    lib = _lib.load()
+    drawn = 0
+    while drawn < n_draws:
+        size = min(block, n_draws - drawn)
+        z = torch.empty((n, size), dtype=torch.float64, device="cuda")
+        torch.cuda.synchronize()
+        _lib.check(lib.hdfe_rademacher(0, n, size, seed, drawn, z.data_ptr()),
+                   "hdfe_rademacher")
+        total += apply_c(lay, b2, z=z, s3=s3, rank_two=rank_two, deflate=deflate, weak=True,
+                         mask=mask)
+        drawn += size
+    trace = total / n_draws
+    out = {}
+    for k in components:
+        s22 = first[k] - 2.0 * trace[k]
+        conservative = not (s22 > 0 and s12[k] ** 2 < s11[k] * s22)
+        if conservative:
+            s22 = first[k]
+        out[COMPONENTS[k]] = {
+            "lambda_1": float(lams[k]), "b1": float(b1[k]), "v_raw": float(v_raw[k]),
+            "sigma": np.array([[s11[k], s12[k]], [s12[k], s22]]),
+            "conservative": bool(conservative), "trace": float(trace[k]),
+            "quadratic": float(quad[k])}
+    return out
+
+
 def component_variances(inv, y, sigma2, leverages, *, psi: int = 0, rows: int | None = None,
                         n_draws: int = 250, block: int | None = None, seed: int = 0,
                         trace: bool = True, smooth_bins=None, b_ii=None, stayer_rows=None,
-                        centering: str = "weighted", reported=(0, 1, 2)) -> ComponentSE:
+                        centering: str = "weighted", reported=(0, 1, 2), weak_id=None,
+                        weak_components=None, sigma2_point=None) -> ComponentSE:
     """V-hat and the standard errors of the three components
     (leaveout_se.py:599-784).  `inv` is the panel's CoefficientInverse, `y`
     the outcome on the sorted rows (device), `sigma2` the leave-out variances
@@ -301,7 +401,11 @@ def component_variances(inv, y, sigma2, leverages, *, psi: int = 0, rows: int | 
     draws use it, the trace's seed + 7_700_017.  At match level
     `stayer_rows` (device bool) zeroes b on stayers' matches, `centering`
     "reference" adds the rank-two term, and `reported` lists the components
-    with a standard error."""
+    with a standard error.  With `weak_id` (the diagnostic's result), every
+    reported component it flags (q >= 1) -- at match level var(psi) alone,
+    since stayers load on the others' weak direction -- gets the moments of
+    the q = 1 interval, from `sigma2_point` (default sigma2) and draws with
+    seed + 8_800_021 (leaveout_se.py:736-756)."""
     import torch
     if centering not in ("reference", "weighted"):
         raise ValueError("centering must be 'reference' or 'weighted'")
@@ -339,12 +443,27 @@ def component_variances(inv, y, sigma2, leverages, *, psi: int = 0, rows: int | 
             drawn += size
             blocks += 1
         total /= n_draws
+    weak = None
+    if weak_id is not None:
+        if weak_components is None:
+            weak_components = [k for k, nm in enumerate(COMPONENTS) if weak_id.q[nm] >= 1]
+        weak_components = [k for k in weak_components
+                           if k in reported and (stayer_rows is None or k == 0)]
+        if weak_components:
+            lams = np.array([float(weak_id.eigenvalues[nm][0]) if weak_id.eigenvalues[nm]
+                             else 0.0 for nm in COMPONENTS])
+            Ul, Ug = weak_id.lead_vectors
+            weak = weak_moments(lay, y, sigma2 if sigma2_point is None else sigma2_point, s3,
+                                leverages, b_ii, lams, Ul, Ug, weak_components,
+                                n_draws=n_draws, block=block, seed=seed + WEAK_SEED_OFFSET,
+                                stayer_rows=stayer_rows, centering=centering)
     variance = first - 2.0 * total
     names = list(COMPONENTS)
     for k in range(3):
         if k not in reported:
             variance[k] = first[k] = total[k] = theta[k] = np.nan
     return ComponentSE(
+        weak=weak,
         se={nm: float(np.sqrt(v)) if v > 0 else float("nan") for nm, v in zip(names, variance)},
         variance=dict(zip(names, variance.tolist())),
         first_term=dict(zip(names, first.tolist())),

@@ -28,7 +28,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import _lib
-from .leaveout_se import COMPONENTS, SELayout, apply_q, smoothed_sigma2
+from .leaveout_se import COMPONENTS, SELayout, apply_q, eigen_rows, smoothed_sigma2
 
 WEAK_ID_THRESHOLD = 0.1            # leaveout_weakid.py:67
 SEED_OFFSET = 9_900_019            # leaveout.py:758
@@ -395,13 +395,7 @@ def weak_id_diagnostics(inv, y, sigma2, leverages, *, psi: int = 0, rows: int | 
                                                   X, Y, k, settled=settled)
     # F: b1 = x1'y~, V[b1] = sum x1^2 sigma2-tilde, x1 = sqrt(w) (A u1)
     sm = smoothed_sigma2(sigma2, leverages.leverage, None, smooth_bins)
-    panel = inv.panel
-    g = panel.tensor("g", torch.int32).long()
-    codes = panel.tensor("codes", torch.int32).view(inv.D, inv.n).long()
-    au = u[1][g]
-    for d in range(inv.D):
-        au = au + u[0][int(inv.offs[d]) + codes[d]]
-    x1 = au * lay.root[:, None]
+    x1 = eigen_rows(lay, u[0], u[1])
     y = y.to(torch.float64)
     ybar = float((lay.w * y).sum() / lay.w.sum())
     yt = lay.root * (y - ybar)

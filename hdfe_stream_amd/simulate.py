@@ -16,7 +16,7 @@ from __future__ import annotations
 
 import numpy as np
 
-__all__ = ["akm_panel", "kss_panel", "weighted_panel", "poisson_models",
+__all__ = ["akm_panel", "kss_panel", "weighted_panel", "bottleneck_panel", "poisson_models",
            "poisson_models_device"]
 
 # Row-level corr(alpha, psi) is about 0.6 times the copula parameter with
@@ -87,6 +87,52 @@ def weighted_panel(n_workers: int, n_firms: int, seed: int = 4, n_years: int = 8
     rng = np.random.default_rng(seed + 1000)
     d["w"] = rng.uniform(0.5, 2.0, len(d["y"])).astype(np.float32)
     return d
+
+
+def bottleneck_panel(n_bridge: int = 6, firms_per_side: int = 8, movers_per_side: int = 150,
+                     stayers_per_firm: int = 8, gap: float = 0.5, seed: int = 0) -> dict:
+    """A panel with weakly identified components (the design of
+    hdfe_stream/simulate.py:279-360, `simulate_bottleneck`): two sides of
+    `firms_per_side` firms, each with `movers_per_side` workers who move
+    between two firms of their side (two years at each) and
+    `stayers_per_firm` three-year stayers per firm, joined only by
+    `n_bridge` workers who move from one side to the other.  The gap
+    between the sides' mean firm effects (`gap` is added to the second
+    side's) is then estimated far worse than any other contrast: one
+    eigenvalue of S^- Q dominates.  Effects alpha ~ N(0, 0.4^2), psi ~
+    N(0, 0.3^2), noise N(0, 0.15^2); columns worker_id, firm_id, y."""
+    rng = np.random.default_rng(seed)
+    F = firms_per_side
+    psi = rng.normal(0.0, 0.3, 2 * F)
+    psi[F:] += gap
+    worker, firm = [], []
+    wid = 0
+
+    def spell(f, years):
+        nonlocal wid
+        worker.extend([wid] * years)
+        firm.extend([f] * years)
+
+    for side in range(2):
+        base = side * F
+        for _ in range(movers_per_side):
+            a, b = rng.choice(F, 2, replace=False)
+            spell(base + a, 2)
+            spell(base + b, 2)
+            wid += 1
+        for f in range(F):
+            for _ in range(stayers_per_firm):
+                spell(base + f, 3)
+                wid += 1
+    for _ in range(n_bridge):
+        spell(int(rng.integers(0, F)), 2)
+        spell(F + int(rng.integers(0, F)), 2)
+        wid += 1
+    worker = np.asarray(worker, np.int64)
+    firm = np.asarray(firm, np.int64)
+    alpha_w = rng.normal(0.0, 0.4, wid)
+    y = alpha_w[worker] + psi[firm] + rng.normal(0.0, 0.15, len(worker))
+    return {"worker_id": worker, "firm_id": firm, "y": y}
 
 
 def poisson_models(n_models: int = 32, lo: int = 50_000, hi: int = 2_000_000,

@@ -331,17 +331,21 @@ void hdfe_components_destroy(void *h);
  * draws z (n, s); out (n, (1 or 3) s + 3 s).  Replaces the stacking of
  * _apply_c_passes (leaveout_se.py:515-530). */
 int hdfe_se_stack(int64_t n, int s, const double *v, const double *z, const double *s3,
-                  const double *b, double *out);
+                  const double *b, int mask, double *out);
 /* C v at row level from the two solves' read-backs base and thq, reduced in
  * a fixed order (host out): mode 0, v = yt (s = 1): 4 sum s_k (C y)_k^2 and
- * yt'(C y)_k; mode 1, per-component draws: sum_i s_k,i sum_j (C v)_kij^2.
- * u, c2, udot, odot: the rank-two term of the reference centring.
- * Replaces _apply_c_passes' pass 3 with collect_y / _cc_trace's collect
- * (leaveout_se.py:565-592, 722-731, 911-925). */
+ * yt'(C y)_k; mode 1, per-component draws: sum_i s_k,i sum_j (C v)_kij^2;
+ * modes 2 and 3 the same for the deflated C2 of the q = 1 interval, s floored
+ * at zero (mode 2 adds 2 sum x1_k s_k (C2 y)_k).  u, c2, udot, odot: the
+ * rank-two term of the reference centring; x1 (3, n) and defl (lambda (3),
+ * x1_k'v_k (3, s)): the deflation; mask: the components with draws.
+ * Replaces _apply_c_passes' pass 3 with collect_y / _cc_trace's collect and
+ * _weak_moments' collect (leaveout_se.py:565-592, 722-731, 843-853, 911-925). */
 int hdfe_se_assemble(int64_t n, int s, int mode, const double *v, const double *z,
                      const double *s3, const double *b, const double *w, const double *base,
                      const double *thq, const double *u, double c2, const double *udot,
-                     const double *odot, const double *yt, double *out);
+                     const double *odot, const double *yt, const double *x1, const double *defl,
+                     int mask, double *out);
 /* out (3, n) += w_i (sum r1^2, sum r2^2, sum r1 r2) of rows (n, 2 s).
  * Replaces _nb_bii_moments (kernels_se.py:75-98). */
 int hdfe_se_bii(int64_t n, int s, const double *rows, const double *w, double *out);

@@ -1174,12 +1174,13 @@ def apply_q(lay: SELayout, k: int, Ul: np.ndarray, Ug: np.ndarray):
 
 
 def apply_c(lay: SELayout, v: np.ndarray, b: np.ndarray, per_component: bool = False,
-            rank_two=None) -> np.ndarray:
+            rank_two=None, deflate=None) -> np.ndarray:
     """C_k v for the three components, (n, 3, s) (leaveout_se.py:426-596):
     v is (n, s) shared or (n, 3, s) one per component; b (3, n).  Solve 1:
     S^- X~'[v, b_k v]; Q_k on the input block; solve 2; then
     C v = B v - (b .* M v + M (b .* v)) / 2, plus rank_two = (u (3, n), c):
-    c (1 u_k' + u_k 1') v."""
+    c (1 u_k' + u_k 1') v.  deflate = (lambdas (3,), x1 (n, 3)) applies C2,
+    built from B2 = B - lambda_1 x1 x1' (leaveout_se.py:497-499, 579-580)."""
     ctx = lay.ctx
     n_in = 3 if per_component else 1
     s = v.shape[-1]
@@ -1202,6 +1203,8 @@ def apply_c(lay: SELayout, v: np.ndarray, b: np.ndarray, per_component: bool = F
         vk = part(k)
         Mv = vk - root * base[:, inc(k)]
         Bv = root * thq[:, k * s:(k + 1) * s]
+        if deflate is not None:
+            Bv = Bv - deflate[0][k] * np.outer(deflate[1][:, k], deflate[1][:, k] @ vk)
         Mbv = vk * b[k][:, None] - root * base[:, bc(k)]
         cv[:, k, :] = Bv - 0.5 * (b[k][:, None] * Mv + Mbv)
         if rank_two is not None:
@@ -1686,3 +1689,143 @@ def weak_id_diagnostics(lay: SELayout, y: np.ndarray, sigma2: np.ndarray, levera
         out["q_borderline"].append(bool(np.any(np.abs(r - WEAK_ID_THRESHOLD)
                                                < 2.0 * rel_se * r)))
     return out
+
+
+
+# ------------------------------------------------ the q = 1 weak interval
+#
+# leaveout_se.py:786-889 (the deflated moments) and am_interval.py (the
+# Andrews-Mikusheva projection), restated.
+
+WEAK_MOMENTS_OFFSET = 8_800_021    # leaveout_se.py:747
+
+
+def eigen_rows(lay: SELayout, Ul: np.ndarray, Ug: np.ndarray) -> np.ndarray:
+    """sqrt(w_i) (A u)_i for coefficient blocks (L, c), (G, c)."""
+    ctx = lay.ctx
+    return lay.root[:, None] * (ctx.Do @ Ul + Ug[ctx.g])
+
+
+def weak_moments(lay: SELayout, y, sigma2, s, leverage, complement, factor, b_ii, lams,
+                 Ul, Ug, components, n_draws: int, block: int, seed: int, stayer_rows=None,
+                 centering: str = "weighted") -> dict:
+    """(b1, raw V[b1], the 2x2 covariance of (b1, theta1)) per weak component:
+    b1 = x1'y~ with x1 = X~ u1, and theta1 = y~'C2 y~ for C2 built from
+    B2 = B - lambda_1 x1 x1'; `s` (3, n) is the smoothed sigma2, floored at
+    zero here; `sigma2` the point estimate's (for V[b1] raw)."""
+    ctx = lay.ctx
+    n = len(ctx.w)
+    x1 = eigen_rows(lay, Ul, Ug)
+    w = ctx.w
+    ybar = float(w @ y / w.sum())
+    yt_c = lay.root * (y - ybar)
+    b1 = x1.T @ yt_c
+    v_raw = (x1 ** 2).T @ sigma2
+    sf = np.maximum(s, 0.0)
+    s11 = np.array([(x1[:, k] ** 2) @ sf[k] for k in range(3)])
+    inv_m = factor / complement
+    b2 = np.stack([(b_ii[k] - lams[k] * x1[:, k] ** 2) * inv_m for k in range(3)])
+    if stayer_rows is not None:
+        b2[:, stayer_rows] = 0.0
+    rank_two = (m_times(lay, b2), 0.5 / n) if centering == "reference" else None
+    yt = lay.root * (y - (ybar if centering == "weighted" else 0.0))
+    deflate = (np.asarray(lams, np.float64), x1)
+    cy = apply_c(lay, yt[:, None], b2, rank_two=rank_two, deflate=deflate)[:, :, 0]
+    s12 = np.array([2.0 * (x1[:, k] * sf[k]) @ cy[:, k] for k in range(3)])
+    first = np.array([4.0 * sf[k] @ cy[:, k] ** 2 for k in range(3)])
+    quad = np.array([yt @ cy[:, k] for k in range(3)])
+    total = np.zeros(3)
+    drawn = 0
+    while drawn < n_draws:
+        size = min(block, n_draws - drawn)
+        z = rademacher(0, n, size, seed, drawn)
+        v = np.zeros((n, 3, size))
+        for k in components:
+            v[:, k, :] = z * np.sqrt(sf[k])[:, None]
+        cv = apply_c(lay, v, b2, per_component=True, rank_two=rank_two, deflate=deflate)
+        for k in components:
+            total[k] += sf[k] @ (cv[:, k, :] ** 2).sum(axis=1)
+        drawn += size
+    trace = total / n_draws
+    out = {}
+    for k in components:
+        s22 = first[k] - 2.0 * trace[k]
+        conservative = not (s22 > 0 and s12[k] ** 2 < s11[k] * s22)
+        if conservative:
+            s22 = first[k]
+        out[k] = {"lambda_1": float(lams[k]), "b1": float(b1[k]), "v_raw": float(v_raw[k]),
+                  "sigma": np.array([[s11[k], s12[k]], [s12[k], s22]]),
+                  "conservative": bool(conservative), "trace": float(trace[k]),
+                  "quadratic": float(quad[k])}
+    return out
+
+
+def am_cdf(z: float, kappa: float) -> float:
+    """P(sqrt(a^2 + (b + c)^2) - c <= z), c = 1/kappa, for independent
+    half-normals a, b (am_interval.py:45-62; KSS Appendix C.6.1 for q = 1):
+    over b in [0, z] with density 2 phi(b), a reaches
+    sqrt((z - b)(z + b + 2c)) -- the circle of radius z + c about (0, -c),
+    factored so nothing cancels as c grows -- with probability
+    2 Phi(.) - 1."""
+    from scipy import integrate, special
+    if z <= 0:
+        return 0.0
+    if kappa <= 0:
+        return float(special.erf(z / np.sqrt(2.0)))
+    c = 1.0 / kappa
+
+    def inner(b):
+        top = np.sqrt(max((z - b) * (z + b + 2.0 * c), 0.0))
+        return 2.0 * np.exp(-0.5 * b * b) / np.sqrt(2.0 * np.pi) * special.erf(top / np.sqrt(2.0))
+
+    # for small kappa the erf leaves one only within ~8 kappa of b = z: a
+    # breakpoint there keeps quad from stepping over that sliver
+    edge = z - 20.0 * kappa
+    pts = [edge] if 0.0 < edge < z else None
+    val, _ = integrate.quad(inner, 0.0, z, epsabs=1e-14, epsrel=1e-13, limit=200, points=pts)
+    return float(val)
+
+
+def am_critical_value(kappa: float, alpha: float = 0.05) -> float:
+    """The (1 - alpha) quantile of the above (am_interval.py:65-79), by
+    bisection between the chi-squared(1) and chi-squared(2) quantiles'
+    roots (its limits)."""
+    from scipy import special
+    lo = float(np.sqrt(special.chdtri(1, alpha)))
+    if kappa <= 0:
+        return lo
+    hi = float(np.sqrt(special.chdtri(2, alpha)))
+    a, b = lo * (1 - 1e-9), hi * (1 + 1e-9)
+    for _ in range(60):
+        mid = 0.5 * (a + b)
+        if am_cdf(mid, kappa) < 1.0 - alpha:
+            a = mid
+        else:
+            b = mid
+    return 0.5 * (a + b)
+
+
+def am_interval(lam1: float, b1: float, theta1: float, sigma, alpha: float = 0.05):
+    """The q = 1 interval (am_interval.py:90-123): the extremes of lambda_1 b^2 + t on the boundary
+    of the ellipse of (b, t) at the critical value, from the first-order
+    condition in closed form (KSS Appendix C.6.2): with the boundary
+    (b1, theta1) + z L (cos p, sin p), L = chol(sigma), and s = tan(p / 2),
+    the stationary points are the real roots of a quartic in s."""
+    sigma = np.asarray(sigma, np.float64)
+    vb, vt, cv = sigma[0, 0], sigma[1, 1], sigma[0, 1]
+    kappa = 2.0 * abs(lam1) * vb / np.sqrt(vt * (1.0 - cv * cv / (vb * vt)))
+    z = am_critical_value(kappa, alpha)
+    Lc = np.linalg.cholesky(sigma)
+    A, B, Cc = z * Lc[0, 0], z * Lc[1, 0], z * Lc[1, 1]
+
+    def f(p):
+        return lam1 * (b1 + A * np.cos(p)) ** 2 + theta1 + B * np.cos(p) + Cc * np.sin(p)
+
+    # f'(p) (1 + s^2)^2 = -Cc s^4 - (4 lam A (b1 - A) + 2 B) s^3
+    #                     - (4 lam A (b1 + A) + 2 B) s + Cc
+    coef = [-Cc, -(4 * lam1 * A * (b1 - A) + 2 * B), 0.0, -(4 * lam1 * A * (b1 + A) + 2 * B), Cc]
+    roots = np.roots(coef) if np.any(np.abs(coef) > 0) else np.array([])
+    cand = [np.pi] + [2.0 * np.arctan(r.real) for r in roots if abs(r.imag) <= 1e-9 * (1 + abs(r))]
+    vals = np.array([f(p) for p in cand])
+    return float(vals.min()), float(vals.max()), {"kappa": float(kappa), "z": float(z),
+                                                  "rho": float(cv / np.sqrt(vb * vt))}

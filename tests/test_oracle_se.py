@@ -257,3 +257,165 @@ def test_ritz_filter_drops_a_lone_spurious_value(oracle_mod):
     # merged copies: two equal eigenvalues count once
     theta, _b, _S = O.ritz_top(np.array([1.0, 1.0]), np.array([0.0, 1e-3]), 4)
     assert len(theta) == 1 and theta[0] == pytest.approx(1.0)
+
+
+# ------------------------------------------------ the q = 1 weak interval
+
+def _lead(dense, k):
+    """The leading eigenvector of S^- Q_k, S-normalised, split into the
+    level and group blocks, and its eigenvalue."""
+    A, Sfull, _Q, _lay = _dense_A(dense, k)
+    ev, V = np.linalg.eig(A)
+    j = int(np.argmax(np.abs(ev.real)))
+    u = V[:, j].real
+    u = u / np.sqrt(u @ Sfull @ u)
+    G = dense["ctx"].G
+    return float(ev[j].real), u[G:], u[:G]
+
+
+def _c2(dense, k, lam, x1):
+    B2 = dense["B"][k] - lam * np.outer(x1, x1)
+    b2 = np.diag(B2) / np.diag(dense["M"])
+    M = dense["M"]
+    return B2 - 0.5 * (np.diag(b2) @ M + M @ np.diag(b2)), b2
+
+
+def test_eigen_rows_are_x_tilde_u(dense):
+    O, lay = dense["O"], dense["lay"]
+    lam, ul, ug = _lead(dense, 0)
+    x1 = O.eigen_rows(lay, ul[:, None], ug[:, None])[:, 0]
+    assert rel(x1, dense["Xt"] @ np.concatenate([ug, ul])) < 1e-12
+    # u1'S u1 = 1 makes x1 a unit vector, and the Rayleigh quotient is lambda
+    assert x1 @ x1 == pytest.approx(1.0, rel=1e-9)
+    assert x1 @ dense["B"][0] @ x1 == pytest.approx(lam, rel=1e-8)
+
+
+@pytest.mark.parametrize("per_component", [False, True])
+def test_deflated_kernel_is_dense_c2(dense, per_component):
+    O, lay, n = dense["O"], dense["lay"], dense["n"]
+    leads = [_lead(dense, k) for k in range(3)]
+    lams = np.array([t[0] for t in leads])
+    x1 = np.stack([O.eigen_rows(lay, t[1][:, None], t[2][:, None])[:, 0] for t in leads], 1)
+    C2, b2 = zip(*[_c2(dense, k, lams[k], x1[:, k]) for k in range(3)])
+    rng = np.random.default_rng(11)
+    v = rng.standard_normal((n, 3, 4) if per_component else (n, 4))
+    u = rng.standard_normal((3, n))
+    cv = O.apply_c(lay, v, np.stack(b2), per_component=per_component, rank_two=(u, 0.2),
+                   deflate=(lams, x1))
+    one = np.ones(n)
+    for k in range(3):
+        K = C2[k] + 0.2 * (np.outer(one, u[k]) + np.outer(u[k], one))
+        want = K @ (v[:, k, :] if per_component else v)
+        assert rel(cv[:, k, :], want) < 1e-8
+        assert np.abs(np.diag(C2[k])).max() < 1e-12 * np.abs(C2[k]).max()
+
+
+def test_weak_moments_dense(dense):
+    O, lay, ctx, n = dense["O"], dense["lay"], dense["ctx"], dense["n"]
+    leads = [_lead(dense, k) for k in range(3)]
+    lams = np.array([t[0] for t in leads])
+    Ul = np.stack([t[1] for t in leads], 1)
+    Ug = np.stack([t[2] for t in leads], 1)
+    x1 = dense["Xt"] @ np.vstack([Ug, Ul])
+    y = dense["f"]["p0"]["V"][:, 0]
+    yt = lay.root * (y - ctx.w @ y / ctx.w.sum())
+    M = dense["M"]
+    # the exact leave-out sigma2 of the observation: y~_i (M y~)_i / M_ii
+    s2 = yt * (M @ yt) / np.diag(M)
+    rng = np.random.default_rng(5)
+    s = rng.uniform(-0.05, 0.4, (3, n))            # some negative: the floor matters
+    b_ii = np.stack([np.diag(B) for B in dense["B"]])
+    nd, seed = 12, 77
+    out = O.weak_moments(lay, y, s2, s, 1.0 - np.diag(M), np.diag(M), np.ones(n), b_ii, lams,
+                         Ul, Ug, (0, 1, 2), nd, 5, seed)
+    sf = np.maximum(s, 0.0)
+    Z = O.rademacher(0, n, nd, seed, 0)
+    names = range(3)
+    for k in names:
+        C2, _ = _c2(dense, k, lams[k], x1[:, k])
+        c2y = C2 @ yt
+        p = out[k]
+        assert p["b1"] == pytest.approx(x1[:, k] @ yt, rel=1e-10)
+        assert p["v_raw"] == pytest.approx((x1[:, k] ** 2) @ s2, rel=1e-10)
+        assert p["sigma"][0, 0] == pytest.approx((x1[:, k] ** 2) @ sf[k], rel=1e-10)
+        assert p["sigma"][0, 1] == pytest.approx(2 * (x1[:, k] * sf[k]) @ c2y, rel=1e-7)
+        Om = Z * np.sqrt(sf[k])[:, None]
+        tr = (sf[k][:, None] * (C2 @ Om) ** 2).sum() / nd
+        assert p["trace"] == pytest.approx(tr, rel=1e-7)
+        s22 = 4 * sf[k] @ c2y ** 2 - 2 * tr
+        if not p["conservative"]:
+            assert p["sigma"][1, 1] == pytest.approx(s22, rel=1e-7)
+        # theta1 = y~'C2 y~ = y~'C y~ - lambda (b1^2 - V[b1] raw) when sigma2
+        # is the leave-out one: the deflation moves lambda b1^2 out of the
+        # quadratic form and lambda sum x1^2 sigma2 out of its bias
+        assert p["quadratic"] == pytest.approx(yt @ c2y, rel=1e-8)
+        theta = yt @ dense["C"][k] @ yt
+        assert p["quadratic"] == pytest.approx(
+            theta - lams[k] * (p["b1"] ** 2 - p["v_raw"]), rel=1e-7, abs=1e-10 * abs(theta))
+
+
+def test_am_cdf_is_pinned(oracle_mod):
+    from scipy import integrate, special
+    O = oracle_mod
+    # the limits: |N(0,1)| at kappa = 0, the chi(2) radius as kappa -> inf
+    for z in (0.7, 1.96, 2.6):
+        assert O.am_cdf(z, 0.0) == pytest.approx(special.erf(z / np.sqrt(2)), abs=1e-15)
+        assert O.am_cdf(z, 1e9) == pytest.approx(1 - np.exp(-z * z / 2), abs=1e-8)
+        # first order in small kappa: rho ~ b + kappa a^2 / 2, so the law
+        # moves by kappa E[a^2] / 2 times b's density 2 phi(z)
+        k = 1e-5
+        first = special.erf(z / np.sqrt(2)) - k * np.exp(-z * z / 2) / np.sqrt(2 * np.pi)
+        assert O.am_cdf(z, k) == pytest.approx(first, abs=1e-9)
+    # the region itself, integrated in two dimensions, at moderate curvature
+    for z, k in ((1.3, 0.4), (2.1, 2.5)):
+        c = 1.0 / k
+        hn = lambda x: np.sqrt(2 / np.pi) * np.exp(-0.5 * x * x)
+        top = lambda b: np.sqrt(max((z + c) ** 2 - (b + c) ** 2, 0.0))
+        want, _ = integrate.dblquad(lambda a, b: hn(a) * hn(b), 0.0, z, lambda b: 0.0, top,
+                                    epsabs=1e-12, epsrel=1e-10)
+        assert O.am_cdf(z, k) == pytest.approx(want, abs=1e-9)
+    # and by simulation
+    rng = np.random.default_rng(0)
+    a, b = np.abs(rng.standard_normal((2, 2_000_000)))
+    for z, k in ((1.5, 0.5), (2.2, 3.0)):
+        c = 1.0 / k
+        p = np.mean(np.sqrt(a * a + (b + c) ** 2) - c <= z)
+        assert abs(O.am_cdf(z, k) - p) < 5 * np.sqrt(p * (1 - p) / a.size)
+
+
+def test_am_critical_value_limits(oracle_mod):
+    O = oracle_mod
+    assert O.am_critical_value(0.0) == pytest.approx(1.959963984540054, abs=1e-12)
+    assert O.am_critical_value(1e9) == pytest.approx(np.sqrt(-2 * np.log(0.05)), abs=1e-7)
+    z = O.am_critical_value(1.5, 0.1)
+    assert O.am_cdf(z, 1.5) == pytest.approx(0.9, abs=1e-12)
+    prev = 0.0
+    for k in (0.01, 0.1, 1.0, 10.0):
+        cur = O.am_critical_value(k)
+        assert cur > prev
+        prev = cur
+
+
+def test_am_interval_is_the_projected_ellipse(oracle_mod):
+    O = oracle_mod
+    rng = np.random.default_rng(2)
+    p = np.linspace(0.0, 2 * np.pi, 400_001)
+    for _ in range(6):
+        lam, b1, th = rng.normal() * rng.choice([0.05, 1.0, 20.0]), rng.normal(), rng.normal()
+        R = rng.normal(size=(2, 2))
+        S = R @ R.T + 0.1 * np.eye(2)
+        lo, hi, info = O.am_interval(lam, b1, th, S)
+        z = info["z"]
+        Lc = np.linalg.cholesky(S)
+        b = b1 + z * Lc[0, 0] * np.cos(p)
+        t = th + z * (Lc[1, 0] * np.cos(p) + Lc[1, 1] * np.sin(p))
+        g = lam * b * b + t
+        assert lo == pytest.approx(g.min(), abs=1e-9 * (1 + abs(g).max()))
+        assert hi == pytest.approx(g.max(), abs=1e-9 * (1 + abs(g).max()))
+        assert lo <= lam * b1 ** 2 + th <= hi
+    # a linear map (lambda = 0): theta1 -/+ z sqrt(V[theta1]) at the |N| critical value
+    S = np.array([[0.3, 0.1], [0.1, 0.5]])
+    lo, hi, info = O.am_interval(0.0, 0.4, 1.0, S)
+    assert info["kappa"] == 0.0 and info["z"] == pytest.approx(1.959963984540054)
+    assert (lo, hi) == pytest.approx((1.0 - info["z"] * np.sqrt(0.5),
+                                      1.0 + info["z"] * np.sqrt(0.5)), abs=1e-12)

@@ -172,3 +172,169 @@ def test_weak_id_matches_the_oracle(oracle_mod):
         np.testing.assert_allclose(out.ratios[nm], ref["ratios"][k], rtol=1e-5)
     assert out.diagnostics["n_draws"] == ref["n_draws"]
     assert "weak-identification" in out.summary()
+
+
+# ------------------------------------------------ the q = 1 weak interval
+
+def _device_to_oracle(res, f, Ul, Ug):
+    """Coefficient blocks in the device's level/group order, moved to the
+    oracle's through the rows (both hold the rows in the same order)."""
+    import torch
+    info = res.panel.info
+    n = info["n"]
+    g = res.panel.tensor("g", torch.int32).long().cpu().numpy()
+    c0 = res.panel.tensor("codes", torch.int32).view(info["D"], n)[0].long().cpu().numpy()
+    dl = int(res.panel.offs[0]) + c0
+    ol = f["cells"]["offs"][0] + f["p0"]["codes"][:, 0]
+    og = f["p0"]["g"]
+    Ul, Ug = Ul.cpu().numpy(), Ug.cpu().numpy()
+    L, G = int(ol.max()) + 1, int(og.max()) + 1
+    out_l, out_g = np.zeros((L, Ul.shape[1])), np.zeros((G, Ug.shape[1]))
+    out_l[ol] = Ul[dl]
+    out_g[og] = Ug[g]
+    return out_l, out_g
+
+
+@pytest.mark.parametrize("centering", ["weighted", "reference"])
+def test_weak_moments_match_the_oracle(oracle_mod, centering):
+    import torch
+    from hdfe_stream_amd.estimator import feols_stream
+    from hdfe_stream_amd.leaveout import CoefficientInverse, JLALeverages
+    from hdfe_stream_amd.leaveout_se import SELayout, eigen_rows, weak_moments
+    from hdfe_stream_amd.simulate import kss_panel
+    O = oracle_mod
+    weighted = centering == "weighted"
+    d = kss_panel(1200, 150, seed=5)
+    if weighted:
+        d["w"] = np.random.default_rng(2).uniform(0.5, 2.0, len(d["y"]))
+    d = _loo(O, d)
+    wkw = {"weights": "w"} if weighted else {}
+    print(f"weak_moments centering={centering}", flush=True)
+    res = feols_stream("y ~ 1 | worker_id + firm_id", d, stream="worker_id", fixef_rm="none",
+                       keep_intermediates=True, tol=1e-12, **wkw)
+    f = O.fit(d, "y", (), FE, weights="w" if weighted else None, tol=1e-12, fixef_rm="none",
+              stream="worker_id")
+    ctx = O.KSSContext(f, tol=1e-12)
+    n = len(ctx.w)
+    rng = np.random.default_rng(8)
+    M_bar, factor = rng.uniform(0.3, 0.95, n), rng.uniform(0.9, 1.1, n)
+    s2 = rng.uniform(0.05, 0.3, n)
+    s = rng.uniform(-0.05, 0.4, (3, n))
+    b_ii = rng.uniform(0.0, 0.02, (3, n))
+    lams = np.array([0.3, 0.2, -0.1])
+    stayers = rng.random(n) < 0.3 if centering == "reference" else None
+    comps = (0, 2) if centering == "reference" else (0, 1, 2)
+    cu = lambda a: torch.from_numpy(np.ascontiguousarray(a, np.float64)).cuda()
+    inv = CoefficientInverse(res.panel, tol=1e-12, precision="fp64")
+    lay = SELayout(inv, 0, n)
+    Ul_d = torch.randn((inv.L, 3), dtype=torch.float64, device="cuda",
+                       generator=torch.Generator("cuda").manual_seed(1))
+    Ug_d = torch.randn((inv.G, 3), dtype=torch.float64, device="cuda",
+                       generator=torch.Generator("cuda").manual_seed(2))
+    Ul, Ug = _device_to_oracle(res, f, Ul_d, Ug_d)
+    olay = O.SELayout(ctx, 0, n)
+    x1 = eigen_rows(lay, Ul_d, Ug_d).cpu().numpy()
+    assert rel(x1, O.eigen_rows(olay, Ul, Ug)) < 1e-13
+    y = f["p0"]["V"][:, 0]
+    ref = O.weak_moments(olay, y, s2, s, 1.0 - M_bar, M_bar, factor, b_ii, lams, Ul, Ug, comps,
+                         10, 4, 123, stayer_rows=stayers, centering=centering)
+    lev = JLALeverages(leverage=cu(1.0 - M_bar), complement=cu(M_bar), factor=cu(factor),
+                       n_draws=1)
+    got = weak_moments(lay, res.panel.tensor("V", torch.float64), cu(s2), cu(s), lev,
+                       cu(b_ii), lams, Ul_d, Ug_d, comps, n_draws=10, block=4, seed=123,
+                       stayer_rows=None if stayers is None else torch.from_numpy(stayers).cuda(),
+                       centering=centering)
+    names = ["var(psi)", "var(alpha)", "cov(psi, alpha)"]
+    assert set(got) == {names[k] for k in comps}
+    for k in comps:
+        p, q = got[names[k]], ref[k]
+        for key in ("b1", "v_raw", "trace", "quadratic"):
+            assert p[key] == pytest.approx(q[key], rel=1e-8, abs=1e-12), key
+        np.testing.assert_allclose(p["sigma"], q["sigma"], rtol=1e-8, atol=1e-13)
+        assert p["conservative"] == q["conservative"]
+
+
+@pytest.fixture(scope="module")
+def bottleneck():
+    from hdfe_stream_amd.leaveout import leave_out_kss
+    from hdfe_stream_amd.simulate import bottleneck_panel
+    d = bottleneck_panel(n_bridge=6, seed=0)
+    print("leave_out_kss on a bottleneck, se=True", flush=True)
+    lo = leave_out_kss("y ~ 1 | worker_id + firm_id", d, n_draws=128, seed=3,
+                       precision="fp64", tol=1e-12, leave_out="observation", se=True,
+                       se_draws=64, block=16, diagnose_draws=64)
+    return d, lo
+
+
+def test_weak_interval_on_a_bottleneck(oracle_mod, bottleneck):
+    import torch
+    O = oracle_mod
+    d, lo = bottleneck
+    flagged = {k for k, q in lo.weak_id.q.items() if q >= 1}
+    assert flagged, "the bottleneck panel should be weakly identified"
+    assert set(lo.se.weak) == flagged
+    # the moments again in the oracle, from the device's own inputs
+    f = O.fit(d, "y", (), FE, tol=1e-12, fixef_rm="none", stream="worker_id")
+    ctx = O.KSSContext(f, tol=1e-12)
+    n = len(ctx.w)
+    Ul, Ug = _device_to_oracle(lo.fit, f, *lo.weak_id.lead_vectors)
+    lev = lo.leverages
+    host = lambda t: t.detach().cpu().numpy()
+    names = ["var(psi)", "var(alpha)", "cov(psi, alpha)"]
+    comps = tuple(k for k, nm in enumerate(names) if nm in flagged)
+    lams = np.array([lo.weak_id.eigenvalues[nm][0] for nm in names])
+    ref = O.weak_moments(O.SELayout(ctx, 0, n), f["p0"]["V"][:, 0],
+                         host(lo.diagnostics["sigma2"]),
+                         host(lo.se.diagnostics["smoothed_sigma2"]), host(lev.leverage),
+                         host(lev.complement), host(lev.factor), host(lo.se.diagnostics["b_ii"]),
+                         lams, Ul, Ug, comps, 64, 16, 3 + O.SE_BII_OFFSET + O.WEAK_MOMENTS_OFFSET)
+    for k in comps:
+        p, q = lo.se.weak[names[k]], ref[k]
+        for key in ("b1", "v_raw", "trace", "quadratic"):
+            assert p[key] == pytest.approx(q[key], rel=1e-7, abs=1e-12), key
+        np.testing.assert_allclose(p["sigma"], q["sigma"], rtol=1e-7, atol=1e-13)
+        theta1 = lo.leave_out[names[k]] - p["lambda_1"] * (p["b1"] ** 2 - p["v_raw"])
+        assert p["theta1"] == pytest.approx(theta1, rel=1e-12)
+        olo, ohi, info = O.am_interval(p["lambda_1"], p["b1"], theta1, p["sigma"], 0.05)
+        assert p["interval"] == pytest.approx((olo, ohi), rel=1e-9, abs=1e-12)
+        assert p["kappa"] == pytest.approx(info["kappa"], rel=1e-9)
+        assert p["interval"][0] < p["interval"][1] and p["confidence"] == 0.95
+        # theta1 = y~'C2 y~ - exactly, for the leave-out sigma2 - so its
+        # plug-in form and the quadratic form agree up to the bias estimate
+        assert abs(p["quadratic"] - theta1) < 0.5 * abs(lo.plug_in[names[k]]) + 1e-3
+    text = lo.summary()
+    assert "q = 1 intervals" in text and "*" in text
+    del torch
+
+
+def test_weak_interval_switches_and_level(bottleneck):
+    from hdfe_stream_amd.leaveout import leave_out_kss
+    d, wide = bottleneck
+    kw = dict(n_draws=128, seed=3, precision="fp64", tol=1e-12, leave_out="observation",
+              se=True, se_draws=64, block=16, diagnose_draws=64)
+    print("weak interval off / at 80%", flush=True)
+    off = leave_out_kss("y ~ 1 | worker_id + firm_id", d, weak_interval=False, **kw)
+    assert off.se.weak is None and "q = 1 intervals" not in off.summary()
+    narrow = leave_out_kss("y ~ 1 | worker_id + firm_id", d, confidence=0.8, **kw)
+    assert "80% interval" in narrow.summary()
+    for nm, piece in narrow.se.weak.items():
+        w_lo, w_hi = wide.se.weak[nm]["interval"]
+        n_lo, n_hi = piece["interval"]
+        assert n_hi - n_lo < w_hi - w_lo
+        # same draws, same moments: only the level moved
+        np.testing.assert_array_equal(piece["sigma"], wide.se.weak[nm]["sigma"])
+
+
+def test_match_level_weak_interval_is_var_psi_alone():
+    from hdfe_stream_amd.leaveout import leave_out_kss
+    from hdfe_stream_amd.simulate import bottleneck_panel
+    d = bottleneck_panel(n_bridge=6, seed=1)
+    print("leave_out_kss match on a bottleneck", flush=True)
+    lo = leave_out_kss("y ~ 1 | worker_id + firm_id", d, n_draws=64, seed=2, se=True,
+                       se_draws=32, block=16, diagnose_draws=48, se_variance="match")
+    if lo.weak_id.q["var(psi)"] >= 1:
+        assert set(lo.se.weak) == {"var(psi)"}
+        lo_, hi_ = lo.se.weak["var(psi)"]["interval"]
+        assert lo_ < hi_
+    else:
+        assert not lo.se.weak
