@@ -54,7 +54,9 @@ SIGNATURES = [
     ("hdfe_rademacher", I32, [I64, I64, I32, C.c_uint64, I64, P]),
     ("hdfe_rows_create", I32, [I64, I64, I32, P, P, P, P]),
     ("hdfe_rows_center", I32, [P, P, P, P, I32, C.c_uint64, I64, I32, P]),
-    ("hdfe_rows_project_moments", I32, [P, P, P, P, I32, C.c_uint64, I64, I32, P, P, P]),
+    ("hdfe_rows_project_moments", I32, [P, P, P, P, I32, C.c_uint64, I64, I32, P, P, I32, P,
+                                         P, P]),
+    ("hdfe_rows_cov_center", I32, [P, I32, P, P, I32, C.c_uint64, I64, I32, P]),
     ("hdfe_jla_combine", I32, [I64, I64, P, P]),
     ("hdfe_rows_destroy", None, [P]),
     ("hdfe_loo_prune", I32, [I64, P, P, P, P]),

@@ -27,6 +27,8 @@ namespace hdfe {
 constexpr int kRowChunk = 1024;  // rows of a level per wave (a year holds millions)
 constexpr int kMaxDraws = 1024;  // draws per block
 constexpr int kDPL = kMaxDraws / 64;
+constexpr int kMaxCov = 64;       // covariates in a leave-out design
+constexpr int kCovBlocks = 512;   // row blocks of the covariate reduction
 
 // The splitmix64-finalised counter hash of (seed, row ordinal, draw); the top
 // bit gives the sign (core/kernels_groups.py:71-77).
@@ -123,18 +125,60 @@ __global__ void k_center_sum(int64_t L, int k, const int *__restrict__ cptr,
     }
 }
 
+// The covariate block of the reduced right-hand side of a block of draws
+// (inverse.py:162-180 with a border): part[b, c, j] = sum over row block b
+// of w_i Xd[i, c] r_ij, Xd the covariates less their weighted group means,
+// so that the sum is X'W M_0 r; then a fixed-order sum over the blocks.
+__global__ __launch_bounds__(256) void k_cov_center(int64_t n, int kc, int k,
+                                                    const double *__restrict__ Xd,
+                                                    const double *__restrict__ w, int sqrtw,
+                                                    uint64_t seed, int64_t first,
+                                                    double *__restrict__ part) {
+    const int64_t per = (n + gridDim.x - 1) / gridDim.x;
+    const int64_t r0 = min(n, (int64_t)blockIdx.x * per), r1 = min(n, r0 + per);
+    for (int j = threadIdx.x; j < k; j += blockDim.x) {
+        for (int c0 = 0; c0 < kc; c0 += 8) {
+            const int cn = min(8, kc - c0);
+            double acc[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+            for (int64_t i = r0; i < r1; ++i) {
+                const double v = (w ? w[i] : 1.0) * row_value(seed, i, first + j, w, sqrtw);
+                const double *x = Xd + i * kc + c0;
+#pragma unroll
+                for (int c = 0; c < 8; ++c)
+                    if (c < cn) acc[c] += v * x[c];
+            }
+            for (int c = 0; c < cn; ++c)
+                part[((int64_t)blockIdx.x * kc + c0 + c) * k + j] = acc[c];
+        }
+    }
+}
+
+__global__ void k_cov_finish(int nb, int64_t m, const double *__restrict__ part,
+                             double *__restrict__ out) {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < m;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        double s = 0.0;
+        for (int b = 0; b < nb; ++b) s += part[(int64_t)b * m + e];
+        out[e] = s;
+    }
+}
+
 // Projection back to rows fused with the JLA moments (kernels_inverse.py:
 // 74-108, leaveout.py:181-190): per group, h_ij = sum_d U[level_d(i), j],
 // u0_j = weighted mean over the group of (r_ij - h_ij), proj = h + u0 (times
-// sqrt(w_i) in the square-root metric); then for each row, over the block's
-// draws, the sums of P = proj^2, M = (r - proj)^2, P^2, M^2 and P M are added
-// to moments[0..4][i].  One wave per group.
+// sqrt(w_i) in the square-root metric); with covariates in the design, plus
+// Xd_i' C_j (Xd the covariates less their weighted group means, C (kc, k)
+// the covariate block of the solution), which is x_i'C with its group mean
+// taken out as u0 takes it.  Then for each row, over the block's draws, the
+// sums of P = proj^2, M = (r - proj)^2, P^2, M^2 and P M are added to
+// moments[0..4][i].  One wave per group.
 __global__ void k_project_moments(int64_t G, int D, int k, const int *__restrict__ gstart,
                                   const int *__restrict__ codes, int64_t ld,
                                   const int64_t *__restrict__ offs, const double *__restrict__ w,
                                   int sqrtw, uint64_t seed, int64_t first,
-                                  const double *__restrict__ U, double *__restrict__ moments,
-                                  int64_t n) {
+                                  const double *__restrict__ U, int kc,
+                                  const double *__restrict__ Xd, const double *__restrict__ Cb,
+                                  double *__restrict__ moments, int64_t n) {
     const int lane = threadIdx.x & 63;
     const int64_t waves = (int64_t)gridDim.x * (blockDim.x / 64);
     for (int64_t g = blockIdx.x * (int64_t)(blockDim.x / 64) + threadIdx.x / 64; g < G;
@@ -164,6 +208,7 @@ __global__ void k_project_moments(int64_t G, int D, int k, const int *__restrict
                 if (j >= k) break;
                 double h = 0.0;
                 for (int d = 0; d < D; ++d) h += U[(offs[d] + codes[d * ld + i]) * k + j];
+                for (int c = 0; c < kc; ++c) h += Xd[(int64_t)i * kc + c] * Cb[c * k + j];
                 const double proj = (h + u0[t]) * root;
                 const double r = rademacher(seed, (uint64_t)i, (uint64_t)(first + j));
                 const double P = proj * proj, dm = r - proj, M = dm * dm;
@@ -304,8 +349,8 @@ struct RowPasses {
     int D = 0;
     DevBuf<int> rptr, rrow, cptr, clvl, cbeg;
     int64_t nchunk = 0;
-    DevBuf<double> Rbar, part;
-    int64_t kcap = 0;
+    DevBuf<double> Rbar, part, cpart;
+    int64_t kcap = 0, ccap = 0;
     Temp tmp;
     hipStream_t s = nullptr;
 };
@@ -420,17 +465,46 @@ int hdfe_rows_center(void *h, const int *g, const int *gstart, const double *w, 
     return HDFE_STATUS_OK;
 }
 
+// The covariate block of a block of draws' reduced right-hand side: out
+// (kc, k) device = X'W M_0 r from Xd (n, kc), the covariates less their
+// weighted group means.
+int hdfe_rows_cov_center(void *h, int kc, const double *Xd, const double *w, int sqrtw,
+                         uint64_t seed, int64_t first_draw, int k, double *out) {
+    ARG_CHECK(h && Xd && out, "hdfe_rows_cov_center: null argument");
+    ARG_CHECK(kc >= 1 && kc <= kMaxCov && k >= 1 && k <= kMaxDraws,
+              "hdfe_rows_cov_center: kc must be in [1, 64] and k in [1, 1024]");
+    RowPasses &R = *(RowPasses *)h;
+    const int64_t m = (int64_t)kc * k;
+    const int nb = (int)std::max<int64_t>(
+        1, std::min<int64_t>({(int64_t)kCovBlocks, (R.n + 255) / 256, (int64_t)(64 << 20) / (8 * m)}));
+    if (R.ccap < nb * m) {
+        DEV_ALLOC(R.cpart, nb * m, R.s);
+        R.ccap = nb * m;
+    }
+    hipLaunchKernelGGL(k_cov_center, dim3(nb), dim3(256), 0, R.s, R.n, kc, k, Xd, w, sqrtw, seed,
+                       first_draw, R.cpart.p);
+    hipLaunchKernelGGL(k_cov_finish, dim3(grid1(m)), dim3(256), 0, R.s, nb, m, R.cpart.p, out);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(R.s));
+    return HDFE_STATUS_OK;
+}
+
 // Back to rows and the JLA moments of the block: moments (5, n) device,
-// accumulated; U (L, k) device, the reduced solution.
+// accumulated; U (L, k) device, the reduced solution; with covariates in the
+// design, Xd (n, kc) and the solution's covariate block Cb (kc, k).
 int hdfe_rows_project_moments(void *h, const int *gstart, const int *codes, const double *w,
                               int sqrtw, uint64_t seed, int64_t first_draw, int k,
-                              const int64_t *offs_d, const double *U, double *moments) {
+                              const int64_t *offs_d, const double *U, int kc, const double *Xd,
+                              const double *Cb, double *moments) {
     ARG_CHECK(h && gstart && codes && U && moments && offs_d, "hdfe_rows_project_moments: null");
     ARG_CHECK(k >= 1 && k <= kMaxDraws, "hdfe_rows_project_moments: k must be in [1, 1024]");
+    ARG_CHECK(kc >= 0 && kc <= kMaxCov && (kc == 0 || (Xd && Cb)),
+              "hdfe_rows_project_moments: kc must be in [0, 64] with Xd and Cb");
     RowPasses &R = *(RowPasses *)h;
     hipStream_t s = R.s;
     hipLaunchKernelGGL(k_project_moments, dim3(grid1(R.G * 64)), dim3(256), 0, s, R.G, R.D, k,
-                       gstart, codes, R.n, offs_d, w, sqrtw, seed, first_draw, U, moments, R.n);
+                       gstart, codes, R.n, offs_d, w, sqrtw, seed, first_draw, U, kc, Xd, Cb,
+                       moments, R.n);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(s));
     return HDFE_STATUS_OK;

@@ -94,21 +94,103 @@ def _rows(panel):
     return rp
 
 
+class DesignBorder:
+    """The covariate border of the leave-out design (inverse.py:182-231).
+
+    With covariates X beside the fixed effects, partialling out the streamed
+    groups leaves the system [[S, T_dx], [T_dx', T_xx]] over (levels,
+    covariates), with T_dx = D_o'W M_0 X and T_xx = X'W M_0 X (M_0 the
+    weighted within-group projector).  The covariate corner is handled by a
+    Schur complement, computed once per design and kept on the panel:
+    Z = S^- T_dx (k solves; a column is zero when its norm is at most
+    1e-11 sqrt(sum w x^2)) and Schur^+ = pinv(T_xx - T_dx'Z).  A reduced
+    right-hand side (r, r_x) then solves as C = Schur^+ (r_x - Z'r),
+    U = S^- r - Z C.  Device tensors: X (n, k) the covariates, Xd (n, k) less
+    their weighted group means, Xg (G, k) their weighted group sums."""
+
+    def __init__(self, panel, cols, tol, maxiter, precision):
+        import torch
+        info = panel.info
+        n = info["n"]
+        self.cols = tuple(int(c) for c in cols)
+        self.kc = len(self.cols)
+        if self.kc > 64:
+            raise ValueError("at most 64 covariates in the leave-out design")
+        inv0 = CoefficientInverse(panel, tol, maxiter, precision)
+        V = panel.tensor("V", torch.float64).view(n, info["m"])
+        self.X = V[:, list(self.cols)].contiguous()
+        g = panel.tensor("g", torch.int32).long()
+        self.Xg = inv0.group_sums(self.X)
+        Wg = torch.where(inv0.Wg > 0, inv0.Wg, torch.ones_like(inv0.Wg))
+        self.Xd = (self.X - (self.Xg / Wg[:, None])[g]).contiguous()
+        w = panel.tensor("w", torch.float64) if info["weighted"] else None
+        T_dx = inv0.reduce(self.kc, R=self.Xd, levels_only=True)
+        wXd = self.Xd if w is None else w[:, None] * self.Xd
+        T_xx = (self.Xd.T @ wXd).cpu().numpy()
+        wx2 = (self.X * self.X) if w is None else w[:, None] * self.X * self.X
+        scale = np.sqrt(wx2.sum(0).cpu().numpy())
+        torch.cuda.synchronize()
+        zero = _col_norms(T_dx, info["L"], self.kc) <= RHS_ZERO_TOL * scale
+        Z, it, conv, rel = panel.matrix.solve(T_dx, zero, None, tol=tol, maxiter=maxiter,
+                                              precision=precision)
+        torch.cuda.synchronize()
+        self.Z = Z
+        schur = T_xx - (T_dx.T @ Z).cpu().numpy()
+        self.schur_inv = torch.from_numpy(np.linalg.pinv(schur, hermitian=True)).cuda()
+        self.info = {"iterations": it, "converged": bool(conv), "max_rel_residual": rel}
+
+    def solve(self, rhs, rhs_x, U):
+        """(U - Z C, C) for the reduced solution U = S^- rhs."""
+        C = self.schur_inv @ (rhs_x - self.Z.T @ rhs)
+        return (U - self.Z @ C).contiguous(), C.contiguous()
+
+
+def design_border(panel, cols, tol=1e-10, maxiter=5000, precision="mixed"):
+    """The panel's `DesignBorder` for covariate columns `cols` (V's columns,
+    1-based: column 0 is the outcome), built on first use; None for none."""
+    cols = tuple(int(c) for c in cols)
+    if not cols:
+        return None
+    cache = getattr(panel, "_borders", None)
+    if cache is None:
+        cache = panel._borders = {}
+    if cols not in cache:
+        cache[cols] = DesignBorder(panel, cols, tol, maxiter, precision)
+    return cache[cols]
+
+
+def design_columns(result, covariates=None):
+    """V's columns of the covariates in the leave-out design: the fit's own
+    by default (leaveout.py:111-127), none for (), or those named."""
+    names = list(result.coefnames)
+    if covariates is None:
+        return tuple(range(1, len(names) + 1))
+    cols = []
+    for c in covariates:
+        if c not in names:
+            raise ValueError(f"{c!r} is not a covariate of the fit; it has {names}")
+        cols.append(names.index(c) + 1)
+    return tuple(cols)
+
+
 def jla_leverages(panel, n_draws: int = 250, block: int | None = None, seed: int = 0,
                   tol: float = 1e-10, maxiter: int = 5000, precision: str = "mixed",
-                  sqrt_weights: bool | None = None, timings: dict | None = None) -> JLALeverages:
+                  sqrt_weights: bool | None = None, timings: dict | None = None,
+                  covariates=()) -> JLALeverages:
     """Every row's leverage by random projection (leaveout.py:130-198).
 
     `panel` is a fitted estimator's Panel (its rows, S and solver stay on the
     device).  `block` draws share one block solve (default: all of them, up
     to 1024).  With weights the leverages are those of the square-root-weight
-    hat matrix, as in the reference."""
+    hat matrix, as in the reference.  `covariates` are V's columns of the
+    covariates that join the design (the border, inverse.py:263-281)."""
     import torch
     block = min(int(n_draws), 1024) if block is None else int(block)
     if block < 1 or n_draws < 1:
         raise ValueError("n_draws and block must be at least 1")
     if panel.matrix is None:
         panel.system(with_rhs=False)
+    bd = design_border(panel, covariates, tol, maxiter, precision)
     lib = _lib.load()
     info = panel.info
     n, L = info["n"], info["L"]
@@ -139,14 +221,24 @@ def jla_leverages(panel, n_draws: int = 250, block: int | None = None, seed: int
         bn = np.empty(size)
         _lib.check(lib.hdfe_col_norms(rhs.data_ptr(), L, size, bn.ctypes.data), "hdfe_col_norms")
         zero = bn <= RHS_ZERO_TOL * scale
+        rhs_x = Cb = None
+        if bd is not None:
+            rhs_x = torch.empty((bd.kc, size), dtype=torch.float64, device="cuda")
+            _lib.check(lib.hdfe_rows_cov_center(rp._h, bd.kc, bd.Xd.data_ptr(), w, sqrtw, seed,
+                                                drawn, size, rhs_x.data_ptr()),
+                       "hdfe_rows_cov_center")
         t1 = time.perf_counter()
         U, it, c, r = panel.matrix.solve(rhs, zero, None, tol=tol, maxiter=maxiter,
                                          precision=precision)
+        if bd is not None:
+            U, Cb = bd.solve(rhs, rhs_x, U)
         torch.cuda.synchronize()
         t2 = time.perf_counter()
-        _lib.check(lib.hdfe_rows_project_moments(rp._h, gstart, codes, w, sqrtw, seed, drawn, size,
-                                                 offs_d, U.data_ptr(), moments.data_ptr()),
-                   "hdfe_rows_project_moments")
+        _lib.check(lib.hdfe_rows_project_moments(
+            rp._h, gstart, codes, w, sqrtw, seed, drawn, size, offs_d, U.data_ptr(),
+            0 if bd is None else bd.kc, None if bd is None else bd.Xd.data_ptr(),
+            None if Cb is None else Cb.data_ptr(), moments.data_ptr()),
+            "hdfe_rows_project_moments")
         t3 = time.perf_counter()
         t_rows += (t1 - t0) + (t3 - t2)
         t_solve += t2 - t1
@@ -171,7 +263,7 @@ def jla_leverages(panel, n_draws: int = 250, block: int | None = None, seed: int
                      "leave_one_out_connected": bool(stats[1] < 1.0),
                      "raw_sum_deviation": float(stats[3]), "solver": "explicit",
                      "converged": bool(conv), "iterations": its, "max_rel_residual": rel,
-                     "precision": precision, "covariates": []})
+                     "precision": precision, "covariates": list(covariates)})
 
 
 # ------------------------------------------------------------ bias term
@@ -188,10 +280,12 @@ def _col_norms(X, rows, k):
 
 class CoefficientInverse:
     """u = S^- v for coefficient-space blocks of a fitted Panel
-    (`apply_inverse`, inverse.py:311-374): levels (L, k) and a streamed block
-    (G, k), device tensors."""
+    (`apply_inverse`, inverse.py:311-374): a level block and a streamed block
+    (G, k), device tensors.  With covariates in the design (`covariates`,
+    V's columns) the level block is (K, k), K = L + k_cov: the fixed-effect
+    levels, then the covariates; without, K = L."""
 
-    def __init__(self, panel, tol=1e-10, maxiter=5000, precision="mixed"):
+    def __init__(self, panel, tol=1e-10, maxiter=5000, precision="mixed", covariates=()):
         import torch
         if panel.matrix is None:
             panel.system(with_rhs=False)
@@ -216,6 +310,12 @@ class CoefficientInverse:
                    "hdfe_kss_group_weight")
         self.rows_state = _rows(panel)
         self.iterations, self.converged, self.max_rel = [], True, 0.0
+        self.border = design_border(panel, covariates, tol, maxiter, precision)
+        self.kc = 0 if self.border is None else self.border.kc
+        self.K = self.L + self.kc
+        if self.kc:
+            self.g_long = panel.tensor("g", torch.int32).long()
+            self.Wg_safe = torch.where(self.Wg > 0, self.Wg, torch.ones_like(self.Wg))
 
     def drop_null(self, levels, streamed):
         """In place: the component along S's null space removed
@@ -226,15 +326,27 @@ class CoefficientInverse:
             self.offs.ctypes.data, self.nlev.ctypes.data, streamed.data_ptr(),
             levels.data_ptr()), "hdfe_kss_drop_null")
 
-    def reduce(self, k, R=None, V=None, x=None):
-        """D_o' W diag(x) R for rows R (n, k), or D_o' W rho for the spread
-        streamed block V (G, k)."""
+    def reduce(self, k, R=None, V=None, x=None, levels_only=False):
+        """A'W diag(x) R onto the level block for rows R (n, k), or A'W rho
+        for the spread streamed block V (G, k), rho_i = V[g(i)] / W_g: D_o'
+        then, with covariates and not `levels_only`, X' (K rows)."""
         import torch
-        out = torch.empty((self.L, k), dtype=torch.float64, device="cuda")
+        kc = 0 if levels_only else self.kc
+        out = torch.empty((self.L + kc, k), dtype=torch.float64, device="cuda")
         _lib.check(_lib.load().hdfe_kss_reduce(
             self.rows_state._h, k, self.g, self.w, None if x is None else x.data_ptr(),
             None if R is None else R.data_ptr(), None if V is None else V.data_ptr(),
             self.Wg.data_ptr(), out.data_ptr()), "hdfe_kss_reduce")
+        if kc:
+            bd = self.border
+            if R is not None:
+                wx = torch.ones(self.n, dtype=torch.float64, device="cuda") if self.w is None \
+                    else self.panel.tensor("w", torch.float64)
+                if x is not None:
+                    wx = wx * x
+                out[self.L:] = bd.X.T @ (wx[:, None] * R)
+            else:
+                out[self.L:] = bd.Xg.T @ (V / self.Wg_safe[:, None])
         return out
 
     def group_sums(self, R, x=None):
@@ -247,20 +359,28 @@ class CoefficientInverse:
         return out
 
     def __call__(self, levels, streamed, want_rows=True):
-        """(U_levels (L, k), U_streamed (G, k), rows (A u)_i (n, k) or None)."""
+        """(U (K, k), U_streamed (G, k), rows (A u)_i (n, k) or None).
+        `levels` has L or K rows (a missing covariate block is zero)."""
         import torch
-        levels = levels.to(torch.float64).contiguous().clone()
+        levels = levels.to(torch.float64)
+        cov = levels[self.L:] if (self.kc and levels.shape[0] > self.L) else None
+        levels = levels[:self.L].contiguous().clone()
         streamed = streamed.to(torch.float64).contiguous().clone()
         k = levels.shape[1]
         torch.cuda.synchronize()
         self.drop_null(levels, streamed)
         scale = np.sqrt(_col_norms(levels, self.L, k) ** 2 + _col_norms(streamed, self.G, k) ** 2)
-        rhs = levels - self.reduce(k, V=streamed)
+        red = self.reduce(k, V=streamed)
+        rhs = (levels - red[:self.L]).contiguous()
         torch.cuda.synchronize()
         zero = _col_norms(rhs, self.L, k) <= RHS_ZERO_TOL * scale
         U, it, conv, rel = self.panel.matrix.solve(rhs, zero, None, tol=self.tol,
                                                    maxiter=self.maxiter,
                                                    precision=self.precision)
+        Cb = None
+        if self.kc:
+            rhs_x = -red[self.L:] if cov is None else cov - red[self.L:]
+            U, Cb = self.border.solve(rhs, rhs_x, U)
         torch.cuda.synchronize()
         self.iterations.append(it)
         self.converged &= conv
@@ -272,7 +392,16 @@ class CoefficientInverse:
             self.G, self.D, k, self.gstart, self.codes, self.n, self.offs_d, self.w,
             self.Wg.data_ptr(), U.data_ptr(), streamed.data_ptr(),
             None if rows is None else rows.data_ptr(), Ug.data_ptr()), "hdfe_kss_project")
-        return U, Ug, rows
+        if Cb is None:
+            return U, Ug, rows
+        # the covariates' share: h gains x_i'C, so the streamed block loses
+        # its weighted group mean, and the rows gain (x_i - x_bar_g)'C
+        bd = self.border
+        shift = (bd.Xg @ Cb) / self.Wg_safe[:, None]
+        Ug -= shift
+        if rows is not None:
+            rows += bd.Xd @ Cb
+        return torch.cat([U, Cb]), Ug, rows
 
 
 def _moment_divisor(total, rows):
@@ -297,7 +426,7 @@ class TraceEstimate:
 def hutchinson_trace(panel, sigma2, n_draws: int = 250, block: int | None = None,
                      seed: int = 0, psi: int = 0, tol: float = 1e-10, maxiter: int = 5000,
                      precision: str = "mixed", timings: dict | None = None,
-                     person_years: int | None = None) -> TraceEstimate:
+                     person_years: int | None = None, covariates=()) -> TraceEstimate:
     """tr(Q S^- A'W Omega A S^-) for var(psi), var(alpha) and their
     covariance by Hutchinson's estimator (leaveout.py:321-448).
 
@@ -305,12 +434,14 @@ def hutchinson_trace(panel, sigma2, n_draws: int = 250, block: int | None = None
     order); `psi` the index of the non-streamed dimension whose variance is
     wanted (0: the first).  Per block of draws: Z (levels, then groups) from
     the counter-based generator, u = S^- Z with its rows, v = A'W Sigma A u,
-    u2 = S^- v, and the eight moments of (Z, u2)."""
+    u2 = S^- v, and the eight moments of (Z, u2).  With `covariates` (V's
+    columns) in the design the draws take positions levels, groups, then
+    covariates (leaveout.py:410-419)."""
     import torch
     block = min(int(n_draws), 256) if block is None else int(block)
     if block < 1 or n_draws < 1:
         raise ValueError("n_draws and block must be at least 1")
-    inv = CoefficientInverse(panel, tol, maxiter, precision)
+    inv = CoefficientInverse(panel, tol, maxiter, precision, covariates=covariates)
     if not 0 <= psi < inv.D:
         raise ValueError(f"psi must index a non-streamed dimension (0..{inv.D - 1})")
     lib = _lib.load()
@@ -322,11 +453,12 @@ def hutchinson_trace(panel, sigma2, n_draws: int = 250, block: int | None = None
     t0 = time.perf_counter()
     while drawn < n_draws:
         size = min(block, n_draws - drawn)
-        Z = torch.empty((L + G, size), dtype=torch.float64, device="cuda")
+        Z = torch.empty((L + G + inv.kc, size), dtype=torch.float64, device="cuda")
         torch.cuda.synchronize()
-        _lib.check(lib.hdfe_rademacher(0, L + G, size, seed, drawn, Z.data_ptr()),
+        _lib.check(lib.hdfe_rademacher(0, L + G + inv.kc, size, seed, drawn, Z.data_ptr()),
                    "hdfe_rademacher")
-        Zl, Zg = Z[:L], Z[L:]
+        Zl, Zg = (Z[:L], Z[L:L + G]) if not inv.kc else \
+            (torch.cat([Z[:L], Z[L + G:]]), Z[L:L + G].contiguous())
         _, _, rows = inv(Zl, Zg)
         vl = inv.reduce(size, R=rows, x=sig)
         vg = inv.group_sums(rows, x=sig)
@@ -460,7 +592,7 @@ def leave_out_components(result, n_draws: int = 250, block: int | None = None, s
                          se_trace: bool = True, smooth_bins=None,
                          se_variance: str = "person_year", match_within=None,
                          diagnose=None, diagnose_draws: int = 64, weak_interval: bool = True,
-                         confidence: float = 0.95) -> LeaveOutComponents:
+                         confidence: float = 0.95, covariates=None) -> LeaveOutComponents:
     """KSS components for a fit (leaveout.py:634-805): JLA leverages, the
     leverage check, the plug-in moments, sigma2, the Hutchinson bias with
     seed + 1_000_003, and plug-in minus bias.
@@ -484,7 +616,11 @@ def leave_out_components(result, n_draws: int = 250, block: int | None = None, s
     With `se` and `weak_interval`, each component it flags (q >= 1) also
     gets KSS's q = 1 interval at `confidence` in se.weak[name]["interval"]:
     the Andrews-Mikusheva projection of the ellipse of (b1, theta1), theta1
-    = leave-out - lambda_1 (b1^2 - V[b1] raw) (leaveout.py:777-793)."""
+    = leave-out - lambda_1 (b1^2 - V[b1] raw) (leaveout.py:777-793).
+    `covariates` join the fixed effects in the leave-out design: the fit's
+    own by default, () for the fixed effects alone, or a list of names
+    (leaveout.py:111-127); the solves then go through the border's Schur
+    complement (`DesignBorder`)."""
     if not 0.0 < confidence < 1.0:
         raise ValueError("confidence must lie in (0, 1)")
     if stayers not in ("own", "within_match"):
@@ -496,10 +632,8 @@ def leave_out_components(result, n_draws: int = 250, block: int | None = None, s
     import torch
     panel = result.panel
     info = panel.info
-    if info["m"] != 1:
-        raise NotImplementedError(
-            "leave-out components with covariates need the covariates in the design "
-            "(the Schur border); fit the fixed effects alone, as 'y ~ 1 | worker + firm'")
+    cols = design_columns(result, covariates)
+    cov_names = [result.coefnames[c - 1] for c in cols]
     alpha = result.fe_names[0]
     others = result.fe_names[1:]
     psi = psi or others[0]
@@ -511,7 +645,8 @@ def leave_out_components(result, n_draws: int = 250, block: int | None = None, s
     t = {} if timings is None else timings
     t0 = time.perf_counter()
     lev = jla_leverages(panel, n_draws=n_draws, block=block, seed=seed, tol=tol,
-                        maxiter=maxiter, precision=precision, timings=t)
+                        maxiter=maxiter, precision=precision, timings=t, covariates=cols)
+    lev.diagnostics["covariates"] = cov_names
     t1 = time.perf_counter()
     movers = _movers(panel, pi)
     match_level = stayers == "within_match"
@@ -556,7 +691,7 @@ def leave_out_components(result, n_draws: int = 250, block: int | None = None, s
     t2 = time.perf_counter()
     trace = hutchinson_trace(panel, sigma2, n_draws=n_draws, block=block,
                              seed=seed + TRACE_SEED_OFFSET, psi=pi, tol=tol, maxiter=maxiter,
-                             precision=precision, timings=t, person_years=R)
+                             precision=precision, timings=t, person_years=R, covariates=cols)
     t3 = time.perf_counter()
     bias = trace.as_dict()
     leave_out = {k: plug_in[k] - bias[k] for k in plug_in}
@@ -569,17 +704,18 @@ def leave_out_components(result, n_draws: int = 250, block: int | None = None, s
     if bool(se) if diagnose is None else diagnose:
         from .leaveout_weakid import SEED_OFFSET, weak_id_diagnostics
         y_d = panel.tensor("V", torch.float64).view(n, info["m"])[:, 0].contiguous()
-        weak_id = weak_id_diagnostics(CoefficientInverse(panel, tol, maxiter, precision), y_d,
+        weak_id = weak_id_diagnostics(CoefficientInverse(panel, tol, maxiter, precision,
+                                                         covariates=cols), y_d,
                                       sigma2, lev, psi=pi, rows=R, n_draws=int(diagnose_draws),
                                       block=8 if block is None else int(block),
                                       seed=seed + SEED_OFFSET)
-        weak_id.diagnostics.update(psi=psi, alpha=alpha, covariates=[])
+        weak_id.diagnostics.update(psi=psi, alpha=alpha, covariates=cov_names)
     errors = None
     if se:
         from .leaveout_se import BII_SEED_OFFSET, component_variances
         if se_variance not in ("person_year", "match"):
             raise ValueError("se_variance must be 'person_year' or 'match'")
-        inv = CoefficientInverse(panel, tol, maxiter, precision)
+        inv = CoefficientInverse(panel, tol, maxiter, precision, covariates=cols)
         y = panel.tensor("V", torch.float64).view(n, info["m"])[:, 0].contiguous()
         sigma2_se, options = sigma2, {}
         if match_level:
@@ -605,7 +741,7 @@ def leave_out_components(result, n_draws: int = 250, block: int | None = None, s
             n_draws=n_draws if se_draws is None else int(se_draws), block=block,
             seed=seed + BII_SEED_OFFSET, trace=se_trace, smooth_bins=smooth_bins,
             weak_id=weak_id if weak_interval else None, **options)
-        errors.diagnostics.update(psi=psi, alpha=alpha, covariates=[],
+        errors.diagnostics.update(psi=psi, alpha=alpha, covariates=cov_names,
                                   se_variance=se_variance if match_level else None,
                                   sigma2=sigma2_se)
         if errors.weak:
@@ -627,7 +763,7 @@ def leave_out_components(result, n_draws: int = 250, block: int | None = None, s
                      "confidence": confidence,
                      "leave_out": "match" if match_level else "observation",
                      "centering": centering, "sigma2_mean": sigma2_mean,
-                     "ybar": float(ybar.value), "covariates": [],
+                     "ybar": float(ybar.value), "covariates": cov_names,
                      "n_levels": {psi: panel.n_levels[pi], alpha: G},
                      "sigma2": sigma2})
 

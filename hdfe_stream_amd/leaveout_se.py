@@ -85,6 +85,7 @@ class SELayout:
         panel = inv.panel
         n = inv.n
         self.n, self.L, self.G = n, inv.L, inv.G
+        self.K = inv.K                      # levels, then the design's covariates
         if inv.w is not None:
             w = panel.tensor("w", torch.float64)
         else:
@@ -106,7 +107,8 @@ class SELayout:
         self.psi_codes = inv.codes + 4 * psi * n
 
     def reduce(self, R):
-        """X~'R for rows R (n, k): (levels (L, k), groups (G, k))."""
+        """X~'R for rows R (n, k): (levels and covariates (K, k), groups
+        (G, k))."""
         x = self.inv_root if self.weighted else None
         R = R.contiguous()
         return self.inv.reduce(R.shape[1], R=R, x=x), self.inv.group_sums(R, x=x)
@@ -152,7 +154,7 @@ def apply_q(lay: SELayout, k: int, Ul, Ug):
     dimensions over the rows (kernels_se.py:44-72)."""
     import torch
     s = Ul.shape[1]
-    ol = torch.zeros((lay.L, s), dtype=torch.float64, device="cuda")
+    ol = torch.zeros((lay.K, s), dtype=torch.float64, device="cuda")
     og = torch.zeros((lay.G, s), dtype=torch.float64, device="cuda")
     lw = lay.level_w[lay.psi]
     psi_mean = (lw[None, :] @ Ul[lay.psi])[0] / lay.total
@@ -218,7 +220,7 @@ def apply_c(lay: SELayout, b, v=None, z=None, s3=None, rank_two=None, yt=None, d
     lv, gs = lay.reduce(stack)
     U0l, U0g, base = lay.inv(lv, gs)
     del lv, gs
-    ql = torch.zeros((lay.L, 3 * s), dtype=torch.float64, device="cuda")
+    ql = torch.zeros((lay.K, 3 * s), dtype=torch.float64, device="cuda")
     qg = torch.zeros((lay.G, 3 * s), dtype=torch.float64, device="cuda")
     for k in range(3):
         j = k if per else 0
@@ -310,7 +312,7 @@ def smoothed_sigma2(sigma2, leverage, b_ii=None, n_bins=None):
 
 
 def eigen_rows(lay: SELayout, Ul, Ug):
-    """x1 = X~ u: sqrt(w_i) (A u)_i for coefficient blocks Ul (L, c) and
+    """x1 = X~ u: sqrt(w_i) (A u)_i for coefficient blocks Ul (K, c) and
     Ug (G, c) (leaveout_se.py:486-495); device (n, c)."""
     import torch
     inv = lay.inv
@@ -320,6 +322,8 @@ def eigen_rows(lay: SELayout, Ul, Ug):
     au = Ug[g]
     for d in range(inv.D):
         au = au + Ul[int(inv.offs[d]) + codes[d]]
+    if inv.kc:
+        au = au + inv.border.X @ Ul[inv.L:]
     return au * lay.root[:, None]
 
 

@@ -80,7 +80,8 @@ def _q_block(lay, comps, Ul, Ug):
     """Q applied column by column, column j with component comps[j]."""
     import torch
     comps = np.asarray(comps)
-    ol, og = torch.zeros_like(Ul), torch.zeros_like(Ug)
+    ol = torch.zeros((lay.K, Ul.shape[1]), dtype=torch.float64, device="cuda")
+    og = torch.zeros_like(Ug)
     for k in range(3):
         cols = np.flatnonzero(comps == k)
         if len(cols):
@@ -104,7 +105,7 @@ def _krylov_plan(lay, max_iter):
     """Steps per component and whether to keep its basis
     (leaveout_weakid.py:186-215)."""
     n_psi = lay.psi.stop - lay.psi.start
-    dims = [max(n_psi - 1, 0), max(lay.L + 1, 1), max(2 * (n_psi - 1), 0)]
+    dims = [max(n_psi - 1, 0), max(lay.K + 1, 1), max(2 * (n_psi - 1), 0)]
     support = [n_psi, lay.G, n_psi + lay.G]
     caps = [min(max_iter, d) for d in dims]
     reorth = [cap * 2 * rows * 8 <= SCRATCH_BYTES for cap, rows in zip(caps, support)]
@@ -163,7 +164,7 @@ def _lanczos(lay, n_iter, seed, ritz=None, check=None, every=20, caps=None, reor
     inv_norm = np.where(norm > 0, 1.0 / np.where(norm > 0, norm, 1.0), 0.0)
     scale(x, inv_norm, live)
     scale(y, inv_norm, live)
-    x_prev = [torch.zeros((L, 3), dtype=torch.float64, device="cuda"),
+    x_prev = [torch.zeros((lay.K, 3), dtype=torch.float64, device="cuda"),
               torch.zeros((G, 3), dtype=torch.float64, device="cuda")]
     beta_prev = np.zeros(3)
     basis = [([], []) for _ in comps]
@@ -172,9 +173,9 @@ def _lanczos(lay, n_iter, seed, ritz=None, check=None, every=20, caps=None, reor
     acc = None
     if ritz is not None:
         k = ritz[0].shape[1]
-        acc = ([torch.zeros((L, 3 * k), dtype=torch.float64, device="cuda"),
+        acc = ([torch.zeros((lay.K, 3 * k), dtype=torch.float64, device="cuda"),
                 torch.zeros((G, 3 * k), dtype=torch.float64, device="cuda")],
-               [torch.zeros((L, 3 * k), dtype=torch.float64, device="cuda"),
+               [torch.zeros((lay.K, 3 * k), dtype=torch.float64, device="cuda"),
                 torch.zeros((G, 3 * k), dtype=torch.float64, device="cuda")])
         ritz_d = [torch.from_numpy(np.ascontiguousarray(r)).cuda() for r in ritz]
     beta_max = np.zeros(3)
@@ -371,7 +372,7 @@ def weak_id_diagnostics(inv, y, sigma2, leverages, *, psi: int = 0, rows: int | 
     alphas, betas, lengths, n_iter = _lanczos(lay, max_iter, seed, check=converged_all, every=10,
                                               caps=caps, reorth=reorth)
     checked = [_ritz_top(alphas[c], betas[c], top) for c in range(3)]
-    k = top if 2 * 3 * top * (lay.L + lay.G) * 8 <= SCRATCH_BYTES else 1
+    k = top if 2 * 3 * top * (lay.K + lay.G) * 8 <= SCRATCH_BYTES else 1
     ritz = []
     for _t, _b, S in checked:
         coef = np.zeros((S.shape[0], k))

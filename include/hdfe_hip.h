@@ -205,10 +205,21 @@ int hdfe_rows_center(void *h, const int *g, const int *gstart, const double *w, 
 
 /* The projection back to rows (A S^- A' r)_i from the reduced solution U
  * (L, k), fused with the JLA sink: moments (5, n) += the block's sums of P,
- * M, P^2, M^2, P M per row (kernels_inverse.py:74-108, leaveout.py:181-190). */
+ * M, P^2, M^2, P M per row (kernels_inverse.py:74-108, leaveout.py:181-190).
+ * With covariates in the design (kc >= 1): Xd (n, kc) the covariates less
+ * their weighted group means and Cb (kc, k) the solution's covariate block
+ * (the x_i'C term of _nb_project_rows). */
 int hdfe_rows_project_moments(void *h, const int *gstart, const int *codes, const double *w,
                               int sqrtw, uint64_t seed, int64_t first_draw, int k,
-                              const int64_t *offs_d, const double *U, double *moments);
+                              const int64_t *offs_d, const double *U, int kc, const double *Xd,
+                              const double *Cb, double *moments);
+
+/* The covariate block of a block of draws' reduced right-hand side, out
+ * (kc, k) = X'W M_0 r = sum_i w_i Xd_i r_i' (the acc_x of
+ * _nb_center_rows, kernels_inverse.py:34-71, inverse.py:162-180); the draws
+ * as hdfe_rows_center's. */
+int hdfe_rows_cov_center(void *h, int kc, const double *Xd, const double *w, int sqrtw,
+                         uint64_t seed, int64_t first_draw, int k, double *out);
 
 /* Leverages from the moments, in place: rows 0, 1, 2 become P_bar, M_bar and
  * the non-linearity factor (leaveout.py:209-282); stats (host, 4) = min, max,

@@ -465,32 +465,57 @@ def combine_moments(moments: np.ndarray, n_draws: int):
     return P_bar, M_bar, factor
 
 
+def border(p0: dict, c: dict, S, dinv, X: np.ndarray, tol: float = 1e-10,
+           maxiter: int = 5000):
+    """The covariate border of the reduced system (inverse.py:182-231):
+    T_dx = D_o'W M_0 X and T_xx = X'W M_0 X from one centring pass of X
+    itself, Z = S^- T_dx (k solves; a column is zero when its norm is at most
+    1e-11 sqrt(sum w x^2)), and pinv of the Schur complement T_xx - T_dx'Z.
+    Returns (Z, Schur^+)."""
+    g, codes = p0["g"], p0["codes"]
+    n = len(g)
+    starts = np.concatenate(([0], np.flatnonzero(g[1:] != g[:-1]) + 1, [n])).astype(np.int64)
+    w = np.ones(n) if p0["w"] is None else np.asarray(p0["w"], np.float64)
+    X = np.ascontiguousarray(X, np.float64)
+    T_dx, T_xx = center_rows(starts, codes, c["offs"], w, X, X, c["L"])
+    scale = np.sqrt(w @ (X * X))
+    zero = np.linalg.norm(T_dx, axis=0) <= RHS_ZERO_TOL * scale
+    Z, _, _, _ = pcg(S, dinv, T_dx, zero, None, tol, maxiter)
+    return Z, np.linalg.pinv(T_xx - T_dx.T @ Z, hermitian=True)
+
+
 def jla_leverages(p0: dict, c: dict, S, dinv, n_draws: int, block: int, seed: int = 0,
-                  tol: float = 1e-10, maxiter: int = 5000):
-    """Leverages of the fixed-effect design (no covariates) by the
-    reference's JLA: per block, centre, solve, project, accumulate the five
-    moments (leaveout.py:130-198, inverse.py:240-309).  With weights the
-    operator is the square-root-weight one, W^1/2 A S^- A' W^1/2: the draws
-    enter divided by sqrt(w) and the projection leaves multiplied by it
-    (inverse.py:299-305, leaveout.py:179)."""
+                  tol: float = 1e-10, maxiter: int = 5000, X=None):
+    """Leverages by the reference's JLA: per block, centre, solve, project,
+    accumulate the five moments (leaveout.py:130-198, inverse.py:240-309).
+    With weights the operator is the square-root-weight one,
+    W^1/2 A S^- A' W^1/2: the draws enter divided by sqrt(w) and the
+    projection leaves multiplied by it (inverse.py:299-305, leaveout.py:179).
+    With covariates X (n, k) the design is [X, fixed effects]: the border's
+    Schur complement on the covariate corner (inverse.py:263-281)."""
     g, codes = p0["g"], p0["codes"]
     n = len(g)
     starts = np.concatenate(([0], np.flatnonzero(g[1:] != g[:-1]) + 1, [n])).astype(np.int64)
     weighted = p0["w"] is not None
     w = np.ones(n) if not weighted else np.asarray(p0["w"], np.float64)
     root = np.sqrt(w)
-    X0 = np.zeros((n, 0))
+    X0 = np.zeros((n, 0)) if X is None else np.ascontiguousarray(X, np.float64)
+    bz = border(p0, c, S, dinv, X0, tol, maxiter) if X0.shape[1] else None
     moments = np.zeros((5, n))
     drawn = 0
     while drawn < n_draws:
         size = min(block, n_draws - drawn)
         R = rademacher(0, n, size, seed, drawn)
         Rin = R / root[:, None] if weighted else R
-        acc, _ = center_rows(starts, codes, c["offs"], w, X0, Rin, c["L"])
+        acc, acc_x = center_rows(starts, codes, c["offs"], w, X0, Rin, c["L"])
         scale = np.sqrt(w @ (Rin * Rin))
         zero = np.linalg.norm(acc, axis=0) <= RHS_ZERO_TOL * scale
         U, _, _, _ = pcg(S, dinv, acc, zero, None, tol, maxiter)
-        proj = project_rows(starts, codes, c["offs"], w, X0, np.zeros((0, size)), Rin, U)
+        Cm = np.zeros((0, size))
+        if bz is not None:
+            Cm = bz[1] @ (acc_x - bz[0].T @ acc)
+            U = U - bz[0] @ Cm
+        proj = project_rows(starts, codes, c["offs"], w, X0, Cm, Rin, U)
         if weighted:
             proj = proj * root[:, None]
         jla_moments(R, proj, moments)
@@ -519,7 +544,7 @@ class KSSContext:
     weights, the reduced S with its Jacobi inverse, and the null-space basis
     of the full coefficient space (inverse.py:376-433)."""
 
-    def __init__(self, fitres: dict, tol: float = 1e-10, maxiter: int = 5000):
+    def __init__(self, fitres: dict, tol: float = 1e-10, maxiter: int = 5000, X=None):
         p0, c = fitres["p0"], fitres["cells"]
         self.p0, self.c = p0, c
         self.Dg, self.Do = design(p0, c)
@@ -547,6 +572,13 @@ class KSSContext:
             vecs.append(np.concatenate([np.zeros(self.G), lv]))
         q, _ = np.linalg.qr(np.column_stack(vecs))
         self.q_groups, self.q_levels = q[:self.G], q[self.G:]
+        # covariates of the design: the level block of coefficient space is
+        # [levels (L); covariates (k)], K = L + k rows
+        self.X = np.zeros((n, 0)) if X is None else np.ascontiguousarray(X, np.float64)
+        self.kc = self.X.shape[1]
+        self.K = self.L + self.kc
+        self.border = (border(p0, c, self.S, self.dinv, self.X, tol, maxiter) if self.kc
+                       else None)
 
 
 def drop_null(ctx: KSSContext, levels: np.ndarray, streamed: np.ndarray):
@@ -557,21 +589,32 @@ def drop_null(ctx: KSSContext, levels: np.ndarray, streamed: np.ndarray):
 
 def apply_inverse(ctx: KSSContext, levels: np.ndarray, streamed: np.ndarray):
     """u = S^- v for a coefficient-space block (inverse.py:311-374): drop the
-    null component, reduce the spread streamed block onto the levels
-    (kernels_inverse.py:111-152), solve, then recover the streamed block and
-    the rows (A u)_i (kernels_inverse.py:156-188).  Returns (U_levels,
-    U_groups, rows)."""
-    levels, streamed = drop_null(ctx, np.asarray(levels, np.float64),
-                                 np.asarray(streamed, np.float64))
-    scale = np.sqrt((levels * levels).sum(axis=0) + (streamed * streamed).sum(axis=0))
+    null component, reduce the spread streamed block onto the levels and the
+    covariates (kernels_inverse.py:111-152), solve with the border's Schur
+    complement, then recover the streamed block and the rows (A u)_i
+    (kernels_inverse.py:156-188).  `levels` has L rows, or K = L + k with
+    the covariate block below (missing: zero).  Returns (U (K, m), U_groups,
+    rows)."""
+    levels = np.asarray(levels, np.float64)
+    lev, cov = levels[:ctx.L], levels[ctx.L:]
+    lev, streamed = drop_null(ctx, lev, np.asarray(streamed, np.float64))
+    scale = np.sqrt((lev * lev).sum(axis=0) + (streamed * streamed).sum(axis=0))
     Wg = np.where(ctx.Wg > 0, ctx.Wg, 1.0)
     rho = (streamed / Wg[:, None])[ctx.g]
-    rhs = levels - ctx.Do.T @ (ctx.w[:, None] * rho)
+    rhs = lev - ctx.Do.T @ (ctx.w[:, None] * rho)
     zero = np.linalg.norm(rhs, axis=0) <= RHS_ZERO_TOL * scale
     U, _, _, _ = pcg(ctx.S, ctx.dinv, rhs, zero, None, ctx.tol, ctx.maxiter)
-    h = ctx.Do @ U
+    Cm = np.zeros((ctx.kc, levels.shape[1]))
+    if ctx.kc:
+        rhs_x = -(ctx.X.T @ (ctx.w[:, None] * rho))
+        if cov.shape[0]:
+            rhs_x = rhs_x + cov
+        Z, sinv = ctx.border
+        Cm = sinv @ (rhs_x - Z.T @ rhs)
+        U = U - Z @ Cm
+    h = ctx.Do @ U + ctx.X @ Cm
     u0 = (ctx.Dg.T @ (ctx.w[:, None] * (rho - h))) / Wg[:, None]
-    return U, u0, h + u0[ctx.g]
+    return np.vstack([U, Cm]), u0, h + u0[ctx.g]
 
 
 def trace_moments(ctx: KSSContext, psi_col: int, Zl, Zg, Ul, Ug) -> np.ndarray:
@@ -604,17 +647,19 @@ def hutchinson_trace(ctx: KSSContext, sigma2: np.ndarray, n_draws: int, block: i
                      seed: int, psi_col: int = 0, n_rows: int | None = None):
     """tr(Q S^- A'W Omega A S^-) for the three components by Hutchinson's
     estimator with coefficient-space Rademacher draws keyed on (position,
-    draw), levels first then groups (leaveout.py:321-448)."""
+    draw), levels first, then groups, then covariates (leaveout.py:321-448)."""
     L, G = ctx.L, ctx.G
     totals = np.zeros(8)
     drawn = 0
     while drawn < n_draws:
         size = min(block, n_draws - drawn)
-        Z = rademacher(0, L + G, size, seed, drawn)
-        Zl, Zg = Z[:L], Z[L:]
+        # positions: levels, groups, then covariates (leaveout.py:410-419)
+        Z = rademacher(0, L + G + ctx.kc, size, seed, drawn)
+        Zl, Zg = np.vstack([Z[:L], Z[L + G:]]), Z[L:L + G]
         _, _, rows = apply_inverse(ctx, Zl, Zg)
         weighted = rows * sigma2[:, None]
-        vl = ctx.Do.T @ (ctx.w[:, None] * weighted)
+        vl = np.vstack([ctx.Do.T @ (ctx.w[:, None] * weighted),
+                        ctx.X.T @ (ctx.w[:, None] * weighted)])
         vg = ctx.Dg.T @ (ctx.w[:, None] * weighted)
         Ul, Ug, _ = apply_inverse(ctx, vl, vg)
         totals += trace_moments(ctx, psi_col, Zl, Zg, Ul, Ug)
@@ -641,12 +686,14 @@ def plug_in(a: np.ndarray, b: np.ndarray, w: np.ndarray, rows: int):
 
 def leave_out_kss(data: dict, y: str = "y", xs=(), alpha: str = "worker_id",
                   psi: str = "firm_id", others=(), n_draws: int = 250, block: int = 256,
-                  seed: int = 0, tol: float = 1e-10) -> dict:
+                  seed: int = 0, tol: float = 1e-10, design_covariates: bool = True) -> dict:
     """KSS leaving out an observation, end to end (leaveout.py:1069-1255,
     634-805): leave-one-out pruning of the (alpha, psi) graph, the fit with
     alpha streamed and no singleton drop, JLA leverages, sigma2 ("own"
     stayers, weighted centring), the Hutchinson bias with seed + 1_000_003,
-    and the components plug-in minus bias.  Unweighted."""
+    and the components plug-in minus bias.  Unweighted.  The model's
+    covariates `xs` join the leave-out design unless `design_covariates`
+    is False (leaveout.py:111-127)."""
     a_ids, b_ids = np.asarray(data[alpha]), np.asarray(data[psi])
     ua, ia = np.unique(a_ids, return_inverse=True)
     ub, ib = np.unique(b_ids, return_inverse=True)
@@ -658,9 +705,10 @@ def leave_out_kss(data: dict, y: str = "y", xs=(), alpha: str = "worker_id",
     f = fit(sub, y, xs, fe, tol=tol, fixef_rm="none", stream=alpha)
     p0, c = f["p0"], f["cells"]
     n = len(p0["g"])
-    moments = jla_leverages(p0, c, f["S"], f["dinv"], n_draws, block, seed, tol)
+    X = p0["V"][:, 1:] if (len(xs) and design_covariates) else None
+    moments = jla_leverages(p0, c, f["S"], f["dinv"], n_draws, block, seed, tol, X=X)
     _, M_bar, factor = combine_moments(moments, n_draws)
-    ctx = KSSContext(f, tol)
+    ctx = KSSContext(f, tol, X=X)
     w = ctx.w
     s2 = sigma2(p0["V"][:, 0], f["resid"], w, M_bar, factor)
     psi_col = 0
@@ -1114,11 +1162,12 @@ class SELayout:
 
 
 def se_reduce(lay: SELayout, R: np.ndarray):
-    """X~' R in coefficient space: (levels, groups) of sum_i sqrt(w_i) R_i (the
-    reference feeds R / sqrt(w) to reductions that apply w)."""
+    """X~' R in coefficient space: (levels and covariates (K), groups) of
+    sum_i sqrt(w_i) R_i (the reference feeds R / sqrt(w) to reductions that
+    apply w)."""
     ctx = lay.ctx
     x = lay.root[:, None] * R
-    return ctx.Do.T @ x, ctx.Dg.T @ x
+    return np.vstack([ctx.Do.T @ x, ctx.X.T @ x]), ctx.Dg.T @ x
 
 
 def bii_rows(lay: SELayout, n_draws: int, block: int, seed: int) -> np.ndarray:
@@ -1156,7 +1205,7 @@ def apply_q(lay: SELayout, k: int, Ul: np.ndarray, Ug: np.ndarray):
     over the rows."""
     ctx = lay.ctx
     s = Ul.shape[1]
-    ol, og = np.zeros((ctx.L, s)), np.zeros((ctx.G, s))
+    ol, og = np.zeros((ctx.K, s)), np.zeros((ctx.G, s))
     lw = lay.level_w[lay.psi]
     psi_mean = (lw @ Ul[lay.psi]) / lay.total
     alpha_mean = (lay.group_w @ Ug) / lay.total
@@ -1189,7 +1238,7 @@ def apply_c(lay: SELayout, v: np.ndarray, b: np.ndarray, per_component: bool = F
                            + [part(k) * b[k][:, None] for k in range(3)], axis=1)
     lv, gs = se_reduce(lay, stack)
     U0l, U0g, base = apply_inverse(ctx, lv, gs)
-    ql, qg = np.zeros((ctx.L, 3 * s)), np.zeros((ctx.G, 3 * s))
+    ql, qg = np.zeros((ctx.K, 3 * s)), np.zeros((ctx.G, 3 * s))
     inc = lambda k: slice((k if per_component else 0) * s, ((k if per_component else 0) + 1) * s)
     bc = lambda k: slice((n_in + k) * s, (n_in + k + 1) * s)
     for k in range(3):
@@ -1343,7 +1392,7 @@ def q_block(lay: SELayout, comps, Ul: np.ndarray, Ug: np.ndarray):
     """Q applied column by column, column j with component comps[j]
     (leaveout_weakid.py:164-179)."""
     comps = np.asarray(comps)
-    ol, og = np.zeros_like(Ul), np.zeros_like(Ug)
+    ol, og = np.zeros((lay.ctx.K, Ul.shape[1])), np.zeros_like(Ug)
     for k in range(3):
         cols = np.flatnonzero(comps == k)
         if len(cols):
@@ -1360,8 +1409,8 @@ def krylov_plan(lay: SELayout, max_iter: int):
     """Steps per component and whether its basis is kept for full
     reorthogonalization (leaveout_weakid.py:186-215)."""
     n_psi = lay.psi.stop - lay.psi.start
-    G, L = lay.ctx.G, lay.ctx.L
-    dims = [max(n_psi - 1, 0), max(L + 1, 1), max(2 * (n_psi - 1), 0)]
+    G, K = lay.ctx.G, lay.ctx.K
+    dims = [max(n_psi - 1, 0), max(K + 1, 1), max(2 * (n_psi - 1), 0)]
     support = [n_psi, G, n_psi + G]
     caps = [min(max_iter, d) for d in dims]
     reorth = [cap * 2 * rows * 8 <= SCRATCH_BYTES for cap, rows in zip(caps, support)]
@@ -1415,7 +1464,7 @@ def lanczos(lay: SELayout, n_iter: int, seed: int, ritz=None, check=None, every:
     for blk in (*x, *y):
         blk[:, live] /= norm[live]
         blk[:, ~live] = 0.0
-    x_prev = [np.zeros((L, 3)), np.zeros((G, 3))]
+    x_prev = [np.zeros((ctx.K, 3)), np.zeros((G, 3))]
     beta_prev = np.zeros(3)
     basis = [([], []) for _ in comps]
     alphas, betas = [[] for _ in comps], [[] for _ in comps]
@@ -1423,8 +1472,8 @@ def lanczos(lay: SELayout, n_iter: int, seed: int, ritz=None, check=None, every:
     acc = None
     if ritz is not None:
         k = ritz[0].shape[1]
-        acc = ([np.zeros((L, 3 * k)), np.zeros((G, 3 * k))],
-               [np.zeros((L, 3 * k)), np.zeros((G, 3 * k))])
+        acc = ([np.zeros((ctx.K, 3 * k)), np.zeros((G, 3 * k))],
+               [np.zeros((ctx.K, 3 * k)), np.zeros((G, 3 * k))])
     beta_max = np.zeros(3)
     steps = 0
     for j in range(n_iter):
@@ -1632,7 +1681,7 @@ def weak_id_diagnostics(lay: SELayout, y: np.ndarray, sigma2: np.ndarray, levera
     alphas, betas, lengths, n_iter = lanczos(lay, max_iter, seed, check=converged_all,
                                              every=10, caps=caps, reorth=reorth)
     checked = [ritz_top(alphas[c], betas[c], top) for c in range(3)]
-    n_coef = ctx.L + ctx.G
+    n_coef = ctx.K + ctx.G
     k = top if 2 * 3 * top * n_coef * 8 <= SCRATCH_BYTES else 1
     ritz = []
     for _t, _b, S in checked:
@@ -1660,8 +1709,7 @@ def weak_id_diagnostics(lay: SELayout, y: np.ndarray, sigma2: np.ndarray, levera
     w = ctx.w
     ybar = float(w @ y / w.sum())
     yt = lay.root * (y - ybar)
-    rows = ctx.Do @ u[0] + u[1][ctx.g]
-    x1 = rows * lay.root[:, None]
+    x1 = eigen_rows(lay, u[0], u[1])
     num, den = x1.T @ yt, (x1 ** 2).T @ sm
     f_stat = np.where(den > 0, num ** 2 / np.where(den > 0, den, 1.0), np.nan)
     out = {"eigenvalues": [], "bounds": [], "ratios": [], "q": [], "q_capped": [],
@@ -1701,9 +1749,10 @@ WEAK_MOMENTS_OFFSET = 8_800_021    # leaveout_se.py:747
 
 
 def eigen_rows(lay: SELayout, Ul: np.ndarray, Ug: np.ndarray) -> np.ndarray:
-    """sqrt(w_i) (A u)_i for coefficient blocks (L, c), (G, c)."""
+    """sqrt(w_i) (A u)_i for coefficient blocks (K, c), (G, c)
+    (leaveout_se.py:486-495)."""
     ctx = lay.ctx
-    return lay.root[:, None] * (ctx.Do @ Ul + Ug[ctx.g])
+    return lay.root[:, None] * (ctx.Do @ Ul[:ctx.L] + ctx.X @ Ul[ctx.L:] + Ug[ctx.g])
 
 
 def weak_moments(lay: SELayout, y, sigma2, s, leverage, complement, factor, b_ii, lams,

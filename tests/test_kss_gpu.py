@@ -135,12 +135,168 @@ def test_unit_weights_give_the_unweighted_components():
         assert b.plug_in[k] == pytest.approx(a.plug_in[k], rel=1e-12, abs=1e-15)
 
 
-def test_covariates_are_refused():
+# ------------------------------------------------ covariates in the design
+
+def _cov_fits(O, d, weighted, tol=1e-12):
+    from hdfe_stream_amd.estimator import feols_stream
+    kw = {"weights": "w"} if weighted else {}
+    res = feols_stream("y ~ x1 + x2 | worker_id + firm_id", d, stream="worker_id",
+                       fixef_rm="none", keep_intermediates=True, tol=tol, **kw)
+    ref = O.fit(d, "y", ("x1", "x2"), FE2, weights="w" if weighted else None, tol=tol,
+                fixef_rm="none", stream="worker_id")
+    return res, ref
+
+
+@pytest.fixture(scope="module", params=[False, True], ids=["unweighted", "weighted"])
+def cov_panel(oracle_mod, request):
+    from hdfe_stream_amd.simulate import akm_panel
+    O = oracle_mod
+    d = akm_panel(1500, 120, seed=6, p_move=0.2)
+    if request.param:
+        d["w"] = np.random.default_rng(7).uniform(0.5, 2.0, len(d["y"]))
+    d = _loo(O, d)
+    res, ref = _cov_fits(O, d, request.param)
+    return O, d, res, ref, request.param
+
+
+def test_apply_inverse_with_covariates_matches_the_oracle(cov_panel):
+    import torch
+    from hdfe_stream_amd.leaveout import CoefficientInverse
+    O, d, res, ref, weighted = cov_panel
+    print(f"apply_inverse with covariates weighted={weighted}", flush=True)
+    inv = CoefficientInverse(res.panel, tol=1e-12, precision="fp64", covariates=(1, 2))
+    ctx = O.KSSContext(ref, tol=1e-12, X=ref["p0"]["V"][:, 1:])
+    assert (inv.K, inv.kc) == (ctx.K, 2)
+    rng = np.random.default_rng(3)
+    vl = rng.standard_normal((ctx.K, 5))
+    vg = rng.standard_normal((ctx.G, 5))
+    Ul, Ug, rows = inv(torch.from_numpy(vl).cuda(), torch.from_numpy(vg).cuda())
+    rUl, rUg, rrows = O.apply_inverse(ctx, vl, vg)
+    assert Ul.shape == (ctx.K, 5)
+    assert rel(rows.cpu().numpy(), rrows) < 1e-9
+    assert rel(Ul.cpu().numpy(), rUl) < 1e-8
+    assert rel(Ug.cpu().numpy(), rUg) < 1e-8
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_leverages_with_covariates_match_the_oracle(cov_panel, precision):
+    from hdfe_stream_amd.leaveout import jla_leverages
+    O, d, res, ref, weighted = cov_panel
+    print(f"jla with covariates weighted={weighted} {precision}", flush=True)
+    tol = 1e-12 if precision == "fp64" else 1e-10
+    t = {"keep_raw": True}
+    lev = jla_leverages(res.panel, n_draws=40, block=16, seed=4, tol=tol, precision=precision,
+                        timings=t, covariates=(1, 2))
+    p0, c = ref["p0"], ref["cells"]
+    mom = O.jla_leverages(p0, c, ref["S"], ref["dinv"], 40, 16, 4, tol=1e-12,
+                          X=p0["V"][:, 1:])
+    bound = 1e-8 if precision == "fp64" else 1e-6
+    assert rel(t["raw_moments"].cpu().numpy(), mom) < bound
+    P, M, F = O.combine_moments(mom, 40)
+    assert rel(lev.leverage.cpu().numpy(), P) < bound
+    # the covariates enter the design: the leverages are not the fixed effects' alone
+    lev0 = jla_leverages(res.panel, n_draws=40, block=16, seed=4, tol=tol, precision=precision)
+    assert float((lev.leverage - lev0.leverage).abs().max()) > 1e-4
+
+
+def test_hutchinson_with_covariates_matches_the_oracle(cov_panel):
+    import torch
+    from hdfe_stream_amd.leaveout import hutchinson_trace
+    O, d, res, ref, weighted = cov_panel
+    n = res.n_obs
+    print(f"hutchinson with covariates weighted={weighted}", flush=True)
+    sigma2 = np.random.default_rng(5).uniform(0.05, 0.4, n)
+    tr = hutchinson_trace(res.panel, torch.from_numpy(sigma2).cuda(), n_draws=30, block=12,
+                          seed=17, tol=1e-12, precision="fp64", covariates=(1, 2))
+    ctx = O.KSSContext(ref, tol=1e-12, X=ref["p0"]["V"][:, 1:])
+    (vp, va, vc), totals = O.hutchinson_trace(ctx, sigma2, 30, 12, seed=17)
+    got = np.array([tr.var_psi, tr.var_alpha, tr.cov])
+    want = np.array([vp, va, vc])
+    assert np.abs(got - want).max() <= 1e-8 * np.abs(want).max()
+
+
+def test_standard_errors_with_covariates_match_the_oracle(cov_panel):
+    import torch
+    from hdfe_stream_amd.leaveout import CoefficientInverse, JLALeverages
+    from hdfe_stream_amd.leaveout_se import component_variances
+    O, d, res, ref, weighted = cov_panel
+    print(f"component_variances with covariates weighted={weighted}", flush=True)
+    ctx = O.KSSContext(ref, tol=1e-12, X=ref["p0"]["V"][:, 1:])
+    n = len(ctx.w)
+    rng = np.random.default_rng(4)
+    M_bar, factor = rng.uniform(0.3, 0.95, n), rng.uniform(0.9, 1.1, n)
+    y = ref["p0"]["V"][:, 0]
+    s2 = O.sigma2(y, ref["resid"], ctx.w, M_bar, factor)
+    want = O.component_variances(O.SELayout(ctx, 0, n), y, s2, 1.0 - M_bar, M_bar, factor, 12,
+                                 6, 9)
+    cu = lambda a: torch.from_numpy(np.ascontiguousarray(a, np.float64)).cuda()
+    lev = JLALeverages(leverage=cu(1.0 - M_bar), complement=cu(M_bar), factor=cu(factor),
+                       n_draws=12)
+    inv = CoefficientInverse(res.panel, tol=1e-12, precision="fp64", covariates=(1, 2))
+    out = component_variances(inv, res.panel.tensor("V", torch.float64).view(n, 3)[:, 0]
+                              .contiguous(), cu(s2), lev, n_draws=12, block=6, seed=9)
+    names = ["var(psi)", "var(alpha)", "cov(psi, alpha)"]
+    for k in range(3):
+        assert rel(out.diagnostics["b_ii"][k].cpu().numpy(), want["b_ii"][k]) < 1e-8
+    for key, part in (("first", out.first_term), ("theta", out.theta),
+                      ("trace", out.trace_term), ("variance", out.variance)):
+        assert rel([part[nm] for nm in names], want[key]) < 1e-7, key
+
+
+def test_weak_id_with_covariates_matches_the_oracle(cov_panel):
+    import torch
+    from hdfe_stream_amd.leaveout import CoefficientInverse, JLALeverages
+    from hdfe_stream_amd.leaveout_weakid import weak_id_diagnostics
+    O, d, res, ref, weighted = cov_panel
+    print(f"weak_id with covariates weighted={weighted}", flush=True)
+    ctx = O.KSSContext(ref, tol=1e-12, X=ref["p0"]["V"][:, 1:])
+    n = len(ctx.w)
+    rng = np.random.default_rng(6)
+    s2, lv = rng.uniform(0.1, 0.4, n), rng.uniform(0.2, 0.9, n)
+    y = ref["p0"]["V"][:, 0]
+    want = O.weak_id_diagnostics(O.SELayout(ctx, 0, n), y, s2, lv, n_draws=16, block=8, seed=7)
+    cu = lambda a: torch.from_numpy(np.ascontiguousarray(a, np.float64)).cuda()
+    levs = JLALeverages(leverage=cu(lv), complement=cu(1.0 - lv), factor=cu(np.ones(n)),
+                        n_draws=1)
+    inv = CoefficientInverse(res.panel, tol=1e-12, precision="fp64", covariates=(1, 2))
+    out = weak_id_diagnostics(inv, res.panel.tensor("V", torch.float64).view(n, 3)[:, 0]
+                              .contiguous(), cu(s2), levs, n_draws=16, block=8, seed=7)
+    assert out.diagnostics["lanczos_steps"] == want["steps"]
+    for k, nm in enumerate(["var(psi)", "var(alpha)", "cov(psi, alpha)"]):
+        np.testing.assert_allclose(out.eigenvalues[nm], want["eigenvalues"][k], rtol=1e-6)
+        assert out.sum_sq[nm] == pytest.approx(want["sum_sq"][k], rel=1e-6)
+        assert out.f_stat[nm] == pytest.approx(want["f_stat"][k], rel=1e-5)
+        assert out.q[nm] == want["q"][k]
+    assert out.lead_vectors[0].shape[0] == ctx.K
+
+
+def test_leave_out_kss_with_covariates_matches_the_oracle(oracle_mod):
     from hdfe_stream_amd.leaveout import leave_out_kss
     from hdfe_stream_amd.simulate import kss_panel
-    d = kss_panel(500, 60, seed=4)
-    with pytest.raises(NotImplementedError):
-        leave_out_kss("y ~ x1 | worker_id + firm_id", d, n_draws=8, leave_out="observation")
+    O = oracle_mod
+    d = kss_panel(4000, 400, seed=3)
+    print("leave_out_kss with covariates", flush=True)
+    lo = leave_out_kss("y ~ x1 + x2 | worker_id + firm_id", d, n_draws=64, seed=2,
+                       precision="fp64", tol=1e-12, leave_out="observation")
+    ref = O.leave_out_kss(d, xs=("x1", "x2"), n_draws=64, block=64, seed=2, tol=1e-12)
+    assert lo.diagnostics["covariates"] == ["x1", "x2"]
+    assert lo.leverages.diagnostics["covariates"] == ["x1", "x2"]
+    names = ["var(psi)", "var(alpha)", "cov(psi, alpha)"]
+    assert rel([lo.plug_in[k] for k in names], ref["plug_in"]) < 1e-9
+    assert rel([lo.bias[k] for k in names], ref["bias"]) < 1e-7
+    assert rel([lo.leave_out[k] for k in names], ref["leave_out"]) < 1e-7
+    assert rel(lo.diagnostics["sigma2"].cpu().numpy(), ref["sigma2"]) < 1e-5
+    # () asks for the fixed-effect design alone: another estimate
+    fe_only = O.leave_out_kss(d, xs=("x1", "x2"), n_draws=64, block=64, seed=2, tol=1e-12,
+                              design_covariates=False)
+    assert rel(ref["bias"], fe_only["bias"]) > 1e-6
+    from hdfe_stream_amd.leaveout import leave_out_components
+    lo0 = leave_out_components(lo.fit, n_draws=64, seed=2, precision="fp64", tol=1e-12,
+                               covariates=())
+    assert lo0.diagnostics["covariates"] == []
+    assert rel([lo0.bias[k] for k in names], fe_only["bias"]) < 1e-7
+    with pytest.raises(ValueError):
+        leave_out_components(lo.fit, n_draws=4, covariates=("x9",))
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])

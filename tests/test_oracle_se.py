@@ -21,23 +21,28 @@ def _loo(O, d):
     return {k: np.asarray(v)[keep] for k, v in d.items()}
 
 
-@pytest.fixture(scope="module")
-def dense(oracle_mod):
+@pytest.fixture(scope="module", params=["fixed effects", "covariates"])
+def dense(oracle_mod, request):
+    """The design [D_g, D_o] or, with covariates, [D_g, D_o, X] (x1, x2 in
+    the model and in the leave-out design: the border)."""
     from hdfe_stream_amd.simulate import kss_panel
     O = oracle_mod
     d = kss_panel(220, 30, seed=12)
     d["w"] = np.random.default_rng(3).uniform(0.5, 2.0, len(d["y"]))
     d = _loo(O, d)
-    f = O.fit(d, "y", (), FE, weights="w", tol=1e-13, fixef_rm="none", stream="worker_id")
-    ctx = O.KSSContext(f, tol=1e-13)
+    xs = ("x1", "x2") if request.param == "covariates" else ()
+    f = O.fit(d, "y", xs, FE, weights="w", tol=1e-13, fixef_rm="none", stream="worker_id")
+    X = f["p0"]["V"][:, 1:] if xs else None
+    ctx = O.KSSContext(f, tol=1e-13, X=X)
     n = len(ctx.w)
     lay = O.SELayout(ctx, 0, n)
-    A = np.hstack([ctx.Dg.toarray(), ctx.Do.toarray()])          # coefficients: groups, levels
+    # coefficients: groups, levels, covariates
+    A = np.hstack([ctx.Dg.toarray(), ctx.Do.toarray(), ctx.X])
     Xt = lay.root[:, None] * A
     Sm = np.linalg.pinv(Xt.T @ Xt)
     P = Xt @ Sm @ Xt.T
     M = np.eye(n) - P
-    G, L = ctx.G, ctx.L
+    G, L = ctx.G, ctx.K
     psi = np.zeros((n, G + L))
     psi[np.arange(n), G + lay.psi_level] = 1.0
     alp = np.zeros((n, G + L))
@@ -54,7 +59,7 @@ def dense(oracle_mod):
     bvec = np.stack([np.diag(Bk) / np.diag(M) for Bk in B])
     Cm = [B[k] - 0.5 * (np.diag(bvec[k]) @ M + M @ np.diag(bvec[k])) for k in range(3)]
     return dict(O=O, f=f, ctx=ctx, lay=lay, Xt=Xt, Sm=Sm, M=M, Lp=Lp, La=La, B=B, b=bvec,
-                C=Cm, d=d, n=n)
+                C=Cm, d=d, n=n, A=A)
 
 
 def test_the_kernel_has_a_zero_diagonal(dense):
@@ -235,7 +240,7 @@ def test_first_stage_f(dense):
     tab = O.fit_smoother_1d(s2, lev)
     sm = O.smooth_eval_1d(tab, lev)
     yt = lay.root * (y - ctx.w @ y / ctx.w.sum())
-    A = np.hstack([ctx.Dg.toarray(), ctx.Do.toarray()])
+    A = dense["A"]
     for k in range(3):
         u = np.concatenate([out["lead"][1][:, k], out["lead"][0][:, k]])
         x1 = lay.root * (A @ u)
