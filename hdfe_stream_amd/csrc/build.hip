@@ -137,12 +137,20 @@ __global__ void k_level_count(int64_t n, const int *__restrict__ code,
         for (int t = threadIdx.x; t < nl; t += blockDim.x) bins[t] = 0;
         __syncthreads();
     }
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x)
-        if (keep[i]) {
-            if (lds) atomicAdd(&bins[code[i]], 1);
-            else atomicAdd(&cnt[code[i]], 1);
+    // wave-uniform trip count: one atomic per run of equal codes
+    const int lane = threadIdx.x & (kWave - 1);
+    for (int64_t i0 = blockIdx.x * (int64_t)blockDim.x + (threadIdx.x - lane); i0 < n;
+         i0 += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = i0 + lane;
+        const bool in = i < n;
+        const int key = in ? code[i] : -1;
+        bool last;
+        const int tot = run_total<int>(key, (in && keep[i]) ? 1 : 0, last);
+        if (last && key >= 0 && tot) {
+            if (lds) atomicAdd(&bins[key], tot);
+            else atomicAdd(&cnt[key], tot);
         }
+    }
     if (lds) {
         __syncthreads();
         for (int t = threadIdx.x; t < nl; t += blockDim.x)
@@ -355,6 +363,105 @@ __global__ void k_cells(int64_t C, int m, const int *__restrict__ cstart,
     }
 }
 
+// k_cells for a compile-time m <= 8: the sums and the within-cell scatter
+// stay in registers (the runtime-m form indexes a private array, which the
+// compiler places in scratch memory).  The scatter is accumulated on its
+// upper triangle, in the runtime-m form's order, and mirrored, so it is
+// exactly symmetric (the runtime-m form rounds its two triangles apart).
+template <int M>
+__global__ __launch_bounds__(256) void k_cells_m(int64_t C, const int *__restrict__ cstart,
+                                                 const double *__restrict__ V,
+                                                 const double *__restrict__ w,
+                                                 const double *__restrict__ mean,
+                                                 double *__restrict__ cn, double *__restrict__ crun,
+                                                 double *__restrict__ csums,
+                                                 double *__restrict__ wpart) {
+    __shared__ double lds[T256];
+    constexpr int NT = M * (M + 1) / 2;
+    double W[NT];
+    double mu[M];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) W[t] = 0.0;
+#pragma unroll
+    for (int j = 0; j < M; ++j) mu[j] = mean[j];
+    const int64_t c0 = part_begin(C, blockIdx.x, gridDim.x), c1 = part_begin(C, blockIdx.x + 1, gridDim.x);
+    for (int64_t c = c0 + threadIdx.x; c < c1; c += blockDim.x) {
+        const int s = cstart[c], e = cstart[c + 1];
+        double nn = 0.0;
+        double sm[M];
+#pragma unroll
+        for (int j = 0; j < M; ++j) sm[j] = 0.0;
+        for (int i = s; i < e; ++i) {
+            const double wi = w ? w[i] : 1.0;
+            nn += wi;
+#pragma unroll
+            for (int j = 0; j < M; ++j) sm[j] += wi * (V[(int64_t)i * M + j] - mu[j]);
+        }
+        cn[c] = nn;
+        crun[c] = (double)(e - s);
+#pragma unroll
+        for (int j = 0; j < M; ++j) csums[c * M + j] = sm[j];
+        for (int i = s; i < e; ++i) {
+            const double wi = w ? w[i] : 1.0;
+            double dv[M];
+#pragma unroll
+            for (int j = 0; j < M; ++j) dv[j] = V[(int64_t)i * M + j] - mu[j] - sm[j] / nn;
+            int t = 0;
+#pragma unroll
+            for (int a = 0; a < M; ++a)
+#pragma unroll
+                for (int b = a; b < M; ++b) W[t++] += wi * dv[a] * dv[b];
+        }
+    }
+    int t = 0;
+#pragma unroll
+    for (int a = 0; a < M; ++a)
+#pragma unroll
+        for (int b = a; b < M; ++b) {
+            const double v = block_sum<double>(W[t++], lds);
+            if (threadIdx.x == 0) {
+                wpart[(int64_t)blockIdx.x * M * M + a * M + b] = v;
+                wpart[(int64_t)blockIdx.x * M * M + b * M + a] = v;
+            }
+        }
+}
+
+// The S build's level tables, one thread per level: its dimension, whether
+// the dimension is sparse, its index among the sparse or the dense levels
+// (-1 in the other table) and the inverse lists.  Dimensions are contiguous
+// in level order, so a level's index is its dimension's base plus its offset
+// within the dimension.
+struct LevelDims {
+    int64_t off[8];
+    int64_t base[8];      // sparse (or dense) levels in the dimensions before
+    int sparse[8];
+    int D;
+};
+
+__global__ void k_level_tables(int64_t L, LevelDims ld, uint8_t *__restrict__ sparse_lv,
+                               int *__restrict__ sparse_idx, int *__restrict__ dense_idx,
+                               int *__restrict__ lvl_dim, int *__restrict__ sparse_levels,
+                               int *__restrict__ dense_levels) {
+    for (int64_t a = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; a < L;
+         a += (int64_t)gridDim.x * blockDim.x) {
+        int t = 0;
+        for (int d = 1; d < ld.D; ++d)
+            if (a >= ld.off[d]) t = d;
+        const int64_t idx = ld.base[t] + (a - ld.off[t]);
+        lvl_dim[a] = t;
+        sparse_lv[a] = (uint8_t)ld.sparse[t];
+        if (ld.sparse[t]) {
+            sparse_idx[a] = (int)idx;
+            dense_idx[a] = -1;
+            sparse_levels[idx] = (int)a;
+        } else {
+            dense_idx[a] = (int)idx;
+            sparse_idx[a] = -1;
+            dense_levels[idx] = (int)a;
+        }
+    }
+}
+
 // obs / cnt per level of one dimension over all cells (cnt in integer
 // atomics: exact; obs, with weights, in float atomics).  Small dimensions
 // count in LDS bins first.
@@ -372,17 +479,26 @@ __global__ void k_obs_cnt(int64_t C, const int *__restrict__ cstart,
         }
         __syncthreads();
     }
-    for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < C;
-         c += (int64_t)gridDim.x * blockDim.x) {
-        const int s0 = cstart[c];
-        const unsigned r = (unsigned)(cstart[c + 1] - s0);
-        const int a = code[s0];
-        if (lds) {
-            atomicAdd(&bins[a], r);
-            if (weighted) atomicAdd(&wbins[a], cn[c]);
-        } else {
-            atomicAdd(&cnt[off + a], (unsigned long long)r);
-            if (weighted) atomicAdd(&obs[off + a], cn[c]);
+    // wave-uniform trip count: one atomic per run of cells at one level
+    const int lane = threadIdx.x & (kWave - 1);
+    for (int64_t c0 = blockIdx.x * (int64_t)blockDim.x + (threadIdx.x - lane); c0 < C;
+         c0 += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t c = c0 + lane;
+        const bool in = c < C;
+        const int s0 = in ? cstart[c] : 0;
+        const unsigned long long r = in ? (unsigned long long)(cstart[c + 1] - s0) : 0ull;
+        const int a = in ? code[s0] : -1;
+        bool last;
+        const unsigned long long rt = run_total<unsigned long long>(a, r, last);
+        const double wt = weighted ? run_total<double>(a, in ? cn[c] : 0.0, last) : 0.0;
+        if (last && a >= 0) {
+            if (lds) {
+                atomicAdd(&bins[a], (unsigned)rt);
+                if (weighted) atomicAdd(&wbins[a], wt);
+            } else {
+                atomicAdd(&cnt[off + a], rt);
+                if (weighted) atomicAdd(&obs[off + a], wt);
+            }
         }
     }
     if (lds) {
@@ -994,8 +1110,14 @@ __global__ void k_uf_union(int64_t Gi, const int *__restrict__ igstart,
          g += (int64_t)gridDim.x * blockDim.x) {
         const int s = igstart[g], e = igstart[g + 1];
         const int a0 = icodes[s];
+        int prev = a0;
         for (int c = s + 1; c < e; ++c) {
-            int a = uf_root(parent, a0), b = uf_root(parent, icodes[c]);
+            const int lc = icodes[c];
+            // a group's cells repeat its levels (one cell per level of every
+            // other dimension): a level just joined needs no second union
+            if (lc == prev || lc == a0) continue;
+            prev = lc;
+            int a = uf_root(parent, a0), b = uf_root(parent, lc);
             while (a != b) {
                 if (a > b) {
                     int t = a;
@@ -1532,9 +1654,28 @@ static int run_pass1(Panel &P) {
     DEV_ALLOC(csums, (int64_t)nc * m, s);
     const int mm = m <= 8 ? m * m : 0;
     DEV_ALLOC(wpart, (int64_t)nb * std::max(mm, 1), s);
-    hipLaunchKernelGGL(k_cells, dim3(nb), dim3(T256), 0, s, (int64_t)nc, m, P.cstart.p, P.V.p,
-                       P.weighted ? P.w.p : (const double *)nullptr, P.means.p, cn.p, crun.p,
-                       csums.p, wpart.p);
+    {
+        const double *wp = P.weighted ? P.w.p : (const double *)nullptr;
+#define HDFE_CELLS(MM)                                                                         \
+    case MM:                                                                                   \
+        hipLaunchKernelGGL(k_cells_m<MM>, dim3(nb), dim3(T256), 0, s, (int64_t)nc, P.cstart.p, \
+                           P.V.p, wp, P.means.p, cn.p, crun.p, csums.p, wpart.p);              \
+        break;
+        switch (m) {
+            HDFE_CELLS(1)
+            HDFE_CELLS(2)
+            HDFE_CELLS(3)
+            HDFE_CELLS(4)
+            HDFE_CELLS(5)
+            HDFE_CELLS(6)
+            HDFE_CELLS(7)
+            HDFE_CELLS(8)
+        default:
+            hipLaunchKernelGGL(k_cells, dim3(nb), dim3(T256), 0, s, (int64_t)nc, m, P.cstart.p,
+                               P.V.p, wp, P.means.p, cn.p, crun.p, csums.p, wpart.p);
+        }
+#undef HDFE_CELLS
+    }
     DEV_ALLOC(P.W, std::max(mm, 1), s);
     if (mm) {
         HIP_TRY(to_host(hp, wpart.p, (int64_t)nb * mm, s));
@@ -1930,8 +2071,7 @@ int hdfe_panel_system(void *h, int with_rhs, void **matrix, double *seconds) {
     HIP_TRY(hipGetLastError());
 
     // dense / sparse dimensions and their levels (ascending global index)
-    std::vector<uint8_t> sparse_dim(D), sparse_lv(L);
-    std::vector<int> sparse_idx(L, -1), dense_idx(L, -1), lvl_dim(L), sparse_levels, dense_levels;
+    std::vector<uint8_t> sparse_dim(D);
     {
         // smallest dimensions first while the dense levels fit kDenseMax
         std::vector<int> order(D);
@@ -1951,21 +2091,16 @@ int hdfe_panel_system(void *h, int with_rhs, void **matrix, double *seconds) {
         for (int t = 0; t < D; ++t) any_sparse |= (bool)sparse_dim[t];
         if (!any_sparse) sparse_dim[order.back()] = 1;
     }
-    for (int t = 0; t < D; ++t)
-        for (int64_t l = 0; l < P.nlev[t]; ++l) {
-            const int a = (int)(P.offs_h[t] + l);
-            lvl_dim[a] = t;
-            sparse_lv[a] = sparse_dim[t];
-            if (sparse_dim[t]) {
-                sparse_idx[a] = (int)sparse_levels.size();
-                sparse_levels.push_back(a);
-            } else {
-                dense_idx[a] = (int)dense_levels.size();
-                dense_levels.push_back(a);
-            }
-        }
-    const int64_t Ls = (int64_t)sparse_levels.size();
-    const int Ld = (int)dense_levels.size();
+    LevelDims ldims{};
+    ldims.D = D;
+    int64_t Ls = 0, Ld64 = 0;
+    for (int t = 0; t < D; ++t) {
+        ldims.off[t] = P.offs_h[t];
+        ldims.sparse[t] = sparse_dim[t];
+        ldims.base[t] = sparse_dim[t] ? Ls : Ld64;
+        (sparse_dim[t] ? Ls : Ld64) += P.nlev[t];
+    }
+    const int Ld = (int)Ld64;
     int ncross = 0;
     for (int d1 = 0; d1 < D; ++d1)
         for (int d2 = 0; d2 < D; ++d2) ncross += (d1 != d2 && sparse_dim[d1] && sparse_dim[d2]);
@@ -1978,13 +2113,10 @@ int hdfe_panel_system(void *h, int with_rhs, void **matrix, double *seconds) {
     DEV_ALLOC(d_lvl_dim, L, s);
     DEV_ALLOC(d_sparse_levels, std::max<int64_t>(Ls, 1), s);
     DEV_ALLOC(d_dense_levels, std::max(Ld, 1), s);
-    HIP_TRY(hipMemcpyAsync(d_sparse_lv.p, sparse_lv.data(), L, hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemcpyAsync(d_sparse_dim.p, sparse_dim.data(), D, hipMemcpyHostToDevice, s));
-    HIP_TRY(hipMemcpyAsync(d_sparse_idx.p, sparse_idx.data(), L * 4, hipMemcpyHostToDevice, s));
-    HIP_TRY(hipMemcpyAsync(d_dense_idx.p, dense_idx.data(), L * 4, hipMemcpyHostToDevice, s));
-    HIP_TRY(hipMemcpyAsync(d_lvl_dim.p, lvl_dim.data(), L * 4, hipMemcpyHostToDevice, s));
-    if (Ls) HIP_TRY(hipMemcpyAsync(d_sparse_levels.p, sparse_levels.data(), Ls * 4, hipMemcpyHostToDevice, s));
-    if (Ld) HIP_TRY(hipMemcpyAsync(d_dense_levels.p, dense_levels.data(), Ld * 4, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_level_tables, dim3(grid1(L)), dim3(T256), 0, s, (int64_t)L, ldims,
+                       d_sparse_lv.p, d_sparse_idx.p, d_dense_idx.p, d_lvl_dim.p,
+                       d_sparse_levels.p, d_dense_levels.p);
 
     // sparse x sparse: emit, sort, reduce
     DevBuf<int64_t> cnt, pos;

@@ -76,6 +76,30 @@ __device__ __forceinline__ T block_sum(T v, T *lds) {
     return s;
 }
 
+// Runs of equal keys among consecutive lanes of a wave (rows sorted by group
+// repeat a level over neighbouring rows): the sum of v over the caller's run,
+// valid in the run's last lane (`last` is true there), so that one atomic per
+// run replaces one per lane.  Every lane of the wave calls it; a lane with
+// nothing to add passes v = 0, and one past the end a key no row has.
+template <typename T>
+__device__ __forceinline__ T run_total(int key, T v, bool &last) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int up = __shfl_up(key, 1, kWave);
+    const bool head = lane == 0 || up != key;
+    const unsigned long long hm = __ballot(head);
+    T p = v;
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        const T t = __shfl_up(p, d, kWave);
+        if (lane >= d) p += t;
+    }
+    last = lane == kWave - 1 || ((hm >> (lane + 1)) & 1ull);
+    const unsigned long long mine = lane == kWave - 1 ? hm : (hm & ((2ull << lane) - 1ull));
+    const int hl = 63 - __clzll((long long)mine);
+    const T before = __shfl(p, hl > 0 ? hl - 1 : 0, kWave);
+    return hl > 0 ? p - before : p;
+}
+
 // Row range of part `w` of `nparts` equal contiguous parts of [0, n).
 __host__ __device__ __forceinline__ int64_t part_begin(int64_t n, int64_t w, int64_t nparts) {
     return (n * w) / nparts;

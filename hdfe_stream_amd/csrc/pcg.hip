@@ -872,6 +872,19 @@ static int grid_for(int64_t n, int64_t per, int lo, int hi) {
     return (int)std::max<int64_t>(lo, std::min<int64_t>(hi, g));
 }
 
+// The matrix's device arrays and work buffers come from the stream-ordered
+// pool on the matrix's stream (a repeated fit reuses the same memory instead
+// of mapping new pages), and go back to it in stream order.
+static hipError_t palloc(void *p, size_t bytes, hipStream_t s) {
+    hipError_t e = pool_prepare();
+    if (e != hipSuccess) return e;
+    return hipMallocAsync((void **)p, std::max<size_t>(bytes, 1), s);
+}
+
+static void pfree(void *p, hipStream_t s) {
+    if (p) (void)hipFreeAsync(p, s);
+}
+
 // A pair of scratch allocations freed on scope exit.
 struct DevBuf2 {
     double *p = nullptr;
@@ -893,11 +906,10 @@ struct Work {
     double *sd = nullptr;  // rz alpha beta bnorm rel scale : 6k
     int *si = nullptr;     // active(k) zero(k) outer_active(k) done it it_final
     unsigned *cnt = nullptr;
-    void release() {
+    void release(hipStream_t s) {
         void *ps[] = {X64, R64, P64, AP64, B64, X32, R32, P32, AP32, part, lpart, colbuf, gpart,
                       sd, si, cnt};
-        for (void *p : ps)
-            if (p) (void)hipFree(p);
+        for (void *p : ps) pfree(p, s);
         *this = Work();
     }
 };
@@ -959,35 +971,35 @@ static int alloc_work(Matrix &M, int k) {
         M.blk_tried = true;
         if (M.L * (int64_t)kSmallK * 8 > kL2Budget && M.L < (INT32_MAX - 1) / 8) {
             M.drop_graphs();
-            M.w.release();
+            M.w.release(M.stream);
             const int st = build_blocked(M, M.ip_host);
             if (st) return st;
         }
     }
     if (M.w.k == k && M.w.L == M.L) return HDFE_STATUS_OK;
     M.drop_graphs();  // their kernels point at the work buffers
-    M.w.release();
+    M.w.release(M.stream);
     const int64_t Lk = std::max<int64_t>(1, M.L * (int64_t)k);
     Work &W = M.w;
-    HIP_TRY(hipMalloc(&W.X64, Lk * 8));
-    HIP_TRY(hipMalloc(&W.R64, Lk * 8));
-    HIP_TRY(hipMalloc(&W.P64, Lk * 8));
-    HIP_TRY(hipMalloc(&W.AP64, Lk * 8));
-    HIP_TRY(hipMalloc(&W.B64, Lk * 8));
-    HIP_TRY(hipMalloc(&W.X32, Lk * 4));
-    HIP_TRY(hipMalloc(&W.R32, Lk * 4));
-    HIP_TRY(hipMalloc(&W.P32, Lk * 4));
-    HIP_TRY(hipMalloc(&W.AP32, Lk * 4));
-    HIP_TRY(hipMalloc(&W.part, (int64_t)4096 * 2 * k * 8));
+    HIP_TRY(palloc(&W.X64, Lk * 8, M.stream));
+    HIP_TRY(palloc(&W.R64, Lk * 8, M.stream));
+    HIP_TRY(palloc(&W.P64, Lk * 8, M.stream));
+    HIP_TRY(palloc(&W.AP64, Lk * 8, M.stream));
+    HIP_TRY(palloc(&W.B64, Lk * 8, M.stream));
+    HIP_TRY(palloc(&W.X32, Lk * 4, M.stream));
+    HIP_TRY(palloc(&W.R32, Lk * 4, M.stream));
+    HIP_TRY(palloc(&W.P32, Lk * 4, M.stream));
+    HIP_TRY(palloc(&W.AP32, Lk * 4, M.stream));
+    HIP_TRY(palloc(&W.part, (int64_t)4096 * 2 * k * 8, M.stream));
     const int64_t dlp = M.dl ? (int64_t)M.nlong * spmm_grid(M.L, k) : 0;
-    HIP_TRY(hipMalloc(&W.lpart, std::max<int64_t>(1, std::max<int64_t>(M.nchunk, dlp) * k) * 8));
-    HIP_TRY(hipMalloc(&W.colbuf, (int64_t)k * 8));
-    if (M.blk && k <= kSmallK) HIP_TRY(hipMalloc(&W.gpart, (int64_t)8 * Lk * 8));
-    HIP_TRY(hipMalloc(&W.sd, ((int64_t)6 * k + 1) * 8));
-    HIP_TRY(hipMalloc(&W.si, ((int64_t)3 * k + 8) * 4));
-    HIP_TRY(hipMalloc(&W.cnt, 8 * sizeof(unsigned)));
-    HIP_TRY(hipMemset(W.cnt, 0, 8 * sizeof(unsigned)));
-    HIP_TRY(hipMemset(W.sd, 0, ((int64_t)6 * k + 1) * 8));
+    HIP_TRY(palloc(&W.lpart, std::max<int64_t>(1, std::max<int64_t>(M.nchunk, dlp) * k) * 8, M.stream));
+    HIP_TRY(palloc(&W.colbuf, (int64_t)k * 8, M.stream));
+    if (M.blk && k <= kSmallK) HIP_TRY(palloc(&W.gpart, (int64_t)8 * Lk * 8, M.stream));
+    HIP_TRY(palloc(&W.sd, ((int64_t)6 * k + 1) * 8, M.stream));
+    HIP_TRY(palloc(&W.si, ((int64_t)3 * k + 8) * 4, M.stream));
+    HIP_TRY(palloc(&W.cnt, 8 * sizeof(unsigned), M.stream));
+    HIP_TRY(hipMemsetAsync(W.cnt, 0, 8 * sizeof(unsigned), M.stream));
+    HIP_TRY(hipMemsetAsync(W.sd, 0, ((int64_t)6 * k + 1) * 8, M.stream));
     W.k = k;
     W.L = M.L;
     return tune_blocked_both(M, k);
@@ -1082,35 +1094,35 @@ static int build_blocked(Matrix &M, const std::vector<int> &ip32) {
         for (int k = 1; k <= kSmallK; ++k)
             for (int t = 0; t <= M.nlong; ++t)
                 cptr[(k - 1) * (M.nlong + 1) + t] = t * spmm_grid(L, k);
-        HIP_TRY(hipMalloc(&M.dl_cptr, cptr.size() * 4));
+        HIP_TRY(palloc(&M.dl_cptr, cptr.size() * 4, ms));
         HIP_TRY(hipMemcpyAsync(M.dl_cptr, cptr.data(), cptr.size() * 4, hipMemcpyHostToDevice, ms));
-        HIP_TRY(hipMalloc(&M.dl_slot, L * 4));
+        HIP_TRY(palloc(&M.dl_slot, L * 4, ms));
         HIP_TRY(hipMemcpyAsync(M.dl_slot, slot_h.data(), L * 4, hipMemcpyHostToDevice, ms));
-        HIP_TRY(hipMalloc(&M.dt64, (int64_t)M.nlong * L * 8));
-        HIP_TRY(hipMalloc(&M.dt32, (int64_t)M.nlong * L * 4));
-        HIP_TRY(hipMalloc(&M.ll64, (int64_t)M.nlong * M.nlong * 8));
-        HIP_TRY(hipMalloc(&M.ll32, (int64_t)M.nlong * M.nlong * 4));
+        HIP_TRY(palloc(&M.dt64, (int64_t)M.nlong * L * 8, ms));
+        HIP_TRY(palloc(&M.dt32, (int64_t)M.nlong * L * 4, ms));
+        HIP_TRY(palloc(&M.ll64, (int64_t)M.nlong * M.nlong * 8, ms));
+        HIP_TRY(palloc(&M.ll32, (int64_t)M.nlong * M.nlong * 4, ms));
         HIP_TRY(hipMemsetAsync(M.dt64, 0, (int64_t)M.nlong * L * 8, ms));
         HIP_TRY(hipMemsetAsync(M.dt32, 0, (int64_t)M.nlong * L * 4, ms));
         HIP_TRY(hipMemsetAsync(M.ll64, 0, (int64_t)M.nlong * M.nlong * 8, ms));
         HIP_TRY(hipMemsetAsync(M.ll32, 0, (int64_t)M.nlong * M.nlong * 4, ms));
     }
     M.col_lo_h[8] = (int)L;
-    HIP_TRY(hipMalloc(&M.col_lo, 9 * 4));
+    HIP_TRY(palloc(&M.col_lo, 9 * 4, ms));
     HIP_TRY(hipMemcpyAsync(M.col_lo, M.col_lo_h, 9 * 4, hipMemcpyHostToDevice, ms));
     DevBuf<int> cnt;
     DEV_ALLOC(cnt, 8 * L + 1, ms);
     HIP_TRY(hipMemsetAsync(cnt.p + 8 * L, 0, 4, ms));
     hipLaunchKernelGGL(k_blk_rowcount, dim3(grid1(L)), dim3(kThreads), 0, ms, L, M.indptr,
                        M.indices, M.col_lo, M.dl ? (const int *)M.dl_slot : nullptr, cnt.p);
-    HIP_TRY(hipMalloc(&M.bptr, (8 * L + 1) * 4));
+    HIP_TRY(palloc(&M.bptr, (8 * L + 1) * 4, ms));
     HIP_TRY(exclusive_sum(tmp, cnt.p, M.bptr, 8 * L + 1, ms));
     int last = 0;
     HIP_TRY(scalar_to_host(&last, M.bptr + 8 * L, ms));
     M.bnnz = last;
-    HIP_TRY(hipMalloc(&M.bidx, std::max<int64_t>(M.bnnz, 1) * 4));
-    HIP_TRY(hipMalloc(&M.bval64, std::max<int64_t>(M.bnnz, 1) * 8));
-    HIP_TRY(hipMalloc(&M.bval32, std::max<int64_t>(M.bnnz, 1) * 4));
+    HIP_TRY(palloc(&M.bidx, std::max<int64_t>(M.bnnz, 1) * 4, ms));
+    HIP_TRY(palloc(&M.bval64, std::max<int64_t>(M.bnnz, 1) * 8, ms));
+    HIP_TRY(palloc(&M.bval32, std::max<int64_t>(M.bnnz, 1) * 4, ms));
     hipLaunchKernelGGL(k_blk_fill, dim3(grid1(L)), dim3(kThreads), 0, ms, L, M.indptr, M.indices,
                        M.val64, M.val32, M.col_lo, M.bptr, M.bidx, M.bval64, M.bval32,
                        M.dl ? (const int *)M.dl_slot : nullptr, M.dt64, M.dt32);
@@ -1580,15 +1592,15 @@ int matrix_create(int64_t L, int64_t nnz, const std::vector<int> &ip32, const in
     M->nchunk = (int)crow.size();
     HIP_TRY(hipStreamCreateWithFlags(&M->stream, hipStreamNonBlocking));
     HIP_TRY(hipEventCreateWithFlags(&M->ev, hipEventDisableTiming));
-    HIP_TRY(hipMalloc(&M->indptr, (L + 1) * 4));
-    HIP_TRY(hipMalloc(&M->indices, std::max<int64_t>(nnz, 1) * 4));
-    HIP_TRY(hipMalloc(&M->val64, std::max<int64_t>(nnz, 1) * 8));
-    HIP_TRY(hipMalloc(&M->val32, std::max<int64_t>(nnz, 1) * 4));
-    HIP_TRY(hipMalloc(&M->dinv64, std::max<int64_t>(L, 1) * 8));
-    HIP_TRY(hipMalloc(&M->dinv32, std::max<int64_t>(L, 1) * 4));
+    hipStream_t ms = M->stream;
+    HIP_TRY(palloc(&M->indptr, (L + 1) * 4, ms));
+    HIP_TRY(palloc(&M->indices, std::max<int64_t>(nnz, 1) * 4, ms));
+    HIP_TRY(palloc(&M->val64, std::max<int64_t>(nnz, 1) * 8, ms));
+    HIP_TRY(palloc(&M->val32, std::max<int64_t>(nnz, 1) * 4, ms));
+    HIP_TRY(palloc(&M->dinv64, std::max<int64_t>(L, 1) * 8, ms));
+    HIP_TRY(palloc(&M->dinv32, std::max<int64_t>(L, 1) * 4, ms));
     // every upload and the FP32 casts in order on the matrix's own stream (a
     // device-to-device hipMemcpy may return before the copy has finished)
-    hipStream_t ms = M->stream;
     HIP_TRY(hipMemcpyAsync(M->indptr, ip32.data(), (L + 1) * 4, hipMemcpyHostToDevice, ms));
     const hipMemcpyKind up = on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
     if (nnz) {
@@ -1604,14 +1616,15 @@ int matrix_create(int64_t L, int64_t nnz, const std::vector<int> &ip32, const in
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(ms));
     if (M->nlong) {
-        HIP_TRY(hipMalloc(&M->long_rows, M->nlong * 4));
-        HIP_TRY(hipMalloc(&M->long_cptr, (M->nlong + 1) * 4));
-        HIP_TRY(hipMalloc(&M->chunk_row, M->nchunk * 4));
-        HIP_TRY(hipMalloc(&M->chunk_beg, M->nchunk * 4));
-        HIP_TRY(hipMemcpy(M->long_rows, lrows.data(), M->nlong * 4, hipMemcpyHostToDevice));
-        HIP_TRY(hipMemcpy(M->long_cptr, lcptr.data(), (M->nlong + 1) * 4, hipMemcpyHostToDevice));
-        HIP_TRY(hipMemcpy(M->chunk_row, crow.data(), M->nchunk * 4, hipMemcpyHostToDevice));
-        HIP_TRY(hipMemcpy(M->chunk_beg, cbeg.data(), M->nchunk * 4, hipMemcpyHostToDevice));
+        HIP_TRY(palloc(&M->long_rows, M->nlong * 4, ms));
+        HIP_TRY(palloc(&M->long_cptr, (M->nlong + 1) * 4, ms));
+        HIP_TRY(palloc(&M->chunk_row, M->nchunk * 4, ms));
+        HIP_TRY(palloc(&M->chunk_beg, M->nchunk * 4, ms));
+        HIP_TRY(hipMemcpyAsync(M->long_rows, lrows.data(), M->nlong * 4, hipMemcpyHostToDevice, ms));
+        HIP_TRY(hipMemcpyAsync(M->long_cptr, lcptr.data(), (M->nlong + 1) * 4,
+                               hipMemcpyHostToDevice, ms));
+        HIP_TRY(hipMemcpyAsync(M->chunk_row, crow.data(), M->nchunk * 4, hipMemcpyHostToDevice, ms));
+        HIP_TRY(hipMemcpyAsync(M->chunk_beg, cbeg.data(), M->nchunk * 4, hipMemcpyHostToDevice, ms));
     }
     HIP_TRY(hipStreamSynchronize(M->stream));
     M->ip_host = ip32;  // kept for the blocked layout, built at the first small-k use
@@ -1879,13 +1892,13 @@ void hdfe_destroy(void *h) {
     (void)hipSetDevice(M->device);
     if (M->stream) (void)hipStreamSynchronize(M->stream);
     M->drop_graphs();
-    M->w.release();
+    M->w.release(M->stream);
     void *ps[] = {M->indptr, M->indices, M->val64, M->val32, M->dinv64, M->dinv32,
                   M->long_rows, M->long_cptr, M->chunk_row, M->chunk_beg, M->col_lo, M->bptr,
                   M->bidx, M->bval64, M->bval32, M->dl_slot, M->dl_cptr, M->dt64, M->dt32,
                   M->ll64, M->ll32};
-    for (void *p : ps)
-        if (p) (void)hipFree(p);
+    for (void *p : ps) pfree(p, M->stream);
+    if (M->stream) (void)hipStreamSynchronize(M->stream);
     if (M->ev) (void)hipEventDestroy(M->ev);
     if (M->stream) (void)hipStreamDestroy(M->stream);
     delete M;
