@@ -580,6 +580,40 @@ __global__ void k_inc_keys(int64_t M, int D, const int *__restrict__ cgroup,
     }
 }
 
+// Each identifying group's incidence keys put in order within the group's
+// segment (keys are emitted cell-major and cells are in group order, so a
+// group's keys are contiguous): W lanes per group, every key's place counted
+// against the segment's others, ties by position, so the order is the one a
+// stable sort gives.  A segment longer than W raises `too_long` and is left
+// for the caller's radix sort.
+template <int W>
+__global__ void k_seg_rank_sort(int64_t Gi, int D, const int *__restrict__ igstart,
+                                const uint64_t *__restrict__ kin, const int *__restrict__ vin,
+                                uint64_t *__restrict__ kout, int *__restrict__ vout,
+                                int *__restrict__ too_long) {
+    const int sub = threadIdx.x % W;
+    for (int64_t g = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / W; g < Gi;
+         g += (int64_t)gridDim.x * blockDim.x / W) {
+        const int64_t s = (int64_t)igstart[g] * D, e = (int64_t)igstart[g + 1] * D;
+        const int64_t len = e - s;
+        if (len > W) {
+            if (sub == 0) *too_long = 1;
+            continue;
+        }
+        const bool in = sub < len;
+        const uint64_t k = in ? kin[s + sub] : ~0ull;
+        int rank = 0;
+        for (int t = 0; t < W; ++t) {
+            const uint64_t kt = __shfl(k, t, W);
+            rank += (t < len) && (kt < k || (kt == k && t < sub));
+        }
+        if (in) {
+            kout[s + rank] = k;
+            vout[s + rank] = vin[s + sub];
+        }
+    }
+}
+
 template <int W>
 __global__ void k_cell_group(int64_t Gi, const int *__restrict__ igstart, int *__restrict__ cg) {
     const int sub = threadIdx.x % W;
@@ -1498,8 +1532,36 @@ struct Panel {
     double t_build = 0.0;
 };
 
+// Dense-range factorization: values within a range of at most a few times
+// n are coded through a presence table and its prefix sum (no sort).
+__global__ void k_mark_present(int64_t n, const int64_t *__restrict__ col, int64_t lo,
+                               int *__restrict__ present) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        present[col[i] - lo] = 1;
+}
+
+__global__ void k_dense_codes(int64_t n, const int64_t *__restrict__ col, int64_t lo,
+                              const int *__restrict__ rank, int *__restrict__ code) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        code[i] = rank[col[i] - lo];
+}
+
+__global__ void k_dense_levels(int64_t range, const int *__restrict__ present,
+                               const int *__restrict__ rank, int64_t lo,
+                               int64_t *__restrict__ levels) {
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < range;
+         r += (int64_t)gridDim.x * blockDim.x)
+        if (present[r]) levels[rank[r]] = lo + r;
+}
+
 // Codes of an int64 column in sorted level order (core/passes.py:329-335):
-// code (n) and the level values (nl), on stream s.
+// code (n) and the level values (nl), on stream s.  A column whose values
+// span at most max(2n, 2^22) integers is coded through a presence table
+// (the usual case for ids, years, and ids already coded); any other by a
+// radix sort of the values.  Both give every value its rank among the
+// distinct values.
 int factorize_col(hipStream_t s, Temp &tmp, const int64_t *col, int64_t n, DevBuf<int> &code,
                   DevBuf<int64_t> &levels, int64_t &nl) {
     // min / max for the key range
@@ -1513,6 +1575,29 @@ int factorize_col(hipStream_t s, Temp &tmp, const int64_t *col, int64_t n, DevBu
     for (int b = 0; b < nb; ++b) {
         lo = std::min(lo, h[2 * b]);
         hi = std::max(hi, h[2 * b + 1]);
+    }
+    const uint64_t span = (uint64_t)hi - (uint64_t)lo;
+    if (n > 0 && span < (uint64_t)std::max<int64_t>(2 * n, (int64_t)1 << 22) &&
+        span < (uint64_t)INT32_MAX - 1) {
+        const int64_t range = (int64_t)span + 1;
+        DevBuf<int> present, rank;
+        DEV_ALLOC(present, range + 1, s);
+        DEV_ALLOC(rank, range + 1, s);
+        HIP_TRY(hipMemsetAsync(present.p, 0, (range + 1) * 4, s));
+        hipLaunchKernelGGL(k_mark_present, dim3(grid1(n)), dim3(T256), 0, s, n, col, lo,
+                           present.p);
+        HIP_TRY(exclusive_sum(tmp, present.p, rank.p, range + 1, s));
+        int cnt = 0;
+        HIP_TRY(scalar_to_host(&cnt, rank.p + range, s));
+        nl = cnt;
+        DEV_ALLOC(code, n, s);
+        DEV_ALLOC(levels, nl, s);
+        hipLaunchKernelGGL(k_dense_codes, dim3(grid1(n)), dim3(T256), 0, s, n, col, lo, rank.p,
+                           code.p);
+        hipLaunchKernelGGL(k_dense_levels, dim3(grid1(range)), dim3(T256), 0, s, range,
+                           present.p, rank.p, lo, levels.p);
+        HIP_TRY(hipGetLastError());
+        return HDFE_STATUS_OK;
     }
     const int bits = std::max(1, bits_for((uint64_t)(hi - lo)));
     DevBuf<uint64_t> k1, k2;
@@ -2008,7 +2093,35 @@ int hdfe_panel_system(void *h, int with_rhs, void **matrix, double *seconds) {
     if (nE) {
         hipLaunchKernelGGL(k_inc_keys, dim3(grid1(nE)), dim3(T256), 0, s, M, D, P.cgroup.p,
                            P.icodes.p, M, P.offs.p, lbits, k1.p, v1.p);
-        HIP_TRY(sort_pairs(P.tmp, k1.p, k2.p, v1.p, v2.p, nE, 0, lbits + gbits, s));
+        // within-group ranks when every group's segment fits a few lanes,
+        // else the radix sort (the same order either way)
+        int W = 4;
+        while (W < 64 && W < 2 * D * cw) W *= 2;
+        bool sorted = false;
+        DevBuf<int> flag;
+        DEV_ALLOC(flag, 1, s);
+        for (int attempt = 0; attempt < 2 && !sorted; ++attempt) {
+            if (attempt == 1) {
+                if (W == 64) break;
+                W = 64;  // the widest segments a wave can rank
+            }
+            HIP_TRY(hipMemsetAsync(flag.p, 0, 4, s));
+#define HDFE_SEG(WW)                                                                            \
+    hipLaunchKernelGGL(k_seg_rank_sort<WW>, dim3(grid1(Gi * WW)), dim3(T256), 0, s, Gi, D,      \
+                       P.igstart.p, k1.p, v1.p, k2.p, v2.p, flag.p)
+            switch (W) {
+            case 4: HDFE_SEG(4); break;
+            case 8: HDFE_SEG(8); break;
+            case 16: HDFE_SEG(16); break;
+            case 32: HDFE_SEG(32); break;
+            default: HDFE_SEG(64); break;
+            }
+#undef HDFE_SEG
+            int tl = 0;
+            HIP_TRY(scalar_to_host(&tl, flag.p, s));
+            sorted = !tl;
+        }
+        if (!sorted) HIP_TRY(sort_pairs(P.tmp, k1.p, k2.p, v1.p, v2.p, nE, 0, lbits + gbits, s));
         hipLaunchKernelGGL(k_heads_u64, dim3(grid1(nE)), dim3(T256), 0, s, nE, k2.p, head.p);
         HIP_TRY(inclusive_sum(P.tmp, head.p, rank.p, nE, s));
         HIP_TRY(scalar_to_host(&E, rank.p + nE - 1, s));
@@ -2040,7 +2153,9 @@ int hdfe_panel_system(void *h, int with_rhs, void **matrix, double *seconds) {
     if (E) {
         hipLaunchKernelGGL(k_level_keys, dim3(grid1(E)), dim3(T256), 0, s, (int64_t)E, P.ig.p,
                            P.il.p, gbits, k1.p, v1.p);
-        HIP_TRY(sort_pairs(P.tmp, k1.p, k2.p, v1.p, P.lq.p, E, 0, lbits + gbits, s));
+        // the incidences are in (group, level) order: a stable sort on the
+        // level bits alone leaves each level's groups in order
+        HIP_TRY(sort_pairs(P.tmp, k1.p, k2.p, v1.p, P.lq.p, E, gbits, lbits + gbits, s));
     }
     hipLaunchKernelGGL(k_level_ptr, dim3(grid1(L + 1)), dim3(T256), 0, s, L, (int64_t)E, k2.p,
                        gbits, P.lptr.p);
