@@ -614,6 +614,26 @@ __global__ void k_seg_rank_sort(int64_t Gi, int D, const int *__restrict__ igsta
     }
 }
 
+// Whether the rows are already in group order, and each group's first row
+// (group codes are dense, so in order every group has a first row).
+__global__ void k_group_order(int64_t n, int64_t G, const int *__restrict__ g,
+                              int *__restrict__ start, int *__restrict__ unsorted) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int gi = g[i];
+        if (i > 0 && g[i - 1] > gi) *unsorted = 1;
+        if (i == 0 || g[i - 1] != gi) start[gi] = (int)i;
+        if (i == n - 1) start[G] = (int)n;
+    }
+}
+
+// Ranks within segments of at most W keys (k_seg_rank_sort) at the widths a
+// wave allows, trying the given width first and then 64 lanes; false when a
+// segment is longer (the caller's radix sort then orders them).
+static int seg_rank_sort(hipStream_t s, int64_t nseg, int mult, const int *start,
+                         const uint64_t *kin, const int *vin, uint64_t *kout, int *vout,
+                         int W, bool &done);
+
 template <int W>
 __global__ void k_cell_group(int64_t Gi, const int *__restrict__ igstart, int *__restrict__ cg) {
     const int sub = threadIdx.x % W;
@@ -710,46 +730,52 @@ __global__ void k_level_keys(int64_t E, const int *__restrict__ ig, const int *_
 // partials of a level are then added in chunk order.
 constexpr int kChunk = 2048;
 
-__global__ void k_diag_rhs_part(int64_t nchunk, int m, const int *__restrict__ chunk_lvl,
-                                const int *__restrict__ chunk_beg, const int *__restrict__ lptr,
-                                const int *__restrict__ lq, const int *__restrict__ ig,
-                                const double *__restrict__ iu, const double *__restrict__ is,
-                                const double *__restrict__ Ng, const double *__restrict__ vbar,
-                                double *__restrict__ part) {
-    // one sweep of the chunk's incidences per block of eight columns (one
-    // sweep in all for m <= 8): diag and the rhs columns together
+// Each incidence's terms, in incidence order (its group's N_g and vbar are
+// read next to its neighbours'): contrib[q] = (u - u^2 / N_g, s_j - u vbar_j).
+__global__ void k_inc_contrib(int64_t E, int m, const int *__restrict__ ig,
+                              const double *__restrict__ iu, const double *__restrict__ is,
+                              const double *__restrict__ Ng, const double *__restrict__ vbar,
+                              double *__restrict__ contrib) {
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < E;
+         q += (int64_t)gridDim.x * blockDim.x) {
+        const double u = iu[q];
+        const int gq = ig[q];
+        double *o = contrib + q * (m + 1);
+        o[0] = u - u * u / Ng[gq];
+        for (int j = 0; j < m; ++j) o[1 + j] = is[q * m + j] - u * vbar[(int64_t)gq * m + j];
+    }
+}
+
+// The level sums of the contributions: one wave per fixed chunk of a level's
+// incidences (lanes strided, then a fixed butterfly) writes a partial, one
+// gather of a contiguous (m + 1)-row per incidence.
+__global__ void k_diag_rhs_part_c(int64_t nchunk, int m, const int *__restrict__ chunk_lvl,
+                                  const int *__restrict__ chunk_beg, const int *__restrict__ lptr,
+                                  const int *__restrict__ lq, const double *__restrict__ contrib,
+                                  double *__restrict__ part) {
     const int lane = threadIdx.x & 63;
     const int64_t waves = (int64_t)gridDim.x * (blockDim.x / 64);
     for (int64_t c = blockIdx.x * (int64_t)(blockDim.x / 64) + threadIdx.x / 64; c < nchunk;
          c += waves) {
         const int a = chunk_lvl[c];
         const int b = chunk_beg[c], e = min(b + kChunk, lptr[a + 1]);
-        for (int j0 = 0; j0 < m || j0 == 0; j0 += 8) {
-            const int nj = min(8, m - j0);
-            double d = 0.0, r[8];
+        for (int j0 = 0; j0 <= m; j0 += 8) {
+            const int nj = min(8, m + 1 - j0);
+            double r[8];
 #pragma unroll
             for (int j = 0; j < 8; ++j) r[j] = 0.0;
             for (int t = b + lane; t < e; t += 64) {
-                const int q = lq[t];
-                const double u = iu[q];
-                const int gq = ig[q];
-                if (j0 == 0) d += u - u * u / Ng[gq];
+                const double *o = contrib + (int64_t)lq[t] * (m + 1) + j0;
 #pragma unroll
                 for (int j = 0; j < 8; ++j)
-                    if (j < nj)
-                        r[j] += is[(int64_t)q * m + j0 + j] - u * vbar[(int64_t)gq * m + j0 + j];
-            }
-            if (j0 == 0) {
-                d = wave_sum(d);
-                if (lane == 0) part[c * (m + 1)] = d;
+                    if (j < nj) r[j] += o[j];
             }
 #pragma unroll
             for (int j = 0; j < 8; ++j) {
                 if (j >= nj) break;
                 const double v = wave_sum(r[j]);
-                if (lane == 0) part[c * (m + 1) + 1 + j0 + j] = v;
+                if (lane == 0) part[c * (m + 1) + j0 + j] = v;
             }
-            if (m == 0) break;
         }
     }
 }
@@ -946,37 +972,65 @@ __global__ void k_sparse_flag(int64_t E, const int *__restrict__ il,
 // Dense x dense block: per workgroup of groups, thread per (t1, t2) pair:
 // sum of n_g(t1, t2) - U_t1 U_t2 / N_g and a presence count; partials
 // reduced in a fixed order by k_dd_finish.
-__global__ void k_dd(int64_t Gi, int Ld, const int *__restrict__ dense_levels,
+__global__ __launch_bounds__(256) void k_dd(int64_t Gi, int Ld, const int *__restrict__ dense_levels,
                      const int *__restrict__ lvl_dim, const int64_t *__restrict__ offs,
                      const double *__restrict__ U, const double *__restrict__ Ng,
                      const int *__restrict__ igstart, const int *__restrict__ icodes,
                      int64_t ild, const double *__restrict__ in, double *__restrict__ part,
                      int *__restrict__ ppart) {
+    __shared__ double sv[T256];
+    __shared__ int sp[T256];
     const int npair = Ld * Ld;
+    // with few pairs, nsub slices of the block's groups run side by side
+    // and are added in slice order (fixed)
+    const int nsub = npair * 2 <= (int)blockDim.x ? (int)blockDim.x / npair : 1;
     const int64_t g0 = part_begin(Gi, blockIdx.x, gridDim.x), g1 = part_begin(Gi, blockIdx.x + 1, gridDim.x);
-    for (int pr = threadIdx.x; pr < npair; pr += blockDim.x) {
-        const int t1 = pr / Ld, t2 = pr % Ld;
-        const int b1 = dense_levels[t1], b2 = dense_levels[t2];
-        const int d1 = lvl_dim[b1], d2 = lvl_dim[b2];
-        const int l1 = b1 - (int)offs[d1], l2 = b2 - (int)offs[d2];
+    const int stride = nsub > 1 ? npair : (int)blockDim.x;
+    const int sub = nsub > 1 ? (int)threadIdx.x / npair : 0;
+    for (int pr0 = 0; pr0 < npair; pr0 += stride) {
+        const int pr = pr0 + (nsub > 1 ? (int)threadIdx.x % npair : (int)threadIdx.x);
+        const bool act = sub < nsub && pr < npair;
         double v = 0.0;
         int pres = 0;
-        for (int64_t g = g0; g < g1; ++g) {
-            const double u1 = U[g * Ld + t1], u2 = U[g * Ld + t2];
-            if (u1 == 0.0 || u2 == 0.0) continue;
-            pres = 1;
-            double nat;
-            if (d1 == d2) {
-                nat = (t1 == t2) ? u1 : 0.0;
-            } else {
-                nat = 0.0;
-                for (int c = igstart[g]; c < igstart[g + 1]; ++c)
-                    if (icodes[d1 * ild + c] == l1 && icodes[d2 * ild + c] == l2) nat += in[c];
+        if (act) {
+            const int t1 = pr / Ld, t2 = pr % Ld;
+            const int b1 = dense_levels[t1], b2 = dense_levels[t2];
+            const int d1 = lvl_dim[b1], d2 = lvl_dim[b2];
+            const int l1 = b1 - (int)offs[d1], l2 = b2 - (int)offs[d2];
+            for (int64_t g = g0 + sub; g < g1; g += nsub) {
+                const double u1 = U[g * Ld + t1], u2 = U[g * Ld + t2];
+                if (u1 == 0.0 || u2 == 0.0) continue;
+                pres = 1;
+                double nat;
+                if (d1 == d2) {
+                    nat = (t1 == t2) ? u1 : 0.0;
+                } else {
+                    nat = 0.0;
+                    for (int c = igstart[g]; c < igstart[g + 1]; ++c)
+                        if (icodes[d1 * ild + c] == l1 && icodes[d2 * ild + c] == l2) nat += in[c];
+                }
+                v += nat - u1 * u2 / Ng[g];
             }
-            v += nat - u1 * u2 / Ng[g];
         }
-        part[(int64_t)blockIdx.x * npair + pr] = v;
-        ppart[(int64_t)blockIdx.x * npair + pr] = pres;
+        if (nsub > 1) {
+            sv[threadIdx.x] = v;
+            sp[threadIdx.x] = pres;
+            __syncthreads();
+            if ((int)threadIdx.x < npair) {
+                double tv = 0.0;
+                int tp = 0;
+                for (int q = 0; q < nsub; ++q) {
+                    tv += sv[q * npair + threadIdx.x];
+                    tp |= sp[q * npair + threadIdx.x];
+                }
+                part[(int64_t)blockIdx.x * npair + threadIdx.x] = tv;
+                ppart[(int64_t)blockIdx.x * npair + threadIdx.x] = tp;
+            }
+            __syncthreads();
+        } else if (act) {
+            part[(int64_t)blockIdx.x * npair + pr] = v;
+            ppart[(int64_t)blockIdx.x * npair + pr] = pres;
+        }
     }
 }
 
@@ -1556,6 +1610,36 @@ __global__ void k_dense_levels(int64_t range, const int *__restrict__ present,
         if (present[r]) levels[rank[r]] = lo + r;
 }
 
+static int seg_rank_sort(hipStream_t s, int64_t nseg, int mult, const int *start,
+                         const uint64_t *kin, const int *vin, uint64_t *kout, int *vout,
+                         int W, bool &done) {
+    done = false;
+    DevBuf<int> flag;
+    DEV_ALLOC(flag, 1, s);
+    for (int attempt = 0; attempt < 2 && !done; ++attempt) {
+        if (attempt == 1) {
+            if (W >= 64) break;
+            W = 64;
+        }
+        HIP_TRY(hipMemsetAsync(flag.p, 0, 4, s));
+#define HDFE_SEG(WW)                                                                             \
+    hipLaunchKernelGGL(k_seg_rank_sort<WW>, dim3(grid1(nseg * WW)), dim3(T256), 0, s, nseg, mult, \
+                       start, kin, vin, kout, vout, flag.p)
+        switch (W) {
+        case 4: HDFE_SEG(4); break;
+        case 8: HDFE_SEG(8); break;
+        case 16: HDFE_SEG(16); break;
+        case 32: HDFE_SEG(32); break;
+        default: HDFE_SEG(64); break;
+        }
+#undef HDFE_SEG
+        int tl = 0;
+        HIP_TRY(scalar_to_host(&tl, flag.p, s));
+        done = !tl;
+    }
+    return HDFE_STATUS_OK;
+}
+
 // Codes of an int64 column in sorted level order (core/passes.py:329-335):
 // code (n) and the level values (nl), on stream s.  A column whose values
 // span at most max(2n, 2^22) integers is coded through a presence table
@@ -1938,6 +2022,24 @@ int hdfe_panel_create(int64_t n_rows, int nfe, const int64_t *const *fe, int m,
     DEV_ALLOC(i2, n, s);
     hipLaunchKernelGGL(k_pack_key, dim3(grid1(n)), dim3(T256), 0, s, n, D, code[gd].p, codes_in.p,
                        n, shift_d.p, tot, keep.p, full.p, i1.p);
+    // rows stored in group order (a panel kept by worker) need only each
+    // group's rows ranked by their codes: the order the radix sort gives
+    bool rows_sorted = false;
+    if (!sort_vars && P->singletons == 0 && P->G > 0 && n < INT32_MAX) {
+        DevBuf<int> gst, unsorted;
+        DEV_ALLOC(gst, P->G + 1, s);
+        DEV_ALLOC(unsorted, 1, s);
+        HIP_TRY(hipMemsetAsync(unsorted.p, 0, 4, s));
+        hipLaunchKernelGGL(k_group_order, dim3(grid1(n)), dim3(T256), 0, s, n, P->G, code[gd].p,
+                           gst.p, unsorted.p);
+        int un = 0;
+        HIP_TRY(scalar_to_host(&un, unsorted.p, s));
+        if (!un) {
+            int W = 4;
+            while (W < 64 && (int64_t)W * P->G < n) W *= 2;
+            TRY(seg_rank_sort(s, P->G, 1, gst.p, full.p, i1.p, k2.p, i2.p, W, rows_sorted));
+        }
+    }
     DevBuf<double> Vin;
     DevBuf<const double *> colp;
     DEV_ALLOC(colp, m, s);
@@ -1958,11 +2060,12 @@ int hdfe_panel_create(int64_t n_rows, int nfe, const int64_t *const *fe, int m,
             first = false;
         }
         hipLaunchKernelGGL(k_gather_key, dim3(grid1(n)), dim3(T256), 0, s, n, full.p, i1.p, k1.p);
-    } else {
+    } else if (!rows_sorted) {
         HIP_TRY(hipMemcpyAsync(k1.p, full.p, n * 8, hipMemcpyDeviceToDevice, s));
     }
     // kept keys are below 2^(tot + gbits); a dropped row's key has that bit set
-    HIP_TRY(sort_pairs(P->tmp, k1.p, k2.p, i1.p, i2.p, n, 0, tot + gbits + 1, s));
+    if (!rows_sorted)
+        HIP_TRY(sort_pairs(P->tmp, k1.p, k2.p, i1.p, i2.p, n, 0, tot + gbits + 1, s));
     const int64_t nk = P->n;
     P->perm = std::move(i2);
     // gather the sorted columns (kept rows come first)
@@ -2096,31 +2199,9 @@ int hdfe_panel_system(void *h, int with_rhs, void **matrix, double *seconds) {
         // within-group ranks when every group's segment fits a few lanes,
         // else the radix sort (the same order either way)
         int W = 4;
-        while (W < 64 && W < 2 * D * cw) W *= 2;
+        while (W < 64 && W < D * cw) W *= 2;
         bool sorted = false;
-        DevBuf<int> flag;
-        DEV_ALLOC(flag, 1, s);
-        for (int attempt = 0; attempt < 2 && !sorted; ++attempt) {
-            if (attempt == 1) {
-                if (W == 64) break;
-                W = 64;  // the widest segments a wave can rank
-            }
-            HIP_TRY(hipMemsetAsync(flag.p, 0, 4, s));
-#define HDFE_SEG(WW)                                                                            \
-    hipLaunchKernelGGL(k_seg_rank_sort<WW>, dim3(grid1(Gi * WW)), dim3(T256), 0, s, Gi, D,      \
-                       P.igstart.p, k1.p, v1.p, k2.p, v2.p, flag.p)
-            switch (W) {
-            case 4: HDFE_SEG(4); break;
-            case 8: HDFE_SEG(8); break;
-            case 16: HDFE_SEG(16); break;
-            case 32: HDFE_SEG(32); break;
-            default: HDFE_SEG(64); break;
-            }
-#undef HDFE_SEG
-            int tl = 0;
-            HIP_TRY(scalar_to_host(&tl, flag.p, s));
-            sorted = !tl;
-        }
+        TRY(seg_rank_sort(s, Gi, D, P.igstart.p, k1.p, v1.p, k2.p, v2.p, W, sorted));
         if (!sorted) HIP_TRY(sort_pairs(P.tmp, k1.p, k2.p, v1.p, v2.p, nE, 0, lbits + gbits, s));
         hipLaunchKernelGGL(k_heads_u64, dim3(grid1(nE)), dim3(T256), 0, s, nE, k2.p, head.p);
         HIP_TRY(inclusive_sum(P.tmp, head.p, rank.p, nE, s));
@@ -2175,9 +2256,15 @@ int hdfe_panel_system(void *h, int with_rhs, void **matrix, double *seconds) {
         DEV_ALLOC(part, (int64_t)nchunk * (m + 1), s);
         hipLaunchKernelGGL(k_level_chunks, dim3(grid1(L)), dim3(T256), 0, s, L, P.lptr.p, cptr.p,
                            clvl.p, cbeg.p);
-        hipLaunchKernelGGL(k_diag_rhs_part, dim3(grid1((int64_t)nchunk * 64)), dim3(T256), 0, s,
-                           (int64_t)nchunk, m, clvl.p, cbeg.p, P.lptr.p, P.lq.p, P.ig.p, P.iu.p,
-                           P.is.p, P.Ng.p, P.vbar.p, part.p);
+        {
+            DevBuf<double> contrib;
+            DEV_ALLOC(contrib, std::max<int64_t>((int64_t)P.E * (m + 1), 1), s);
+            hipLaunchKernelGGL(k_inc_contrib, dim3(grid1(P.E)), dim3(T256), 0, s, (int64_t)P.E, m,
+                               P.ig.p, P.iu.p, P.is.p, P.Ng.p, P.vbar.p, contrib.p);
+            hipLaunchKernelGGL(k_diag_rhs_part_c, dim3(grid1((int64_t)nchunk * 64)), dim3(T256), 0,
+                               s, (int64_t)nchunk, m, clvl.p, cbeg.p, P.lptr.p, P.lq.p, contrib.p,
+                               part.p);
+        }
         hipLaunchKernelGGL(k_diag_rhs_sum, dim3(grid1(L)), dim3(T256), 0, s, L, m, cptr.p, part.p,
                            P.diag.p, with_rhs ? P.rhs.p : (double *)nullptr);
         HIP_TRY(hipGetLastError());

@@ -2,23 +2,27 @@
 #pragma once
 #include <hipcub/hipcub.hpp>
 
+#include <algorithm>
+#include <map>
 #include <memory>
 #include <mutex>
+#include <unordered_map>
 #include <vector>
 
 #include "common.h"
 
 namespace hdfe {
 
-// Device memory comes from the device's stream-ordered pool
-// (hipMallocAsync / hipFreeAsync).  The pool keeps up to kPoolKeep bytes of
-// freed memory for reuse instead of returning it at every synchronisation, so
-// the many large temporaries of a fit cost no driver allocation after the
-// first fit, and a free neither blocks the host nor synchronises the device.
-// A buffer is freed on the stream it was allocated on: reuse is ordered after
-// the work queued there.  Owners that hand their memory to other streams
-// synchronise the device before freeing it.
-constexpr uint64_t kPoolKeep = 16ull << 30;
+// Small device buffers come from the device's stream-ordered pool
+// (hipMallocAsync / hipFreeAsync), large ones from the library's own cache
+// (big_alloc below).  The pool keeps its freed memory for reuse (no release
+// threshold) instead of returning it at synchronisations, so the many
+// temporaries of a fit cost no driver allocation after the first fit, and a
+// free neither blocks the host nor synchronises the device.  A buffer is
+// freed on the stream it was allocated on: reuse is ordered after the work
+// queued there.  Owners that hand their memory to other streams synchronise
+// the device before freeing it.
+constexpr uint64_t kPoolKeep = ~0ull;
 
 inline hipError_t pool_prepare() {
     static std::mutex mu;
@@ -38,7 +42,110 @@ inline hipError_t pool_prepare() {
     return hipSuccess;
 }
 
-// An owning device allocation from the stream-ordered pool.
+// Large buffers (kBigBytes and up) are kept by the library once allocated:
+// a freed block waits in a free list with an event recorded on the stream it
+// was freed on, and a later allocation that it fits (at most twice its size,
+// same device) takes it after making its own stream wait for that event.  The
+// stream-ordered pool was measured spending milliseconds inside hipFreeAsync
+// for buffers of hundreds of megabytes; with the cache a repeated fit maps no
+// memory and frees none.  Blocks go back to the driver only when an
+// allocation fails for want of memory.  Small buffers use the pool.
+constexpr size_t kBigBytes = size_t(4) << 20;
+
+struct BigCache {
+    struct Block {
+        void *p = nullptr;
+        size_t bytes = 0;
+        hipEvent_t ev = nullptr;
+        int dev = 0;
+    };
+    std::mutex mu;
+    std::multimap<size_t, Block> free_list;
+    std::unordered_map<void *, Block> live;
+};
+
+inline BigCache &big_cache() {
+    static BigCache *c = new BigCache();  // lives to process exit (used by static owners)
+    return *c;
+}
+
+inline hipError_t big_release_free(BigCache &c) {
+    // caller holds c.mu
+    if (c.free_list.empty()) return hipSuccess;
+    hipError_t e = hipDeviceSynchronize();
+    for (auto &kv : c.free_list) {
+        (void)hipEventDestroy(kv.second.ev);
+        (void)hipFree(kv.second.p);
+    }
+    c.free_list.clear();
+    return e;
+}
+
+inline hipError_t big_alloc(void **p, size_t bytes, hipStream_t s) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    BigCache &c = big_cache();
+    std::lock_guard<std::mutex> lock(c.mu);
+    for (auto it = c.free_list.lower_bound(bytes);
+         it != c.free_list.end() && it->first <= 2 * bytes; ++it) {
+        if (it->second.dev != dev) continue;
+        BigCache::Block b = it->second;
+        c.free_list.erase(it);
+        if ((e = hipStreamWaitEvent(s, b.ev, 0)) != hipSuccess) return e;
+        c.live[b.p] = b;
+        *p = b.p;
+        return hipSuccess;
+    }
+    const size_t grain = size_t(2) << 20;
+    BigCache::Block b;
+    b.bytes = (bytes + grain - 1) / grain * grain;
+    b.dev = dev;
+    e = hipMalloc(&b.p, b.bytes);
+    if (e == hipErrorOutOfMemory) {
+        (void)hipGetLastError();
+        if ((e = big_release_free(c)) != hipSuccess) return e;
+        e = hipMalloc(&b.p, b.bytes);
+    }
+    if (e != hipSuccess) return e;
+    if ((e = hipEventCreateWithFlags(&b.ev, hipEventDisableTiming)) != hipSuccess) {
+        (void)hipFree(b.p);
+        return e;
+    }
+    c.live[b.p] = b;
+    *p = b.p;
+    return hipSuccess;
+}
+
+// Returns a cached block to the free list, usable after the work queued on
+// s; false when p is not one (a pool allocation).
+inline bool big_free(void *p, hipStream_t s) {
+    BigCache &c = big_cache();
+    std::lock_guard<std::mutex> lock(c.mu);
+    auto it = c.live.find(p);
+    if (it == c.live.end()) return false;
+    BigCache::Block b = it->second;
+    c.live.erase(it);
+    (void)hipEventRecord(b.ev, s);
+    c.free_list.emplace(b.bytes, b);
+    return true;
+}
+
+// Device memory of any size on stream s: the cache for large buffers, the
+// pool for the rest; dev_free takes either.
+inline hipError_t dev_malloc(void **p, size_t bytes, hipStream_t s) {
+    bytes = std::max<size_t>(bytes, 1);
+    if (bytes >= kBigBytes) return big_alloc(p, bytes, s);
+    hipError_t e = pool_prepare();
+    if (e != hipSuccess) return e;
+    return hipMallocAsync(p, bytes, s);
+}
+
+inline void dev_free(void *p, hipStream_t s) {
+    if (p && !big_free(p, s)) (void)hipFreeAsync(p, s);
+}
+
+// An owning device allocation (dev_malloc: the cache or the pool).
 template <typename T>
 struct DevBuf {
     T *p = nullptr;
@@ -64,16 +171,15 @@ struct DevBuf {
     }
     ~DevBuf() { reset(); }
     void reset() {
-        if (p) (void)hipFreeAsync(p, st);
+        dev_free(p, st);
         p = nullptr;
         n = 0;
     }
     hipError_t alloc(int64_t count, hipStream_t s) {
         reset();
-        hipError_t e = pool_prepare();
-        if (e != hipSuccess) return e;
         st = s;
-        e = hipMallocAsync((void **)&p, sizeof(T) * (size_t)std::max<int64_t>(count, 1), s);
+        const size_t b = sizeof(T) * (size_t)std::max<int64_t>(count, 1);
+        hipError_t e = dev_malloc((void **)&p, b, s);
         if (e != hipSuccess) {
             p = nullptr;
             return e;

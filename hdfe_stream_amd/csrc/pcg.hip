@@ -872,18 +872,14 @@ static int grid_for(int64_t n, int64_t per, int lo, int hi) {
     return (int)std::max<int64_t>(lo, std::min<int64_t>(hi, g));
 }
 
-// The matrix's device arrays and work buffers come from the stream-ordered
-// pool on the matrix's stream (a repeated fit reuses the same memory instead
-// of mapping new pages), and go back to it in stream order.
+// The matrix's device arrays and work buffers come from dev_malloc on the
+// matrix's stream (a repeated fit reuses the same memory instead of mapping
+// new pages), and go back in stream order.
 static hipError_t palloc(void *p, size_t bytes, hipStream_t s) {
-    hipError_t e = pool_prepare();
-    if (e != hipSuccess) return e;
-    return hipMallocAsync((void **)p, std::max<size_t>(bytes, 1), s);
+    return dev_malloc((void **)p, bytes, s);
 }
 
-static void pfree(void *p, hipStream_t s) {
-    if (p) (void)hipFreeAsync(p, s);
-}
+static void pfree(void *p, hipStream_t s) { dev_free(p, s); }
 
 // A pair of scratch allocations freed on scope exit.
 struct DevBuf2 {

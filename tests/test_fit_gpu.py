@@ -122,3 +122,28 @@ def test_many_sparse_rows_and_split_rows(oracle_mod):
     d = akm_panel(60_000, 9_000, seed=3)
     res, ref = _check_fit(oracle_mod, d)
     assert res.panel.matrix.nlong() > 0
+
+
+def test_rows_in_group_order_and_shuffled_build_the_same_panel():
+    """Rows stored by worker take the within-group ranking; shuffled rows the
+    radix sort.  Both must give the same sorted panel: the same cells, codes,
+    system and fit (only duplicate cells' sums may round apart)."""
+    from hdfe_stream_amd.estimator import StreamingHDFE
+    from hdfe_stream_amd.simulate import akm_panel
+    d = akm_panel(4000, 300, seed=12)
+    d["year"] = d["year"] % 3            # repeat (worker, firm, year) cells
+    perm = np.random.default_rng(1).permutation(len(d["y"]))
+    shuffled = {k: v[perm] for k, v in d.items()}
+    print("ordered vs shuffled rows", flush=True)
+    fits = []
+    for data in (d, shuffled):
+        est = StreamingHDFE("y", ["x1", "x2"], FE, tol=1e-12, precision="fp64")
+        fits.append(est.fit(data))
+    a, b = fits
+    for name in ("igstart", "icodes", "cstart"):
+        np.testing.assert_array_equal(a.panel.array(name, np.int32), b.panel.array(name, np.int32))
+    np.testing.assert_allclose(a.panel.array("diag", np.float64), b.panel.array("diag", np.float64),
+                               rtol=1e-13)
+    assert rel(a.beta, b.beta) < 1e-11
+    for f in a.fe_names:
+        np.testing.assert_allclose(a.fe_values[f], b.fe_values[f], rtol=0, atol=1e-9)
