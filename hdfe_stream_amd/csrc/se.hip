@@ -253,7 +253,77 @@ __global__ void k_se_smooth(int64_t n, const double *__restrict__ lev,
 
 using namespace hdfe;
 
+// A' diag(w) B for tall A (K, a) and B (K, b), rows strided by lda and ldb (a b small,
+// K in the millions: the shapes of x1'v, u'v and the Krylov projections, for
+// which a library GEMM without a split over K runs one tile down all K rows).
+// Each workgroup sums its row range for every pair (p, q) in a fixed order;
+// the finish adds the workgroups' partials in order.
+__global__ __launch_bounds__(256) void k_tall_dot(int64_t K, int a, int b,
+                                                  const double *__restrict__ A, int64_t lda,
+                                                  const double *__restrict__ B, int64_t ldb,
+                                                  const double *__restrict__ w,
+                                                  double *__restrict__ part) {
+    __shared__ double lds[256];
+    const int P = a * b;
+    // with few pairs, nrl row lanes per pair, added in lane order
+    const int nrl = P < (int)blockDim.x ? (int)blockDim.x / P : 1;
+    const int rl = (int)threadIdx.x / (P < (int)blockDim.x ? P : (int)blockDim.x);
+    const int64_t r0 = part_begin(K, blockIdx.x, gridDim.x), r1 = part_begin(K, blockIdx.x + 1, gridDim.x);
+    for (int t0 = 0; t0 < P; t0 += (nrl > 1 ? P : (int)blockDim.x)) {
+        const int t = t0 + (nrl > 1 ? (int)threadIdx.x % P : (int)threadIdx.x);
+        const bool act = rl < nrl && t < P;
+        double acc = 0.0;
+        if (act) {
+            const int p = t / b, q = t - p * b;
+            if (w)
+                for (int64_t i = r0 + rl; i < r1; i += nrl)
+                    acc += w[i] * A[i * lda + p] * B[i * ldb + q];
+            else
+                for (int64_t i = r0 + rl; i < r1; i += nrl) acc += A[i * lda + p] * B[i * ldb + q];
+        }
+        if (nrl > 1) {
+            lds[threadIdx.x] = act ? acc : 0.0;
+            __syncthreads();
+            if ((int)threadIdx.x < P) {
+                double v = 0.0;
+                for (int r = 0; r < nrl; ++r) v += lds[r * P + threadIdx.x];
+                part[(int64_t)blockIdx.x * P + threadIdx.x] = v;
+            }
+            __syncthreads();
+        } else if (act) {
+            part[(int64_t)blockIdx.x * P + t] = acc;
+        }
+    }
+}
+
+__global__ void k_tall_dot_finish(int nb, int P, const double *__restrict__ part,
+                                  double *__restrict__ out) {
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < P; t += gridDim.x * blockDim.x) {
+        double v = 0.0;
+        for (int w = 0; w < nb; ++w) v += part[(int64_t)w * P + t];
+        out[t] = v;
+    }
+}
+
 extern "C" {
+
+// out (a, b) device, row major = A' diag(w) B for A (K, a), B (K, b) device
+// with row strides lda, ldb (elements), w (K) or NULL; fixed-order sums.
+int hdfe_tall_dot(int64_t K, int a, int b, const double *A, int64_t lda, const double *B,
+                  int64_t ldb, const double *w, double *out) {
+    ARG_CHECK(K >= 0 && a >= 1 && b >= 1 && (int64_t)a * b <= (1 << 16) && A && B && out &&
+                  lda >= a && ldb >= b,
+              "hdfe_tall_dot: bad argument");
+    const int P = a * b;
+    const int nb = (int)std::max<int64_t>(1, std::min<int64_t>(kSeBlocks, (K + 4095) / 4096));
+    DevBuf<double> part;
+    DEV_ALLOC(part, (int64_t)nb * P, nullptr);
+    hipLaunchKernelGGL(k_tall_dot, dim3(nb), dim3(256), 0, 0, K, a, b, A, lda, B, ldb, w, part.p);
+    hipLaunchKernelGGL(k_tall_dot_finish, dim3(grid1(P)), dim3(256), 0, 0, nb, P, part.p, out);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipDeviceSynchronize());
+    return HDFE_STATUS_OK;
+}
 
 // The stacked first-solve input (see k_se_stack); out (n, (nin + 3) s); mask
 // the components with draws (bits 0-2).

@@ -39,6 +39,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import _lib
+from .leaveout_se import tall_dot
 
 RHS_ZERO_TOL = 1e-11      # core/solve.py:44
 TRACE_SEED_OFFSET = 1_000_003     # leaveout.py:730
@@ -125,8 +126,7 @@ class DesignBorder:
         self.Xd = (self.X - (self.Xg / Wg[:, None])[g]).contiguous()
         w = panel.tensor("w", torch.float64) if info["weighted"] else None
         T_dx = inv0.reduce(self.kc, R=self.Xd, levels_only=True)
-        wXd = self.Xd if w is None else w[:, None] * self.Xd
-        T_xx = (self.Xd.T @ wXd).cpu().numpy()
+        T_xx = tall_dot(self.Xd, self.Xd, w).cpu().numpy()
         wx2 = (self.X * self.X) if w is None else w[:, None] * self.X * self.X
         scale = np.sqrt(wx2.sum(0).cpu().numpy())
         torch.cuda.synchronize()
@@ -135,13 +135,13 @@ class DesignBorder:
                                               precision=precision)
         torch.cuda.synchronize()
         self.Z = Z
-        schur = T_xx - (T_dx.T @ Z).cpu().numpy()
+        schur = T_xx - tall_dot(T_dx, Z).cpu().numpy()
         self.schur_inv = torch.from_numpy(np.linalg.pinv(schur, hermitian=True)).cuda()
         self.info = {"iterations": it, "converged": bool(conv), "max_rel_residual": rel}
 
     def solve(self, rhs, rhs_x, U):
         """(U - Z C, C) for the reduced solution U = S^- rhs."""
-        C = self.schur_inv @ (rhs_x - self.Z.T @ rhs)
+        C = self.schur_inv @ (rhs_x - tall_dot(self.Z, rhs))
         return (U - self.Z @ C).contiguous(), C.contiguous()
 
 
@@ -344,9 +344,9 @@ class CoefficientInverse:
                     else self.panel.tensor("w", torch.float64)
                 if x is not None:
                     wx = wx * x
-                out[self.L:] = bd.X.T @ (wx[:, None] * R)
+                out[self.L:] = tall_dot(bd.X, R, wx)
             else:
-                out[self.L:] = bd.Xg.T @ (V / self.Wg_safe[:, None])
+                out[self.L:] = tall_dot(bd.Xg, V, 1.0 / self.Wg_safe)
         return out
 
     def group_sums(self, R, x=None):

@@ -114,6 +114,30 @@ class SELayout:
         return self.inv.reduce(R.shape[1], R=R, x=x), self.inv.group_sums(R, x=x)
 
 
+def tall_dot(A, B, w=None):
+    """A' diag(w) B on the device for tall A (K, a) and B (K, b) (vectors count
+    as one column), FP64, fixed-order sums (hdfe_tall_dot)."""
+    import torch
+    A2 = A[:, None] if A.dim() == 1 else A
+    B2 = B[:, None] if B.dim() == 1 else B
+    A2 = A2.to(torch.float64)
+    B2 = B2.to(torch.float64)
+    if A2.stride(1) != 1 or A2.stride(0) < A2.shape[1]:
+        A2 = A2.contiguous()
+    if B2.stride(1) != 1 or B2.stride(0) < B2.shape[1]:
+        B2 = B2.contiguous()
+    K, a, b = A2.shape[0], A2.shape[1], B2.shape[1]
+    out = torch.empty((a, b), dtype=torch.float64, device="cuda")
+    if K == 0:
+        return out.zero_()
+    wd = None if w is None else w.to(torch.float64).contiguous()
+    torch.cuda.synchronize()
+    _lib.check(_lib.load().hdfe_tall_dot(K, a, b, A2.data_ptr(), A2.stride(0), B2.data_ptr(),
+                                         B2.stride(0), None if wd is None else wd.data_ptr(),
+                                         out.data_ptr()), "hdfe_tall_dot")
+    return out
+
+
 def bii_rows(lay: SELayout, n_draws: int, block: int, seed: int):
     """B_ii of the three components, (3, n) device, by random projection
     (leaveout_se.py:228-327): per block of draws q, the two solves of the
@@ -157,8 +181,8 @@ def apply_q(lay: SELayout, k: int, Ul, Ug):
     ol = torch.zeros((lay.K, s), dtype=torch.float64, device="cuda")
     og = torch.zeros((lay.G, s), dtype=torch.float64, device="cuda")
     lw = lay.level_w[lay.psi]
-    psi_mean = (lw[None, :] @ Ul[lay.psi])[0] / lay.total
-    alpha_mean = (lay.group_w[None, :] @ Ug)[0] / lay.total
+    psi_mean = tall_dot(lw, Ul[lay.psi])[0] / lay.total
+    alpha_mean = tall_dot(lay.group_w, Ug)[0] / lay.total
     if k == 0:
         ol[lay.psi] = lw[:, None] * (Ul[lay.psi] - psi_mean) / lay.norm
     elif k == 1:
@@ -209,12 +233,12 @@ def apply_c(lay: SELayout, b, v=None, z=None, s3=None, rank_two=None, yt=None, d
     udot = odot = None
     if rank_two is not None:
         u = rank_two[0]
-        udot = torch.stack([u[k][None, :] @ vin[k] for k in range(3)]).reshape(3, s).contiguous()
+        udot = torch.stack([tall_dot(u[k], vin[k]) for k in range(3)]).reshape(3, s).contiguous()
         odot = torch.stack([vin[k].sum(0) for k in range(3)]).contiguous()
     x1 = defl = None
     if deflate is not None:
         x1 = deflate[1]
-        along = torch.stack([x1[k][None, :] @ vin[k] for k in range(3)]).reshape(3 * s)
+        along = torch.stack([tall_dot(x1[k], vin[k]) for k in range(3)]).reshape(3 * s)
         lam = torch.as_tensor(np.asarray(deflate[0], np.float64)).cuda()
         defl = torch.cat([lam, along]).contiguous()
     lv, gs = lay.reduce(stack)
@@ -349,8 +373,8 @@ def weak_moments(lay: SELayout, y, sigma2, s3, leverages, b_ii, lams, Ul, Ug, co
     y = y.to(torch.float64).contiguous()
     ybar = float((w * y).sum() / w.sum())
     yt_c = lay.root * (y - ybar)
-    b1 = (x1 @ yt_c).cpu().numpy()
-    v_raw = ((x1 * x1) @ sigma2.to(torch.float64)).cpu().numpy()
+    b1 = tall_dot(yt_c, x1.T)[0].cpu().numpy()
+    v_raw = tall_dot(sigma2, (x1 * x1).T)[0].cpu().numpy()
     sf = s3.clamp_min(0.0)
     s11 = (x1 * x1 * sf).sum(1).cpu().numpy()
     lam_d = torch.as_tensor(np.asarray(lams, np.float64)).cuda()

@@ -28,7 +28,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import _lib
-from .leaveout_se import COMPONENTS, SELayout, apply_q, eigen_rows, smoothed_sigma2
+from .leaveout_se import COMPONENTS, SELayout, apply_q, eigen_rows, smoothed_sigma2, tall_dot
 
 WEAK_ID_THRESHOLD = 0.1            # leaveout_weakid.py:67
 SEED_OFFSET = 9_900_019            # leaveout.py:758
@@ -219,7 +219,7 @@ def _lanczos(lay, n_iter, seed, ritz=None, check=None, every=20, caps=None, reor
             Xs, Ys = torch.stack(basis[c][0], 1), torch.stack(basis[c][1], 1)
             for _ in range(2):
                 wc = support(c, w)
-                scatter(c, Xs @ (Ys.T @ wc), w)
+                scatter(c, Xs @ tall_dot(Ys, wc)[:, 0], w)
         project(w)
         z = _solve(lay, w)
         beta = np.sqrt(np.maximum(_dot(w, z), 0.0))
@@ -326,7 +326,7 @@ def _sum_sq_eigenvalues(lay, n_draws, block, seed, X, Y, k, settled=None, min_dr
         yd = [torch.empty((b.shape[0], 3 * m), dtype=torch.float64, device="cuda") for b in y]
         for c in range(3):
             cols = slice(c * k, (c + 1) * k)
-            proj = sum(zb.T @ Yb[:, cols] for zb, Yb in zip(zeta, Y))
+            proj = sum(tall_dot(zb, Yb[:, cols]) for zb, Yb in zip(zeta, Y))
             out = slice(c * m, (c + 1) * m)
             for d, yb, Yb in zip(yd, y, Y):
                 d[:, out] = yb - Yb[:, cols] @ proj.T

@@ -337,6 +337,14 @@ void hdfe_components_destroy(void *h);
  * Row passes of the components' standard errors, in the square-root-weight
  * metric; blocks of s vectors are row-major (rows, s); b and s3 are (3, n). */
 
+/* out (a, b) device = A' diag(w) B for tall A (K, a) and B (K, b), device,
+ * row strides lda and ldb, w (K) or NULL; fixed-order sums.  No reference counterpart: it replaces the
+ * numpy products x1'v, u'v and the Krylov projections of leaveout_se.py and
+ * leaveout_weakid.py (e.g. leaveout_se.py:536-540, leaveout_weakid.py:505-512)
+ * on the device. */
+int hdfe_tall_dot(int64_t K, int a, int b, const double *A, int64_t lda, const double *B,
+                  int64_t ldb, const double *w, double *out);
+
 /* The first solve's stacked input of applying C: [v, b_k v] for a shared v
  * (n, s), or v_k = z sqrt(max(s_k, 0)) and [v_k, b_k v_k] per component from
  * draws z (n, s); out (n, (1 or 3) s + 3 s).  Replaces the stacking of

@@ -338,3 +338,24 @@ def test_match_level_weak_interval_is_var_psi_alone():
         assert lo_ < hi_
     else:
         assert not lo.se.weak
+
+
+def test_tall_dot_matches_a_plain_product():
+    import torch
+    from hdfe_stream_amd.leaveout_se import tall_dot
+    g = torch.Generator("cuda").manual_seed(3)
+    for K, a, b in ((1, 1, 1), (1000, 3, 7), (200_003, 1, 40), (50_000, 20, 30), (7, 300, 2)):
+        A = torch.randn((K, a), dtype=torch.float64, device="cuda", generator=g)
+        B = torch.randn((K, b + 3), dtype=torch.float64, device="cuda", generator=g)[:, 2:2 + b]
+        w = torch.rand(K, dtype=torch.float64, device="cuda", generator=g)
+        print(f"tall_dot K={K} a={a} b={b}", flush=True)
+        want = (A.cpu().T @ B.cpu()).numpy()
+        got = tall_dot(A, B).cpu().numpy()
+        np.testing.assert_allclose(got, want, rtol=1e-11, atol=1e-11 * np.abs(want).max())
+        want_w = (A.cpu().T @ (w.cpu()[:, None] * B.cpu())).numpy()
+        got_w = tall_dot(A, B, w).cpu().numpy()
+        np.testing.assert_allclose(got_w, want_w, rtol=1e-11, atol=1e-11 * np.abs(want_w).max())
+        # fixed-order sums: the same bits twice
+        assert np.array_equal(tall_dot(A, B).cpu().numpy(), got)
+    v = torch.randn(1000, dtype=torch.float64, device="cuda", generator=g)
+    assert tall_dot(v, v).item() == pytest.approx(float((v * v).sum()), rel=1e-12)
