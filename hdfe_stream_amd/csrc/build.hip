@@ -780,19 +780,27 @@ __global__ void k_diag_rhs_part_c(int64_t nchunk, int m, const int *__restrict__
     }
 }
 
+// One wave per level adds its chunks' partials (lanes strided, a fixed
+// butterfly): a level of millions of incidences (a year) has thousands of
+// chunks, which one thread would add in a long dependent chain.
 __global__ void k_diag_rhs_sum(int64_t L, int m, const int *__restrict__ cptr,
                                const double *__restrict__ part, double *__restrict__ diag,
                                double *__restrict__ rhs) {
-    for (int64_t a = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; a < L;
-         a += (int64_t)gridDim.x * blockDim.x) {
+    const int lane = threadIdx.x & 63;
+    const int64_t waves = (int64_t)gridDim.x * (blockDim.x / 64);
+    for (int64_t a = blockIdx.x * (int64_t)(blockDim.x / 64) + threadIdx.x / 64; a < L;
+         a += waves) {
+        const int c0 = cptr[a], c1 = cptr[a + 1];
         double d = 0.0;
-        for (int c = cptr[a]; c < cptr[a + 1]; ++c) d += part[(int64_t)c * (m + 1)];
-        diag[a] = d;
+        for (int c = c0 + lane; c < c1; c += 64) d += part[(int64_t)c * (m + 1)];
+        d = wave_sum(d);
+        if (lane == 0) diag[a] = d;
         if (rhs)
             for (int j = 0; j < m; ++j) {
                 double r = 0.0;
-                for (int c = cptr[a]; c < cptr[a + 1]; ++c) r += part[(int64_t)c * (m + 1) + 1 + j];
-                rhs[a * m + j] = r;
+                for (int c = c0 + lane; c < c1; c += 64) r += part[(int64_t)c * (m + 1) + 1 + j];
+                r = wave_sum(r);
+                if (lane == 0) rhs[a * m + j] = r;
             }
     }
 }
@@ -2265,7 +2273,7 @@ int hdfe_panel_system(void *h, int with_rhs, void **matrix, double *seconds) {
                                s, (int64_t)nchunk, m, clvl.p, cbeg.p, P.lptr.p, P.lq.p, contrib.p,
                                part.p);
         }
-        hipLaunchKernelGGL(k_diag_rhs_sum, dim3(grid1(L)), dim3(T256), 0, s, L, m, cptr.p, part.p,
+        hipLaunchKernelGGL(k_diag_rhs_sum, dim3(grid1(L * 64)), dim3(T256), 0, s, L, m, cptr.p, part.p,
                            P.diag.p, with_rhs ? P.rhs.p : (double *)nullptr);
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipStreamSynchronize(s));
