@@ -565,18 +565,146 @@ __global__ void k_ident_cells(int64_t G, int D, int m, const int *__restrict__ g
     }
 }
 
-// Incidence keys: (group, level) for every (identifying cell, dimension).
-__global__ void k_inc_keys(int64_t M, int D, const int *__restrict__ cgroup,
-                           const int *__restrict__ icodes, int64_t ild,
-                           const int64_t *__restrict__ offs, int lbits,
-                           uint64_t *__restrict__ key, int *__restrict__ val) {
-    const int64_t n = M * D;
+// The sparse dimensions (those whose levels are rows of S's sparse part).
+struct DimList {
+    int d[8];
+    int n;
+};
+
+// Incidence keys of the sparse dimensions: (group, level) for every
+// (identifying cell, sparse dimension j), e = c * n + j.  The dense
+// dimensions' (a handful of levels each: years) incidences are never formed:
+// their diagonal, right-hand side and weights come from the cells directly.
+__global__ void k_inc_keys_s(int64_t M, DimList dl, const int *__restrict__ cgroup,
+                             const int *__restrict__ icodes, int64_t ild,
+                             const int64_t *__restrict__ offs, int lbits,
+                             uint64_t *__restrict__ key, int *__restrict__ val) {
+    const int64_t n = M * dl.n;
     for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n;
          e += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t c = e / D;
-        const int d = (int)(e - c * D);
+        const int64_t c = e / dl.n;
+        const int d = dl.d[(int)(e - c * dl.n)];
         key[e] = ((uint64_t)cgroup[c] << lbits) | (uint64_t)(offs[d] + icodes[d * ild + c]);
         val[e] = (int)c;
+    }
+}
+
+// U[g, t] = the weight of group g's cells at dense level t: W lanes per
+// group, per level a lane sum then a fixed butterfly.
+template <int W>
+__global__ void k_dense_u_cells(int64_t Gi, int Ld, const int *__restrict__ igstart,
+                                const int *__restrict__ icodes, int64_t ild,
+                                const int64_t *__restrict__ offs, DimList dense,
+                                const int *__restrict__ dense_idx, const double *__restrict__ in,
+                                double *__restrict__ U) {
+    const int sub = threadIdx.x % W;
+    for (int64_t g = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / W; g < Gi;
+         g += (int64_t)gridDim.x * blockDim.x / W) {
+        const int c0 = igstart[g], c1 = igstart[g + 1];
+        for (int t = 0; t < Ld; ++t) {
+            double u = 0.0;
+            for (int c = c0 + sub; c < c1; c += W)
+                for (int q = 0; q < dense.n; ++q) {
+                    const int d = dense.d[q];
+                    if (dense_idx[offs[d] + icodes[d * ild + c]] == t) u += in[c];
+                }
+            u = run_sum<W>(u);
+            if (sub == 0) U[g * Ld + t] = u;
+        }
+    }
+}
+
+// The dense levels' cell sums: per workgroup over a contiguous range of
+// identifying cells, for each level t, the sum of n_c (col 0) and of the
+// cell sums s_cj (col j + 1) over the cells at level t; a block sum (fixed
+// tree) per (t, col).  part[b, t, col].
+__global__ __launch_bounds__(256) void k_dense_cell_sums(
+    int64_t M, int Ld, int m, const int *__restrict__ icodes, int64_t ild,
+    const int64_t *__restrict__ offs, DimList dense, const int *__restrict__ dense_idx,
+    const double *__restrict__ in, const double *__restrict__ isums, double *__restrict__ part) {
+    __shared__ double lds[T256 / 64];
+    const int64_t c0 = part_begin(M, blockIdx.x, gridDim.x), c1 = part_begin(M, blockIdx.x + 1, gridDim.x);
+    const int np = m + 1;
+    for (int t = 0; t < Ld; ++t)
+        for (int j0 = 0; j0 < np; j0 += 8) {
+            const int nj = min(8, np - j0);
+            double acc[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+            for (int64_t c = c0 + threadIdx.x; c < c1; c += blockDim.x)
+                for (int q = 0; q < dense.n; ++q) {
+                    const int d = dense.d[q];
+                    if (dense_idx[offs[d] + icodes[d * ild + c]] != t) continue;
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) {
+                        if (j >= nj) break;
+                        const int col = j0 + j;
+                        acc[j] += col == 0 ? in[c] : isums[c * m + col - 1];
+                    }
+                }
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                if (j >= nj) break;
+                const double v = block_sum<double>(acc[j], lds);
+                if (threadIdx.x == 0) part[((int64_t)blockIdx.x * Ld + t) * np + j0 + j] = v;
+            }
+        }
+}
+
+// diag and rhs of the dense levels (kernels_base.py:64-110 restricted to
+// them): diag_t = sum_c n_c - sum_g U[g,t]^2 / N_g and rhs_tj = sum_c s_cj -
+// sum_g U[g,t] vbar_gj (the cells' part from k_dense_cell_sums).  Here the
+// groups' part: per pair (t, col) over a range of groups in nsub slices
+// added in order (fixed), like k_dd.  part[b, t, col].
+__global__ __launch_bounds__(256) void k_dense_diag(int64_t Gi, int Ld, int m,
+                                                    const double *__restrict__ U,
+                                                    const double *__restrict__ Ng,
+                                                    const double *__restrict__ vbar,
+                                                    double *__restrict__ part) {
+    __shared__ double sv[T256];
+    const int npair = Ld * (m + 1);
+    const int nsub = npair * 2 <= (int)blockDim.x ? (int)blockDim.x / npair : 1;
+    const int64_t g0 = part_begin(Gi, blockIdx.x, gridDim.x), g1 = part_begin(Gi, blockIdx.x + 1, gridDim.x);
+    const int stride = nsub > 1 ? npair : (int)blockDim.x;
+    const int sub = nsub > 1 ? (int)threadIdx.x / npair : 0;
+    for (int pr0 = 0; pr0 < npair; pr0 += stride) {
+        const int pr = pr0 + (nsub > 1 ? (int)threadIdx.x % npair : (int)threadIdx.x);
+        const bool act = sub < nsub && pr < npair;
+        double v = 0.0;
+        if (act) {
+            const int t = pr / (m + 1), col = pr - t * (m + 1);
+            for (int64_t g = g0 + sub; g < g1; g += nsub) {
+                const double u = U[g * Ld + t];
+                v += col == 0 ? u * u / Ng[g] : u * vbar[g * m + col - 1];
+            }
+        }
+        if (nsub > 1) {
+            sv[threadIdx.x] = act ? v : 0.0;
+            __syncthreads();
+            if ((int)threadIdx.x < npair) {
+                double tv = 0.0;
+                for (int q = 0; q < nsub; ++q) tv += sv[q * npair + threadIdx.x];
+                part[(int64_t)blockIdx.x * npair + threadIdx.x] = tv;
+            }
+            __syncthreads();
+        } else if (act) {
+            part[(int64_t)blockIdx.x * npair + pr] = v;
+        }
+    }
+}
+
+__global__ void k_dense_diag_finish(int nb, int nbc, int Ld, int m, const double *__restrict__ part,
+                                    const double *__restrict__ cpart,
+                                    const int *__restrict__ dense_levels, double *__restrict__ diag,
+                                    double *__restrict__ rhs) {
+    const int npair = Ld * (m + 1);
+    for (int pr = blockIdx.x * blockDim.x + threadIdx.x; pr < npair; pr += gridDim.x * blockDim.x) {
+        double cs = 0.0, gs = 0.0;
+        for (int b = 0; b < nbc; ++b) cs += cpart[(int64_t)b * npair + pr];
+        for (int b = 0; b < nb; ++b) gs += part[(int64_t)b * npair + pr];
+        const double v = cs - gs;
+        const int t = pr / (m + 1), col = pr - t * (m + 1);
+        const int a = dense_levels[t];
+        if (col == 0) diag[a] = v;
+        else if (rhs) rhs[(int64_t)a * m + col - 1] = v;
     }
 }
 
@@ -894,15 +1022,6 @@ __global__ void k_reduce_runs(int64_t n, const uint64_t *__restrict__ key,
 }
 
 // U[g, t] = u of dense level t in group g (0 when absent).
-__global__ void k_dense_u(int64_t E, const int *__restrict__ ig, const int *__restrict__ il,
-                          const double *__restrict__ iu, const int *__restrict__ dense_idx,
-                          int Ld, double *__restrict__ U) {
-    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < E;
-         q += (int64_t)gridDim.x * blockDim.x) {
-        const int t = dense_idx[il[q]];
-        if (t >= 0) U[(int64_t)ig[q] * Ld + t] = iu[q];
-    }
-}
 
 // Sparse x dense block, first per incidence: for incidence q = (group g,
 // sparse level a) and dense level t, C[q, t] = n_g(a, t) - u_a U[g, t] / N_g
@@ -2180,11 +2299,65 @@ int hdfe_panel_system(void *h, int with_rhs, void **matrix, double *seconds) {
     const int D = P.D, m = P.m;
     const int64_t L = P.L, M = P.M, Gi = P.Gi;
     ARG_CHECK(D >= 1 && L >= 1, "hdfe_panel_system: no dimension besides the streamed one");
+    // dense / sparse dimensions and their levels (ascending global index)
+    std::vector<uint8_t> sparse_dim(D);
+    {
+        // smallest dimensions first while the dense levels fit kDenseMax
+        std::vector<int> order(D);
+        std::iota(order.begin(), order.end(), 0);
+        std::stable_sort(order.begin(), order.end(),
+                         [&](int a, int b) { return P.nlev[a] < P.nlev[b]; });
+        int64_t ld = 0;
+        for (int t = 0; t < D; ++t) sparse_dim[t] = 1;
+        for (int t : order) {
+            if (P.nlev[t] <= kDenseMax && ld + P.nlev[t] <= kDenseMax && D > 1) {
+                sparse_dim[t] = 0;
+                ld += P.nlev[t];
+            }
+        }
+        // keep at least one sparse dimension
+        bool any_sparse = false;
+        for (int t = 0; t < D; ++t) any_sparse |= (bool)sparse_dim[t];
+        if (!any_sparse) sparse_dim[order.back()] = 1;
+    }
+    LevelDims ldims{};
+    ldims.D = D;
+    int64_t Ls = 0, Ld64 = 0;
+    for (int t = 0; t < D; ++t) {
+        ldims.off[t] = P.offs_h[t];
+        ldims.sparse[t] = sparse_dim[t];
+        ldims.base[t] = sparse_dim[t] ? Ls : Ld64;
+        (sparse_dim[t] ? Ls : Ld64) += P.nlev[t];
+    }
+    const int Ld = (int)Ld64;
+    int ncross = 0;
+    for (int d1 = 0; d1 < D; ++d1)
+        for (int d2 = 0; d2 < D; ++d2) ncross += (d1 != d2 && sparse_dim[d1] && sparse_dim[d2]);
+    DevBuf<uint8_t> d_sparse_lv, d_sparse_dim;
+    DevBuf<int> d_sparse_idx, d_dense_idx, d_lvl_dim, d_sparse_levels, d_dense_levels;
+    DEV_ALLOC(d_sparse_lv, L, s);
+    DEV_ALLOC(d_sparse_dim, D, s);
+    DEV_ALLOC(d_sparse_idx, L, s);
+    DEV_ALLOC(d_dense_idx, L, s);
+    DEV_ALLOC(d_lvl_dim, L, s);
+    DEV_ALLOC(d_sparse_levels, std::max<int64_t>(Ls, 1), s);
+    DEV_ALLOC(d_dense_levels, std::max(Ld, 1), s);
+    HIP_TRY(hipMemcpyAsync(d_sparse_dim.p, sparse_dim.data(), D, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_level_tables, dim3(grid1(L)), dim3(T256), 0, s, (int64_t)L, ldims,
+                       d_sparse_lv.p, d_sparse_idx.p, d_dense_idx.p, d_lvl_dim.p,
+                       d_sparse_levels.p, d_dense_levels.p);
+
     // incidences (group, level) of the identifying cells
     const int lbits = std::max(1, bits_for((uint64_t)L));
     const int gbits = std::max(1, bits_for((uint64_t)std::max<int64_t>(Gi, 1)));
     ARG_CHECK(lbits + gbits <= 64, "hdfe_panel_system: incidence keys need more than 64 bits");
-    const int64_t nE = M * D;
+    DimList sdl{}, ddl{};
+    for (int t = 0; t < D; ++t) {
+        if (sparse_dim[t]) sdl.d[sdl.n++] = t;
+        else ddl.d[ddl.n++] = t;
+    }
+    const int Ds = sdl.n;
+    const int64_t nE = M * Ds;
     DEV_ALLOC(P.cgroup, std::max<int64_t>(M, 1), s);
     const int cw = run_width(M, Gi);
 #define HDFE_L(W)                                                                       \
@@ -2202,14 +2375,14 @@ int hdfe_panel_system(void *h, int with_rhs, void **matrix, double *seconds) {
     DEV_ALLOC(rank, std::max<int64_t>(nE, 1), s);
     int E = 0;
     if (nE) {
-        hipLaunchKernelGGL(k_inc_keys, dim3(grid1(nE)), dim3(T256), 0, s, M, D, P.cgroup.p,
+        hipLaunchKernelGGL(k_inc_keys_s, dim3(grid1(nE)), dim3(T256), 0, s, M, sdl, P.cgroup.p,
                            P.icodes.p, M, P.offs.p, lbits, k1.p, v1.p);
         // within-group ranks when every group's segment fits a few lanes,
         // else the radix sort (the same order either way)
         int W = 4;
-        while (W < 64 && W < D * cw) W *= 2;
+        while (W < 64 && W < Ds * cw) W *= 2;
         bool sorted = false;
-        TRY(seg_rank_sort(s, Gi, D, P.igstart.p, k1.p, v1.p, k2.p, v2.p, W, sorted));
+        TRY(seg_rank_sort(s, Gi, Ds, P.igstart.p, k1.p, v1.p, k2.p, v2.p, W, sorted));
         if (!sorted) HIP_TRY(sort_pairs(P.tmp, k1.p, k2.p, v1.p, v2.p, nE, 0, lbits + gbits, s));
         hipLaunchKernelGGL(k_heads_u64, dim3(grid1(nE)), dim3(T256), 0, s, nE, k2.p, head.p);
         HIP_TRY(inclusive_sum(P.tmp, head.p, rank.p, nE, s));
@@ -2276,57 +2449,32 @@ int hdfe_panel_system(void *h, int with_rhs, void **matrix, double *seconds) {
         hipLaunchKernelGGL(k_diag_rhs_sum, dim3(grid1(L * 64)), dim3(T256), 0, s, L, m, cptr.p, part.p,
                            P.diag.p, with_rhs ? P.rhs.p : (double *)nullptr);
         HIP_TRY(hipGetLastError());
-        HIP_TRY(hipStreamSynchronize(s));
     }
+    // the dense levels (no incidences): weights per group, then their sums
+    DevBuf<double> U;
+    if (Ld) {
+        DEV_ALLOC(U, std::max<int64_t>(Gi * Ld, 1), s);
+        HIP_TRY(hipMemsetAsync(U.p, 0, std::max<int64_t>(Gi * Ld, 1) * 8, s));
+#define HDFE_L(W)                                                                          \
+    hipLaunchKernelGGL(k_dense_u_cells<W>, dim3(grid1(Gi * W)), dim3(T256), 0, s, Gi, Ld,  \
+                       P.igstart.p, P.icodes.p, M, P.offs.p, ddl, d_dense_idx.p, P.in.p, U.p)
+        HDFE_BY_WIDTH(cw, HDFE_L)
+#undef HDFE_L
+        const int nbd = 1024, nbc = 1024, npd = Ld * (m + 1);
+        DevBuf<double> dpart, cpart;
+        DEV_ALLOC(dpart, (int64_t)nbd * npd, s);
+        DEV_ALLOC(cpart, (int64_t)nbc * npd, s);
+        hipLaunchKernelGGL(k_dense_cell_sums, dim3(nbc), dim3(T256), 0, s, M, Ld, m, P.icodes.p,
+                           M, P.offs.p, ddl, d_dense_idx.p, P.in.p, P.isums.p, cpart.p);
+        hipLaunchKernelGGL(k_dense_diag, dim3(nbd), dim3(T256), 0, s, Gi, Ld, m, U.p, P.Ng.p,
+                           P.vbar.p, dpart.p);
+        hipLaunchKernelGGL(k_dense_diag_finish, dim3(grid1(npd)), dim3(T256), 0, s, nbd, nbc, Ld,
+                           m, dpart.p, cpart.p, d_dense_levels.p, P.diag.p,
+                           with_rhs ? P.rhs.p : (double *)nullptr);
+        HIP_TRY(hipGetLastError());
+    }
+    HIP_TRY(hipStreamSynchronize(s));
     HIP_TRY(hipGetLastError());
-
-    // dense / sparse dimensions and their levels (ascending global index)
-    std::vector<uint8_t> sparse_dim(D);
-    {
-        // smallest dimensions first while the dense levels fit kDenseMax
-        std::vector<int> order(D);
-        std::iota(order.begin(), order.end(), 0);
-        std::stable_sort(order.begin(), order.end(),
-                         [&](int a, int b) { return P.nlev[a] < P.nlev[b]; });
-        int64_t ld = 0;
-        for (int t = 0; t < D; ++t) sparse_dim[t] = 1;
-        for (int t : order) {
-            if (P.nlev[t] <= kDenseMax && ld + P.nlev[t] <= kDenseMax && D > 1) {
-                sparse_dim[t] = 0;
-                ld += P.nlev[t];
-            }
-        }
-        // keep at least one sparse dimension
-        bool any_sparse = false;
-        for (int t = 0; t < D; ++t) any_sparse |= (bool)sparse_dim[t];
-        if (!any_sparse) sparse_dim[order.back()] = 1;
-    }
-    LevelDims ldims{};
-    ldims.D = D;
-    int64_t Ls = 0, Ld64 = 0;
-    for (int t = 0; t < D; ++t) {
-        ldims.off[t] = P.offs_h[t];
-        ldims.sparse[t] = sparse_dim[t];
-        ldims.base[t] = sparse_dim[t] ? Ls : Ld64;
-        (sparse_dim[t] ? Ls : Ld64) += P.nlev[t];
-    }
-    const int Ld = (int)Ld64;
-    int ncross = 0;
-    for (int d1 = 0; d1 < D; ++d1)
-        for (int d2 = 0; d2 < D; ++d2) ncross += (d1 != d2 && sparse_dim[d1] && sparse_dim[d2]);
-    DevBuf<uint8_t> d_sparse_lv, d_sparse_dim;
-    DevBuf<int> d_sparse_idx, d_dense_idx, d_lvl_dim, d_sparse_levels, d_dense_levels;
-    DEV_ALLOC(d_sparse_lv, L, s);
-    DEV_ALLOC(d_sparse_dim, D, s);
-    DEV_ALLOC(d_sparse_idx, L, s);
-    DEV_ALLOC(d_dense_idx, L, s);
-    DEV_ALLOC(d_lvl_dim, L, s);
-    DEV_ALLOC(d_sparse_levels, std::max<int64_t>(Ls, 1), s);
-    DEV_ALLOC(d_dense_levels, std::max(Ld, 1), s);
-    HIP_TRY(hipMemcpyAsync(d_sparse_dim.p, sparse_dim.data(), D, hipMemcpyHostToDevice, s));
-    hipLaunchKernelGGL(k_level_tables, dim3(grid1(L)), dim3(T256), 0, s, (int64_t)L, ldims,
-                       d_sparse_lv.p, d_sparse_idx.p, d_dense_idx.p, d_lvl_dim.p,
-                       d_sparse_levels.p, d_dense_levels.p);
 
     // sparse x sparse: emit, sort, reduce
     DevBuf<int64_t> cnt, pos;
@@ -2385,7 +2533,7 @@ int hdfe_panel_system(void *h, int with_rhs, void **matrix, double *seconds) {
     P.n_sparse_ss = nss;
 
     // blocks with a dense dimension
-    DevBuf<double> U, PD, DD;
+    DevBuf<double> PD, DD;
     DevBuf<uint8_t> PP, DP;
     DevBuf<int> colpres, cpos, colbuf;
     DEV_ALLOC(PD, std::max<int64_t>(Ls * Ld, 1), s);
@@ -2395,10 +2543,6 @@ int hdfe_panel_system(void *h, int with_rhs, void **matrix, double *seconds) {
     DEV_ALLOC(colpres, std::max(Ld, 1), s);
     DEV_ALLOC(cpos, std::max<int64_t>((int64_t)Ld * (Ls + 1), 1), s);
     if (Ld) {
-        DEV_ALLOC(U, std::max<int64_t>(Gi * Ld, 1), s);
-        HIP_TRY(hipMemsetAsync(U.p, 0, std::max<int64_t>(Gi * Ld, 1) * 8, s));
-        hipLaunchKernelGGL(k_dense_u, dim3(grid1(E)), dim3(T256), 0, s, (int64_t)E, P.ig.p, P.il.p,
-                           P.iu.p, d_dense_idx.p, Ld, U.p);
         {
             DevBuf<int> sflag, sidx;
             DEV_ALLOC(sflag, (int64_t)E + 1, s);
