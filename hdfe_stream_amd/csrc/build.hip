@@ -691,6 +691,82 @@ __global__ __launch_bounds__(256) void k_dense_diag(int64_t Gi, int Ld, int m,
     }
 }
 
+// The dense levels in one pass over the groups (m <= 8 variables, LDS slots
+// for every sub-group's (level, column) pairs within 64 KB): W lanes per
+// group find, per dense level t, the
+// weight u and the cell sums s_j of the group's cells at t (lane sums, a
+// fixed butterfly); the group's lane 0 writes U[g, t] and adds u - u^2 / N_g
+// and s_j - u vbar_gj (kernels_base.py:64-110) to its own LDS slots; the
+// slots are added in sub-group order into part[b, t, col] (fixed order).
+template <int W>
+__global__ __launch_bounds__(256) void k_dense_sums(
+    int64_t Gi, int Ld, int m, const int *__restrict__ igstart, const int *__restrict__ icodes,
+    int64_t ild, const int64_t *__restrict__ offs, DimList dense,
+    const int *__restrict__ dense_idx, const double *__restrict__ in,
+    const double *__restrict__ isums, const double *__restrict__ Ng,
+    const double *__restrict__ vbar, double *__restrict__ U, double *__restrict__ part) {
+    extern __shared__ double acc[];
+    const int np = m + 1, npair = Ld * np;
+    const int sub = threadIdx.x % W, sg = threadIdx.x / W, nsg = blockDim.x / W;
+    for (int e = threadIdx.x; e < nsg * npair; e += blockDim.x) acc[e] = 0.0;
+    __syncthreads();
+    const int64_t g0 = part_begin(Gi, blockIdx.x, gridDim.x), g1 = part_begin(Gi, blockIdx.x + 1, gridDim.x);
+    for (int64_t g = g0 + sg; g < g1; g += nsg) {
+        const int c0 = igstart[g], c1 = igstart[g + 1];
+        for (int t = 0; t < Ld; ++t) {
+            double u = 0.0, sj[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+            for (int c = c0 + sub; c < c1; c += W)
+                for (int q = 0; q < dense.n; ++q) {
+                    const int d = dense.d[q];
+                    if (dense_idx[offs[d] + icodes[d * ild + c]] != t) continue;
+                    u += in[c];
+#pragma unroll
+                    for (int j = 0; j < 8; ++j)
+                        if (j < m) sj[j] += isums[(int64_t)c * m + j];
+                }
+            u = run_sum<W>(u);
+#pragma unroll
+            for (int j = 0; j < 8; ++j)
+                if (j < m) sj[j] = run_sum<W>(sj[j]);
+            if (sub == 0) {
+                U[g * Ld + t] = u;
+                double *a = acc + (sg * Ld + t) * np;
+                a[0] += u - u * u / Ng[g];
+#pragma unroll
+                for (int j = 0; j < 8; ++j)
+                    if (j < m) a[1 + j] += sj[j] - u * vbar[g * m + j];
+            }
+        }
+    }
+    __syncthreads();
+    for (int pr = threadIdx.x; pr < npair; pr += blockDim.x) {
+        double v = 0.0;
+        for (int q = 0; q < nsg; ++q) v += acc[q * npair + pr];
+        part[(int64_t)blockIdx.x * npair + pr] = v;
+    }
+}
+
+// The dense levels' sums from k_dense_sums' partials: a wave per (level,
+// column) pair, lanes strided over the workgroups, a fixed butterfly.
+__global__ void k_dense_sums_finish(int nb, int Ld, int m, const double *__restrict__ part,
+                                    const int *__restrict__ dense_levels,
+                                    double *__restrict__ diag, double *__restrict__ rhs) {
+    const int npair = Ld * (m + 1);
+    const int lane = threadIdx.x & 63;
+    const int waves = gridDim.x * (blockDim.x / 64);
+    for (int pr = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64; pr < npair; pr += waves) {
+        double v = 0.0;
+        for (int b = lane; b < nb; b += 64) v += part[(int64_t)b * npair + pr];
+        v = wave_sum(v);
+        if (lane == 0) {
+            const int t = pr / (m + 1), col = pr - t * (m + 1);
+            const int a = dense_levels[t];
+            if (col == 0) diag[a] = v;
+            else if (rhs) rhs[(int64_t)a * m + col - 1] = v;
+        }
+    }
+}
+
 __global__ void k_dense_diag_finish(int nb, int nbc, int Ld, int m, const double *__restrict__ part,
                                     const double *__restrict__ cpart,
                                     const int *__restrict__ dense_levels, double *__restrict__ diag,
@@ -2455,12 +2531,28 @@ int hdfe_panel_system(void *h, int with_rhs, void **matrix, double *seconds) {
     if (Ld) {
         DEV_ALLOC(U, std::max<int64_t>(Gi * Ld, 1), s);
         HIP_TRY(hipMemsetAsync(U.p, 0, std::max<int64_t>(Gi * Ld, 1) * 8, s));
+        const int npd = Ld * (m + 1);
+        if (m <= 8 && (int64_t)(T256 / cw) * npd * 8 <= 65536) {
+            const int nbs = 2048;
+            DevBuf<double> spart;
+            DEV_ALLOC(spart, (int64_t)nbs * npd, s);
+#define HDFE_L(W)                                                                              \
+    hipLaunchKernelGGL(k_dense_sums<W>, dim3(nbs), dim3(T256), (T256 / W) * npd * sizeof(double), \
+                       s, Gi, Ld, m, P.igstart.p, P.icodes.p, M, P.offs.p, ddl, d_dense_idx.p,  \
+                       P.in.p, P.isums.p, P.Ng.p, P.vbar.p, U.p, spart.p)
+            HDFE_BY_WIDTH(cw, HDFE_L)
+#undef HDFE_L
+            hipLaunchKernelGGL(k_dense_sums_finish, dim3(grid1((int64_t)npd * 64)), dim3(T256), 0,
+                               s, nbs, Ld, m, spart.p, d_dense_levels.p, P.diag.p,
+                               with_rhs ? P.rhs.p : (double *)nullptr);
+            HIP_TRY(hipGetLastError());
+        } else {
 #define HDFE_L(W)                                                                          \
     hipLaunchKernelGGL(k_dense_u_cells<W>, dim3(grid1(Gi * W)), dim3(T256), 0, s, Gi, Ld,  \
                        P.igstart.p, P.icodes.p, M, P.offs.p, ddl, d_dense_idx.p, P.in.p, U.p)
         HDFE_BY_WIDTH(cw, HDFE_L)
 #undef HDFE_L
-        const int nbd = 1024, nbc = 1024, npd = Ld * (m + 1);
+        const int nbd = 1024, nbc = 1024;
         DevBuf<double> dpart, cpart;
         DEV_ALLOC(dpart, (int64_t)nbd * npd, s);
         DEV_ALLOC(cpart, (int64_t)nbc * npd, s);
@@ -2471,6 +2563,7 @@ int hdfe_panel_system(void *h, int with_rhs, void **matrix, double *seconds) {
         hipLaunchKernelGGL(k_dense_diag_finish, dim3(grid1(npd)), dim3(T256), 0, s, nbd, nbc, Ld,
                            m, dpart.p, cpart.p, d_dense_levels.p, P.diag.p,
                            with_rhs ? P.rhs.p : (double *)nullptr);
+        }
         HIP_TRY(hipGetLastError());
     }
     HIP_TRY(hipStreamSynchronize(s));

@@ -147,3 +147,18 @@ def test_rows_in_group_order_and_shuffled_build_the_same_panel():
     assert rel(a.beta, b.beta) < 1e-11
     for f in a.fe_names:
         np.testing.assert_allclose(a.fe_values[f], b.fe_values[f], rtol=0, atol=1e-9)
+
+
+def test_many_covariates_with_a_dense_dimension(oracle_mod):
+    """Nine covariates (m = 10 variables with y) and the year dimension: the
+    dense levels' sums take the general path (more variables than the
+    one-pass kernel's registers hold)."""
+    from hdfe_stream_amd.simulate import akm_panel
+    d = akm_panel(2500, 200, seed=21)
+    rng = np.random.default_rng(4)
+    xs = ["x1", "x2"]
+    for j in range(3, 10):
+        d[f"x{j}"] = rng.standard_normal(len(d["y"]))
+        xs.append(f"x{j}")
+    print("fit with nine covariates", flush=True)
+    _check_fit(oracle_mod, d, xs=tuple(xs))
