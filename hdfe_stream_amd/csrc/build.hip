@@ -713,17 +713,56 @@ __global__ __launch_bounds__(256) void k_dense_sums(
     const int64_t g0 = part_begin(Gi, blockIdx.x, gridDim.x), g1 = part_begin(Gi, blockIdx.x + 1, gridDim.x);
     for (int64_t g = g0 + sg; g < g1; g += nsg) {
         const int c0 = igstart[g], c1 = igstart[g + 1];
+        // one dense dimension and at most two cells per lane (the usual
+        // case): each lane's cells read once, into registers
+        const bool simple = dense.n == 1 && c1 - c0 <= 2 * W;
+        int ta = -1, tb = -1;
+        double ina = 0.0, inb = 0.0, sa[8], sb[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) sa[j] = sb[j] = 0.0;
+        if (simple) {
+            const int d = dense.d[0];
+            const int ca = c0 + sub, cb = ca + W;
+            if (ca < c1) {
+                ta = dense_idx[offs[d] + icodes[d * ild + ca]];
+                ina = in[ca];
+#pragma unroll
+                for (int j = 0; j < 8; ++j)
+                    if (j < m) sa[j] = isums[(int64_t)ca * m + j];
+            }
+            if (cb < c1) {
+                tb = dense_idx[offs[d] + icodes[d * ild + cb]];
+                inb = in[cb];
+#pragma unroll
+                for (int j = 0; j < 8; ++j)
+                    if (j < m) sb[j] = isums[(int64_t)cb * m + j];
+            }
+        }
         for (int t = 0; t < Ld; ++t) {
             double u = 0.0, sj[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-            for (int c = c0 + sub; c < c1; c += W)
-                for (int q = 0; q < dense.n; ++q) {
-                    const int d = dense.d[q];
-                    if (dense_idx[offs[d] + icodes[d * ild + c]] != t) continue;
-                    u += in[c];
+            if (simple) {
+                // the cells in order, as the general loop adds them
+                if (ta == t) {
+                    u += ina;
 #pragma unroll
-                    for (int j = 0; j < 8; ++j)
-                        if (j < m) sj[j] += isums[(int64_t)c * m + j];
+                    for (int j = 0; j < 8; ++j) sj[j] += sa[j];
                 }
+                if (tb == t) {
+                    u += inb;
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) sj[j] += sb[j];
+                }
+            } else {
+                for (int c = c0 + sub; c < c1; c += W)
+                    for (int q = 0; q < dense.n; ++q) {
+                        const int d = dense.d[q];
+                        if (dense_idx[offs[d] + icodes[d * ild + c]] != t) continue;
+                        u += in[c];
+#pragma unroll
+                        for (int j = 0; j < 8; ++j)
+                            if (j < m) sj[j] += isums[(int64_t)c * m + j];
+                    }
+            }
             u = run_sum<W>(u);
 #pragma unroll
             for (int j = 0; j < 8; ++j)
@@ -1237,18 +1276,26 @@ __global__ __launch_bounds__(256) void k_dd(int64_t Gi, int Ld, const int *__res
     }
 }
 
+// A wave per pair: lanes strided over the workgroups' partials, a fixed
+// butterfly.
 __global__ void k_dd_finish(int nb, int npair, const double *__restrict__ part,
                             const int *__restrict__ ppart, double *__restrict__ DD,
                             uint8_t *__restrict__ DP) {
-    for (int pr = blockIdx.x * blockDim.x + threadIdx.x; pr < npair; pr += gridDim.x * blockDim.x) {
+    const int lane = threadIdx.x & 63;
+    const int waves = gridDim.x * (blockDim.x / 64);
+    for (int pr = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64; pr < npair; pr += waves) {
         double v = 0.0;
         int p = 0;
-        for (int b = 0; b < nb; ++b) {
+        for (int b = lane; b < nb; b += 64) {
             v += part[(int64_t)b * npair + pr];
             p |= ppart[(int64_t)b * npair + pr];
         }
-        DD[pr] = v;
-        DP[pr] = (uint8_t)p;
+        v = wave_sum(v);
+        p = __any(p) ? 1 : 0;
+        if (lane == 0) {
+            DD[pr] = v;
+            DP[pr] = (uint8_t)p;
+        }
     }
 }
 
@@ -2659,7 +2706,7 @@ int hdfe_panel_system(void *h, int with_rhs, void **matrix, double *seconds) {
             HIP_TRY(hipGetLastError());
             HIP_TRY(hipStreamSynchronize(s));
         }
-        const int nbd = 1024;
+        const int nbd = Gi > (int64_t)1 << 20 ? 4096 : 1024;
         DevBuf<double> ddp;
         DevBuf<int> ddpp;
         DEV_ALLOC(ddp, (int64_t)nbd * Ld * Ld, s);
@@ -2667,7 +2714,7 @@ int hdfe_panel_system(void *h, int with_rhs, void **matrix, double *seconds) {
         hipLaunchKernelGGL(k_dd, dim3(nbd), dim3(T256), 0, s, Gi, Ld, d_dense_levels.p,
                            d_lvl_dim.p, P.offs.p, U.p, P.Ng.p, P.igstart.p, P.icodes.p, M,
                            P.in.p, ddp.p, ddpp.p);
-        hipLaunchKernelGGL(k_dd_finish, dim3(grid1(Ld * Ld)), dim3(T256), 0, s, nbd, Ld * Ld,
+        hipLaunchKernelGGL(k_dd_finish, dim3(grid1((int64_t)Ld * Ld * 64)), dim3(T256), 0, s, nbd, Ld * Ld,
                            ddp.p, ddpp.p, DD.p, DP.p);
         // per dense level: positions of the present sparse levels
         DEV_ALLOC(colbuf, Ls + 1, s);

@@ -149,16 +149,12 @@ def test_rows_in_group_order_and_shuffled_build_the_same_panel():
         np.testing.assert_allclose(a.fe_values[f], b.fe_values[f], rtol=0, atol=1e-9)
 
 
-def test_many_covariates_with_a_dense_dimension(oracle_mod):
-    """Nine covariates (m = 10 variables with y) and the year dimension: the
-    dense levels' sums take the general path (more variables than the
-    one-pass kernel's registers hold)."""
+def test_a_wide_dense_dimension_on_short_groups(oracle_mod):
+    """Forty levels of a dense dimension on groups of three rows: the dense
+    levels' (level, column) slots outgrow the one-pass kernel's LDS, and the
+    general path sums them."""
     from hdfe_stream_amd.simulate import akm_panel
-    d = akm_panel(2500, 200, seed=21)
-    rng = np.random.default_rng(4)
-    xs = ["x1", "x2"]
-    for j in range(3, 10):
-        d[f"x{j}"] = rng.standard_normal(len(d["y"]))
-        xs.append(f"x{j}")
-    print("fit with nine covariates", flush=True)
-    _check_fit(oracle_mod, d, xs=tuple(xs))
+    d = akm_panel(3000, 200, seed=21, n_years=3)
+    d["month"] = np.random.default_rng(4).integers(0, 40, len(d["y"]))
+    print("fit with a 40-level dense dimension", flush=True)
+    _check_fit(oracle_mod, d, fe=["worker_id", "firm_id", "month"])
