@@ -1718,6 +1718,120 @@ __global__ __launch_bounds__(T256) void k_assemble(
     }
 }
 
+// V'M_D V for more than eight variables, by 8 x 8 tiles (columns ja + a,
+// jb + b for a < na, b < nb): k_assemble's between-cell part and k_cells'
+// within-cell scatter for one tile, the same products in the same order as
+// the one-tile kernels take them per entry.
+template <int W>
+__global__ __launch_bounds__(T256) void k_assemble_tile(
+    int64_t Gi, int D, int m, int ja, int na, int jb, int nbc, const int *__restrict__ igstart,
+    const int *__restrict__ icodes, int64_t ild, const int64_t *__restrict__ offs,
+    const double *__restrict__ in, const double *__restrict__ isums,
+    const double *__restrict__ Gm, double *__restrict__ part) {
+    __shared__ double lds[T256 / 64];
+    double A[64];
+#pragma unroll
+    for (int t = 0; t < 64; ++t) A[t] = 0.0;
+    const int sub = threadIdx.x % W;
+    const int64_t g0 = part_begin(Gi, blockIdx.x, gridDim.x), g1 = part_begin(Gi, blockIdx.x + 1, gridDim.x);
+    for (int64_t g = g0 + threadIdx.x / W; g < g1; g += blockDim.x / W) {
+        const int s = igstart[g], e = igstart[g + 1];
+        double qa[8], qb[8], N = 0.0;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) qa[j] = qb[j] = 0.0;
+        for (int c = s + sub; c < e; c += W) {
+            const double ic = in[c];
+            N += ic;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                if (j < na) {
+                    double v = isums[(int64_t)c * m + ja + j] / ic;
+                    for (int d = 0; d < D; ++d) v -= Gm[(offs[d] + icodes[d * ild + c]) * m + ja + j];
+                    qa[j] += ic * v;
+                }
+                if (j < nbc) {
+                    double v = isums[(int64_t)c * m + jb + j] / ic;
+                    for (int d = 0; d < D; ++d) v -= Gm[(offs[d] + icodes[d * ild + c]) * m + jb + j];
+                    qb[j] += ic * v;
+                }
+            }
+        }
+        N = run_sum<W>(N);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            qa[j] = run_sum<W>(qa[j]) / N;
+            qb[j] = run_sum<W>(qb[j]) / N;
+        }
+        for (int c = s + sub; c < e; c += W) {
+            const double ic = in[c];
+            double pa[8], pb[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                pa[j] = pb[j] = 0.0;
+                if (j < na) {
+                    double v = isums[(int64_t)c * m + ja + j] / ic;
+                    for (int d = 0; d < D; ++d) v -= Gm[(offs[d] + icodes[d * ild + c]) * m + ja + j];
+                    pa[j] = v - qa[j];
+                }
+                if (j < nbc) {
+                    double v = isums[(int64_t)c * m + jb + j] / ic;
+                    for (int d = 0; d < D; ++d) v -= Gm[(offs[d] + icodes[d * ild + c]) * m + jb + j];
+                    pb[j] = v - qb[j];
+                }
+            }
+#pragma unroll
+            for (int a = 0; a < 8; ++a)
+#pragma unroll
+                for (int b = 0; b < 8; ++b) A[a * 8 + b] += ic * pa[a] * pb[b];
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < 64; ++t) {
+        const double v = block_sum<double>(A[t], lds);
+        if (threadIdx.x == 0) part[(int64_t)blockIdx.x * 64 + t] = v;
+    }
+}
+
+// The within-cell scatter of one 8 x 8 tile (k_cells' W for m > 8): per
+// cell, sum over its rows of w (V_a - mean_a - s_a / n)(V_b - mean_b - s_b / n).
+__global__ __launch_bounds__(T256) void k_cells_tile(int64_t C, int m, int ja, int na, int jb,
+                                                     int nbc, const int *__restrict__ cstart,
+                                                     const double *__restrict__ V,
+                                                     const double *__restrict__ w,
+                                                     const double *__restrict__ mean,
+                                                     const double *__restrict__ cn,
+                                                     const double *__restrict__ csums,
+                                                     double *__restrict__ wpart) {
+    __shared__ double lds[T256];
+    double Wt[64];
+#pragma unroll
+    for (int t = 0; t < 64; ++t) Wt[t] = 0.0;
+    const int64_t c0 = part_begin(C, blockIdx.x, gridDim.x), c1 = part_begin(C, blockIdx.x + 1, gridDim.x);
+    for (int64_t c = c0 + threadIdx.x; c < c1; c += blockDim.x) {
+        const int s = cstart[c], e = cstart[c + 1];
+        if (e - s < 2) continue;  // one row: every deviation is zero
+        const double nn = cn[c];
+        for (int i = s; i < e; ++i) {
+            const double wi = w ? w[i] : 1.0;
+            double da[8], db[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                da[j] = j < na ? V[(int64_t)i * m + ja + j] - mean[ja + j] - csums[c * m + ja + j] / nn : 0.0;
+                db[j] = j < nbc ? V[(int64_t)i * m + jb + j] - mean[jb + j] - csums[c * m + jb + j] / nn : 0.0;
+            }
+#pragma unroll
+            for (int a = 0; a < 8; ++a)
+#pragma unroll
+                for (int b = 0; b < 8; ++b) Wt[a * 8 + b] += wi * da[a] * db[b];
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < 64; ++t) {
+        const double v = block_sum<double>(Wt[t], lds);
+        if (threadIdx.x == 0) wpart[(int64_t)blockIdx.x * 64 + t] = v;
+    }
+}
+
 // Streamed effects and residuals (core/kernels_base.py:543-619): per group
 // the weighted mean of y - x'beta - sum_d gam_y[level], the residual the
 // rest.  V column 0 is y, columns xcols[...] the regressors.
@@ -2095,13 +2209,39 @@ static int run_pass1(Panel &P) {
         }
 #undef HDFE_CELLS
     }
-    DEV_ALLOC(P.W, std::max(mm, 1), s);
+    DEV_ALLOC(P.W, std::max(m * m, 1), s);
     if (mm) {
         HIP_TRY(to_host(hp, wpart.p, (int64_t)nb * mm, s));
         std::vector<double> Wh(mm, 0.0);
         for (int b = 0; b < nb; ++b)
             for (int t = 0; t < mm; ++t) Wh[t] += hp[(size_t)b * mm + t];
         HIP_TRY(hipMemcpyAsync(P.W.p, Wh.data(), mm * 8, hipMemcpyHostToDevice, s));
+    } else {
+        // more than eight variables: the scatter by 8 x 8 tiles
+        const double *wp = P.weighted ? P.w.p : (const double *)nullptr;
+        DevBuf<double> tpart;
+        DEV_ALLOC(tpart, (int64_t)nb * 64, s);
+        std::vector<double> Wh((size_t)m * m, 0.0), th;
+        const int nt = (m + 7) / 8;
+        for (int ta = 0; ta < nt; ++ta)
+            for (int tb = ta; tb < nt; ++tb) {
+                const int ja = 8 * ta, jb = 8 * tb;
+                const int na = std::min(8, m - ja), nbc = std::min(8, m - jb);
+                hipLaunchKernelGGL(k_cells_tile, dim3(nb), dim3(T256), 0, s, (int64_t)nc, m, ja, na,
+                                   jb, nbc, P.cstart.p, P.V.p, wp, P.means.p, cn.p, csums.p,
+                                   tpart.p);
+                HIP_TRY(hipGetLastError());
+                HIP_TRY(to_host(th, tpart.p, (int64_t)nb * 64, s));
+                for (int a = 0; a < na; ++a)
+                    for (int b = 0; b < nbc; ++b) {
+                        double v = 0.0;
+                        for (int q = 0; q < nb; ++q) v += th[(size_t)q * 64 + a * 8 + b];
+                        Wh[(size_t)(ja + a) * m + jb + b] = v;
+                        Wh[(size_t)(jb + b) * m + ja + a] = v;
+                    }
+            }
+        HIP_TRY(hipMemcpyAsync(P.W.p, Wh.data(), (size_t)m * m * 8, hipMemcpyHostToDevice, s));
+        HIP_TRY(hipStreamSynchronize(s));
     }
     const int64_t L = P.L;
     DEV_ALLOC(P.obs, std::max<int64_t>(L, 1), s);
@@ -2385,7 +2525,7 @@ int hdfe_panel_array(void *h, const char *name, void **ptr, int64_t *count) {
     else if (nm == "glevels") { p = P.glevels.p; c = P.G; }
     else if (nm == "means") { p = P.means.p; c = P.m; }
     else if (nm == "raw_ss") { p = P.raw_ss.p; c = P.m; }
-    else if (nm == "W") { p = P.W.p; c = P.m <= 8 ? P.m * P.m : 0; }
+    else if (nm == "W") { p = P.W.p; c = (int64_t)P.m * P.m; }
     else if (nm == "obs") { p = P.obs.p; c = P.L; }
     else if (nm == "cnt") { p = P.cnt.p; c = P.L; }
     else if (nm == "icodes") { p = P.icodes.p; c = P.D * P.M; }
@@ -2888,13 +3028,41 @@ int hdfe_panel_normalize(void *h, double *Gm, int k, void *stream) {
     return HDFE_STATUS_OK;
 }
 
-// A = W_within + V' M_D V between cells (m x m, host out; m <= 8).
+// A = W_within + V' M_D V between cells (m x m, host out; more than eight
+// variables by 8 x 8 tiles).
 int hdfe_panel_assemble(void *h, const double *Gm, double *A) {
     ARG_CHECK(h && Gm && A, "hdfe_panel_assemble: null argument");
     Panel &P = *(Panel *)h;
-    ARG_CHECK(P.m <= 8, "hdfe_panel_assemble: cell assembly needs m <= 8");
     hipStream_t s = P.s;
     const int m = P.m, nb = 4096, cw = run_width(P.M, P.Gi);
+    if (m > 8) {
+        // by 8 x 8 tiles of the variables
+        DevBuf<double> tpart;
+        DEV_ALLOC(tpart, (int64_t)nb * 64, s);
+        std::vector<double> th, Wh;
+        HIP_TRY(to_host(Wh, P.W.p, (int64_t)m * m, s));
+        const int nt = (m + 7) / 8;
+        for (int ta = 0; ta < nt; ++ta)
+            for (int tb = ta; tb < nt; ++tb) {
+                const int ja = 8 * ta, jb = 8 * tb;
+                const int na = std::min(8, m - ja), nbc = std::min(8, m - jb);
+#define HDFE_L(W)                                                                               \
+    hipLaunchKernelGGL(k_assemble_tile<W>, dim3(nb), dim3(T256), 0, s, P.Gi, P.D, m, ja, na, jb, \
+                       nbc, P.igstart.p, P.icodes.p, P.M, P.offs.p, P.in.p, P.isums.p, Gm, tpart.p)
+                HDFE_BY_WIDTH(cw, HDFE_L)
+#undef HDFE_L
+                HIP_TRY(hipGetLastError());
+                HIP_TRY(to_host(th, tpart.p, (int64_t)nb * 64, s));
+                for (int a = 0; a < na; ++a)
+                    for (int b = 0; b < nbc; ++b) {
+                        double v = Wh[(size_t)(ja + a) * m + jb + b];
+                        for (int q = 0; q < nb; ++q) v += th[(size_t)q * 64 + a * 8 + b];
+                        A[(ja + a) * m + jb + b] = v;
+                        A[(jb + b) * m + ja + a] = v;
+                    }
+            }
+        return HDFE_STATUS_OK;
+    }
     DevBuf<double> part;
     DEV_ALLOC(part, (int64_t)nb * m * m, s);
     switch (m) {

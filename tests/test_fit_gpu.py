@@ -158,3 +158,26 @@ def test_a_wide_dense_dimension_on_short_groups(oracle_mod):
     d["month"] = np.random.default_rng(4).integers(0, 40, len(d["y"]))
     print("fit with a 40-level dense dimension", flush=True)
     _check_fit(oracle_mod, d, fe=["worker_id", "firm_id", "month"])
+
+
+@pytest.mark.parametrize("weighted", [False, True])
+def test_twelve_covariates(oracle_mod, weighted):
+    """Thirteen variables (y and twelve regressors): the within-cell scatter
+    and the between-cell assembly go by 8 x 8 tiles, the dense levels by the
+    general path.  Repeated (worker, firm, year) cells give the within-cell
+    part weight."""
+    from hdfe_stream_amd.simulate import akm_panel
+    d = akm_panel(3000, 250, seed=31)
+    d["year"] = d["year"] % 3
+    rng = np.random.default_rng(5)
+    n = len(d["y"])
+    xs = []
+    for j in range(12):
+        x = rng.standard_normal(n) + 0.3 * np.sin(d["firm_id"] * (j + 1.0)) + 0.1 * j * d["year"]
+        d[f"z{j}"] = x
+        d["y"] = d["y"] + 0.05 * (j - 5.5) * x
+        xs.append(f"z{j}")
+    if weighted:
+        d["w"] = rng.uniform(0.5, 2.0, n)
+    print(f"fit with twelve covariates, weighted={weighted}", flush=True)
+    _check_fit(oracle_mod, d, xs=xs, weights="w" if weighted else None)
