@@ -340,14 +340,19 @@ def main():
     def fit(precision):
         return feols_stream(FML, data, precision=precision, tol=1e-10)
 
+    # each precision's last result is kept (for the comparison below) and
+    # released before that precision's next fit, in the warm-up as in the
+    # timed steps, so every timed fit finds the same device memory free
+    results = {}
     for w in range(args.warmup):
-        fit("fp64")
-        fit("mixed")
+        for prec in ("fp64", "mixed"):
+            results.pop(prec, None)
+            results[prec] = fit(prec)
         log(f"warmup {w + 1}/{args.warmup} done")
     times = {"fp64": [], "mixed": []}
-    results = {}
     for s in range(args.steps):
         for prec in ("fp64", "mixed"):
+            results.pop(prec, None)
             torch.cuda.synchronize()
             t0 = time.perf_counter()
             r = fit(prec)

@@ -1915,6 +1915,7 @@ __global__ void k_gam_y(int64_t L, int m, const double *__restrict__ Gm,
 
 struct Panel {
     hipStream_t s = nullptr;
+    int dev = 0;  // the device of s
     Temp tmp;
     int64_t n_in = 0, n = 0;  // rows given / kept
     int nfe = 0, D = 0, m = 0, gdim = 0;
@@ -2310,7 +2311,8 @@ int hdfe_panel_create(int64_t n_rows, int nfe, const int64_t *const *fe, int m,
     ARG_CHECK(m >= 1 && m <= 64, "hdfe_panel_create: 1 to 64 variables");
     std::unique_ptr<Panel> P(new Panel());
     const auto t0 = std::chrono::steady_clock::now();
-    HIP_TRY(hipStreamCreateWithFlags(&P->s, hipStreamNonBlocking));
+    HIP_TRY(hipGetDevice(&P->dev));
+    HIP_TRY(stream_get(&P->s));
     hipStream_t s = P->s;
     const int64_t n = n_rows;
     P->n_in = n;
@@ -3168,8 +3170,9 @@ void hdfe_panel_destroy(void *h) {
     (void)hipDeviceSynchronize();
     if (P->matrix) hdfe_destroy(P->matrix);
     hipStream_t s = P->s;
+    const int dev = P->dev;
     delete P;
-    if (s) (void)hipStreamDestroy(s);
+    stream_put(s, dev);
 }
 
 }  // extern "C"

@@ -13,50 +13,30 @@
 
 namespace hdfe {
 
-// Small device buffers come from the device's stream-ordered pool
-// (hipMallocAsync / hipFreeAsync), large ones from the library's own cache
-// (big_alloc below).  The pool keeps its freed memory for reuse (no release
-// threshold) instead of returning it at synchronisations, so the many
-// temporaries of a fit cost no driver allocation after the first fit, and a
-// free neither blocks the host nor synchronises the device.  A buffer is
-// freed on the stream it was allocated on: reuse is ordered after the work
-// queued there.  Owners that hand their memory to other streams synchronise
-// the device before freeing it.
-constexpr uint64_t kPoolKeep = ~0ull;
-
-inline hipError_t pool_prepare() {
-    static std::mutex mu;
-    static std::vector<int> done;
-    int dev = 0;
-    hipError_t e = hipGetDevice(&dev);
-    if (e != hipSuccess) return e;
-    std::lock_guard<std::mutex> lock(mu);
-    if ((int)done.size() > dev && done[dev]) return hipSuccess;
-    hipMemPool_t pool;
-    if ((e = hipDeviceGetDefaultMemPool(&pool, dev)) != hipSuccess) return e;
-    uint64_t keep = kPoolKeep;
-    if ((e = hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep)) != hipSuccess)
-        return e;
-    if ((int)done.size() <= dev) done.resize(dev + 1, 0);
-    done[dev] = 1;
-    return hipSuccess;
-}
-
-// Large buffers (kBigBytes and up) are kept by the library once allocated:
-// a freed block waits in a free list with an event recorded on the stream it
-// was freed on, and a later allocation that it fits (at most twice its size,
-// same device) takes it after making its own stream wait for that event.  The
+// Device buffers come from the library's own cache (big_alloc below), which
+// keeps freed blocks for reuse instead of returning them to the driver, so
+// the many temporaries of a fit cost no driver allocation after the first
+// fit, and a free neither blocks the host nor synchronises the device.  A
+// buffer is freed on the stream it was allocated on: reuse is ordered after
+// the work queued there.  Owners that hand their memory to other streams
+// synchronise the device before freeing it.
+//
+// Buffers are kept by the library once allocated: a freed block waits in a
+// free list with an event recorded on the stream it was freed on, and a later
+// allocation that it fits (at most twice its size, same device) takes it,
+// making its own stream wait for that event unless it is the same stream.
+// Blocks are sized in powers of two up to 2 MB and in 2 MB steps above.  The
 // stream-ordered pool was measured spending milliseconds inside hipFreeAsync
-// for buffers of hundreds of megabytes; with the cache a repeated fit maps no
-// memory and frees none.  Blocks go back to the driver only when an
-// allocation fails for want of memory.  Small buffers use the pool.
-constexpr size_t kBigBytes = size_t(4) << 20;
-
+// for buffers of hundreds of megabytes, and 10-25 us per call for small ones
+// (about a hundred per fit); with the cache a repeated fit maps no memory and
+// frees none.  Blocks go back to the driver only when an allocation fails for
+// want of memory.
 struct BigCache {
     struct Block {
         void *p = nullptr;
         size_t bytes = 0;
         hipEvent_t ev = nullptr;
+        hipStream_t freed_on = nullptr;
         int dev = 0;
     };
     std::mutex mu;
@@ -92,14 +72,19 @@ inline hipError_t big_alloc(void **p, size_t bytes, hipStream_t s) {
         if (it->second.dev != dev) continue;
         BigCache::Block b = it->second;
         c.free_list.erase(it);
-        if ((e = hipStreamWaitEvent(s, b.ev, 0)) != hipSuccess) return e;
+        if (b.freed_on != s && (e = hipStreamWaitEvent(s, b.ev, 0)) != hipSuccess) return e;
         c.live[b.p] = b;
         *p = b.p;
         return hipSuccess;
     }
     const size_t grain = size_t(2) << 20;
     BigCache::Block b;
-    b.bytes = (bytes + grain - 1) / grain * grain;
+    if (bytes >= grain) {
+        b.bytes = (bytes + grain - 1) / grain * grain;
+    } else {
+        b.bytes = 256;
+        while (b.bytes < bytes) b.bytes <<= 1;
+    }
     b.dev = dev;
     e = hipMalloc(&b.p, b.bytes);
     if (e == hipErrorOutOfMemory) {
@@ -127,25 +112,67 @@ inline bool big_free(void *p, hipStream_t s) {
     BigCache::Block b = it->second;
     c.live.erase(it);
     (void)hipEventRecord(b.ev, s);
+    b.freed_on = s;
     c.free_list.emplace(b.bytes, b);
     return true;
 }
 
-// Device memory of any size on stream s: the cache for large buffers, the
-// pool for the rest; dev_free takes either.
+// Device memory of any size on stream s, from the cache.
 inline hipError_t dev_malloc(void **p, size_t bytes, hipStream_t s) {
-    bytes = std::max<size_t>(bytes, 1);
-    if (bytes >= kBigBytes) return big_alloc(p, bytes, s);
-    hipError_t e = pool_prepare();
-    if (e != hipSuccess) return e;
-    return hipMallocAsync(p, bytes, s);
+    return big_alloc(p, std::max<size_t>(bytes, 1), s);
 }
 
 inline void dev_free(void *p, hipStream_t s) {
-    if (p && !big_free(p, s)) (void)hipFreeAsync(p, s);
+    if (p && !big_free(p, s)) (void)hipFree(p);  // not the cache's: a plain allocation
 }
 
-// An owning device allocation (dev_malloc: the cache or the pool).
+// Streams of the library's handles (panels, solver matrices, row passes) come
+// from a per-device free list: creating a HIP stream was measured at 0.1 to
+// 5 ms of host time and destroying one at 0.1 to 0.5 ms, twice per fit.  A
+// returned stream is synchronised first, so the next owner finds it idle.
+// All are non-blocking streams of the default priority.
+struct StreamPool {
+    std::mutex mu;
+    std::vector<std::pair<int, hipStream_t>> free_list;
+};
+
+inline StreamPool &stream_pool() {
+    static StreamPool *sp = new StreamPool();  // lives to process exit
+    return *sp;
+}
+
+inline hipError_t stream_get(hipStream_t *s) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    StreamPool &sp = stream_pool();
+    {
+        std::lock_guard<std::mutex> lock(sp.mu);
+        for (size_t i = sp.free_list.size(); i-- > 0;) {
+            if (sp.free_list[i].first != dev) continue;
+            *s = sp.free_list[i].second;
+            sp.free_list.erase(sp.free_list.begin() + (std::ptrdiff_t)i);
+            return hipSuccess;
+        }
+    }
+    return hipStreamCreateWithFlags(s, hipStreamNonBlocking);
+}
+
+// Returns s (made on the current device's list when dev < 0) to the pool.
+inline void stream_put(hipStream_t s, int dev = -1) {
+    if (!s) return;
+    if (hipStreamSynchronize(s) != hipSuccess) {
+        (void)hipGetLastError();
+        (void)hipStreamDestroy(s);
+        return;
+    }
+    if (dev < 0 && hipGetDevice(&dev) != hipSuccess) return;
+    StreamPool &sp = stream_pool();
+    std::lock_guard<std::mutex> lock(sp.mu);
+    sp.free_list.emplace_back(dev, s);
+}
+
+// An owning device allocation (dev_malloc: the cache).
 template <typename T>
 struct DevBuf {
     T *p = nullptr;

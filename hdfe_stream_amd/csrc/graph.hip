@@ -383,10 +383,10 @@ int hdfe_loo_prune(int64_t n, const int64_t *a_ids, const int64_t *b_ids, uint8_
                    int64_t *info) {
     ARG_CHECK(a_ids && b_ids && keep && info && n > 0 && n < INT32_MAX, "hdfe_loo_prune: bad argument");
     hipStream_t s = nullptr;
-    HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    HIP_TRY(stream_get(&s));
     struct StreamGuard {
         hipStream_t s;
-        ~StreamGuard() { (void)hipStreamDestroy(s); }
+        ~StreamGuard() { stream_put(s); }
     } guard{s};
     Temp tmp;
     PairTables T;
@@ -426,10 +426,10 @@ int hdfe_components(int64_t n, const int64_t *a_ids, const int64_t *b_ids, int m
     ARG_CHECK(min_deg0 >= 1 && min_deg1 >= 1, "hdfe_components: min_degree must be >= 1");
     ARG_CHECK(rank_by >= -1 && rank_by <= 1, "hdfe_components: rank_by must be -1, 0 or 1");
     hipStream_t s = nullptr;
-    HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    HIP_TRY(stream_get(&s));
     struct StreamGuard {
         hipStream_t s;
-        ~StreamGuard() { (void)hipStreamDestroy(s); }
+        ~StreamGuard() { stream_put(s); }
     } guard{s};
     std::unique_ptr<Components> R(new Components());
     {

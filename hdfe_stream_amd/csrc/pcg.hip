@@ -1586,7 +1586,7 @@ int matrix_create(int64_t L, int64_t nnz, const std::vector<int> &ip32, const in
     }
     M->nlong = (int)lrows.size();
     M->nchunk = (int)crow.size();
-    HIP_TRY(hipStreamCreateWithFlags(&M->stream, hipStreamNonBlocking));
+    HIP_TRY(stream_get(&M->stream));
     HIP_TRY(hipEventCreateWithFlags(&M->ev, hipEventDisableTiming));
     hipStream_t ms = M->stream;
     HIP_TRY(palloc(&M->indptr, (L + 1) * 4, ms));
@@ -1896,7 +1896,7 @@ void hdfe_destroy(void *h) {
     for (void *p : ps) pfree(p, M->stream);
     if (M->stream) (void)hipStreamSynchronize(M->stream);
     if (M->ev) (void)hipEventDestroy(M->ev);
-    if (M->stream) (void)hipStreamDestroy(M->stream);
+    stream_put(M->stream, M->device);
     delete M;
 }
 

@@ -345,6 +345,8 @@ using namespace hdfe;
 // The row-pass state kept per panel: the level-to-rows CSR of every
 // dimension and its chunks, and scratch for the group means.
 struct RowPasses {
+    hipStream_t s = nullptr;
+    int dev = 0;  // the device of s
     int64_t n = 0, G = 0, L = 0;
     int D = 0;
     DevBuf<int> rptr, rrow, cptr, clvl, cbeg;
@@ -352,7 +354,6 @@ struct RowPasses {
     DevBuf<double> Rbar, part, cpart;
     int64_t kcap = 0, ccap = 0;
     Temp tmp;
-    hipStream_t s = nullptr;
 };
 
 extern "C" {
@@ -375,7 +376,8 @@ int hdfe_rows_create(int64_t n, int64_t G, int D, const int *codes, const int64_
                      const int64_t *nlev_h, void **out) {
     ARG_CHECK(out && codes && offs_h && nlev_h && D >= 1, "hdfe_rows_create: bad argument");
     std::unique_ptr<RowPasses> R(new RowPasses());
-    HIP_TRY(hipStreamCreateWithFlags(&R->s, hipStreamNonBlocking));
+    HIP_TRY(hipGetDevice(&R->dev));
+    HIP_TRY(stream_get(&R->s));
     hipStream_t s = R->s;
     R->n = n;
     R->G = G;
@@ -567,8 +569,9 @@ void hdfe_rows_destroy(void *h) {
     // buffers are freed on the passes' stream, which must outlive them
     (void)hipDeviceSynchronize();
     hipStream_t s = R->s;
+    const int dev = R->dev;
     delete R;
-    if (s) (void)hipStreamDestroy(s);
+    stream_put(s, dev);
 }
 
 }  // extern "C"
