@@ -68,6 +68,9 @@ struct CgState {
 // ----------------------------------------------------------------- kernels
 
 constexpr int kFinThreads = 1024;  // threads of the one-workgroup finishing kernels
+// CG batches are captured as graphs only below this many bytes moved per
+// iteration (about 16 us of device time; six launches cost ~20 us of host time)
+constexpr int64_t kGraphMaxBytes = int64_t(48) << 20;
 
 // out[j] = the sum over w < nw of part[w * stride + j], for every column
 // j < k, by the whole workgroup: TJ threads across columns and TR = blockDim
@@ -1259,6 +1262,13 @@ static int run_cg(Matrix &M, int k, const T *val, const T *dinv, T *X, T *R, T *
         hipLaunchKernelGGL((k_direction<T>), dim3(gd), dim3(kThreads), 0, s, M.L, P, R, dinv,
                            st);
     };
+    // Graphs pay where launching an iteration's six kernels costs more host
+    // time than they run on the device: small systems.  A large one keeps
+    // the device busy from eager launches, and capturing and instantiating a
+    // graph there idles it (0.3-0.4 ms per capture measured on 400,000
+    // levels); its batches run eagerly.
+    const int64_t iter_bytes = M.nnz * (int64_t)(4 + sizeof(T)) + M.L * (int64_t)k * sizeof(T) * 6;
+    const bool use_graphs = !M.timing && iter_bytes < kGraphMaxBytes;
     int host[2] = {0, 0};
     int batch = 8;
     int64_t done_before = 0;  // iterations launched before this batch
@@ -1271,7 +1281,7 @@ static int run_cg(Matrix &M, int k, const T *val, const T *dinv, T *X, T *R, T *
                 M.evpool.push_back(e);
             }
         hipGraphExec_t ex = nullptr;
-        if (!M.timing) {
+        if (use_graphs) {
             // a batch of nb iterations as a graph: its kernels' arguments are
             // fixed for these buffers (tol and maxiter are read on the device)
             const std::array<uintptr_t, 9> key = {
@@ -1786,6 +1796,14 @@ int hdfe_csr_blk_parts(void *h, int k, int precision, int set, int *parts) {
         M.blk_parts[precision][k] = set;
     }
     *parts = M.blk_parts[precision][k];
+    return HDFE_STATUS_OK;
+}
+
+int hdfe_csr_graphs(void *h, int64_t *count) {
+    ARG_CHECK(h && count, "hdfe_csr_graphs: null argument");
+    Matrix &M = *(Matrix *)h;
+    std::lock_guard<std::mutex> lock(M.mu);
+    *count = (int64_t)M.cg_graphs.size();
     return HDFE_STATUS_OK;
 }
 

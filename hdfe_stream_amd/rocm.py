@@ -173,6 +173,12 @@ class DeviceCSR:
                                                   int(set), C.byref(out)), "hdfe_csr_blk_parts")
         return int(out.value)
 
+    def graphs(self) -> int:
+        """CG batches held as HIP graphs (captured on small systems only)."""
+        n = C.c_int64()
+        _lib.check(_lib.load().hdfe_csr_graphs(self._h, C.byref(n)), "hdfe_csr_graphs")
+        return int(n.value)
+
     def timing_read(self) -> dict:
         ms = C.c_double()
         n = C.c_int64()

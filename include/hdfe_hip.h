@@ -91,6 +91,12 @@ int hdfe_csr_timing_read(void *h, double *ms, int64_t *launches);
  * launch autotuner's report and tests. */
 int hdfe_csr_blk_parts(void *h, int k, int precision, int set, int *parts);
 
+/* The number of CG batches held as HIP graphs on this matrix.  Batches are
+ * captured only on systems small enough that launching costs more than the
+ * device time of an iteration.  No counterpart in the reference (it has no
+ * graphs); for tests. */
+int hdfe_csr_graphs(void *h, int64_t *count);
+
 /* Average duration in microseconds of the solve's SpMM kernel on this
  * matrix (k right-hand sides, fused pAp sums): reps back-to-back launches
  * between two HIP events on the solver's stream; precision 0 FP64, 1 FP32.
