@@ -695,9 +695,11 @@ __global__ __launch_bounds__(256) void k_dense_diag(int64_t Gi, int Ld, int m,
 // for every sub-group's (level, column) pairs within 64 KB): W lanes per
 // group find, per dense level t, the
 // weight u and the cell sums s_j of the group's cells at t (lane sums, a
-// fixed butterfly); the group's lane 0 writes U[g, t] and adds u - u^2 / N_g
-// and s_j - u vbar_gj (kernels_base.py:64-110) to its own LDS slots; the
-// slots are added in sub-group order into part[b, t, col] (fixed order).
+// fixed butterfly, which leaves the sums on every lane); lane t of the group
+// (lane 0 for every t when there are more levels than lanes) writes U[g, t]
+// and adds u - u^2 / N_g and s_j - u vbar_gj (kernels_base.py:64-110) to the
+// sub-group's LDS slots; the slots are added in sub-group order into
+// part[b, t, col] (fixed order).
 template <int W>
 __global__ __launch_bounds__(256) void k_dense_sums(
     int64_t Gi, int Ld, int m, const int *__restrict__ igstart, const int *__restrict__ icodes,
@@ -711,8 +713,10 @@ __global__ __launch_bounds__(256) void k_dense_sums(
     for (int e = threadIdx.x; e < nsg * npair; e += blockDim.x) acc[e] = 0.0;
     __syncthreads();
     const int64_t g0 = part_begin(Gi, blockIdx.x, gridDim.x), g1 = part_begin(Gi, blockIdx.x + 1, gridDim.x);
+    const bool own = Ld <= W;  // lane t keeps level t's sums
     for (int64_t g = g0 + sg; g < g1; g += nsg) {
         const int c0 = igstart[g], c1 = igstart[g + 1];
+        double myu = 0.0, mys[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
         // one dense dimension and at most two cells per lane (the usual
         // case): each lane's cells read once, into registers
         const bool simple = dense.n == 1 && c1 - c0 <= 2 * W;
@@ -767,7 +771,13 @@ __global__ __launch_bounds__(256) void k_dense_sums(
 #pragma unroll
             for (int j = 0; j < 8; ++j)
                 if (j < m) sj[j] = run_sum<W>(sj[j]);
-            if (sub == 0) {
+            if (own) {
+                if (sub == t) {
+                    myu = u;
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) mys[j] = sj[j];
+                }
+            } else if (sub == 0) {
                 U[g * Ld + t] = u;
                 double *a = acc + (sg * Ld + t) * np;
                 a[0] += u - u * u / Ng[g];
@@ -775,6 +785,14 @@ __global__ __launch_bounds__(256) void k_dense_sums(
                 for (int j = 0; j < 8; ++j)
                     if (j < m) a[1 + j] += sj[j] - u * vbar[g * m + j];
             }
+        }
+        if (own && sub < Ld) {
+            U[g * Ld + sub] = myu;
+            double *a = acc + (sg * Ld + sub) * np;
+            a[0] += myu - myu * myu / Ng[g];
+#pragma unroll
+            for (int j = 0; j < 8; ++j)
+                if (j < m) a[1 + j] += mys[j] - myu * vbar[g * m + j];
         }
     }
     __syncthreads();

@@ -790,54 +790,83 @@ namespace hdfe {
 // Blocked layout of the non-split rows (see k_spmm_blk); with slot (the
 // dense-long slot of each column, -1 for the others), a row's entries at the
 // split rows' columns are left out of the blocks.
-__global__ void k_blk_rowcount(int64_t L, const int *__restrict__ indptr,
-                               const int *__restrict__ indices, const int *__restrict__ col_lo,
-                               const int *__restrict__ slot, int *__restrict__ cnt) {
-    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < L;
-         r += (int64_t)gridDim.x * blockDim.x) {
-        int c[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+// The blocked layout's counts and fill, a wave per row: lanes take the row's
+// entries 64 at a time (coalesced), each entry's column block found from
+// col_lo; per block a ballot ranks the wave's entries in it.  The fill puts
+// each entry at its block's position for the row plus the block's entries on
+// lower lanes and in earlier passes, so every block keeps the row's order.
+// Split rows (over kLongThresh nonzeros) are left out; entries at dense long
+// columns (slot >= 0) go to dt instead.
+__device__ inline int blk_of(int col, const int *__restrict__ col_lo) {
+    int b = 0;
+    while (b < 7 && col >= col_lo[b + 1]) ++b;
+    return b;
+}
+
+__global__ __launch_bounds__(kThreads) void k_blk_rowcount(
+    int64_t L, const int *__restrict__ indptr, const int *__restrict__ indices,
+    const int *__restrict__ col_lo, const int *__restrict__ slot, int *__restrict__ cnt) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int64_t nwave = (int64_t)gridDim.x * (blockDim.x / kWave);
+    for (int64_t r = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / kWave; r < L; r += nwave) {
         const int s = indptr[r], e = indptr[r + 1];
+        int c = 0;  // lane b < 8: block b's entries
         if (e - s <= kLongThresh) {
-            int b = 0;
-            for (int p = s; p < e; ++p) {
-                const int col = indices[p];
-                while (b < 7 && col >= col_lo[b + 1]) ++b;
-                if (!slot || slot[col] < 0) ++c[b];
+            for (int p0 = s; p0 < e; p0 += kWave) {
+                const int p = p0 + lane;
+                int b = -1;
+                if (p < e) {
+                    const int col = indices[p];
+                    if (!slot || slot[col] < 0) b = blk_of(col, col_lo);
+                }
+#pragma unroll
+                for (int bb = 0; bb < 8; ++bb) {
+                    const uint64_t m = __ballot(b == bb);
+                    if (lane == bb) c += __popcll(m);
+                }
             }
         }
-#pragma unroll
-        for (int b = 0; b < 8; ++b) cnt[(int64_t)b * L + r] = c[b];
+        if (lane < 8) cnt[(int64_t)lane * L + r] = c;
     }
 }
 
-__global__ void k_blk_fill(int64_t L, const int *__restrict__ indptr,
-                           const int *__restrict__ indices, const double *__restrict__ v64,
-                           const float *__restrict__ v32, const int *__restrict__ col_lo,
-                           const int *__restrict__ bptr, int *__restrict__ bidx,
-                           double *__restrict__ b64, float *__restrict__ b32,
-                           const int *__restrict__ slot, double *__restrict__ dt64,
-                           float *__restrict__ dt32) {
-    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < L;
-         r += (int64_t)gridDim.x * blockDim.x) {
+__global__ __launch_bounds__(kThreads) void k_blk_fill(
+    int64_t L, const int *__restrict__ indptr, const int *__restrict__ indices,
+    const double *__restrict__ v64, const float *__restrict__ v32, const int *__restrict__ col_lo,
+    const int *__restrict__ bptr, int *__restrict__ bidx, double *__restrict__ b64,
+    float *__restrict__ b32, const int *__restrict__ slot, double *__restrict__ dt64,
+    float *__restrict__ dt32) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const uint64_t below = lane ? (~0ull >> (kWave - lane)) : 0ull;
+    const int64_t nwave = (int64_t)gridDim.x * (blockDim.x / kWave);
+    for (int64_t r = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / kWave; r < L; r += nwave) {
         const int s = indptr[r], e = indptr[r + 1];
         if (e - s > kLongThresh) continue;
-        int b = 0;
-        int q = bptr[r];
-        for (int p = s; p < e; ++p) {
-            const int col = indices[p];
-            while (b < 7 && col >= col_lo[b + 1]) {
-                ++b;
-                q = bptr[(int64_t)b * L + r];
+        int next = lane < 8 ? bptr[(int64_t)lane * L + r] : 0;  // lane b: block b's next slot
+        for (int p0 = s; p0 < e; p0 += kWave) {
+            const int p = p0 + lane;
+            int col = 0, b = -1, sl = -1;
+            if (p < e) {
+                col = indices[p];
+                sl = slot ? slot[col] : -1;
+                if (sl < 0) b = blk_of(col, col_lo);
             }
-            if (slot && slot[col] >= 0) {
-                dt64[(int64_t)slot[col] * L + r] = v64[p];
-                dt32[(int64_t)slot[col] * L + r] = v32[p];
-                continue;
+            int q = 0;
+#pragma unroll
+            for (int bb = 0; bb < 8; ++bb) {
+                const uint64_t m = __ballot(b == bb);
+                const int at = __shfl(next, bb);
+                if (b == bb) q = at + __popcll(m & below);
+                if (lane == bb) next += __popcll(m);
             }
-            bidx[q] = col;
-            b64[q] = v64[p];
-            b32[q] = v32[p];
-            ++q;
+            if (sl >= 0) {
+                dt64[(int64_t)sl * L + r] = v64[p];
+                dt32[(int64_t)sl * L + r] = v32[p];
+            } else if (b >= 0) {
+                bidx[q] = col;
+                b64[q] = v64[p];
+                b32[q] = v32[p];
+            }
         }
     }
 }
@@ -1112,7 +1141,8 @@ static int build_blocked(Matrix &M, const std::vector<int> &ip32) {
     DevBuf<int> cnt;
     DEV_ALLOC(cnt, 8 * L + 1, ms);
     HIP_TRY(hipMemsetAsync(cnt.p + 8 * L, 0, 4, ms));
-    hipLaunchKernelGGL(k_blk_rowcount, dim3(grid1(L)), dim3(kThreads), 0, ms, L, M.indptr,
+    const int gwave = (int)std::min<int64_t>((L + kThreads / kWave - 1) / (kThreads / kWave), 65536);
+    hipLaunchKernelGGL(k_blk_rowcount, dim3(gwave), dim3(kThreads), 0, ms, L, M.indptr,
                        M.indices, M.col_lo, M.dl ? (const int *)M.dl_slot : nullptr, cnt.p);
     HIP_TRY(palloc(&M.bptr, (8 * L + 1) * 4, ms));
     HIP_TRY(exclusive_sum(tmp, cnt.p, M.bptr, 8 * L + 1, ms));
@@ -1122,7 +1152,7 @@ static int build_blocked(Matrix &M, const std::vector<int> &ip32) {
     HIP_TRY(palloc(&M.bidx, std::max<int64_t>(M.bnnz, 1) * 4, ms));
     HIP_TRY(palloc(&M.bval64, std::max<int64_t>(M.bnnz, 1) * 8, ms));
     HIP_TRY(palloc(&M.bval32, std::max<int64_t>(M.bnnz, 1) * 4, ms));
-    hipLaunchKernelGGL(k_blk_fill, dim3(grid1(L)), dim3(kThreads), 0, ms, L, M.indptr, M.indices,
+    hipLaunchKernelGGL(k_blk_fill, dim3(gwave), dim3(kThreads), 0, ms, L, M.indptr, M.indices,
                        M.val64, M.val32, M.col_lo, M.bptr, M.bidx, M.bval64, M.bval32,
                        M.dl ? (const int *)M.dl_slot : nullptr, M.dt64, M.dt32);
     if (M.dl)

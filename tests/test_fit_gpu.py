@@ -181,3 +181,17 @@ def test_twelve_covariates(oracle_mod, weighted):
         d["w"] = rng.uniform(0.5, 2.0, n)
     print(f"fit with twelve covariates, weighted={weighted}", flush=True)
     _check_fit(oracle_mod, d, xs=xs, weights="w" if weighted else None)
+
+
+def test_level_values_contiguous_and_gapped(oracle_mod):
+    """Each dimension's level values: the sorted distinct ids of the rows
+    kept (worker ids filling a range, firm ids spaced by three, years with a
+    gap)."""
+    d = _panel_with_singletons(seed=13)
+    d["firm_id"] = d["firm_id"] * 3 + 7
+    d["year"] = np.where(d["year"] >= 4, d["year"] + 10, d["year"])
+    res, ref = _check_fit(oracle_mod, d)
+    keep = ref["p0"]["keep"]
+    for f in FE:
+        np.testing.assert_array_equal(res.fe_levels[f], np.unique(d[f][keep]))
+        assert res.fe_levels[f].dtype == np.int64
