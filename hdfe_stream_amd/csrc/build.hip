@@ -742,7 +742,43 @@ __global__ __launch_bounds__(256) void k_dense_sums(
                     if (j < m) sb[j] = isums[(int64_t)cb * m + j];
             }
         }
-        for (int t = 0; t < Ld; ++t) {
+        // distinct levels among the group's cells (one cell a worker-year):
+        // level t's sums are one cell's values, which lane t takes directly
+        // (the same bits as the lane sums, whose other terms are zeros)
+        bool fast = false;
+        if (own && simple) {
+            int bits = (ta >= 0 ? 1 << ta : 0) | (tb >= 0 ? 1 << tb : 0);
+            int cnt = (ta >= 0) + (tb >= 0);
+#pragma unroll
+            for (int o = W / 2; o > 0; o >>= 1) {
+                bits |= __shfl_xor(bits, o, W);
+                cnt += __shfl_xor(cnt, o, W);
+            }
+            fast = __popc(bits) == cnt;
+        }
+        if (fast) {
+            const int base = (int)(threadIdx.x & (kWave - 1)) - sub;
+            const uint64_t own_bits = (W == 64 ? ~0ull : ((1ull << W) - 1)) << base;
+            int srca = -1, srcb = -1;
+            for (int t = 0; t < Ld; ++t) {
+                const uint64_t ma = __ballot(ta == t) & own_bits, mb = __ballot(tb == t) & own_bits;
+                if (sub == t) {
+                    srca = ma ? __builtin_ctzll(ma) : -1;
+                    srcb = mb ? __builtin_ctzll(mb) : -1;
+                }
+            }
+            const int la = srca >= 0 ? srca : base, lb = srcb >= 0 ? srcb : base;
+            const double ua = __shfl(ina, la), ub = __shfl(inb, lb);
+            myu = srca >= 0 ? ua : (srcb >= 0 ? ub : 0.0);
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                if (j < m) {
+                    const double va = __shfl(sa[j], la), vb = __shfl(sb[j], lb);
+                    mys[j] = srca >= 0 ? va : (srcb >= 0 ? vb : 0.0);
+                }
+            }
+        }
+        for (int t = 0; t < Ld && !fast; ++t) {
             double u = 0.0, sj[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
             if (simple) {
                 // the cells in order, as the general loop adds them
