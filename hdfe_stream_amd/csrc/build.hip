@@ -952,6 +952,23 @@ __global__ void k_inc_reduce(int64_t E, int m, int lbits, const uint64_t *__rest
         ig[q] = (int)(k >> lbits);
         il[q] = (int)(k & ((1ull << lbits) - 1));
         double u = 0.0;
+        if (m <= 8) {
+            // one pass over the run, the m sums in registers (each still
+            // added in run order)
+            double v[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+            for (int t = s; t < e; ++t) {
+                const int64_t c = cell[t];
+                u += in[c];
+#pragma unroll
+                for (int j = 0; j < 8; ++j)
+                    if (j < m) v[j] += isums[c * m + j];
+            }
+            iu[q] = u;
+#pragma unroll
+            for (int j = 0; j < 8; ++j)
+                if (j < m) is[q * m + j] = v[j];
+            continue;
+        }
         for (int t = s; t < e; ++t) u += in[cell[t]];
         iu[q] = u;
         for (int j = 0; j < m; ++j) {
@@ -1213,9 +1230,36 @@ __global__ void k_sd_inc(int64_t E, int D, int Ld, const int *__restrict__ ig,
         const int g = ig[q];
         const int da = lvl_dim[a];
         const int la = a - (int)offs[da];
-        double *row = Cq + r * Ld;
         const double f = iu[q] / Ng[g];
         uint64_t bits = 0;
+        if (Ld <= 8) {
+            // the row in registers (the same terms in the same order)
+            double rr[8];
+#pragma unroll
+            for (int t = 0; t < 8; ++t) {
+                const double ut = t < Ld ? U[(int64_t)g * Ld + t] : 0.0;
+                rr[t] = -f * ut;
+                if (ut != 0.0) bits |= 1ull << t;
+            }
+            for (int c = igstart[g]; c < igstart[g + 1]; ++c) {
+                if (icodes[da * ild + c] != la) continue;
+                for (int d = 0; d < D; ++d) {
+                    if (sparse_dim[d]) continue;
+                    const int tc = dense_idx[offs[d] + icodes[d * ild + c]];
+                    const double v = in[c];
+#pragma unroll
+                    for (int t = 0; t < 8; ++t)
+                        if (tc == t) rr[t] += v;
+                }
+            }
+            double *row = Cq + r * Ld;
+#pragma unroll
+            for (int t = 0; t < 8; ++t)
+                if (t < Ld) row[t] = rr[t];
+            Fq[r] = bits;
+            continue;
+        }
+        double *row = Cq + r * Ld;
         for (int t = 0; t < Ld; ++t) {
             const double ut = U[(int64_t)g * Ld + t];
             row[t] = -f * ut;
@@ -1734,9 +1778,9 @@ __global__ __launch_bounds__(T256) void k_assemble(
     const int64_t g0 = part_begin(Gi, blockIdx.x, gridDim.x), g1 = part_begin(Gi, blockIdx.x + 1, gridDim.x);
     for (int64_t g = g0 + threadIdx.x / W; g < g1; g += blockDim.x / W) {
         const int s = igstart[g], e = igstart[g + 1];
-        double qg[MM], N = 0.0;
+        double qg[MM], v0[MM], N = 0.0;
 #pragma unroll
-        for (int j = 0; j < MM; ++j) qg[j] = 0.0;
+        for (int j = 0; j < MM; ++j) qg[j] = v0[j] = 0.0;
         for (int c = s + sub; c < e; c += W) {
             const double ic = in[c];
             N += ic;
@@ -1744,19 +1788,26 @@ __global__ __launch_bounds__(T256) void k_assemble(
             for (int j = 0; j < MM; ++j) {
                 double v = isums[(int64_t)c * MM + j] / ic;
                 for (int d = 0; d < D; ++d) v -= Gm[(offs[d] + icodes[d * ild + c]) * MM + j];
+                if (c == s + sub) v0[j] = v;
                 qg[j] += ic * v;
             }
         }
         N = run_sum<W>(N);
 #pragma unroll
         for (int j = 0; j < MM; ++j) qg[j] = run_sum<W>(qg[j]) / N;
+        // a lane's first cell: its values kept from the pass above
         for (int c = s + sub; c < e; c += W) {
             const double ic = in[c];
             double q[MM];
 #pragma unroll
             for (int j = 0; j < MM; ++j) {
-                double v = isums[(int64_t)c * MM + j] / ic;
-                for (int d = 0; d < D; ++d) v -= Gm[(offs[d] + icodes[d * ild + c]) * MM + j];
+                double v;
+                if (c == s + sub) {
+                    v = v0[j];
+                } else {
+                    v = isums[(int64_t)c * MM + j] / ic;
+                    for (int d = 0; d < D; ++d) v -= Gm[(offs[d] + icodes[d * ild + c]) * MM + j];
+                }
                 q[j] = v - qg[j];
             }
 #pragma unroll
