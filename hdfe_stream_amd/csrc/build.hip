@@ -401,6 +401,8 @@ __global__ __launch_bounds__(256) void k_cells_m(int64_t C, const int *__restric
         crun[c] = (double)(e - s);
 #pragma unroll
         for (int j = 0; j < M; ++j) csums[c * M + j] = sm[j];
+        // an unweighted one-row cell: every deviation is exactly zero
+        if (!w && e - s == 1) continue;
         for (int i = s; i < e; ++i) {
             const double wi = w ? w[i] : 1.0;
             double dv[M];
@@ -956,7 +958,25 @@ __global__ void k_inc_reduce(int64_t E, int m, int lbits, const uint64_t *__rest
             // one pass over the run, the m sums in registers (each still
             // added in run order)
             double v[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-            for (int t = s; t < e; ++t) {
+            int t = s;
+            for (; t + 1 < e; t += 2) {  // two cells' loads in flight
+                const int64_t c0 = cell[t], c1 = cell[t + 1];
+                const double n0 = in[c0], n1 = in[c1];
+                double s0[8], s1[8];
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    s0[j] = j < m ? isums[c0 * m + j] : 0.0;
+                    s1[j] = j < m ? isums[c1 * m + j] : 0.0;
+                }
+                u += n0;
+                u += n1;
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    v[j] += s0[j];
+                    v[j] += s1[j];
+                }
+            }
+            if (t < e) {
                 const int64_t c = cell[t];
                 u += in[c];
 #pragma unroll
@@ -1337,7 +1357,28 @@ __global__ __launch_bounds__(256) void k_dd(int64_t Gi, int Ld, const int *__res
             const int b1 = dense_levels[t1], b2 = dense_levels[t2];
             const int d1 = lvl_dim[b1], d2 = lvl_dim[b2];
             const int l1 = b1 - (int)offs[d1], l2 = b2 - (int)offs[d2];
-            for (int64_t g = g0 + sub; g < g1; g += nsub) {
+            int64_t g = g0 + sub;
+            if (d1 == d2) {
+                // eight groups' loads issued together, added in group order
+                for (; g + 7 * nsub < g1; g += 8 * nsub) {
+                    double a[8], b[8], n[8];
+#pragma unroll
+                    for (int i = 0; i < 8; ++i) {
+                        const int64_t gi = g + i * nsub;
+                        a[i] = U[gi * Ld + t1];
+                        b[i] = U[gi * Ld + t2];
+                        n[i] = Ng[gi];
+                    }
+#pragma unroll
+                    for (int i = 0; i < 8; ++i) {
+                        if (a[i] == 0.0 || b[i] == 0.0) continue;
+                        pres = 1;
+                        const double nat = (t1 == t2) ? a[i] : 0.0;
+                        v += nat - a[i] * b[i] / n[i];
+                    }
+                }
+            }
+            for (; g < g1; g += nsub) {
                 const double u1 = U[g * Ld + t1], u2 = U[g * Ld + t2];
                 if (u1 == 0.0 || u2 == 0.0) continue;
                 pres = 1;
