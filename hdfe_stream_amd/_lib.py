@@ -45,6 +45,8 @@ SIGNATURES = [
     ("hdfe_panel_create", I32, [I64, I32, P, I32, P, P, I32, I32, I32, I32, P]),
     ("hdfe_panel_info", I32, [P, P, P, P]),
     ("hdfe_panel_array", I32, [P, C.c_char_p, P, P]),
+    ("hdfe_panel_copy_async", I32, [P, P, P, I64]),
+    ("hdfe_panel_copy_wait", I32, [P]),
     ("hdfe_panel_system", I32, [P, I32, P, P]),
     ("hdfe_panel_components", I32, [P, P]),
     ("hdfe_panel_normalize", I32, [P, P, I32, P]),

@@ -1581,19 +1581,27 @@ __global__ void k_uf_init(int64_t n, int *parent) {
         parent[i] = (int)i;
 }
 
+// One pass of unions over the groups: a group's cells (W lanes per group)
+// join their level to the first cell's.  Each union walks both roots and
+// hooks the larger root under the smaller with a CAS, retrying from the
+// values a failed CAS returns until both walks meet, so when the pass ends
+// every group's levels share a root, and each set's root is its smallest
+// level whatever the order the unions ran in.  Every parent pointer, however
+// stale the copy a lane reads, points within its set, so a walk that meets
+// proves the union.  A level equal to the cell before it is skipped (a
+// group's cells repeat its levels, one cell per level of every other
+// dimension).
+template <int W>
 __global__ void k_uf_union(int64_t Gi, const int *__restrict__ igstart,
-                           const int *__restrict__ icodes, int *parent, int *changed) {
-    for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < Gi;
-         g += (int64_t)gridDim.x * blockDim.x) {
+                           const int *__restrict__ icodes, int *parent) {
+    const int sub = threadIdx.x % W;
+    for (int64_t g = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / W; g < Gi;
+         g += (int64_t)gridDim.x * blockDim.x / W) {
         const int s = igstart[g], e = igstart[g + 1];
         const int a0 = icodes[s];
-        int prev = a0;
-        for (int c = s + 1; c < e; ++c) {
+        for (int c = s + 1 + sub; c < e; c += W) {
             const int lc = icodes[c];
-            // a group's cells repeat its levels (one cell per level of every
-            // other dimension): a level just joined needs no second union
-            if (lc == prev || lc == a0) continue;
-            prev = lc;
+            if (lc == a0 || lc == icodes[c - 1]) continue;
             int a = uf_root(parent, a0), b = uf_root(parent, lc);
             while (a != b) {
                 if (a > b) {
@@ -1601,12 +1609,8 @@ __global__ void k_uf_union(int64_t Gi, const int *__restrict__ igstart,
                     a = b;
                     b = t;
                 }
-                // hook the larger root under the smaller one
                 int old = atomicCAS(&parent[b], b, a);
-                if (old == b) {
-                    *changed = 1;
-                    break;
-                }
+                if (old == b) break;
                 b = uf_root(parent, old);
                 a = uf_root(parent, a);
             }
@@ -2062,6 +2066,8 @@ __global__ void k_gam_y(int64_t L, int m, const double *__restrict__ Gm,
 struct Panel {
     hipStream_t s = nullptr;
     int dev = 0;  // the device of s
+    hipStream_t side = nullptr;  // copies to the host beside the panel's work
+    hipEvent_t side_ev = nullptr;
     Temp tmp;
     int64_t n_in = 0, n = 0;  // rows given / kept
     int nfe = 0, D = 0, m = 0, gdim = 0;
@@ -2657,6 +2663,30 @@ int hdfe_panel_info(void *h, int64_t *info, int64_t *nlev, int *dims) {
 }
 
 // Device pointers of the panel's arrays, by name.
+// A copy of `bytes` of device memory at src (a panel array) into page-locked
+// host memory at dst, on a stream of its own, after the work queued on the
+// panel's stream so far: it overlaps the panel's later kernels.
+int hdfe_panel_copy_async(void *h, const void *src, void *dst, int64_t bytes) {
+    ARG_CHECK(h && src && dst && bytes >= 0, "hdfe_panel_copy_async: bad argument");
+    Panel &P = *(Panel *)h;
+    if (!P.side) {
+        HIP_TRY(stream_get(&P.side));
+        HIP_TRY(hipEventCreateWithFlags(&P.side_ev, hipEventDisableTiming));
+    }
+    HIP_TRY(hipEventRecord(P.side_ev, P.s));
+    HIP_TRY(hipStreamWaitEvent(P.side, P.side_ev, 0));
+    if (bytes) HIP_TRY(hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyDeviceToHost, P.side));
+    return HDFE_STATUS_OK;
+}
+
+// Waits for the panel's copies to the host.
+int hdfe_panel_copy_wait(void *h) {
+    ARG_CHECK(h, "hdfe_panel_copy_wait: null handle");
+    Panel &P = *(Panel *)h;
+    if (P.side) HIP_TRY(hipStreamSynchronize(P.side));
+    return HDFE_STATUS_OK;
+}
+
 int hdfe_panel_array(void *h, const char *name, void **ptr, int64_t *count) {
     ARG_CHECK(h && name && ptr, "hdfe_panel_array: null argument");
     Panel &P = *(Panel *)h;
@@ -3068,21 +3098,19 @@ int hdfe_panel_components(void *h, int64_t *ncomp) {
     hipStream_t s = P.s;
     ARG_CHECK(P.D >= 1, "hdfe_panel_components: no non-streamed dimension");
     const int64_t nl = P.nlev[0];
-    DevBuf<int> parent, changed, is_root, rpos;
+    DevBuf<int> parent, is_root, rpos;
     DEV_ALLOC(parent, nl, s);
-    DEV_ALLOC(changed, 1, s);
     DEV_ALLOC(is_root, nl + 1, s);
     DEV_ALLOC(rpos, nl + 1, s);
     hipLaunchKernelGGL(k_uf_init, dim3(grid1(nl)), dim3(T256), 0, s, nl, parent.p);
-    for (int it = 0; it < 64; ++it) {
-        int ch = 0;
-        HIP_TRY(hipMemsetAsync(changed.p, 0, 4, s));
-        hipLaunchKernelGGL(k_uf_union, dim3(grid1(P.Gi)), dim3(T256), 0, s, P.Gi, P.igstart.p,
-                           P.icodes.p, parent.p, changed.p);
-        hipLaunchKernelGGL(k_uf_flatten, dim3(grid1(nl)), dim3(T256), 0, s, nl, parent.p);
-        HIP_TRY(scalar_to_host(&ch, changed.p, s));
-        if (!ch) break;
-    }
+    // one pass joins every group's levels (see k_uf_union)
+    const int uw = run_width(P.M, P.Gi);
+#define HDFE_L(W)                                                                          \
+    hipLaunchKernelGGL(k_uf_union<W>, dim3(grid1(P.Gi * W)), dim3(T256), 0, s, P.Gi,        \
+                       P.igstart.p, P.icodes.p, parent.p)
+    HDFE_BY_WIDTH(uw, HDFE_L)
+#undef HDFE_L
+    hipLaunchKernelGGL(k_uf_flatten, dim3(grid1(nl)), dim3(T256), 0, s, nl, parent.p);
     DevBuf<unsigned long long> rows;
     DEV_ALLOC(rows, nl, s);
     HIP_TRY(hipMemsetAsync(rows.p, 0, nl * 8, s));
@@ -3315,10 +3343,13 @@ void hdfe_panel_destroy(void *h) {
     // are freed on its stream, which must outlive them
     (void)hipDeviceSynchronize();
     if (P->matrix) hdfe_destroy(P->matrix);
-    hipStream_t s = P->s;
+    hipStream_t s = P->s, side = P->side;
+    hipEvent_t side_ev = P->side_ev;
     const int dev = P->dev;
     delete P;
     stream_put(s, dev);
+    stream_put(side, dev);
+    if (side_ev) (void)hipEventDestroy(side_ev);
 }
 
 }  // extern "C"

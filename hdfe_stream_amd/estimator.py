@@ -145,6 +145,23 @@ class Panel:
                        "hdfe_memcpy")
         return out
 
+    def array_async(self, name, dtype):
+        """A panel array's copy to page-locked host memory, started now on a
+        stream beside the panel's own (ready after copy_wait)."""
+        p, n = self.ptr(name)
+        out = _pinned(n, dtype) if n else np.empty(0, dtype)
+        # held with the panel, whose release waits for the device first, so
+        # the buffer outlives the copy however the fit ends
+        self._keep.append(out)
+        if n:
+            _lib.check(_lib.load().hdfe_panel_copy_async(self._h, C.c_void_p(p),
+                                                         out.ctypes.data, out.nbytes),
+                       "hdfe_panel_copy_async")
+        return out
+
+    def copy_wait(self):
+        _lib.check(_lib.load().hdfe_panel_copy_wait(self._h), "hdfe_panel_copy_wait")
+
     def tensor(self, name, dtype):
         """A copy of a panel array on the device (torch)."""
         import torch
@@ -304,6 +321,9 @@ class StreamingHDFE:
                       self.fixef_rm, -1 if self.stream is None else self.fe.index(self.stream),
                       sort_vars=self.keep_intermediates)
         info = {"solver": "explicit", "device": self.backend.name}
+        # the streamed dimension's level values go to the host beside the
+        # system build and the solve
+        glevels = panel.array_async("glevels", np.int64)
         t1 = time.time()
         s_secs = panel.system(with_rhs=True)
         info.update(S_nnz=panel.info["nnz"],
@@ -356,7 +376,8 @@ class StreamingHDFE:
         # effects per dimension, with their level values
         fe_order = [self.fe[panel.info["gdim"]]] + [self.fe[d] for d in panel.dims]
         gy = _host(gam_y)
-        levels = {fe_order[0]: panel.array("glevels", np.int64, pinned=True)}
+        panel.copy_wait()
+        levels = {fe_order[0]: glevels}
         values = {fe_order[0]: _host(geff)}
         for t, d in enumerate(fe_order[1:]):
             o, nl = int(panel.offs[t]), panel.n_levels[t]

@@ -139,6 +139,15 @@ int hdfe_panel_info(void *h, int64_t *info, int64_t *nlev, int *dims);
  * "comp", "offs"). */
 int hdfe_panel_array(void *h, const char *name, void **ptr, int64_t *count);
 
+/* Copies `bytes` of device memory at src (a panel array) into page-locked
+ * host memory at dst on a stream of the panel's own, after the work queued on
+ * the panel so far, so that it overlaps the panel's later kernels;
+ * hdfe_panel_copy_wait waits for every such copy.  No counterpart in the
+ * reference (its level values are host arrays throughout); the fit starts
+ * the streamed dimension's level values this way after pass 0. */
+int hdfe_panel_copy_async(void *h, const void *src, void *dst, int64_t bytes);
+int hdfe_panel_copy_wait(void *h);
+
 /* Step 2's system on the device: diag (core/kernels_base.py:86-110), the
  * right-hand side (:64-83) and the explicit S (core/solve.py:148-194,
  * core/kernels_base.py:192-358) with its solver handle in *matrix (owned by the
