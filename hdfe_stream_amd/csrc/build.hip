@@ -79,8 +79,14 @@ __global__ void k_minmax_partial(int64_t n, const int64_t *__restrict__ col,
                                  int64_t *__restrict__ part) {
     __shared__ int64_t lo[T256], hi[T256];
     int64_t a = INT64_MAX, b = INT64_MIN;
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t step = (int64_t)gridDim.x * blockDim.x;
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    for (; i + 3 * step < n; i += 4 * step) {  // four loads in flight
+        const int64_t v0 = col[i], v1 = col[i + step], v2 = col[i + 2 * step], v3 = col[i + 3 * step];
+        a = min(a, min(min(v0, v1), min(v2, v3)));
+        b = max(b, max(max(v0, v1), max(v2, v3)));
+    }
+    for (; i < n; i += step) {
         a = min(a, col[i]);
         b = max(b, col[i]);
     }
@@ -288,6 +294,22 @@ __global__ void k_colsum_rows(int64_t n, int m, const double *__restrict__ V,
                               double *__restrict__ part) {
     __shared__ double lds[T256];
     const int64_t r0 = part_begin(n, blockIdx.x, gridDim.x), r1 = part_begin(n, blockIdx.x + 1, gridDim.x);
+    if (m <= 8) {
+        // every column in one pass over the rows (each column's sums in the
+        // same order as a pass per column)
+        double sj[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        for (int64_t i = r0 + threadIdx.x; i < r1; i += blockDim.x)
+#pragma unroll
+            for (int j = 0; j < 8; ++j)
+                if (j < m) sj[j] += V[i * m + j];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            if (j >= m) break;
+            const double t = block_sum<double>(sj[j], lds);
+            if (threadIdx.x == 0) part[(int64_t)blockIdx.x * m + j] = t;
+        }
+        return;
+    }
     for (int j = 0; j < m; ++j) {
         double s = 0.0;
         for (int64_t i = r0 + threadIdx.x; i < r1; i += blockDim.x) s += V[i * m + j];
@@ -301,6 +323,28 @@ __global__ void k_colss_rows(int64_t n, int m, const double *__restrict__ V,
                              const double *__restrict__ mean, double *__restrict__ part) {
     __shared__ double lds[T256];
     const int64_t r0 = part_begin(n, blockIdx.x, gridDim.x), r1 = part_begin(n, blockIdx.x + 1, gridDim.x);
+    if (m <= 8) {
+        double mu[8], sj[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            mu[j] = j < m ? mean[j] : 0.0;
+            sj[j] = 0.0;
+        }
+        for (int64_t i = r0 + threadIdx.x; i < r1; i += blockDim.x)
+#pragma unroll
+            for (int j = 0; j < 8; ++j)
+                if (j < m) {
+                    const double d = V[i * m + j] - mu[j];
+                    sj[j] += d * d;
+                }
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            if (j >= m) break;
+            const double t = block_sum<double>(sj[j], lds);
+            if (threadIdx.x == 0) part[(int64_t)blockIdx.x * m + j] = t;
+        }
+        return;
+    }
     for (int j = 0; j < m; ++j) {
         double s = 0.0;
         for (int64_t i = r0 + threadIdx.x; i < r1; i += blockDim.x) {
@@ -2166,7 +2210,7 @@ static int seg_rank_sort(hipStream_t s, int64_t nseg, int mult, const int *start
 int factorize_col(hipStream_t s, Temp &tmp, const int64_t *col, int64_t n, DevBuf<int> &code,
                   DevBuf<int64_t> &levels, int64_t &nl) {
     // min / max for the key range
-    const int nb = 256;
+    const int nb = 1024;
     DevBuf<int64_t> part;
     DEV_ALLOC(part, 2 * nb, s);
     hipLaunchKernelGGL(k_minmax_partial, dim3(nb), dim3(T256), 0, s, n, col, part.p);
