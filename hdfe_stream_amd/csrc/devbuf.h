@@ -23,9 +23,11 @@ namespace hdfe {
 //
 // Buffers are kept by the library once allocated: a freed block waits in a
 // free list with an event recorded on the stream it was freed on, and a later
-// allocation that it fits (at most twice its size, same device) takes it,
-// making its own stream wait for that event unless it is the same stream.
-// Blocks are sized in powers of two up to 2 MB and in 2 MB steps above.  The
+// allocation that it fits (at most twice its size class, same device) takes
+// it, making its own stream wait for that event unless it is the same stream.
+// Blocks are sized in powers of two (from 256 bytes) up to 2 MB and in 2 MB
+// steps above; classes under 256 KB are cut 64 at a time from one hipMalloc
+// (a slab), so a cold start maps few allocations.  The
 // stream-ordered pool was measured spending milliseconds inside hipFreeAsync
 // for buffers of hundreds of megabytes, and 10-25 us per call for small ones
 // (about a hundred per fit); with the cache a repeated fit maps no memory and
@@ -38,6 +40,7 @@ struct BigCache {
         hipEvent_t ev = nullptr;
         hipStream_t freed_on = nullptr;
         int dev = 0;
+        bool slab = false;  // cut from a slab: never returned alone
     };
     std::mutex mu;
     std::multimap<size_t, Block> free_list;
@@ -53,12 +56,24 @@ inline hipError_t big_release_free(BigCache &c) {
     // caller holds c.mu
     if (c.free_list.empty()) return hipSuccess;
     hipError_t e = hipDeviceSynchronize();
-    for (auto &kv : c.free_list) {
-        (void)hipEventDestroy(kv.second.ev);
-        (void)hipFree(kv.second.p);
+    for (auto it = c.free_list.begin(); it != c.free_list.end();) {
+        if (it->second.slab) {
+            ++it;
+            continue;
+        }
+        if (it->second.ev) (void)hipEventDestroy(it->second.ev);
+        (void)hipFree(it->second.p);
+        it = c.free_list.erase(it);
     }
-    c.free_list.clear();
     return e;
+}
+
+inline size_t big_class(size_t bytes) {
+    const size_t grain = size_t(2) << 20;
+    if (bytes >= grain) return (bytes + grain - 1) / grain * grain;
+    size_t b = 256;
+    while (b < bytes) b <<= 1;
+    return b;
 }
 
 inline hipError_t big_alloc(void **p, size_t bytes, hipStream_t s) {
@@ -67,35 +82,39 @@ inline hipError_t big_alloc(void **p, size_t bytes, hipStream_t s) {
     if (e != hipSuccess) return e;
     BigCache &c = big_cache();
     std::lock_guard<std::mutex> lock(c.mu);
-    for (auto it = c.free_list.lower_bound(bytes);
-         it != c.free_list.end() && it->first <= 2 * bytes; ++it) {
+    const size_t cls = big_class(bytes);
+    for (auto it = c.free_list.lower_bound(cls);
+         it != c.free_list.end() && it->first <= 2 * cls; ++it) {
         if (it->second.dev != dev) continue;
         BigCache::Block b = it->second;
         c.free_list.erase(it);
-        if (b.freed_on != s && (e = hipStreamWaitEvent(s, b.ev, 0)) != hipSuccess) return e;
+        // a block never freed yet has no event: nothing to wait for
+        if (b.ev && b.freed_on != s && (e = hipStreamWaitEvent(s, b.ev, 0)) != hipSuccess)
+            return e;
         c.live[b.p] = b;
         *p = b.p;
         return hipSuccess;
     }
-    const size_t grain = size_t(2) << 20;
     BigCache::Block b;
-    if (bytes >= grain) {
-        b.bytes = (bytes + grain - 1) / grain * grain;
-    } else {
-        b.bytes = 256;
-        while (b.bytes < bytes) b.bytes <<= 1;
-    }
+    b.bytes = cls;
     b.dev = dev;
-    e = hipMalloc(&b.p, b.bytes);
+    const bool slab = cls < (size_t(256) << 10);
+    const size_t total = slab ? 64 * cls : cls;
+    e = hipMalloc(&b.p, total);
     if (e == hipErrorOutOfMemory) {
         (void)hipGetLastError();
         if ((e = big_release_free(c)) != hipSuccess) return e;
-        e = hipMalloc(&b.p, b.bytes);
+        e = hipMalloc(&b.p, total);
     }
     if (e != hipSuccess) return e;
-    if ((e = hipEventCreateWithFlags(&b.ev, hipEventDisableTiming)) != hipSuccess) {
-        (void)hipFree(b.p);
-        return e;
+    if (slab) {
+        // the slab's first block is returned; the other 63 wait in the list
+        b.slab = true;
+        for (int q = 1; q < 64; ++q) {
+            BigCache::Block f = b;
+            f.p = (char *)b.p + (size_t)q * cls;
+            c.free_list.emplace(cls, f);
+        }
     }
     c.live[b.p] = b;
     *p = b.p;
@@ -103,7 +122,7 @@ inline hipError_t big_alloc(void **p, size_t bytes, hipStream_t s) {
 }
 
 // Returns a cached block to the free list, usable after the work queued on
-// s; false when p is not one (a pool allocation).
+// s; false when p is not the cache's.
 inline bool big_free(void *p, hipStream_t s) {
     BigCache &c = big_cache();
     std::lock_guard<std::mutex> lock(c.mu);
@@ -111,7 +130,14 @@ inline bool big_free(void *p, hipStream_t s) {
     if (it == c.live.end()) return false;
     BigCache::Block b = it->second;
     c.live.erase(it);
-    (void)hipEventRecord(b.ev, s);
+    if (!b.ev && hipEventCreateWithFlags(&b.ev, hipEventDisableTiming) != hipSuccess) {
+        // no event to order a later reuse by: wait for the block's work now
+        (void)hipGetLastError();
+        b.ev = nullptr;
+        (void)hipStreamSynchronize(s);
+    } else {
+        (void)hipEventRecord(b.ev, s);
+    }
     b.freed_on = s;
     c.free_list.emplace(b.bytes, b);
     return true;
