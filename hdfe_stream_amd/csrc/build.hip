@@ -611,6 +611,16 @@ __global__ void k_ident_cells(int64_t G, int D, int m, const int *__restrict__ g
     }
 }
 
+// Each cell's codes (those of its first row), D x C, dimension-major.
+__global__ void k_cell_codes(int64_t C, int D, const int *__restrict__ cstart,
+                             const int *__restrict__ codes, int64_t ld, int *__restrict__ icodes) {
+    for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < C;
+         c += (int64_t)gridDim.x * blockDim.x) {
+        const int r = cstart[c];
+        for (int d = 0; d < D; ++d) icodes[d * C + c] = codes[d * ld + r];
+    }
+}
+
 // The sparse dimensions (those whose levels are rows of S's sparse part).
 struct DimList {
     int d[8];
@@ -2481,10 +2491,25 @@ static int run_pass1(Panel &P) {
     }
     const int64_t M = P.M;
     DEV_ALLOC(P.icodes, std::max<int64_t>(1, (int64_t)D * M), s);
-    DEV_ALLOC(P.in, std::max<int64_t>(M, 1), s);
-    DEV_ALLOC(P.isums, std::max<int64_t>(M * m, 1), s);
     DEV_ALLOC(P.igstart, Gi + 1, s);
     DEV_ALLOC(P.igroup, std::max(Gi, 1), s);
+    if (Gi == ng && M == nc) {
+        // every group identifies: the identifying cells are the cells, whose
+        // weights and sums become the panel's as they are; only the codes
+        // at the cells' first rows are gathered
+        P.in = std::move(cn);
+        P.isums = std::move(csums);
+        hipLaunchKernelGGL(k_cell_codes, dim3(grid1(M)), dim3(T256), 0, s, M, D, P.cstart.p,
+                           P.codes.p, nk, P.icodes.p);
+        HIP_TRY(hipMemcpyAsync(P.igstart.p, P.cgstart.p, (size_t)(ng + 1) * 4,
+                               hipMemcpyDeviceToDevice, s));
+        hipLaunchKernelGGL(k_iota, dim3(grid1(Gi)), dim3(T256), 0, s, (int64_t)Gi, P.igroup.p);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipStreamSynchronize(s));
+        return HDFE_STATUS_OK;
+    }
+    DEV_ALLOC(P.in, std::max<int64_t>(M, 1), s);
+    DEV_ALLOC(P.isums, std::max<int64_t>(M * m, 1), s);
 #define HDFE_L(W)                                                                                \
     hipLaunchKernelGGL(k_ident_cells<W>, dim3(grid1((int64_t)ng * W)), dim3(T256), 0, s,        \
                        (int64_t)ng, D, m, P.cgstart.p, ident.p, gpos.p, cpos.p, P.cstart.p,     \
