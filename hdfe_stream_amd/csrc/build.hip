@@ -3152,7 +3152,13 @@ int hdfe_panel_system(void *h, int with_rhs, void **matrix, double *seconds) {
         P.matrix = nullptr;
     }
     void *mat = nullptr;
-    TRY(matrix_create(L, nnz, ip, indices.p, data.p, dinv.p, true, &mat));
+    // the matrix takes over the row pointers, indices and values (the copy
+    // to the host above ordered every write to them)
+    TRY(matrix_create(L, nnz, ip, indices.p, data.p, dinv.p, true, &mat, indptr.p, indices.p,
+                      data.p));
+    indptr.p = nullptr;
+    indices.p = nullptr;
+    data.p = nullptr;
     P.matrix = mat;
     if (matrix) *matrix = mat;
     if (seconds) *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();

@@ -1302,6 +1302,15 @@ static int run_cg(Matrix &M, int k, const T *val, const T *dinv, T *X, T *R, T *
     int host[2] = {0, 0};
     int batch = 8;
     int64_t done_before = 0;  // iterations launched before this batch
+    // Eager batches are sized from the convergence so far: the worst
+    // column's relative residual at the last two polls (1 before the first
+    // iteration) gives a rate per iteration, and the next batch is the
+    // iterations that rate needs to reach the tolerance, plus one.  Every
+    // iteration after the stop returns at once but still costs its launches,
+    // so this trims the overshoot that doubling leaves.  Graph batches keep
+    // the doubling sizes, whose graphs are reused.
+    std::vector<double> relh(k, 0.0);
+    double rel_prev = 1.0;
     for (int64_t launched = 0; launched < st.maxiter;) {
         const int nb = (int)std::min<int64_t>(batch, st.maxiter - launched);
         if (M.timing)
@@ -1343,6 +1352,8 @@ static int run_cg(Matrix &M, int k, const T *val, const T *dinv, T *X, T *R, T *
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipMemcpyAsync(host, st.done, sizeof(int), hipMemcpyDeviceToHost, s));
         HIP_TRY(hipMemcpyAsync(host + 1, st.it_final, sizeof(int), hipMemcpyDeviceToHost, s));
+        if (!use_graphs)
+            HIP_TRY(hipMemcpyAsync(relh.data(), st.rel, k * sizeof(double), hipMemcpyDeviceToHost, s));
         HIP_TRY(hipStreamSynchronize(s));
         if (M.timing) {
             // iterations that ran (the ones after the stop return at once)
@@ -1355,7 +1366,20 @@ static int run_cg(Matrix &M, int k, const T *val, const T *dinv, T *X, T *R, T *
             }
         }
         if (host[0]) break;
-        if (batch < 64) batch *= 2;
+        if (use_graphs) {
+            if (batch < 64) batch *= 2;
+        } else {
+            double r = 0.0;
+            for (int j = 0; j < k; ++j) r = std::max(r, relh[j]);
+            int next = std::min(64, 2 * batch);
+            if (r > 0.0 && r < rel_prev && st.tol > 0.0 && st.tol < r) {
+                const double rate = std::pow(r / rel_prev, 1.0 / nb);
+                const double need = std::log(st.tol / r) / std::log(rate);
+                if (std::isfinite(need)) next = (int)std::min(64.0, std::max(2.0, std::ceil(need) + 1.0));
+            }
+            rel_prev = r;
+            batch = next;
+        }
     }
     if (!host[0]) {  // maxiter reached without the flag (maxiter == 0)
         *it_out = 0;
@@ -1607,7 +1631,8 @@ static int solve_mixed(Matrix &M, int k, const uint8_t *zero_h, bool has_x0, dou
 // A device matrix from a host int32 indptr and host or device indices,
 // values and inverse diagonal.
 int matrix_create(int64_t L, int64_t nnz, const std::vector<int> &ip32, const int *indices,
-                  const double *data, const double *dinv, bool on_device, void **out) {
+                  const double *data, const double *dinv, bool on_device, void **out,
+                  int *own_indptr, int *own_indices, double *own_data) {
     std::unique_ptr<Matrix> M(new Matrix());
     HIP_TRY(hipGetDevice(&M->device));
     M->L = L;
@@ -1629,17 +1654,25 @@ int matrix_create(int64_t L, int64_t nnz, const std::vector<int> &ip32, const in
     HIP_TRY(stream_get(&M->stream));
     HIP_TRY(hipEventCreateWithFlags(&M->ev, hipEventDisableTiming));
     hipStream_t ms = M->stream;
-    HIP_TRY(palloc(&M->indptr, (L + 1) * 4, ms));
-    HIP_TRY(palloc(&M->indices, std::max<int64_t>(nnz, 1) * 4, ms));
-    HIP_TRY(palloc(&M->val64, std::max<int64_t>(nnz, 1) * 8, ms));
+    const bool adopt = own_indptr && own_indices && own_data;
+    if (adopt) {
+        M->indptr = own_indptr;
+        M->indices = own_indices;
+        M->val64 = own_data;
+    } else {
+        HIP_TRY(palloc(&M->indptr, (L + 1) * 4, ms));
+        HIP_TRY(palloc(&M->indices, std::max<int64_t>(nnz, 1) * 4, ms));
+        HIP_TRY(palloc(&M->val64, std::max<int64_t>(nnz, 1) * 8, ms));
+    }
     HIP_TRY(palloc(&M->val32, std::max<int64_t>(nnz, 1) * 4, ms));
     HIP_TRY(palloc(&M->dinv64, std::max<int64_t>(L, 1) * 8, ms));
     HIP_TRY(palloc(&M->dinv32, std::max<int64_t>(L, 1) * 4, ms));
     // every upload and the FP32 casts in order on the matrix's own stream (a
     // device-to-device hipMemcpy may return before the copy has finished)
-    HIP_TRY(hipMemcpyAsync(M->indptr, ip32.data(), (L + 1) * 4, hipMemcpyHostToDevice, ms));
+    if (!adopt)
+        HIP_TRY(hipMemcpyAsync(M->indptr, ip32.data(), (L + 1) * 4, hipMemcpyHostToDevice, ms));
     const hipMemcpyKind up = on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
-    if (nnz) {
+    if (nnz && !adopt) {
         HIP_TRY(hipMemcpyAsync(M->indices, indices, nnz * 4, up, ms));
         HIP_TRY(hipMemcpyAsync(M->val64, data, nnz * 8, up, ms));
     }
