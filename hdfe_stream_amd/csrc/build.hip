@@ -484,10 +484,12 @@ struct LevelDims {
     int D;
 };
 
-__global__ void k_level_tables(int64_t L, LevelDims ld, uint8_t *__restrict__ sparse_lv,
+__global__ void k_level_tables(int64_t L, LevelDims ld, uint8_t *__restrict__ sparse_dim,
+                               uint8_t *__restrict__ sparse_lv,
                                int *__restrict__ sparse_idx, int *__restrict__ dense_idx,
                                int *__restrict__ lvl_dim, int *__restrict__ sparse_levels,
                                int *__restrict__ dense_levels) {
+    if (blockIdx.x == 0 && (int)threadIdx.x < ld.D) sparse_dim[threadIdx.x] = (uint8_t)ld.sparse[threadIdx.x];
     for (int64_t a = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; a < L;
          a += (int64_t)gridDim.x * blockDim.x) {
         int t = 0;
@@ -2852,9 +2854,10 @@ int hdfe_panel_system(void *h, int with_rhs, void **matrix, double *seconds) {
     DEV_ALLOC(d_lvl_dim, L, s);
     DEV_ALLOC(d_sparse_levels, std::max<int64_t>(Ls, 1), s);
     DEV_ALLOC(d_dense_levels, std::max(Ld, 1), s);
-    HIP_TRY(hipMemcpyAsync(d_sparse_dim.p, sparse_dim.data(), D, hipMemcpyHostToDevice, s));
+    // (the dimensions' flags written by the kernel from its arguments: a
+    // pageable copy here would wait behind the level values' copy to the host)
     hipLaunchKernelGGL(k_level_tables, dim3(grid1(L)), dim3(T256), 0, s, (int64_t)L, ldims,
-                       d_sparse_lv.p, d_sparse_idx.p, d_dense_idx.p, d_lvl_dim.p,
+                       d_sparse_dim.p, d_sparse_lv.p, d_sparse_idx.p, d_dense_idx.p, d_lvl_dim.p,
                        d_sparse_levels.p, d_dense_levels.p);
 
     // incidences (group, level) of the identifying cells
