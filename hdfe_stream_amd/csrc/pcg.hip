@@ -52,6 +52,7 @@ constexpr int kSmallK = 8;         // k <= 8: lanes over nonzeros; wider: over c
 constexpr int64_t kL2Budget = 3 << 20;  // bytes of P an XCD's 4 MB L2 should hold (3 MB)
 constexpr int kBlkParts = 128;     // row parts per XCD of the blocked SpMM
 constexpr int kDenseLongMax = 16;  // split rows the blocked SpMM takes as dense columns
+constexpr int kCombineMax = 512;   // workgroups of the blocked SpMM's combine
 
 // Device-side solver state (one allocation; see State layout in Work).
 struct CgState {
@@ -1117,11 +1118,12 @@ static int build_blocked(Matrix &M, const std::vector<int> &ip32) {
         M.col_lo_h[b] = (int)c;
     }
     if (M.dl) {
-        // per k <= kSmallK, the combine's partials of split row t start at t * grid
+        // per k <= kSmallK, the combine's partials of split row t start at
+        // t * (the combine's workgroups)
         std::vector<int> cptr((int64_t)kSmallK * (M.nlong + 1));
         for (int k = 1; k <= kSmallK; ++k)
             for (int t = 0; t <= M.nlong; ++t)
-                cptr[(k - 1) * (M.nlong + 1) + t] = t * spmm_grid(L, k);
+                cptr[(k - 1) * (M.nlong + 1) + t] = t * std::min(spmm_grid(L, k), kCombineMax);
         HIP_TRY(palloc(&M.dl_cptr, cptr.size() * 4, ms));
         HIP_TRY(hipMemcpyAsync(M.dl_cptr, cptr.data(), cptr.size() * 4, hipMemcpyHostToDevice, ms));
         HIP_TRY(palloc(&M.dl_slot, L * 4, ms));
@@ -1195,6 +1197,9 @@ static void launch_spmm(Matrix &M, const T *val, const T *P, T *AP, double *part
         const T *dt = dl_values<T>(M), *ll = M.dl ? ll_values<T>(M) : nullptr;
         const int *lcp = M.dl ? M.dl_cptr + (k - 1) * (M.nlong + 1) : M.long_cptr;
         const int nchunk = M.dl ? 0 : M.nchunk;
+        // the combine in at most 512 workgroups (a few rows per thread): the
+        // one-workgroup finish then adds 512 partials per split row, not 2,048
+        const int gc = std::min(g, kCombineMax);
 #define HDFE_BLK(KK, GG)                                                                          \
     hipLaunchKernelGGL((k_spmm_blk<T, KK, GG>), dim3(nblk + nchunk), dim3(kThreads), 0, s,        \
                        M.bptr, M.bidx, bval, M.L, ngrp, nblk, P, gp, M.indptr, M.indices, val,   \
@@ -1204,11 +1209,11 @@ static void launch_spmm(Matrix &M, const T *val, const T *P, T *AP, double *part
         if (G == 1) HDFE_BLK(KK, 1);    \
         else if (G == 2) HDFE_BLK(KK, 2); \
         else HDFE_BLK(KK, 4);           \
-        hipLaunchKernelGGL((k_spmm_combine<T, KK>), dim3(g), dim3(kThreads), 0, s, M.L, ngrp, gp, \
+        hipLaunchKernelGGL((k_spmm_combine<T, KK>), dim3(gc), dim3(kThreads), 0, s, M.L, ngrp, gp, \
                            P, AP, part, with_dot, nld, dt, M.long_rows, lpart, st);             \
         if (with_dot || M.nlong)                                                                  \
             hipLaunchKernelGGL((k_fin_small<T, KK>), dim3(1), dim3(kFinThreads), 0, s, P, AP, part, \
-                               g, with_dot, M.long_rows, lcp, M.nlong, lpart, st, ll);            \
+                               gc, with_dot, M.long_rows, lcp, M.nlong, lpart, st, ll);           \
         break;
         switch (k) {
             HDFE_BLKG(1) HDFE_BLKG(2) HDFE_BLKG(3) HDFE_BLKG(4)
