@@ -1547,6 +1547,52 @@ __global__ void k_fill_sparse_rows(int64_t Ls, int Ld, const int *__restrict__ s
     }
 }
 
+// The same rows with W lanes per row: each entry's place is found, not
+// walked to.  A sparse entry lands after the row's sparse entries before it
+// and the present dense columns below its column; a dense entry after the
+// present dense columns before it and the sparse columns below it (a binary
+// search of the row).  Dense and sparse columns never coincide, so the
+// places are the merge's, and the sparse entries' stores are coalesced.
+template <int W>
+__global__ void k_fill_sparse_rows_w(int64_t Ls, int Ld, const int *__restrict__ sparse_levels,
+                                     const int *__restrict__ dense_levels,
+                                     const int *__restrict__ ss_rowptr,
+                                     const int *__restrict__ ss_col,
+                                     const double *__restrict__ ss_val,
+                                     const uint8_t *__restrict__ PP, const double *__restrict__ PD,
+                                     const int *__restrict__ indptr, int *__restrict__ indices,
+                                     double *__restrict__ data) {
+    const int sub = threadIdx.x % W;
+    for (int64_t s = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / W; s < Ls;
+         s += (int64_t)gridDim.x * blockDim.x / W) {
+        const int o0 = indptr[sparse_levels[s]];
+        const int p0 = ss_rowptr[s], pe = ss_rowptr[s + 1];
+        const uint8_t *pp = PP + s * Ld;
+        for (int p = p0 + sub; p < pe; p += W) {
+            const int col = ss_col[p];
+            int below = 0;
+            for (int t = 0; t < Ld; ++t) below += (pp[t] && dense_levels[t] < col) ? 1 : 0;
+            indices[o0 + (p - p0) + below] = col;
+            data[o0 + (p - p0) + below] = ss_val[p];
+        }
+        for (int t = sub; t < Ld; t += W) {
+            if (!pp[t]) continue;
+            const int dl = dense_levels[t];
+            int rank = 0;
+            for (int u = 0; u < t; ++u) rank += pp[u] ? 1 : 0;
+            int lo = p0, hi = pe;
+            while (lo < hi) {
+                const int mid = lo + (hi - lo) / 2;
+                if (ss_col[mid] < dl) lo = mid + 1;
+                else hi = mid;
+            }
+            const int o = o0 + rank + (lo - p0);
+            indices[o] = dl;
+            data[o] = PD[s * Ld + t];
+        }
+    }
+}
+
 // Dense rows, sparse columns: entry (t, a) at the rank of a among the
 // present sparse columns plus the present dense columns below a.
 __global__ void k_fill_dense_sparse(int64_t Ls, int Ld, const int *__restrict__ sparse_levels,
@@ -3135,9 +3181,20 @@ int hdfe_panel_system(void *h, int with_rhs, void **matrix, double *seconds) {
     DEV_ALLOC(indices, std::max(nnz, 1), s);
     DEV_ALLOC(data, std::max(nnz, 1), s);
     DEV_ALLOC(dinv, L, s);
-    hipLaunchKernelGGL(k_fill_sparse_rows, dim3(grid1(Ls)), dim3(T256), 0, s, Ls, Ld,
-                       d_sparse_levels.p, d_dense_levels.p, ss_rowptr.p, ss_col.p, ss_val.p, PP.p,
-                       PD.p, indptr.p, indices.p, data.p);
+    if (Ld <= 16 && Ls > 0) {
+        // W lanes per row, near the row's entries
+        const int rw = run_width(nnz, Ls);
+#define HDFE_L(W)                                                                              \
+    hipLaunchKernelGGL(k_fill_sparse_rows_w<W>, dim3(grid1(Ls * W)), dim3(T256), 0, s, Ls, Ld, \
+                       d_sparse_levels.p, d_dense_levels.p, ss_rowptr.p, ss_col.p, ss_val.p,    \
+                       PP.p, PD.p, indptr.p, indices.p, data.p)
+        HDFE_BY_WIDTH(rw, HDFE_L)
+#undef HDFE_L
+    } else {
+        hipLaunchKernelGGL(k_fill_sparse_rows, dim3(grid1(Ls)), dim3(T256), 0, s, Ls, Ld,
+                           d_sparse_levels.p, d_dense_levels.p, ss_rowptr.p, ss_col.p, ss_val.p,
+                           PP.p, PD.p, indptr.p, indices.p, data.p);
+    }
     if (Ld) {
         hipLaunchKernelGGL(k_fill_dense_sparse, dim3(grid1(Ls * Ld)), dim3(T256), 0, s, Ls, Ld,
                            d_sparse_levels.p, d_dense_levels.p, PP.p, PD.p, cpos.p, DP.p, indptr.p,
