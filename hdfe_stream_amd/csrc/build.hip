@@ -1275,7 +1275,24 @@ __global__ void k_reduce_runs(int64_t n, const uint64_t *__restrict__ key,
         if (!head[i]) continue;
         const uint64_t k = key[i];
         double s = val[i];
-        for (int64_t t = i + 1; t < n && key[t] == k; ++t) s += val[t];
+        int64_t t = i + 1;
+        // four keys and values loaded together, added in order up to the run's end
+        bool more = true;
+        while (more && t + 3 < n) {
+            const uint64_t k0 = key[t], k1 = key[t + 1], k2 = key[t + 2], k3 = key[t + 3];
+            const double v0 = val[t], v1 = val[t + 1], v2 = val[t + 2], v3 = val[t + 3];
+            if (k0 != k) { more = false; break; }
+            s += v0;
+            if (k1 != k) { more = false; break; }
+            s += v1;
+            if (k2 != k) { more = false; break; }
+            s += v2;
+            if (k3 != k) { more = false; break; }
+            s += v3;
+            t += 4;
+        }
+        if (more)
+            for (; t < n && key[t] == k; ++t) s += val[t];
         const int r = rank[i] - 1;
         row[r] = (int)(k >> lbits);
         col[r] = (int)(k & ((1ull << lbits) - 1));
@@ -1368,7 +1385,20 @@ __global__ void k_sd_sum(int64_t Ls, int Ld, const int *__restrict__ sparse_leve
         const int a = sparse_levels[s];
         double v = 0.0;
         uint64_t pres = 0;
-        for (int p = lptr[a]; p < lptr[a + 1]; ++p) {
+        int p = lptr[a];
+        const int pe = lptr[a + 1];
+        for (; p + 3 < pe; p += 4) {  // four incidences' loads in flight, added in order
+            const int64_t r0 = sidx[lq[p]], r1 = sidx[lq[p + 1]], r2 = sidx[lq[p + 2]],
+                          r3 = sidx[lq[p + 3]];
+            const double c0 = Cq[r0 * Ld + t], c1 = Cq[r1 * Ld + t], c2 = Cq[r2 * Ld + t],
+                         c3 = Cq[r3 * Ld + t];
+            pres |= Fq[r0] | Fq[r1] | Fq[r2] | Fq[r3];
+            v += c0;
+            v += c1;
+            v += c2;
+            v += c3;
+        }
+        for (; p < pe; ++p) {
             const int64_t r = sidx[lq[p]];
             v += Cq[r * Ld + t];
             pres |= Fq[r];
