@@ -130,6 +130,10 @@ def kss_phase(n_workers, n_firms, draws):
     out = {"workload": f"KSS leave-out y ~ 1 | worker_id + firm_id + year, "
                        f"{n_workers:,} workers x 8 years, {n_firms:,} firms, p_move 0.15, "
                        f"{draws} JLA + {draws} Hutchinson draws, mixed precision"}
+    # one untimed run first, as the fits have their warm-up: the first call in
+    # the process also pays its host allocations (page faults of fresh arrays)
+    leave_out_kss("y ~ 1 | worker_id + firm_id + year", cols, n_draws=draws, seed=0,
+                  block=draws, precision="mixed", leave_out="match")
     for level in ("match", "observation"):
         tm = {}
         torch.cuda.synchronize()
