@@ -1187,7 +1187,7 @@ static void launch_spmm(Matrix &M, const T *val, const T *P, T *AP, double *part
             }
         }
         const double per_row = (double)M.bnnz / ((double)std::max<int64_t>(M.L, 1) * ngrp);
-        const int G = per_row < 6.0 ? 1 : (per_row < 12.0 ? 2 : 4);
+        const int G = per_row < 4.0 ? 1 : (per_row < 12.0 ? 2 : 4);
         const int tuned = M.blk_parts[sizeof(T) == 4 ? 1 : 0][k];
         const int nblk = 8 * (force_parts ? force_parts : (tuned ? tuned : kBlkParts));
         const T *bval = blk_values<T>(M);
