@@ -2448,13 +2448,20 @@ static int run_pass1(Panel &P) {
     DEV_ALLOC(P.gstart, ng + 1, s);
     DEV_ALLOC(P.cstart, nc + 1, s);
     hipLaunchKernelGGL(k_starts, dim3(grid1(nk)), dim3(T256), 0, s, nk, gh.p, grank.p, P.gstart.p);
-    hipLaunchKernelGGL(k_starts, dim3(grid1(nk)), dim3(T256), 0, s, nk, ch.p, crank.p, P.cstart.p);
-    // group starts over cells: the cell rank of each group's first row
-    DevBuf<int> cghead, cgrank;
-    DEV_ALLOC(cghead, nc, s);
-    DEV_ALLOC(cgrank, nc, s);
     DEV_ALLOC(P.cgstart, ng + 1, s);
-    {
+    if (nc == nk) {
+        // every row its own cell: the cells' starts are the rows, and the
+        // groups' starts over cells are their starts over rows
+        hipLaunchKernelGGL(k_iota, dim3(grid1(nk + 1)), dim3(T256), 0, s, nk + 1, P.cstart.p);
+        HIP_TRY(hipMemcpyAsync(P.cgstart.p, P.gstart.p, (size_t)(ng + 1) * 4,
+                               hipMemcpyDeviceToDevice, s));
+    } else {
+        hipLaunchKernelGGL(k_starts, dim3(grid1(nk)), dim3(T256), 0, s, nk, ch.p, crank.p,
+                           P.cstart.p);
+        // group starts over cells: the cell rank of each group's first row
+        DevBuf<int> cghead, cgrank;
+        DEV_ALLOC(cghead, nc, s);
+        DEV_ALLOC(cgrank, nc, s);
         // a cell starts a group when its first row does
         hipLaunchKernelGGL(k_gather<int>, dim3(grid1(nc)), dim3(T256), 0, s, (int64_t)nc,
                            P.cstart.p, gh.p, cghead.p);
