@@ -17,6 +17,8 @@
 // result is independent of scheduling.
 #include <algorithm>
 #include <cmath>
+#include <mutex>
+#include <string>
 #include <vector>
 
 #include "devbuf.h"
@@ -305,6 +307,34 @@ __global__ void k_tall_dot_finish(int nb, int P, const double *__restrict__ part
     }
 }
 
+// The allocation-free variant's partials: a file-scope device array, one copy
+// per GPU, held until the process ends.  Host code never passes `scratch`
+// itself to a kernel (on the host the name is a stand-in, not a device
+// address): it uses the address hipGetSymbolAddress returns, kept per device.
+constexpr int64_t kNoAllocScratch = 1 << 20;  // doubles (8 MiB)
+static __device__ __align__(256) double scratch[kNoAllocScratch];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static double *g_scratch_ptr[kMaxDevices];
+
+// This device's address of `scratch`, looked up once per device with that
+// device current.
+static int scratch_ptr(double **out) {
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    ARG_CHECK(dev >= 0 && dev < kMaxDevices, "hdfe_tall_dot_noalloc: device ordinal out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        ARG_CHECK(p != nullptr, "hdfe_tall_dot_noalloc: no device address for the scratch array");
+        g_scratch_ptr[dev] = static_cast<double *>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return HDFE_STATUS_OK;
+}
+
 extern "C" {
 
 // out (a, b) device, row major = A' diag(w) B for A (K, a), B (K, b) device
@@ -322,6 +352,33 @@ int hdfe_tall_dot(int64_t K, int a, int b, const double *A, int64_t lda, const d
     hipLaunchKernelGGL(k_tall_dot_finish, dim3(grid1(P)), dim3(256), 0, 0, nb, P, part.p, out);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipDeviceSynchronize());
+    return HDFE_STATUS_OK;
+}
+
+// hdfe_tall_dot without any allocation: the partials go to the device's
+// `scratch` array (nb * a * b doubles, at most kNoAllocScratch), the launches
+// to `stream` (NULL: the null stream), and nothing waits for them.  Same
+// partition and order of sums as hdfe_tall_dot, so the same result bit for
+// bit.  Calls that overlap on one GPU share its array and race.
+int hdfe_tall_dot_noalloc(int64_t K, int a, int b, const double *A, int64_t lda, const double *B,
+                          int64_t ldb, const double *w, double *out, void *stream) {
+    ARG_CHECK(K >= 0 && a >= 1 && b >= 1 && (int64_t)a * b <= (1 << 16) && A && B && out &&
+                  lda >= a && ldb >= b,
+              "hdfe_tall_dot_noalloc: bad argument");
+    const int P = a * b;
+    const int nb = (int)std::max<int64_t>(1, std::min<int64_t>(kSeBlocks, (K + 4095) / 4096));
+    if ((int64_t)nb * P > kNoAllocScratch) {
+        set_error("hdfe_tall_dot_noalloc: " + std::to_string(nb) + " partials of " +
+                  std::to_string(P) + " pairs exceed the " + std::to_string(kNoAllocScratch) +
+                  "-double scratch array (use hdfe_tall_dot)");
+        return HDFE_STATUS_ARG;
+    }
+    double *part = nullptr;
+    TRY(scratch_ptr(&part));
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    hipLaunchKernelGGL(k_tall_dot, dim3(nb), dim3(256), 0, st, K, a, b, A, lda, B, ldb, w, part);
+    hipLaunchKernelGGL(k_tall_dot_finish, dim3(grid1(P)), dim3(256), 0, st, nb, P, part, out);
+    HIP_TRY(hipGetLastError());
     return HDFE_STATUS_OK;
 }
 

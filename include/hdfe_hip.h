@@ -360,6 +360,20 @@ void hdfe_components_destroy(void *h);
 int hdfe_tall_dot(int64_t K, int a, int b, const double *A, int64_t lda, const double *B,
                   int64_t ldb, const double *w, double *out);
 
+/* hdfe_tall_dot that allocates nothing.  No counterpart in the reference: it
+ * is for callers that cannot call hipMalloc (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid allocation).  The per-workgroup
+ * partials live in a file-scope device array of 1,048,576 doubles (8 MiB) that
+ * each GPU the library runs on holds until the process ends; a call whose
+ * partials (min(1024, ceil(K / 4096)) * a * b doubles) exceed it returns
+ * HDFE_STATUS_ARG.  Launches go to `stream` (NULL: the null stream) and the
+ * call does not wait for them; the result is the same as hdfe_tall_dot's, bit
+ * for bit.  Every call on one GPU shares that GPU's array: calls that overlap
+ * on one GPU race. */
+int hdfe_tall_dot_noalloc(int64_t K, int a, int b, const double *A, int64_t lda,
+                          const double *B, int64_t ldb, const double *w, double *out,
+                          void *stream);
+
 /* The first solve's stacked input of applying C: [v, b_k v] for a shared v
  * (n, s), or v_k = z sqrt(max(s_k, 0)) and [v_k, b_k v_k] per component from
  * draws z (n, s); out (n, (1 or 3) s + 3 s).  Replaces the stacking of
