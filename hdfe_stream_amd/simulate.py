@@ -16,7 +16,7 @@ from __future__ import annotations
 
 import numpy as np
 
-__all__ = ["akm_panel", "kss_panel", "weighted_panel", "bottleneck_panel", "poisson_models",
+__all__ = ["simulate_akm", "akm_panel", "kss_panel", "weighted_panel", "bottleneck_panel", "poisson_models",
            "poisson_models_device"]
 
 # Row-level corr(alpha, psi) is about 0.6 times the copula parameter with
@@ -70,6 +70,71 @@ def akm_panel(n_workers: int, n_firms: int, n_years: int = 8, p_move: float = 0.
     y = a + p + g + 0.3 * x1 - 0.2 * x2 + eps
     return {"worker_id": worker, "firm_id": firm, "year": year, "x1": x1, "x2": x2, "y": y,
             "alpha": a, "psi": p, "gamma": g}
+
+
+def simulate_akm(n_workers: int = 50_000, n_firms: int | None = None, years=range(2005, 2015),
+                 p_move: float = 0.08, p_obs: float = 0.85, sd_worker: float = 0.4,
+                 sd_firm: float = 0.25, sd_noise: float = 0.3, seed: int = 0,
+                 keep_effects: bool = False) -> dict:
+    """The reference's AKM panel (hdfe_stream/simulate.py:42-143,
+    `simulate_akm`) in numpy, draw for draw: the same generator calls in the
+    same order, so the same seed gives the same rows.  Kept alike on purpose:
+    the reference's benchmark coefficients (benchmarks/results/akm.csv) are
+    quoted on these panels.
+
+    Firms are ranked by effect; a worker's first firm and every mover's new
+    firm are, with probability 1/2 each, the firm at the worker's noisy rank
+    or a size-weighted (Pareto 1.2 sizes) draw; each year a worker moves with
+    p_move and is observed with p_obs; ages run from a birth year in
+    [22, 55).  log_earn = 10 + worker + firm + 0.08 age^2/100 - 0.012
+    age^3/1000 + N(0, sd_noise^2) noise drawn from seed + 1 over the rows in
+    year order.
+
+    Returns a dict of numpy columns in generation order (year by year, workers
+    ascending within a year); the reference then shuffles the rows with
+    Polars, which changes no estimate.  `firm_id` is the firm's index (int64;
+    the reference's string ids are "F" + this index), `worker_id` is
+    7 * worker + 1,000,000 as there."""
+    if n_firms is None:
+        n_firms = max(n_workers // 15, 50)
+    rng = np.random.default_rng(seed)
+    years = list(years)
+    worker_effect = rng.normal(0, sd_worker, n_workers)
+    firm_effect = rng.normal(0, sd_firm, n_firms)
+    ranked = np.argsort(firm_effect)
+    size = rng.pareto(1.2, n_firms) + 1
+    weights = size / size.sum()
+    birth = rng.integers(22, 55, n_workers)
+
+    def new_firm(effect):
+        pos = (effect / max(sd_worker, 1e-12) + rng.normal(0, 1.5, effect.size)) / 6 + 0.5
+        by_rank = ranked[(np.clip(pos, 0, 0.999) * n_firms).astype(int)]
+        by_size = rng.choice(n_firms, effect.size, p=weights)
+        return np.where(rng.random(effect.size) < 0.5, by_rank, by_size)
+
+    firm = new_firm(worker_effect)
+    cols = {k: [] for k in ("worker_id", "firm_id", "year", "age", "w", "f")}
+    for t, yr in enumerate(years):
+        if t:
+            firm = np.where(rng.random(n_workers) < p_move, new_firm(worker_effect), firm)
+        seen = np.flatnonzero(rng.random(n_workers) < p_obs)
+        cols["worker_id"].append(seen.astype(np.int64) * 7 + 1_000_000)
+        cols["firm_id"].append(firm[seen].astype(np.int64))
+        cols["year"].append(np.full(seen.size, yr, np.int64))
+        cols["age"].append((birth[seen] + t).astype(float))
+        cols["w"].append(worker_effect[seen])
+        cols["f"].append(firm_effect[firm[seen]])
+    out = {k: np.concatenate(v) for k, v in cols.items()}
+    noise = np.random.default_rng(seed + 1).normal(0, sd_noise, out["age"].size)
+    age = out["age"]
+    out["age_squared"] = age ** 2 / 100
+    out["age_cubed"] = age ** 3 / 1000
+    out["log_earn"] = (10 + out["w"] + out["f"] + 0.08 * out["age_squared"]
+                       - 0.012 * out["age_cubed"] + noise)
+    w, f = out.pop("w"), out.pop("f")
+    if keep_effects:
+        out["true_worker_effect"], out["true_firm_effect"] = w, f
+    return out
 
 
 def kss_panel(n_workers: int = 1_000_000, n_firms: int = 100_000, seed: int = 3,
