@@ -725,7 +725,8 @@ def leave_out_kss(data: dict, y: str = "y", xs=(), alpha: str = "worker_id",
 def leave_out_match(data: dict, y: str = "y", xs=(), alpha: str = "worker_id",
                     psi: str = "firm_id", others=(), n_draws: int = 250, block: int = 256,
                     seed: int = 0, tol: float = 1e-10, se: bool = False, se_draws=None,
-                    se_block: int = 16, se_variance: str = "person_year") -> dict:
+                    se_block: int = 16, se_variance: str = "person_year",
+                    exact: bool = False) -> dict:
     """KSS leaving out a whole match (leaveout_match.py:156-214, 63-132;
     leaveout.py:634-805 with stayers "within_match" and the reference
     centring): prune as leave_out_kss, fit the row model, collapse y~ = the
@@ -740,7 +741,10 @@ def leave_out_match(data: dict, y: str = "y", xs=(), alpha: str = "worker_id",
     sigma2 recomputed with weighted centring on movers' matches, per match
     the within-match variance plus that over T ("person_year") or that alone
     with the stayers' values ("match"), b zero on stayers' matches, and the
-    reference-centred kernel's rank-two term."""
+    reference-centred kernel's rank-two term.  With `exact` (small panels)
+    the leverages and the bias come from the dense pseudo-inverse of the
+    weighted match design instead of JLA and Hutchinson draws (the exact
+    counterpart of prototypes/kss_exact.py:398-486)."""
     a_ids, b_ids = np.asarray(data[alpha]), np.asarray(data[psi])
     ua, ia = np.unique(a_ids, return_inverse=True)
     ub, ib = np.unique(b_ids, return_inverse=True)
@@ -768,8 +772,13 @@ def leave_out_match(data: dict, y: str = "y", xs=(), alpha: str = "worker_id",
     mg, mpc = mp0["g"], mp0["codes"][:, 0]
     # the match fit keeps the table's (worker, firm) order
     assert np.array_equal(mg, g[first]) and np.array_equal(mpc, pc[first])
-    moments = jla_leverages(mp0, mc, mf["S"], mf["dinv"], n_draws, block, seed, tol)
-    _, M_bar, factor = combine_moments(moments, n_draws)
+    if exact:
+        hat = _exact_match_design(mg, mpc, mp0["w"], R)
+        moments = None
+        M_bar, factor = 1.0 - hat["P"], np.ones(nm)
+    else:
+        moments = jla_leverages(mp0, mc, mf["S"], mf["dinv"], n_draws, block, seed, tol)
+        _, M_bar, factor = combine_moments(moments, n_draws)
     P_bar = 1.0 - M_bar
     nf = np.bincount(g, minlength=int(g.max()) + 1)          # rows per worker
     firms = np.bincount(mg, minlength=int(g.max()) + 1)      # matches per worker
@@ -794,7 +803,12 @@ def leave_out_match(data: dict, y: str = "y", xs=(), alpha: str = "worker_id",
     den = np.bincount(mid[st_rows], weights=w_row[st_rows], minlength=nm)
     s2[~movers] = num[~movers] / den[~movers]
     ctx = KSSContext(mf, tol)
-    bias, totals = hutchinson_trace(ctx, s2, n_draws, block, seed + 1_000_003, 0, n_rows=R)
+    if exact:
+        bias = tuple(float((Tm * b_ii) @ s2) for b_ii in hat["B"])
+        totals = None
+    else:
+        bias, totals = hutchinson_trace(ctx, s2, n_draws, block, seed + 1_000_003, 0,
+                                        n_rows=R)
     del nf
     out = {"plug_in": plug, "bias": bias,
            "leave_out": tuple(p - q for p, q in zip(plug, bias)), "keep": keep,
@@ -818,6 +832,40 @@ def leave_out_match(data: dict, y: str = "y", xs=(), alpha: str = "worker_id",
                                         centering="reference", reported=(0, 2))
         out["se_sigma2"] = sig
     return out
+
+
+def _exact_match_design(worker, firm, T, person_years):
+    """Exact leverages and bias kernels of the weighted match regression
+    y_m = alpha_worker + psi_firm (weights T, the spells' lengths), densely:
+    P_m = T_m a_m' S^+ a_m with S = A'TA, and B_m = a_m' S^+ Q S^+ a_m for
+    the three person-year moments' Q (var(psi), var(alpha), cov), each the
+    T-weighted, T-centred second moment divided by person_years - 1
+    (prototypes/kss_exact.py:175-221, 398-486).  Both are invariant to the
+    choice of generalised inverse, since A's null direction (one on the
+    workers, minus one on the firms) is also Q's."""
+    nw, nfm = int(worker.max()) + 1, int(firm.max()) + 1
+    nm = len(worker)
+    A = np.zeros((nm, nw + nfm))
+    A[np.arange(nm), worker] = 1.0
+    A[np.arange(nm), nw + firm] = 1.0
+    S = A.T @ (T[:, None] * A)
+    Sp = np.linalg.pinv(S, hermitian=True)
+    AS = A @ Sp
+    P = T * np.einsum("ij,ij->i", AS, A)
+    total = T.sum()
+    norm = total * (person_years - 1) / person_years
+    Wc = A[:, :nw] - (T @ A[:, :nw]) / total
+    Jc = A[:, nw:] - (T @ A[:, nw:]) / total
+    K = nw + nfm
+    Qpsi = np.zeros((K, K))
+    Qpsi[nw:, nw:] = Jc.T @ (T[:, None] * Jc) / norm
+    Qalpha = np.zeros((K, K))
+    Qalpha[:nw, :nw] = Wc.T @ (T[:, None] * Wc) / norm
+    Qcov = np.zeros((K, K))
+    Qcov[nw:, :nw] = Jc.T @ (T[:, None] * Wc) / (2 * norm)
+    Qcov[:nw, nw:] = Qcov[nw:, :nw].T
+    B = [np.einsum("ij,jk,ik->i", AS, Q, AS) for Q in (Qpsi, Qalpha, Qcov)]
+    return {"P": P, "B": B}
 
 
 # ------------------------------------------------- differentiable fits

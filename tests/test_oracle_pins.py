@@ -47,3 +47,51 @@ def test_a_plausible_generator_slip_breaks_the_pin(oracle_mod):
     f = oracle_mod.fit(d, "log_earn", FML_X, FE3, tol=1e-10)
     want = np.array([FITS[25_000]["coef_age_squared"], FITS[25_000]["coef_age_cubed"]])
     assert np.max(np.abs(f["beta"] - want) / np.abs(want)) > 1e-6
+
+
+XHDFE = json.loads((Path(__file__).parent / "golden" / "xhdfe_match.json").read_text())
+
+
+def anchor_panel(demeaned: bool, oracle):
+    """The reference's anchor panel (tests/test_leaveout.py:2236-2244):
+    simulate_akm(200, 20, seed=11) pruned to its leave-one-out connected
+    set; the outcome raw or demeaned over the pruned rows."""
+    from hdfe_stream_amd.simulate import simulate_akm
+    d = simulate_akm(200, 20, seed=11)
+    ua, ia = np.unique(d["worker_id"], return_inverse=True)
+    ub, ib = np.unique(d["firm_id"], return_inverse=True)
+    alive, info = oracle.leave_one_out(ia, ib, len(ua), len(ub))
+    keep = alive[ia] & alive[len(ua) + ib]
+    y = d["log_earn"][keep]
+    return {"worker_id": d["worker_id"][keep], "firm_id": d["firm_id"][keep],
+            "y": y - y.mean() if demeaned else y}, info
+
+
+@pytest.mark.parametrize("outcome", ["raw", "demeaned"])
+def test_oracle_match_leave_out_reproduces_xhdfe(oracle_mod, outcome):
+    """The oracle's match-level leave-out with exact leverages gives xhdfe's
+    KSS var(psi) on the anchor panel to 1e-10: pruning, the collapse to
+    matches, the stayers' sigma, the reference centring and the person-year
+    divisor are all pinned by it."""
+    data, info = anchor_panel(outcome == "demeaned", oracle_mod)
+    assert info["components_before"] == 2
+    out = oracle_mod.leave_out_match(data, exact=True, tol=1e-13)
+    assert out["leave_out"][0] == pytest.approx(XHDFE["kss_var_psi"][outcome], abs=1e-10)
+
+
+def test_weighted_centring_would_miss_xhdfe(oracle_mod):
+    """The pin separates the two centrings: centring sigma2 at the weighted
+    mean of the match outcomes (not the reference's plain mean of sqrt(T)
+    ybar) moves the raw-outcome estimate far beyond 1e-10."""
+    data, _ = anchor_panel(False, oracle_mod)
+    out = oracle_mod.leave_out_match(data, exact=True, tol=1e-13)
+    mf, movers = out["match_fit"], out["movers"]
+    T = mf["p0"]["w"]
+    yv = mf["p0"]["V"][:, 0]
+    s2 = out["sigma2"].copy()
+    ybw = float(T @ yv / T.sum())
+    s2[movers] = (T * (yv - ybw) * mf["resid"] / out["M_bar"])[movers]
+    hat = oracle_mod._exact_match_design(mf["p0"]["g"], mf["p0"]["codes"][:, 0], T,
+                                         out["person_years"])
+    moved = out["plug_in"][0] - float((T * hat["B"][0]) @ s2)
+    assert abs(moved - XHDFE["kss_var_psi"]["raw"]) > 1e-6

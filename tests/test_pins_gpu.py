@@ -45,3 +45,27 @@ def test_device_fit_matches_the_reference_benchmark_coefficients(n_workers):
 def test_device_fit_at_five_million_workers_matches_the_reference():
     """The reference's 5M-worker benchmark panel (42,498,893 rows)."""
     _check(5_000_000)
+
+
+XHDFE = json.loads((Path(__file__).parent / "golden" / "xhdfe_match.json").read_text())
+
+
+@pytest.mark.parametrize("outcome", ["raw", "demeaned"])
+def test_device_match_leave_out_on_the_xhdfe_anchor_panel(oracle_mod, outcome):
+    """The device's match-level leave-out on the reference's anchor panel:
+    draw for draw equal to the oracle's (4,096 JLA and Hutchinson draws), and
+    within the draws' Monte Carlo error of xhdfe's exact KSS var(psi); the
+    oracle's exact-leverage version reproduces xhdfe to 1e-10 (CPU test)."""
+    from hdfe_stream_amd.leaveout import leave_out_kss
+    from test_oracle_pins import anchor_panel
+    data, _ = anchor_panel(outcome == "demeaned", oracle_mod)
+    print(f"xhdfe anchor panel, {outcome}", flush=True)
+    lo = leave_out_kss("y ~ 1 | worker_id + firm_id", data, n_draws=4096, seed=1,
+                       precision="fp64", tol=1e-12)
+    ref = oracle_mod.leave_out_match(data, n_draws=4096, block=256, seed=1, tol=1e-12)
+    names = ["var(psi)", "var(alpha)", "cov(psi, alpha)"]
+    got = np.array([lo.leave_out[k] for k in names])
+    assert np.max(np.abs(got - np.array(ref["leave_out"]))) < 1e-8 * np.max(np.abs(got))
+    exact = oracle_mod.leave_out_match(data, exact=True, tol=1e-13)
+    assert exact["leave_out"][0] == pytest.approx(XHDFE["kss_var_psi"][outcome], abs=1e-10)
+    assert abs(got[0] - XHDFE["kss_var_psi"][outcome]) < 0.004
