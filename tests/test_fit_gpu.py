@@ -64,15 +64,15 @@ def _check_fit(O, d, fe=FE, xs=("x1", "x2"), weights=None, precision="fp64", tol
         assert all(abs(a - b) <= 1 for a, b in zip(its, ref["iterations"]))
     assert res.solver_info["converged"]
     G = res.Gamma.cpu().numpy()
-    assert np.abs(G - ref["Gamma"]).max() <= 1e-7 * np.abs(ref["Gamma"]).max()
+    assert np.abs(G - ref["Gamma"]).max() <= 1e-8 * np.abs(ref["Gamma"]).max()
     assert rel(res.beta, ref["beta"]) < beta_tol
     assert rel(res.A, ref["A"]) < 1e-9
     fe_order = res.fe_names
     gy = np.concatenate([res.fe_values[f] for f in fe_order[1:]])
-    assert np.abs(gy - ref["gam_y"]).max() <= 1e-7 * max(np.abs(ref["gam_y"]).max(), 1)
+    assert np.abs(gy - ref["gam_y"]).max() <= 1e-8 * max(np.abs(ref["gam_y"]).max(), 1)
     np.testing.assert_allclose(res.fe_values[fe_order[0]], ref["worker_effects"], rtol=0,
-                               atol=1e-7)
-    np.testing.assert_allclose(res.resid(), ref["resid_source_order"], rtol=0, atol=1e-7)
+                               atol=1e-8)
+    np.testing.assert_allclose(res.resid(), ref["resid_source_order"], rtol=0, atol=1e-8)
     return res, ref
 
 

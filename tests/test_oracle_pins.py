@@ -95,3 +95,61 @@ def test_weighted_centring_would_miss_xhdfe(oracle_mod):
                                          out["person_years"])
     moved = out["plug_in"][0] - float((T * hat["B"][0]) @ s2)
     assert abs(moved - XHDFE["kss_var_psi"]["raw"]) > 1e-6
+
+
+def _two_way_design(n_workers, n_firms, n_years, p_move, seed):
+    from hdfe_stream_amd.simulate import akm_panel
+    d = akm_panel(n_workers, n_firms, n_years=n_years, p_move=p_move, seed=seed)
+    n = len(d["y"])
+    wi = np.unique(d["worker_id"], return_inverse=True)[1]
+    fi = np.unique(d["firm_id"], return_inverse=True)[1]
+    A = np.zeros((n, wi.max() + 1 + fi.max() + 1))
+    A[np.arange(n), wi] = 1.0
+    A[np.arange(n), wi.max() + 1 + fi] = 1.0
+    return A
+
+
+def test_nonlinearity_factor_removes_the_bias_of_one_over_m(oracle_mod):
+    """The JLA non-linearity factor F (leaveout.py:240-253): with p = 32
+    Rademacher draws, F_i / M_bar_i is an unbiased estimate of 1 / M_ii to
+    second order while 1 / M_bar_i is biased upwards.  On a 300-row two-way
+    design with exact M_ii, over 2,000 independent draw sets, the mean of
+    F / M_bar matches 1 / M_ii within its standard error (per row and
+    pooled over rows), the naive 1 / M_bar misses by dozens of standard
+    errors, and so does F with the sign of its variance term V or of its bias
+    term B flipped."""
+    O = oracle_mod
+    A = _two_way_design(100, 25, 3, 0.5, 2)
+    n = A.shape[0]
+    assert n <= 300
+    P = A @ np.linalg.pinv(A.T @ A) @ A.T
+    M_exact = 1.0 - np.diag(P)
+    assert 0.3 < M_exact.min() and M_exact.max() < 0.7
+    p, S = 32, 2000
+    rng = np.random.default_rng(0)
+    est = {k: np.empty((S, n)) for k in ("factor", "naive", "flip_V", "flip_B")}
+    for s in range(S):
+        r = rng.integers(0, 2, (n, p)) * 2.0 - 1.0
+        mom = np.zeros((5, n))
+        O.jla_moments(np.ascontiguousarray(r), np.ascontiguousarray(P @ r), mom)
+        P_bar, M_bar, F = O.combine_moments(mom, p)
+        # the mutants: V and B as the 2021 note defines them, one sign flipped
+        _, _, mPP, mMM, mPM = (mom[i] / p for i in range(5))
+        V = (M_bar ** 2 * mPP + P_bar ** 2 * mMM - 2 * P_bar * M_bar * mPM) / p
+        B = (M_bar * mPP - P_bar * mMM + (M_bar - P_bar) * mPM) / p
+        assert np.allclose(F, 1.0 - V / M_bar ** 2 + B / M_bar, rtol=1e-12, atol=1e-14)
+        est["factor"][s] = F / M_bar
+        est["naive"][s] = 1.0 / M_bar
+        est["flip_V"][s] = (1.0 + V / M_bar ** 2 + B / M_bar) / M_bar
+        est["flip_B"][s] = (1.0 - V / M_bar ** 2 - B / M_bar) / M_bar
+
+    def z_scores(e):
+        per_row = (e.mean(0) - 1.0 / M_exact) / (e.std(0, ddof=1) / np.sqrt(S))
+        pooled = (e * M_exact[None, :]).mean(1) - 1.0     # one number per draw set
+        return per_row, pooled.mean() / (pooled.std(ddof=1) / np.sqrt(S))
+
+    per_row, pooled = z_scores(est["factor"])
+    assert np.max(np.abs(per_row)) < 4.5 and abs(pooled) < 3.0, (per_row, pooled)
+    for mutant in ("naive", "flip_V", "flip_B"):
+        _, z = z_scores(est[mutant])
+        assert z > 8.0, (mutant, z)
