@@ -100,3 +100,27 @@ def test_separated_levels_are_dropped(oracle_mod):
     assert r.diagnostics["separation"]["levels"].get("worker_id", 0) >= 25
     assert r.n_obs == int(ref["keep"].sum())
     assert rel(r.beta, ref["beta"]) < 1e-7
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_every_model_meets_its_own_tolerance(oracle_mod, precision):
+    """A small model whose right-hand sides are a thousandth of a large
+    one's, batched with it: each model's own relative residual
+    ||b_m - S_m x_m|| / ||b_m|| meets tol after every IRLS solve (not only
+    the stacked one), and each model's slopes equal the oracle's IRLS on
+    that model alone."""
+    from hdfe_stream_amd.glm import fepois_batch
+    O = oracle_mod
+    ds = _models([40000, 800], seed=21)
+    ds[1]["X"] = (ds[1]["X"] * 1e-3).astype(np.float32)
+    print(f"fepois batch, uneven norms, {precision}", flush=True)
+    tol = 1e-10
+    res = fepois_batch("y ~ X | worker_id + firm_id + year", ds, iwls_tol=1e-10,
+                       iwls_maxiter=50, tol=tol, precision=precision, check_residuals=True)
+    for d, r in zip(ds, res):
+        per_solve = r.diagnostics["model_rel_residual"]
+        assert per_solve and max(per_solve) <= 2 * tol, per_solve
+        ref = _oracle(O, d)
+        assert r.converged and ref["converged"]
+        assert rel(r.beta, ref["beta"]) < 1e-7
+        assert r.deviance == pytest.approx(ref["deviance"], rel=1e-9)
