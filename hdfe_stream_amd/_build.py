@@ -20,7 +20,7 @@ CSRC = PKG / "csrc"
 LIB = PKG / "libhdfe_hip.so"
 OBJ = PKG / "_build"
 SOURCES = ["util.hip", "pcg.hip", "build.hip", "rows.hip", "kss.hip", "graph.hip", "adjoint.hip", "se.hip",
-           "glm.hip"]
+           "glm.hip", "ingest.hip"]
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
          # packed FP32 instructions stay off: hipcc would otherwise emit them
          # from plain float code

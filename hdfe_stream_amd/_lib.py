@@ -102,6 +102,10 @@ SIGNATURES = [
     ("hdfe_glm_coefficients", I32, [I64, I64, I32, P, P, P, P, P, P, P, P, I64, I32, P, P, P,
                                     P]),
     ("hdfe_glm_toward", I32, [I64, P, I32, P, P, P, P]),
+    ("hdfe_stage_create", I32, [I32, P, I64, I32, I32, P]),
+    ("hdfe_stage_columns", I32, [P, I64, P, P, I64, I32, P]),
+    ("hdfe_stage_wait", I32, [P]),
+    ("hdfe_stage_destroy", None, [P]),
 ]
 
 

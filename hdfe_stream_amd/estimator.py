@@ -8,7 +8,8 @@ the same FP64 algorithm -- pass 0 (singletons, codes, row sort), pass 1
 and beta) and pass 2 (streamed effects, residuals) -- all of it on the device
 (libhdfe_hip.so), with the panel's rows resident in HBM.
 
-Not mirrored (outside the solve path): Parquet streaming through Polars, the
+Host columns and Parquet files reach HBM through the batched ingest (`ingest.py`).
+Not mirrored (outside the solve path): Polars buckets and hash partitions, the
 pyfixest formula language beyond `y ~ x1 + x2 | fe1 + fe2 + ...`, standard
 errors, IV, collinearity checks.
 """
@@ -309,6 +310,23 @@ class StreamingHDFE:
         self.verbose = verbose
         self.spmm_timing = spmm_timing
 
+    def _ingest(self, data, names):
+        """The model's columns on the device: host arrays streamed to HBM in
+        batches of rows, a Parquet file read and streamed batch by batch,
+        device tensors as they are (`ingest`)."""
+        import torch
+        from .ingest import parquet_to_device, to_device
+        dtypes = {f: torch.int64 for f in self.fe}
+        dtypes.update({v: torch.float64 for v in names})
+        needed = list(dict.fromkeys(list(self.fe) + names +
+                                    ([self.weights] if self.weights is not None else [])))
+        stats = {}
+        if isinstance(data, (str, bytes)) or hasattr(data, "__fspath__"):
+            cols = parquet_to_device(data, needed, dtypes, stats=stats)
+        else:
+            cols = to_device({k: _column(data, k) for k in needed}, dtypes, stats=stats)
+        return cols, {k: (round(v, 4) if isinstance(v, float) else v) for k, v in stats.items()}
+
     def fit(self, data) -> HDFEResult:
         import torch
         t0 = time.time()
@@ -316,8 +334,10 @@ class StreamingHDFE:
         if len(self.fe) < 2:
             raise ValueError("device='rocm' fits need at least two fixed effects "
                              "(one streamed, one solved for)")
-        panel = Panel([_column(data, f) for f in self.fe], [_column(data, v) for v in names],
-                      None if self.weights is None else _column(data, self.weights),
+        cols, ingest = self._ingest(data, names)
+        t_in = time.time()
+        panel = Panel([cols[f] for f in self.fe], [cols[v] for v in names],
+                      None if self.weights is None else cols[self.weights],
                       self.fixef_rm, -1 if self.stream is None else self.fe.index(self.stream),
                       sort_vars=self.keep_intermediates)
         info = {"solver": "explicit", "device": self.backend.name}
@@ -383,7 +403,8 @@ class StreamingHDFE:
             o, nl = int(panel.offs[t]), panel.n_levels[t]
             levels[d] = panel.array(f"levels{t}", np.int64, pinned=True)
             values[d] = gy[o:o + nl]
-        diag = {"seconds_pass01": round(t1 - t0, 4), "seconds_system": round(t2 - t1, 4),
+        diag = {"seconds_ingest": round(t_in - t0, 4), "ingest": ingest,
+                "seconds_pass01": round(t1 - t_in, 4), "seconds_system": round(t2 - t1, 4),
                 "seconds_solve": round(t3 - t2, 4), "seconds_pass2": round(t4 - t3, 4),
                 "seconds_total": round(t4 - t0, 4), "singletons": panel.info["singletons"],
                 "cells": panel.info["C"], "identifying_cells": panel.info["M"],

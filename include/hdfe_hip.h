@@ -492,6 +492,31 @@ int hdfe_glm_coefficients(int64_t L, int64_t G, int kx, const double *Gm, const 
 int hdfe_glm_toward(int64_t n, const int *model, int per, const double *t, const double *base,
                     const double *target, double *out);
 
+/* ------------------------------------------ ingest (ingest.hip)
+ * Host columns to HBM in batches of rows through page-locked slots: the
+ * batch loop of the reference's Parquet passes (iter_group_chunks,
+ * core/utils.py:26-60, and the chunk loop of _pass1_cells,
+ * core/passes.py:397-531), whose batches this build moves to the device
+ * instead of working on them on the CPU. */
+
+/* A stager for ncols columns of elem_bytes[c] bytes per element: `depth`
+ * page-locked slots of batch_rows rows, `threads` host threads packing them. */
+int hdfe_stage_create(int ncols, const int *elem_bytes, int64_t batch_rows, int depth,
+                      int threads, void **out);
+
+/* Rows [0, n_rows) of host_cols[c] (ordinary host memory) into dev_cols[c]
+ * from row dev_row on, batch by batch (the pack of one batch overlapping the
+ * DMA of the previous).  The host columns are free again on return; with
+ * `wait` the copies have also landed, else hdfe_stage_wait waits for them.
+ * stats (or NULL): seconds in all, seconds packing, bytes, batches. */
+int hdfe_stage_columns(void *h, int64_t n_rows, const void *const *host_cols,
+                       void *const *dev_cols, int64_t dev_row, int wait, double *stats);
+
+/* Waits for every copy the stager has enqueued. */
+int hdfe_stage_wait(void *h);
+
+void hdfe_stage_destroy(void *h);
+
 #ifdef __cplusplus
 }
 #endif
