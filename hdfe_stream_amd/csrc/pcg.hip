@@ -1572,6 +1572,7 @@ static int solve_mixed(Matrix &M, int k, const uint8_t *zero_h, bool has_x0, dou
     int64_t total = 0;
     bool first = true;
     double prev = 1e300;
+    int64_t first_inner = 0;
     for (int outer = 0; outer < 64; ++outer) {
         // the true residual, in FP64
         if (first && !has_x0) {
@@ -1612,8 +1613,13 @@ static int solve_mixed(Matrix &M, int k, const uint8_t *zero_h, bool has_x0, dou
         }
         prev = rmax;
         *outer_out = outer + 1;
-        // the FP32 inner solve of S d = r / ||r||
-        CgState st = make_state(W, k, inner_tol, (int)std::min<int64_t>(maxiter - total, INT32_MAX));
+        // the FP32 inner solve of S d = r / ||r||; after the first, each may
+        // take at most three times the first's iterations (and 20): once the
+        // residual nears its rounding floor the inner target can be out of
+        // reach, and the outer test then ends the refinement
+        int64_t cap = maxiter - total;
+        if (first_inner > 0) cap = std::min<int64_t>(cap, std::max<int64_t>(20, 3 * first_inner));
+        CgState st = make_state(W, k, inner_tol, (int)std::min<int64_t>(cap, INT32_MAX));
         HIP_TRY(hipMemcpyAsync(scale, sc.data(), k * 8, hipMemcpyHostToDevice, s));
         HIP_TRY(hipMemcpyAsync(st.active, act.data(), k * 4, hipMemcpyHostToDevice, s));
         HIP_TRY(hipMemcpyAsync(outer_act, act.data(), k * 4, hipMemcpyHostToDevice, s));
@@ -1625,6 +1631,7 @@ static int solve_mixed(Matrix &M, int k, const uint8_t *zero_h, bool has_x0, dou
         int conv = 0;
         TRY(run_cg<float>(M, k, M.val32, M.dinv32, W.X32, W.R32, W.P32, W.AP32, st, s, &it, &conv));
         total += std::max<int64_t>(it, 1);
+        if (first_inner == 0) first_inner = std::max<int64_t>(it, 1);
         hipLaunchKernelGGL(k_refine_update, dim3(ge), dim3(kThreads), 0, s, M.L, k, W.X64, W.X32,
                            scale, outer_act);
         HIP_TRY(hipGetLastError());

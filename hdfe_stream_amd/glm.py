@@ -85,6 +85,39 @@ def _design(fml, data):
     return yname, blocks, fe
 
 
+def _null_groups(panel, lmodel, n_models, ncomp):
+    """Each level's group of S's null space: S = D_o'W M_0 D_o vanishes on
+    every vector that is constant over the first solved dimension's levels
+    of one connected component (each worker's rows lie in one component),
+    and on every vector constant over one model's levels of a further
+    dimension (every row of the model moves alike).  These indicators are
+    disjoint, so the null space is spanned by orthogonal group indicators:
+    component c of the first dimension is group c, dimension t >= 1 of model
+    m is group ncomp + m (D - 1) + t - 1.  Device (L,) int64."""
+    import torch
+    D = len(panel.n_levels)
+    gid = torch.empty(panel.info["L"], dtype=torch.int64, device="cuda")
+    o0, n0 = int(panel.offs[0]), int(panel.n_levels[0])
+    gid[o0:o0 + n0] = panel.tensor("comp", torch.int32)[:n0].long()
+    for t in range(1, D):
+        o, nl = int(panel.offs[t]), int(panel.n_levels[t])
+        gid[o:o + nl] = ncomp + lmodel[o:o + nl] * (D - 1) + (t - 1)
+    return gid
+
+
+def _to_range(rhs, null_id):
+    """rhs less its part in S's null space (the group means of `null_id`,
+    whose indicators span it): the rounding of D_o'W M_0 v leaves the sums
+    over a null group nonzero by a few ulps of their terms, and no x can
+    reduce a model's own relative residual below that part."""
+    import torch
+    ng = int(null_id.max()) + 1
+    sums = torch.zeros((ng, rhs.shape[1]), dtype=torch.float64, device=rhs.device)
+    sums.index_add_(0, null_id, rhs)
+    cnt = torch.bincount(null_id, minlength=ng).to(torch.float64).clamp_(min=1)
+    return rhs - (sums / cnt[:, None])[null_id]
+
+
 def _per_model_solve(matrix, rhs, x0, lmodel, n_models, tol, maxiter, precision, info):
     """Solve the block-diagonal stacked system so that every model meets `tol`
     on its own relative residual ||r_m|| / ||b_m||, column by column.
@@ -106,8 +139,16 @@ def _per_model_solve(matrix, rhs, x0, lmodel, n_models, tol, maxiter, precision,
     live = max(int((norm > 0).sum(0).max()), 1)
     scale = inv[lmodel]
     x0s = None if x0 is None else x0 * scale
-    X, its, ok, rel = matrix.solve(rhs * scale, None, x0s, tol=tol / np.sqrt(live),
+    b = rhs * scale
+    X, its, ok, rel = matrix.solve(b, None, x0s, tol=tol / np.sqrt(live),
                                    maxiter=maxiter, precision=precision, info=info)
+    if not ok and precision == "mixed":
+        # the refinement stopped at FP32's reach above the tightened
+        # tolerance: FP64 CG finishes from its solution
+        X, its2, ok, rel = matrix.solve(b, None, X, tol=tol / np.sqrt(live),
+                                        maxiter=max(maxiter - its, 1), precision="fp64")
+        its += its2
+        info["fp64_finish_iterations"] = its2
     return X * norm[lmodel], its, ok, rel
 
 
@@ -328,6 +369,7 @@ def fepois_batch(fml: str, datasets: list, iwls_tol: float = 1e-8, iwls_maxiter:
         panel.system(with_rhs=False)
         if ncomp is None:
             ncomp = panel.components()
+            null_id = _null_groups(panel, lmodel_l, M, ncomp)
         clock("system", t0)
         t0 = time.perf_counter()
         rhs = torch.empty((L, Cc), dtype=torch.float64, device=dev)
@@ -338,6 +380,7 @@ def fepois_batch(fml: str, datasets: list, iwls_tol: float = 1e-8, iwls_maxiter:
         clock("rhs", t0)
         t0 = time.perf_counter()
         sinfo = {}
+        rhs = _to_range(rhs, null_id)
         Gamma, its, ok, rel = _per_model_solve(panel.matrix, rhs, Gamma, lmodel_l, M, tol,
                                                maxiter, precision, sinfo)
         solver_its.append((its, sinfo.get("refinement_steps", 0), bool(ok), float(rel)))
