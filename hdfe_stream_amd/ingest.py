@@ -83,6 +83,23 @@ class Stager:
             pass
 
 
+_CACHE: dict = {}
+
+
+def cached_stager(elem_bytes, batch_rows: int = DEFAULT_BATCH_ROWS, depth: int = DEFAULT_DEPTH,
+                  threads: int | None = None) -> Stager:
+    """A process-wide stager per (element sizes, batch, depth, threads): its
+    page-locked slots and host threads are made once (hipHostMalloc of the
+    slots costs milliseconds per call), then reused by every ingest of that
+    shape.  Calls from one thread only, as the fits."""
+    key = (tuple(int(b) for b in elem_bytes), int(batch_rows), int(depth),
+           int(threads or default_threads()))
+    st = _CACHE.get(key)
+    if st is None or st._h is None:
+        st = _CACHE[key] = Stager(key[0], key[1], key[2], key[3])
+    return st
+
+
 def _torch_dtype(np_dtype):
     import torch
     return torch.from_numpy(np.empty(0, np_dtype)).dtype
@@ -117,12 +134,9 @@ def to_device(columns: dict, dtypes: dict | None = None, batch_rows: int = DEFAU
             raise ValueError("columns of different lengths")
         dev = [torch.empty(n, dtype=_torch_dtype(host[k].dtype), device="cuda") for k in names]
         torch.cuda.synchronize()
-        st = Stager([host[k].itemsize for k in names], batch_rows, threads=threads)
-        try:
-            s = st.stage([host[k] for k in names], dev) if n else \
-                {"seconds": 0.0, "pack_seconds": 0.0, "bytes": 0, "batches": 0}
-        finally:
-            st.close()
+        st = cached_stager([host[k].itemsize for k in names], batch_rows, threads=threads)
+        s = st.stage([host[k] for k in names], dev) if n else \
+            {"seconds": 0.0, "pack_seconds": 0.0, "bytes": 0, "batches": 0}
         out.update(zip(names, dev))
         if stats is not None:
             stats.update(s)
@@ -153,7 +167,7 @@ def parquet_to_device(path, columns, dtypes: dict | None = None,
                         else np.dtype(schema.field(c).type.to_pandas_dtype()))
     dev = [torch.empty(n, dtype=_torch_dtype(t), device="cuda") for t in np_types]
     torch.cuda.synchronize()
-    st = Stager([t.itemsize for t in np_types], batch_rows, threads=threads)
+    st = cached_stager([t.itemsize for t in np_types], batch_rows, threads=threads)
     row, nbytes, pack = 0, 0, 0.0
     try:
         for batch in pf.iter_batches(batch_size=batch_rows, columns=columns):
@@ -163,9 +177,8 @@ def parquet_to_device(path, columns, dtypes: dict | None = None,
             row += len(arrs[0])
             nbytes += s["bytes"]
             pack += s["pack_seconds"]
-        st.wait()
     finally:
-        st.close()
+        st.wait()
     if row != n:
         raise RuntimeError(f"read {row} rows of {n}")
     if stats is not None:
