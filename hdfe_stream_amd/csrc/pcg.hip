@@ -471,7 +471,13 @@ __global__ __launch_bounds__(kThreads) void k_spmm_combine(
 // Wide k: one wave per row, lanes over columns (j = lane + 64 t, t < CPL).
 // Each nonzero's column index and value are loaded once (wave-uniform) and
 // feed CPL coalesced gathers of the neighbour's row of P; two nonzeros are in
-// flight at a time.
+// flight at a time.  Rows are dealt to the waves grid-stride (wave v of
+// workgroup w takes rows 4 w + v, 4 (w + nw) + v, ...), so the rows in flight
+// at any moment form one narrow window that sweeps the matrix: their
+// neighbours' rows of P -- for a block-diagonal S of many models, one model's
+// block -- stay resident in the memory-side cache instead of the whole of P
+// (a contiguous row range per workgroup keeps every part of the matrix in
+// flight at once).
 template <typename T, int CPL>
 __global__ __launch_bounds__(kThreads) void k_spmm_wide(
     const int *__restrict__ indptr, const int *__restrict__ indices, const T *__restrict__ val,
@@ -481,12 +487,11 @@ __global__ __launch_bounds__(kThreads) void k_spmm_wide(
     extern __shared__ double wlds[];  // [4][k]
     const int k = st.k;
     const int w = blockIdx.x, nw = gridDim.x;
-    const int64_t r0 = part_begin(L, w, nw), r1 = part_begin(L, w + 1, nw);
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     double dot[CPL];
 #pragma unroll
     for (int t = 0; t < CPL; ++t) dot[t] = 0.0;
-    for (int64_t row = r0 + wv; row < r1; row += 4) {
+    for (int64_t row = (int64_t)w * 4 + wv; row < L; row += (int64_t)nw * 4) {
         const int s = __builtin_amdgcn_readfirstlane(indptr[row]);
         const int e = __builtin_amdgcn_readfirstlane(indptr[row + 1]);
         const bool is_long = (e - s) > kLongThresh;
@@ -533,6 +538,84 @@ __global__ __launch_bounds__(kThreads) void k_spmm_wide(
     __syncthreads();
     for (int j = threadIdx.x; j < k; j += blockDim.x)
         part[(int64_t)w * k + j] = wlds[j] + wlds[k + j] + wlds[2 * k + j] + wlds[3 * k + j];
+}
+
+// Wide k in column slabs, a slab per XCD: the same product AP = S P with the
+// columns cut into slabs of kSlabW, and in each launch ("round") XCD x (the
+// workgroups w with w % 8 == x) working on one slab only, several XCDs
+// sharing a slab when a round has fewer than eight.  Within an XCD the rows
+// go grid-stride, kSlabW lanes per row (four rows per wave), so an XCD's
+// gathers in flight are one slab of its window's neighbour rows: for a
+// block-diagonal S of many models a slab of one model's block (kSlabW x 8 B
+// per level), which its 4 MB L2 can hold where the full rows (k x 8 B per
+// level) could not.  Each row's sum runs over its nonzeros in order and in
+// pairs, as in k_spmm_wide; the pAp partials are per (round, workgroup), zero
+// outside the workgroup's slab.
+constexpr int kSlabW = 16;
+constexpr int kSlabGroups = 128;  // workgroups per XCD
+constexpr int kPartRows = 8192;   // rows of k partials in the work buffer
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_spmm_slab(
+    const int *__restrict__ indptr, const int *__restrict__ indices, const T *__restrict__ val,
+    int64_t L, const T *__restrict__ P, T *__restrict__ AP, double *__restrict__ part,
+    int with_dot, int slab0, int nslab, CgState st) {
+    if (st.done && *st.done) return;
+    __shared__ double lds[kThreads];
+    const int k = st.k;
+    const int w = blockIdx.x, x = w & 7, lw = w >> 3, gx = (int)gridDim.x >> 3;
+    const int si = x % nslab, q = x / nslab, nq = (8 - si + nslab - 1) / nslab;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int j = (slab0 + si) * kSlabW + (lane & (kSlabW - 1));
+    const bool colok = j < k;
+    constexpr int kRowsPerIter = kThreads / kSlabW;
+    const int64_t stride = (int64_t)gx * nq * kRowsPerIter;
+    double dot = 0.0;
+    for (int64_t base = ((int64_t)lw * nq + q) * kRowsPerIter; base < L; base += stride) {
+        const int64_t row = base + threadIdx.x / kSlabW;
+        if (row >= L) continue;
+        const int s = indptr[row], e = indptr[row + 1];
+        T acc = T(0);
+        if (e - s > kLongThresh) {
+            // a split row: its product is already in AP
+            if (colok) acc = AP[row * k + j];
+        } else {
+            const T *Pj = P + (colok ? j : 0);
+            // the nonzeros in pairs, as k_spmm_wide adds them; four in flight
+            int p = s;
+            for (; p + 3 < e; p += 4) {
+                const int c0 = indices[p], c1 = indices[p + 1], c2 = indices[p + 2],
+                          c3 = indices[p + 3];
+                const T v0 = val[p], v1 = val[p + 1], v2 = val[p + 2], v3 = val[p + 3];
+                const T a0 = Pj[(int64_t)c0 * k], a1 = Pj[(int64_t)c1 * k];
+                const T a2 = Pj[(int64_t)c2 * k], a3 = Pj[(int64_t)c3 * k];
+                acc += v0 * a0 + v1 * a1;
+                acc += v2 * a2 + v3 * a3;
+            }
+            if (p + 1 < e) {
+                const int c0 = indices[p], c1 = indices[p + 1];
+                const T v0 = val[p], v1 = val[p + 1];
+                acc += v0 * Pj[(int64_t)c0 * k] + v1 * Pj[(int64_t)c1 * k];
+                p += 2;
+            }
+            if (p < e) acc += val[p] * Pj[(int64_t)indices[p] * k];
+            if (colok) AP[row * k + j] = acc;
+        }
+        if (colok) dot += (double)P[row * k + j] * (double)acc;
+    }
+    (void)wv;
+    if (!with_dot) return;
+    lds[threadIdx.x] = colok ? dot : 0.0;
+    __syncthreads();
+    // the workgroup's partial: the kSlabW columns of its slab summed over its
+    // row lanes in a fixed order, every other column zero
+    for (int c = threadIdx.x; c < k; c += blockDim.x) {
+        const int cs = c - (slab0 + si) * kSlabW;
+        double v = 0.0;
+        if (cs >= 0 && cs < kSlabW)
+            for (int r = 0; r < kRowsPerIter; ++r) v += lds[r * kSlabW + cs];
+        part[(int64_t)w * k + c] = v;
+    }
 }
 
 // Column-blocked mapping for the vector kernels: TJ threads across columns,
@@ -967,6 +1050,9 @@ struct Matrix {
     // the blocked SpMM's row parts per XCD, chosen by timing (0: the default),
     // per precision (0: FP64, 1: FP32) and k
     int blk_parts[2][kSmallK + 1] = {};
+    // the wide SpMM's form per (precision, k), chosen by timing: 0 a wave per
+    // row over all columns (k_spmm_wide), 1 column slabs per XCD (k_spmm_slab)
+    std::map<std::pair<int, int>, int> wide_mode;
     // captured graphs of batches of CG iterations, per (buffers, k, precision,
     // batch); a key's first use runs eagerly, the second captures
     std::map<std::array<uintptr_t, 9>, hipGraphExec_t> cg_graphs;
@@ -1019,7 +1105,7 @@ static int alloc_work(Matrix &M, int k) {
     HIP_TRY(palloc(&W.R32, Lk * 4, M.stream));
     HIP_TRY(palloc(&W.P32, Lk * 4, M.stream));
     HIP_TRY(palloc(&W.AP32, Lk * 4, M.stream));
-    HIP_TRY(palloc(&W.part, (int64_t)4096 * 2 * k * 8, M.stream));
+    HIP_TRY(palloc(&W.part, (int64_t)kPartRows * k * 8, M.stream));
     const int64_t dlp = M.dl ? (int64_t)M.nlong * spmm_grid(M.L, k) : 0;
     HIP_TRY(palloc(&W.lpart, std::max<int64_t>(1, std::max<int64_t>(M.nchunk, dlp) * k) * 8, M.stream));
     HIP_TRY(palloc(&W.colbuf, (int64_t)k * 8, M.stream));
@@ -1169,7 +1255,8 @@ static int build_blocked(Matrix &M, const std::vector<int> &ip32) {
 // AP = S P (and, with with_dot, pAp sums and alpha).  `st.done` may be null.
 template <typename T>
 static void launch_spmm(Matrix &M, const T *val, const T *P, T *AP, double *part, double *lpart,
-                        int with_dot, CgState st, hipStream_t s, int force_parts = 0) {
+                        int with_dot, CgState st, hipStream_t s, int force_parts = 0,
+                        int force_wide = -1) {
     const int k = st.k;
     const int g = spmm_grid(M.L, k);
     if (k <= kSmallK && M.blk && M.L * (int64_t)k * (int64_t)sizeof(T) > kL2Budget &&
@@ -1248,6 +1335,28 @@ static void launch_spmm(Matrix &M, const T *val, const T *P, T *AP, double *part
                            M.long_rows, M.long_cptr, M.nlong, P, AP, lpart, st);
         hipLaunchKernelGGL((k_fin_long_wide<T>), dim3(1), dim3(kFinThreads), 0, s, M.long_rows,
                            M.long_cptr, M.nlong, lpart, AP, st);
+    }
+    int mode = force_wide;
+    if (mode < 0) {
+        auto it = M.wide_mode.find({sizeof(T) == 4 ? 1 : 0, k});
+        mode = it == M.wide_mode.end() ? 0 : it->second;
+    }
+    const int nslab = (k + kSlabW - 1) / kSlabW, rounds = (nslab + 7) / 8;
+    // the slab form's pAp partials, one row of k per (round, workgroup), must
+    // fit the work buffer's kPartRows rows: fewer workgroups per XCD for wide
+    // k, and the wave-per-row form when not even eight per XCD would fit
+    const int gs = std::min(8 * kSlabGroups, (kPartRows / rounds) / 8 * 8);
+    if (mode == 1 && gs >= 64) {
+        for (int r = 0; r < rounds; ++r) {
+            const int slab0 = 8 * r, ns = std::min(8, nslab - slab0);
+            hipLaunchKernelGGL((k_spmm_slab<T>), dim3(gs), dim3(kThreads), 0, s, M.indptr,
+                               M.indices, val, M.L, P, AP, part + (int64_t)r * gs * k, with_dot,
+                               slab0, ns, st);
+        }
+        if (with_dot)
+            hipLaunchKernelGGL(k_fin_alpha, dim3(1), dim3(kFinThreads), 0, s, part, rounds * gs,
+                               st);
+        return;
     }
     const int cpl = (k + 63) / 64;
 #define HDFE_WIDE(C)                                                                        \
@@ -1467,9 +1576,64 @@ static int tune_blocked(Matrix &M, int k) {
     return HDFE_STATUS_OK;
 }
 
+// The wide SpMM's form (k > kSmallK), per precision: each candidate timed
+// with HIP events over three launches after one untimed, the fastest kept,
+// process-wide per matrix shape (size, nonzeros, split rows, k).
+static std::map<std::array<int64_t, 5>, int> g_wide_tune;
+
+template <typename T>
+static int tune_wide(Matrix &M, int k) {
+    if (k <= kSmallK) return HDFE_STATUS_OK;
+    const int prec = sizeof(T) == 4 ? 1 : 0;
+    const std::array<int64_t, 5> key = {M.L, M.nnz, M.nlong, k, prec};
+    {
+        std::lock_guard<std::mutex> lock(g_tune_mu);
+        auto it = g_wide_tune.find(key);
+        if (it != g_wide_tune.end()) {
+            M.wide_mode[{prec, k}] = it->second;
+            return HDFE_STATUS_OK;
+        }
+    }
+    hipStream_t s = M.stream;
+    T *P = sizeof(T) == 4 ? (T *)M.w.P32 : (T *)M.w.P64;
+    T *AP = sizeof(T) == 4 ? (T *)M.w.AP32 : (T *)M.w.AP64;
+    const T *val = sizeof(T) == 4 ? (const T *)M.val32 : (const T *)M.val64;
+    HIP_TRY(hipMemsetAsync(P, 0, M.L * (int64_t)k * sizeof(T), s));
+    CgState cs = make_state(M.w, k, 0.0, 1);
+    cs.done = nullptr;
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    float best = 1e30f;
+    int choice = 0;
+    for (int mode = 0; mode < 2; ++mode) {
+        launch_spmm<T>(M, val, P, AP, M.w.part, M.w.lpart, 1, cs, s, 0, mode);
+        HIP_TRY(hipEventRecord(e0, s));
+        for (int r = 0; r < 3; ++r)
+            launch_spmm<T>(M, val, P, AP, M.w.part, M.w.lpart, 1, cs, s, 0, mode);
+        HIP_TRY(hipEventRecord(e1, s));
+        HIP_TRY(hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best) {
+            best = ms;
+            choice = mode;
+        }
+    }
+    HIP_TRY(hipEventDestroy(e0));
+    HIP_TRY(hipEventDestroy(e1));
+    HIP_TRY(hipGetLastError());
+    M.wide_mode[{prec, k}] = choice;
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    g_wide_tune[key] = choice;
+    return HDFE_STATUS_OK;
+}
+
 static int tune_blocked_both(Matrix &M, int k) {
     TRY(tune_blocked<double>(M, k));
-    return tune_blocked<float>(M, k);
+    TRY(tune_blocked<float>(M, k));
+    TRY(tune_wide<double>(M, k));
+    return tune_wide<float>(M, k);
 }
 
 // FP64 block PCG (core/solve.py:66-140).
@@ -1871,6 +2035,18 @@ int hdfe_csr_blk_parts(void *h, int k, int precision, int set, int *parts) {
         M.blk_parts[precision][k] = set;
     }
     *parts = M.blk_parts[precision][k];
+    return HDFE_STATUS_OK;
+}
+
+int hdfe_csr_wide_mode(void *h, int k, int precision, int set, int *mode) {
+    ARG_CHECK(h && mode && k > kSmallK && k <= kMaxK && (precision == 0 || precision == 1) &&
+                  set >= -1 && set <= 1,
+              "hdfe_csr_wide_mode: bad argument");
+    Matrix &M = *(Matrix *)h;
+    std::lock_guard<std::mutex> lock(M.mu);
+    if (set >= 0) M.wide_mode[{precision, k}] = set;
+    auto it = M.wide_mode.find({precision, k});
+    *mode = it == M.wide_mode.end() ? -1 : it->second;
     return HDFE_STATUS_OK;
 }
 

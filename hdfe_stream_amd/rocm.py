@@ -173,6 +173,14 @@ class DeviceCSR:
                                                   int(set), C.byref(out)), "hdfe_csr_blk_parts")
         return int(out.value)
 
+    def wide_mode(self, k: int, precision: str = "fp64", set: int = -1) -> int:
+        """The wide SpMM's form for k > 8 columns (-1 not tuned yet, 0 a wave
+        per row, 1 and 2 column slabs per XCD); `set` overrides it."""
+        out = C.c_int()
+        _lib.check(_lib.load().hdfe_csr_wide_mode(self._h, int(k), PRECISIONS[precision],
+                                                  int(set), C.byref(out)), "hdfe_csr_wide_mode")
+        return int(out.value)
+
     def graphs(self) -> int:
         """CG batches held as HIP graphs (captured on small systems only)."""
         n = C.c_int64()

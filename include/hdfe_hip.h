@@ -91,6 +91,13 @@ int hdfe_csr_timing_read(void *h, double *ms, int64_t *launches);
  * launch autotuner's report and tests. */
 int hdfe_csr_blk_parts(void *h, int k, int precision, int set, int *parts);
 
+/* The wide SpMM's form for k > 8 columns at `precision` (0 FP64, 1 FP32), as
+ * the launch autotuner chose it at the matrix's first use with that k (-1: not
+ * yet): 0 a wave per row over all columns, 1 and 2 column slabs per XCD with
+ * 64 or 128 workgroups per XCD; set >= 0 overrides it.  No counterpart in the
+ * reference; for the autotuner's report and tests. */
+int hdfe_csr_wide_mode(void *h, int k, int precision, int set, int *mode);
+
 /* The number of CG batches held as HIP graphs on this matrix.  Batches are
  * captured only on systems small enough that launching costs more than the
  * device time of an iteration.  No counterpart in the reference (it has no

@@ -258,3 +258,30 @@ def test_graph_replays_match_the_eager_solve(system, big_system, size, precision
     assert ok8 and rel8 <= 1e-6
     if precision == "fp64":
         assert it8 < its[0]
+
+
+@pytest.mark.parametrize("k", [9, 64, 301, 1024])
+def test_wide_spmm_forms_agree_with_scipy(k):
+    """Both forms of the wide product (a wave per row; column slabs per XCD,
+    whose pAp partials are bounded by the work buffer at k = 1024) against
+    scipy's S @ X, with a split (long) row in the matrix."""
+    import scipy.sparse as sp
+    from hdfe_stream_amd.rocm import DeviceCSR
+    rng = np.random.default_rng(k)
+    L = 6000
+    A = sp.random(L, L, density=0.005, random_state=k, format="csr")
+    A = A + A.T
+    A = A.tolil()
+    A[7, :] = rng.standard_normal(L)          # one row longer than the split threshold
+    A[:, 7] = A[7, :].T
+    S = sp.csr_matrix(A) + sp.identity(L) * 50.0
+    S.sort_indices()
+    M = DeviceCSR.from_scipy(S, 1.0 / S.diagonal())
+    X = rng.standard_normal((L, k))
+    want = S @ X
+    M.matmat(X)                                 # first use with k: the autotuner runs
+    for mode in (0, 1):
+        print(f"wide spmm k={k} form {mode}", flush=True)
+        assert M.wide_mode(k, "fp64", set=mode) == mode
+        got = M.matmat(X)
+        assert np.abs(got - want).max() <= 1e-12 * np.abs(want).max(), mode
