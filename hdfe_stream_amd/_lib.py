@@ -94,8 +94,8 @@ SIGNATURES = [
     ("hdfe_adj_xwr", I32, [I64, I32, P, P, I32, P, P, P]),
     ("hdfe_adj_gradients", I32, [I64, I32, P, P, P, P, P, P, P, P]),
     ("hdfe_panel_reweight", I32, [P, P, P]),
-    ("hdfe_glm_rows", I32, [I32, P, P, P, I32, P, P, P, I32, P, P, I64, I32, P, P, P, P, P, P,
-                            P, P]),
+    ("hdfe_glm_rows", I32, [I32, P, P, P, I32, I32, P, P, P, I32, P, P, I64, I32, P, P, P, P, P,
+                            P, P, P]),
     ("hdfe_glm_rhs", I32, [P, I64, P, P, P, P, I32, P, P, P]),
     ("hdfe_glm_partial", I32, [I32, P, P, P, P, P, P, I32, I32, P, P, I32, P, P, I64, I32, P, P,
                                P, P, P, P]),

@@ -37,16 +37,19 @@ constexpr int kSyrkMaxBlk = 8;       // 32x32 output blocks per wave
 
 // ------------------------------------------------------------ row pass
 
-// One IRLS row pass (core/kernels_glm.py:99-211, Poisson): per row its
-// linear predictor at (beta_m, fe, a) -- or the start log((y + ybar_m) / 2)
-// -- then mu = exp(eta), W = mu f, u = y / mu - 1, z = eta + u, and the
-// deviance and log-likelihood less its constant.  A wave per row (lanes over
+// One IRLS row pass (core/kernels_glm.py:99-211): per row its linear
+// predictor at (beta_m, fe, a) -- or the start: log((y + ybar_m) / 2) for
+// Poisson, ybar_m for Gaussian -- then, Poisson (family 0): mu = exp(eta),
+// W = mu f, u = y / mu - 1, z = eta + u, and the deviance and
+// log-likelihood less its constant; Gaussian (family 1, identity link, the
+// OLS batch): W = f, z = y, deviance the weighted sum of squared residuals;
+// family 2: the linear predictor alone, written to w.  A wave per row (lanes over
 // the covariates); a workgroup per chunk of one model's rows; part[c] =
 // (dev, sum f (y eta - mu), f y, f) of chunk c (the fifth slot is
 // k_glm_lgamma's).
 __global__ __launch_bounds__(256) void k_glm_rows(
     int nchunk, const int *__restrict__ ch_model, const int64_t *__restrict__ ch_begin,
-    const int64_t *__restrict__ ch_end, int init, const double *__restrict__ ybar,
+    const int64_t *__restrict__ ch_end, int init, int family, const double *__restrict__ ybar,
     const double *__restrict__ y, const float *__restrict__ X, int kx,
     const int *__restrict__ g, const int *__restrict__ codes, int64_t ld, int D,
     const int64_t *__restrict__ offs, const double *__restrict__ fe,
@@ -62,7 +65,7 @@ __global__ __launch_bounds__(256) void k_glm_rows(
         const double yi = y[i];
         double eta;
         if (init) {
-            eta = log(0.5 * (yi + ybar[m]));
+            eta = family == 1 ? ybar[m] : log(0.5 * (yi + ybar[m]));
         } else {
             double s = 0.0;
             for (int j = lane; j < kx; j += 64) s += (double)X[i * kx + j] * beta[(int64_t)m * kx + j];
@@ -70,7 +73,20 @@ __global__ __launch_bounds__(256) void k_glm_rows(
             eta = s + a[g[i]];
             for (int d = 0; d < D; ++d) eta += fe[offs[d] + codes[d * ld + i]];
         }
-        if (lane == 0) {
+        if (lane == 0 && family == 2) {
+            // the linear predictor alone (w = eta), for the fitted values
+            w[i] = eta;
+            z[i] = yi;
+        } else if (lane == 0 && family == 1) {
+            // Gaussian, identity link: W = f, z = y, deviance = f (y - eta)^2
+            const double fi = f ? f[i] : 1.0;
+            const double r = yi - eta;
+            acc[0] += fi * r * r;
+            acc[2] += fi * yi;
+            acc[3] += fi;
+            w[i] = fi;
+            z[i] = yi;
+        } else if (lane == 0) {
             const double mu = exp(eta);
             const double fi = f ? f[i] : 1.0;
             const double u = mu > 0.0 ? yi / mu - 1.0 : 0.0;
@@ -418,11 +434,12 @@ int hdfe_rows_chunk_sum(void *h, int k, const double *part, double *out);
 // chunk lists; device in/out.  part (host, nchunk x 5): deviance, sum f (y
 // eta - mu), f y, f, and with init the sum of f lgamma(y + 1).
 int hdfe_glm_rows(int nchunk, const int *ch_model, const int64_t *ch_begin,
-                  const int64_t *ch_end, int init, const double *ybar, const double *y,
+                  const int64_t *ch_end, int init, int family, const double *ybar, const double *y,
                   const float *X, int kx, const int *g, const int *codes, int64_t ld, int D,
                   const int64_t *offs, const double *fe, const double *a, const double *beta,
                   const double *f, double *w, double *z, double *part_h) {
-    ARG_CHECK(nchunk >= 1 && y && w && z && part_h && (init || (X || kx == 0)),
+    ARG_CHECK(nchunk >= 1 && y && w && z && part_h && (init || (X || kx == 0)) &&
+                  family >= 0 && family <= 2,
               "hdfe_glm_rows: bad argument");
     DevBuf<int> cm;
     DevBuf<int64_t> cb, ce;
@@ -436,8 +453,9 @@ int hdfe_glm_rows(int nchunk, const int *ch_model, const int64_t *ch_begin,
     HIP_TRY(hipMemcpy(ce.p, ch_end, nchunk * 8, hipMemcpyHostToDevice));
     HIP_TRY(hipMemset(part.p, 0, (int64_t)nchunk * 5 * 8));
     hipLaunchKernelGGL(k_glm_rows, dim3(nchunk), dim3(256), 0, 0, nchunk, cm.p, cb.p, ce.p, init,
-                       ybar, y, X, kx, g, codes, ld, D, offs, fe, a, beta, f, w, z, part.p);
-    if (init)
+                       family, ybar, y, X, kx, g, codes, ld, D, offs, fe, a, beta, f, w, z,
+                       part.p);
+    if (init && family == 0)
         hipLaunchKernelGGL(k_glm_lgamma, dim3(nchunk), dim3(256), 0, 0, cb.p, ce.p, y, f, part.p);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpy(part_h, part.p, (int64_t)nchunk * 5 * 8, hipMemcpyDeviceToHost));

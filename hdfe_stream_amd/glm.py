@@ -168,13 +168,38 @@ def _model_residuals(matrix, rhs, X, lmodel, n_models):
 def fepois_batch(fml: str, datasets: list, iwls_tol: float = 1e-8, iwls_maxiter: int = 25,
                  separation_check: bool = True, tol: float = 1e-10, maxiter: int = 5000,
                  precision: str = "mixed", rows_per_syrk: int = 32768,
-                 timings: dict | None = None, check_residuals: bool = False) -> list:
+                 timings: dict | None = None, check_residuals: bool = False,
+                 weights: str | None = None) -> list:
     """Fit `fml` by Poisson IRLS on each dataset, all on the device at once.
     Returns one GLMResult per dataset.  Every model's weighted fit of [z, X]
     on its fixed effects meets `tol` on its own relative residual (the
     reference solves each model alone, core/solve.py:66-140); with
     `check_residuals` each model's true residuals b - S x are measured after
     every solve and reported (diagnostics["model_rel_residual"])."""
+    return _glm_batch(fml, datasets, "poisson", iwls_tol, iwls_maxiter, separation_check, tol,
+                      maxiter, precision, rows_per_syrk, timings, check_residuals, weights)
+
+
+def feols_batch(fml: str, datasets: list, weights: str | None = None, tol: float = 1e-10,
+                maxiter: int = 5000, precision: str = "mixed", rows_per_syrk: int = 32768,
+                max_refinements: int = 6, timings: dict | None = None,
+                check_residuals: bool = False) -> list:
+    """OLS with fixed effects on many datasets at once (feols_stream,
+    api.py:22-107, of each; the wide-design path of the reference's
+    assembly="rows": core/inference.py:109-137, `_nb_residualize_rows` and
+    V'V).  The same machinery as `fepois_batch` with the identity link: one
+    weighted fit of [y, X] on the fixed effects (every model to `tol` on its
+    own residual), then the slopes by Newton steps with the split-BF16
+    Hessian X~'W X~ on the matrix cores and the FP64 score X~'W (y~ - X~
+    beta), repeated until they stop moving (|d beta| <= 1e-13 |beta|), so
+    that they solve the FP64 normal equations.  `deviance` is the weighted
+    residual sum of squares; `weights` names a column of analytic weights."""
+    return _glm_batch(fml, datasets, "gaussian", 1e-13, max_refinements, False, tol, maxiter,
+                      precision, rows_per_syrk, timings, check_residuals, weights)
+
+
+def _glm_batch(fml, datasets, family, iwls_tol, iwls_maxiter, separation_check, tol, maxiter,
+               precision, rows_per_syrk, timings, check_residuals, weights):
     import torch
 
     from .estimator import Panel, _column
@@ -193,21 +218,30 @@ def fepois_batch(fml: str, datasets: list, iwls_tol: float = 1e-8, iwls_maxiter:
     for d in datasets:
         yname, blocks, fe = _design(fml, d)
         if len(fe) < 2:
-            raise ValueError("device Poisson fits need at least two fixed effects")
+            raise ValueError("device batch fits need at least two fixed effects")
         y = torch.as_tensor(np.asarray(_column(d, yname)) if not torch.is_tensor(
             _column(d, yname)) else _column(d, yname)).to(dev, torch.float64)
-        if float(y.min()) < 0:
-            raise ValueError("Poisson regression needs a nonnegative dependent variable; "
-                             f"its minimum is {float(y.min())}")
-        if float(y.max()) <= 0:
-            raise ValueError("the dependent variable is zero on every row")
+        if family == "poisson":
+            if float(y.min()) < 0:
+                raise ValueError("Poisson regression needs a nonnegative dependent variable; "
+                                 f"its minimum is {float(y.min())}")
+            if float(y.max()) <= 0:
+                raise ValueError("the dependent variable is zero on every row")
+        wt = None
+        if weights is not None:
+            col = _column(d, weights)
+            wt = torch.as_tensor(np.asarray(col) if not torch.is_tensor(col) else col).to(
+                dev, torch.float64)
+            if float(wt.min()) < 0:
+                raise ValueError("weights must be nonnegative")
         xs = [torch.as_tensor(b).to(dev, torch.float32).reshape(len(y), -1)
               for _, b in blocks]
         X = torch.cat(xs, dim=1).contiguous() if xs else torch.zeros((len(y), 0), device=dev)
         ids = [torch.as_tensor(np.asarray(_column(d, f)) if not torch.is_tensor(_column(d, f))
                                else _column(d, f)).to(dev, torch.int64) for f in fe]
         names = [n for nm, _ in blocks for n in nm]
-        per.append({"y": y, "X": X, "ids": ids, "names": names, "fe": fe, "n_in": len(y)})
+        per.append({"y": y, "X": X, "ids": ids, "names": names, "fe": fe, "n_in": len(y),
+                    "w": wt})
     kx = per[0]["X"].shape[1]
     if any(p["X"].shape[1] != kx for p in per):
         raise ValueError("every model of a batch needs the same covariates")
@@ -252,6 +286,7 @@ def fepois_batch(fml: str, datasets: list, iwls_tol: float = 1e-8, iwls_maxiter:
     fe_cols = [torch.cat(s) for s in stacked]
     smodel = torch.cat(src_model)
     y_all = torch.cat([p["y"][r] for p, r in zip(per, src_row)])
+    f_all = None if weights is None else torch.cat([p["w"][r] for p, r in zip(per, src_row)])
     X_all = torch.cat([p["X"][r] for p, r in zip(per, src_row)])
     n_stack = len(y_all)
     ones = torch.ones(n_stack, dtype=torch.float64, device=dev)
@@ -261,6 +296,7 @@ def fepois_batch(fml: str, datasets: list, iwls_tol: float = 1e-8, iwls_maxiter:
     perm = panel.tensor("perm", torch.int32).long()[:n]
     rmodel = smodel[perm].contiguous()
     y_s = y_all[perm].contiguous()
+    f_s = None if f_all is None else f_all[perm].contiguous()
     X_s = X_all[perm].contiguous()
     del X_all
     rm_host = rmodel.cpu().numpy()
@@ -286,8 +322,16 @@ def fepois_batch(fml: str, datasets: list, iwls_tol: float = 1e-8, iwls_maxiter:
             ptr[k] = panel.ptr(k)[0]
     refresh_ptrs()
     n_m = np.diff(rs)
-    ybar = torch.tensor([float(y_s[rs[m]:rs[m + 1]].mean()) if n_m[m] else 1.0
-                         for m in range(M)], dtype=torch.float64, device=dev)
+
+    def wmean(v, m):
+        if not n_m[m]:
+            return 1.0
+        if f_s is None:
+            return float(v[rs[m]:rs[m + 1]].mean())
+        fw = f_s[rs[m]:rs[m + 1]]
+        return float((fw * v[rs[m]:rs[m + 1]]).sum() / fw.sum().clamp(min=1e-300))
+    ybar = torch.tensor([wmean(y_s, m) for m in range(M)], dtype=torch.float64, device=dev)
+    fam = {"poisson": 0, "gaussian": 1}[family]
     # launch lists
     ch_model, ch_b, ch_e = [], [], []
     for m in range(M):
@@ -319,17 +363,18 @@ def fepois_batch(fml: str, datasets: list, iwls_tol: float = 1e-8, iwls_maxiter:
     rp = RowPasses(panel)
     Vz = torch.empty((n, 1), dtype=torch.float64, device=dev)
 
-    def row_pass(beta, fe, a, init):
+    def row_pass(beta, fe, a, init, family_code=None):
         w = torch.empty(n, dtype=torch.float64, device=dev)
         z = torch.empty(n, dtype=torch.float64, device=dev)
         part = np.zeros((len(ch_model), 5))
         torch.cuda.synchronize()
         _lib.check(lib.hdfe_glm_rows(
             len(ch_model), ch_model.ctypes.data, ch_b.ctypes.data, ch_e.ctypes.data, int(init),
+            fam if family_code is None else family_code,
             ybar.data_ptr(), y_s.data_ptr(), X_s.data_ptr(), kx, ptr["g"], ptr["codes"], n, D,
             ptr["offs"], None if init else fe.data_ptr(), None if init else a.data_ptr(),
-            None if init else beta.data_ptr(), None, w.data_ptr(), z.data_ptr(),
-            part.ctypes.data), "hdfe_glm_rows")
+            None if init else beta.data_ptr(), None if f_s is None else f_s.data_ptr(),
+            w.data_ptr(), z.data_ptr(), part.ctypes.data), "hdfe_glm_rows")
         sums = np.zeros((M, 5))
         np.add.at(sums, ch_model, part)
         return w, z, sums
@@ -452,7 +497,10 @@ def fepois_batch(fml: str, datasets: list, iwls_tol: float = 1e-8, iwls_maxiter:
                 dev_new[m] < dev_cur[m]
                 or abs(dev_new[m] - dev_cur[m]) / (0.1 + abs(dev_cur[m])) < tol_acc)
 
-        bad = [m for m in range(M) if active[m] and not init[m] and not accept(m)]
+        # least squares: the Newton step contracts whatever the deviance's
+        # rounding says, so it is always taken
+        bad = [] if fam == 1 else \
+            [m for m in range(M) if active[m] and not init[m] and not accept(m)]
         for m in range(M):
             if active[m] and init[m] and not np.isfinite(dev_new[m]):
                 raise FloatingPointError(f"model {m}: the first IRLS step gives a "
@@ -475,8 +523,15 @@ def fepois_batch(fml: str, datasets: list, iwls_tol: float = 1e-8, iwls_maxiter:
         for m in range(M):
             if not active[m]:
                 continue
-            crit = abs(dev_new[m] - dev_cur[m]) / (0.1 + abs(dev_cur[m]))
-            flat = (not init[m]) and dev_new[m] >= dev_cur[m]
+            if fam == 1:
+                # the slopes' relative change: the Newton steps refine them
+                # to the FP64 normal equations
+                step = float((cb[m] - beta[m]).abs().max())
+                crit = step / max(float(cb[m].abs().max()), 1e-300)
+                flat = False
+            else:
+                crit = abs(dev_new[m] - dev_cur[m]) / (0.1 + abs(dev_cur[m]))
+                flat = (not init[m]) and dev_new[m] >= dev_cur[m]
             iters[m] = it
             paths[m].append(float(dev_new[m]))
             dev_cur[m] = dev_new[m]
@@ -491,7 +546,7 @@ def fepois_batch(fml: str, datasets: list, iwls_tol: float = 1e-8, iwls_maxiter:
     # first at obs-weighted mean zero, the first per component (the
     # normalisation of core/solve.py:383-399), the group effects absorbing
     # the shifts; the linear predictor is unchanged
-    eta_s = _eta(y_s, fe, a, beta, row_pass)
+    eta_s = _eta(y_s, fe, a, beta, row_pass, fam)
     fe_h = fe.cpu().numpy()
     a_h = a.cpu().numpy()
     lmod_h = lmodel.cpu().numpy()
@@ -545,7 +600,8 @@ def fepois_batch(fml: str, datasets: list, iwls_tol: float = 1e-8, iwls_maxiter:
         nm = int(n_m[m])
         out.append(GLMResult(
             coefnames=p["names"], beta=beta[m].cpu().numpy(), deviance=float(dev_cur[m]),
-            loglik=float(sums[m, 1] - slg[m]), iterations=int(iters[m]),
+            loglik=float(sums[m, 1] - slg[m]) if fam == 0 else float("nan"),
+            iterations=int(iters[m]),
             converged=bool(converged[m]), n_obs=nm, fe_names=[fe_names[0]] +
             [fe_names[d] for d in panel.dims], fe_levels=levels, fe_values=values,
             deviance_path=paths[m], eta_source=e_src,
@@ -567,11 +623,11 @@ def fepois_batch(fml: str, datasets: list, iwls_tol: float = 1e-8, iwls_maxiter:
     return out
 
 
-def _eta(y_s, fe, a, beta, row_pass):
-    """The linear predictor per sorted row at (beta, fe, a): z - u with the
-    row pass's u = y / mu - 1 recovered from w = mu."""
-    w, z, _ = row_pass(beta, fe, a, False)
-    return z - (y_s / w - 1.0)
+def _eta(y_s, fe, a, beta, row_pass, fam=0):
+    """The linear predictor per sorted row at (beta, fe, a) (the row pass's
+    family 2 writes it to w)."""
+    w, _, _ = row_pass(beta, fe, a, False, 2)
+    return w
 
 
 def fepois_stream(fml: str, data, iwls_tol: float = 1e-8, iwls_maxiter: int = 25,

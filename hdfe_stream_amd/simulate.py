@@ -17,7 +17,7 @@ from __future__ import annotations
 import numpy as np
 
 __all__ = ["simulate_akm", "akm_panel", "kss_panel", "weighted_panel", "bottleneck_panel", "poisson_models",
-           "poisson_models_device"]
+           "poisson_models_device", "ols_models"]
 
 # Row-level corr(alpha, psi) is about 0.6 times the copula parameter with
 # p_move 0.15 and 8 years (movers' later firms are uniform), so a copula of
@@ -250,4 +250,29 @@ def poisson_models_device(n_models: int = 32, lo: int = 50_000, hi: int = 2_000_
         mu = torch.exp(0.5 + 0.1 * X[:, 0].double() + a[worker] + p[firm])
         y = torch.poisson(mu, generator=gen)
         out.append({"worker_id": worker, "firm_id": firm, "year": year, "X": X, "y": y})
+    return out
+
+
+def ols_models(n_models: int = 32, lo: int = 50_000, hi: int = 2_000_000, n_cov: int = 300,
+               seed: int = 6) -> list:
+    """Uneven linear models of `poisson_models`' shapes: rows log-uniform in
+    [lo, hi], worker rows/8 levels (rows sorted by worker), firm rows/80,
+    year 10, `n_cov` float32 covariates N(0, 1), and y = 0.3 x1 - 0.2 x2 +
+    alpha + psi + gamma + eps with alpha ~ N(0, 0.5^2), psi ~ N(0, 0.3^2),
+    gamma ~ N(0, 0.1^2), eps ~ N(0, 0.4^2); float64."""
+    rng = np.random.default_rng(seed)
+    sizes = np.exp(rng.uniform(np.log(lo), np.log(hi), n_models))
+    out = []
+    for n in sizes.astype(np.int64):
+        n = max(int(n), 160)
+        nw, nf = max(n // 8, 2), max(n // 80, 2)
+        worker = np.sort(rng.integers(0, nw, n))
+        firm = rng.integers(0, nf, n)
+        year = rng.integers(0, 10, n)
+        X = rng.standard_normal((n, n_cov)).astype(np.float32)
+        y = (0.3 * X[:, 0].astype(np.float64) - 0.2 * X[:, 1].astype(np.float64)
+             + rng.normal(0.0, 0.5, nw)[worker] + rng.normal(0.0, 0.3, nf)[firm]
+             + rng.normal(0.0, 0.1, 10)[year] + rng.normal(0.0, 0.4, n))
+        out.append({"worker_id": worker.astype(np.int64), "firm_id": firm.astype(np.int64),
+                    "year": year.astype(np.int64), "X": X, "y": y})
     return out

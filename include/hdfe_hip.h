@@ -453,13 +453,17 @@ int hdfe_adj_gradients(int64_t n, int kx, const double *rows, const double *Xt, 
  * model contiguous in sorted order.  X: covariates (n, kx) float32, row
  * major, sorted order. */
 
-/* One IRLS row pass (_nb_glm_rows / _row_terms, core/kernels_glm.py:99-211,
- * Poisson): per row the linear predictor at (beta_m, fe, a), or the start
- * log((y + ybar_m) / 2) with init; w = mu f, z = eta + y / mu - 1.  Chunks
- * (host lists: model, row range) of one model each; part (host, nchunk x 5)
- * = deviance, sum f (y eta - mu), f y, f, and with init sum f lgamma(y + 1). */
+/* One IRLS row pass (_nb_glm_rows / _row_terms, core/kernels_glm.py:99-211):
+ * per row the linear predictor at (beta_m, fe, a), or the start with init
+ * (Poisson log((y + ybar_m) / 2), Gaussian ybar_m); family 0 (Poisson):
+ * w = mu f, z = eta + y / mu - 1; family 1 (Gaussian, identity link: the
+ * OLS batch): w = f, z = y; family 2: w = eta alone.  Chunks (host lists:
+ * model, row range) of one
+ * model each; part (host, nchunk x 5) = deviance (Gaussian: the weighted
+ * residual sum of squares), sum f (y eta - mu) (Poisson), f y, f, and with
+ * init (Poisson) sum f lgamma(y + 1). */
 int hdfe_glm_rows(int nchunk, const int *ch_model, const int64_t *ch_begin,
-                  const int64_t *ch_end, int init, const double *ybar, const double *y,
+                  const int64_t *ch_end, int init, int family, const double *ybar, const double *y,
                   const float *X, int kx, const int *g, const int *codes, int64_t ld, int D,
                   const int64_t *offs, const double *fe, const double *a, const double *beta,
                   const double *f, double *w, double *z, double *part_h);
