@@ -34,6 +34,7 @@ SIGNATURES = [
     ("hdfe_csr_info", I32, [P, P, P, P]),
     ("hdfe_pcg_solve", I32, [P, I64, P, P, P, D, I64, I32, D, P, P, P, P, P, I32, P]),
     ("hdfe_csr_matmat", I32, [P, I64, P, P, I32]),
+    ("hdfe_stream_create", I32, [I64, I64, I64, I32, P, P, I64, P, P, P, P]),
     ("hdfe_csr_matmat_f32", I32, [P, I64, P, P, I32]),
     ("hdfe_csr_export", I32, [P, P, P, P, P]),
     ("hdfe_col_norms", I32, [P, I64, I64, P]),
@@ -48,6 +49,7 @@ SIGNATURES = [
     ("hdfe_panel_copy_async", I32, [P, P, P, I64]),
     ("hdfe_panel_copy_wait", I32, [P]),
     ("hdfe_panel_system", I32, [P, I32, P, P]),
+    ("hdfe_panel_budget", I32, [P, I64, P, P]),
     ("hdfe_panel_components", I32, [P, P]),
     ("hdfe_panel_normalize", I32, [P, P, I32, P]),
     ("hdfe_panel_assemble", I32, [P, P, P]),

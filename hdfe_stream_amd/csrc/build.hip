@@ -2206,6 +2206,8 @@ struct Panel {
     std::vector<int> dims;         // user index of each non-streamed dimension
     std::vector<int64_t> nlev;     // their levels
     std::vector<int64_t> offs_h;
+    int64_t s_budget = -1;   // bytes the explicit S build may take (-1: no limit)
+    int64_t s_skipped = 0;   // the estimate of an S not built for the budget (0: built)
     int64_t G = 0, L = 0, C = 0, M = 0, Gi = 0, E = 0;
     int weighted = 0;
     int64_t singletons = 0, single_rounds = 0;
@@ -2841,6 +2843,17 @@ int hdfe_panel_copy_wait(void *h) {
     return HDFE_STATUS_OK;
 }
 
+// The explicit S build's budget in bytes (-1: none), and the estimate of the
+// last S that was not built for it (0: built).
+int hdfe_panel_budget(void *h, int64_t set, int64_t *budget, int64_t *skipped) {
+    ARG_CHECK(h, "hdfe_panel_budget: null handle");
+    Panel &P = *(Panel *)h;
+    if (set >= -1) P.s_budget = set;
+    if (budget) *budget = P.s_budget;
+    if (skipped) *skipped = P.s_skipped;
+    return HDFE_STATUS_OK;
+}
+
 int hdfe_panel_array(void *h, const char *name, void **ptr, int64_t *count) {
     ARG_CHECK(h && name && ptr, "hdfe_panel_array: null argument");
     Panel &P = *(Panel *)h;
@@ -3099,7 +3112,24 @@ int hdfe_panel_system(void *h, int with_rhs, void **matrix, double *seconds) {
     HIP_TRY(exclusive_sum(P.tmp, cnt.p, pos.p, Gi + 1, s));
     int64_t nent = 0;
     HIP_TRY(scalar_to_host(&nent, pos.p + Gi, s));
-    ARG_CHECK(nent < INT32_MAX, "hdfe_panel_system: too many sparse entries for one sort");
+    // the explicit build's size from here: the emitted pairs sorted with their
+    // values (two key and two value arrays) and S itself (at most one entry
+    // per pair, plus the dense levels' rows); over the budget, or past what
+    // one sort can take, the panel keeps diag and rhs and no S (the caller's
+    // stream_cg fallback: hdfe_stream_create)
+    const int64_t est = nent * 32 + (nent + L * Ld) * 12 + (L + 1) * 4;
+    if ((P.s_budget >= 0 && est > P.s_budget) || nent >= INT32_MAX) {
+        if (P.matrix) {
+            hdfe_destroy(P.matrix);
+            P.matrix = nullptr;
+        }
+        P.s_skipped = est;
+        if (matrix) *matrix = nullptr;
+        if (seconds)
+            *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        return HDFE_STATUS_OK;
+    }
+    P.s_skipped = 0;
     DevBuf<uint64_t> ek1, ek2;
     DevBuf<double> ev1, ev2;
     DEV_ALLOC(ek1, std::max<int64_t>(nent, 1), s);

@@ -1008,6 +1008,125 @@ struct DevBuf2 {
 };
 
 // Work buffers, cached on the matrix for the last block width used.
+// ---------------------------------------------- matrix-free S P (stream_cg)
+//
+// The reference's fallback when S does not fit (`_nb_stream_matvec`,
+// core/kernels_base.py:477-507; core/solve.py:251-271): S P from the
+// identifying cells without S.  With a_c the cell's levels (one per
+// non-streamed dimension), n_c its weight and N_g its group's total,
+//   (S P)_l = sum over the cells c containing l of v_c,
+//   v_c = n_c (t_c - m_g),  t_c = sum_d P[a_cd],  m_g = sum_c n_c t_c / N_g.
+// v is formed per group (a thread per group and column, its cells in order),
+// then summed per level through the level-to-cells CSR in chunks of
+// kMfChunk cells (a wave per chunk, a fixed shuffle tree) and the chunks in
+// order: no atomics, the same result every run.
+
+constexpr int kMfMaxD = 8;
+constexpr int kMfChunk = 2048;
+struct MfDims {
+    int D;
+    int64_t off[kMfMaxD];
+};
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_mf_cells(int64_t Gi, const int *__restrict__ igstart,
+                                                     const int *__restrict__ icodes, int64_t ld,
+                                                     MfDims dm, const double *__restrict__ cn,
+                                                     const T *__restrict__ P, int k,
+                                                     T *__restrict__ v, CgState st) {
+    if (st.done && *st.done) return;
+    const int64_t nth = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < Gi * k; q += nth) {
+        const int64_t g = q / k;
+        const int j = (int)(q - g * k);
+        const int s = igstart[g], e = igstart[g + 1];
+        double N = 0.0, S = 0.0;
+        for (int c = s; c < e; ++c) {
+            double t = 0.0;
+            for (int d = 0; d < dm.D; ++d)
+                t += (double)P[(dm.off[d] + icodes[d * ld + c]) * k + j];
+            N += cn[c];
+            S += cn[c] * t;
+        }
+        const double mg = N > 0.0 ? S / N : 0.0;
+        for (int c = s; c < e; ++c) {
+            double t = 0.0;
+            for (int d = 0; d < dm.D; ++d)
+                t += (double)P[(dm.off[d] + icodes[d * ld + c]) * k + j];
+            v[(int64_t)c * k + j] = (T)(cn[c] * (t - mg));
+        }
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_mf_chunks(int64_t nchunk, const int *__restrict__ ch_beg,
+                                                      const int *__restrict__ ch_end,
+                                                      const int *__restrict__ lcell,
+                                                      const T *__restrict__ v, int k,
+                                                      double *__restrict__ cpart, CgState st) {
+    if (st.done && *st.done) return;
+    const int lane = threadIdx.x & 63;
+    const int64_t nwave = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    int tj = 1;
+    while (tj < k && tj < 64) tj <<= 1;
+    const int tr = 64 / tj, cj = lane % tj, rj = lane / tj;
+    for (int64_t q = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; q < nchunk;
+         q += nwave) {
+        const int b = ch_beg[q], e = ch_end[q];
+        for (int j0 = 0; j0 < k; j0 += tj) {
+            const int j = j0 + cj;
+            double acc = 0.0;
+            if (j < k)
+                for (int i = b + rj; i < e; i += tr) acc += (double)v[(int64_t)lcell[i] * k + j];
+            for (int o = tj; o < 64; o <<= 1) acc += __shfl_xor(acc, o, 64);
+            if (rj == 0 && j < k) cpart[q * k + j] = acc;
+        }
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_mf_finish(int64_t L, const int *__restrict__ lch,
+                                                      const double *__restrict__ cpart, int k,
+                                                      T *__restrict__ AP, CgState st) {
+    if (st.done && *st.done) return;
+    const int64_t nth = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < L * k; q += nth) {
+        const int64_t l = q / k;
+        const int j = (int)(q - l * k);
+        double acc = 0.0;
+        for (int c = lch[l]; c < lch[l + 1]; ++c) acc += cpart[(int64_t)c * k + j];
+        AP[q] = (T)acc;
+    }
+}
+
+// The level key of every (dimension, cell) pair and its cell, for the sort
+// that makes the level-to-cells CSR.
+__global__ void k_mf_pairs(int64_t Mc, const int *__restrict__ icodes, int64_t ld, MfDims dm,
+                           int *__restrict__ key, int *__restrict__ cell,
+                           int *__restrict__ count) {
+    const int64_t n = Mc * dm.D;
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n;
+         q += (int64_t)gridDim.x * blockDim.x) {
+        const int d = (int)(q / Mc);
+        const int64_t c = q - d * Mc;
+        const int l = (int)(dm.off[d] + icodes[d * ld + c]);
+        key[q] = l;
+        cell[q] = (int)c;
+        atomicAdd(count + l, 1);
+    }
+}
+
+struct MatFree {
+    int64_t Gi = 0, Mc = 0, ld = 0, nchunk = 0;
+    MfDims dm{};
+    const int *igstart = nullptr, *icodes = nullptr;
+    const double *cn = nullptr;
+    int *lcell = nullptr, *ch_beg = nullptr, *ch_end = nullptr, *lch = nullptr;
+    void *v = nullptr;          // (Mc, k) of the current k
+    double *cpart = nullptr;    // (nchunk, k)
+    int vk = 0;
+};
+
 struct Work {
     int k = 0;
     int64_t L = 0;
@@ -1064,6 +1183,7 @@ struct Matrix {
     }
     hipStream_t stream = nullptr;  // the solver's own stream
     hipEvent_t ev = nullptr;
+    MatFree *mf = nullptr;         // set: S P from the cells (no explicit S)
     Work w;
     std::mutex mu;
     // optional HIP-event timing of the SpMM launches of the CG iterations
@@ -1117,6 +1237,17 @@ static int alloc_work(Matrix &M, int k) {
     HIP_TRY(hipMemsetAsync(W.sd, 0, ((int64_t)6 * k + 1) * 8, M.stream));
     W.k = k;
     W.L = M.L;
+    if (M.mf) {
+        MatFree &F = *M.mf;
+        pfree(F.v, M.stream);
+        pfree(F.cpart, M.stream);
+        F.v = nullptr;
+        F.cpart = nullptr;
+        HIP_TRY(palloc(&F.v, std::max<int64_t>(F.Mc * k, 1) * 8, M.stream));
+        HIP_TRY(palloc(&F.cpart, std::max<int64_t>(F.nchunk * k, 1) * 8, M.stream));
+        F.vk = k;
+        return HDFE_STATUS_OK;
+    }
     return tune_blocked_both(M, k);
 }
 
@@ -1258,6 +1389,30 @@ static void launch_spmm(Matrix &M, const T *val, const T *P, T *AP, double *part
                         int with_dot, CgState st, hipStream_t s, int force_parts = 0,
                         int force_wide = -1) {
     const int k = st.k;
+    if (M.mf) {
+        MatFree &F = *M.mf;
+        T *v = (T *)F.v;
+        const int gc = grid_for(std::max<int64_t>(F.Gi * k, 1), 1024, 1, 8192);
+        hipLaunchKernelGGL((k_mf_cells<T>), dim3(gc), dim3(kThreads), 0, s, F.Gi, F.igstart,
+                           F.icodes, F.ld, F.dm, F.cn, P, k, v, st);
+        const int gk = (int)std::min<int64_t>(std::max<int64_t>((F.nchunk + 3) / 4, 1), 16384);
+        hipLaunchKernelGGL((k_mf_chunks<T>), dim3(gk), dim3(kThreads), 0, s, F.nchunk, F.ch_beg,
+                           F.ch_end, F.lcell, v, k, F.cpart, st);
+        const int gf = grid_for(std::max<int64_t>(M.L * k, 1), 1024, 1, 8192);
+        hipLaunchKernelGGL((k_mf_finish<T>), dim3(gf), dim3(kThreads), 0, s, M.L, F.lch, F.cpart,
+                           k, AP, st);
+        if (with_dot) {
+            const int gv = vec_grid(M.L, k);
+            hipLaunchKernelGGL((k_colsum<T>), dim3(gv), dim3(kThreads), 0, s, M.L, k, P, AP,
+                               (const T *)nullptr, part);
+            hipLaunchKernelGGL(k_fin_alpha, dim3(1), dim3(kFinThreads), 0, s, part, gv, st);
+        }
+        (void)force_parts;
+        (void)force_wide;
+        (void)val;
+        (void)lpart;
+        return;
+    }
     const int g = spmm_grid(M.L, k);
     if (k <= kSmallK && M.blk && M.L * (int64_t)k * (int64_t)sizeof(T) > kL2Budget &&
         M.w.gpart) {
@@ -1877,6 +2032,88 @@ int matrix_create(int64_t L, int64_t nnz, const std::vector<int> &ip32, const in
     return HDFE_STATUS_OK;
 }
 
+// A matrix-free operator over a panel's identifying cells (see k_mf_cells):
+// the level-to-cells CSR by a stable radix sort of the (level, cell) pairs
+// (each level's cells ascending), its chunks, dinv64 copied and cast.
+int matfree_create(int64_t L, int64_t Gi, int64_t Mc, int D, const int *igstart,
+                   const int *icodes, int64_t ld, const int64_t *offs_h, const double *cell_n,
+                   const double *dinv, void **out) {
+    std::unique_ptr<Matrix> M(new Matrix());
+    HIP_TRY(hipGetDevice(&M->device));
+    M->L = L;
+    M->nnz = 0;
+    M->blk_tried = true;
+    HIP_TRY(stream_get(&M->stream));
+    HIP_TRY(hipEventCreateWithFlags(&M->ev, hipEventDisableTiming));
+    hipStream_t ms = M->stream;
+    auto *F = new MatFree();
+    M->mf = F;
+    F->Gi = Gi;
+    F->Mc = Mc;
+    F->ld = ld;
+    F->dm.D = D;
+    for (int d = 0; d < D; ++d) F->dm.off[d] = offs_h[d];
+    F->igstart = igstart;
+    F->icodes = icodes;
+    F->cn = cell_n;
+    HIP_TRY(palloc(&M->dinv64, std::max<int64_t>(L, 1) * 8, ms));
+    HIP_TRY(palloc(&M->dinv32, std::max<int64_t>(L, 1) * 4, ms));
+    if (L) HIP_TRY(hipMemcpyAsync(M->dinv64, dinv, L * 8, hipMemcpyDeviceToDevice, ms));
+    hipLaunchKernelGGL(k_cast_f64_to_f32, dim3(grid_for(std::max<int64_t>(L, 1), 1024, 1, 4096)),
+                       dim3(kThreads), 0, ms, L, M->dinv64, M->dinv32);
+    const int64_t n = Mc * D;
+    int *key = nullptr, *cell = nullptr, *key2 = nullptr, *count = nullptr;
+    void *tmp = nullptr;
+    HIP_TRY(palloc(&key, std::max<int64_t>(n, 1) * 4, ms));
+    HIP_TRY(palloc(&cell, std::max<int64_t>(n, 1) * 4, ms));
+    HIP_TRY(palloc(&key2, std::max<int64_t>(n, 1) * 4, ms));
+    HIP_TRY(palloc(&F->lcell, std::max<int64_t>(n, 1) * 4, ms));
+    HIP_TRY(palloc(&count, (L + 1) * 4, ms));
+    HIP_TRY(hipMemsetAsync(count, 0, (L + 1) * 4, ms));
+    hipLaunchKernelGGL(k_mf_pairs, dim3(grid_for(std::max<int64_t>(n, 1), 1024, 1, 8192)),
+                       dim3(kThreads), 0, ms, Mc, icodes, ld, F->dm, key, cell, count);
+    size_t tb = 0;
+    const int bits = std::max(1, 64 - __builtin_clzll((unsigned long long)std::max<int64_t>(L, 1)));
+    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, key, key2, cell, F->lcell, (int)n, 0,
+                                               bits, ms));
+    HIP_TRY(palloc(&tmp, std::max<size_t>(tb, 1), ms));
+    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(tmp, tb, key, key2, cell, F->lcell, (int)n, 0,
+                                               bits, ms));
+    std::vector<int> cnt(L);
+    if (L) HIP_TRY(hipMemcpyAsync(cnt.data(), count, L * 4, hipMemcpyDeviceToHost, ms));
+    HIP_TRY(hipStreamSynchronize(ms));
+    pfree(key, ms);
+    pfree(cell, ms);
+    pfree(key2, ms);
+    pfree(count, ms);
+    pfree(tmp, ms);
+    // chunks of at most kMfChunk cells, level by level
+    std::vector<int> cb, ce, lch(L + 1, 0);
+    int64_t at = 0;
+    for (int64_t l = 0; l < L; ++l) {
+        lch[l] = (int)cb.size();
+        for (int64_t b = 0; b < cnt[l]; b += kMfChunk) {
+            cb.push_back((int)(at + b));
+            ce.push_back((int)(at + std::min<int64_t>(cnt[l], b + kMfChunk)));
+        }
+        at += cnt[l];
+    }
+    lch[L] = (int)cb.size();
+    F->nchunk = (int64_t)cb.size();
+    HIP_TRY(palloc(&F->ch_beg, std::max<int64_t>(F->nchunk, 1) * 4, ms));
+    HIP_TRY(palloc(&F->ch_end, std::max<int64_t>(F->nchunk, 1) * 4, ms));
+    HIP_TRY(palloc(&F->lch, (L + 1) * 4, ms));
+    if (F->nchunk) {
+        HIP_TRY(hipMemcpyAsync(F->ch_beg, cb.data(), F->nchunk * 4, hipMemcpyHostToDevice, ms));
+        HIP_TRY(hipMemcpyAsync(F->ch_end, ce.data(), F->nchunk * 4, hipMemcpyHostToDevice, ms));
+    }
+    HIP_TRY(hipMemcpyAsync(F->lch, lch.data(), (L + 1) * 4, hipMemcpyHostToDevice, ms));
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(ms));
+    *out = M.release();
+    return HDFE_STATUS_OK;
+}
+
 }  // namespace hdfe
 
 // ================================================================== C ABI
@@ -1986,6 +2223,8 @@ int hdfe_csr_matmat(void *h, int64_t k, const double *x, double *y, int on_devic
 int hdfe_csr_export(void *h, int64_t *indptr, int32_t *indices, double *data, double *dinv) {
     ARG_CHECK(h != nullptr, "hdfe_csr_export: null handle");
     Matrix &M = *(Matrix *)h;
+    ARG_CHECK(!M.mf || (!indptr && !indices && !data),
+              "hdfe_csr_export: a matrix-free operator has no explicit S");
     HIP_TRY(hipSetDevice(M.device));
     if (indptr) {
         std::vector<int> ip(M.L + 1);
@@ -2151,6 +2390,18 @@ int hdfe_csr_matmat_f32(void *h, int64_t k, const float *x, float *y, int on_dev
     return HDFE_STATUS_OK;
 }
 
+int hdfe_stream_create(int64_t L, int64_t Gi, int64_t M, int D, const int *igstart,
+                       const int *icodes, int64_t ld, const int64_t *offs, const double *cell_n,
+                       const double *dinv, void **out) {
+    ARG_CHECK(out && igstart && icodes && cell_n && dinv && offs && L >= 1 && Gi >= 0 &&
+                  M >= 0 && D >= 1 && D <= kMfMaxD && ld >= M,
+              "hdfe_stream_create: bad argument");
+    ARG_CHECK(M * D < INT32_MAX && L < INT32_MAX, "hdfe_stream_create: too many cells");
+    for (int d = 0; d < D; ++d)
+        ARG_CHECK(offs[d] >= 0 && offs[d] < L, "hdfe_stream_create: level offset out of range");
+    return matfree_create(L, Gi, M, D, igstart, icodes, ld, offs, cell_n, dinv, out);
+}
+
 void hdfe_destroy(void *h) {
     if (!h) return;
     Matrix *M = (Matrix *)h;
@@ -2163,6 +2414,13 @@ void hdfe_destroy(void *h) {
                   M->bidx, M->bval64, M->bval32, M->dl_slot, M->dl_cptr, M->dt64, M->dt32,
                   M->ll64, M->ll32};
     for (void *p : ps) pfree(p, M->stream);
+    if (M->mf) {
+        MatFree *F = M->mf;
+        void *fs[] = {F->lcell, F->ch_beg, F->ch_end, F->lch, F->v, F->cpart};
+        for (void *p : fs) pfree(p, M->stream);
+        delete F;
+        M->mf = nullptr;
+    }
     if (M->stream) (void)hipStreamSynchronize(M->stream);
     if (M->ev) (void)hipEventDestroy(M->ev);
     stream_put(M->stream, M->device);

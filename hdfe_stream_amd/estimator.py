@@ -174,15 +174,47 @@ class Panel:
                                                out.numel() * out.element_size(), 2), "hdfe_memcpy")
         return out
 
-    def system(self, with_rhs=True):
+    def system(self, with_rhs=True, budget=None):
+        """diag, rhs and the explicit S (`matrix`).  With `budget` (bytes) an
+        S whose build would take more is not built: `matrix` is None and
+        `s_skipped` its estimate (the caller's fallback is stream_matrix)."""
         h = C.c_void_p()
         secs = C.c_double()
-        _lib.check(_lib.load().hdfe_panel_system(self._h, int(with_rhs), C.byref(h),
-                                                 C.byref(secs)), "hdfe_panel_system")
-        self.matrix = DeviceCSR(h, self.info["L"], 0, owned=False)
+        lib = _lib.load()
+        _lib.check(lib.hdfe_panel_budget(self._h, -1 if budget is None else int(budget), None,
+                                         None), "hdfe_panel_budget")
+        _lib.check(lib.hdfe_panel_system(self._h, int(with_rhs), C.byref(h), C.byref(secs)),
+                   "hdfe_panel_system")
+        skipped = C.c_int64()
+        _lib.check(lib.hdfe_panel_budget(self._h, -2, None, C.byref(skipped)), "hdfe_panel_budget")
+        self.s_skipped = int(skipped.value)
         self.refresh()
-        self.matrix.nnz = self.info["nnz"]
+        if h.value:
+            self.matrix = DeviceCSR(h, self.info["L"], 0, owned=False)
+            self.matrix.nnz = self.info["nnz"]
+        else:
+            self.matrix = None
         return float(secs.value)
+
+    def stream_matrix(self):
+        """The matrix-free S over the identifying cells (hdfe_stream_create):
+        the same solves as the explicit S, for an S too large to build (the
+        reference's stream_cg, core/solve.py:251-271).  Kept as `matrix`."""
+        import torch
+        info = self.info
+        L, Gi, Mc, D = info["L"], info["Gi"], info["M"], info["D"]
+        diag = self.tensor("diag", torch.float64)
+        self._mf_dinv = torch.where(diag > 1e-12, 1.0 / torch.where(diag > 1e-12, diag,
+                                                                     torch.ones_like(diag)),
+                                    torch.zeros_like(diag)).contiguous()
+        offs = np.ascontiguousarray(self.offs, np.int64)
+        h = C.c_void_p()
+        torch.cuda.synchronize()
+        _lib.check(_lib.load().hdfe_stream_create(
+            L, Gi, Mc, D, self.ptr("igstart")[0], self.ptr("icodes")[0], Mc, offs.ctypes.data,
+            self.ptr("in")[0], self._mf_dinv.data_ptr(), C.byref(h)), "hdfe_stream_create")
+        self.matrix = DeviceCSR(h, L, 0, owned=True)
+        return self.matrix
 
     def components(self):
         n = C.c_int64()
@@ -276,17 +308,19 @@ class StreamingHDFE:
     def __init__(self, y, x, fe, *, tol=1e-10, maxiter=5000, rhs_block=8, solver="auto",
                  precond="jacobi", fixef_rm="singleton", weights=None, stream=None,
                  device="rocm", precision="fp64", inner_tol=1e-5, keep_intermediates=False,
-                 verbose=False, spmm_timing=False, **ignored):
+                 verbose=False, spmm_timing=False, max_memory_gb=None, **ignored):
         if device != "rocm":
             raise ValueError("device must be 'rocm' (this package is the MI355X backend)")
         if solver not in ("auto", "explicit", "stream_cg", "within"):
             raise ValueError("solver must be 'auto', 'explicit', 'stream_cg' or 'within'")
         if precond not in ("jacobi", "amg"):
             raise ValueError("precond must be 'jacobi' or 'amg'")
-        if solver not in ("auto", "explicit") or precond != "jacobi":
-            raise ValueError("device='rocm' runs the explicit solver with the Jacobi "
-                             "preconditioner: use solver='auto' or 'explicit' and "
-                             "precond='jacobi'")
+        if solver not in ("auto", "explicit", "stream_cg") or precond != "jacobi":
+            raise ValueError("device='rocm' runs the explicit or the matrix-free (stream_cg) "
+                             "solver with the Jacobi preconditioner: use solver='auto', "
+                             "'explicit' or 'stream_cg' and precond='jacobi'")
+        if max_memory_gb is not None and not max_memory_gb > 0:
+            raise ValueError("max_memory_gb must be positive")
         if fixef_rm not in ("none", "singleton"):
             raise ValueError("fixef_rm must be 'none' or 'singleton'")
         if precision not in PRECISIONS:
@@ -300,6 +334,8 @@ class StreamingHDFE:
         if stream is not None and stream not in self.fe:
             raise ValueError(f"stream={stream!r} is not one of the fixed effects {self.fe}")
         self.tol, self.maxiter = tol, maxiter
+        self.solver = solver
+        self.max_memory_gb = max_memory_gb
         self.rhs_block = max(1, int(rhs_block))
         self.fixef_rm = fixef_rm
         self.weights = weights
@@ -309,6 +345,20 @@ class StreamingHDFE:
         self.keep_intermediates = keep_intermediates
         self.verbose = verbose
         self.spmm_timing = spmm_timing
+
+    def _s_budget(self, panel):
+        """Bytes the explicit S build may take: `max_memory_gb` if given,
+        else 80% of the device's free memory less the block PCG's work
+        arrays (the reference's headroom rule, core/solve.py:172-176);
+        None (no limit) for solver='explicit', 0 for 'stream_cg'."""
+        if self.solver == "stream_cg":
+            return 0
+        work = 7 * panel.info["L"] * min(panel.info["m"], self.rhs_block) * 8
+        if self.max_memory_gb is not None:
+            return max(int(self.max_memory_gb * 1e9) - work, 0)
+        if self.solver == "explicit":
+            return None
+        return max(int(0.8 * _lib.device_memory()[0]) - work, 0)
 
     def _ingest(self, data, names):
         """The model's columns on the device: host arrays streamed to HBM in
@@ -345,10 +395,27 @@ class StreamingHDFE:
         # system build and the solve
         glevels = panel.array_async("glevels", np.int64)
         t1 = time.time()
-        s_secs = panel.system(with_rhs=True)
-        info.update(S_nnz=panel.info["nnz"],
-                    S_mb=round(panel.info["nnz"] * 12 / 1e6 + (panel.info["L"] + 1) * 4 / 1e6, 1),
-                    S_build_seconds=round(s_secs, 4), device_setup_seconds=0.0)
+        s_secs = panel.system(with_rhs=True, budget=self._s_budget(panel))
+        if panel.matrix is None:
+            # S would not fit the headroom: the matrix-free operator over the
+            # cells (core/solve.py:205-218, 251-271), on the device
+            if self.solver == "explicit":
+                raise MemoryError(f"S would take about {panel.s_skipped / 1e9:.3g} GB, more "
+                                  "than the device memory headroom; raise max_memory_gb or "
+                                  "use solver='stream_cg'")
+            panel.stream_matrix()
+            info["solver"] = "stream_cg"
+            if self.solver == "auto":
+                info["fallback"] = (f"S would take about {panel.s_skipped / 1e9:.3g} GB > the "
+                                    f"{self._s_budget(panel) / 1e9:.3g} GB of device memory "
+                                    "headroom: matrix-free stream_cg on the device")
+            info.update(S_nnz=0, S_mb=0.0, S_build_seconds=0.0)
+        else:
+            info.update(S_nnz=panel.info["nnz"],
+                        S_mb=round(panel.info["nnz"] * 12 / 1e6
+                                   + (panel.info["L"] + 1) * 4 / 1e6, 1),
+                        S_build_seconds=round(s_secs, 4))
+        info["device_setup_seconds"] = 0.0
         ncomp = panel.components()
         L, m = panel.info["L"], panel.info["m"]
         rhs = panel.tensor("rhs", torch.float64).view(L, m)

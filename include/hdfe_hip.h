@@ -62,6 +62,18 @@ int hdfe_pcg_solve(void *h, int64_t k, const double *b, const uint8_t *zero, con
                    int64_t *iterations, int *converged, double *max_rel, int64_t *outer,
                    int on_device, void *stream);
 
+/* A matrix-free S over a panel's identifying cells, for the same solves as
+ * an explicit S (hdfe_pcg_solve, hdfe_csr_matmat, hdfe_destroy): the
+ * reference's stream_cg fallback (`_nb_stream_matvec`,
+ * core/kernels_base.py:477-507, chosen at core/solve.py:205-218, 251-271
+ * when S does not fit).  Device arrays: igstart (Gi + 1) cells per group,
+ * icodes (D, ld) the cells' level codes per non-streamed dimension, cell_n
+ * (M) the cells' weights, dinv (L); offs (host, D) each dimension's first
+ * level.  The arrays must outlive the handle (they stay the panel's). */
+int hdfe_stream_create(int64_t L, int64_t Gi, int64_t M, int D, const int *igstart,
+                       const int *icodes, int64_t ld, const int64_t *offs, const double *cell_n,
+                       const double *dinv, void **out);
+
 /* y = S x (k columns, FP64), host or device.  The operator of
  * _nb_csr_matmat, core/kernels_base.py:361-371. */
 int hdfe_csr_matmat(void *h, int64_t k, const double *x, double *y, int on_device);
@@ -160,6 +172,13 @@ int hdfe_panel_copy_wait(void *h);
  * core/kernels_base.py:192-358) with its solver handle in *matrix (owned by the
  * panel: use it with hdfe_pcg_solve, never hdfe_destroy).  *seconds: wall time. */
 int hdfe_panel_system(void *h, int with_rhs, void **matrix, double *seconds);
+
+/* The explicit S build's budget in bytes (set >= 0; -1: no limit; < -1:
+ * unchanged): over it hdfe_panel_system builds diag and rhs but no S
+ * (*matrix NULL), the reference's _STooLarge test before allocating S
+ * (core/solve.py:148-194); *skipped is the estimate of the S not built (0:
+ * built).  Either output may be NULL. */
+int hdfe_panel_budget(void *h, int64_t set, int64_t *budget, int64_t *skipped);
 
 /* Components of the first non-streamed dimension through the identifying
  * groups, ranked by rows with ties to the smallest level
