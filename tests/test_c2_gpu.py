@@ -97,3 +97,21 @@ def test_c2_sample_matches_the_oracle(oracle_mod):
         assert rel(r.beta, ref["beta"]) < 1e-8, precision
         G = r.Gamma.cpu().numpy()
         assert np.abs(G - ref["Gamma"]).max() <= 1e-8 * np.abs(ref["Gamma"]).max(), precision
+
+
+def test_c2_stream_cg_matches_the_explicit_fit(c2):
+    """The matrix-free fallback at the full size: the same coefficients and
+    effects as the explicit S."""
+    import torch
+    from hdfe_stream_amd.estimator import feols_stream
+    d, fits = c2
+    print("C2 stream_cg", flush=True)
+    cols = {k: torch.from_numpy(d[k]).cuda() for k in ("worker_id", "firm_id", "year",
+                                                        "x1", "x2", "y")}
+    r = feols_stream(FML, cols, precision="fp64", tol=1e-10, solver="stream_cg")
+    a = fits["fp64"]
+    assert r.solver_info["solver"] == "stream_cg" and r.solver_info["converged"]
+    assert rel(r.beta, a.beta) <= 1e-8
+    for dim in ("firm_id", "year"):
+        va, vb = np.asarray(a.fixef(dim)[1]), np.asarray(r.fixef(dim)[1])
+        assert np.abs(va - vb).max() <= 1e-8 * np.abs(va).max(), dim
