@@ -293,6 +293,12 @@ class HDFEResult:
         """(level values, effects) of one fixed-effect dimension."""
         return self.fe_levels[dim], self.fe_values[dim]
 
+    def solve_normal_equations(self, data, columns, block=16, **options):
+        """u_j = (Z'WZ)^- Z'v_j for the columns v_j of `data`
+        (normal_equations.py:115-231; see `normal_equations`)."""
+        from .normal_equations import solve_normal_equations
+        return solve_normal_equations(self, data, columns, block=block, **options)
+
     def resid(self):
         """Residuals in the source's row order (NaN for dropped rows)."""
         perm = self.panel.array("perm", np.int32)[: self.n_obs]
