@@ -293,6 +293,23 @@ class HDFEResult:
         """(level values, effects) of one fixed-effect dimension."""
         return self.fe_levels[dim], self.fe_values[dim]
 
+    def vcov(self, vcov="iid", data=None):
+        """The coefficients' variance matrix: 'iid', 'hetero' ('HC1'), or
+        'CRV1:<cols>' / {'CRV1': '<cols>'} with '+' for multi-way clustering
+        and '^' for intersections (core/inference.py:430-668, 880-920).
+        Cluster columns that are not fixed effects are read from `data` (the
+        data the fit was given).  Returns (V, details)."""
+        from .vcov import VcovEngine
+        eng = getattr(self, "_vcov_engine", None)
+        if eng is None or (data is not None and eng.data is not data):
+            eng = self._vcov_engine = VcovEngine(self, data)
+        return eng.vcov(vcov)
+
+    def se(self, vcov="iid", data=None):
+        """Standard errors under `vcov` (see `vcov`)."""
+        V, _ = self.vcov(vcov, data)
+        return np.sqrt(np.diag(V))
+
     def solve_normal_equations(self, data, columns, block=16, **options):
         """u_j = (Z'WZ)^- Z'v_j for the columns v_j of `data`
         (normal_equations.py:115-231; see `normal_equations`)."""

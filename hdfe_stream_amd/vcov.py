@@ -1,0 +1,199 @@
+"""Variance-covariance matrices of a fit's coefficients on the device (the
+reference's step 4: the meats of `_pass2`, core/inference.py:430-668, and
+their small-sample factors, :880-920).
+
+With X~ the covariates partialled on the fixed effects (M_F x, per row),
+e the residuals, w the weights and B = X~'W X~ (the fit's V'M_D V block):
+
+    iid     rss / (N - K) B^-1
+    hetero  N / (N - K) B^-1 M B^-1,          M = sum_i w_i^2 e_i^2 x~_i x~_i'
+    CRV1    G/(G-1) (N-1)/(N-Kc) B^-1 M_c B^-1, M_c = sum_c s_c s_c',
+            s_c = sum_{i in c} w_i e_i x~_i
+
+K = k + k_fe, k_fe the fixed effects' free parameters: every level less the
+redundant ones (the connected components of the first two dimensions, and
+one per further dimension; the reference's fe_dof="exact",
+core/estimator.py:446-464).  For CRV1, Kc drops the levels of fixed effects
+nested in the clusters (every level inside one cluster) and adds one back per
+nested effect; a multi-way request "a+b" combines the terms a, b, a^b by
+inclusion-exclusion with G the smallest one-way count (pyfixest's
+conventions, which the reference follows).  Scores, cluster sums and the
+meats' tall products run on the device (fixed-order sums); the k x k algebra
+on the host.
+
+Assumed: the (streamed, first) pair of fixed effects has at least as many
+components as any other pair (true when the further dimensions, e.g. years,
+link every group).
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+from itertools import combinations
+
+import numpy as np
+
+from . import _lib
+
+
+def _parse(vcov):
+    """('iid' | 'hetero' | 'CRV1', [cluster terms]) from 'iid', 'hetero',
+    'HC1', 'CRV1:a+b' or {'CRV1': 'a+b'}."""
+    if isinstance(vcov, dict):
+        if len(vcov) != 1:
+            raise ValueError("vcov dict must have one key")
+        (kind, req), = vcov.items()
+        vcov = f"{kind}:{req}"
+    v = str(vcov).strip()
+    if v in ("iid",):
+        return "iid", None
+    if v in ("hetero", "HC1"):
+        return "hetero", None
+    if v.startswith("CRV1:"):
+        req = v.split(":", 1)[1]
+        parts = [[c.strip() for c in p.split("^")] for p in req.split("+") if p.strip()]
+        if not parts:
+            raise ValueError(f"no cluster variable in {vcov!r}")
+        return "CRV1", parts
+    raise ValueError(f"unknown vcov {vcov!r}: use 'iid', 'hetero', 'HC1' or 'CRV1:<cols>'")
+
+
+class VcovEngine:
+    """The pieces every vcov of one fit shares: X~, the scores w e x~ and B
+    (device), the fixed effects' dof."""
+
+    def __init__(self, result, data=None):
+        import torch
+        self.res = result
+        panel = result.panel
+        info = panel.info
+        self.n, self.k = info["n"], len(result.coefnames)
+        m = info["m"]
+        self.data = data
+        self.w = panel.tensor("w", torch.float64) if info["weighted"] else None
+        e = result.resid_sorted.to(torch.float64)
+        self.rss = float(((e * e) * (self.w if self.w is not None else 1.0)).sum())
+        self.N = self.n
+        xj = list(range(1, m))
+        self.B = np.asarray(result.A)[np.ix_(xj, xj)] if self.k else np.zeros((0, 0))
+        if self.k:
+            Xt = torch.empty((self.n, self.k), dtype=torch.float64, device="cuda")
+            cols = np.arange(1, m, dtype=np.int32)
+            torch.cuda.synchronize()
+            _lib.check(_lib.load().hdfe_panel_partial(panel._h, C.c_void_p(result.Gamma.data_ptr()),
+                                                      cols.ctypes.data, self.k,
+                                                      C.c_void_p(Xt.data_ptr())),
+                       "hdfe_panel_partial")
+            f = e if self.w is None else self.w * e
+            self.U = (Xt * f[:, None]).contiguous()
+        else:
+            self.U = None
+        # the fixed effects' parameters: every level, less the redundant
+        G = info["G"]
+        nlev = [int(v) for v in panel.n_levels]
+        self.levels = {result.fe_names[0]: G}
+        for t, name in enumerate(result.fe_names[1:]):
+            self.levels[name] = nlev[t]
+        self.ncomp = max(int(result.n_components), 1)
+        self.D = len(result.fe_names)
+
+    def k_fe(self, nested=()):
+        total = sum(self.levels.values())
+        if self.D < 2:
+            return total
+        # the first pair's components count unless both of its dimensions are
+        # nested (then its redundant levels are among those not counted)
+        first = self.res.fe_names[:2]
+        comps = 1 if all(d in nested for d in first) else self.ncomp
+        return total - (comps + self.D - 2)
+
+    def _codes(self, cols):
+        """Each sorted row's code of the cluster formed by `cols` (fixed
+        effects of the fit, or columns of the data the fit was given)."""
+        import torch
+        res, panel = self.res, self.res.panel
+        perm = panel.tensor("perm", torch.int32).long()[:self.n]
+        keys = []
+        for c in cols:
+            if c == res.fe_names[0]:
+                keys.append(panel.tensor("g", torch.int32).long()[:self.n])
+            elif c in res.fe_names:
+                t = res.fe_names.index(c) - 1
+                keys.append(panel.tensor("codes", torch.int32).view(-1, self.n)[t].long())
+            else:
+                if self.data is None:
+                    raise ValueError(f"cluster column {c!r} is not a fixed effect: pass the "
+                                     "fit's data to vcov()")
+                from .estimator import _column
+                col = _column(self.data, c)
+                v = (col if torch.is_tensor(col) else torch.from_numpy(np.asarray(col))).to(
+                    "cuda")
+                keys.append(torch.unique(v[perm], return_inverse=True)[1])
+        code = keys[0]
+        for kk in keys[1:]:
+            code = torch.unique(torch.stack([code, kk], 1), dim=0, return_inverse=True)[1]
+        if len(keys) == 1:
+            code = torch.unique(code, return_inverse=True)[1]
+        return code
+
+    def _meat(self, code):
+        import torch
+        from .leaveout_se import tall_dot
+        G = int(code.max()) + 1 if code.numel() else 0
+        S = torch.zeros((G, self.k), dtype=torch.float64, device="cuda")
+        S.index_add_(0, code, self.U)
+        return tall_dot(S, S).cpu().numpy(), G
+
+    def _nested(self, cols):
+        """Fixed effects all of whose levels lie in one cluster of `cols`."""
+        import torch
+        out = []
+        code = None
+        for name in self.res.fe_names:
+            if set(cols) <= {name}:
+                out.append(name)
+                continue
+            if code is None:
+                code = self._codes(cols)
+            lv = self._codes([name])
+            nl = int(lv.max()) + 1
+            lo = torch.full((nl,), 2**62, dtype=torch.int64, device="cuda")
+            hi = torch.full((nl,), -1, dtype=torch.int64, device="cuda")
+            lo.scatter_reduce_(0, lv, code, "amin")
+            hi.scatter_reduce_(0, lv, code, "amax")
+            if bool((lo == hi).all()):
+                out.append(name)
+        return out
+
+    def vcov(self, spec):
+        kind, parts = _parse(spec)
+        k, N = self.k, self.N
+        if not k:
+            return np.zeros((0, 0)), {"kind": kind}
+        Binv = np.linalg.inv(self.B)
+        K = k + self.k_fe()
+        ratio = (lambda a, b: a / b if b else np.nan)
+        if kind == "iid":
+            return ratio(self.rss, N - K) * Binv, {"kind": kind, "K": K, "df": N - K}
+        if kind == "hetero":
+            from .leaveout_se import tall_dot
+            M = tall_dot(self.U, self.U).cpu().numpy()
+            return ratio(N, N - K) * Binv @ M @ Binv, {"kind": kind, "K": K, "df": N - K}
+        terms = []
+        for r in range(1, len(parts) + 1):
+            for combo in combinations(range(len(parts)), r):
+                cols = list(dict.fromkeys(c for i in combo for c in parts[i]))
+                terms.append((cols, (-1) ** (r + 1), r == 1))
+        meats, Gs = [], []
+        for cols, sign, single in terms:
+            M, G = self._meat(self._codes(cols))
+            meats.append((sign, M))
+            if single:
+                Gs.append(G)
+        Gm = min(Gs)
+        nest = list(dict.fromkeys(d for cols, _, _ in terms for d in self._nested(cols)))
+        Kc = k + self.k_fe(nest) - sum(self.levels[d] for d in nest) + len(nest)
+        adj = ratio(Gm, Gm - 1) * ratio(N - 1, N - Kc)
+        V = sum(sign * adj * (Binv @ M @ Binv) for sign, M in meats)
+        return V, {"kind": "CRV1", "clusters": Gs, "Kc": Kc, "adj": adj, "df": Gm - 1,
+                   "nested": nest}
