@@ -285,6 +285,8 @@ class HDFEResult:
     resid_sorted: object = None
     gam_y: object = None
     geff: object = None
+    vcov_matrix: object = None      # with the fit's vcov= option
+    vcov_details: dict = None
 
     def coef(self):
         return dict(zip(self.coefnames, self.beta.tolist()))
@@ -331,7 +333,7 @@ class StreamingHDFE:
     def __init__(self, y, x, fe, *, tol=1e-10, maxiter=5000, rhs_block=8, solver="auto",
                  precond="jacobi", fixef_rm="singleton", weights=None, stream=None,
                  device="rocm", precision="fp64", inner_tol=1e-5, keep_intermediates=False,
-                 verbose=False, spmm_timing=False, max_memory_gb=None, **ignored):
+                 verbose=False, spmm_timing=False, max_memory_gb=None, vcov=None, **ignored):
         if device != "rocm":
             raise ValueError("device must be 'rocm' (this package is the MI355X backend)")
         if solver not in ("auto", "explicit", "stream_cg", "within"):
@@ -359,6 +361,10 @@ class StreamingHDFE:
         self.tol, self.maxiter = tol, maxiter
         self.solver = solver
         self.max_memory_gb = max_memory_gb
+        self.vcov = vcov
+        if vcov is not None:
+            from .vcov import _parse
+            _parse(vcov)
         self.rhs_block = max(1, int(rhs_block))
         self.fixef_rm = fixef_rm
         self.weights = weights
@@ -503,6 +509,13 @@ class StreamingHDFE:
                          fe_levels=levels, fe_values=values, n_obs=panel.info["n"],
                          n_components=ncomp, A=A, diagnostics=diag, panel=panel, Gamma=Gamma,
                          resid_sorted=resid, gam_y=gam_y, geff=geff)
+        if self.vcov is not None:
+            # the coefficients' variance at fit time (the reference's vcov=
+            # of fit, core/estimator.py:467): cluster columns that are not
+            # fixed effects come from the fit's data
+            t5 = time.time()
+            res.vcov_matrix, res.vcov_details = res.vcov(self.vcov, data)
+            res.diagnostics["seconds_vcov"] = round(time.time() - t5, 4)
         if self.verbose:
             print(f"fit: {diag}")
         return res

@@ -18,8 +18,8 @@ nested in the clusters (every level inside one cluster) and adds one back per
 nested effect; a multi-way request "a+b" combines the terms a, b, a^b by
 inclusion-exclusion with G the smallest one-way count (pyfixest's
 conventions, which the reference follows).  Scores, cluster sums and the
-meats' tall products run on the device (fixed-order sums); the k x k algebra
-on the host.
+meats' tall products run on the device in a fixed order (the same result
+every run); the k x k algebra on the host.
 
 Assumed: the (streamed, first) pair of fixed effects has at least as many
 components as any other pair (true when the further dimensions, e.g. years,
@@ -109,7 +109,15 @@ class VcovEngine:
 
     def _codes(self, cols):
         """Each sorted row's code of the cluster formed by `cols` (fixed
-        effects of the fit, or columns of the data the fit was given)."""
+        effects of the fit, or columns of the data the fit was given); codes
+        are dense, 0 .. clusters - 1.  Kept per column set."""
+        key = tuple(cols)
+        cache = self.__dict__.setdefault("_code_cache", {})
+        if key not in cache:
+            cache[key] = self._make_codes(cols)
+        return cache[key]
+
+    def _make_codes(self, cols):
         import torch
         res, panel = self.res, self.res.panel
         perm = panel.tensor("perm", torch.int32).long()[:self.n]
@@ -136,16 +144,32 @@ class VcovEngine:
             code = torch.unique(code, return_inverse=True)[1]
         return code
 
-    def _meat(self, code):
+    def _meat(self, cols):
+        """sum_c s_c s_c' and the cluster count, s_c the scores' sum over the
+        cluster's rows in a fixed order (no float atomics): the streamed
+        dimension's groups are the panel's contiguous row ranges; any other
+        clustering is sorted stably by cluster first."""
         import torch
         from .leaveout_se import tall_dot
-        G = int(code.max()) + 1 if code.numel() else 0
-        S = torch.zeros((G, self.k), dtype=torch.float64, device="cuda")
-        S.index_add_(0, code, self.U)
+        res, panel = self.res, self.res.panel
+        if list(cols) == [res.fe_names[0]]:
+            G = panel.info["G"]
+            S = torch.empty((G, self.k), dtype=torch.float64, device="cuda")
+            torch.cuda.synchronize()
+            _lib.check(_lib.load().hdfe_kss_group_sums(G, self.k, panel.ptr("gstart")[0], None,
+                                                       None, self.U.data_ptr(), S.data_ptr()),
+                       "hdfe_kss_group_sums")
+        else:
+            code = self._codes(cols)
+            G = int(code.max()) + 1 if code.numel() else 0
+            order = torch.argsort(code, stable=True)
+            lengths = torch.bincount(code, minlength=G)
+            S = torch.segment_reduce(self.U[order], "sum", lengths=lengths, axis=0)
         return tall_dot(S, S).cpu().numpy(), G
 
     def _nested(self, cols):
-        """Fixed effects all of whose levels lie in one cluster of `cols`."""
+        """Fixed effects all of whose levels lie in one cluster of `cols`: as
+        many distinct (level, cluster) pairs over the rows as levels."""
         import torch
         out = []
         code = None
@@ -155,13 +179,10 @@ class VcovEngine:
                 continue
             if code is None:
                 code = self._codes(cols)
+                nc = int(code.max()) + 1
             lv = self._codes([name])
-            nl = int(lv.max()) + 1
-            lo = torch.full((nl,), 2**62, dtype=torch.int64, device="cuda")
-            hi = torch.full((nl,), -1, dtype=torch.int64, device="cuda")
-            lo.scatter_reduce_(0, lv, code, "amin")
-            hi.scatter_reduce_(0, lv, code, "amax")
-            if bool((lo == hi).all()):
+            pairs = torch.unique(lv * nc + code).numel()
+            if pairs == int(lv.max()) + 1:
                 out.append(name)
         return out
 
@@ -186,7 +207,7 @@ class VcovEngine:
                 terms.append((cols, (-1) ** (r + 1), r == 1))
         meats, Gs = [], []
         for cols, sign, single in terms:
-            M, G = self._meat(self._codes(cols))
+            M, G = self._meat(cols)
             meats.append((sign, M))
             if single:
                 Gs.append(G)
