@@ -51,6 +51,8 @@ SIGNATURES = [
     ("hdfe_panel_system", I32, [P, I32, P, P]),
     ("hdfe_panel_budget", I32, [P, I64, P, P]),
     ("hdfe_panel_components", I32, [P, P]),
+    ("hdfe_panel_pair_components", I32, [P, I32, I32, P]),
+    ("hdfe_panel_scores", I32, [P, P, P, I32, P, P, P]),
     ("hdfe_panel_normalize", I32, [P, P, I32, P]),
     ("hdfe_panel_assemble", I32, [P, P, P]),
     ("hdfe_panel_pass2", I32, [P, P, P, P, I32, P, P, P]),

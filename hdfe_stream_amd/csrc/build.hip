@@ -1707,6 +1707,23 @@ __device__ int uf_root(int *parent, int x) {
     }
 }
 
+// Joins the sets of x and y: hooks the larger root under the smaller with a
+// CAS, retrying from the values a failed CAS returns until both walks meet.
+__device__ void uf_join(int *parent, int x, int y) {
+    int a = uf_root(parent, x), b = uf_root(parent, y);
+    while (a != b) {
+        if (a > b) {
+            int t = a;
+            a = b;
+            b = t;
+        }
+        int old = atomicCAS(&parent[b], b, a);
+        if (old == b) break;
+        b = uf_root(parent, old);
+        a = uf_root(parent, a);
+    }
+}
+
 __global__ void k_uf_init(int64_t n, int *parent) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x)
@@ -1734,20 +1751,143 @@ __global__ void k_uf_union(int64_t Gi, const int *__restrict__ igstart,
         for (int c = s + 1 + sub; c < e; c += W) {
             const int lc = icodes[c];
             if (lc == a0 || lc == icodes[c - 1]) continue;
-            int a = uf_root(parent, a0), b = uf_root(parent, lc);
-            while (a != b) {
-                if (a > b) {
-                    int t = a;
-                    a = b;
-                    b = t;
-                }
-                int old = atomicCAS(&parent[b], b, a);
-                if (old == b) break;
-                b = uf_root(parent, old);
-                a = uf_root(parent, a);
-            }
+            uf_join(parent, a0, lc);
         }
     }
+}
+
+// Unions of the rows' pairs of levels (a_i, nb0 + b_i) of two non-streamed
+// dimensions; a row repeating the pair of the row before it is skipped.
+__global__ void k_uf_rows(int64_t n, const int *__restrict__ ca, const int *__restrict__ cb,
+                          int nb0, int *parent) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int x = ca[i], y = nb0 + cb[i];
+        if (i > 0 && ca[i - 1] == x && nb0 + cb[i - 1] == y) continue;
+        uf_join(parent, x, y);
+    }
+}
+
+// Sets bit x * ny + y of a bitmap.  A bit already set is only read, so the
+// many rows of a few-level pair do not contend on atomics.
+__device__ __forceinline__ void set_edge(unsigned *bits, int x, int y, int64_t ny) {
+    const uint64_t b = (uint64_t)x * (uint64_t)ny + (uint64_t)y;
+    unsigned *wd = bits + (b >> 5);
+    const unsigned mk = 1u << (b & 31);
+    if (!(*wd & mk)) atomicOr(wd, mk);
+}
+
+// The distinct edges (ca[i], cb[i]) of two non-streamed dimensions' rows.
+__global__ void k_pair_bits(int64_t n, const int *__restrict__ ca, const int *__restrict__ cb,
+                            int64_t ny, unsigned *bits) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        set_edge(bits, ca[i], cb[i], ny);
+}
+
+// The streamed dimension's groups join their levels of cb: the edges
+// (level on the group's first row, level on each other row), W lanes per
+// group.
+template <int W>
+__global__ void k_group_pair_bits(int64_t G, const int *__restrict__ gstart,
+                                  const int *__restrict__ cb, int64_t ny, unsigned *bits) {
+    const int sub = threadIdx.x % W;
+    for (int64_t g = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / W; g < G;
+         g += (int64_t)gridDim.x * blockDim.x / W) {
+        const int s = gstart[g], e = gstart[g + 1];
+        const int x = cb[s];
+        for (int i = s + 1 + sub; i < e; i += W) {
+            const int y = cb[i];
+            if (y != x) set_edge(bits, x, y, ny);
+        }
+    }
+}
+
+// A pair with a side of at most 64 levels (years, say): its components are
+// those of that side's levels, two of them linked when some level of the
+// other side (or group of the streamed dimension) has rows in both.  Each
+// such level's 64-bit mask of small-side levels m is ORed into adj[y] for
+// every y in m: first into the workgroup's copy in LDS, then into adj once
+// per workgroup (integer ORs: the result is exact).
+__device__ __forceinline__ void adj_add(unsigned long long *lds, unsigned long long m) {
+    for (unsigned long long v = m; v; v &= v - 1) atomicOr(&lds[__builtin_ctzll(v)], m);
+}
+
+__device__ __forceinline__ void adj_flush(unsigned long long *lds, unsigned long long *adj) {
+    __syncthreads();
+    for (int y = threadIdx.x; y < 64; y += blockDim.x)
+        if (lds[y]) atomicOr(&adj[y], lds[y]);
+}
+
+// The streamed dimension's groups: each group's mask of its levels of cb.
+template <int W>
+__global__ void k_group_masks(int64_t G, const int *__restrict__ gstart,
+                              const int *__restrict__ cb, unsigned long long *adj) {
+    __shared__ unsigned long long lds[64];
+    for (int y = threadIdx.x; y < 64; y += blockDim.x) lds[y] = 0;
+    __syncthreads();
+    const int sub = threadIdx.x % W;
+    for (int64_t g = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / W; g < G;
+         g += (int64_t)gridDim.x * blockDim.x / W) {
+        unsigned long long m = 0;
+        for (int i = gstart[g] + sub; i < gstart[g + 1]; i += W) m |= 1ull << cb[i];
+#pragma unroll
+        for (int o = W / 2; o > 0; o >>= 1) m |= __shfl_xor(m, o, W);
+        if (sub == 0 && (m & (m - 1))) adj_add(lds, m);
+    }
+    adj_flush(lds, adj);
+}
+
+// Another dimension: each of its levels' mask of small-side levels, from
+// the rows (masks (na), zeroed first), then the masks into adj.
+__global__ void k_row_masks(int64_t n, const int *__restrict__ ca, const int *__restrict__ cb,
+                            unsigned long long *masks) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const unsigned long long bit = 1ull << cb[i];
+        unsigned long long *mp = masks + ca[i];
+        if (!(*mp & bit)) atomicOr(mp, bit);
+    }
+}
+
+__global__ void k_mask_adj(int64_t na, const unsigned long long *__restrict__ masks,
+                           unsigned long long *adj) {
+    __shared__ unsigned long long lds[64];
+    for (int y = threadIdx.x; y < 64; y += blockDim.x) lds[y] = 0;
+    __syncthreads();
+    for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x < na;
+         x += (int64_t)gridDim.x * blockDim.x) {
+        const unsigned long long m = masks[x];
+        if (m & (m - 1)) adj_add(lds, m);
+    }
+    adj_flush(lds, adj);
+}
+
+// Unions of the edges a bitmap marks (bit x * ny + y joins x and yoff + y).
+__global__ void k_uf_bits(int64_t nwords, const unsigned *__restrict__ bits, int64_t ny,
+                          int yoff, int *parent) {
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < nwords;
+         q += (int64_t)gridDim.x * blockDim.x) {
+        unsigned v = bits[q];
+        while (v) {
+            const int t = __builtin_ctz(v);
+            v &= v - 1;
+            const uint64_t b = ((uint64_t)q << 5) + t;
+            uf_join(parent, (int)(b / ny), yoff + (int)(b % ny));
+        }
+    }
+}
+
+// The roots among n union-find entries (one per wave's lanes, summed with
+// integer atomics: exact).
+__global__ void k_uf_count_roots(int64_t n, const int *__restrict__ parent,
+                                 unsigned long long *count) {
+    unsigned long long c = 0;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        c += parent[i] == (int)i;
+    c = wave_sum(c);
+    if ((threadIdx.x & (kWave - 1)) == 0 && c) atomicAdd(count, c);
 }
 
 __global__ void k_uf_flatten(int64_t n, int *parent) {
@@ -2178,6 +2318,105 @@ __global__ void k_partial(int64_t G, int D, int m, const int *__restrict__ gstar
             mu = run_sum<W>(mu) / Wg;
             for (int i = s + sub; i < e; i += W) out[(int64_t)i * kc + c] -= mu;
         }
+    }
+}
+
+// The coefficients' scores u_i = w_i e_i x~_i (x~ the covariates cols,
+// kc <= K of them, partialled on the fixed effects as k_partial makes them;
+// e the residuals) and their sums in one pass over the groups
+// (core/inference.py:430-668, the meats of the residual pass): each
+// workgroup writes part[b] = [sum w e^2, sum_i u_i u_i' (upper triangle),
+// sum_g s_g s_g' (upper triangle; s_g = the sum of u over group g's rows)],
+// summed in a fixed order (lanes by a butterfly, waves in order), and U
+// (n, kc) holds the scores when not null.  Covariates c >= kc are zero.
+template <int K>
+struct ScoreAcc {
+    static constexpr int T = K * (K + 1) / 2;
+    static constexpr int NV = 1 + 2 * T;
+};
+
+template <int W, int K>
+__global__ void __launch_bounds__(256)
+    k_scores(int64_t G, int D, int m, const int *__restrict__ gstart, const int *__restrict__ codes,
+             int64_t ld, const int64_t *__restrict__ offs, const double *__restrict__ V,
+             const double *__restrict__ w, const double *__restrict__ Gm,
+             const int *__restrict__ cols, int kc, const double *__restrict__ e,
+             double *__restrict__ U, double *__restrict__ part) {
+    constexpr int T = ScoreAcc<K>::T, NV = ScoreAcc<K>::NV;
+    const int sub = threadIdx.x % W;
+    double acc[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) acc[v] = 0.0;
+    int col[K];
+#pragma unroll
+    for (int c = 0; c < K; ++c) col[c] = c < kc ? cols[c] : -1;
+    for (int64_t g = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / W; g < G;
+         g += (int64_t)gridDim.x * blockDim.x / W) {
+        const int s = gstart[g], en = gstart[g + 1];
+        // the group's weighted means of the covariates less their fits
+        double Wg = 0.0, mu[K];
+#pragma unroll
+        for (int c = 0; c < K; ++c) mu[c] = 0.0;
+        for (int i = s + sub; i < en; i += W) {
+            const double wi = w ? w[i] : 1.0;
+            Wg += wi;
+#pragma unroll
+            for (int c = 0; c < K; ++c) {
+                if (col[c] < 0) continue;
+                double u = V[(int64_t)i * m + col[c]];
+                for (int d = 0; d < D; ++d) u -= Gm[(offs[d] + codes[d * ld + i]) * m + col[c]];
+                mu[c] += wi * u;
+            }
+        }
+        Wg = run_sum<W>(Wg);
+#pragma unroll
+        for (int c = 0; c < K; ++c) mu[c] = run_sum<W>(mu[c]) / Wg;
+        double sg[K];
+#pragma unroll
+        for (int c = 0; c < K; ++c) sg[c] = 0.0;
+        for (int i = s + sub; i < en; i += W) {
+            const double wi = w ? w[i] : 1.0, f = wi * e[i];
+            acc[0] += f * e[i];
+            double x[K];
+#pragma unroll
+            for (int c = 0; c < K; ++c) {
+                x[c] = 0.0;
+                if (col[c] < 0) continue;
+                double u = V[(int64_t)i * m + col[c]];
+                for (int d = 0; d < D; ++d) u -= Gm[(offs[d] + codes[d * ld + i]) * m + col[c]];
+                x[c] = f * (u - mu[c]);
+                sg[c] += x[c];
+                if (U) U[(int64_t)i * kc + c] = x[c];
+            }
+            int t = 1;
+#pragma unroll
+            for (int a = 0; a < K; ++a)
+#pragma unroll
+                for (int b = a; b < K; ++b) acc[t++] += x[a] * x[b];
+        }
+#pragma unroll
+        for (int c = 0; c < K; ++c) sg[c] = run_sum<W>(sg[c]);
+        if (sub == 0) {
+            int t = 1 + T;
+#pragma unroll
+            for (int a = 0; a < K; ++a)
+#pragma unroll
+                for (int b = a; b < K; ++b) acc[t++] += sg[a] * sg[b];
+        }
+    }
+    // the workgroup's sums: each wave by a butterfly, then the waves in order
+    __shared__ double red[256 / kWave][NV];
+    const int lane = threadIdx.x % kWave, wv = threadIdx.x / kWave;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        const double t = wave_sum(acc[v]);
+        if (lane == 0) red[wv][v] = t;
+    }
+    __syncthreads();
+    for (int v = threadIdx.x; v < NV; v += blockDim.x) {
+        double t = 0.0;
+        for (int q = 0; q < (int)(blockDim.x / kWave); ++q) t += red[q][v];
+        part[(int64_t)blockIdx.x * NV + v] = t;
     }
 }
 
@@ -3374,6 +3613,115 @@ int hdfe_panel_components(void *h, int64_t *ncomp) {
     return HDFE_STATUS_OK;
 }
 
+// Connected components of the bipartite graph of two fixed-effect
+// dimensions' levels linked by the rows (core/passes.py:964-979 for pairs
+// with the streamed dimension, :525-527 for the others): a, b index the
+// dimensions as the fit orders them, 0 the streamed one, t + 1 the
+// non-streamed t, a < b.  Every level is in the sample.  With the streamed
+// dimension every group has a row, so the count is that of b's levels
+// joined through the groups (unions within each group's rows).
+int hdfe_panel_pair_components(void *h, int a, int b, int64_t *count) {
+    ARG_CHECK(h && count, "hdfe_panel_pair_components: null argument");
+    Panel &P = *(Panel *)h;
+    ARG_CHECK(a >= 0 && a < b && b <= P.D, "hdfe_panel_pair_components: bad pair");
+    hipStream_t s = P.s;
+    int ta = a - 1, tb = b - 1;
+    // the side of at most 64 levels as b where there is one (a non-streamed
+    // pair is symmetric)
+    if (a > 0 && P.nlev[tb] > 64 && P.nlev[ta] <= 64) std::swap(ta, tb);
+    const int64_t nb = P.nlev[tb];
+    const int64_t na = a == 0 ? 0 : P.nlev[ta];
+    const int64_t nv = na + nb;
+    ARG_CHECK(nv < INT32_MAX, "hdfe_panel_pair_components: too many levels");
+    if (nb <= 64) {
+        DevBuf<unsigned long long> adj, masks;
+        DEV_ALLOC(adj, 64, s);
+        HIP_TRY(hipMemsetAsync(adj.p, 0, 64 * 8, s));
+        const int *cbp = P.codes.p + (int64_t)tb * P.n;
+        if (P.n > 0 && a == 0) {
+#define HDFE_L(W)                                                                           \
+    hipLaunchKernelGGL(k_group_masks<W>, dim3(grid1(P.G * W, 256, 2048)), dim3(T256), 0, s,  \
+                       P.G, P.gstart.p, cbp, adj.p)
+            HDFE_BY_WIDTH(run_width(P.n, P.G), HDFE_L)
+#undef HDFE_L
+        } else if (P.n > 0) {
+            DEV_ALLOC(masks, na, s);
+            HIP_TRY(hipMemsetAsync(masks.p, 0, na * 8, s));
+            hipLaunchKernelGGL(k_row_masks, dim3(grid1(P.n)), dim3(T256), 0, s, P.n,
+                               P.codes.p + (int64_t)ta * P.n, cbp, masks.p);
+            hipLaunchKernelGGL(k_mask_adj, dim3(grid1(na, 256, 2048)), dim3(T256), 0, s, na,
+                               masks.p, adj.p);
+        }
+        HIP_TRY(hipGetLastError());
+        std::vector<unsigned long long> A;
+        HIP_TRY(to_host(A, adj.p, 64, s));
+        // components of the small side's nb levels (every level has rows)
+        int par[64];
+        for (int y = 0; y < 64; ++y) par[y] = y;
+        auto root = [&](int y) {
+            while (par[y] != y) y = par[y] = par[par[y]];
+            return y;
+        };
+        for (int y = 0; y < nb; ++y)
+            for (unsigned long long v = A[y]; v; v &= v - 1) {
+                const int z = __builtin_ctzll(v), ry = root(y), rz = root(z);
+                if (ry != rz) par[std::max(ry, rz)] = std::min(ry, rz);
+            }
+        int64_t c = 0;
+        for (int y = 0; y < nb; ++y) c += root(y) == y;
+        *count = c;
+        return HDFE_STATUS_OK;
+    }
+    DevBuf<int> parent;
+    DevBuf<unsigned long long> cnt;
+    DEV_ALLOC(parent, std::max<int64_t>(nv, 1), s);
+    DEV_ALLOC(cnt, 1, s);
+    hipLaunchKernelGGL(k_uf_init, dim3(grid1(nv)), dim3(T256), 0, s, nv, parent.p);
+    // the distinct edges first when their bitmap is modest (a few-level
+    // dimension: its rows repeat few pairs), else a union per row
+    const int64_t nx = a == 0 ? nb : na;
+    const int64_t nbits = nx * nb;
+    if (P.n > 0 && nbits <= ((int64_t)1 << 31)) {
+        const int64_t nw = (nbits + 31) / 32;
+        DevBuf<unsigned> bits;
+        DEV_ALLOC(bits, nw, s);
+        HIP_TRY(hipMemsetAsync(bits.p, 0, nw * 4, s));
+        if (a == 0) {
+#define HDFE_L(W)                                                                           \
+    hipLaunchKernelGGL(k_group_pair_bits<W>, dim3(grid1(P.G * W)), dim3(T256), 0, s, P.G,    \
+                       P.gstart.p, P.codes.p + (int64_t)tb * P.n, nb, bits.p)
+            HDFE_BY_WIDTH(run_width(P.n, P.G), HDFE_L)
+#undef HDFE_L
+        } else {
+            hipLaunchKernelGGL(k_pair_bits, dim3(grid1(P.n)), dim3(T256), 0, s, P.n,
+                                   P.codes.p + (int64_t)ta * P.n,
+                               P.codes.p + (int64_t)tb * P.n, nb, bits.p);
+        }
+        hipLaunchKernelGGL(k_uf_bits, dim3(grid1(nw)), dim3(T256), 0, s, nw, bits.p, nb,
+                           a == 0 ? 0 : (int)na, parent.p);
+    } else if (P.n > 0) {
+        if (a == 0) {
+            const int uw = run_width(P.n, P.G);
+#define HDFE_L(W)                                                                          \
+    hipLaunchKernelGGL(k_uf_union<W>, dim3(grid1(P.G * W)), dim3(T256), 0, s, P.G,          \
+                       P.gstart.p, P.codes.p + (int64_t)tb * P.n, parent.p)
+            HDFE_BY_WIDTH(uw, HDFE_L)
+#undef HDFE_L
+        } else {
+            hipLaunchKernelGGL(k_uf_rows, dim3(grid1(P.n)), dim3(T256), 0, s, P.n,
+                               P.codes.p + (int64_t)ta * P.n,
+                               P.codes.p + (int64_t)tb * P.n, (int)na, parent.p);
+        }
+    }
+    HIP_TRY(hipMemsetAsync(cnt.p, 0, 8, s));
+    hipLaunchKernelGGL(k_uf_count_roots, dim3(grid1(nv)), dim3(T256), 0, s, nv, parent.p, cnt.p);
+    HIP_TRY(hipGetLastError());
+    unsigned long long c = 0;
+    HIP_TRY(scalar_to_host(&c, cnt.p, s));
+    *count = (int64_t)c;
+    return HDFE_STATUS_OK;
+}
+
 // Normalise Gamma (L, k) on the device in place (core/solve.py:383-399).
 int hdfe_panel_normalize(void *h, double *Gm, int k, void *stream) {
     ARG_CHECK(h && Gm, "hdfe_panel_normalize: null argument");
@@ -3535,6 +3883,56 @@ int hdfe_panel_partial(void *h, const double *Gm, const int *cols, int kc, doubl
 #undef HDFE_L
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(s));
+    return HDFE_STATUS_OK;
+}
+
+// The scores of the covariates cols (host, kc <= 8) with residuals e
+// (sorted rows, device) and their sums (see k_scores): out (host) = [rss,
+// sum_i u_i u_i' (kc x kc), sum_g s_g s_g' (kc x kc)]; U (n, kc, device)
+// receives the scores when not null.
+int hdfe_panel_scores(void *h, const double *Gm, const int *cols, int kc, const double *e,
+                      double *U, double *out) {
+    ARG_CHECK(h && Gm && cols && e && out && kc >= 1 && kc <= 8,
+              "hdfe_panel_scores: bad argument (1 <= kc <= 8)");
+    Panel &P = *(Panel *)h;
+    for (int c = 0; c < kc; ++c)
+        ARG_CHECK(cols[c] >= 0 && cols[c] < P.m, "hdfe_panel_scores: column out of range");
+    hipStream_t s = P.s;
+    DevBuf<int> cc;
+    DEV_ALLOC(cc, kc, s);
+    HIP_TRY(hipMemcpyAsync(cc.p, cols, kc * 4, hipMemcpyHostToDevice, s));
+    const int W = run_width(P.n, P.G);
+    const int nb = grid1(P.G * W, 256, 1024);
+    const int K = kc <= 2 ? 2 : kc <= 4 ? 4 : 8;
+    const int NV = K == 2 ? ScoreAcc<2>::NV : K == 4 ? ScoreAcc<4>::NV : ScoreAcc<8>::NV;
+    DevBuf<double> part;
+    DEV_ALLOC(part, (int64_t)nb * NV, s);
+#define HDFE_L2(WW, KK)                                                                        \
+    hipLaunchKernelGGL((k_scores<WW, KK>), dim3(nb), dim3(T256), 0, s, P.G, P.D, P.m,         \
+                       P.gstart.p, P.codes.p, P.n, P.offs.p, P.V.p,                           \
+                       P.weighted ? P.w.p : (const double *)nullptr, Gm, cc.p, kc, e, U, part.p)
+#define HDFE_L(WW)                 \
+    if (K == 2) HDFE_L2(WW, 2);    \
+    else if (K == 4) HDFE_L2(WW, 4); \
+    else HDFE_L2(WW, 8)
+    HDFE_BY_WIDTH(W, HDFE_L)
+#undef HDFE_L
+#undef HDFE_L2
+    HIP_TRY(hipGetLastError());
+    std::vector<double> hp;
+    HIP_TRY(to_host(hp, part.p, (int64_t)nb * NV, s));
+    std::vector<double> tot(NV, 0.0);
+    for (int b = 0; b < nb; ++b)
+        for (int v = 0; v < NV; ++v) tot[v] += hp[(size_t)b * NV + v];
+    const int T = K * (K + 1) / 2;
+    out[0] = tot[0];
+    int t = 0;
+    for (int a = 0; a < K; ++a)
+        for (int b = a; b < K; ++b, ++t)
+            if (a < kc && b < kc) {
+                out[1 + a * kc + b] = out[1 + b * kc + a] = tot[1 + t];
+                out[1 + kc * kc + a * kc + b] = out[1 + kc * kc + b * kc + a] = tot[1 + T + t];
+            }
     return HDFE_STATUS_OK;
 }
 

@@ -285,8 +285,10 @@ class HDFEResult:
     resid_sorted: object = None
     gam_y: object = None
     geff: object = None
-    vcov_matrix: object = None      # with the fit's vcov= option
+    vcov_matrix: object = None      # the fit's default vcov (its vcov= option)
     vcov_details: dict = None
+    vcovs: dict = field(default_factory=dict)   # spec -> (V, details) made at fit time
+    fe_dof: str = "exact"
 
     def coef(self):
         return dict(zip(self.coefnames, self.beta.tolist()))
@@ -295,21 +297,22 @@ class HDFEResult:
         """(level values, effects) of one fixed-effect dimension."""
         return self.fe_levels[dim], self.fe_values[dim]
 
-    def vcov(self, vcov="iid", data=None):
+    def vcov(self, vcov="iid", data=None, fe_dof=None):
         """The coefficients' variance matrix: 'iid', 'hetero' ('HC1'), or
         'CRV1:<cols>' / {'CRV1': '<cols>'} with '+' for multi-way clustering
         and '^' for intersections (core/inference.py:430-668, 880-920).
         Cluster columns that are not fixed effects are read from `data` (the
-        data the fit was given).  Returns (V, details)."""
+        data the fit was given); fe_dof ('exact' or 'pyfixest', the fit's by
+        default) counts the fixed effects' parameters.  Returns (V, details)."""
         from .vcov import VcovEngine
         eng = getattr(self, "_vcov_engine", None)
         if eng is None or (data is not None and eng.data is not data):
             eng = self._vcov_engine = VcovEngine(self, data)
-        return eng.vcov(vcov)
+        return eng.vcov(vcov, fe_dof or self.fe_dof)
 
-    def se(self, vcov="iid", data=None):
+    def se(self, vcov="iid", data=None, fe_dof=None):
         """Standard errors under `vcov` (see `vcov`)."""
-        V, _ = self.vcov(vcov, data)
+        V, _ = self.vcov(vcov, data, fe_dof)
         return np.sqrt(np.diag(V))
 
     def solve_normal_equations(self, data, columns, block=16, **options):
@@ -333,7 +336,8 @@ class StreamingHDFE:
     def __init__(self, y, x, fe, *, tol=1e-10, maxiter=5000, rhs_block=8, solver="auto",
                  precond="jacobi", fixef_rm="singleton", weights=None, stream=None,
                  device="rocm", precision="fp64", inner_tol=1e-5, keep_intermediates=False,
-                 verbose=False, spmm_timing=False, max_memory_gb=None, vcov=None, **ignored):
+                 verbose=False, spmm_timing=False, max_memory_gb=None, vcov="iid", fe_dof="exact",
+                 **ignored):
         if device != "rocm":
             raise ValueError("device must be 'rocm' (this package is the MI355X backend)")
         if solver not in ("auto", "explicit", "stream_cg", "within"):
@@ -365,6 +369,9 @@ class StreamingHDFE:
         if vcov is not None:
             from .vcov import _parse
             _parse(vcov)
+        if fe_dof not in ("exact", "pyfixest"):
+            raise ValueError("fe_dof must be 'exact' or 'pyfixest'")
+        self.fe_dof = fe_dof
         self.rhs_block = max(1, int(rhs_block))
         self.fixef_rm = fixef_rm
         self.weights = weights
@@ -508,13 +515,17 @@ class StreamingHDFE:
         res = HDFEResult(coefnames=names[1:], beta=beta, solver_info=info, fe_names=fe_order,
                          fe_levels=levels, fe_values=values, n_obs=panel.info["n"],
                          n_components=ncomp, A=A, diagnostics=diag, panel=panel, Gamma=Gamma,
-                         resid_sorted=resid, gam_y=gam_y, geff=geff)
-        if self.vcov is not None:
-            # the coefficients' variance at fit time (the reference's vcov=
-            # of fit, core/estimator.py:467): cluster columns that are not
-            # fixed effects come from the fit's data
+                         resid_sorted=resid, gam_y=gam_y, geff=geff, fe_dof=self.fe_dof)
+        if self.vcov is not None and res.coefnames:
+            # the coefficients' variance at fit time, as the reference's fit
+            # (core/estimator.py:467-481): iid and hetero always, and the
+            # default `vcov` (cluster columns that are not fixed effects come
+            # from the fit's data); vcov=None skips them
             t5 = time.time()
-            res.vcov_matrix, res.vcov_details = res.vcov(self.vcov, data)
+            res.vcovs = {spec: res.vcov(spec, data) for spec in ("iid", "hetero")}
+            if not isinstance(self.vcov, str) or self.vcov not in res.vcovs:
+                res.vcovs[str(self.vcov)] = res.vcov(self.vcov, data)
+            res.vcov_matrix, res.vcov_details = res.vcovs[str(self.vcov)]
             res.diagnostics["seconds_vcov"] = round(time.time() - t5, 4)
         if self.verbose:
             print(f"fit: {diag}")

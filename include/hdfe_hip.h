@@ -185,6 +185,23 @@ int hdfe_panel_budget(void *h, int64_t set, int64_t *budget, int64_t *skipped);
  * (core/kernels_base.py:155-174, core/passes.py:933-979). */
 int hdfe_panel_components(void *h, int64_t *ncomp);
 
+/* Connected components of two fixed-effect dimensions' levels linked by the
+ * rows (core/passes.py:964-979 with the streamed dimension, :525-527 for two
+ * others; the `pair_components` that fe_dof="exact" takes the largest of,
+ * core/estimator.py:446-464).  a < b index the dimensions in fit order: 0
+ * the streamed one, t + 1 the non-streamed t. */
+int hdfe_panel_pair_components(void *h, int a, int b, int64_t *count);
+
+/* The coefficients' scores u_i = w_i e_i x~_i for the covariates cols (host,
+ * 1 <= kc <= 8; x~ partialled on the fixed effects from the reduced
+ * solutions Gm, e the residuals in sorted row order) summed in one pass in a
+ * fixed order, as the reference's residual pass collects its meats
+ * (core/inference.py:430-668): out (host, 1 + 2 kc^2) = [sum w e^2,
+ * sum_i u_i u_i', sum_g s_g s_g' with s_g the scores' sum over group g of
+ * the streamed dimension]; U (n, kc, device) receives u when not NULL. */
+int hdfe_panel_scores(void *h, const double *Gm, const int *cols, int kc, const double *e,
+                      double *U, double *out);
+
 /* Normalise Gamma (L, k), device, in place: obs-weighted mean zero per
  * component for the first dimension, overall for the others
  * (core/solve.py:383-399).  stream may be NULL. */
