@@ -31,7 +31,7 @@ sys.path.insert(0, str(ROOT))
 FML = "y ~ x1 + x2 | worker_id + firm_id + year"
 # The FP64 SpMM of the CG at k = 3 on the 400,008-level S: P (9.6 MB) outgrows
 # an XCD's L2, so it runs column-blocked as two launches, timed together.
-SPMM_KERNEL = "k_spmm_blk<double,3,G> + k_spmm_combine<double,3>"
+SPMM_KERNEL = "k_spmm_tiles<double,3,2> + k_fin_small<double,3>"
 HBM_PEAK_GBS = 8000.0
 BF16_PEAK_TFLOPS = 2500.0     # MI355X dense bf16 matrix peak
 
@@ -428,8 +428,15 @@ def main():
     nnz = f64.panel.info["nnz"]
     k = 3
     info = f64.solver_info
-    avg64 = f64.panel.matrix.spmm_time(k, reps=100, precision="fp64") * 1e-6
-    avg32 = f64.panel.matrix.spmm_time(k, reps=100, precision="mixed") * 1e-6
+    M = f64.panel.matrix
+    avg64 = M.spmm_time(k, reps=100, precision="fp64") * 1e-6
+    avg32 = M.spmm_time(k, reps=100, precision="mixed") * 1e-6
+    # the other form of the column-blocked product, for comparison
+    grouped = {}
+    for prec in ("fp64", "mixed"):
+        M.small_form(k, prec, set=1)
+        grouped[prec] = round(M.spmm_time(k, reps=100, precision=prec), 2)
+        M.small_form(k, prec, set=0)
     byt = spmm_bytes(nnz, L, k, 8)
     ach = byt / avg64 / 1e9
     kname = SPMM_KERNEL
@@ -438,7 +445,7 @@ def main():
             "traffic": measured_traffic(kname, f"c2:{args.workers}x{args.firms}"),
             "kernel": kname, "bytes_per_launch": int(byt),
             "avg_launch_us": round(avg64 * 1e6, 2),
-            "tuned_row_parts_per_xcd": f64.panel.matrix.blk_parts(k, "fp64")}
+            "grouped_form_us": grouped["fp64"]}
     b32 = spmm_bytes(nnz, L, k, 4)
     line = {
         "metric": "AKM fit wall-s @5M workers (mixed FP32/FP64-refined, 1e-10 rel. residual)",
@@ -462,10 +469,10 @@ def main():
         "se_crv1_worker": [float(v) for v in np.sqrt(np.diag(mix.vcov_matrix))],
         "fe_rel_err_mixed_vs_fp64": fe_err,
         "roofline": roof,
-        "spmm_fp32": {"kernel": "k_spmm_blk<float,3,G> + k_spmm_combine<float,3>",
+        "spmm_fp32": {"kernel": "k_spmm_tiles<float,3,1> + k_fin_small<float,3>",
                       "avg_launch_us": round(avg32 * 1e6, 2),
                       "bytes_per_launch": int(b32), "achieved_GBps": round(b32 / avg32 / 1e9, 1),
-                      "tuned_row_parts_per_xcd": f64.panel.matrix.blk_parts(k, "mixed")},
+                      "grouped_form_us": grouped["mixed"]},
         "copy_calibration": copy_calibration(),
     }
     if not args.no_host:

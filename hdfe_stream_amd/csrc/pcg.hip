@@ -52,6 +52,7 @@ constexpr int kSmallK = 8;         // k <= 8: lanes over nonzeros; wider: over c
 constexpr int64_t kL2Budget = 3 << 20;  // bytes of P an XCD's 4 MB L2 should hold (3 MB)
 constexpr int kBlkParts = 128;     // row parts per XCD of the blocked SpMM
 constexpr int kDenseLongMax = 16;  // split rows the blocked SpMM takes as dense columns
+constexpr int kNB = 16;            // column blocks of the blocked layout
 constexpr int kCombineMax = 512;   // workgroups of the blocked SpMM's combine
 
 // Device-side solver state (one allocation; see State layout in Work).
@@ -178,26 +179,63 @@ __device__ void spmm_finish(const T *__restrict__ P, T *__restrict__ AP,
                             const T *__restrict__ ll = nullptr) {
     const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
     const int nwv = blockDim.x / kWave;
-    for (int q = wv; q < nlong * K; q += nwv) {
-        const int l = q / K, j = q % K;
-        // four loads in flight per lane, their sums added in a fixed order
-        double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-        const int c1 = long_cptr[l + 1];
-        int c = long_cptr[l] + lane;
+    // split row l's partials (its range of partial rows, each K values) in
+    // H stretches, one wave each: every lane adds whole partial rows (all K
+    // values in one contiguous load), four in flight; the stretches' sums
+    // then in order
+    __shared__ double hsum[kFinThreads / kWave][K];
+    // more split rows than waves (only without ll): a wave per row, written
+    // directly
+    const bool direct = nlong > nwv;
+    const int H = nlong > 0 && !direct ? max(1, min(nwv / nlong, 4)) : 1;
+    for (int q = wv; q < nlong * H; q += nwv) {
+        const int l = q / H, h = q % H;
+        // without long_cptr, split row l's partials are [l nw, (l + 1) nw)
+        const int cb = long_cptr ? long_cptr[l] : l * nw;
+        const int ce = long_cptr ? long_cptr[l + 1] : (l + 1) * nw;
+        const int c1 = cb + (int)((int64_t)(ce - cb) * (h + 1) / H);
+        int c = cb + (int)((int64_t)(ce - cb) * h / H) + lane;
+        double a0[K], a1[K], a2[K], a3[K];
+#pragma unroll
+        for (int j = 0; j < K; ++j) a0[j] = a1[j] = a2[j] = a3[j] = 0.0;
         for (; c + 3 * kWave < c1; c += 4 * kWave) {
-            a0 += lpart[(int64_t)c * K + j];
-            a1 += lpart[(int64_t)(c + kWave) * K + j];
-            a2 += lpart[(int64_t)(c + 2 * kWave) * K + j];
-            a3 += lpart[(int64_t)(c + 3 * kWave) * K + j];
+#pragma unroll
+            for (int j = 0; j < K; ++j) {
+                a0[j] += lpart[(int64_t)c * K + j];
+                a1[j] += lpart[(int64_t)(c + kWave) * K + j];
+                a2[j] += lpart[(int64_t)(c + 2 * kWave) * K + j];
+                a3[j] += lpart[(int64_t)(c + 3 * kWave) * K + j];
+            }
         }
-        for (; c < c1; c += kWave) a0 += lpart[(int64_t)c * K + j];
-        double s2 = wave_sum((a0 + a1) + (a2 + a3));
-        if (lane == 0) {
-            if (ll)
-                for (int t = 0; t < nlong; ++t)
-                    s2 += (double)ll[l * nlong + t] * (double)P[(int64_t)long_rows[t] * K + j];
-            AP[(int64_t)long_rows[l] * K + j] = (T)s2;
+        for (; c < c1; c += kWave)
+#pragma unroll
+            for (int j = 0; j < K; ++j) a0[j] += lpart[(int64_t)c * K + j];
+#pragma unroll
+        for (int j = 0; j < K; ++j) {
+            const double v = wave_sum((a0[j] + a1[j]) + (a2[j] + a3[j]));
+            if (lane == 0) {
+                if (!direct) {
+                    hsum[q][j] = v;
+                } else {
+                    double s2 = v;
+                    if (ll)
+                        for (int u = 0; u < nlong; ++u)
+                            s2 += (double)ll[l * nlong + u] *
+                                  (double)P[(int64_t)long_rows[u] * K + j];
+                    AP[(int64_t)long_rows[l] * K + j] = (T)s2;
+                }
+            }
         }
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < (direct ? 0 : nlong * K); t += blockDim.x) {
+        const int l = t / K, j = t % K;
+        double s2 = 0.0;
+        for (int h = 0; h < H; ++h) s2 += hsum[l * H + h][j];
+        if (ll)
+            for (int u = 0; u < nlong; ++u)
+                s2 += (double)ll[l * nlong + u] * (double)P[(int64_t)long_rows[u] * K + j];
+        AP[(int64_t)long_rows[l] * K + j] = (T)s2;
     }
     if (!with_dot) return;
     __shared__ double pap[K];
@@ -317,9 +355,9 @@ __global__ __launch_bounds__(kThreads) void k_spmm_small(
 }
 
 // Blocked small-k SpMM, launch 1 of 2: the non-split rows' nonzeros are
-// stored again by column block (eight contiguous column ranges of equal
+// stored again by column block (kNB contiguous column ranges of equal
 // nonzero counts, block-major, rows ascending within a block, the row
-// pointers of all blocks in one array bptr (8 L + 1)).  Workgroup w < nblk
+// pointers of all blocks in one array bptr (kNB L + 1)).  Workgroup w < nblk
 // runs on XCD w % 8 (the dispatcher deals workgroups to the eight XCDs in
 // turn) and takes one group of ngrp groups of consecutive blocks -- the
 // XCD's group is xcd % ngrp -- over one part of the rows, so the rows of P
@@ -362,7 +400,7 @@ __global__ __launch_bounds__(kThreads) void k_spmm_blk(
     const int nparts = (nblk / 8) * share;    // row parts of one group
     const int rp = (w >> 3) * share + xcd / ngrp;
     const int64_t r0 = part_begin(L, rp, nparts), r1 = part_begin(L, rp + 1, nparts);
-    const int b0 = grp * share, b1 = b0 + share;
+    const int b0 = grp * (kNB / ngrp), b1 = b0 + kNB / ngrp;
     T *out = gpart + (int64_t)grp * L * K;
     const int sub = threadIdx.x % G, slot = threadIdx.x / G, nslot = kThreads / G;
     for (int64_t base = r0; base < r1; base += nslot) {
@@ -464,6 +502,134 @@ __global__ __launch_bounds__(kThreads) void k_spmm_combine(
         for (int j = 0; j < K; ++j) {
             double t = block_sum<double>(dot[j], lds);
             if (threadIdx.x == 0) part[(int64_t)w * K + j] = t;
+        }
+    }
+}
+
+// Tiled small-k SpMM (one launch before k_fin_small), the default form of
+// the column-blocked product.  Workgroup w owns a tile of R x 256
+// consecutive rows (R fixed by L, k and the precision, never by timing); in
+// column block b the tile's nonzeros are one
+// contiguous range of the blocked arrays, which the workgroup streams in
+// chunks of CH nonzeros: every thread loads its nonzeros' column indices and
+// values (coalesced) and gathers their P rows with four in flight, writing
+// val x P[col] to LDS; then each row's thread adds its row's products from
+// LDS in ascending order.  No per-row pointer chase sits between a load and
+// its gather, and no lane waits on a longer row.  The blocks are taken in
+// order, so the workgroups, started together, gather from one block's slice
+// of P at a time (kNB blocks: a slice of P small enough to stay in an XCD's
+// L2 while S streams through it).  Per tile: the pAp partial part[w, j] and,
+// for every split row t, the tile's partial of AP[long t] = sum_row dt[t, row]
+// P[row] in dlpart[(t nw + w) K + j] (k_fin_small adds them in tile order).
+// Measured against the groups-of-blocks form (k_spmm_blk + k_spmm_combine)
+// on the 5M-worker system: 107 vs 123 us at k = 3 (FP64), 77 vs 93 (FP32),
+// 172 vs 315 at k = 8; non-temporal loads of S made it slower (135 us).
+template <typename T, int K, int R>
+__global__ __launch_bounds__(kThreads) void k_spmm_tiles(
+    const int *__restrict__ bptr, const int *__restrict__ bidx, const T *__restrict__ bval,
+    int64_t L, const T *__restrict__ P, T *__restrict__ AP, double *__restrict__ part,
+    int with_dot, int nld, const T *__restrict__ dt, const int *__restrict__ long_rows,
+    double *__restrict__ dlpart, CgState st) {
+    if (st.done && *st.done) return;
+    constexpr int CH = K <= 4 ? 1024 : 512;
+    __shared__ T prod[CH * K];
+    __shared__ double lds[kThreads / kWave];
+    __shared__ T plong[kDenseLongMax * K];
+    const int nw = gridDim.x, w = blockIdx.x, tid = threadIdx.x;
+    const int64_t r0 = (int64_t)w * R * kThreads, r1 = min(L, r0 + R * kThreads);
+    for (int q = tid; q < nld * K; q += blockDim.x)
+        plong[q] = P[(int64_t)long_rows[q / K] * K + q % K];
+    T acc[R][K];
+#pragma unroll
+    for (int q = 0; q < R; ++q)
+#pragma unroll
+        for (int j = 0; j < K; ++j) acc[q][j] = T(0);
+    for (int b = 0; b < kNB; ++b) {
+        const int64_t boff = (int64_t)b * L;
+        const int s0 = bptr[boff + r0], s1 = bptr[boff + r1];
+        int rs[R], re[R];
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+            const int64_t row = r0 + q * kThreads + tid;
+            rs[q] = row < r1 ? bptr[boff + row] : 0;
+            re[q] = row < r1 ? bptr[boff + row + 1] : 0;
+        }
+        for (int c0 = s0; c0 < s1; c0 += CH) {
+            const int cn = min(CH, s1 - c0);
+            // the chunk's products, four nonzeros per thread in flight
+            for (int t = tid; t < cn; t += 4 * kThreads) {
+                int cc[4];
+                T vv[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int q = t + u * kThreads;
+                    cc[u] = q < cn ? bidx[c0 + q] : 0;
+                    vv[u] = q < cn ? bval[c0 + q] : T(0);
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int q = t + u * kThreads;
+                    if (q < cn) {
+#pragma unroll
+                        for (int j = 0; j < K; ++j)
+                            prod[q * K + j] = vv[u] * P[(int64_t)cc[u] * K + j];
+                    }
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int q = 0; q < R; ++q) {
+                const int a = max(rs[q], c0) - c0, e = min(re[q], c0 + cn) - c0;
+                for (int p = a; p < e; ++p)
+#pragma unroll
+                    for (int j = 0; j < K; ++j) acc[q][j] += prod[p * K + j];
+            }
+            __syncthreads();
+        }
+    }
+    if (nld) __syncthreads();  // plong (no chunk may have run)
+    double dot[K];
+#pragma unroll
+    for (int j = 0; j < K; ++j) dot[j] = 0.0;
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+        const int64_t row = r0 + q * kThreads + tid;
+        if (row < r1) {
+            for (int t = 0; t < nld; ++t) {
+                const T d = dt[t * L + row];
+#pragma unroll
+                for (int j = 0; j < K; ++j) acc[q][j] += d * plong[t * K + j];
+            }
+#pragma unroll
+            for (int j = 0; j < K; ++j) {
+                AP[row * K + j] = acc[q][j];
+                dot[j] += (double)P[row * K + j] * (double)acc[q][j];
+            }
+        }
+    }
+    if (nld) {
+        const int lane = tid & (kWave - 1), wv = tid / kWave;
+        for (int t = wv; t < nld; t += kThreads / kWave) {
+            double ta[K];
+#pragma unroll
+            for (int j = 0; j < K; ++j) ta[j] = 0.0;
+            for (int64_t row = r0 + lane; row < r1; row += kWave) {
+                const double d = (double)dt[t * L + row];
+#pragma unroll
+                for (int j = 0; j < K; ++j) ta[j] += d * (double)P[row * K + j];
+            }
+#pragma unroll
+            for (int j = 0; j < K; ++j) {
+                const double v = wave_sum(ta[j]);
+                if (lane == 0) dlpart[((int64_t)t * nw + w) * K + j] = v;
+            }
+        }
+    }
+    if (with_dot) {
+#pragma unroll
+        for (int j = 0; j < K; ++j) {
+            double t = block_sum<double>(dot[j], lds);
+            if (tid == 0) part[(int64_t)w * K + j] = t;
         }
     }
 }
@@ -883,7 +1049,7 @@ namespace hdfe {
 // columns (slot >= 0) go to dt instead.
 __device__ inline int blk_of(int col, const int *__restrict__ col_lo) {
     int b = 0;
-    while (b < 7 && col >= col_lo[b + 1]) ++b;
+    while (b < kNB - 1 && col >= col_lo[b + 1]) ++b;
     return b;
 }
 
@@ -894,7 +1060,7 @@ __global__ __launch_bounds__(kThreads) void k_blk_rowcount(
     const int64_t nwave = (int64_t)gridDim.x * (blockDim.x / kWave);
     for (int64_t r = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / kWave; r < L; r += nwave) {
         const int s = indptr[r], e = indptr[r + 1];
-        int c = 0;  // lane b < 8: block b's entries
+        int c = 0;  // lane b < kNB: block b's entries
         if (e - s <= kLongThresh) {
             for (int p0 = s; p0 < e; p0 += kWave) {
                 const int p = p0 + lane;
@@ -904,13 +1070,13 @@ __global__ __launch_bounds__(kThreads) void k_blk_rowcount(
                     if (!slot || slot[col] < 0) b = blk_of(col, col_lo);
                 }
 #pragma unroll
-                for (int bb = 0; bb < 8; ++bb) {
+                for (int bb = 0; bb < kNB; ++bb) {
                     const uint64_t m = __ballot(b == bb);
                     if (lane == bb) c += __popcll(m);
                 }
             }
         }
-        if (lane < 8) cnt[(int64_t)lane * L + r] = c;
+        if (lane < kNB) cnt[(int64_t)lane * L + r] = c;
     }
 }
 
@@ -926,7 +1092,7 @@ __global__ __launch_bounds__(kThreads) void k_blk_fill(
     for (int64_t r = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / kWave; r < L; r += nwave) {
         const int s = indptr[r], e = indptr[r + 1];
         if (e - s > kLongThresh) continue;
-        int next = lane < 8 ? bptr[(int64_t)lane * L + r] : 0;  // lane b: block b's next slot
+        int next = lane < kNB ? bptr[(int64_t)lane * L + r] : 0;  // lane b: block b's next slot
         for (int p0 = s; p0 < e; p0 += kWave) {
             const int p = p0 + lane;
             int col = 0, b = -1, sl = -1;
@@ -937,7 +1103,7 @@ __global__ __launch_bounds__(kThreads) void k_blk_fill(
             }
             int q = 0;
 #pragma unroll
-            for (int bb = 0; bb < 8; ++bb) {
+            for (int bb = 0; bb < kNB; ++bb) {
                 const uint64_t m = __ballot(b == bb);
                 const int at = __shfl(next, bb);
                 if (b == bb) q = at + __popcll(m & below);
@@ -1156,7 +1322,7 @@ struct Matrix {
     // blocked copy of the non-split rows for the small-k SpMM (k_spmm_blk)
     bool blk = false, blk_tried = false;
     std::vector<int> ip_host;  // host row pointers (the blocked layout's balance)
-    int col_lo_h[9] = {0};
+    int col_lo_h[kNB + 1] = {0};
     int *col_lo = nullptr, *bptr = nullptr, *bidx = nullptr;
     double *bval64 = nullptr;
     float *bval32 = nullptr;
@@ -1169,6 +1335,9 @@ struct Matrix {
     // the blocked SpMM's row parts per XCD, chosen by timing (0: the default),
     // per precision (0: FP64, 1: FP32) and k
     int blk_parts[2][kSmallK + 1] = {};
+    // the column-blocked product's form per precision and k: 0 the default
+    // (tiled where it applies), 1 groups of blocks + combine, 2 tiled
+    int small_form[2][kSmallK + 1] = {};
     // the wide SpMM's form per (precision, k), chosen by timing: 0 a wave per
     // row over all columns (k_spmm_wide), 1 column slabs per XCD (k_spmm_slab)
     std::map<std::pair<int, int>, int> wide_mode;
@@ -1204,7 +1373,7 @@ static int alloc_work(Matrix &M, int k) {
     // the blocked layout for small k, once, when P outgrows an XCD's L2
     if (k <= kSmallK && !M.blk_tried) {
         M.blk_tried = true;
-        if (M.L * (int64_t)kSmallK * 8 > kL2Budget && M.L < (INT32_MAX - 1) / 8) {
+        if (M.L * (int64_t)kSmallK * 8 > kL2Budget && M.L < (INT32_MAX - 1) / kNB) {
             M.drop_graphs();
             M.w.release(M.stream);
             const int st = build_blocked(M, M.ip_host);
@@ -1274,6 +1443,23 @@ static CgState make_state(Work &W, int k, double tol, int maxiter) {
 static int spmm_grid(int64_t L, int k) {
     return k <= kSmallK ? grid_for(L, 256, 1, 2048) : grid_for(L, 128, 1, 512);
 }
+
+// The tiled form's shape (k_spmm_tiles) from L, k and the precision alone
+// (so that its sums never depend on timing): R x 256 rows a tile, R = 2 for
+// FP64 with k >= 2 (measured faster on the 5M-worker system: 107 vs 118 us
+// at k = 3, 172 vs 183 at k = 8), else 1 (77 vs 80 us FP32 at k = 3), raised
+// to 4 if the tiles would outnumber the work buffers' partials (spmm_grid's
+// count); 0 when even that does not fit.
+static int tiles_shape(int64_t L, int k, bool fp64, int *r) {
+    for (int R = fp64 && k >= 2 ? 2 : 1; R <= 4; R *= 2) {
+        const int64_t nw = (L + R * (int64_t)kThreads - 1) / (R * (int64_t)kThreads);
+        if (nw <= spmm_grid(L, k)) {
+            *r = R;
+            return (int)nw;
+        }
+    }
+    return 0;
+}
 static int vec_grid(int64_t L, int k) {
     return grid_for(L * (int64_t)k, 2048, 1, k <= kSmallK ? 2048 : 512);
 }
@@ -1297,7 +1483,7 @@ const double *ll_values<double>(const Matrix &M) { return M.ll64; }
 template <>
 const float *ll_values<float>(const Matrix &M) { return M.ll32; }
 
-// The blocked copy of the non-split rows (k_spmm_blk): eight column blocks
+// The blocked copy of the non-split rows (k_spmm_blk): kNB column blocks
 // of about equal nonzero counts, on the matrix's stream.  S is symmetric, so
 // a column's nonzeros are counted by its row's length (ip32, host).
 static int build_blocked(Matrix &M, const std::vector<int> &ip32) {
@@ -1319,15 +1505,15 @@ static int build_blocked(Matrix &M, const std::vector<int> &ip32) {
     }
     M.dl = M.nlong >= 1 && M.nlong <= kDenseLongMax &&
            4 * long_nnz >= (int64_t)M.nlong * (L - M.nlong) && (int64_t)M.nlong * L < INT32_MAX;
-    // col_lo[b] = the first column whose prefix count reaches b / 8 of the
+    // col_lo[b] = the first column whose prefix count reaches b / kNB of the
     // total (S is symmetric: a column's count is its row's length); dense
     // long columns are not in the blocks
     int64_t btotal = total;
     if (M.dl) btotal -= long_nnz;
     M.col_lo_h[0] = 0;
     int64_t acc = 0, c = 0;
-    for (int b = 1; b < 8; ++b) {
-        const int64_t target = btotal * b / 8;
+    for (int b = 1; b < kNB; ++b) {
+        const int64_t target = btotal * b / kNB;
         while (c < L && acc < target) {
             if (!(M.dl && slot_h[c] >= 0)) acc += (int64_t)ip32[c + 1] - ip32[c];
             ++c;
@@ -1354,19 +1540,19 @@ static int build_blocked(Matrix &M, const std::vector<int> &ip32) {
         HIP_TRY(hipMemsetAsync(M.ll64, 0, (int64_t)M.nlong * M.nlong * 8, ms));
         HIP_TRY(hipMemsetAsync(M.ll32, 0, (int64_t)M.nlong * M.nlong * 4, ms));
     }
-    M.col_lo_h[8] = (int)L;
-    HIP_TRY(palloc(&M.col_lo, 9 * 4, ms));
-    HIP_TRY(hipMemcpyAsync(M.col_lo, M.col_lo_h, 9 * 4, hipMemcpyHostToDevice, ms));
+    M.col_lo_h[kNB] = (int)L;
+    HIP_TRY(palloc(&M.col_lo, (kNB + 1) * 4, ms));
+    HIP_TRY(hipMemcpyAsync(M.col_lo, M.col_lo_h, (kNB + 1) * 4, hipMemcpyHostToDevice, ms));
     DevBuf<int> cnt;
-    DEV_ALLOC(cnt, 8 * L + 1, ms);
-    HIP_TRY(hipMemsetAsync(cnt.p + 8 * L, 0, 4, ms));
+    DEV_ALLOC(cnt, kNB * L + 1, ms);
+    HIP_TRY(hipMemsetAsync(cnt.p + kNB * L, 0, 4, ms));
     const int gwave = (int)std::min<int64_t>((L + kThreads / kWave - 1) / (kThreads / kWave), 65536);
     hipLaunchKernelGGL(k_blk_rowcount, dim3(gwave), dim3(kThreads), 0, ms, L, M.indptr,
                        M.indices, M.col_lo, M.dl ? (const int *)M.dl_slot : nullptr, cnt.p);
-    HIP_TRY(palloc(&M.bptr, (8 * L + 1) * 4, ms));
-    HIP_TRY(exclusive_sum(tmp, cnt.p, M.bptr, 8 * L + 1, ms));
+    HIP_TRY(palloc(&M.bptr, (kNB * L + 1) * 4, ms));
+    HIP_TRY(exclusive_sum(tmp, cnt.p, M.bptr, kNB * L + 1, ms));
     int last = 0;
-    HIP_TRY(scalar_to_host(&last, M.bptr + 8 * L, ms));
+    HIP_TRY(scalar_to_host(&last, M.bptr + kNB * L, ms));
     M.bnnz = last;
     HIP_TRY(palloc(&M.bidx, std::max<int64_t>(M.bnnz, 1) * 4, ms));
     HIP_TRY(palloc(&M.bval64, std::max<int64_t>(M.bnnz, 1) * 8, ms));
@@ -1422,7 +1608,7 @@ static void launch_spmm(Matrix &M, const T *val, const T *P, T *AP, double *part
             int64_t widest = 0;
             for (int q = 0; q < ng; ++q)
                 widest = std::max<int64_t>(
-                    widest, (int64_t)M.col_lo_h[(q + 1) * 8 / ng] - M.col_lo_h[q * 8 / ng]);
+                    widest, (int64_t)M.col_lo_h[(q + 1) * kNB / ng] - M.col_lo_h[q * kNB / ng]);
             if (widest * k * (int64_t)sizeof(T) <= kL2Budget) {
                 ngrp = ng;
                 break;
@@ -1439,6 +1625,34 @@ static void launch_spmm(Matrix &M, const T *val, const T *P, T *AP, double *part
         const T *dt = dl_values<T>(M), *ll = M.dl ? ll_values<T>(M) : nullptr;
         const int *lcp = M.dl ? M.dl_cptr + (k - 1) * (M.nlong + 1) : M.long_cptr;
         const int nchunk = M.dl ? 0 : M.nchunk;
+        // the tiled form (the default): one pass over the rows, blocks in order
+        const int prec = sizeof(T) == 4 ? 1 : 0;
+        int rt = 0;
+        const int nwt = M.small_form[prec][k] != 1 && (M.dl || M.nlong == 0)
+                            ? tiles_shape(M.L, k, sizeof(T) == 8, &rt)
+                            : 0;
+        if (nwt > 0) {
+#define HDFE_TL(KK, RR)                                                                          \
+    hipLaunchKernelGGL((k_spmm_tiles<T, KK, RR>), dim3(nwt), dim3(kThreads), 0, s, M.bptr,       \
+                       M.bidx, bval, M.L, P, AP, part, with_dot, nld, dt, M.long_rows, lpart, st)
+#define HDFE_TLG(KK)                                                                               \
+    case KK:                                                                                       \
+        if (rt == 1) { HDFE_TL(KK, 1); }                                                           \
+        else if (rt == 2) { HDFE_TL(KK, 2); }                                                      \
+        else { HDFE_TL(KK, 4); }                                                                   \
+        if (with_dot || M.nlong)                                                                   \
+            hipLaunchKernelGGL((k_fin_small<T, KK>), dim3(1), dim3(kFinThreads), 0, s, P, AP, part, \
+                               nwt, with_dot, M.long_rows, (const int *)nullptr, M.nlong, lpart,   \
+                               st, ll);                                                           \
+        break;
+            switch (k) {
+                HDFE_TLG(1) HDFE_TLG(2) HDFE_TLG(3) HDFE_TLG(4)
+                HDFE_TLG(5) HDFE_TLG(6) HDFE_TLG(7) HDFE_TLG(8)
+            }
+#undef HDFE_TLG
+#undef HDFE_TL
+            return;
+        }
         // the combine in at most 512 workgroups (a few rows per thread): the
         // one-workgroup finish then adds 512 partials per split row, not 2,048
         const int gc = std::min(g, kCombineMax);
@@ -1688,6 +1902,11 @@ static int tune_blocked(Matrix &M, int k) {
           M.w.gpart))
         return HDFE_STATUS_OK;
     const int prec = sizeof(T) == 4 ? 1 : 0;
+    // the tiled form, where it is taken, has no launch shape to choose
+    int rt = 0;
+    if (M.small_form[prec][k] != 1 && (M.dl || M.nlong == 0) &&
+        tiles_shape(M.L, k, sizeof(T) == 8, &rt) > 0)
+        return HDFE_STATUS_OK;
     const std::array<int64_t, 6> key = {M.L, M.nnz, M.bnnz, M.nlong, k, prec};
     {
         std::lock_guard<std::mutex> lock(g_tune_mu);
@@ -2274,6 +2493,20 @@ int hdfe_csr_blk_parts(void *h, int k, int precision, int set, int *parts) {
         M.blk_parts[precision][k] = set;
     }
     *parts = M.blk_parts[precision][k];
+    return HDFE_STATUS_OK;
+}
+
+int hdfe_csr_small_form(void *h, int k, int precision, int set, int *form) {
+    ARG_CHECK(h && form && k >= 1 && k <= kSmallK && (precision == 0 || precision == 1) &&
+                  set >= -1 && set <= 2,
+              "hdfe_csr_small_form: bad argument");
+    Matrix &M = *(Matrix *)h;
+    std::lock_guard<std::mutex> lock(M.mu);
+    if (set >= 0 && set != M.small_form[precision][k]) {
+        M.drop_graphs();  // their kernels are the old form's
+        M.small_form[precision][k] = set;
+    }
+    *form = M.small_form[precision][k];
     return HDFE_STATUS_OK;
 }
 

@@ -181,6 +181,15 @@ class DeviceCSR:
                                                   int(set), C.byref(out)), "hdfe_csr_wide_mode")
         return int(out.value)
 
+    def small_form(self, k: int, precision: str = "fp64", set: int = -1) -> int:
+        """The column-blocked SpMM's form for k <= 8 columns (0 the default:
+        tiled where it applies; 1 groups of column blocks with partial
+        products and a combine; 2 tiled); `set` overrides it."""
+        out = C.c_int()
+        _lib.check(_lib.load().hdfe_csr_small_form(self._h, int(k), PRECISIONS[precision],
+                                                   int(set), C.byref(out)), "hdfe_csr_small_form")
+        return int(out.value)
+
     def graphs(self) -> int:
         """CG batches held as HIP graphs (captured on small systems only)."""
         n = C.c_int64()

@@ -110,6 +110,14 @@ int hdfe_csr_blk_parts(void *h, int k, int precision, int set, int *parts);
  * reference; for the autotuner's report and tests. */
 int hdfe_csr_wide_mode(void *h, int k, int precision, int set, int *mode);
 
+/* The column-blocked small-k product's form for k <= 8 columns (no
+ * counterpart in the reference, whose cuSPARSE SpMM has one form): 0 the
+ * default (tiled where it applies), 1 groups of column blocks with their
+ * partial products and a combine, 2 tiled (a workgroup's rows streamed block
+ * by block through LDS, no partial products); `set` >= 0 overrides it.  Each
+ * form is deterministic; they differ in rounding. */
+int hdfe_csr_small_form(void *h, int k, int precision, int set, int *form);
+
 /* The number of CG batches held as HIP graphs on this matrix.  Batches are
  * captured only on systems small enough that launching costs more than the
  * device time of an iteration.  No counterpart in the reference (it has no

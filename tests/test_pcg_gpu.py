@@ -203,11 +203,13 @@ def test_blocked_spmm_solve(oracle_mod, big_system, precision):
 
 @pytest.mark.parametrize("k", [1, 3])
 def test_autotuned_launch_shape_leaves_results_bit_identical(big_system, k):
-    """The blocked SpMM's row parts per XCD are chosen by timing; every choice
+    """The grouped form's row parts per XCD are chosen by timing; every choice
     partitions the same per-row sums, so the products are bit-identical."""
     from hdfe_stream_amd.rocm import DeviceCSR
     S, dinv = big_system
     A = DeviceCSR.from_scipy(S, dinv)
+    A.small_form(k, "fp64", set=1)
+    A.small_form(k, "mixed", set=1)
     X = np.random.default_rng(k).standard_normal((S.shape[0], k))
     first = A.matmat(X)
     tuned = A.blk_parts(k, "fp64")
@@ -219,6 +221,79 @@ def test_autotuned_launch_shape_leaves_results_bit_identical(big_system, k):
         a32 = A.matmat_f32(X.astype(np.float32))
         A.blk_parts(k, "mixed", set=128)
         np.testing.assert_array_equal(A.matmat_f32(X.astype(np.float32)), a32)
+
+
+@pytest.mark.parametrize("k", [1, 3, 9])
+def test_many_split_rows(k):
+    """More split rows (20 dense levels, each coupled to every other level)
+    than the finishing workgroup has waves: each split row's sum is then
+    written by its own wave."""
+    import scipy.sparse as sp
+    from hdfe_stream_amd.rocm import DeviceCSR
+    rng = np.random.default_rng(31)
+    Ls, Ld = 10_000, 20
+    r = np.repeat(np.arange(Ls), 6)
+    c = rng.integers(0, Ls, len(r))
+    rd = np.repeat(np.arange(Ls), Ld)
+    cd = Ls + np.tile(np.arange(Ld), Ls)
+    B = sp.csr_matrix((np.concatenate([-rng.uniform(0.1, 1, len(r)), -rng.uniform(0.01, 0.1, len(rd))]),
+                       (np.concatenate([r, rd]), np.concatenate([c, cd]))), shape=(Ls + Ld, Ls + Ld))
+    S = (B + B.T).tocsr()
+    S.setdiag(0.0)
+    S.eliminate_zeros()
+    S = (S + sp.diags(np.asarray(abs(S).sum(axis=1)).ravel() + 1.0)).tocsr()
+    S.sort_indices()
+    A = DeviceCSR.from_scipy(S, 1.0 / S.diagonal())
+    assert A.nlong() == Ld
+    X = rng.standard_normal((S.shape[0], k))
+    ref = S @ X
+    np.testing.assert_allclose(A.matmat(X), ref, rtol=1e-12, atol=1e-12 * np.abs(ref).max())
+    b = ref
+    Xd, _, ok, rel = A.solve(b, tol=1e-10)
+    assert ok and rel <= 1e-10
+
+
+@pytest.mark.parametrize("k", [1, 3, 8])
+def test_tiled_and_grouped_forms(big_system, k):
+    """The column-blocked product's two forms (groups of blocks with partial
+    products and a combine; tiles of rows streamed block by block) are both
+    S X, each the same bits on every call; the default is the tiled form."""
+    from hdfe_stream_amd.rocm import DeviceCSR
+    S, dinv = big_system
+    A = DeviceCSR.from_scipy(S, dinv)
+    X = np.random.default_rng(20 + k).standard_normal((S.shape[0], k))
+    ref = S @ X
+    out = {}
+    for form in (1, 2):
+        A.small_form(k, "fp64", set=form)
+        A.small_form(k, "mixed", set=form)
+        out[form] = (A.matmat(X), A.matmat_f32(X.astype(np.float32)))
+        np.testing.assert_allclose(out[form][0], ref, rtol=1e-12, atol=1e-12 * np.abs(ref).max())
+        assert np.max(np.abs(out[form][1] - ref)) <= 1e-5 * np.abs(ref).max()
+        np.testing.assert_array_equal(A.matmat(X), out[form][0])
+        np.testing.assert_array_equal(A.matmat_f32(X.astype(np.float32)), out[form][1])
+    np.testing.assert_allclose(out[1][0], out[2][0], rtol=1e-13, atol=1e-13 * np.abs(ref).max())
+    A.small_form(k, "fp64", set=0)
+    A.small_form(k, "mixed", set=0)
+    np.testing.assert_array_equal(A.matmat(X), out[2][0])
+    np.testing.assert_array_equal(A.matmat_f32(X.astype(np.float32)), out[2][1])
+
+
+@pytest.mark.parametrize("form", [1, 2])
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_blocked_forms_solve(oracle_mod, big_system, form, precision):
+    from hdfe_stream_amd.rocm import DeviceCSR
+    S, dinv = big_system
+    A = DeviceCSR.from_scipy(S, dinv)
+    A.small_form(3, "fp64", set=form)
+    A.small_form(3, "mixed", set=form)
+    b = _rhs(S, 3, seed=19)
+    Xc, itc, okc, _ = oracle_mod.pcg(S, dinv, b, None, None, 1e-10, 5000)
+    Xd, itd, okd, rel = A.solve(b, tol=1e-10, precision=precision)
+    assert okc and okd and rel <= 1e-10
+    if precision == "fp64":
+        assert abs(itc - itd) <= 1
+    assert np.max(np.abs(Xd - Xc)) <= 1e-8 * np.max(np.abs(Xc))
 
 
 @pytest.mark.parametrize("size", ["small", "big"])
