@@ -149,6 +149,83 @@ __global__ __launch_bounds__(kFinThreads) void k_fin_long_wide(
     }
 }
 
+// Split rows, wide k, when they are dense (every level of a small dimension
+// -- years -- couples to most others): AP[long t] = sum_r dw[r, t] P[r], with
+// dw (L, nlong) the split rows' values by column, row major.  Workgroup (b, c)
+// takes row block b of kDwBlocks and the 64 columns of tile c: each wave a
+// stride of the block's rows, each lane one column, every split row's sum in
+// registers, so P is read once for all split rows (k_long_wide gathers the
+// whole of P once per split row).  The four waves' sums are added in LDS in
+// wave order into dwpart[(b nlong + t) k + j].
+constexpr int kDwBlocks = 256;
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_dense_long_wide(
+    int64_t L, int nlong, const T *__restrict__ dw, const T *__restrict__ P,
+    double *__restrict__ dwpart, CgState st) {
+    if (st.done && *st.done) return;
+    __shared__ double red[kThreads / kWave][kDenseLongMax][kWave];
+    const int k = st.k;
+    const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
+    const int b = blockIdx.x, j = blockIdx.y * kWave + lane;
+    const int64_t r0 = part_begin(L, b, kDwBlocks), r1 = part_begin(L, b + 1, kDwBlocks);
+    double acc[kDenseLongMax];
+#pragma unroll
+    for (int t = 0; t < kDenseLongMax; ++t) acc[t] = 0.0;
+    if (j < k) {
+        for (int64_t r = r0 + wv; r < r1; r += kThreads / kWave) {
+            const double pv = (double)P[r * k + j];
+#pragma unroll
+            for (int t = 0; t < kDenseLongMax; ++t)
+                if (t < nlong) acc[t] += (double)dw[r * nlong + t] * pv;
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < kDenseLongMax; ++t)
+        if (t < nlong) red[wv][t][lane] = acc[t];
+    __syncthreads();
+    if (wv == 0 && j < k)
+        for (int t = 0; t < nlong; ++t)
+            dwpart[((int64_t)b * nlong + t) * k + j] =
+                ((red[0][t][lane] + red[1][t][lane]) + red[2][t][lane]) + red[3][t][lane];
+}
+
+// After k_dense_long_wide: each split row's AP from the row blocks' sums in
+// block order, a thread per (split row, column).
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_fin_dense_long(
+    int nlong, const int *__restrict__ long_rows, const double *__restrict__ dwpart,
+    T *__restrict__ AP, CgState st) {
+    if (st.done && *st.done) return;
+    const int k = st.k;
+    const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (q >= (int64_t)nlong * k) return;
+    const int t = (int)(q / k), j = (int)(q % k);
+    double s0 = 0.0, s1 = 0.0;
+    int b = 0;
+    for (; b + 1 < kDwBlocks; b += 2) {
+        s0 += dwpart[((int64_t)b * nlong + t) * k + j];
+        s1 += dwpart[((int64_t)(b + 1) * nlong + t) * k + j];
+    }
+    for (; b < kDwBlocks; ++b) s0 += dwpart[((int64_t)b * nlong + t) * k + j];
+    AP[(int64_t)long_rows[t] * k + j] = (T)(s0 + s1);
+}
+
+// The dense split rows' values by column: dw[col, t] = S[long t, col].
+__global__ void k_dw_fill(int nlong, int64_t L, const int *__restrict__ long_rows,
+                          const int *__restrict__ indptr, const int *__restrict__ indices,
+                          const double *__restrict__ v64, const float *__restrict__ v32,
+                          double *__restrict__ dw64, float *__restrict__ dw32) {
+    const int t = blockIdx.y;
+    const int row = long_rows[t];
+    const int s = indptr[row], e = indptr[row + 1];
+    for (int64_t p = s + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < e;
+         p += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t c = indices[p];
+        dw64[c * nlong + t] = v64[p];
+        dw32[c * nlong + t] = v32[p];
+    }
+}
+
 // One workgroup, after a wide SpMM: alpha = rz / pAp where active and
 // pAp > 0, else 0 (core/solve.py:120), from the nw workgroups' pAp sums.
 __global__ __launch_bounds__(kFinThreads) void k_fin_alpha(const double *__restrict__ part, int nw,
@@ -710,14 +787,20 @@ __global__ __launch_bounds__(kThreads) void k_spmm_wide(
 // columns cut into slabs of kSlabW, and in each launch ("round") XCD x (the
 // workgroups w with w % 8 == x) working on one slab only, several XCDs
 // sharing a slab when a round has fewer than eight.  Within an XCD the rows
-// go grid-stride, kSlabW lanes per row (four rows per wave), so an XCD's
+// go grid-stride, kSlabW lanes per row (64 / kSlabW rows per wave), so an XCD's
 // gathers in flight are one slab of its window's neighbour rows: for a
 // block-diagonal S of many models a slab of one model's block (kSlabW x 8 B
 // per level), which its 4 MB L2 can hold where the full rows (k x 8 B per
 // level) could not.  Each row's sum runs over its nonzeros in order and in
 // pairs, as in k_spmm_wide; the pAp partials are per (round, workgroup), zero
 // outside the workgroup's slab.
-constexpr int kSlabW = 16;
+// a slab's row is one 128-byte line of P: 16 columns in FP64, 32 in FP32
+// (at 16 FP32 columns a gather moved the same lines as in FP64: the FP32
+// product took 0.93 ms against FP64's 1.22 on the 256-column JLA block)
+template <typename T>
+constexpr int slab_w() {
+    return sizeof(T) == 4 ? 32 : 16;
+}
 constexpr int kSlabGroups = 128;  // workgroups per XCD
 constexpr int kPartRows = 8192;   // rows of k partials in the work buffer
 
@@ -727,6 +810,7 @@ __global__ __launch_bounds__(kThreads) void k_spmm_slab(
     int64_t L, const T *__restrict__ P, T *__restrict__ AP, double *__restrict__ part,
     int with_dot, int slab0, int nslab, CgState st) {
     if (st.done && *st.done) return;
+    constexpr int kSlabW = slab_w<T>();
     __shared__ double lds[kThreads];
     const int k = st.k;
     const int w = blockIdx.x, x = w & 7, lw = w >> 3, gx = (int)gridDim.x >> 3;
@@ -1300,12 +1384,13 @@ struct Work {
     float *X32 = nullptr, *R32 = nullptr, *P32 = nullptr, *AP32 = nullptr;
     double *part = nullptr, *lpart = nullptr, *colbuf = nullptr;
     void *gpart = nullptr;  // blocked SpMM: the groups' partial AP (8, L, k)
+    double *dwpart = nullptr;  // dense split rows, wide k: (kDwBlocks, nlong, k)
     double *sd = nullptr;  // rz alpha beta bnorm rel scale : 6k
     int *si = nullptr;     // active(k) zero(k) outer_active(k) done it it_final
     unsigned *cnt = nullptr;
     void release(hipStream_t s) {
         void *ps[] = {X64, R64, P64, AP64, B64, X32, R32, P32, AP32, part, lpart, colbuf, gpart,
-                      sd, si, cnt};
+                      dwpart, sd, si, cnt};
         for (void *p : ps) pfree(p, s);
         *this = Work();
     }
@@ -1329,6 +1414,11 @@ struct Matrix {
     int64_t bnnz = 0;
     // dense long columns: the split rows taken by symmetry in the blocked SpMM
     bool dl = false;
+    // dense split rows for wide k (k_dense_long_wide): -1 not yet looked at,
+    // 0 not dense, 1 built
+    int dw = -1;
+    double *dw64 = nullptr;  // (L, nlong)
+    float *dw32 = nullptr;
     int *dl_slot = nullptr, *dl_cptr = nullptr;  // (L) slot or -1; (kSmallK, nlong + 1)
     double *dt64 = nullptr, *ll64 = nullptr;     // (nlong, L) column-major; (nlong, nlong)
     float *dt32 = nullptr, *ll32 = nullptr;
@@ -1369,6 +1459,8 @@ static int build_blocked(Matrix &M, const std::vector<int> &ip32);
 static int spmm_grid(int64_t L, int k);
 static int tune_blocked_both(Matrix &M, int k);
 
+static int alloc_work_buffers(Matrix &M, int k);
+
 static int alloc_work(Matrix &M, int k) {
     // the blocked layout for small k, once, when P outgrows an XCD's L2
     if (k <= kSmallK && !M.blk_tried) {
@@ -1380,6 +1472,35 @@ static int alloc_work(Matrix &M, int k) {
             if (st) return st;
         }
     }
+    // the split rows by column for wide k, once, when they are dense
+    if (k > kSmallK && M.nlong && M.dw < 0) {
+        M.dw = 0;
+        if ((int64_t)M.ip_host.size() != M.L + 1) return alloc_work_buffers(M, k);
+        int64_t long_nnz = 0;
+        std::vector<int> lr(M.nlong);
+        HIP_TRY(hipMemcpy(lr.data(), M.long_rows, M.nlong * 4, hipMemcpyDeviceToHost));
+        for (int t = 0; t < M.nlong; ++t)
+            long_nnz += (int64_t)M.ip_host[lr[t] + 1] - M.ip_host[lr[t]];
+        if (M.nlong <= kDenseLongMax && 4 * long_nnz >= (int64_t)M.nlong * M.L) {
+            M.drop_graphs();
+            M.w.release(M.stream);
+            const int64_t n = (int64_t)M.nlong * M.L;
+            HIP_TRY(palloc(&M.dw64, n * 8, M.stream));
+            HIP_TRY(palloc(&M.dw32, n * 4, M.stream));
+            HIP_TRY(hipMemsetAsync(M.dw64, 0, n * 8, M.stream));
+            HIP_TRY(hipMemsetAsync(M.dw32, 0, n * 4, M.stream));
+            hipLaunchKernelGGL(k_dw_fill, dim3(64, M.nlong), dim3(kThreads), 0, M.stream, M.nlong,
+                               M.L, M.long_rows, M.indptr, M.indices, M.val64, M.val32, M.dw64,
+                               M.dw32);
+            HIP_TRY(hipGetLastError());
+            HIP_TRY(hipStreamSynchronize(M.stream));
+            M.dw = 1;
+        }
+    }
+    return alloc_work_buffers(M, k);
+}
+
+static int alloc_work_buffers(Matrix &M, int k) {
     if (M.w.k == k && M.w.L == M.L) return HDFE_STATUS_OK;
     M.drop_graphs();  // their kernels point at the work buffers
     M.w.release(M.stream);
@@ -1399,6 +1520,8 @@ static int alloc_work(Matrix &M, int k) {
     HIP_TRY(palloc(&W.lpart, std::max<int64_t>(1, std::max<int64_t>(M.nchunk, dlp) * k) * 8, M.stream));
     HIP_TRY(palloc(&W.colbuf, (int64_t)k * 8, M.stream));
     if (M.blk && k <= kSmallK) HIP_TRY(palloc(&W.gpart, (int64_t)8 * Lk * 8, M.stream));
+    if (M.dw == 1 && k > kSmallK)
+        HIP_TRY(palloc(&W.dwpart, (int64_t)kDwBlocks * M.nlong * k * 8, M.stream));
     HIP_TRY(palloc(&W.sd, ((int64_t)6 * k + 1) * 8, M.stream));
     HIP_TRY(palloc(&W.si, ((int64_t)3 * k + 8) * 4, M.stream));
     HIP_TRY(palloc(&W.cnt, 8 * sizeof(unsigned), M.stream));
@@ -1698,7 +1821,13 @@ static void launch_spmm(Matrix &M, const T *val, const T *P, T *AP, double *part
 #undef HDFE_SPMM
         return;
     }
-    if (M.nlong) {
+    if (M.nlong && M.dw == 1 && M.w.dwpart) {
+        const T *dw = sizeof(T) == 4 ? (const T *)M.dw32 : (const T *)M.dw64;
+        hipLaunchKernelGGL((k_dense_long_wide<T>), dim3(kDwBlocks, (k + kWave - 1) / kWave),
+                           dim3(kThreads), 0, s, M.L, M.nlong, dw, P, M.w.dwpart, st);
+        hipLaunchKernelGGL((k_fin_dense_long<T>), dim3((M.nlong * k + kThreads - 1) / kThreads),
+                           dim3(kThreads), 0, s, M.nlong, M.long_rows, M.w.dwpart, AP, st);
+    } else if (M.nlong) {
         hipLaunchKernelGGL((k_long_wide<T>), dim3(M.nchunk), dim3(kThreads), 4 * k * 8, s,
                            M.indptr, M.indices, val, M.chunk_row, M.chunk_beg, M.nchunk,
                            M.long_rows, M.long_cptr, M.nlong, P, AP, lpart, st);
@@ -1710,7 +1839,7 @@ static void launch_spmm(Matrix &M, const T *val, const T *P, T *AP, double *part
         auto it = M.wide_mode.find({sizeof(T) == 4 ? 1 : 0, k});
         mode = it == M.wide_mode.end() ? 0 : it->second;
     }
-    const int nslab = (k + kSlabW - 1) / kSlabW, rounds = (nslab + 7) / 8;
+    const int nslab = (k + slab_w<T>() - 1) / slab_w<T>(), rounds = (nslab + 7) / 8;
     // the slab form's pAp partials, one row of k per (round, workgroup), must
     // fit the work buffer's kPartRows rows: fewer workgroups per XCD for wide
     // k, and the wave-per-row form when not even eight per XCD would fit
@@ -2645,7 +2774,7 @@ void hdfe_destroy(void *h) {
     void *ps[] = {M->indptr, M->indices, M->val64, M->val32, M->dinv64, M->dinv32,
                   M->long_rows, M->long_cptr, M->chunk_row, M->chunk_beg, M->col_lo, M->bptr,
                   M->bidx, M->bval64, M->bval32, M->dl_slot, M->dl_cptr, M->dt64, M->dt32,
-                  M->ll64, M->ll32};
+                  M->ll64, M->ll32, M->dw64, M->dw32};
     for (void *p : ps) pfree(p, M->stream);
     if (M->mf) {
         MatFree *F = M->mf;

@@ -80,8 +80,10 @@ def test_device_spmm_is_the_operator(system, k):
     np.testing.assert_allclose(A.matmat(X), S @ X, rtol=1e-12, atol=1e-12 * np.abs(S.data).max())
 
 
-@pytest.mark.parametrize("k", [1, 3, 40])
+@pytest.mark.parametrize("k", [1, 3, 40, 256])
 def test_split_rows(long_system, k):
+    """Split rows (the years, dense: each couples to most levels); for wide k
+    they are taken by column in one pass over P."""
     from hdfe_stream_amd.rocm import DeviceCSR
     S, dinv = long_system
     A = DeviceCSR.from_scipy(S, dinv)
@@ -89,20 +91,31 @@ def test_split_rows(long_system, k):
     X = np.random.default_rng(k).standard_normal((S.shape[0], k))
     ref = S @ X
     np.testing.assert_allclose(A.matmat(X), ref, rtol=1e-11, atol=1e-11 * np.abs(ref).max())
+    got32 = A.matmat_f32(X.astype(np.float32))
+    assert np.max(np.abs(got32 - ref)) <= 1e-5 * np.abs(ref).max()
+    np.testing.assert_array_equal(A.matmat(X), A.matmat(X))
 
 
+@pytest.mark.parametrize("k", [3, 64])
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_split_rows_solve(oracle_mod, long_system, precision):
+def test_split_rows_solve(oracle_mod, long_system, precision, k):
     from hdfe_stream_amd.rocm import DeviceCSR
     S, dinv = long_system
     A = DeviceCSR.from_scipy(S, dinv)
-    b = _rhs(S, 3, seed=5)
+    b = _rhs(S, k, seed=5)
     Xc, itc, okc, _ = oracle_mod.pcg(S, dinv, b, None, None, 1e-10, 5000)
     Xd, itd, okd, rel = A.solve(b, tol=1e-10, precision=precision)
     assert okc and okd and rel <= 1e-10
+    # the device's solution meets the tolerance in the exact FP64 operator
+    r = np.linalg.norm(b - S @ Xd, axis=0) / np.linalg.norm(b, axis=0)
+    assert r.max() <= 1.5e-10
     if precision == "fp64":
         assert abs(itc - itd) <= 1
-    assert _agree(long_system, Xd, Xc, precision, 1e-7)
+    # FP64 follows the oracle's iterates; mixed reaches another point of the
+    # same residual ball, and on this system (years coupled to every level:
+    # condition ~1e4 on range(S)) two such points can differ by ~1e-6 at 64
+    # columns
+    assert _agree(long_system, Xd, Xc, precision, 1e-7 if precision == "fp64" or k <= 8 else 1e-6)
 
 
 @pytest.mark.parametrize("k", [16, 256])
@@ -360,3 +373,6 @@ def test_wide_spmm_forms_agree_with_scipy(k):
         assert M.wide_mode(k, "fp64", set=mode) == mode
         got = M.matmat(X)
         assert np.abs(got - want).max() <= 1e-12 * np.abs(want).max(), mode
+        assert M.wide_mode(k, "mixed", set=mode) == mode
+        got32 = M.matmat_f32(X.astype(np.float32))
+        assert np.abs(got32 - want).max() <= 1e-5 * np.abs(want).max(), mode
