@@ -1695,35 +1695,8 @@ __global__ void k_ss_rowcnt(int64_t n, const int *__restrict__ row, const int *_
 // ------------------------------------------------------- components
 
 // Union-find on the levels of dimension 0 linked through identifying groups
-// (core/kernels_base.py:155-174); roots end as each component's smallest
-// level, whatever the order of the unions.
-__device__ int uf_root(int *parent, int x) {
-    while (true) {
-        int p = parent[x];
-        if (p == x) return x;
-        int gp = parent[p];
-        if (gp != p) atomicCAS(&parent[x], p, gp);
-        x = p;
-    }
-}
-
-// Joins the sets of x and y: hooks the larger root under the smaller with a
-// CAS, retrying from the values a failed CAS returns until both walks meet.
-__device__ void uf_join(int *parent, int x, int y) {
-    int a = uf_root(parent, x), b = uf_root(parent, y);
-    while (a != b) {
-        if (a > b) {
-            int t = a;
-            a = b;
-            b = t;
-        }
-        int old = atomicCAS(&parent[b], b, a);
-        if (old == b) break;
-        b = uf_root(parent, old);
-        a = uf_root(parent, a);
-    }
-}
-
+// (core/kernels_base.py:155-174) with uf_root / uf_join (common.h); roots end
+// as each component's smallest level, whatever the order of the unions.
 __global__ void k_uf_init(int64_t n, int *parent) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x)

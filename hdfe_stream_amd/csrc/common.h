@@ -100,6 +100,37 @@ __device__ __forceinline__ T run_total(int key, T v, bool &last) {
     return hl > 0 ? p - before : p;
 }
 
+// Lock-free union-find over int parent links with parent[x] <= x: the root
+// of a set is its smallest member whatever the order of the unions.
+// uf_root halves the path it walks (a CAS, so a link only moves up).
+__device__ __forceinline__ int uf_root(int *parent, int x) {
+    while (true) {
+        int p = parent[x];
+        if (p == x) return x;
+        int gp = parent[p];
+        if (gp != p) atomicCAS(&parent[x], p, gp);
+        x = p;
+    }
+}
+
+// Joins the sets of x and y: hooks the larger root under the smaller with a
+// CAS, retrying from the values a failed CAS returns until both walks meet
+// (each retry lowers the larger root, so the loop ends).
+__device__ __forceinline__ void uf_join(int *parent, int x, int y) {
+    int a = uf_root(parent, x), b = uf_root(parent, y);
+    while (a != b) {
+        if (a > b) {
+            int t = a;
+            a = b;
+            b = t;
+        }
+        int old = atomicCAS(&parent[b], b, a);
+        if (old == b) break;
+        b = uf_root(parent, old);
+        a = uf_root(parent, a);
+    }
+}
+
 // Row range of part `w` of `nparts` equal contiguous parts of [0, n).
 __host__ __device__ __forceinline__ int64_t part_begin(int64_t n, int64_t w, int64_t nparts) {
     return (n * w) / nparts;

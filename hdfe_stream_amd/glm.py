@@ -169,7 +169,7 @@ def fepois_batch(fml: str, datasets: list, iwls_tol: float = 1e-8, iwls_maxiter:
                  separation_check: bool = True, tol: float = 1e-10, maxiter: int = 5000,
                  precision: str = "mixed", rows_per_syrk: int = 32768,
                  timings: dict | None = None, check_residuals: bool = False,
-                 weights: str | None = None) -> list:
+                 weights: str | None = None, solver: str = "explicit") -> list:
     """Fit `fml` by Poisson IRLS on each dataset, all on the device at once.
     Returns one GLMResult per dataset.  Every model's weighted fit of [z, X]
     on its fixed effects meets `tol` on its own relative residual (the
@@ -177,13 +177,14 @@ def fepois_batch(fml: str, datasets: list, iwls_tol: float = 1e-8, iwls_maxiter:
     `check_residuals` each model's true residuals b - S x are measured after
     every solve and reported (diagnostics["model_rel_residual"])."""
     return _glm_batch(fml, datasets, "poisson", iwls_tol, iwls_maxiter, separation_check, tol,
-                      maxiter, precision, rows_per_syrk, timings, check_residuals, weights)
+                      maxiter, precision, rows_per_syrk, timings, check_residuals, weights,
+                      solver)
 
 
 def feols_batch(fml: str, datasets: list, weights: str | None = None, tol: float = 1e-10,
                 maxiter: int = 5000, precision: str = "mixed", rows_per_syrk: int = 32768,
                 max_refinements: int = 6, timings: dict | None = None,
-                check_residuals: bool = False) -> list:
+                check_residuals: bool = False, solver: str = "explicit") -> list:
     """OLS with fixed effects on many datasets at once (feols_stream,
     api.py:22-107, of each; the wide-design path of the reference's
     assembly="rows": core/inference.py:109-137, `_nb_residualize_rows` and
@@ -195,11 +196,11 @@ def feols_batch(fml: str, datasets: list, weights: str | None = None, tol: float
     that they solve the FP64 normal equations.  `deviance` is the weighted
     residual sum of squares; `weights` names a column of analytic weights."""
     return _glm_batch(fml, datasets, "gaussian", 1e-13, max_refinements, False, tol, maxiter,
-                      precision, rows_per_syrk, timings, check_residuals, weights)
+                      precision, rows_per_syrk, timings, check_residuals, weights, solver)
 
 
 def _glm_batch(fml, datasets, family, iwls_tol, iwls_maxiter, separation_check, tol, maxiter,
-               precision, rows_per_syrk, timings, check_residuals, weights):
+               precision, rows_per_syrk, timings, check_residuals, weights, solver="explicit"):
     import torch
 
     from .estimator import Panel, _column
@@ -411,7 +412,11 @@ def _glm_batch(fml, datasets, family, iwls_tol, iwls_maxiter, separation_check, 
         _lib.check(lib.hdfe_panel_reweight(panel._h, Vz.data_ptr(), w.data_ptr()),
                    "hdfe_panel_reweight")
         refresh_ptrs()
-        panel.system(with_rhs=False)
+        if solver == "stream_cg":
+            panel.system(with_rhs=False, budget=0)
+            panel.stream_matrix()
+        else:
+            panel.system(with_rhs=False)
         if ncomp is None:
             ncomp = panel.components()
             null_id = _null_groups(panel, lmodel_l, M, ncomp)

@@ -9,7 +9,8 @@ pytestmark = pytest.mark.gpu
 
 
 @pytest.mark.parametrize("n_workers,n_firms,p_move,seed", [
-    (400, 60, 0.15, 4), (3000, 400, 0.05, 9), (2000, 80, 0.3, 2), (20000, 2000, 0.15, 3)])
+    (400, 60, 0.15, 4), (3000, 400, 0.05, 9), (2000, 80, 0.3, 2), (20000, 2000, 0.15, 3),
+    (200000, 20000, 0.05, 7)])
 def test_leave_one_out_matches_the_oracle(oracle_mod, n_workers, n_firms, p_move, seed):
     from hdfe_stream_amd.connected import leave_one_out
     from hdfe_stream_amd.simulate import akm_panel
