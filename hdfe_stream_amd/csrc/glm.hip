@@ -180,8 +180,9 @@ __global__ __launch_bounds__(256) void k_glm_center(
 // For the groups of one model per workgroup (gb..ge): x~ = x less the levels'
 // effects in Gm (L, C) less its weighted group mean, likewise z~; the score
 // partial gpart[wg, j] = sum w_i x~_ij (z~_i - x~_i'beta_m); and s = sqrt(w)
-// x~ split into bf16 hi + lo, stored in rows of eight: element (p, j) of the
-// model's padded rows at ((p / 8) * Cp + j) * 8 + p % 8 from pbase.
+// x~ split into bf16 hi + lo, row-major: element (p, j) of the padded rows at
+// p * Cp + j (a row's lanes store one contiguous stretch; the rows-of-eight
+// layout the matrix cores read is made while k_glm_syrk stages them).
 // A wave per group, lanes over columns (two sweeps of the group's rows: the
 // means, then the values).
 __global__ __launch_bounds__(256) void k_glm_partial(
@@ -237,7 +238,7 @@ __global__ __launch_bounds__(256) void k_glm_partial(
             zt = __shfl(zt, 0, 64);
             const double r = zt - dot;
             const int64_t p = pstart[m] + (i - rs[m]);
-            const int64_t base = (p >> 3) * (int64_t)Cp * 8 + (p & 7);
+            const int64_t base = p * (int64_t)Cp;
             for (int t = 0; t < kMaxJ; ++t) {
                 const int j = lane + 64 * t;
                 if (j < 1 || j >= C) continue;
@@ -245,8 +246,8 @@ __global__ __launch_bounds__(256) void k_glm_partial(
                 const float sv = (float)(sw * v[t]);
                 const __bf16 h = (__bf16)sv;
                 const __bf16 l = (__bf16)(sv - (float)h);
-                hi[base + (int64_t)(j - 1) * 8] = h;
-                lo[base + (int64_t)(j - 1) * 8] = l;
+                hi[base + j - 1] = h;
+                lo[base + j - 1] = l;
             }
         }
     }
@@ -265,7 +266,9 @@ __global__ __launch_bounds__(256) void k_glm_partial(
 
 // Upper 32x32 blocks (bi <= bj) of s's over one model's padded rows
 // [kb, ke) (multiples of 16) for the blocks q0 + wave + 8 t of this launch:
-// the workgroup stages 16 rows of every column (hi and lo) in LDS, each wave
+// the workgroup stages 16 rows of every column (hi and lo, row-major in
+// memory) in LDS as rows of eight (a column's eight rows in 16 bytes: one
+// MFMA operand per lane), each wave
 // runs three MFMAs per block and step.  out[wg, q, lane * 16 + r]: the MFMA
 // accumulator as it lies in the lanes.
 __global__ __launch_bounds__(kSyrkThreads) void k_glm_syrk(
@@ -302,12 +305,21 @@ __global__ __launch_bounds__(kSyrkThreads) void k_glm_syrk(
     const int64_t kb = wg_kb[blockIdx.x], ke = wg_ke[blockIdx.x];
     const int nvec = (int)(slice / 8);  // 16-byte vectors per array slice
     for (int64_t k0 = kb; k0 < ke; k0 += 16) {
-        const uint4 *gh = (const uint4 *)(hi + (k0 >> 3) * (int64_t)Cp * 8);
-        const uint4 *gl = (const uint4 *)(lo + (k0 >> 3) * (int64_t)Cp * 8);
+        // rows k0..k0+15 (row-major, 8 columns per vector) into the LDS
+        // rows of eight: element (r, c) at ((r / 8) * Cp + c) * 8 + r % 8
+        const bf16x8 *gh = (const bf16x8 *)(hi + k0 * (int64_t)Cp);
+        const bf16x8 *gl = (const bf16x8 *)(lo + k0 * (int64_t)Cp);
         __syncthreads();
         for (int v = threadIdx.x; v < nvec; v += blockDim.x) {
-            ((uint4 *)sh)[v] = gh[v];
-            ((uint4 *)sl)[v] = gl[v];
+            const int e = v * 8, r = e / Cp, c = e - r * Cp;
+            const bf16x8 a = gh[v], b = gl[v];
+            __bf16 *dh = sh + ((r >> 3) * Cp + c) * 8 + (r & 7);
+            __bf16 *dl = sl + ((r >> 3) * Cp + c) * 8 + (r & 7);
+#pragma unroll
+            for (int t = 0; t < 8; ++t) {
+                dh[t * 8] = a[t];
+                dl[t * 8] = b[t];
+            }
         }
         __syncthreads();
         const int rg = lane >> 5, col = lane & 31;

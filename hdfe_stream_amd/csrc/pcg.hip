@@ -31,6 +31,7 @@
 // when that true FP64 residual meets tol.
 #include "common.h"
 
+#include <climits>
 #include <cmath>
 #include <algorithm>
 #include <array>
@@ -1366,6 +1367,36 @@ __global__ void k_mf_pairs(int64_t Mc, const int *__restrict__ icodes, int64_t l
     }
 }
 
+// ----------------------------------- matrix-free S P for wide blocks (k > 8)
+//
+// With hundreds of right-hand-side columns (the GLM's [z, X], the JLA draws)
+// the per-cell vector v of k_mf_cells would be M x k (45 GB at 18.7M cells
+// and 301 columns).  The wide form keeps only the group means:
+//   m_g = sum_{c in g} n_c t_c / N_g,   t_c = sum_d P[a_cd]       (G x k)
+// and forms each level's row from them.  Levels of a *sparse* dimension
+// (firms: tens of cells each) are rows of their own:
+//   (S P)_l = sum_{c contains l} n_c (t_c - m_g(c))               k_mfw_levels
+// over the level-to-cells CSR, one workgroup per level, lanes over columns,
+// gathering the cells' other P rows and their groups' m rows.  Levels of a
+// *dense* dimension (years: many thousands of cells each, whose rows would
+// each read most of m) are summed where m_g is made: k_mfw_groups takes a
+// planned range of consecutive groups per workgroup, every lane one column
+// for all of the range's groups in order, and adds n_c (t_c - m_g) of each
+// cell into a per-lane LDS slot of the cell's dense level (the range's
+// levels of a dense dimension fall in a window of at most kMfwSlots); the
+// range's slots go to rows of dpart, and k_mfw_dense adds each dense level's
+// rows over the ranges that cover it, in range order.  Every sum runs in a
+// fixed order in FP64: results do not depend on scheduling.
+
+struct MfList {
+    int n = 0;
+    int d[kMfMaxD] = {0};
+};
+
+constexpr int kMfwSlots = 16;           // LDS slots of dense levels per range
+constexpr int kMfwRanges = 8192;        // workgroups of k_mfw_groups (target)
+constexpr int64_t kMfwDenseAvg = 1024;  // cells per level that make a dimension dense
+
 struct MatFree {
     int64_t Gi = 0, Mc = 0, ld = 0, nchunk = 0;
     MfDims dm{};
@@ -1375,7 +1406,266 @@ struct MatFree {
     void *v = nullptr;          // (Mc, k) of the current k
     double *cpart = nullptr;    // (nchunk, k)
     int vk = 0;
+    // the wide form (k > kSmallK), when a plan exists (wide)
+    bool wide = false;
+    MfList dd;                          // the dense dimensions
+    int64_t nsl = 0;                    // levels of the sparse dimensions
+    int *slev = nullptr;                // those levels
+    int *lptr = nullptr;                // (L + 1) each level's cells in lcell
+    int *cgroup = nullptr;              // (Mc) each cell's group
+    int64_t nr = 0, dprows = 0, ndl = 0;
+    int maxslot = 0;                    // the widest range's slots
+    int *rng = nullptr;                 // (nr + 1) first group of each range
+    int *rbase = nullptr, *rsoff = nullptr;  // (nr, nds) window start, slot offset
+    int *dpoff = nullptr;               // (nr + 1) first dpart row of each range
+    int *dlev = nullptr, *dcptr = nullptr, *dcrow = nullptr;  // dense levels, their rows
+    int64_t *offs_d = nullptr;          // (D) the dimensions' level offsets, on the device
+    void *mg = nullptr;                 // (Gi, k) group means, of the current k
+    double *dpart = nullptr;            // (dprows, k)
 };
+
+// Cells of a range or a level are staged kMfwStage at a time in LDS (their
+// levels, weights and dense slots or groups, loaded by all lanes at once),
+// then every lane walks them in order with kMfwIlp cells' gathers in flight.
+constexpr int kMfwStage = 256;
+constexpr int kMfwIlp = 4;
+constexpr int kMfwLanes = 512;      // k_mfw_groups' workgroup at most (a column each)
+constexpr int kMfwMaxDense = 2;     // dense dimensions summed per range (more: as sparse)
+constexpr int kMfwStageBytes = kMfwStage * (4 * (kMfMaxD + kMfwMaxDense) + 8);
+
+// Stages cells [p0, p0 + np) of range r: their levels, dense slots and
+// weights (every thread of the workgroup calls it).
+__device__ __forceinline__ void mfw_stage_cells(int p0, int np, const int *__restrict__ icodes,
+                                                int64_t ld, int D, const int64_t *__restrict__ offs,
+                                                int nds, int dd0, int dd1, int r,
+                                                const int *__restrict__ rbase,
+                                                const int *__restrict__ rsoff,
+                                                const double *__restrict__ cn,
+                                                int (*s_lv)[kMfwStage],
+                                                int (*s_sl)[kMfwStage], double *s_w) {
+    __syncthreads();
+    for (int i = threadIdx.x; i < np; i += blockDim.x) {
+        const int c = p0 + i;
+        for (int d = 0; d < D; ++d) s_lv[d][i] = (int)(offs[d] + icodes[d * ld + c]);
+        if (nds > 0) s_sl[0][i] = rsoff[r * nds] + icodes[dd0 * ld + c] - rbase[r * nds];
+        if (nds > 1)
+            s_sl[1][i] = rsoff[r * nds + 1] + icodes[dd1 * ld + c] - rbase[r * nds + 1];
+        s_w[i] = cn[c];
+    }
+    __syncthreads();
+}
+
+// Staged cells [a, e): N += n_c, S += n_c t_c and n_c t_c into each cell's
+// dense slots, kMfwIlp cells' gathers in flight (column j, if on).
+template <typename T>
+__device__ __forceinline__ void mfw_add_cells(int a, int e, bool on, int j, int k, int D, int nds,
+                                              const T *__restrict__ P, int (*s_lv)[kMfwStage],
+                                              int (*s_sl)[kMfwStage], const double *s_w,
+                                              double *slot, int bd, double &N, double &S) {
+    for (int i0 = a; i0 < e; i0 += kMfwIlp) {
+        const int nu = min(kMfwIlp, e - i0);
+        double t[kMfwIlp];
+#pragma unroll
+        for (int u = 0; u < kMfwIlp; ++u) t[u] = 0.0;
+        if (on)
+            for (int d = 0; d < D; ++d)
+#pragma unroll
+                for (int u = 0; u < kMfwIlp; ++u)
+                    if (u < nu) t[u] += (double)P[(int64_t)s_lv[d][i0 + u] * k + j];
+#pragma unroll
+        for (int u = 0; u < kMfwIlp; ++u)
+            if (u < nu) {
+                const double w = s_w[i0 + u];
+                N += w;
+                S += w * t[u];
+                for (int q = 0; q < nds; ++q) slot[s_sl[q][i0 + u] * bd + threadIdx.x] += w * t[u];
+            }
+    }
+}
+
+// Range r's groups [rng[r], rng[r+1]): m_g into mg, and each dense level's
+// part into dpart rows dpoff[r] + rsoff[r, q] + (code - rbase[r, q]).  One
+// workgroup per range; lane t takes columns j = t, t + blockDim, ...  The
+// groups are staged whole, as many as fit kMfwStage cells (a larger group
+// alone, in pieces).  A cell adds n_c t_c to its dense slots as it is read;
+// once the group's mean is known, each of its cells takes n_c m_g back out.
+template <typename T>
+__global__ __launch_bounds__(kMfwLanes) void k_mfw_groups(const int *__restrict__ rng, const int *__restrict__ rbase,
+                             const int *__restrict__ rsoff, const int *__restrict__ dpoff,
+                             const int *__restrict__ igstart, const int *__restrict__ icodes,
+                             int64_t ld, int D, const int64_t *__restrict__ offs, int nds, int dd0,
+                             int dd1, const double *__restrict__ cn, const T *__restrict__ P, int k,
+                             T *__restrict__ mg, double *__restrict__ dpart, CgState st) {
+    if (st.done && *st.done) return;
+    __shared__ int s_lv[kMfMaxD][kMfwStage];
+    __shared__ int s_sl[kMfwMaxDense][kMfwStage];
+    __shared__ double s_w[kMfwStage];
+    extern __shared__ double slot[];   // (the plan's widest range, blockDim)
+    const int r = blockIdx.x, tid = threadIdx.x, bd = blockDim.x;
+    const int gb = rng[r], ge = rng[r + 1];
+    const int nslot = dpoff[r + 1] - dpoff[r];
+    for (int j0 = 0; j0 < k; j0 += bd) {
+        const int j = j0 + tid;
+        const bool on = j < k;
+        for (int q = 0; q < nslot; ++q) slot[q * bd + tid] = 0.0;
+        for (int g = gb; g < ge;) {
+            // the whole groups [g, g2) that fit the stage (at least group g)
+            const int c0 = igstart[g];
+            int g2 = g + 1;
+            while (g2 < ge && igstart[g2 + 1] - c0 <= kMfwStage) ++g2;
+            const int cend = igstart[g2];
+            if (cend - c0 > kMfwStage) {
+                // one group larger than the stage: its pieces once for the
+                // mean (and n_c t_c into the slots), again for n_c m_g out
+                double N = 0.0, S = 0.0;
+                for (int p0 = c0; p0 < cend; p0 += kMfwStage) {
+                    const int np = min(kMfwStage, cend - p0);
+                    mfw_stage_cells(p0, np, icodes, ld, D, offs, nds, dd0, dd1, r, rbase, rsoff, cn,
+                                    s_lv, s_sl, s_w);
+                    mfw_add_cells(0, np, on, j, k, D, nds, P, s_lv, s_sl, s_w, slot, bd, N, S);
+                }
+                const double m = N > 0.0 ? S / N : 0.0;
+                if (on) mg[(int64_t)g * k + j] = (T)m;
+                if (nds)
+                    for (int p0 = c0; p0 < cend; p0 += kMfwStage) {
+                        const int np = min(kMfwStage, cend - p0);
+                        mfw_stage_cells(p0, np, icodes, ld, D, offs, nds, dd0, dd1, r, rbase, rsoff,
+                                        cn, s_lv, s_sl, s_w);
+                        for (int i = 0; i < np; ++i)
+                            for (int q = 0; q < nds; ++q) slot[s_sl[q][i] * bd + tid] -= s_w[i] * m;
+                    }
+            } else {
+                mfw_stage_cells(c0, cend - c0, icodes, ld, D, offs, nds, dd0, dd1, r, rbase, rsoff, cn,
+                                s_lv, s_sl, s_w);
+                for (int gi = g; gi < g2; ++gi) {
+                    const int a = igstart[gi] - c0, e = igstart[gi + 1] - c0;
+                    double N = 0.0, S = 0.0;
+                    mfw_add_cells(a, e, on, j, k, D, nds, P, s_lv, s_sl, s_w, slot, bd, N, S);
+                    const double m = N > 0.0 ? S / N : 0.0;
+                    if (on) mg[(int64_t)gi * k + j] = (T)m;
+                    for (int i = a; i < e; ++i)
+                        for (int q = 0; q < nds; ++q) slot[s_sl[q][i] * bd + tid] -= s_w[i] * m;
+                }
+            }
+            g = g2;
+        }
+        if (on)
+            for (int q = 0; q < nslot; ++q)
+                dpart[(int64_t)(dpoff[r] + q) * k + j] = slot[q * bd + tid];
+        __syncthreads();
+    }
+}
+
+// The sparse dimensions' rows: (S P)_l = sum over l's cells of
+// n_c (t_c - m_g(c)) = N_l P[l] + sum_c n_c (sum over the cell's other
+// dimensions of P[a_cd] - m_g(c)), the cells in level order (the
+// level-to-cells CSR), staged as in k_mfw_groups.
+template <typename T>
+__global__ void k_mfw_levels(int64_t nsl, const int *__restrict__ slev,
+                             const int *__restrict__ lptr, const int *__restrict__ lcell,
+                             const int *__restrict__ cgroup, const int *__restrict__ icodes,
+                             int64_t ld, MfDims dm, const double *__restrict__ cn,
+                             const T *__restrict__ P, const T *__restrict__ mg, int k,
+                             T *__restrict__ AP, CgState st) {
+    if (st.done && *st.done) return;
+    __shared__ int s_lv[kMfMaxD][kMfwStage];
+    __shared__ int s_g[kMfwStage];
+    __shared__ double s_w[kMfwStage];
+    const int tid = threadIdx.x, bd = blockDim.x, D = dm.D;
+    for (int64_t li = blockIdx.x; li < nsl; li += gridDim.x) {
+        const int64_t l = slev[li];
+        int own = 0;   // the level's own dimension: the highest offset at or below l
+        for (int d = 1; d < D; ++d)
+            if (dm.off[d] <= l && dm.off[d] > dm.off[own]) own = d;
+        const int q0 = lptr[l], q1 = lptr[l + 1];
+        for (int j0 = 0; j0 < k; j0 += bd) {
+            const int j = j0 + tid;
+            const bool on = j < k;
+            double out = 0.0, N = 0.0;
+            for (int p0 = q0; p0 < q1; p0 += kMfwStage) {
+                const int np = min(kMfwStage, q1 - p0);
+                __syncthreads();
+                for (int i = tid; i < np; i += bd) {
+                    const int c = lcell[p0 + i];
+                    for (int d = 0; d < D; ++d)
+                        s_lv[d][i] = (int)(dm.off[d] + icodes[d * ld + c]);
+                    s_g[i] = cgroup[c];
+                    s_w[i] = cn[c];
+                }
+                __syncthreads();
+                if (!on) continue;
+                for (int i0 = 0; i0 < np; i0 += kMfwIlp) {
+                    const int nu = min(kMfwIlp, np - i0);
+                    double t[kMfwIlp];
+#pragma unroll
+                    for (int u = 0; u < kMfwIlp; ++u)
+                        t[u] = u < nu ? -(double)mg[(int64_t)s_g[i0 + u] * k + j] : 0.0;
+                    for (int d = 0; d < D; ++d) {
+                        if (d == own) continue;
+#pragma unroll
+                        for (int u = 0; u < kMfwIlp; ++u)
+                            if (u < nu) t[u] += (double)P[(int64_t)s_lv[d][i0 + u] * k + j];
+                    }
+#pragma unroll
+                    for (int u = 0; u < kMfwIlp; ++u)
+                        if (u < nu) {
+                            out += s_w[i0 + u] * t[u];
+                            N += s_w[i0 + u];
+                        }
+                }
+            }
+            if (on) AP[l * k + j] = (T)(out + N * (double)P[l * k + j]);
+        }
+    }
+}
+
+// The dense levels' rows: each one's dpart rows over the ranges covering
+// it, in range order.
+template <typename T>
+__global__ void k_mfw_dense(int64_t ndl, const int *__restrict__ dlev,
+                            const int *__restrict__ dcptr, const int *__restrict__ dcrow,
+                            const double *__restrict__ dpart, int k, T *__restrict__ AP,
+                            CgState st) {
+    if (st.done && *st.done) return;
+    const int tid = threadIdx.x, bd = blockDim.x;
+    for (int64_t i = blockIdx.x; i < ndl; i += gridDim.x) {
+        const int64_t l = dlev[i];
+        for (int j = tid; j < k; j += bd) {
+            double out = 0.0;
+            for (int q = dcptr[i]; q < dcptr[i + 1]; ++q) out += dpart[(int64_t)dcrow[q] * k + j];
+            AP[l * k + j] = (T)out;
+        }
+    }
+}
+
+// Each identifying cell's group.
+__global__ void k_mfw_cell_groups(int64_t Gi, const int *__restrict__ igstart,
+                                  int *__restrict__ cgroup) {
+    for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < Gi;
+         g += (int64_t)gridDim.x * blockDim.x)
+        for (int c = igstart[g]; c < igstart[g + 1]; ++c) cgroup[c] = (int)g;
+}
+
+// Each group's lowest and highest code in dense dimension q (out (Gi, nds, 2);
+// a group with no cells gets (INT_MAX, -1)).
+__global__ void k_mfw_group_window(int64_t Gi, const int *__restrict__ igstart,
+                                   const int *__restrict__ icodes, int64_t ld, MfList dd,
+                                   int *__restrict__ out) {
+    const int nds = dd.n;
+    for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < Gi;
+         g += (int64_t)gridDim.x * blockDim.x)
+        for (int q = 0; q < nds; ++q) {
+            const int d = dd.d[q];
+            int lo = INT_MAX, hi = -1;
+            for (int c = igstart[g]; c < igstart[g + 1]; ++c) {
+                const int a = icodes[d * ld + c];
+                lo = min(lo, a);
+                hi = max(hi, a);
+            }
+            out[(g * nds + q) * 2] = lo;
+            out[(g * nds + q) * 2 + 1] = hi;
+        }
+}
 
 struct Work {
     int k = 0;
@@ -1531,12 +1821,19 @@ static int alloc_work_buffers(Matrix &M, int k) {
     W.L = M.L;
     if (M.mf) {
         MatFree &F = *M.mf;
-        pfree(F.v, M.stream);
-        pfree(F.cpart, M.stream);
+        void *old[] = {F.v, F.cpart, F.mg, F.dpart};
+        for (void *q : old) pfree(q, M.stream);
         F.v = nullptr;
         F.cpart = nullptr;
-        HIP_TRY(palloc(&F.v, std::max<int64_t>(F.Mc * k, 1) * 8, M.stream));
-        HIP_TRY(palloc(&F.cpart, std::max<int64_t>(F.nchunk * k, 1) * 8, M.stream));
+        F.mg = nullptr;
+        F.dpart = nullptr;
+        if (F.wide && k > kSmallK) {
+            HIP_TRY(palloc(&F.mg, std::max<int64_t>(F.Gi * k, 1) * 8, M.stream));
+            HIP_TRY(palloc(&F.dpart, std::max<int64_t>(F.dprows * k, 1) * 8, M.stream));
+        } else {
+            HIP_TRY(palloc(&F.v, std::max<int64_t>(F.Mc * k, 1) * 8, M.stream));
+            HIP_TRY(palloc(&F.cpart, std::max<int64_t>(F.nchunk * k, 1) * 8, M.stream));
+        }
         F.vk = k;
         return HDFE_STATUS_OK;
     }
@@ -1698,6 +1995,37 @@ static void launch_spmm(Matrix &M, const T *val, const T *P, T *AP, double *part
                         int with_dot, CgState st, hipStream_t s, int force_parts = 0,
                         int force_wide = -1) {
     const int k = st.k;
+    if (M.mf && M.mf->wide && k > kSmallK) {
+        MatFree &F = *M.mf;
+        T *mg = (T *)F.mg;
+        const int bd = (int)std::min<int64_t>(1024, (k + kWave - 1) / kWave * kWave);
+        // k_mfw_groups' lanes: its slots and stage within 64 KB of LDS
+        int bg = std::min(bd, kMfwLanes);
+        while (bg > kWave && (int64_t)F.maxslot * bg * 8 + kMfwStageBytes > 65536) bg -= kWave;
+        if (F.nr)
+            hipLaunchKernelGGL((k_mfw_groups<T>), dim3(F.nr), dim3(bg),
+                               (size_t)F.maxslot * bg * sizeof(double), s, F.rng, F.rbase,
+                               F.rsoff, F.dpoff, F.igstart, F.icodes, F.ld, F.dm.D, F.offs_d,
+                               F.dd.n, F.dd.d[0], F.dd.d[1], F.cn, P, k, mg, F.dpart, st);
+        if (F.nsl)
+            hipLaunchKernelGGL((k_mfw_levels<T>), dim3((int)std::min<int64_t>(F.nsl, 16384)),
+                               dim3(bd), 0, s, F.nsl, F.slev, F.lptr, F.lcell, F.cgroup, F.icodes,
+                               F.ld, F.dm, F.cn, P, (const T *)mg, k, AP, st);
+        if (F.ndl)
+            hipLaunchKernelGGL((k_mfw_dense<T>), dim3((int)std::min<int64_t>(F.ndl, 4096)), dim3(bd),
+                               0, s, F.ndl, F.dlev, F.dcptr, F.dcrow, F.dpart, k, AP, st);
+        if (with_dot) {
+            const int gv = vec_grid(M.L, k);
+            hipLaunchKernelGGL((k_colsum<T>), dim3(gv), dim3(kThreads), 0, s, M.L, k, P, AP,
+                               (const T *)nullptr, part);
+            hipLaunchKernelGGL(k_fin_alpha, dim3(1), dim3(kFinThreads), 0, s, part, gv, st);
+        }
+        (void)force_parts;
+        (void)force_wide;
+        (void)val;
+        (void)lpart;
+        return;
+    }
     if (M.mf) {
         MatFree &F = *M.mf;
         T *v = (T *)F.v;
@@ -2380,6 +2708,135 @@ int matrix_create(int64_t L, int64_t nnz, const std::vector<int> &ip32, const in
     return HDFE_STATUS_OK;
 }
 
+// The wide form's plan (matfree_create): the dense dimensions, the sparse
+// levels, each cell's group, the ranges of consecutive groups with their
+// dense windows (at most kMfwSlots slots in all), and for every dense level
+// the dpart rows of the ranges that cover it, in range order.  A group whose
+// own window exceeds the slots leaves the wide form off (k_mf_cells serves).
+static int mfw_plan(Matrix &M, MatFree &F, const std::vector<int> &cnt, hipStream_t ms) {
+    const int64_t L = M.L, Gi = F.Gi, Mc = F.Mc;
+    const int D = F.dm.D;
+    std::vector<int> lptr(L + 1, 0);
+    for (int64_t l = 0; l < L; ++l) lptr[l + 1] = lptr[l] + cnt[l];
+    // a dimension's levels run from its offset to the next offset above it
+    int64_t dlo[kMfMaxD], dhi[kMfMaxD];
+    bool dense[kMfMaxD] = {false};
+    F.dd.n = 0;
+    for (int d = 0; d < D; ++d) {
+        dlo[d] = F.dm.off[d];
+        dhi[d] = L;
+        for (int e = 0; e < D; ++e)
+            if (F.dm.off[e] > dlo[d]) dhi[d] = std::min<int64_t>(dhi[d], F.dm.off[e]);
+        const int64_t nl = dhi[d] - dlo[d];
+        if (nl > 0 && Mc / nl >= kMfwDenseAvg && F.dd.n < kMfwMaxDense) {
+            dense[d] = true;
+            F.dd.d[F.dd.n++] = d;
+        }
+    }
+    const int nds = F.dd.n;
+    std::vector<int> slev, dlev;
+    for (int d = 0; d < D; ++d)
+        for (int64_t l = dlo[d]; l < dhi[d]; ++l) (dense[d] ? dlev : slev).push_back((int)l);
+    std::vector<int> win;
+    if (nds) {
+        int *wd = nullptr;
+        HIP_TRY(palloc(&wd, std::max<int64_t>(Gi * nds * 2, 1) * 4, ms));
+        hipLaunchKernelGGL(k_mfw_group_window, dim3(grid_for(std::max<int64_t>(Gi, 1), 256, 1, 8192)),
+                           dim3(kThreads), 0, ms, Gi, F.igstart, F.icodes, F.ld, F.dd, wd);
+        win.resize(Gi * nds * 2);
+        if (Gi) HIP_TRY(hipMemcpyAsync(win.data(), wd, Gi * nds * 2 * 4, hipMemcpyDeviceToHost, ms));
+        HIP_TRY(hipStreamSynchronize(ms));
+        pfree(wd, ms);
+    }
+    const int64_t R = std::max<int64_t>(1, (Gi + kMfwRanges - 1) / kMfwRanges);
+    std::vector<int> rng{0}, rbase, rsoff, dpoff{0};
+    F.maxslot = 0;
+    for (int64_t g = 0; g < Gi;) {
+        int wl[kMfMaxD], wh[kMfMaxD];
+        for (int q = 0; q < nds; ++q) {
+            wl[q] = INT_MAX;
+            wh[q] = -1;
+        }
+        int64_t e = g;
+        while (e < Gi && e - g < R) {
+            int nl[kMfMaxD], nh[kMfMaxD], tot = 0;
+            for (int q = 0; q < nds; ++q) {
+                nl[q] = std::min(wl[q], win[(e * nds + q) * 2]);
+                nh[q] = std::max(wh[q], win[(e * nds + q) * 2 + 1]);
+                tot += nh[q] >= nl[q] ? nh[q] - nl[q] + 1 : 0;
+            }
+            if (tot > kMfwSlots) break;
+            for (int q = 0; q < nds; ++q) {
+                wl[q] = nl[q];
+                wh[q] = nh[q];
+            }
+            ++e;
+        }
+        if (e == g) return HDFE_STATUS_OK;   // one group alone is too wide: no wide form
+        int off = 0;
+        for (int q = 0; q < nds; ++q) {
+            rbase.push_back(wh[q] >= wl[q] ? wl[q] : 0);
+            rsoff.push_back(off);
+            off += wh[q] >= wl[q] ? wh[q] - wl[q] + 1 : 0;
+        }
+        dpoff.push_back(dpoff.back() + off);
+        F.maxslot = std::max(F.maxslot, off);
+        rng.push_back((int)e);
+        g = e;
+    }
+    const int64_t nr = (int64_t)rng.size() - 1;
+    // every dense level's rows, range by range
+    std::vector<int> dindex(L, -1);
+    for (size_t i = 0; i < dlev.size(); ++i) dindex[dlev[i]] = (int)i;
+    std::vector<int> dcnt(dlev.size() + 1, 0), dcptr(dlev.size() + 1, 0), dcrow;
+    for (int pass = 0; pass < 2; ++pass) {
+        std::vector<int> fill;
+        if (pass) {
+            for (size_t i = 0; i < dlev.size(); ++i) dcptr[i + 1] = dcptr[i] + dcnt[i];
+            dcrow.assign(dcptr.back(), 0);
+            fill.assign(dcptr.begin(), dcptr.end() - 1);
+        }
+        for (int64_t r = 0; r < nr; ++r)
+            for (int q = 0; q < nds; ++q) {
+                const int qs = rsoff[r * nds + q];
+                const int span = (q + 1 < nds ? rsoff[r * nds + q + 1] : dpoff[r + 1] - dpoff[r]) - qs;
+                for (int c = 0; c < span; ++c) {
+                    const int l = (int)dlo[F.dd.d[q]] + rbase[r * nds + q] + c;
+                    const int i = dindex[l];
+                    if (pass) dcrow[fill[i]++] = dpoff[r] + qs + c;
+                    else ++dcnt[i];
+                }
+            }
+    }
+    F.nsl = (int64_t)slev.size();
+    F.ndl = (int64_t)dlev.size();
+    F.nr = nr;
+    F.dprows = dpoff.back();
+    auto up = [&](int **dst, const std::vector<int> &src) -> hipError_t {
+        hipError_t e = palloc(dst, std::max<size_t>(src.size(), 1) * 4, ms);
+        if (e != hipSuccess || src.empty()) return e;
+        return hipMemcpyAsync(*dst, src.data(), src.size() * 4, hipMemcpyHostToDevice, ms);
+    };
+    HIP_TRY(palloc(&F.offs_d, kMfMaxD * 8, ms));
+    HIP_TRY(hipMemcpyAsync(F.offs_d, F.dm.off, D * 8, hipMemcpyHostToDevice, ms));
+    HIP_TRY(up(&F.lptr, lptr));
+    HIP_TRY(up(&F.slev, slev));
+    HIP_TRY(up(&F.rng, rng));
+    HIP_TRY(up(&F.rbase, rbase));
+    HIP_TRY(up(&F.rsoff, rsoff));
+    HIP_TRY(up(&F.dpoff, dpoff));
+    HIP_TRY(up(&F.dlev, dlev));
+    HIP_TRY(up(&F.dcptr, dcptr));
+    HIP_TRY(up(&F.dcrow, dcrow));
+    HIP_TRY(palloc(&F.cgroup, std::max<int64_t>(Mc, 1) * 4, ms));
+    hipLaunchKernelGGL(k_mfw_cell_groups, dim3(grid_for(std::max<int64_t>(Gi, 1), 256, 1, 8192)),
+                       dim3(kThreads), 0, ms, Gi, F.igstart, F.cgroup);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(ms));   // the host vectors above go out of scope
+    F.wide = true;
+    return HDFE_STATUS_OK;
+}
+
 // A matrix-free operator over a panel's identifying cells (see k_mf_cells):
 // the level-to-cells CSR by a stable radix sort of the (level, cell) pairs
 // (each level's cells ascending), its chunks, dinv64 copied and cast.
@@ -2458,6 +2915,7 @@ int matfree_create(int64_t L, int64_t Gi, int64_t Mc, int D, const int *igstart,
     HIP_TRY(hipMemcpyAsync(F->lch, lch.data(), (L + 1) * 4, hipMemcpyHostToDevice, ms));
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(ms));
+    TRY(mfw_plan(*M, *F, cnt, ms));
     *out = M.release();
     return HDFE_STATUS_OK;
 }
@@ -2778,7 +3236,9 @@ void hdfe_destroy(void *h) {
     for (void *p : ps) pfree(p, M->stream);
     if (M->mf) {
         MatFree *F = M->mf;
-        void *fs[] = {F->lcell, F->ch_beg, F->ch_end, F->lch, F->v, F->cpart};
+        void *fs[] = {F->lcell, F->ch_beg, F->ch_end, F->lch, F->v, F->cpart, F->slev,
+                      F->lptr, F->cgroup, F->rng, F->rbase, F->rsoff, F->dpoff, F->dlev,
+                      F->dcptr, F->dcrow, F->mg, F->dpart, F->offs_d};
         for (void *p : fs) pfree(p, M->stream);
         delete F;
         M->mf = nullptr;

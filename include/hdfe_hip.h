@@ -518,8 +518,8 @@ int hdfe_glm_rhs(void *rows_h, int64_t G, const int *gstart, const int *g, const
                  const float *X, int kx, const double *w, double *vbar, double *rhs);
 
 /* The covariates and working response partialled on the fixed effects
- * (from Gm (L, kx + 1)); sqrt(w) X~ split into bf16 hi + lo in rows of eight
- * ((p / 8, Cp, 8), model m's padded rows from pstart[m]); and per workgroup
+ * (from Gm (L, kx + 1)); sqrt(w) X~ split into bf16 hi + lo, row-major
+ * ((padded rows, Cp), model m's padded rows from pstart[m]); and per workgroup
  * (host lists: model, group range) the score X~'W (z~ - X~ beta_m), gpart
  * (host, nwg x kx). */
 int hdfe_glm_partial(int nwg, const int *wg_model, const int64_t *wg_gb, const int64_t *wg_ge,
@@ -529,7 +529,8 @@ int hdfe_glm_partial(int nwg, const int *wg_model, const int64_t *wg_gb, const i
                      const double *beta, void *hi, void *lo, double *gpart_h);
 
 /* A (nmodels, Cp, Cp, FP64, device) = (hi + lo)'(hi + lo) per model over its
- * padded rows, as hi'hi + hi'lo + lo'hi on the matrix cores (split-BF16, FP32
+ * padded rows (hi, lo row-major (padded rows, Cp) bf16), as hi'hi + hi'lo +
+ * lo'hi on the matrix cores (split-BF16, FP32
  * accumulation per workgroup of rows_per_wg rows, FP64 across them): the
  * dense X'WX of glm.py:990. */
 int hdfe_glm_syrk(int nmodels, const int64_t *pstart_h, int Cp, const void *hi, const void *lo,

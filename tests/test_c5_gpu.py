@@ -23,17 +23,18 @@ GOLDEN = np.load(Path(__file__).parent / "golden" / "c5_poisson.npz")
 FML = "y ~ X | worker_id + firm_id + year"
 
 
-@pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_c5_batch_matches_the_oracle_model_by_model(precision):
+@pytest.mark.parametrize("precision,solver", [("fp64", "explicit"), ("mixed", "explicit"),
+                                              ("fp64", "stream_cg"), ("mixed", "stream_cg")])
+def test_c5_batch_matches_the_oracle_model_by_model(precision, solver):
     from hdfe_stream_amd.glm import fepois_batch
     from hdfe_stream_amd.simulate import poisson_models
     n_models, lo, hi, n_cov, seed = (int(v) for v in GOLDEN["spec"])
     ds = poisson_models(n_models, lo=lo, hi=hi, n_cov=n_cov, seed=seed)
     assert [len(d["y"]) for d in ds] == GOLDEN["rows"].tolist()
-    print(f"C5 batch of {n_models} x {n_cov} covariates, {precision}", flush=True)
+    print(f"C5 batch of {n_models} x {n_cov} covariates, {precision}, {solver}", flush=True)
     tol = 1e-10
     res = fepois_batch(FML, ds, iwls_tol=1e-10, iwls_maxiter=50, tol=tol, precision=precision,
-                       check_residuals=True)
+                       check_residuals=True, solver=solver)
     for m, r in enumerate(res):
         assert r.converged, m
         assert r.n_obs == GOLDEN["kept"][m], m

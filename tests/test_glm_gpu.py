@@ -29,9 +29,8 @@ def test_split_bf16_syrk_matches_fp64():
     hi = St.to(torch.bfloat16)
     lo = (St - hi.float()).to(torch.bfloat16)
     H, Lo = hi.double().numpy(), lo.double().numpy()
-    k8 = lambda t: t.reshape(npad // 8, 8, Cp).permute(0, 2, 1).contiguous().cuda()
     A = torch.empty((len(n_m), Cp, Cp), dtype=torch.float64, device="cuda")
-    hi_d, lo_d = k8(hi), k8(lo)          # held: the launch reads them
+    hi_d, lo_d = hi.contiguous().cuda(), lo.contiguous().cuda()   # held: the launch reads them
     torch.cuda.synchronize()
     _lib.check(_lib.load().hdfe_glm_syrk(len(n_m), pstart.ctypes.data, Cp, hi_d.data_ptr(),
                                          lo_d.data_ptr(), 256, A.data_ptr()), "hdfe_glm_syrk")

@@ -31,12 +31,16 @@ def test_stream_product_equals_the_explicit_product(c1_panel, weighted):
     L = P.info["L"]
     rng = np.random.default_rng(1)
     print(f"stream product weighted={weighted}", flush=True)
-    for k in (1, 3, 12):
+    for k in (1, 3, 12, 64, 301):
+        # k > 8 takes the wide form (group means; the years, a dense dimension,
+        # summed per range of groups)
         X = torch.from_numpy(rng.standard_normal((L, k))).cuda()
         want = explicit.matmat(X).cpu().numpy()
         stream = P.stream_matrix()
         got = stream.matmat(X).cpu().numpy()
         assert np.abs(got - want).max() <= 1e-11 * np.abs(want).max(), k
+        got32 = stream.matmat_f32(X.cpu().numpy())
+        assert np.abs(got32 - want).max() <= 1e-5 * np.abs(want).max(), k
     P.close()
 
 
