@@ -29,6 +29,7 @@
 namespace hdfe {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 constexpr int kGlmRowChunk = 1024;   // level chunks of the row CSR (rows.hip)
@@ -303,23 +304,30 @@ __global__ __launch_bounds__(kSyrkThreads) void k_glm_syrk(
 #pragma unroll
         for (int r = 0; r < 16; ++r) acc[t][r] = 0.f;
     const int64_t kb = wg_kb[blockIdx.x], ke = wg_ke[blockIdx.x];
-    const int nvec = (int)(slice / 8);  // 16-byte vectors per array slice
+    const int cpair = Cp / 2;  // column pairs per eight rows
     for (int64_t k0 = kb; k0 < ke; k0 += 16) {
         // rows k0..k0+15 (row-major, 8 columns per vector) into the LDS
         // rows of eight: element (r, c) at ((r / 8) * Cp + c) * 8 + r % 8
-        const bf16x8 *gh = (const bf16x8 *)(hi + k0 * (int64_t)Cp);
-        const bf16x8 *gl = (const bf16x8 *)(lo + k0 * (int64_t)Cp);
+        // a thread per two columns of eight rows: eight 4-byte loads per
+        // array (a row's pairs are contiguous across the threads), two
+        // columns' 16 bytes out
         __syncthreads();
-        for (int v = threadIdx.x; v < nvec; v += blockDim.x) {
-            const int e = v * 8, r = e / Cp, c = e - r * Cp;
-            const bf16x8 a = gh[v], b = gl[v];
-            __bf16 *dh = sh + ((r >> 3) * Cp + c) * 8 + (r & 7);
-            __bf16 *dl = sl + ((r >> 3) * Cp + c) * 8 + (r & 7);
+        for (int tile = threadIdx.x; tile < Cp; tile += blockDim.x) {
+            const int rg = tile / cpair, c0 = (tile - rg * cpair) * 2;
+            bf16x8 h0, h1, l0, l1;
 #pragma unroll
-            for (int t = 0; t < 8; ++t) {
-                dh[t * 8] = a[t];
-                dl[t * 8] = b[t];
+            for (int r = 0; r < 8; ++r) {
+                const int64_t off = (k0 + rg * 8 + r) * (int64_t)Cp + c0;
+                const bf16x2 a = *(const bf16x2 *)(hi + off), b = *(const bf16x2 *)(lo + off);
+                h0[r] = a[0];
+                h1[r] = a[1];
+                l0[r] = b[0];
+                l1[r] = b[1];
             }
+            *(bf16x8 *)(sh + (rg * Cp + c0) * 8) = h0;
+            *(bf16x8 *)(sh + (rg * Cp + c0 + 1) * 8) = h1;
+            *(bf16x8 *)(sl + (rg * Cp + c0) * 8) = l0;
+            *(bf16x8 *)(sl + (rg * Cp + c0 + 1) * 8) = l1;
         }
         __syncthreads();
         const int rg = lane >> 5, col = lane & 31;

@@ -1429,12 +1429,14 @@ struct MatFree {
 // then every lane walks them in order with kMfwIlp cells' gathers in flight.
 constexpr int kMfwStage = 256;
 constexpr int kMfwIlp = 4;
+constexpr int kMfwIlpGroups = 8;    // k_mfw_groups' cells in flight across group ends
 constexpr int kMfwLanes = 512;      // k_mfw_groups' workgroup at most (a column each)
 constexpr int kMfwMaxDense = 2;     // dense dimensions summed per range (more: as sparse)
 constexpr int kMfwStageBytes = kMfwStage * (4 * (kMfMaxD + kMfwMaxDense) + 8);
 
 // Stages cells [p0, p0 + np) of range r: their levels, dense slots and
-// weights (every thread of the workgroup calls it).
+// weights, and the ends of groups [g, g2) relative to p0 (every thread of
+// the workgroup calls it).
 __device__ __forceinline__ void mfw_stage_cells(int p0, int np, const int *__restrict__ icodes,
                                                 int64_t ld, int D, const int64_t *__restrict__ offs,
                                                 int nds, int dd0, int dd1, int r,
@@ -1442,8 +1444,11 @@ __device__ __forceinline__ void mfw_stage_cells(int p0, int np, const int *__res
                                                 const int *__restrict__ rsoff,
                                                 const double *__restrict__ cn,
                                                 int (*s_lv)[kMfwStage],
-                                                int (*s_sl)[kMfwStage], double *s_w) {
+                                                int (*s_sl)[kMfwStage], double *s_w,
+                                                const int *__restrict__ igstart, int g, int g2,
+                                                int *s_gend) {
     __syncthreads();
+    for (int q = threadIdx.x; q < g2 - g; q += blockDim.x) s_gend[q] = igstart[g + 1 + q] - p0;
     for (int i = threadIdx.x; i < np; i += blockDim.x) {
         const int c = p0 + i;
         for (int d = 0; d < D; ++d) s_lv[d][i] = (int)(offs[d] + icodes[d * ld + c]);
@@ -1500,6 +1505,7 @@ __global__ __launch_bounds__(kMfwLanes) void k_mfw_groups(const int *__restrict_
     __shared__ int s_lv[kMfMaxD][kMfwStage];
     __shared__ int s_sl[kMfwMaxDense][kMfwStage];
     __shared__ double s_w[kMfwStage];
+    __shared__ int s_gend[kMfwStage];
     extern __shared__ double slot[];   // (the plan's widest range, blockDim)
     const int r = blockIdx.x, tid = threadIdx.x, bd = blockDim.x;
     const int gb = rng[r], ge = rng[r + 1];
@@ -1521,7 +1527,7 @@ __global__ __launch_bounds__(kMfwLanes) void k_mfw_groups(const int *__restrict_
                 for (int p0 = c0; p0 < cend; p0 += kMfwStage) {
                     const int np = min(kMfwStage, cend - p0);
                     mfw_stage_cells(p0, np, icodes, ld, D, offs, nds, dd0, dd1, r, rbase, rsoff, cn,
-                                    s_lv, s_sl, s_w);
+                                    s_lv, s_sl, s_w, igstart, g, g + 1, s_gend);
                     mfw_add_cells(0, np, on, j, k, D, nds, P, s_lv, s_sl, s_w, slot, bd, N, S);
                 }
                 const double m = N > 0.0 ? S / N : 0.0;
@@ -1530,22 +1536,51 @@ __global__ __launch_bounds__(kMfwLanes) void k_mfw_groups(const int *__restrict_
                     for (int p0 = c0; p0 < cend; p0 += kMfwStage) {
                         const int np = min(kMfwStage, cend - p0);
                         mfw_stage_cells(p0, np, icodes, ld, D, offs, nds, dd0, dd1, r, rbase, rsoff,
-                                        cn, s_lv, s_sl, s_w);
+                                        cn, s_lv, s_sl, s_w, igstart, g, g + 1, s_gend);
                         for (int i = 0; i < np; ++i)
                             for (int q = 0; q < nds; ++q) slot[s_sl[q][i] * bd + tid] -= s_w[i] * m;
                     }
             } else {
                 mfw_stage_cells(c0, cend - c0, icodes, ld, D, offs, nds, dd0, dd1, r, rbase, rsoff, cn,
-                                s_lv, s_sl, s_w);
-                for (int gi = g; gi < g2; ++gi) {
-                    const int a = igstart[gi] - c0, e = igstart[gi + 1] - c0;
-                    double N = 0.0, S = 0.0;
-                    mfw_add_cells(a, e, on, j, k, D, nds, P, s_lv, s_sl, s_w, slot, bd, N, S);
+                                s_lv, s_sl, s_w, igstart, g, g2, s_gend);
+                // every staged cell in order, kMfwIlpGroups cells' gathers in flight
+                // across group ends; a group is finished when its end passes
+                const int ncell = cend - c0;
+                int gi = g, gbeg = 0, gend = s_gend[0];
+                double N = 0.0, S = 0.0;
+                auto finish = [&]() {
                     const double m = N > 0.0 ? S / N : 0.0;
                     if (on) mg[(int64_t)gi * k + j] = (T)m;
-                    for (int i = a; i < e; ++i)
+                    for (int i = gbeg; i < gend; ++i)
                         for (int q = 0; q < nds; ++q) slot[s_sl[q][i] * bd + tid] -= s_w[i] * m;
+                    N = 0.0;
+                    S = 0.0;
+                    ++gi;
+                    gbeg = gend;
+                    gend = gi < g2 ? s_gend[gi - g] : ncell;
+                };
+                for (int i0 = 0; i0 < ncell; i0 += kMfwIlpGroups) {
+                    const int nu = min(kMfwIlpGroups, ncell - i0);
+                    double t[kMfwIlpGroups];
+#pragma unroll
+                    for (int u = 0; u < kMfwIlpGroups; ++u) t[u] = 0.0;
+                    if (on)
+                        for (int d = 0; d < D; ++d)
+#pragma unroll
+                            for (int u = 0; u < kMfwIlpGroups; ++u)
+                                if (u < nu) t[u] += (double)P[(int64_t)s_lv[d][i0 + u] * k + j];
+#pragma unroll
+                    for (int u = 0; u < kMfwIlpGroups; ++u)
+                        if (u < nu) {
+                            const int i = i0 + u;
+                            while (i == gend) finish();
+                            const double w = s_w[i];
+                            N += w;
+                            S += w * t[u];
+                            for (int q = 0; q < nds; ++q) slot[s_sl[q][i] * bd + tid] += w * t[u];
+                        }
                 }
+                while (gi < g2) finish();
             }
             g = g2;
         }
