@@ -482,16 +482,20 @@ def main():
     if not (args.no_jla and args.no_kss and args.no_adjoint and args.no_glm):
         del results, f64, mix
     if not args.no_jla:
+        _lib.release_memory()   # the previous phase's cached blocks back to the driver
         log("JLA phase")
         line["jla"] = jla_phase(args.jla_workers, args.jla_firms, args.jla_draws)
     if not args.no_kss:
+        _lib.release_memory()   # the previous phase's cached blocks back to the driver
         log("KSS phase")
         line["kss"] = kss_phase(args.jla_workers, args.jla_firms, args.jla_draws)
     if not args.no_adjoint:
+        _lib.release_memory()   # the previous phase's cached blocks back to the driver
         log("adjoint phase")
         line["adjoint"] = adjoint_phase(args.adjoint_workers, args.adjoint_firms,
                                         args.adjoint_fd_rows)
     if not args.no_glm:
+        _lib.release_memory()   # the previous phase's cached blocks back to the driver
         log("GLM phase")
         line["glm"] = glm_phase(args.glm_models, args.glm_lo, args.glm_hi, args.glm_cov)
     if not args.no_cpu:

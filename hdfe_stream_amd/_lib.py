@@ -42,6 +42,7 @@ SIGNATURES = [
     ("hdfe_csr_spmm_time", I32, [P, I64, I32, I32, P]),
     ("hdfe_csr_timing_read", I32, [P, P, P]),
     ("hdfe_memcpy", I32, [P, P, I64, I32]),
+    ("hdfe_cache_release", I32, [P]),
     ("hdfe_destroy", None, [P]),
     ("hdfe_panel_create", I32, [I64, I32, P, I32, P, P, I32, I32, I32, I32, P]),
     ("hdfe_panel_info", I32, [P, P, P, P]),
@@ -169,6 +170,17 @@ def require_device() -> str:
         raise RuntimeError("device='rocm': no HIP device visible")
     check(st, "hdfe_device_info")
     return name.value.decode()
+
+
+def release_memory() -> int:
+    """Device memory the library and torch hold for reuse, returned to the
+    driver (torch's cache emptied, the library's free blocks freed); the
+    bytes the library returned."""
+    import torch
+    freed = C.c_int64(0)
+    check(load().hdfe_cache_release(C.byref(freed)), "hdfe_cache_release")
+    torch.cuda.empty_cache()
+    return int(freed.value)
 
 
 def device_memory() -> tuple[int, int]:

@@ -358,11 +358,17 @@ struct RowPasses {
 
 extern "C" {
 
+// The entry points below that launch on the null stream synchronise the
+// device before their first launch and after their last, so that work the
+// caller queued on any stream (torch's current one included) is done before
+// they read or write its buffers, and theirs is done when they return.
+
 // A draw table on the device: out (n, k) = draws (ordinal + i, first + j).
 int hdfe_rademacher(int64_t ordinal, int64_t n, int k, uint64_t seed, int64_t first_draw,
                     double *out) {
     ARG_CHECK(out && n >= 0 && k >= 1, "hdfe_rademacher: bad argument");
     if (n == 0) return HDFE_STATUS_OK;
+    HIP_TRY(hipDeviceSynchronize());
     hipLaunchKernelGGL(k_rademacher, dim3(grid1(n * k)), dim3(256), 0, 0, ordinal, n, k, seed,
                        first_draw, out);
     HIP_TRY(hipGetLastError());
@@ -516,6 +522,7 @@ int hdfe_rows_project_moments(void *h, const int *gstart, const int *codes, cons
 // P_bar and the largest |P + M - 1| of the raw estimates.
 int hdfe_jla_combine(int64_t n, int64_t n_draws, double *moments, double *stats) {
     ARG_CHECK(moments && stats && n >= 1 && n_draws >= 1, "hdfe_jla_combine: bad argument");
+    HIP_TRY(hipDeviceSynchronize());
     const int nb = 256;
     DevBuf<double> part;
     DEV_ALLOC(part, nb * 4, nullptr);
@@ -557,9 +564,11 @@ int hdfe_rows_csr(void *h, const int **rptr, const int **rrow, const int **cptr,
 int hdfe_rows_chunk_sum(void *h, int k, const double *part, double *out) {
     ARG_CHECK(h && part && out, "hdfe_rows_chunk_sum: null argument");
     RowPasses &R = *(RowPasses *)h;
+    HIP_TRY(hipDeviceSynchronize());
     hipLaunchKernelGGL(k_center_sum, dim3(grid1(R.L * k)), dim3(256), 0, 0, R.L, k, R.cptr.p,
                        part, out);
     HIP_TRY(hipGetLastError());
+    HIP_TRY(hipDeviceSynchronize());
     return HDFE_STATUS_OK;
 }
 

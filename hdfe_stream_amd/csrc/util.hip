@@ -1,5 +1,7 @@
-// Error reporting and device facts for the C ABI.
+// Error reporting, device facts and the device-memory cache's release for
+// the C ABI.
 #include "common.h"
+#include "devbuf.h"
 
 #include <cstring>
 
@@ -46,6 +48,19 @@ int hdfe_device_info(char *name, int len, int64_t *free_bytes, int64_t *total_by
     HIP_TRY(hipMemGetInfo(&f, &t));
     if (free_bytes) *free_bytes = (int64_t)f;
     if (total_bytes) *total_bytes = (int64_t)t;
+    return HDFE_STATUS_OK;
+}
+
+// Returns the cache's free blocks to the driver (slabs of small blocks
+// stay); *freed: bytes returned.  Synchronises the device first.
+int hdfe_cache_release(int64_t *freed) {
+    hdfe::BigCache &c = hdfe::big_cache();
+    std::lock_guard<std::mutex> lock(c.mu);
+    int64_t b = 0;
+    for (const auto &kv : c.free_list)
+        if (!kv.second.slab) b += (int64_t)kv.second.bytes;
+    HIP_TRY(hdfe::big_release_free(c));
+    if (freed) *freed = b;
     return HDFE_STATUS_OK;
 }
 

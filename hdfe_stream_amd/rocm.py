@@ -36,7 +36,7 @@ def rocm_backend():
     return SimpleNamespace(
         lib=lib, name=name,
         free_bytes=lambda: _lib.device_memory()[0],
-        release=lambda: None,
+        release=_lib.release_memory,
         csr=lambda S: DeviceCSR.from_scipy(S, None),
     )
 

@@ -134,6 +134,16 @@ int hdfe_csr_spmm_time(void *h, int64_t k, int reps, int precision, double *avg_
  * in a fixed order: the zero-column test of core/solve.py:354-355. */
 int hdfe_col_norms(const double *X, int64_t L, int64_t k, double *out);
 
+/* Returns the library's cached free device blocks to the driver (*freed:
+ * their bytes; the slabs of blocks under 256 KB stay), after synchronising
+ * the device.  The library keeps freed blocks for reuse and otherwise
+ * returns them only when one of its own allocations fails, so a caller that
+ * allocates elsewhere (torch) calls this after a large phase or before
+ * retrying an allocation that failed.  No counterpart in the reference (its
+ * device memory is CuPy's pool, released by the backend's release(),
+ * core/gpu.py:33-51). */
+int hdfe_cache_release(int64_t *freed);
+
 /* dst <- src; kind 0 device to host, 1 host to device, 2 device to device. */
 int hdfe_memcpy(void *dst, const void *src, int64_t bytes, int kind);
 

@@ -195,3 +195,18 @@ def test_level_values_contiguous_and_gapped(oracle_mod):
     for f in FE:
         np.testing.assert_array_equal(res.fe_levels[f], np.unique(d[f][keep]))
         assert res.fe_levels[f].dtype == np.int64
+
+
+def test_fit_after_the_memory_cache_is_released(oracle_mod, c1_panel):
+    """The library's cached blocks go back to the driver (the backend's
+    release()), and the next fit allocates afresh with the same result."""
+    from hdfe_stream_amd import _lib
+    from hdfe_stream_amd.estimator import feols_stream
+    fml = "y ~ x1 + x2 | worker_id + firm_id + year"
+    a = feols_stream(fml, c1_panel, tol=1e-10)
+    del a
+    freed = _lib.release_memory()
+    assert freed >= 0
+    b = feols_stream(fml, c1_panel, tol=1e-10)
+    ref = oracle_mod.fit(c1_panel, tol=1e-10)
+    assert rel(b.beta, ref["beta"]) < 1e-8
