@@ -795,12 +795,19 @@ __global__ __launch_bounds__(kThreads) void k_spmm_wide(
 // level) could not.  Each row's sum runs over its nonzeros in order and in
 // pairs, as in k_spmm_wide; the pAp partials are per (round, workgroup), zero
 // outside the workgroup's slab.
-// a slab's row is one 128-byte line of P: 16 columns in FP64, 32 in FP32
-// (at 16 FP32 columns a gather moved the same lines as in FP64: the FP32
-// product took 0.93 ms against FP64's 1.22 on the 256-column JLA block)
+// a slab's row is one 128-byte line of P: 16 columns in FP64, 32 in FP32,
+// always 16 lanes per row (a lane takes two FP32 columns): at 16 FP32 columns
+// a gather moved the same lines as in FP64 (0.93 ms against FP64's 1.22 on
+// the 256-column JLA block), and at 32 lanes per row a wave had half FP64's
+// gathers in flight (FP32 0.86 ms against FP64's 0.65)
+template <typename T>
+constexpr int slab_v() {
+    return sizeof(T) == 4 ? 2 : 1;
+}
+constexpr int kSlabLanes = 16;
 template <typename T>
 constexpr int slab_w() {
-    return sizeof(T) == 4 ? 32 : 16;
+    return kSlabLanes * slab_v<T>();
 }
 constexpr int kSlabGroups = 128;  // workgroups per XCD
 constexpr int kPartRows = 8192;   // rows of k partials in the work buffer
@@ -811,61 +818,89 @@ __global__ __launch_bounds__(kThreads) void k_spmm_slab(
     int64_t L, const T *__restrict__ P, T *__restrict__ AP, double *__restrict__ part,
     int with_dot, int slab0, int nslab, CgState st) {
     if (st.done && *st.done) return;
+    constexpr int V = slab_v<T>();
     constexpr int kSlabW = slab_w<T>();
-    __shared__ double lds[kThreads];
+    __shared__ double lds[kThreads * V];
     const int k = st.k;
     const int w = blockIdx.x, x = w & 7, lw = w >> 3, gx = (int)gridDim.x >> 3;
     const int si = x % nslab, q = x / nslab, nq = (8 - si + nslab - 1) / nslab;
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int j = (slab0 + si) * kSlabW + (lane & (kSlabW - 1));
-    const bool colok = j < k;
-    constexpr int kRowsPerIter = kThreads / kSlabW;
+    const int lane = threadIdx.x & 63;
+    const int jb = (slab0 + si) * kSlabW + (lane & (kSlabLanes - 1)) * V;
+    bool colok[V];
+    int jc[V];   // the column read (clamped into the row when past k)
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+        colok[v] = jb + v < k;
+        jc[v] = colok[v] ? jb + v : 0;
+    }
+    constexpr int kRowsPerIter = kThreads / kSlabLanes;
     const int64_t stride = (int64_t)gx * nq * kRowsPerIter;
-    double dot = 0.0;
+    double dot[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) dot[v] = 0.0;
     for (int64_t base = ((int64_t)lw * nq + q) * kRowsPerIter; base < L; base += stride) {
-        const int64_t row = base + threadIdx.x / kSlabW;
+        const int64_t row = base + threadIdx.x / kSlabLanes;
         if (row >= L) continue;
         const int s = indptr[row], e = indptr[row + 1];
-        T acc = T(0);
+        T acc[V];
+#pragma unroll
+        for (int v = 0; v < V; ++v) acc[v] = T(0);
         if (e - s > kLongThresh) {
             // a split row: its product is already in AP
-            if (colok) acc = AP[row * k + j];
+#pragma unroll
+            for (int v = 0; v < V; ++v)
+                if (colok[v]) acc[v] = AP[row * k + jc[v]];
         } else {
-            const T *Pj = P + (colok ? j : 0);
             // the nonzeros in pairs, as k_spmm_wide adds them; four in flight
             int p = s;
             for (; p + 3 < e; p += 4) {
                 const int c0 = indices[p], c1 = indices[p + 1], c2 = indices[p + 2],
                           c3 = indices[p + 3];
                 const T v0 = val[p], v1 = val[p + 1], v2 = val[p + 2], v3 = val[p + 3];
-                const T a0 = Pj[(int64_t)c0 * k], a1 = Pj[(int64_t)c1 * k];
-                const T a2 = Pj[(int64_t)c2 * k], a3 = Pj[(int64_t)c3 * k];
-                acc += v0 * a0 + v1 * a1;
-                acc += v2 * a2 + v3 * a3;
+#pragma unroll
+                for (int v = 0; v < V; ++v) {
+                    const T *Pj = P + jc[v];
+                    const T a0 = Pj[(int64_t)c0 * k], a1 = Pj[(int64_t)c1 * k];
+                    const T a2 = Pj[(int64_t)c2 * k], a3 = Pj[(int64_t)c3 * k];
+                    acc[v] += v0 * a0 + v1 * a1;
+                    acc[v] += v2 * a2 + v3 * a3;
+                }
             }
             if (p + 1 < e) {
                 const int c0 = indices[p], c1 = indices[p + 1];
                 const T v0 = val[p], v1 = val[p + 1];
-                acc += v0 * Pj[(int64_t)c0 * k] + v1 * Pj[(int64_t)c1 * k];
+#pragma unroll
+                for (int v = 0; v < V; ++v) {
+                    const T *Pj = P + jc[v];
+                    acc[v] += v0 * Pj[(int64_t)c0 * k] + v1 * Pj[(int64_t)c1 * k];
+                }
                 p += 2;
             }
-            if (p < e) acc += val[p] * Pj[(int64_t)indices[p] * k];
-            if (colok) AP[row * k + j] = acc;
+            if (p < e)
+#pragma unroll
+                for (int v = 0; v < V; ++v) acc[v] += val[p] * P[(int64_t)indices[p] * k + jc[v]];
+#pragma unroll
+            for (int v = 0; v < V; ++v)
+                if (colok[v]) AP[row * k + jc[v]] = acc[v];
         }
-        if (colok) dot += (double)P[row * k + j] * (double)acc;
+#pragma unroll
+        for (int v = 0; v < V; ++v)
+            if (colok[v]) dot[v] += (double)P[row * k + jc[v]] * (double)acc[v];
     }
-    (void)wv;
     if (!with_dot) return;
-    lds[threadIdx.x] = colok ? dot : 0.0;
+    // lds[r * kSlabW + cs]: row lane r's sum for slab column cs
+    const int r = threadIdx.x / kSlabLanes, cl = (threadIdx.x & (kSlabLanes - 1)) * V;
+#pragma unroll
+    for (int v = 0; v < V; ++v) lds[r * kSlabW + cl + v] = colok[v] ? dot[v] : 0.0;
     __syncthreads();
     // the workgroup's partial: the kSlabW columns of its slab summed over its
     // row lanes in a fixed order, every other column zero
     for (int c = threadIdx.x; c < k; c += blockDim.x) {
         const int cs = c - (slab0 + si) * kSlabW;
-        double v = 0.0;
+        double vsum = 0.0;
         if (cs >= 0 && cs < kSlabW)
-            for (int r = 0; r < kRowsPerIter; ++r) v += lds[r * kSlabW + cs];
-        part[(int64_t)w * k + c] = v;
+            for (int rr = 0; rr < kRowsPerIter; ++rr) vsum += lds[rr * kSlabW + cs];
+        part[(int64_t)w * k + c] = vsum;
     }
 }
 
