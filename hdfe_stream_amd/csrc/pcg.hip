@@ -1795,6 +1795,9 @@ struct Matrix {
     // batch); a key's first use runs eagerly, the second captures
     std::map<std::array<uintptr_t, 9>, hipGraphExec_t> cg_graphs;
     std::map<std::array<uintptr_t, 9>, int> cg_seen;
+    // CG solves run to the end on this matrix, per (k, precision): a large
+    // system's batches become graphs once it has been solved twice with them
+    std::map<std::pair<int, int>, int> cg_solves;
     void drop_graphs() {
         for (auto &kv : cg_graphs) (void)hipGraphExecDestroy(kv.second);
         cg_graphs.clear();
@@ -2306,9 +2309,12 @@ static int run_cg(Matrix &M, int k, const T *val, const T *dinv, T *X, T *R, T *
     // time than they run on the device: small systems.  A large one keeps
     // the device busy from eager launches, and capturing and instantiating a
     // graph there idles it (0.3-0.4 ms per capture measured on 400,000
-    // levels); its batches run eagerly.
+    // levels): its batches run eagerly until the matrix has been solved twice
+    // with this k and precision (the leave-out paths solve one matrix
+    // hundreds of times; a fit builds a new one each time), then as graphs.
     const int64_t iter_bytes = M.nnz * (int64_t)(4 + sizeof(T)) + M.L * (int64_t)k * sizeof(T) * 6;
-    const bool use_graphs = !M.timing && iter_bytes < kGraphMaxBytes;
+    const std::pair<int, int> sk = {k, (int)sizeof(T)};
+    const bool use_graphs = !M.timing && (iter_bytes < kGraphMaxBytes || M.cg_solves[sk] >= 2);
     int host[2] = {0, 0};
     int batch = 8;
     int64_t done_before = 0;  // iterations launched before this batch
@@ -2391,6 +2397,7 @@ static int run_cg(Matrix &M, int k, const T *val, const T *dinv, T *X, T *R, T *
             batch = next;
         }
     }
+    ++M.cg_solves[sk];
     if (!host[0]) {  // maxiter reached without the flag (maxiter == 0)
         *it_out = 0;
         *conv_out = 0;
