@@ -607,8 +607,11 @@ __global__ __launch_bounds__(kThreads) void k_spmm_tiles(
     const int *__restrict__ bptr, const int *__restrict__ bidx, const T *__restrict__ bval,
     int64_t L, const T *__restrict__ P, T *__restrict__ AP, double *__restrict__ part,
     int with_dot, int nld, const T *__restrict__ dt, const int *__restrict__ long_rows,
-    double *__restrict__ dlpart, CgState st) {
+    double *__restrict__ dlpart, CgState st, const int *__restrict__ tbp,
+    const uint16_t *__restrict__ off16, const uint16_t *__restrict__ bidx16,
+    const int *__restrict__ col_lo) {
     if (st.done && *st.done) return;
+    const bool C16 = off16 != nullptr;   // the compact indices (tbp, off16, bidx16)
     constexpr int CH = K <= 4 ? 1024 : 512;
     __shared__ T prod[CH * K];
     __shared__ double lds[kThreads / kWave];
@@ -622,15 +625,39 @@ __global__ __launch_bounds__(kThreads) void k_spmm_tiles(
     for (int q = 0; q < R; ++q)
 #pragma unroll
         for (int j = 0; j < K; ++j) acc[q][j] = T(0);
+    const int64_t nt = (L + kThreads - 1) / kThreads;   // 256-row stretches (tbp's rows)
     for (int b = 0; b < kNB; ++b) {
         const int64_t boff = (int64_t)b * L;
-        const int s0 = bptr[boff + r0], s1 = bptr[boff + r1];
+        int s0, s1, cb = 0;
         int rs[R], re[R];
+        if (C16) {
+            // the stretch starts in tbp, each row's offset from its stretch's
+            // start in 16 bits; column indices relative to the block's first
+            const int *tb = tbp + b * (nt + 1);
+            s0 = tb[r0 / kThreads];
+            s1 = tb[(r1 + kThreads - 1) / kThreads];
+            cb = col_lo[b];
 #pragma unroll
-        for (int q = 0; q < R; ++q) {
-            const int64_t row = r0 + q * kThreads + tid;
-            rs[q] = row < r1 ? bptr[boff + row] : 0;
-            re[q] = row < r1 ? bptr[boff + row + 1] : 0;
+            for (int q = 0; q < R; ++q) {
+                const int64_t row = r0 + q * kThreads + tid;
+                rs[q] = re[q] = 0;
+                if (row < r1) {
+                    const int64_t si = r0 / kThreads + q;   // row's stretch, < nt
+                    const int st0 = tb[si];
+                    rs[q] = st0 + off16[boff + row];
+                    re[q] = (tid + 1 == kThreads || row + 1 == L) ? tb[si + 1]
+                                                                   : st0 + off16[boff + row + 1];
+                }
+            }
+        } else {
+            s0 = bptr[boff + r0];
+            s1 = bptr[boff + r1];
+#pragma unroll
+            for (int q = 0; q < R; ++q) {
+                const int64_t row = r0 + q * kThreads + tid;
+                rs[q] = row < r1 ? bptr[boff + row] : 0;
+                re[q] = row < r1 ? bptr[boff + row + 1] : 0;
+            }
         }
         for (int c0 = s0; c0 < s1; c0 += CH) {
             const int cn = min(CH, s1 - c0);
@@ -641,7 +668,7 @@ __global__ __launch_bounds__(kThreads) void k_spmm_tiles(
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
                     const int q = t + u * kThreads;
-                    cc[u] = q < cn ? bidx[c0 + q] : 0;
+                    cc[u] = q < cn ? (C16 ? cb + (int)bidx16[c0 + q] : bidx[c0 + q]) : 0;
                     vv[u] = q < cn ? bval[c0 + q] : T(0);
                 }
 #pragma unroll
@@ -1241,6 +1268,37 @@ __global__ __launch_bounds__(kThreads) void k_blk_fill(
     }
 }
 
+// The tiled form's compact indices from the blocked arrays: each block's
+// 256-row stretch starts (tbp, kNB x (nt + 1), the last the block's end),
+// each row's start less its stretch's (off16), each column less its block's
+// first (bidx16).  The caller checks that both fit 16 bits.
+__global__ void k_tbp_fill(int64_t L, int64_t nt, const int *__restrict__ bptr,
+                           int *__restrict__ tbp) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < kNB * (nt + 1);
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t b = i / (nt + 1), t = i - b * (nt + 1);
+        tbp[i] = bptr[b * L + min(t * kThreads, L)];
+    }
+}
+
+__global__ void k_off16_fill(int64_t L, int64_t nt, const int *__restrict__ bptr,
+                             const int *__restrict__ tbp, uint16_t *__restrict__ off16) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < kNB * L;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t b = i / L, r = i - b * L;
+        off16[i] = (uint16_t)(bptr[i] - tbp[b * (nt + 1) + r / kThreads]);
+    }
+}
+
+__global__ void k_bidx16_fill(int64_t n, const int *__restrict__ bidx,
+                              const int *__restrict__ col_lo, uint16_t *__restrict__ bidx16) {
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n;
+         q += (int64_t)gridDim.x * blockDim.x) {
+        const int col = bidx[q];
+        bidx16[q] = (uint16_t)(col - col_lo[blk_of(col, col_lo)]);
+    }
+}
+
 // The split rows' block among themselves: ll[t, slot(col)] = S[long t, col]
 // for the split rows' entries at split rows' columns (kLlParts workgroups per
 // row, each a stride of its entries).
@@ -1769,6 +1827,13 @@ struct Matrix {
     std::vector<int> ip_host;  // host row pointers (the blocked layout's balance)
     int col_lo_h[kNB + 1] = {0};
     int *col_lo = nullptr, *bptr = nullptr, *bidx = nullptr;
+    // the tiled form's compact indices (when every block spans at most 65,536
+    // columns and every 256-row stretch of a block holds under 65,536
+    // nonzeros): tbp (kNB, stretches + 1) the stretches' starts, off16 (kNB L)
+    // each row's start less its stretch's, bidx16 the column less the block's
+    // first
+    int *tbp = nullptr;
+    uint16_t *off16 = nullptr, *bidx16 = nullptr;
     double *bval64 = nullptr;
     float *bval32 = nullptr;
     int64_t bnnz = 0;
@@ -2057,6 +2122,40 @@ static int build_blocked(Matrix &M, const std::vector<int> &ip32) {
         hipLaunchKernelGGL(k_ll_fill, dim3(M.nlong * kLlParts), dim3(kThreads), 0, ms, M.nlong,
                            M.long_rows, M.indptr, M.indices, M.val64, M.val32, M.dl_slot, M.ll64, M.ll32);
     HIP_TRY(hipGetLastError());
+    // the tiled form's 16-bit indices, when every block spans at most 65,536
+    // columns and every stretch of 256 rows holds under 65,536 of a block's
+    // nonzeros (38 of the 64 MB of 32-bit indices and row pointers at 5M
+    // workers)
+    bool narrow = M.bnnz > 0;
+    for (int b = 0; b < kNB; ++b)
+        if (M.col_lo_h[b + 1] - M.col_lo_h[b] > 65536) narrow = false;
+    if (narrow) {
+        const int64_t nt = (L + kThreads - 1) / kThreads;
+        HIP_TRY(palloc(&M.tbp, kNB * (nt + 1) * 4, ms));
+        hipLaunchKernelGGL(k_tbp_fill, dim3(grid_for(kNB * (nt + 1), 256, 1, 4096)), dim3(256), 0, ms,
+                           L, nt, M.bptr, M.tbp);
+        std::vector<int> tb(kNB * (nt + 1));
+        HIP_TRY(hipMemcpyAsync(tb.data(), M.tbp, tb.size() * 4, hipMemcpyDeviceToHost, ms));
+        HIP_TRY(hipStreamSynchronize(ms));
+        for (int b = 0; b < kNB && narrow; ++b)
+            for (int64_t t = 0; t < nt; ++t)
+                if (tb[b * (nt + 1) + t + 1] - tb[b * (nt + 1) + t] > 65535) {
+                    narrow = false;
+                    break;
+                }
+        if (narrow) {
+            HIP_TRY(palloc(&M.off16, kNB * L * 2, ms));
+            HIP_TRY(palloc(&M.bidx16, M.bnnz * 2, ms));
+            hipLaunchKernelGGL(k_off16_fill, dim3(grid_for(kNB * L, 256, 1, 16384)), dim3(256), 0,
+                               ms, L, nt, M.bptr, M.tbp, M.off16);
+            hipLaunchKernelGGL(k_bidx16_fill, dim3(grid_for(M.bnnz, 256, 1, 16384)), dim3(256), 0,
+                               ms, M.bnnz, M.bidx, M.col_lo, M.bidx16);
+            HIP_TRY(hipGetLastError());
+        } else {
+            pfree(M.tbp, ms);
+            M.tbp = nullptr;
+        }
+    }
     HIP_TRY(hipStreamSynchronize(ms));
     M.blk = true;
     return HDFE_STATUS_OK;
@@ -2155,10 +2254,13 @@ static void launch_spmm(Matrix &M, const T *val, const T *P, T *AP, double *part
         const int nwt = M.small_form[prec][k] != 1 && (M.dl || M.nlong == 0)
                             ? tiles_shape(M.L, k, sizeof(T) == 8, &rt)
                             : 0;
+        // form 3: the tiled form with the 32-bit indices (for comparison)
+        const bool c16 = M.small_form[prec][k] != 3;
         if (nwt > 0) {
 #define HDFE_TL(KK, RR)                                                                          \
     hipLaunchKernelGGL((k_spmm_tiles<T, KK, RR>), dim3(nwt), dim3(kThreads), 0, s, M.bptr,       \
-                       M.bidx, bval, M.L, P, AP, part, with_dot, nld, dt, M.long_rows, lpart, st)
+                       M.bidx, bval, M.L, P, AP, part, with_dot, nld, dt, M.long_rows, lpart, st,  \
+                       M.tbp, c16 ? M.off16 : nullptr, M.bidx16, M.col_lo)
 #define HDFE_TLG(KK)                                                                               \
     case KK:                                                                                       \
         if (rt == 1) { HDFE_TL(KK, 1); }                                                           \
@@ -3162,7 +3264,7 @@ int hdfe_csr_blk_parts(void *h, int k, int precision, int set, int *parts) {
 
 int hdfe_csr_small_form(void *h, int k, int precision, int set, int *form) {
     ARG_CHECK(h && form && k >= 1 && k <= kSmallK && (precision == 0 || precision == 1) &&
-                  set >= -1 && set <= 2,
+                  set >= -1 && set <= 3,
               "hdfe_csr_small_form: bad argument");
     Matrix &M = *(Matrix *)h;
     std::lock_guard<std::mutex> lock(M.mu);
@@ -3307,6 +3409,7 @@ void hdfe_destroy(void *h) {
     M->drop_graphs();
     M->w.release(M->stream);
     void *ps[] = {M->indptr, M->indices, M->val64, M->val32, M->dinv64, M->dinv32,
+                  M->tbp, M->off16, M->bidx16,
                   M->long_rows, M->long_cptr, M->chunk_row, M->chunk_beg, M->col_lo, M->bptr,
                   M->bidx, M->bval64, M->bval32, M->dl_slot, M->dl_cptr, M->dt64, M->dt32,
                   M->ll64, M->ll32, M->dw64, M->dw32};

@@ -184,7 +184,8 @@ class DeviceCSR:
     def small_form(self, k: int, precision: str = "fp64", set: int = -1) -> int:
         """The column-blocked SpMM's form for k <= 8 columns (0 the default:
         tiled where it applies; 1 groups of column blocks with partial
-        products and a combine; 2 tiled); `set` overrides it."""
+        products and a combine; 2 tiled; 3 tiled with 32-bit indices); `set`
+        overrides it."""
         out = C.c_int()
         _lib.check(_lib.load().hdfe_csr_small_form(self._h, int(k), PRECISIONS[precision],
                                                    int(set), C.byref(out)), "hdfe_csr_small_form")

@@ -114,8 +114,10 @@ int hdfe_csr_wide_mode(void *h, int k, int precision, int set, int *mode);
  * counterpart in the reference, whose cuSPARSE SpMM has one form): 0 the
  * default (tiled where it applies), 1 groups of column blocks with their
  * partial products and a combine, 2 tiled (a workgroup's rows streamed block
- * by block through LDS, no partial products); `set` >= 0 overrides it.  Each
- * form is deterministic; they differ in rounding. */
+ * by block through LDS, no partial products; 16-bit block-local column
+ * indices and row offsets where they fit), 3 tiled with 32-bit indices (the
+ * same sums as 2); `set` >= 0 overrides it.  Each form is deterministic;
+ * 1 and 2 differ in rounding. */
 int hdfe_csr_small_form(void *h, int k, int precision, int set, int *form);
 
 /* The number of CG batches held as HIP graphs on this matrix.  Batches are

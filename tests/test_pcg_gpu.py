@@ -270,14 +270,16 @@ def test_many_split_rows(k):
 def test_tiled_and_grouped_forms(big_system, k):
     """The column-blocked product's two forms (groups of blocks with partial
     products and a combine; tiles of rows streamed block by block) are both
-    S X, each the same bits on every call; the default is the tiled form."""
+    S X, each the same bits on every call; the default is the tiled form,
+    whose 16-bit block-local indices give the same bits as its 32-bit ones
+    (form 3)."""
     from hdfe_stream_amd.rocm import DeviceCSR
     S, dinv = big_system
     A = DeviceCSR.from_scipy(S, dinv)
     X = np.random.default_rng(20 + k).standard_normal((S.shape[0], k))
     ref = S @ X
     out = {}
-    for form in (1, 2):
+    for form in (1, 2, 3):
         A.small_form(k, "fp64", set=form)
         A.small_form(k, "mixed", set=form)
         out[form] = (A.matmat(X), A.matmat_f32(X.astype(np.float32)))
@@ -286,6 +288,8 @@ def test_tiled_and_grouped_forms(big_system, k):
         np.testing.assert_array_equal(A.matmat(X), out[form][0])
         np.testing.assert_array_equal(A.matmat_f32(X.astype(np.float32)), out[form][1])
     np.testing.assert_allclose(out[1][0], out[2][0], rtol=1e-13, atol=1e-13 * np.abs(ref).max())
+    np.testing.assert_array_equal(out[3][0], out[2][0])
+    np.testing.assert_array_equal(out[3][1], out[2][1])
     A.small_form(k, "fp64", set=0)
     A.small_form(k, "mixed", set=0)
     np.testing.assert_array_equal(A.matmat(X), out[2][0])
