@@ -35,6 +35,7 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 constexpr int kGlmRowChunk = 1024;   // level chunks of the row CSR (rows.hip)
 constexpr int kSyrkThreads = 512;    // 8 waves
 constexpr int kSyrkMaxBlk = 8;       // 32x32 output blocks per wave
+constexpr int kGlmRowsPerWave = 4;   // the IRLS row pass's rows in flight per wave
 
 // ------------------------------------------------------------ row pass
 
@@ -44,8 +45,9 @@ constexpr int kSyrkMaxBlk = 8;       // 32x32 output blocks per wave
 // W = mu f, u = y / mu - 1, z = eta + u, and the deviance and
 // log-likelihood less its constant; Gaussian (family 1, identity link, the
 // OLS batch): W = f, z = y, deviance the weighted sum of squared residuals;
-// family 2: the linear predictor alone, written to w.  A wave per row (lanes over
-// the covariates); a workgroup per chunk of one model's rows; part[c] =
+// family 2: the linear predictor alone, written to w.  A wave per four rows
+// (lanes over the covariates for the dots, then a lane per row); a workgroup
+// per chunk of one model's rows; part[c] =
 // (dev, sum f (y eta - mu), f y, f) of chunk c (the fifth slot is
 // k_glm_lgamma's).
 __global__ __launch_bounds__(256) void k_glm_rows(
@@ -61,24 +63,61 @@ __global__ __launch_bounds__(256) void k_glm_rows(
     const int c = blockIdx.x;
     const int m = ch_model[c];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    double acc[5] = {0, 0, 0, 0, 0};
-    for (int64_t i = ch_begin[c] + wv; i < ch_end[c]; i += 4) {
+    const int64_t b0 = ch_begin[c], e0 = ch_end[c];
+    const double *bm = init ? nullptr : beta + (int64_t)m * kx;
+    double acc[4] = {0, 0, 0, 0};
+    // a wave takes kGlmRowsPerWave rows at a time: their X rows' loads in
+    // flight together, one dot per row across the lanes, then lane u does
+    // row u's link, weight and deviance
+    for (int64_t i0 = b0 + (int64_t)wv * kGlmRowsPerWave; i0 < e0; i0 += 4 * kGlmRowsPerWave) {
+        double s[kGlmRowsPerWave];
+#pragma unroll
+        for (int u = 0; u < kGlmRowsPerWave; ++u) s[u] = 0.0;
+        if (!init) {
+            if ((kx & 3) == 0) {
+                // 16-byte loads: four covariates per lane (rows start 16-byte aligned)
+                const int kq = kx >> 2;
+                for (int jq = lane; jq < kq; jq += 64) {
+                    const double b0 = bm[4 * jq], b1 = bm[4 * jq + 1], b2 = bm[4 * jq + 2],
+                                 b3 = bm[4 * jq + 3];
+#pragma unroll
+                    for (int u = 0; u < kGlmRowsPerWave; ++u)
+                        if (i0 + u < e0) {
+                            const float4 xv = *(const float4 *)(X + (i0 + u) * kx + 4 * jq);
+                            s[u] += (double)xv.x * b0 + (double)xv.y * b1 + (double)xv.z * b2 +
+                                    (double)xv.w * b3;
+                        }
+                }
+            } else {
+                for (int j = lane; j < kx; j += 64) {
+                    const double bj = bm[j];
+#pragma unroll
+                    for (int u = 0; u < kGlmRowsPerWave; ++u)
+                        if (i0 + u < e0) s[u] += (double)X[(i0 + u) * kx + j] * bj;
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < kGlmRowsPerWave; ++u) s[u] = wave_sum(s[u]);
+        }
+        double su = s[0];
+#pragma unroll
+        for (int u = 1; u < kGlmRowsPerWave; ++u)
+            if (lane == u) su = s[u];
+        const int64_t i = i0 + lane;
+        if (lane >= kGlmRowsPerWave || i >= e0) continue;
         const double yi = y[i];
         double eta;
         if (init) {
             eta = family == 1 ? ybar[m] : log(0.5 * (yi + ybar[m]));
         } else {
-            double s = 0.0;
-            for (int j = lane; j < kx; j += 64) s += (double)X[i * kx + j] * beta[(int64_t)m * kx + j];
-            s = wave_sum(s);
-            eta = s + a[g[i]];
+            eta = su + a[g[i]];
             for (int d = 0; d < D; ++d) eta += fe[offs[d] + codes[d * ld + i]];
         }
-        if (lane == 0 && family == 2) {
+        if (family == 2) {
             // the linear predictor alone (w = eta), for the fitted values
             w[i] = eta;
             z[i] = yi;
-        } else if (lane == 0 && family == 1) {
+        } else if (family == 1) {
             // Gaussian, identity link: W = f, z = y, deviance = f (y - eta)^2
             const double fi = f ? f[i] : 1.0;
             const double r = yi - eta;
@@ -87,7 +126,7 @@ __global__ __launch_bounds__(256) void k_glm_rows(
             acc[3] += fi;
             w[i] = fi;
             z[i] = yi;
-        } else if (lane == 0) {
+        } else {
             const double mu = exp(eta);
             const double fi = f ? f[i] : 1.0;
             const double u = mu > 0.0 ? yi / mu - 1.0 : 0.0;
@@ -100,8 +139,12 @@ __global__ __launch_bounds__(256) void k_glm_rows(
             z[i] = eta + u;
         }
     }
-    if (lane == 0)
-        for (int q = 0; q < 4; ++q) lds[q][wv] = acc[q];
+    // the chunk's sums: the wave's lanes (a fixed tree), then the waves in order
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const double v = wave_sum(acc[q]);
+        if (lane == 0) lds[q][wv] = v;
+    }
     __syncthreads();
     if (threadIdx.x < 4) {
         const int q = threadIdx.x;
