@@ -1684,10 +1684,21 @@ __global__ __launch_bounds__(kMfwLanes) void k_mfw_groups(const int *__restrict_
     }
 }
 
+// Four consecutive columns of a row of T (the row's start need only be
+// aligned to T: the vector type's alignment is T's, so the compiler emits
+// the wide loads the target allows at that alignment).
+template <typename T>
+struct Vec4 {
+    typedef T type __attribute__((ext_vector_type(4), aligned(sizeof(T))));
+};
+
 // The sparse dimensions' rows: (S P)_l = sum over l's cells of
 // n_c (t_c - m_g(c)) = N_l P[l] + sum_c n_c (sum over the cell's other
 // dimensions of P[a_cd] - m_g(c)), the cells in level order (the
-// level-to-cells CSR), staged as in k_mfw_groups.
+// level-to-cells CSR), staged as in k_mfw_groups.  A lane takes four
+// consecutive columns (one vector load per row it gathers; the last lane
+// of a row whose width is not a multiple of four loads its columns one by
+// one).
 template <typename T>
 __global__ void k_mfw_levels(int64_t nsl, const int *__restrict__ slev,
                              const int *__restrict__ lptr, const int *__restrict__ lcell,
@@ -1696,6 +1707,7 @@ __global__ void k_mfw_levels(int64_t nsl, const int *__restrict__ slev,
                              const T *__restrict__ P, const T *__restrict__ mg, int k,
                              T *__restrict__ AP, CgState st) {
     if (st.done && *st.done) return;
+    typedef typename Vec4<T>::type V4;
     __shared__ int s_lv[kMfMaxD][kMfwStage];
     __shared__ int s_g[kMfwStage];
     __shared__ double s_w[kMfwStage];
@@ -1706,10 +1718,11 @@ __global__ void k_mfw_levels(int64_t nsl, const int *__restrict__ slev,
         for (int d = 1; d < D; ++d)
             if (dm.off[d] <= l && dm.off[d] > dm.off[own]) own = d;
         const int q0 = lptr[l], q1 = lptr[l + 1];
-        for (int j0 = 0; j0 < k; j0 += bd) {
-            const int j = j0 + tid;
-            const bool on = j < k;
-            double out = 0.0, N = 0.0;
+        for (int j0 = 0; j0 < k; j0 += 4 * bd) {
+            const int j = j0 + 4 * tid;
+            const int nj = j < k ? min(4, k - j) : 0;   // this lane's columns
+            double out[4] = {0.0, 0.0, 0.0, 0.0};
+            double N = 0.0;
             for (int p0 = q0; p0 < q1; p0 += kMfwStage) {
                 const int np = min(kMfwStage, q1 - p0);
                 __syncthreads();
@@ -1721,28 +1734,56 @@ __global__ void k_mfw_levels(int64_t nsl, const int *__restrict__ slev,
                     s_w[i] = cn[c];
                 }
                 __syncthreads();
-                if (!on) continue;
+                for (int i = 0; i < np; ++i) N += s_w[i];
+                if (!nj) continue;
                 for (int i0 = 0; i0 < np; i0 += kMfwIlp) {
                     const int nu = min(kMfwIlp, np - i0);
-                    double t[kMfwIlp];
+                    double t[kMfwIlp][4];
+                    if (nj == 4) {
 #pragma unroll
-                    for (int u = 0; u < kMfwIlp; ++u)
-                        t[u] = u < nu ? -(double)mg[(int64_t)s_g[i0 + u] * k + j] : 0.0;
-                    for (int d = 0; d < D; ++d) {
-                        if (d == own) continue;
+                        for (int u = 0; u < kMfwIlp; ++u) {
+                            const V4 m4 = u < nu ? *(const V4 *)(mg + (int64_t)s_g[i0 + u] * k + j)
+                                                 : V4{0, 0, 0, 0};
+#pragma unroll
+                            for (int c = 0; c < 4; ++c) t[u][c] = -(double)m4[c];
+                        }
+                        for (int d = 0; d < D; ++d) {
+                            if (d == own) continue;
+#pragma unroll
+                            for (int u = 0; u < kMfwIlp; ++u)
+                                if (u < nu) {
+                                    const V4 p4 = *(const V4 *)(P + (int64_t)s_lv[d][i0 + u] * k + j);
+#pragma unroll
+                                    for (int c = 0; c < 4; ++c) t[u][c] += (double)p4[c];
+                                }
+                        }
+                    } else {
 #pragma unroll
                         for (int u = 0; u < kMfwIlp; ++u)
-                            if (u < nu) t[u] += (double)P[(int64_t)s_lv[d][i0 + u] * k + j];
+#pragma unroll
+                            for (int c = 0; c < 4; ++c)
+                                t[u][c] = u < nu && c < nj
+                                              ? -(double)mg[(int64_t)s_g[i0 + u] * k + j + c]
+                                              : 0.0;
+                        for (int d = 0; d < D; ++d) {
+                            if (d == own) continue;
+#pragma unroll
+                            for (int u = 0; u < kMfwIlp; ++u)
+#pragma unroll
+                                for (int c = 0; c < 4; ++c)
+                                    if (u < nu && c < nj)
+                                        t[u][c] += (double)P[(int64_t)s_lv[d][i0 + u] * k + j + c];
+                        }
                     }
 #pragma unroll
                     for (int u = 0; u < kMfwIlp; ++u)
-                        if (u < nu) {
-                            out += s_w[i0 + u] * t[u];
-                            N += s_w[i0 + u];
-                        }
+                        if (u < nu)
+#pragma unroll
+                            for (int c = 0; c < 4; ++c) out[c] += s_w[i0 + u] * t[u][c];
                 }
             }
-            if (on) AP[l * k + j] = (T)(out + N * (double)P[l * k + j]);
+            for (int c = 0; c < nj; ++c)
+                AP[l * k + j + c] = (T)(out[c] + N * (double)P[l * k + j + c]);
         }
     }
 }
@@ -2179,10 +2220,13 @@ static void launch_spmm(Matrix &M, const T *val, const T *P, T *AP, double *part
                                (size_t)F.maxslot * bg * sizeof(double), s, F.rng, F.rbase,
                                F.rsoff, F.dpoff, F.igstart, F.icodes, F.ld, F.dm.D, F.offs_d,
                                F.dd.n, F.dd.d[0], F.dd.d[1], F.cn, P, k, mg, F.dpart, st);
-        if (F.nsl)
+        if (F.nsl) {
+            // four columns a lane
+            const int bl = (int)std::min<int64_t>(1024, ((k + 3) / 4 + kWave - 1) / kWave * kWave);
             hipLaunchKernelGGL((k_mfw_levels<T>), dim3((int)std::min<int64_t>(F.nsl, 16384)),
-                               dim3(bd), 0, s, F.nsl, F.slev, F.lptr, F.lcell, F.cgroup, F.icodes,
+                               dim3(bl), 0, s, F.nsl, F.slev, F.lptr, F.lcell, F.cgroup, F.icodes,
                                F.ld, F.dm, F.cn, P, (const T *)mg, k, AP, st);
+        }
         if (F.ndl)
             hipLaunchKernelGGL((k_mfw_dense<T>), dim3((int)std::min<int64_t>(F.ndl, 4096)), dim3(bd),
                                0, s, F.ndl, F.dlev, F.dcptr, F.dcrow, F.dpart, k, AP, st);
