@@ -72,6 +72,7 @@ SIGNATURES = [
     ("hdfe_csr_graphs", I32, [P, P]),
     ("hdfe_csr_wide_mode", I32, [P, I32, I32, I32, P]),
     ("hdfe_csr_small_form", I32, [P, I32, I32, I32, P]),
+    ("hdfe_csr_row_pointer_bits", I32, [P, I32, P]),
     ("hdfe_tall_dot", I32, [I64, I32, I32, P, I64, P, I64, P, P]),
     ("hdfe_tall_dot_noalloc", I32, [I64, I32, I32, P, I64, P, I64, P, P, P]),
     ("hdfe_se_stack", I32, [I64, I32, P, P, P, P, I32, P]),

@@ -3327,10 +3327,12 @@ int hdfe_panel_system(void *h, int with_rhs, void **matrix, double *seconds) {
     // the explicit build's size from here: the emitted pairs sorted with their
     // values (two key and two value arrays) and S itself (at most one entry
     // per pair, plus the dense levels' rows); over the budget, or past what
-    // one sort can take, the panel keeps diag and rhs and no S (the caller's
-    // stream_cg fallback: hdfe_stream_create)
+    // this build's 32-bit counts take (2^31 emitted pairs or entries of S),
+    // the panel keeps diag and rhs and no S (the caller's stream_cg
+    // fallback: hdfe_stream_create)
     const int64_t est = nent * 32 + (nent + L * Ld) * 12 + (L + 1) * 4;
-    if ((P.s_budget >= 0 && est > P.s_budget) || nent >= INT32_MAX) {
+    const int64_t nnz_max = nent + 2 * (int64_t)Ls * Ld + (int64_t)Ld * Ld;
+    if ((P.s_budget >= 0 && est > P.s_budget) || nnz_max >= INT32_MAX) {
         if (P.matrix) {
             hdfe_destroy(P.matrix);
             P.matrix = nullptr;

@@ -108,11 +108,13 @@ __device__ void reduce_parts(const double *part, int nw, int k, int stride, doub
     }
 }
 
-// Split rows, wide k: waves over nonzeros, lanes over columns.
-template <typename T>
+// Split rows, wide k: waves over nonzeros, lanes over columns.  I is the
+// row pointers' type (int, or int64_t for 2^31 nonzeros and more), here and
+// in the other plain CSR kernels.
+template <typename T, typename I>
 __global__ __launch_bounds__(kThreads) void k_long_wide(
-    const int *__restrict__ indptr, const int *__restrict__ indices, const T *__restrict__ val,
-    const int *__restrict__ chunk_row, const int *__restrict__ chunk_beg, int nchunk,
+    const I *__restrict__ indptr, const int *__restrict__ indices, const T *__restrict__ val,
+    const int *__restrict__ chunk_row, const I *__restrict__ chunk_beg, int nchunk,
     const int *__restrict__ long_rows, const int *__restrict__ long_cptr, int nlong,
     const T *__restrict__ P, T *__restrict__ AP, double *__restrict__ lpart, CgState st) {
     if (st.done && *st.done) return;
@@ -120,13 +122,13 @@ __global__ __launch_bounds__(kThreads) void k_long_wide(
     const int k = st.k;
     const int c = blockIdx.x;
     const int row = chunk_row[c];
-    const int s = chunk_beg[c], e = min(s + kLongChunk, indptr[row + 1]);
+    const I s = chunk_beg[c], e = min(s + (I)kLongChunk, indptr[row + 1]);
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     for (int j0 = 0; j0 < k; j0 += 64) {
         const int j = j0 + lane;
         T acc = T(0);
         if (j < k)
-            for (int p = s + wv; p < e; p += 4) acc += val[p] * P[(int64_t)indices[p] * k + j];
+            for (I p = s + wv; p < e; p += 4) acc += val[p] * P[(int64_t)indices[p] * k + j];
         if (j < k) wlds[wv * k + j] = (double)acc;
     }
     __syncthreads();
@@ -345,11 +347,11 @@ __global__ __launch_bounds__(kFinThreads) void k_fin_small(
 // nonzeros: the levels of small dimensions), the others ranges of ordinary
 // rows with G lanes per row.  The last workgroup to finish adds each split
 // row's chunk partials in a fixed order, adds their P AP, and forms alpha.
-template <typename T, int K, int G>
+template <typename T, int K, int G, typename I>
 __global__ __launch_bounds__(kThreads) void k_spmm_small(
-    const int *__restrict__ indptr, const int *__restrict__ indices, const T *__restrict__ val,
+    const I *__restrict__ indptr, const int *__restrict__ indices, const T *__restrict__ val,
     int64_t L, const T *__restrict__ P, T *__restrict__ AP, double *__restrict__ part,
-    int with_dot, const int *__restrict__ chunk_row, const int *__restrict__ chunk_beg,
+    int with_dot, const int *__restrict__ chunk_row, const I *__restrict__ chunk_beg,
     int nchunk, const int *__restrict__ long_rows, const int *__restrict__ long_cptr, int nlong,
     double *__restrict__ lpart, CgState st) {
     if (st.done && *st.done) return;
@@ -361,11 +363,11 @@ __global__ __launch_bounds__(kThreads) void k_spmm_small(
     if ((int)blockIdx.x < nchunk) {
         const int c = blockIdx.x;
         const int row = chunk_row[c];
-        const int s = chunk_beg[c], e = min(s + kLongChunk, indptr[row + 1]);
+        const I s = chunk_beg[c], e = min(s + (I)kLongChunk, indptr[row + 1]);
         T acc[K];
 #pragma unroll
         for (int j = 0; j < K; ++j) acc[j] = T(0);
-        for (int p = s + threadIdx.x; p < e; p += blockDim.x) {
+        for (I p = s + threadIdx.x; p < e; p += blockDim.x) {
             const int col = indices[p];
             const T v = val[p];
 #pragma unroll
@@ -383,7 +385,7 @@ __global__ __launch_bounds__(kThreads) void k_spmm_small(
         for (int64_t base = r0; base < r1; base += nslot) {
             const int64_t row = base + slot;
             const bool ok = row < r1;
-            int s = 0, e = 0;
+            I s = 0, e = 0;
             if (ok) {
                 s = indptr[row];
                 e = indptr[row + 1];
@@ -395,7 +397,7 @@ __global__ __launch_bounds__(kThreads) void k_spmm_small(
             if (ok && !is_long) {
                 // four nonzeros per lane in flight: their column indices
                 // first, then all their P gathers
-                int p = s + sub;
+                I p = s + sub;
                 for (; p + 3 * G < e; p += 4 * G) {
                     const int c0 = indices[p], c1 = indices[p + G];
                     const int c2 = indices[p + 2 * G], c3 = indices[p + 3 * G];
@@ -749,9 +751,17 @@ __global__ __launch_bounds__(kThreads) void k_spmm_tiles(
 // block -- stay resident in the memory-side cache instead of the whole of P
 // (a contiguous row range per workgroup keeps every part of the matrix in
 // flight at once).
-template <typename T, int CPL>
+// a wave-uniform value in a scalar register (64-bit: its halves in turn)
+__device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ int64_t uniform(int64_t v) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)((uint64_t)v >> 32));
+    return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
+template <typename T, int CPL, typename I>
 __global__ __launch_bounds__(kThreads) void k_spmm_wide(
-    const int *__restrict__ indptr, const int *__restrict__ indices, const T *__restrict__ val,
+    const I *__restrict__ indptr, const int *__restrict__ indices, const T *__restrict__ val,
     int64_t L, const T *__restrict__ P, T *__restrict__ AP, double *__restrict__ part,
     int with_dot, CgState st) {
     if (st.done && *st.done) return;
@@ -763,14 +773,14 @@ __global__ __launch_bounds__(kThreads) void k_spmm_wide(
 #pragma unroll
     for (int t = 0; t < CPL; ++t) dot[t] = 0.0;
     for (int64_t row = (int64_t)w * 4 + wv; row < L; row += (int64_t)nw * 4) {
-        const int s = __builtin_amdgcn_readfirstlane(indptr[row]);
-        const int e = __builtin_amdgcn_readfirstlane(indptr[row + 1]);
+        const I s = uniform(indptr[row]);
+        const I e = uniform(indptr[row + 1]);
         const bool is_long = (e - s) > kLongThresh;
         T acc[CPL];
 #pragma unroll
         for (int t = 0; t < CPL; ++t) acc[t] = T(0);
         if (!is_long) {
-            int p = s;
+            I p = s;
             for (; p + 1 < e; p += 2) {
                 const int c0 = indices[p], c1 = indices[p + 1];
                 const T v0 = val[p], v1 = val[p + 1];
@@ -839,9 +849,9 @@ constexpr int slab_w() {
 constexpr int kSlabGroups = 128;  // workgroups per XCD
 constexpr int kPartRows = 8192;   // rows of k partials in the work buffer
 
-template <typename T>
+template <typename T, typename I>
 __global__ __launch_bounds__(kThreads) void k_spmm_slab(
-    const int *__restrict__ indptr, const int *__restrict__ indices, const T *__restrict__ val,
+    const I *__restrict__ indptr, const int *__restrict__ indices, const T *__restrict__ val,
     int64_t L, const T *__restrict__ P, T *__restrict__ AP, double *__restrict__ part,
     int with_dot, int slab0, int nslab, CgState st) {
     if (st.done && *st.done) return;
@@ -868,7 +878,7 @@ __global__ __launch_bounds__(kThreads) void k_spmm_slab(
     for (int64_t base = ((int64_t)lw * nq + q) * kRowsPerIter; base < L; base += stride) {
         const int64_t row = base + threadIdx.x / kSlabLanes;
         if (row >= L) continue;
-        const int s = indptr[row], e = indptr[row + 1];
+        const I s = indptr[row], e = indptr[row + 1];
         T acc[V];
 #pragma unroll
         for (int v = 0; v < V; ++v) acc[v] = T(0);
@@ -879,7 +889,7 @@ __global__ __launch_bounds__(kThreads) void k_spmm_slab(
                 if (colok[v]) acc[v] = AP[row * k + jc[v]];
         } else {
             // the nonzeros in pairs, as k_spmm_wide adds them; four in flight
-            int p = s;
+            I p = s;
             for (; p + 3 < e; p += 4) {
                 const int c0 = indices[p], c1 = indices[p + 1], c2 = indices[p + 2],
                           c3 = indices[p + 3];
@@ -1134,6 +1144,12 @@ __global__ void k_cast_f64_to_f32(int64_t n, const double *__restrict__ a, float
     for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n;
          e += (int64_t)gridDim.x * blockDim.x)
         b[e] = (float)a[e];
+}
+
+__global__ void k_widen(int64_t n, const int *__restrict__ a, int64_t *__restrict__ b) {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n;
+         e += (int64_t)gridDim.x * blockDim.x)
+        b[e] = a[e];
 }
 
 // Mixed precision: R32 = r64 / s_j for active columns (0 otherwise), X32 = 0,
@@ -1859,6 +1875,11 @@ struct Matrix {
     int device = 0;
     int64_t L = 0, nnz = 0;
     int *indptr = nullptr, *indices = nullptr;
+    // 64-bit row pointers (2^31 nonzeros and more, or asked for): indptr64
+    // and chunk_beg64 replace indptr and chunk_beg, and only the plain CSR
+    // forms run (no column-blocked copy, no dense split rows)
+    bool p64 = false;
+    int64_t *indptr64 = nullptr, *chunk_beg64 = nullptr;
     double *val64 = nullptr, *dinv64 = nullptr;
     float *val32 = nullptr, *dinv32 = nullptr;
     int nlong = 0, nchunk = 0;
@@ -1934,6 +1955,7 @@ static int alloc_work(Matrix &M, int k) {
     // the blocked layout for small k, once, when P outgrows an XCD's L2
     if (k <= kSmallK && !M.blk_tried) {
         M.blk_tried = true;
+        if (M.p64) return alloc_work_buffers(M, k);
         if (M.L * (int64_t)kSmallK * 8 > kL2Budget && M.L < (INT32_MAX - 1) / kNB) {
             M.drop_graphs();
             M.w.release(M.stream);
@@ -2202,6 +2224,85 @@ static int build_blocked(Matrix &M, const std::vector<int> &ip32) {
     return HDFE_STATUS_OK;
 }
 
+// The plain CSR forms (no column-blocked copy): k <= 8 a few lanes per row,
+// wide k a wave per row or column slabs, split rows in chunks; I the row
+// pointers' type.
+template <typename T, typename I>
+static void launch_plain(Matrix &M, const I *ip, const I *cb, const T *val, const T *P, T *AP,
+                         double *part, double *lpart, int with_dot, CgState st, hipStream_t s,
+                         int force_wide, int g) {
+    const int k = st.k;
+    if (k <= kSmallK) {
+#define HDFE_SPMM(KK)                                                                          \
+    case KK:                                                                                   \
+        hipLaunchKernelGGL((k_spmm_small<T, KK, 8, I>), dim3(g + M.nchunk), dim3(kThreads), 0, s, \
+                           ip, M.indices, val, M.L, P, AP, part, with_dot, M.chunk_row,  \
+                           cb, M.nchunk, M.long_rows, M.long_cptr, M.nlong, lpart,    \
+                           st);                                                                \
+        if (with_dot || M.nlong)                                                               \
+            hipLaunchKernelGGL((k_fin_small<T, KK>), dim3(1), dim3(kFinThreads), 0, s, P, AP,  \
+                               part, g, with_dot, M.long_rows, M.long_cptr, M.nlong, lpart,    \
+                               st);                                                            \
+        break;
+        switch (k) {
+            HDFE_SPMM(1) HDFE_SPMM(2) HDFE_SPMM(3) HDFE_SPMM(4)
+            HDFE_SPMM(5) HDFE_SPMM(6) HDFE_SPMM(7) HDFE_SPMM(8)
+        }
+#undef HDFE_SPMM
+        return;
+    }
+    if (M.nlong && M.dw == 1 && M.w.dwpart) {
+        const T *dw = sizeof(T) == 4 ? (const T *)M.dw32 : (const T *)M.dw64;
+        hipLaunchKernelGGL((k_dense_long_wide<T>), dim3(kDwBlocks, (k + kWave - 1) / kWave),
+                           dim3(kThreads), 0, s, M.L, M.nlong, dw, P, M.w.dwpart, st);
+        hipLaunchKernelGGL((k_fin_dense_long<T>), dim3((M.nlong * k + kThreads - 1) / kThreads),
+                           dim3(kThreads), 0, s, M.nlong, M.long_rows, M.w.dwpart, AP, st);
+    } else if (M.nlong) {
+        hipLaunchKernelGGL((k_long_wide<T, I>), dim3(M.nchunk), dim3(kThreads), 4 * k * 8, s,
+                           ip, M.indices, val, M.chunk_row, cb, M.nchunk,
+                           M.long_rows, M.long_cptr, M.nlong, P, AP, lpart, st);
+        hipLaunchKernelGGL((k_fin_long_wide<T>), dim3(1), dim3(kFinThreads), 0, s, M.long_rows,
+                           M.long_cptr, M.nlong, lpart, AP, st);
+    }
+    int mode = force_wide;
+    if (mode < 0) {
+        auto it = M.wide_mode.find({sizeof(T) == 4 ? 1 : 0, k});
+        mode = it == M.wide_mode.end() ? 0 : it->second;
+    }
+    const int nslab = (k + slab_w<T>() - 1) / slab_w<T>(), rounds = (nslab + 7) / 8;
+    // the slab form's pAp partials, one row of k per (round, workgroup), must
+    // fit the work buffer's kPartRows rows: fewer workgroups per XCD for wide
+    // k, and the wave-per-row form when not even eight per XCD would fit
+    const int gs = std::min(8 * kSlabGroups, (kPartRows / rounds) / 8 * 8);
+    if (mode == 1 && gs >= 64) {
+        for (int r = 0; r < rounds; ++r) {
+            const int slab0 = 8 * r, ns = std::min(8, nslab - slab0);
+            hipLaunchKernelGGL((k_spmm_slab<T, I>), dim3(gs), dim3(kThreads), 0, s, ip,
+                               M.indices, val, M.L, P, AP, part + (int64_t)r * gs * k, with_dot,
+                               slab0, ns, st);
+        }
+        if (with_dot)
+            hipLaunchKernelGGL(k_fin_alpha, dim3(1), dim3(kFinThreads), 0, s, part, rounds * gs,
+                               st);
+        return;
+    }
+    const int cpl = (k + 63) / 64;
+#define HDFE_WIDE(C)                                                                        \
+    hipLaunchKernelGGL((k_spmm_wide<T, C, I>), dim3(g), dim3(kThreads), 4 * k * 8, s, ip, \
+                       M.indices, val, M.L, P, AP, part, with_dot, st)
+    if (cpl <= 1) HDFE_WIDE(1);
+    else if (cpl <= 2) HDFE_WIDE(2);
+    else if (cpl <= 3) HDFE_WIDE(3);
+    else if (cpl <= 4) HDFE_WIDE(4);
+    else if (cpl <= 5) HDFE_WIDE(5);
+    else if (cpl <= 6) HDFE_WIDE(6);
+    else if (cpl <= 8) HDFE_WIDE(8);
+    else HDFE_WIDE(16);
+#undef HDFE_WIDE
+    if (with_dot)
+        hipLaunchKernelGGL(k_fin_alpha, dim3(1), dim3(kFinThreads), 0, s, part, g, st);
+}
+
 // AP = S P (and, with with_dot, pAp sums and alpha).  `st.done` may be null.
 template <typename T>
 static void launch_spmm(Matrix &M, const T *val, const T *P, T *AP, double *part, double *lpart,
@@ -2349,75 +2450,12 @@ static void launch_spmm(Matrix &M, const T *val, const T *P, T *AP, double *part
 #undef HDFE_BLK
         return;
     }
-    if (k <= kSmallK) {
-#define HDFE_SPMM(KK)                                                                          \
-    case KK:                                                                                   \
-        hipLaunchKernelGGL((k_spmm_small<T, KK, 8>), dim3(g + M.nchunk), dim3(kThreads), 0, s, \
-                           M.indptr, M.indices, val, M.L, P, AP, part, with_dot, M.chunk_row,  \
-                           M.chunk_beg, M.nchunk, M.long_rows, M.long_cptr, M.nlong, lpart,    \
-                           st);                                                                \
-        if (with_dot || M.nlong)                                                               \
-            hipLaunchKernelGGL((k_fin_small<T, KK>), dim3(1), dim3(kFinThreads), 0, s, P, AP,  \
-                               part, g, with_dot, M.long_rows, M.long_cptr, M.nlong, lpart,    \
-                               st);                                                            \
-        break;
-        switch (k) {
-            HDFE_SPMM(1) HDFE_SPMM(2) HDFE_SPMM(3) HDFE_SPMM(4)
-            HDFE_SPMM(5) HDFE_SPMM(6) HDFE_SPMM(7) HDFE_SPMM(8)
-        }
-#undef HDFE_SPMM
-        return;
-    }
-    if (M.nlong && M.dw == 1 && M.w.dwpart) {
-        const T *dw = sizeof(T) == 4 ? (const T *)M.dw32 : (const T *)M.dw64;
-        hipLaunchKernelGGL((k_dense_long_wide<T>), dim3(kDwBlocks, (k + kWave - 1) / kWave),
-                           dim3(kThreads), 0, s, M.L, M.nlong, dw, P, M.w.dwpart, st);
-        hipLaunchKernelGGL((k_fin_dense_long<T>), dim3((M.nlong * k + kThreads - 1) / kThreads),
-                           dim3(kThreads), 0, s, M.nlong, M.long_rows, M.w.dwpart, AP, st);
-    } else if (M.nlong) {
-        hipLaunchKernelGGL((k_long_wide<T>), dim3(M.nchunk), dim3(kThreads), 4 * k * 8, s,
-                           M.indptr, M.indices, val, M.chunk_row, M.chunk_beg, M.nchunk,
-                           M.long_rows, M.long_cptr, M.nlong, P, AP, lpart, st);
-        hipLaunchKernelGGL((k_fin_long_wide<T>), dim3(1), dim3(kFinThreads), 0, s, M.long_rows,
-                           M.long_cptr, M.nlong, lpart, AP, st);
-    }
-    int mode = force_wide;
-    if (mode < 0) {
-        auto it = M.wide_mode.find({sizeof(T) == 4 ? 1 : 0, k});
-        mode = it == M.wide_mode.end() ? 0 : it->second;
-    }
-    const int nslab = (k + slab_w<T>() - 1) / slab_w<T>(), rounds = (nslab + 7) / 8;
-    // the slab form's pAp partials, one row of k per (round, workgroup), must
-    // fit the work buffer's kPartRows rows: fewer workgroups per XCD for wide
-    // k, and the wave-per-row form when not even eight per XCD would fit
-    const int gs = std::min(8 * kSlabGroups, (kPartRows / rounds) / 8 * 8);
-    if (mode == 1 && gs >= 64) {
-        for (int r = 0; r < rounds; ++r) {
-            const int slab0 = 8 * r, ns = std::min(8, nslab - slab0);
-            hipLaunchKernelGGL((k_spmm_slab<T>), dim3(gs), dim3(kThreads), 0, s, M.indptr,
-                               M.indices, val, M.L, P, AP, part + (int64_t)r * gs * k, with_dot,
-                               slab0, ns, st);
-        }
-        if (with_dot)
-            hipLaunchKernelGGL(k_fin_alpha, dim3(1), dim3(kFinThreads), 0, s, part, rounds * gs,
-                               st);
-        return;
-    }
-    const int cpl = (k + 63) / 64;
-#define HDFE_WIDE(C)                                                                        \
-    hipLaunchKernelGGL((k_spmm_wide<T, C>), dim3(g), dim3(kThreads), 4 * k * 8, s, M.indptr, \
-                       M.indices, val, M.L, P, AP, part, with_dot, st)
-    if (cpl <= 1) HDFE_WIDE(1);
-    else if (cpl <= 2) HDFE_WIDE(2);
-    else if (cpl <= 3) HDFE_WIDE(3);
-    else if (cpl <= 4) HDFE_WIDE(4);
-    else if (cpl <= 5) HDFE_WIDE(5);
-    else if (cpl <= 6) HDFE_WIDE(6);
-    else if (cpl <= 8) HDFE_WIDE(8);
-    else HDFE_WIDE(16);
-#undef HDFE_WIDE
-    if (with_dot)
-        hipLaunchKernelGGL(k_fin_alpha, dim3(1), dim3(kFinThreads), 0, s, part, g, st);
+    if (M.p64)
+        launch_plain<T, int64_t>(M, M.indptr64, M.chunk_beg64, val, P, AP, part, lpart, with_dot,
+                                 st, s, force_wide, g);
+    else
+        launch_plain<T, int>(M, M.indptr, M.chunk_beg, val, P, AP, part, lpart, with_dot, st, s,
+                             force_wide, g);
 }
 
 template <typename T>
@@ -2858,23 +2896,29 @@ static int solve_mixed(Matrix &M, int k, const uint8_t *zero_h, bool has_x0, dou
     return HDFE_STATUS_OK;
 }
 
-// A device matrix from a host int32 indptr and host or device indices,
-// values and inverse diagonal.
-int matrix_create(int64_t L, int64_t nnz, const std::vector<int> &ip32, const int *indices,
-                  const double *data, const double *dinv, bool on_device, void **out,
-                  int *own_indptr, int *own_indices, double *own_data) {
+// A device matrix from host row pointers (int32, or int64: 64-bit row
+// pointers on the device) and host or device indices, values and inverse
+// diagonal.
+template <typename I>
+static int matrix_create_t(int64_t L, int64_t nnz, const std::vector<I> &ip32,
+                           const int *indices, const double *data, const double *dinv,
+                           bool on_device, void **out, int *own_indptr, int *own_indices,
+                           double *own_data) {
+    constexpr bool p64 = sizeof(I) == 8;
     std::unique_ptr<Matrix> M(new Matrix());
+    M->p64 = p64;
     HIP_TRY(hipGetDevice(&M->device));
     M->L = L;
     M->nnz = nnz;
-    std::vector<int> lrows, lcptr(1, 0), crow, cbeg;
+    std::vector<int> lrows, lcptr(1, 0), crow;
+    std::vector<I> cbeg;
     for (int64_t i = 0; i < L; ++i) {
         const int64_t len = (int64_t)ip32[i + 1] - ip32[i];
         if (len > kLongThresh) {
             lrows.push_back((int)i);
             for (int64_t p = ip32[i]; p < ip32[i + 1]; p += kLongChunk) {
                 crow.push_back((int)i);
-                cbeg.push_back((int)p);
+                cbeg.push_back((I)p);
             }
             lcptr.push_back((int)crow.size());
         }
@@ -2884,8 +2928,12 @@ int matrix_create(int64_t L, int64_t nnz, const std::vector<int> &ip32, const in
     HIP_TRY(stream_get(&M->stream));
     HIP_TRY(hipEventCreateWithFlags(&M->ev, hipEventDisableTiming));
     hipStream_t ms = M->stream;
-    const bool adopt = own_indptr && own_indices && own_data;
-    if (adopt) {
+    const bool adopt = !p64 && own_indptr && own_indices && own_data;
+    if (p64) {
+        HIP_TRY(palloc(&M->indptr64, (L + 1) * 8, ms));
+        HIP_TRY(palloc(&M->indices, std::max<int64_t>(nnz, 1) * 4, ms));
+        HIP_TRY(palloc(&M->val64, std::max<int64_t>(nnz, 1) * 8, ms));
+    } else if (adopt) {
         M->indptr = own_indptr;
         M->indices = own_indices;
         M->val64 = own_data;
@@ -2900,7 +2948,8 @@ int matrix_create(int64_t L, int64_t nnz, const std::vector<int> &ip32, const in
     // every upload and the FP32 casts in order on the matrix's own stream (a
     // device-to-device hipMemcpy may return before the copy has finished)
     if (!adopt)
-        HIP_TRY(hipMemcpyAsync(M->indptr, ip32.data(), (L + 1) * 4, hipMemcpyHostToDevice, ms));
+        HIP_TRY(hipMemcpyAsync(p64 ? (void *)M->indptr64 : (void *)M->indptr, ip32.data(),
+                               (L + 1) * sizeof(I), hipMemcpyHostToDevice, ms));
     const hipMemcpyKind up = on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
     if (nnz && !adopt) {
         HIP_TRY(hipMemcpyAsync(M->indices, indices, nnz * 4, up, ms));
@@ -2918,17 +2967,27 @@ int matrix_create(int64_t L, int64_t nnz, const std::vector<int> &ip32, const in
         HIP_TRY(palloc(&M->long_rows, M->nlong * 4, ms));
         HIP_TRY(palloc(&M->long_cptr, (M->nlong + 1) * 4, ms));
         HIP_TRY(palloc(&M->chunk_row, M->nchunk * 4, ms));
-        HIP_TRY(palloc(&M->chunk_beg, M->nchunk * 4, ms));
+        if (p64) HIP_TRY(palloc(&M->chunk_beg64, M->nchunk * 8, ms));
+        else HIP_TRY(palloc(&M->chunk_beg, M->nchunk * 4, ms));
         HIP_TRY(hipMemcpyAsync(M->long_rows, lrows.data(), M->nlong * 4, hipMemcpyHostToDevice, ms));
         HIP_TRY(hipMemcpyAsync(M->long_cptr, lcptr.data(), (M->nlong + 1) * 4,
                                hipMemcpyHostToDevice, ms));
         HIP_TRY(hipMemcpyAsync(M->chunk_row, crow.data(), M->nchunk * 4, hipMemcpyHostToDevice, ms));
-        HIP_TRY(hipMemcpyAsync(M->chunk_beg, cbeg.data(), M->nchunk * 4, hipMemcpyHostToDevice, ms));
+        HIP_TRY(hipMemcpyAsync(p64 ? (void *)M->chunk_beg64 : (void *)M->chunk_beg, cbeg.data(),
+                               M->nchunk * sizeof(I), hipMemcpyHostToDevice, ms));
     }
     HIP_TRY(hipStreamSynchronize(M->stream));
-    M->ip_host = ip32;  // kept for the blocked layout, built at the first small-k use
+    // kept for the blocked layout, built at the first small-k use
+    if constexpr (!p64) M->ip_host = ip32;
     *out = M.release();
     return HDFE_STATUS_OK;
+}
+
+int matrix_create(int64_t L, int64_t nnz, const std::vector<int> &ip32, const int *indices,
+                  const double *data, const double *dinv, bool on_device, void **out,
+                  int *own_indptr, int *own_indices, double *own_data) {
+    return matrix_create_t<int>(L, nnz, ip32, indices, data, dinv, on_device, out, own_indptr,
+                                own_indices, own_data);
 }
 
 // The wide form's plan (matfree_create): the dense dimensions, the sparse
@@ -3157,18 +3216,68 @@ int hdfe_csr_create(int64_t L, int64_t nnz, const int64_t *indptr, const int32_t
                     const double *data, const double *dinv, int on_device, void **out) {
     ARG_CHECK(out != nullptr, "hdfe_csr_create: null output handle");
     ARG_CHECK(L >= 0 && nnz >= 0, "hdfe_csr_create: negative size");
-    ARG_CHECK(nnz < INT32_MAX && L < INT32_MAX,
-              "hdfe_csr_create: nnz and L must be below 2^31 (int32 CSR on the device)");
+    ARG_CHECK(L < INT32_MAX, "hdfe_csr_create: L must be below 2^31 (int32 column indices)");
     const hipMemcpyKind kind = on_device ? hipMemcpyDeviceToHost : hipMemcpyHostToHost;
     std::vector<int64_t> ip(L + 1);
     HIP_TRY(hipMemcpy(ip.data(), indptr, (L + 1) * 8, kind));
     ARG_CHECK(ip[0] == 0 && ip[L] == nnz, "hdfe_csr_create: indptr must run from 0 to nnz");
-    std::vector<int> ip32(L + 1);
-    for (int64_t i = 0; i <= L; ++i) {
-        if (i < L) ARG_CHECK(ip[i + 1] >= ip[i], "hdfe_csr_create: indptr must be non-decreasing");
-        ip32[i] = (int)ip[i];
-    }
+    for (int64_t i = 0; i < L; ++i)
+        ARG_CHECK(ip[i + 1] >= ip[i], "hdfe_csr_create: indptr must be non-decreasing");
+    // 2^31 nonzeros and more: 64-bit row pointers on the device (the
+    // reference's int64 index arrays, core/solve.py:169, 188-189)
+    if (nnz >= INT32_MAX)
+        return hdfe::matrix_create_t<int64_t>(L, nnz, ip, indices, data, dinv, on_device != 0,
+                                              out, nullptr, nullptr, nullptr);
+    std::vector<int> ip32(ip.begin(), ip.end());
     return hdfe::matrix_create(L, nnz, ip32, indices, data, dinv, on_device != 0, out);
+}
+
+// The row pointers' width on the device (32 or 64 bits); set = 64 widens a
+// matrix's 32-bit row pointers (its column-blocked and dense split-row copies
+// are dropped: the 64-bit form runs the plain CSR kernels only); set = -1
+// reads.  A matrix of 2^31 nonzeros or more is created with 64-bit ones.
+int hdfe_csr_row_pointer_bits(void *h, int set, int *bits) {
+    ARG_CHECK(h && bits && (set == -1 || set == 32 || set == 64),
+              "hdfe_csr_row_pointer_bits: bad argument");
+    Matrix &M = *(Matrix *)h;
+    std::lock_guard<std::mutex> lock(M.mu);
+    ARG_CHECK(!M.mf, "hdfe_csr_row_pointer_bits: a matrix-free operator has no row pointers");
+    ARG_CHECK(!(set == 32 && M.p64), "hdfe_csr_row_pointer_bits: row pointers only widen");
+    if (set == 64 && !M.p64) {
+        HIP_TRY(hipSetDevice(M.device));
+        hipStream_t ms = M.stream;
+        M.drop_graphs();
+        M.w.release(ms);
+        void *ps[] = {M.tbp, M.off16, M.bidx16, M.col_lo, M.bptr, M.bidx, M.bval64, M.bval32,
+                      M.dl_slot, M.dl_cptr, M.dt64, M.dt32, M.ll64, M.ll32, M.dw64, M.dw32};
+        for (void *q : ps) pfree(q, ms);
+        M.tbp = M.col_lo = M.bptr = M.bidx = M.dl_slot = M.dl_cptr = nullptr;
+        M.off16 = M.bidx16 = nullptr;
+        M.bval64 = M.dt64 = M.ll64 = M.dw64 = nullptr;
+        M.bval32 = M.dt32 = M.ll32 = M.dw32 = nullptr;
+        M.blk = M.dl = false;
+        M.blk_tried = true;
+        M.dw = 0;
+        M.bnnz = 0;
+        HIP_TRY(palloc(&M.indptr64, (M.L + 1) * 8, ms));
+        hipLaunchKernelGGL(k_widen, dim3(grid_for(M.L + 1, 1024, 1, 4096)), dim3(kThreads), 0, ms,
+                           M.L + 1, M.indptr, M.indptr64);
+        if (M.nchunk) {
+            HIP_TRY(palloc(&M.chunk_beg64, (int64_t)M.nchunk * 8, ms));
+            hipLaunchKernelGGL(k_widen, dim3(grid_for(M.nchunk, 1024, 1, 4096)), dim3(kThreads), 0,
+                               ms, (int64_t)M.nchunk, M.chunk_beg, M.chunk_beg64);
+        }
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipStreamSynchronize(ms));
+        pfree(M.indptr, ms);
+        pfree(M.chunk_beg, ms);
+        M.indptr = M.chunk_beg = nullptr;
+        M.ip_host.clear();
+        M.ip_host.shrink_to_fit();
+        M.p64 = true;
+    }
+    *bits = M.p64 ? 64 : 32;
+    return HDFE_STATUS_OK;
 }
 
 int hdfe_csr_info(void *h, int64_t *L, int64_t *nnz, int *nlong) {
@@ -3255,7 +3364,9 @@ int hdfe_csr_export(void *h, int64_t *indptr, int32_t *indices, double *data, do
     ARG_CHECK(!M.mf || (!indptr && !indices && !data),
               "hdfe_csr_export: a matrix-free operator has no explicit S");
     HIP_TRY(hipSetDevice(M.device));
-    if (indptr) {
+    if (indptr && M.p64) {
+        HIP_TRY(hipMemcpy(indptr, M.indptr64, (M.L + 1) * 8, hipMemcpyDeviceToHost));
+    } else if (indptr) {
         std::vector<int> ip(M.L + 1);
         HIP_TRY(hipMemcpy(ip.data(), M.indptr, (M.L + 1) * 4, hipMemcpyDeviceToHost));
         for (int64_t i = 0; i <= M.L; ++i) indptr[i] = ip[i];
@@ -3452,7 +3563,8 @@ void hdfe_destroy(void *h) {
     if (M->stream) (void)hipStreamSynchronize(M->stream);
     M->drop_graphs();
     M->w.release(M->stream);
-    void *ps[] = {M->indptr, M->indices, M->val64, M->val32, M->dinv64, M->dinv32,
+    void *ps[] = {M->indptr, M->indptr64, M->chunk_beg64, M->indices, M->val64, M->val32,
+                  M->dinv64, M->dinv32,
                   M->tbp, M->off16, M->bidx16,
                   M->long_rows, M->long_cptr, M->chunk_row, M->chunk_beg, M->col_lo, M->bptr,
                   M->bidx, M->bval64, M->bval32, M->dl_slot, M->dl_cptr, M->dt64, M->dt32,

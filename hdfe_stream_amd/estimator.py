@@ -435,16 +435,18 @@ class StreamingHDFE:
         if panel.matrix is None:
             # S would not fit the headroom: the matrix-free operator over the
             # cells (core/solve.py:205-218, 251-271), on the device
+            budget = self._s_budget(panel)
+            if budget is not None and panel.s_skipped > budget:
+                why = (f"S would take about {panel.s_skipped / 1e9:.3g} GB > the "
+                       f"{budget / 1e9:.3g} GB of device memory headroom")
+            else:
+                why = "S would hold 2**31 entries or more (the device build counts in 32 bits)"
             if self.solver == "explicit":
-                raise MemoryError(f"S would take about {panel.s_skipped / 1e9:.3g} GB, more "
-                                  "than the device memory headroom; raise max_memory_gb or "
-                                  "use solver='stream_cg'")
+                raise MemoryError(f"{why}; raise max_memory_gb or use solver='stream_cg'")
             panel.stream_matrix()
             info["solver"] = "stream_cg"
             if self.solver == "auto":
-                info["fallback"] = (f"S would take about {panel.s_skipped / 1e9:.3g} GB > the "
-                                    f"{self._s_budget(panel) / 1e9:.3g} GB of device memory "
-                                    "headroom: matrix-free stream_cg on the device")
+                info["fallback"] = f"{why}: matrix-free stream_cg on the device"
             info.update(S_nnz=0, S_mb=0.0, S_build_seconds=0.0)
         else:
             info.update(S_nnz=panel.info["nnz"],

@@ -191,6 +191,15 @@ class DeviceCSR:
                                                    int(set), C.byref(out)), "hdfe_csr_small_form")
         return int(out.value)
 
+    def row_pointer_bits(self, set: int = -1) -> int:
+        """The row pointers' width on the device (32 or 64); set=64 widens
+        them (only the plain CSR products run then, as for a matrix of 2**31
+        nonzeros or more, which is created with 64-bit ones)."""
+        out = C.c_int()
+        _lib.check(_lib.load().hdfe_csr_row_pointer_bits(self._h, int(set), C.byref(out)),
+                   "hdfe_csr_row_pointer_bits")
+        return int(out.value)
+
     def graphs(self) -> int:
         """CG batches held as HIP graphs (captured on small systems only)."""
         n = C.c_int64()

@@ -37,7 +37,9 @@ int hdfe_device_info(char *name, int len, int64_t *free_bytes, int64_t *total_by
 
 /* Copy S (CSR, L x L, sorted column indices) and the Jacobi inverse diagonal
  * to the device once; *out receives the handle.  indptr int64 (L+1), indices
- * int32 (nnz), data float64 (nnz), dinv float64 (L); host or device.
+ * int32 (nnz), data float64 (nnz), dinv float64 (L); host or device.  L must
+ * be below 2^31; with 2^31 nonzeros or more the device keeps 64-bit row
+ * pointers (the reference's int64 index arrays, core/solve.py:169, 188-189).
  * Replaces explicit_solver's upload, core/gpu.py:186-192 (and the caller at
  * core/solve.py:221-229). */
 int hdfe_csr_create(int64_t L, int64_t nnz, const int64_t *indptr, const int32_t *indices,
@@ -119,6 +121,13 @@ int hdfe_csr_wide_mode(void *h, int k, int precision, int set, int *mode);
  * same sums as 2); `set` >= 0 overrides it.  Each form is deterministic;
  * 1 and 2 differ in rounding. */
 int hdfe_csr_small_form(void *h, int k, int precision, int set, int *form);
+
+/* The row pointers' width on the device, 32 or 64 (*bits); set = 64 widens
+ * 32-bit ones (the matrix then runs the plain CSR products only, as one of
+ * 2^31 nonzeros or more does), set = -1 reads.  No counterpart in the
+ * reference (scipy picks the index type by size); for tests of the 64-bit
+ * path at sizes that fit a test. */
+int hdfe_csr_row_pointer_bits(void *h, int set, int *bits);
 
 /* The number of CG batches held as HIP graphs on this matrix.  Batches are
  * captured only on systems small enough that launching costs more than the
