@@ -251,14 +251,43 @@ __global__ __launch_bounds__(kFinThreads) void k_fin_alpha(const double *__restr
 // With ll (the split rows' block among themselves, nlong x nlong) the
 // partials hold only the split rows' products with the other rows, and
 // sum_t' ll[l, t'] P[long t'] is added last.
+// Every load is issued up front -- the chunk partials, each thread's share of
+// the pAp partials, and (up to kFinStage split rows) the split rows' P and ll,
+// held in registers until the partials are in -- so the kernel waits on
+// memory about twice, not once per split row (loops over the split rows that
+// each chased long_rows[u] to P cost ~10 us per product at eight rows).
+constexpr int kFinStage = 64;
 template <typename T, int K>
 __device__ void spmm_finish(const T *__restrict__ P, T *__restrict__ AP,
                             const double *__restrict__ part, int nw, int with_dot,
                             const int *__restrict__ long_rows, const int *__restrict__ long_cptr,
                             int nlong, const double *__restrict__ lpart, CgState st,
                             const T *__restrict__ ll = nullptr) {
-    const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
+    const int tid = threadIdx.x;
+    const int lane = tid & (kWave - 1), wv = tid / kWave;
     const int nwv = blockDim.x / kWave;
+    const bool staged = nlong <= kFinStage;
+    const bool lls = ll && nlong <= kDenseLongMax;
+    __shared__ double spl[kFinStage * K], sap[kFinStage * K];
+    __shared__ double sll[kDenseLongMax * kDenseLongMax];
+    // the split rows' P and ll, into registers first
+    double plv = 0.0, llv = 0.0;
+    if (staged && tid < nlong * K) plv = (double)P[(int64_t)long_rows[tid / K] * K + tid % K];
+    if (lls && tid < nlong * nlong) llv = (double)ll[tid];
+    // the pAp partials: thread (r, c) sums rows r, r + TR, ... of column c
+    constexpr int TJ = K <= 1 ? 1 : (K <= 2 ? 2 : (K <= 4 ? 4 : 8));
+    const int TR = blockDim.x / TJ, pc = tid % TJ, pr = tid / TJ;
+    double pa0 = 0.0, pa1 = 0.0, pa2 = 0.0, pa3 = 0.0;
+    if (with_dot && pc < K) {
+        int w = pr;
+        for (; w + 3 * TR < nw; w += 4 * TR) {
+            pa0 += part[(int64_t)w * K + pc];
+            pa1 += part[(int64_t)(w + TR) * K + pc];
+            pa2 += part[(int64_t)(w + 2 * TR) * K + pc];
+            pa3 += part[(int64_t)(w + 3 * TR) * K + pc];
+        }
+        for (; w < nw; w += TR) pa0 += part[(int64_t)w * K + pc];
+    }
     // split row l's partials (its range of partial rows, each K values) in
     // H stretches, one wave each: every lane adds whole partial rows (all K
     // values in one contiguous load), four in flight; the stretches' sums
@@ -287,9 +316,15 @@ __device__ void spmm_finish(const T *__restrict__ P, T *__restrict__ AP,
                 a3[j] += lpart[(int64_t)(c + 3 * kWave) * K + j];
             }
         }
-        for (; c < c1; c += kWave)
+        if (c < c1)
 #pragma unroll
             for (int j = 0; j < K; ++j) a0[j] += lpart[(int64_t)c * K + j];
+        if (c + kWave < c1)
+#pragma unroll
+            for (int j = 0; j < K; ++j) a1[j] += lpart[(int64_t)(c + kWave) * K + j];
+        if (c + 2 * kWave < c1)
+#pragma unroll
+            for (int j = 0; j < K; ++j) a2[j] += lpart[(int64_t)(c + 2 * kWave) * K + j];
 #pragma unroll
         for (int j = 0; j < K; ++j) {
             const double v = wave_sum((a0[j] + a1[j]) + (a2[j] + a3[j]));
@@ -303,30 +338,46 @@ __device__ void spmm_finish(const T *__restrict__ P, T *__restrict__ AP,
                             s2 += (double)ll[l * nlong + u] *
                                   (double)P[(int64_t)long_rows[u] * K + j];
                     AP[(int64_t)long_rows[l] * K + j] = (T)s2;
+                    if (staged) sap[l * K + j] = (double)(T)s2;
                 }
             }
         }
     }
+    if (staged && tid < nlong * K) spl[tid] = plv;
+    if (lls && tid < nlong * nlong) sll[tid] = llv;
+    __shared__ double red[kFinThreads];
+    red[tid] = (pa0 + pa1) + (pa2 + pa3);
     __syncthreads();
-    for (int t = threadIdx.x; t < (direct ? 0 : nlong * K); t += blockDim.x) {
+    for (int t = tid; t < (direct ? 0 : nlong * K); t += blockDim.x) {
         const int l = t / K, j = t % K;
         double s2 = 0.0;
         for (int h = 0; h < H; ++h) s2 += hsum[l * H + h][j];
-        if (ll)
+        if (lls) {
+            for (int u = 0; u < nlong; ++u) s2 += sll[l * nlong + u] * spl[u * K + j];
+        } else if (ll) {
             for (int u = 0; u < nlong; ++u)
                 s2 += (double)ll[l * nlong + u] * (double)P[(int64_t)long_rows[u] * K + j];
+        }
         AP[(int64_t)long_rows[l] * K + j] = (T)s2;
+        if (staged) sap[t] = (double)(T)s2;
     }
     if (!with_dot) return;
-    __shared__ double pap[K];
-    reduce_parts(part, nw, K, K, pap);
+    // the pAp partials' tree over the TR row lanes of each column
+    for (int h = TR / 2; h > 0; h >>= 1) {
+        __syncthreads();
+        if (pr < h) red[pr * TJ + pc] += red[(pr + h) * TJ + pc];
+    }
     __syncthreads();  // the split rows' AP, written above, are visible here
-    if (threadIdx.x < K) {
-        const int j = threadIdx.x;
-        double d = pap[j];
-        for (int l = 0; l < nlong; ++l) {
-            const int64_t row = long_rows[l];
-            d += (double)P[row * K + j] * (double)AP[row * K + j];
+    if (tid < K) {
+        const int j = tid;
+        double d = red[j];
+        if (staged) {
+            for (int l = 0; l < nlong; ++l) d += spl[l * K + j] * sap[l * K + j];
+        } else {
+            for (int l = 0; l < nlong; ++l) {
+                const int64_t row = long_rows[l];
+                d += (double)P[row * K + j] * (double)AP[row * K + j];
+            }
         }
         st.alpha[j] = (st.active[j] && d > 0.0) ? st.rz[j] / d : 0.0;
     }
@@ -591,10 +642,10 @@ __global__ __launch_bounds__(kThreads) void k_spmm_combine(
 // consecutive rows (R fixed by L, k and the precision, never by timing); in
 // column block b the tile's nonzeros are one
 // contiguous range of the blocked arrays, which the workgroup streams in
-// chunks of CH nonzeros: every thread loads its nonzeros' column indices and
-// values (coalesced) and gathers their P rows with four in flight, writing
-// val x P[col] to LDS; then each row's thread adds its row's products from
-// LDS in ascending order.  No per-row pointer chase sits between a load and
+// chunks of CH nonzeros: every thread loads V consecutive nonzeros' column
+// indices and values (one vector load each) and gathers their P rows, all V
+// in flight, writing val x P[col] to LDS; then each row's thread adds its
+// row's products from LDS in ascending order.  No per-row pointer chase sits between a load and
 // its gather, and no lane waits on a longer row.  The blocks are taken in
 // order, so the workgroups, started together, gather from one block's slice
 // of P at a time (kNB blocks: a slice of P small enough to stay in an XCD's
@@ -603,7 +654,24 @@ __global__ __launch_bounds__(kThreads) void k_spmm_combine(
 // P[row] in dlpart[(t nw + w) K + j] (k_fin_small adds them in tile order).
 // Measured against the groups-of-blocks form (k_spmm_blk + k_spmm_combine)
 // on the 5M-worker system: 107 vs 123 us at k = 3 (FP64), 77 vs 93 (FP32),
-// 172 vs 315 at k = 8; non-temporal loads of S made it slower (135 us).
+// 172 vs 315 at k = 8; non-temporal loads of S made it slower (135 us).  The
+// counters put it at the L2's limit (TCC busy 82% of the kernel, 11.5M
+// requests: about one per gathered nonzero); one vector load per thread for
+// its V indices and values (instead of V strided loads of each) took the FP64
+// product from 105 to 99 us, two column blocks a round (half the barriers)
+// or six nonzeros a thread made it slower (114 us).
+// positions of the blocked arrays a thread of the tiled SpMM takes per chunk
+// (its column indices and values in one vector load each); the blocked
+// arrays hold kTilePad spare entries past their end for the last vectors
+template <int K>
+constexpr int tile_vec() {
+    return K <= 4 ? 4 : 2;
+}
+constexpr int kTilePad = 4;
+template <typename T, int N>
+struct VecN {
+    typedef T type __attribute__((ext_vector_type(N)));
+};
 template <typename T, int K, int R>
 __global__ __launch_bounds__(kThreads) void k_spmm_tiles(
     const int *__restrict__ bptr, const int *__restrict__ bidx, const T *__restrict__ bval,
@@ -614,10 +682,14 @@ __global__ __launch_bounds__(kThreads) void k_spmm_tiles(
     const int *__restrict__ col_lo) {
     if (st.done && *st.done) return;
     const bool C16 = off16 != nullptr;   // the compact indices (tbp, off16, bidx16)
-    constexpr int CH = K <= 4 ? 1024 : 512;
+    constexpr int V = tile_vec<K>();
+    constexpr int CH = V * kThreads;
     __shared__ T prod[CH * K];
     __shared__ double lds[kThreads / kWave];
     __shared__ T plong[kDenseLongMax * K];
+    // the rows' starts in the current block: a row's end is the next row's
+    // start (read after the chunk's barrier), one load per row and block
+    __shared__ int rsl[R * kThreads + 1];
     const int nw = gridDim.x, w = blockIdx.x, tid = threadIdx.x;
     const int64_t r0 = (int64_t)w * R * kThreads, r1 = min(L, r0 + R * kThreads);
     for (int q = tid; q < nld * K; q += blockDim.x)
@@ -642,15 +714,11 @@ __global__ __launch_bounds__(kThreads) void k_spmm_tiles(
 #pragma unroll
             for (int q = 0; q < R; ++q) {
                 const int64_t row = r0 + q * kThreads + tid;
-                rs[q] = re[q] = 0;
-                if (row < r1) {
-                    const int64_t si = r0 / kThreads + q;   // row's stretch, < nt
-                    const int st0 = tb[si];
-                    rs[q] = st0 + off16[boff + row];
-                    re[q] = (tid + 1 == kThreads || row + 1 == L) ? tb[si + 1]
-                                                                   : st0 + off16[boff + row + 1];
-                }
+                const int64_t si = r0 / kThreads + q;   // row's stretch
+                rs[q] = row < r1 ? tb[si] + off16[boff + row] : s1;
+                rsl[q * kThreads + tid] = rs[q];
             }
+            if (tid == 0) rsl[R * kThreads] = s1;
         } else {
             s0 = bptr[boff + r0];
             s1 = bptr[boff + r1];
@@ -661,32 +729,47 @@ __global__ __launch_bounds__(kThreads) void k_spmm_tiles(
                 re[q] = row < r1 ? bptr[boff + row + 1] : 0;
             }
         }
-        for (int c0 = s0; c0 < s1; c0 += CH) {
-            const int cn = min(CH, s1 - c0);
-            // the chunk's products, four nonzeros per thread in flight
-            for (int t = tid; t < cn; t += 4 * kThreads) {
-                int cc[4];
-                T vv[4];
+        // chunks of CH positions from s0 rounded down to a multiple of V:
+        // thread t takes positions base + V t .. + V - 1 (one aligned vector
+        // load of their indices and one of their values) and gathers the P
+        // rows of those in [s0, s1), all V in flight
+        for (int base = s0 & ~(V - 1); base < s1; base += CH) {
+            const int p0 = base + V * tid;
+            int cc[V];
+            T vv[V];
+            if (p0 < s1) {
+                if (C16) {
+                    const typename VecN<uint16_t, V>::type ix =
+                        *(const typename VecN<uint16_t, V>::type *)(bidx16 + p0);
 #pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    const int q = t + u * kThreads;
-                    cc[u] = q < cn ? (C16 ? cb + (int)bidx16[c0 + q] : bidx[c0 + q]) : 0;
-                    vv[u] = q < cn ? bval[c0 + q] : T(0);
+                    for (int v = 0; v < V; ++v) cc[v] = cb + (int)ix[v];
+                } else {
+                    const typename VecN<int, V>::type ix =
+                        *(const typename VecN<int, V>::type *)(bidx + p0);
+#pragma unroll
+                    for (int v = 0; v < V; ++v) cc[v] = ix[v];
                 }
+                const typename VecN<T, V>::type xv = *(const typename VecN<T, V>::type *)(bval + p0);
 #pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    const int q = t + u * kThreads;
-                    if (q < cn) {
+                for (int v = 0; v < V; ++v) vv[v] = xv[v];
+#pragma unroll
+                for (int v = 0; v < V; ++v) {
+                    const int p = p0 + v;
+                    if (p >= s0 && p < s1) {
 #pragma unroll
                         for (int j = 0; j < K; ++j)
-                            prod[q * K + j] = vv[u] * P[(int64_t)cc[u] * K + j];
+                            prod[(V * tid + v) * K + j] = vv[v] * P[(int64_t)cc[v] * K + j];
                     }
                 }
             }
             __syncthreads();
+            if (C16) {
+#pragma unroll
+                for (int q = 0; q < R; ++q) re[q] = rsl[q * kThreads + tid + 1];
+            }
 #pragma unroll
             for (int q = 0; q < R; ++q) {
-                const int a = max(rs[q], c0) - c0, e = min(re[q], c0 + cn) - c0;
+                const int a = max(rs[q], base) - base, e = min(re[q], base + CH) - base;
                 for (int p = a; p < e; ++p)
 #pragma unroll
                     for (int j = 0; j < K; ++j) acc[q][j] += prod[p * K + j];
@@ -741,6 +824,14 @@ __global__ __launch_bounds__(kThreads) void k_spmm_tiles(
     }
 }
 
+// a wave-uniform value in a scalar register (64-bit: its halves in turn)
+__device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ int64_t uniform(int64_t v) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)((uint64_t)v >> 32));
+    return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
 // Wide k: one wave per row, lanes over columns (j = lane + 64 t, t < CPL).
 // Each nonzero's column index and value are loaded once (wave-uniform) and
 // feed CPL coalesced gathers of the neighbour's row of P; two nonzeros are in
@@ -751,14 +842,6 @@ __global__ __launch_bounds__(kThreads) void k_spmm_tiles(
 // block -- stay resident in the memory-side cache instead of the whole of P
 // (a contiguous row range per workgroup keeps every part of the matrix in
 // flight at once).
-// a wave-uniform value in a scalar register (64-bit: its halves in turn)
-__device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
-__device__ __forceinline__ int64_t uniform(int64_t v) {
-    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
-    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)((uint64_t)v >> 32));
-    return (int64_t)(((uint64_t)hi << 32) | lo);
-}
-
 template <typename T, int CPL, typename I>
 __global__ __launch_bounds__(kThreads) void k_spmm_wide(
     const I *__restrict__ indptr, const int *__restrict__ indices, const T *__restrict__ val,
@@ -2175,9 +2258,10 @@ static int build_blocked(Matrix &M, const std::vector<int> &ip32) {
     int last = 0;
     HIP_TRY(scalar_to_host(&last, M.bptr + kNB * L, ms));
     M.bnnz = last;
-    HIP_TRY(palloc(&M.bidx, std::max<int64_t>(M.bnnz, 1) * 4, ms));
-    HIP_TRY(palloc(&M.bval64, std::max<int64_t>(M.bnnz, 1) * 8, ms));
-    HIP_TRY(palloc(&M.bval32, std::max<int64_t>(M.bnnz, 1) * 4, ms));
+    // kTilePad spare entries: the tiled SpMM's last vector loads
+    HIP_TRY(palloc(&M.bidx, (M.bnnz + kTilePad) * 4, ms));
+    HIP_TRY(palloc(&M.bval64, (M.bnnz + kTilePad) * 8, ms));
+    HIP_TRY(palloc(&M.bval32, (M.bnnz + kTilePad) * 4, ms));
     hipLaunchKernelGGL(k_blk_fill, dim3(gwave), dim3(kThreads), 0, ms, L, M.indptr, M.indices,
                        M.val64, M.val32, M.col_lo, M.bptr, M.bidx, M.bval64, M.bval32,
                        M.dl ? (const int *)M.dl_slot : nullptr, M.dt64, M.dt32);
@@ -2208,7 +2292,7 @@ static int build_blocked(Matrix &M, const std::vector<int> &ip32) {
                 }
         if (narrow) {
             HIP_TRY(palloc(&M.off16, kNB * L * 2, ms));
-            HIP_TRY(palloc(&M.bidx16, M.bnnz * 2, ms));
+            HIP_TRY(palloc(&M.bidx16, (M.bnnz + kTilePad) * 2, ms));
             hipLaunchKernelGGL(k_off16_fill, dim3(grid_for(kNB * L, 256, 1, 16384)), dim3(256), 0,
                                ms, L, nt, M.bptr, M.tbp, M.off16);
             hipLaunchKernelGGL(k_bidx16_fill, dim3(grid_for(M.bnnz, 256, 1, 16384)), dim3(256), 0,
