@@ -227,76 +227,143 @@ __global__ __launch_bounds__(256) void k_glm_center(
 // x~ split into bf16 hi + lo, row-major: element (p, j) of the padded rows at
 // p * Cp + j (a row's lanes store one contiguous stretch; the rows-of-eight
 // layout the matrix cores read is made while k_glm_syrk stages them).
-// A wave per group, lanes over columns (two sweeps of the group's rows: the
-// means, then the values).
+// A wave per group, lanes over columns, J = ceil(C / 64) columns a lane.  A
+// group of at most kGlmStage rows (a worker's years) is read once: its rows'
+// x - sum_d Gm[level] stay in registers for the mean and then the values,
+// every row's loads in flight together; a longer group takes two sweeps of its
+// rows (the means, then the values), gathering the effects twice.
+constexpr int kGlmStage = 8;
+// Row i's x - sum_d Gm[level_d] (and z likewise in column 0) for the lane's
+// columns j = lane + 64 t: every load's address is valid (columns past C and
+// column 0's X read clamped ones, the values selected afterwards), so a row's
+// loads carry no branches and the rows' loads of a staged group all fly
+// together.  i is wave-uniform.
+template <int J, int D>
+__device__ __forceinline__ void glm_partial_row(int lane, int C, int i, const double *__restrict__ z,
+                                                const float *__restrict__ X, int kx,
+                                                const int *__restrict__ codes, int64_t ld,
+                                                const int64_t *__restrict__ offs,
+                                                const double *__restrict__ Gm, double (&u)[J]) {
+    int64_t lv[D > 0 ? D : 1];
+#pragma unroll
+    for (int d = 0; d < D; ++d) lv[d] = (offs[d] + codes[d * ld + i]) * C;
+    const double zi = z[i];
+    const float *xr = X + (int64_t)i * kx;
+#pragma unroll
+    for (int t = 0; t < J; ++t) {
+        const int j = lane + 64 * t;
+        const int jj = min(j, C - 1);
+        const float xv = xr[max(jj - 1, 0)];   // loaded by every lane
+        double v = j == 0 ? zi : (double)xv;
+#pragma unroll
+        for (int d = 0; d < D; ++d) v -= Gm[lv[d] + jj];
+        u[t] = j < C ? v : 0.0;
+    }
+}
+
+template <int J>
+__device__ __forceinline__ void glm_partial_out(int lane, int C, int64_t base, double wi,
+                                                const double (&v)[J], const double *__restrict__ bm,
+                                                double (&gacc)[J], __bf16 *__restrict__ hi,
+                                                __bf16 *__restrict__ lo) {
+    double dot = 0.0, zt = 0.0;
+#pragma unroll
+    for (int t = 0; t < J; ++t) {
+        const int j = lane + 64 * t;
+        const double b = bm[max(min(j, C - 1) - 1, 0)];
+        if (j == 0) zt = v[t];
+        else dot += v[t] * b;   // v is 0 past C
+    }
+    dot = wave_sum(dot);
+    zt = __shfl(zt, 0, 64);
+    const double r = zt - dot;
+    const double sw = sqrt(wi);
+#pragma unroll
+    for (int t = 0; t < J; ++t) {
+        const int j = lane + 64 * t;
+        const float sv = (float)(sw * v[t]);
+        const __bf16 h = (__bf16)sv;
+        const __bf16 l = (__bf16)(sv - (float)h);
+        if (j >= 1 && j < C) {
+            gacc[t] += wi * v[t] * r;
+            hi[base + j - 1] = h;
+            lo[base + j - 1] = l;
+        }
+    }
+}
+
+template <int J, int D>
 __global__ __launch_bounds__(256) void k_glm_partial(
     const int *__restrict__ wg_model, const int64_t *__restrict__ wg_gb,
     const int64_t *__restrict__ wg_ge, const int *__restrict__ gstart,
     const int64_t *__restrict__ rs, const int64_t *__restrict__ pstart, int Cp,
     const double *__restrict__ z, const float *__restrict__ X, int kx,
-    const double *__restrict__ w, const int *__restrict__ codes, int64_t ld, int D,
+    const double *__restrict__ w, const int *__restrict__ codes, int64_t ld,
     const int64_t *__restrict__ offs, const double *__restrict__ Gm,
     const double *__restrict__ beta, __bf16 *__restrict__ hi, __bf16 *__restrict__ lo,
     double *__restrict__ gpart) {
     extern __shared__ double glds[];  // [4][kx]
     const int C = kx + 1;
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int m = wg_model[blockIdx.x];
     const double *bm = beta + (int64_t)m * kx;
-    constexpr int kMaxJ = 8;  // columns per lane: C <= 512
-    double gacc[kMaxJ];
-    for (int t = 0; t < kMaxJ; ++t) gacc[t] = 0.0;
+    const int64_t prow = pstart[m] - rs[m];   // padded row of panel row i: prow + i
+    double gacc[J];
+#pragma unroll
+    for (int t = 0; t < J; ++t) gacc[t] = 0.0;
     for (int64_t gg = wg_gb[blockIdx.x] + wv; gg < wg_ge[blockIdx.x]; gg += 4) {
         const int s = gstart[gg], e = gstart[gg + 1];
-        double mean[kMaxJ], W = 0.0;
-        for (int t = 0; t < kMaxJ; ++t) mean[t] = 0.0;
-        for (int i = s; i < e; ++i) {
-            const double wi = w[i];
-            W += wi;
-            for (int t = 0; t < kMaxJ; ++t) {
-                const int j = lane + 64 * t;
-                if (j >= C) break;
-                double u = j == 0 ? z[i] : (double)X[(int64_t)i * kx + j - 1];
-                for (int d = 0; d < D; ++d) u -= Gm[(offs[d] + codes[d * ld + i]) * C + j];
-                mean[t] += wi * u;
+        double mean[J], W = 0.0;
+#pragma unroll
+        for (int t = 0; t < J; ++t) mean[t] = 0.0;
+        if (e - s <= kGlmStage) {
+            // rows past the group's end repeat its last row, weight 0
+            double u[kGlmStage][J], wr[kGlmStage];
+#pragma unroll
+            for (int r = 0; r < kGlmStage; ++r) {
+                const int i = s + min(r, e - s - 1);
+                const double wi = w[i];
+                wr[r] = r < e - s ? wi : 0.0;
+                glm_partial_row<J, D>(lane, C, i, z, X, kx, codes, ld, offs, Gm, u[r]);
             }
-        }
-        const double inv = W > 0.0 ? 1.0 / W : 0.0;
-        for (int i = s; i < e; ++i) {
-            const double wi = w[i];
-            const double sw = sqrt(wi);
-            double v[kMaxJ];
-            double dot = 0.0, zt = 0.0;
-            for (int t = 0; t < kMaxJ; ++t) {
-                const int j = lane + 64 * t;
-                v[t] = 0.0;
-                if (j >= C) continue;
-                double u = j == 0 ? z[i] : (double)X[(int64_t)i * kx + j - 1];
-                for (int d = 0; d < D; ++d) u -= Gm[(offs[d] + codes[d * ld + i]) * C + j];
-                u -= mean[t] * inv;
-                v[t] = u;
-                if (j == 0) zt = u;
-                else dot += u * bm[j - 1];
+#pragma unroll
+            for (int r = 0; r < kGlmStage; ++r) {
+                W += wr[r];
+#pragma unroll
+                for (int t = 0; t < J; ++t) mean[t] += wr[r] * u[r][t];
             }
-            dot = wave_sum(dot);
-            zt = __shfl(zt, 0, 64);
-            const double r = zt - dot;
-            const int64_t p = pstart[m] + (i - rs[m]);
-            const int64_t base = p * (int64_t)Cp;
-            for (int t = 0; t < kMaxJ; ++t) {
-                const int j = lane + 64 * t;
-                if (j < 1 || j >= C) continue;
-                gacc[t] += wi * v[t] * r;
-                const float sv = (float)(sw * v[t]);
-                const __bf16 h = (__bf16)sv;
-                const __bf16 l = (__bf16)(sv - (float)h);
-                hi[base + j - 1] = h;
-                lo[base + j - 1] = l;
+            const double inv = W > 0.0 ? 1.0 / W : 0.0;
+#pragma unroll
+            for (int r = 0; r < kGlmStage; ++r) {
+                if (r >= e - s) break;
+#pragma unroll
+                for (int t = 0; t < J; ++t) u[r][t] -= mean[t] * inv;
+                glm_partial_out<J>(lane, C, (prow + s + r) * (int64_t)Cp, wr[r], u[r], bm, gacc,
+                                   hi, lo);
+            }
+        } else {
+            for (int i = s; i < e; ++i) {
+                const double wi = w[i];
+                double u[J];
+                glm_partial_row<J, D>(lane, C, i, z, X, kx, codes, ld, offs, Gm, u);
+                W += wi;
+#pragma unroll
+                for (int t = 0; t < J; ++t) mean[t] += wi * u[t];
+            }
+            const double inv = W > 0.0 ? 1.0 / W : 0.0;
+            for (int i = s; i < e; ++i) {
+                double u[J];
+                glm_partial_row<J, D>(lane, C, i, z, X, kx, codes, ld, offs, Gm, u);
+#pragma unroll
+                for (int t = 0; t < J; ++t) u[t] -= mean[t] * inv;
+                glm_partial_out<J>(lane, C, (prow + i) * (int64_t)Cp, w[i], u, bm, gacc, hi, lo);
             }
         }
     }
     // the workgroup's score partial: waves in order
-    for (int t = 0; t < kMaxJ; ++t) {
+#pragma unroll
+    for (int t = 0; t < J; ++t) {
         const int j = lane + 64 * t;
         if (j >= 1 && j < C) glds[wv * kx + j - 1] = gacc[t];
     }
@@ -548,15 +615,16 @@ int hdfe_glm_rhs(void *rows_h, int64_t G, const int *gstart, const int *g, const
     return HDFE_STATUS_OK;
 }
 
-// The partialled rows (split bf16, hi and lo (npad / 8, Cp, 8), device) and
-// the score partials of nwg workgroups (host lists: model, group range);
-// gpart (host, nwg x kx).
+// The partialled rows (split bf16, hi and lo (npad, Cp) row-major, device)
+// and the score partials of nwg workgroups (host lists: model, group range);
+// gpart (host, nwg x kx).  D <= 4 fixed-effect dimensions besides the groups.
 int hdfe_glm_partial(int nwg, const int *wg_model, const int64_t *wg_gb, const int64_t *wg_ge,
                      const int *gstart, const int64_t *rs_h, const int64_t *pstart_h, int nmodels,
                      int Cp, const double *z, const float *X, int kx, const double *w,
                      const int *codes, int64_t ld, int D, const int64_t *offs, const double *Gm,
                      const double *beta, void *hi, void *lo, double *gpart_h) {
-    ARG_CHECK(nwg >= 1 && kx >= 1 && kx + 1 <= 512 && Cp >= kx && Cp % 32 == 0,
+    ARG_CHECK(nwg >= 1 && kx >= 1 && kx + 1 <= 512 && Cp >= kx && Cp % 32 == 0 && D >= 0 &&
+                  D <= 4,
               "hdfe_glm_partial: bad argument");
     DevBuf<int> wm;
     DevBuf<int64_t> gb, ge, rs, ps;
@@ -572,9 +640,30 @@ int hdfe_glm_partial(int nwg, const int *wg_model, const int64_t *wg_gb, const i
     HIP_TRY(hipMemcpy(ge.p, wg_ge, nwg * 8, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(rs.p, rs_h, (nmodels + 1) * 8, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(ps.p, pstart_h, (nmodels + 1) * 8, hipMemcpyHostToDevice));
-    hipLaunchKernelGGL(k_glm_partial, dim3(nwg), dim3(256), 4 * kx * 8, 0, wm.p, gb.p, ge.p,
-                       gstart, rs.p, ps.p, Cp, z, X, kx, w, codes, ld, D, offs, Gm, beta,
-                       (__bf16 *)hi, (__bf16 *)lo, gpart.p);
+#define HDFE_GP(JJ, DD)                                                                          \
+    hipLaunchKernelGGL((k_glm_partial<JJ, DD>), dim3(nwg), dim3(256), 4 * kx * 8, 0, wm.p, gb.p,   \
+                       ge.p, gstart, rs.p, ps.p, Cp, z, X, kx, w, codes, ld, offs, Gm, beta,      \
+                       (__bf16 *)hi, (__bf16 *)lo, gpart.p)
+#define HDFE_GPD(JJ)                      \
+    switch (D) {                          \
+        case 0: HDFE_GP(JJ, 0); break;    \
+        case 1: HDFE_GP(JJ, 1); break;    \
+        case 2: HDFE_GP(JJ, 2); break;    \
+        case 3: HDFE_GP(JJ, 3); break;    \
+        default: HDFE_GP(JJ, 4); break;   \
+    }
+    switch ((kx + 1 + 63) / 64) {
+        case 1: HDFE_GPD(1); break;
+        case 2: HDFE_GPD(2); break;
+        case 3: HDFE_GPD(3); break;
+        case 4: HDFE_GPD(4); break;
+        case 5: HDFE_GPD(5); break;
+        case 6: HDFE_GPD(6); break;
+        case 7: HDFE_GPD(7); break;
+        default: HDFE_GPD(8); break;
+    }
+#undef HDFE_GPD
+#undef HDFE_GP
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpy(gpart_h, gpart.p, (int64_t)nwg * kx * 8, hipMemcpyDeviceToHost));
     return HDFE_STATUS_OK;

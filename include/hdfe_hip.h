@@ -542,7 +542,8 @@ int hdfe_glm_rhs(void *rows_h, int64_t G, const int *gstart, const int *g, const
  * (from Gm (L, kx + 1)); sqrt(w) X~ split into bf16 hi + lo, row-major
  * ((padded rows, Cp), model m's padded rows from pstart[m]); and per workgroup
  * (host lists: model, group range) the score X~'W (z~ - X~ beta_m), gpart
- * (host, nwg x kx). */
+ * (host, nwg x kx).  At most four fixed-effect dimensions besides the
+ * groups (D <= 4). */
 int hdfe_glm_partial(int nwg, const int *wg_model, const int64_t *wg_gb, const int64_t *wg_ge,
                      const int *gstart, const int64_t *rs_h, const int64_t *pstart_h, int nmodels,
                      int Cp, const double *z, const float *X, int kx, const double *w,
