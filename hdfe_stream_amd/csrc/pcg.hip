@@ -1626,9 +1626,22 @@ constexpr int kMfwLanes = 512;      // k_mfw_groups' workgroup at most (a column
 constexpr int kMfwMaxDense = 2;     // dense dimensions summed per range (more: as sparse)
 constexpr int kMfwStageBytes = kMfwStage * (4 * (kMfMaxD + kMfwMaxDense) + 8);
 
+// The matrix-free kernels take the cells' dimension count DT as a template
+// parameter (0: the runtime D, up to kMfMaxD): with the loops over the
+// dimensions unrolled and every gather's address valid (cells and columns
+// past the end clamped to the last, their terms weighted 0), a lane's loads
+// carry no branches and fly together -- with a runtime loop and guarded loads
+// the compiler waited on each gather in turn (k_mfw_groups 22.8 ms per FP32
+// product at C5).
+template <int DT>
+constexpr int mfw_dmax() {
+    return DT > 0 ? DT : kMfMaxD;
+}
+
 // Stages cells [p0, p0 + np) of range r: their levels, dense slots and
 // weights, and the ends of groups [g, g2) relative to p0 (every thread of
 // the workgroup calls it).
+template <int DT>
 __device__ __forceinline__ void mfw_stage_cells(int p0, int np, const int *__restrict__ icodes,
                                                 int64_t ld, int D, const int64_t *__restrict__ offs,
                                                 int nds, int dd0, int dd1, int r,
@@ -1639,36 +1652,50 @@ __device__ __forceinline__ void mfw_stage_cells(int p0, int np, const int *__res
                                                 int (*s_sl)[kMfwStage], double *s_w,
                                                 const int *__restrict__ igstart, int g, int g2,
                                                 int *s_gend) {
+    constexpr int DM = mfw_dmax<DT>();
     __syncthreads();
     for (int q = threadIdx.x; q < g2 - g; q += blockDim.x) s_gend[q] = igstart[g + 1 + q] - p0;
     for (int i = threadIdx.x; i < np; i += blockDim.x) {
         const int c = p0 + i;
-        for (int d = 0; d < D; ++d) s_lv[d][i] = (int)(offs[d] + icodes[d * ld + c]);
-        if (nds > 0) s_sl[0][i] = rsoff[r * nds] + icodes[dd0 * ld + c] - rbase[r * nds];
-        if (nds > 1)
-            s_sl[1][i] = rsoff[r * nds + 1] + icodes[dd1 * ld + c] - rbase[r * nds + 1];
-        s_w[i] = cn[c];
+        int code[DM];
+#pragma unroll
+        for (int d = 0; d < DM; ++d) code[d] = d < D ? icodes[d * ld + c] : 0;
+        const double wc = cn[c];
+        int c0 = 0, c1 = 0;
+#pragma unroll
+        for (int d = 0; d < DM; ++d) {
+            if (d < D) s_lv[d][i] = (int)(offs[d] + code[d]);
+            c0 = d == dd0 ? code[d] : c0;
+            c1 = d == dd1 ? code[d] : c1;
+        }
+        if (nds > 0) s_sl[0][i] = rsoff[r * nds] + c0 - rbase[r * nds];
+        if (nds > 1) s_sl[1][i] = rsoff[r * nds + 1] + c1 - rbase[r * nds + 1];
+        s_w[i] = wc;
     }
     __syncthreads();
 }
 
 // Staged cells [a, e): N += n_c, S += n_c t_c and n_c t_c into each cell's
-// dense slots, kMfwIlp cells' gathers in flight (column j, if on).
-template <typename T>
-__device__ __forceinline__ void mfw_add_cells(int a, int e, bool on, int j, int k, int D, int nds,
+// dense slots, kMfwIlp cells' gathers in flight (column jc, the lane's column
+// clamped into the row).
+template <typename T, int DT>
+__device__ __forceinline__ void mfw_add_cells(int a, int e, int jc, int k, int D, int nds,
                                               const T *__restrict__ P, int (*s_lv)[kMfwStage],
                                               int (*s_sl)[kMfwStage], const double *s_w,
                                               double *slot, int bd, double &N, double &S) {
+    constexpr int DM = mfw_dmax<DT>();
     for (int i0 = a; i0 < e; i0 += kMfwIlp) {
         const int nu = min(kMfwIlp, e - i0);
         double t[kMfwIlp];
 #pragma unroll
-        for (int u = 0; u < kMfwIlp; ++u) t[u] = 0.0;
-        if (on)
-            for (int d = 0; d < D; ++d)
+        for (int u = 0; u < kMfwIlp; ++u) {
+            const int iu = min(i0 + u, e - 1);
+            double x = 0.0;
 #pragma unroll
-                for (int u = 0; u < kMfwIlp; ++u)
-                    if (u < nu) t[u] += (double)P[(int64_t)s_lv[d][i0 + u] * k + j];
+            for (int d = 0; d < DM; ++d)
+                if (DT > 0 || d < D) x += (double)P[(int64_t)s_lv[d][iu] * k + jc];
+            t[u] = x;
+        }
 #pragma unroll
         for (int u = 0; u < kMfwIlp; ++u)
             if (u < nu) {
@@ -1686,7 +1713,7 @@ __device__ __forceinline__ void mfw_add_cells(int a, int e, bool on, int j, int 
 // groups are staged whole, as many as fit kMfwStage cells (a larger group
 // alone, in pieces).  A cell adds n_c t_c to its dense slots as it is read;
 // once the group's mean is known, each of its cells takes n_c m_g back out.
-template <typename T>
+template <typename T, int DT>
 __global__ __launch_bounds__(kMfwLanes) void k_mfw_groups(const int *__restrict__ rng, const int *__restrict__ rbase,
                              const int *__restrict__ rsoff, const int *__restrict__ dpoff,
                              const int *__restrict__ igstart, const int *__restrict__ icodes,
@@ -1702,9 +1729,11 @@ __global__ __launch_bounds__(kMfwLanes) void k_mfw_groups(const int *__restrict_
     const int r = blockIdx.x, tid = threadIdx.x, bd = blockDim.x;
     const int gb = rng[r], ge = rng[r + 1];
     const int nslot = dpoff[r + 1] - dpoff[r];
+    constexpr int DM = mfw_dmax<DT>();
     for (int j0 = 0; j0 < k; j0 += bd) {
         const int j = j0 + tid;
         const bool on = j < k;
+        const int jc = min(j, k - 1);
         for (int q = 0; q < nslot; ++q) slot[q * bd + tid] = 0.0;
         for (int g = gb; g < ge;) {
             // the whole groups [g, g2) that fit the stage (at least group g)
@@ -1718,22 +1747,22 @@ __global__ __launch_bounds__(kMfwLanes) void k_mfw_groups(const int *__restrict_
                 double N = 0.0, S = 0.0;
                 for (int p0 = c0; p0 < cend; p0 += kMfwStage) {
                     const int np = min(kMfwStage, cend - p0);
-                    mfw_stage_cells(p0, np, icodes, ld, D, offs, nds, dd0, dd1, r, rbase, rsoff, cn,
+                    mfw_stage_cells<DT>(p0, np, icodes, ld, D, offs, nds, dd0, dd1, r, rbase, rsoff, cn,
                                     s_lv, s_sl, s_w, igstart, g, g + 1, s_gend);
-                    mfw_add_cells(0, np, on, j, k, D, nds, P, s_lv, s_sl, s_w, slot, bd, N, S);
+                    mfw_add_cells<T, DT>(0, np, jc, k, D, nds, P, s_lv, s_sl, s_w, slot, bd, N, S);
                 }
                 const double m = N > 0.0 ? S / N : 0.0;
                 if (on) mg[(int64_t)g * k + j] = (T)m;
                 if (nds)
                     for (int p0 = c0; p0 < cend; p0 += kMfwStage) {
                         const int np = min(kMfwStage, cend - p0);
-                        mfw_stage_cells(p0, np, icodes, ld, D, offs, nds, dd0, dd1, r, rbase, rsoff,
+                        mfw_stage_cells<DT>(p0, np, icodes, ld, D, offs, nds, dd0, dd1, r, rbase, rsoff,
                                         cn, s_lv, s_sl, s_w, igstart, g, g + 1, s_gend);
                         for (int i = 0; i < np; ++i)
                             for (int q = 0; q < nds; ++q) slot[s_sl[q][i] * bd + tid] -= s_w[i] * m;
                     }
             } else {
-                mfw_stage_cells(c0, cend - c0, icodes, ld, D, offs, nds, dd0, dd1, r, rbase, rsoff, cn,
+                mfw_stage_cells<DT>(c0, cend - c0, icodes, ld, D, offs, nds, dd0, dd1, r, rbase, rsoff, cn,
                                 s_lv, s_sl, s_w, igstart, g, g2, s_gend);
                 // every staged cell in order, kMfwIlpGroups cells' gathers in flight
                 // across group ends; a group is finished when its end passes
@@ -1755,12 +1784,14 @@ __global__ __launch_bounds__(kMfwLanes) void k_mfw_groups(const int *__restrict_
                     const int nu = min(kMfwIlpGroups, ncell - i0);
                     double t[kMfwIlpGroups];
 #pragma unroll
-                    for (int u = 0; u < kMfwIlpGroups; ++u) t[u] = 0.0;
-                    if (on)
-                        for (int d = 0; d < D; ++d)
+                    for (int u = 0; u < kMfwIlpGroups; ++u) {
+                        const int iu = min(i0 + u, ncell - 1);
+                        double x = 0.0;
 #pragma unroll
-                            for (int u = 0; u < kMfwIlpGroups; ++u)
-                                if (u < nu) t[u] += (double)P[(int64_t)s_lv[d][i0 + u] * k + j];
+                        for (int d = 0; d < DM; ++d)
+                            if (DT > 0 || d < D) x += (double)P[(int64_t)s_lv[d][iu] * k + jc];
+                        t[u] = x;
+                    }
 #pragma unroll
                     for (int u = 0; u < kMfwIlpGroups; ++u)
                         if (u < nu) {
@@ -1798,7 +1829,7 @@ struct Vec4 {
 // consecutive columns (one vector load per row it gathers; the last lane
 // of a row whose width is not a multiple of four loads its columns one by
 // one).
-template <typename T>
+template <typename T, int DT>
 __global__ void k_mfw_levels(int64_t nsl, const int *__restrict__ slev,
                              const int *__restrict__ lptr, const int *__restrict__ lcell,
                              const int *__restrict__ cgroup, const int *__restrict__ icodes,
@@ -1807,10 +1838,12 @@ __global__ void k_mfw_levels(int64_t nsl, const int *__restrict__ slev,
                              T *__restrict__ AP, CgState st) {
     if (st.done && *st.done) return;
     typedef typename Vec4<T>::type V4;
-    __shared__ int s_lv[kMfMaxD][kMfwStage];
+    constexpr int DM = mfw_dmax<DT>();
+    // the cells' levels in the dimensions other than the level's own
+    __shared__ int s_lv[kMfMaxD - 1][kMfwStage];
     __shared__ int s_g[kMfwStage];
     __shared__ double s_w[kMfwStage];
-    const int tid = threadIdx.x, bd = blockDim.x, D = dm.D;
+    const int tid = threadIdx.x, bd = blockDim.x, D = DT > 0 ? DT : dm.D;
     for (int64_t li = blockIdx.x; li < nsl; li += gridDim.x) {
         const int64_t l = slev[li];
         int own = 0;   // the level's own dimension: the highest offset at or below l
@@ -1827,56 +1860,58 @@ __global__ void k_mfw_levels(int64_t nsl, const int *__restrict__ slev,
                 __syncthreads();
                 for (int i = tid; i < np; i += bd) {
                     const int c = lcell[p0 + i];
-                    for (int d = 0; d < D; ++d)
-                        s_lv[d][i] = (int)(dm.off[d] + icodes[d * ld + c]);
-                    s_g[i] = cgroup[c];
-                    s_w[i] = cn[c];
+                    int code[DM];
+#pragma unroll
+                    for (int d = 0; d < DM; ++d) code[d] = d < D ? icodes[d * ld + c] : 0;
+                    const int gc = cgroup[c];
+                    const double wc = cn[c];
+#pragma unroll
+                    for (int d = 0; d < DM; ++d)
+                        if (d < D && d != own)
+                            s_lv[d < own ? d : d - 1][i] = (int)(dm.off[d] + code[d]);
+                    s_g[i] = gc;
+                    s_w[i] = wc;
                 }
                 __syncthreads();
                 for (int i = 0; i < np; ++i) N += s_w[i];
                 if (!nj) continue;
                 for (int i0 = 0; i0 < np; i0 += kMfwIlp) {
-                    const int nu = min(kMfwIlp, np - i0);
                     double t[kMfwIlp][4];
                     if (nj == 4) {
 #pragma unroll
                         for (int u = 0; u < kMfwIlp; ++u) {
-                            const V4 m4 = u < nu ? *(const V4 *)(mg + (int64_t)s_g[i0 + u] * k + j)
-                                                 : V4{0, 0, 0, 0};
+                            const int iu = min(i0 + u, np - 1);
+                            const V4 m4 = *(const V4 *)(mg + (int64_t)s_g[iu] * k + j);
 #pragma unroll
                             for (int c = 0; c < 4; ++c) t[u][c] = -(double)m4[c];
-                        }
-                        for (int d = 0; d < D; ++d) {
-                            if (d == own) continue;
 #pragma unroll
-                            for (int u = 0; u < kMfwIlp; ++u)
-                                if (u < nu) {
-                                    const V4 p4 = *(const V4 *)(P + (int64_t)s_lv[d][i0 + u] * k + j);
+                            for (int e = 0; e < DM - 1; ++e)
+                                if (DT > 0 || e < D - 1) {
+                                    const V4 p4 = *(const V4 *)(P + (int64_t)s_lv[e][iu] * k + j);
 #pragma unroll
                                     for (int c = 0; c < 4; ++c) t[u][c] += (double)p4[c];
                                 }
                         }
                     } else {
+                        // the row's last lane: its columns one by one, clamped
 #pragma unroll
-                        for (int u = 0; u < kMfwIlp; ++u)
+                        for (int u = 0; u < kMfwIlp; ++u) {
+                            const int iu = min(i0 + u, np - 1);
 #pragma unroll
-                            for (int c = 0; c < 4; ++c)
-                                t[u][c] = u < nu && c < nj
-                                              ? -(double)mg[(int64_t)s_g[i0 + u] * k + j + c]
-                                              : 0.0;
-                        for (int d = 0; d < D; ++d) {
-                            if (d == own) continue;
+                            for (int c = 0; c < 4; ++c) {
+                                const int jj = j + min(c, nj - 1);
+                                double x = -(double)mg[(int64_t)s_g[iu] * k + jj];
 #pragma unroll
-                            for (int u = 0; u < kMfwIlp; ++u)
-#pragma unroll
-                                for (int c = 0; c < 4; ++c)
-                                    if (u < nu && c < nj)
-                                        t[u][c] += (double)P[(int64_t)s_lv[d][i0 + u] * k + j + c];
+                                for (int e = 0; e < DM - 1; ++e)
+                                    if (DT > 0 || e < D - 1)
+                                        x += (double)P[(int64_t)s_lv[e][iu] * k + jj];
+                                t[u][c] = x;
+                            }
                         }
                     }
 #pragma unroll
                     for (int u = 0; u < kMfwIlp; ++u)
-                        if (u < nu)
+                        if (i0 + u < np)
 #pragma unroll
                             for (int c = 0; c < 4; ++c) out[c] += s_w[i0 + u] * t[u][c];
                 }
@@ -2400,18 +2435,29 @@ static void launch_spmm(Matrix &M, const T *val, const T *P, T *AP, double *part
         // k_mfw_groups' lanes: its slots and stage within 64 KB of LDS
         int bg = std::min(bd, kMfwLanes);
         while (bg > kWave && (int64_t)F.maxslot * bg * 8 + kMfwStageBytes > 65536) bg -= kWave;
-        if (F.nr)
-            hipLaunchKernelGGL((k_mfw_groups<T>), dim3(F.nr), dim3(bg),
-                               (size_t)F.maxslot * bg * sizeof(double), s, F.rng, F.rbase,
-                               F.rsoff, F.dpoff, F.igstart, F.icodes, F.ld, F.dm.D, F.offs_d,
-                               F.dd.n, F.dd.d[0], F.dd.d[1], F.cn, P, k, mg, F.dpart, st);
-        if (F.nsl) {
-            // four columns a lane
-            const int bl = (int)std::min<int64_t>(1024, ((k + 3) / 4 + kWave - 1) / kWave * kWave);
-            hipLaunchKernelGGL((k_mfw_levels<T>), dim3((int)std::min<int64_t>(F.nsl, 16384)),
-                               dim3(bl), 0, s, F.nsl, F.slev, F.lptr, F.lcell, F.cgroup, F.icodes,
-                               F.ld, F.dm, F.cn, P, (const T *)mg, k, AP, st);
+        // the cells' dimension count as a template parameter up to four
+#define HDFE_MFW(DD)                                                                               \
+    do {                                                                                           \
+        if (F.nr)                                                                                  \
+            hipLaunchKernelGGL((k_mfw_groups<T, DD>), dim3(F.nr), dim3(bg),                        \
+                               (size_t)F.maxslot * bg * sizeof(double), s, F.rng, F.rbase,         \
+                               F.rsoff, F.dpoff, F.igstart, F.icodes, F.ld, F.dm.D, F.offs_d,      \
+                               F.dd.n, F.dd.d[0], F.dd.d[1], F.cn, P, k, mg, F.dpart, st);         \
+        if (F.nsl)                                                                                 \
+            hipLaunchKernelGGL((k_mfw_levels<T, DD>), dim3((int)std::min<int64_t>(F.nsl, 16384)),  \
+                               dim3(bl), 0, s, F.nsl, F.slev, F.lptr, F.lcell, F.cgroup, F.icodes, \
+                               F.ld, F.dm, F.cn, P, (const T *)mg, k, AP, st);                     \
+    } while (0)
+        // k_mfw_levels: four columns a lane
+        const int bl = (int)std::min<int64_t>(1024, ((k + 3) / 4 + kWave - 1) / kWave * kWave);
+        switch (F.dm.D) {
+            case 1: HDFE_MFW(1); break;
+            case 2: HDFE_MFW(2); break;
+            case 3: HDFE_MFW(3); break;
+            case 4: HDFE_MFW(4); break;
+            default: HDFE_MFW(0); break;
         }
+#undef HDFE_MFW
         if (F.ndl)
             hipLaunchKernelGGL((k_mfw_dense<T>), dim3((int)std::min<int64_t>(F.ndl, 4096)), dim3(bd),
                                0, s, F.ndl, F.dlev, F.dcptr, F.dcrow, F.dpart, k, AP, st);
