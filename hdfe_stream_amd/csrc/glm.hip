@@ -170,25 +170,57 @@ __global__ __launch_bounds__(256) void k_glm_lgamma(const int64_t *__restrict__ 
 
 // ----------------------------------------------- right-hand sides [z, X]
 
+// The row kernels below walk a group's or a level chunk's rows eight at a
+// time with every load's address valid (rows past the end repeat the last,
+// weight 0; columns past C and column 0's X read clamped ones), so the rows'
+// loads carry no branches and fly together; with per-row guarded loads the
+// compiler waited on each row's loads in turn.  Sums keep the rows' order.
+constexpr int kGlmBatch = 8;
+
 // Weighted group means of v = [z, x] (G, C): a wave per group, lanes over
-// the C = kx + 1 columns.
+// the C = kx + 1 columns (J a lane).
+template <int J>
 __global__ __launch_bounds__(256) void k_glm_group_means(
     int64_t G, const int *__restrict__ gstart, const double *__restrict__ z,
     const float *__restrict__ X, int kx, const double *__restrict__ w,
     double *__restrict__ vbar) {
     const int C = kx + 1;
     const int lane = threadIdx.x & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int64_t waves = (int64_t)gridDim.x * 4;
-    for (int64_t gg = blockIdx.x * 4 + (threadIdx.x >> 6); gg < G; gg += waves) {
+    for (int64_t gg = blockIdx.x * 4 + wv; gg < G; gg += waves) {
         const int s = gstart[gg], e = gstart[gg + 1];
-        double W = 0.0;
-        for (int i = s; i < e; ++i) W += w[i];
+        double W = 0.0, acc[J];
+#pragma unroll
+        for (int t = 0; t < J; ++t) acc[t] = 0.0;
+        for (int i0 = s; i0 < e; i0 += kGlmBatch) {
+            double wr[kGlmBatch], v[kGlmBatch][J];
+#pragma unroll
+            for (int u = 0; u < kGlmBatch; ++u) {
+                const int i = min(i0 + u, e - 1);
+                const double wi = w[i];
+                wr[u] = i0 + u < e ? wi : 0.0;
+                const double zi = z[i];
+                const float *xr = X + (int64_t)i * kx;
+#pragma unroll
+                for (int t = 0; t < J; ++t) {
+                    const int j = lane + 64 * t;
+                    const float xv = xr[max(min(j, C - 1) - 1, 0)];
+                    v[u][t] = j == 0 ? zi : (double)xv;
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < kGlmBatch; ++u) {
+                W += wr[u];
+#pragma unroll
+                for (int t = 0; t < J; ++t) acc[t] += wr[u] * v[u][t];
+            }
+        }
         const double inv = W > 0.0 ? 1.0 / W : 0.0;
-        for (int j = lane; j < C; j += 64) {
-            double acc = 0.0;
-            for (int i = s; i < e; ++i)
-                acc += w[i] * (j == 0 ? z[i] : (double)X[(int64_t)i * kx + j - 1]);
-            vbar[gg * C + j] = acc * inv;
+#pragma unroll
+        for (int t = 0; t < J; ++t) {
+            const int j = lane + 64 * t;
+            if (j < C) vbar[gg * C + j] = acc[t] * inv;
         }
     }
 }
@@ -196,6 +228,7 @@ __global__ __launch_bounds__(256) void k_glm_group_means(
 // part[chunk, j] = sum over the chunk's rows i (of one level, the level-to-
 // rows CSR of rows.hip) of w_i (v_ij - vbar[g_i, j]): D_o' W M_0 v
 // (kernels_inverse.py:34-71 applied to the data), a wave per chunk.
+template <int J>
 __global__ __launch_bounds__(256) void k_glm_center(
     int64_t nchunk, const int *__restrict__ chunk_lvl, const int *__restrict__ chunk_beg,
     const int *__restrict__ rptr, const int *__restrict__ rrow, const int *__restrict__ g,
@@ -203,18 +236,41 @@ __global__ __launch_bounds__(256) void k_glm_center(
     const double *__restrict__ w, const double *__restrict__ vbar, double *__restrict__ part) {
     const int C = kx + 1;
     const int lane = threadIdx.x & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int64_t waves = (int64_t)gridDim.x * 4;
-    for (int64_t c = blockIdx.x * 4 + (threadIdx.x >> 6); c < nchunk; c += waves) {
+    for (int64_t c = blockIdx.x * 4 + wv; c < nchunk; c += waves) {
         const int lv = chunk_lvl[c];
         const int b = chunk_beg[c], e = min(b + kGlmRowChunk, rptr[lv + 1]);
-        for (int j = lane; j < C; j += 64) {
-            double acc = 0.0;
-            for (int p = b; p < e; ++p) {
-                const int i = rrow[p];
-                const double v = j == 0 ? z[i] : (double)X[(int64_t)i * kx + j - 1];
-                acc += w[i] * (v - vbar[(int64_t)g[i] * C + j]);
+        double acc[J];
+#pragma unroll
+        for (int t = 0; t < J; ++t) acc[t] = 0.0;
+        for (int p0 = b; p0 < e; p0 += kGlmBatch) {
+            double wr[kGlmBatch], v[kGlmBatch][J];
+#pragma unroll
+            for (int u = 0; u < kGlmBatch; ++u) {
+                const int i = rrow[min(p0 + u, e - 1)];
+                const double wi = w[i];
+                wr[u] = p0 + u < e ? wi : 0.0;
+                const double zi = z[i];
+                const double *vb = vbar + (int64_t)g[i] * C;
+                const float *xr = X + (int64_t)i * kx;
+#pragma unroll
+                for (int t = 0; t < J; ++t) {
+                    const int j = lane + 64 * t;
+                    const int jj = min(j, C - 1);
+                    const float xv = xr[max(jj - 1, 0)];
+                    v[u][t] = (j == 0 ? zi : (double)xv) - vb[jj];
+                }
             }
-            part[c * C + j] = acc;
+#pragma unroll
+            for (int u = 0; u < kGlmBatch; ++u)
+#pragma unroll
+                for (int t = 0; t < J; ++t) acc[t] += wr[u] * v[u][t];
+        }
+#pragma unroll
+        for (int t = 0; t < J; ++t) {
+            const int j = lane + 64 * t;
+            if (j < C) part[c * C + j] = acc[t];
         }
     }
 }
@@ -512,26 +568,53 @@ __global__ void k_glm_fe(int64_t L, int kx, const double *__restrict__ Gm,
 }
 
 // a[g] = the weighted group mean of z - x'beta_m - sum_d fe (the group's own
-// effect, core/kernels_glm.py:11-33): a wave per group.
+// effect, core/kernels_glm.py:11-33): a wave per group, lanes over the kx
+// slopes (J a lane), its rows kGlmBatch at a time as in the row kernels above.
+template <int J, int D>
 __global__ __launch_bounds__(256) void k_glm_group_effects(
     int64_t G, const int *__restrict__ gstart, const int *__restrict__ gmodel,
     const double *__restrict__ z, const float *__restrict__ X, int kx,
-    const double *__restrict__ w, const int *__restrict__ codes, int64_t ld, int D,
+    const double *__restrict__ w, const int *__restrict__ codes, int64_t ld,
     const int64_t *__restrict__ offs, const double *__restrict__ fe,
     const double *__restrict__ beta, double *__restrict__ a) {
     const int lane = threadIdx.x & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int64_t waves = (int64_t)gridDim.x * 4;
-    for (int64_t gg = blockIdx.x * 4 + (threadIdx.x >> 6); gg < G; gg += waves) {
+    for (int64_t gg = blockIdx.x * 4 + wv; gg < G; gg += waves) {
         const double *bm = beta + (int64_t)gmodel[gg] * kx;
+        double bl[J > 0 ? J : 1];
+#pragma unroll
+        for (int t = 0; t < J; ++t) {
+            const int j = lane + 64 * t;
+            bl[t] = j < kx ? bm[min(j, kx - 1)] : 0.0;
+        }
+        const int s = gstart[gg], e = gstart[gg + 1];
         double W = 0.0, S = 0.0;
-        for (int i = gstart[gg]; i < gstart[gg + 1]; ++i) {
-            double s = 0.0;
-            for (int j = lane; j < kx; j += 64) s += (double)X[(int64_t)i * kx + j] * bm[j];
-            s = wave_sum(s);
-            double u = z[i] - s;
-            for (int d = 0; d < D; ++d) u -= fe[offs[d] + codes[d * ld + i]];
-            W += w[i];
-            S += w[i] * u;
+        for (int i0 = s; i0 < e; i0 += kGlmBatch) {
+            double sr[kGlmBatch], ur[kGlmBatch], wr[kGlmBatch];
+#pragma unroll
+            for (int u = 0; u < kGlmBatch; ++u) {
+                const int i = min(i0 + u, e - 1);
+                const float *xr = X + (int64_t)i * kx;
+                double x = 0.0;
+#pragma unroll
+                for (int t = 0; t < J; ++t) x += (double)xr[min(lane + 64 * t, kx - 1)] * bl[t];
+                sr[u] = x;
+                double v = z[i];
+#pragma unroll
+                for (int d = 0; d < D; ++d) v -= fe[offs[d] + codes[d * ld + i]];
+                ur[u] = v;
+                const double wi = w[i];
+                wr[u] = i0 + u < e ? wi : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < kGlmBatch; ++u) {
+                const double xb = wave_sum(sr[u]);
+                if (i0 + u < e) {
+                    W += wr[u];
+                    S += wr[u] * (ur[u] - xb);
+                }
+            }
         }
         if (lane == 0) a[gg] = W > 0.0 ? S / W : 0.0;
     }
@@ -602,12 +685,27 @@ int hdfe_glm_rhs(void *rows_h, int64_t G, const int *gstart, const int *g, const
     int64_t nchunk;
     int st = hdfe_rows_csr(rows_h, &rptr, &rrow, &cptr, &clvl, &cbeg, &nchunk);
     if (st) return st;
-    hipLaunchKernelGGL(k_glm_group_means, dim3(grid1(G * 64)), dim3(256), 0, 0, G, gstart, z, X,
-                       kx, w, vbar);
     DevBuf<double> part;
     DEV_ALLOC(part, nchunk * (int64_t)C, nullptr);
-    hipLaunchKernelGGL(k_glm_center, dim3(grid1(nchunk * 64)), dim3(256), 0, 0, nchunk, clvl,
-                       cbeg, rptr, rrow, g, z, X, kx, w, vbar, part.p);
+    // without covariates the kernels' clamped X loads read z's first bytes
+    // (their values unused)
+    const float *Xs = kx > 0 ? X : (const float *)z;
+#define HDFE_RHS(JJ)                                                                              \
+    hipLaunchKernelGGL((k_glm_group_means<JJ>), dim3(grid1(G * 64)), dim3(256), 0, 0, G, gstart, \
+                       z, Xs, kx, w, vbar);                                                       \
+    hipLaunchKernelGGL((k_glm_center<JJ>), dim3(grid1(nchunk * 64)), dim3(256), 0, 0, nchunk,    \
+                       clvl, cbeg, rptr, rrow, g, z, Xs, kx, w, vbar, part.p)
+    switch ((C + 63) / 64) {
+        case 1: HDFE_RHS(1); break;
+        case 2: HDFE_RHS(2); break;
+        case 3: HDFE_RHS(3); break;
+        case 4: HDFE_RHS(4); break;
+        case 5: HDFE_RHS(5); break;
+        case 6: HDFE_RHS(6); break;
+        case 7: HDFE_RHS(7); break;
+        default: HDFE_RHS(8); break;
+    }
+#undef HDFE_RHS
     HIP_TRY(hipGetLastError());
     st = hdfe_rows_chunk_sum(rows_h, C, part.p, rhs);
     if (st) return st;
@@ -724,9 +822,32 @@ int hdfe_glm_coefficients(int64_t L, int64_t G, int kx, const double *Gm, const 
                           const int *gmodel, const int *gstart, const double *z, const float *X,
                           const double *w, const int *codes, int64_t ld, int D,
                           const int64_t *offs, const double *beta, double *fe, double *a) {
+    ARG_CHECK(kx >= 0 && kx <= 512 && D >= 0 && D <= 4, "hdfe_glm_coefficients: bad argument");
     hipLaunchKernelGGL(k_glm_fe, dim3(grid1(L)), dim3(256), 0, 0, L, kx, Gm, lmodel, beta, fe);
-    hipLaunchKernelGGL(k_glm_group_effects, dim3(grid1(G * 64)), dim3(256), 0, 0, G, gstart,
-                       gmodel, z, X, kx, w, codes, ld, D, offs, fe, beta, a);
+#define HDFE_GE(JJ, DD)                                                                       \
+    hipLaunchKernelGGL((k_glm_group_effects<JJ, DD>), dim3(grid1(G * 64)), dim3(256), 0, 0, G, \
+                       gstart, gmodel, z, X, kx, w, codes, ld, offs, fe, beta, a)
+#define HDFE_GED(JJ)                    \
+    switch (D) {                        \
+        case 0: HDFE_GE(JJ, 0); break;  \
+        case 1: HDFE_GE(JJ, 1); break;  \
+        case 2: HDFE_GE(JJ, 2); break;  \
+        case 3: HDFE_GE(JJ, 3); break;  \
+        default: HDFE_GE(JJ, 4); break; \
+    }
+    switch ((kx + 63) / 64) {
+        case 0: HDFE_GED(0); break;   // no slopes: X is not read
+        case 1: HDFE_GED(1); break;
+        case 2: HDFE_GED(2); break;
+        case 3: HDFE_GED(3); break;
+        case 4: HDFE_GED(4); break;
+        case 5: HDFE_GED(5); break;
+        case 6: HDFE_GED(6); break;
+        case 7: HDFE_GED(7); break;
+        default: HDFE_GED(8); break;
+    }
+#undef HDFE_GED
+#undef HDFE_GE
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipDeviceSynchronize());
     return HDFE_STATUS_OK;
