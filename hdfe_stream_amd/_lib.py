@@ -35,6 +35,7 @@ SIGNATURES = [
     ("hdfe_pcg_solve", I32, [P, I64, P, P, P, D, I64, I32, D, P, P, P, P, P, I32, P]),
     ("hdfe_csr_matmat", I32, [P, I64, P, P, I32]),
     ("hdfe_stream_create", I32, [I64, I64, I64, I32, P, P, I64, P, P, P, P]),
+    ("hdfe_stream_rebind", I32, [P, I64, I64, I64, I32, P, P, I64, P, P, P]),
     ("hdfe_csr_matmat_f32", I32, [P, I64, P, P, I32]),
     ("hdfe_csr_export", I32, [P, P, P, P, P]),
     ("hdfe_col_norms", I32, [P, I64, I64, P]),

@@ -1580,6 +1580,35 @@ __global__ void k_mf_pairs(int64_t Mc, const int *__restrict__ icodes, int64_t l
 // rows over the ranges that cover it, in range order.  Every sum runs in a
 // fixed order in FP64: results do not depend on scheduling.
 
+__global__ void k_fill_u64(int64_t n, unsigned long long v, unsigned long long *__restrict__ x) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        x[i] = v;
+}
+
+// An order-independent 64-bit fingerprint of int arrays: the sum over
+// elements of a mix of (position, value) -- integer sums, so the same arrays
+// give the same fingerprint whatever the schedule.  Row r of `rows` rows
+// starts at a + r * ld.
+__global__ void k_fingerprint(int64_t n, int rows, int64_t ld, const int *__restrict__ a,
+                              unsigned long long *__restrict__ out) {
+    unsigned long long h = 0;
+    for (int r = 0; r < rows; ++r)
+        for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+             i += (int64_t)gridDim.x * blockDim.x) {
+            unsigned long long x = ((unsigned long long)(r * n + i) << 32) ^
+                                   (unsigned int)a[r * ld + i];
+            x ^= x >> 31;
+            x *= 0x7fb5d329728ea185ull;
+            x ^= x >> 27;
+            x *= 0x81dadef4bc2dd44dull;
+            x ^= x >> 33;
+            h += x;
+        }
+    for (int o = 32; o > 0; o >>= 1) h += __shfl_xor(h, o, 64);
+    if ((threadIdx.x & 63) == 0) atomicAdd(out, h);
+}
+
 struct MfList {
     int n = 0;
     int d[kMfMaxD] = {0};
@@ -1614,14 +1643,23 @@ struct MatFree {
     int64_t *offs_d = nullptr;          // (D) the dimensions' level offsets, on the device
     void *mg = nullptr;                 // (Gi, k) group means, of the current k
     double *dpart = nullptr;            // (dprows, k)
+    unsigned long long fp = 0;          // the cells' fingerprint (igstart, icodes)
 };
 
 // Cells of a range or a level are staged kMfwStage at a time in LDS (their
 // levels, weights and dense slots or groups, loaded by all lanes at once),
 // then every lane walks them in order with kMfwIlp cells' gathers in flight.
+// A lane adds into its dense slots with LDS atomics that return nothing
+// (ds_add_f64: the lane goes on without waiting; one lane's adds to a slot
+// land in program order, so the sums are the same every run).
 constexpr int kMfwStage = 256;
 constexpr int kMfwIlp = 4;
-constexpr int kMfwIlpGroups = 8;    // k_mfw_groups' cells in flight across group ends
+// k_mfw_groups' cells in flight across group ends (16 FP64 cells cost
+// occupancy: 12.4 -> 16.2 ms at C5, where FP32 went 11.8 -> 10.3)
+template <typename T>
+constexpr int mfw_ilp_groups() {
+    return sizeof(T) == 4 ? 16 : 8;
+}
 constexpr int kMfwLanes = 512;      // k_mfw_groups' workgroup at most (a column each)
 constexpr int kMfwMaxDense = 2;     // dense dimensions summed per range (more: as sparse)
 constexpr int kMfwStageBytes = kMfwStage * (4 * (kMfMaxD + kMfwMaxDense) + 8);
@@ -1702,7 +1740,8 @@ __device__ __forceinline__ void mfw_add_cells(int a, int e, int jc, int k, int D
                 const double w = s_w[i0 + u];
                 N += w;
                 S += w * t[u];
-                for (int q = 0; q < nds; ++q) slot[s_sl[q][i0 + u] * bd + threadIdx.x] += w * t[u];
+                for (int q = 0; q < nds; ++q)
+                    unsafeAtomicAdd(&slot[s_sl[q][i0 + u] * bd + threadIdx.x], w * t[u]);
             }
     }
 }
@@ -1730,6 +1769,7 @@ __global__ __launch_bounds__(kMfwLanes) void k_mfw_groups(const int *__restrict_
     const int gb = rng[r], ge = rng[r + 1];
     const int nslot = dpoff[r + 1] - dpoff[r];
     constexpr int DM = mfw_dmax<DT>();
+    constexpr int kIlp = mfw_ilp_groups<T>();
     for (int j0 = 0; j0 < k; j0 += bd) {
         const int j = j0 + tid;
         const bool on = j < k;
@@ -1759,12 +1799,13 @@ __global__ __launch_bounds__(kMfwLanes) void k_mfw_groups(const int *__restrict_
                         mfw_stage_cells<DT>(p0, np, icodes, ld, D, offs, nds, dd0, dd1, r, rbase, rsoff,
                                         cn, s_lv, s_sl, s_w, igstart, g, g + 1, s_gend);
                         for (int i = 0; i < np; ++i)
-                            for (int q = 0; q < nds; ++q) slot[s_sl[q][i] * bd + tid] -= s_w[i] * m;
+                            for (int q = 0; q < nds; ++q)
+                                unsafeAtomicAdd(&slot[s_sl[q][i] * bd + tid], -(s_w[i] * m));
                     }
             } else {
                 mfw_stage_cells<DT>(c0, cend - c0, icodes, ld, D, offs, nds, dd0, dd1, r, rbase, rsoff, cn,
                                 s_lv, s_sl, s_w, igstart, g, g2, s_gend);
-                // every staged cell in order, kMfwIlpGroups cells' gathers in flight
+                // every staged cell in order, kIlp cells' gathers in flight
                 // across group ends; a group is finished when its end passes
                 const int ncell = cend - c0;
                 int gi = g, gbeg = 0, gend = s_gend[0];
@@ -1773,18 +1814,19 @@ __global__ __launch_bounds__(kMfwLanes) void k_mfw_groups(const int *__restrict_
                     const double m = N > 0.0 ? S / N : 0.0;
                     if (on) mg[(int64_t)gi * k + j] = (T)m;
                     for (int i = gbeg; i < gend; ++i)
-                        for (int q = 0; q < nds; ++q) slot[s_sl[q][i] * bd + tid] -= s_w[i] * m;
+                        for (int q = 0; q < nds; ++q)
+                            unsafeAtomicAdd(&slot[s_sl[q][i] * bd + tid], -(s_w[i] * m));
                     N = 0.0;
                     S = 0.0;
                     ++gi;
                     gbeg = gend;
                     gend = gi < g2 ? s_gend[gi - g] : ncell;
                 };
-                for (int i0 = 0; i0 < ncell; i0 += kMfwIlpGroups) {
-                    const int nu = min(kMfwIlpGroups, ncell - i0);
-                    double t[kMfwIlpGroups];
+                for (int i0 = 0; i0 < ncell; i0 += kIlp) {
+                    const int nu = min(kIlp, ncell - i0);
+                    double t[kIlp];
 #pragma unroll
-                    for (int u = 0; u < kMfwIlpGroups; ++u) {
+                    for (int u = 0; u < kIlp; ++u) {
                         const int iu = min(i0 + u, ncell - 1);
                         double x = 0.0;
 #pragma unroll
@@ -1793,14 +1835,15 @@ __global__ __launch_bounds__(kMfwLanes) void k_mfw_groups(const int *__restrict_
                         t[u] = x;
                     }
 #pragma unroll
-                    for (int u = 0; u < kMfwIlpGroups; ++u)
+                    for (int u = 0; u < kIlp; ++u)
                         if (u < nu) {
                             const int i = i0 + u;
                             while (i == gend) finish();
                             const double w = s_w[i];
                             N += w;
                             S += w * t[u];
-                            for (int q = 0; q < nds; ++q) slot[s_sl[q][i] * bd + tid] += w * t[u];
+                            for (int q = 0; q < nds; ++q)
+                                unsafeAtomicAdd(&slot[s_sl[q][i] * bd + tid], w * t[u]);
                         }
                 }
                 while (gi < g2) finish();
@@ -3252,6 +3295,26 @@ static int mfw_plan(Matrix &M, MatFree &F, const std::vector<int> &cnt, hipStrea
 // A matrix-free operator over a panel's identifying cells (see k_mf_cells):
 // the level-to-cells CSR by a stable radix sort of the (level, cell) pairs
 // (each level's cells ascending), its chunks, dinv64 copied and cast.
+// The fingerprint of a cell structure: igstart (Gi + 1) and icodes (D rows).
+static int cells_fingerprint(int64_t Gi, int64_t Mc, int D, const int *igstart,
+                             const int *icodes, int64_t ld, hipStream_t s,
+                             unsigned long long *fp) {
+    unsigned long long *d = nullptr;
+    HIP_TRY(palloc(&d, 8, s));
+    // a fill kernel, not hipMemsetAsync: a caller may capture this stream
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, (int64_t)1, 0ull, d);
+    hipLaunchKernelGGL(k_fingerprint, dim3(grid_for(Gi + 1, 1024, 1, 1024)), dim3(kThreads), 0, s,
+                       Gi + 1, 1, (int64_t)0, igstart, d);
+    if (Mc)
+        hipLaunchKernelGGL(k_fingerprint, dim3(grid_for(Mc, 1024, 1, 2048)), dim3(kThreads), 0, s,
+                           Mc, D, ld, icodes, d);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(fp, d, 8, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    pfree(d, s);
+    return HDFE_STATUS_OK;
+}
+
 int matfree_create(int64_t L, int64_t Gi, int64_t Mc, int D, const int *igstart,
                    const int *icodes, int64_t ld, const int64_t *offs_h, const double *cell_n,
                    const double *dinv, void **out) {
@@ -3328,6 +3391,7 @@ int matfree_create(int64_t L, int64_t Gi, int64_t Mc, int D, const int *igstart,
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(ms));
     TRY(mfw_plan(*M, *F, cnt, ms));
+    TRY(cells_fingerprint(Gi, Mc, D, igstart, icodes, ld, ms, &F->fp));
     *out = M.release();
     return HDFE_STATUS_OK;
 }
@@ -3684,6 +3748,41 @@ int hdfe_stream_create(int64_t L, int64_t Gi, int64_t M, int D, const int *igsta
     for (int d = 0; d < D; ++d)
         ARG_CHECK(offs[d] >= 0 && offs[d] < L, "hdfe_stream_create: level offset out of range");
     return matfree_create(L, Gi, M, D, igstart, icodes, ld, offs, cell_n, dinv, out);
+}
+
+// A matrix-free operator over new cell arrays of the same structure (the
+// IRLS reweighting: the same cells, new weights): when the new igstart and
+// icodes have the operator's sizes and fingerprint, it takes the new
+// pointers, weights and inverse diagonal and keeps its plan (*rebound = 1);
+// otherwise nothing changes (*rebound = 0).
+int hdfe_stream_rebind(void *h, int64_t L, int64_t Gi, int64_t M, int D, const int *igstart,
+                       const int *icodes, int64_t ld, const double *cell_n, const double *dinv,
+                       int *rebound) {
+    ARG_CHECK(h && igstart && icodes && cell_n && dinv && rebound && ld >= M,
+              "hdfe_stream_rebind: bad argument");
+    Matrix &Mx = *(Matrix *)h;
+    ARG_CHECK(Mx.mf != nullptr, "hdfe_stream_rebind: not a matrix-free operator");
+    std::lock_guard<std::mutex> lock(Mx.mu);
+    HIP_TRY(hipSetDevice(Mx.device));
+    MatFree &F = *Mx.mf;
+    *rebound = 0;
+    if (L != Mx.L || Gi != F.Gi || M != F.Mc || D != F.dm.D) return HDFE_STATUS_OK;
+    unsigned long long fp = 0;
+    TRY(cells_fingerprint(Gi, M, D, igstart, icodes, ld, Mx.stream, &fp));
+    if (fp != F.fp) return HDFE_STATUS_OK;
+    Mx.drop_graphs();  // their kernels hold the old pointers
+    F.igstart = igstart;
+    F.icodes = icodes;
+    F.ld = ld;
+    F.cn = cell_n;
+    hipStream_t ms = Mx.stream;
+    if (L) HIP_TRY(hipMemcpyAsync(Mx.dinv64, dinv, L * 8, hipMemcpyDeviceToDevice, ms));
+    hipLaunchKernelGGL(k_cast_f64_to_f32, dim3(grid_for(std::max<int64_t>(L, 1), 1024, 1, 4096)),
+                       dim3(kThreads), 0, ms, L, Mx.dinv64, Mx.dinv32);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(ms));
+    *rebound = 1;
+    return HDFE_STATUS_OK;
 }
 
 void hdfe_destroy(void *h) {

@@ -196,10 +196,13 @@ class Panel:
             self.matrix = None
         return float(secs.value)
 
-    def stream_matrix(self):
+    def stream_matrix(self, reuse=False):
         """The matrix-free S over the identifying cells (hdfe_stream_create):
         the same solves as the explicit S, for an S too large to build (the
-        reference's stream_cg, core/solve.py:251-271).  Kept as `matrix`."""
+        reference's stream_cg, core/solve.py:251-271).  Kept as `matrix`.
+        With `reuse`, the panel's previous operator is rebound to the current
+        cells when their structure is the same (hdfe_stream_rebind: a
+        reweighting changes only the weights), keeping its plan."""
         import torch
         info = self.info
         L, Gi, Mc, D = info["L"], info["Gi"], info["M"], info["D"]
@@ -207,13 +210,23 @@ class Panel:
         self._mf_dinv = torch.where(diag > 1e-12, 1.0 / torch.where(diag > 1e-12, diag,
                                                                      torch.ones_like(diag)),
                                     torch.zeros_like(diag)).contiguous()
+        torch.cuda.synchronize()
+        prev = getattr(self, "_stream", None)
+        if reuse and prev is not None and prev._h:
+            ok = C.c_int(0)
+            _lib.check(_lib.load().hdfe_stream_rebind(
+                prev._h, L, Gi, Mc, D, self.ptr("igstart")[0], self.ptr("icodes")[0], Mc,
+                self.ptr("in")[0], self._mf_dinv.data_ptr(), C.byref(ok)), "hdfe_stream_rebind")
+            if ok.value:
+                self.matrix = prev
+                return self.matrix
         offs = np.ascontiguousarray(self.offs, np.int64)
         h = C.c_void_p()
-        torch.cuda.synchronize()
         _lib.check(_lib.load().hdfe_stream_create(
             L, Gi, Mc, D, self.ptr("igstart")[0], self.ptr("icodes")[0], Mc, offs.ctypes.data,
             self.ptr("in")[0], self._mf_dinv.data_ptr(), C.byref(h)), "hdfe_stream_create")
         self.matrix = DeviceCSR(h, L, 0, owned=True)
+        self._stream = self.matrix
         return self.matrix
 
     def components(self):

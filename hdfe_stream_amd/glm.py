@@ -449,7 +449,7 @@ def _glm_batch(fml, datasets, family, iwls_tol, iwls_maxiter, separation_check, 
         refresh_ptrs()
         if solver == "stream_cg":
             panel.system(with_rhs=False, budget=0)
-            panel.stream_matrix()
+            panel.stream_matrix(reuse=True)
         else:
             panel.system(with_rhs=False)
         if ncomp is None:

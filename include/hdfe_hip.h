@@ -76,6 +76,17 @@ int hdfe_stream_create(int64_t L, int64_t Gi, int64_t M, int D, const int *igsta
                        const int *icodes, int64_t ld, const int64_t *offs, const double *cell_n,
                        const double *dinv, void **out);
 
+/* The operator of hdfe_stream_create over new cell arrays of the same
+ * structure -- the IRLS reweighting keeps the cells and changes their
+ * weights: when igstart and icodes have the operator's sizes and content
+ * (a 64-bit fingerprint of both), it takes the new pointers, weights and
+ * inverse diagonal and keeps its plan (*rebound = 1); otherwise it is left
+ * as it was (*rebound = 0) and the caller creates a new one.  No
+ * counterpart in the reference, which rebuilds nothing for stream_cg. */
+int hdfe_stream_rebind(void *h, int64_t L, int64_t Gi, int64_t M, int D, const int *igstart,
+                       const int *icodes, int64_t ld, const double *cell_n, const double *dinv,
+                       int *rebound);
+
 /* y = S x (k columns, FP64), host or device.  The operator of
  * _nb_csr_matmat, core/kernels_base.py:361-371. */
 int hdfe_csr_matmat(void *h, int64_t k, const double *x, double *y, int on_device);
