@@ -1630,6 +1630,7 @@ struct MatFree {
     // the wide form (k > kSmallK), when a plan exists (wide)
     bool wide = false;
     MfList dd;                          // the dense dimensions
+    MfList sd;                          // the others (gathered per cell)
     int64_t nsl = 0;                    // levels of the sparse dimensions
     int *slev = nullptr;                // those levels
     int *lptr = nullptr;                // (L + 1) each level's cells in lcell
@@ -1663,6 +1664,7 @@ constexpr int mfw_ilp_groups() {
 constexpr int kMfwLanes = 512;      // k_mfw_groups' workgroup at most (a column each)
 constexpr int kMfwMaxDense = 2;     // dense dimensions summed per range (more: as sparse)
 constexpr int kMfwStageBytes = kMfwStage * (4 * (kMfMaxD + kMfwMaxDense) + 8);
+constexpr int64_t kMfwLds = 96 * 1024;   // k_mfw_groups' LDS at most (gfx950: 160 KB a workgroup)
 
 // The matrix-free kernels take the cells' dimension count DT as a template
 // parameter (0: the runtime D, up to kMfMaxD): with the loops over the
@@ -1676,13 +1678,14 @@ constexpr int mfw_dmax() {
     return DT > 0 ? DT : kMfMaxD;
 }
 
-// Stages cells [p0, p0 + np) of range r: their levels, dense slots and
-// weights, and the ends of groups [g, g2) relative to p0 (every thread of
-// the workgroup calls it).
-template <int DT>
+// Stages cells [p0, p0 + np) of range r: their levels in the sparse
+// dimensions (sd, ns of them), their dense slots and weights, and the ends of
+// groups [g, g2) relative to p0 (every thread of the workgroup calls it).
+template <int DS>
 __device__ __forceinline__ void mfw_stage_cells(int p0, int np, const int *__restrict__ icodes,
-                                                int64_t ld, int D, const int64_t *__restrict__ offs,
-                                                int nds, int dd0, int dd1, int r,
+                                                int64_t ld, const MfList &sd,
+                                                const int64_t *__restrict__ offs, int nds,
+                                                int dd0, int dd1, int r,
                                                 const int *__restrict__ rbase,
                                                 const int *__restrict__ rsoff,
                                                 const double *__restrict__ cn,
@@ -1690,22 +1693,21 @@ __device__ __forceinline__ void mfw_stage_cells(int p0, int np, const int *__res
                                                 int (*s_sl)[kMfwStage], double *s_w,
                                                 const int *__restrict__ igstart, int g, int g2,
                                                 int *s_gend) {
-    constexpr int DM = mfw_dmax<DT>();
+    constexpr int DM = mfw_dmax<DS>();
+    const int ns = DS > 0 ? DS : sd.n;
     __syncthreads();
     for (int q = threadIdx.x; q < g2 - g; q += blockDim.x) s_gend[q] = igstart[g + 1 + q] - p0;
     for (int i = threadIdx.x; i < np; i += blockDim.x) {
         const int c = p0 + i;
         int code[DM];
 #pragma unroll
-        for (int d = 0; d < DM; ++d) code[d] = d < D ? icodes[d * ld + c] : 0;
+        for (int e = 0; e < DM; ++e) code[e] = e < ns ? icodes[sd.d[e] * ld + c] : 0;
+        const int c0 = nds > 0 ? icodes[dd0 * ld + c] : 0;
+        const int c1 = nds > 1 ? icodes[dd1 * ld + c] : 0;
         const double wc = cn[c];
-        int c0 = 0, c1 = 0;
 #pragma unroll
-        for (int d = 0; d < DM; ++d) {
-            if (d < D) s_lv[d][i] = (int)(offs[d] + code[d]);
-            c0 = d == dd0 ? code[d] : c0;
-            c1 = d == dd1 ? code[d] : c1;
-        }
+        for (int e = 0; e < DM; ++e)
+            if (e < ns) s_lv[e][i] = (int)(offs[sd.d[e]] + code[e]);
         if (nds > 0) s_sl[0][i] = rsoff[r * nds] + c0 - rbase[r * nds];
         if (nds > 1) s_sl[1][i] = rsoff[r * nds + 1] + c1 - rbase[r * nds + 1];
         s_w[i] = wc;
@@ -1713,34 +1715,45 @@ __device__ __forceinline__ void mfw_stage_cells(int p0, int np, const int *__res
     __syncthreads();
 }
 
+// t_c for staged cell i and the lane's column: its sparse dimensions' rows of
+// P gathered (column jc, the lane's clamped into the row), its dense levels'
+// from the LDS copy pw (slot-major, bd lanes)
+template <typename T, int DS, int NDS>
+__device__ __forceinline__ double mfw_cell_t(int i, int jc, int k, int ns,
+                                             const T *__restrict__ P, int (*s_lv)[kMfwStage],
+                                             int (*s_sl)[kMfwStage], const T *pw, int bd) {
+    constexpr int DM = mfw_dmax<DS>();
+    double x = 0.0;
+#pragma unroll
+    for (int e = 0; e < DM; ++e)
+        if (DS > 0 || e < ns) x += (double)P[(int64_t)s_lv[e][i] * k + jc];
+#pragma unroll
+    for (int q = 0; q < NDS; ++q) x += (double)pw[s_sl[q][i] * bd + threadIdx.x];
+    return x;
+}
+
 // Staged cells [a, e): N += n_c, S += n_c t_c and n_c t_c into each cell's
-// dense slots, kMfwIlp cells' gathers in flight (column jc, the lane's column
-// clamped into the row).
-template <typename T, int DT>
-__device__ __forceinline__ void mfw_add_cells(int a, int e, int jc, int k, int D, int nds,
+// dense slots, kMfwIlp cells' gathers in flight.
+template <typename T, int DS, int NDS>
+__device__ __forceinline__ void mfw_add_cells(int a, int e, int jc, int k, int ns,
                                               const T *__restrict__ P, int (*s_lv)[kMfwStage],
                                               int (*s_sl)[kMfwStage], const double *s_w,
-                                              double *slot, int bd, double &N, double &S) {
-    constexpr int DM = mfw_dmax<DT>();
+                                              double *slot, const T *pw, int bd, double &N,
+                                              double &S) {
     for (int i0 = a; i0 < e; i0 += kMfwIlp) {
         const int nu = min(kMfwIlp, e - i0);
         double t[kMfwIlp];
 #pragma unroll
-        for (int u = 0; u < kMfwIlp; ++u) {
-            const int iu = min(i0 + u, e - 1);
-            double x = 0.0;
-#pragma unroll
-            for (int d = 0; d < DM; ++d)
-                if (DT > 0 || d < D) x += (double)P[(int64_t)s_lv[d][iu] * k + jc];
-            t[u] = x;
-        }
+        for (int u = 0; u < kMfwIlp; ++u)
+            t[u] = mfw_cell_t<T, DS, NDS>(min(i0 + u, e - 1), jc, k, ns, P, s_lv, s_sl, pw, bd);
 #pragma unroll
         for (int u = 0; u < kMfwIlp; ++u)
             if (u < nu) {
                 const double w = s_w[i0 + u];
                 N += w;
                 S += w * t[u];
-                for (int q = 0; q < nds; ++q)
+#pragma unroll
+                for (int q = 0; q < NDS; ++q)
                     unsafeAtomicAdd(&slot[s_sl[q][i0 + u] * bd + threadIdx.x], w * t[u]);
             }
     }
@@ -1752,29 +1765,46 @@ __device__ __forceinline__ void mfw_add_cells(int a, int e, int jc, int k, int D
 // groups are staged whole, as many as fit kMfwStage cells (a larger group
 // alone, in pieces).  A cell adds n_c t_c to its dense slots as it is read;
 // once the group's mean is known, each of its cells takes n_c m_g back out.
-template <typename T, int DT>
+// The range's dense levels' rows of P (its window: as many as its slots) are
+// copied to LDS (pw) at the start, so only the sparse dimensions' rows are
+// gathered per cell: with the years gathered too, every cell's 1.2 KB year
+// row went through the L2 as well, which ran 98% busy (C5).  Dynamic LDS:
+// slot (ms x bd doubles), then pw (ms x bd T).
+template <typename T, int DS, int NDS>
 __global__ __launch_bounds__(kMfwLanes) void k_mfw_groups(const int *__restrict__ rng, const int *__restrict__ rbase,
                              const int *__restrict__ rsoff, const int *__restrict__ dpoff,
                              const int *__restrict__ igstart, const int *__restrict__ icodes,
-                             int64_t ld, int D, const int64_t *__restrict__ offs, int nds, int dd0,
-                             int dd1, const double *__restrict__ cn, const T *__restrict__ P, int k,
-                             T *__restrict__ mg, double *__restrict__ dpart, CgState st) {
+                             int64_t ld, MfList sd, const int64_t *__restrict__ offs,
+                             int dd0, int dd1, int ms, const double *__restrict__ cn,
+                             const T *__restrict__ P, int k, T *__restrict__ mg,
+                             double *__restrict__ dpart, CgState st) {
     if (st.done && *st.done) return;
-    __shared__ int s_lv[kMfMaxD][kMfwStage];
+    constexpr int DM = mfw_dmax<DS>();
+    __shared__ int s_lv[DM][kMfwStage];
     __shared__ int s_sl[kMfwMaxDense][kMfwStage];
     __shared__ double s_w[kMfwStage];
     __shared__ int s_gend[kMfwStage];
-    extern __shared__ double slot[];   // (the plan's widest range, blockDim)
+    extern __shared__ double slot[];
     const int r = blockIdx.x, tid = threadIdx.x, bd = blockDim.x;
+    T *pw = (T *)(slot + (int64_t)ms * bd);
+    const int ns = DS > 0 ? DS : sd.n;
+    constexpr int nds = NDS;
     const int gb = rng[r], ge = rng[r + 1];
     const int nslot = dpoff[r + 1] - dpoff[r];
-    constexpr int DM = mfw_dmax<DT>();
     constexpr int kIlp = mfw_ilp_groups<T>();
     for (int j0 = 0; j0 < k; j0 += bd) {
         const int j = j0 + tid;
         const bool on = j < k;
         const int jc = min(j, k - 1);
         for (int q = 0; q < nslot; ++q) slot[q * bd + tid] = 0.0;
+        // the window's rows of P, each lane its column (no lane reads
+        // another's, so no barrier)
+        for (int q = 0; q < nds; ++q) {
+            const int qs = rsoff[r * nds + q];
+            const int qe = q + 1 < nds ? rsoff[r * nds + q + 1] : nslot;
+            const int64_t row0 = offs[q == 0 ? dd0 : dd1] + rbase[r * nds + q] - qs;
+            for (int sl = qs; sl < qe; ++sl) pw[sl * bd + tid] = P[(row0 + sl) * k + jc];
+        }
         for (int g = gb; g < ge;) {
             // the whole groups [g, g2) that fit the stage (at least group g)
             const int c0 = igstart[g];
@@ -1787,24 +1817,26 @@ __global__ __launch_bounds__(kMfwLanes) void k_mfw_groups(const int *__restrict_
                 double N = 0.0, S = 0.0;
                 for (int p0 = c0; p0 < cend; p0 += kMfwStage) {
                     const int np = min(kMfwStage, cend - p0);
-                    mfw_stage_cells<DT>(p0, np, icodes, ld, D, offs, nds, dd0, dd1, r, rbase, rsoff, cn,
-                                    s_lv, s_sl, s_w, igstart, g, g + 1, s_gend);
-                    mfw_add_cells<T, DT>(0, np, jc, k, D, nds, P, s_lv, s_sl, s_w, slot, bd, N, S);
+                    mfw_stage_cells<DS>(p0, np, icodes, ld, sd, offs, nds, dd0, dd1, r, rbase,
+                                        rsoff, cn, s_lv, s_sl, s_w, igstart, g, g + 1, s_gend);
+                    mfw_add_cells<T, DS, NDS>(0, np, jc, k, ns, P, s_lv, s_sl, s_w, slot, pw, bd,
+                                              N, S);
                 }
                 const double m = N > 0.0 ? S / N : 0.0;
                 if (on) mg[(int64_t)g * k + j] = (T)m;
                 if (nds)
                     for (int p0 = c0; p0 < cend; p0 += kMfwStage) {
                         const int np = min(kMfwStage, cend - p0);
-                        mfw_stage_cells<DT>(p0, np, icodes, ld, D, offs, nds, dd0, dd1, r, rbase, rsoff,
-                                        cn, s_lv, s_sl, s_w, igstart, g, g + 1, s_gend);
+                        mfw_stage_cells<DS>(p0, np, icodes, ld, sd, offs, nds, dd0, dd1, r, rbase,
+                                            rsoff, cn, s_lv, s_sl, s_w, igstart, g, g + 1,
+                                            s_gend);
                         for (int i = 0; i < np; ++i)
                             for (int q = 0; q < nds; ++q)
                                 unsafeAtomicAdd(&slot[s_sl[q][i] * bd + tid], -(s_w[i] * m));
                     }
             } else {
-                mfw_stage_cells<DT>(c0, cend - c0, icodes, ld, D, offs, nds, dd0, dd1, r, rbase, rsoff, cn,
-                                s_lv, s_sl, s_w, igstart, g, g2, s_gend);
+                mfw_stage_cells<DS>(c0, cend - c0, icodes, ld, sd, offs, nds, dd0, dd1, r, rbase,
+                                    rsoff, cn, s_lv, s_sl, s_w, igstart, g, g2, s_gend);
                 // every staged cell in order, kIlp cells' gathers in flight
                 // across group ends; a group is finished when its end passes
                 const int ncell = cend - c0;
@@ -1826,14 +1858,9 @@ __global__ __launch_bounds__(kMfwLanes) void k_mfw_groups(const int *__restrict_
                     const int nu = min(kIlp, ncell - i0);
                     double t[kIlp];
 #pragma unroll
-                    for (int u = 0; u < kIlp; ++u) {
-                        const int iu = min(i0 + u, ncell - 1);
-                        double x = 0.0;
-#pragma unroll
-                        for (int d = 0; d < DM; ++d)
-                            if (DT > 0 || d < D) x += (double)P[(int64_t)s_lv[d][iu] * k + jc];
-                        t[u] = x;
-                    }
+                    for (int u = 0; u < kIlp; ++u)
+                        t[u] = mfw_cell_t<T, DS, NDS>(min(i0 + u, ncell - 1), jc, k, ns, P, s_lv,
+                                                      s_sl, pw, bd);
 #pragma unroll
                     for (int u = 0; u < kIlp; ++u)
                         if (u < nu) {
@@ -1842,7 +1869,8 @@ __global__ __launch_bounds__(kMfwLanes) void k_mfw_groups(const int *__restrict_
                             const double w = s_w[i];
                             N += w;
                             S += w * t[u];
-                            for (int q = 0; q < nds; ++q)
+#pragma unroll
+                            for (int q = 0; q < NDS; ++q)
                                 unsafeAtomicAdd(&slot[s_sl[q][i] * bd + tid], w * t[u]);
                         }
                 }
@@ -2475,32 +2503,61 @@ static void launch_spmm(Matrix &M, const T *val, const T *P, T *AP, double *part
         MatFree &F = *M.mf;
         T *mg = (T *)F.mg;
         const int bd = (int)std::min<int64_t>(1024, (k + kWave - 1) / kWave * kWave);
-        // k_mfw_groups' lanes: its slots and stage within 64 KB of LDS
+        // k_mfw_groups' lanes: its slots (double) and the window's rows of P
+        // (T) in dynamic LDS, with the stage, within kMfwLds; the columns
+        // split evenly over the passes a workgroup makes
         int bg = std::min(bd, kMfwLanes);
-        while (bg > kWave && (int64_t)F.maxslot * bg * 8 + kMfwStageBytes > 65536) bg -= kWave;
-        // the cells' dimension count as a template parameter up to four
-#define HDFE_MFW(DD)                                                                               \
-    do {                                                                                           \
-        if (F.nr)                                                                                  \
-            hipLaunchKernelGGL((k_mfw_groups<T, DD>), dim3(F.nr), dim3(bg),                        \
-                               (size_t)F.maxslot * bg * sizeof(double), s, F.rng, F.rbase,         \
-                               F.rsoff, F.dpoff, F.igstart, F.icodes, F.ld, F.dm.D, F.offs_d,      \
-                               F.dd.n, F.dd.d[0], F.dd.d[1], F.cn, P, k, mg, F.dpart, st);         \
-        if (F.nsl)                                                                                 \
-            hipLaunchKernelGGL((k_mfw_levels<T, DD>), dim3((int)std::min<int64_t>(F.nsl, 16384)),  \
-                               dim3(bl), 0, s, F.nsl, F.slev, F.lptr, F.lcell, F.cgroup, F.icodes, \
-                               F.ld, F.dm, F.cn, P, (const T *)mg, k, AP, st);                     \
-    } while (0)
+        while (bg > kWave &&
+               (int64_t)F.maxslot * bg * (8 + sizeof(T)) + kMfwStageBytes > kMfwLds)
+            bg -= kWave;
+        {
+            const int passes = (k + bg - 1) / bg;
+            bg = (int)std::min<int64_t>(bg, ((k + passes - 1) / passes + kWave - 1) / kWave * kWave);
+        }
+        const size_t dyn = (size_t)F.maxslot * bg * (8 + sizeof(T));
         // k_mfw_levels: four columns a lane
         const int bl = (int)std::min<int64_t>(1024, ((k + 3) / 4 + kWave - 1) / kWave * kWave);
-        switch (F.dm.D) {
-            case 1: HDFE_MFW(1); break;
-            case 2: HDFE_MFW(2); break;
-            case 3: HDFE_MFW(3); break;
-            case 4: HDFE_MFW(4); break;
-            default: HDFE_MFW(0); break;
+        // the sparse dimensions' count as a template parameter up to four
+#define HDFE_MFW(SS, NN)                                                                           \
+    do {                                                                                           \
+        if (dyn > 65536)                                                                           \
+            (void)hipFuncSetAttribute((const void *)k_mfw_groups<T, SS, NN>,                       \
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn);       \
+        hipLaunchKernelGGL((k_mfw_groups<T, SS, NN>), dim3(F.nr), dim3(bg), dyn, s, F.rng, F.rbase, \
+                           F.rsoff, F.dpoff, F.igstart, F.icodes, F.ld, F.sd, F.offs_d, F.dd.d[0], \
+                           F.dd.d[1], F.maxslot, F.cn, P, k, mg, F.dpart, st);                     \
+    } while (0)
+#define HDFE_MFWN(SS)                          \
+    switch (F.dd.n) {                          \
+        case 0: HDFE_MFW(SS, 0); break;        \
+        case 1: HDFE_MFW(SS, 1); break;        \
+        default: HDFE_MFW(SS, 2); break;       \
+    }
+        // the sparse and the dense dimensions' counts as template parameters
+        if (F.nr) {
+            switch (F.sd.n) {
+                case 1: HDFE_MFWN(1); break;
+                case 2: HDFE_MFWN(2); break;
+                case 3: HDFE_MFWN(3); break;
+                default: HDFE_MFWN(0); break;
+            }
         }
+#undef HDFE_MFWN
 #undef HDFE_MFW
+#define HDFE_MFL(DD)                                                                              \
+    hipLaunchKernelGGL((k_mfw_levels<T, DD>), dim3((int)std::min<int64_t>(F.nsl, 16384)), dim3(bl), \
+                       0, s, F.nsl, F.slev, F.lptr, F.lcell, F.cgroup, F.icodes, F.ld, F.dm, F.cn,  \
+                       P, (const T *)mg, k, AP, st)
+        if (F.nsl) {
+            switch (F.dm.D) {
+                case 1: HDFE_MFL(1); break;
+                case 2: HDFE_MFL(2); break;
+                case 3: HDFE_MFL(3); break;
+                case 4: HDFE_MFL(4); break;
+                default: HDFE_MFL(0); break;
+            }
+        }
+#undef HDFE_MFL
         if (F.ndl)
             hipLaunchKernelGGL((k_mfw_dense<T>), dim3((int)std::min<int64_t>(F.ndl, 4096)), dim3(bd),
                                0, s, F.ndl, F.dlev, F.dcptr, F.dcrow, F.dpart, k, AP, st);
@@ -3189,6 +3246,9 @@ static int mfw_plan(Matrix &M, MatFree &F, const std::vector<int> &cnt, hipStrea
         }
     }
     const int nds = F.dd.n;
+    F.sd.n = 0;
+    for (int d = 0; d < D; ++d)
+        if (!dense[d]) F.sd.d[F.sd.n++] = d;
     std::vector<int> slev, dlev;
     for (int d = 0; d < D; ++d)
         for (int64_t l = dlo[d]; l < dhi[d]; ++l) (dense[d] ? dlev : slev).push_back((int)l);
@@ -3206,6 +3266,19 @@ static int mfw_plan(Matrix &M, MatFree &F, const std::vector<int> &cnt, hipStrea
     const int64_t R = std::max<int64_t>(1, (Gi + kMfwRanges - 1) / kMfwRanges);
     std::vector<int> rng{0}, rbase, rsoff, dpoff{0};
     F.maxslot = 0;
+    // a range's window is cut at the widest single group's (when the dense
+    // levels are per model, the years of C5, a range stays within a model:
+    // fewer slots, less LDS)
+    int soft = 0;
+    for (int64_t g = 0; g < Gi; ++g) {
+        int tot = 0;
+        for (int q = 0; q < nds; ++q) {
+            const int lo = win[(g * nds + q) * 2], hi = win[(g * nds + q) * 2 + 1];
+            tot += hi >= lo ? hi - lo + 1 : 0;
+        }
+        soft = std::max(soft, tot);
+    }
+    if (soft > kMfwSlots) return HDFE_STATUS_OK;   // one group alone is too wide: no wide form
     for (int64_t g = 0; g < Gi;) {
         int wl[kMfMaxD], wh[kMfMaxD];
         for (int q = 0; q < nds; ++q) {
@@ -3220,7 +3293,7 @@ static int mfw_plan(Matrix &M, MatFree &F, const std::vector<int> &cnt, hipStrea
                 nh[q] = std::max(wh[q], win[(e * nds + q) * 2 + 1]);
                 tot += nh[q] >= nl[q] ? nh[q] - nl[q] + 1 : 0;
             }
-            if (tot > kMfwSlots) break;
+            if (tot > soft) break;
             for (int q = 0; q < nds; ++q) {
                 wl[q] = nl[q];
                 wh[q] = nh[q];
@@ -3292,9 +3365,6 @@ static int mfw_plan(Matrix &M, MatFree &F, const std::vector<int> &cnt, hipStrea
     return HDFE_STATUS_OK;
 }
 
-// A matrix-free operator over a panel's identifying cells (see k_mf_cells):
-// the level-to-cells CSR by a stable radix sort of the (level, cell) pairs
-// (each level's cells ascending), its chunks, dinv64 copied and cast.
 // The fingerprint of a cell structure: igstart (Gi + 1) and icodes (D rows).
 static int cells_fingerprint(int64_t Gi, int64_t Mc, int D, const int *igstart,
                              const int *icodes, int64_t ld, hipStream_t s,
@@ -3315,6 +3385,9 @@ static int cells_fingerprint(int64_t Gi, int64_t Mc, int D, const int *igstart,
     return HDFE_STATUS_OK;
 }
 
+// A matrix-free operator over a panel's identifying cells (see k_mf_cells):
+// the level-to-cells CSR by a stable radix sort of the (level, cell) pairs
+// (each level's cells ascending), its chunks, dinv64 copied and cast.
 int matfree_create(int64_t L, int64_t Gi, int64_t Mc, int D, const int *igstart,
                    const int *icodes, int64_t ld, const int64_t *offs_h, const double *cell_n,
                    const double *dinv, void **out) {
