@@ -270,6 +270,10 @@ class Panel:
         if getattr(self, "_h", None):
             if self.matrix is not None:
                 self.matrix.close()         # the handle itself is the panel's
+            prev = getattr(self, "_stream", None)
+            if prev is not None and prev is not self.matrix:
+                prev.close()
+            self._stream = None
             _lib.load().hdfe_panel_destroy(self._h)
             self._h = None
 

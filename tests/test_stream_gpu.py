@@ -44,6 +44,49 @@ def test_stream_product_equals_the_explicit_product(c1_panel, weighted):
     P.close()
 
 
+def test_rebound_operator_equals_a_new_one(c1_panel):
+    """A reweighting keeps the cells (IRLS): hdfe_stream_rebind keeps the
+    operator's plan and takes the new cell weights, its products equal a new
+    operator's bit for bit; an operator over other cells is not rebound."""
+    import ctypes as C
+    import torch
+    from hdfe_stream_amd import _lib
+    from hdfe_stream_amd.estimator import Panel
+    d = c1_panel
+    cols = [torch.from_numpy(d[k]).cuda() for k in ("worker_id", "firm_id", "year")]
+    P = Panel(cols, [torch.from_numpy(d["y"]).cuda()],
+              weights=torch.from_numpy(np.random.default_rng(2).uniform(0.5, 2.0, len(d["y"]))).cuda())
+    P.system(with_rhs=False, budget=0)
+    A = P.stream_matrix()
+    n, L = P.info["n"], P.info["L"]
+    rng = np.random.default_rng(4)
+    V = torch.from_numpy(rng.standard_normal((n, P.info["m"]))).cuda().contiguous()
+    w = torch.from_numpy(rng.uniform(0.2, 3.0, n)).cuda()
+    torch.cuda.synchronize()
+    _lib.check(_lib.load().hdfe_panel_reweight(P._h, V.data_ptr(), w.data_ptr()),
+               "hdfe_panel_reweight")
+    P.system(with_rhs=False, budget=0)
+    print("stream rebind", flush=True)
+    B = P.stream_matrix(reuse=True)
+    assert B is A                       # the same operator, rebound
+    fresh = P.stream_matrix()            # a new one over the same cells
+    assert fresh is not A
+    for k in (3, 301):
+        X = torch.from_numpy(rng.standard_normal((L, k))).cuda()
+        np.testing.assert_array_equal(A.matmat(X).cpu().numpy(), fresh.matmat(X).cpu().numpy())
+    # other cells: the rebind leaves the operator as it was
+    Q = Panel([c[: len(c) // 2] for c in cols], [torch.from_numpy(d["y"][: len(d["y"]) // 2]).cuda()])
+    Q.system(with_rhs=False, budget=0)
+    ok = C.c_int(7)
+    _lib.check(_lib.load().hdfe_stream_rebind(
+        A._h, Q.info["L"], Q.info["Gi"], Q.info["M"], Q.info["D"], Q.ptr("igstart")[0],
+        Q.ptr("icodes")[0], Q.info["M"], Q.ptr("in")[0], P._mf_dinv.data_ptr(), C.byref(ok)),
+        "hdfe_stream_rebind")
+    assert ok.value == 0
+    Q.close()
+    P.close()
+
+
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_stream_cg_fit_equals_the_explicit_fit(c1_panel, precision):
     from hdfe_stream_amd.estimator import feols_stream
