@@ -185,7 +185,7 @@ def _model_residuals(matrix, rhs, X, lmodel, n_models):
 
 def fepois_batch(fml: str, datasets: list, iwls_tol: float = 1e-8, iwls_maxiter: int = 25,
                  separation_check: bool = True, tol: float = 1e-10, maxiter: int = 5000,
-                 precision: str = "mixed", rows_per_syrk: int = 32768,
+                 precision: str = "auto", rows_per_syrk: int = 32768,
                  timings: dict | None = None, check_residuals: bool = False,
                  weights: str | None = None, solver: str = "auto") -> list:
     """Fit `fml` by Poisson IRLS on each dataset, all on the device at once.
@@ -196,14 +196,16 @@ def fepois_batch(fml: str, datasets: list, iwls_tol: float = 1e-8, iwls_maxiter:
     every solve and reported (diagnostics["model_rel_residual"]).  `solver`:
     "explicit" builds S at every reweighting; "stream_cg" solves through the
     matrix-free S over the cells (the reference's stream_cg); "auto" takes
-    the matrix-free S for more than eight columns of [z, X]."""
+    the matrix-free S for more than eight columns of [z, X].  `precision`:
+    "fp64", "mixed" (FP32 CG with FP64 refinement) or "auto" (FP64 through
+    the matrix-free S, mixed through the explicit one)."""
     return _glm_batch(fml, datasets, "poisson", iwls_tol, iwls_maxiter, separation_check, tol,
                       maxiter, precision, rows_per_syrk, timings, check_residuals, weights,
                       solver)
 
 
 def feols_batch(fml: str, datasets: list, weights: str | None = None, tol: float = 1e-10,
-                maxiter: int = 5000, precision: str = "mixed", rows_per_syrk: int = 32768,
+                maxiter: int = 5000, precision: str = "auto", rows_per_syrk: int = 32768,
                 max_refinements: int = 6, timings: dict | None = None,
                 check_residuals: bool = False, solver: str = "auto") -> list:
     """OLS with fixed effects on many datasets at once (feols_stream,
@@ -233,6 +235,8 @@ def _glm_batch(fml, datasets, family, iwls_tol, iwls_maxiter, separation_check, 
         raise ValueError("iwls_maxiter must be at least 1")
     if solver not in ("auto", "explicit", "stream_cg"):
         raise ValueError("solver must be 'auto', 'explicit' or 'stream_cg'")
+    if precision not in ("auto", "fp64", "mixed"):
+        raise ValueError("precision must be 'auto', 'fp64' or 'mixed'")
     lib = _lib.load()
     t_start = time.perf_counter()
     dev = "cuda"
@@ -277,6 +281,12 @@ def _glm_batch(fml, datasets, family, iwls_tol, iwls_maxiter, separation_check, 
         # S (group means, no S built at each reweighting: C5's S has 133M
         # nonzeros from 18.7M cells); a narrow one through the explicit S
         solver = "stream_cg" if kx + 1 > 8 else "explicit"
+    if precision == "auto":
+        # through the wide matrix-free S an FP32 product costs 0.6 of an FP64
+        # one, and the refinements' restarts and FP64 residuals cost more than
+        # that saves (C5: 2.32 s FP64 against 2.76 s mixed); the explicit S's
+        # narrow block keeps the mixed solve
+        precision = "fp64" if solver == "stream_cg" else "mixed"
     fe_names = per[0]["fe"]
     nfe = len(fe_names)
     # separation: levels whose outcome is zero on every row, one pass over
