@@ -291,6 +291,9 @@ def glm_phase(n_models, lo, hi, n_cov):
     ds = poisson_models_device(n_models, lo, hi, n_cov, seed=5)
     rows = int(sum(len(d["y"]) for d in ds))
     fml = "y ~ X | worker_id + firm_id + year"
+    # an untimed batch of two models first: the kernels' code objects load
+    # and the library's caches fill (a process's first batch pays ~0.3 s)
+    fepois_batch(fml, ds[:2])
     tm = {}
     torch.cuda.synchronize()
     t0 = time.perf_counter()
