@@ -63,9 +63,20 @@ class VcovEngine:
     """The pieces every vcov of one fit shares: X~, the scores w e x~ and B
     (device), the fixed effects' dof."""
 
+    @property
+    def res(self):
+        r = self._res()
+        if r is None:
+            raise RuntimeError("the fit this vcov engine belongs to is gone")
+        return r
+
     def __init__(self, result, data=None):
         import torch
-        self.res = result
+        import weakref
+        # the result owns the engine: a weak reference back, so that the
+        # pair is freed (device arrays included) as soon as the result is,
+        # without waiting for the cycle collector
+        self._res = weakref.ref(result)
         panel = result.panel
         info = panel.info
         self.n, self.k = info["n"], len(result.coefnames)
