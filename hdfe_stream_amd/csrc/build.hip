@@ -2308,6 +2308,43 @@ struct ScoreAcc {
     static constexpr int NV = 1 + 2 * T;
 };
 
+// Row i's covariates col[c] less their fitted effects, x - sum_d Gm[level_d,
+// col] (0 where col[c] < 0): the row's first kRowDims dimensions' codes are
+// loaded together, then their effects -- two waits, not two per dimension
+// (a runtime loop over the dimensions waited on each code and each effect
+// in turn).
+constexpr int kRowDims = 4;
+template <int K>
+__device__ __forceinline__ void partial_row(int64_t i, int D, int m, const int *__restrict__ codes,
+                                            int64_t ld, const int64_t *__restrict__ offs,
+                                            const double *__restrict__ V,
+                                            const double *__restrict__ Gm, const int (&col)[K],
+                                            double (&u)[K]) {
+    int64_t lv[kRowDims];
+#pragma unroll
+    for (int d = 0; d < kRowDims; ++d) {
+        const int dd = min(d, D - 1);   // a valid dimension (its effect weighted 0 past D)
+        lv[d] = (offs[dd] + codes[dd * ld + i]) * m;
+    }
+    // every column's value and effects loaded before any is used
+    double x[K], g[K][kRowDims];
+#pragma unroll
+    for (int c = 0; c < K; ++c) {
+        const int cc = max(col[c], 0);
+        x[c] = V[i * m + cc];
+#pragma unroll
+        for (int d = 0; d < kRowDims; ++d) g[c][d] = Gm[lv[d] + cc];
+    }
+#pragma unroll
+    for (int c = 0; c < K; ++c) {
+        const int cc = max(col[c], 0);
+#pragma unroll
+        for (int d = 0; d < kRowDims; ++d) x[c] -= d < D ? g[c][d] : 0.0;
+        for (int d = kRowDims; d < D; ++d) x[c] -= Gm[(offs[d] + codes[d * ld + i]) * m + cc];
+        u[c] = col[c] < 0 ? 0.0 : x[c];
+    }
+}
+
 template <int W, int K>
 __global__ void __launch_bounds__(256)
     k_scores(int64_t G, int D, int m, const int *__restrict__ gstart, const int *__restrict__ codes,
@@ -2330,15 +2367,17 @@ __global__ void __launch_bounds__(256)
         double Wg = 0.0, mu[K];
 #pragma unroll
         for (int c = 0; c < K; ++c) mu[c] = 0.0;
+        // the lane's first row kept from this sweep for the next
+        double u0[K];
         for (int i = s + sub; i < en; i += W) {
             const double wi = w ? w[i] : 1.0;
             Wg += wi;
+            double u[K];
+            partial_row<K>(i, D, m, codes, ld, offs, V, Gm, col, u);
 #pragma unroll
             for (int c = 0; c < K; ++c) {
-                if (col[c] < 0) continue;
-                double u = V[(int64_t)i * m + col[c]];
-                for (int d = 0; d < D; ++d) u -= Gm[(offs[d] + codes[d * ld + i]) * m + col[c]];
-                mu[c] += wi * u;
+                mu[c] += wi * u[c];
+                if (i == s + sub) u0[c] = u[c];
             }
         }
         Wg = run_sum<W>(Wg);
@@ -2350,16 +2389,18 @@ __global__ void __launch_bounds__(256)
         for (int i = s + sub; i < en; i += W) {
             const double wi = w ? w[i] : 1.0, f = wi * e[i];
             acc[0] += f * e[i];
-            double x[K];
+            double x[K], u[K];
+            if (i == s + sub) {
+#pragma unroll
+                for (int c = 0; c < K; ++c) u[c] = u0[c];
+            } else {
+                partial_row<K>(i, D, m, codes, ld, offs, V, Gm, col, u);
+            }
 #pragma unroll
             for (int c = 0; c < K; ++c) {
-                x[c] = 0.0;
-                if (col[c] < 0) continue;
-                double u = V[(int64_t)i * m + col[c]];
-                for (int d = 0; d < D; ++d) u -= Gm[(offs[d] + codes[d * ld + i]) * m + col[c]];
-                x[c] = f * (u - mu[c]);
+                x[c] = col[c] < 0 ? 0.0 : f * (u[c] - mu[c]);
                 sg[c] += x[c];
-                if (U) U[(int64_t)i * kc + c] = x[c];
+                if (U && col[c] >= 0) U[(int64_t)i * kc + c] = x[c];
             }
             int t = 1;
 #pragma unroll
