@@ -2051,6 +2051,73 @@ __global__ void k_sub_global_mean(int64_t nl, int k, int64_t o, int nb,
 
 // --------------------------------------------------- assembly / pass 2
 
+// Row i's covariates col[c] less their fitted effects, x - sum_d Gm[level_d,
+// col] (0 where col[c] < 0): the row's first kRowDims dimensions' codes are
+// loaded together, then their effects -- two waits, not two per dimension
+// (a runtime loop over the dimensions waited on each code and each effect
+// in turn).
+constexpr int kRowDims = 3;
+template <int K>
+__device__ __forceinline__ void partial_row(int64_t i, int D, int m, const int *__restrict__ codes,
+                                            int64_t ld, const int64_t *__restrict__ offs,
+                                            const double *__restrict__ V,
+                                            const double *__restrict__ Gm, const int (&col)[K],
+                                            double (&u)[K]) {
+    int64_t lv[kRowDims];
+#pragma unroll
+    for (int d = 0; d < kRowDims; ++d) {
+        const int dd = min(d, D - 1);   // a valid dimension (its effect weighted 0 past D)
+        lv[d] = (offs[dd] + codes[dd * ld + i]) * m;
+    }
+    // every column's value and effects loaded before any is used
+    double x[K], g[K][kRowDims];
+#pragma unroll
+    for (int c = 0; c < K; ++c) {
+        const int cc = max(col[c], 0);
+        x[c] = V[i * m + cc];
+#pragma unroll
+        for (int d = 0; d < kRowDims; ++d) g[c][d] = Gm[lv[d] + cc];
+    }
+#pragma unroll
+    for (int c = 0; c < K; ++c) {
+        const int cc = max(col[c], 0);
+#pragma unroll
+        for (int d = 0; d < kRowDims; ++d) x[c] -= d < D ? g[c][d] : 0.0;
+        for (int d = kRowDims; d < D; ++d) x[c] -= Gm[(offs[d] + codes[d * ld + i]) * m + cc];
+        u[c] = col[c] < 0 ? 0.0 : x[c];
+    }
+}
+
+// v[j] -= sum_d Gm[level_d(r), j] for the MM columns of row (or cell) r,
+// its codes first and then every column's effects in flight together, as
+// partial_row.
+template <int MM>
+__device__ __forceinline__ void minus_effects(int64_t r, int D, const int *__restrict__ codes,
+                                              int64_t ld, const int64_t *__restrict__ offs,
+                                              const double *__restrict__ Gm, double (&v)[MM]) {
+    int64_t lv[kRowDims];
+#pragma unroll
+    for (int d = 0; d < kRowDims; ++d) {
+        const int dd = min(d, D - 1);
+        lv[d] = (offs[dd] + codes[dd * ld + r]) * MM;
+    }
+    double g[MM][kRowDims];
+#pragma unroll
+    for (int j = 0; j < MM; ++j)
+#pragma unroll
+        for (int d = 0; d < kRowDims; ++d) g[j][d] = Gm[lv[d] + j];
+#pragma unroll
+    for (int j = 0; j < MM; ++j)
+#pragma unroll
+        for (int d = 0; d < kRowDims; ++d) v[j] -= d < D ? g[j][d] : 0.0;
+    for (int d = kRowDims; d < D; ++d) {
+        const int64_t l = (offs[d] + codes[d * ld + r]) * MM;
+#pragma unroll
+        for (int j = 0; j < MM; ++j) v[j] -= Gm[l + j];
+    }
+}
+
+
 // V' M_D V between cells (core/kernels_base.py:510-540), m <= 8: per group
 // q_c = sums_c / n_c - sum_d Gamma[level] less the group mean.  MM is m,
 // fixed at compile time so the m x m sums stay in registers.
@@ -2074,12 +2141,14 @@ __global__ __launch_bounds__(T256) void k_assemble(
         for (int c = s + sub; c < e; c += W) {
             const double ic = in[c];
             N += ic;
+            double v[MM];
+#pragma unroll
+            for (int j = 0; j < MM; ++j) v[j] = isums[(int64_t)c * MM + j] / ic;
+            minus_effects<MM>(c, D, icodes, ild, offs, Gm, v);
 #pragma unroll
             for (int j = 0; j < MM; ++j) {
-                double v = isums[(int64_t)c * MM + j] / ic;
-                for (int d = 0; d < D; ++d) v -= Gm[(offs[d] + icodes[d * ild + c]) * MM + j];
-                if (c == s + sub) v0[j] = v;
-                qg[j] += ic * v;
+                if (c == s + sub) v0[j] = v[j];
+                qg[j] += ic * v[j];
             }
         }
         N = run_sum<W>(N);
@@ -2088,18 +2157,17 @@ __global__ __launch_bounds__(T256) void k_assemble(
         // a lane's first cell: its values kept from the pass above
         for (int c = s + sub; c < e; c += W) {
             const double ic = in[c];
-            double q[MM];
+            double q[MM], v[MM];
+            if (c == s + sub) {
 #pragma unroll
-            for (int j = 0; j < MM; ++j) {
-                double v;
-                if (c == s + sub) {
-                    v = v0[j];
-                } else {
-                    v = isums[(int64_t)c * MM + j] / ic;
-                    for (int d = 0; d < D; ++d) v -= Gm[(offs[d] + icodes[d * ild + c]) * MM + j];
-                }
-                q[j] = v - qg[j];
+                for (int j = 0; j < MM; ++j) v[j] = v0[j];
+            } else {
+#pragma unroll
+                for (int j = 0; j < MM; ++j) v[j] = isums[(int64_t)c * MM + j] / ic;
+                minus_effects<MM>(c, D, icodes, ild, offs, Gm, v);
             }
+#pragma unroll
+            for (int j = 0; j < MM; ++j) q[j] = v[j] - qg[j];
 #pragma unroll
             for (int a = 0; a < MM; ++a)
 #pragma unroll
@@ -2307,43 +2375,6 @@ struct ScoreAcc {
     static constexpr int T = K * (K + 1) / 2;
     static constexpr int NV = 1 + 2 * T;
 };
-
-// Row i's covariates col[c] less their fitted effects, x - sum_d Gm[level_d,
-// col] (0 where col[c] < 0): the row's first kRowDims dimensions' codes are
-// loaded together, then their effects -- two waits, not two per dimension
-// (a runtime loop over the dimensions waited on each code and each effect
-// in turn).
-constexpr int kRowDims = 4;
-template <int K>
-__device__ __forceinline__ void partial_row(int64_t i, int D, int m, const int *__restrict__ codes,
-                                            int64_t ld, const int64_t *__restrict__ offs,
-                                            const double *__restrict__ V,
-                                            const double *__restrict__ Gm, const int (&col)[K],
-                                            double (&u)[K]) {
-    int64_t lv[kRowDims];
-#pragma unroll
-    for (int d = 0; d < kRowDims; ++d) {
-        const int dd = min(d, D - 1);   // a valid dimension (its effect weighted 0 past D)
-        lv[d] = (offs[dd] + codes[dd * ld + i]) * m;
-    }
-    // every column's value and effects loaded before any is used
-    double x[K], g[K][kRowDims];
-#pragma unroll
-    for (int c = 0; c < K; ++c) {
-        const int cc = max(col[c], 0);
-        x[c] = V[i * m + cc];
-#pragma unroll
-        for (int d = 0; d < kRowDims; ++d) g[c][d] = Gm[lv[d] + cc];
-    }
-#pragma unroll
-    for (int c = 0; c < K; ++c) {
-        const int cc = max(col[c], 0);
-#pragma unroll
-        for (int d = 0; d < kRowDims; ++d) x[c] -= d < D ? g[c][d] : 0.0;
-        for (int d = kRowDims; d < D; ++d) x[c] -= Gm[(offs[d] + codes[d * ld + i]) * m + cc];
-        u[c] = col[c] < 0 ? 0.0 : x[c];
-    }
-}
 
 template <int W, int K>
 __global__ void __launch_bounds__(256)
