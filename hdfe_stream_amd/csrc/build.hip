@@ -998,6 +998,9 @@ __global__ void k_cell_group(int64_t Gi, const int *__restrict__ igstart, int *_
 }
 
 // Per incidence (a run of equal keys): group, level, u = sum n, s = sums.
+constexpr int kIncBatch = 8;   // a run's cells loaded together up to this many
+// MM: m at compile time for the batched short runs (0: runtime m, no batch)
+template <int MM>
 __global__ void k_inc_reduce(int64_t E, int m, int lbits, const uint64_t *__restrict__ key,
                              const int *__restrict__ cell, const int *__restrict__ istart,
                              const double *__restrict__ in, const double *__restrict__ isums,
@@ -1010,6 +1013,36 @@ __global__ void k_inc_reduce(int64_t E, int m, int lbits, const uint64_t *__rest
         ig[q] = (int)(k >> lbits);
         il[q] = (int)(k & ((1ull << lbits) - 1));
         double u = 0.0;
+        if (MM > 0 && e - s <= kIncBatch) {
+            // a short run (the usual case): its cells' indices, then their
+            // weights and sums, loaded together (cells past the run repeat
+            // its last and are not added); each sum still in run order
+            constexpr int M1 = MM > 0 ? MM : 1;
+            int64_t cl[kIncBatch];
+#pragma unroll
+            for (int b = 0; b < kIncBatch; ++b) cl[b] = cell[s + min(b, e - s - 1)];
+            double nb[kIncBatch], sb[kIncBatch][M1];
+#pragma unroll
+            for (int b = 0; b < kIncBatch; ++b) {
+                nb[b] = in[cl[b]];
+#pragma unroll
+                for (int j = 0; j < M1; ++j) sb[b][j] = isums[cl[b] * M1 + j];
+            }
+            double v[M1];
+#pragma unroll
+            for (int j = 0; j < M1; ++j) v[j] = 0.0;
+#pragma unroll
+            for (int b = 0; b < kIncBatch; ++b)
+                if (b < e - s) {
+                    u += nb[b];
+#pragma unroll
+                    for (int j = 0; j < M1; ++j) v[j] += sb[b][j];
+                }
+            iu[q] = u;
+#pragma unroll
+            for (int j = 0; j < M1; ++j) is[q * M1 + j] = v[j];
+            continue;
+        }
         if (m <= 8) {
             // one pass over the run, the m sums in registers (each still
             // added in run order)
@@ -1314,7 +1347,7 @@ __global__ void k_sd_inc(int64_t E, int D, int Ld, const int *__restrict__ ig,
                          const double *__restrict__ U, const double *__restrict__ Ng,
                          const int *__restrict__ igstart, const int *__restrict__ icodes,
                          int64_t ild, const double *__restrict__ in, double *__restrict__ Cq,
-                         uint64_t *__restrict__ Fq) {
+                         uint64_t *__restrict__ Fq, int dd1) {
     for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < E;
          q += (int64_t)gridDim.x * blockDim.x) {
         const int a = il[q];
@@ -1334,15 +1367,39 @@ __global__ void k_sd_inc(int64_t E, int D, int Ld, const int *__restrict__ ig,
                 rr[t] = -f * ut;
                 if (ut != 0.0) bits |= 1ull << t;
             }
-            for (int c = igstart[g]; c < igstart[g + 1]; ++c) {
-                if (icodes[da * ild + c] != la) continue;
-                for (int d = 0; d < D; ++d) {
-                    if (sparse_dim[d]) continue;
-                    const int tc = dense_idx[offs[d] + icodes[d * ild + c]];
-                    const double v = in[c];
+            const int c0 = igstart[g], c1 = igstart[g + 1];
+            if (dd1 >= 0 && c1 - c0 <= kIncBatch) {
+                // one dense dimension (dd1) and a short group (the usual
+                // case): the cells' codes and weights, then their dense
+                // slots, loaded together; added in cell order as below
+                int ca[kIncBatch], ct[kIncBatch];
+                double cv[kIncBatch];
 #pragma unroll
-                    for (int t = 0; t < 8; ++t)
-                        if (tc == t) rr[t] += v;
+                for (int b = 0; b < kIncBatch; ++b) {
+                    const int c = c0 + min(b, c1 - c0 - 1);
+                    ca[b] = icodes[da * ild + c];
+                    ct[b] = icodes[dd1 * ild + c];
+                    cv[b] = in[c];
+                }
+#pragma unroll
+                for (int b = 0; b < kIncBatch; ++b) ct[b] = dense_idx[offs[dd1] + ct[b]];
+#pragma unroll
+                for (int b = 0; b < kIncBatch; ++b)
+                    if (b < c1 - c0 && ca[b] == la)
+#pragma unroll
+                        for (int t = 0; t < 8; ++t)
+                            if (ct[b] == t) rr[t] += cv[b];
+            } else {
+                for (int c = c0; c < c1; ++c) {
+                    if (icodes[da * ild + c] != la) continue;
+                    for (int d = 0; d < D; ++d) {
+                        if (sparse_dim[d]) continue;
+                        const int tc = dense_idx[offs[d] + icodes[d * ild + c]];
+                        const double v = in[c];
+#pragma unroll
+                        for (int t = 0; t < 8; ++t)
+                            if (tc == t) rr[t] += v;
+                    }
                 }
             }
             double *row = Cq + r * Ld;
@@ -3293,8 +3350,17 @@ int hdfe_panel_system(void *h, int with_rhs, void **matrix, double *seconds) {
     DEV_ALLOC(P.Ng, std::max<int64_t>(Gi, 1), s);
     DEV_ALLOC(P.vbar, std::max<int64_t>(Gi * m, 1), s);
     DEV_ALLOC(P.gis, Gi + 1, s);
-    hipLaunchKernelGGL(k_inc_reduce, dim3(grid1(E)), dim3(T256), 0, s, (int64_t)E, m, lbits, k2.p,
-                       v2.p, istart.p, P.in.p, P.isums.p, P.ig.p, P.il.p, P.iu.p, P.is.p);
+#define HDFE_IR(MM)                                                                             \
+    hipLaunchKernelGGL(k_inc_reduce<MM>, dim3(grid1(E)), dim3(T256), 0, s, (int64_t)E, m, lbits, \
+                       k2.p, v2.p, istart.p, P.in.p, P.isums.p, P.ig.p, P.il.p, P.iu.p, P.is.p)
+    switch (m) {
+        case 1: HDFE_IR(1); break;
+        case 2: HDFE_IR(2); break;
+        case 3: HDFE_IR(3); break;
+        case 4: HDFE_IR(4); break;
+        default: HDFE_IR(0); break;
+    }
+#undef HDFE_IR
 #define HDFE_L(W)                                                                        \
     hipLaunchKernelGGL(k_group_totals<W>, dim3(grid1(Gi * W)), dim3(T256), 0, s, Gi, m,  \
                        P.igstart.p, P.in.p, P.isums.p, P.Ng.p, P.vbar.p)
@@ -3472,6 +3538,14 @@ int hdfe_panel_system(void *h, int with_rhs, void **matrix, double *seconds) {
     DEV_ALLOC(colpres, std::max(Ld, 1), s);
     DEV_ALLOC(cpos, std::max<int64_t>((int64_t)Ld * (Ls + 1), 1), s);
     if (Ld) {
+        // the one dense dimension, if there is exactly one (k_sd_inc's short path)
+        int dense1 = -1, ndense = 0;
+        for (int t = 0; t < D; ++t)
+            if (!sparse_dim[t]) {
+                dense1 = t;
+                ++ndense;
+            }
+        if (ndense != 1) dense1 = -1;
         {
             DevBuf<int> sflag, sidx;
             DEV_ALLOC(sflag, (int64_t)E + 1, s);
@@ -3488,7 +3562,7 @@ int hdfe_panel_system(void *h, int with_rhs, void **matrix, double *seconds) {
             hipLaunchKernelGGL(k_sd_inc, dim3(grid1(E)), dim3(T256), 0, s, (int64_t)E, D, Ld,
                                P.ig.p, P.il.p, P.iu.p, sidx.p, d_sparse_lv.p, d_lvl_dim.p,
                                P.offs.p, d_sparse_dim.p, d_dense_idx.p, U.p, P.Ng.p,
-                               P.igstart.p, P.icodes.p, M, P.in.p, Cq.p, Fq.p);
+                               P.igstart.p, P.icodes.p, M, P.in.p, Cq.p, Fq.p, dense1);
             hipLaunchKernelGGL(k_sd_sum, dim3(grid1(Ls * Ld)), dim3(T256), 0, s, Ls, Ld,
                                d_sparse_levels.p, P.lptr.p, P.lq.p, sidx.p, Cq.p, Fq.p, PD.p,
                                PP.p);
