@@ -290,10 +290,29 @@ __global__ void k_starts(int64_t n, const int *__restrict__ head, const int *__r
 }
 
 // Column sums of V (n, m) in fixed row blocks: part[b * m + j].
+// MM: m at compile time (1..8; 0 runtime): the row's loads unguarded and
+// four rows' loads in flight (each column still summed row by row in order)
+template <int MM>
 __global__ void k_colsum_rows(int64_t n, int m, const double *__restrict__ V,
                               double *__restrict__ part) {
     __shared__ double lds[T256];
     const int64_t r0 = part_begin(n, blockIdx.x, gridDim.x), r1 = part_begin(n, blockIdx.x + 1, gridDim.x);
+    if (MM > 0) {
+        constexpr int M1 = MM > 0 ? MM : 1;
+        double sj[M1];
+#pragma unroll
+        for (int j = 0; j < M1; ++j) sj[j] = 0.0;
+#pragma unroll 4
+        for (int64_t i = r0 + threadIdx.x; i < r1; i += blockDim.x)
+#pragma unroll
+            for (int j = 0; j < M1; ++j) sj[j] += V[i * M1 + j];
+#pragma unroll
+        for (int j = 0; j < M1; ++j) {
+            const double t = block_sum<double>(sj[j], lds);
+            if (threadIdx.x == 0) part[(int64_t)blockIdx.x * M1 + j] = t;
+        }
+        return;
+    }
     if (m <= 8) {
         // every column in one pass over the rows (each column's sums in the
         // same order as a pass per column)
@@ -319,10 +338,33 @@ __global__ void k_colsum_rows(int64_t n, int m, const double *__restrict__ V,
 }
 
 // Squared deviations from the mean, same blocks.
+template <int MM>
 __global__ void k_colss_rows(int64_t n, int m, const double *__restrict__ V,
                              const double *__restrict__ mean, double *__restrict__ part) {
     __shared__ double lds[T256];
     const int64_t r0 = part_begin(n, blockIdx.x, gridDim.x), r1 = part_begin(n, blockIdx.x + 1, gridDim.x);
+    if (MM > 0) {
+        constexpr int M1 = MM > 0 ? MM : 1;
+        double mu[M1], sj[M1];
+#pragma unroll
+        for (int j = 0; j < M1; ++j) {
+            mu[j] = mean[j];
+            sj[j] = 0.0;
+        }
+#pragma unroll 4
+        for (int64_t i = r0 + threadIdx.x; i < r1; i += blockDim.x)
+#pragma unroll
+            for (int j = 0; j < M1; ++j) {
+                const double d = V[i * M1 + j] - mu[j];
+                sj[j] += d * d;
+            }
+#pragma unroll
+        for (int j = 0; j < M1; ++j) {
+            const double t = block_sum<double>(sj[j], lds);
+            if (threadIdx.x == 0) part[(int64_t)blockIdx.x * M1 + j] = t;
+        }
+        return;
+    }
     if (m <= 8) {
         double mu[8], sj[8];
 #pragma unroll
@@ -2762,14 +2804,31 @@ static int run_pass1(Panel &P) {
     std::vector<double> hp;
     DEV_ALLOC(P.means, m, s);
     DEV_ALLOC(P.raw_ss, m, s);
-    hipLaunchKernelGGL(k_colsum_rows, dim3(nb), dim3(T256), 0, s, nk, m, P.V.p, part.p);
+#define HDFE_CS(MM) hipLaunchKernelGGL(k_colsum_rows<MM>, dim3(nb), dim3(T256), 0, s, nk, m, P.V.p, part.p)
+    switch (m) {
+        case 1: HDFE_CS(1); break;
+        case 2: HDFE_CS(2); break;
+        case 3: HDFE_CS(3); break;
+        case 4: HDFE_CS(4); break;
+        default: HDFE_CS(0); break;
+    }
+#undef HDFE_CS
     HIP_TRY(to_host(hp, part.p, (int64_t)nb * m, s));
     std::vector<double> mean(m, 0.0), ss(m, 0.0);
     for (int b = 0; b < nb; ++b)
         for (int j = 0; j < m; ++j) mean[j] += hp[(size_t)b * m + j];
     for (int j = 0; j < m; ++j) mean[j] /= (double)nk;
     HIP_TRY(hipMemcpyAsync(P.means.p, mean.data(), m * 8, hipMemcpyHostToDevice, s));
-    hipLaunchKernelGGL(k_colss_rows, dim3(nb), dim3(T256), 0, s, nk, m, P.V.p, P.means.p, part.p);
+#define HDFE_CS(MM) \
+    hipLaunchKernelGGL(k_colss_rows<MM>, dim3(nb), dim3(T256), 0, s, nk, m, P.V.p, P.means.p, part.p)
+    switch (m) {
+        case 1: HDFE_CS(1); break;
+        case 2: HDFE_CS(2); break;
+        case 3: HDFE_CS(3); break;
+        case 4: HDFE_CS(4); break;
+        default: HDFE_CS(0); break;
+    }
+#undef HDFE_CS
     HIP_TRY(to_host(hp, part.p, (int64_t)nb * m, s));
     for (int b = 0; b < nb; ++b)
         for (int j = 0; j < m; ++j) ss[j] += hp[(size_t)b * m + j];
