@@ -800,7 +800,7 @@ __global__ __launch_bounds__(256) void k_dense_diag(int64_t Gi, int Ld, int m,
 // and adds u - u^2 / N_g and s_j - u vbar_gj (kernels_base.py:64-110) to the
 // sub-group's LDS slots; the slots are added in sub-group order into
 // part[b, t, col] (fixed order).
-template <int W>
+template <int W, int MM>
 __global__ __launch_bounds__(256) void k_dense_sums(
     int64_t Gi, int Ld, int m, const int *__restrict__ igstart, const int *__restrict__ icodes,
     int64_t ild, const int64_t *__restrict__ offs, DimList dense,
@@ -808,7 +808,10 @@ __global__ __launch_bounds__(256) void k_dense_sums(
     const double *__restrict__ isums, const double *__restrict__ Ng,
     const double *__restrict__ vbar, double *__restrict__ U, double *__restrict__ part) {
     extern __shared__ double acc[];
-    const int np = m + 1, npair = Ld * np;
+    // MM: m at compile time (0: runtime), so the columns' loads carry no guards
+    constexpr int MJ = MM > 0 ? MM : 8;
+    const int mm = MM > 0 ? MM : m;
+    const int np = mm + 1, npair = Ld * np;
     const int sub = threadIdx.x % W, sg = threadIdx.x / W, nsg = blockDim.x / W;
     for (int e = threadIdx.x; e < nsg * npair; e += blockDim.x) acc[e] = 0.0;
     __syncthreads();
@@ -825,21 +828,30 @@ __global__ __launch_bounds__(256) void k_dense_sums(
 #pragma unroll
         for (int j = 0; j < 8; ++j) sa[j] = sb[j] = 0.0;
         if (simple) {
+            // both cells' loads together (a lane without a cell reads the
+            // group's first and drops it)
             const int d = dense.d[0];
             const int ca = c0 + sub, cb = ca + W;
-            if (ca < c1) {
-                ta = dense_idx[offs[d] + icodes[d * ild + ca]];
-                ina = in[ca];
+            const bool ha = ca < c1, hb = cb < c1;
+            const int xa = ha ? ca : c0, xb = hb ? cb : c0;
+            const int ka = icodes[d * ild + xa], kb = icodes[d * ild + xb];
+            const double na = in[xa], nb = in[xb];
+            double va[MJ], vb[MJ];
 #pragma unroll
-                for (int j = 0; j < 8; ++j)
-                    if (j < m) sa[j] = isums[(int64_t)ca * m + j];
+            for (int j = 0; j < MJ; ++j) {
+                const int jj = MM > 0 ? j : min(j, mm - 1);
+                va[j] = isums[(int64_t)xa * mm + jj];
+                vb[j] = isums[(int64_t)xb * mm + jj];
             }
-            if (cb < c1) {
-                tb = dense_idx[offs[d] + icodes[d * ild + cb]];
-                inb = in[cb];
+            const int da = dense_idx[offs[d] + ka], db = dense_idx[offs[d] + kb];
+            ta = ha ? da : -1;
+            tb = hb ? db : -1;
+            ina = ha ? na : 0.0;
+            inb = hb ? nb : 0.0;
 #pragma unroll
-                for (int j = 0; j < 8; ++j)
-                    if (j < m) sb[j] = isums[(int64_t)cb * m + j];
+            for (int j = 0; j < MJ; ++j) {
+                sa[j] = ha && j < mm ? va[j] : 0.0;
+                sb[j] = hb && j < mm ? vb[j] : 0.0;
             }
         }
         // distinct levels among the group's cells (one cell a worker-year):
@@ -871,8 +883,8 @@ __global__ __launch_bounds__(256) void k_dense_sums(
             const double ua = __shfl(ina, la), ub = __shfl(inb, lb);
             myu = srca >= 0 ? ua : (srcb >= 0 ? ub : 0.0);
 #pragma unroll
-            for (int j = 0; j < 8; ++j) {
-                if (j < m) {
+            for (int j = 0; j < MJ; ++j) {
+                if (j < mm) {
                     const double va = __shfl(sa[j], la), vb = __shfl(sb[j], lb);
                     mys[j] = srca >= 0 ? va : (srcb >= 0 ? vb : 0.0);
                 }
@@ -899,27 +911,27 @@ __global__ __launch_bounds__(256) void k_dense_sums(
                         if (dense_idx[offs[d] + icodes[d * ild + c]] != t) continue;
                         u += in[c];
 #pragma unroll
-                        for (int j = 0; j < 8; ++j)
-                            if (j < m) sj[j] += isums[(int64_t)c * m + j];
+                        for (int j = 0; j < MJ; ++j)
+                            if (j < mm) sj[j] += isums[(int64_t)c * mm + j];
                     }
             }
             u = run_sum<W>(u);
 #pragma unroll
-            for (int j = 0; j < 8; ++j)
-                if (j < m) sj[j] = run_sum<W>(sj[j]);
+            for (int j = 0; j < MJ; ++j)
+                if (j < mm) sj[j] = run_sum<W>(sj[j]);
             if (own) {
                 if (sub == t) {
                     myu = u;
 #pragma unroll
-                    for (int j = 0; j < 8; ++j) mys[j] = sj[j];
+                    for (int j = 0; j < MJ; ++j) mys[j] = sj[j];
                 }
             } else if (sub == 0) {
                 U[g * Ld + t] = u;
                 double *a = acc + (sg * Ld + t) * np;
                 a[0] += u - u * u / Ng[g];
 #pragma unroll
-                for (int j = 0; j < 8; ++j)
-                    if (j < m) a[1 + j] += sj[j] - u * vbar[g * m + j];
+                for (int j = 0; j < MJ; ++j)
+                    if (j < mm) a[1 + j] += sj[j] - u * vbar[g * mm + j];
             }
         }
         if (own && sub < Ld) {
@@ -927,8 +939,8 @@ __global__ __launch_bounds__(256) void k_dense_sums(
             double *a = acc + (sg * Ld + sub) * np;
             a[0] += myu - myu * myu / Ng[g];
 #pragma unroll
-            for (int j = 0; j < 8; ++j)
-                if (j < m) a[1 + j] += mys[j] - myu * vbar[g * m + j];
+            for (int j = 0; j < MJ; ++j)
+                if (j < mm) a[1 + j] += mys[j] - myu * vbar[g * mm + j];
         }
     }
     __syncthreads();
@@ -3478,12 +3490,21 @@ int hdfe_panel_system(void *h, int with_rhs, void **matrix, double *seconds) {
             const int nbs = 2048;
             DevBuf<double> spart;
             DEV_ALLOC(spart, (int64_t)nbs * npd, s);
-#define HDFE_L(W)                                                                              \
-    hipLaunchKernelGGL(k_dense_sums<W>, dim3(nbs), dim3(T256), (T256 / W) * npd * sizeof(double), \
-                       s, Gi, Ld, m, P.igstart.p, P.icodes.p, M, P.offs.p, ddl, d_dense_idx.p,  \
+#define HDFE_L2(W, MM)                                                                              \
+    hipLaunchKernelGGL((k_dense_sums<W, MM>), dim3(nbs), dim3(T256), (T256 / W) * npd * sizeof(double), \
+                       s, Gi, Ld, m, P.igstart.p, P.icodes.p, M, P.offs.p, ddl, d_dense_idx.p,       \
                        P.in.p, P.isums.p, P.Ng.p, P.vbar.p, U.p, spart.p)
+#define HDFE_L(W)                       \
+    switch (m) {                        \
+        case 1: HDFE_L2(W, 1); break;   \
+        case 2: HDFE_L2(W, 2); break;   \
+        case 3: HDFE_L2(W, 3); break;   \
+        case 4: HDFE_L2(W, 4); break;   \
+        default: HDFE_L2(W, 0); break;  \
+    }
             HDFE_BY_WIDTH(cw, HDFE_L)
 #undef HDFE_L
+#undef HDFE_L2
             hipLaunchKernelGGL(k_dense_sums_finish, dim3(grid1((int64_t)npd * 64)), dim3(T256), 0,
                                s, nbs, Ld, m, spart.p, d_dense_levels.p, P.diag.p,
                                with_rhs ? P.rhs.p : (double *)nullptr);
