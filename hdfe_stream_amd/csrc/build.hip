@@ -554,14 +554,19 @@ __global__ void k_level_tables(int64_t L, LevelDims ld, uint8_t *__restrict__ sp
 
 // obs / cnt per level of one dimension over all cells (cnt in integer
 // atomics: exact; obs, with weights, in float atomics).  Small dimensions
-// count in LDS bins first.
+// count in LDS bins first.  ONE: unweighted cells of one row each (cell c is
+// row c), so a run's count is its length, read off the wave's head mask
+// without the run scan.  LDSB: the dimension counts in LDS bins (a larger one
+// allocates none, which keeps the kernel's occupancy for its gathers).
+template <bool ONE, bool LDSB>
 __global__ void k_obs_cnt(int64_t C, const int *__restrict__ cstart,
                           const int *__restrict__ code, int64_t off, int nl,
                           const double *__restrict__ cn, unsigned long long *__restrict__ cnt,
                           double *__restrict__ obs, int weighted) {
-    __shared__ unsigned bins[kLdsBins / 2];
-    __shared__ double wbins[kLdsBins / 2];
-    const bool lds = nl <= kLdsBins / 2;
+    constexpr int NB = LDSB ? kLdsBins / 2 : 1;
+    __shared__ unsigned bins[NB];
+    __shared__ double wbins[NB];
+    constexpr bool lds = LDSB;
     if (lds) {
         for (int t = threadIdx.x; t < nl; t += blockDim.x) {
             bins[t] = 0;
@@ -575,12 +580,24 @@ __global__ void k_obs_cnt(int64_t C, const int *__restrict__ cstart,
          c0 += (int64_t)gridDim.x * blockDim.x) {
         const int64_t c = c0 + lane;
         const bool in = c < C;
-        const int s0 = in ? cstart[c] : 0;
-        const unsigned long long r = in ? (unsigned long long)(cstart[c + 1] - s0) : 0ull;
-        const int a = in ? code[s0] : -1;
         bool last;
-        const unsigned long long rt = run_total<unsigned long long>(a, r, last);
-        const double wt = weighted ? run_total<double>(a, in ? cn[c] : 0.0, last) : 0.0;
+        unsigned long long rt;
+        double wt = 0.0;
+        int a;
+        if (ONE) {
+            a = in ? code[c] : -1;
+            const int up = __shfl_up(a, 1, kWave);
+            const unsigned long long hm = __ballot(lane == 0 || up != a);
+            last = lane == kWave - 1 || ((hm >> (lane + 1)) & 1ull);
+            const int hl = 63 - __clzll((long long)(hm & ((2ull << lane) - 1ull)));
+            rt = (unsigned long long)(lane - hl + 1);
+        } else {
+            const int s0 = in ? cstart[c] : 0;
+            const unsigned long long r = in ? (unsigned long long)(cstart[c + 1] - s0) : 0ull;
+            a = in ? code[s0] : -1;
+            rt = run_total<unsigned long long>(a, r, last);
+            wt = weighted ? run_total<double>(a, in ? cn[c] : 0.0, last) : 0.0;
+        }
         if (last && a >= 0) {
             if (lds) {
                 atomicAdd(&bins[a], (unsigned)rt);
@@ -598,6 +615,68 @@ __global__ void k_obs_cnt(int64_t C, const int *__restrict__ cstart,
                 atomicAdd(&cnt[off + t], (unsigned long long)bins[t]);
                 if (weighted) atomicAdd(&obs[off + t], wbins[t]);
             }
+    }
+}
+
+// obs / cnt of a dimension of at most NL levels (the years): every lane keeps
+// its own counts and weights per level in registers (a select per level, no
+// atomics and no run scan per cell; a wave's 64 cells spread over the few
+// levels made 8-way LDS atomic conflicts), four cells' loads in flight at a
+// time (addresses clamped to the last cell, their counts masked), then one
+// block sum and one atomic per block and level.  ONE as in k_obs_cnt.
+constexpr int kObsBatch = 4;
+
+template <int NL, bool ONE>
+__global__ __launch_bounds__(256) void k_obs_cnt_small(int64_t C, const int *__restrict__ cstart,
+                                                       const int *__restrict__ code, int64_t off,
+                                                       int nl, const double *__restrict__ cn,
+                                                       unsigned long long *__restrict__ cnt,
+                                                       double *__restrict__ obs, int weighted) {
+    __shared__ unsigned long long ldsk[T256 / kWave];
+    __shared__ double ldsw[T256 / kWave];
+    unsigned long long k[NL];
+    double wv[NL];
+#pragma unroll
+    for (int t = 0; t < NL; ++t) {
+        k[t] = 0ull;
+        wv[t] = 0.0;
+    }
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t c0 = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c0 < C;
+         c0 += kObsBatch * stride) {
+        int s[kObsBatch], e[kObsBatch], a[kObsBatch];
+        double wc[kObsBatch];
+#pragma unroll
+        for (int b = 0; b < kObsBatch; ++b) {
+            const int64_t c = min(c0 + b * stride, C - 1);
+            s[b] = ONE ? (int)c : cstart[c];
+            e[b] = ONE ? (int)c + 1 : cstart[c + 1];
+            wc[b] = weighted ? cn[c] : 0.0;
+        }
+#pragma unroll
+        for (int b = 0; b < kObsBatch; ++b) a[b] = code[s[b]];
+#pragma unroll
+        for (int b = 0; b < kObsBatch; ++b) {
+            const bool in = c0 + b * stride < C;
+            const unsigned long long r = in ? (unsigned long long)(e[b] - s[b]) : 0ull;
+            const double x = in ? wc[b] : 0.0;
+#pragma unroll
+            for (int t = 0; t < NL; ++t) {
+                const bool h = a[b] == t;
+                k[t] += h ? r : 0ull;
+                wv[t] += h ? x : 0.0;
+            }
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < NL; ++t) {
+        if (t >= nl) break;
+        const unsigned long long kt = block_sum<unsigned long long>(k[t], ldsk);
+        const double wt = weighted ? block_sum<double>(wv[t], ldsw) : 0.0;
+        if (threadIdx.x == 0 && kt) {
+            atomicAdd(&cnt[off + t], kt);
+            if (weighted) atomicAdd(&obs[off + t], wt);
+        }
     }
 }
 
@@ -2957,11 +3036,37 @@ static int run_pass1(Panel &P) {
         DEV_ALLOC(ic, std::max<int64_t>(L, 1), s);
         HIP_TRY(hipMemsetAsync(ic.p, 0, std::max<int64_t>(L, 1) * 8, s));
         HIP_TRY(hipMemsetAsync(P.obs.p, 0, std::max<int64_t>(L, 1) * 8, s));
-        for (int t = 0; t < D; ++t)
-            hipLaunchKernelGGL(k_obs_cnt, dim3(grid1(nc, T256, 2048)), dim3(T256), 0, s,
-                               (int64_t)nc, P.cstart.p, P.codes.p + (int64_t)t * nk,
-                               P.offs_h[t], (int)P.nlev[t], cn.p, ic.p, P.obs.p,
-                               P.weighted);
+        for (int t = 0; t < D; ++t) {
+            const int nl = (int)P.nlev[t];
+            const int *ct = P.codes.p + (int64_t)t * nk;
+            const bool one = nc == nk && !P.weighted;
+#define HDFE_OCS(NL, ONE)                                                                       \
+    hipLaunchKernelGGL((k_obs_cnt_small<NL, ONE>), dim3(grid1(nc, T256, 1024)), dim3(T256), 0, s, \
+                       (int64_t)nc, P.cstart.p, ct, P.offs_h[t], nl, cn.p, ic.p, P.obs.p,        \
+                       P.weighted)
+            if (nl <= 8 && one)
+                HDFE_OCS(8, true);
+            else if (nl <= 8)
+                HDFE_OCS(8, false);
+            else if (nl <= 16 && one)
+                HDFE_OCS(16, true);
+            else if (nl <= 16)
+                HDFE_OCS(16, false);
+#undef HDFE_OCS
+#define HDFE_OC(ONE, LDSB)                                                                   \
+    hipLaunchKernelGGL((k_obs_cnt<ONE, LDSB>), dim3(grid1(nc, T256, 2048)), dim3(T256), 0, s, \
+                       (int64_t)nc, P.cstart.p, ct, P.offs_h[t], nl, cn.p, ic.p, P.obs.p,     \
+                       P.weighted)
+            else if (nl <= kLdsBins / 2 && one)
+                HDFE_OC(true, true);
+            else if (nl <= kLdsBins / 2)
+                HDFE_OC(false, true);
+            else if (one)
+                HDFE_OC(true, false);
+            else
+                HDFE_OC(false, false);
+#undef HDFE_OC
+        }
         hipLaunchKernelGGL(k_u64_to_f64, dim3(grid1(L)), dim3(T256), 0, s, L, ic.p, P.cnt.p,
                            P.weighted ? (double *)nullptr : P.obs.p);
         HIP_TRY(hipStreamSynchronize(s));
