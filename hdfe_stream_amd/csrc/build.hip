@@ -2566,12 +2566,18 @@ struct ScoreAcc {
     static constexpr int NV = 1 + 2 * T;
 };
 
+// The covariates' columns, by value (no host-to-device copy before the
+// launch).
+struct ColSel {
+    int c[8];
+};
+
 template <int W, int K>
 __global__ void __launch_bounds__(256)
     k_scores(int64_t G, int D, int m, const int *__restrict__ gstart, const int *__restrict__ codes,
              int64_t ld, const int64_t *__restrict__ offs, const double *__restrict__ V,
              const double *__restrict__ w, const double *__restrict__ Gm,
-             const int *__restrict__ cols, int kc, const double *__restrict__ e,
+             ColSel cols, int kc, const double *__restrict__ e,
              double *__restrict__ U, double *__restrict__ part) {
     constexpr int T = ScoreAcc<K>::T, NV = ScoreAcc<K>::NV;
     const int sub = threadIdx.x % W;
@@ -2580,7 +2586,7 @@ __global__ void __launch_bounds__(256)
     for (int v = 0; v < NV; ++v) acc[v] = 0.0;
     int col[K];
 #pragma unroll
-    for (int c = 0; c < K; ++c) col[c] = c < kc ? cols[c] : -1;
+    for (int c = 0; c < K; ++c) col[c] = c < kc ? cols.c[c] : -1;
     for (int64_t g = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / W; g < G;
          g += (int64_t)gridDim.x * blockDim.x / W) {
         const int s = gstart[g], en = gstart[g + 1];
@@ -4204,9 +4210,8 @@ int hdfe_panel_scores(void *h, const double *Gm, const int *cols, int kc, const 
     for (int c = 0; c < kc; ++c)
         ARG_CHECK(cols[c] >= 0 && cols[c] < P.m, "hdfe_panel_scores: column out of range");
     hipStream_t s = P.s;
-    DevBuf<int> cc;
-    DEV_ALLOC(cc, kc, s);
-    HIP_TRY(hipMemcpyAsync(cc.p, cols, kc * 4, hipMemcpyHostToDevice, s));
+    ColSel cc{};
+    for (int c = 0; c < kc; ++c) cc.c[c] = cols[c];
     const int W = run_width(P.n, P.G);
     const int nb = grid1(P.G * W, 256, 1024);
     const int K = kc <= 2 ? 2 : kc <= 4 ? 4 : 8;
@@ -4216,7 +4221,7 @@ int hdfe_panel_scores(void *h, const double *Gm, const int *cols, int kc, const 
 #define HDFE_L2(WW, KK)                                                                        \
     hipLaunchKernelGGL((k_scores<WW, KK>), dim3(nb), dim3(T256), 0, s, P.G, P.D, P.m,         \
                        P.gstart.p, P.codes.p, P.n, P.offs.p, P.V.p,                           \
-                       P.weighted ? P.w.p : (const double *)nullptr, Gm, cc.p, kc, e, U, part.p)
+                       P.weighted ? P.w.p : (const double *)nullptr, Gm, cc, kc, e, U, part.p)
 #define HDFE_L(WW)                 \
     if (K == 2) HDFE_L2(WW, 2);    \
     else if (K == 4) HDFE_L2(WW, 4); \

@@ -68,6 +68,31 @@ def _pinned(n, dtype):
     return t.numpy()
 
 
+_side_stream = None
+
+
+def _host_async(t):
+    """Start a device tensor's copy to page-locked host memory on a side
+    stream, after the work queued so far on the current stream; returns the
+    array and an event (None for an empty tensor) that must complete before
+    the array is read."""
+    import torch
+    global _side_stream
+    if _side_stream is None:
+        _side_stream = torch.cuda.Stream()
+    pin = torch.empty(t.numel(), dtype=t.dtype, pin_memory=True)
+    ev = None
+    if t.numel():
+        cur = torch.cuda.current_stream()
+        _side_stream.wait_stream(cur)
+        with torch.cuda.stream(_side_stream):
+            pin.copy_(t.reshape(-1), non_blocking=True)
+            ev = torch.cuda.Event()
+            ev.record(_side_stream)
+        t.record_stream(_side_stream)
+    return pin.numpy().reshape(tuple(t.shape)), ev
+
+
 def _host(t):
     """A device tensor's host copy through page-locked memory."""
     out = _pinned(t.numel(), np.dtype(str(t.dtype).replace("torch.", "")))
@@ -169,7 +194,7 @@ class Panel:
         p, n = self.ptr(name)
         out = torch.empty(n, dtype=dtype, device="cuda")
         if n:
-            torch.cuda.synchronize()
+            torch.cuda.current_stream().synchronize()
             _lib.check(_lib.load().hdfe_memcpy(C.c_void_p(out.data_ptr()), C.c_void_p(p),
                                                out.numel() * out.element_size(), 2), "hdfe_memcpy")
         return out
@@ -520,11 +545,16 @@ class StreamingHDFE:
         gy = _host(gam_y)
         panel.copy_wait()
         levels = {fe_order[0]: glevels}
-        values = {fe_order[0]: _host(geff)}
+        values = {}
         for t, d in enumerate(fe_order[1:]):
             o, nl = int(panel.offs[t]), panel.n_levels[t]
             levels[d] = panel.array(f"levels{t}", np.int64, pinned=True)
             values[d] = gy[o:o + nl]
+        # the streamed dimension's effects (one per group: the largest host
+        # copy of the fit, after the smaller ones, which would queue behind it
+        # on the bus) travel while the variances are computed
+        geff_h, geff_ev = _host_async(geff)
+        values = {fe_order[0]: geff_h, **values}
         diag = {"seconds_ingest": round(t_in - t0, 4), "ingest": ingest,
                 "seconds_pass01": round(t1 - t_in, 4), "seconds_system": round(t2 - t1, 4),
                 "seconds_solve": round(t3 - t2, 4), "seconds_pass2": round(t4 - t3, 4),
@@ -535,17 +565,21 @@ class StreamingHDFE:
                          fe_levels=levels, fe_values=values, n_obs=panel.info["n"],
                          n_components=ncomp, A=A, diagnostics=diag, panel=panel, Gamma=Gamma,
                          resid_sorted=resid, gam_y=gam_y, geff=geff, fe_dof=self.fe_dof)
-        if self.vcov is not None and res.coefnames:
-            # the coefficients' variance at fit time, as the reference's fit
-            # (core/estimator.py:467-481): iid and hetero always, and the
-            # default `vcov` (cluster columns that are not fixed effects come
-            # from the fit's data); vcov=None skips them
-            t5 = time.time()
-            res.vcovs = {spec: res.vcov(spec, data) for spec in ("iid", "hetero")}
-            if not isinstance(self.vcov, str) or self.vcov not in res.vcovs:
-                res.vcovs[str(self.vcov)] = res.vcov(self.vcov, data)
-            res.vcov_matrix, res.vcov_details = res.vcovs[str(self.vcov)]
-            res.diagnostics["seconds_vcov"] = round(time.time() - t5, 4)
+        try:
+            if self.vcov is not None and res.coefnames:
+                # the coefficients' variance at fit time, as the reference's
+                # fit (core/estimator.py:467-481): iid and hetero always, and
+                # the default `vcov` (cluster columns that are not fixed
+                # effects come from the fit's data); vcov=None skips them
+                t5 = time.time()
+                res.vcovs = {spec: res.vcov(spec, data) for spec in ("iid", "hetero")}
+                if not isinstance(self.vcov, str) or self.vcov not in res.vcovs:
+                    res.vcovs[str(self.vcov)] = res.vcov(self.vcov, data)
+                res.vcov_matrix, res.vcov_details = res.vcovs[str(self.vcov)]
+                res.diagnostics["seconds_vcov"] = round(time.time() - t5, 4)
+        finally:
+            if geff_ev is not None:
+                geff_ev.synchronize()
         if self.verbose:
             print(f"fit: {diag}")
         return res

@@ -112,7 +112,7 @@ class VcovEngine:
         import torch
         k = self.k
         out = np.zeros(1 + 2 * k * k)
-        torch.cuda.synchronize()
+        torch.cuda.current_stream().synchronize()
         _lib.check(_lib.load().hdfe_panel_scores(
             self.res.panel._h, C.c_void_p(self.res.Gamma.data_ptr()), self._cols.ctypes.data, k,
             C.c_void_p(self._e.data_ptr()), C.c_void_p(U.data_ptr() if U is not None else None),
@@ -128,7 +128,7 @@ class VcovEngine:
             if self.k <= SCORES_MAX_K:
                 self._scores(U)
             else:
-                torch.cuda.synchronize()
+                torch.cuda.current_stream().synchronize()
                 _lib.check(_lib.load().hdfe_panel_partial(
                     self.res.panel._h, C.c_void_p(self.res.Gamma.data_ptr()),
                     self._cols.ctypes.data, self.k, C.c_void_p(U.data_ptr())),
@@ -219,7 +219,7 @@ class VcovEngine:
         if list(cols) == [res.fe_names[0]]:
             G = panel.info["G"]
             S = torch.empty((G, self.k), dtype=torch.float64, device="cuda")
-            torch.cuda.synchronize()
+            torch.cuda.current_stream().synchronize()
             _lib.check(_lib.load().hdfe_kss_group_sums(G, self.k, panel.ptr("gstart")[0], None,
                                                        None, self.U.data_ptr(), S.data_ptr()),
                        "hdfe_kss_group_sums")
