@@ -454,7 +454,10 @@ __global__ void k_cells(int64_t C, int m, const int *__restrict__ cstart,
 // compiler places in scratch memory).  The scatter is accumulated on its
 // upper triangle, in the runtime-m form's order, and mirrored, so it is
 // exactly symmetric (the runtime-m form rounds its two triangles apart).
-template <int M>
+// ONE: unweighted cells of one row each (cell c is row c): n = 1, one row,
+// the sums the centred row itself and no scatter (every deviation is exactly
+// zero), four cells' loads in flight a lane.
+template <int M, bool ONE>
 __global__ __launch_bounds__(256) void k_cells_m(int64_t C, const int *__restrict__ cstart,
                                                  const double *__restrict__ V,
                                                  const double *__restrict__ w,
@@ -471,7 +474,29 @@ __global__ __launch_bounds__(256) void k_cells_m(int64_t C, const int *__restric
 #pragma unroll
     for (int j = 0; j < M; ++j) mu[j] = mean[j];
     const int64_t c0 = part_begin(C, blockIdx.x, gridDim.x), c1 = part_begin(C, blockIdx.x + 1, gridDim.x);
-    for (int64_t c = c0 + threadIdx.x; c < c1; c += blockDim.x) {
+    if (ONE) {
+        constexpr int B = 4;
+        for (int64_t c = c0 + threadIdx.x; c < c1; c += B * (int64_t)blockDim.x) {
+            double v[B][M];
+#pragma unroll
+            for (int b = 0; b < B; ++b) {
+                const int64_t cb = min(c + b * (int64_t)blockDim.x, c1 - 1);
+#pragma unroll
+                for (int j = 0; j < M; ++j) v[b][j] = V[cb * M + j];
+            }
+#pragma unroll
+            for (int b = 0; b < B; ++b) {
+                const int64_t cb = c + b * (int64_t)blockDim.x;
+                if (cb < c1) {
+                    cn[cb] = 1.0;
+                    crun[cb] = 1.0;
+#pragma unroll
+                    for (int j = 0; j < M; ++j) csums[cb * M + j] = 0.0 + 1.0 * (v[b][j] - mu[j]);
+                }
+            }
+        }
+    }
+    for (int64_t c = c0 + threadIdx.x; !ONE && c < c1; c += blockDim.x) {
         const int s = cstart[c], e = cstart[c + 1];
         double nn = 0.0;
         double sm[M];
@@ -2980,10 +3005,17 @@ static int run_pass1(Panel &P) {
     DEV_ALLOC(wpart, (int64_t)nb * std::max(mm, 1), s);
     {
         const double *wp = P.weighted ? P.w.p : (const double *)nullptr;
-#define HDFE_CELLS(MM)                                                                         \
-    case MM:                                                                                   \
-        hipLaunchKernelGGL(k_cells_m<MM>, dim3(nb), dim3(T256), 0, s, (int64_t)nc, P.cstart.p, \
-                           P.V.p, wp, P.means.p, cn.p, crun.p, csums.p, wpart.p);              \
+        const bool one = nc == nk && !P.weighted;
+#define HDFE_CELLS(MM)                                                                          \
+    case MM:                                                                                    \
+        if (one)                                                                                \
+            hipLaunchKernelGGL((k_cells_m<MM, true>), dim3(nb), dim3(T256), 0, s, (int64_t)nc,  \
+                               P.cstart.p, P.V.p, wp, P.means.p, cn.p, crun.p, csums.p,         \
+                               wpart.p);                                                        \
+        else                                                                                    \
+            hipLaunchKernelGGL((k_cells_m<MM, false>), dim3(nb), dim3(T256), 0, s, (int64_t)nc, \
+                               P.cstart.p, P.V.p, wp, P.means.p, cn.p, crun.p, csums.p,         \
+                               wpart.p);                                                        \
         break;
         switch (m) {
             HDFE_CELLS(1)
