@@ -1716,6 +1716,61 @@ __global__ __launch_bounds__(256) void k_dd(int64_t Gi, int Ld, const int *__res
     }
 }
 
+// Dense x dense block when every dense level belongs to one dimension (the
+// years) and there are LD of them: a lane per group reads the group's LD
+// sums U_g (one row) and Ng once and adds every pair (t1 <= t2) of
+// [t1 == t2] U_t1 - U_t1 U_t2 / N_g in registers (each group's row was read
+// once per pair by k_dd), the groups of a lane in order, then a fixed block
+// tree per pair; presence as a bit mask per lane, OR-ed in LDS.  The
+// partials' layout is k_dd's (both triangles written).
+template <int LD>
+__global__ __launch_bounds__(256) void k_dd_rows(int64_t Gi, const double *__restrict__ U,
+                                                 const double *__restrict__ Ng,
+                                                 double *__restrict__ part,
+                                                 int *__restrict__ ppart) {
+    constexpr int NP = LD * (LD + 1) / 2;
+    __shared__ double lds[T256 / kWave];
+    __shared__ unsigned long long pm;
+    double acc[NP];
+#pragma unroll
+    for (int q = 0; q < NP; ++q) acc[q] = 0.0;
+    unsigned long long mask = 0ull;
+    if (threadIdx.x == 0) pm = 0ull;
+    const int64_t g0 = part_begin(Gi, blockIdx.x, gridDim.x), g1 = part_begin(Gi, blockIdx.x + 1, gridDim.x);
+    for (int64_t g = g0 + threadIdx.x; g < g1; g += blockDim.x) {
+        double u[LD];
+#pragma unroll
+        for (int t = 0; t < LD; ++t) u[t] = U[g * LD + t];
+        const double n = Ng[g];
+        int q = 0;
+#pragma unroll
+        for (int t1 = 0; t1 < LD; ++t1)
+#pragma unroll
+            for (int t2 = t1; t2 < LD; ++t2, ++q) {
+                const bool h = u[t1] != 0.0 && u[t2] != 0.0;
+                const double nat = t1 == t2 ? u[t1] : 0.0;
+                acc[q] += h ? nat - u[t1] * u[t2] / n : 0.0;
+                mask |= h ? 1ull << q : 0ull;
+            }
+    }
+    __syncthreads();
+    if (mask) atomicOr(&pm, mask);
+    int q = 0;
+#pragma unroll
+    for (int t1 = 0; t1 < LD; ++t1)
+#pragma unroll
+        for (int t2 = t1; t2 < LD; ++t2, ++q) {
+            const double v = block_sum<double>(acc[q], lds);
+            if (threadIdx.x == 0) {
+                const int pres = (int)((pm >> q) & 1ull);
+                part[(int64_t)blockIdx.x * LD * LD + t1 * LD + t2] = v;
+                part[(int64_t)blockIdx.x * LD * LD + t2 * LD + t1] = v;
+                ppart[(int64_t)blockIdx.x * LD * LD + t1 * LD + t2] = pres;
+                ppart[(int64_t)blockIdx.x * LD * LD + t2 * LD + t1] = pres;
+            }
+        }
+}
+
 // A wave per pair: lanes strided over the workgroups' partials, a fixed
 // butterfly.
 __global__ void k_dd_finish(int nb, int npair, const double *__restrict__ part,
@@ -3797,9 +3852,25 @@ int hdfe_panel_system(void *h, int with_rhs, void **matrix, double *seconds) {
         DevBuf<int> ddpp;
         DEV_ALLOC(ddp, (int64_t)nbd * Ld * Ld, s);
         DEV_ALLOC(ddpp, (int64_t)nbd * Ld * Ld, s);
-        hipLaunchKernelGGL(k_dd, dim3(nbd), dim3(T256), 0, s, Gi, Ld, d_dense_levels.p,
-                           d_lvl_dim.p, P.offs.p, U.p, P.Ng.p, P.igstart.p, P.icodes.p, M,
-                           P.in.p, ddp.p, ddpp.p);
+#define HDFE_DDR(LL)                                                                   \
+    case LL:                                                                           \
+        hipLaunchKernelGGL(k_dd_rows<LL>, dim3(nbd), dim3(T256), 0, s, Gi, U.p, P.Ng.p, \
+                           ddp.p, ddpp.p);                                             \
+        break;
+        switch (dense1 >= 0 ? Ld : 0) {
+            HDFE_DDR(2)
+            HDFE_DDR(3)
+            HDFE_DDR(4)
+            HDFE_DDR(5)
+            HDFE_DDR(6)
+            HDFE_DDR(7)
+            HDFE_DDR(8)
+            default:
+                hipLaunchKernelGGL(k_dd, dim3(nbd), dim3(T256), 0, s, Gi, Ld, d_dense_levels.p,
+                                   d_lvl_dim.p, P.offs.p, U.p, P.Ng.p, P.igstart.p, P.icodes.p,
+                                   M, P.in.p, ddp.p, ddpp.p);
+        }
+#undef HDFE_DDR
         hipLaunchKernelGGL(k_dd_finish, dim3(grid1((int64_t)Ld * Ld * 64)), dim3(T256), 0, s, nbd, Ld * Ld,
                            ddp.p, ddpp.p, DD.p, DP.p);
         // per dense level: positions of the present sparse levels
